@@ -1,0 +1,62 @@
+// Shared device-side types of libgsx (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gsx {
+
+constexpr int kMaxDeg = 4;         // degrees 1..4 on the device
+constexpr int kMaxQ1 = kMaxDeg + 1; // Gauss nodes per direction (quA=quB=1)
+constexpr int kWave = 64;
+
+// One patch as the kernels see it.  Elements are lexicographic (direction 0
+// fastest); element e_k along direction k covers knots [p+e_k, p+e_k+1] and
+// its active functions are e_k..e_k+p (open knots, no repeated interior
+// knots -- checked on the host).
+struct PatchDev {
+    int d;
+    int p[3];
+    int nb[3];          // basis functions per direction
+    int nel[3];         // elements per direction
+    int fkind;          // 1: d*pi^2*prod sin(pi x); 0: felem
+    int pad0;
+    const double *knots[3];
+    const double *ctrl;     // patch control points (nb x d)
+    const double *kappa;    // per element or nullptr
+    const double *felem;    // per element or nullptr
+    const int *dofs;        // global DOF index per local basis function
+    int64_t nelem;
+    int64_t kmat_off;       // element matrices: doubles offset of element 0
+    int64_t fvec_off;       // element load vectors: doubles offset
+    int64_t elem_off;       // global element id of element 0
+};
+
+// Gauss-Legendre nodes/weights for n = 1..kMaxQ1 points, ascending.
+struct GaussTab {
+    double x[kMaxQ1 + 1][kMaxQ1];
+    double w[kMaxQ1 + 1][kMaxQ1];
+};
+
+// Device-resident scalars of one CG solve.
+struct CGState {
+    double rz;        // (r, z) of the current iterate
+    double pap;       // (p, A p)
+    double alpha;
+    double beta;
+    double rr;        // ||r||^2
+    double err;       // ||r|| / ||b||
+    double bnorm;
+    double tol;
+    double tau;       // Jacobi damping
+    double rz_new;
+    double rel_since_update; // mixed: ||r32||^2 relative tracking
+    double rr_ref;    // mixed: ||r||^2 at the last reliable update
+    int it;           // completed iterations
+    int done;         // 1 once err < tol (later launches exit at once)
+    int maxit;
+    int need_update;  // mixed: a reliable update is due
+};
+
+}  // namespace gsx

@@ -1,0 +1,82 @@
+// Kernel launchers of libgsx (defined in gsx_assembly.hip, gsx_solve.hip).
+#pragma once
+
+#include "gsx_common.h"
+
+namespace gsx {
+
+// assembly
+hipError_t launch_elem(int d, int p, const PatchDev *patches, const int *gpatch,
+                       const int64_t *gprefix, int ng, int64_t nelem, const GaussTab &gt,
+                       double *Kbuf, double *Fbuf, double *detmin, hipStream_t s);
+hipError_t launch_l2err(int d, int p, const PatchDev *patches, const int *gpatch,
+                        const int64_t *gprefix, int ng, int64_t nelem, const GaussTab &gt,
+                        const double *sol, int nfree, double *out, hipStream_t s);
+hipError_t launch_rowcount(const PatchDev *patches, const int *occ_ptr, const int *occ_patch,
+                           const int64_t *occ_local, int nfree, int *counts, int *err,
+                           hipStream_t s);
+hipError_t launch_rowfill(const PatchDev *patches, const int *occ_ptr, const int *occ_patch,
+                          const int64_t *occ_local, int nfree, const int *rowptr, int *col,
+                          hipStream_t s);
+hipError_t launch_scan(const int *counts, int n, int *rowptr, long long *bsum, long long *total,
+                       hipStream_t s);
+int gather_maxrow_bucket(int maxrow);
+hipError_t launch_gather(const PatchDev *patches, const int *occ_ptr, const int *occ_patch,
+                         const int64_t *occ_local, int nfree, const int *rowptr, const int *col,
+                         const double *Kbuf, const double *Fbuf, double *val, double *rhs,
+                         unsigned char *keep, int maxrow, hipStream_t s);
+hipError_t launch_keep(const int *rowptr, const int *col, const double *val,
+                       const unsigned char *keep, int n, int64_t nnz, int *counts, int *nrowptr,
+                       int *ncol, double *nval, long long *bsum, long long *total,
+                       unsigned long long *dropped, int phase, hipStream_t s);
+hipError_t launch_min(const double *v, int64_t n, double *out, hipStream_t s);
+hipError_t launch_sum(const double *v, int64_t n, double *out, hipStream_t s);
+
+// solve
+int spmv_lpr(double avg_row);
+int spmv_blocks(int n, int lpr);
+hipError_t launch_spmv_f64(int lpr, int n, const int *rp, const int *ci, const double *v,
+                           const double *x, double *y, const double *b, double *part, int mode,
+                           const CGState *st, hipStream_t s);
+hipError_t launch_spmv_f32(int lpr, int n, const int *rp, const int *ci, const float *v,
+                           const float *x, float *y, const float *b, double *part, int mode,
+                           const CGState *st, hipStream_t s);
+hipError_t launch_cg_update_f64(int n, double *x, double *r, const double *p, const double *t,
+                                double *z, const double *diag, double *prr, double *prz,
+                                const CGState *st, hipStream_t s);
+hipError_t launch_cg_update_f32(int n, float *x, float *r, const float *p, const float *t,
+                                float *z, const float *diag, double *prr, double *prz,
+                                const CGState *st, hipStream_t s);
+hipError_t launch_cg_xr_f64(int n, double *x, double *r, const double *p, const double *t,
+                            double *prr, const CGState *st, hipStream_t s);
+hipError_t launch_cg_pupdate_f64(int n, double *p, const double *z, const CGState *st,
+                                 hipStream_t s);
+hipError_t launch_cg_pupdate_f32(int n, float *p, const float *z, const CGState *st,
+                                 hipStream_t s);
+hipError_t launch_dot_f64(int n, const double *a, const double *b, double *part,
+                          const CGState *st, hipStream_t s);
+hipError_t launch_dot_f32(int n, const float *a, const float *b, double *part,
+                          const CGState *st, hipStream_t s);
+hipError_t launch_jacobi_apply_f64(int n, const double *r, double *z, const double *diag,
+                                   double tau, double *prz, const CGState *st, hipStream_t s);
+hipError_t launch_jacobi_apply_f32(int n, const float *r, float *z, const float *diag,
+                                   double tau, double *prz, const CGState *st, hipStream_t s);
+hipError_t launch_jacobi_step_f64(int lpr, int n, const int *rp, const int *ci, const double *v,
+                                  const double *diag, const double *b, const double *xin,
+                                  double *xout, double tau, int x_is_zero, hipStream_t s);
+hipError_t launch_jacobi_step_f32(int lpr, int n, const int *rp, const int *ci, const float *v,
+                                  const float *diag, const float *b, const float *xin,
+                                  float *xout, float tau, int x_is_zero, hipStream_t s);
+hipError_t launch_diag_f64(int n, const int *rp, const int *ci, const double *v, double *diag,
+                           hipStream_t s);
+hipError_t launch_diag_f32(int n, const int *rp, const int *ci, const float *v, float *diag,
+                           hipStream_t s);
+hipError_t launch_f64_to_f32(int64_t n, const double *a, float *b, hipStream_t s);
+hipError_t launch_accum_correction(int n, double *x, float *xc, hipStream_t s);
+hipError_t launch_axpby(int n, double a, const double *x, double b, double *y, hipStream_t s);
+hipError_t launch_cg_scalar(int step, const double *p0, int np0, const double *p1, int np1,
+                            CGState *st, double *hist, hipStream_t s);
+hipError_t launch_cg_reliable(const double *p0, int np, CGState *st, double *hist,
+                              hipStream_t s);
+
+}  // namespace gsx

@@ -1,0 +1,1198 @@
+// Host runtime of libgsx: the C ABI declared in include/gsx.h.
+//
+// Owns device memory, builds the DOF mapper (host, integer), uploads the
+// patches, drives the assembly kernels and the CG / mixed-precision CG
+// iterations.  All device work of a context runs on its one HIP stream.
+#include "gsx.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "gsx_common.h"
+#include "gsx_kernels.h"
+
+using gsx::CGState;
+using gsx::GaussTab;
+using gsx::PatchDev;
+
+namespace {
+
+using DevPtr = std::shared_ptr<void>;
+
+struct Counter {
+    std::atomic<int64_t> bytes{0};
+};
+
+}  // namespace
+
+struct gsx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    bool timing = false;
+    std::string timing_filter;  // empty: time every scope
+    struct Rec {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, std::pair<int64_t, double>> stats;
+    std::shared_ptr<Counter> mem = std::make_shared<Counter>();
+    GaussTab gt;
+};
+
+namespace {
+
+gsx_status fail(gsx_ctx *c, gsx_status s, const std::string &msg)
+{
+    if (c) c->err = msg;
+    return s;
+}
+
+#define HIP_OK(ctx, expr)                                                                      \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess)                                                                  \
+            return fail((ctx), GSX_ERR_HIP,                                                    \
+                        std::string(#expr) + ": " + hipGetErrorString(_e));                   \
+    } while (0)
+
+#define GSX_TRY(expr)                   \
+    do {                                \
+        gsx_status _s = (expr);         \
+        if (_s != GSX_OK) return _s;    \
+    } while (0)
+
+DevPtr dalloc(gsx_ctx *c, size_t bytes)
+{
+    void *p = nullptr;
+    if (bytes == 0) bytes = 16;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    auto mem = c->mem;
+    mem->bytes += (int64_t)bytes;
+    return DevPtr(p, [mem, bytes](void *q) {
+        (void)hipFree(q);
+        mem->bytes -= (int64_t)bytes;
+    });
+}
+
+template <typename T>
+T *dp(const DevPtr &p) { return static_cast<T *>(p.get()); }
+
+// Kernel timing scope: events on the context stream when timing is on.
+struct KScope {
+    gsx_ctx *c;
+    const char *name;
+    hipEvent_t a = nullptr, b = nullptr;
+    bool on = false;
+    KScope(gsx_ctx *c_, const char *n) : c(c_), name(n)
+    {
+        on = c->timing && (c->timing_filter.empty() || c->timing_filter == n);
+        if (!on) return;
+        a = get();
+        b = get();
+        (void)hipEventRecord(a, c->stream);
+    }
+    ~KScope()
+    {
+        if (!on) return;
+        (void)hipEventRecord(b, c->stream);
+        c->recs.push_back({name, a, b});
+    }
+    hipEvent_t get()
+    {
+        if (!c->pool.empty()) {
+            hipEvent_t e = c->pool.back();
+            c->pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        return e;
+    }
+};
+
+void resolve_timing(gsx_ctx *c)
+{
+    for (auto &r : c->recs) {
+        (void)hipEventSynchronize(r.b);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, r.a, r.b);
+        auto &s = c->stats[r.name];
+        s.first += 1;
+        s.second += (double)ms;
+        c->pool.push_back(r.a);
+        c->pool.push_back(r.b);
+    }
+    c->recs.clear();
+}
+
+// Gauss-Legendre nodes by Newton's method on P_n in long double, ascending,
+// symmetric (the values of the 30-digit tables of gsGaussRule::
+// lookupReference, src/gsAssembler/gsGaussRule.hpp:218-547, rounded).
+void gauss_rule(int n, double *x, double *w)
+{
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int i = 0; i < (n + 1) / 2; ++i) {
+        long double z = cosl(pi * ((long double)i + 0.75L) / ((long double)n + 0.5L));
+        long double dp = 0;
+        for (int it = 0; it < 100; ++it) {
+            long double p0 = 1.0L, p1 = 0.0L;
+            for (int k = 1; k <= n; ++k) {
+                const long double p2 = p1;
+                p1 = p0;
+                p0 = ((2.0L * k - 1.0L) * z * p1 - (k - 1.0L) * p2) / (long double)k;
+            }
+            dp = (long double)n * (z * p0 - p1) / (z * z - 1.0L);
+            const long double dz = p0 / dp;
+            z -= dz;
+            if (fabsl(dz) < 1e-30L) break;
+        }
+        long double p0 = 1.0L, p1 = 0.0L;
+        for (int k = 1; k <= n; ++k) {
+            const long double p2 = p1;
+            p1 = p0;
+            p0 = ((2.0L * k - 1.0L) * z * p1 - (k - 1.0L) * p2) / (long double)k;
+        }
+        dp = (long double)n * (z * p0 - p1) / (z * z - 1.0L);
+        const long double wi = 2.0L / ((1.0L - z * z) * dp * dp);
+        if (n % 2 == 1 && i == n / 2) z = 0.0L;
+        x[n - 1 - i] = (double)z;
+        w[n - 1 - i] = (double)wi;
+        x[i] = -(double)z;
+        w[i] = (double)wi;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// problem
+// ---------------------------------------------------------------------------
+struct HostPatch {
+    int d;
+    int p[3];
+    int nb[3];
+    int nel[3];
+    int64_t nbasis, nelem;
+};
+
+struct Group {
+    int d, p;
+    std::vector<int> patches;
+    std::vector<int64_t> prefix;
+    int64_t nelem = 0;
+    DevPtr d_gpatch, d_gprefix;
+};
+
+struct gsx_problem {
+    gsx_ctx *ctx = nullptr;
+    int d = 0, npatch = 0;
+    std::vector<HostPatch> hp;
+    std::vector<int64_t> boff;     // basis offset per patch
+    std::vector<int> dofs;         // concatenated global index
+    int nfree = 0;
+    int64_t nbasis = 0, nelem = 0, nglued = 0;
+    std::vector<PatchDev> pdev;
+    DevPtr d_patches, d_dofs;
+    std::vector<DevPtr> keep;      // knots, ctrl, kappa, felem
+    DevPtr d_occ_ptr, d_occ_patch, d_occ_local;
+    std::vector<Group> groups;
+    // pattern cache
+    bool have_pattern = false;
+    DevPtr d_rowptr, d_col;
+    int64_t nnz = 0;
+    int maxrow = 0;
+    // element scratch
+    DevPtr d_K, d_F, d_detmin, d_scan_bsum, d_scan_total;
+    int64_t kdoubles = 0, fdoubles = 0;
+    double min_det = 0.0;
+};
+
+// DOF numbering with the rules of gsDofMapper::finalizeComp
+// (src/gsCore/gsDofMapper.cpp:271-340): standard free DOFs (never matched,
+// not eliminated) in (patch, local) scan order, then the coupled free DOFs
+// by first appearance, then the eliminated ones by first appearance.  Groups
+// of matched DOFs are the connected components of the interface matching
+// (gsDofMapper::matchDof/mergeDofsGlobally, gsDofMapper.cpp:99-157,445-478);
+// a group is eliminated if any member lies on a Dirichlet side.
+static int build_mapper(gsx_problem *P, int ndir, const int32_t *dir_sides, int nif,
+                        const int32_t *ifaces, std::string &err)
+{
+    const int64_t total = P->nbasis;
+    std::vector<int64_t> parent(total);
+    std::iota(parent.begin(), parent.end(), 0);
+    std::vector<char> matched(total, 0), elim(total, 0);
+    auto find = [&](int64_t a) {
+        while (parent[a] != a) {
+            parent[a] = parent[parent[a]];
+            a = parent[a];
+        }
+        return a;
+    };
+    auto side_list = [&](int k, int side, std::vector<int64_t> &out) {
+        const HostPatch &h = P->hp[k];
+        const int dir = (side - 1) / 2, par = (side - 1) % 2;
+        const int fix = par ? h.nb[dir] - 1 : 0;
+        int64_t tot = 1;
+        for (int j = 0; j < h.d; ++j)
+            if (j != dir) tot *= h.nb[j];
+        out.clear();
+        for (int64_t t = 0; t < tot; ++t) {
+            int idx[3] = {0, 0, 0};
+            int64_t r = t;
+            for (int j = 0; j < h.d; ++j) {
+                if (j == dir) { idx[j] = fix; continue; }
+                idx[j] = (int)(r % h.nb[j]);
+                r /= h.nb[j];
+            }
+            out.push_back(idx[0] + (int64_t)h.nb[0] * (idx[1] + (int64_t)h.nb[1] * idx[2]));
+        }
+    };
+    std::vector<int64_t> lst;
+    for (int f = 0; f < nif; ++f) {
+        const int32_t *I = ifaces + 8 * f;
+        const int p1 = I[0], s1 = I[1], p2 = I[2], s2 = I[3];
+        if (p1 < 0 || p1 >= P->npatch || p2 < 0 || p2 >= P->npatch || s1 < 1 || s1 > 2 * P->d ||
+            s2 < 1 || s2 > 2 * P->d) {
+            err = "invalid interface record";
+            return -1;
+        }
+        const HostPatch &A = P->hp[p1], &B = P->hp[p2];
+        const int dir1 = (s1 - 1) / 2, dir2 = (s2 - 1) / 2, par2 = (s2 - 1) % 2;
+        side_list(p1, s1, lst);
+        for (int64_t la : lst) {
+            int ia[3] = {(int)(la % A.nb[0]), (int)((la / A.nb[0]) % A.nb[1]),
+                         A.d == 3 ? (int)(la / ((int64_t)A.nb[0] * A.nb[1])) : 0};
+            int ib[3] = {0, 0, 0};
+            ib[dir2] = par2 ? B.nb[dir2] - 1 : 0;
+            int j = 0;
+            for (int k = 0; k < A.d; ++k) {
+                if (k == dir1) continue;
+                const int m = I[4 + 2 * j], flip = I[5 + 2 * j];
+                if (m < 0 || m >= B.d || A.nb[k] != B.nb[m]) {
+                    err = "non-conforming interface";
+                    return -1;
+                }
+                ib[m] = flip ? (B.nb[m] - 1 - ia[k]) : ia[k];
+                ++j;
+            }
+            const int64_t lb = ib[0] + (int64_t)B.nb[0] * (ib[1] + (int64_t)B.nb[1] * ib[2]);
+            const int64_t ga = P->boff[p1] + la, gb = P->boff[p2] + lb;
+            matched[ga] = matched[gb] = 1;
+            const int64_t ra = find(ga), rb = find(gb);
+            if (ra != rb) parent[std::max(ra, rb)] = std::min(ra, rb);
+        }
+    }
+    for (int s = 0; s < ndir; ++s) {
+        const int k = dir_sides[2 * s], side = dir_sides[2 * s + 1];
+        if (k < 0 || k >= P->npatch || side < 1 || side > 2 * P->d) {
+            err = "invalid Dirichlet side";
+            return -1;
+        }
+        side_list(k, side, lst);
+        for (int64_t l : lst) elim[P->boff[k] + l] = 1;
+    }
+    std::vector<char> comp_elim(total, 0);
+    for (int64_t i = 0; i < total; ++i)
+        if (elim[i]) comp_elim[find(i)] = 1;
+    int64_t nstd = 0, ncpl = 0;
+    std::vector<int64_t> comp_id(total, -1);
+    for (int64_t i = 0; i < total; ++i) {
+        const int64_t r = find(i);
+        if (comp_elim[r]) continue;
+        if (!matched[i]) ++nstd;
+        else if (comp_id[r] < 0) { comp_id[r] = 0; ++ncpl; }
+    }
+    std::fill(comp_id.begin(), comp_id.end(), -1);
+    const int64_t nfree = nstd + ncpl;
+    if (nfree >= (int64_t)INT32_MAX) {
+        err = "too many DOFs for int32 indices";
+        return -1;
+    }
+    int64_t cur_std = 0, cur_cpl = nstd, cur_elim = nfree;
+    P->dofs.assign(total, -1);
+    for (int64_t i = 0; i < total; ++i) {
+        const int64_t r = find(i);
+        if (comp_elim[r]) {
+            if (comp_id[r] < 0) comp_id[r] = cur_elim++;
+            P->dofs[i] = (int)comp_id[r];
+        } else if (!matched[i]) {
+            P->dofs[i] = (int)cur_std++;
+        } else {
+            if (comp_id[r] < 0) comp_id[r] = cur_cpl++;
+            P->dofs[i] = (int)comp_id[r];
+        }
+    }
+    P->nglued = cur_elim;
+    return (int)nfree;
+}
+
+extern "C" {
+
+gsx_status gsx_ctx_create(int32_t device, gsx_ctx **out)
+{
+    if (!out) return GSX_ERR_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        return GSX_ERR_NO_DEVICE;
+    }
+    if (device < 0 || device >= n) return GSX_ERR_ARG;
+    gsx_ctx *c = new gsx_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return GSX_ERR_HIP;
+    }
+    std::memset(&c->gt, 0, sizeof(c->gt));
+    for (int q = 1; q <= gsx::kMaxQ1; ++q) gauss_rule(q, c->gt.x[q], c->gt.w[q]);
+    *out = c;
+    return GSX_OK;
+}
+
+gsx_status gsx_ctx_destroy(gsx_ctx *c)
+{
+    if (!c) return GSX_ERR_ARG;
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &r : c->recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    for (auto e : c->pool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return GSX_OK;
+}
+
+const char *gsx_last_error(const gsx_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+gsx_status gsx_ctx_synchronize(gsx_ctx *c)
+{
+    if (!c) return GSX_ERR_ARG;
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return GSX_OK;
+}
+
+gsx_status gsx_ctx_enable_timing(gsx_ctx *c, int32_t enable)
+{
+    if (!c) return GSX_ERR_ARG;
+    c->timing = enable != 0;
+    return GSX_OK;
+}
+
+gsx_status gsx_ctx_timing_filter(gsx_ctx *c, const char *name)
+{
+    if (!c) return GSX_ERR_ARG;
+    c->timing_filter = name ? name : "";
+    return GSX_OK;
+}
+
+gsx_status gsx_ctx_reset_timing(gsx_ctx *c)
+{
+    if (!c) return GSX_ERR_ARG;
+    resolve_timing(c);
+    c->stats.clear();
+    return GSX_OK;
+}
+
+gsx_status gsx_ctx_kernel_stats(gsx_ctx *c, const char *name, int64_t *launches, double *total_ms)
+{
+    if (!c || !name) return GSX_ERR_ARG;
+    resolve_timing(c);
+    auto it = c->stats.find(name);
+    if (launches) *launches = it == c->stats.end() ? 0 : it->second.first;
+    if (total_ms) *total_ms = it == c->stats.end() ? 0.0 : it->second.second;
+    return GSX_OK;
+}
+
+gsx_status gsx_device_alloc(gsx_ctx *c, int64_t bytes, void **out)
+{
+    if (!c || !out || bytes < 0) return GSX_ERR_ARG;
+    HIP_OK(c, hipMalloc(out, (size_t)(bytes ? bytes : 16)));
+    c->mem->bytes += bytes ? bytes : 16;
+    return GSX_OK;
+}
+
+gsx_status gsx_device_free(gsx_ctx *c, void *p)
+{
+    if (!c) return GSX_ERR_ARG;
+    if (!p) return GSX_OK;
+    size_t sz = 0;
+    (void)hipMemPtrGetInfo(p, &sz);
+    HIP_OK(c, hipFree(p));
+    c->mem->bytes -= (int64_t)sz;
+    return GSX_OK;
+}
+
+gsx_status gsx_copy_to_device(gsx_ctx *c, void *dev, const void *host, int64_t bytes)
+{
+    if (!c || (bytes && (!dev || !host))) return GSX_ERR_ARG;
+    HIP_OK(c, hipMemcpyAsync(dev, host, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return GSX_OK;
+}
+
+gsx_status gsx_copy_to_host(gsx_ctx *c, void *host, const void *dev, int64_t bytes)
+{
+    if (!c || (bytes && (!dev || !host))) return GSX_ERR_ARG;
+    HIP_OK(c, hipMemcpyAsync(host, dev, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return GSX_OK;
+}
+
+gsx_status gsx_ctx_memory(gsx_ctx *c, int64_t *bytes)
+{
+    if (!c || !bytes) return GSX_ERR_ARG;
+    *bytes = c->mem->bytes;
+    return GSX_OK;
+}
+
+// ---------------------------------------------------------------------------
+gsx_status gsx_problem_create(gsx_ctx *c, int32_t npatch, const gsx_patch_desc *patches,
+                              int32_t ndir, const int32_t *dir_sides, int32_t nif,
+                              const int32_t *ifaces, gsx_problem **out)
+{
+    if (!c || !patches || npatch <= 0 || !out || (ndir > 0 && !dir_sides) || (nif > 0 && !ifaces))
+        return fail(c, GSX_ERR_ARG, "gsx_problem_create: null argument");
+    auto P = std::make_unique<gsx_problem>();
+    P->ctx = c;
+    P->npatch = npatch;
+    P->d = patches[0].d;
+    if (P->d != 2 && P->d != 3) return fail(c, GSX_ERR_ARG, "dimension must be 2 or 3");
+    int64_t boff = 0, eoff = 0;
+    for (int k = 0; k < npatch; ++k) {
+        const gsx_patch_desc &D = patches[k];
+        if (D.d != P->d) return fail(c, GSX_ERR_ARG, "all patches must share the dimension");
+        HostPatch h{};
+        h.d = D.d;
+        h.nbasis = 1;
+        h.nelem = 1;
+        for (int j = 0; j < D.d; ++j) {
+            const int p = D.p[j], nk = D.nknots[j];
+            const double *U = D.knots_host[j];
+            if (p < 1 || p > gsx::kMaxDeg) return fail(c, GSX_ERR_ARG, "degree must be in 1..4");
+            if (p != D.p[0]) return fail(c, GSX_ERR_ARG, "degree must be equal in all directions");
+            if (!U || nk < 2 * p + 2) return fail(c, GSX_ERR_ARG, "knot vector too short");
+            for (int i = 0; i < p; ++i)
+                if (U[i] != U[p] || U[nk - 1 - i] != U[nk - 1 - p])
+                    return fail(c, GSX_ERR_ARG, "knot vectors must be open (clamped)");
+            for (int i = p; i < nk - p - 1; ++i)
+                if (!(U[i + 1] > U[i]))
+                    return fail(c, GSX_ERR_ARG, "interior knots must be strictly increasing");
+            h.p[j] = p;
+            h.nb[j] = nk - p - 1;
+            h.nel[j] = nk - 2 * p - 1;
+            h.nbasis *= h.nb[j];
+            h.nelem *= h.nel[j];
+        }
+        for (int j = D.d; j < 3; ++j) { h.p[j] = 0; h.nb[j] = 1; h.nel[j] = 1; }
+        if (!D.ctrl_host) return fail(c, GSX_ERR_ARG, "missing control points");
+        if (D.fkind == 0 && !D.felem_host) return fail(c, GSX_ERR_ARG, "fkind 0 needs felem");
+        P->hp.push_back(h);
+        P->boff.push_back(boff);
+        boff += h.nbasis;
+        eoff += h.nelem;
+    }
+    P->nbasis = boff;
+    P->nelem = eoff;
+    std::string err;
+    const int nfree = build_mapper(P.get(), ndir, dir_sides, nif, ifaces, err);
+    if (nfree < 0) return fail(c, GSX_ERR_ARG, err);
+    P->nfree = nfree;
+
+    // upload
+    hipStream_t s = c->stream;
+    P->d_dofs = dalloc(c, sizeof(int) * P->nbasis);
+    if (!P->d_dofs) return fail(c, GSX_ERR_OOM, "device allocation (dofs)");
+    HIP_OK(c, hipMemcpyAsync(P->d_dofs.get(), P->dofs.data(), sizeof(int) * P->nbasis,
+                             hipMemcpyHostToDevice, s));
+    eoff = 0;
+    int64_t kdoubles = 0, fdoubles = 0;
+    for (int k = 0; k < npatch; ++k) {
+        const gsx_patch_desc &D = patches[k];
+        const HostPatch &h = P->hp[k];
+        PatchDev pd{};
+        pd.d = h.d;
+        for (int j = 0; j < 3; ++j) {
+            pd.p[j] = h.p[j];
+            pd.nb[j] = h.nb[j];
+            pd.nel[j] = h.nel[j];
+            pd.knots[j] = nullptr;
+        }
+        for (int j = 0; j < h.d; ++j) {
+            DevPtr kb = dalloc(c, sizeof(double) * D.nknots[j]);
+            if (!kb) return fail(c, GSX_ERR_OOM, "device allocation (knots)");
+            HIP_OK(c, hipMemcpyAsync(kb.get(), D.knots_host[j], sizeof(double) * D.nknots[j],
+                                     hipMemcpyHostToDevice, s));
+            pd.knots[j] = dp<double>(kb);
+            P->keep.push_back(kb);
+        }
+        DevPtr cb = dalloc(c, sizeof(double) * h.nbasis * h.d);
+        if (!cb) return fail(c, GSX_ERR_OOM, "device allocation (ctrl)");
+        HIP_OK(c, hipMemcpyAsync(cb.get(), D.ctrl_host, sizeof(double) * h.nbasis * h.d,
+                                 hipMemcpyHostToDevice, s));
+        pd.ctrl = dp<double>(cb);
+        P->keep.push_back(cb);
+        pd.kappa = nullptr;
+        pd.felem = nullptr;
+        if (D.kappa_host) {
+            DevPtr b = dalloc(c, sizeof(double) * h.nelem);
+            if (!b) return fail(c, GSX_ERR_OOM, "device allocation (kappa)");
+            HIP_OK(c, hipMemcpyAsync(b.get(), D.kappa_host, sizeof(double) * h.nelem,
+                                     hipMemcpyHostToDevice, s));
+            pd.kappa = dp<double>(b);
+            P->keep.push_back(b);
+        }
+        if (D.felem_host && D.fkind == 0) {
+            DevPtr b = dalloc(c, sizeof(double) * h.nelem);
+            if (!b) return fail(c, GSX_ERR_OOM, "device allocation (felem)");
+            HIP_OK(c, hipMemcpyAsync(b.get(), D.felem_host, sizeof(double) * h.nelem,
+                                     hipMemcpyHostToDevice, s));
+            pd.felem = dp<double>(b);
+            P->keep.push_back(b);
+        }
+        pd.fkind = D.fkind;
+        pd.dofs = dp<int>(P->d_dofs) + P->boff[k];
+        pd.nelem = h.nelem;
+        pd.elem_off = eoff;
+        int nb1 = h.p[0] + 1;
+        int NB = (h.d == 2) ? nb1 * nb1 : nb1 * nb1 * nb1;
+        pd.kmat_off = kdoubles;
+        pd.fvec_off = fdoubles;
+        kdoubles += h.nelem * (int64_t)NB * NB;
+        fdoubles += h.nelem * (int64_t)NB;
+        eoff += h.nelem;
+        P->pdev.push_back(pd);
+    }
+    P->kdoubles = kdoubles;
+    P->fdoubles = fdoubles;
+    P->d_patches = dalloc(c, sizeof(PatchDev) * npatch);
+    if (!P->d_patches) return fail(c, GSX_ERR_OOM, "device allocation (patches)");
+    HIP_OK(c, hipMemcpyAsync(P->d_patches.get(), P->pdev.data(), sizeof(PatchDev) * npatch,
+                             hipMemcpyHostToDevice, s));
+
+    // groups by (d, p)
+    for (int k = 0; k < npatch; ++k) {
+        const HostPatch &h = P->hp[k];
+        Group *g = nullptr;
+        for (auto &gg : P->groups)
+            if (gg.d == h.d && gg.p == h.p[0]) g = &gg;
+        if (!g) {
+            P->groups.push_back(Group());
+            g = &P->groups.back();
+            g->d = h.d;
+            g->p = h.p[0];
+        }
+        g->patches.push_back(k);
+    }
+    for (auto &g : P->groups) {
+        g.prefix.assign(1, 0);
+        for (int k : g.patches) g.prefix.push_back(g.prefix.back() + P->hp[k].nelem);
+        g.nelem = g.prefix.back();
+        g.d_gpatch = dalloc(c, sizeof(int) * g.patches.size());
+        g.d_gprefix = dalloc(c, sizeof(int64_t) * g.prefix.size());
+        if (!g.d_gpatch || !g.d_gprefix) return fail(c, GSX_ERR_OOM, "device allocation (groups)");
+        HIP_OK(c, hipMemcpyAsync(g.d_gpatch.get(), g.patches.data(), sizeof(int) * g.patches.size(),
+                                 hipMemcpyHostToDevice, s));
+        HIP_OK(c, hipMemcpyAsync(g.d_gprefix.get(), g.prefix.data(), sizeof(int64_t) * g.prefix.size(),
+                                 hipMemcpyHostToDevice, s));
+    }
+
+    // occurrences of every free DOF (patch order, then local order)
+    std::vector<int> occ_cnt(nfree + 1, 0);
+    for (int64_t i = 0; i < P->nbasis; ++i)
+        if (P->dofs[i] < nfree) occ_cnt[P->dofs[i] + 1]++;
+    for (int r = 0; r < nfree; ++r) occ_cnt[r + 1] += occ_cnt[r];
+    std::vector<int> occ_patch(occ_cnt[nfree]);
+    std::vector<int64_t> occ_local(occ_cnt[nfree]);
+    std::vector<int> fillp(occ_cnt.begin(), occ_cnt.end() - 1);
+    for (int k = 0; k < npatch; ++k)
+        for (int64_t l = 0; l < P->hp[k].nbasis; ++l) {
+            const int g = P->dofs[P->boff[k] + l];
+            if (g < nfree) {
+                const int o = fillp[g]++;
+                occ_patch[o] = k;
+                occ_local[o] = l;
+            }
+        }
+    P->d_occ_ptr = dalloc(c, sizeof(int) * (nfree + 1));
+    P->d_occ_patch = dalloc(c, sizeof(int) * std::max<size_t>(1, occ_patch.size()));
+    P->d_occ_local = dalloc(c, sizeof(int64_t) * std::max<size_t>(1, occ_local.size()));
+    if (!P->d_occ_ptr || !P->d_occ_patch || !P->d_occ_local)
+        return fail(c, GSX_ERR_OOM, "device allocation (occurrences)");
+    HIP_OK(c, hipMemcpyAsync(P->d_occ_ptr.get(), occ_cnt.data(), sizeof(int) * (nfree + 1),
+                             hipMemcpyHostToDevice, s));
+    if (!occ_patch.empty()) {
+        HIP_OK(c, hipMemcpyAsync(P->d_occ_patch.get(), occ_patch.data(), sizeof(int) * occ_patch.size(),
+                                 hipMemcpyHostToDevice, s));
+        HIP_OK(c, hipMemcpyAsync(P->d_occ_local.get(), occ_local.data(),
+                                 sizeof(int64_t) * occ_local.size(), hipMemcpyHostToDevice, s));
+    }
+    HIP_OK(c, hipStreamSynchronize(s));
+    *out = P.release();
+    return GSX_OK;
+}
+
+gsx_status gsx_problem_destroy(gsx_problem *P)
+{
+    if (!P) return GSX_ERR_ARG;
+    (void)hipStreamSynchronize(P->ctx->stream);
+    delete P;
+    return GSX_OK;
+}
+
+gsx_status gsx_problem_info(const gsx_problem *P, int64_t *nbasis, int64_t *nfree, int64_t *nelem,
+                            int64_t *nglued)
+{
+    if (!P) return GSX_ERR_ARG;
+    if (nbasis) *nbasis = P->nbasis;
+    if (nfree) *nfree = P->nfree;
+    if (nelem) *nelem = P->nelem;
+    if (nglued) *nglued = P->nglued;
+    return GSX_OK;
+}
+
+gsx_status gsx_problem_dofs(const gsx_problem *P, int32_t *dofs)
+{
+    if (!P || !dofs) return GSX_ERR_ARG;
+    std::memcpy(dofs, P->dofs.data(), sizeof(int) * P->nbasis);
+    return GSX_OK;
+}
+
+gsx_status gsx_problem_min_detj(const gsx_problem *P, double *m)
+{
+    if (!P || !m) return GSX_ERR_ARG;
+    *m = P->min_det;
+    return GSX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// matrix + solver workspace
+// ---------------------------------------------------------------------------
+struct CGWork {
+    int n = 0;
+    int nparts = 0;
+    DevPtr r, z, p, t;           // FP64 vectors
+    DevPtr parts;                // 4 partial arrays
+    DevPtr state, hist;
+    int hist_cap = 0;
+    // mixed
+    DevPtr r32, z32, p32, t32, xc32, diag32;
+};
+
+struct gsx_matrix {
+    gsx_ctx *ctx = nullptr;
+    int n = 0;
+    int64_t nnz = 0;
+    int lpr = 8;
+    DevPtr rowptr, col, val, val32, diag;
+    CGWork ws;
+    std::vector<double> hist;
+};
+
+static gsx_status pattern_build(gsx_problem *P)
+{
+    gsx_ctx *c = P->ctx;
+    hipStream_t s = c->stream;
+    const int n = P->nfree;
+    DevPtr counts = dalloc(c, sizeof(int) * (n + 1));
+    DevPtr errf = dalloc(c, sizeof(int));
+    P->d_rowptr = dalloc(c, sizeof(int) * (n + 1));
+    const int nblk = (n + 1023) / 1024 + 1;
+    P->d_scan_bsum = dalloc(c, sizeof(long long) * nblk);
+    P->d_scan_total = dalloc(c, sizeof(long long));
+    if (!counts || !errf || !P->d_rowptr || !P->d_scan_bsum || !P->d_scan_total)
+        return fail(c, GSX_ERR_OOM, "device allocation (pattern)");
+    HIP_OK(c, hipMemsetAsync(errf.get(), 0, sizeof(int), s));
+    {
+        KScope k(c, "pattern_count");
+        HIP_OK(c, gsx::launch_rowcount(dp<PatchDev>(P->d_patches), dp<int>(P->d_occ_ptr),
+                                       dp<int>(P->d_occ_patch), dp<int64_t>(P->d_occ_local), n,
+                                       dp<int>(counts), dp<int>(errf), s));
+    }
+    {
+        KScope k(c, "pattern_scan");
+        HIP_OK(c, gsx::launch_scan(dp<int>(counts), n, dp<int>(P->d_rowptr),
+                                   dp<long long>(P->d_scan_bsum), dp<long long>(P->d_scan_total), s));
+    }
+    int errh = 0;
+    long long total = 0;
+    HIP_OK(c, hipMemcpyAsync(&errh, errf.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(c, hipMemcpyAsync(&total, P->d_scan_total.get(), sizeof(long long), hipMemcpyDeviceToHost, s));
+    HIP_OK(c, hipStreamSynchronize(s));
+    if (errh) return fail(c, GSX_ERR_ARG, "row of an interface DOF exceeds the pattern buffer");
+    if (total >= (long long)INT32_MAX) return fail(c, GSX_ERR_ARG, "nnz exceeds int32 indexing");
+    // max row length
+    std::vector<int> cnt(n);
+    HIP_OK(c, hipMemcpyAsync(cnt.data(), counts.get(), sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HIP_OK(c, hipStreamSynchronize(s));
+    P->maxrow = n ? *std::max_element(cnt.begin(), cnt.end()) : 0;
+    P->nnz = total;
+    P->d_col = dalloc(c, sizeof(int) * std::max<long long>(1, total));
+    if (!P->d_col) return fail(c, GSX_ERR_OOM, "device allocation (col)");
+    {
+        KScope k(c, "pattern_fill");
+        HIP_OK(c, gsx::launch_rowfill(dp<PatchDev>(P->d_patches), dp<int>(P->d_occ_ptr),
+                                      dp<int>(P->d_occ_patch), dp<int64_t>(P->d_occ_local), n,
+                                      dp<int>(P->d_rowptr), dp<int>(P->d_col), s));
+    }
+    P->have_pattern = true;
+    return GSX_OK;
+}
+
+extern "C" {
+
+gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_matrix **A_out,
+                        double *rhs_dev)
+{
+    if (!P || !A_out) return GSX_ERR_ARG;
+    gsx_ctx *c = P->ctx;
+    if ((mode & ~GSX_ASM_REBUILD_PATTERN) != GSX_PATTERN_EXPR &&
+        (mode & ~GSX_ASM_REBUILD_PATTERN) != GSX_PATTERN_FULL)
+        return fail(c, GSX_ERR_ARG, "unknown pattern mode");
+    if (precision != GSX_FP64 && precision != GSX_MIXED)
+        return fail(c, GSX_ERR_ARG, "unknown precision");
+    hipStream_t s = c->stream;
+    const bool rebuild = (mode & GSX_ASM_REBUILD_PATTERN) != 0;
+    mode &= ~GSX_ASM_REBUILD_PATTERN;
+    if (!P->have_pattern || rebuild) {
+        gsx_status st = pattern_build(P);
+        if (st != GSX_OK) return st;
+    }
+    if (gsx::gather_maxrow_bucket(P->maxrow) < 0)
+        return fail(c, GSX_ERR_ARG, "rows longer than 1024 entries are not supported");
+    if (!P->d_K) {
+        P->d_K = dalloc(c, sizeof(double) * std::max<int64_t>(1, P->kdoubles));
+        P->d_F = dalloc(c, sizeof(double) * std::max<int64_t>(1, P->fdoubles));
+        P->d_detmin = dalloc(c, sizeof(double) * std::max<int64_t>(1, P->nelem) + 64);
+        if (!P->d_K || !P->d_F || !P->d_detmin)
+            return fail(c, GSX_ERR_OOM, "device allocation (element matrices)");
+    }
+    for (auto &g : P->groups) {
+        KScope k(c, g.d == 3 ? "elem3d" : "elem2d");
+        HIP_OK(c, gsx::launch_elem(g.d, g.p, dp<PatchDev>(P->d_patches), dp<int>(g.d_gpatch),
+                                   dp<int64_t>(g.d_gprefix), (int)g.patches.size(), g.nelem, c->gt,
+                                   dp<double>(P->d_K), dp<double>(P->d_F), dp<double>(P->d_detmin), s));
+    }
+    double *dmin_slot = dp<double>(P->d_detmin) + std::max<int64_t>(1, P->nelem);
+    HIP_OK(c, gsx::launch_min(dp<double>(P->d_detmin), P->nelem, dmin_slot, s));
+
+    auto A = std::make_unique<gsx_matrix>();
+    A->ctx = c;
+    A->n = P->nfree;
+    A->nnz = P->nnz;
+    A->rowptr = P->d_rowptr;
+    A->col = P->d_col;
+    A->val = dalloc(c, sizeof(double) * std::max<int64_t>(1, P->nnz));
+    DevPtr rhs_tmp;
+    double *rhs = rhs_dev;
+    if (!rhs) {
+        rhs_tmp = dalloc(c, sizeof(double) * std::max(1, P->nfree));
+        rhs = dp<double>(rhs_tmp);
+    }
+    DevPtr keep;
+    if (mode == GSX_PATTERN_EXPR) keep = dalloc(c, std::max<int64_t>(1, P->nnz));
+    if (!A->val || !rhs || (mode == GSX_PATTERN_EXPR && !keep))
+        return fail(c, GSX_ERR_OOM, "device allocation (values)");
+    {
+        KScope k(c, "gather");
+        HIP_OK(c, gsx::launch_gather(dp<PatchDev>(P->d_patches), dp<int>(P->d_occ_ptr),
+                                     dp<int>(P->d_occ_patch), dp<int64_t>(P->d_occ_local), P->nfree,
+                                     dp<int>(P->d_rowptr), dp<int>(P->d_col), dp<double>(P->d_K),
+                                     dp<double>(P->d_F), dp<double>(A->val), rhs,
+                                     keep ? dp<unsigned char>(keep) : nullptr, P->maxrow, s));
+    }
+    double mind = 0.0;
+    HIP_OK(c, hipMemcpyAsync(&mind, dmin_slot, sizeof(double), hipMemcpyDeviceToHost, s));
+    if (mode == GSX_PATTERN_EXPR) {
+        DevPtr dropped = dalloc(c, sizeof(unsigned long long));
+        if (!dropped) return fail(c, GSX_ERR_OOM, "device allocation");
+        HIP_OK(c, hipMemsetAsync(dropped.get(), 0, sizeof(unsigned long long), s));
+        HIP_OK(c, gsx::launch_keep(nullptr, nullptr, nullptr, dp<unsigned char>(keep), P->nfree, P->nnz,
+                                   nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                   dp<unsigned long long>(dropped), 0, s));
+        unsigned long long ndrop = 0;
+        HIP_OK(c, hipMemcpyAsync(&ndrop, dropped.get(), sizeof(ndrop), hipMemcpyDeviceToHost, s));
+        HIP_OK(c, hipStreamSynchronize(s));
+        if (ndrop > 0) {
+            // compaction: entries whose every element contribution was an
+            // exact zero are not part of the gsExprAssembler pattern
+            const int n = P->nfree;
+            DevPtr counts = dalloc(c, sizeof(int) * (n + 1));
+            DevPtr nrp = dalloc(c, sizeof(int) * (n + 1));
+            const int64_t nn = P->nnz - (int64_t)ndrop;
+            DevPtr ncol = dalloc(c, sizeof(int) * std::max<int64_t>(1, nn));
+            DevPtr nval = dalloc(c, sizeof(double) * std::max<int64_t>(1, nn));
+            if (!counts || !nrp || !ncol || !nval) return fail(c, GSX_ERR_OOM, "device allocation");
+            HIP_OK(c, gsx::launch_keep(dp<int>(A->rowptr), dp<int>(A->col), dp<double>(A->val),
+                                       dp<unsigned char>(keep), n, P->nnz, dp<int>(counts), dp<int>(nrp),
+                                       dp<int>(ncol), dp<double>(nval), dp<long long>(P->d_scan_bsum),
+                                       dp<long long>(P->d_scan_total), nullptr, 1, s));
+            A->rowptr = nrp;
+            A->col = ncol;
+            A->val = nval;
+            A->nnz = nn;
+        }
+    }
+    HIP_OK(c, hipStreamSynchronize(s));
+    P->min_det = mind;
+    if (!(mind > 0.0)) {
+        char buf[128];
+        std::snprintf(buf, sizeof(buf), "non-positive Jacobian determinant (min det J = %.6e)", mind);
+        return fail(c, GSX_ERR_JACOBIAN, buf);
+    }
+    A->lpr = gsx::spmv_lpr(A->n ? (double)A->nnz / A->n : 1.0);
+    if (precision == GSX_MIXED) {
+        A->val32 = dalloc(c, sizeof(float) * std::max<int64_t>(1, A->nnz));
+        if (!A->val32) return fail(c, GSX_ERR_OOM, "device allocation (fp32 values)");
+        KScope k(c, "convert_f32");
+        HIP_OK(c, gsx::launch_f64_to_f32(A->nnz, dp<double>(A->val), dp<float>(A->val32), s));
+    }
+    *A_out = A.release();
+    return GSX_OK;
+}
+
+gsx_status gsx_matrix_destroy(gsx_matrix *A)
+{
+    if (!A) return GSX_ERR_ARG;
+    (void)hipStreamSynchronize(A->ctx->stream);
+    delete A;
+    return GSX_OK;
+}
+
+gsx_status gsx_matrix_info(const gsx_matrix *A, int64_t *n, int64_t *nnz)
+{
+    if (!A) return GSX_ERR_ARG;
+    if (n) *n = A->n;
+    if (nnz) *nnz = A->nnz;
+    return GSX_OK;
+}
+
+gsx_status gsx_matrix_download(const gsx_matrix *A, int32_t *rp, int32_t *ci, double *v)
+{
+    if (!A) return GSX_ERR_ARG;
+    gsx_ctx *c = A->ctx;
+    hipStream_t s = c->stream;
+    if (rp) HIP_OK(c, hipMemcpyAsync(rp, A->rowptr.get(), sizeof(int) * (A->n + 1), hipMemcpyDeviceToHost, s));
+    if (ci) HIP_OK(c, hipMemcpyAsync(ci, A->col.get(), sizeof(int) * A->nnz, hipMemcpyDeviceToHost, s));
+    if (v) HIP_OK(c, hipMemcpyAsync(v, A->val.get(), sizeof(double) * A->nnz, hipMemcpyDeviceToHost, s));
+    HIP_OK(c, hipStreamSynchronize(s));
+    return GSX_OK;
+}
+
+gsx_status gsx_matrix_device_ptrs(const gsx_matrix *A, const int32_t **rp, const int32_t **ci,
+                                  const double **v)
+{
+    if (!A) return GSX_ERR_ARG;
+    if (rp) *rp = dp<int>(A->rowptr);
+    if (ci) *ci = dp<int>(A->col);
+    if (v) *v = dp<double>(A->val);
+    return GSX_OK;
+}
+
+gsx_status gsx_spmv(gsx_matrix *A, const double *x, double *y)
+{
+    if (!A || !x || !y) return GSX_ERR_ARG;
+    gsx_ctx *c = A->ctx;
+    KScope k(c, "spmv_f64");
+    HIP_OK(c, gsx::launch_spmv_f64(A->lpr, A->n, dp<int>(A->rowptr), dp<int>(A->col),
+                                   dp<double>(A->val), x, y, nullptr, nullptr, 0, nullptr, c->stream));
+    return GSX_OK;
+}
+
+}  // extern "C"
+
+static gsx_status cg_workspace(gsx_matrix *A, int maxit, bool mixed)
+{
+    gsx_ctx *c = A->ctx;
+    CGWork &w = A->ws;
+    const int n = A->n;
+    if (w.n != n || !w.r) {
+        w = CGWork();
+        w.n = n;
+        w.nparts = std::max(gsx::spmv_blocks(n, A->lpr), (n + 255) / 256) + 1;
+        w.r = dalloc(c, sizeof(double) * n);
+        w.z = dalloc(c, sizeof(double) * n);
+        w.p = dalloc(c, sizeof(double) * n);
+        w.t = dalloc(c, sizeof(double) * n);
+        w.parts = dalloc(c, sizeof(double) * 4 * w.nparts);
+        w.state = dalloc(c, sizeof(CGState));
+        if (!w.r || !w.z || !w.p || !w.t || !w.parts || !w.state)
+            return fail(c, GSX_ERR_OOM, "device allocation (CG workspace)");
+        if (!A->diag) {
+            A->diag = dalloc(c, sizeof(double) * n);
+            if (!A->diag) return fail(c, GSX_ERR_OOM, "device allocation (diag)");
+            HIP_OK(c, gsx::launch_diag_f64(n, dp<int>(A->rowptr), dp<int>(A->col), dp<double>(A->val),
+                                           dp<double>(A->diag), c->stream));
+        }
+    }
+    if (w.hist_cap < maxit + 1) {
+        w.hist = dalloc(c, sizeof(double) * (maxit + 1));
+        if (!w.hist) return fail(c, GSX_ERR_OOM, "device allocation (history)");
+        w.hist_cap = maxit + 1;
+    }
+    if (mixed && !w.r32) {
+        w.r32 = dalloc(c, sizeof(float) * n);
+        w.z32 = dalloc(c, sizeof(float) * n);
+        w.p32 = dalloc(c, sizeof(float) * n);
+        w.t32 = dalloc(c, sizeof(float) * n);
+        w.xc32 = dalloc(c, sizeof(float) * n);
+        w.diag32 = dalloc(c, sizeof(float) * n);
+        if (!w.r32 || !w.z32 || !w.p32 || !w.t32 || !w.xc32 || !w.diag32)
+            return fail(c, GSX_ERR_OOM, "device allocation (mixed workspace)");
+        HIP_OK(c, gsx::launch_diag_f32(n, dp<int>(A->rowptr), dp<int>(A->col), dp<float>(A->val32),
+                                       dp<float>(w.diag32), c->stream));
+    }
+    return GSX_OK;
+}
+
+static gsx_status read_state(gsx_ctx *c, const CGWork &w, CGState &h)
+{
+    HIP_OK(c, hipMemcpyAsync(&h, w.state.get(), sizeof(CGState), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return GSX_OK;
+}
+
+// FP64 preconditioned CG with the reference's step order and stopping rule.
+static gsx_status cg_fp64(gsx_matrix *A, int precond, double tau, const double *b, double *x,
+                          double tol, int maxit, int *iters, double *relres)
+{
+    gsx_ctx *c = A->ctx;
+    hipStream_t s = c->stream;
+    CGWork &w = A->ws;
+    const int n = A->n;
+    const int np = w.nparts;
+    // partial counts: SpMV kernels write ns per-block partials, vector kernels nv
+    const int ns = gsx::spmv_blocks(n, A->lpr), nv = (n + 255) / 256;
+    double *P0 = dp<double>(w.parts), *P1 = P0 + np, *P2 = P1 + np;
+    CGState *st = dp<CGState>(w.state);
+    double *hist = dp<double>(w.hist);
+    CGState h{};
+    h.tol = tol;
+    h.tau = tau;
+    h.maxit = maxit;
+    HIP_OK(c, hipMemcpyAsync(st, &h, sizeof(CGState), hipMemcpyHostToDevice, s));
+    const int *rp = dp<int>(A->rowptr), *ci = dp<int>(A->col);
+    const double *va = dp<double>(A->val);
+    double *r = dp<double>(w.r), *z = dp<double>(w.z), *p = dp<double>(w.p), *t = dp<double>(w.t);
+    const double *diag = dp<double>(A->diag);
+    {
+        KScope k(c, "cg_init");
+        // r = b - A x ; ||r||^2 ; ||b||^2
+        HIP_OK(c, gsx::launch_spmv_f64(A->lpr, n, rp, ci, va, x, r, b, P1, 2, nullptr, s));
+        HIP_OK(c, gsx::launch_dot_f64(n, b, b, P0, nullptr, s));
+        HIP_OK(c, gsx::launch_cg_scalar(0, P0, nv, P1, ns, st, hist, s));
+    }
+    GSX_TRY(read_state(c, w, h));
+    if (h.done == 3) {
+        HIP_OK(c, hipMemsetAsync(x, 0, sizeof(double) * n, s));
+    }
+    if (!h.done) {
+        if (precond == GSX_PREC_JACOBI) {
+            HIP_OK(c, gsx::launch_jacobi_apply_f64(n, r, p, diag, tau, P0, st, s));
+        } else {
+            HIP_OK(c, hipMemcpyAsync(p, r, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+            HIP_OK(c, gsx::launch_dot_f64(n, r, r, P0, st, s));
+        }
+        HIP_OK(c, gsx::launch_cg_scalar(1, P0, nv, nullptr, 0, st, hist, s));
+        const int chunk = 16;
+        for (int done_it = 0; done_it < maxit;) {
+            const int k_end = std::min(maxit, done_it + chunk);
+            for (int k = done_it; k < k_end; ++k) {
+                {
+                    KScope ks(c, "spmv_f64");
+                    HIP_OK(c, gsx::launch_spmv_f64(A->lpr, n, rp, ci, va, p, t, nullptr, P0, 1, st, s));
+                }
+                KScope kv(c, "cg_vec");
+                HIP_OK(c, gsx::launch_cg_scalar(2, P0, ns, nullptr, 0, st, hist, s));
+                HIP_OK(c, gsx::launch_cg_update_f64(n, x, r, p, t, z,
+                                                    precond == GSX_PREC_JACOBI ? diag : nullptr, P1, P2, st, s));
+                if (precond == GSX_PREC_JACOBI) {
+                    HIP_OK(c, gsx::launch_cg_scalar(3, P1, nv, P2, nv, st, hist, s));
+                    HIP_OK(c, gsx::launch_cg_pupdate_f64(n, p, z, st, s));
+                } else {
+                    HIP_OK(c, gsx::launch_cg_scalar(3, P1, nv, P1, nv, st, hist, s));
+                    HIP_OK(c, gsx::launch_cg_pupdate_f64(n, p, r, st, s));
+                }
+            }
+            done_it = k_end;
+            GSX_TRY(read_state(c, w, h));
+            if (h.done) break;
+        }
+    }
+    *iters = h.it;
+    *relres = h.err;
+    A->hist.resize(h.it + 1);
+    HIP_OK(c, hipMemcpyAsync(A->hist.data(), hist, sizeof(double) * (h.it + 1), hipMemcpyDeviceToHost, s));
+    HIP_OK(c, hipStreamSynchronize(s));
+    if (h.done == 1 || h.done == 3) return GSX_OK;
+    if (h.err < tol) return GSX_OK;
+    return fail(c, GSX_ERR_NOT_CONVERGED, "CG reached the maximum number of iterations");
+}
+
+// Mixed precision: CG iterations on the FP32 copy of A with FP64 reliable
+// updates: every time the FP32 recurrence residual has dropped by a factor
+// delta (or claims convergence), the FP32 correction is added to the FP64
+// solution and the residual is recomputed as b - A x with the FP64 matrix.
+// The stopping test uses that FP64 residual only, so the fixed point is the
+// solution of the FP64 system.
+static gsx_status cg_mixed(gsx_matrix *A, int precond, double tau, const double *b, double *x,
+                           double tol, int maxit, int *iters, double *relres)
+{
+    gsx_ctx *c = A->ctx;
+    hipStream_t s = c->stream;
+    CGWork &w = A->ws;
+    const int n = A->n;
+    const int np = w.nparts;
+    // partial counts: SpMV kernels write ns per-block partials, vector kernels nv
+    const int ns = gsx::spmv_blocks(n, A->lpr), nv = (n + 255) / 256;
+    double *P0 = dp<double>(w.parts), *P1 = P0 + np, *P2 = P1 + np;
+    CGState *st = dp<CGState>(w.state);
+    CGState h{};
+    const double delta = 0.01;
+    h.tol = tol;
+    h.tau = tau;
+    h.maxit = maxit;
+    h.rel_since_update = delta * delta;
+    HIP_OK(c, hipMemcpyAsync(st, &h, sizeof(CGState), hipMemcpyHostToDevice, s));
+    const int *rp = dp<int>(A->rowptr), *ci = dp<int>(A->col);
+    const double *va = dp<double>(A->val);
+    const float *va32 = dp<float>(A->val32);
+    double *r = dp<double>(w.r);
+    float *r32 = dp<float>(w.r32), *z32 = dp<float>(w.z32), *p32 = dp<float>(w.p32),
+          *t32 = dp<float>(w.t32), *xc = dp<float>(w.xc32);
+    const float *d32 = dp<float>(w.diag32);
+    const bool jac = precond == GSX_PREC_JACOBI;
+    {
+        KScope k(c, "cg_init");
+        HIP_OK(c, gsx::launch_spmv_f64(A->lpr, n, rp, ci, va, x, r, b, P1, 2, nullptr, s));
+        HIP_OK(c, gsx::launch_dot_f64(n, b, b, P0, nullptr, s));
+        HIP_OK(c, gsx::launch_cg_scalar(0, P0, nv, P1, ns, st, nullptr, s));
+    }
+    GSX_TRY(read_state(c, w, h));
+    if (h.done == 3) HIP_OK(c, hipMemsetAsync(x, 0, sizeof(double) * n, s));
+
+    // r32 = fp32(r64); z32 = M r32; rz = (r32, z32); the direction p32 is kept
+    auto restart_from_r64 = [&]() -> gsx_status {
+        HIP_OK(c, gsx::launch_f64_to_f32(n, r, r32, s));
+        if (jac) {
+            HIP_OK(c, gsx::launch_jacobi_apply_f32(n, r32, z32, d32, tau, P0, nullptr, s));
+        } else {
+            HIP_OK(c, hipMemcpyAsync(z32, r32, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+            HIP_OK(c, gsx::launch_dot_f32(n, r32, r32, P0, nullptr, s));
+        }
+        HIP_OK(c, gsx::launch_cg_scalar(1, P0, nv, nullptr, 0, st, nullptr, s));
+        return GSX_OK;
+    };
+    if (!h.done) {
+        HIP_OK(c, hipMemsetAsync(xc, 0, sizeof(float) * n, s));
+        gsx_status e = restart_from_r64();
+        if (e != GSX_OK) return e;
+        HIP_OK(c, hipMemcpyAsync(p32, z32, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
+        int chunk = 8;
+        while (true) {
+            for (int k = 0; k < chunk; ++k) {
+                {
+                    KScope ks(c, "spmv_f32");
+                    HIP_OK(c, gsx::launch_spmv_f32(A->lpr, n, rp, ci, va32, p32, t32, nullptr, P0, 1, st, s));
+                }
+                KScope kv(c, "cg_vec_f32");
+                HIP_OK(c, gsx::launch_cg_scalar(2, P0, ns, nullptr, 0, st, nullptr, s));
+                HIP_OK(c, gsx::launch_cg_update_f32(n, xc, r32, p32, t32, z32, jac ? d32 : nullptr, P1, P2, st, s));
+                HIP_OK(c, gsx::launch_cg_scalar(4, P1, nv, jac ? P2 : P1, nv, st, nullptr, s));
+                HIP_OK(c, gsx::launch_cg_pupdate_f32(n, p32, jac ? z32 : r32, st, s));
+            }
+            GSX_TRY(read_state(c, w, h));
+            const bool due = h.need_update || h.done == 4 || h.done == 2 ||
+                             h.rr < delta * delta * h.rr_ref || h.err < tol;
+            if (due) {
+                KScope kr(c, "reliable_update");
+
+                HIP_OK(c, gsx::launch_accum_correction(n, x, xc, s));
+                HIP_OK(c, gsx::launch_spmv_f64(A->lpr, n, rp, ci, va, x, r, b, P1, 2, nullptr, s));
+                HIP_OK(c, gsx::launch_cg_reliable(P1, ns, st, nullptr, s));
+                GSX_TRY(read_state(c, w, h));
+                if (h.done == 1 || h.done == 2) break;
+                gsx_status e2 = restart_from_r64();
+                if (e2 != GSX_OK) return e2;
+            }
+            if (h.it >= maxit) break;
+        }
+    }
+    *iters = h.it;
+    *relres = h.err;
+    HIP_OK(c, hipStreamSynchronize(s));
+    if (h.done == 1 || h.done == 3 || h.err < tol) return GSX_OK;
+    return fail(c, GSX_ERR_NOT_CONVERGED, "mixed CG reached the maximum number of iterations");
+}
+
+extern "C" {
+
+gsx_status gsx_cg_solve(gsx_matrix *A, int32_t precond, double tau, gsx_mg *mg, int32_t precision,
+                        const double *b, double *x, double tol, int32_t maxit, int32_t *iters,
+                        double *relres)
+{
+    if (!A || !b || !x || maxit < 0) return GSX_ERR_ARG;
+    gsx_ctx *c = A->ctx;
+    if (precond == GSX_PREC_MG || mg) return fail(c, GSX_ERR_ARG, "multigrid preconditioner not available");
+    if (precond != GSX_PREC_NONE && precond != GSX_PREC_JACOBI)
+        return fail(c, GSX_ERR_ARG, "unknown preconditioner");
+    if (precision == GSX_MIXED && !A->val32)
+        return fail(c, GSX_ERR_ARG, "mixed precision needs a matrix assembled with GSX_MIXED");
+    const bool mixed = precision == GSX_MIXED;
+    gsx_status e = cg_workspace(A, std::max(1, maxit), mixed);
+    if (e != GSX_OK) return e;
+    int it = 0;
+    double rr = 0.0;
+    e = mixed ? cg_mixed(A, precond, tau, b, x, tol, maxit, &it, &rr)
+              : cg_fp64(A, precond, tau, b, x, tol, maxit, &it, &rr);
+    if (iters) *iters = it;
+    if (relres) *relres = rr;
+    return e;
+}
+
+gsx_status gsx_cg_history(gsx_matrix *A, double *hist, int32_t cap, int32_t *count)
+{
+    if (!A || !count) return GSX_ERR_ARG;
+    *count = (int32_t)A->hist.size();
+    if (hist)
+        for (int i = 0; i < std::min<int>(cap, (int)A->hist.size()); ++i) hist[i] = A->hist[i];
+    return GSX_OK;
+}
+
+gsx_status gsx_l2_error(gsx_problem *P, const double *sol, double *err)
+{
+    if (!P || !sol || !err) return GSX_ERR_ARG;
+    gsx_ctx *c = P->ctx;
+    hipStream_t s = c->stream;
+    DevPtr per = dalloc(c, sizeof(double) * (std::max<int64_t>(1, P->nelem) + 1));
+    if (!per) return fail(c, GSX_ERR_OOM, "device allocation (l2)");
+    for (auto &g : P->groups)
+        HIP_OK(c, gsx::launch_l2err(g.d, g.p, dp<PatchDev>(P->d_patches), dp<int>(g.d_gpatch),
+                                    dp<int64_t>(g.d_gprefix), (int)g.patches.size(), g.nelem, c->gt,
+                                    sol, P->nfree, dp<double>(per), s));
+    double *slot = dp<double>(per) + std::max<int64_t>(1, P->nelem);
+    HIP_OK(c, gsx::launch_sum(dp<double>(per), P->nelem, slot, s));
+    double tot = 0.0;
+    HIP_OK(c, hipMemcpyAsync(&tot, slot, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_OK(c, hipStreamSynchronize(s));
+    *err = std::sqrt(tot);
+    return GSX_OK;
+}
+
+}  // extern "C"

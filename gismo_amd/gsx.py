@@ -1,0 +1,328 @@
+"""ctypes binding of libgsx (include/gsx.h) and a thin Python mirror of the
+G+Smo call sequence for the isogeometric Poisson path:
+
+    gsMultiPatch/gsMultiBasis + gsFeSpace::setup   -> Problem(...)
+    gsExprAssembler::assemble / gsPoissonAssembler -> Problem.assemble()
+    gsConjugateGradient(A, makeJacobiOp(A)).solve  -> Matrix.cg_solve()
+    gsExprEvaluator::integral((u - u_ex)^2 meas)   -> Problem.l2_error()
+
+The product path is libgsx only: if the shared library is missing this
+module raises instead of computing anything on the CPU.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgsx.so")
+
+GSX_OK = 0
+GSX_ERR_ARG = 1
+GSX_ERR_OOM = 2
+GSX_ERR_JACOBIAN = 3
+GSX_ERR_HIP = 4
+GSX_ERR_NOT_CONVERGED = 5
+GSX_ERR_NO_DEVICE = 6
+GSX_ERR_BUDGET = 7
+
+FP64 = 0
+MIXED = 1
+PATTERN_EXPR = 0
+PATTERN_FULL = 1
+ASM_REBUILD_PATTERN = 0x100
+PREC_NONE = 0
+PREC_JACOBI = 1
+PREC_MG = 2
+
+_STATUS_NAMES = {
+    0: "GSX_OK", 1: "GSX_ERR_ARG", 2: "GSX_ERR_OOM", 3: "GSX_ERR_JACOBIAN",
+    4: "GSX_ERR_HIP", 5: "GSX_ERR_NOT_CONVERGED", 6: "GSX_ERR_NO_DEVICE", 7: "GSX_ERR_BUDGET",
+}
+
+
+class GsxError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{_STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class PatchDesc(C.Structure):
+    _fields_ = [
+        ("d", C.c_int32),
+        ("p", C.c_int32 * 3),
+        ("nknots", C.c_int32 * 3),
+        ("knots_host", C.POINTER(C.c_double) * 3),
+        ("ctrl_host", C.POINTER(C.c_double)),
+        ("kappa_host", C.POINTER(C.c_double)),
+        ("felem_host", C.POINTER(C.c_double)),
+        ("fkind", C.c_int32),
+    ]
+
+
+_lib = None
+_vp = C.c_void_p
+_pp = C.POINTER(C.c_void_p)
+_i64p = C.POINTER(C.c_int64)
+_i32p = C.POINTER(C.c_int32)
+_dp = C.POINTER(C.c_double)
+
+# (name, argtypes) of every exported function; restype is c_int (gsx_status)
+# unless listed in _RESTYPES.
+SIGNATURES = {
+    "gsx_ctx_create": [C.c_int32, _pp],
+    "gsx_ctx_destroy": [_vp],
+    "gsx_last_error": [_vp],
+    "gsx_ctx_synchronize": [_vp],
+    "gsx_ctx_enable_timing": [_vp, C.c_int32],
+    "gsx_ctx_reset_timing": [_vp],
+    "gsx_ctx_timing_filter": [_vp, C.c_char_p],
+    "gsx_ctx_kernel_stats": [_vp, C.c_char_p, _i64p, _dp],
+    "gsx_device_alloc": [_vp, C.c_int64, _pp],
+    "gsx_device_free": [_vp, _vp],
+    "gsx_copy_to_device": [_vp, _vp, _vp, C.c_int64],
+    "gsx_copy_to_host": [_vp, _vp, _vp, C.c_int64],
+    "gsx_ctx_memory": [_vp, _i64p],
+    "gsx_problem_create": [_vp, C.c_int32, C.POINTER(PatchDesc), C.c_int32, _i32p, C.c_int32,
+                           _i32p, _pp],
+    "gsx_problem_destroy": [_vp],
+    "gsx_problem_info": [_vp, _i64p, _i64p, _i64p, _i64p],
+    "gsx_problem_dofs": [_vp, _i32p],
+    "gsx_problem_min_detj": [_vp, _dp],
+    "gsx_assemble": [_vp, C.c_int32, C.c_int32, _pp, _vp],
+    "gsx_matrix_destroy": [_vp],
+    "gsx_matrix_info": [_vp, _i64p, _i64p],
+    "gsx_matrix_download": [_vp, _i32p, _i32p, _dp],
+    "gsx_matrix_device_ptrs": [_vp, _pp, _pp, _pp],
+    "gsx_spmv": [_vp, _vp, _vp],
+    "gsx_cg_solve": [_vp, C.c_int32, C.c_double, _vp, C.c_int32, _vp, _vp, C.c_double, C.c_int32,
+                     _i32p, _dp],
+    "gsx_cg_history": [_vp, _dp, C.c_int32, _i32p],
+    "gsx_l2_error": [_vp, _vp, _dp],
+}
+_RESTYPES = {"gsx_last_error": C.c_char_p}
+
+
+def lib():
+    """Load libgsx.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libgsx not built: {LIB_PATH} is missing (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPES.get(name, C.c_int)
+        _lib = L
+    return _lib
+
+
+def _check(ctx, status, allow=()):
+    if status != GSX_OK and status not in allow:
+        msg = lib().gsx_last_error(ctx).decode() if ctx else ""
+        raise GsxError(status, msg)
+    return status
+
+
+class DeviceArray:
+    """Device buffer owned by a Context (freed with it or on close())."""
+
+    def __init__(self, ctx, n, dtype=np.float64):
+        self.ctx = ctx
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        p = C.c_void_p()
+        _check(ctx.h, lib().gsx_device_alloc(ctx.h, self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    @property
+    def nbytes(self):
+        return self.n * self.dtype.itemsize
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        _check(self.ctx.h, lib().gsx_copy_to_device(self.ctx.h, self.ptr, a.ctypes.data, self.nbytes))
+        return self
+
+    def download(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        _check(self.ctx.h, lib().gsx_copy_to_host(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def zero(self):
+        return self.upload(np.zeros(self.n, self.dtype))
+
+    def close(self):
+        if self.ptr:
+            lib().gsx_device_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        st = lib().gsx_ctx_create(device, C.byref(h))
+        if st != GSX_OK:
+            raise GsxError(st, "gsx_ctx_create failed")
+        self.h = h
+
+    def array(self, n, dtype=np.float64):
+        return DeviceArray(self, n, dtype)
+
+    def to_device(self, a, dtype=np.float64):
+        a = np.asarray(a, dtype=dtype)
+        return DeviceArray(self, a.size, dtype).upload(a)
+
+    def synchronize(self):
+        _check(self.h, lib().gsx_ctx_synchronize(self.h))
+
+    def enable_timing(self, on=True):
+        _check(self.h, lib().gsx_ctx_enable_timing(self.h, 1 if on else 0))
+
+    def timing_filter(self, name=None):
+        _check(self.h, lib().gsx_ctx_timing_filter(self.h, name.encode() if name else None))
+
+    def reset_timing(self):
+        _check(self.h, lib().gsx_ctx_reset_timing(self.h))
+
+    def kernel_stats(self, name):
+        n = C.c_int64()
+        ms = C.c_double()
+        _check(self.h, lib().gsx_ctx_kernel_stats(self.h, name.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def memory(self):
+        b = C.c_int64()
+        _check(self.h, lib().gsx_ctx_memory(self.h, C.byref(b)))
+        return b.value
+
+    def close(self):
+        if self.h:
+            lib().gsx_ctx_destroy(self.h)
+            self.h = None
+
+
+class Problem:
+    """Patches + DOF mapper on the device (gsMultiPatch, gsMultiBasis,
+    gsFeSpace::setup with glued interfaces and eliminated Dirichlet sides)."""
+
+    def __init__(self, ctx, problem):
+        self.ctx = ctx
+        self.src = problem
+        patches = problem.patches
+        n = len(patches)
+        arr = (PatchDesc * n)()
+        self._keep = []
+        for i, P in enumerate(patches):
+            o = arr[i]
+            o.d = P.d
+            for k in range(3):
+                if k < P.d:
+                    kn = np.ascontiguousarray(P.knots[k], dtype=np.float64)
+                    self._keep.append(kn)
+                    o.p[k] = int(P.p[k])
+                    o.nknots[k] = kn.size
+                    o.knots_host[k] = kn.ctypes.data_as(_dp)
+            ctrl = np.ascontiguousarray(P.ctrl, dtype=np.float64)
+            self._keep.append(ctrl)
+            o.ctrl_host = ctrl.ctypes.data_as(_dp)
+            for name, attr in (("kappa_host", "kappa"), ("felem_host", "felem")):
+                a = getattr(P, attr, None)
+                if a is None:
+                    setattr(o, name, _dp())
+                else:
+                    a = np.ascontiguousarray(a, dtype=np.float64)
+                    self._keep.append(a)
+                    setattr(o, name, a.ctypes.data_as(_dp))
+            o.fkind = int(P.fkind)
+        ds = np.ascontiguousarray(np.asarray(problem.dir_sides, dtype=np.int32).reshape(-1))
+        ifs = np.ascontiguousarray(np.asarray(problem.ifaces, dtype=np.int32).reshape(-1))
+        self._keep += [ds, ifs]
+        h = C.c_void_p()
+        _check(ctx.h, lib().gsx_problem_create(
+            ctx.h, n, arr, len(problem.dir_sides), ds.ctypes.data_as(_i32p) if ds.size else None,
+            len(problem.ifaces), ifs.ctypes.data_as(_i32p) if ifs.size else None, C.byref(h)))
+        self.h = h
+        nb, nf, ne, ng = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        _check(ctx.h, lib().gsx_problem_info(h, C.byref(nb), C.byref(nf), C.byref(ne), C.byref(ng)))
+        self.nbasis, self.nfree, self.nelem, self.nglued = nb.value, nf.value, ne.value, ng.value
+
+    def dofs(self):
+        out = np.zeros(self.nbasis, np.int32)
+        _check(self.ctx.h, lib().gsx_problem_dofs(self.h, out.ctypes.data_as(_i32p)))
+        return out
+
+    def min_detj(self):
+        v = C.c_double()
+        _check(self.ctx.h, lib().gsx_problem_min_detj(self.h, C.byref(v)))
+        return v.value
+
+    def assemble(self, mode=PATTERN_EXPR, precision=FP64, rhs=None):
+        """Returns (Matrix, rhs DeviceArray)."""
+        if rhs is None:
+            rhs = self.ctx.array(max(1, self.nfree))
+        h = C.c_void_p()
+        _check(self.ctx.h, lib().gsx_assemble(self.h, mode, precision, C.byref(h), rhs.ptr))
+        return Matrix(self.ctx, h), rhs
+
+    def l2_error(self, sol):
+        v = C.c_double()
+        _check(self.ctx.h, lib().gsx_l2_error(self.h, sol.ptr, C.byref(v)))
+        return v.value
+
+    def close(self):
+        if self.h:
+            lib().gsx_problem_destroy(self.h)
+            self.h = None
+
+
+class Matrix:
+    """Device CSR matrix (symmetric: the CSR arrays are also the CSC arrays
+    of gsSparseMatrix)."""
+
+    def __init__(self, ctx, h):
+        self.ctx = ctx
+        self.h = h
+        n, nnz = C.c_int64(), C.c_int64()
+        _check(ctx.h, lib().gsx_matrix_info(h, C.byref(n), C.byref(nnz)))
+        self.n, self.nnz = n.value, nnz.value
+
+    def download(self):
+        rp = np.zeros(self.n + 1, np.int32)
+        ci = np.zeros(max(1, self.nnz), np.int32)
+        va = np.zeros(max(1, self.nnz), np.float64)
+        _check(self.ctx.h, lib().gsx_matrix_download(self.h, rp.ctypes.data_as(_i32p),
+                                                     ci.ctypes.data_as(_i32p), va.ctypes.data_as(_dp)))
+        return rp, ci[: self.nnz], va[: self.nnz]
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        rp, ci, va = self.download()
+        return sp.csr_matrix((va, ci, rp), shape=(self.n, self.n))
+
+    def spmv(self, x, y):
+        _check(self.ctx.h, lib().gsx_spmv(self.h, x.ptr, y.ptr))
+
+    def cg_solve(self, b, x, tol=1e-10, maxit=1000, precond=PREC_JACOBI, tau=1.0,
+                 precision=FP64, raise_on_nonconvergence=False):
+        it = C.c_int32()
+        rr = C.c_double()
+        st = lib().gsx_cg_solve(self.h, precond, tau, None, precision, b.ptr, x.ptr, tol, maxit,
+                                C.byref(it), C.byref(rr))
+        allow = () if raise_on_nonconvergence else (GSX_ERR_NOT_CONVERGED,)
+        _check(self.ctx.h, st, allow)
+        return it.value, rr.value, st == GSX_OK
+
+    def history(self):
+        cnt = C.c_int32()
+        _check(self.ctx.h, lib().gsx_cg_history(self.h, None, 0, C.byref(cnt)))
+        h = np.zeros(max(1, cnt.value))
+        _check(self.ctx.h, lib().gsx_cg_history(self.h, h.ctypes.data_as(_dp), cnt.value, C.byref(cnt)))
+        return h[: cnt.value]
+
+    def close(self):
+        if self.h:
+            lib().gsx_matrix_destroy(self.h)
+            self.h = None

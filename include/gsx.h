@@ -1,0 +1,178 @@
+/*
+ * gsx.h -- C ABI of libgsx, the MI355X (gfx950) isogeometric Poisson
+ * assemble + solve library.
+ *
+ * Plain pointers and sizes only.  Host pointers are named *_host; device
+ * pointers (from gsx_device_alloc or any HIP allocation) are named *_dev.
+ * Every entry point returns a gsx_status (0 = success); the message of the
+ * last failure of a context is returned by gsx_last_error().
+ *
+ * Each entry point replaces one step of the G+Smo hot path; the comment
+ * names the reference interface (paths relative to the G+Smo source tree).
+ * The C++ drop-in classes (gismo_amd/host/gismo) and the Python mirror
+ * (gismo_amd) are written on top of these functions.
+ */
+#ifndef GSX_H
+#define GSX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    GSX_OK = 0,
+    GSX_ERR_ARG = 1,           /* invalid argument / unsupported input       */
+    GSX_ERR_OOM = 2,           /* device allocation failed                   */
+    GSX_ERR_JACOBIAN = 3,      /* non-positive Jacobian determinant          */
+    GSX_ERR_HIP = 4,           /* HIP runtime error                          */
+    GSX_ERR_NOT_CONVERGED = 5, /* solver stopped at max iterations           */
+    GSX_ERR_NO_DEVICE = 6,     /* no GPU visible                             */
+    GSX_ERR_BUDGET = 7         /* request exceeds the HBM budget             */
+} gsx_status;
+
+typedef enum {
+    GSX_FP64 = 0,   /* plain FP64 everywhere                                  */
+    GSX_MIXED = 1   /* FP64 operator + FP32 inner iterations, FP64 residuals  */
+} gsx_precision;
+
+typedef enum {
+    GSX_PATTERN_EXPR = 0,    /* gsExprAssembler: skip exact-zero local entries */
+    GSX_PATTERN_FULL = 1,    /* gsPoissonAssembler: every free-free pair       */
+    GSX_ASM_REBUILD_PATTERN = 0x100  /* flag: recompute the cached pattern     */
+} gsx_pattern_mode;
+
+typedef enum {
+    GSX_PREC_NONE = 0,
+    GSX_PREC_JACOBI = 1,
+    GSX_PREC_MG = 2
+} gsx_precond;
+
+typedef struct gsx_ctx gsx_ctx;
+typedef struct gsx_problem gsx_problem;
+typedef struct gsx_matrix gsx_matrix;
+typedef struct gsx_mg gsx_mg;
+
+/* One tensor-product B-spline patch, geometry and discretisation sharing the
+ * basis (isoparametric).  Knot vectors are open with strictly increasing
+ * interior knots; degree 1..4 per direction; d = 2 or 3.  Control points are
+ * row-major nbasis x d, lexicographic with direction 0 fastest.  kappa and
+ * felem (per element, lexicographic) may be NULL.
+ * fkind 1: f = d*pi^2*prod_r sin(pi x_r); fkind 0: f = felem[e]. */
+typedef struct {
+    int32_t d;
+    int32_t p[3];
+    int32_t nknots[3];
+    const double *knots_host[3];
+    const double *ctrl_host;
+    const double *kappa_host;
+    const double *felem_host;
+    int32_t fkind;
+} gsx_patch_desc;
+
+/* ---- context --------------------------------------------------------- */
+/* No reference counterpart: owns the HIP stream, events and the scratch
+ * pool of one device. */
+gsx_status gsx_ctx_create(int32_t device, gsx_ctx **out);
+gsx_status gsx_ctx_destroy(gsx_ctx *ctx);
+const char *gsx_last_error(const gsx_ctx *ctx);
+gsx_status gsx_ctx_synchronize(gsx_ctx *ctx);
+/* Kernel timing: when enabled, every launch is bracketed by HIP events on
+ * the context's stream; gsx_ctx_kernel_stats returns, for kernel name
+ * `name`, the number of launches and the summed milliseconds since the last
+ * reset. */
+gsx_status gsx_ctx_enable_timing(gsx_ctx *ctx, int32_t enable);
+gsx_status gsx_ctx_reset_timing(gsx_ctx *ctx);
+/* Restrict timing to the scopes named `name` (NULL or "": all scopes). */
+gsx_status gsx_ctx_timing_filter(gsx_ctx *ctx, const char *name);
+gsx_status gsx_ctx_kernel_stats(gsx_ctx *ctx, const char *name,
+                                int64_t *launches, double *total_ms);
+/* Device memory helpers for callers without their own allocator. */
+gsx_status gsx_device_alloc(gsx_ctx *ctx, int64_t bytes, void **dev_out);
+gsx_status gsx_device_free(gsx_ctx *ctx, void *dev);
+gsx_status gsx_copy_to_device(gsx_ctx *ctx, void *dev, const void *host, int64_t bytes);
+gsx_status gsx_copy_to_host(gsx_ctx *ctx, void *host, const void *dev, int64_t bytes);
+/* Bytes of device memory currently held by the context. */
+gsx_status gsx_ctx_memory(gsx_ctx *ctx, int64_t *bytes_in_use);
+
+/* ---- problem: patches + DOF mapper ------------------------------------ */
+/* Replaces gsMultiPatch/gsMultiBasis + gsFeSpace::setup(bc, dirichlet, 0)
+ * building the gsDofMapper (src/gsAssembler/gsExpressions.h:1091-1121,
+ * src/gsCore/gsDofMapper.cpp:99-340): interfaces glued (C0), Dirichlet sides
+ * eliminated, free DOFs numbered standard-first, then coupled, then
+ * eliminated.  dir_sides: ndir pairs (patch, side 1..2d).  ifaces: nif
+ * records (p1, s1, p2, s2, m0, f0, m1, f1). */
+gsx_status gsx_problem_create(gsx_ctx *ctx, int32_t npatch,
+                              const gsx_patch_desc *patches, int32_t ndir,
+                              const int32_t *dir_sides_host, int32_t nif,
+                              const int32_t *ifaces_host, gsx_problem **out);
+gsx_status gsx_problem_destroy(gsx_problem *prob);
+/* nbasis: all basis functions (summed over patches, before gluing);
+ * nfree: free DOFs; nelem: elements; nglued: glued space dimension. */
+gsx_status gsx_problem_info(const gsx_problem *prob, int64_t *nbasis,
+                            int64_t *nfree, int64_t *nelem, int64_t *nglued);
+/* Global DOF index of each (patch, local basis function), concatenated over
+ * patches (gsDofMapper::index, src/gsCore/gsDofMapper.h:325). */
+gsx_status gsx_problem_dofs(const gsx_problem *prob, int32_t *dofs_host);
+
+/* ---- assembly ---------------------------------------------------------- */
+/* Replaces gsExprAssembler<T>::assemble(igrad(u,G)*igrad(u,G).tr()*meas(G)
+ * [*kappa], u*ff*meas(G)) (src/gsAssembler/gsExprAssembler.h:754-833) and
+ * gsPoissonAssembler<T>::assemble() (src/gsAssembler/gsPoissonAssembler.hpp:
+ * 41-78): builds the CSR pattern and FP64 values of the free-free stiffness
+ * matrix and the load vector on the device.  The matrix is symmetric
+ * bitwise, so its CSR arrays are also its CSC arrays (gsSparseMatrix is
+ * column-major, src/gsCore/gsForwardDeclarations.h:279-280).
+ * precision GSX_MIXED additionally keeps an FP32 copy of the values for the
+ * inner iterations.  rhs_dev: nfree doubles (device) or NULL. */
+gsx_status gsx_assemble(gsx_problem *prob, int32_t pattern_mode,
+                        int32_t precision, gsx_matrix **A_out,
+                        double *rhs_dev);
+gsx_status gsx_matrix_destroy(gsx_matrix *A);
+gsx_status gsx_matrix_info(const gsx_matrix *A, int64_t *n, int64_t *nnz);
+/* Host copies of the CSR arrays (n+1, nnz, nnz). */
+gsx_status gsx_matrix_download(const gsx_matrix *A, int32_t *rowptr_host,
+                               int32_t *col_host, double *val_host);
+/* Device pointers of the CSR arrays (owned by the matrix). */
+gsx_status gsx_matrix_device_ptrs(const gsx_matrix *A, const int32_t **rowptr_dev,
+                                  const int32_t **col_dev, const double **val_dev);
+/* Smallest Jacobian determinant met at any Gauss point of the last
+ * assembly of prob (a non-positive value makes gsx_assemble fail with
+ * GSX_ERR_JACOBIAN). */
+gsx_status gsx_problem_min_detj(const gsx_problem *prob, double *min_det);
+
+/* ---- operators ----------------------------------------------------------- */
+/* y = A x on the device (gsMatrixOp::apply, src/gsSolver/gsMatrixOp.h:74-75). */
+gsx_status gsx_spmv(gsx_matrix *A, const double *x_dev, double *y_dev);
+
+/* ---- solvers --------------------------------------------------------------- */
+/* Preconditioned CG: gsConjugateGradient::solve (src/gsSolver/
+ * gsConjugateGradient.hpp:20-93, src/gsSolver/gsIterativeSolver.h:114-200)
+ * with gsJacobiOp (src/gsSolver/gsSimplePreconditioners.h:146-244, damping
+ * tau) or gsMultiGridOp (mg != NULL) as preconditioner.  Stops when
+ * ||r||_2/||b||_2 < tol; x_dev holds the initial guess on entry.
+ * precision GSX_MIXED: iterations in FP32 with FP64 reliable-update
+ * residuals b - A x (the FP64 operator), the stopping test on the FP64
+ * residual.  Returns GSX_ERR_NOT_CONVERGED (not an error of the call) when
+ * maxit is reached; iters/relres are always written. */
+gsx_status gsx_cg_solve(gsx_matrix *A, int32_t precond, double tau,
+                        gsx_mg *mg, int32_t precision, const double *b_dev,
+                        double *x_dev, double tol, int32_t maxit,
+                        int32_t *iters, double *relres);
+/* Relative residual history of the last FP64 solve of A (iters+1 values,
+ * gsIterativeSolver::solveDetailed, gsIterativeSolver.h:134-165). */
+gsx_status gsx_cg_history(gsx_matrix *A, double *hist_host, int32_t cap,
+                          int32_t *count);
+
+/* ---- reporting ---------------------------------------------------------- */
+/* L2 error of the discrete solution (free coefficients, eliminated = 0)
+ * against u = prod_r sin(pi x_r) with the assembly quadrature
+ * (gsExprEvaluator::integral, src/gsAssembler/gsExprEvaluator.h:382-450). */
+gsx_status gsx_l2_error(gsx_problem *prob, const double *sol_dev,
+                        double *err);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

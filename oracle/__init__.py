@@ -1,0 +1,316 @@
+"""ctypes bindings of the CPU FP64 oracle (oracle/libgsx_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: the oracle restates the reference algorithm on the
+CPU and is the parity checker.  Only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py may import this module; the product path
+(gismo_amd, libgsx) never does.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libgsx_oracle.so")
+_lib = None
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class OcPatch(C.Structure):
+    _fields_ = [
+        ("d", C.c_int),
+        ("p", C.c_int * 3),
+        ("nknots", C.c_int * 3),
+        ("knots", C.POINTER(C.c_double) * 3),
+        ("ctrl", C.POINTER(C.c_double)),
+        ("kappa", C.POINTER(C.c_double)),
+        ("felem", C.POINTER(C.c_double)),
+        ("fkind", C.c_int),
+    ]
+
+
+class OcCsr(C.Structure):
+    _fields_ = [
+        ("n", C.c_int),
+        ("nnz", C.c_int64),
+        ("rowptr", C.POINTER(C.c_int)),
+        ("col", C.POINTER(C.c_int)),
+        ("val", C.POINTER(C.c_double)),
+    ]
+
+
+def build():
+    """Compile the oracle (gcc) in place."""
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.oc_gauss_legendre.argtypes = [C.c_int, _f64p, _f64p]
+        L.oc_knots_uniform.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _f64p]
+        L.oc_find_span.argtypes = [_f64p, C.c_int, C.c_int, C.c_double]
+        L.oc_find_span.restype = C.c_int
+        L.oc_bspline_ders.argtypes = [_f64p, C.c_int, C.c_int, C.c_double, C.c_int, _f64p]
+        L.oc_element_geometry.argtypes = [C.POINTER(OcPatch), C.c_int, _f64p, _f64p, _f64p, _f64p]
+        L.oc_element_geometry.restype = C.c_int
+        L.oc_mapper_build.argtypes = [C.c_int, C.POINTER(OcPatch), C.c_int, _i32p, C.c_int, _i32p, _i32p]
+        L.oc_mapper_build.restype = C.c_int
+        L.oc_assemble.argtypes = [C.c_int, C.POINTER(OcPatch), _i32p, C.c_int, C.c_int, _f64p]
+        L.oc_assemble.restype = C.POINTER(OcCsr)
+        L.oc_csr_free.argtypes = [C.POINTER(OcCsr)]
+        L.oc_assemble_rows.argtypes = [C.POINTER(OcPatch), _i32p, C.c_int, C.c_int, _i32p, C.c_int,
+                                       _i32p, _i32p, _f64p, _f64p]
+        L.oc_cg_solve.argtypes = [C.POINTER(OcCsr), _f64p, _f64p, C.c_double, C.c_int, C.c_int,
+                                  C.c_double, C.c_void_p, C.POINTER(C.c_double), C.c_void_p]
+        L.oc_cg_solve.restype = C.c_int
+        L.oc_transfer_1d.argtypes = [_f64p, C.c_int, C.c_int, _f64p]
+        L.oc_transfer_1d.restype = C.c_int
+        L.oc_transfer_multipatch.argtypes = [C.c_int, C.POINTER(OcPatch), _i32p, C.c_int, _i32p, C.c_int]
+        L.oc_transfer_multipatch.restype = C.POINTER(OcCsr)
+        L.oc_mg_create.argtypes = [C.POINTER(OcCsr), C.c_int, C.POINTER(C.POINTER(OcCsr)),
+                                   C.c_int, C.c_int, C.c_double, C.c_double]
+        L.oc_mg_create.restype = C.c_void_p
+        L.oc_mg_apply.argtypes = [C.c_void_p, _f64p, _f64p]
+        L.oc_mg_matrix.argtypes = [C.c_void_p, C.c_int]
+        L.oc_mg_matrix.restype = C.POINTER(OcCsr)
+        L.oc_mg_free.argtypes = [C.c_void_p]
+        L.oc_l2_error.argtypes = [C.c_int, C.POINTER(OcPatch), _i32p, C.c_int, _f64p]
+        L.oc_l2_error.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+# ---------------------------------------------------------------------------
+# numpy-friendly wrappers
+# ---------------------------------------------------------------------------
+
+def gauss_legendre(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().oc_gauss_legendre(n, x, w)
+    return x, w
+
+
+def knots_uniform(nelem, p, a=0.0, b=1.0):
+    k = np.zeros(nelem + 2 * p + 1)
+    lib().oc_knots_uniform(nelem, p, a, b, k)
+    return k
+
+
+def find_span(knots, p, u):
+    knots = np.ascontiguousarray(knots, dtype=np.float64)
+    return lib().oc_find_span(knots, len(knots), p, float(u))
+
+
+def bspline_ders(knots, p, span, u, nder=1):
+    knots = np.ascontiguousarray(knots, dtype=np.float64)
+    out = np.zeros((nder + 1) * (p + 1))
+    lib().oc_bspline_ders(knots, p, span, float(u), nder, out)
+    return out.reshape(nder + 1, p + 1)
+
+
+class Patches:
+    """Keeps the numpy arrays behind an array of OcPatch alive."""
+
+    def __init__(self, patch_list):
+        # patch_list: sequence of objects with attributes d, p, knots, ctrl,
+        # kappa (or None), felem (or None), fkind
+        self._keep = []
+        n = len(patch_list)
+        self.arr = (OcPatch * n)()
+        for i, P in enumerate(patch_list):
+            o = self.arr[i]
+            o.d = P.d
+            for k in range(3):
+                if k < P.d:
+                    kn = np.ascontiguousarray(P.knots[k], dtype=np.float64)
+                    self._keep.append(kn)
+                    o.p[k] = int(P.p[k])
+                    o.nknots[k] = len(kn)
+                    o.knots[k] = kn.ctypes.data_as(C.POINTER(C.c_double))
+                else:
+                    o.p[k] = 0
+                    o.nknots[k] = 0
+            ctrl = np.ascontiguousarray(P.ctrl, dtype=np.float64)
+            self._keep.append(ctrl)
+            o.ctrl = ctrl.ctypes.data_as(C.POINTER(C.c_double))
+            for name in ("kappa", "felem"):
+                a = getattr(P, name, None)
+                if a is None:
+                    setattr(o, name, C.POINTER(C.c_double)())
+                else:
+                    a = np.ascontiguousarray(a, dtype=np.float64)
+                    self._keep.append(a)
+                    setattr(o, name, a.ctypes.data_as(C.POINTER(C.c_double)))
+            o.fkind = int(P.fkind)
+        self.n = n
+
+
+def element_geometry(patch, elem):
+    P = Patches([patch])
+    nq = 1
+    for k in range(patch.d):
+        nq *= patch.p[k] + 1
+    d = patch.d
+    xq = np.zeros(nq * d)
+    jac = np.zeros(nq * d * d)
+    meas = np.zeros(nq)
+    wq = np.zeros(nq)
+    lib().oc_element_geometry(P.arr, elem, xq, jac, meas, wq)
+    return xq.reshape(nq, d), jac.reshape(nq, d, d), meas, wq
+
+
+def mapper(patch_list, dir_sides, ifaces):
+    """dir_sides: list of (patch, side); ifaces: list of 8-int records.
+    Returns (dofs concatenated per patch, nfree)."""
+    P = Patches(patch_list)
+    total = 0
+    for Pp in patch_list:
+        s = 1
+        for k in range(Pp.d):
+            s *= len(Pp.knots[k]) - Pp.p[k] - 1
+        total += s
+    ds = np.ascontiguousarray(np.asarray(dir_sides, dtype=np.int32).reshape(-1))
+    if ds.size == 0:
+        ds = np.zeros(2, np.int32)
+    ifs = np.ascontiguousarray(np.asarray(ifaces, dtype=np.int32).reshape(-1))
+    if ifs.size == 0:
+        ifs = np.zeros(8, np.int32)
+    out = np.zeros(total, np.int32)
+    nfree = lib().oc_mapper_build(P.n, P.arr, len(dir_sides), ds, len(ifaces), ifs, out)
+    return out, nfree
+
+
+def _csr_to_scipy(ptr, free=True):
+    import scipy.sparse as sp
+    A = ptr.contents
+    n, nnz = A.n, A.nnz
+    rp = np.ctypeslib.as_array(A.rowptr, shape=(n + 1,)).copy()
+    if nnz:
+        ci = np.ctypeslib.as_array(A.col, shape=(nnz,)).copy()
+        va = np.ctypeslib.as_array(A.val, shape=(nnz,)).copy()
+    else:
+        ci = np.zeros(0, np.int32)
+        va = np.zeros(0)
+    ncols = n
+    M = sp.csr_matrix((va, ci, rp), shape=(n, max(ncols, int(ci.max()) + 1 if nnz else n)))
+    if free:
+        lib().oc_csr_free(ptr)
+    return M
+
+
+def assemble(patch_list, dofs, nfree, mode=0):
+    """Returns (scipy CSR matrix with sorted indices, rhs)."""
+    P = Patches(patch_list)
+    rhs = np.zeros(nfree)
+    dofs = np.ascontiguousarray(dofs, dtype=np.int32)
+    ptr = lib().oc_assemble(P.n, P.arr, dofs, nfree, mode, rhs)
+    return _csr_to_scipy(ptr), rhs
+
+
+def assemble_rows(patch, dofs, nfree, rows, maxc=1024):
+    """Full rows (cols, vals) and rhs entries of a single-patch problem for the
+    requested free DOFs, from the elements in their supports."""
+    P = Patches([patch])
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    n = rows.size
+    counts = np.zeros(n, np.int32)
+    cols = np.zeros(n * maxc, np.int32)
+    vals = np.zeros(n * maxc)
+    rhs = np.zeros(n)
+    lib().oc_assemble_rows(P.arr, np.ascontiguousarray(dofs, np.int32), nfree, n, rows, maxc,
+                           counts, cols, vals, rhs)
+    out = []
+    for t in range(n):
+        c = int(counts[t])
+        out.append((cols[t * maxc:t * maxc + c].copy(), vals[t * maxc:t * maxc + c].copy()))
+    return out, rhs
+
+
+class _CsrHolder:
+    def __init__(self, M):
+        M = M.tocsr()
+        M.sort_indices()
+        self.rp = np.ascontiguousarray(M.indptr, dtype=np.int32)
+        self.ci = np.ascontiguousarray(M.indices, dtype=np.int32)
+        self.va = np.ascontiguousarray(M.data, dtype=np.float64)
+        self.s = OcCsr()
+        self.s.n = M.shape[0]
+        self.s.nnz = M.nnz
+        self.s.rowptr = self.rp.ctypes.data_as(C.POINTER(C.c_int))
+        self.s.col = self.ci.ctypes.data_as(C.POINTER(C.c_int))
+        self.s.val = self.va.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def cg_solve(A, b, x0=None, tol=1e-10, maxit=1000, precond="jacobi", tau=1.0, mg=None,
+             history=False):
+    H = _CsrHolder(A)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, dtype=np.float64).copy()
+    err = C.c_double(0)
+    pc = {"none": 0, "jacobi": 1, "mg": 2}[precond]
+    hist = np.zeros(maxit + 1) if history else None
+    it = lib().oc_cg_solve(C.byref(H.s), b, x, tol, maxit, pc, tau,
+                           mg.handle if mg is not None else None, C.byref(err),
+                           hist.ctypes.data_as(C.c_void_p) if history else None)
+    if history:
+        return x, it, err.value, hist[: it + 1]
+    return x, it, err.value
+
+
+def transfer_1d(knots, p):
+    knots = np.ascontiguousarray(knots, dtype=np.float64)
+    nc = len(knots) - p - 1
+    T = np.zeros((2 * nc + 8) * nc)
+    nf = lib().oc_transfer_1d(knots, len(knots), p, T)
+    return T[: nf * nc].reshape(nf, nc)
+
+
+def transfer_multipatch(coarse_patches, dofs_c, nfree_c, dofs_f, nfree_f):
+    P = Patches(coarse_patches)
+    ptr = lib().oc_transfer_multipatch(P.n, P.arr, np.ascontiguousarray(dofs_c, np.int32), nfree_c,
+                                       np.ascontiguousarray(dofs_f, np.int32), nfree_f)
+    import scipy.sparse as sp
+    M = _csr_to_scipy(ptr)
+    return sp.csr_matrix((M.data, M.indices, M.indptr), shape=(nfree_f, nfree_c))
+
+
+class Multigrid:
+    def __init__(self, A_fine, transfers, npre=1, npost=1, damping=1.0, cgc_damping=1.0):
+        self._A = _CsrHolder(A_fine)
+        self._P = [_CsrHolder(P) for P in transfers]
+        arr = (C.POINTER(OcCsr) * max(1, len(self._P)))()
+        for i, h in enumerate(self._P):
+            arr[i] = C.pointer(h.s)
+        self._arr = arr
+        self.nlev = len(transfers) + 1
+        self.handle = lib().oc_mg_create(C.byref(self._A.s), self.nlev, arr, npre, npost,
+                                         damping, cgc_damping)
+
+    def apply(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros_like(r)
+        lib().oc_mg_apply(self.handle, r, z)
+        return z
+
+    def matrix(self, lvl):
+        return _csr_to_scipy(lib().oc_mg_matrix(self.handle, lvl), free=False)
+
+    def __del__(self):
+        try:
+            lib().oc_mg_free(self.handle)
+        except Exception:
+            pass
+
+
+def l2_error(patch_list, dofs, nfree, sol):
+    P = Patches(patch_list)
+    return lib().oc_l2_error(P.n, P.arr, np.ascontiguousarray(dofs, np.int32), nfree,
+                             np.ascontiguousarray(sol, np.float64))

@@ -1,0 +1,141 @@
+/*
+ * gsx_oracle.h -- CPU FP64 restatement of G+Smo's isogeometric Poisson
+ * assemble + solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the
+ * HIP product path (libgsx).  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it; nothing in the product links it.
+ *
+ * Every function cites the reference file:line whose algorithm it restates
+ * (paths relative to the G+Smo source tree).  Plain C99, single-threaded, so
+ * the accumulation order equals the reference's single-threaded order
+ * (patch by patch, elements lexicographic with direction 0 fastest).
+ *
+ * Pinning (see tests/test_oracle_*.py):
+ *   oc_gauss_legendre   -- reference 30-digit tables (golden fixture), exactness
+ *                          on monomials, numpy leggauss.
+ *   oc_find_span,
+ *   oc_bspline_ders     -- brute-force Cox-de Boor in exact rationals,
+ *                          partition of unity, sum of derivatives = 0,
+ *                          finite differences.
+ *   oc_mapper_build     -- hand-counted small multipatch cases, closed-form
+ *                          free-DOF counts of the glued grids.
+ *   oc_assemble         -- Kronecker closed form of affine patches computed in
+ *                          exact rationals, row sums, energy identity, symmetry.
+ *   oc_cg_solve         -- 1D FD Poisson with known solution, iterate-by-iterate
+ *                          brute force in numpy on tiny systems.
+ *   oc_mg_*             -- Galerkin identity P^T A_f P = A_c (nested spaces),
+ *                          binomial subdivision masks, two-grid brute force.
+ */
+#ifndef GSX_ORACLE_H
+#define GSX_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OC_MAXP 8
+
+/* One tensor-product B-spline patch (geometry and discretisation share the
+ * basis: isoparametric).  Control points are row-major nbasis x d in
+ * lexicographic order with direction 0 fastest (gsTensorBasis numbering,
+ * gsTensorBasis.hpp:203-236). */
+typedef struct {
+    int d;                  /* parametric = physical dimension, 2 or 3 */
+    int p[3];               /* degree per direction */
+    int nknots[3];          /* knot vector length per direction */
+    const double *knots[3]; /* open knot vectors */
+    const double *ctrl;     /* control points */
+    const double *kappa;    /* per-element coefficient (NULL: 1) */
+    const double *felem;    /* per-element constant source (fkind 0) */
+    int fkind;              /* 0: felem; 1: d*pi^2*prod sin(pi x_r) */
+} oc_patch;
+
+typedef struct {
+    int n;          /* rows = cols = free DOFs */
+    int64_t nnz;
+    int *rowptr;    /* n+1 */
+    int *col;       /* nnz, sorted per row */
+    double *val;    /* nnz */
+} oc_csr;
+
+/* --- quadrature, knots, basis ------------------------------------------ */
+void oc_gauss_legendre(int n, double *x, double *w);
+void oc_knots_uniform(int nelem, int p, double a, double b, double *knots);
+int  oc_find_span(const double *knots, int nknots, int p, double u);
+void oc_bspline_ders(const double *knots, int p, int span, double u,
+                     int nder, double *out);
+int  oc_patch_nbasis(const oc_patch *P, int dir);
+int  oc_patch_nelem(const oc_patch *P, int dir);
+
+/* --- geometry at the Gauss points of one element ------------------------ */
+/* Returns physical points (nq x d), Jacobians (nq x d x d, J[r][a] =
+ * dx_r/dxi_a), measures |det J| (nq) and weights (nq). */
+int  oc_element_geometry(const oc_patch *P, int elem, double *xq,
+                         double *jac, double *meas, double *wq);
+
+/* --- DOF mapper --------------------------------------------------------- */
+/* dir_sides: ndir pairs (patch, side) with side 1..2d (west,east,south,north,
+ * front,back).  ifaces: nif records of 8 ints (p1,s1,p2,s2,m0,f0,m1,f1):
+ * for the j-th tangential direction of patch p1 (increasing order), mj is the
+ * matching direction of patch p2 and fj=1 if its orientation is reversed.
+ * dofs_out: sum of patch basis sizes; global index per (patch, local).
+ * Returns the number of free DOFs, or -1 on error. */
+int  oc_mapper_build(int npatch, const oc_patch *patches, int ndir,
+                     const int *dir_sides, int nif, const int *ifaces,
+                     int *dofs_out);
+
+/* --- assembly ----------------------------------------------------------- */
+/* mode 0: gsExprAssembler semantics (exact-zero local entries skipped);
+ * mode 1: gsPoissonAssembler semantics (every free-free pair inserted).
+ * rhs_out: nfree.  Returns a malloc'd CSR (free with oc_csr_free). */
+oc_csr *oc_assemble(int npatch, const oc_patch *patches, const int *dofs,
+                    int nfree, int mode, double *rhs_out);
+void oc_csr_free(oc_csr *A);
+int  oc_assemble_rows(const oc_patch *P, const int *dofs, int nfree, int nrows,
+                      const int *rows, int maxc, int *counts, int *cols,
+                      double *vals, double *rhs);
+void oc_csr_spmv(const oc_csr *A, const double *x, double *y);
+
+/* --- solvers ------------------------------------------------------------ */
+typedef struct oc_mg oc_mg;
+
+/* precond: 0 none, 1 Jacobi(tau), 2 multigrid (mg != NULL).
+ * x: in/out (initial guess).  hist (optional, maxit+1): error history.
+ * Returns iterations; *err_out = final ||r||/||b||. */
+int  oc_cg_solve(const oc_csr *A, const double *b, double *x, double tol,
+                 int maxit, int precond, double tau, const oc_mg *mg,
+                 double *err_out, double *hist);
+
+/* 1D uniform-refinement transfer (midpoint insertion) for an open knot
+ * vector: fine (nf x nc) dense, row-major.  Returns nf. */
+int  oc_transfer_1d(const double *knots, int nknots, int p, double *T);
+
+/* Multigrid over nlev levels; A_fine and transfers P_l (fine_l x coarse_l,
+ * CSR) for l = 0..nlev-2 (P_l maps level l to level l+1).  Galerkin coarse
+ * operators are formed inside. */
+oc_mg *oc_mg_create(const oc_csr *A_fine, int nlev, const oc_csr *const *P,
+                    int npre, int npost, double damping, double cgc_damping);
+void oc_mg_apply(const oc_mg *mg, const double *r, double *z);
+const oc_csr *oc_mg_matrix(const oc_mg *mg, int lvl);
+void oc_mg_free(oc_mg *mg);
+
+/* Transfer between two mapped multipatch spaces (coarse -> fine) following
+ * gsMultiBasis::combineTransferMatrices; patches of the fine space are the
+ * coarse ones uniformly refined once in every direction. */
+oc_csr *oc_transfer_multipatch(int npatch, const oc_patch *coarse,
+                               const int *dofs_c, int nfree_c,
+                               const int *dofs_f, int nfree_f);
+
+/* --- reporting ---------------------------------------------------------- */
+/* L2 error of the discrete solution (free coefficients, eliminated = 0)
+ * against u = prod sin(pi x_r), with the assembly quadrature. */
+double oc_l2_error(int npatch, const oc_patch *patches, const int *dofs,
+                   int nfree, const double *sol);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,152 @@
+"""GPU parity: libgsx assembly and solve against the CPU oracle on the same
+seeded inputs.
+
+Bars: DOF map and CSR pattern bit-exact; matrix entries within 1e-12
+relative to the largest entry of their row (entries that cancel are judged
+against the row, see the survey's hard parts), load vector within 1e-12
+relative to its largest entry; Jacobi-CG iteration counts equal and
+solutions within 1e-8 relative.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from gismo_amd import gsx, problems
+
+pytestmark = pytest.mark.gpu
+
+ENTRY_TOL = 1e-12
+
+
+def _oracle(prob, mode=0):
+    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    A, b = O.assemble(prob.patches, dofs, nfree, mode)
+    return dofs, nfree, A, b
+
+
+def _row_scaled_err(rp, va, ref):
+    err = 0.0
+    for i in range(len(rp) - 1):
+        a, b = rp[i], rp[i + 1]
+        if b > a:
+            s = np.abs(ref[a:b]).max()
+            err = max(err, np.abs(va[a:b] - ref[a:b]).max() / s)
+    return err
+
+
+def _check_assembly(ctx, prob, mode=gsx.PATTERN_EXPR):
+    print(f"[case] {prob.name} mode={mode}", flush=True)
+    dofs, nfree, Ao, bo = _oracle(prob, 0 if mode == gsx.PATTERN_EXPR else 1)
+    gp = gsx.Problem(ctx, prob)
+    assert gp.nfree == nfree
+    assert np.array_equal(gp.dofs(), dofs)
+    A, b = gp.assemble(mode, gsx.FP64)
+    rp, ci, va = A.download()
+    assert np.array_equal(rp, Ao.indptr)
+    assert np.array_equal(ci, Ao.indices)
+    assert _row_scaled_err(rp, va, Ao.data) <= ENTRY_TOL
+    bg = b.download()[:nfree]
+    assert np.abs(bg - bo).max() <= ENTRY_TOL * np.abs(bo).max()
+    # bitwise symmetric: CSR arrays are the CSC arrays
+    import scipy.sparse as sp
+    M = sp.csr_matrix((va, ci, rp), shape=(nfree, nfree))
+    assert (M != M.T).nnz == 0
+    return gp, A, b, (dofs, nfree, Ao, bo)
+
+
+def test_square_p2_n64(ctx):
+    _check_assembly(ctx, problems.poisson_square(64, 2))
+
+
+@pytest.mark.parametrize("d,p,n", [(2, 1, 5), (2, 3, 7), (2, 4, 6), (3, 1, 4), (3, 2, 5), (3, 3, 6), (3, 4, 3)])
+def test_noisy_patch(ctx, d, p, n):
+    if d == 3:
+        prob = problems.poisson_noisy_cube(n=n, p=p, seed=100 + p)
+    else:
+        P = problems.box_patch(2, p, n)
+        problems.perturb_interior(P, 0.15 / n, 200 + p)
+        P.kappa = problems.lognormal_kappa(P.nelem, 0.5, 200 + p)
+        prob = problems.Problem([P], problems.all_sides(2), [], name=f"noisy2d-p{p}-n{n}")
+    _check_assembly(ctx, prob)
+
+
+def test_full_pattern_mode(ctx):
+    _check_assembly(ctx, problems.poisson_noisy_cube(n=4, p=2, seed=5), gsx.PATTERN_FULL)
+
+
+@pytest.mark.parametrize("nx,ny,n,p", [(2, 2, 4, 3), (4, 4, 3, 2), (3, 2, 4, 1)])
+def test_multipatch_grid(ctx, nx, ny, n, p):
+    _check_assembly(ctx, problems.square_grid(nx, ny, n, p))
+
+
+def test_batch_style_patch(ctx):
+    # per-element random source, lognormal kappa, p=4 3D
+    probs = problems.batch_problems(count=4, seed=3, n2d=(5, 7), n3d=(3, 4))
+    for prob in probs:
+        _check_assembly(ctx, prob)
+
+
+def test_cg_jacobi_matches_oracle(ctx):
+    prob = problems.poisson_square(64, 2)
+    gp, A, b, (dofs, nfree, Ao, bo) = _check_assembly(ctx, prob)
+    x = ctx.array(nfree).zero()
+    it, rel, ok = A.cg_solve(b, x, tol=1e-10, maxit=1000)
+    xo, ito, erro, histo = O.cg_solve(Ao, bo, tol=1e-10, maxit=1000, history=True)
+    assert ok and it == ito
+    xs = x.download()
+    assert np.linalg.norm(xs - xo) <= 1e-8 * np.linalg.norm(xo)
+    h = A.history()
+    assert len(h) == len(histo)
+    assert np.allclose(h, histo, rtol=1e-6, atol=0)
+    # L2 error against the manufactured solution
+    l2 = gp.l2_error(x)
+    l2o = O.l2_error(prob.patches, dofs, nfree, xo)
+    assert abs(l2 - l2o) <= 1e-8 * l2o
+    assert l2 < 1e-6
+
+
+def test_cg_noisy_cube_and_mixed(ctx):
+    prob = problems.poisson_noisy_cube(n=8, p=3, seed=42)
+    dofs, nfree, Ao, bo = _oracle(prob)
+    gp = gsx.Problem(ctx, prob)
+    A, b = gp.assemble(gsx.PATTERN_EXPR, gsx.MIXED)
+    x64 = ctx.array(nfree).zero()
+    it, rel, ok = A.cg_solve(b, x64, tol=1e-10, maxit=2000)
+    xo, ito, erro = O.cg_solve(Ao, bo, tol=1e-10, maxit=2000)
+    assert ok and it == ito
+    ref = x64.download()
+    assert np.linalg.norm(ref - xo) <= 1e-8 * np.linalg.norm(xo)
+    xm = ctx.array(nfree).zero()
+    itm, relm, okm = A.cg_solve(b, xm, tol=1e-10, maxit=4000, precision=gsx.MIXED)
+    assert okm and relm < 1e-10
+    xmh = xm.download()
+    # error of the mixed solution against the FP64 oracle solution
+    assert np.linalg.norm(xmh - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
+def test_determinism(ctx):
+    prob = problems.poisson_noisy_cube(n=6, p=3, seed=9)
+    gp = gsx.Problem(ctx, prob)
+    A1, b1 = gp.assemble()
+    A2, b2 = gp.assemble()
+    r1, c1, v1 = A1.download()
+    r2, c2, v2 = A2.download()
+    assert np.array_equal(v1, v2) and np.array_equal(b1.download(), b2.download())
+    x1 = ctx.array(gp.nfree).zero()
+    x2 = ctx.array(gp.nfree).zero()
+    A1.cg_solve(b1, x1, tol=1e-10)
+    A2.cg_solve(b2, x2, tol=1e-10)
+    assert np.array_equal(x1.download(), x2.download())
+
+
+def test_negative_jacobian_raises(ctx):
+    prob = problems.poisson_noisy_cube(n=4, p=2, seed=1)
+    P = prob.patches[0]
+    nb = P.nbasis_dir()
+    # fold an interior control point far across its neighbours
+    i = 2 + nb[0] * (2 + nb[1] * 2)
+    P.ctrl[i, 0] += 3.0
+    gp = gsx.Problem(ctx, prob)
+    with pytest.raises(gsx.GsxError) as e:
+        gp.assemble()
+    assert e.value.status == gsx.GSX_ERR_JACOBIAN
