@@ -342,6 +342,47 @@ static int build_mapper(gsx_problem *P, int ndir, const int32_t *dir_sides, int 
 
 extern "C" {
 
+gsx_status gsx_gauss_legendre(int32_t n, double *x, double *w)
+{
+    if (n < 1 || n > 64 || !x || !w) return GSX_ERR_ARG;
+    gauss_rule(n, x, w);
+    return GSX_OK;
+}
+
+gsx_status gsx_mapper_build(int32_t d, int32_t npatch, const int32_t *nbasis_dir, int32_t ndir,
+                            const int32_t *dir_sides, int32_t nif, const int32_t *ifaces,
+                            int32_t *dofs_out, int64_t *nfree, int64_t *nglued)
+{
+    if ((d != 2 && d != 3) || npatch <= 0 || !nbasis_dir || !dofs_out || (ndir > 0 && !dir_sides) ||
+        (nif > 0 && !ifaces))
+        return GSX_ERR_ARG;
+    gsx_problem P;
+    P.d = d;
+    P.npatch = npatch;
+    int64_t off = 0;
+    for (int k = 0; k < npatch; ++k) {
+        HostPatch h{};
+        h.d = d;
+        h.nbasis = 1;
+        for (int j = 0; j < 3; ++j) {
+            h.nb[j] = j < d ? nbasis_dir[3 * k + j] : 1;
+            if (h.nb[j] < 1) return GSX_ERR_ARG;
+            h.nbasis *= h.nb[j];
+        }
+        P.hp.push_back(h);
+        P.boff.push_back(off);
+        off += h.nbasis;
+    }
+    P.nbasis = off;
+    std::string err;
+    const int nf = build_mapper(&P, ndir, dir_sides, nif, ifaces, err);
+    if (nf < 0) return GSX_ERR_ARG;
+    std::memcpy(dofs_out, P.dofs.data(), sizeof(int) * off);
+    if (nfree) *nfree = nf;
+    if (nglued) *nglued = P.nglued;
+    return GSX_OK;
+}
+
 gsx_status gsx_ctx_create(int32_t device, gsx_ctx **out)
 {
     if (!out) return GSX_ERR_ARG;

@@ -70,6 +70,9 @@ _dp = C.POINTER(C.c_double)
 # (name, argtypes) of every exported function; restype is c_int (gsx_status)
 # unless listed in _RESTYPES.
 SIGNATURES = {
+    "gsx_gauss_legendre": [C.c_int32, _dp, _dp],
+    "gsx_mapper_build": [C.c_int32, C.c_int32, _i32p, C.c_int32, _i32p, C.c_int32, _i32p, _i32p,
+                         _i64p, _i64p],
     "gsx_ctx_create": [C.c_int32, _pp],
     "gsx_ctx_destroy": [_vp],
     "gsx_last_error": [_vp],
@@ -123,6 +126,32 @@ def _check(ctx, status, allow=()):
         msg = lib().gsx_last_error(ctx).decode() if ctx else ""
         raise GsxError(status, msg)
     return status
+
+
+def gauss_legendre(n):
+    """Host-side Gauss-Legendre rule of libgsx (used by the kernels)."""
+    x = np.zeros(n)
+    w = np.zeros(n)
+    _check(None, lib().gsx_gauss_legendre(n, x.ctypes.data_as(_dp), w.ctypes.data_as(_dp)))
+    return x, w
+
+
+def mapper(problem):
+    """libgsx's DOF mapper (host): (dofs per (patch, local), nfree, nglued)."""
+    d = problem.d
+    nbd = np.ones((len(problem.patches), 3), np.int32)
+    for k, P in enumerate(problem.patches):
+        nbd[k, :d] = P.nbasis_dir()
+    ds = np.ascontiguousarray(np.asarray(problem.dir_sides, dtype=np.int32).reshape(-1))
+    ifs = np.ascontiguousarray(np.asarray(problem.ifaces, dtype=np.int32).reshape(-1))
+    out = np.zeros(int(nbd.prod(axis=1).sum()), np.int32)
+    nf, ng = C.c_int64(), C.c_int64()
+    _check(None, lib().gsx_mapper_build(
+        d, len(problem.patches), nbd.ctypes.data_as(_i32p), len(problem.dir_sides),
+        ds.ctypes.data_as(_i32p) if ds.size else None, len(problem.ifaces),
+        ifs.ctypes.data_as(_i32p) if ifs.size else None, out.ctypes.data_as(_i32p),
+        C.byref(nf), C.byref(ng)))
+    return out, nf.value, ng.value
 
 
 class DeviceArray:

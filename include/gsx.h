@@ -96,6 +96,19 @@ gsx_status gsx_copy_to_host(gsx_ctx *ctx, void *host, const void *dev, int64_t b
 /* Bytes of device memory currently held by the context. */
 gsx_status gsx_ctx_memory(gsx_ctx *ctx, int64_t *bytes_in_use);
 
+/* ---- host-only building blocks (no device needed) ---------------------- */
+/* Gauss-Legendre rule with n nodes on [-1,1], ascending: the values of
+ * gsGaussRule::lookupReference (src/gsAssembler/gsGaussRule.hpp:218-547). */
+gsx_status gsx_gauss_legendre(int32_t n, double *x_host, double *w_host);
+/* The DOF mapper alone (what gsx_problem_create uses): nbasis_dir holds
+ * npatch x 3 basis sizes per direction; dofs_out receives the global index
+ * of every (patch, local) function; gsDofMapper::finalize numbering
+ * (src/gsCore/gsDofMapper.cpp:240-340). */
+gsx_status gsx_mapper_build(int32_t d, int32_t npatch, const int32_t *nbasis_dir_host,
+                            int32_t ndir, const int32_t *dir_sides_host, int32_t nif,
+                            const int32_t *ifaces_host, int32_t *dofs_out_host,
+                            int64_t *nfree, int64_t *nglued);
+
 /* ---- problem: patches + DOF mapper ------------------------------------ */
 /* Replaces gsMultiPatch/gsMultiBasis + gsFeSpace::setup(bc, dirichlet, 0)
  * building the gsDofMapper (src/gsAssembler/gsExpressions.h:1091-1121,

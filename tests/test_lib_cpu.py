@@ -1,0 +1,138 @@
+"""libgsx on the CPU (no GPU calls): the library loads and exports every
+function include/gsx.h declares; its host-side building blocks (Gauss rule,
+DOF mapper) agree with the reference tables and with the oracle."""
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gismo_amd import gsx, problems
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = []
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        if h.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", h)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names += re.findall(r"\b(gsx_[a-z0-9_]+)\s*\(", txt)
+    return sorted(set(names))
+
+
+def test_exports_every_declared_symbol():
+    L = gsx.lib()
+    names = _declared()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    # and the Python binding covers them
+    assert not [n for n in names if n not in gsx.SIGNATURES]
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    try:
+        gsx.Context(0)
+    except gsx.GsxError as e:
+        assert e.status in (gsx.GSX_ERR_NO_DEVICE, gsx.GSX_ERR_HIP)
+    else:
+        pytest.skip("a GPU is visible")
+
+
+def test_gauss_rule_matches_reference_tables():
+    tab = json.load(open(os.path.join(ROOT, "tests", "golden", "gauss_legendre_reference.json")))["rules"]
+    for n_str, r in tab.items():
+        x, w = gsx.gauss_legendre(int(n_str))
+        xr = np.array([float(s) for s in r["x"]])
+        wr = np.array([float(s) for s in r["w"]])
+        assert np.max(np.abs(x - xr)) <= 2.3e-16
+        assert np.max(np.abs(w - wr)) <= 2.3e-16 * max(1.0, wr.max())
+        xo, wo = O.gauss_legendre(int(n_str))
+        assert np.array_equal(x, xo) and np.array_equal(w, wo)
+
+
+def _cube_pair(flip_axes, perm, n=3, p=2):
+    """Two unit cubes glued along x=1; the second one's parametrisation is
+    permuted/flipped so the interface needs the orientation map."""
+    A = problems.box_patch(3, p, n, lo=[0, 0, 0], hi=[1, 1, 1])
+    B = problems.box_patch(3, p, n, lo=[1, 0, 0], hi=[2, 1, 1])
+    # reparametrise B: new parametric direction k follows old direction perm[k]
+    nb = B.nbasis_dir()
+    C = B.ctrl.reshape(nb[2], nb[1], nb[0], 3)           # [z][y][x]
+    C = np.transpose(C, (2, 1, 0, 3))                     # [x][y][z]
+    C = np.transpose(C, (perm[0], perm[1], perm[2], 3))   # [new0][new1][new2]
+    for ax in flip_axes:
+        C = np.flip(C, axis=ax)
+    B.ctrl = np.ascontiguousarray(np.transpose(C, (2, 1, 0, 3)).reshape(-1, 3))
+    # which side of B is the physical plane x = 1 ?
+    old_dir0 = perm.index(0)              # new direction that carries old x
+    side = 2 * old_dir0 + (2 if old_dir0 in flip_axes else 1)
+    # tangential directions of A (1, 2) map to the new directions carrying y, z
+    m0, m1 = perm.index(1), perm.index(2)
+    f0, f1 = int(m0 in flip_axes), int(m1 in flip_axes)
+    iface = (0, 2, 1, side, m0, f0, m1, f1)
+    prob = problems.Problem([A, B], [], [iface])
+    return prob
+
+
+@pytest.mark.parametrize("flip,perm", [((), (0, 1, 2)), ((1,), (0, 1, 2)), ((0, 2), (0, 1, 2)),
+                                       ((), (1, 0, 2)), ((2,), (2, 0, 1)), ((0, 1), (1, 2, 0))])
+def test_mapper_3d_orientations(flip, perm):
+    prob = _cube_pair(list(flip), list(perm))
+    # Dirichlet on every side except the glued ones
+    dside = [s for s in range(1, 7) if s != 2]
+    bside = prob.ifaces[0][3]
+    prob.dir_sides = [(0, s) for s in dside] + [(1, s) for s in range(1, 7) if s != bside]
+    dg, nf, ng = gsx.mapper(prob)
+    do, nfo = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    assert nf == nfo and np.array_equal(dg, do)
+    # glued functions sit at the same physical control point
+    A, B = prob.patches
+    pos = {}
+    for k, P in enumerate(prob.patches):
+        for l in range(P.nbasis):
+            g = dg[k * A.nbasis + l]
+            x = tuple(np.round(P.ctrl[l], 12))
+            if g in pos:
+                assert pos[g] == x
+            pos[g] = x
+    assert ng == len(pos)
+
+
+@pytest.mark.parametrize("nx,ny,n,p", [(4, 4, 5, 3), (3, 2, 4, 1), (1, 3, 6, 2), (2, 2, 1, 1)])
+def test_mapper_grids_match_oracle(nx, ny, n, p):
+    prob = problems.square_grid(nx, ny, n, p)
+    dg, nf, ng = gsx.mapper(prob)
+    do, nfo = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    assert nf == nfo and np.array_equal(dg, do)
+
+
+def test_mapper_partial_dirichlet_and_flipped_2d():
+    # two squares glued with the second one's y direction reversed, Dirichlet
+    # only on two sides (coupled DOFs that are also eliminated at the ends)
+    A = problems.box_patch(2, 2, 4, lo=[0, 0], hi=[1, 1])
+    B = problems.box_patch(2, 2, 4, lo=[1, 0], hi=[2, 1])
+    nb = B.nbasis_dir()
+    C = B.ctrl.reshape(nb[1], nb[0], 2)[::-1]
+    B.ctrl = np.ascontiguousarray(C.reshape(-1, 2))
+    prob = problems.Problem([A, B], [(0, 3), (1, 2)], [(0, 2, 1, 1, 1, 1, 0, 0)])
+    dg, nf, ng = gsx.mapper(prob)
+    do, nfo = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    assert nf == nfo and np.array_equal(dg, do)
+    pos = {}
+    for k, P in enumerate(prob.patches):
+        for l in range(P.nbasis):
+            g = dg[k * A.nbasis + l]
+            x = tuple(np.round(P.ctrl[l], 12))
+            assert pos.setdefault(g, x) == x
+
+
+def test_mapper_rejects_bad_input():
+    prob = problems.square_grid(2, 1, 3, 2)
+    prob.ifaces = [(0, 2, 5, 1, 1, 0, 0, 0)]
+    with pytest.raises(gsx.GsxError):
+        gsx.mapper(prob)
