@@ -689,6 +689,211 @@ __global__ __launch_bounds__(256) void k_gather(const PatchDev *__restrict__ pat
     if (lane == 0) rhs[r] = racc;
 }
 
+// ---------------------------------------------------------------------------
+// Single-patch gather.  On one patch whose free functions form the box
+// [flo, fhi] (Dirichlet sides eliminated), row i and column j = i + o with
+// |o_k| <= P meet in the elements e_k in [max(0, max(i_k, j_k) - P),
+// min(nel_k - 1, min(i_k, j_k))]; the contribution of element e is
+// K_e(i - e, j - e).  The wavefront of row i sums, element by element in
+// lexicographic order (the reference's order), every K_e row of i into the
+// (2P+1)^D stencil slots (slot = o + P per direction: no search), then
+// writes the CSR values of the row (window clipped to the free box,
+// lexicographic = ascending column) and the structured operator's slots.
+// ---------------------------------------------------------------------------
+struct PatchBox {
+    int flo[3], fhi[3];   // free functions per direction (inclusive)
+    int nb[3], nel[3];
+    int padded[3];        // padded grid dims (structured operator)
+};
+
+template <int D, int P>
+__global__ __launch_bounds__(256) void k_gather_patch(PatchBox bx, int nfree,
+                                                      const int *__restrict__ f2g,
+                                                      const int *__restrict__ rowptr,
+                                                      const double *__restrict__ Kbuf,
+                                                      const double *__restrict__ Fbuf,
+                                                      double *__restrict__ val,
+                                                      double *__restrict__ rhs,
+                                                      unsigned char *__restrict__ keep,
+                                                      double *__restrict__ s64,
+                                                      float *__restrict__ s32)
+{
+    constexpr int W = 2 * P + 1;
+    constexpr int NS = (D == 3) ? W * W * W : W * W;
+    constexpr int NB1 = P + 1;
+    constexpr int NB = (D == 3) ? NB1 * NB1 * NB1 : NB1 * NB1;
+    __shared__ double sAcc[4][NS];
+    __shared__ unsigned char sNz[4][NS];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int r = blockIdx.x * 4 + w;
+    if (r >= nfree) return;
+    for (int k = lane; k < NS; k += kWave) {
+        sAcc[w][k] = 0.0;
+        sNz[w][k] = 0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int g = f2g[r];
+    int ic[3];
+    ic[0] = g % bx.padded[0] - P;
+    ic[1] = (g / bx.padded[0]) % bx.padded[1] - P;
+    ic[2] = (D == 3) ? g / (bx.padded[0] * bx.padded[1]) - P : 0;
+    int elo[3] = {0, 0, 0}, ehi[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        elo[k] = max(0, ic[k] - P);
+        ehi[k] = min(bx.nel[k] - 1, ic[k]);
+    }
+    double racc = 0.0;
+    const int ne0 = ehi[0] - elo[0] + 1, ne1 = ehi[1] - elo[1] + 1, ne2 = ehi[2] - elo[2] + 1;
+    const int ne = ne0 * ne1 * ne2;
+    // element rows in flight per lane; with more than one b chunk per element
+    // (NB > 64) one element at a time keeps the per-column element order
+    constexpr int U = (NB > kWave) ? 1 : 8;
+    for (int t0 = 0; t0 < ne; t0 += U) {
+        for (int b0 = 0; b0 < NB; b0 += kWave) {
+            const int b = b0 + lane;
+            const int bxx = b % NB1, byy = (b / NB1) % NB1, bzz = (D == 3) ? b / (NB1 * NB1) : 0;
+            double v[U];
+            int slot[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int t = t0 + u;
+                slot[u] = -1;
+                v[u] = 0.0;
+                if (t < ne && b < NB) {
+                    const int ex = elo[0] + t % ne0, ey = elo[1] + (t / ne0) % ne1;
+                    const int ez = elo[2] + t / (ne0 * ne1);
+                    const int jx = ex + bxx, jy = ey + byy, jz = ez + bzz;
+                    bool fr = jx >= bx.flo[0] && jx <= bx.fhi[0] && jy >= bx.flo[1] && jy <= bx.fhi[1];
+                    if (D == 3) fr = fr && jz >= bx.flo[2] && jz <= bx.fhi[2];
+                    if (fr) {
+                        const int64_t el = ex + (int64_t)bx.nel[0] * (ey + (int64_t)bx.nel[1] * ez);
+                        const int la = (ic[0] - ex) + NB1 * ((ic[1] - ey) + NB1 * (ic[2] - ez));
+                        v[u] = Kbuf[el * (NB * NB) + (int64_t)la * NB + b];
+                        slot[u] = (D == 3) ? ((jz - ic[2] + P) * W + (jy - ic[1] + P)) * W + (jx - ic[0] + P)
+                                           : (jy - ic[1] + P) * W + (jx - ic[0] + P);
+                    }
+                }
+            }
+            // element order is kept: the slots of one element are distinct,
+            // successive elements are applied one after the other
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (slot[u] >= 0) {
+                    sAcc[w][slot[u]] += v[u];
+                    sNz[w][slot[u]] |= (v[u] != 0.0);
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        for (int u = 0; u < U && t0 + u < ne; ++u) {
+            const int t = t0 + u;
+            const int ex = elo[0] + t % ne0, ey = elo[1] + (t / ne0) % ne1, ez = elo[2] + t / (ne0 * ne1);
+            const int64_t el = ex + (int64_t)bx.nel[0] * (ey + (int64_t)bx.nel[1] * ez);
+            const int la = (ic[0] - ex) + NB1 * ((ic[1] - ey) + NB1 * (ic[2] - ez));
+            racc += Fbuf[el * NB + la];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (s64)
+        for (int k = lane; k < NS; k += kWave) s64[(int64_t)k * nfree + r] = sAcc[w][k];
+    if (s32)
+        for (int k = lane; k < NS; k += kWave) s32[(int64_t)k * nfree + r] = (float)sAcc[w][k];
+    // CSR row: window clipped to the free box, lexicographic
+    int lo[3] = {0, 0, 0}, wd[3] = {1, 1, 1};
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        lo[k] = max(bx.flo[k], ic[k] - P);
+        wd[k] = min(bx.fhi[k], ic[k] + P) - lo[k] + 1;
+    }
+    const int rb = rowptr[r], len = rowptr[r + 1] - rb;
+    for (int q = lane; q < len; q += kWave) {
+        const int qx = q % wd[0], qy = (q / wd[0]) % wd[1], qz = (D == 3) ? q / (wd[0] * wd[1]) : 0;
+        const int slot = (D == 3)
+            ? ((lo[2] + qz - ic[2] + P) * W + (lo[1] + qy - ic[1] + P)) * W + (lo[0] + qx - ic[0] + P)
+            : (lo[1] + qy - ic[1] + P) * W + (lo[0] + qx - ic[0] + P);
+        val[rb + q] = sAcc[w][slot];
+        if (keep) keep[rb + q] = sNz[w][slot];
+    }
+    if (lane == 0) rhs[r] = racc;
+}
+
+// Single-patch pattern: row r's columns are the free functions of the
+// window [i - P, i + P] clipped to the free box, lexicographic (= ascending
+// free index).  One wavefront per row, lanes write consecutive columns.
+template <int D, int P>
+__global__ __launch_bounds__(256) void k_rowfill_patch(PatchBox bx, int nfree, const int *__restrict__ f2g,
+                                                       const int *__restrict__ rowptr,
+                                                       int *__restrict__ col)
+{
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int r = blockIdx.x * 4 + w;
+    if (r >= nfree) return;
+    const int g = f2g[r];
+    int ic[3];
+    ic[0] = g % bx.padded[0] - P;
+    ic[1] = (g / bx.padded[0]) % bx.padded[1] - P;
+    ic[2] = (D == 3) ? g / (bx.padded[0] * bx.padded[1]) - P : 0;
+    int lo[3] = {0, 0, 0}, wd[3] = {1, 1, 1}, fw[3] = {1, 1, 1};
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        lo[k] = max(bx.flo[k], ic[k] - P);
+        wd[k] = min(bx.fhi[k], ic[k] + P) - lo[k] + 1;
+        fw[k] = bx.fhi[k] - bx.flo[k] + 1;
+    }
+    const int rb = rowptr[r], len = rowptr[r + 1] - rb;
+    for (int q = lane; q < len; q += kWave) {
+        const int qx = q % wd[0], qy = (q / wd[0]) % wd[1], qz = (D == 3) ? q / (wd[0] * wd[1]) : 0;
+        const int jx = lo[0] + qx - bx.flo[0], jy = lo[1] + qy - bx.flo[1];
+        const int jz = (D == 3) ? lo[2] + qz - bx.flo[2] : 0;
+        col[rb + q] = jx + fw[0] * (jy + fw[1] * jz);
+    }
+}
+
+hipError_t launch_rowfill_patch(int d, int p, const int *flo, const int *fhi, const int *nb,
+                                const int *nel, const int *padded, int nfree, const int *f2g,
+                                const int *rowptr, int *col, hipStream_t s)
+{
+    if (nfree <= 0) return hipSuccess;
+    PatchBox bx;
+    for (int k = 0; k < 3; ++k) {
+        bx.flo[k] = flo[k];
+        bx.fhi[k] = fhi[k];
+        bx.nb[k] = nb[k];
+        bx.nel[k] = nel[k];
+        bx.padded[k] = padded[k];
+    }
+    const dim3 grid((nfree + 3) / 4), blk(256);
+#define GSX_F(D, P) if (d == D && p == P) { hipLaunchKernelGGL((k_rowfill_patch<D, P>), grid, blk, 0, s, bx, nfree, f2g, rowptr, col); return hipGetLastError(); }
+    GSX_F(2, 1) GSX_F(2, 2) GSX_F(2, 3) GSX_F(2, 4)
+    GSX_F(3, 1) GSX_F(3, 2) GSX_F(3, 3) GSX_F(3, 4)
+#undef GSX_F
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_gather_patch(int d, int p, const int *flo, const int *fhi, const int *nb,
+                               const int *nel, const int *padded, int nfree, const int *f2g,
+                               const int *rowptr, const double *Kbuf, const double *Fbuf,
+                               double *val, double *rhs, unsigned char *keep, double *s64,
+                               float *s32, hipStream_t s)
+{
+    if (nfree <= 0) return hipSuccess;
+    PatchBox bx;
+    for (int k = 0; k < 3; ++k) {
+        bx.flo[k] = flo[k];
+        bx.fhi[k] = fhi[k];
+        bx.nb[k] = nb[k];
+        bx.nel[k] = nel[k];
+        bx.padded[k] = padded[k];
+    }
+    const dim3 grid((nfree + 3) / 4), blk(256);
+#define GSX_G(D, P) if (d == D && p == P) { hipLaunchKernelGGL((k_gather_patch<D, P>), grid, blk, 0, s, bx, nfree, f2g, rowptr, Kbuf, Fbuf, val, rhs, keep, s64, s32); return hipGetLastError(); }
+    GSX_G(2, 1) GSX_G(2, 2) GSX_G(2, 3) GSX_G(2, 4)
+    GSX_G(3, 1) GSX_G(3, 2) GSX_G(3, 3) GSX_G(3, 4)
+#undef GSX_G
+    return hipErrorInvalidValue;
+}
+
 // Compaction after the exact-zero rule removed entries: new counts per row.
 __global__ void k_keep_count(const int *__restrict__ rowptr, const unsigned char *__restrict__ keep,
                              int n, int *__restrict__ counts)
@@ -726,18 +931,21 @@ __global__ void k_count_zero_flags(const unsigned char *__restrict__ keep, int64
     if ((threadIdx.x % kWave) == 0 && bal) atomicAdd(dropped, (unsigned long long)__popcll(bal));
 }
 
-__global__ void k_min_reduce(const double *__restrict__ v, int64_t n, double *__restrict__ out)
+// min over v[0..n): stage 1 writes one minimum per block into part[],
+// stage 2 (one block) reduces part[0..nb).
+__global__ void k_min_stage(const double *__restrict__ v, int64_t n, double *__restrict__ out)
 {
     __shared__ double s[256];
     double m = 1e300;
-    for (int64_t i = threadIdx.x; i < n; i += 256) m = fmin(m, v[i]);
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        m = fmin(m, v[i]);
     s[threadIdx.x] = m;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
         if (threadIdx.x < off) s[threadIdx.x] = fmin(s[threadIdx.x], s[threadIdx.x + off]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = s[0];
+    if (threadIdx.x == 0) out[blockIdx.x] = s[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -945,9 +1153,14 @@ hipError_t launch_keep(const int *rowptr, const int *col, const double *val,
     return hipGetLastError();
 }
 
+// out must hold 1 + 256 doubles (the last 256 are block minima)
 hipError_t launch_min(const double *v, int64_t n, double *out, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_min_reduce, dim3(1), dim3(256), 0, s, v, n, out);
+    int64_t nb = (n + 255) / 256;
+    if (nb > 256) nb = 256;
+    if (nb < 1) nb = 1;
+    hipLaunchKernelGGL(k_min_stage, dim3((unsigned)nb), dim3(256), 0, s, v, n, out + 1);
+    hipLaunchKernelGGL(k_min_stage, dim3(1), dim3(256), 0, s, out + 1, nb, out);
     return hipGetLastError();
 }
 

@@ -53,6 +53,7 @@ struct CGState {
     double rz_new;
     double rel_since_update; // mixed: ||r32||^2 relative tracking
     double rr_ref;    // mixed: ||r||^2 at the last reliable update
+    double rzs[2];    // (r, z) by iteration parity (grid-layout solver)
     int it;           // completed iterations
     int done;         // 1 once err < tol (later launches exit at once)
     int maxit;

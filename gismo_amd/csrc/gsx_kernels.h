@@ -25,6 +25,14 @@ hipError_t launch_gather(const PatchDev *patches, const int *occ_ptr, const int 
                          const int64_t *occ_local, int nfree, const int *rowptr, const int *col,
                          const double *Kbuf, const double *Fbuf, double *val, double *rhs,
                          unsigned char *keep, int maxrow, hipStream_t s);
+hipError_t launch_rowfill_patch(int d, int p, const int *flo, const int *fhi, const int *nb,
+                                const int *nel, const int *padded, int nfree, const int *f2g,
+                                const int *rowptr, int *col, hipStream_t s);
+hipError_t launch_gather_patch(int d, int p, const int *flo, const int *fhi, const int *nb,
+                               const int *nel, const int *padded, int nfree, const int *f2g,
+                               const int *rowptr, const double *Kbuf, const double *Fbuf,
+                               double *val, double *rhs, unsigned char *keep, double *s64,
+                               float *s32, hipStream_t s);
 hipError_t launch_keep(const int *rowptr, const int *col, const double *val,
                        const unsigned char *keep, int n, int64_t nnz, int *counts, int *nrowptr,
                        int *ncol, double *nval, long long *bsum, long long *total,
@@ -78,5 +86,45 @@ hipError_t launch_cg_scalar(int step, const double *p0, int np0, const double *p
                             CGState *st, double *hist, hipStream_t s);
 hipError_t launch_cg_reliable(const double *p0, int np, CGState *st, double *hist,
                               hipStream_t s);
+
+// structured operator + grid-layout CG (gsx_stencil.hip)
+int sspmv_blocks(int nfree);
+int vec_blocks(int64_t n);
+hipError_t launch_fill_f64(double *a, int64_t n, double v, hipStream_t s);
+hipError_t launch_fill_f32(float *a, int64_t n, float v, hipStream_t s);
+hipError_t launch_csr_to_stencil(int d, int p, int nfree, const int *rp, const int *ci,
+                                 const double *v, const int *f2g, int nbx, int nby, double *s64,
+                                 float *s32, hipStream_t s);
+hipError_t launch_scatter_f64_to_grid(int nfree, const int *f2g, const double *src, double *dst,
+                                      hipStream_t s);
+hipError_t launch_gather_f64_from_grid(int nfree, const int *f2g, const double *src, double *dst,
+                                       hipStream_t s);
+hipError_t launch_sspmv_f64(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz,
+                            const double *sv, const double *x, double *y, const double *b,
+                            double *part, const CGState *st, hipStream_t s);
+hipError_t launch_sspmv_f32(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz,
+                            const float *sv, const float *x, float *y, const float *b,
+                            double *part, const CGState *st, hipStream_t s);
+hipError_t launch_cg2_update_f64(int n, double *x, double *r, const double *p, const double *t,
+                                 double *z, const double *diag, double tau, const double *pap,
+                                 int npap, double *prr, double *prz, CGState *st, int k,
+                                 hipStream_t s);
+hipError_t launch_cg2_update_f32(int n, float *x, float *r, const float *p, const float *t,
+                                 float *z, const float *diag, double tau, const double *pap,
+                                 int npap, double *prr, double *prz, CGState *st, int k,
+                                 hipStream_t s);
+hipError_t launch_cg2_direction_f64(int n, double *p, const double *z, const double *prr,
+                                    const double *prz, int np, CGState *st, int k, double *hist,
+                                    hipStream_t s);
+hipError_t launch_cg2_direction_f32_mixed(int n, float *p, const float *z, const double *prr,
+                                          const double *prz, int np, CGState *st, int k,
+                                          hipStream_t s);
+hipError_t launch_cg2_precond_f64(int n, const double *r, double *z, const double *diag,
+                                  double tau, double *prz, hipStream_t s);
+hipError_t launch_cg2_precond_f32(int n, const float *r, float *z, const float *diag, double tau,
+                                  double *prz, hipStream_t s);
+hipError_t launch_set_rz(const double *parts, int np, CGState *st, int slot, hipStream_t s);
+hipError_t launch_grid_f64_to_f32(int64_t n, const double *a, float *b, hipStream_t s);
+hipError_t launch_accum_grid(int64_t n, double *x, float *xc, hipStream_t s);
 
 }  // namespace gsx

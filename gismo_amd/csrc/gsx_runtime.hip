@@ -27,8 +27,28 @@ namespace {
 
 using DevPtr = std::shared_ptr<void>;
 
+// Device memory of a context: bytes in use plus a cache of released blocks
+// keyed by (rounded) size, so repeated assemblies and solves of the same
+// shapes reuse memory instead of calling hipMalloc/hipFree each time.
 struct Counter {
-    std::atomic<int64_t> bytes{0};
+    std::atomic<int64_t> bytes{0};     // handed out
+    int64_t cached = 0;                // held in the cache
+    std::multimap<size_t, void *> cache;
+    ~Counter()
+    {
+        for (auto &kv : cache) (void)hipFree(kv.second);
+    }
+    static size_t round(size_t b)
+    {
+        const size_t g = b >= (1u << 20) ? (size_t)(2u << 20) : 256;
+        return (b + g - 1) / g * g;
+    }
+    void trim()
+    {
+        for (auto &kv : cache) (void)hipFree(kv.second);
+        cache.clear();
+        cached = 0;
+    }
 };
 
 }  // namespace
@@ -75,16 +95,27 @@ gsx_status fail(gsx_ctx *c, gsx_status s, const std::string &msg)
 DevPtr dalloc(gsx_ctx *c, size_t bytes)
 {
     void *p = nullptr;
-    if (bytes == 0) bytes = 16;
-    if (hipMalloc(&p, bytes) != hipSuccess) {
-        (void)hipGetLastError();
-        return nullptr;
-    }
+    bytes = Counter::round(bytes ? bytes : 16);
     auto mem = c->mem;
+    auto it = mem->cache.find(bytes);
+    if (it != mem->cache.end()) {
+        p = it->second;
+        mem->cache.erase(it);
+        mem->cached -= (int64_t)bytes;
+    } else if (hipMalloc(&p, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        // release the cache and retry once
+        mem->trim();
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+    }
     mem->bytes += (int64_t)bytes;
     return DevPtr(p, [mem, bytes](void *q) {
-        (void)hipFree(q);
         mem->bytes -= (int64_t)bytes;
+        mem->cache.emplace(bytes, q);
+        mem->cached += (int64_t)bytes;
     });
 }
 
@@ -215,6 +246,11 @@ struct gsx_problem {
     DevPtr d_rowptr, d_col;
     int64_t nnz = 0;
     int maxrow = 0;
+    // single patch: free DOF -> index on the tensor grid padded by p zero
+    // layers (structured operator of the solvers)
+    DevPtr d_f2g;
+    int grid_dims[3] = {1, 1, 1};
+    int box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};   // free functions per direction
     // element scratch
     DevPtr d_K, d_F, d_detmin, d_scan_bsum, d_scan_total;
     int64_t kdoubles = 0, fdoubles = 0;
@@ -682,6 +718,39 @@ gsx_status gsx_problem_create(gsx_ctx *c, int32_t npatch, const gsx_patch_desc *
         HIP_OK(c, hipMemcpyAsync(P->d_occ_local.get(), occ_local.data(),
                                  sizeof(int64_t) * occ_local.size(), hipMemcpyHostToDevice, s));
     }
+    if (npatch == 1 && nfree > 0) {
+        const HostPatch &h = P->hp[0];
+        const int p = h.p[0];
+        P->grid_dims[0] = h.nb[0] + 2 * p;
+        P->grid_dims[1] = h.nb[1] + 2 * p;
+        P->grid_dims[2] = h.d == 3 ? h.nb[2] + 2 * p : 1;
+        std::vector<int> f2g(nfree, 0);
+        for (int64_t l = 0; l < h.nbasis; ++l) {
+            const int g = P->dofs[l];
+            if (g < nfree) {
+                const int ix = (int)(l % h.nb[0]), iy = (int)((l / h.nb[0]) % h.nb[1]);
+                const int iz = h.d == 3 ? (int)(l / ((int64_t)h.nb[0] * h.nb[1])) : 0;
+                const int pz = h.d == 3 ? p : 0;
+                f2g[g] = (ix + p) + P->grid_dims[0] * ((iy + p) + P->grid_dims[1] * (iz + pz));
+            }
+        }
+        // free functions of a single patch form a box: [1 or 0, nb-2 or nb-1]
+        int64_t boxsize = 1;
+        for (int k = 0; k < 3; ++k) {
+            P->box_lo[k] = 0;
+            P->box_hi[k] = k < h.d ? h.nb[k] - 1 : 0;
+        }
+        for (int sidx = 0; sidx < ndir; ++sidx) {
+            const int side = dir_sides[2 * sidx + 1], dir = (side - 1) / 2;
+            if (side % 2 == 1) P->box_lo[dir] = 1; else P->box_hi[dir] = h.nb[dir] - 2;
+        }
+        for (int k = 0; k < h.d; ++k) boxsize *= std::max(0, P->box_hi[k] - P->box_lo[k] + 1);
+        if (boxsize != nfree) goto no_structure;
+        P->d_f2g = dalloc(c, sizeof(int) * nfree);
+        if (!P->d_f2g) return fail(c, GSX_ERR_OOM, "device allocation (grid map)");
+        HIP_OK(c, hipMemcpyAsync(P->d_f2g.get(), f2g.data(), sizeof(int) * nfree, hipMemcpyHostToDevice, s));
+    }
+no_structure:
     HIP_OK(c, hipStreamSynchronize(s));
     *out = P.release();
     return GSX_OK;
@@ -736,6 +805,27 @@ struct CGWork {
     DevPtr r32, z32, p32, t32, xc32, diag32;
 };
 
+// Structured operator of a single-patch matrix (see gsx_stencil.hip):
+// coefficients [slot][free row], vectors on the tensor grid padded by p
+// zero layers on every side.
+struct StencilOp {
+    bool available = false;   // single patch: the structure applies
+    bool built = false;       // grid workspace allocated
+    bool s64_ready = false;   // slots filled (by the gather or by conversion)
+    bool s32_ready = false;
+    int d = 0, p = 0;
+    int nx = 1, ny = 1, nz = 1;   // padded grid
+    int64_t ngrid = 0;
+    int slots = 0;
+    std::vector<int> f2g_host;
+    DevPtr f2g, s64, s32;
+    // grid-layout workspace
+    DevPtr diag64, diag32;
+    DevPtr xg, rg, bg, zg, pg, tg;            // FP64
+    DevPtr r32, z32, p32, t32, xc32;          // FP32
+    DevPtr parts;
+};
+
 struct gsx_matrix {
     gsx_ctx *ctx = nullptr;
     int n = 0;
@@ -743,6 +833,9 @@ struct gsx_matrix {
     int lpr = 8;
     DevPtr rowptr, col, val, val32, diag;
     CGWork ws;
+    StencilOp st;
+    bool use_stencil = true;
+    bool mixed = false;       // assembled with GSX_MIXED (FP32 operator available)
     std::vector<double> hist;
 };
 
@@ -788,9 +881,16 @@ static gsx_status pattern_build(gsx_problem *P)
     if (!P->d_col) return fail(c, GSX_ERR_OOM, "device allocation (col)");
     {
         KScope k(c, "pattern_fill");
-        HIP_OK(c, gsx::launch_rowfill(dp<PatchDev>(P->d_patches), dp<int>(P->d_occ_ptr),
-                                      dp<int>(P->d_occ_patch), dp<int64_t>(P->d_occ_local), n,
-                                      dp<int>(P->d_rowptr), dp<int>(P->d_col), s));
+        if (P->npatch == 1 && P->d_f2g) {
+            const HostPatch &h = P->hp[0];
+            HIP_OK(c, gsx::launch_rowfill_patch(h.d, h.p[0], P->box_lo, P->box_hi, h.nb, h.nel,
+                                                P->grid_dims, n, dp<int>(P->d_f2g), dp<int>(P->d_rowptr),
+                                                dp<int>(P->d_col), s));
+        } else {
+            HIP_OK(c, gsx::launch_rowfill(dp<PatchDev>(P->d_patches), dp<int>(P->d_occ_ptr),
+                                          dp<int>(P->d_occ_patch), dp<int64_t>(P->d_occ_local), n,
+                                          dp<int>(P->d_rowptr), dp<int>(P->d_col), s));
+        }
     }
     P->have_pattern = true;
     return GSX_OK;
@@ -820,7 +920,7 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
     if (!P->d_K) {
         P->d_K = dalloc(c, sizeof(double) * std::max<int64_t>(1, P->kdoubles));
         P->d_F = dalloc(c, sizeof(double) * std::max<int64_t>(1, P->fdoubles));
-        P->d_detmin = dalloc(c, sizeof(double) * std::max<int64_t>(1, P->nelem) + 64);
+        P->d_detmin = dalloc(c, sizeof(double) * (std::max<int64_t>(1, P->nelem) + 260));
         if (!P->d_K || !P->d_F || !P->d_detmin)
             return fail(c, GSX_ERR_OOM, "device allocation (element matrices)");
     }
@@ -850,7 +950,34 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
     if (mode == GSX_PATTERN_EXPR) keep = dalloc(c, std::max<int64_t>(1, P->nnz));
     if (!A->val || !rhs || (mode == GSX_PATTERN_EXPR && !keep))
         return fail(c, GSX_ERR_OOM, "device allocation (values)");
-    {
+    const bool single = P->npatch == 1 && P->d_f2g && P->nfree > 0;
+    if (single) {
+        // structured operator filled by the same gather
+        StencilOp &S = A->st;
+        const HostPatch &h = P->hp[0];
+        S.available = true;
+        S.d = h.d;
+        S.p = h.p[0];
+        S.nx = P->grid_dims[0];
+        S.ny = P->grid_dims[1];
+        S.nz = P->grid_dims[2];
+        S.ngrid = (int64_t)S.nx * S.ny * S.nz;
+        const int W = 2 * S.p + 1;
+        S.slots = h.d == 3 ? W * W * W : W * W;
+        S.f2g = P->d_f2g;
+        S.s64 = dalloc(c, sizeof(double) * (int64_t)S.slots * A->n);
+        if (precision == GSX_MIXED) S.s32 = dalloc(c, sizeof(float) * (int64_t)S.slots * A->n);
+        if (!S.s64 || (precision == GSX_MIXED && !S.s32))
+            return fail(c, GSX_ERR_OOM, "device allocation (structured operator)");
+        S.s64_ready = true;
+        S.s32_ready = precision == GSX_MIXED;
+        KScope k(c, "gather");
+        HIP_OK(c, gsx::launch_gather_patch(h.d, h.p[0], P->box_lo, P->box_hi, h.nb, h.nel, P->grid_dims,
+                                           P->nfree, dp<int>(P->d_f2g), dp<int>(P->d_rowptr),
+                                           dp<double>(P->d_K), dp<double>(P->d_F), dp<double>(A->val), rhs,
+                                           keep ? dp<unsigned char>(keep) : nullptr, dp<double>(S.s64),
+                                           S.s32 ? dp<float>(S.s32) : nullptr, s));
+    } else {
         KScope k(c, "gather");
         HIP_OK(c, gsx::launch_gather(dp<PatchDev>(P->d_patches), dp<int>(P->d_occ_ptr),
                                      dp<int>(P->d_occ_patch), dp<int64_t>(P->d_occ_local), P->nfree,
@@ -898,7 +1025,8 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
         return fail(c, GSX_ERR_JACOBIAN, buf);
     }
     A->lpr = gsx::spmv_lpr(A->n ? (double)A->nnz / A->n : 1.0);
-    if (precision == GSX_MIXED) {
+    A->mixed = precision == GSX_MIXED;
+    if (precision == GSX_MIXED && !A->st.available) {
         A->val32 = dalloc(c, sizeof(float) * std::max<int64_t>(1, A->nnz));
         if (!A->val32) return fail(c, GSX_ERR_OOM, "device allocation (fp32 values)");
         KScope k(c, "convert_f32");
@@ -958,11 +1086,21 @@ gsx_status gsx_spmv(gsx_matrix *A, const double *x, double *y)
 
 }  // extern "C"
 
-static gsx_status cg_workspace(gsx_matrix *A, int maxit, bool mixed)
+static gsx_status cg_workspace(gsx_matrix *A, int maxit, bool mixed, bool grid = false)
 {
     gsx_ctx *c = A->ctx;
     CGWork &w = A->ws;
     const int n = A->n;
+    if (grid) {
+        // grid-layout solver: its vectors live in the structured operator
+        if (!w.state) w.state = dalloc(c, sizeof(CGState));
+        if (w.hist_cap < maxit + 1) {
+            w.hist = dalloc(c, sizeof(double) * (maxit + 1));
+            w.hist_cap = maxit + 1;
+        }
+        if (!w.state || !w.hist) return fail(c, GSX_ERR_OOM, "device allocation (CG state)");
+        return GSX_OK;
+    }
     if (w.n != n || !w.r) {
         w = CGWork();
         w.n = n;
@@ -1182,7 +1320,192 @@ static gsx_status cg_mixed(gsx_matrix *A, int precond, double tau, const double 
     return fail(c, GSX_ERR_NOT_CONVERGED, "mixed CG reached the maximum number of iterations");
 }
 
+// ---------------------------------------------------------------------------
+// structured operator + grid-layout CG
+// ---------------------------------------------------------------------------
+constexpr int kPartStride = 2048;
+
+static gsx_status stencil_prepare(gsx_matrix *A, bool mixed)
+{
+    gsx_ctx *c = A->ctx;
+    hipStream_t s = c->stream;
+    StencilOp &S = A->st;
+    const int n = A->n;
+    const int64_t K = S.slots, N = S.ngrid;
+    KScope k(c, "stencil_build");
+    if (!S.s64_ready) {
+        S.s64 = dalloc(c, sizeof(double) * K * n);
+        if (!S.s64) return fail(c, GSX_ERR_OOM, "device allocation (structured operator)");
+        HIP_OK(c, gsx::launch_fill_f64(dp<double>(S.s64), K * n, 0.0, s));
+        HIP_OK(c, gsx::launch_csr_to_stencil(S.d, S.p, n, dp<int>(A->rowptr), dp<int>(A->col),
+                                             dp<double>(A->val), dp<int>(S.f2g), S.nx, S.ny,
+                                             dp<double>(S.s64), nullptr, s));
+        S.s64_ready = true;
+    }
+    if (mixed && !S.s32_ready) {
+        S.s32 = dalloc(c, sizeof(float) * K * n);
+        if (!S.s32) return fail(c, GSX_ERR_OOM, "device allocation (structured operator, fp32)");
+        HIP_OK(c, gsx::launch_fill_f32(dp<float>(S.s32), K * n, 0.0f, s));
+        HIP_OK(c, gsx::launch_csr_to_stencil(S.d, S.p, n, dp<int>(A->rowptr), dp<int>(A->col),
+                                             dp<double>(A->val), dp<int>(S.f2g), S.nx, S.ny, nullptr,
+                                             dp<float>(S.s32), s));
+        S.s32_ready = true;
+    }
+    if (!S.built) {
+        S.diag64 = dalloc(c, sizeof(double) * N);
+        S.xg = dalloc(c, sizeof(double) * N);
+        S.rg = dalloc(c, sizeof(double) * N);
+        S.bg = dalloc(c, sizeof(double) * N);
+        S.zg = dalloc(c, sizeof(double) * N);
+        S.pg = dalloc(c, sizeof(double) * N);
+        S.tg = dalloc(c, sizeof(double) * N);
+        S.parts = dalloc(c, sizeof(double) * 4 * kPartStride);
+        if (!S.diag64 || !S.xg || !S.rg || !S.bg || !S.zg || !S.pg || !S.tg || !S.parts)
+            return fail(c, GSX_ERR_OOM, "device allocation (grid workspace)");
+        // diagonal = centre slot of the structured operator
+        const int64_t centre = (K - 1) / 2;
+        HIP_OK(c, gsx::launch_fill_f64(dp<double>(S.diag64), N, 1.0, s));
+        HIP_OK(c, gsx::launch_scatter_f64_to_grid(n, dp<int>(S.f2g), dp<double>(S.s64) + centre * n,
+                                                  dp<double>(S.diag64), s));
+        for (DevPtr *v : {&S.xg, &S.rg, &S.bg, &S.zg, &S.pg, &S.tg})
+            HIP_OK(c, gsx::launch_fill_f64(dp<double>(*v), N, 0.0, s));
+        S.built = true;
+    }
+    if (mixed && !S.r32) {
+        S.diag32 = dalloc(c, sizeof(float) * N);
+        S.r32 = dalloc(c, sizeof(float) * N);
+        S.z32 = dalloc(c, sizeof(float) * N);
+        S.p32 = dalloc(c, sizeof(float) * N);
+        S.t32 = dalloc(c, sizeof(float) * N);
+        S.xc32 = dalloc(c, sizeof(float) * N);
+        if (!S.diag32 || !S.r32 || !S.z32 || !S.p32 || !S.t32 || !S.xc32)
+            return fail(c, GSX_ERR_OOM, "device allocation (grid workspace, fp32)");
+        HIP_OK(c, gsx::launch_grid_f64_to_f32(N, dp<double>(S.diag64), dp<float>(S.diag32), s));
+        for (DevPtr *v : {&S.r32, &S.z32, &S.p32, &S.t32, &S.xc32})
+            HIP_OK(c, gsx::launch_fill_f32(dp<float>(*v), N, 0.0f, s));
+    }
+    return GSX_OK;
+}
+
+// Grid-layout CG.  mixed == false: FP64 with the reference's recurrences and
+// stopping rule.  mixed == true: FP32 recurrences on the FP32 copy of the
+// operator with FP64 reliable updates (see cg_mixed).
+static gsx_status cg_grid(gsx_matrix *A, int precond, double tau, const double *b, double *x,
+                          double tol, int maxit, bool mixed, int *iters, double *relres)
+{
+    gsx_ctx *c = A->ctx;
+    hipStream_t s = c->stream;
+    StencilOp &S = A->st;
+    CGWork &w = A->ws;
+    const int n = A->n;
+    const int64_t N = S.ngrid;
+    const int nsp = gsx::sspmv_blocks(n), nv = gsx::vec_blocks(N);
+    double *P0 = dp<double>(S.parts), *P1 = P0 + kPartStride, *P2 = P1 + kPartStride;
+    CGState *st = dp<CGState>(w.state);
+    double *hist = mixed ? nullptr : dp<double>(w.hist);
+    const bool jac = precond == GSX_PREC_JACOBI;
+    const int *f2g = dp<int>(S.f2g);
+    const int sy = S.nx, sz = S.nx * S.ny;
+    double *xg = dp<double>(S.xg), *rg = dp<double>(S.rg), *bg = dp<double>(S.bg);
+    double *zg = dp<double>(S.zg), *pg = dp<double>(S.pg), *tg = dp<double>(S.tg);
+    const double *sv = dp<double>(S.s64), *dg = dp<double>(S.diag64);
+    CGState h{};
+    h.tol = tol;
+    h.tau = tau;
+    h.maxit = maxit;
+    h.rel_since_update = 1e-4;   // mixed: reliable update every 1e-2 of residual reduction
+    HIP_OK(c, hipMemcpyAsync(st, &h, sizeof(CGState), hipMemcpyHostToDevice, s));
+    {
+        KScope k(c, "cg_init");
+        HIP_OK(c, gsx::launch_scatter_f64_to_grid(n, f2g, b, bg, s));
+        HIP_OK(c, gsx::launch_scatter_f64_to_grid(n, f2g, x, xg, s));
+        HIP_OK(c, gsx::launch_sspmv_f64(S.d, S.p, 2, n, f2g, sy, sz, sv, xg, rg, bg, P1, nullptr, s));
+        HIP_OK(c, gsx::launch_cg2_precond_f64((int)N, bg, zg, nullptr, 1.0, P0, s));   // ||b||^2
+        HIP_OK(c, gsx::launch_cg_scalar(0, P0, nv, P1, nsp, st, hist, s));
+    }
+    GSX_TRY(read_state(c, w, h));
+    if (h.done == 3) HIP_OK(c, gsx::launch_fill_f64(x, n, 0.0, s));
+    int k = 0;
+    if (!h.done && !mixed) {
+        HIP_OK(c, gsx::launch_cg2_precond_f64((int)N, rg, zg, jac ? dg : nullptr, tau, P0, s));
+        HIP_OK(c, gsx::launch_set_rz(P0, nv, st, 0, s));
+        HIP_OK(c, hipMemcpyAsync(pg, zg, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+        const int chunk = 16;
+        while (k < maxit) {
+            const int kend = std::min(maxit, k + chunk);
+            for (; k < kend; ++k) {
+                {
+                    KScope ks(c, "spmv_f64");
+                    HIP_OK(c, gsx::launch_sspmv_f64(S.d, S.p, 1, n, f2g, sy, sz, sv, pg, tg, nullptr, P0, st, s));
+                }
+                KScope kv(c, "cg_vec");
+                HIP_OK(c, gsx::launch_cg2_update_f64((int)N, xg, rg, pg, tg, zg, jac ? dg : nullptr, tau, P0,
+                                                     nsp, P1, P2, st, k, s));
+                HIP_OK(c, gsx::launch_cg2_direction_f64((int)N, pg, zg, P1, P2, nv, st, k, hist, s));
+            }
+            GSX_TRY(read_state(c, w, h));
+            if (h.done) break;
+        }
+        HIP_OK(c, gsx::launch_gather_f64_from_grid(n, f2g, xg, x, s));
+    } else if (!h.done && mixed) {
+        float *r32 = dp<float>(S.r32), *z32 = dp<float>(S.z32), *p32 = dp<float>(S.p32);
+        float *t32 = dp<float>(S.t32), *xc = dp<float>(S.xc32);
+        const float *sv32 = dp<float>(S.s32), *d32 = dp<float>(S.diag32);
+        auto restart = [&](int slot) -> gsx_status {
+            HIP_OK(c, gsx::launch_grid_f64_to_f32(N, rg, r32, s));
+            HIP_OK(c, gsx::launch_cg2_precond_f32((int)N, r32, z32, jac ? d32 : nullptr, tau, P0, s));
+            HIP_OK(c, gsx::launch_set_rz(P0, nv, st, slot, s));
+            return GSX_OK;
+        };
+        HIP_OK(c, gsx::launch_fill_f32(xc, N, 0.0f, s));
+        GSX_TRY(restart(0));
+        HIP_OK(c, hipMemcpyAsync(p32, z32, sizeof(float) * N, hipMemcpyDeviceToDevice, s));
+        const int chunk = 8;
+        while (true) {
+            for (int j = 0; j < chunk; ++j, ++k) {
+                {
+                    KScope ks(c, "spmv_f32");
+                    HIP_OK(c, gsx::launch_sspmv_f32(S.d, S.p, 1, n, f2g, sy, sz, sv32, p32, t32, nullptr, P0, st, s));
+                }
+                KScope kv(c, "cg_vec_f32");
+                HIP_OK(c, gsx::launch_cg2_update_f32((int)N, xc, r32, p32, t32, z32, jac ? d32 : nullptr, tau, P0,
+                                                     nsp, P1, P2, st, k, s));
+                HIP_OK(c, gsx::launch_cg2_direction_f32_mixed((int)N, p32, z32, P1, P2, nv, st, k, s));
+            }
+            GSX_TRY(read_state(c, w, h));
+            k = h.it;
+            if (h.need_update || h.done == 2 || h.err < tol) {
+                KScope kr(c, "reliable_update");
+                HIP_OK(c, gsx::launch_accum_grid(N, xg, xc, s));
+                HIP_OK(c, gsx::launch_sspmv_f64(S.d, S.p, 2, n, f2g, sy, sz, sv, xg, rg, bg, P1, nullptr, s));
+                HIP_OK(c, gsx::launch_cg_reliable(P1, nsp, st, nullptr, s));
+                GSX_TRY(read_state(c, w, h));
+                if (h.done == 1 || h.done == 2) break;
+                GSX_TRY(restart(k));
+            }
+            if (h.it >= maxit) break;
+        }
+        HIP_OK(c, gsx::launch_gather_f64_from_grid(n, f2g, xg, x, s));
+    }
+    *iters = h.it;
+    *relres = h.err;
+    if (!mixed) {
+        A->hist.resize(h.it + 1);
+        HIP_OK(c, hipMemcpyAsync(A->hist.data(), hist, sizeof(double) * (h.it + 1), hipMemcpyDeviceToHost, s));
+    }
+    HIP_OK(c, hipStreamSynchronize(s));
+    if (h.done == 1 || h.done == 3 || h.err < tol) return GSX_OK;
+    return fail(c, GSX_ERR_NOT_CONVERGED, "CG reached the maximum number of iterations");
+}
+
 extern "C" {
+
+gsx_status gsx_matrix_set_operator(gsx_matrix *A, int32_t structured)
+{
+    if (!A) return GSX_ERR_ARG;
+    A->use_stencil = structured != 0;
+    return GSX_OK;
+}
 
 gsx_status gsx_cg_solve(gsx_matrix *A, int32_t precond, double tau, gsx_mg *mg, int32_t precision,
                         const double *b, double *x, double tol, int32_t maxit, int32_t *iters,
@@ -1193,15 +1516,29 @@ gsx_status gsx_cg_solve(gsx_matrix *A, int32_t precond, double tau, gsx_mg *mg, 
     if (precond == GSX_PREC_MG || mg) return fail(c, GSX_ERR_ARG, "multigrid preconditioner not available");
     if (precond != GSX_PREC_NONE && precond != GSX_PREC_JACOBI)
         return fail(c, GSX_ERR_ARG, "unknown preconditioner");
-    if (precision == GSX_MIXED && !A->val32)
+    if (precision == GSX_MIXED && !A->mixed)
         return fail(c, GSX_ERR_ARG, "mixed precision needs a matrix assembled with GSX_MIXED");
     const bool mixed = precision == GSX_MIXED;
-    gsx_status e = cg_workspace(A, std::max(1, maxit), mixed);
+    const bool grid = A->st.available && A->use_stencil;
+    if (mixed && !grid && !A->val32) {
+        // CSR mixed path on a single-patch matrix: FP32 copy made on demand
+        A->val32 = dalloc(c, sizeof(float) * std::max<int64_t>(1, A->nnz));
+        if (!A->val32) return fail(c, GSX_ERR_OOM, "device allocation (fp32 values)");
+        KScope k(c, "convert_f32");
+        HIP_OK(c, gsx::launch_f64_to_f32(A->nnz, dp<double>(A->val), dp<float>(A->val32), c->stream));
+    }
+    gsx_status e = cg_workspace(A, std::max(1, maxit), mixed && !grid, grid);
     if (e != GSX_OK) return e;
     int it = 0;
     double rr = 0.0;
-    e = mixed ? cg_mixed(A, precond, tau, b, x, tol, maxit, &it, &rr)
-              : cg_fp64(A, precond, tau, b, x, tol, maxit, &it, &rr);
+    if (grid) {
+        e = stencil_prepare(A, mixed);
+        if (e != GSX_OK) return e;
+        e = cg_grid(A, precond, tau, b, x, tol, maxit, mixed, &it, &rr);
+    } else {
+        e = mixed ? cg_mixed(A, precond, tau, b, x, tol, maxit, &it, &rr)
+                  : cg_fp64(A, precond, tau, b, x, tol, maxit, &it, &rr);
+    }
     if (iters) *iters = it;
     if (relres) *relres = rr;
     return e;

@@ -98,6 +98,7 @@ SIGNATURES = {
     "gsx_matrix_download": [_vp, _i32p, _i32p, _dp],
     "gsx_matrix_device_ptrs": [_vp, _pp, _pp, _pp],
     "gsx_spmv": [_vp, _vp, _vp],
+    "gsx_matrix_set_operator": [_vp, C.c_int32],
     "gsx_cg_solve": [_vp, C.c_int32, C.c_double, _vp, C.c_int32, _vp, _vp, C.c_double, C.c_int32,
                      _i32p, _dp],
     "gsx_cg_history": [_vp, _dp, C.c_int32, _i32p],
@@ -333,6 +334,10 @@ class Matrix:
 
     def spmv(self, x, y):
         _check(self.ctx.h, lib().gsx_spmv(self.h, x.ptr, y.ptr))
+
+    def set_operator(self, structured=True):
+        """Solver operator format: structured stencil (single patch) or CSR."""
+        _check(self.ctx.h, lib().gsx_matrix_set_operator(self.h, 1 if structured else 0))
 
     def cg_solve(self, b, x, tol=1e-10, maxit=1000, precond=PREC_JACOBI, tau=1.0,
                  precision=FP64, raise_on_nonconvergence=False):

@@ -156,6 +156,12 @@ gsx_status gsx_matrix_device_ptrs(const gsx_matrix *A, const int32_t **rowptr_de
 gsx_status gsx_problem_min_detj(const gsx_problem *prob, double *min_det);
 
 /* ---- operators ----------------------------------------------------------- */
+/* Operator format used by the solvers for A: structured = 1 (default) uses
+ * the (2p+1)^d stencil form of single-patch matrices (no column indices,
+ * vectors on the padded tensor grid); 0 forces CSR.  Multipatch matrices
+ * always use CSR.  No reference counterpart (gsMatrixOp always applies the
+ * sparse matrix). */
+gsx_status gsx_matrix_set_operator(gsx_matrix *A, int32_t structured);
 /* y = A x on the device (gsMatrixOp::apply, src/gsSolver/gsMatrixOp.h:74-75). */
 gsx_status gsx_spmv(gsx_matrix *A, const double *x_dev, double *y_dev);
 

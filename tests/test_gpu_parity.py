@@ -150,3 +150,56 @@ def test_negative_jacobian_raises(ctx):
     with pytest.raises(gsx.GsxError) as e:
         gp.assemble()
     assert e.value.status == gsx.GSX_ERR_JACOBIAN
+
+
+@pytest.mark.parametrize("d,p,n", [(3, 3, 8), (3, 2, 6), (2, 2, 16), (3, 4, 4)])
+def test_structured_operator_matches_csr(ctx, d, p, n):
+    """The stencil form of a single-patch matrix gives the CSR path's
+    iterations and solution (same products, same per-row order)."""
+    if d == 3:
+        prob = problems.poisson_noisy_cube(n=n, p=p, seed=300 + p)
+    else:
+        P = problems.box_patch(2, p, n)
+        problems.perturb_interior(P, 0.15 / n, 301)
+        P.kappa = problems.lognormal_kappa(P.nelem, 0.5, 301)
+        prob = problems.Problem([P], problems.all_sides(2), [], name="noisy2d")
+    print(f"[case] {prob.name}", flush=True)
+    dofs, nfree, Ao, bo = _oracle(prob)
+    # relres 1e-12 so that two converged solutions agree to ~cond(A)*1e-12
+    xo, ito, erro = O.cg_solve(Ao, bo, tol=1e-12, maxit=3000)
+    gp = gsx.Problem(ctx, prob)
+    A, b = gp.assemble(gsx.PATTERN_EXPR, gsx.MIXED)
+    sols = {}
+    for structured in (True, False):
+        A.set_operator(structured)
+        for prec in (gsx.FP64, gsx.MIXED):
+            x = ctx.array(nfree).zero()
+            it, rel, ok = A.cg_solve(b, x, tol=1e-12, maxit=3000, precision=prec)
+            assert ok and rel < 1e-12, (structured, prec, it, rel)
+            sols[(structured, prec)] = (it, x.download())
+    for key, (it, xs) in sols.items():
+        assert np.linalg.norm(xs - xo) <= 1e-8 * np.linalg.norm(xo), key
+    # FP64 recurrences: the oracle's iteration count; at relres 1e-12 the
+    # residual history is at rounding level, where the block-wise dot
+    # products may move the last iteration by one (exact equality is
+    # asserted at 1e-10 in test_cg_jacobi_matches_oracle)
+    assert abs(sols[(True, gsx.FP64)][0] - ito) <= 1
+    assert abs(sols[(False, gsx.FP64)][0] - ito) <= 1
+
+
+def test_structured_partial_dirichlet(ctx):
+    # free functions on the patch edge: the padded grid keeps every
+    # neighbour read inside the vector
+    P = problems.box_patch(3, 3, 5)
+    problems.perturb_interior(P, 0.1 / 5, 7)
+    prob = problems.Problem([P], [(0, 1), (0, 4)], [], name="cube-two-sides")
+    dofs, nfree, Ao, bo = _oracle(prob)
+    xo, ito, erro = O.cg_solve(Ao, bo, tol=1e-10, maxit=3000)
+    gp = gsx.Problem(ctx, prob)
+    A, b = gp.assemble(gsx.PATTERN_EXPR, gsx.MIXED)
+    for prec in (gsx.FP64, gsx.MIXED):
+        x = ctx.array(nfree).zero()
+        it, rel, ok = A.cg_solve(b, x, tol=1e-10, maxit=3000, precision=prec)
+        assert ok
+        xs = x.download()
+        assert np.linalg.norm(xs - xo) <= 1e-7 * np.linalg.norm(xo)
