@@ -87,6 +87,40 @@ hipError_t launch_cg_scalar(int step, const double *p0, int np0, const double *p
 hipError_t launch_cg_reliable(const double *p0, int np, CGState *st, double *hist,
                               hipStream_t s);
 
+// multigrid (gsx_mg.hip)
+hipError_t launch_transfer(int phase, int nfree_f, const int *occ_ptr, const int *occ_patch,
+                           const int64_t *occ_local, const int *fine_nb, const void *t1d, const int *trp,
+                           const int *tcol, const double *tval, const int *cdofs_off, const int *cdofs,
+                           const int *coarse_nb, int d, int nfree_c, int *counts, int *err,
+                           const int *rowptr, int *col, double *val, hipStream_t s);
+hipError_t launch_transpose(int phase, int n, const int *rp, const int *ci, const double *v, int *counts,
+                            const int *trp, int *cursor, int *tci, double *tv, int nt, hipStream_t s);
+int ap_width(int d);
+hipError_t launch_ap(int d, int n, const int *arp, const int *aci, const double *av, const int *prp,
+                     const int *pci, const double *pv, int *bcnt, int *bci, double *bv, int *err,
+                     hipStream_t s);
+hipError_t launch_rap(int d, int nc, const int *rrp, const int *rci, const double *rv, const int *bcnt,
+                      const int *bci, const double *bv, const int *crp, const int *cci, double *cval,
+                      int *err, hipStream_t s);
+hipError_t launch_dense_inverse(int n, int npad, const int *rp, const int *ci, const double *v,
+                                double *D, double *W, double *Pinv, hipStream_t s);
+hipError_t launch_dense_gemv_f64(int n, int npad, const double *M, const double *x, double *y,
+                                 hipStream_t s);
+hipError_t launch_prolong_add_f64(int n, const int *rp, const int *ci, const double *v,
+                                  const double *x, double *y, double alpha, hipStream_t s);
+hipError_t launch_prolong_add_f32(int n, const int *rp, const int *ci, const float *v,
+                                  const float *x, float *y, float alpha, hipStream_t s);
+hipError_t launch_restrict_f64(int n, const int *rp, const int *ci, const double *v, const double *x,
+                               double *y, hipStream_t s);
+hipError_t launch_restrict_f32(int n, const int *rp, const int *ci, const float *v, const float *x,
+                               float *y, hipStream_t s);
+hipError_t launch_residual_f64(int lpr, int n, const int *rp, const int *ci, const double *v,
+                               const double *x, const double *b, double *r, hipStream_t s);
+hipError_t launch_residual_f32(int lpr, int n, const int *rp, const int *ci, const float *v,
+                               const float *x, const float *b, float *r, hipStream_t s);
+hipError_t launch_conv_f64_f32(int64_t n, const double *a, float *b, hipStream_t s);
+hipError_t launch_conv_f32_f64(int64_t n, const float *a, double *b, hipStream_t s);
+
 // structured operator + grid-layout CG (gsx_stencil.hip)
 int sspmv_blocks(int nfree);
 int vec_blocks(int64_t n);

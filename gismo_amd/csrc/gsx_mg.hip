@@ -1,0 +1,592 @@
+// Geometric multigrid kernels of libgsx (gfx950): transfer matrices,
+// Galerkin coarse operators, dense coarse inverse, V-cycle vector steps.
+//
+// Reference path (G+Smo source tree):
+//   gsMultiGridOp::init (Galerkin A_c = P^T A P)   src/gsMultiGrid/gsMultiGrid.hpp:70-94
+//   gsMultiGridOp::multiGridStep (V/W cycle)        src/gsMultiGrid/gsMultiGrid.hpp:148-194
+//   gsMultiBasis::combineTransferMatrices           src/gsCore/gsMultiBasis.hpp:137-178
+//   gsTensorBasis::uniformRefine_withTransfer       src/gsTensor/gsTensorBasis.hpp:861-880
+//   coarse direct solve (SparseLU / Cholesky)       src/gsSolver/gsMatrixOp.h:306-370
+#include "gsx_common.h"
+#include "gsx_kernels.h"
+
+namespace gsx {
+
+// 1D transfer of one (patch, direction): CSR rows = fine 1D index.
+struct T1D {
+    int rp_off;     // offset of this matrix's row pointer array (nf + 1 ints)
+    int ent_off;    // offset of its first entry
+    int nf, nc;
+};
+
+// ---------------------------------------------------------------------------
+// Transfer P (fine free x coarse free).  Row f takes the tensor products of
+// the 1D transfer rows of its first occurrence (patch, local index);
+// interface functions get the same entries from every patch they belong to
+// (conforming interfaces), which is what take_first keeps.  Columns sorted.
+// ---------------------------------------------------------------------------
+constexpr int kMaxPRow = 128;
+
+template <bool WRITE>
+__device__ int transfer_row(int f, const int *__restrict__ occ_ptr, const int *__restrict__ occ_patch,
+                            const int64_t *__restrict__ occ_local, const int *__restrict__ fine_nb,
+                            const T1D *__restrict__ t1d, const int *__restrict__ trp,
+                            const int *__restrict__ tcol, const double *__restrict__ tval,
+                            const int *__restrict__ cdofs_off, const int *__restrict__ cdofs,
+                            const int *__restrict__ coarse_nb, int d, int nfree_c,
+                            int *__restrict__ ocol, double *__restrict__ oval)
+{
+    const int o = occ_ptr[f];
+    const int k = occ_patch[o];
+    const int64_t l = occ_local[o];
+    const int nbx = fine_nb[3 * k], nby = fine_nb[3 * k + 1];
+    int lc[3] = {(int)(l % nbx), (int)((l / nbx) % nby), d == 3 ? (int)(l / ((int64_t)nbx * nby)) : 0};
+    int b0[3], e0[3];
+    for (int j = 0; j < 3; ++j) {
+        if (j < d) {
+            const T1D &T = t1d[3 * k + j];
+            b0[j] = trp[T.rp_off + lc[j]];
+            e0[j] = trp[T.rp_off + lc[j] + 1];
+        } else {
+            b0[j] = 0;
+            e0[j] = 1;
+        }
+    }
+    const int cnx = coarse_nb[3 * k], cny = coarse_nb[3 * k + 1];
+    int cnt = 0;
+    int cc[kMaxPRow];
+    double cv[kMaxPRow];
+    for (int iz = b0[2]; iz < e0[2]; ++iz)
+        for (int iy = b0[1]; iy < e0[1]; ++iy)
+            for (int ix = b0[0]; ix < e0[0]; ++ix) {
+                const T1D &Tx = t1d[3 * k], &Ty = t1d[3 * k + 1];
+                double v = tval[Tx.ent_off + ix];
+                const int cx = tcol[Tx.ent_off + ix];
+                v *= tval[Ty.ent_off + iy];
+                const int cy = tcol[Ty.ent_off + iy];
+                int cz = 0;
+                if (d == 3) {
+                    const T1D &Tz = t1d[3 * k + 2];
+                    v *= tval[Tz.ent_off + iz];
+                    cz = tcol[Tz.ent_off + iz];
+                }
+                if (v == 0.0) continue;
+                const int g = cdofs[cdofs_off[k] + cx + cnx * (cy + cny * cz)];
+                if (g >= nfree_c) continue;
+                if (cnt >= kMaxPRow) return -1;
+                // insertion by column
+                int pos = cnt;
+                while (pos > 0 && cc[pos - 1] > g) {
+                    cc[pos] = cc[pos - 1];
+                    cv[pos] = cv[pos - 1];
+                    --pos;
+                }
+                cc[pos] = g;
+                cv[pos] = v;
+                ++cnt;
+            }
+    if (WRITE)
+        for (int i = 0; i < cnt; ++i) {
+            ocol[i] = cc[i];
+            oval[i] = cv[i];
+        }
+    return cnt;
+}
+
+__global__ void k_transfer_count(int nfree_f, const int *occ_ptr, const int *occ_patch, const int64_t *occ_local,
+                                 const int *fine_nb, const T1D *t1d, const int *trp, const int *tcol,
+                                 const double *tval, const int *cdofs_off, const int *cdofs,
+                                 const int *coarse_nb, int d, int nfree_c, int *counts, int *err)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfree_f) return;
+    const int c = transfer_row<false>(f, occ_ptr, occ_patch, occ_local, fine_nb, t1d, trp, tcol, tval,
+                                      cdofs_off, cdofs, coarse_nb, d, nfree_c, nullptr, nullptr);
+    if (c < 0) { err[0] = 1; counts[f] = 0; } else counts[f] = c;
+}
+
+__global__ void k_transfer_fill(int nfree_f, const int *occ_ptr, const int *occ_patch, const int64_t *occ_local,
+                                const int *fine_nb, const T1D *t1d, const int *trp, const int *tcol,
+                                const double *tval, const int *cdofs_off, const int *cdofs,
+                                const int *coarse_nb, int d, int nfree_c, const int *rowptr, int *col,
+                                double *val)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nfree_f) return;
+    transfer_row<true>(f, occ_ptr, occ_patch, occ_local, fine_nb, t1d, trp, tcol, tval, cdofs_off, cdofs,
+                       coarse_nb, d, nfree_c, col + rowptr[f], val + rowptr[f]);
+}
+
+// ---------------------------------------------------------------------------
+// CSR transpose (R = P^T): counts by integer atomics, then each row sorted
+// by column so the result does not depend on the atomic order.
+// ---------------------------------------------------------------------------
+__global__ void k_tr_count(int n, const int *__restrict__ rp, const int *__restrict__ ci, int *__restrict__ counts)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    for (int k = rp[r]; k < rp[r + 1]; ++k) atomicAdd(&counts[ci[k]], 1);
+}
+
+__global__ void k_tr_fill(int n, const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ v,
+                          const int *__restrict__ trp, int *__restrict__ cursor, int *__restrict__ tci,
+                          double *__restrict__ tv)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    for (int k = rp[r]; k < rp[r + 1]; ++k) {
+        const int c = ci[k];
+        const int pos = trp[c] + atomicAdd(&cursor[c], 1);
+        tci[pos] = r;
+        tv[pos] = v[k];
+    }
+}
+
+__global__ void k_sort_rows(int n, const int *__restrict__ rp, int *__restrict__ ci, double *__restrict__ v)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int a = rp[r], b = rp[r + 1];
+    for (int i = a + 1; i < b; ++i) {
+        const int c = ci[i];
+        const double x = v[i];
+        int j = i - 1;
+        while (j >= a && ci[j] > c) {
+            ci[j + 1] = ci[j];
+            v[j + 1] = v[j];
+            --j;
+        }
+        ci[j + 1] = c;
+        v[j + 1] = x;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Galerkin product A_c = P^T (A P).
+// Stage 1: B = A P, one thread per fine row, ELL rows of width WB with
+// columns sorted; products accumulated in (A column, P column) order.
+// Stage 2: A_c = R B into the coarse pattern (known: the coarse level's
+// element-sharing pattern), one thread per coarse row.
+// ---------------------------------------------------------------------------
+template <int WB>
+__global__ void k_ap(int n, const int *__restrict__ arp, const int *__restrict__ aci, const double *__restrict__ av,
+                     const int *__restrict__ prp, const int *__restrict__ pci, const double *__restrict__ pv,
+                     int *__restrict__ bcnt, int *__restrict__ bci, double *__restrict__ bv, int *__restrict__ err)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= n) return;
+    int cc[WB];
+    double cv[WB];
+    int cnt = 0;
+    for (int k = arp[f]; k < arp[f + 1]; ++k) {
+        const int f2 = aci[k];
+        const double a = av[k];
+        for (int m = prp[f2]; m < prp[f2 + 1]; ++m) {
+            const int c = pci[m];
+            const double x = a * pv[m];
+            int lo = 0, hi = cnt;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (cc[mid] < c) lo = mid + 1; else hi = mid;
+            }
+            if (lo < cnt && cc[lo] == c) {
+                cv[lo] += x;
+            } else {
+                if (cnt >= WB) { err[0] = 1; bcnt[f] = 0; return; }
+                for (int i = cnt; i > lo; --i) { cc[i] = cc[i - 1]; cv[i] = cv[i - 1]; }
+                cc[lo] = c;
+                cv[lo] = x;
+                ++cnt;
+            }
+        }
+    }
+    bcnt[f] = cnt;
+    for (int i = 0; i < cnt; ++i) {
+        bci[(int64_t)f * WB + i] = cc[i];
+        bv[(int64_t)f * WB + i] = cv[i];
+    }
+}
+
+template <int WB, int MAXC>
+__global__ void k_rap(int nc, const int *__restrict__ rrp, const int *__restrict__ rci, const double *__restrict__ rv,
+                      const int *__restrict__ bcnt, const int *__restrict__ bci, const double *__restrict__ bv,
+                      const int *__restrict__ crp, const int *__restrict__ cci, double *__restrict__ cval,
+                      int *__restrict__ err)
+{
+    const int c1 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c1 >= nc) return;
+    const int a = crp[c1], len = crp[c1 + 1] - a;
+    if (len > MAXC) { err[0] = 2; return; }
+    double acc[MAXC];
+    for (int i = 0; i < len; ++i) acc[i] = 0.0;
+    for (int k = rrp[c1]; k < rrp[c1 + 1]; ++k) {
+        const int f1 = rci[k];
+        const double p = rv[k];
+        const int nb = bcnt[f1];
+        for (int m = 0; m < nb; ++m) {
+            const int c2 = bci[(int64_t)f1 * WB + m];
+            int lo = 0, hi = len - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (cci[a + mid] < c2) lo = mid + 1; else hi = mid;
+            }
+            if (cci[a + lo] != c2) { err[0] = 3; continue; }
+            acc[lo] += p * bv[(int64_t)f1 * WB + m];
+        }
+    }
+    for (int i = 0; i < len; ++i) cval[a + i] = acc[i];
+}
+
+// ---------------------------------------------------------------------------
+// Dense coarse inverse by blocked Gauss-Jordan (SPD, no pivoting), block 64.
+// ---------------------------------------------------------------------------
+constexpr int kGJ = 64;
+
+__global__ void k_densify(int n, int npad, const int *__restrict__ rp, const int *__restrict__ ci,
+                          const double *__restrict__ v, double *__restrict__ D)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)npad * npad) return;
+    const int r = (int)(i / npad), c = (int)(i % npad);
+    double x = 0.0;
+    if (r >= n || c >= n) {
+        x = (r == c) ? 1.0 : 0.0;
+    } else {
+        int lo = rp[r], hi = rp[r + 1] - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (ci[mid] < c) lo = mid + 1; else hi = mid;
+        }
+        if (lo < rp[r + 1] && ci[lo] == c) x = v[lo];
+    }
+    D[i] = x;
+}
+
+// invert the pivot block (kb) in LDS; result to Pinv (64x64, row-major)
+__global__ __launch_bounds__(256) void k_gj_pivot(double *__restrict__ D, int npad, int kb, double *__restrict__ Pinv)
+{
+    __shared__ double M[kGJ][kGJ + 1];
+    const int t = threadIdx.x;
+    for (int i = t; i < kGJ * kGJ; i += 256) M[i / kGJ][i % kGJ] = D[(int64_t)(kb + i / kGJ) * npad + kb + i % kGJ];
+    __syncthreads();
+    for (int p = 0; p < kGJ; ++p) {
+        const double piv = M[p][p];
+        __syncthreads();
+        // row p scaled; others eliminated (in-place Gauss-Jordan)
+        for (int i = t; i < kGJ * kGJ; i += 256) {
+            const int r = i / kGJ, c = i % kGJ;
+            if (r != p) {
+                const double f = M[r][p] / piv;
+                if (c != p) M[r][c] -= f * M[p][c];
+            }
+        }
+        __syncthreads();
+        for (int i = t; i < kGJ; i += 256) {
+            if (i != p) {
+                M[i][p] = -M[i][p] / piv;
+                M[p][i] = M[p][i] / piv;
+            }
+        }
+        __syncthreads();
+        if (t == 0) M[p][p] = 1.0 / piv;
+        __syncthreads();
+    }
+    for (int i = t; i < kGJ * kGJ; i += 256) Pinv[i] = M[i / kGJ][i % kGJ];
+}
+
+// W[r][j] = sum_m Pinv[r][m] D[kb+m][j] (all j); one block per 64 columns
+__global__ __launch_bounds__(256) void k_gj_rowpanel(const double *__restrict__ D, int npad, int kb,
+                                                     const double *__restrict__ Pinv, double *__restrict__ W)
+{
+    __shared__ double sP[kGJ][kGJ + 1];
+    __shared__ double sR[kGJ][kGJ + 1];
+    const int t = threadIdx.x, j0 = blockIdx.x * kGJ;
+    for (int i = t; i < kGJ * kGJ; i += 256) {
+        sP[i / kGJ][i % kGJ] = Pinv[i];
+        sR[i / kGJ][i % kGJ] = D[(int64_t)(kb + i / kGJ) * npad + j0 + i % kGJ];
+    }
+    __syncthreads();
+    for (int i = t; i < kGJ * kGJ; i += 256) {
+        const int r = i / kGJ, c = i % kGJ;
+        double s = 0.0;
+        for (int m = 0; m < kGJ; ++m) s += sP[r][m] * sR[m][c];
+        W[(int64_t)r * npad + j0 + c] = s;
+    }
+}
+
+// D[I][J] -= C[I] W[J] for tiles I, J != kblk (C = column panel of D)
+__global__ __launch_bounds__(256) void k_gj_update(double *__restrict__ D, int npad, int kb,
+                                                   const double *__restrict__ W)
+{
+    const int I = blockIdx.y * kGJ, J = blockIdx.x * kGJ;
+    if (I == kb || J == kb) return;
+    __shared__ double sC[kGJ][kGJ + 1];
+    __shared__ double sW[kGJ][kGJ + 1];
+    const int t = threadIdx.x;
+    for (int i = t; i < kGJ * kGJ; i += 256) {
+        sC[i / kGJ][i % kGJ] = D[(int64_t)(I + i / kGJ) * npad + kb + i % kGJ];
+        sW[i / kGJ][i % kGJ] = W[(int64_t)(i / kGJ) * npad + J + i % kGJ];
+    }
+    __syncthreads();
+    // each thread 4x4 outputs
+    const int tr = (t / 16) * 4, tc = (t % 16) * 4;
+    double acc[4][4] = {{0}};
+    for (int m = 0; m < kGJ; ++m) {
+        double a[4], b[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            a[u] = sC[tr + u][m];
+            b[u] = sW[m][tc + u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) D[(int64_t)(I + tr + u) * npad + J + tc + v] -= acc[u][v];
+}
+
+// column panel D[I][kb] = -C[I] Pinv, row panel D[kb][J] = W[J], pivot = Pinv
+__global__ __launch_bounds__(256) void k_gj_panels(double *__restrict__ D, int npad, int kb,
+                                                   const double *__restrict__ W, const double *__restrict__ Pinv)
+{
+    const int I = blockIdx.x * kGJ;
+    __shared__ double sC[kGJ][kGJ + 1];
+    __shared__ double sP[kGJ][kGJ + 1];
+    const int t = threadIdx.x;
+    if (I == kb) {
+        for (int i = t; i < kGJ * kGJ; i += 256)
+            D[(int64_t)(kb + i / kGJ) * npad + kb + i % kGJ] = Pinv[i];
+        return;
+    }
+    for (int i = t; i < kGJ * kGJ; i += 256) {
+        sC[i / kGJ][i % kGJ] = D[(int64_t)(I + i / kGJ) * npad + kb + i % kGJ];
+        sP[i / kGJ][i % kGJ] = Pinv[i];
+    }
+    __syncthreads();
+    for (int i = t; i < kGJ * kGJ; i += 256) {
+        const int r = i / kGJ, c = i % kGJ;
+        double s = 0.0;
+        for (int m = 0; m < kGJ; ++m) s += sC[r][m] * sP[m][c];
+        D[(int64_t)(I + r) * npad + kb + c] = -s;
+        // row panel block (kb rows, columns I..I+63) from W
+        D[(int64_t)(kb + r) * npad + I + c] = W[(int64_t)r * npad + I + c];
+    }
+}
+
+// y = M x, dense n x n (row stride npad), one wavefront per row
+template <typename T>
+__global__ __launch_bounds__(256) void k_dense_gemv(int n, int npad, const T *__restrict__ M,
+                                                    const T *__restrict__ x, T *__restrict__ y)
+{
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int r = blockIdx.x * 4 + w;
+    if (r >= n) return;
+    const T *row = M + (int64_t)r * npad;
+    T acc = 0;
+    for (int c = lane; c < n; c += kWave) acc += row[c] * x[c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+    if (lane == 0) y[r] = acc;
+}
+
+// y += alpha * (P x) (prolongation), one thread per fine row (rows short)
+template <typename V, typename T>
+__global__ void k_prolong_add(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                              const V *__restrict__ v, const T *__restrict__ x, T *__restrict__ y, T alpha)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    T s = 0;
+    for (int k = rp[r]; k < rp[r + 1]; ++k) s += (T)v[k] * x[ci[k]];
+    y[r] += alpha * s;
+}
+
+// r = b - A x (CSR), one wavefront-group per row
+template <int LPR, typename V, typename T>
+__global__ __launch_bounds__(256) void k_residual(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                  const V *__restrict__ v, const T *__restrict__ x,
+                                                  const T *__restrict__ b, T *__restrict__ r)
+{
+    constexpr int RPB = 256 / LPR;
+    const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+    const int row = blockIdx.x * RPB + g;
+    T acc = 0;
+    if (row < n)
+        for (int k = rp[row] + lane; k < rp[row + 1]; k += LPR) acc += (T)v[k] * x[ci[k]];
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+    if (row < n && lane == 0) r[row] = b[row] - acc;
+}
+
+// y = R x (restriction), one thread per coarse row
+template <typename V, typename T>
+__global__ void k_restrict(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                           const V *__restrict__ v, const T *__restrict__ x, T *__restrict__ y)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    T s = 0;
+    for (int k = rp[r]; k < rp[r + 1]; ++k) s += (T)v[k] * x[ci[k]];
+    y[r] = s;
+}
+
+template <typename S, typename Dd>
+__global__ void k_conv(int64_t n, const S *__restrict__ a, Dd *__restrict__ b)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (Dd)a[i];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline dim3 g1(int64_t n, int b = 256) { return dim3((unsigned)((n + b - 1) / b)); }
+
+hipError_t launch_transfer(int phase, int nfree_f, const int *occ_ptr, const int *occ_patch,
+                           const int64_t *occ_local, const int *fine_nb, const void *t1d, const int *trp,
+                           const int *tcol, const double *tval, const int *cdofs_off, const int *cdofs,
+                           const int *coarse_nb, int d, int nfree_c, int *counts, int *err,
+                           const int *rowptr, int *col, double *val, hipStream_t s)
+{
+    if (nfree_f <= 0) return hipSuccess;
+    if (phase == 0)
+        hipLaunchKernelGGL(k_transfer_count, g1(nfree_f, 128), dim3(128), 0, s, nfree_f, occ_ptr, occ_patch, occ_local,
+                           fine_nb, (const T1D *)t1d, trp, tcol, tval, cdofs_off, cdofs, coarse_nb, d, nfree_c,
+                           counts, err);
+    else
+        hipLaunchKernelGGL(k_transfer_fill, g1(nfree_f, 128), dim3(128), 0, s, nfree_f, occ_ptr, occ_patch, occ_local,
+                           fine_nb, (const T1D *)t1d, trp, tcol, tval, cdofs_off, cdofs, coarse_nb, d, nfree_c,
+                           rowptr, col, val);
+    return hipGetLastError();
+}
+
+hipError_t launch_transpose(int phase, int n, const int *rp, const int *ci, const double *v, int *counts,
+                            const int *trp, int *cursor, int *tci, double *tv, int nt, hipStream_t s)
+{
+    if (phase == 0) {
+        if (n > 0) hipLaunchKernelGGL(k_tr_count, g1(n), dim3(256), 0, s, n, rp, ci, counts);
+    } else if (phase == 1) {
+        if (n > 0) hipLaunchKernelGGL(k_tr_fill, g1(n), dim3(256), 0, s, n, rp, ci, v, trp, cursor, tci, tv);
+    } else {
+        if (nt > 0) hipLaunchKernelGGL(k_sort_rows, g1(nt, 128), dim3(128), 0, s, nt, trp, tci, tv);
+    }
+    return hipGetLastError();
+}
+
+int ap_width(int d) { return d == 2 ? 128 : 512; }
+
+hipError_t launch_ap(int d, int n, const int *arp, const int *aci, const double *av, const int *prp,
+                     const int *pci, const double *pv, int *bcnt, int *bci, double *bv, int *err, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    if (d == 2)
+        hipLaunchKernelGGL((k_ap<128>), g1(n, 128), dim3(128), 0, s, n, arp, aci, av, prp, pci, pv, bcnt, bci, bv, err);
+    else
+        hipLaunchKernelGGL((k_ap<512>), g1(n, 64), dim3(64), 0, s, n, arp, aci, av, prp, pci, pv, bcnt, bci, bv, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_rap(int d, int nc, const int *rrp, const int *rci, const double *rv, const int *bcnt,
+                      const int *bci, const double *bv, const int *crp, const int *cci, double *cval, int *err,
+                      hipStream_t s)
+{
+    if (nc <= 0) return hipSuccess;
+    if (d == 2)
+        hipLaunchKernelGGL((k_rap<128, 128>), g1(nc, 128), dim3(128), 0, s, nc, rrp, rci, rv, bcnt, bci, bv, crp, cci, cval, err);
+    else
+        hipLaunchKernelGGL((k_rap<512, 1024>), g1(nc, 64), dim3(64), 0, s, nc, rrp, rci, rv, bcnt, bci, bv, crp, cci, cval, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_inverse(int n, int npad, const int *rp, const int *ci, const double *v, double *D,
+                                double *W, double *Pinv, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_densify, g1((int64_t)npad * npad), dim3(256), 0, s, n, npad, rp, ci, v, D);
+    const int nb = npad / kGJ;
+    for (int k = 0; k < nb; ++k) {
+        const int kb = k * kGJ;
+        hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, s, D, npad, kb, Pinv);
+        hipLaunchKernelGGL(k_gj_rowpanel, dim3(nb), dim3(256), 0, s, D, npad, kb, Pinv, W);
+        hipLaunchKernelGGL(k_gj_update, dim3(nb, nb), dim3(256), 0, s, D, npad, kb, W);
+        hipLaunchKernelGGL(k_gj_panels, dim3(nb), dim3(256), 0, s, D, npad, kb, W, Pinv);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_dense_gemv_f64(int n, int npad, const double *M, const double *x, double *y, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_dense_gemv<double>), g1(n, 4), dim3(256), 0, s, n, npad, M, x, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_prolong_add_f64(int n, const int *rp, const int *ci, const double *v, const double *x,
+                                  double *y, double alpha, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_prolong_add<double, double>), g1(n), dim3(256), 0, s, n, rp, ci, v, x, y, alpha);
+    return hipGetLastError();
+}
+
+hipError_t launch_prolong_add_f32(int n, const int *rp, const int *ci, const float *v, const float *x,
+                                  float *y, float alpha, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_prolong_add<float, float>), g1(n), dim3(256), 0, s, n, rp, ci, v, x, y, alpha);
+    return hipGetLastError();
+}
+
+hipError_t launch_restrict_f64(int n, const int *rp, const int *ci, const double *v, const double *x,
+                               double *y, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_restrict<double, double>), g1(n), dim3(256), 0, s, n, rp, ci, v, x, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_restrict_f32(int n, const int *rp, const int *ci, const float *v, const float *x, float *y,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL((k_restrict<float, float>), g1(n), dim3(256), 0, s, n, rp, ci, v, x, y);
+    return hipGetLastError();
+}
+
+template <typename V, typename T>
+static hipError_t residual_dispatch(int lpr, int n, const int *rp, const int *ci, const V *v, const T *x,
+                                    const T *b, T *r, hipStream_t s)
+{
+    const dim3 grid((n + 256 / lpr - 1) / (256 / lpr)), blk(256);
+    switch (lpr) {
+    case 64: hipLaunchKernelGGL((k_residual<64, V, T>), grid, blk, 0, s, n, rp, ci, v, x, b, r); break;
+    case 32: hipLaunchKernelGGL((k_residual<32, V, T>), grid, blk, 0, s, n, rp, ci, v, x, b, r); break;
+    case 16: hipLaunchKernelGGL((k_residual<16, V, T>), grid, blk, 0, s, n, rp, ci, v, x, b, r); break;
+    default: hipLaunchKernelGGL((k_residual<8, V, T>), grid, blk, 0, s, n, rp, ci, v, x, b, r); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_residual_f64(int lpr, int n, const int *rp, const int *ci, const double *v, const double *x,
+                               const double *b, double *r, hipStream_t s)
+{
+    return residual_dispatch<double, double>(lpr, n, rp, ci, v, x, b, r, s);
+}
+
+hipError_t launch_residual_f32(int lpr, int n, const int *rp, const int *ci, const float *v, const float *x,
+                               const float *b, float *r, hipStream_t s)
+{
+    return residual_dispatch<float, float>(lpr, n, rp, ci, v, x, b, r, s);
+}
+
+hipError_t launch_conv_f64_f32(int64_t n, const double *a, float *b, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL((k_conv<double, float>), g1(n), dim3(256), 0, s, n, a, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_conv_f32_f64(int64_t n, const float *a, double *b, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL((k_conv<float, double>), g1(n), dim3(256), 0, s, n, a, b);
+    return hipGetLastError();
+}
+
+}  // namespace gsx

@@ -248,6 +248,9 @@ struct gsx_problem {
     int maxrow = 0;
     // single patch: free DOF -> index on the tensor grid padded by p zero
     // layers (structured operator of the solvers)
+    // host copies of the inputs needed to build coarser levels
+    std::vector<std::vector<double>> knots_h;   // npatch * 3 (empty beyond d)
+    std::vector<int> dir_sides_h, ifaces_h;
     DevPtr d_f2g;
     int grid_dims[3] = {1, 1, 1};
     int box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};   // free functions per direction
@@ -584,6 +587,12 @@ gsx_status gsx_problem_create(gsx_ctx *c, int32_t npatch, const gsx_patch_desc *
         eoff += h.nelem;
     }
     P->nbasis = boff;
+    P->knots_h.resize((size_t)npatch * 3);
+    for (int k = 0; k < npatch; ++k)
+        for (int j = 0; j < patches[k].d; ++j)
+            P->knots_h[3 * k + j].assign(patches[k].knots_host[j], patches[k].knots_host[j] + patches[k].nknots[j]);
+    if (ndir > 0) P->dir_sides_h.assign(dir_sides, dir_sides + 2 * ndir);
+    if (nif > 0) P->ifaces_h.assign(ifaces, ifaces + 8 * nif);
     P->nelem = eoff;
     std::string err;
     const int nfree = build_mapper(P.get(), ndir, dir_sides, nif, ifaces, err);
@@ -1320,6 +1329,9 @@ static gsx_status cg_mixed(gsx_matrix *A, int precond, double tau, const double 
     return fail(c, GSX_ERR_NOT_CONVERGED, "mixed CG reached the maximum number of iterations");
 }
 
+static gsx_status cg_mg_entry(gsx_matrix *A, gsx_mg *mg, const double *b, double *x, double tol, int maxit,
+                              int *iters, double *relres);
+
 // ---------------------------------------------------------------------------
 // structured operator + grid-layout CG
 // ---------------------------------------------------------------------------
@@ -1513,7 +1525,15 @@ gsx_status gsx_cg_solve(gsx_matrix *A, int32_t precond, double tau, gsx_mg *mg, 
 {
     if (!A || !b || !x || maxit < 0) return GSX_ERR_ARG;
     gsx_ctx *c = A->ctx;
-    if (precond == GSX_PREC_MG || mg) return fail(c, GSX_ERR_ARG, "multigrid preconditioner not available");
+    if (precond == GSX_PREC_MG || mg) {
+        if (!mg) return fail(c, GSX_ERR_ARG, "multigrid preconditioner needs an gsx_mg");
+        int it = 0;
+        double rr = 0.0;
+        gsx_status e = cg_mg_entry(A, mg, b, x, tol, maxit, &it, &rr);
+        if (iters) *iters = it;
+        if (relres) *relres = rr;
+        return e;
+    }
     if (precond != GSX_PREC_NONE && precond != GSX_PREC_JACOBI)
         return fail(c, GSX_ERR_ARG, "unknown preconditioner");
     if (precision == GSX_MIXED && !A->mixed)
@@ -1574,3 +1594,588 @@ gsx_status gsx_l2_error(gsx_problem *P, const double *sol, double *err)
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Geometric multigrid (gsMultiGridOp, src/gsMultiGrid/gsMultiGrid.hpp).
+// Levels are built by coarsening every patch's knot vectors (dropping every
+// other interior knot, the inverse of gsGridHierarchy::buildByRefinement,
+// src/gsMultiGrid/gsGridHierarchy.hpp:27-101).  Transfers are the tensor
+// products of 1D knot-insertion matrices restricted to free DOFs
+// (gsMultiBasis::combineTransferMatrices, src/gsCore/gsMultiBasis.hpp:
+// 137-178); coarse operators are Galerkin products P^T A P computed on the
+// device; the coarsest level is solved exactly with a dense FP64 inverse.
+// ===========================================================================
+struct MGLevel {
+    gsx_problem *prob = nullptr;       // coarse levels own their problem
+    int n = 0;
+    int64_t nnz = 0;
+    int lpr = 8;
+    DevPtr rowptr, col, val, val32, diag, diag32;
+    // transfer to the next finer level: P (n_fine x n), R = P^T (n x n_fine)
+    int nfine = 0;
+    int64_t pnnz = 0;
+    DevPtr prp, pci, pv, pv32, rrp, rci, rv, rv32;
+    DevPtr x, b, r, t;                 // FP64
+    DevPtr x32, b32, r32, t32;         // FP32
+};
+
+struct gsx_mg {
+    gsx_ctx *ctx = nullptr;
+    std::vector<MGLevel> L;            // 0 = coarsest
+    int npre = 1, npost = 1, ncycles = 1;
+    double tau = 1.0, cgc = 1.0;
+    int precision = GSX_FP64;
+    int npad = 0;
+    DevPtr Dinv;
+    DevPtr zin, zout;                  // finest FP64 staging for the mixed cycle
+    ~gsx_mg()
+    {
+        for (auto &l : L)
+            if (l.prob) gsx_problem_destroy(l.prob);
+    }
+};
+
+// 1D knot-insertion transfer coarse -> fine (fine knots = coarse knots plus
+// the inserted ones), by single-knot Boehm insertion applied to the
+// identity coefficients (gsBoehmRefine, src/gsNurbs/gsBoehm.hpp:161-226).
+// Returns dense nf x nc (row-major).
+static std::vector<double> transfer_1d(const std::vector<double> &coarse, const std::vector<double> &fine,
+                                       int p, int &nf, int &nc)
+{
+    nc = (int)coarse.size() - p - 1;
+    nf = (int)fine.size() - p - 1;
+    std::vector<double> U = coarse;
+    std::vector<double> C((size_t)nc * nc, 0.0), Cn;
+    for (int i = 0; i < nc; ++i) C[(size_t)i * nc + i] = 1.0;
+    int ncur = nc;
+    // knots of `fine` missing from `coarse` (multiset difference, in order)
+    std::vector<double> ins;
+    {
+        size_t a = 0;
+        for (size_t i = 0; i < fine.size(); ++i) {
+            if (a < coarse.size() && coarse[a] == fine[i]) ++a;
+            else ins.push_back(fine[i]);
+        }
+    }
+    for (double u : ins) {
+        int k = p;
+        while (k + 1 < (int)U.size() && U[k + 1] <= u) ++k;
+        Cn.assign((size_t)(ncur + 1) * nc, 0.0);
+        for (int i = 0; i <= ncur; ++i) {
+            double *dst = &Cn[(size_t)i * nc];
+            if (i <= k - p) {
+                std::memcpy(dst, &C[(size_t)i * nc], sizeof(double) * nc);
+            } else if (i >= k + 1) {
+                std::memcpy(dst, &C[(size_t)(i - 1) * nc], sizeof(double) * nc);
+            } else {
+                const double a = (u - U[i]) / (U[i + p] - U[i]);
+                for (int c = 0; c < nc; ++c) dst[c] = (1.0 - a) * C[(size_t)(i - 1) * nc + c] + a * C[(size_t)i * nc + c];
+            }
+        }
+        C.swap(Cn);
+        ++ncur;
+        U.insert(U.begin() + k + 1, u);
+    }
+    return C;
+}
+
+static gsx_status mg_level_vectors(gsx_ctx *c, MGLevel &l, bool mixed)
+{
+    const int n = std::max(1, l.n);
+    l.x = dalloc(c, sizeof(double) * n);
+    l.b = dalloc(c, sizeof(double) * n);
+    l.r = dalloc(c, sizeof(double) * n);
+    l.t = dalloc(c, sizeof(double) * n);
+    if (!l.x || !l.b || !l.r || !l.t) return fail(c, GSX_ERR_OOM, "device allocation (multigrid vectors)");
+    if (mixed) {
+        l.x32 = dalloc(c, sizeof(float) * n);
+        l.b32 = dalloc(c, sizeof(float) * n);
+        l.r32 = dalloc(c, sizeof(float) * n);
+        l.t32 = dalloc(c, sizeof(float) * n);
+        if (!l.x32 || !l.b32 || !l.r32 || !l.t32) return fail(c, GSX_ERR_OOM, "device allocation (multigrid vectors)");
+    }
+    return GSX_OK;
+}
+
+static gsx_status scan_counts(gsx_ctx *c, const int *counts, int n, DevPtr &rowptr, int64_t &total)
+{
+    hipStream_t s = c->stream;
+    rowptr = dalloc(c, sizeof(int) * (n + 1));
+    DevPtr bsum = dalloc(c, sizeof(long long) * ((n + 1023) / 1024 + 2));
+    DevPtr tot = dalloc(c, sizeof(long long));
+    if (!rowptr || !bsum || !tot) return fail(c, GSX_ERR_OOM, "device allocation (scan)");
+    HIP_OK(c, gsx::launch_scan(counts, n, dp<int>(rowptr), dp<long long>(bsum), dp<long long>(tot), s));
+    long long t = 0;
+    HIP_OK(c, hipMemcpyAsync(&t, tot.get(), sizeof(long long), hipMemcpyDeviceToHost, s));
+    HIP_OK(c, hipStreamSynchronize(s));
+    total = t;
+    return GSX_OK;
+}
+
+extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nlevels, int32_t npre,
+                                    int32_t npost, int32_t ncycles, double damping, double cgc_damping,
+                                    int32_t precision, gsx_mg **out)
+{
+    if (!fine || !A || !out || nlevels < 1 || npre < 0 || npost < 0 || ncycles < 1) return GSX_ERR_ARG;
+    gsx_ctx *c = fine->ctx;
+    if (ncycles != 1) return fail(c, GSX_ERR_ARG, "only V-cycles (NumCycles = 1) are implemented");
+    if (A->n != fine->nfree) return fail(c, GSX_ERR_ARG, "matrix does not belong to the problem");
+    if (precision != GSX_FP64 && precision != GSX_MIXED) return fail(c, GSX_ERR_ARG, "unknown precision");
+    hipStream_t s = c->stream;
+    KScope ks(c, "mg_setup");
+    auto mg = std::make_unique<gsx_mg>();
+    mg->ctx = c;
+    mg->npre = npre;
+    mg->npost = npost;
+    mg->ncycles = ncycles;
+    mg->tau = damping;
+    mg->cgc = cgc_damping;
+    mg->precision = precision;
+    const bool mixed = precision == GSX_MIXED;
+    mg->L.resize(nlevels);
+    // finest level = the given matrix
+    {
+        MGLevel &f = mg->L[nlevels - 1];
+        f.n = A->n;
+        f.nnz = A->nnz;
+        f.lpr = A->lpr;
+        f.rowptr = A->rowptr;
+        f.col = A->col;
+        f.val = A->val;
+    }
+    // knot vectors of all levels
+    const int npatch = fine->npatch, d = fine->d;
+    std::vector<std::vector<std::vector<double>>> knots(nlevels);
+    knots[nlevels - 1] = fine->knots_h;
+    for (int l = nlevels - 2; l >= 0; --l) {
+        knots[l].resize((size_t)npatch * 3);
+        for (int k = 0; k < npatch; ++k)
+            for (int j = 0; j < d; ++j) {
+                const std::vector<double> &U = knots[l + 1][3 * k + j];
+                const int p = fine->hp[k].p[j];
+                const int nel = (int)U.size() - 2 * p - 1;
+                if (nel % 2 != 0 || nel < 2)
+                    return fail(c, GSX_ERR_ARG, "multigrid needs an even number of elements per direction on every level");
+                std::vector<double> V(U.begin(), U.begin() + p + 1);
+                for (int e = 2; e < nel; e += 2) V.push_back(U[p + e]);
+                V.insert(V.end(), U.end() - p - 1, U.end());
+                knots[l][3 * k + j] = V;
+            }
+    }
+    // coarse problems (geometry is not used by the Galerkin levels)
+    for (int l = nlevels - 2; l >= 0; --l) {
+        std::vector<gsx_patch_desc> desc(npatch);
+        std::vector<std::vector<double>> ctrl(npatch);
+        for (int k = 0; k < npatch; ++k) {
+            gsx_patch_desc &D = desc[k];
+            std::memset(&D, 0, sizeof(D));
+            D.d = d;
+            int64_t nbas = 1;
+            for (int j = 0; j < d; ++j) {
+                D.p[j] = fine->hp[k].p[j];
+                D.nknots[j] = (int)knots[l][3 * k + j].size();
+                D.knots_host[j] = knots[l][3 * k + j].data();
+                nbas *= D.nknots[j] - D.p[j] - 1;
+            }
+            ctrl[k].assign((size_t)nbas * d, 0.0);
+            D.ctrl_host = ctrl[k].data();
+            D.fkind = 1;
+        }
+        gsx_problem *cp = nullptr;
+        gsx_status st = gsx_problem_create(c, npatch, desc.data(), (int)fine->dir_sides_h.size() / 2,
+                                           fine->dir_sides_h.empty() ? nullptr : fine->dir_sides_h.data(),
+                                           (int)fine->ifaces_h.size() / 8,
+                                           fine->ifaces_h.empty() ? nullptr : fine->ifaces_h.data(), &cp);
+        if (st != GSX_OK) return st;
+        mg->L[l].prob = cp;
+        mg->L[l].n = cp->nfree;
+        st = pattern_build(cp);
+        if (st != GSX_OK) return st;
+    }
+    // transfers and Galerkin products, finest first
+    for (int l = nlevels - 2; l >= 0; --l) {
+        MGLevel &C = mg->L[l], &F = mg->L[l + 1];
+        gsx_problem *fp = (l + 1 == nlevels - 1) ? fine : F.prob;
+        gsx_problem *cp = C.prob;
+        // 1D transfers per (patch, direction)
+        struct T1Dh { int rp_off, ent_off, nf, nc; };
+        std::vector<T1Dh> t1d((size_t)npatch * 3, T1Dh{0, 0, 1, 1});
+        std::vector<int> trp, tcol;
+        std::vector<double> tval;
+        std::vector<int> fnb((size_t)npatch * 3, 1), cnb((size_t)npatch * 3, 1), cdoff(npatch, 0);
+        for (int k = 0; k < npatch; ++k) {
+            cdoff[k] = (int)cp->boff[k];
+            for (int j = 0; j < 3; ++j) {
+                fnb[3 * k + j] = fp->hp[k].nb[j];
+                cnb[3 * k + j] = cp->hp[k].nb[j];
+            }
+            for (int j = 0; j < 3; ++j) {
+                T1Dh &T = t1d[3 * k + j];
+                T.rp_off = (int)trp.size();
+                T.ent_off = (int)tcol.size();
+                if (j >= d) {
+                    trp.push_back(0);
+                    trp.push_back(1);
+                    tcol.push_back(0);
+                    tval.push_back(1.0);
+                    continue;
+                }
+                int nf = 0, nc = 0;
+                std::vector<double> Tm = transfer_1d(knots[l][3 * k + j], knots[l + 1][3 * k + j], fp->hp[k].p[j], nf, nc);
+                T.nf = nf;
+                T.nc = nc;
+                int cnt = 0;
+                for (int i = 0; i < nf; ++i) {
+                    trp.push_back(cnt);
+                    for (int cc = 0; cc < nc; ++cc)
+                        if (Tm[(size_t)i * nc + cc] != 0.0) {
+                            tcol.push_back(cc);
+                            tval.push_back(Tm[(size_t)i * nc + cc]);
+                            ++cnt;
+                        }
+                }
+                trp.push_back(cnt);
+            }
+        }
+        // rp entries are relative to ent_off
+        DevPtr d_t1d = dalloc(c, sizeof(T1Dh) * t1d.size());
+        DevPtr d_trp = dalloc(c, sizeof(int) * trp.size()), d_tcol = dalloc(c, sizeof(int) * tcol.size());
+        DevPtr d_tval = dalloc(c, sizeof(double) * tval.size());
+        DevPtr d_fnb = dalloc(c, sizeof(int) * fnb.size()), d_cnb = dalloc(c, sizeof(int) * cnb.size());
+        DevPtr d_cdoff = dalloc(c, sizeof(int) * npatch);
+        DevPtr counts = dalloc(c, sizeof(int) * (F.n + 1)), errf = dalloc(c, sizeof(int) * 2);
+        if (!d_t1d || !d_trp || !d_tcol || !d_tval || !d_fnb || !d_cnb || !d_cdoff || !counts || !errf)
+            return fail(c, GSX_ERR_OOM, "device allocation (transfer)");
+        HIP_OK(c, hipMemcpyAsync(d_t1d.get(), t1d.data(), sizeof(T1Dh) * t1d.size(), hipMemcpyHostToDevice, s));
+        HIP_OK(c, hipMemcpyAsync(d_trp.get(), trp.data(), sizeof(int) * trp.size(), hipMemcpyHostToDevice, s));
+        HIP_OK(c, hipMemcpyAsync(d_tcol.get(), tcol.data(), sizeof(int) * tcol.size(), hipMemcpyHostToDevice, s));
+        HIP_OK(c, hipMemcpyAsync(d_tval.get(), tval.data(), sizeof(double) * tval.size(), hipMemcpyHostToDevice, s));
+        HIP_OK(c, hipMemcpyAsync(d_fnb.get(), fnb.data(), sizeof(int) * fnb.size(), hipMemcpyHostToDevice, s));
+        HIP_OK(c, hipMemcpyAsync(d_cnb.get(), cnb.data(), sizeof(int) * cnb.size(), hipMemcpyHostToDevice, s));
+        HIP_OK(c, hipMemcpyAsync(d_cdoff.get(), cdoff.data(), sizeof(int) * npatch, hipMemcpyHostToDevice, s));
+        HIP_OK(c, hipMemsetAsync(errf.get(), 0, sizeof(int) * 2, s));
+        HIP_OK(c, gsx::launch_transfer(0, F.n, dp<int>(fp->d_occ_ptr), dp<int>(fp->d_occ_patch),
+                                       dp<int64_t>(fp->d_occ_local), dp<int>(d_fnb), d_t1d.get(), dp<int>(d_trp),
+                                       dp<int>(d_tcol), dp<double>(d_tval), dp<int>(d_cdoff), dp<int>(cp->d_dofs),
+                                       dp<int>(d_cnb), d, C.n, dp<int>(counts), dp<int>(errf), nullptr, nullptr,
+                                       nullptr, s));
+        int64_t pn = 0;
+        GSX_TRY(scan_counts(c, dp<int>(counts), F.n, C.prp, pn));
+        C.nfine = F.n;
+        C.pnnz = pn;
+        C.pci = dalloc(c, sizeof(int) * std::max<int64_t>(1, pn));
+        C.pv = dalloc(c, sizeof(double) * std::max<int64_t>(1, pn));
+        if (!C.pci || !C.pv) return fail(c, GSX_ERR_OOM, "device allocation (transfer)");
+        HIP_OK(c, gsx::launch_transfer(1, F.n, dp<int>(fp->d_occ_ptr), dp<int>(fp->d_occ_patch),
+                                       dp<int64_t>(fp->d_occ_local), dp<int>(d_fnb), d_t1d.get(), dp<int>(d_trp),
+                                       dp<int>(d_tcol), dp<double>(d_tval), dp<int>(d_cdoff), dp<int>(cp->d_dofs),
+                                       dp<int>(d_cnb), d, C.n, nullptr, nullptr, dp<int>(C.prp), dp<int>(C.pci),
+                                       dp<double>(C.pv), s));
+        // R = P^T
+        DevPtr tcount = dalloc(c, sizeof(int) * (C.n + 1)), cursor = dalloc(c, sizeof(int) * (C.n + 1));
+        if (!tcount || !cursor) return fail(c, GSX_ERR_OOM, "device allocation (restriction)");
+        HIP_OK(c, hipMemsetAsync(tcount.get(), 0, sizeof(int) * (C.n + 1), s));
+        HIP_OK(c, hipMemsetAsync(cursor.get(), 0, sizeof(int) * (C.n + 1), s));
+        HIP_OK(c, gsx::launch_transpose(0, F.n, dp<int>(C.prp), dp<int>(C.pci), nullptr, dp<int>(tcount), nullptr,
+                                        nullptr, nullptr, nullptr, 0, s));
+        int64_t rn = 0;
+        GSX_TRY(scan_counts(c, dp<int>(tcount), C.n, C.rrp, rn));
+        C.rci = dalloc(c, sizeof(int) * std::max<int64_t>(1, rn));
+        C.rv = dalloc(c, sizeof(double) * std::max<int64_t>(1, rn));
+        if (!C.rci || !C.rv) return fail(c, GSX_ERR_OOM, "device allocation (restriction)");
+        HIP_OK(c, gsx::launch_transpose(1, F.n, dp<int>(C.prp), dp<int>(C.pci), dp<double>(C.pv), nullptr,
+                                        dp<int>(C.rrp), dp<int>(cursor), dp<int>(C.rci), dp<double>(C.rv), 0, s));
+        HIP_OK(c, gsx::launch_transpose(2, 0, nullptr, nullptr, nullptr, nullptr, dp<int>(C.rrp), nullptr,
+                                        dp<int>(C.rci), dp<double>(C.rv), C.n, s));
+        // Galerkin: B = A P (ELL), A_c = R B into the coarse pattern
+        const int WB = gsx::ap_width(d);
+        DevPtr bcnt = dalloc(c, sizeof(int) * F.n), bci = dalloc(c, sizeof(int) * (size_t)F.n * WB);
+        DevPtr bv = dalloc(c, sizeof(double) * (size_t)F.n * WB);
+        C.rowptr = cp->d_rowptr;
+        C.col = cp->d_col;
+        C.nnz = cp->nnz;
+        C.lpr = gsx::spmv_lpr(C.n ? (double)C.nnz / C.n : 1.0);
+        C.val = dalloc(c, sizeof(double) * std::max<int64_t>(1, C.nnz));
+        if (!bcnt || !bci || !bv || !C.val) return fail(c, GSX_ERR_OOM, "device allocation (Galerkin product)");
+        HIP_OK(c, gsx::launch_ap(d, F.n, dp<int>(F.rowptr), dp<int>(F.col), dp<double>(F.val), dp<int>(C.prp),
+                                 dp<int>(C.pci), dp<double>(C.pv), dp<int>(bcnt), dp<int>(bci), dp<double>(bv),
+                                 dp<int>(errf), s));
+        HIP_OK(c, gsx::launch_rap(d, C.n, dp<int>(C.rrp), dp<int>(C.rci), dp<double>(C.rv), dp<int>(bcnt),
+                                  dp<int>(bci), dp<double>(bv), dp<int>(C.rowptr), dp<int>(C.col),
+                                  dp<double>(C.val), dp<int>(errf) + 1, s));
+        int errh[2] = {0, 0};
+        HIP_OK(c, hipMemcpyAsync(errh, errf.get(), sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        HIP_OK(c, hipStreamSynchronize(s));
+        if (errh[0] || errh[1]) {
+            char buf[160];
+            std::snprintf(buf, sizeof(buf), "multigrid level %d: transfer/Galerkin buffer overflow (codes %d, %d)", l,
+                          errh[0], errh[1]);
+            return fail(c, GSX_ERR_ARG, buf);
+        }
+    }
+    // per-level diagonals, FP32 copies, vectors
+    for (int l = 0; l < nlevels; ++l) {
+        MGLevel &V = mg->L[l];
+        V.diag = dalloc(c, sizeof(double) * std::max(1, V.n));
+        if (!V.diag) return fail(c, GSX_ERR_OOM, "device allocation (diag)");
+        HIP_OK(c, gsx::launch_diag_f64(V.n, dp<int>(V.rowptr), dp<int>(V.col), dp<double>(V.val), dp<double>(V.diag), s));
+        if (mixed) {
+            V.val32 = dalloc(c, sizeof(float) * std::max<int64_t>(1, V.nnz));
+            V.diag32 = dalloc(c, sizeof(float) * std::max(1, V.n));
+            if (!V.val32 || !V.diag32) return fail(c, GSX_ERR_OOM, "device allocation (fp32 level)");
+            HIP_OK(c, gsx::launch_conv_f64_f32(V.nnz, dp<double>(V.val), dp<float>(V.val32), s));
+            HIP_OK(c, gsx::launch_conv_f64_f32(V.n, dp<double>(V.diag), dp<float>(V.diag32), s));
+            if (l < nlevels - 1) {
+                V.pv32 = dalloc(c, sizeof(float) * std::max<int64_t>(1, V.pnnz));
+                V.rv32 = dalloc(c, sizeof(float) * std::max<int64_t>(1, V.pnnz));
+                if (!V.pv32 || !V.rv32) return fail(c, GSX_ERR_OOM, "device allocation (fp32 transfer)");
+                HIP_OK(c, gsx::launch_conv_f64_f32(V.pnnz, dp<double>(V.pv), dp<float>(V.pv32), s));
+                HIP_OK(c, gsx::launch_conv_f64_f32(V.pnnz, dp<double>(V.rv), dp<float>(V.rv32), s));
+            }
+        }
+        GSX_TRY(mg_level_vectors(c, V, mixed));
+    }
+    // exact coarse solve: dense FP64 inverse of the coarsest operator
+    {
+        MGLevel &C0 = mg->L[0];
+        const int n0 = C0.n;
+        if ((int64_t)n0 > 32768) return fail(c, GSX_ERR_ARG, "coarsest level too large for the dense inverse");
+        mg->npad = std::max(64, (n0 + 63) / 64 * 64);
+        mg->Dinv = dalloc(c, sizeof(double) * (size_t)mg->npad * mg->npad);
+        DevPtr W = dalloc(c, sizeof(double) * (size_t)64 * mg->npad), Pinv = dalloc(c, sizeof(double) * 64 * 64);
+        if (!mg->Dinv || !W || !Pinv) return fail(c, GSX_ERR_OOM, "device allocation (coarse inverse)");
+        HIP_OK(c, gsx::launch_dense_inverse(n0, mg->npad, dp<int>(C0.rowptr), dp<int>(C0.col), dp<double>(C0.val),
+                                            dp<double>(mg->Dinv), dp<double>(W), dp<double>(Pinv), s));
+        HIP_OK(c, hipStreamSynchronize(s));
+    }
+    *out = mg.release();
+    return GSX_OK;
+}
+
+extern "C" gsx_status gsx_mg_destroy(gsx_mg *mg)
+{
+    if (!mg) return GSX_ERR_ARG;
+    (void)hipStreamSynchronize(mg->ctx->stream);
+    delete mg;
+    return GSX_OK;
+}
+
+extern "C" gsx_status gsx_mg_level_info(const gsx_mg *mg, int32_t lvl, int64_t *n, int64_t *nnz, int64_t *pnnz)
+{
+    if (!mg || lvl < 0 || lvl >= (int)mg->L.size()) return GSX_ERR_ARG;
+    const MGLevel &l = mg->L[lvl];
+    if (n) *n = l.n;
+    if (nnz) *nnz = l.nnz;
+    if (pnnz) *pnnz = l.pnnz;
+    return GSX_OK;
+}
+
+// level operator (CSR) and, for lvl < nlevels-1, the transfer to the next
+// finer level (CSR, n_fine rows)
+extern "C" gsx_status gsx_mg_level_download(const gsx_mg *mg, int32_t lvl, int32_t *rp, int32_t *ci, double *v,
+                                            int32_t *prp, int32_t *pci, double *pv)
+{
+    if (!mg || lvl < 0 || lvl >= (int)mg->L.size()) return GSX_ERR_ARG;
+    gsx_ctx *c = mg->ctx;
+    hipStream_t s = c->stream;
+    const MGLevel &l = mg->L[lvl];
+    if (rp) HIP_OK(c, hipMemcpyAsync(rp, l.rowptr.get(), sizeof(int) * (l.n + 1), hipMemcpyDeviceToHost, s));
+    if (ci) HIP_OK(c, hipMemcpyAsync(ci, l.col.get(), sizeof(int) * l.nnz, hipMemcpyDeviceToHost, s));
+    if (v) HIP_OK(c, hipMemcpyAsync(v, l.val.get(), sizeof(double) * l.nnz, hipMemcpyDeviceToHost, s));
+    if (lvl < (int)mg->L.size() - 1) {
+        if (prp) HIP_OK(c, hipMemcpyAsync(prp, l.prp.get(), sizeof(int) * (l.nfine + 1), hipMemcpyDeviceToHost, s));
+        if (pci) HIP_OK(c, hipMemcpyAsync(pci, l.pci.get(), sizeof(int) * l.pnnz, hipMemcpyDeviceToHost, s));
+        if (pv) HIP_OK(c, hipMemcpyAsync(pv, l.pv.get(), sizeof(double) * l.pnnz, hipMemcpyDeviceToHost, s));
+    }
+    HIP_OK(c, hipStreamSynchronize(s));
+    return GSX_OK;
+}
+
+// One multigrid step at level l on (b, x) with x = 0 on entry (the
+// preconditioner's apply).  FP64 version.
+static gsx_status mg_cycle64(gsx_mg *mg, int l)
+{
+    gsx_ctx *c = mg->ctx;
+    hipStream_t s = c->stream;
+    MGLevel &V = mg->L[l];
+    double *x = dp<double>(V.x), *t = dp<double>(V.t), *b = dp<double>(V.b), *r = dp<double>(V.r);
+    if (l == 0) {
+        KScope k(c, "mg_coarse");
+        HIP_OK(c, gsx::launch_dense_gemv_f64(V.n, mg->npad, dp<double>(mg->Dinv), b, x, s));
+        return GSX_OK;
+    }
+    const int *rp = dp<int>(V.rowptr), *ci = dp<int>(V.col);
+    const double *va = dp<double>(V.val), *dg = dp<double>(V.diag);
+    bool zero = true;
+    {
+        KScope k(c, "mg_smooth");
+        for (int i = 0; i < mg->npre; ++i) {
+            HIP_OK(c, gsx::launch_jacobi_step_f64(V.lpr, V.n, rp, ci, va, dg, b, x, t, mg->tau, zero ? 1 : 0, s));
+            std::swap(x, t);
+            zero = false;
+        }
+    }
+    MGLevel &C = mg->L[l - 1];
+    {
+        KScope k(c, "mg_transfer");
+        if (zero) HIP_OK(c, hipMemcpyAsync(r, b, sizeof(double) * V.n, hipMemcpyDeviceToDevice, s));
+        else HIP_OK(c, gsx::launch_residual_f64(V.lpr, V.n, rp, ci, va, x, b, r, s));
+        HIP_OK(c, gsx::launch_restrict_f64(C.n, dp<int>(C.rrp), dp<int>(C.rci), dp<double>(C.rv), r, dp<double>(C.b), s));
+    }
+    // coarse correction from a zero start (one V-cycle step)
+    GSX_TRY(mg_cycle64(mg, l - 1));
+    {
+        KScope k(c, "mg_transfer");
+        if (zero) HIP_OK(c, gsx::launch_fill_f64(x, V.n, 0.0, s));
+        HIP_OK(c, gsx::launch_prolong_add_f64(V.n, dp<int>(C.prp),
+                                              dp<int>(C.pci), dp<double>(C.pv), dp<double>(C.x), x, mg->cgc, s));
+    }
+    {
+        KScope k(c, "mg_smooth");
+        for (int i = 0; i < mg->npost; ++i) {
+            HIP_OK(c, gsx::launch_jacobi_step_f64(V.lpr, V.n, rp, ci, va, dg, b, x, t, mg->tau, 0, s));
+            std::swap(x, t);
+        }
+    }
+    if (x != dp<double>(V.x)) HIP_OK(c, hipMemcpyAsync(dp<double>(V.x), x, sizeof(double) * V.n, hipMemcpyDeviceToDevice, s));
+    return GSX_OK;
+}
+
+// FP32 cycle (operators, transfers and vectors in FP32; the coarsest solve
+// in FP64 through conversions).
+static gsx_status mg_cycle32(gsx_mg *mg, int l)
+{
+    gsx_ctx *c = mg->ctx;
+    hipStream_t s = c->stream;
+    MGLevel &V = mg->L[l];
+    float *x = dp<float>(V.x32), *t = dp<float>(V.t32), *b = dp<float>(V.b32), *r = dp<float>(V.r32);
+    if (l == 0) {
+        KScope k(c, "mg_coarse");
+        HIP_OK(c, gsx::launch_conv_f32_f64(V.n, b, dp<double>(V.b), s));
+        HIP_OK(c, gsx::launch_dense_gemv_f64(V.n, mg->npad, dp<double>(mg->Dinv), dp<double>(V.b), dp<double>(V.x), s));
+        HIP_OK(c, gsx::launch_conv_f64_f32(V.n, dp<double>(V.x), x, s));
+        return GSX_OK;
+    }
+    const int *rp = dp<int>(V.rowptr), *ci = dp<int>(V.col);
+    const float *va = dp<float>(V.val32), *dg = dp<float>(V.diag32);
+    bool zero = true;
+    {
+        KScope k(c, "mg_smooth");
+        for (int i = 0; i < mg->npre; ++i) {
+            HIP_OK(c, gsx::launch_jacobi_step_f32(V.lpr, V.n, rp, ci, va, dg, b, x, t, (float)mg->tau, zero ? 1 : 0, s));
+            std::swap(x, t);
+            zero = false;
+        }
+    }
+    MGLevel &C = mg->L[l - 1];
+    {
+        KScope k(c, "mg_transfer");
+        if (zero) HIP_OK(c, hipMemcpyAsync(r, b, sizeof(float) * V.n, hipMemcpyDeviceToDevice, s));
+        else HIP_OK(c, gsx::launch_residual_f32(V.lpr, V.n, rp, ci, va, x, b, r, s));
+        HIP_OK(c, gsx::launch_restrict_f32(C.n, dp<int>(C.rrp), dp<int>(C.rci), dp<float>(C.rv32), r, dp<float>(C.b32), s));
+    }
+    GSX_TRY(mg_cycle32(mg, l - 1));
+    {
+        KScope k(c, "mg_transfer");
+        if (zero) HIP_OK(c, gsx::launch_fill_f32(x, V.n, 0.0f, s));
+        HIP_OK(c, gsx::launch_prolong_add_f32(V.n, dp<int>(C.prp), dp<int>(C.pci), dp<float>(C.pv32), dp<float>(C.x32), x,
+                                              (float)mg->cgc, s));
+    }
+    {
+        KScope k(c, "mg_smooth");
+        for (int i = 0; i < mg->npost; ++i) {
+            HIP_OK(c, gsx::launch_jacobi_step_f32(V.lpr, V.n, rp, ci, va, dg, b, x, t, (float)mg->tau, 0, s));
+            std::swap(x, t);
+        }
+    }
+    if (x != dp<float>(V.x32)) HIP_OK(c, hipMemcpyAsync(dp<float>(V.x32), x, sizeof(float) * V.n, hipMemcpyDeviceToDevice, s));
+    return GSX_OK;
+}
+
+// z = MG(r) (gsPreconditionerOp::apply: x = 0, one step)
+static gsx_status mg_apply_dev(gsx_mg *mg, const double *r, double *z)
+{
+    gsx_ctx *c = mg->ctx;
+    hipStream_t s = c->stream;
+    MGLevel &F = mg->L.back();
+    const int nl = (int)mg->L.size();
+    if (mg->precision == GSX_MIXED) {
+        HIP_OK(c, gsx::launch_conv_f64_f32(F.n, r, dp<float>(F.b32), s));
+        GSX_TRY(mg_cycle32(mg, nl - 1));
+        HIP_OK(c, gsx::launch_conv_f32_f64(F.n, dp<float>(F.x32), z, s));
+    } else {
+        HIP_OK(c, hipMemcpyAsync(dp<double>(F.b), r, sizeof(double) * F.n, hipMemcpyDeviceToDevice, s));
+        GSX_TRY(mg_cycle64(mg, nl - 1));
+        HIP_OK(c, hipMemcpyAsync(z, dp<double>(F.x), sizeof(double) * F.n, hipMemcpyDeviceToDevice, s));
+    }
+    return GSX_OK;
+}
+
+extern "C" gsx_status gsx_mg_apply(gsx_mg *mg, const double *r_dev, double *z_dev)
+{
+    if (!mg || !r_dev || !z_dev) return GSX_ERR_ARG;
+    return mg_apply_dev(mg, r_dev, z_dev);
+}
+
+// MG-preconditioned CG (FP64 outer iteration on A; the cycle in the
+// precision of mg).  Host-controlled: one state read per iteration.
+static gsx_status cg_mg(gsx_matrix *A, gsx_mg *mg, const double *b, double *x, double tol, int maxit,
+                        int *iters, double *relres)
+{
+    gsx_ctx *c = A->ctx;
+    hipStream_t s = c->stream;
+    gsx_status e = cg_workspace(A, std::max(1, maxit), false, false);
+    if (e != GSX_OK) return e;
+    CGWork &w = A->ws;
+    const int n = A->n, np = w.nparts;
+    const int ns = gsx::spmv_blocks(n, A->lpr), nv = (n + 255) / 256;
+    double *P0 = dp<double>(w.parts), *P1 = P0 + np;
+    CGState *st = dp<CGState>(w.state);
+    double *hist = dp<double>(w.hist);
+    const int *rp = dp<int>(A->rowptr), *ci = dp<int>(A->col);
+    const double *va = dp<double>(A->val);
+    double *r = dp<double>(w.r), *z = dp<double>(w.z), *p = dp<double>(w.p), *t = dp<double>(w.t);
+    CGState h{};
+    h.tol = tol;
+    h.maxit = maxit;
+    HIP_OK(c, hipMemcpyAsync(st, &h, sizeof(CGState), hipMemcpyHostToDevice, s));
+    HIP_OK(c, gsx::launch_spmv_f64(A->lpr, n, rp, ci, va, x, r, b, P1, 2, nullptr, s));
+    HIP_OK(c, gsx::launch_dot_f64(n, b, b, P0, nullptr, s));
+    HIP_OK(c, gsx::launch_cg_scalar(0, P0, nv, P1, ns, st, hist, s));
+    GSX_TRY(read_state(c, w, h));
+    if (h.done == 3) HIP_OK(c, gsx::launch_fill_f64(x, n, 0.0, s));
+    if (!h.done) {
+        GSX_TRY(mg_apply_dev(mg, r, z));
+        HIP_OK(c, hipMemcpyAsync(p, z, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        HIP_OK(c, gsx::launch_dot_f64(n, r, z, P0, nullptr, s));
+        HIP_OK(c, gsx::launch_cg_scalar(1, P0, nv, nullptr, 0, st, hist, s));
+        while (true) {
+            {
+                KScope k(c, "spmv_f64");
+                HIP_OK(c, gsx::launch_spmv_f64(A->lpr, n, rp, ci, va, p, t, nullptr, P0, 1, st, s));
+            }
+            HIP_OK(c, gsx::launch_cg_scalar(2, P0, ns, nullptr, 0, st, hist, s));
+            HIP_OK(c, gsx::launch_cg_xr_f64(n, x, r, p, t, P1, st, s));
+            HIP_OK(c, gsx::launch_cg_scalar(5, P1, nv, nullptr, 0, st, hist, s));
+            GSX_TRY(read_state(c, w, h));
+            if (h.done) break;
+            GSX_TRY(mg_apply_dev(mg, r, z));
+            HIP_OK(c, gsx::launch_dot_f64(n, r, z, P0, nullptr, s));
+            HIP_OK(c, gsx::launch_cg_scalar(6, P0, nv, nullptr, 0, st, hist, s));
+            HIP_OK(c, gsx::launch_cg_pupdate_f64(n, p, z, st, s));
+        }
+    }
+    *iters = h.it;
+    *relres = h.err;
+    A->hist.resize(h.it + 1);
+    HIP_OK(c, hipMemcpyAsync(A->hist.data(), hist, sizeof(double) * (h.it + 1), hipMemcpyDeviceToHost, s));
+    HIP_OK(c, hipStreamSynchronize(s));
+    if (h.done == 1 || h.done == 3 || h.err < tol) return GSX_OK;
+    return fail(c, GSX_ERR_NOT_CONVERGED, "MG-PCG reached the maximum number of iterations");
+}
+
+static gsx_status cg_mg_entry(gsx_matrix *A, gsx_mg *mg, const double *b, double *x, double tol, int maxit,
+                              int *iters, double *relres)
+{
+    return cg_mg(A, mg, b, x, tol, maxit, iters, relres);
+}

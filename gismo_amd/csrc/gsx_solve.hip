@@ -343,6 +343,21 @@ __global__ __launch_bounds__(256) void k_cg_scalar(int step, const double *__res
         if (st->it >= st->maxit) st->done = 2;
         break;
     }
+    case 5: {
+        // residual check of a preconditioned iteration whose preconditioner
+        // is applied afterwards by the host (multigrid)
+        st->it += 1;
+        st->rr = a;
+        st->err = sqrt(a) / st->bnorm;
+        if (hist) hist[st->it] = st->err;
+        if (st->err < st->tol) st->done = 1;
+        else if (st->it >= st->maxit) st->done = 2;
+        break;
+    }
+    case 6:
+        st->beta = a / st->rz;
+        st->rz = a;
+        break;
     default:
         break;
     }

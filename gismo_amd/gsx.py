@@ -103,6 +103,12 @@ SIGNATURES = {
                      _i32p, _dp],
     "gsx_cg_history": [_vp, _dp, C.c_int32, _i32p],
     "gsx_l2_error": [_vp, _vp, _dp],
+    "gsx_mg_create": [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                      C.c_int32, _pp],
+    "gsx_mg_destroy": [_vp],
+    "gsx_mg_apply": [_vp, _vp, _vp],
+    "gsx_mg_level_info": [_vp, C.c_int32, _i64p, _i64p, _i64p],
+    "gsx_mg_level_download": [_vp, C.c_int32, _i32p, _i32p, _dp, _i32p, _i32p, _dp],
 }
 _RESTYPES = {"gsx_last_error": C.c_char_p}
 
@@ -360,3 +366,63 @@ class Matrix:
         if self.h:
             lib().gsx_matrix_destroy(self.h)
             self.h = None
+
+
+class Multigrid:
+    """Geometric multigrid preconditioner (gsMultiGridOp) on the device."""
+
+    def __init__(self, problem, matrix, nlevels, npre=1, npost=1, damping=1.0, cgc_damping=1.0,
+                 precision=FP64):
+        self.ctx = problem.ctx
+        h = C.c_void_p()
+        _check(self.ctx.h, lib().gsx_mg_create(problem.h, matrix.h, nlevels, npre, npost, 1, damping,
+                                               cgc_damping, precision, C.byref(h)))
+        self.h = h
+        self.nlevels = nlevels
+
+    def apply(self, r, z):
+        _check(self.ctx.h, lib().gsx_mg_apply(self.h, r.ptr, z.ptr))
+
+    def level(self, lvl):
+        """(operator CSR as scipy, transfer to the next finer level or None)."""
+        import scipy.sparse as sp
+        n, nnz, pnnz = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(self.ctx.h, lib().gsx_mg_level_info(self.h, lvl, C.byref(n), C.byref(nnz), C.byref(pnnz)))
+        rp = np.zeros(n.value + 1, np.int32)
+        ci = np.zeros(max(1, nnz.value), np.int32)
+        va = np.zeros(max(1, nnz.value))
+        nf = None
+        if lvl < self.nlevels - 1:
+            nf_, _, _ = C.c_int64(), None, None
+            _check(self.ctx.h, lib().gsx_mg_level_info(self.h, lvl + 1, C.byref(nf_), None, None))
+            nf = nf_.value
+            prp = np.zeros(nf + 1, np.int32)
+            pci = np.zeros(max(1, pnnz.value), np.int32)
+            pva = np.zeros(max(1, pnnz.value))
+            _check(self.ctx.h, lib().gsx_mg_level_download(
+                self.h, lvl, rp.ctypes.data_as(_i32p), ci.ctypes.data_as(_i32p), va.ctypes.data_as(_dp),
+                prp.ctypes.data_as(_i32p), pci.ctypes.data_as(_i32p), pva.ctypes.data_as(_dp)))
+            P = sp.csr_matrix((pva[:pnnz.value], pci[:pnnz.value], prp), shape=(nf, n.value))
+        else:
+            _check(self.ctx.h, lib().gsx_mg_level_download(
+                self.h, lvl, rp.ctypes.data_as(_i32p), ci.ctypes.data_as(_i32p), va.ctypes.data_as(_dp),
+                None, None, None))
+            P = None
+        A = sp.csr_matrix((va[:nnz.value], ci[:nnz.value], rp), shape=(n.value, n.value))
+        return A, P
+
+    def close(self):
+        if self.h:
+            lib().gsx_mg_destroy(self.h)
+            self.h = None
+
+
+def mg_cg_solve(matrix, mg, b, x, tol=1e-8, maxit=500):
+    """CG preconditioned by the multigrid cycle (gsConjugateGradient with a
+    gsMultiGridOp preconditioner)."""
+    it = C.c_int32()
+    rr = C.c_double()
+    st = lib().gsx_cg_solve(matrix.h, PREC_MG, 1.0, mg.h, FP64, b.ptr, x.ptr, tol, maxit, C.byref(it),
+                            C.byref(rr))
+    _check(matrix.ctx.h, st, (GSX_ERR_NOT_CONVERGED,))
+    return it.value, rr.value, st == GSX_OK

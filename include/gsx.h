@@ -184,6 +184,36 @@ gsx_status gsx_cg_solve(gsx_matrix *A, int32_t precond, double tau,
 gsx_status gsx_cg_history(gsx_matrix *A, double *hist_host, int32_t cap,
                           int32_t *count);
 
+/* ---- geometric multigrid -------------------------------------------------- */
+/* gsMultiGridOp::make + init (src/gsMultiGrid/gsMultiGrid.hpp:19-94) on a
+ * hierarchy of nlevels obtained by coarsening every patch's knot vectors
+ * (gsGridHierarchy, src/gsMultiGrid/gsGridHierarchy.hpp:27-101; the fine
+ * problem needs an even element count per direction on every level):
+ * transfers from knot insertion restricted to free DOFs
+ * (gsMultiBasis::combineTransferMatrices, src/gsCore/gsMultiBasis.hpp:
+ * 137-178), Galerkin coarse operators P^T A P, damped Jacobi smoothing
+ * (npre/npost steps, damping), coarse-grid correction damping, exact FP64
+ * coarse solve.  precision GSX_MIXED runs the cycle in FP32.  Only
+ * V-cycles (ncycles = 1). */
+gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nlevels,
+                         int32_t npre, int32_t npost, int32_t ncycles,
+                         double damping, double cgc_damping, int32_t precision,
+                         gsx_mg **out);
+gsx_status gsx_mg_destroy(gsx_mg *mg);
+/* z = MG(r): one cycle from a zero start (gsPreconditionerOp::apply,
+ * src/gsSolver/gsPreconditioner.h:79-84). */
+gsx_status gsx_mg_apply(gsx_mg *mg, const double *r_dev, double *z_dev);
+/* Level lvl (0 = coarsest): operator size/nnz and, below the finest level,
+ * the nnz of the transfer to the next finer level. */
+gsx_status gsx_mg_level_info(const gsx_mg *mg, int32_t lvl, int64_t *n,
+                             int64_t *nnz, int64_t *transfer_nnz);
+/* Host copies of the level operator (CSR) and of its transfer (CSR, rows =
+ * next finer level's free DOFs); any pointer may be NULL. */
+gsx_status gsx_mg_level_download(const gsx_mg *mg, int32_t lvl, int32_t *rowptr_host,
+                                 int32_t *col_host, double *val_host,
+                                 int32_t *prowptr_host, int32_t *pcol_host,
+                                 double *pval_host);
+
 /* ---- reporting ---------------------------------------------------------- */
 /* L2 error of the discrete solution (free coefficients, eliminated = 0)
  * against u = prod_r sin(pi x_r) with the assembly quadrature

@@ -1,0 +1,92 @@
+"""GPU multigrid against the oracle: transfers, Galerkin level operators,
+one cycle, and MG-preconditioned CG (FP64 and FP32 cycles)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from gismo_amd import gsx, problems
+
+pytestmark = pytest.mark.gpu
+
+
+def _hierarchy(make, ns):
+    levels = [make(n) for n in ns]
+    maps = [O.mapper(p.patches, p.dir_sides, p.ifaces) for p in levels]
+    return levels, maps
+
+
+def _oracle_mg(levels, maps, npre=2, npost=2, damping=0.6):
+    Af, bf = O.assemble(levels[-1].patches, maps[-1][0], maps[-1][1], 0)
+    Ps = [O.transfer_multipatch(levels[i].patches, maps[i][0], maps[i][1], maps[i + 1][0], maps[i + 1][1])
+          for i in range(len(levels) - 1)]
+    return Af, bf, Ps, O.Multigrid(Af, Ps, npre=npre, npost=npost, damping=damping)
+
+
+def _rowscaled(A, B):
+    A = A.tocsr()
+    B = B.tocsr()
+    assert np.array_equal(A.indptr, B.indptr) and np.array_equal(A.indices, B.indices)
+    err = 0.0
+    for i in range(A.shape[0]):
+        s, e = A.indptr[i], A.indptr[i + 1]
+        if e > s:
+            err = max(err, np.abs(A.data[s:e] - B.data[s:e]).max() / np.abs(B.data[s:e]).max())
+    return err
+
+
+CASES = {
+    "grid2x2-p3": (lambda n: problems.square_grid(2, 2, n, 3), (2, 4, 8)),
+    "grid3x2-p2": (lambda n: problems.square_grid(3, 2, n, 2), (2, 4, 8)),
+    "cube-p2": (lambda n: problems.poisson_noisy_cube(n=n, p=2, seed=4), (2, 4, 8)),
+}
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_transfers_and_galerkin_levels(ctx, name):
+    make, ns = CASES[name]
+    print(f"[case] {name}", flush=True)
+    levels, maps = _hierarchy(make, ns)
+    Af, bf, Ps, omg = _oracle_mg(levels, maps)
+    gp = gsx.Problem(ctx, levels[-1])
+    A, b = gp.assemble()
+    mg = gsx.Multigrid(gp, A, len(ns), 2, 2, 0.6)
+    for lvl in range(len(ns)):
+        Ag, Pg = mg.level(lvl)
+        Ao = omg.matrix(lvl)
+        assert _rowscaled(Ag, Ao) <= 1e-12, lvl
+        if Pg is not None:
+            Po = Ps[lvl]
+            Po.sort_indices()
+            assert np.array_equal(Pg.indptr, Po.indptr) and np.array_equal(Pg.indices, Po.indices)
+            assert np.array_equal(Pg.data, Po.data)
+    # one cycle
+    rng = np.random.default_rng(1)
+    r = rng.standard_normal(gp.nfree)
+    rd = ctx.to_device(r)
+    zd = ctx.array(gp.nfree)
+    mg.apply(rd, zd)
+    zo = omg.apply(r)
+    assert np.linalg.norm(zd.download() - zo) <= 1e-10 * np.linalg.norm(zo)
+    mg.close()
+
+
+@pytest.mark.parametrize("name", ["grid2x2-p3", "cube-p2"])
+def test_mg_pcg_matches_oracle(ctx, name):
+    make, ns = CASES[name]
+    levels, maps = _hierarchy(make, ns)
+    Af, bf, Ps, omg = _oracle_mg(levels, maps)
+    xo, ito, erro = O.cg_solve(Af, bf, tol=1e-10, maxit=200, precond="mg", mg=omg)
+    gp = gsx.Problem(ctx, levels[-1])
+    A, b = gp.assemble()
+    for prec in (gsx.FP64, gsx.MIXED):
+        mg = gsx.Multigrid(gp, A, len(ns), 2, 2, 0.6, precision=prec)
+        x = ctx.array(gp.nfree).zero()
+        it, rel, ok = gsx.mg_cg_solve(A, mg, b, x, tol=1e-10, maxit=200)
+        assert ok and rel < 1e-10
+        if prec == gsx.FP64:
+            assert abs(it - ito) <= 1
+        else:
+            assert it <= ito + 3
+        xs = x.download()
+        assert np.linalg.norm(xs - xo) <= 1e-8 * np.linalg.norm(xo)
+        mg.close()
