@@ -1045,6 +1045,50 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
     return GSX_OK;
 }
 
+gsx_status gsx_matrix_from_csr(gsx_ctx *c, int32_t n, const int32_t *rp, const int32_t *ci,
+                               const double *v, int32_t with_fp32, gsx_matrix **out)
+{
+    if (!c || n < 0 || !rp || !out) return GSX_ERR_ARG;
+    const int64_t nnz = rp[n];
+    if (nnz < 0 || (nnz > 0 && (!ci || !v))) return fail(c, GSX_ERR_ARG, "invalid CSR arrays");
+    for (int i = 0; i < n; ++i)
+        if (rp[i + 1] < rp[i]) return fail(c, GSX_ERR_ARG, "row pointers must not decrease");
+    for (int64_t k = 0; k < nnz; ++k)
+        if (ci[k] < 0 || ci[k] >= n) return fail(c, GSX_ERR_ARG, "column index out of range");
+    hipStream_t s = c->stream;
+    auto A = std::make_unique<gsx_matrix>();
+    A->ctx = c;
+    A->n = n;
+    A->nnz = nnz;
+    A->rowptr = dalloc(c, sizeof(int) * (n + 1));
+    A->col = dalloc(c, sizeof(int) * std::max<int64_t>(1, nnz));
+    A->val = dalloc(c, sizeof(double) * std::max<int64_t>(1, nnz));
+    if (!A->rowptr || !A->col || !A->val) return fail(c, GSX_ERR_OOM, "device allocation (matrix)");
+    HIP_OK(c, hipMemcpyAsync(A->rowptr.get(), rp, sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+    if (nnz) {
+        HIP_OK(c, hipMemcpyAsync(A->col.get(), ci, sizeof(int) * nnz, hipMemcpyHostToDevice, s));
+        HIP_OK(c, hipMemcpyAsync(A->val.get(), v, sizeof(double) * nnz, hipMemcpyHostToDevice, s));
+    }
+    A->lpr = gsx::spmv_lpr(n ? (double)nnz / n : 1.0);
+    A->mixed = with_fp32 != 0;
+    if (A->mixed) {
+        A->val32 = dalloc(c, sizeof(float) * std::max<int64_t>(1, nnz));
+        if (!A->val32) return fail(c, GSX_ERR_OOM, "device allocation (fp32 values)");
+        HIP_OK(c, gsx::launch_f64_to_f32(nnz, dp<double>(A->val), dp<float>(A->val32), s));
+    }
+    HIP_OK(c, hipStreamSynchronize(s));
+    *out = A.release();
+    return GSX_OK;
+}
+
+gsx_status gsx_ctx_release_cache(gsx_ctx *c)
+{
+    if (!c) return GSX_ERR_ARG;
+    (void)hipStreamSynchronize(c->stream);
+    c->mem->trim();
+    return GSX_OK;
+}
+
 gsx_status gsx_matrix_destroy(gsx_matrix *A)
 {
     if (!A) return GSX_ERR_ARG;

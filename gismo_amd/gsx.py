@@ -94,6 +94,8 @@ SIGNATURES = {
     "gsx_problem_min_detj": [_vp, _dp],
     "gsx_assemble": [_vp, C.c_int32, C.c_int32, _pp, _vp],
     "gsx_matrix_destroy": [_vp],
+    "gsx_matrix_from_csr": [_vp, C.c_int32, _i32p, _i32p, _dp, C.c_int32, _pp],
+    "gsx_ctx_release_cache": [_vp],
     "gsx_matrix_info": [_vp, _i64p, _i64p],
     "gsx_matrix_download": [_vp, _i32p, _i32p, _dp],
     "gsx_matrix_device_ptrs": [_vp, _pp, _pp, _pp],

@@ -1,0 +1,1200 @@
+// gismo.h -- drop-in C++ interface for G+Smo's isogeometric Poisson path,
+// backed by libgsx (HIP, gfx950) through its C ABI (include/gsx.h).
+//
+// Mirrors the names, signatures and call sequences of the reference for the
+// path it replaces (paths relative to the G+Smo source tree):
+//   containers    gsMatrix, gsSparseMatrix (column-major, int indices)
+//                 src/gsMatrix/gsMatrix.h:38, src/gsMatrix/gsSparseMatrix.h:139-448
+//   geometry      gsKnotVector, gsTensorBSpline, gsMultiPatch, gsMultiBasis,
+//                 gsNurbsCreator::BSplineSquare/BSplineSquareGrid/BSplineCube
+//                 src/gsNurbs/gsKnotVector.h, src/gsCore/gsMultiPatch.h,
+//                 src/gsNurbs/gsNurbsCreator.hpp:655-745
+//   conditions    gsBoundaryConditions (homogeneous Dirichlet sides)
+//                 src/gsPde/gsBoundaryConditions.h:25-981
+//   assembly      gsExprAssembler (igrad(u,G)*igrad(u,G).tr()*meas(G) [*kappa],
+//                 u*ff*meas(G)) src/gsAssembler/gsExprAssembler.h:29-833;
+//                 gsPoissonAssembler src/gsAssembler/gsPoissonAssembler.h:35-142
+//   solvers       gsLinearOperator, gsMatrixOp, gsJacobiOp/makeJacobiOp,
+//                 gsIterativeSolver, gsConjugateGradient, gsMultiGridOp
+//                 src/gsSolver/*.h, src/gsMultiGrid/gsMultiGrid.h
+//   evaluation    gsExprEvaluator::integral of the L2 error against the
+//                 manufactured solution src/gsAssembler/gsExprEvaluator.h:382-450
+//
+// Restrictions of this drop-in (each raises std::runtime_error when hit):
+// isoparametric patches, or multilinear single-element geometries (the
+// gsNurbsCreator boxes) mapped into the discretisation basis; equal degree
+// in all directions, degree 1..4; homogeneous Dirichlet conditions; the
+// source d*pi^2*prod sin(pi x_i) (as its gsFunctionExpr string) or a
+// per-element constant.
+#pragma once
+
+#include <gsx.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace gismo {
+
+typedef double real_t;
+typedef int index_t;
+typedef short short_t;
+
+// ---------------------------------------------------------------- errors
+namespace detail {
+
+inline gsx_ctx *ctx()
+{
+    struct Holder {
+        gsx_ctx *h = nullptr;
+        Holder()
+        {
+            if (gsx_ctx_create(0, &h) != GSX_OK) h = nullptr;
+        }
+        ~Holder()
+        {
+            if (h) gsx_ctx_destroy(h);
+        }
+    };
+    static Holder holder;
+    if (!holder.h) throw std::runtime_error("gismo (libgsx): no GPU device available");
+    return holder.h;
+}
+
+inline void check(gsx_status s, const char *what)
+{
+    if (s == GSX_OK) return;
+    std::ostringstream os;
+    os << what << ": " << gsx_last_error(ctx());
+    if (s == GSX_ERR_ARG || s == GSX_ERR_JACOBIAN) throw std::runtime_error(os.str());
+    throw std::runtime_error(os.str());
+}
+
+struct DeviceVec {
+    void *p = nullptr;
+    int64_t bytes = 0;
+    explicit DeviceVec(int64_t b) : bytes(b) { check(gsx_device_alloc(ctx(), b, &p), "device allocation"); }
+    ~DeviceVec() { if (p) gsx_device_free(ctx(), p); }
+    DeviceVec(const DeviceVec &) = delete;
+    DeviceVec &operator=(const DeviceVec &) = delete;
+    double *d() const { return static_cast<double *>(p); }
+};
+
+}  // namespace detail
+
+// ---------------------------------------------------------------- gsMatrix
+/// Dense column-major matrix (the subset of gsMatrix the path uses).
+template <class T = real_t>
+class gsMatrix {
+public:
+    gsMatrix() = default;
+    gsMatrix(index_t r, index_t c) { resize(r, c); }
+    index_t rows() const { return m_rows; }
+    index_t cols() const { return m_cols; }
+    index_t size() const { return m_rows * m_cols; }
+    void resize(index_t r, index_t c)
+    {
+        m_rows = r;
+        m_cols = c;
+        m_data.assign((size_t)r * c, T(0));
+    }
+    void setZero() { std::fill(m_data.begin(), m_data.end(), T(0)); }
+    void setZero(index_t r, index_t c) { resize(r, c); }
+    T &operator()(index_t i, index_t j) { return m_data[(size_t)j * m_rows + i]; }
+    const T &operator()(index_t i, index_t j) const { return m_data[(size_t)j * m_rows + i]; }
+    T &operator()(index_t i) { return m_data[i]; }
+    const T &operator()(index_t i) const { return m_data[i]; }
+    T &at(index_t i) { return m_data.at(i); }
+    const T &at(index_t i) const { return m_data.at(i); }
+    T *data() { return m_data.data(); }
+    const T *data() const { return m_data.data(); }
+    T norm() const
+    {
+        T s = 0;
+        for (T v : m_data) s += v * v;
+        return std::sqrt(s);
+    }
+    static gsMatrix Zero(index_t r, index_t c) { return gsMatrix(r, c); }
+
+private:
+    index_t m_rows = 0, m_cols = 0;
+    std::vector<T> m_data;
+};
+
+template <class T = real_t>
+using gsVector = gsMatrix<T>;
+
+// ---------------------------------------------------------------- gsSparseMatrix
+/// Column-major compressed sparse matrix with int indices (Eigen's
+/// compressed storage accessors).  Matrices produced by the assemblers are
+/// symmetric bitwise and keep a handle to their device copy, which the
+/// solvers use without a host round trip.
+template <class T = real_t>
+class gsSparseMatrix {
+public:
+    gsSparseMatrix() = default;
+    gsSparseMatrix(index_t r, index_t c) : m_rows(r), m_cols(c), m_outer(c + 1, 0) {}
+    index_t rows() const { return m_rows; }
+    index_t cols() const { return m_cols; }
+    index_t nonZeros() const { return (index_t)m_val.size(); }
+    bool isCompressed() const { return true; }
+    const index_t *outerIndexPtr() const { return m_outer.data(); }
+    const index_t *innerIndexPtr() const { return m_inner.data(); }
+    const T *valuePtr() const { return m_val.data(); }
+    T coeff(index_t i, index_t j) const
+    {
+        auto b = m_inner.begin() + m_outer[j], e = m_inner.begin() + m_outer[j + 1];
+        auto it = std::lower_bound(b, e, i);
+        return (it != e && *it == i) ? m_val[it - m_inner.begin()] : T(0);
+    }
+    T at(index_t i, index_t j) const { return coeff(i, j); }
+    /// Fill from compressed columns (takes the arrays).
+    void setFromCompressed(index_t r, index_t c, std::vector<index_t> outer, std::vector<index_t> inner,
+                           std::vector<T> val)
+    {
+        m_rows = r;
+        m_cols = c;
+        m_outer = std::move(outer);
+        m_inner = std::move(inner);
+        m_val = std::move(val);
+        m_dev.reset();
+    }
+    /// y = A x on the host
+    gsMatrix<T> operator*(const gsMatrix<T> &x) const
+    {
+        gsMatrix<T> y(m_rows, 1);
+        for (index_t j = 0; j < m_cols; ++j)
+            for (index_t k = m_outer[j]; k < m_outer[j + 1]; ++k) y(m_inner[k]) += m_val[k] * x(j);
+        return y;
+    }
+    // device twin (set by the assemblers)
+    std::shared_ptr<gsx_matrix> devicePtr() const { return m_dev; }
+    void setDevice(std::shared_ptr<gsx_matrix> d) { m_dev = std::move(d); }
+    // the problem it was assembled on (multigrid needs the discretisation)
+    std::shared_ptr<gsx_problem> problemPtr() const { return m_prob; }
+    void setProblem(std::shared_ptr<gsx_problem> p) { m_prob = std::move(p); }
+
+private:
+    index_t m_rows = 0, m_cols = 0;
+    std::vector<index_t> m_outer, m_inner;
+    std::vector<T> m_val;
+    std::shared_ptr<gsx_matrix> m_dev;
+    std::shared_ptr<gsx_problem> m_prob;
+};
+
+// ---------------------------------------------------------------- options
+/// Typed key/value options (gsOptionList, src/gsIO/gsOptionList.h:32).
+class gsOptionList {
+public:
+    gsOptionList &addInt(const std::string &k, const std::string & = "", index_t v = 0) { m_i[k] = v; return *this; }
+    gsOptionList &addReal(const std::string &k, const std::string & = "", real_t v = 0) { m_r[k] = v; return *this; }
+    gsOptionList &addSwitch(const std::string &k, const std::string & = "", bool v = false) { m_b[k] = v; return *this; }
+    gsOptionList &addString(const std::string &k, const std::string & = "", const std::string &v = "") { m_s[k] = v; return *this; }
+    void setInt(const std::string &k, index_t v) { m_i[k] = v; }
+    void setReal(const std::string &k, real_t v) { m_r[k] = v; }
+    void setSwitch(const std::string &k, bool v) { m_b[k] = v; }
+    void setString(const std::string &k, const std::string &v) { m_s[k] = v; }
+    index_t askInt(const std::string &k, index_t def = 0) const { auto it = m_i.find(k); return it == m_i.end() ? def : it->second; }
+    real_t askReal(const std::string &k, real_t def = 0) const { auto it = m_r.find(k); return it == m_r.end() ? def : it->second; }
+    bool askSwitch(const std::string &k, bool def = false) const { auto it = m_b.find(k); return it == m_b.end() ? def : it->second; }
+    std::string askString(const std::string &k, const std::string &def = "") const { auto it = m_s.find(k); return it == m_s.end() ? def : it->second; }
+    index_t getInt(const std::string &k) const { return m_i.at(k); }
+    real_t getReal(const std::string &k) const { return m_r.at(k); }
+
+private:
+    std::map<std::string, index_t> m_i;
+    std::map<std::string, real_t> m_r;
+    std::map<std::string, bool> m_b;
+    std::map<std::string, std::string> m_s;
+};
+
+// ---------------------------------------------------------------- boundary
+namespace boundary {
+enum side { none = 0, west = 1, east = 2, south = 3, north = 4, front = 5, back = 6 };
+}
+namespace condition_type {
+enum type { dirichlet = 0, neumann = 1, robin = 2 };
+}
+namespace dirichlet {
+enum strategy { elimination = 11, nitsche = 12, interpolation = 1, l2Projection = 2, homogeneous = 3 };
+}
+namespace iFace {
+enum strategy { glue = 1, dg = 2, conforming = 1 };
+}
+
+// ---------------------------------------------------------------- knots
+/// Knot vector; uniform construction follows gsKnotVector::initUniform
+/// (src/gsNurbs/gsKnotVector.hpp:643-688): interior knot i = first + i*h.
+class gsKnotVector {
+public:
+    gsKnotVector() = default;
+    gsKnotVector(real_t first, real_t last, unsigned interior, unsigned mult_ends, unsigned mult_interior = 1,
+                 int degree = -1)
+    {
+        m_deg = degree == -1 ? (int)mult_ends - 1 : degree;
+        const real_t h = (last - first) / (real_t)(interior + 1);
+        m_k.assign(mult_ends, first);
+        for (unsigned i = 1; i <= interior; ++i) m_k.insert(m_k.end(), mult_interior, first + (real_t)i * h);
+        m_k.insert(m_k.end(), mult_ends, last);
+    }
+    gsKnotVector(std::vector<real_t> knots, int degree) : m_k(std::move(knots)), m_deg(degree) {}
+    int degree() const { return m_deg; }
+    size_t size() const { return m_k.size(); }
+    real_t operator[](size_t i) const { return m_k[i]; }
+    const std::vector<real_t> &knots() const { return m_k; }
+    index_t numBasis() const { return (index_t)m_k.size() - m_deg - 1; }
+    /// insert numKnots uniformly spaced knots in every non-empty span
+    /// (gsKnotVector::uniformRefine / getUniformRefinementKnots,
+    /// src/gsNurbs/gsKnotVector.hpp:820-834,1048-1062)
+    void uniformRefine(int numKnots = 1)
+    {
+        std::vector<real_t> out;
+        for (size_t i = 0; i + 1 < m_k.size(); ++i) {
+            out.push_back(m_k[i]);
+            if (m_k[i + 1] > m_k[i]) {
+                const real_t step = (m_k[i + 1] - m_k[i]) / (real_t)(numKnots + 1);
+                for (int j = 1; j <= numKnots; ++j) out.push_back(m_k[i] + (real_t)j * step);
+            }
+        }
+        out.push_back(m_k.back());
+        m_k.swap(out);
+    }
+    /// raise the degree by r, keeping the continuity: every distinct knot
+    /// gains r in multiplicity (gsBSplineBasis::degreeElevate)
+    void degreeElevate(int r = 1)
+    {
+        std::vector<real_t> out;
+        for (size_t i = 0; i < m_k.size(); ++i) {
+            out.push_back(m_k[i]);
+            if (i + 1 == m_k.size() || m_k[i + 1] != m_k[i]) out.insert(out.end(), r, m_k[i]);
+        }
+        m_k.swap(out);
+        m_deg += r;
+    }
+
+private:
+    std::vector<real_t> m_k;
+    int m_deg = 0;
+};
+
+// ---------------------------------------------------------------- patches
+/// Tensor-product B-spline patch: knot vectors and control points
+/// (rows = basis functions, lexicographic with direction 0 fastest).
+class gsTensorBSplinePatch {
+public:
+    gsTensorBSplinePatch() = default;
+    gsTensorBSplinePatch(std::vector<gsKnotVector> kv, gsMatrix<real_t> coefs)
+        : m_kv(std::move(kv)), m_coefs(std::move(coefs)) {}
+    short_t domainDim() const { return (short_t)m_kv.size(); }
+    short_t targetDim() const { return (short_t)m_coefs.cols(); }
+    const gsKnotVector &knots(int k) const { return m_kv[k]; }
+    const gsMatrix<real_t> &coefs() const { return m_coefs; }
+    gsMatrix<real_t> &coefs() { return m_coefs; }
+    index_t basisSize() const
+    {
+        index_t n = 1;
+        for (auto &k : m_kv) n *= k.numBasis();
+        return n;
+    }
+    /// x = G(xi) of a multilinear single-element patch, as nested linear
+    /// interpolation along direction 0, then 1, then 2 (a box [a, b]
+    /// gives a + t (b - a) in every coordinate)
+    void evalMultilinear(const real_t *xi, real_t *x) const
+    {
+        const int d = domainDim(), D = targetDim();
+        for (int r = 0; r < D; ++r) {
+            real_t v[8];
+            for (int c = 0; c < (1 << d); ++c) v[c] = m_coefs(c, r);
+            for (int k = 0; k < d; ++k) {
+                const real_t t = (xi[k] - m_kv[k][0]) / (m_kv[k][m_kv[k].size() - 1] - m_kv[k][0]);
+                for (int c = 0; c < (1 << (d - k - 1)); ++c) v[c] = v[2 * c] + t * (v[2 * c + 1] - v[2 * c]);
+            }
+            x[r] = v[0];
+        }
+    }
+    bool isMultilinearBox() const
+    {
+        for (auto &k : m_kv)
+            if (k.degree() != 1 || k.numBasis() != 2) return false;
+        return true;
+    }
+
+private:
+    std::vector<gsKnotVector> m_kv;
+    gsMatrix<real_t> m_coefs;
+};
+
+template <short_t d, class T = real_t>
+using gsTensorBSpline = gsTensorBSplinePatch;
+
+/// Patches plus their topology (conforming interfaces, boundary sides).
+template <class T = real_t>
+class gsMultiPatch {
+public:
+    struct Interface { int p1, s1, p2, s2, m0, f0, m1, f1; };
+    void addPatch(const gsTensorBSplinePatch &p) { m_p.push_back(p); }
+    void addPatch(std::unique_ptr<gsTensorBSplinePatch> p) { m_p.push_back(*p); }
+    index_t nPatches() const { return (index_t)m_p.size(); }
+    const gsTensorBSplinePatch &patch(index_t i) const { return m_p[i]; }
+    short_t dim() const { return m_p.empty() ? 0 : m_p[0].domainDim(); }
+    /// find conforming interfaces by matching side corners
+    /// (gsMultiPatch::computeTopology, src/gsCore/gsMultiPatch.hpp:366-493),
+    /// for the multilinear box patches and isoparametric patches whose side
+    /// corners are control points
+    bool computeTopology(real_t tol = 1e-4)
+    {
+        m_if.clear();
+        const int d = dim();
+        auto side_corners = [&](int k, int side, std::vector<std::vector<real_t>> &cs,
+                                std::vector<std::vector<int>> &idx) {
+            const gsTensorBSplinePatch &P = m_p[k];
+            int nb[3] = {1, 1, 1};
+            for (int j = 0; j < d; ++j) nb[j] = P.knots(j).numBasis();
+            const int dir = (side - 1) / 2, par = (side - 1) % 2;
+            cs.clear();
+            idx.clear();
+            for (int c = 0; c < (1 << (d - 1)); ++c) {
+                int ii[3] = {0, 0, 0}, t = 0;
+                for (int j = 0; j < d; ++j) {
+                    if (j == dir) { ii[j] = par ? nb[j] - 1 : 0; continue; }
+                    ii[j] = (c >> t) & 1 ? nb[j] - 1 : 0;
+                    ++t;
+                }
+                const int l = ii[0] + nb[0] * (ii[1] + nb[1] * ii[2]);
+                std::vector<real_t> x(P.targetDim());
+                for (int r = 0; r < P.targetDim(); ++r) x[r] = P.coefs()(l, r);
+                cs.push_back(x);
+                idx.push_back({ii[0], ii[1], ii[2]});
+            }
+        };
+        std::vector<std::vector<int>> used(m_p.size(), std::vector<int>(2 * d + 1, 0));
+        for (int a = 0; a < (int)m_p.size(); ++a)
+            for (int sa = 1; sa <= 2 * d; ++sa) {
+                if (used[a][sa]) continue;
+                std::vector<std::vector<real_t>> ca, cb;
+                std::vector<std::vector<int>> ia, ib;
+                side_corners(a, sa, ca, ia);
+                for (int b = a + 1; b < (int)m_p.size() && !used[a][sa]; ++b)
+                    for (int sb = 1; sb <= 2 * d && !used[a][sa]; ++sb) {
+                        if (used[b][sb]) continue;
+                        side_corners(b, sb, cb, ib);
+                        // corner correspondence
+                        std::vector<int> match(ca.size(), -1);
+                        bool ok = true;
+                        for (size_t i = 0; i < ca.size() && ok; ++i) {
+                            for (size_t j = 0; j < cb.size(); ++j) {
+                                real_t dist = 0;
+                                for (size_t r = 0; r < ca[i].size(); ++r) dist += std::fabs(ca[i][r] - cb[j][r]);
+                                if (dist < tol) { match[i] = (int)j; break; }
+                            }
+                            ok = match[i] >= 0;
+                        }
+                        if (!ok) continue;
+                        // tangential directions of a -> directions/orientation of b
+                        const int dira = (sa - 1) / 2, dirb = (sb - 1) / 2;
+                        int rec[8] = {a, sa, b, sb, 0, 0, 0, 0}, t = 0;
+                        for (int j = 0; j < d; ++j) {
+                            if (j == dira) continue;
+                            // corner 0 vs the corner that moves along j in a
+                            const int c1 = 1 << t;
+                            const std::vector<int> &p0 = ib[match[0]], &p1 = ib[match[c1]];
+                            int mj = -1, fj = 0;
+                            for (int m = 0; m < d; ++m)
+                                if (m != dirb && p0[m] != p1[m]) { mj = m; fj = p1[m] < p0[m]; }
+                            rec[4 + 2 * t] = mj;
+                            rec[5 + 2 * t] = fj;
+                            ++t;
+                        }
+                        m_if.push_back({rec[0], rec[1], rec[2], rec[3], rec[4], rec[5], rec[6], rec[7]});
+                        used[a][sa] = used[b][sb] = 1;
+                    }
+            }
+        m_bdr.clear();
+        for (int a = 0; a < (int)m_p.size(); ++a)
+            for (int s = 1; s <= 2 * d; ++s)
+                if (!used[a][s]) m_bdr.push_back({a, s});
+        return true;
+    }
+    const std::vector<Interface> &interfaces() const { return m_if; }
+    const std::vector<std::pair<int, int>> &boundaries() const { return m_bdr; }
+
+private:
+    std::vector<gsTensorBSplinePatch> m_p;
+    std::vector<Interface> m_if;
+    std::vector<std::pair<int, int>> m_bdr;
+};
+
+/// Discretisation bases, one tensor B-spline basis per patch.
+template <class T = real_t>
+class gsMultiBasis {
+public:
+    gsMultiBasis() = default;
+    explicit gsMultiBasis(const gsMultiPatch<T> &mp, bool = false)
+    {
+        for (index_t k = 0; k < mp.nPatches(); ++k) {
+            std::vector<gsKnotVector> kv;
+            for (int j = 0; j < mp.patch(k).domainDim(); ++j) kv.push_back(mp.patch(k).knots(j));
+            m_kv.push_back(kv);
+        }
+    }
+    index_t nBases() const { return (index_t)m_kv.size(); }
+    void degreeElevate(int r = 1)
+    {
+        for (auto &p : m_kv)
+            for (auto &k : p) k.degreeElevate(r);
+    }
+    void setDegree(int deg)
+    {
+        for (auto &p : m_kv)
+            for (auto &k : p)
+                if (k.degree() < deg) k.degreeElevate(deg - k.degree());
+    }
+    void uniformRefine(int numKnots = 1)
+    {
+        for (auto &p : m_kv)
+            for (auto &k : p) k.uniformRefine(numKnots);
+    }
+    const gsKnotVector &knots(int patch, int dir) const { return m_kv[patch][dir]; }
+    int degree(int patch = 0, int dir = 0) const { return m_kv[patch][dir].degree(); }
+    int maxCwiseDegree() const
+    {
+        int m = 0;
+        for (auto &p : m_kv)
+            for (auto &k : p) m = std::max(m, k.degree());
+        return m;
+    }
+    int minCwiseDegree() const
+    {
+        int m = 1 << 20;
+        for (auto &p : m_kv)
+            for (auto &k : p) m = std::min(m, k.degree());
+        return m;
+    }
+    index_t size(int patch) const
+    {
+        index_t n = 1;
+        for (auto &k : m_kv[patch]) n *= k.numBasis();
+        return n;
+    }
+    index_t totalSize() const
+    {
+        index_t n = 0;
+        for (int k = 0; k < nBases(); ++k) n += size(k);
+        return n;
+    }
+
+private:
+    std::vector<std::vector<gsKnotVector>> m_kv;
+};
+
+/// gsNurbsCreator subset (src/gsNurbs/gsNurbsCreator.hpp:655-745): degree-1
+/// single-element boxes.
+template <class T = real_t>
+struct gsNurbsCreator {
+    static std::unique_ptr<gsTensorBSplinePatch> BSplineSquare(real_t r = 1, real_t x = 0, real_t y = 0)
+    {
+        gsMatrix<real_t> C(4, 2);
+        const real_t pts[4][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}};
+        for (int i = 0; i < 4; ++i) {
+            C(i, 0) = r * pts[i][0] + x;
+            C(i, 1) = r * pts[i][1] + y;
+        }
+        return std::unique_ptr<gsTensorBSplinePatch>(
+            new gsTensorBSplinePatch({gsKnotVector(0, 1, 0, 2), gsKnotVector(0, 1, 0, 2)}, C));
+    }
+    /// n x m squares; patch index i*m + j at [lx + r i, ..] x [ly + r j, ..]
+    static gsMultiPatch<T> BSplineSquareGrid(int n, int m, real_t r = 1, real_t lx = 0, real_t ly = 0)
+    {
+        gsMultiPatch<T> mp;
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < m; ++j) mp.addPatch(BSplineSquare(r, lx + r * (real_t)i, ly + r * (real_t)j));
+        mp.computeTopology();
+        return mp;
+    }
+    static std::unique_ptr<gsTensorBSplinePatch> BSplineCube(real_t r = 1, real_t x = 0, real_t y = 0, real_t z = 0)
+    {
+        gsMatrix<real_t> C(8, 3);
+        for (int i = 0; i < 8; ++i) {
+            C(i, 0) = r * (i & 1) + x;
+            C(i, 1) = r * ((i >> 1) & 1) + y;
+            C(i, 2) = r * ((i >> 2) & 1) + z;
+        }
+        return std::unique_ptr<gsTensorBSplinePatch>(new gsTensorBSplinePatch(
+            {gsKnotVector(0, 1, 0, 2), gsKnotVector(0, 1, 0, 2), gsKnotVector(0, 1, 0, 2)}, C));
+    }
+};
+
+// ---------------------------------------------------------------- functions
+/// Source functions the device path evaluates itself.
+template <class T = real_t>
+class gsFunctionExpr {
+public:
+    gsFunctionExpr() = default;
+    /// accepts "0" and the manufactured sources "2*pi^2*sin(pi*x)*sin(pi*y)"
+    /// and "3*pi^2*sin(pi*x)*sin(pi*y)*sin(pi*z)" (blanks ignored)
+    gsFunctionExpr(const std::string &expr, short_t dim) : m_dim(dim)
+    {
+        std::string e;
+        for (char ch : expr)
+            if (ch != ' ') e += ch;
+        if (e == "0") { m_kind = 2; return; }
+        const std::string u2 = "sin(pi*x)*sin(pi*y)", u3 = "sin(pi*x)*sin(pi*y)*sin(pi*z)";
+        if ((dim == 2 && e == "2*pi^2*" + u2) || (dim == 3 && e == "3*pi^2*" + u3)) { m_kind = 1; return; }
+        if ((dim == 2 && e == u2) || (dim == 3 && e == u3)) { m_kind = 3; return; }
+        throw std::runtime_error("gsFunctionExpr: only 0, the manufactured solution prod sin(pi x_i) and its "
+                                 "source d*pi^2*prod sin(pi x_i) are evaluated on the device, got '" + expr + "'");
+    }
+    int kind() const { return m_kind; }
+    short_t domainDim() const { return m_dim; }
+
+private:
+    int m_kind = 2;   // 1 source d pi^2 prod sin, 2 zero, 3 solution prod sin
+    short_t m_dim = 2;
+};
+
+/// Piecewise constant per element (patch-wise arrays, lexicographic);
+/// used for the diffusion coefficient and random sources.
+template <class T = real_t>
+class gsPiecewiseElementFunction {
+public:
+    gsPiecewiseElementFunction() = default;
+    explicit gsPiecewiseElementFunction(std::vector<std::vector<real_t>> perPatch) : m_v(std::move(perPatch)) {}
+    const std::vector<real_t> &patch(int k) const { return m_v.at(k); }
+    bool empty() const { return m_v.empty(); }
+
+private:
+    std::vector<std::vector<real_t>> m_v;
+};
+
+/// Boundary conditions: homogeneous Dirichlet sides (others natural).
+template <class T = real_t>
+class gsBoundaryConditions {
+public:
+    void addCondition(int patch, boundary::side s, condition_type::type t, const gsFunctionExpr<T> *g = nullptr,
+                      short_t = 0, bool = false, int = -1)
+    {
+        if (t != condition_type::dirichlet)
+            throw std::runtime_error("gsBoundaryConditions: only Dirichlet conditions are supported");
+        if (g && g->kind() != 2)
+            throw std::runtime_error("gsBoundaryConditions: only homogeneous Dirichlet values are supported");
+        m_dir.push_back({patch, (int)s});
+    }
+    void addCondition(boundary::side s, condition_type::type t, const gsFunctionExpr<T> *g = nullptr)
+    {
+        addCondition(0, s, t, g);
+    }
+    const std::vector<std::pair<int, int>> &dirichletSides() const { return m_dir; }
+
+private:
+    std::vector<std::pair<int, int>> m_dir;
+};
+
+// ---------------------------------------------------------------- device problem
+namespace detail {
+
+/// Geometry of patch k expressed in the discretisation basis of mb.
+inline std::vector<double> isoCoefs(const gsTensorBSplinePatch &P, const gsMultiBasis<real_t> &mb, int k)
+{
+    const int d = P.domainDim();
+    bool same = true;
+    for (int j = 0; j < d; ++j)
+        same = same && P.knots(j).knots() == mb.knots(k, j).knots() && P.knots(j).degree() == mb.knots(k, j).degree();
+    const index_t nb = mb.size(k);
+    std::vector<double> out((size_t)nb * d);
+    if (same) {
+        for (index_t i = 0; i < nb; ++i)
+            for (int r = 0; r < d; ++r) out[(size_t)i * d + r] = P.coefs()(i, r);
+        return out;
+    }
+    if (!P.isMultilinearBox())
+        throw std::runtime_error("gismo (libgsx): the geometry must use the discretisation basis or be a "
+                                 "multilinear single-element patch");
+    // multilinear maps are reproduced exactly by their values at the
+    // Greville abscissae of the finer/elevated basis
+    std::vector<std::vector<double>> gr(d);
+    int nbd[3] = {1, 1, 1};
+    for (int j = 0; j < d; ++j) {
+        const gsKnotVector &U = mb.knots(k, j);
+        const int p = U.degree();
+        nbd[j] = U.numBasis();
+        for (int i = 0; i < nbd[j]; ++i) {
+            double s = 0;
+            for (int m = 1; m <= p; ++m) s += U[i + m];
+            gr[j].push_back(s / p);
+        }
+    }
+    for (index_t i = 0; i < nb; ++i) {
+        const int ix = i % nbd[0], iy = (i / nbd[0]) % nbd[1], iz = d == 3 ? i / (nbd[0] * nbd[1]) : 0;
+        double xi[3] = {gr[0][ix], gr[1][iy], d == 3 ? gr[2][iz] : 0.0}, x[3];
+        P.evalMultilinear(xi, x);
+        for (int r = 0; r < d; ++r) out[(size_t)i * d + r] = x[r];
+    }
+    return out;
+}
+
+struct ProblemDeleter {
+    void operator()(gsx_problem *p) const { if (p) gsx_problem_destroy(p); }
+};
+struct MatrixDeleter {
+    void operator()(gsx_matrix *p) const { if (p) gsx_matrix_destroy(p); }
+};
+
+inline std::shared_ptr<gsx_problem> makeProblem(const gsMultiPatch<real_t> &mp, const gsMultiBasis<real_t> &mb,
+                                                const gsBoundaryConditions<real_t> &bc, int fkind,
+                                                const gsPiecewiseElementFunction<real_t> *kappa,
+                                                const gsPiecewiseElementFunction<real_t> *felem)
+{
+    const int np = mp.nPatches();
+    if (np == 0 || mb.nBases() != np) throw std::runtime_error("gismo (libgsx): patches and bases do not match");
+    std::vector<gsx_patch_desc> desc(np);
+    std::vector<std::vector<double>> ctrl(np), zeros(np);
+    for (int k = 0; k < np; ++k) {
+        gsx_patch_desc &D = desc[k];
+        std::memset(&D, 0, sizeof(D));
+        D.d = mp.patch(k).domainDim();
+        for (int j = 0; j < D.d; ++j) {
+            D.p[j] = mb.knots(k, j).degree();
+            D.nknots[j] = (int32_t)mb.knots(k, j).size();
+            D.knots_host[j] = mb.knots(k, j).knots().data();
+        }
+        ctrl[k] = isoCoefs(mp.patch(k), mb, k);
+        D.ctrl_host = ctrl[k].data();
+        D.kappa_host = (kappa && !kappa->empty()) ? kappa->patch(k).data() : nullptr;
+        D.felem_host = (felem && !felem->empty()) ? felem->patch(k).data() : nullptr;
+        D.fkind = felem && !felem->empty() ? 0 : fkind;
+        if (D.fkind == 2) {   // zero source: per-element zeros
+            index_t ne = 1;
+            for (int j = 0; j < D.d; ++j) ne *= (index_t)mb.knots(k, j).size() - 2 * D.p[j] - 1;
+            zeros[k].assign(ne, 0.0);
+            D.felem_host = zeros[k].data();
+            D.fkind = 0;
+        }
+    }
+    std::vector<int32_t> ds, ifs;
+    for (auto &s : bc.dirichletSides()) { ds.push_back(s.first); ds.push_back(s.second); }
+    for (auto &f : mp.interfaces()) {
+        const int32_t r[8] = {f.p1, f.s1, f.p2, f.s2, f.m0, f.f0, f.m1, f.f1};
+        ifs.insert(ifs.end(), r, r + 8);
+    }
+    gsx_problem *p = nullptr;
+    check(gsx_problem_create(ctx(), np, desc.data(), (int32_t)ds.size() / 2, ds.empty() ? nullptr : ds.data(),
+                             (int32_t)ifs.size() / 8, ifs.empty() ? nullptr : ifs.data(), &p),
+          "gsx_problem_create");
+    return std::shared_ptr<gsx_problem>(p, ProblemDeleter());
+}
+
+/// Device assembly -> host gsSparseMatrix (the CSR arrays of a bitwise
+/// symmetric matrix are its CSC arrays) + rhs.
+inline void assembleInto(const std::shared_ptr<gsx_problem> &prob, int mode, int precision,
+                         gsSparseMatrix<real_t> &M, gsMatrix<real_t> &rhs)
+{
+    int64_t nbasis = 0, nfree = 0, nelem = 0, nglued = 0;
+    check(gsx_problem_info(prob.get(), &nbasis, &nfree, &nelem, &nglued), "gsx_problem_info");
+    DeviceVec b(sizeof(double) * std::max<int64_t>(1, nfree));
+    gsx_matrix *A = nullptr;
+    check(gsx_assemble(prob.get(), mode, precision, &A, b.d()), "gsx_assemble");
+    std::shared_ptr<gsx_matrix> dev(A, MatrixDeleter());
+    int64_t n = 0, nnz = 0;
+    check(gsx_matrix_info(A, &n, &nnz), "gsx_matrix_info");
+    std::vector<index_t> outer(n + 1), inner(nnz);
+    std::vector<real_t> val(nnz);
+    check(gsx_matrix_download(A, outer.data(), inner.data(), val.data()), "gsx_matrix_download");
+    M.setFromCompressed((index_t)n, (index_t)n, std::move(outer), std::move(inner), std::move(val));
+    M.setDevice(dev);
+    M.setProblem(prob);
+    rhs.resize((index_t)nfree, 1);
+    check(gsx_copy_to_host(ctx(), rhs.data(), b.d(), sizeof(double) * nfree), "download rhs");
+}
+
+}  // namespace detail
+
+// ---------------------------------------------------------------- expressions
+namespace expr {
+struct geometryMap { const gsMultiPatch<real_t> *mp = nullptr; };
+struct gsFeSpace;
+struct igrad_expr { bool tr = false; igrad_expr tr_() const { igrad_expr e; e.tr = true; return e; } };
+struct meas_expr {};
+struct coeff_expr { int kind = 2; };
+struct kappa_expr { const gsPiecewiseElementFunction<real_t> *k = nullptr; };
+struct gg_expr {};
+struct bilinear_form { const gsPiecewiseElementFunction<real_t> *kappa = nullptr; };
+struct uf_expr { int kind = 2; };
+struct linear_form { int kind = 2; const gsPiecewiseElementFunction<real_t> *felem = nullptr; };
+
+struct gsFeSpace {
+    const gsMultiBasis<real_t> *mb = nullptr;
+    // shared by the copies handed out by getSpace, so that setup() on any
+    // of them reaches the assembler
+    std::shared_ptr<const gsBoundaryConditions<real_t> *> bc = std::make_shared<const gsBoundaryConditions<real_t> *>(nullptr);
+    /// gsFeSpace::setup (src/gsAssembler/gsExpressions.h:1091-1351):
+    /// interfaces glued (icont = 0), Dirichlet sides eliminated
+    void setup(const gsBoundaryConditions<real_t> &b, int = dirichlet::interpolation, int = 0) const { *bc = &b; }
+};
+/// exact solution u = prod sin(pi x_i) (gsExprEvaluator::getVariable)
+struct exact_var { int kind = 3; };
+/// discrete solution (gsExprAssembler::getSolution): the free coefficients
+struct solution_var { const gsMatrix<real_t> *coefs = nullptr; };
+struct diff_expr {
+    int kind = 3;
+    const gsMatrix<real_t> *coefs = nullptr;
+    diff_expr sqNorm() const { return *this; }
+};
+struct l2_integrand { diff_expr d; };
+inline diff_expr operator-(const exact_var &e, const solution_var &s) { diff_expr d; d.kind = e.kind; d.coefs = s.coefs; return d; }
+inline diff_expr operator-(const solution_var &s, const exact_var &e) { return e - s; }
+inline l2_integrand operator*(const diff_expr &d, const meas_expr &) { l2_integrand i; i.d = d; return i; }
+}  // namespace expr
+
+typedef expr::geometryMap geometryMap;
+typedef expr::gsFeSpace space;
+
+struct igrad_t {
+    expr::igrad_expr e;
+    igrad_t tr() const { igrad_t t; t.e.tr = true; return t; }
+};
+inline igrad_t igrad(const space &, const geometryMap &) { return igrad_t(); }
+inline expr::meas_expr meas(const geometryMap &) { return expr::meas_expr(); }
+inline expr::gg_expr operator*(const igrad_t &a, const igrad_t &b)
+{
+    if (a.e.tr || !b.e.tr) throw std::runtime_error("gsExprAssembler: expected igrad(u,G) * igrad(u,G).tr()");
+    return expr::gg_expr();
+}
+inline expr::bilinear_form operator*(const expr::gg_expr &, const expr::meas_expr &) { return expr::bilinear_form(); }
+inline expr::bilinear_form operator*(const expr::bilinear_form &f, const expr::kappa_expr &k)
+{
+    expr::bilinear_form g = f;
+    g.kappa = k.k;
+    return g;
+}
+inline expr::bilinear_form operator*(const expr::kappa_expr &k, const expr::bilinear_form &f) { return f * k; }
+inline expr::uf_expr operator*(const space &, const expr::coeff_expr &c) { expr::uf_expr u; u.kind = c.kind; return u; }
+inline expr::linear_form operator*(const expr::uf_expr &u, const expr::meas_expr &)
+{
+    expr::linear_form l;
+    l.kind = u.kind;
+    return l;
+}
+inline expr::linear_form operator*(const space &, const expr::kappa_expr &k)
+{
+    expr::linear_form l;
+    l.felem = k.k;
+    return l;
+}
+inline expr::linear_form operator*(const expr::linear_form &l, const expr::meas_expr &) { return l; }
+
+// ---------------------------------------------------------------- assemblers
+/// gsExprAssembler (src/gsAssembler/gsExprAssembler.h:29-833) for the
+/// Poisson/diffusion forms above.  Options: "Precision" ("fp64" default or
+/// "mixed": keeps an FP32 copy of the operator for mixed solves).
+template <class T = real_t>
+class gsExprAssembler {
+public:
+    typedef gismo::geometryMap geometryMap;
+    typedef gismo::space space;
+    typedef expr::solution_var solution;
+    typedef expr::exact_var variable;
+
+    explicit gsExprAssembler(index_t = 1, index_t = 1) {}
+    void setIntegrationElements(const gsMultiBasis<T> &mb) { m_mb = &mb; }
+    void setIntegrationElements(const gsMultiBasis<T> &&) = delete;
+    geometryMap getMap(const gsMultiPatch<T> &mp) { m_mp = &mp; geometryMap G; G.mp = &mp; return G; }
+    space getSpace(const gsMultiBasis<T> &mb, index_t = 1, index_t = 0) { m_u.mb = &mb; return m_u; }
+    solution getSolution(const space &, gsMatrix<T> &coefs) const { solution s; s.coefs = &coefs; return s; }
+    expr::coeff_expr getCoeff(const gsFunctionExpr<T> &f, const geometryMap &) { expr::coeff_expr c; c.kind = f.kind(); return c; }
+    expr::coeff_expr getCoeff(const gsFunctionExpr<T> &f) { expr::coeff_expr c; c.kind = f.kind(); return c; }
+    expr::kappa_expr getCoeff(const gsPiecewiseElementFunction<T> &k) { expr::kappa_expr e; e.k = &k; return e; }
+    gsOptionList &options() { return m_opt; }
+    void setOptions(const gsOptionList &o) { m_opt = o; }
+    void initSystem(index_t = 1) {}
+    index_t numDofs() const { return m_matrix.rows(); }
+    void assemble(const expr::bilinear_form &a, const expr::linear_form &l)
+    {
+        if (!m_mb || !m_mp) throw std::runtime_error("gsExprAssembler: call setIntegrationElements and getMap first");
+        if (!l.felem && l.kind != 1 && l.kind != 2)
+            throw std::runtime_error("gsExprAssembler: unsupported source function");
+        const gsBoundaryConditions<T> *bc = *m_u.bc ? *m_u.bc : m_bc;
+        static const gsBoundaryConditions<T> none;
+        m_prob = detail::makeProblem(*m_mp, *m_mb, bc ? *bc : none, l.kind, a.kappa, l.felem);
+        const int prec = m_opt.askString("Precision", "fp64") == "mixed" ? GSX_MIXED : GSX_FP64;
+        detail::assembleInto(m_prob, GSX_PATTERN_EXPR, prec, m_matrix, m_rhs);
+    }
+    const gsSparseMatrix<T> &matrix() const { return m_matrix; }
+    void matrix_into(gsSparseMatrix<T> &out) const { out = m_matrix; }
+    gsSparseMatrix<T> giveMatrix() { return std::move(m_matrix); }
+    const gsMatrix<T> &rhs() const { return m_rhs; }
+    std::shared_ptr<gsx_problem> problem() const { return m_prob; }
+    /// conditions used when the space's setup() was not called
+    void setBoundaryConditions(const gsBoundaryConditions<T> &bc) { m_bc = &bc; }
+
+private:
+    const gsMultiBasis<T> *m_mb = nullptr;
+    const gsMultiPatch<T> *m_mp = nullptr;
+    const gsBoundaryConditions<T> *m_bc = nullptr;
+    space m_u;
+    gsOptionList m_opt;
+    std::shared_ptr<gsx_problem> m_prob;
+    gsSparseMatrix<T> m_matrix;
+    gsMatrix<T> m_rhs;
+};
+
+/// gsPoissonAssembler (src/gsAssembler/gsPoissonAssembler.h:35-142):
+/// every free-free pair is in the pattern (gsSparseSystem::push).
+template <class T = real_t>
+class gsPoissonAssembler {
+public:
+    gsPoissonAssembler(const gsMultiPatch<T> &mp, const gsMultiBasis<T> &mb, const gsBoundaryConditions<T> &bc,
+                       const gsFunctionExpr<T> &f, dirichlet::strategy ds = dirichlet::elimination,
+                       iFace::strategy = iFace::glue)
+        : m_mp(&mp), m_mb(&mb), m_bc(&bc), m_f(f)
+    {
+        if (ds != dirichlet::elimination && ds != dirichlet::homogeneous)
+            throw std::runtime_error("gsPoissonAssembler: only Dirichlet elimination is supported");
+    }
+    gsOptionList &options() { return m_opt; }
+    virtual ~gsPoissonAssembler() = default;
+    virtual void assemble()
+    {
+        m_prob = detail::makeProblem(*m_mp, *m_mb, *m_bc, m_f.kind(), nullptr, nullptr);
+        const int prec = m_opt.askString("Precision", "fp64") == "mixed" ? GSX_MIXED : GSX_FP64;
+        detail::assembleInto(m_prob, GSX_PATTERN_FULL, prec, m_matrix, m_rhs);
+    }
+    const gsSparseMatrix<T> &matrix() const { return m_matrix; }
+    const gsMatrix<T> &rhs() const { return m_rhs; }
+    index_t numDofs() const { return m_matrix.rows(); }
+    std::shared_ptr<gsx_problem> problem() const { return m_prob; }
+
+private:
+    const gsMultiPatch<T> *m_mp;
+    const gsMultiBasis<T> *m_mb;
+    const gsBoundaryConditions<T> *m_bc;
+    gsFunctionExpr<T> m_f;
+    gsOptionList m_opt;
+    std::shared_ptr<gsx_problem> m_prob;
+    gsSparseMatrix<T> m_matrix;
+    gsMatrix<T> m_rhs;
+};
+
+// ---------------------------------------------------------------- operators / solvers
+/// gsLinearOperator (src/gsSolver/gsLinearOperator.h:28-70)
+template <class T = real_t>
+class gsLinearOperator {
+public:
+    typedef std::shared_ptr<gsLinearOperator> Ptr;
+    virtual ~gsLinearOperator() = default;
+    virtual void apply(const gsMatrix<T> &input, gsMatrix<T> &x) const = 0;
+    virtual index_t rows() const = 0;
+    virtual index_t cols() const = 0;
+};
+
+namespace detail {
+inline std::shared_ptr<gsx_matrix> deviceMatrix(const gsSparseMatrix<real_t> &M, bool fp32)
+{
+    if (M.devicePtr()) return M.devicePtr();
+    // column-major arrays -> row-major arrays
+    const index_t n = M.rows();
+    if (M.cols() != n) throw std::runtime_error("gismo (libgsx): square matrices only");
+    std::vector<int32_t> rp(n + 1, 0), ci(M.nonZeros());
+    std::vector<double> v(M.nonZeros());
+    for (index_t k = 0; k < M.nonZeros(); ++k) rp[M.innerIndexPtr()[k] + 1]++;
+    for (index_t i = 0; i < n; ++i) rp[i + 1] += rp[i];
+    std::vector<int32_t> pos(rp.begin(), rp.end() - 1);
+    for (index_t j = 0; j < n; ++j)
+        for (index_t k = M.outerIndexPtr()[j]; k < M.outerIndexPtr()[j + 1]; ++k) {
+            const int r = M.innerIndexPtr()[k];
+            ci[pos[r]] = j;
+            v[pos[r]++] = M.valuePtr()[k];
+        }
+    gsx_matrix *A = nullptr;
+    check(gsx_matrix_from_csr(ctx(), n, rp.data(), ci.data(), v.data(), fp32 ? 1 : 0, &A), "gsx_matrix_from_csr");
+    return std::shared_ptr<gsx_matrix>(A, MatrixDeleter());
+}
+}  // namespace detail
+
+/// gsMatrixOp (src/gsSolver/gsMatrixOp.h:64-75): y = A x on the device
+template <class T = real_t>
+class gsMatrixOp : public gsLinearOperator<T> {
+public:
+    explicit gsMatrixOp(const gsSparseMatrix<T> &M) : m_n(M.rows()), m_dev(detail::deviceMatrix(M, false)) {}
+    void apply(const gsMatrix<T> &input, gsMatrix<T> &x) const override
+    {
+        detail::DeviceVec a(sizeof(double) * m_n), b(sizeof(double) * m_n);
+        detail::check(gsx_copy_to_device(detail::ctx(), a.p, input.data(), sizeof(double) * m_n), "upload");
+        detail::check(gsx_spmv(m_dev.get(), a.d(), b.d()), "gsx_spmv");
+        x.resize(m_n, 1);
+        detail::check(gsx_copy_to_host(detail::ctx(), x.data(), b.p, sizeof(double) * m_n), "download");
+    }
+    index_t rows() const override { return m_n; }
+    index_t cols() const override { return m_n; }
+
+private:
+    index_t m_n;
+    std::shared_ptr<gsx_matrix> m_dev;
+};
+
+template <class T>
+typename gsLinearOperator<T>::Ptr makeMatrixOp(const gsSparseMatrix<T> &M)
+{
+    return std::make_shared<gsMatrixOp<T>>(M);
+}
+
+/// gsPreconditionerOp base (src/gsSolver/gsPreconditioner.h:42-130)
+template <class T = real_t>
+class gsPreconditionerOp : public gsLinearOperator<T> {
+public:
+    typedef std::shared_ptr<gsPreconditionerOp> Ptr;
+    virtual int kind() const = 0;
+};
+
+/// gsJacobiOp (src/gsSolver/gsSimplePreconditioners.h:146-244)
+template <class T = real_t>
+class gsJacobiOp : public gsPreconditionerOp<T> {
+public:
+    explicit gsJacobiOp(const gsSparseMatrix<T> &M, T tau = 1) : m_M(&M), m_tau(tau) {}
+    void apply(const gsMatrix<T> &input, gsMatrix<T> &x) const override
+    {
+        x.resize(input.rows(), 1);
+        for (index_t i = 0; i < input.rows(); ++i) x(i) = m_tau * input(i) / m_M->coeff(i, i);
+    }
+    index_t rows() const override { return m_M->rows(); }
+    index_t cols() const override { return m_M->cols(); }
+    int kind() const override { return GSX_PREC_JACOBI; }
+    void setDamping(T tau) { m_tau = tau; }
+    T damping() const { return m_tau; }
+    void setOptions(const gsOptionList &o) { m_tau = o.askReal("Damping", m_tau); }
+
+private:
+    const gsSparseMatrix<T> *m_M;
+    T m_tau;
+};
+
+template <class T>
+std::shared_ptr<gsJacobiOp<T>> makeJacobiOp(const gsSparseMatrix<T> &M, T tau = 1)
+{
+    return std::make_shared<gsJacobiOp<T>>(M, tau);
+}
+
+/// gsMultiGridOp (src/gsMultiGrid/gsMultiGrid.h): built here from the
+/// assembled problem by coarsening its knot vectors (the transfers and
+/// Galerkin products are formed on the device).  Options: NumPreSmooth,
+/// NumPostSmooth, CoarseGridCorrectionDamping, Damping (of the Jacobi
+/// smoother), Precision ("fp64" or "mixed": FP32 cycle).
+template <class T = real_t>
+class gsMultiGridOp : public gsPreconditionerOp<T> {
+public:
+    typedef std::shared_ptr<gsMultiGridOp> Ptr;
+    static Ptr make(const gsSparseMatrix<T> &fine, index_t numLevels, const gsOptionList &opt = gsOptionList())
+    {
+        Ptr p(new gsMultiGridOp());
+        p->m_n = fine.rows();
+        p->m_A = fine.devicePtr();
+        p->m_prob = fine.problemPtr();
+        if (!p->m_A || !p->m_prob)
+            throw std::runtime_error("gsMultiGridOp: the matrix must come from gsExprAssembler/gsPoissonAssembler");
+        p->m_levels = numLevels;
+        p->m_opt = opt;
+        return p;
+    }
+    void setOptions(const gsOptionList &o) { m_opt = o; m_mg.reset(); }
+    void apply(const gsMatrix<T> &input, gsMatrix<T> &x) const override
+    {
+        detail::DeviceVec a(sizeof(double) * m_n), b(sizeof(double) * m_n);
+        detail::check(gsx_copy_to_device(detail::ctx(), a.p, input.data(), sizeof(double) * m_n), "upload");
+        detail::check(gsx_mg_apply(device(), a.d(), b.d()), "gsx_mg_apply");
+        x.resize(m_n, 1);
+        detail::check(gsx_copy_to_host(detail::ctx(), x.data(), b.p, sizeof(double) * m_n), "download");
+    }
+    index_t rows() const override { return m_n; }
+    index_t cols() const override { return m_n; }
+    int kind() const override { return GSX_PREC_MG; }
+    gsx_mg *device() const
+    {
+        if (!m_mg) {
+            gsx_mg *h = nullptr;
+            const int prec = m_opt.askString("Precision", "fp64") == "mixed" ? GSX_MIXED : GSX_FP64;
+            detail::check(gsx_mg_create(m_prob.get(), m_A.get(), m_levels, m_opt.askInt("NumPreSmooth", 1),
+                                        m_opt.askInt("NumPostSmooth", 1), 1, m_opt.askReal("Damping", 1.0),
+                                        m_opt.askReal("CoarseGridCorrectionDamping", 1.0), prec, &h),
+                          "gsx_mg_create");
+            m_mg.reset(h, [](gsx_mg *q) { gsx_mg_destroy(q); });
+        }
+        return m_mg.get();
+    }
+
+private:
+    index_t m_n = 0;
+    index_t m_levels = 1;
+    gsOptionList m_opt;
+    std::shared_ptr<gsx_matrix> m_A;
+    std::shared_ptr<gsx_problem> m_prob;
+    mutable std::shared_ptr<gsx_mg> m_mg;
+};
+
+/// gsIterativeSolver / gsConjugateGradient (src/gsSolver/gsIterativeSolver.h:
+/// 20-230, src/gsSolver/gsConjugateGradient.hpp:20-93).  Options:
+/// MaxIterations (1000), Tolerance (1e-10), Precision ("fp64" or "mixed").
+template <class T = real_t>
+class gsIterativeSolver {
+public:
+    gsIterativeSolver(const gsSparseMatrix<T> &A, typename gsPreconditionerOp<T>::Ptr precond = nullptr)
+        : m_A(&A), m_prec(std::move(precond)) {}
+    virtual ~gsIterativeSolver() = default;
+    gsIterativeSolver &setOptions(const gsOptionList &o)
+    {
+        m_max = o.askInt("MaxIterations", m_max);
+        m_tol = o.askReal("Tolerance", m_tol);
+        m_precision = o.askString("Precision", m_precision);
+        return *this;
+    }
+    gsIterativeSolver &setTolerance(T t) { m_tol = t; return *this; }
+    gsIterativeSolver &setMaxIterations(index_t m) { m_max = m; return *this; }
+    T tolerance() const { return m_tol; }
+    index_t maxIterations() const { return m_max; }
+    index_t iterations() const { return m_it; }
+    T error() const { return m_err; }
+    void solve(const gsMatrix<T> &rhs, gsMatrix<T> &x) { run(rhs, x, nullptr); }
+    void solveDetailed(const gsMatrix<T> &rhs, gsMatrix<T> &x, gsMatrix<T> &hist) { run(rhs, x, &hist); }
+
+protected:
+    void run(const gsMatrix<T> &rhs, gsMatrix<T> &x, gsMatrix<T> *hist)
+    {
+        const index_t n = m_A->rows();
+        const bool mixed = m_precision == "mixed";
+        std::shared_ptr<gsx_matrix> dev = detail::deviceMatrix(*m_A, mixed);
+        if (x.size() == 0) x.setZero(n, 1);
+        detail::DeviceVec b(sizeof(double) * n), xd(sizeof(double) * n);
+        detail::check(gsx_copy_to_device(detail::ctx(), b.p, rhs.data(), sizeof(double) * n), "upload rhs");
+        detail::check(gsx_copy_to_device(detail::ctx(), xd.p, x.data(), sizeof(double) * n), "upload x");
+        int32_t it = 0;
+        double rel = 0;
+        int kind = m_prec ? m_prec->kind() : GSX_PREC_NONE;
+        double tau = 1.0;
+        gsx_mg *mg = nullptr;
+        if (kind == GSX_PREC_JACOBI) tau = static_cast<gsJacobiOp<T> *>(m_prec.get())->damping();
+        if (kind == GSX_PREC_MG) mg = static_cast<gsMultiGridOp<T> *>(m_prec.get())->device();
+        const gsx_status st = gsx_cg_solve(dev.get(), kind, tau, mg, mixed ? GSX_MIXED : GSX_FP64, b.d(), xd.d(),
+                                           m_tol, m_max, &it, &rel);
+        if (st != GSX_OK && st != GSX_ERR_NOT_CONVERGED) detail::check(st, "gsx_cg_solve");
+        m_it = it;
+        m_err = rel;
+        detail::check(gsx_copy_to_host(detail::ctx(), x.data(), xd.p, sizeof(double) * n), "download x");
+        if (hist) {
+            int32_t cnt = 0;
+            gsx_cg_history(dev.get(), nullptr, 0, &cnt);
+            hist->resize(cnt, 1);
+            gsx_cg_history(dev.get(), hist->data(), cnt, &cnt);
+        }
+    }
+    const gsSparseMatrix<T> *m_A;
+    typename gsPreconditionerOp<T>::Ptr m_prec;
+    index_t m_max = 1000;
+    T m_tol = 1e-10;
+    std::string m_precision = "fp64";
+    index_t m_it = 0;
+    T m_err = 0;
+};
+
+template <class T = real_t>
+class gsConjugateGradient : public gsIterativeSolver<T> {
+public:
+    gsConjugateGradient(const gsSparseMatrix<T> &A, typename gsPreconditionerOp<T>::Ptr precond = nullptr)
+        : gsIterativeSolver<T>(A, std::move(precond)) {}
+    static gsOptionList defaultOptions()
+    {
+        gsOptionList o;
+        o.addInt("MaxIterations", "", 1000);
+        o.addReal("Tolerance", "", 1e-10);
+        o.addString("Precision", "", "fp64");
+        return o;
+    }
+};
+
+// ---------------------------------------------------------------- evaluation
+/// gsExprEvaluator (src/gsAssembler/gsExprEvaluator.h:38,382-450) for the
+/// L2 error of a discrete solution against u = prod sin(pi x_i):
+///   math::sqrt(ev.integral((u_ex - u_sol).sqNorm() * meas(G)))
+template <class T = real_t>
+class gsExprEvaluator {
+public:
+    explicit gsExprEvaluator(const gsExprAssembler<T> &A) : m_ea(&A) {}
+    explicit gsExprEvaluator(const gsPoissonAssembler<T> &A) : m_pa(&A) {}
+    expr::exact_var getVariable(const gsFunctionExpr<T> &f, const geometryMap & = geometryMap()) const
+    {
+        if (f.kind() != 3)
+            throw std::runtime_error("gsExprEvaluator: only the manufactured solution prod sin(pi x_i) is evaluated");
+        return expr::exact_var();
+    }
+    T integral(const expr::l2_integrand &e) const
+    {
+        const T err = l2ErrorSinProduct(*e.d.coefs);
+        return err * err;
+    }
+    T l2ErrorSinProduct(const gsMatrix<T> &sol) const
+    {
+        std::shared_ptr<gsx_problem> prob = m_ea ? m_ea->problem() : m_pa->problem();
+        if (!prob) throw std::runtime_error("gsExprEvaluator: nothing assembled yet");
+        int64_t nbasis = 0, nfree = 0, nelem = 0, nglued = 0;
+        detail::check(gsx_problem_info(prob.get(), &nbasis, &nfree, &nelem, &nglued), "gsx_problem_info");
+        if (sol.rows() != nfree) throw std::runtime_error("gsExprEvaluator: solution size does not match the space");
+        detail::DeviceVec s(sizeof(double) * std::max<int64_t>(1, nfree));
+        detail::check(gsx_copy_to_device(detail::ctx(), s.p, sol.data(), sizeof(double) * nfree), "upload");
+        double e = 0;
+        detail::check(gsx_l2_error(prob.get(), s.d(), &e), "gsx_l2_error");
+        return e;
+    }
+
+private:
+    const gsExprAssembler<T> *m_ea = nullptr;
+    const gsPoissonAssembler<T> *m_pa = nullptr;
+};
+
+namespace math {
+using std::sqrt;
+using std::log;
+using std::abs;
+}
+
+/// gsSparseSolver<T>::CGDiagonal: Eigen's diagonally preconditioned CG
+/// (tolerance machine epsilon, at most 2n iterations) on the device.
+template <class T = real_t>
+struct gsSparseSolver {
+    class CGDiagonal {
+    public:
+        CGDiagonal &compute(const gsSparseMatrix<T> &A)
+        {
+            m_A = A;
+            m_prec = makeJacobiOp(m_A, T(1));
+            return *this;
+        }
+        gsMatrix<T> solve(const gsMatrix<T> &b)
+        {
+            gsConjugateGradient<T> cg(m_A, m_prec);
+            cg.setTolerance(m_tol).setMaxIterations(m_max > 0 ? m_max : 2 * m_A.rows());
+            gsMatrix<T> x;
+            x.setZero(b.rows(), 1);
+            cg.solve(b, x);
+            m_it = cg.iterations();
+            m_err = cg.error();
+            return x;
+        }
+        void setTolerance(T t) { m_tol = t; }
+        void setMaxIterations(index_t m) { m_max = m; }
+        index_t iterations() const { return m_it; }
+        T error() const { return m_err; }
+
+    private:
+        gsSparseMatrix<T> m_A;
+        std::shared_ptr<gsJacobiOp<T>> m_prec;
+        T m_tol = std::numeric_limits<T>::epsilon();
+        index_t m_max = 0, m_it = 0;
+        T m_err = 0;
+    };
+};
+
+}  // namespace gismo

@@ -143,6 +143,15 @@ gsx_status gsx_assemble(gsx_problem *prob, int32_t pattern_mode,
                         int32_t precision, gsx_matrix **A_out,
                         double *rhs_dev);
 gsx_status gsx_matrix_destroy(gsx_matrix *A);
+/* Device matrix from host CSR arrays (any square sparse matrix, e.g. one a
+ * caller assembled itself): what gsMatrixOp wraps (src/gsSolver/
+ * gsMatrixOp.h:64-75).  with_fp32 keeps an FP32 copy for mixed solves. */
+gsx_status gsx_matrix_from_csr(gsx_ctx *ctx, int32_t n, const int32_t *rowptr_host,
+                               const int32_t *col_host, const double *val_host,
+                               int32_t with_fp32, gsx_matrix **out);
+/* Return the context's cached device blocks to the driver.  No reference
+ * counterpart. */
+gsx_status gsx_ctx_release_cache(gsx_ctx *ctx);
 gsx_status gsx_matrix_info(const gsx_matrix *A, int64_t *n, int64_t *nnz);
 /* Host copies of the CSR arrays (n+1, nnz, nnz). */
 gsx_status gsx_matrix_download(const gsx_matrix *A, int32_t *rowptr_host,

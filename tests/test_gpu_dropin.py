@@ -1,0 +1,109 @@
+"""The C++ drop-in example drivers (gismo_amd/host/*_example.cpp, built by
+build()) on the GPU, against the oracle and the Python mirror on the same
+problems: per-level DOF counts, pattern sizes and CG iteration counts of
+the poisson2_example call sequence (gsExprAssembler and gsPoissonAssembler
+paths), the L2 convergence order, and the multigrid-preconditioned CG of the
+multiGrid_example call sequence."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from gismo_amd import gsx, problems
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "gismo_amd", "host", "bin")
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(name, *args):
+    exe = os.path.join(BIN, name)
+    if not os.path.exists(exe):
+        pytest.fail(f"{exe} missing: run __graft_entry__.build()")
+    print(f"running {name} {' '.join(args)}", flush=True)
+    out = subprocess.run([exe, *args], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    return out.stdout
+
+
+def _levels(text):
+    rows = []
+    for m in re.finditer(r"level (\d+) dofs (\d+) nnz (\d+) iterations (\d+) relres (\S+) L2 (\S+)", text):
+        rows.append((int(m[2]), int(m[3]), int(m[4]), float(m[5]), float(m[6])))
+    return rows
+
+
+def test_poisson2_example_square_p2():
+    rows = _levels(_run("poisson2_example", "-r", "3", "-e", "1"))
+    assert len(rows) == 4
+    for lvl, (dofs, nnz, its, rel, l2) in enumerate(rows):
+        n = 2 ** (lvl + 1)
+        prob = problems.poisson_square(n=n, p=2)
+        dmap, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+        Ao, bo = O.assemble(prob.patches, dmap, nfree, 0)
+        _, ito, _ = O.cg_solve(Ao, bo, tol=1e-10, maxit=10000)
+        assert (dofs, nnz) == (nfree, Ao.nnz)
+        assert its == ito, (n, its, ito)
+        assert rel < 1e-10
+    l2 = np.array([r[4] for r in rows])
+    rates = np.log2(l2[:-1] / l2[1:])
+    assert rates[-1] > 2.7, rates
+
+
+def test_poisson2_example_cube_full_pattern_mixed():
+    rows = _levels(_run("poisson2_example", "-d", "3", "-e", "2", "-r", "1", "--full", "--mixed"))
+    assert len(rows) == 2
+    for lvl, (dofs, nnz, its, rel, l2) in enumerate(rows):
+        n = 2 ** (lvl + 1)
+        prob = problems.box_patch(3, 3, n)
+        prob = problems.Problem([prob], problems.all_sides(3), [], name="cube")
+        prob.patches[0].fkind = 1
+        dmap, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+        Ao, bo = O.assemble(prob.patches, dmap, nfree, 1)
+        assert (dofs, nnz) == (nfree, Ao.nnz)
+        assert rel < 1e-10
+
+
+def test_poisson2_example_grid_matches_python_mirror(ctx):
+    rows = _levels(_run("poisson2_example", "--grid", "3", "2", "-e", "2", "-r", "2", "--last"))
+    assert len(rows) == 1
+    dofs, nnz, its, rel, l2 = rows[0]
+    prob = problems.square_grid(3, 2, n=8, p=3)
+    gp = gsx.Problem(ctx, prob)
+    A, b = gp.assemble(gsx.PATTERN_EXPR, gsx.FP64)
+    x = ctx.array(gp.nfree).zero()
+    it, rel_py, ok = A.cg_solve(b, x, tol=1e-10, maxit=100000)
+    assert (dofs, nnz, its) == (gp.nfree, A.nnz, it)
+    assert abs(l2 - gp.l2_error(x)) <= 1e-12 * l2
+    A.close()
+    gp.close()
+
+
+def test_multigrid_example_matches_python_mirror(ctx):
+    text = _run("multiGrid_example", "--grid", "2", "2", "-r", "3", "-p", "2", "-l", "3",
+                "--MG.Damping", "0.8", "-t", "1e-8")
+    m = re.search(r"dofs (\d+) nnz (\d+)", text)
+    s = re.search(r"solver cg iterations (\d+) relres (\S+)", text)
+    l2 = float(re.search(r"L2 (\S+)", text)[1])
+    prob = problems.square_grid(2, 2, n=8, p=2)
+    gp = gsx.Problem(ctx, prob)
+    A, b = gp.assemble(gsx.PATTERN_FULL, gsx.FP64)
+    mg = gsx.Multigrid(gp, A, 3, 1, 1, 0.8)
+    x = ctx.array(gp.nfree).zero()
+    it, rel, ok = gsx.mg_cg_solve(A, mg, b, x, tol=1e-8, maxit=100)
+    assert (int(m[1]), int(m[2])) == (gp.nfree, A.nnz)
+    assert int(s[1]) == it and ok
+    assert float(s[2]) < 1e-8
+    assert abs(l2 - gp.l2_error(x)) <= 1e-10 * l2
+    # the stationary multigrid iteration converges too
+    text = _run("multiGrid_example", "--grid", "2", "2", "-r", "3", "-p", "2", "-l", "3", "-i", "d",
+                "--MG.Damping", "0.8", "-t", "1e-8")
+    s = re.search(r"solver d iterations (\d+) relres (\S+)", text)
+    assert float(s[2]) < 1e-8 and int(s[1]) < 100
+    mg.close()
+    A.close()
+    gp.close()
