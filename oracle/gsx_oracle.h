@@ -11,21 +11,32 @@
  * the accumulation order equals the reference's single-threaded order
  * (patch by patch, elements lexicographic with direction 0 fastest).
  *
- * Pinning (see tests/test_oracle_*.py):
+ * Pinning (tests/test_oracle.py, tests/test_lib_cpu.py):
  *   oc_gauss_legendre   -- reference 30-digit tables (golden fixture), exactness
  *                          on monomials, numpy leggauss.
- *   oc_find_span,
+ *   oc_knots_uniform,
+ *   oc_find_span        -- span search cases of the reference's knot vector
+ *                          tests, uniform spacing by construction.
  *   oc_bspline_ders     -- brute-force Cox-de Boor in exact rationals,
- *                          partition of unity, sum of derivatives = 0,
- *                          finite differences.
+ *                          partition of unity, sum of derivatives = 0.
+ *   oc_element_geometry -- scipy BSpline evaluation of the same patch.
  *   oc_mapper_build     -- hand-counted small multipatch cases, closed-form
- *                          free-DOF counts of the glued grids.
- *   oc_assemble         -- Kronecker closed form of affine patches computed in
- *                          exact rationals, row sums, energy identity, symmetry.
- *   oc_cg_solve         -- 1D FD Poisson with known solution, iterate-by-iterate
- *                          brute force in numpy on tiny systems.
+ *                          free-DOF counts of the glued grids, glued p=1
+ *                          grid = single patch.
+ *   oc_assemble         -- Kronecker closed form of affine patches (matrix and
+ *                          load), isoparametric identities (constant in the
+ *                          kernel, energy identity, cross term, load sum),
+ *                          symmetry, EXPR pattern = FULL pattern.
+ *   oc_assemble_rows    -- rows of oc_assemble.
+ *   oc_csr_spmv         -- inside oc_cg_solve, pinned by its checks.
+ *   oc_cg_solve         -- 1D FD Poisson with known solution, residual
+ *                          history against a numpy CG on tiny systems.
+ *   oc_transfer_1d,
+ *   oc_transfer_multipatch -- refinement identity (coarse spline = fine
+ *                          spline at sample points), binomial subdivision masks.
  *   oc_mg_*             -- Galerkin identity P^T A_f P = A_c (nested spaces),
- *                          binomial subdivision masks, two-grid brute force.
+ *                          two-grid cycle against numpy, MG-PCG convergence.
+ *   oc_l2_error         -- convergence order p+1 on the manufactured solution.
  */
 #ifndef GSX_ORACLE_H
 #define GSX_ORACLE_H
