@@ -1,0 +1,483 @@
+// Batched preconditioned CG of libgsx (gfx950): many independent solves
+// advanced together, one launch per CG stage for all of them.
+//
+// Reference path (G+Smo source tree): every member runs the algorithm of
+//   gsConjugateGradient::initIteration/step  src/gsSolver/gsConjugateGradient.hpp:20-93
+//   gsIterativeSolver::solve (stopping rule)  src/gsSolver/gsIterativeSolver.h:114-200
+//   gsJacobiOp::apply                        src/gsSolver/gsSimplePreconditioners.h:178-202
+// The reference has no batching: it runs the solves one after another.
+//
+// Layout.  A member ("segment") is one CG solve: rows [row0, row0 + nrows)
+// of a CSR matrix and its own vectors (base pointers indexed by row and by
+// column).  Its rows are cut into chunks of (256 / lpr) * 64 rows, one
+// thread block per chunk, and chunks never straddle two members.  Every
+// reduction is per chunk (block partial) and then per member (the member's
+// chunk partials summed in chunk order by every block that needs the
+// scalar), so a member's arithmetic does not depend on which other members
+// share the launch: a solve gives bitwise the same iterates alone and in
+// any mix.
+//
+// Iteration k of a member (three launches for all members):
+//   spmv:       t = A p, chunk partial (p, t)
+//   update:     alpha = rz_k / (p, A p); x += alpha p; r -= alpha t;
+//               z = M r; chunk partials (r, r), (r, z)
+//   direction:  stop test on (r, r); beta = rz_{k+1} / rz_k; p = z + beta p
+// rz_k lives in slot k & 1 of the member's state, so the blocks of the
+// direction kernel read rz_k while the member's first block writes rz_{k+1}.
+// The iteration count is published through it_next and taken over by the
+// next spmv launch, for the same reason.
+//
+// Mixed precision (FP32 inner iterations, FP64 reliable updates): the FP32
+// recurrence never declares convergence itself.  When its residual has
+// dropped by delta^2 since the last reliable update (or claims the
+// tolerance), the member is flagged, and the next reliable-update pass
+// forms x64 += xc32, r64 = b - A64 x64 in FP64, decides convergence on
+// ||r64|| / ||b||, and restarts the FP32 recurrence from fp32(r64) with the
+// same direction.
+#include "gsx_common.h"
+#include "gsx_kernels.h"
+
+namespace gsx {
+
+namespace {
+
+__device__ inline double bsum256(double v, double *s)
+{
+    s[threadIdx.x] = v;
+    __syncthreads();
+#pragma unroll
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    const double r = s[0];
+    __syncthreads();
+    return r;
+}
+
+// sum of a member's chunk partials, in chunk order, by the whole block
+__device__ inline double member_sum(const double *__restrict__ part, int c0, int nc, double *s)
+{
+    double v = 0.0;
+    for (int k = threadIdx.x; k < nc; k += 256) v += part[c0 + k];
+    return bsum256(v, s);
+}
+
+template <int LPR, typename V, typename T>
+__device__ inline void chunk_spmv(const BSeg &g, int lo, int hi, const V *__restrict__ val,
+                                  const T *__restrict__ x, T *__restrict__ y, double &dot)
+{
+    constexpr int RPB = 256 / LPR;
+    const int grp = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+    for (int r = lo + grp; r < hi; r += RPB) {
+        const int k0 = g.rp[r], k1 = g.rp[r + 1];
+        T acc = 0;
+        for (int k = k0 + lane; k < k1; k += LPR) acc += (T)val[k] * x[g.ci[k]];
+#pragma unroll
+        for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+        if (lane == 0) {
+            y[r] = acc;
+            dot += (double)x[r] * (double)acc;
+        }
+    }
+}
+
+template <typename V, typename T>
+__device__ inline void chunk_spmv_any(const BSeg &g, int lo, int hi, const V *val, const T *x, T *y,
+                                      double &dot)
+{
+    switch (g.lpr) {
+    case 64: chunk_spmv<64, V, T>(g, lo, hi, val, x, y, dot); break;
+    case 32: chunk_spmv<32, V, T>(g, lo, hi, val, x, y, dot); break;
+    case 16: chunk_spmv<16, V, T>(g, lo, hi, val, x, y, dot); break;
+    default: chunk_spmv<8, V, T>(g, lo, hi, val, x, y, dot); break;
+    }
+}
+
+// r = b - A x (FP64), partial (r, r)
+template <int LPR>
+__device__ inline void chunk_residual(const BSeg &g, int lo, int hi, double &rr)
+{
+    constexpr int RPB = 256 / LPR;
+    const int grp = threadIdx.x / LPR, lane = threadIdx.x % LPR;
+    for (int r = lo + grp; r < hi; r += RPB) {
+        const int k0 = g.rp[r], k1 = g.rp[r + 1];
+        double acc = 0;
+        for (int k = k0 + lane; k < k1; k += LPR) acc += g.v64[k] * g.x[g.ci[k]];
+#pragma unroll
+        for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
+        if (lane == 0) {
+            const double v = g.b[r] - acc;
+            g.r[r] = v;
+            rr += v * v;
+        }
+    }
+}
+
+__device__ inline void chunk_residual_any(const BSeg &g, int lo, int hi, double &rr)
+{
+    switch (g.lpr) {
+    case 64: chunk_residual<64>(g, lo, hi, rr); break;
+    case 32: chunk_residual<32>(g, lo, hi, rr); break;
+    case 16: chunk_residual<16>(g, lo, hi, rr); break;
+    default: chunk_residual<8>(g, lo, hi, rr); break;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Initialisation of members in phase 0 (three launches):
+//   I1: r = b - A x (FP64), partials (r, r) and (b, b)
+//   I2: bnorm, err, done; z = M r (or the FP32 copies), p = z; partial (r, z)
+//   I3: rz_0, phase 1
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_b_init1(const BChunk *__restrict__ chunks,
+                                                 const BSeg *__restrict__ segs, BState *__restrict__ st,
+                                                 double *__restrict__ p0, double *__restrict__ p1)
+{
+    __shared__ double s[256];
+    const BChunk ch = chunks[blockIdx.x];
+    if (st[ch.seg].phase != 0) return;
+    const BSeg g = segs[ch.seg];
+    double rr = 0.0, bb = 0.0;
+    chunk_residual_any(g, ch.lo, ch.hi, rr);
+    for (int i = ch.lo + threadIdx.x; i < ch.hi; i += 256) bb += g.b[i] * g.b[i];
+    const double a = bsum256(rr, s), c = bsum256(bb, s);
+    if (threadIdx.x == 0) {
+        p0[blockIdx.x] = a;
+        p1[blockIdx.x] = c;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_b_init2(const BChunk *__restrict__ chunks,
+                                                 const BSeg *__restrict__ segs, BState *__restrict__ st,
+                                                 const double *__restrict__ p0, const double *__restrict__ p1,
+                                                 double *__restrict__ p2, int mixed)
+{
+    __shared__ double s[256];
+    const BChunk ch = chunks[blockIdx.x];
+    BState &S = st[ch.seg];
+    if (S.phase != 0) return;
+    const BSeg g = segs[ch.seg];
+    const double rr = member_sum(p0, g.c0, g.nc, s), bb = member_sum(p1, g.c0, g.nc, s);
+    const double bnorm = sqrt(bb);
+    int done = 0;
+    double err = 0.0;
+    if (bnorm == 0.0) done = 3;   // zero rhs: x = 0 is the solution
+    else {
+        err = sqrt(rr) / bnorm;
+        if (err < g.tol) done = 1;
+    }
+    double rz = 0.0;
+    for (int i = ch.lo + threadIdx.x; i < ch.hi; i += 256) {
+        if (done == 3) g.x[i] = 0.0;
+        if (done) continue;
+        if (mixed) {
+            const float r32 = (float)g.r[i];
+            const float z32 = g.d32 ? (float)g.tau * r32 / g.d32[i] : r32;
+            g.r32[i] = r32;
+            g.z32[i] = z32;
+            g.p32[i] = z32;
+            g.xc32[i] = 0.0f;
+            rz += (double)r32 * (double)z32;
+        } else {
+            const double ri = g.r[i];
+            const double zi = g.d64 ? g.tau * ri / g.d64[i] : ri;
+            g.z[i] = zi;
+            g.p[i] = zi;
+            rz += ri * zi;
+        }
+    }
+    const double a = bsum256(rz, s);
+    if (threadIdx.x == 0) p2[blockIdx.x] = a;
+    if (blockIdx.x == (unsigned)g.c0 && threadIdx.x == 0) {
+        S.bnorm = bnorm;
+        S.rr = rr;
+        S.rr_ref = rr;
+        S.err = err;
+        S.done = done;
+        S.it = 0;
+        S.it_next = 0;
+        S.need_update = 0;
+        S.stop = 0;
+        S.pending = 0;
+        if (g.hist) g.hist[0] = err;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_b_init3(const BChunk *__restrict__ chunks,
+                                                 const BSeg *__restrict__ segs, BState *__restrict__ st,
+                                                 const double *__restrict__ p2)
+{
+    __shared__ double s[256];
+    const BChunk ch = chunks[blockIdx.x];
+    BState &S = st[ch.seg];
+    if (S.phase != 0) return;
+    const BSeg g = segs[ch.seg];
+    if ((int)blockIdx.x != g.c0) return;
+    const double rz = member_sum(p2, g.c0, g.nc, s);
+    if (threadIdx.x == 0) {
+        S.rzs[0] = rz;
+        S.phase = 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One CG iteration (three launches).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_b_spmv(const BChunk *__restrict__ chunks,
+                                                const BSeg *__restrict__ segs, BState *__restrict__ st,
+                                                double *__restrict__ p0)
+{
+    __shared__ double s[256];
+    const BChunk ch = chunks[blockIdx.x];
+    BState &S = st[ch.seg];
+    if (S.phase != 1 || S.done || S.stop) return;
+    const BSeg g = segs[ch.seg];
+    if ((int)blockIdx.x == g.c0 && threadIdx.x == 0) S.it = S.it_next;
+    double dot = 0.0;
+    if constexpr (sizeof(T) == 4) chunk_spmv_any<float, float>(g, ch.lo, ch.hi, g.v32, g.p32, g.t32, dot);
+    else chunk_spmv_any<double, double>(g, ch.lo, ch.hi, g.v64, g.p, g.t, dot);
+    const double a = bsum256(dot, s);
+    if (threadIdx.x == 0) p0[blockIdx.x] = a;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_b_update(const BChunk *__restrict__ chunks,
+                                                  const BSeg *__restrict__ segs, BState *__restrict__ st,
+                                                  const double *__restrict__ p0, double *__restrict__ p1,
+                                                  double *__restrict__ p2)
+{
+    __shared__ double s[256];
+    const BChunk ch = chunks[blockIdx.x];
+    const BState &S = st[ch.seg];
+    if (S.phase != 1 || S.done || S.stop) return;
+    const BSeg g = segs[ch.seg];
+    const double pap = member_sum(p0, g.c0, g.nc, s);
+    const double alpha = S.rzs[S.it & 1] / pap;
+    double rr = 0.0, rz = 0.0;
+    if constexpr (sizeof(T) == 4) {
+        const float a = (float)alpha, tau = (float)g.tau;
+        for (int i = ch.lo + threadIdx.x; i < ch.hi; i += 256) {
+            g.xc32[i] += a * g.p32[i];
+            const float ri = g.r32[i] - a * g.t32[i];
+            g.r32[i] = ri;
+            const float zi = g.d32 ? tau * ri / g.d32[i] : ri;
+            g.z32[i] = zi;
+            rr += (double)ri * (double)ri;
+            rz += (double)ri * (double)zi;
+        }
+    } else {
+        for (int i = ch.lo + threadIdx.x; i < ch.hi; i += 256) {
+            g.x[i] += alpha * g.p[i];
+            const double ri = g.r[i] - alpha * g.t[i];
+            g.r[i] = ri;
+            const double zi = g.d64 ? g.tau * ri / g.d64[i] : ri;
+            g.z[i] = zi;
+            rr += ri * ri;
+            rz += ri * zi;
+        }
+    }
+    const double a = bsum256(rr, s), c = bsum256(rz, s);
+    if (threadIdx.x == 0) {
+        p1[blockIdx.x] = a;
+        p2[blockIdx.x] = c;
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_b_direction(const BChunk *__restrict__ chunks,
+                                                     const BSeg *__restrict__ segs, BState *__restrict__ st,
+                                                     const double *__restrict__ p1,
+                                                     const double *__restrict__ p2, double delta2)
+{
+    __shared__ double s[256];
+    const BChunk ch = chunks[blockIdx.x];
+    BState &S = st[ch.seg];
+    if (S.phase != 1 || S.done || S.stop) return;
+    const BSeg g = segs[ch.seg];
+    const double rr = member_sum(p1, g.c0, g.nc, s), rz = member_sum(p2, g.c0, g.nc, s);
+    const int k = S.it;
+    const double err = sqrt(rr) / S.bnorm;
+    const bool mixed = sizeof(T) == 4;
+    const bool conv = !mixed && err < g.tol;
+    const bool last = k + 1 >= g.maxit;
+    const double beta = rz / S.rzs[k & 1];
+    if (!conv) {
+        if constexpr (sizeof(T) == 4) {
+            const float b = (float)beta;
+            for (int i = ch.lo + threadIdx.x; i < ch.hi; i += 256) g.p32[i] = g.z32[i] + b * g.p32[i];
+        } else {
+            for (int i = ch.lo + threadIdx.x; i < ch.hi; i += 256) g.p[i] = g.z[i] + beta * g.p[i];
+        }
+    }
+    if ((int)blockIdx.x == g.c0 && threadIdx.x == 0) {
+        S.it_next = k + 1;
+        S.rr = rr;
+        S.err = err;
+        S.rzs[(k + 1) & 1] = rz;
+        if (g.hist && !mixed) g.hist[k + 1] = err;
+        if (mixed && (rr < delta2 * S.rr_ref || err < g.tol)) S.need_update = 1;
+        if (conv) {
+            S.done = 1;
+        } else if (last) {
+            S.stop = 1;
+            if (mixed) S.need_update = 1;
+            else S.done = 2;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reliable update of flagged mixed members (three launches):
+//   R1: x64 += xc32, xc32 = 0
+//   R2: r64 = b - A64 x64, partial (r, r)
+//   R3: decide; restart r32 = fp32(r64), z32 = M r32, partial (r32, z32)
+//   R4: rz into the current slot, clear the flag
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_b_rel1(const BChunk *__restrict__ chunks,
+                                                const BSeg *__restrict__ segs, BState *__restrict__ st)
+{
+    const BChunk ch = chunks[blockIdx.x];
+    const BState &S = st[ch.seg];
+    if (S.phase != 1 || S.done || !S.need_update) return;
+    const BSeg g = segs[ch.seg];
+    for (int i = ch.lo + threadIdx.x; i < ch.hi; i += 256) {
+        g.x[i] += (double)g.xc32[i];
+        g.xc32[i] = 0.0f;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_b_rel2(const BChunk *__restrict__ chunks,
+                                                const BSeg *__restrict__ segs, BState *__restrict__ st,
+                                                double *__restrict__ p0)
+{
+    __shared__ double s[256];
+    const BChunk ch = chunks[blockIdx.x];
+    const BState &S = st[ch.seg];
+    if (S.phase != 1 || S.done || !S.need_update) return;
+    const BSeg g = segs[ch.seg];
+    double rr = 0.0;
+    chunk_residual_any(g, ch.lo, ch.hi, rr);
+    const double a = bsum256(rr, s);
+    if (threadIdx.x == 0) p0[blockIdx.x] = a;
+}
+
+__global__ __launch_bounds__(256) void k_b_rel3(const BChunk *__restrict__ chunks,
+                                                const BSeg *__restrict__ segs, BState *__restrict__ st,
+                                                const double *__restrict__ p0, double *__restrict__ p2)
+{
+    __shared__ double s[256];
+    const BChunk ch = chunks[blockIdx.x];
+    BState &S = st[ch.seg];
+    if (S.phase != 1 || S.done || !S.need_update) return;
+    const BSeg g = segs[ch.seg];
+    const double rr = member_sum(p0, g.c0, g.nc, s);
+    const double err = sqrt(rr) / S.bnorm;
+    const int decided = err < g.tol ? 1 : (S.stop ? 2 : 0);
+    double rz = 0.0;
+    if (!decided) {
+        const float tau = (float)g.tau;
+        for (int i = ch.lo + threadIdx.x; i < ch.hi; i += 256) {
+            const float r32 = (float)g.r[i];
+            const float z32 = g.d32 ? tau * r32 / g.d32[i] : r32;
+            g.r32[i] = r32;
+            g.z32[i] = z32;
+            rz += (double)r32 * (double)z32;
+        }
+    }
+    const double a = bsum256(rz, s);
+    if (threadIdx.x == 0) p2[blockIdx.x] = a;
+    if ((int)blockIdx.x == g.c0 && threadIdx.x == 0) {
+        S.rr = rr;
+        S.rr_ref = rr;
+        S.err = err;
+        if (g.hist) g.hist[S.it_next] = err;
+        S.pending = decided;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_b_rel4(const BChunk *__restrict__ chunks,
+                                                const BSeg *__restrict__ segs, BState *__restrict__ st,
+                                                const double *__restrict__ p2)
+{
+    __shared__ double s[256];
+    const BChunk ch = chunks[blockIdx.x];
+    BState &S = st[ch.seg];
+    if (S.phase != 1 || S.done || !S.need_update) return;
+    const BSeg g = segs[ch.seg];
+    if ((int)blockIdx.x != g.c0) return;
+    const double rz = member_sum(p2, g.c0, g.nc, s);
+    if (threadIdx.x == 0) {
+        if (S.pending) S.done = S.pending;
+        else S.rzs[S.it_next & 1] = rz;
+        S.pending = 0;
+        S.need_update = 0;
+    }
+}
+
+// flag[0] = 1 when a row in [row0, row0 + nrows) has a column outside it
+// (every writer stores the same value)
+__global__ __launch_bounds__(256) void k_b_check(int row0, int nrows, const int *__restrict__ rp,
+                                                 const int *__restrict__ ci, int *__restrict__ flag)
+{
+    const int r = row0 + blockIdx.x * 4 + threadIdx.x / 64, lane = threadIdx.x % 64;
+    if (r >= row0 + nrows) return;
+    bool bad = false;
+    for (int k = rp[r] + lane; k < rp[r + 1]; k += 64) bad |= ci[k] < row0 || ci[k] >= row0 + nrows;
+    if (bad) flag[0] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+int batch_chunk_rows(int lpr) { return (256 / lpr) * 64; }
+
+hipError_t launch_batch_check(int row0, int nrows, const int *rp, const int *ci, int *flag, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_b_check, dim3((nrows + 3) / 4), dim3(256), 0, s, row0, nrows, rp, ci, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_init(int nchunks, const BChunk *chunks, const BSeg *segs, BState *st, double *p0,
+                             double *p1, double *p2, int mixed, hipStream_t s)
+{
+    if (nchunks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_b_init1, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p0, p1);
+    hipLaunchKernelGGL(k_b_init2, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p0, p1, p2, mixed);
+    hipLaunchKernelGGL(k_b_init3, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p2);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_iteration(int nchunks, const BChunk *chunks, const BSeg *segs, BState *st, double *p0,
+                                  double *p1, double *p2, int mixed, double delta2, hipStream_t s)
+{
+    if (nchunks <= 0) return hipSuccess;
+    if (mixed) {
+        hipLaunchKernelGGL(k_b_spmv<float>, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p0);
+        hipLaunchKernelGGL(k_b_update<float>, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p0, p1, p2);
+        hipLaunchKernelGGL(k_b_direction<float>, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p1, p2,
+                           delta2);
+    } else {
+        hipLaunchKernelGGL(k_b_spmv<double>, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p0);
+        hipLaunchKernelGGL(k_b_update<double>, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p0, p1, p2);
+        hipLaunchKernelGGL(k_b_direction<double>, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p1, p2,
+                           delta2);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_reliable(int nchunks, const BChunk *chunks, const BSeg *segs, BState *st, double *p0,
+                                 double *p2, hipStream_t s)
+{
+    if (nchunks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_b_rel1, dim3(nchunks), dim3(256), 0, s, chunks, segs, st);
+    hipLaunchKernelGGL(k_b_rel2, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p0);
+    hipLaunchKernelGGL(k_b_rel3, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p0, p2);
+    hipLaunchKernelGGL(k_b_rel4, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p2);
+    return hipGetLastError();
+}
+
+}  // namespace gsx

@@ -60,4 +60,42 @@ struct CGState {
     int need_update;  // mixed: a reliable update is due
 };
 
+// One member of a batched CG solve: rows [row0, row0 + nrows) of a CSR
+// matrix whose columns stay inside that range.  Vector pointers are bases
+// indexed by row/column number (the member's own arrays shifted by -row0).
+struct BSeg {
+    const int *rp;
+    const int *ci;
+    const double *v64;
+    const float *v32;
+    const double *b;
+    double *x, *r, *z, *p, *t;
+    float *r32, *z32, *p32, *t32, *xc32;
+    const double *d64;   // Jacobi diagonal (null: no preconditioner)
+    const float *d32;
+    double *hist;        // relative residual per iteration (maxit + 1) or null
+    double tau;
+    double tol;
+    int maxit;
+    int lpr;             // lanes per row of the SpMV
+    int c0, nc;          // first chunk and chunk count
+};
+
+struct BChunk {
+    int seg;
+    int lo, hi;          // rows [lo, hi)
+};
+
+struct BState {
+    double rzs[2];       // (r, z) of iteration k in slot k & 1
+    double bnorm, rr, rr_ref, err;
+    int it, it_next;     // completed iterations (it_next published by the direction step)
+    int phase;           // 0: to initialise, 1: iterating
+    int done;            // 1 converged, 2 iteration limit, 3 zero rhs
+    int need_update;     // mixed: reliable update due
+    int stop;            // mixed: iteration limit reached, decide at the next update
+    int pending;         // mixed: decision of the running update
+    int pad;
+};
+
 }  // namespace gsx

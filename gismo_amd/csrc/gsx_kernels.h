@@ -161,4 +161,14 @@ hipError_t launch_set_rz(const double *parts, int np, CGState *st, int slot, hip
 hipError_t launch_grid_f64_to_f32(int64_t n, const double *a, float *b, hipStream_t s);
 hipError_t launch_accum_grid(int64_t n, double *x, float *xc, hipStream_t s);
 
+// batched CG (gsx_batch.hip)
+int batch_chunk_rows(int lpr);
+hipError_t launch_batch_check(int row0, int nrows, const int *rp, const int *ci, int *flag, hipStream_t s);
+hipError_t launch_batch_init(int nchunks, const BChunk *chunks, const BSeg *segs, BState *st, double *p0,
+                             double *p1, double *p2, int mixed, hipStream_t s);
+hipError_t launch_batch_iteration(int nchunks, const BChunk *chunks, const BSeg *segs, BState *st, double *p0,
+                                  double *p1, double *p2, int mixed, double delta2, hipStream_t s);
+hipError_t launch_batch_reliable(int nchunks, const BChunk *chunks, const BSeg *segs, BState *st, double *p0,
+                                 double *p2, hipStream_t s);
+
 }  // namespace gsx

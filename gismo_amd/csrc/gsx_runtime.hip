@@ -2223,3 +2223,278 @@ static gsx_status cg_mg_entry(gsx_matrix *A, gsx_mg *mg, const double *b, double
 {
     return cg_mg(A, mg, b, x, tol, maxit, iters, relres);
 }
+
+// ===========================================================================
+// Batched CG: independent solves advanced together (gsx_batch.hip)
+// ===========================================================================
+namespace {
+
+struct BMember {
+    bool used = false;
+    int status = 0;           // 0 running, else gsx::BState::done
+    gsx_matrix *A = nullptr;
+    int row0 = 0, nrows = 0;
+    int maxit = 0;
+    DevPtr r, z, p, t, r32, z32, p32, t32, xc32, d32, hist;
+    gsx::BSeg seg{};
+    gsx::BState st{};
+};
+
+template <typename T>
+T *shift(T *p, int row0)
+{
+    return p ? reinterpret_cast<T *>(reinterpret_cast<uintptr_t>(p) - sizeof(T) * (size_t)row0) : nullptr;
+}
+
+}  // namespace
+
+struct gsx_batch {
+    gsx_ctx *c = nullptr;
+    bool mixed = false;
+    int cap = 0;
+    std::vector<BMember> m;
+    DevPtr d_segs, d_state, d_chunks, d_parts, d_bad;
+    int chunk_cap = 0, nchunks = 0;
+    bool dirty = true;
+    std::vector<gsx::BChunk> chunks_h;
+    std::vector<gsx::BState> state_h;
+};
+
+gsx_status gsx_batch_create(gsx_ctx *c, int32_t precision, int32_t capacity, gsx_batch **out)
+{
+    if (!c || !out || capacity <= 0 || (precision != GSX_FP64 && precision != GSX_MIXED)) return GSX_ERR_ARG;
+    gsx_batch *B = new gsx_batch();
+    B->c = c;
+    B->mixed = precision == GSX_MIXED;
+    B->cap = capacity;
+    B->m.resize(capacity);
+    B->state_h.resize(capacity);
+    B->d_segs = dalloc(c, sizeof(gsx::BSeg) * capacity);
+    B->d_state = dalloc(c, sizeof(gsx::BState) * capacity);
+    B->d_bad = dalloc(c, sizeof(int));
+    if (!B->d_segs || !B->d_state || !B->d_bad) {
+        delete B;
+        return fail(c, GSX_ERR_OOM, "device allocation (batch)");
+    }
+    *out = B;
+    return GSX_OK;
+}
+
+gsx_status gsx_batch_destroy(gsx_batch *B)
+{
+    if (!B) return GSX_ERR_ARG;
+    (void)hipStreamSynchronize(B->c->stream);
+    delete B;
+    return GSX_OK;
+}
+
+gsx_status gsx_batch_add(gsx_batch *B, gsx_matrix *A, int32_t row0, int32_t nrows, int32_t precond, double tau,
+                         const double *b, double *x, double tol, int32_t maxit, int32_t *id)
+{
+    if (!B || !A || !b || !x || !id || maxit < 1 || nrows <= 0 || row0 < 0 || (int64_t)row0 + nrows > A->n)
+        return GSX_ERR_ARG;
+    gsx_ctx *c = B->c;
+    if (A->ctx != c) return fail(c, GSX_ERR_ARG, "matrix belongs to another context");
+    if (precond != GSX_PREC_NONE && precond != GSX_PREC_JACOBI)
+        return fail(c, GSX_ERR_ARG, "batched CG supports no preconditioner or Jacobi");
+    if (B->mixed && !A->mixed) return fail(c, GSX_ERR_ARG, "mixed batch needs matrices assembled with GSX_MIXED");
+    int slot = -1;
+    for (int k = 0; k < B->cap; ++k)
+        if (!B->m[k].used) {
+            slot = k;
+            break;
+        }
+    if (slot < 0) return fail(c, GSX_ERR_ARG, "batch is full");
+    hipStream_t s = c->stream;
+    // the member's rows must only couple to its own rows
+    int rp2[2] = {0, 0};
+    HIP_OK(c, hipMemcpyAsync(&rp2[0], dp<int>(A->rowptr) + row0, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(c, hipMemcpyAsync(&rp2[1], dp<int>(A->rowptr) + row0 + nrows, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(c, hipMemsetAsync(B->d_bad.get(), 0, sizeof(int), s));
+    HIP_OK(c, gsx::launch_batch_check(row0, nrows, dp<int>(A->rowptr), dp<int>(A->col), dp<int>(B->d_bad), s));
+    int bad = 0;
+    HIP_OK(c, hipMemcpyAsync(&bad, B->d_bad.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_OK(c, hipStreamSynchronize(s));
+    if (bad) return fail(c, GSX_ERR_ARG, "member rows couple to rows outside [row0, row0 + nrows)");
+    if (!A->diag && precond == GSX_PREC_JACOBI) {
+        A->diag = dalloc(c, sizeof(double) * A->n);
+        if (!A->diag) return fail(c, GSX_ERR_OOM, "device allocation (diag)");
+        HIP_OK(c, gsx::launch_diag_f64(A->n, dp<int>(A->rowptr), dp<int>(A->col), dp<double>(A->val),
+                                       dp<double>(A->diag), s));
+    }
+    if (B->mixed && !A->val32) {
+        A->val32 = dalloc(c, sizeof(float) * std::max<int64_t>(1, A->nnz));
+        if (!A->val32) return fail(c, GSX_ERR_OOM, "device allocation (fp32 values)");
+        HIP_OK(c, gsx::launch_f64_to_f32(A->nnz, dp<double>(A->val), dp<float>(A->val32), s));
+    }
+    BMember M;
+    M.used = true;
+    M.A = A;
+    M.row0 = row0;
+    M.nrows = nrows;
+    M.maxit = maxit;
+    const size_t n8 = sizeof(double) * nrows, n4 = sizeof(float) * nrows;
+    M.r = dalloc(c, n8);
+    M.z = dalloc(c, n8);
+    M.p = dalloc(c, n8);
+    M.t = dalloc(c, n8);
+    M.hist = dalloc(c, sizeof(double) * (maxit + 1));
+    bool ok = M.r && M.z && M.p && M.t && M.hist;
+    if (B->mixed) {
+        M.r32 = dalloc(c, n4);
+        M.z32 = dalloc(c, n4);
+        M.p32 = dalloc(c, n4);
+        M.t32 = dalloc(c, n4);
+        M.xc32 = dalloc(c, n4);
+        ok = ok && M.r32 && M.z32 && M.p32 && M.t32 && M.xc32;
+        if (precond == GSX_PREC_JACOBI) {
+            M.d32 = dalloc(c, n4);
+            ok = ok && M.d32;
+            if (ok) HIP_OK(c, gsx::launch_f64_to_f32(nrows, dp<double>(A->diag) + row0, dp<float>(M.d32), s));
+        }
+    }
+    if (!ok) return fail(c, GSX_ERR_OOM, "device allocation (batch member)");
+    gsx::BSeg &g = M.seg;
+    g.rp = dp<int>(A->rowptr);
+    g.ci = dp<int>(A->col);
+    g.v64 = dp<double>(A->val);
+    g.v32 = B->mixed ? dp<float>(A->val32) : nullptr;
+    g.b = shift(b, row0);
+    g.x = shift(x, row0);
+    g.r = shift(dp<double>(M.r), row0);
+    g.z = shift(dp<double>(M.z), row0);
+    g.p = shift(dp<double>(M.p), row0);
+    g.t = shift(dp<double>(M.t), row0);
+    g.r32 = shift(dp<float>(M.r32), row0);
+    g.z32 = shift(dp<float>(M.z32), row0);
+    g.p32 = shift(dp<float>(M.p32), row0);
+    g.t32 = shift(dp<float>(M.t32), row0);
+    g.xc32 = shift(dp<float>(M.xc32), row0);
+    g.d64 = precond == GSX_PREC_JACOBI ? dp<double>(A->diag) : nullptr;
+    g.d32 = shift(dp<float>(M.d32), row0);
+    g.hist = dp<double>(M.hist);
+    g.tau = tau;
+    g.tol = tol;
+    g.maxit = maxit;
+    g.lpr = gsx::spmv_lpr((double)(rp2[1] - rp2[0]) / nrows);
+    std::memset(&M.st, 0, sizeof(M.st));
+    // fresh state: phase 0 (initialised by the next step)
+    HIP_OK(c, hipMemcpyAsync(dp<gsx::BState>(B->d_state) + slot, &M.st, sizeof(gsx::BState),
+                             hipMemcpyHostToDevice, s));
+    HIP_OK(c, hipStreamSynchronize(s));
+    B->m[slot] = std::move(M);
+    B->dirty = true;
+    *id = slot;
+    return GSX_OK;
+}
+
+static gsx_status batch_rebuild(gsx_batch *B)
+{
+    gsx_ctx *c = B->c;
+    hipStream_t s = c->stream;
+    B->chunks_h.clear();
+    for (int k = 0; k < B->cap; ++k) {
+        BMember &M = B->m[k];
+        if (!M.used || M.status) continue;
+        const int rows = gsx::batch_chunk_rows(M.seg.lpr);
+        M.seg.c0 = (int)B->chunks_h.size();
+        for (int lo = M.row0; lo < M.row0 + M.nrows; lo += rows)
+            B->chunks_h.push_back({k, lo, std::min(lo + rows, M.row0 + M.nrows)});
+        M.seg.nc = (int)B->chunks_h.size() - M.seg.c0;
+        HIP_OK(c, hipMemcpyAsync(dp<gsx::BSeg>(B->d_segs) + k, &M.seg, sizeof(gsx::BSeg), hipMemcpyHostToDevice, s));
+    }
+    B->nchunks = (int)B->chunks_h.size();
+    if (B->nchunks > B->chunk_cap) {
+        HIP_OK(c, hipStreamSynchronize(s));
+        B->chunk_cap = std::max(B->nchunks, 2 * B->chunk_cap);
+        B->d_chunks = dalloc(c, sizeof(gsx::BChunk) * B->chunk_cap);
+        B->d_parts = dalloc(c, sizeof(double) * 3 * B->chunk_cap);
+        if (!B->d_chunks || !B->d_parts) return fail(c, GSX_ERR_OOM, "device allocation (batch chunks)");
+    }
+    if (B->nchunks)
+        HIP_OK(c, hipMemcpyAsync(B->d_chunks.get(), B->chunks_h.data(), sizeof(gsx::BChunk) * B->nchunks,
+                                 hipMemcpyHostToDevice, s));
+    // the host arrays are reused by the next rebuild: finish the copies
+    HIP_OK(c, hipStreamSynchronize(s));
+    B->dirty = false;
+    return GSX_OK;
+}
+
+gsx_status gsx_batch_step(gsx_batch *B, int32_t iters, int32_t *running)
+{
+    if (!B || iters < 0) return GSX_ERR_ARG;
+    gsx_ctx *c = B->c;
+    hipStream_t s = c->stream;
+    if (B->dirty) GSX_TRY(batch_rebuild(B));
+    const int nc = B->nchunks;
+    if (nc > 0) {
+        const gsx::BChunk *ch = dp<gsx::BChunk>(B->d_chunks);
+        const gsx::BSeg *sg = dp<gsx::BSeg>(B->d_segs);
+        gsx::BState *st = dp<gsx::BState>(B->d_state);
+        double *P0 = dp<double>(B->d_parts), *P1 = P0 + B->chunk_cap, *P2 = P1 + B->chunk_cap;
+        {
+            KScope k(c, "batch_init");
+            HIP_OK(c, gsx::launch_batch_init(nc, ch, sg, st, P0, P1, P2, B->mixed, s));
+        }
+        {
+            KScope k(c, B->mixed ? "batch_iter_f32" : "batch_iter_f64");
+            for (int i = 0; i < iters; ++i)
+                HIP_OK(c, gsx::launch_batch_iteration(nc, ch, sg, st, P0, P1, P2, B->mixed, 1e-4, s));
+        }
+        if (B->mixed) {
+            KScope k(c, "batch_reliable");
+            HIP_OK(c, gsx::launch_batch_reliable(nc, ch, sg, st, P0, P2, s));
+        }
+        HIP_OK(c, hipMemcpyAsync(B->state_h.data(), st, sizeof(gsx::BState) * B->cap, hipMemcpyDeviceToHost, s));
+        HIP_OK(c, hipStreamSynchronize(s));
+    }
+    int run = 0;
+    for (int k = 0; k < B->cap; ++k) {
+        BMember &M = B->m[k];
+        if (!M.used || M.status) continue;
+        M.st = B->state_h[k];
+        if (M.st.phase == 1 && M.st.done) {
+            M.status = M.st.done;
+            B->dirty = true;
+        } else {
+            ++run;
+        }
+    }
+    if (running) *running = run;
+    return GSX_OK;
+}
+
+gsx_status gsx_batch_result(const gsx_batch *B, int32_t id, int32_t *status, int32_t *iters, double *relres)
+{
+    if (!B || id < 0 || id >= B->cap || !B->m[id].used) return GSX_ERR_ARG;
+    const BMember &M = B->m[id];
+    if (status) *status = M.status;
+    if (iters) *iters = M.st.it_next;
+    if (relres) *relres = M.st.err;
+    return GSX_OK;
+}
+
+gsx_status gsx_batch_history(const gsx_batch *B, int32_t id, double *hist, int32_t cap, int32_t *count)
+{
+    if (!B || id < 0 || id >= B->cap || !B->m[id].used || !count) return GSX_ERR_ARG;
+    const BMember &M = B->m[id];
+    const int n = std::min(M.st.it_next + 1, M.maxit + 1);
+    *count = n;
+    if (hist && cap > 0) {
+        gsx_ctx *c = B->c;
+        HIP_OK(c, hipMemcpyAsync(hist, M.hist.get(), sizeof(double) * std::min(n, cap), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIP_OK(c, hipStreamSynchronize(c->stream));
+    }
+    return GSX_OK;
+}
+
+gsx_status gsx_batch_remove(gsx_batch *B, int32_t id)
+{
+    if (!B || id < 0 || id >= B->cap || !B->m[id].used) return GSX_ERR_ARG;
+    // kernels that may still read the member's buffers
+    HIP_OK(B->c, hipStreamSynchronize(B->c->stream));
+    B->m[id] = BMember();
+    B->dirty = true;
+    return GSX_OK;
+}

@@ -111,6 +111,14 @@ SIGNATURES = {
     "gsx_mg_apply": [_vp, _vp, _vp],
     "gsx_mg_level_info": [_vp, C.c_int32, _i64p, _i64p, _i64p],
     "gsx_mg_level_download": [_vp, C.c_int32, _i32p, _i32p, _dp, _i32p, _i32p, _dp],
+    "gsx_batch_create": [_vp, C.c_int32, C.c_int32, _pp],
+    "gsx_batch_destroy": [_vp],
+    "gsx_batch_add": [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_double, _vp, _vp, C.c_double,
+                      C.c_int32, _i32p],
+    "gsx_batch_step": [_vp, C.c_int32, _i32p],
+    "gsx_batch_result": [_vp, C.c_int32, _i32p, _i32p, _dp],
+    "gsx_batch_history": [_vp, C.c_int32, _dp, C.c_int32, _i32p],
+    "gsx_batch_remove": [_vp, C.c_int32],
 }
 _RESTYPES = {"gsx_last_error": C.c_char_p}
 
@@ -192,10 +200,19 @@ class DeviceArray:
     def zero(self):
         return self.upload(np.zeros(self.n, self.dtype))
 
+    def view(self, offset, n):
+        """Entries [offset, offset + n) (shares the memory; not freed on its own)."""
+        assert 0 <= offset and offset + n <= self.n
+        v = DeviceArray.__new__(DeviceArray)
+        v.ctx, v.n, v.dtype = self.ctx, int(n), self.dtype
+        v.ptr = C.c_void_p(self.ptr.value + offset * self.dtype.itemsize)
+        v._base = self
+        return v
+
     def close(self):
-        if self.ptr:
+        if self.ptr and getattr(self, "_base", None) is None:
             lib().gsx_device_free(self.ctx.h, self.ptr)
-            self.ptr = None
+        self.ptr = None
 
 
 class Context:
@@ -428,3 +445,55 @@ def mg_cg_solve(matrix, mg, b, x, tol=1e-8, maxit=500):
                             C.byref(rr))
     _check(matrix.ctx.h, st, (GSX_ERR_NOT_CONVERGED,))
     return it.value, rr.value, st == GSX_OK
+
+
+class Batch:
+    """Independent CG solves advanced together on the device (one launch per
+    CG stage for all members; a member's result does not depend on the
+    others)."""
+
+    def __init__(self, ctx, precision=FP64, capacity=1024):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(ctx.h, lib().gsx_batch_create(ctx.h, precision, capacity, C.byref(h)))
+        self.h = h
+
+    def add(self, matrix, b, x, row0=0, nrows=None, tol=1e-10, maxit=1000, precond=PREC_JACOBI, tau=1.0):
+        nrows = matrix.n - row0 if nrows is None else nrows
+        i = C.c_int32()
+        _check(self.ctx.h, lib().gsx_batch_add(self.h, matrix.h, row0, nrows, precond, tau, b.ptr, x.ptr, tol,
+                                               maxit, C.byref(i)))
+        return i.value
+
+    def step(self, iters=8):
+        run = C.c_int32()
+        _check(self.ctx.h, lib().gsx_batch_step(self.h, iters, C.byref(run)))
+        return run.value
+
+    def run(self, iters=8, max_steps=100000):
+        for _ in range(max_steps):
+            if self.step(iters) == 0:
+                return
+        raise RuntimeError("batch did not finish")
+
+    def result(self, i):
+        """(status, iterations, relative residual); status 1 converged,
+        2 iteration limit, 3 zero rhs, 0 running."""
+        st, it, rr = C.c_int32(), C.c_int32(), C.c_double()
+        _check(self.ctx.h, lib().gsx_batch_result(self.h, i, C.byref(st), C.byref(it), C.byref(rr)))
+        return st.value, it.value, rr.value
+
+    def history(self, i):
+        n = C.c_int32()
+        _check(self.ctx.h, lib().gsx_batch_history(self.h, i, None, 0, C.byref(n)))
+        out = np.zeros(n.value)
+        _check(self.ctx.h, lib().gsx_batch_history(self.h, i, out.ctypes.data_as(_dp), n.value, C.byref(n)))
+        return out
+
+    def remove(self, i):
+        _check(self.ctx.h, lib().gsx_batch_remove(self.h, i))
+
+    def close(self):
+        if self.h:
+            lib().gsx_batch_destroy(self.h)
+            self.h = None

@@ -121,7 +121,7 @@ int main(int argc, char *argv[])
             for (index_t i = 0; i < errorHistory.rows(); ++i) std::printf(" %.3e", errorHistory(i));
             std::printf("\n");
         }
-        std::printf("L2 %.6e\n", gsExprEvaluator<>(assembler).l2ErrorSinProduct(x));
+        std::printf("L2 %.17g\n", gsExprEvaluator<>(assembler).l2ErrorSinProduct(x));
     } catch (const std::exception &ex) {
         std::fprintf(stderr, "error: %s\n", ex.what());
         return 2;

@@ -108,7 +108,7 @@ int main(int argc, char *argv[])
             const double e = full ? gsExprEvaluator<>(*pa).l2ErrorSinProduct(solVector)
                                   : math::sqrt(ev.integral((u_ex - u_sol).sqNorm() * meas(G)));
             l2err.push_back(e);
-            std::printf("level %d dofs %d nnz %d iterations %d relres %.3e L2 %.6e\n", r, (int)K->rows(),
+            std::printf("level %d dofs %d nnz %d iterations %d relres %.3e L2 %.17g\n", r, (int)K->rows(),
                         (int)K->nonZeros(), (int)cg.iterations(), cg.error(), e);
         }
         std::printf("Assembly: %.4f s  Solving: %.4f s\n", ma_time, slv_time);

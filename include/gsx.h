@@ -223,6 +223,45 @@ gsx_status gsx_mg_level_download(const gsx_mg *mg, int32_t lvl, int32_t *rowptr_
                                  int32_t *prowptr_host, int32_t *pcol_host,
                                  double *pval_host);
 
+/* ---- batched solves ------------------------------------------------------ */
+/* Many independent Jacobi-preconditioned (or plain) CG solves advanced
+ * together: every CG stage is one launch for all running members.  Each
+ * member runs gsConjugateGradient's algorithm and stopping rule
+ * (src/gsSolver/gsConjugateGradient.hpp:20-93, gsIterativeSolver.h:114-200);
+ * the reference solves such problems one after another and has no batch
+ * object.  A member's iterates do not depend on the other members: a solve
+ * gives bitwise the same result alone and in any mix.
+ * precision GSX_MIXED: FP32 iterations with FP64 reliable updates (as in
+ * gsx_cg_solve), for members whose matrix was assembled with GSX_MIXED.
+ * capacity: maximum number of members at one time. */
+typedef struct gsx_batch gsx_batch;
+gsx_status gsx_batch_create(gsx_ctx *ctx, int32_t precision, int32_t capacity,
+                            gsx_batch **out);
+gsx_status gsx_batch_destroy(gsx_batch *batch);
+/* Add the solve A[r, r] x = b for the rows r in [row0, row0 + nrows) of A,
+ * which must not couple to rows outside that range (a diagonal block, e.g.
+ * one patch of a matrix assembled from disjoint patches, or the whole
+ * matrix).  b_dev/x_dev hold nrows doubles; x_dev is the initial guess and
+ * receives the solution.  Members may share a matrix (several right-hand
+ * sides).  precond: GSX_PREC_NONE or GSX_PREC_JACOBI (damping tau).
+ * Returns the member id. */
+gsx_status gsx_batch_add(gsx_batch *batch, gsx_matrix *A, int32_t row0,
+                         int32_t nrows, int32_t precond, double tau,
+                         const double *b_dev, double *x_dev, double tol,
+                         int32_t maxit, int32_t *id);
+/* Initialise newly added members, run iters CG iterations of every running
+ * member (members that finish stop on their own), then the reliable update
+ * (mixed).  running: members still iterating afterwards. */
+gsx_status gsx_batch_step(gsx_batch *batch, int32_t iters, int32_t *running);
+/* status: 0 running, 1 converged, 2 iteration limit, 3 zero right-hand side. */
+gsx_status gsx_batch_result(const gsx_batch *batch, int32_t id, int32_t *status,
+                            int32_t *iters, double *relres);
+/* Relative residual history of a member (iters + 1 values; FP64 members). */
+gsx_status gsx_batch_history(const gsx_batch *batch, int32_t id, double *hist_host,
+                             int32_t cap, int32_t *count);
+/* Release a member's slot and workspace (its x_dev keeps the solution). */
+gsx_status gsx_batch_remove(gsx_batch *batch, int32_t id);
+
 /* ---- reporting ---------------------------------------------------------- */
 /* L2 error of the discrete solution (free coefficients, eliminated = 0)
  * against u = prod_r sin(pi x_r) with the assembly quadrature

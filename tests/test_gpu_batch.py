@@ -1,0 +1,190 @@
+"""Batched CG (gsx_batch_*) on the GPU against the oracle: every member of a
+mixed 2D/3D batch of uneven sizes and degrees reaches the oracle's solution
+with the oracle's iteration count; a member's result is bitwise the same
+alone, in a mix, added while others run, and as a block of a matrix
+assembled from disjoint patches; mixed precision reaches the FP64 solution;
+zero right-hand sides and iteration limits end with their status."""
+import numpy as np
+import pytest
+
+import oracle as O
+from gismo_amd import gsx, problems
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_batch(count=12, seed=3):
+    return problems.batch_problems(count=count, seed=seed, n2d=(4, 14), n3d=(3, 6))
+
+
+def _oracle(prob, tol):
+    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    A, b = O.assemble(prob.patches, dofs, nfree, 0)
+    x, it, err = O.cg_solve(A, b, tol=tol, maxit=5000)
+    return A, b, x, it
+
+
+class _Member:
+    def __init__(self, ctx, prob, precision):
+        self.prob = prob
+        self.gp = gsx.Problem(ctx, prob)
+        self.A, self.b = self.gp.assemble(gsx.PATTERN_EXPR, precision)
+        self.x = ctx.array(self.gp.nfree).zero()
+
+    def close(self):
+        self.x.close()
+        self.b.close()
+        self.A.close()
+        self.gp.close()
+
+
+def _solve(ctx, members, precision, tol, iters=8, maxit=5000):
+    B = gsx.Batch(ctx, precision, capacity=max(4, len(members)))
+    ids = []
+    for m in members:
+        m.x.zero()
+        ids.append(B.add(m.A, m.b, m.x, tol=tol, maxit=maxit))
+    B.run(iters)
+    res = [B.result(i) for i in ids]
+    hist = [B.history(i) for i in ids] if precision == gsx.FP64 else None
+    B.close()
+    return res, [m.x.download() for m in members], hist
+
+
+def test_batch_fp64_matches_oracle(ctx):
+    probs = _small_batch()
+    members = [_Member(ctx, p, gsx.FP64) for p in probs]
+    res, xs, hist = _solve(ctx, members, gsx.FP64, 1e-10)
+    for prob, (st, it, rr), x, h in zip(probs, res, xs, hist):
+        _, _, xo, ito = _oracle(prob, 1e-10)
+        print(prob.name, it, ito, flush=True)
+        assert st == 1 and rr < 1e-10
+        # counts beyond ~nfree (small, badly conditioned 3D p=4 members) are
+        # set by rounding: allow 2% there
+        assert abs(it - ito) <= max(1, 0.02 * ito), (prob.name, it, ito)
+        assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+        assert len(h) == it + 1 and h[-1] == pytest.approx(rr, rel=1e-12) and h[0] == pytest.approx(1.0)
+    for m in members:
+        m.close()
+
+
+def test_batch_member_is_bitwise_the_same_alone_and_in_any_mix(ctx):
+    probs = _small_batch(count=8, seed=11)
+    members = [_Member(ctx, p, gsx.FP64) for p in probs]
+    _, xs_mix, _ = _solve(ctx, members, gsx.FP64, 1e-10)
+    for k in (0, 5):
+        _, xs_alone, _ = _solve(ctx, [members[k]], gsx.FP64, 1e-10)
+        assert np.array_equal(xs_alone[0], xs_mix[k])
+    # reversed order, another step length
+    _, xs_rev, _ = _solve(ctx, members[::-1], gsx.FP64, 1e-10, iters=3)
+    for a, b in zip(xs_rev[::-1], xs_mix):
+        assert np.array_equal(a, b)
+    for m in members:
+        m.close()
+
+
+def test_batch_admits_members_while_others_run(ctx):
+    probs = _small_batch(count=6, seed=5)
+    members = [_Member(ctx, p, gsx.FP64) for p in probs]
+    _, xs_ref, _ = _solve(ctx, members, gsx.FP64, 1e-10)
+    for m in members:
+        m.x.zero()
+    B = gsx.Batch(ctx, gsx.FP64, capacity=8)
+    ids = {}
+    for k in range(3):
+        ids[k] = B.add(members[k].A, members[k].b, members[k].x, tol=1e-10, maxit=5000)
+    B.step(5)
+    for k in range(3, 6):
+        ids[k] = B.add(members[k].A, members[k].b, members[k].x, tol=1e-10, maxit=5000)
+        B.step(2)
+    # a finished member leaves, its slot is reused by a new solve of member 0
+    B.run(4)
+    B.remove(ids[0])
+    x0 = ctx.array(members[0].gp.nfree).zero()
+    j = B.add(members[0].A, members[0].b, x0, tol=1e-10, maxit=5000)
+    B.run(4)
+    assert B.result(j)[0] == 1
+    for k in range(6):
+        assert np.array_equal(members[k].x.download(), xs_ref[k]), k
+    assert np.array_equal(x0.download(), xs_ref[0])
+    B.close()
+    x0.close()
+    for m in members:
+        m.close()
+
+
+def test_batch_mixed_precision_reaches_fp64_solution(ctx):
+    probs = _small_batch(count=10, seed=9)
+    members = [_Member(ctx, p, gsx.MIXED) for p in probs]
+    # tolerance 1e-12: the error bound 1e-8 vs the exact FP64 solution
+    # needs it on the worst-conditioned members (3D, p=4)
+    res, xs, _ = _solve(ctx, members, gsx.MIXED, 1e-12)
+    for prob, (st, it, rr), x in zip(probs, res, xs):
+        _, _, xo, ito = _oracle(prob, 1e-14)
+        print(prob.name, it, ito, rr, np.linalg.norm(x - xo) / np.linalg.norm(xo), flush=True)
+        assert st == 1 and rr < 1e-12
+        assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+    for m in members:
+        m.close()
+
+
+def test_batch_blocks_of_disjoint_patch_matrix(ctx):
+    probs = [p for p in _small_batch(count=12, seed=21) if p.d == 2]
+    joint = problems.Problem([p.patches[0] for p in probs],
+                             [(k, s) for k in range(len(probs)) for s in range(1, 5)], [], name="joint")
+    gj = gsx.Problem(ctx, joint)
+    Aj, bj = gj.assemble(gsx.PATTERN_EXPR, gsx.FP64)
+    xj = ctx.array(gj.nfree).zero()
+    members = [_Member(ctx, p, gsx.FP64) for p in probs]
+    _, xs, _ = _solve(ctx, members, gsx.FP64, 1e-10)
+    B = gsx.Batch(ctx, gsx.FP64, capacity=len(probs))
+    off, ids, offs = 0, [], []
+    for m in members:
+        n = m.gp.nfree
+        ids.append(B.add(Aj, bj.view(off, n), xj.view(off, n), row0=off, nrows=n, tol=1e-10, maxit=5000))
+        offs.append(off)
+        off += n
+    assert off == gj.nfree
+    B.run(8)
+    xall = xj.download()
+    for m, o, x in zip(members, offs, xs):
+        xb = xall[o:o + m.gp.nfree]
+        assert np.linalg.norm(xb - x) <= 1e-10 * np.linalg.norm(x)
+    # a member whose rows couple outside its range is refused
+    with pytest.raises(gsx.GsxError):
+        grid = problems.square_grid(2, 1, n=4, p=2)
+        gg = gsx.Problem(ctx, grid)
+        Ag, bg = gg.assemble(gsx.PATTERN_EXPR, gsx.FP64)
+        half = gg.nfree // 2
+        B.add(Ag, bg.view(0, half), xj.view(0, half), row0=0, nrows=half)
+    B.close()
+    for m in members:
+        m.close()
+
+
+def test_batch_zero_rhs_and_iteration_limit(ctx):
+    probs = _small_batch(count=4, seed=2)
+    members = [_Member(ctx, p, gsx.FP64) for p in probs]
+    zero = ctx.array(members[0].gp.nfree).zero()
+    xz = ctx.to_device(np.ones(members[0].gp.nfree))
+    B = gsx.Batch(ctx, gsx.FP64, capacity=4)
+    i0 = B.add(members[0].A, zero, xz, tol=1e-10)
+    i1 = B.add(members[1].A, members[1].b, members[1].x.zero(), tol=1e-14, maxit=3)
+    B.run(8)
+    assert B.result(i0)[0] == 3 and not np.any(xz.download())
+    st, it, rr = B.result(i1)
+    assert st == 2 and it == 3 and rr > 1e-14
+    B.close()
+    # the same limit in mixed precision
+    mm = _Member(ctx, probs[1], gsx.MIXED)
+    B = gsx.Batch(ctx, gsx.MIXED, capacity=2)
+    i = B.add(mm.A, mm.b, mm.x, tol=1e-14, maxit=3)
+    B.run(2)
+    st, it, rr = B.result(i)
+    assert st == 2 and it == 3
+    B.close()
+    mm.close()
+    zero.close()
+    xz.close()
+    for m in members:
+        m.close()
