@@ -32,6 +32,10 @@ METRIC = "assembled+solved DOFs/s at rel err ≤1e-8, mixed vs plain FP64; p99 l
 HBM_PEAK_GBS = 8000.0
 
 
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -83,11 +87,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--workload", choices=["cube", "mg", "batch"], default="cube",
+                    help="cube: single-patch 3D assemble + Jacobi-CG (default); mg: 16-patch 2D "
+                         "assemble + multigrid-preconditioned CG; batch: 512 independent problems")
+    ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--p", type=int, default=3)
-    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--tol", type=float, default=None)
     ap.add_argument("--maxit", type=int, default=5000)
-    ap.add_argument("--cpu-sample", type=int, default=20)
+    ap.add_argument("--count", type=int, default=512, help="batch: number of problems")
+    ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -95,6 +103,13 @@ def main():
               "the path shards by problem/patch across GPUs, which this build does not run.",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "mg":
+        return bench_mg(args)
+    if args.workload == "batch":
+        return bench_batch(args)
+    args.n = 64 if args.n is None else args.n
+    args.tol = 1e-10 if args.tol is None else args.tol
+    args.cpu_sample = 20 if args.cpu_sample is None else args.cpu_sample
 
     from gismo_amd import gsx, problems
 
@@ -204,6 +219,264 @@ def main():
     gp.close()
     ctx.close()
     print(json.dumps(out))
+
+
+def _common(args, metric_value, ms, dtype, workload, extra):
+    out = {
+        "metric": METRIC,
+        "value": metric_value,
+        "unit": "DOFs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (seeded geometry noise, coefficients and sources)",
+        "config": {"workload": workload, "parallelism": "single GPU"},
+    }
+    out["config"].update(extra.pop("config", {}))
+    out.update(extra)
+    return out
+
+
+def bench_mg(args):
+    """16 degree-3 patches tiling [0,4]^2, n elements per patch direction,
+    C0-glued; one step = assemble + multigrid setup (knot-coarsened
+    hierarchy, Galerkin levels, dense coarse inverse) + CG preconditioned by
+    V(2,2) with damped Jacobi (omega 0.6) to relres tol."""
+    from gismo_amd import gsx, problems
+    n = 256 if args.n is None else args.n
+    tol = 1e-8 if args.tol is None else args.tol
+    levels = int(round(np.log2(n))) - 3   # coarsest 16 elements per patch direction
+    prob = problems.square_grid(4, 4, n=n, p=args.p)
+    ctx = gsx.Context(0)
+    gp = gsx.Problem(ctx, prob)
+    ndofs = gp.nglued
+
+    # tight FP64 reference (outside the timing)
+    A, b = gp.assemble(gsx.PATTERN_FULL, gsx.FP64)
+    mg = gsx.Multigrid(gp, A, levels, 2, 2, 0.6)
+    xr = ctx.array(gp.nfree).zero()
+    it_ref, rel_ref, _ = gsx.mg_cg_solve(A, mg, b, xr, tol=1e-13, maxit=500)
+    xref = xr.download()
+    l2_ref = gp.l2_error(xr)
+    nnz = A.nnz
+    mg.close(); A.close(); b.close(); xr.close()
+
+    log(f"mg reference: {it_ref} iterations, relres {rel_ref:.2e}")
+    x = ctx.array(gp.nfree)
+    results = {}
+    for name, prec in (("fp64", gsx.FP64), ("mixed", gsx.MIXED)):
+        def step():
+            A, b = gp.assemble(gsx.PATTERN_FULL | gsx.ASM_REBUILD_PATTERN, prec)
+            mg = gsx.Multigrid(gp, A, levels, 2, 2, 0.6, precision=prec)
+            it, rel, ok = gsx.mg_cg_solve(A, mg, b, x, tol=tol, maxit=500)
+            ctx.synchronize()
+            mg.close(); A.close(); b.close()
+            return it, rel
+        for _ in range(args.warmup):
+            x.zero()
+            step()
+        times = []
+        for _ in range(args.steps):
+            x.zero()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            it, rel = step()
+            times.append(time.perf_counter() - t0)
+        log(f"mg {name}: {1e3 * np.mean(times):.1f} ms per step, {it} iterations")
+        xs = x.download()
+        results[name] = dict(ms=1e3 * float(np.mean(times)), ms_min=1e3 * float(np.min(times)), iters=it,
+                             relres=rel, rel_err=float(np.linalg.norm(xs - xref) / np.linalg.norm(xref)),
+                             l2=gp.l2_error(x), dofs_per_s=ndofs / float(np.mean(times)))
+    # phase split of one mixed step
+    ctx.enable_timing(True)
+    ctx.timing_filter(None)
+    ctx.reset_timing()
+    x.zero()
+    A, b = gp.assemble(gsx.PATTERN_FULL | gsx.ASM_REBUILD_PATTERN, gsx.MIXED)
+    mg = gsx.Multigrid(gp, A, levels, 2, 2, 0.6, precision=gsx.MIXED)
+    gsx.mg_cg_solve(A, mg, b, x, tol=tol, maxit=500)
+    ctx.synchronize()
+    phases = {}
+    for k in ("pattern_count", "pattern_scan", "pattern_fill", "elem2d", "gather", "mg_setup", "mg_smooth",
+              "mg_transfer", "mg_coarse", "cg_init", "spmv_f64", "cg_vec"):
+        nl, ms = ctx.kernel_stats(k)
+        if nl:
+            phases[k] = round(ms, 3)
+    ctx.enable_timing(False)
+    mg.close(); A.close(); b.close()
+    mixed = results["mixed"]
+    extra = {
+        "config": {"dofs": ndofs, "free_dofs": gp.nfree, "nnz": nnz, "levels": levels,
+                   "cycle": "V(2,2), damped Jacobi 0.6, exact FP64 coarse solve"},
+        "fp64": results["fp64"], "mixed": mixed,
+        "speedup_mixed_vs_fp64": results["fp64"]["ms"] / mixed["ms"],
+        "reference_solution": {"relres": rel_ref, "iters": it_ref, "l2_error_vs_exact": l2_ref},
+        "phase_ms_one_mixed_step": phases,
+    }
+    if not args.no_cpu:
+        extra["cpu_baseline"] = cpu_baseline_mg(16 if args.cpu_sample is None else args.cpu_sample, args.p)
+    out = _common(args, mixed["dofs_per_s"], mixed["ms"], "f32+f64",
+                  f"2D multipatch: 16 degree-{args.p} patches on [0,4]^2, {n}x{n} elements per patch, "
+                  f"assemble + MG-preconditioned CG to relres {tol:g}", extra)
+    x.close()
+    gp.close()
+    ctx.close()
+    print(json.dumps(out))
+
+
+def cpu_baseline_mg(n_sample, p):
+    """Oracle assemble + MG-PCG (V(2,2), Jacobi 0.6) of the 16-patch grid at a
+    reduced per-patch size."""
+    import oracle as O
+    from gismo_amd import problems
+    t0 = time.perf_counter()
+    levels = [problems.square_grid(4, 4, n=n_sample >> k, p=p) for k in range(2, -1, -1)]
+    maps = [O.mapper(l.patches, l.dir_sides, l.ifaces) for l in levels]
+    Af, bf = O.assemble(levels[-1].patches, maps[-1][0], maps[-1][1], 1)
+    Ps = [O.transfer_multipatch(levels[i].patches, maps[i][0], maps[i][1], maps[i + 1][0], maps[i + 1][1])
+          for i in range(len(levels) - 1)]
+    mg = O.Multigrid(Af, Ps, npre=2, npost=2, damping=0.6)
+    x, it, err = O.cg_solve(Af, bf, tol=1e-8, maxit=500, precond="mg", mg=mg)
+    dt = time.perf_counter() - t0
+    nb = sum(P.nbasis for P in levels[-1].patches)
+    return {"value": nb / dt, "unit": "DOFs/s", "cores": 1, "kind": "port",
+            "sample": f"oracle assemble + MG-PCG(1e-8) of the 16-patch grid at {n_sample}x{n_sample} "
+                      f"elements per patch, 3 levels ({dt:.1f} s, {it} iterations), 1 thread of {cpu_model()}"}
+
+
+def bench_batch(args):
+    """Independent single-patch problems (half 2D with 8..96, half 3D with
+    8..32 elements per direction, degree 2..4, +-0.1h noise, lognormal
+    kappa, random per-element sources).  One step = assembly of every
+    problem (disjoint patches grouped into a few multipatch assemblies up to
+    an element-matrix budget) + one batched Jacobi-CG over all of them."""
+    from gismo_amd import gsx, problems
+    tol = 1e-10 if args.tol is None else args.tol
+    probs = problems.batch_problems(count=args.count, seed=7)
+    ctx = gsx.Context(0)
+    budget = 24e9   # bytes of element matrices per assembly
+    groups, cur, cur_bytes = [], [], 0.0
+    for d in (2, 3):
+        for pr in [q for q in probs if q.d == d]:
+            P = pr.patches[0]
+            bytes_ = P.nelem * (P.p[0] + 1) ** (2 * d) * 8.0
+            if cur and (cur_bytes + bytes_ > budget or cur[0].d != d):
+                groups.append(cur)
+                cur, cur_bytes = [], 0.0
+            cur.append(pr)
+            cur_bytes += bytes_
+    if cur:
+        groups.append(cur)
+    gps = []
+    for g in groups:
+        joint = problems.Problem([q.patches[0] for q in g],
+                                 [(k, s) for k in range(len(g)) for s in range(1, 2 * g[0].d + 1)], [])
+        gps.append((g, gsx.Problem(ctx, joint)))
+    ndofs = sum(gp.nglued for _, gp in gps)
+    nfree = sum(gp.nfree for _, gp in gps)
+
+    def run(prec, tol_, xs, keep=False):
+        mats = []
+        B = gsx.Batch(ctx, prec, capacity=len(probs))
+        for (g, gp), xg in zip(gps, xs):
+            A, b = gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_REBUILD_PATTERN, prec)
+            mats.append((A, b))
+            off = 0
+            for q in g:
+                nf = int(np.prod([nb - 2 for nb in q.patches[0].nbasis_dir()]))
+                B.add(A, b.view(off, nf), xg.view(off, nf), row0=off, nrows=nf, tol=tol_, maxit=args.maxit)
+                off += nf
+            assert off == gp.nfree
+        B.run(8)
+        res = [B.result(i) for i in range(len(probs))]
+        ctx.synchronize()
+        B.close()
+        nnz = sum(A.nnz for A, _ in mats)
+        for A, b in mats:
+            A.close(); b.close()
+        return res, nnz
+
+    xs = [ctx.array(gp.nfree) for _, gp in gps]
+    for x in xs:
+        x.zero()
+    log(f"batch: {len(probs)} problems in {len(gps)} assemblies, {ndofs} DOFs")
+    ref_res, nnz = run(gsx.FP64, 1e-12, xs)
+    log(f"batch reference done: nnz {nnz}")
+    xref = np.concatenate([x.download() for x in xs])
+    results = {}
+    for name, prec in (("fp64", gsx.FP64), ("mixed", gsx.MIXED)):
+        for _ in range(args.warmup):
+            for x in xs:
+                x.zero()
+            run(prec, tol, xs)
+        times = []
+        for _ in range(args.steps):
+            for x in xs:
+                x.zero()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            res, _ = run(prec, tol, xs)
+            times.append(time.perf_counter() - t0)
+        log(f"batch {name}: {1e3 * np.mean(times):.1f} ms per step")
+        xall = np.concatenate([x.download() for x in xs])
+        # per-member relative error vs the tight FP64 solve
+        errs, off = [], 0
+        for g, gp in gps:
+            for q in g:
+                nf = int(np.prod([nb - 2 for nb in q.patches[0].nbasis_dir()]))
+                a, r = xall[off:off + nf], xref[off:off + nf]
+                errs.append(float(np.linalg.norm(a - r) / max(1e-300, np.linalg.norm(r))))
+                off += nf
+        its = [r[1] for r in res]
+        results[name] = dict(ms=1e3 * float(np.mean(times)), ms_min=1e3 * float(np.min(times)),
+                             max_rel_err=max(errs), converged=sum(r[0] == 1 for r in res),
+                             iters_max=max(its), iters_mean=float(np.mean(its)),
+                             dofs_per_s=ndofs / float(np.mean(times)))
+    mixed = results["mixed"]
+    extra = {"config": {"problems": len(probs), "dofs": ndofs, "free_dofs": nfree, "nnz": nnz,
+                        "assemblies": len(gps)},
+             "fp64": results["fp64"], "mixed": mixed,
+             "speedup_mixed_vs_fp64": results["fp64"]["ms"] / mixed["ms"],
+             "reference": {"tol": 1e-12, "converged": sum(r[0] == 1 for r in ref_res)}}
+    if not args.no_cpu:
+        extra["cpu_baseline"] = cpu_baseline_batch(probs, tol)
+    out = _common(args, mixed["dofs_per_s"], mixed["ms"], "f32+f64",
+                  f"{len(probs)} independent single-patch problems (2D 8..96, 3D 8..32 elements per "
+                  f"direction, p 2..4), assemble + batched Jacobi-CG to relres {tol:g}", extra)
+    for x in xs:
+        x.close()
+    for _, gp in gps:
+        gp.close()
+    ctx.close()
+    print(json.dumps(out))
+
+
+def cpu_baseline_batch(probs, tol, budget_s=15.0):
+    """Oracle assemble + Jacobi-CG of the batch's problems one after another
+    (the reference's way), until budget_s seconds: DOFs/s of that sample."""
+    import oracle as O
+    t0 = time.perf_counter()
+    dofs = 0
+    k = 0
+    order = sorted(range(len(probs)), key=lambda i: probs[i].patches[0].nbasis)
+    for i in order[::7]:
+        pr = probs[i]
+        dm, nf = O.mapper(pr.patches, pr.dir_sides, pr.ifaces)
+        A, b = O.assemble(pr.patches, dm, nf, 0)
+        O.cg_solve(A, b, tol=tol, maxit=5000)
+        dofs += pr.patches[0].nbasis
+        k += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": dofs / dt, "unit": "DOFs/s", "cores": 1, "kind": "port",
+            "sample": f"oracle assemble + Jacobi-CG({tol:g}) of {k} of the batch's problems "
+                      f"(every 7th by size, {dofs} DOFs, {dt:.1f} s), 1 thread of {cpu_model()}"}
 
 
 if __name__ == "__main__":

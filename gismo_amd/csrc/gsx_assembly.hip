@@ -121,7 +121,7 @@ __global__ __launch_bounds__(128) void k_elem2d(const PatchDev *__restrict__ pat
                                                 const int64_t *__restrict__ gprefix, int ng,
                                                 GaussTab gt, double *__restrict__ Kbuf,
                                                 double *__restrict__ Fbuf,
-                                                double *__restrict__ detmin)
+                                                double *__restrict__ detmin, ElemRange rg)
 {
     constexpr int NB1 = P + 1, NB = NB1 * NB1, NQ1 = P + 1, NQ = NQ1 * NQ1;
     __shared__ double sV[2][NQ1][NB1], sD[2][NQ1][NB1], sW[2][NQ1];
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(128) void k_elem2d(const PatchDev *__restrict__ pat
     __shared__ double sG[NQ][2][NB];
     __shared__ double sDet[NQ];
 
-    const int64_t E = blockIdx.x;
+    const int64_t E = rg.e0 + blockIdx.x;
     const int gi = find_patch(gprefix, ng, E);
     const PatchDev &pd = patches[gpatch[gi]];
     const int64_t e = E - gprefix[gi];
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(128) void k_elem2d(const PatchDev *__restrict__ pat
     }
     __syncthreads();
 
-    double *Ke = Kbuf + pd.kmat_off + e * (int64_t)(NB * NB);
+    double *Ke = Kbuf + pd.kmat_off + e * (int64_t)(NB * NB) - rg.kshift;
     for (int idx = t; idx < NB * NB; idx += blockDim.x) {
         const int i = idx / NB, j = idx % NB;
         if (i > j) continue;
@@ -209,7 +209,7 @@ __global__ __launch_bounds__(128) void k_elem2d(const PatchDev *__restrict__ pat
             const double v = sV[0][qx][bx] * sV[1][qy][by];
             acc += sWq[q] * ((v * sF[q]) * sMeas[q]);
         }
-        Fbuf[pd.fvec_off + e * NB + t] = acc;
+        Fbuf[pd.fvec_off + e * NB + t - rg.fshift] = acc;
     }
     if (t == 0) {
         double m = sDet[0];
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256) void k_elem3d(const PatchDev *__restrict__ pat
                                                 const int64_t *__restrict__ gprefix, int ng,
                                                 GaussTab gt, double *__restrict__ Kbuf,
                                                 double *__restrict__ Fbuf,
-                                                double *__restrict__ detmin)
+                                                double *__restrict__ detmin, ElemRange rg)
 {
     using C = Elem3Cfg<P>;
     constexpr int NB1 = C::NB1, NB = C::NB, NQ1 = C::NQ1, NQ = C::NQ, S = C::S;
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void k_elem3d(const PatchDev *__restrict__ pat
     __shared__ double sWq[NQ], sF[NQ], sMeas[NQ], sDet[NQ];
     __shared__ double sT1[NB1 * NB1][NQ1 * NQ1];
 
-    const int64_t E = blockIdx.x;
+    const int64_t E = rg.e0 + blockIdx.x;
     const int gi = find_patch(gprefix, ng, E);
     const PatchDev &pd = patches[gpatch[gi]];
     const int64_t e = E - gprefix[gi];
@@ -358,7 +358,7 @@ __global__ __launch_bounds__(256) void k_elem3d(const PatchDev *__restrict__ pat
             const double v = sV[0][qx][bx] * sV[1][qy][by] * sV[2][qz][bz];
             acc += sWq[q] * ((v * sF[q]) * sMeas[q]);
         }
-        Fbuf[pd.fvec_off + e * NB + t] = acc;
+        Fbuf[pd.fvec_off + e * NB + t - rg.fshift] = acc;
     }
     if (t == 0) {
         double m = sDet[0];
@@ -429,7 +429,7 @@ __global__ __launch_bounds__(256) void k_elem3d(const PatchDev *__restrict__ pat
         }
     }
 
-    double *Ke = Kbuf + pd.kmat_off + e * (int64_t)(NB * NB);
+    double *Ke = Kbuf + pd.kmat_off + e * (int64_t)(NB * NB) - rg.kshift;
 #pragma unroll
     for (int mc = 0; mc < MC; ++mc) {
         const int c = t + 256 * mc;
@@ -707,7 +707,7 @@ struct PatchBox {
 };
 
 template <int D, int P>
-__global__ __launch_bounds__(256) void k_gather_patch(PatchBox bx, int nfree,
+__global__ __launch_bounds__(256) void k_gather_patch(PatchBox bx, int nfree, int r0, int r1, int64_t elbase,
                                                       const int *__restrict__ f2g,
                                                       const int *__restrict__ rowptr,
                                                       const double *__restrict__ Kbuf,
@@ -725,8 +725,8 @@ __global__ __launch_bounds__(256) void k_gather_patch(PatchBox bx, int nfree,
     __shared__ double sAcc[4][NS];
     __shared__ unsigned char sNz[4][NS];
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int r = blockIdx.x * 4 + w;
-    if (r >= nfree) return;
+    const int r = r0 + blockIdx.x * 4 + w;
+    if (r >= r1) return;
     for (int k = lane; k < NS; k += kWave) {
         sAcc[w][k] = 0.0;
         sNz[w][k] = 0;
@@ -767,7 +767,7 @@ __global__ __launch_bounds__(256) void k_gather_patch(PatchBox bx, int nfree,
                     bool fr = jx >= bx.flo[0] && jx <= bx.fhi[0] && jy >= bx.flo[1] && jy <= bx.fhi[1];
                     if (D == 3) fr = fr && jz >= bx.flo[2] && jz <= bx.fhi[2];
                     if (fr) {
-                        const int64_t el = ex + (int64_t)bx.nel[0] * (ey + (int64_t)bx.nel[1] * ez);
+                        const int64_t el = ex + (int64_t)bx.nel[0] * (ey + (int64_t)bx.nel[1] * ez) - elbase;
                         const int la = (ic[0] - ex) + NB1 * ((ic[1] - ey) + NB1 * (ic[2] - ez));
                         v[u] = Kbuf[el * (NB * NB) + (int64_t)la * NB + b];
                         slot[u] = (D == 3) ? ((jz - ic[2] + P) * W + (jy - ic[1] + P)) * W + (jx - ic[0] + P)
@@ -789,7 +789,7 @@ __global__ __launch_bounds__(256) void k_gather_patch(PatchBox bx, int nfree,
         for (int u = 0; u < U && t0 + u < ne; ++u) {
             const int t = t0 + u;
             const int ex = elo[0] + t % ne0, ey = elo[1] + (t / ne0) % ne1, ez = elo[2] + t / (ne0 * ne1);
-            const int64_t el = ex + (int64_t)bx.nel[0] * (ey + (int64_t)bx.nel[1] * ez);
+            const int64_t el = ex + (int64_t)bx.nel[0] * (ey + (int64_t)bx.nel[1] * ez) - elbase;
             const int la = (ic[0] - ex) + NB1 * ((ic[1] - ey) + NB1 * (ic[2] - ez));
             racc += Fbuf[el * NB + la];
         }
@@ -872,12 +872,12 @@ hipError_t launch_rowfill_patch(int d, int p, const int *flo, const int *fhi, co
 }
 
 hipError_t launch_gather_patch(int d, int p, const int *flo, const int *fhi, const int *nb,
-                               const int *nel, const int *padded, int nfree, const int *f2g,
-                               const int *rowptr, const double *Kbuf, const double *Fbuf,
+                               const int *nel, const int *padded, int nfree, int r0, int r1, int64_t elbase,
+                               const int *f2g, const int *rowptr, const double *Kbuf, const double *Fbuf,
                                double *val, double *rhs, unsigned char *keep, double *s64,
                                float *s32, hipStream_t s)
 {
-    if (nfree <= 0) return hipSuccess;
+    if (r1 <= r0) return hipSuccess;
     PatchBox bx;
     for (int k = 0; k < 3; ++k) {
         bx.flo[k] = flo[k];
@@ -886,8 +886,8 @@ hipError_t launch_gather_patch(int d, int p, const int *flo, const int *fhi, con
         bx.nel[k] = nel[k];
         bx.padded[k] = padded[k];
     }
-    const dim3 grid((nfree + 3) / 4), blk(256);
-#define GSX_G(D, P) if (d == D && p == P) { hipLaunchKernelGGL((k_gather_patch<D, P>), grid, blk, 0, s, bx, nfree, f2g, rowptr, Kbuf, Fbuf, val, rhs, keep, s64, s32); return hipGetLastError(); }
+    const dim3 grid((r1 - r0 + 3) / 4), blk(256);
+#define GSX_G(D, P) if (d == D && p == P) { hipLaunchKernelGGL((k_gather_patch<D, P>), grid, blk, 0, s, bx, nfree, r0, r1, elbase, f2g, rowptr, Kbuf, Fbuf, val, rhs, keep, s64, s32); return hipGetLastError(); }
     GSX_G(2, 1) GSX_G(2, 2) GSX_G(2, 3) GSX_G(2, 4)
     GSX_G(3, 1) GSX_G(3, 2) GSX_G(3, 3) GSX_G(3, 4)
 #undef GSX_G
@@ -1052,10 +1052,23 @@ hipError_t launch_elem(int d, int p, const PatchDev *patches, const int *gpatch,
                        const int64_t *gprefix, int ng, int64_t nelem, const GaussTab &gt,
                        double *Kbuf, double *Fbuf, double *detmin, hipStream_t s)
 {
+    return launch_elem_range(d, p, patches, gpatch, gprefix, ng, 0, nelem, gt, Kbuf, Fbuf, detmin, s);
+}
+
+hipError_t launch_elem_range(int d, int p, const PatchDev *patches, const int *gpatch,
+                             const int64_t *gprefix, int ng, int64_t e0, int64_t e1, const GaussTab &gt,
+                             double *Kbuf, double *Fbuf, double *detmin, hipStream_t s)
+{
+    const int64_t nelem = e1 - e0;
     if (nelem <= 0) return hipSuccess;
+    const int nb = d == 3 ? (p + 1) * (p + 1) * (p + 1) : (p + 1) * (p + 1);
+    ElemRange rg;
+    rg.e0 = e0;
+    rg.kshift = e0 * (int64_t)nb * nb;
+    rg.fshift = e0 * (int64_t)nb;
     const dim3 grid((unsigned)nelem);
-#define GSX_E2(P) case P: hipLaunchKernelGGL(k_elem2d<P>, grid, dim3(128), 0, s, patches, gpatch, gprefix, ng, gt, Kbuf, Fbuf, detmin); break;
-#define GSX_E3(P) case P: hipLaunchKernelGGL(k_elem3d<P>, grid, dim3(256), 0, s, patches, gpatch, gprefix, ng, gt, Kbuf, Fbuf, detmin); break;
+#define GSX_E2(P) case P: hipLaunchKernelGGL(k_elem2d<P>, grid, dim3(128), 0, s, patches, gpatch, gprefix, ng, gt, Kbuf, Fbuf, detmin, rg); break;
+#define GSX_E3(P) case P: hipLaunchKernelGGL(k_elem3d<P>, grid, dim3(256), 0, s, patches, gpatch, gprefix, ng, gt, Kbuf, Fbuf, detmin, rg); break;
     if (d == 2) {
         switch (p) { GSX_E2(1) GSX_E2(2) GSX_E2(3) GSX_E2(4) default: return hipErrorInvalidValue; }
     } else {

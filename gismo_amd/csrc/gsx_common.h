@@ -60,6 +60,15 @@ struct CGState {
     int need_update;  // mixed: a reliable update is due
 };
 
+// Element range of one element-kernel launch: elements [e0, e0 + grid) of
+// the launch's patch group; element matrices and load vectors are written
+// shifted down by kshift / fshift doubles (a slab of a larger patch).
+struct ElemRange {
+    int64_t e0;
+    int64_t kshift;
+    int64_t fshift;
+};
+
 // One member of a batched CG solve: rows [row0, row0 + nrows) of a CSR
 // matrix whose columns stay inside that range.  Vector pointers are bases
 // indexed by row/column number (the member's own arrays shifted by -row0).

@@ -9,6 +9,10 @@ namespace gsx {
 hipError_t launch_elem(int d, int p, const PatchDev *patches, const int *gpatch,
                        const int64_t *gprefix, int ng, int64_t nelem, const GaussTab &gt,
                        double *Kbuf, double *Fbuf, double *detmin, hipStream_t s);
+// elements [e0, e1) of the group, written from Kbuf/Fbuf offset 0
+hipError_t launch_elem_range(int d, int p, const PatchDev *patches, const int *gpatch,
+                             const int64_t *gprefix, int ng, int64_t e0, int64_t e1, const GaussTab &gt,
+                             double *Kbuf, double *Fbuf, double *detmin, hipStream_t s);
 hipError_t launch_l2err(int d, int p, const PatchDev *patches, const int *gpatch,
                         const int64_t *gprefix, int ng, int64_t nelem, const GaussTab &gt,
                         const double *sol, int nfree, double *out, hipStream_t s);
@@ -28,9 +32,11 @@ hipError_t launch_gather(const PatchDev *patches, const int *occ_ptr, const int 
 hipError_t launch_rowfill_patch(int d, int p, const int *flo, const int *fhi, const int *nb,
                                 const int *nel, const int *padded, int nfree, const int *f2g,
                                 const int *rowptr, int *col, hipStream_t s);
+// rows [r0, r1) of the free box; Kbuf/Fbuf hold the elements from global
+// element id elbase on
 hipError_t launch_gather_patch(int d, int p, const int *flo, const int *fhi, const int *nb,
-                               const int *nel, const int *padded, int nfree, const int *f2g,
-                               const int *rowptr, const double *Kbuf, const double *Fbuf,
+                               const int *nel, const int *padded, int nfree, int r0, int r1, int64_t elbase,
+                               const int *f2g, const int *rowptr, const double *Kbuf, const double *Fbuf,
                                double *val, double *rhs, unsigned char *keep, double *s64,
                                float *s32, hipStream_t s);
 hipError_t launch_keep(const int *rowptr, const int *col, const double *val,

@@ -68,6 +68,7 @@ struct gsx_ctx {
     std::map<std::string, std::pair<int64_t, double>> stats;
     std::shared_ptr<Counter> mem = std::make_shared<Counter>();
     GaussTab gt;
+    int64_t elem_budget = int64_t(16) << 30;   // bytes of element matrices per assembly pass
 };
 
 namespace {
@@ -255,7 +256,7 @@ struct gsx_problem {
     int grid_dims[3] = {1, 1, 1};
     int box_lo[3] = {0, 0, 0}, box_hi[3] = {0, 0, 0};   // free functions per direction
     // element scratch
-    DevPtr d_K, d_F, d_detmin, d_scan_bsum, d_scan_total;
+    DevPtr d_detmin, d_scan_bsum, d_scan_total;
     int64_t kdoubles = 0, fdoubles = 0;
     double min_det = 0.0;
 };
@@ -440,6 +441,13 @@ gsx_status gsx_ctx_create(int32_t device, gsx_ctx **out)
     std::memset(&c->gt, 0, sizeof(c->gt));
     for (int q = 1; q <= gsx::kMaxQ1; ++q) gauss_rule(q, c->gt.x[q], c->gt.w[q]);
     *out = c;
+    return GSX_OK;
+}
+
+gsx_status gsx_ctx_set_elem_budget(gsx_ctx *c, int64_t bytes)
+{
+    if (!c || bytes <= 0) return GSX_ERR_ARG;
+    c->elem_budget = bytes;
     return GSX_OK;
 }
 
@@ -926,21 +934,49 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
     }
     if (gsx::gather_maxrow_bucket(P->maxrow) < 0)
         return fail(c, GSX_ERR_ARG, "rows longer than 1024 entries are not supported");
-    if (!P->d_K) {
-        P->d_K = dalloc(c, sizeof(double) * std::max<int64_t>(1, P->kdoubles));
-        P->d_F = dalloc(c, sizeof(double) * std::max<int64_t>(1, P->fdoubles));
+    const bool single = P->npatch == 1 && P->d_f2g && P->nfree > 0;
+    if (!P->d_detmin) {
         P->d_detmin = dalloc(c, sizeof(double) * (std::max<int64_t>(1, P->nelem) + 260));
-        if (!P->d_K || !P->d_F || !P->d_detmin)
-            return fail(c, GSX_ERR_OOM, "device allocation (element matrices)");
+        if (!P->d_detmin) return fail(c, GSX_ERR_OOM, "device allocation (Jacobian check)");
     }
-    for (auto &g : P->groups) {
-        KScope k(c, g.d == 3 ? "elem3d" : "elem2d");
-        HIP_OK(c, gsx::launch_elem(g.d, g.p, dp<PatchDev>(P->d_patches), dp<int>(g.d_gpatch),
-                                   dp<int64_t>(g.d_gprefix), (int)g.patches.size(), g.nelem, c->gt,
-                                   dp<double>(P->d_K), dp<double>(P->d_F), dp<double>(P->d_detmin), s));
+    // Element matrices live only during the assembly, in a buffer of at most
+    // the context's budget: a single patch is assembled in slabs of element
+    // layers along its slowest direction (each slab recomputes the p element
+    // layers its first rows share with the previous slab, so every entry is
+    // still summed over all its elements in element order); several patches
+    // are assembled at once and must fit the budget.
+    const HostPatch &h0 = P->hp[0];
+    const int sd = h0.d - 1;   // slowest direction
+    int64_t layer_elems = 1;
+    for (int k = 0; k < sd; ++k) layer_elems *= h0.nel[k];
+    const int nb0 = h0.d == 3 ? (h0.p[0] + 1) * (h0.p[0] + 1) * (h0.p[0] + 1) : (h0.p[0] + 1) * (h0.p[0] + 1);
+    int slab_rows = 0;   // row layers per slab (single patch)
+    if (single) {
+        const int64_t layer_bytes = layer_elems * ((int64_t)nb0 * nb0 + nb0) * (int64_t)sizeof(double);
+        const int64_t fit = c->elem_budget / std::max<int64_t>(1, layer_bytes);
+        const int free_layers = P->box_hi[sd] - P->box_lo[sd] + 1;
+        slab_rows = (int)std::min<int64_t>(free_layers, fit - h0.p[0]);
+        if (slab_rows < 1)
+            return fail(c, GSX_ERR_BUDGET, "one slab of element matrices exceeds the element budget");
+    } else if ((P->kdoubles + P->fdoubles) * (int64_t)sizeof(double) > c->elem_budget) {
+        return fail(c, GSX_ERR_BUDGET, "element matrices of a multipatch assembly exceed the element budget "
+                                       "(gsx_ctx_set_elem_budget)");
     }
-    double *dmin_slot = dp<double>(P->d_detmin) + std::max<int64_t>(1, P->nelem);
-    HIP_OK(c, gsx::launch_min(dp<double>(P->d_detmin), P->nelem, dmin_slot, s));
+    const int64_t kcap = single ? std::min<int64_t>(h0.nel[sd], slab_rows + h0.p[0]) * layer_elems * nb0 * nb0
+                                : P->kdoubles;
+    const int64_t fcap = single ? std::min<int64_t>(h0.nel[sd], slab_rows + h0.p[0]) * layer_elems * nb0
+                                : P->fdoubles;
+    DevPtr Kbuf = dalloc(c, sizeof(double) * std::max<int64_t>(1, kcap));
+    DevPtr Fbuf = dalloc(c, sizeof(double) * std::max<int64_t>(1, fcap));
+    if (!Kbuf || !Fbuf) return fail(c, GSX_ERR_OOM, "device allocation (element matrices)");
+    if (!single) {
+        for (auto &g : P->groups) {
+            KScope k(c, g.d == 3 ? "elem3d" : "elem2d");
+            HIP_OK(c, gsx::launch_elem(g.d, g.p, dp<PatchDev>(P->d_patches), dp<int>(g.d_gpatch),
+                                       dp<int64_t>(g.d_gprefix), (int)g.patches.size(), g.nelem, c->gt,
+                                       dp<double>(Kbuf), dp<double>(Fbuf), dp<double>(P->d_detmin), s));
+        }
+    }
 
     auto A = std::make_unique<gsx_matrix>();
     A->ctx = c;
@@ -959,7 +995,6 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
     if (mode == GSX_PATTERN_EXPR) keep = dalloc(c, std::max<int64_t>(1, P->nnz));
     if (!A->val || !rhs || (mode == GSX_PATTERN_EXPR && !keep))
         return fail(c, GSX_ERR_OOM, "device allocation (values)");
-    const bool single = P->npatch == 1 && P->d_f2g && P->nfree > 0;
     if (single) {
         // structured operator filled by the same gather
         StencilOp &S = A->st;
@@ -980,20 +1015,42 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
             return fail(c, GSX_ERR_OOM, "device allocation (structured operator)");
         S.s64_ready = true;
         S.s32_ready = precision == GSX_MIXED;
-        KScope k(c, "gather");
-        HIP_OK(c, gsx::launch_gather_patch(h.d, h.p[0], P->box_lo, P->box_hi, h.nb, h.nel, P->grid_dims,
-                                           P->nfree, dp<int>(P->d_f2g), dp<int>(P->d_rowptr),
-                                           dp<double>(P->d_K), dp<double>(P->d_F), dp<double>(A->val), rhs,
-                                           keep ? dp<unsigned char>(keep) : nullptr, dp<double>(S.s64),
-                                           S.s32 ? dp<float>(S.s32) : nullptr, s));
+        // free rows are lexicographic over the free box: a range of row
+        // layers along the slowest direction is a range of rows
+        int64_t plane = 1;
+        for (int k = 0; k < sd; ++k) plane *= P->box_hi[k] - P->box_lo[k] + 1;
+        const Group &g = P->groups[0];
+        for (int L0 = P->box_lo[sd]; L0 <= P->box_hi[sd]; L0 += slab_rows) {
+            const int L1 = std::min(L0 + slab_rows, P->box_hi[sd] + 1);
+            const int64_t E0 = std::max(0, L0 - h.p[0]), E1 = std::min(h.nel[sd], L1);
+            {
+                KScope k(c, h.d == 3 ? "elem3d" : "elem2d");
+                HIP_OK(c, gsx::launch_elem_range(h.d, h.p[0], dp<PatchDev>(P->d_patches), dp<int>(g.d_gpatch),
+                                                 dp<int64_t>(g.d_gprefix), 1, E0 * layer_elems,
+                                                 E1 * layer_elems, c->gt, dp<double>(Kbuf), dp<double>(Fbuf),
+                                                 dp<double>(P->d_detmin), s));
+            }
+            KScope k(c, "gather");
+            const int r0 = (int)((L0 - P->box_lo[sd]) * plane), r1 = (int)((L1 - P->box_lo[sd]) * plane);
+            HIP_OK(c, gsx::launch_gather_patch(h.d, h.p[0], P->box_lo, P->box_hi, h.nb, h.nel, P->grid_dims,
+                                               P->nfree, r0, r1, E0 * layer_elems, dp<int>(P->d_f2g),
+                                               dp<int>(P->d_rowptr), dp<double>(Kbuf), dp<double>(Fbuf),
+                                               dp<double>(A->val), rhs, keep ? dp<unsigned char>(keep) : nullptr,
+                                               dp<double>(S.s64), S.s32 ? dp<float>(S.s32) : nullptr, s));
+        }
     } else {
         KScope k(c, "gather");
         HIP_OK(c, gsx::launch_gather(dp<PatchDev>(P->d_patches), dp<int>(P->d_occ_ptr),
                                      dp<int>(P->d_occ_patch), dp<int64_t>(P->d_occ_local), P->nfree,
-                                     dp<int>(P->d_rowptr), dp<int>(P->d_col), dp<double>(P->d_K),
-                                     dp<double>(P->d_F), dp<double>(A->val), rhs,
+                                     dp<int>(P->d_rowptr), dp<int>(P->d_col), dp<double>(Kbuf),
+                                     dp<double>(Fbuf), dp<double>(A->val), rhs,
                                      keep ? dp<unsigned char>(keep) : nullptr, P->maxrow, s));
     }
+    // element matrices go back to the cache once the gathers have run
+    Kbuf.reset();
+    Fbuf.reset();
+    double *dmin_slot = dp<double>(P->d_detmin) + std::max<int64_t>(1, P->nelem);
+    HIP_OK(c, gsx::launch_min(dp<double>(P->d_detmin), P->nelem, dmin_slot, s));
     double mind = 0.0;
     HIP_OK(c, hipMemcpyAsync(&mind, dmin_slot, sizeof(double), hipMemcpyDeviceToHost, s));
     if (mode == GSX_PATTERN_EXPR) {

@@ -86,6 +86,7 @@ SIGNATURES = {
     "gsx_copy_to_device": [_vp, _vp, _vp, C.c_int64],
     "gsx_copy_to_host": [_vp, _vp, _vp, C.c_int64],
     "gsx_ctx_memory": [_vp, _i64p],
+    "gsx_ctx_set_elem_budget": [_vp, C.c_int64],
     "gsx_problem_create": [_vp, C.c_int32, C.POINTER(PatchDesc), C.c_int32, _i32p, C.c_int32,
                            _i32p, _pp],
     "gsx_problem_destroy": [_vp],
@@ -247,6 +248,11 @@ class Context:
         ms = C.c_double()
         _check(self.h, lib().gsx_ctx_kernel_stats(self.h, name.encode(), C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+    def set_elem_budget(self, nbytes):
+        """Bytes of element matrices per assembly pass (single patches larger
+        than this are assembled in slabs)."""
+        _check(self.h, lib().gsx_ctx_set_elem_budget(self.h, int(nbytes)))
 
     def memory(self):
         b = C.c_int64()

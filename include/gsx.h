@@ -95,6 +95,12 @@ gsx_status gsx_copy_to_device(gsx_ctx *ctx, void *dev, const void *host, int64_t
 gsx_status gsx_copy_to_host(gsx_ctx *ctx, void *host, const void *dev, int64_t bytes);
 /* Bytes of device memory currently held by the context. */
 gsx_status gsx_ctx_memory(gsx_ctx *ctx, int64_t *bytes_in_use);
+/* Device bytes the element matrices of one assembly pass may occupy
+ * (default 16 GiB).  A single patch larger than this is assembled in slabs
+ * of element layers (the result is identical); a multipatch assembly that
+ * does not fit fails with GSX_ERR_BUDGET.  No reference counterpart (the
+ * reference assembles element by element on the host). */
+gsx_status gsx_ctx_set_elem_budget(gsx_ctx *ctx, int64_t bytes);
 
 /* ---- host-only building blocks (no device needed) ---------------------- */
 /* Gauss-Legendre rule with n nodes on [-1,1], ascending: the values of

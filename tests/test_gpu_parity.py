@@ -203,3 +203,65 @@ def test_structured_partial_dirichlet(ctx):
         assert ok
         xs = x.download()
         assert np.linalg.norm(xs - xo) <= 1e-7 * np.linalg.norm(xo)
+
+
+def _noisy_patch_problem(d, p, n, seed):
+    P = problems.box_patch(d, p, n)
+    problems.perturb_interior(P, 0.15 / n, seed)
+    P.kappa = problems.lognormal_kappa(P.nelem, 0.5, seed)
+    return problems.Problem([P], problems.all_sides(d), [], name=f"noisy-d{d}-p{p}-n{n}")
+
+
+@pytest.mark.parametrize("d,p,n", [(2, 3, 24), (3, 2, 10), (3, 4, 6)])
+def test_slab_assembly_is_identical(ctx, d, p, n):
+    """A single patch assembled in slabs of element layers (element budget
+    smaller than the patch) gives bitwise the same matrix, load vector and
+    structured operator as in one pass."""
+    prob = _noisy_patch_problem(d, p, n, seed=40 + d + p)
+    gp = gsx.Problem(ctx, prob)
+    A1, b1 = gp.assemble(gsx.PATTERN_EXPR, gsx.MIXED)
+    nel_layer = n ** (d - 1)
+    nb = (p + 1) ** d
+    layer_bytes = nel_layer * (nb * nb + nb) * 8
+    try:
+        ctx.set_elem_budget(layer_bytes * (p + 2))   # slabs of 2 row layers
+        A2, b2 = gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_REBUILD_PATTERN, gsx.MIXED)
+        with pytest.raises(gsx.GsxError):
+            ctx.set_elem_budget(layer_bytes * p)      # not even one row layer fits
+            gp.assemble(gsx.PATTERN_EXPR, gsx.FP64)
+    finally:
+        ctx.set_elem_budget(16 << 30)
+    r1, c1, v1 = A1.download()
+    r2, c2, v2 = A2.download()
+    assert np.array_equal(r1, r2) and np.array_equal(c1, c2) and np.array_equal(v1, v2)
+    assert np.array_equal(b1.download(), b2.download())
+    # the structured operators agree too: same CG iterates
+    xs = []
+    for A, b in ((A1, b1), (A2, b2)):
+        x = ctx.array(gp.nfree).zero()
+        it, rel, ok = A.cg_solve(b, x, tol=1e-10, maxit=5000, precision=gsx.MIXED)
+        assert ok
+        xs.append((it, x.download()))
+        x.close()
+    assert xs[0][0] == xs[1][0] and np.array_equal(xs[0][1], xs[1][1])
+    # and the oracle
+    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    Ao, bo = O.assemble(prob.patches, dofs, nfree, 0)
+    assert np.array_equal(r2, Ao.indptr) and np.array_equal(c2, Ao.indices)
+    assert np.max(np.abs(v2 - Ao.data)) <= 1e-12 * np.abs(Ao.data).max()
+    for a in (A1, A2, b1, b2):
+        a.close()
+    gp.close()
+
+
+def test_multipatch_over_element_budget_is_refused(ctx):
+    prob = problems.square_grid(2, 2, n=8, p=3)
+    gp = gsx.Problem(ctx, prob)
+    try:
+        ctx.set_elem_budget(1 << 16)   # the 4 patches need 0.5 MiB
+        with pytest.raises(gsx.GsxError) as e:
+            gp.assemble(gsx.PATTERN_EXPR, gsx.FP64)
+        assert e.value.status == gsx.GSX_ERR_BUDGET
+    finally:
+        ctx.set_elem_budget(16 << 30)
+    gp.close()
