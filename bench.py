@@ -87,9 +87,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["cube", "mg", "batch"], default="cube",
+    ap.add_argument("--workload", choices=["cube", "mg", "batch", "serve"], default="cube",
                     help="cube: single-patch 3D assemble + Jacobi-CG (default); mg: 16-patch 2D "
-                         "assemble + multigrid-preconditioned CG; batch: 512 independent problems")
+                         "assemble + multigrid-preconditioned CG; batch: 512 independent problems; "
+                         "serve: Poisson-arrival request stream through the continuous-batching server")
+    ap.add_argument("--load", type=float, default=0.7, help="serve: arrival rate as a fraction of capacity")
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--p", type=int, default=3)
     ap.add_argument("--tol", type=float, default=None)
@@ -107,6 +109,8 @@ def main():
         return bench_mg(args)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     args.n = 64 if args.n is None else args.n
     args.tol = 1e-10 if args.tol is None else args.tol
     args.cpu_sample = 20 if args.cpu_sample is None else args.cpu_sample
@@ -477,6 +481,106 @@ def cpu_baseline_batch(probs, tol, budget_s=15.0):
     return {"value": dofs / dt, "unit": "DOFs/s", "cores": 1, "kind": "port",
             "sample": f"oracle assemble + Jacobi-CG({tol:g}) of {k} of the batch's problems "
                       f"(every 7th by size, {dofs} DOFs, {dt:.1f} s), 1 thread of {cpu_model()}"}
+
+
+def bench_serve(args):
+    """Stream of requests (single-patch 2D/3D problems of 1e3..2e6 DOFs,
+    log-uniform, degree 2..4, 1..8 right-hand sides) with Poisson arrivals
+    at `load` x the measured capacity, through the continuous-batching
+    server (mixed precision; FP64 run beside it).  Capacity is measured
+    first with a closed burst of warm-up requests."""
+    from gismo_amd import gsx, serving
+    tol = 1e-10 if args.tol is None else args.tol
+    nwarm = max(20, args.count // 10)
+    reqs = serving.make_requests(nwarm + args.count, seed=20261202)
+    for r in reqs:
+        r.tol = tol
+    warm, main_reqs = reqs[:nwarm], reqs[nwarm:]
+    log(f"serve: {len(reqs)} requests generated")
+    # sampled requests keep their solutions for the error check
+    check = [r for r in main_reqs if r.problem.patches[0].nbasis <= 40000][:12]
+    for r in check:
+        r.keep = True
+
+    def run(prec, rate, rs):
+        srv = serving.Server(precision=prec, iters_per_step=8)
+        t0 = srv.now()
+        if rate is None:
+            for r in rs:
+                r.arrival = t0
+                srv.submit(r)
+        else:
+            gaps = np.random.default_rng(5).exponential(1.0 / rate, len(rs))
+            arr = t0 + np.cumsum(gaps)
+            i = 0
+            while i < len(rs) or not srv.idle():
+                now = srv.now()
+                while i < len(rs) and arr[i] <= now:
+                    rs[i].arrival = float(arr[i])
+                    srv.submit(rs[i])
+                    i += 1
+                if srv.poll() == 0 and srv.idle():
+                    time.sleep(0.0005)
+        srv.drain()
+        res = srv.results
+        srv.close()
+        return res
+
+    out_runs = {}
+    for name, prec in (("mixed", gsx.MIXED), ("fp64", gsx.FP64)):
+        res = run(prec, None, warm)
+        span = max(r.done for r in res) - min(r.arrival for r in res)
+        work = sum(r.dofs * r.nrhs for r in res)
+        cap_rps = len(res) / span
+        rate = args.load * cap_rps
+        log(f"serve {name}: capacity {cap_rps:.1f} requests/s ({work / span / 1e6:.1f} M DOF-solves/s)")
+        res = run(prec, rate, main_reqs)
+        lat = np.array([r.latency for r in res])
+        span = max(r.done for r in res) - min(r.arrival for r in res)
+        work = sum(r.dofs * r.nrhs for r in res)
+        ok = all(all(s == 1 for s in r.status) for r in res)
+        out_runs[name] = dict(requests=len(res), rate_rps=rate, capacity_rps=cap_rps,
+                              p50_ms=1e3 * float(np.percentile(lat, 50)),
+                              p99_ms=1e3 * float(np.percentile(lat, 99)),
+                              mean_ms=1e3 * float(lat.mean()), dofs_per_s=work / span,
+                              requests_per_s=len(res) / span, all_converged=ok,
+                              span_s=span, results={r.rid: r for r in res if r.solutions is not None})
+        log(f"serve {name}: p50 {out_runs[name]['p50_ms']:.1f} ms, p99 {out_runs[name]['p99_ms']:.1f} ms, "
+            f"{work / span / 1e6:.1f} M DOF-solves/s")
+    # error of the sampled mixed results vs tight FP64 solves, and the
+    # sampled results of both runs against each other
+    ctx = gsx.Context(0)
+    errs = []
+    for r in check:
+        rm = out_runs["mixed"]["results"][r.rid]
+        gp = gsx.Problem(ctx, r.problem)
+        A, b0 = gp.assemble(gsx.PATTERN_EXPR, gsx.FP64)
+        for k, seed in enumerate(r.rhs_seeds):
+            b = b0 if k == 0 else ctx.to_device(serving.rhs_vector(gp.nfree, seed))
+            x = ctx.array(gp.nfree).zero()
+            A.cg_solve(b, x, tol=1e-13, maxit=50000)
+            xr = x.download()
+            errs.append(float(np.linalg.norm(rm.solutions[k] - xr) / np.linalg.norm(xr)))
+            x.close()
+            if k:
+                b.close()
+        A.close(); b0.close(); gp.close()
+    ctx.close()
+    for v in out_runs.values():
+        v.pop("results")
+    mixed = out_runs["mixed"]
+    extra = {"config": {"requests": args.count, "warmup_requests": nwarm, "load": args.load,
+                        "sizes": "log-uniform 1e3..2e6 DOFs, d in {2,3}, p in {2,3,4}, 1..8 rhs",
+                        "tol": tol, "iterations_per_step": 8},
+             "mixed": mixed, "fp64": out_runs["fp64"],
+             "p50_ms": mixed["p50_ms"], "p99_ms": mixed["p99_ms"],
+             "sampled_max_rel_err_vs_fp64": max(errs) if errs else None,
+             "sampled_solutions": len(errs)}
+    out = _common(args, mixed["dofs_per_s"], None, "f32+f64",
+                  f"serving: {args.count} requests, Poisson arrivals at {args.load:g} x capacity, "
+                  f"continuous batching, Jacobi-CG to relres {tol:g}", extra)
+    out["unit"] = "DOF-solves/s"
+    print(json.dumps(out))
 
 
 if __name__ == "__main__":

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -30,10 +31,14 @@ using DevPtr = std::shared_ptr<void>;
 // Device memory of a context: bytes in use plus a cache of released blocks
 // keyed by (rounded) size, so repeated assemblies and solves of the same
 // shapes reuse memory instead of calling hipMalloc/hipFree each time.
+// Blocks may be released from another host thread than the one that uses
+// the context (a matrix assembled on one context and solved on another),
+// so the cache is locked.
 struct Counter {
     std::atomic<int64_t> bytes{0};     // handed out
     int64_t cached = 0;                // held in the cache
     std::multimap<size_t, void *> cache;
+    std::mutex mu;
     ~Counter()
     {
         for (auto &kv : cache) (void)hipFree(kv.second);
@@ -45,9 +50,26 @@ struct Counter {
     }
     void trim()
     {
+        std::lock_guard<std::mutex> lk(mu);
         for (auto &kv : cache) (void)hipFree(kv.second);
         cache.clear();
         cached = 0;
+    }
+    void *take(size_t bytes)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = cache.find(bytes);
+        if (it == cache.end()) return nullptr;
+        void *p = it->second;
+        cache.erase(it);
+        cached -= (int64_t)bytes;
+        return p;
+    }
+    void give(size_t bytes, void *p)
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        cache.emplace(bytes, p);
+        cached += (int64_t)bytes;
     }
 };
 
@@ -98,12 +120,8 @@ DevPtr dalloc(gsx_ctx *c, size_t bytes)
     void *p = nullptr;
     bytes = Counter::round(bytes ? bytes : 16);
     auto mem = c->mem;
-    auto it = mem->cache.find(bytes);
-    if (it != mem->cache.end()) {
-        p = it->second;
-        mem->cache.erase(it);
-        mem->cached -= (int64_t)bytes;
-    } else if (hipMalloc(&p, bytes) != hipSuccess) {
+    p = mem->take(bytes);
+    if (!p && hipMalloc(&p, bytes) != hipSuccess) {
         (void)hipGetLastError();
         // release the cache and retry once
         mem->trim();
@@ -115,8 +133,7 @@ DevPtr dalloc(gsx_ctx *c, size_t bytes)
     mem->bytes += (int64_t)bytes;
     return DevPtr(p, [mem, bytes](void *q) {
         mem->bytes -= (int64_t)bytes;
-        mem->cache.emplace(bytes, q);
-        mem->cached += (int64_t)bytes;
+        mem->give(bytes, q);
     });
 }
 
@@ -920,14 +937,15 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
 {
     if (!P || !A_out) return GSX_ERR_ARG;
     gsx_ctx *c = P->ctx;
-    if ((mode & ~GSX_ASM_REBUILD_PATTERN) != GSX_PATTERN_EXPR &&
-        (mode & ~GSX_ASM_REBUILD_PATTERN) != GSX_PATTERN_FULL)
+    if ((mode & ~(GSX_ASM_REBUILD_PATTERN | GSX_ASM_CSR_ONLY)) != GSX_PATTERN_EXPR &&
+        (mode & ~(GSX_ASM_REBUILD_PATTERN | GSX_ASM_CSR_ONLY)) != GSX_PATTERN_FULL)
         return fail(c, GSX_ERR_ARG, "unknown pattern mode");
     if (precision != GSX_FP64 && precision != GSX_MIXED)
         return fail(c, GSX_ERR_ARG, "unknown precision");
     hipStream_t s = c->stream;
     const bool rebuild = (mode & GSX_ASM_REBUILD_PATTERN) != 0;
-    mode &= ~GSX_ASM_REBUILD_PATTERN;
+    const bool csr_only = (mode & GSX_ASM_CSR_ONLY) != 0;
+    mode &= ~(GSX_ASM_REBUILD_PATTERN | GSX_ASM_CSR_ONLY);
     if (!P->have_pattern || rebuild) {
         gsx_status st = pattern_build(P);
         if (st != GSX_OK) return st;
@@ -996,10 +1014,10 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
     if (!A->val || !rhs || (mode == GSX_PATTERN_EXPR && !keep))
         return fail(c, GSX_ERR_OOM, "device allocation (values)");
     if (single) {
-        // structured operator filled by the same gather
+        // structured operator filled by the same gather (unless CSR only)
         StencilOp &S = A->st;
         const HostPatch &h = P->hp[0];
-        S.available = true;
+        S.available = !csr_only;
         S.d = h.d;
         S.p = h.p[0];
         S.nx = P->grid_dims[0];
@@ -1009,12 +1027,14 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
         const int W = 2 * S.p + 1;
         S.slots = h.d == 3 ? W * W * W : W * W;
         S.f2g = P->d_f2g;
-        S.s64 = dalloc(c, sizeof(double) * (int64_t)S.slots * A->n);
-        if (precision == GSX_MIXED) S.s32 = dalloc(c, sizeof(float) * (int64_t)S.slots * A->n);
-        if (!S.s64 || (precision == GSX_MIXED && !S.s32))
-            return fail(c, GSX_ERR_OOM, "device allocation (structured operator)");
-        S.s64_ready = true;
-        S.s32_ready = precision == GSX_MIXED;
+        if (!csr_only) {
+            S.s64 = dalloc(c, sizeof(double) * (int64_t)S.slots * A->n);
+            if (precision == GSX_MIXED) S.s32 = dalloc(c, sizeof(float) * (int64_t)S.slots * A->n);
+            if (!S.s64 || (precision == GSX_MIXED && !S.s32))
+                return fail(c, GSX_ERR_OOM, "device allocation (structured operator)");
+        }
+        S.s64_ready = !csr_only;
+        S.s32_ready = !csr_only && precision == GSX_MIXED;
         // free rows are lexicographic over the free box: a range of row
         // layers along the slowest direction is a range of rows
         int64_t plane = 1;
@@ -1036,7 +1056,8 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
                                                P->nfree, r0, r1, E0 * layer_elems, dp<int>(P->d_f2g),
                                                dp<int>(P->d_rowptr), dp<double>(Kbuf), dp<double>(Fbuf),
                                                dp<double>(A->val), rhs, keep ? dp<unsigned char>(keep) : nullptr,
-                                               dp<double>(S.s64), S.s32 ? dp<float>(S.s32) : nullptr, s));
+                                               S.s64 ? dp<double>(S.s64) : nullptr,
+                                               S.s32 ? dp<float>(S.s32) : nullptr, s));
         }
     } else {
         KScope k(c, "gather");
@@ -2351,7 +2372,7 @@ gsx_status gsx_batch_add(gsx_batch *B, gsx_matrix *A, int32_t row0, int32_t nrow
     if (!B || !A || !b || !x || !id || maxit < 1 || nrows <= 0 || row0 < 0 || (int64_t)row0 + nrows > A->n)
         return GSX_ERR_ARG;
     gsx_ctx *c = B->c;
-    if (A->ctx != c) return fail(c, GSX_ERR_ARG, "matrix belongs to another context");
+    if (A->ctx->device != c->device) return fail(c, GSX_ERR_ARG, "matrix lives on another device");
     if (precond != GSX_PREC_NONE && precond != GSX_PREC_JACOBI)
         return fail(c, GSX_ERR_ARG, "batched CG supports no preconditioner or Jacobi");
     if (B->mixed && !A->mixed) return fail(c, GSX_ERR_ARG, "mixed batch needs matrices assembled with GSX_MIXED");

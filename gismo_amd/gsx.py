@@ -31,6 +31,7 @@ MIXED = 1
 PATTERN_EXPR = 0
 PATTERN_FULL = 1
 ASM_REBUILD_PATTERN = 0x100
+ASM_CSR_ONLY = 0x200
 PREC_NONE = 0
 PREC_JACOBI = 1
 PREC_MG = 2

@@ -148,6 +148,31 @@ def lognormal_kappa(nelem, sigma, seed, stream=10):
     return np.exp(sigma * normal01(seed, stream, np.arange(nelem)))
 
 
+def noisy_box_fast(d, p, n, noise, sigma, seed):
+    """Degree-p patch on the unit box with n elements per direction, interior
+    control points moved by U(+-noise/n), lognormal(0, sigma) kappa and
+    U(-1, 1) sources per element, drawn from numpy's PCG64 stream `seed`
+    (the serving workload: fast enough to generate requests at their
+    arrival rate)."""
+    rng = np.random.default_rng(seed)
+    kn = knots_uniform(n, p)
+    g = greville(kn, p)
+    nb = g.size
+    # C-ordered (z, y, x, coordinate): direction 0 is the fastest index
+    ctrl = np.empty((nb,) * d + (d,))
+    for k in range(d):
+        shape = [1] * d
+        shape[d - 1 - k] = nb
+        ctrl[..., k] = g.reshape(shape)
+    inner = (slice(1, nb - 1),) * d
+    ctrl[inner] += rng.uniform(-noise / n, noise / n, (nb - 2,) * d + (d,))
+    P = Patch(d=d, p=[p] * d, knots=[kn] * d, ctrl=ctrl.reshape(-1, d))
+    P.kappa = np.exp(sigma * rng.standard_normal(P.nelem))
+    P.felem = rng.uniform(-1.0, 1.0, P.nelem)
+    P.fkind = 0
+    return P
+
+
 # ---------------------------------------------------------------------------
 # Workloads
 # ---------------------------------------------------------------------------

@@ -1,0 +1,237 @@
+"""Continuous-batching solve server on one GPU.
+
+Requests (a single-patch Poisson/diffusion problem plus 1..8 right-hand
+sides) arrive at any time.  An assembler thread admits them in arrival
+order while their device memory fits the HBM budget, assembles each on its
+own context (its own HIP stream, so assembly overlaps the running solves),
+and hands the resident operator to the solver thread.  The solver thread
+adds one batched-CG member per right-hand side to the running
+`gsx_batch` and advances every member together, a few iterations per
+step; members join and leave between steps.  A request is complete when
+all its right-hand sides have converged and its solutions are back on the
+host; its operator and workspace are then released and its budget freed.
+
+Every member's iterates are independent of the other members (the batched
+CG reduces per member in a fixed order), so a request's solutions are
+bitwise the same alone and in any mix.
+
+The reference has no serving layer: it assembles and solves one problem at
+a time (src/gsAssembler/gsExprAssembler.h:754-833,
+src/gsSolver/gsConjugateGradient.hpp:20-93); each member runs that
+algorithm.
+"""
+import queue
+import threading
+import time
+from dataclasses import dataclass
+from typing import List, Optional
+
+import numpy as np
+
+from . import gsx, problems
+
+
+@dataclass
+class Request:
+    rid: int
+    problem: problems.Problem
+    rhs_seeds: List[int]          # one per right-hand side; the first is the assembled load vector
+    arrival: float = 0.0          # seconds on the server clock
+    tol: float = 1e-10
+    maxit: int = 20000
+    keep: bool = False            # return the solutions with the result
+
+
+@dataclass
+class Result:
+    rid: int
+    dofs: int
+    nfree: int
+    nrhs: int
+    arrival: float
+    admitted: float
+    assembled: float
+    done: float
+    iterations: List[int]
+    relres: List[float]
+    status: List[int]
+    solutions: Optional[List[np.ndarray]] = None
+
+    @property
+    def latency(self):
+        return self.done - self.arrival
+
+
+def estimate_bytes(prob, nrhs, mixed):
+    """Device bytes a request keeps resident: CSR pattern + values (+ FP32
+    copy), the load vector, and per right-hand side the solution, the rhs
+    and the CG workspace."""
+    P = prob.patches[0]
+    nfree = int(np.prod([nb - 2 for nb in P.nbasis_dir()]))
+    nnz = nfree * (2 * P.p[0] + 1) ** P.d
+    per_rhs = nfree * (8 * 6 + (4 * 6 if mixed else 0))
+    return int(nnz * (4 + 8 + (4 if mixed else 0)) + nfree * 8 * 3 + nrhs * per_rhs + (64 << 20))
+
+
+def rhs_vector(nfree, seed):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-1.0, 1.0, nfree)
+
+
+class _Entry:
+    def __init__(self, req, t_admit):
+        self.req = req
+        self.t_admit = t_admit
+        self.t_asm = None
+        self.gp = None
+        self.A = None
+        self.bs = []
+        self.xs = []
+        self.ids = []
+        self.bytes = 0
+
+
+class Server:
+    """One GPU, two contexts: one assembles, one solves."""
+
+    def __init__(self, precision=gsx.MIXED, budget_bytes=160 << 30, iters_per_step=8, capacity=4096,
+                 keep_solutions=False, device=0):
+        self.precision = precision
+        self.mixed = precision == gsx.MIXED
+        self.budget = budget_bytes
+        self.iters = iters_per_step
+        self.keep = keep_solutions
+        self.ctx_asm = gsx.Context(device)
+        self.ctx = gsx.Context(device)
+        self.batch = gsx.Batch(self.ctx, precision, capacity=capacity)
+        self.capacity = capacity
+        self.t0 = time.perf_counter()
+        self._incoming = queue.Queue()     # submitted requests
+        self._ready = queue.Queue()        # assembled entries
+        self._lock = threading.Lock()
+        self._used = 0                     # admitted bytes
+        self._members = 0                  # batch slots in use (solver thread)
+        self._stop = False
+        self._error = None
+        self.results = []
+        self._active = {}                  # rid -> entry (solver thread)
+        self._asm_thread = threading.Thread(target=self._assembler, daemon=True)
+        self._asm_thread.start()
+
+    def now(self):
+        return time.perf_counter() - self.t0
+
+    def submit(self, req):
+        self._incoming.put(req)
+
+    # ----- assembler thread --------------------------------------------
+    def _assembler(self):
+        pending = []
+        try:
+            while not self._stop:
+                try:
+                    pending.append(self._incoming.get(timeout=0.002))
+                except queue.Empty:
+                    pass
+                while pending:
+                    req = pending[0]
+                    need = estimate_bytes(req.problem, len(req.rhs_seeds), self.mixed)
+                    with self._lock:
+                        if self._used + need > self.budget and self._used > 0:
+                            break
+                        self._used += need
+                    pending.pop(0)
+                    e = _Entry(req, self.now())
+                    e.bytes = need
+                    e.gp = gsx.Problem(self.ctx_asm, req.problem)
+                    e.A, b0 = e.gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_CSR_ONLY, self.precision)
+                    e.bs.append(b0)
+                    for s in req.rhs_seeds[1:]:
+                        e.bs.append(self.ctx_asm.to_device(rhs_vector(e.gp.nfree, s)))
+                    self.ctx_asm.synchronize()
+                    e.t_asm = self.now()
+                    self._ready.put(e)
+        except Exception as ex:   # reported by the solver thread
+            self._error = ex
+
+    # ----- solver thread (the caller) -----------------------------------
+    def poll(self):
+        """Admit assembled requests into the batch, run one step, collect the
+        requests that finished.  Returns the number of running members."""
+        if self._error is not None:
+            raise self._error
+        while self._members < self.capacity:
+            try:
+                e = self._ready.get_nowait()
+            except queue.Empty:
+                break
+            for b in e.bs:
+                x = self.ctx.array(e.gp.nfree).zero()
+                e.xs.append(x)
+                e.ids.append(self.batch.add(e.A, b, x, tol=e.req.tol, maxit=e.req.maxit))
+                self._members += 1
+            self._active[e.req.rid] = e
+        running = self.batch.step(self.iters) if self._active else 0
+        for rid in list(self._active):
+            e = self._active[rid]
+            res = [self.batch.result(i) for i in e.ids]
+            if any(r[0] == 0 for r in res):
+                continue
+            sols = [x.download() for x in e.xs]   # results go back to the host
+            t_done = self.now()
+            for i in e.ids:
+                self.batch.remove(i)
+                self._members -= 1
+            self.results.append(Result(
+                rid=rid, dofs=e.gp.nglued, nfree=e.gp.nfree, nrhs=len(e.ids), arrival=e.req.arrival,
+                admitted=e.t_admit, assembled=e.t_asm, done=t_done,
+                iterations=[r[1] for r in res], relres=[r[2] for r in res], status=[r[0] for r in res],
+                solutions=sols if (self.keep or e.req.keep) else None))
+            for x in e.xs:
+                x.close()
+            e.A.close()
+            for b in e.bs:
+                b.close()
+            e.gp.close()
+            del self._active[rid]
+            with self._lock:
+                self._used -= e.bytes
+        return running
+
+    def idle(self):
+        return not self._active and self._ready.empty() and self._incoming.empty()
+
+    def drain(self, timeout=600.0):
+        t_end = time.perf_counter() + timeout
+        while not self.idle():
+            self.poll()
+            if time.perf_counter() > t_end:
+                raise RuntimeError("server did not drain")
+
+    def close(self):
+        self._stop = True
+        self._asm_thread.join()
+        self.batch.close()
+        self.ctx.close()
+        self.ctx_asm.close()
+
+
+def make_requests(count, seed=20261202, size_range=(1e3, 2e6), nrhs_range=(1, 8), noise=0.1, sigma=0.5):
+    """Requests of the serving workload: size log-uniform in size_range DOFs,
+    d uniform in {2, 3}, p uniform in {2, 3, 4}, n = round(size^(1/d)) - p
+    (at least 2), seeded geometry noise, lognormal kappa, random per-element
+    sources, 1..8 right-hand sides."""
+    out = []
+    lo, hi = np.log(size_range[0]), np.log(size_range[1])
+    for i in range(count):
+        s = seed * 1000003 + i
+        u = problems.uniform01(s, 60, np.arange(4))
+        size = float(np.exp(lo + (hi - lo) * u[0]))
+        d = 2 + int(u[1] * 2)
+        p = 2 + int(u[2] * 3)
+        n = max(2, int(round(size ** (1.0 / d))) - p)
+        nrhs = nrhs_range[0] + int(u[3] * (nrhs_range[1] - nrhs_range[0] + 1))
+        P = problems.noisy_box_fast(d, p, n, noise, sigma, s)
+        prob = problems.Problem([P], problems.all_sides(d), [], name=f"req{i}-d{d}-p{p}-n{n}")
+        out.append(Request(rid=i, problem=prob, rhs_seeds=[s] + [s * 31 + k for k in range(1, nrhs)]))
+    return out

@@ -40,7 +40,9 @@ typedef enum {
 typedef enum {
     GSX_PATTERN_EXPR = 0,    /* gsExprAssembler: skip exact-zero local entries */
     GSX_PATTERN_FULL = 1,    /* gsPoissonAssembler: every free-free pair       */
-    GSX_ASM_REBUILD_PATTERN = 0x100  /* flag: recompute the cached pattern     */
+    GSX_ASM_REBUILD_PATTERN = 0x100, /* flag: recompute the cached pattern     */
+    GSX_ASM_CSR_ONLY = 0x200         /* flag: no structured operator (CSR
+                                        solves only, e.g. batched members)    */
 } gsx_pattern_mode;
 
 typedef enum {
