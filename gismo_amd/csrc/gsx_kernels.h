@@ -145,6 +145,15 @@ hipError_t launch_sspmv_f64(int d, int p, int mode, int nfree, const int *f2g, i
 hipError_t launch_sspmv_f32(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz,
                             const float *sv, const float *x, float *y, const float *b,
                             double *part, const CGState *st, hipStream_t s);
+// symmetric half stencil (FP32): slots from the centre on, each slot array
+// hstride long with pad zeros before and after its nfree entries; h points
+// at slot 0's first entry
+int stencil_half_slots(int d, int p);
+hipError_t launch_stencil_half(int d, int p, int nfree, const double *s64, float *h, int64_t hstride,
+                               hipStream_t s);
+hipError_t launch_sspmv_sym_f32(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz, int fy,
+                                int fz, int64_t hstride, const float *h, const float *x, float *y,
+                                const float *b, double *part, const CGState *st, hipStream_t s);
 hipError_t launch_cg2_update_f64(int n, double *x, double *r, const double *p, const double *t,
                                  double *z, const double *diag, double tau, const double *pap,
                                  int npap, double *prr, double *prz, CGState *st, int k,

@@ -150,6 +150,96 @@ __global__ __launch_bounds__(256) void k_sspmv(int nfree, const int *__restrict_
 }
 
 // ---------------------------------------------------------------------------
+// Symmetric half-stencil SpMV.  A is symmetric bitwise, so only the centre
+// and the (NS - 1) / 2 slots after it (neighbour offsets lexicographically
+// positive) are stored: h[kk][r] = A(r, r + o_kk) for kk = 0..NH-1.  Row r
+// takes its lower-half coefficients from the rows below it:
+//   A(r, r - o_kk) = A(r - o_kk, r) = h[kk][r - of_kk],
+// where of_kk is the slot offset in the numbering of the free box (the free
+// functions of one patch form a box, numbered lexicographically).  Where the
+// neighbour lies outside the free box that index lands on a padding zero or
+// on another row's coefficient, and the grid value x[g - o_kk] it multiplies
+// is a zero of the grid's halo or eliminated layer.  Each slot array is
+// padded by `pad` zeros on both ends.  Half the coefficient bytes of the
+// full stencil (the mirrored half is re-read from cache).
+// ---------------------------------------------------------------------------
+template <int D, int P, typename T, int MODE>
+__global__ __launch_bounds__(256) void k_sspmv_sym(int nfree, const int *__restrict__ free2grid,
+                                                   int sy, int sz, int fy, int fz, int64_t hstride,
+                                                   const T *__restrict__ h, const T *__restrict__ x,
+                                                   T *__restrict__ y, const T *__restrict__ b,
+                                                   double *__restrict__ partial,
+                                                   const CGState *__restrict__ st)
+{
+    __shared__ double s[256];
+    if (st && st->done) return;
+    constexpr int W = 2 * P + 1;
+    double contrib = 0.0;
+    for (int r = blockIdx.x * 256 + threadIdx.x; r < nfree; r += gridDim.x * 256) {
+        const int g = free2grid[r];
+        const T *xb = x + g;
+        const T *hb = h + r;
+        T acc = hb[0] * xb[0];
+        // dz = 0 plane: dy = 0 with dx > 0, then dy > 0
+#pragma unroll
+        for (int dx = 1; dx <= P; ++dx) {
+            const int kk = dx;
+            acc += hb[kk * hstride] * xb[dx] + hb[kk * hstride - dx] * xb[-dx];
+        }
+#pragma unroll
+        for (int dy = 1; dy <= P; ++dy) {
+#pragma unroll
+            for (int dx = -P; dx <= P; ++dx) {
+                const int kk = P + (dy - 1) * W + (dx + P) + 1;
+                const int o = dy * sy + dx, of = dy * fy + dx;
+                acc += hb[kk * hstride] * xb[o] + hb[kk * hstride - of] * xb[-o];
+            }
+        }
+        if (D == 3) {
+            constexpr int K0 = P + P * W + 1;   // slots of the dz = 0 half plane
+#pragma unroll 1
+            for (int dz = 1; dz <= P; ++dz) {
+#pragma unroll
+                for (int dy = -P; dy <= P; ++dy) {
+#pragma unroll
+                    for (int dx = -P; dx <= P; ++dx) {
+                        const int kk = K0 + ((dz - 1) * W + (dy + P)) * W + (dx + P);
+                        const int o = dz * sz + dy * sy + dx, of = dz * fz + dy * fy + dx;
+                        acc += hb[kk * hstride] * xb[o] + hb[kk * hstride - of] * xb[-o];
+                    }
+                }
+            }
+        }
+        if (MODE == 2) {
+            const T v = b[g] - acc;
+            y[g] = v;
+            contrib += (double)v * (double)v;
+        } else {
+            y[g] = acc;
+            if (MODE == 1) contrib += (double)x[g] * (double)acc;
+        }
+    }
+    if (MODE != 0) {
+        const double tot = bsum256(contrib, s);
+        if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+    }
+}
+
+// full FP64 stencil -> FP32 half stencil (slots from the centre on)
+template <int D, int P>
+__global__ void k_stencil_half(int nfree, const double *__restrict__ s64, float *__restrict__ h,
+                               int64_t hstride)
+{
+    constexpr int W = 2 * P + 1;
+    constexpr int NS = (D == 3) ? W * W * W : W * W;
+    constexpr int C = (NS - 1) / 2, NH = NS - C;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nfree) return;
+#pragma unroll 4
+    for (int kk = 0; kk < NH; ++kk) h[kk * hstride + r] = (float)s64[(int64_t)(C + kk) * nfree + r];
+}
+
+// ---------------------------------------------------------------------------
 // grid-layout CG vector kernels (n = grid size; diag = 1 at eliminated
 // functions so z stays 0 there)
 // ---------------------------------------------------------------------------
@@ -343,6 +433,37 @@ hipError_t launch_sspmv_f32(int d, int p, int mode, int nfree, const int *f2g, i
                             double *part, const CGState *st, hipStream_t s)
 {
     return sspmv_dispatch<float, float>(d, p, mode, nfree, f2g, sy, sz, sv, x, y, b, part, st, s);
+}
+
+int stencil_half_slots(int d, int p)
+{
+    const int w = 2 * p + 1, ns = d == 3 ? w * w * w : w * w;
+    return ns - (ns - 1) / 2;
+}
+
+hipError_t launch_stencil_half(int d, int p, int nfree, const double *s64, float *h, int64_t hstride,
+                               hipStream_t s)
+{
+    const dim3 grid((nfree + 255) / 256), blk(256);
+#define GSX_H(D, P) if (d == D && p == P) { hipLaunchKernelGGL((k_stencil_half<D, P>), grid, blk, 0, s, nfree, s64, h, hstride); return hipGetLastError(); }
+    GSX_H(2, 1) GSX_H(2, 2) GSX_H(2, 3) GSX_H(2, 4)
+    GSX_H(3, 1) GSX_H(3, 2) GSX_H(3, 3) GSX_H(3, 4)
+#undef GSX_H
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_sspmv_sym_f32(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz, int fy,
+                                int fz, int64_t hstride, const float *h, const float *x, float *y,
+                                const float *b, double *part, const CGState *st, hipStream_t s)
+{
+    const dim3 grid(sspmv_blocks(nfree)), blk(256);
+#define GSX_Y(D, P, M) if (d == D && p == P && mode == M) { hipLaunchKernelGGL((k_sspmv_sym<D, P, float, M>), grid, blk, 0, s, nfree, f2g, sy, sz, fy, fz, hstride, h, x, y, b, part, st); return hipGetLastError(); }
+#define GSX_Y3(D, P) GSX_Y(D, P, 0) GSX_Y(D, P, 1) GSX_Y(D, P, 2)
+    GSX_Y3(2, 1) GSX_Y3(2, 2) GSX_Y3(2, 3) GSX_Y3(2, 4)
+    GSX_Y3(3, 1) GSX_Y3(3, 2) GSX_Y3(3, 3) GSX_Y3(3, 4)
+#undef GSX_Y3
+#undef GSX_Y
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_cg2_update_f64(int n, double *x, double *r, const double *p, const double *t,
