@@ -30,7 +30,8 @@
 // Mixed precision (FP32 inner iterations, FP64 reliable updates): the FP32
 // recurrence never declares convergence itself.  When its residual has
 // dropped by delta^2 since the last reliable update (or claims the
-// tolerance), the member is flagged, and the next reliable-update pass
+// tolerance), the member is flagged and its FP32 iterations pause (from
+// iteration flag_next on) until the next reliable-update pass, which
 // forms x64 += xc32, r64 = b - A64 x64 in FP64, decides convergence on
 // ||r64|| / ||b||, and restarts the FP32 recurrence from fp32(r64) with the
 // same direction.
@@ -202,6 +203,7 @@ __global__ __launch_bounds__(256) void k_b_init2(const BChunk *__restrict__ chun
         S.need_update = 0;
         S.stop = 0;
         S.pending = 0;
+        S.flag_next = 0;
         if (g.hist) g.hist[0] = err;
     }
 }
@@ -236,7 +238,10 @@ __global__ __launch_bounds__(256) void k_b_spmv(const BChunk *__restrict__ chunk
     BState &S = st[ch.seg];
     if (S.phase != 1 || S.done || S.stop) return;
     const BSeg g = segs[ch.seg];
+    // the iteration number advances even for a paused member, so that the
+    // update and direction launches of this iteration see it paused too
     if ((int)blockIdx.x == g.c0 && threadIdx.x == 0) S.it = S.it_next;
+    if (S.flag_next && S.flag_next <= S.it_next) return;
     double dot = 0.0;
     if constexpr (sizeof(T) == 4) chunk_spmv_any<float, float>(g, ch.lo, ch.hi, g.v32, g.p32, g.t32, dot);
     else chunk_spmv_any<double, double>(g, ch.lo, ch.hi, g.v64, g.p, g.t, dot);
@@ -253,7 +258,7 @@ __global__ __launch_bounds__(256) void k_b_update(const BChunk *__restrict__ chu
     __shared__ double s[256];
     const BChunk ch = chunks[blockIdx.x];
     const BState &S = st[ch.seg];
-    if (S.phase != 1 || S.done || S.stop) return;
+    if (S.phase != 1 || S.done || S.stop || (S.flag_next && S.flag_next <= S.it)) return;
     const BSeg g = segs[ch.seg];
     const double pap = member_sum(p0, g.c0, g.nc, s);
     const double alpha = S.rzs[S.it & 1] / pap;
@@ -296,7 +301,9 @@ __global__ __launch_bounds__(256) void k_b_direction(const BChunk *__restrict__ 
     __shared__ double s[256];
     const BChunk ch = chunks[blockIdx.x];
     BState &S = st[ch.seg];
-    if (S.phase != 1 || S.done || S.stop) return;
+    // flag_next is only set to it + 1 below: every block of this launch
+    // takes the same decision whether or not the first block has written it
+    if (S.phase != 1 || S.done || S.stop || (S.flag_next && S.flag_next <= S.it)) return;
     const BSeg g = segs[ch.seg];
     const double rr = member_sum(p1, g.c0, g.nc, s), rz = member_sum(p2, g.c0, g.nc, s);
     const int k = S.it;
@@ -319,7 +326,10 @@ __global__ __launch_bounds__(256) void k_b_direction(const BChunk *__restrict__ 
         S.err = err;
         S.rzs[(k + 1) & 1] = rz;
         if (g.hist && !mixed) g.hist[k + 1] = err;
-        if (mixed && (rr < delta2 * S.rr_ref || err < g.tol)) S.need_update = 1;
+        if (mixed && (rr < delta2 * S.rr_ref || err < g.tol)) {
+            S.need_update = 1;
+            S.flag_next = k + 1;
+        }
         if (conv) {
             S.done = 1;
         } else if (last) {
@@ -415,6 +425,7 @@ __global__ __launch_bounds__(256) void k_b_rel4(const BChunk *__restrict__ chunk
         else S.rzs[S.it_next & 1] = rz;
         S.pending = 0;
         S.need_update = 0;
+        S.flag_next = 0;
     }
 }
 

@@ -58,6 +58,13 @@ struct CGState {
     int done;         // 1 once err < tol (later launches exit at once)
     int maxit;
     int need_update;  // mixed: a reliable update is due
+    // mixed: the FP32 recurrence stops once an update is due, until the
+    // update has run (it would otherwise iterate on a residual at rounding
+    // level, where (r, z) can vanish).  Grid solver: iterations k >=
+    // flag_next are skipped (0 = none).  CSR solver: frozen, set by the
+    // alpha step when an update is due.
+    int flag_next;
+    int frozen;
 };
 
 // Element range of one element-kernel launch: elements [e0, e0 + grid) of
@@ -104,7 +111,7 @@ struct BState {
     int need_update;     // mixed: reliable update due
     int stop;            // mixed: iteration limit reached, decide at the next update
     int pending;         // mixed: decision of the running update
-    int pad;
+    int flag_next;       // mixed: iterations k >= flag_next wait for the update (0 = none)
 };
 
 }  // namespace gsx

@@ -153,7 +153,7 @@ hipError_t launch_stencil_half(int d, int p, int nfree, const double *s64, float
                                hipStream_t s);
 hipError_t launch_sspmv_sym_f32(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz, int fy,
                                 int fz, int64_t hstride, const float *h, const float *x, float *y,
-                                const float *b, double *part, const CGState *st, hipStream_t s);
+                                const float *b, double *part, const CGState *st, int k, hipStream_t s);
 hipError_t launch_cg2_update_f64(int n, double *x, double *r, const double *p, const double *t,
                                  double *z, const double *diag, double tau, const double *pap,
                                  int npap, double *prr, double *prz, CGState *st, int k,

@@ -1607,7 +1607,7 @@ static gsx_status cg_grid(gsx_matrix *A, int precond, double tau, const double *
                 {
                     KScope ks(c, "spmv_f32");
                     HIP_OK(c, gsx::launch_sspmv_sym_f32(S.d, S.p, 1, n, f2g, sy, sz, S.fy, S.fz, S.hstride, h32,
-                                                        p32, t32, nullptr, P0, st, s));
+                                                        p32, t32, nullptr, P0, st, k, s));
                 }
                 KScope kv(c, "cg_vec_f32");
                 HIP_OK(c, gsx::launch_cg2_update_f32((int)N, xc, r32, p32, t32, z32, jac ? d32 : nullptr, tau, P0,

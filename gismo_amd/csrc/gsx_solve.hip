@@ -51,7 +51,7 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int *__restrict__ row
                                               const CGState *__restrict__ state)
 {
     __shared__ double s[256];
-    if (state && state->done) return;
+    if (state && (state->done || state->frozen)) return;
     constexpr int RPB = 256 / LPR;
     const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
     const int r = blockIdx.x * RPB + g;
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(256) void k_cg_update(int n, TX *__restrict__ x, T 
                                                    const CGState *__restrict__ st)
 {
     __shared__ double s[256];
-    if (st->done) return;
+    if (st->done || st->frozen) return;
     const int i = blockIdx.x * 256 + threadIdx.x;
     double rr = 0.0, rz = 0.0;
     if (i < n) {
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void k_cg_update_id(int n, TX *__restrict__ x,
                                                       const CGState *__restrict__ st)
 {
     __shared__ double s[256];
-    if (st->done) return;
+    if (st->done || st->frozen) return;
     const int i = blockIdx.x * 256 + threadIdx.x;
     double rr = 0.0;
     if (i < n) {
@@ -160,7 +160,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_cg_pupdate(int n, T *__restrict__ p, const T *__restrict__ z,
                                                     const CGState *__restrict__ st)
 {
-    if (st->done) return;
+    if (st->done || st->frozen) return;
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = z[i] + (T)st->beta * p[i];
 }
@@ -311,6 +311,9 @@ __global__ __launch_bounds__(256) void k_cg_scalar(int step, const double *__res
         st->rz = a;
         break;
     case 2:
+        // mixed: an update is due since the last iteration -> wait for it
+        st->frozen = st->need_update;
+        if (st->frozen) break;
         st->pap = a;
         st->alpha = st->rz / a;
         break;
@@ -330,6 +333,7 @@ __global__ __launch_bounds__(256) void k_cg_scalar(int step, const double *__res
         break;
     }
     case 4: {
+        if (st->frozen) break;
         st->it += 1;
         st->rr = a;
         st->err = sqrt(a) / st->bnorm;
@@ -376,6 +380,8 @@ __global__ __launch_bounds__(256) void k_cg_reliable(const double *__restrict__ 
     st->err = sqrt(a) / st->bnorm;
     if (hist) hist[st->it] = st->err;
     st->need_update = 0;
+    st->flag_next = 0;
+    st->frozen = 0;
     st->done = (st->err < st->tol) ? 1 : (st->it >= st->maxit ? 2 : 0);
 }
 

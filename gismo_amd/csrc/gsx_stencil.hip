@@ -169,10 +169,10 @@ __global__ __launch_bounds__(256) void k_sspmv_sym(int nfree, const int *__restr
                                                    const T *__restrict__ h, const T *__restrict__ x,
                                                    T *__restrict__ y, const T *__restrict__ b,
                                                    double *__restrict__ partial,
-                                                   const CGState *__restrict__ st)
+                                                   const CGState *__restrict__ st, int k)
 {
     __shared__ double s[256];
-    if (st && st->done) return;
+    if (st && (st->done || (st->flag_next && st->flag_next <= k))) return;
     constexpr int W = 2 * P + 1;
     double contrib = 0.0;
     for (int r = blockIdx.x * 256 + threadIdx.x; r < nfree; r += gridDim.x * 256) {
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(256) void k_cg2_update(int n, TX *__restrict__ x, T
                                                     CGState *__restrict__ st, int k)
 {
     __shared__ double s[256];
-    if (st->done) return;
+    if (st->done || (st->flag_next && st->flag_next <= k)) return;
     const double pap = sum_partials(pap_parts, npap, s);
     const double alpha_d = st->rzs[k & 1] / pap;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -291,7 +291,9 @@ __global__ __launch_bounds__(256) void k_cg2_direction(int n, T *__restrict__ p,
                                                        double *__restrict__ hist)
 {
     __shared__ double s[256];
-    if (st->done) return;
+    // flag_next is only ever set to k + 1 below, so the blocks of this
+    // launch read the same decision whether or not block 0 has written it
+    if (st->done || (st->flag_next && st->flag_next <= k)) return;
     const double rr = sum_partials(part_rr, np, s);
     const double rzn = sum_partials(part_rz, np, s);
     const double err = sqrt(rr) / st->bnorm;
@@ -307,7 +309,10 @@ __global__ __launch_bounds__(256) void k_cg2_direction(int n, T *__restrict__ p,
         } else {
             st->beta = beta;
             st->rzs[(k + 1) & 1] = rzn;
-            if (MIXED && (rr < st->rel_since_update * st->rr_ref || err < st->tol)) st->need_update = 1;
+            if (MIXED && (rr < st->rel_since_update * st->rr_ref || err < st->tol)) {
+                st->need_update = 1;
+                st->flag_next = k + 1;
+            }
             if (k + 1 >= st->maxit) st->done = 2;
         }
     }
@@ -454,10 +459,10 @@ hipError_t launch_stencil_half(int d, int p, int nfree, const double *s64, float
 
 hipError_t launch_sspmv_sym_f32(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz, int fy,
                                 int fz, int64_t hstride, const float *h, const float *x, float *y,
-                                const float *b, double *part, const CGState *st, hipStream_t s)
+                                const float *b, double *part, const CGState *st, int k, hipStream_t s)
 {
     const dim3 grid(sspmv_blocks(nfree)), blk(256);
-#define GSX_Y(D, P, M) if (d == D && p == P && mode == M) { hipLaunchKernelGGL((k_sspmv_sym<D, P, float, M>), grid, blk, 0, s, nfree, f2g, sy, sz, fy, fz, hstride, h, x, y, b, part, st); return hipGetLastError(); }
+#define GSX_Y(D, P, M) if (d == D && p == P && mode == M) { hipLaunchKernelGGL((k_sspmv_sym<D, P, float, M>), grid, blk, 0, s, nfree, f2g, sy, sz, fy, fz, hstride, h, x, y, b, part, st, k); return hipGetLastError(); }
 #define GSX_Y3(D, P) GSX_Y(D, P, 0) GSX_Y(D, P, 1) GSX_Y(D, P, 2)
     GSX_Y3(2, 1) GSX_Y3(2, 2) GSX_Y3(2, 3) GSX_Y3(2, 4)
     GSX_Y3(3, 1) GSX_Y3(3, 2) GSX_Y3(3, 3) GSX_Y3(3, 4)

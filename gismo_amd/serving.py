@@ -111,6 +111,7 @@ class Server:
         self._lock = threading.Lock()
         self._used = 0                     # admitted bytes
         self._members = 0                  # batch slots in use (solver thread)
+        self._outstanding = 0              # submitted and not yet finished
         self._stop = False
         self._error = None
         self.results = []
@@ -122,6 +123,8 @@ class Server:
         return time.perf_counter() - self.t0
 
     def submit(self, req):
+        with self._lock:
+            self._outstanding += 1
         self._incoming.put(req)
 
     # ----- assembler thread --------------------------------------------
@@ -196,10 +199,14 @@ class Server:
             del self._active[rid]
             with self._lock:
                 self._used -= e.bytes
+                self._outstanding -= 1
         return running
 
     def idle(self):
-        return not self._active and self._ready.empty() and self._incoming.empty()
+        """No request submitted and unfinished (queued, being assembled,
+        waiting for a batch slot or solving)."""
+        with self._lock:
+            return self._outstanding == 0
 
     def drain(self, timeout=600.0):
         t_end = time.perf_counter() + timeout

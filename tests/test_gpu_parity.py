@@ -265,3 +265,39 @@ def test_multipatch_over_element_budget_is_refused(ctx):
     finally:
         ctx.set_elem_budget(16 << 30)
     gp.close()
+
+
+@pytest.mark.parametrize("d,p,n", [(3, 2, 2), (3, 3, 2), (2, 3, 2), (2, 3, 4), (3, 2, 4)])
+def test_mixed_tiny_patches_converge(ctx, d, p, n):
+    """Tiny systems, where the FP32 recurrence reaches rounding level within
+    a few iterations: the mixed solvers (structured and CSR, single solve
+    and batched) pause their FP32 iterations until the FP64 update and reach
+    the FP64 solution."""
+    P = problems.box_patch(d, p, n)
+    P.fkind = 1
+    prob = problems.Problem([P], problems.all_sides(d), [], name=f"tiny-d{d}-p{p}-n{n}")
+    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    Ao, bo = O.assemble(prob.patches, dofs, nfree, 0)
+    xo, _, _ = O.cg_solve(Ao, bo, tol=1e-14, maxit=1000)
+    gp = gsx.Problem(ctx, prob)
+    A, b = gp.assemble(gsx.PATTERN_EXPR, gsx.MIXED)
+    for structured in (True, False):
+        A.set_operator(structured)
+        x = ctx.array(gp.nfree).zero()
+        it, rel, ok = A.cg_solve(b, x, tol=1e-10, maxit=2000, precision=gsx.MIXED)
+        xs = x.download()
+        assert ok and rel < 1e-10 and it < 200, (structured, it, rel)
+        assert np.linalg.norm(xs - xo) <= 1e-8 * np.linalg.norm(xo)
+        x.close()
+    B = gsx.Batch(ctx, gsx.MIXED, capacity=2)
+    x = ctx.array(gp.nfree).zero()
+    i = B.add(A, b, x, tol=1e-10, maxit=2000)
+    B.run(8)
+    st, it, rel = B.result(i)
+    assert st == 1 and rel < 1e-10 and it < 200, (it, rel)
+    assert np.linalg.norm(x.download() - xo) <= 1e-8 * np.linalg.norm(xo)
+    B.close()
+    x.close()
+    A.close()
+    b.close()
+    gp.close()
