@@ -706,6 +706,17 @@ struct PatchBox {
     int padded[3];        // padded grid dims (structured operator)
 };
 
+// Rows per block of the single-patch gather: the block keeps its rows'
+// stencil slots in LDS so that the slot-major structured operator is written
+// in runs of R consecutive rows (whole cache lines) instead of 8-byte
+// pieces a matrix-width apart.
+template <int D, int P>
+struct GatherCfg {
+    static constexpr int W = 2 * P + 1;
+    static constexpr int NS = (D == 3) ? W * W * W : W * W;
+    static constexpr int R = (NS * 16 * 9 <= 56 * 1024) ? 16 : 8;
+};
+
 template <int D, int P>
 __global__ __launch_bounds__(256) void k_gather_patch(PatchBox bx, int nfree, int r0, int r1, int64_t elbase,
                                                       const int *__restrict__ f2g,
@@ -719,103 +730,117 @@ __global__ __launch_bounds__(256) void k_gather_patch(PatchBox bx, int nfree, in
                                                       float *__restrict__ s32)
 {
     constexpr int W = 2 * P + 1;
-    constexpr int NS = (D == 3) ? W * W * W : W * W;
+    constexpr int NS = GatherCfg<D, P>::NS;
+    constexpr int R = GatherCfg<D, P>::R;
     constexpr int NB1 = P + 1;
     constexpr int NB = (D == 3) ? NB1 * NB1 * NB1 : NB1 * NB1;
-    __shared__ double sAcc[4][NS];
-    __shared__ unsigned char sNz[4][NS];
+    __shared__ double sAcc[R][NS];
+    __shared__ unsigned char sNz[R][NS];
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int r = r0 + blockIdx.x * 4 + w;
-    if (r >= r1) return;
-    for (int k = lane; k < NS; k += kWave) {
-        sAcc[w][k] = 0.0;
-        sNz[w][k] = 0;
+    const int rbase = r0 + blockIdx.x * R;
+    for (int k = threadIdx.x; k < R * NS; k += 256) {
+        sAcc[k / NS][k % NS] = 0.0;
+        sNz[k / NS][k % NS] = 0;
     }
-    __builtin_amdgcn_wave_barrier();
-    const int g = f2g[r];
-    int ic[3];
-    ic[0] = g % bx.padded[0] - P;
-    ic[1] = (g / bx.padded[0]) % bx.padded[1] - P;
-    ic[2] = (D == 3) ? g / (bx.padded[0] * bx.padded[1]) - P : 0;
-    int elo[3] = {0, 0, 0}, ehi[3] = {0, 0, 0};
+    __syncthreads();
+    for (int rr = w; rr < R; rr += 4) {
+        const int r = rbase + rr;
+        if (r >= r1) break;
+        const int g = f2g[r];
+        int ic[3];
+        ic[0] = g % bx.padded[0] - P;
+        ic[1] = (g / bx.padded[0]) % bx.padded[1] - P;
+        ic[2] = (D == 3) ? g / (bx.padded[0] * bx.padded[1]) - P : 0;
+        int elo[3] = {0, 0, 0}, ehi[3] = {0, 0, 0};
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
-        elo[k] = max(0, ic[k] - P);
-        ehi[k] = min(bx.nel[k] - 1, ic[k]);
-    }
-    double racc = 0.0;
-    const int ne0 = ehi[0] - elo[0] + 1, ne1 = ehi[1] - elo[1] + 1, ne2 = ehi[2] - elo[2] + 1;
-    const int ne = ne0 * ne1 * ne2;
-    // element rows in flight per lane; with more than one b chunk per element
-    // (NB > 64) one element at a time keeps the per-column element order
-    constexpr int U = (NB > kWave) ? 1 : 8;
-    for (int t0 = 0; t0 < ne; t0 += U) {
-        for (int b0 = 0; b0 < NB; b0 += kWave) {
-            const int b = b0 + lane;
-            const int bxx = b % NB1, byy = (b / NB1) % NB1, bzz = (D == 3) ? b / (NB1 * NB1) : 0;
-            double v[U];
-            int slot[U];
+        for (int k = 0; k < D; ++k) {
+            elo[k] = max(0, ic[k] - P);
+            ehi[k] = min(bx.nel[k] - 1, ic[k]);
+        }
+        double racc = 0.0;
+        const int ne0 = ehi[0] - elo[0] + 1, ne1 = ehi[1] - elo[1] + 1, ne2 = ehi[2] - elo[2] + 1;
+        const int ne = ne0 * ne1 * ne2;
+        // element rows in flight per lane; with more than one b chunk per
+        // element (NB > 64) one element at a time keeps the per-column
+        // element order
+        constexpr int U = (NB > kWave) ? 1 : 8;
+        for (int t0 = 0; t0 < ne; t0 += U) {
+            for (int b0 = 0; b0 < NB; b0 += kWave) {
+                const int b = b0 + lane;
+                const int bxx = b % NB1, byy = (b / NB1) % NB1, bzz = (D == 3) ? b / (NB1 * NB1) : 0;
+                double v[U];
+                int slot[U];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int t = t0 + u;
-                slot[u] = -1;
-                v[u] = 0.0;
-                if (t < ne && b < NB) {
-                    const int ex = elo[0] + t % ne0, ey = elo[1] + (t / ne0) % ne1;
-                    const int ez = elo[2] + t / (ne0 * ne1);
-                    const int jx = ex + bxx, jy = ey + byy, jz = ez + bzz;
-                    bool fr = jx >= bx.flo[0] && jx <= bx.fhi[0] && jy >= bx.flo[1] && jy <= bx.fhi[1];
-                    if (D == 3) fr = fr && jz >= bx.flo[2] && jz <= bx.fhi[2];
-                    if (fr) {
-                        const int64_t el = ex + (int64_t)bx.nel[0] * (ey + (int64_t)bx.nel[1] * ez) - elbase;
-                        const int la = (ic[0] - ex) + NB1 * ((ic[1] - ey) + NB1 * (ic[2] - ez));
-                        v[u] = Kbuf[el * (NB * NB) + (int64_t)la * NB + b];
-                        slot[u] = (D == 3) ? ((jz - ic[2] + P) * W + (jy - ic[1] + P)) * W + (jx - ic[0] + P)
-                                           : (jy - ic[1] + P) * W + (jx - ic[0] + P);
+                for (int u = 0; u < U; ++u) {
+                    const int t = t0 + u;
+                    slot[u] = -1;
+                    v[u] = 0.0;
+                    if (t < ne && b < NB) {
+                        const int ex = elo[0] + t % ne0, ey = elo[1] + (t / ne0) % ne1;
+                        const int ez = elo[2] + t / (ne0 * ne1);
+                        const int jx = ex + bxx, jy = ey + byy, jz = ez + bzz;
+                        bool fr = jx >= bx.flo[0] && jx <= bx.fhi[0] && jy >= bx.flo[1] && jy <= bx.fhi[1];
+                        if (D == 3) fr = fr && jz >= bx.flo[2] && jz <= bx.fhi[2];
+                        if (fr) {
+                            const int64_t el = ex + (int64_t)bx.nel[0] * (ey + (int64_t)bx.nel[1] * ez) - elbase;
+                            const int la = (ic[0] - ex) + NB1 * ((ic[1] - ey) + NB1 * (ic[2] - ez));
+                            v[u] = Kbuf[el * (NB * NB) + (int64_t)la * NB + b];
+                            slot[u] = (D == 3) ? ((jz - ic[2] + P) * W + (jy - ic[1] + P)) * W + (jx - ic[0] + P)
+                                               : (jy - ic[1] + P) * W + (jx - ic[0] + P);
+                        }
                     }
                 }
-            }
-            // element order is kept: the slots of one element are distinct,
-            // successive elements are applied one after the other
+                // element order is kept: the slots of one element are
+                // distinct, successive elements are applied one after the other
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (slot[u] >= 0) {
-                    sAcc[w][slot[u]] += v[u];
-                    sNz[w][slot[u]] |= (v[u] != 0.0);
+                for (int u = 0; u < U; ++u) {
+                    if (slot[u] >= 0) {
+                        sAcc[rr][slot[u]] += v[u];
+                        sNz[rr][slot[u]] |= (v[u] != 0.0);
+                    }
+                    __builtin_amdgcn_wave_barrier();
                 }
-                __builtin_amdgcn_wave_barrier();
+            }
+            for (int u = 0; u < U && t0 + u < ne; ++u) {
+                const int t = t0 + u;
+                const int ex = elo[0] + t % ne0, ey = elo[1] + (t / ne0) % ne1, ez = elo[2] + t / (ne0 * ne1);
+                const int64_t el = ex + (int64_t)bx.nel[0] * (ey + (int64_t)bx.nel[1] * ez) - elbase;
+                const int la = (ic[0] - ex) + NB1 * ((ic[1] - ey) + NB1 * (ic[2] - ez));
+                racc += Fbuf[el * NB + la];
             }
         }
-        for (int u = 0; u < U && t0 + u < ne; ++u) {
-            const int t = t0 + u;
-            const int ex = elo[0] + t % ne0, ey = elo[1] + (t / ne0) % ne1, ez = elo[2] + t / (ne0 * ne1);
-            const int64_t el = ex + (int64_t)bx.nel[0] * (ey + (int64_t)bx.nel[1] * ez) - elbase;
-            const int la = (ic[0] - ex) + NB1 * ((ic[1] - ey) + NB1 * (ic[2] - ez));
-            racc += Fbuf[el * NB + la];
-        }
-    }
-    __builtin_amdgcn_wave_barrier();
-    if (s64)
-        for (int k = lane; k < NS; k += kWave) s64[(int64_t)k * nfree + r] = sAcc[w][k];
-    if (s32)
-        for (int k = lane; k < NS; k += kWave) s32[(int64_t)k * nfree + r] = (float)sAcc[w][k];
-    // CSR row: window clipped to the free box, lexicographic
-    int lo[3] = {0, 0, 0}, wd[3] = {1, 1, 1};
+        __builtin_amdgcn_wave_barrier();
+        // CSR row: window clipped to the free box, lexicographic
+        int lo[3] = {0, 0, 0}, wd[3] = {1, 1, 1};
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
-        lo[k] = max(bx.flo[k], ic[k] - P);
-        wd[k] = min(bx.fhi[k], ic[k] + P) - lo[k] + 1;
+        for (int k = 0; k < D; ++k) {
+            lo[k] = max(bx.flo[k], ic[k] - P);
+            wd[k] = min(bx.fhi[k], ic[k] + P) - lo[k] + 1;
+        }
+        const int rb = rowptr[r], len = rowptr[r + 1] - rb;
+        for (int q = lane; q < len; q += kWave) {
+            const int qx = q % wd[0], qy = (q / wd[0]) % wd[1], qz = (D == 3) ? q / (wd[0] * wd[1]) : 0;
+            const int slot = (D == 3)
+                ? ((lo[2] + qz - ic[2] + P) * W + (lo[1] + qy - ic[1] + P)) * W + (lo[0] + qx - ic[0] + P)
+                : (lo[1] + qy - ic[1] + P) * W + (lo[0] + qx - ic[0] + P);
+            val[rb + q] = sAcc[rr][slot];
+            if (keep) keep[rb + q] = sNz[rr][slot];
+        }
+        if (lane == 0) rhs[r] = racc;
     }
-    const int rb = rowptr[r], len = rowptr[r + 1] - rb;
-    for (int q = lane; q < len; q += kWave) {
-        const int qx = q % wd[0], qy = (q / wd[0]) % wd[1], qz = (D == 3) ? q / (wd[0] * wd[1]) : 0;
-        const int slot = (D == 3)
-            ? ((lo[2] + qz - ic[2] + P) * W + (lo[1] + qy - ic[1] + P)) * W + (lo[0] + qx - ic[0] + P)
-            : (lo[1] + qy - ic[1] + P) * W + (lo[0] + qx - ic[0] + P);
-        val[rb + q] = sAcc[w][slot];
-        if (keep) keep[rb + q] = sNz[w][slot];
-    }
-    if (lane == 0) rhs[r] = racc;
+    __syncthreads();
+    // structured operator: slot-major, R consecutive rows per slot
+    const int nr = min(R, r1 - rbase);
+    if (s64)
+        for (int k = threadIdx.x; k < R * NS; k += 256) {
+            const int slot = k / R, rr = k % R;
+            if (rr < nr) s64[(int64_t)slot * nfree + rbase + rr] = sAcc[rr][slot];
+        }
+    if (s32)
+        for (int k = threadIdx.x; k < R * NS; k += 256) {
+            const int slot = k / R, rr = k % R;
+            if (rr < nr) s32[(int64_t)slot * nfree + rbase + rr] = (float)sAcc[rr][slot];
+        }
 }
 
 // Single-patch pattern: row r's columns are the free functions of the
@@ -886,8 +911,7 @@ hipError_t launch_gather_patch(int d, int p, const int *flo, const int *fhi, con
         bx.nel[k] = nel[k];
         bx.padded[k] = padded[k];
     }
-    const dim3 grid((r1 - r0 + 3) / 4), blk(256);
-#define GSX_G(D, P) if (d == D && p == P) { hipLaunchKernelGGL((k_gather_patch<D, P>), grid, blk, 0, s, bx, nfree, r0, r1, elbase, f2g, rowptr, Kbuf, Fbuf, val, rhs, keep, s64, s32); return hipGetLastError(); }
+#define GSX_G(D, P) if (d == D && p == P) { const dim3 grid((r1 - r0 + GatherCfg<D, P>::R - 1) / GatherCfg<D, P>::R), blk(256); hipLaunchKernelGGL((k_gather_patch<D, P>), grid, blk, 0, s, bx, nfree, r0, r1, elbase, f2g, rowptr, Kbuf, Fbuf, val, rhs, keep, s64, s32); return hipGetLastError(); }
     GSX_G(2, 1) GSX_G(2, 2) GSX_G(2, 3) GSX_G(2, 4)
     GSX_G(3, 1) GSX_G(3, 2) GSX_G(3, 3) GSX_G(3, 4)
 #undef GSX_G
