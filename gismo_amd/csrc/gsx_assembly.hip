@@ -982,14 +982,15 @@ __global__ __launch_bounds__(128) void k_l2err(const PatchDev *__restrict__ patc
                                                const int *__restrict__ gpatch,
                                                const int64_t *__restrict__ gprefix, int ng,
                                                GaussTab gt, const double *__restrict__ sol,
-                                               int nfree, double *__restrict__ out)
+                                               int nfree, double *__restrict__ out,
+                                               double *__restrict__ out_h1)
 {
     constexpr int NB1 = P + 1, NQ1 = P + 1;
     constexpr int NB = (D == 2) ? NB1 * NB1 : NB1 * NB1 * NB1;
     constexpr int NQ = NB;
     __shared__ double sV[D][NQ1][NB1], sD[D][NQ1][NB1], sW[D][NQ1];
     __shared__ double sC[NB][D], sU[NB];
-    __shared__ double sE[128];
+    __shared__ double sE[128], sH[128];
     const int64_t E = blockIdx.x;
     const int gi = find_patch(gprefix, ng, E);
     const PatchDev &pd = patches[gpatch[gi]];
@@ -1008,10 +1009,10 @@ __global__ __launch_bounds__(128) void k_l2err(const PatchDev *__restrict__ patc
         sU[b] = (g < nfree) ? sol[g] : 0.0;
     }
     __syncthreads();
-    double local = 0.0;
+    double local = 0.0, local_h1 = 0.0;
     for (int q = t; q < NQ; q += 128) {
         const int qx = q % NQ1, qy = (q / NQ1) % NQ1, qz = (D == 3) ? q / (NQ1 * NQ1) : 0;
-        double x[3] = {0, 0, 0}, J[3][3] = {{0}}, uh = 0.0;
+        double x[3] = {0, 0, 0}, J[3][3] = {{0}}, uh = 0.0, gp[3] = {0, 0, 0};
         for (int b = 0; b < NB; ++b) {
             const int bx = b % NB1, by = (b / NB1) % NB1, bz = (D == 3) ? b / (NB1 * NB1) : 0;
             double v = sV[0][qx][bx] * sV[1][qy][by];
@@ -1025,6 +1026,9 @@ __global__ __launch_bounds__(128) void k_l2err(const PatchDev *__restrict__ patc
                 g1 *= sV[2][qz][bz];
             }
             uh += sU[b] * v;
+            gp[0] += sU[b] * g0;
+            gp[1] += sU[b] * g1;
+            if (D == 3) gp[2] += sU[b] * g2;
             for (int r = 0; r < D; ++r) {
                 x[r] += sC[b][r] * v;
                 J[r][0] += sC[b][r] * g0;
@@ -1044,13 +1048,48 @@ __global__ __launch_bounds__(128) void k_l2err(const PatchDev *__restrict__ patc
         for (int r = 0; r < D; ++r) ue *= sin(pi * x[r]);
         const double diff = uh - ue;
         local += w * (diff * diff * fabs(det));
+        if (out_h1) {
+            // physical gradient: grad u_h = J^{-T} grad_xi u_h, inv[a][c] = dxi_a/dx_c
+            double inv[3][3] = {{0}};
+            if (D == 2) {
+                inv[0][0] = J[1][1] / det;
+                inv[0][1] = -J[0][1] / det;
+                inv[1][0] = -J[1][0] / det;
+                inv[1][1] = J[0][0] / det;
+            } else {
+                inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+                inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+                inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+                inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+                inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+                inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+                inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+                inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+                inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+            }
+            double d2 = 0.0;
+            for (int c = 0; c < D; ++c) {
+                double gh = 0.0;
+                for (int a = 0; a < D; ++a) gh += gp[a] * inv[a][c];
+                double ge = pi * cos(pi * x[c]);
+                for (int r = 0; r < D; ++r)
+                    if (r != c) ge *= sin(pi * x[r]);
+                d2 += (gh - ge) * (gh - ge);
+            }
+            local_h1 += w * (d2 * fabs(det));
+        }
     }
     sE[t] = local;
+    sH[t] = local_h1;
     __syncthreads();
     if (t == 0) {
-        double s = 0.0;
-        for (int i = 0; i < 128; ++i) s += sE[i];
+        double s = 0.0, h = 0.0;
+        for (int i = 0; i < 128; ++i) {
+            s += sE[i];
+            h += sH[i];
+        }
         out[pd.elem_off + e] = s;
+        if (out_h1) out_h1[pd.elem_off + e] = h;
     }
 }
 
@@ -1105,11 +1144,11 @@ hipError_t launch_elem_range(int d, int p, const PatchDev *patches, const int *g
 
 hipError_t launch_l2err(int d, int p, const PatchDev *patches, const int *gpatch,
                         const int64_t *gprefix, int ng, int64_t nelem, const GaussTab &gt,
-                        const double *sol, int nfree, double *out, hipStream_t s)
+                        const double *sol, int nfree, double *out, double *out_h1, hipStream_t s)
 {
     if (nelem <= 0) return hipSuccess;
     const dim3 grid((unsigned)nelem);
-#define GSX_L(D, P) if (d == D && p == P) { hipLaunchKernelGGL((k_l2err<D, P>), grid, dim3(128), 0, s, patches, gpatch, gprefix, ng, gt, sol, nfree, out); return hipGetLastError(); }
+#define GSX_L(D, P) if (d == D && p == P) { hipLaunchKernelGGL((k_l2err<D, P>), grid, dim3(128), 0, s, patches, gpatch, gprefix, ng, gt, sol, nfree, out, out_h1); return hipGetLastError(); }
     GSX_L(2, 1) GSX_L(2, 2) GSX_L(2, 3) GSX_L(2, 4)
     GSX_L(3, 1) GSX_L(3, 2) GSX_L(3, 3) GSX_L(3, 4)
 #undef GSX_L

@@ -15,7 +15,7 @@ hipError_t launch_elem_range(int d, int p, const PatchDev *patches, const int *g
                              double *Kbuf, double *Fbuf, double *detmin, hipStream_t s);
 hipError_t launch_l2err(int d, int p, const PatchDev *patches, const int *gpatch,
                         const int64_t *gprefix, int ng, int64_t nelem, const GaussTab &gt,
-                        const double *sol, int nfree, double *out, hipStream_t s);
+                        const double *sol, int nfree, double *out, double *out_h1, hipStream_t s);
 hipError_t launch_rowcount(const PatchDev *patches, const int *occ_ptr, const int *occ_patch,
                            const int64_t *occ_local, int nfree, int *counts, int *err,
                            hipStream_t s);

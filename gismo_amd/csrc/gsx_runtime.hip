@@ -1703,24 +1703,34 @@ gsx_status gsx_cg_history(gsx_matrix *A, double *hist, int32_t cap, int32_t *cou
     return GSX_OK;
 }
 
-gsx_status gsx_l2_error(gsx_problem *P, const double *sol, double *err)
+gsx_status gsx_error_norms(gsx_problem *P, const double *sol, double *l2, double *h1semi)
 {
-    if (!P || !sol || !err) return GSX_ERR_ARG;
+    if (!P || !sol || (!l2 && !h1semi)) return GSX_ERR_ARG;
     gsx_ctx *c = P->ctx;
     hipStream_t s = c->stream;
-    DevPtr per = dalloc(c, sizeof(double) * (std::max<int64_t>(1, P->nelem) + 1));
-    if (!per) return fail(c, GSX_ERR_OOM, "device allocation (l2)");
+    const int64_t ne = std::max<int64_t>(1, P->nelem);
+    DevPtr per = dalloc(c, sizeof(double) * 2 * (ne + 1));
+    if (!per) return fail(c, GSX_ERR_OOM, "device allocation (error norms)");
+    double *pl2 = dp<double>(per), *ph1 = pl2 + ne + 1;
     for (auto &g : P->groups)
         HIP_OK(c, gsx::launch_l2err(g.d, g.p, dp<PatchDev>(P->d_patches), dp<int>(g.d_gpatch),
                                     dp<int64_t>(g.d_gprefix), (int)g.patches.size(), g.nelem, c->gt,
-                                    sol, P->nfree, dp<double>(per), s));
-    double *slot = dp<double>(per) + std::max<int64_t>(1, P->nelem);
-    HIP_OK(c, gsx::launch_sum(dp<double>(per), P->nelem, slot, s));
-    double tot = 0.0;
-    HIP_OK(c, hipMemcpyAsync(&tot, slot, sizeof(double), hipMemcpyDeviceToHost, s));
+                                    sol, P->nfree, pl2, h1semi ? ph1 : nullptr, s));
+    HIP_OK(c, gsx::launch_sum(pl2, P->nelem, pl2 + ne, s));
+    if (h1semi) HIP_OK(c, gsx::launch_sum(ph1, P->nelem, ph1 + ne, s));
+    double tot[2] = {0.0, 0.0};
+    HIP_OK(c, hipMemcpyAsync(&tot[0], pl2 + ne, sizeof(double), hipMemcpyDeviceToHost, s));
+    if (h1semi) HIP_OK(c, hipMemcpyAsync(&tot[1], ph1 + ne, sizeof(double), hipMemcpyDeviceToHost, s));
     HIP_OK(c, hipStreamSynchronize(s));
-    *err = std::sqrt(tot);
+    if (l2) *l2 = std::sqrt(tot[0]);
+    if (h1semi) *h1semi = std::sqrt(tot[1]);
     return GSX_OK;
+}
+
+gsx_status gsx_l2_error(gsx_problem *P, const double *sol, double *err)
+{
+    if (!err) return GSX_ERR_ARG;
+    return gsx_error_norms(P, sol, err, nullptr);
 }
 
 }  // extern "C"

@@ -107,6 +107,7 @@ SIGNATURES = {
                      _i32p, _dp],
     "gsx_cg_history": [_vp, _dp, C.c_int32, _i32p],
     "gsx_l2_error": [_vp, _vp, _dp],
+    "gsx_error_norms": [_vp, _vp, _dp, _dp],
     "gsx_mg_create": [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
                       C.c_int32, _pp],
     "gsx_mg_destroy": [_vp],
@@ -333,6 +334,12 @@ class Problem:
         v = C.c_double()
         _check(self.ctx.h, lib().gsx_l2_error(self.h, sol.ptr, C.byref(v)))
         return v.value
+
+    def error_norms(self, sol):
+        """(L2 error, H1 seminorm error) against prod sin(pi x_r)."""
+        a, b = C.c_double(), C.c_double()
+        _check(self.ctx.h, lib().gsx_error_norms(self.h, sol.ptr, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def close(self):
         if self.h:

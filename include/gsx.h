@@ -276,6 +276,11 @@ gsx_status gsx_batch_remove(gsx_batch *batch, int32_t id);
  * (gsExprEvaluator::integral, src/gsAssembler/gsExprEvaluator.h:382-450). */
 gsx_status gsx_l2_error(gsx_problem *prob, const double *sol_dev,
                         double *err);
+/* Both error norms of examples/poisson2_example.cpp:173-176: the L2 error
+ * and the H1 seminorm error |u_h - u|_H1 (igrad(u_ex) - igrad(u_sol, G)),
+ * each integrated with the assembly quadrature; either output may be NULL. */
+gsx_status gsx_error_norms(gsx_problem *prob, const double *sol_dev,
+                           double *l2, double *h1semi);
 
 #ifdef __cplusplus
 }

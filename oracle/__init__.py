@@ -83,6 +83,8 @@ def lib():
         L.oc_mg_free.argtypes = [C.c_void_p]
         L.oc_l2_error.argtypes = [C.c_int, C.POINTER(OcPatch), _i32p, C.c_int, _f64p]
         L.oc_l2_error.restype = C.c_double
+        L.oc_h1_error.argtypes = [C.c_int, C.POINTER(OcPatch), _i32p, C.c_int, _f64p]
+        L.oc_h1_error.restype = C.c_double
         _lib = L
     return _lib
 
@@ -313,4 +315,11 @@ class Multigrid:
 def l2_error(patch_list, dofs, nfree, sol):
     P = Patches(patch_list)
     return lib().oc_l2_error(P.n, P.arr, np.ascontiguousarray(dofs, np.int32), nfree,
+                             np.ascontiguousarray(sol, np.float64))
+
+
+def h1_error(patch_list, dofs, nfree, sol):
+    """H1 seminorm error of the discrete solution against prod sin(pi x_r)."""
+    P = Patches(patch_list)
+    return lib().oc_h1_error(P.n, P.arr, np.ascontiguousarray(dofs, np.int32), nfree,
                              np.ascontiguousarray(sol, np.float64))

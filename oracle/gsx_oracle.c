@@ -1176,6 +1176,59 @@ void oc_mg_free(oc_mg *mg)
 /* L2 error against u = prod sin(pi x_r): gsExprEvaluator::integral of
  * (u_h - u)^2 meas (src/gsAssembler/gsExprEvaluator.h:382-450,
  * examples/poisson2_example.cpp:173-176). */
+/* H1 seminorm error |u_h - u|_{H1} against u = prod sin(pi x_r) with the
+ * assembly quadrature: the second term of the error printed by
+ * examples/poisson2_example.cpp:173-176 (igrad(u_ex) - igrad(u_sol, G)),
+ * integrated as gsExprEvaluator::integral (src/gsAssembler/gsExprEvaluator.h:
+ * 382-450); physical gradients as in the stiffness (gsExpressions.h:
+ * 3263-3297). */
+double oc_h1_error(int npatch, const oc_patch *patches, const int *dofs,
+                   int nfree, const double *sol)
+{
+    const double pi = 3.14159265358979323846;
+    elemdata *D = ed_get();
+    double total = 0.0;
+    int off = 0;
+    for (int pi_ = 0; pi_ < npatch; ++pi_) {
+        const oc_patch *P = &patches[pi_];
+        const int d = P->d;
+        elem1d E[3];
+        int nel = 1, nbas = 1;
+        for (int k = 0; k < d; ++k) {
+            elem1d_build(P, k, &E[k]);
+            nel *= E[k].n;
+            nbas *= oc_patch_nbasis(P, k);
+        }
+        for (int e = 0; e < nel; ++e) {
+            elem_eval(P, E, e, D);
+            double esum = 0.0;
+            for (int q = 0; q < D->nq; ++q) {
+                double gp[3] = {0, 0, 0};   /* parametric gradient of u_h */
+                for (int b = 0; b < D->nb; ++b) {
+                    const int g = dofs[off + D->act[b]];
+                    if (g >= nfree) continue;
+                    for (int a = 0; a < d; ++a) gp[a] += sol[g] * D->dN[(b * d + a) * D->nq + q];
+                }
+                const double *Ji = D->Jinv + q * 9;
+                double diff2 = 0.0;
+                for (int c = 0; c < d; ++c) {
+                    double gh = 0.0;
+                    for (int a = 0; a < d; ++a) gh += gp[a] * Ji[a * 3 + c];
+                    double ge = pi * cos(pi * D->x[q * 3 + c]);
+                    for (int r = 0; r < d; ++r)
+                        if (r != c) ge *= sin(pi * D->x[q * 3 + r]);
+                    diff2 += (gh - ge) * (gh - ge);
+                }
+                esum += D->w[q] * (diff2 * D->meas[q]);
+            }
+            total += esum;
+        }
+        for (int k = 0; k < d; ++k) elem1d_free(&E[k]);
+        off += nbas;
+    }
+    return sqrt(total);
+}
+
 double oc_l2_error(int npatch, const oc_patch *patches, const int *dofs,
                    int nfree, const double *sol)
 {

@@ -36,7 +36,10 @@
  *                          spline at sample points), binomial subdivision masks.
  *   oc_mg_*             -- Galerkin identity P^T A_f P = A_c (nested spaces),
  *                          two-grid cycle against numpy, MG-PCG convergence.
- *   oc_l2_error         -- convergence order p+1 on the manufactured solution.
+ *   oc_l2_error         -- convergence order p+1 on the manufactured solution,
+ *                          ||u||_L2 of the zero solution in closed form.
+ *   oc_h1_error         -- convergence order p, |u|_H1 of the zero solution
+ *                          in closed form (pi/sqrt(2) in 2D, pi sqrt(3/8) in 3D).
  */
 #ifndef GSX_ORACLE_H
 #define GSX_ORACLE_H
@@ -144,6 +147,9 @@ oc_csr *oc_transfer_multipatch(int npatch, const oc_patch *coarse,
 /* L2 error of the discrete solution (free coefficients, eliminated = 0)
  * against u = prod sin(pi x_r), with the assembly quadrature. */
 double oc_l2_error(int npatch, const oc_patch *patches, const int *dofs,
+                   int nfree, const double *sol);
+/* H1 seminorm error against the same u (examples/poisson2_example.cpp:173-176). */
+double oc_h1_error(int npatch, const oc_patch *patches, const int *dofs,
                    int nfree, const double *sol);
 
 #ifdef __cplusplus

@@ -301,3 +301,36 @@ def test_mixed_tiny_patches_converge(ctx, d, p, n):
     A.close()
     b.close()
     gp.close()
+
+
+@pytest.mark.parametrize("name", ["square", "grid", "cube"])
+def test_error_norms_match_oracle(ctx, name):
+    """L2 and H1-seminorm errors of a discrete solution (device quadrature)
+    equal the oracle's on the same coefficients."""
+    prob = {"square": problems.poisson_square(n=16, p=3),
+            "grid": problems.square_grid(2, 2, n=8, p=2),
+            "cube": _noisy_patch_problem(3, 3, 6, seed=5)}[name]
+    if name == "cube":
+        prob.patches[0].kappa = None
+    gp = gsx.Problem(ctx, prob)
+    A, b = gp.assemble(gsx.PATTERN_EXPR, gsx.FP64)
+    x = ctx.array(gp.nfree).zero()
+    A.cg_solve(b, x, tol=1e-12, maxit=5000)
+    l2, h1 = gp.error_norms(x)
+    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    xs = x.download()
+    assert l2 == pytest.approx(O.l2_error(prob.patches, dofs, nfree, xs), rel=1e-10)
+    assert h1 == pytest.approx(O.h1_error(prob.patches, dofs, nfree, xs), rel=1e-10)
+    assert gp.l2_error(x) == l2
+    z = ctx.array(gp.nfree).zero()
+    l2z, h1z = gp.error_norms(z)
+    # u_h = 0: the norms of u itself; closed forms on the affine domains
+    # ([0,1]^2 and [0,2]^2: ||u||^2 = 1/4 and 1, |u|_1^2 = pi^2/2 and 2 pi^2)
+    if name == "square":
+        assert l2z == pytest.approx(0.5, rel=1e-7) and h1z == pytest.approx(np.pi / np.sqrt(2), rel=1e-7)
+    if name == "grid":
+        assert l2z == pytest.approx(1.0, rel=1e-7) and h1z == pytest.approx(np.pi * np.sqrt(2), rel=1e-7)
+    assert h1z == pytest.approx(O.h1_error(prob.patches, dofs, nfree, np.zeros(nfree)), rel=1e-10)
+    for a in (A, b, x, z):
+        a.close()
+    gp.close()

@@ -549,3 +549,28 @@ def test_l2_convergence_rate():
         errs.append(O.l2_error(prob.patches, dofs, nfree, x))
     rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
     assert np.all(rates > 2.8), rates   # p+1 = 3 (the reference asserts > 1.85)
+
+
+@pytest.mark.parametrize("d,p,n", [(2, 3, 16), (3, 3, 6)])
+def test_error_norms_of_zero_solution_closed_form(d, p, n):
+    """With u_h = 0 the errors are the norms of u = prod sin(pi x_r):
+    ||u||_L2 = 2^(-d/2), |u|_H1 = pi sqrt(d) 2^(-d/2) (up to the Gauss rule's
+    error on sin^2, tiny at these sizes)."""
+    P = problems.box_patch(d, p, n)
+    prob = problems.Problem([P], problems.all_sides(d), [])
+    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    z = np.zeros(nfree)
+    assert O.l2_error(prob.patches, dofs, nfree, z) == pytest.approx(2.0 ** (-d / 2), rel=1e-7)
+    assert O.h1_error(prob.patches, dofs, nfree, z) == pytest.approx(np.pi * np.sqrt(d) * 2.0 ** (-d / 2), rel=1e-7)
+
+
+def test_h1_convergence_rate():
+    errs = []
+    for n in (4, 8, 16):
+        prob = problems.poisson_square(n, 2)
+        dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+        A, b = O.assemble(prob.patches, dofs, nfree, 0)
+        x = np.linalg.solve(A.toarray(), b)
+        errs.append(O.h1_error(prob.patches, dofs, nfree, x))
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(rates > 1.9) and np.all(rates < 2.3), rates   # order p = 2
