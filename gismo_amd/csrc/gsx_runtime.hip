@@ -1560,7 +1560,7 @@ static gsx_status cg_grid(gsx_matrix *A, int precond, double tau, const double *
         KScope k(c, "cg_init");
         HIP_OK(c, gsx::launch_scatter_f64_to_grid(n, f2g, b, bg, s));
         HIP_OK(c, gsx::launch_scatter_f64_to_grid(n, f2g, x, xg, s));
-        HIP_OK(c, gsx::launch_sspmv_f64(S.d, S.p, 2, n, f2g, sy, sz, sv, xg, rg, bg, P1, nullptr, s));
+        HIP_OK(c, gsx::launch_sspmv_f64(S.d, S.p, 2, n, f2g, sy, sz, sv, xg, rg, bg, P1, nullptr, 0, s));
         HIP_OK(c, gsx::launch_cg2_precond_f64((int)N, bg, zg, nullptr, 1.0, P0, s));   // ||b||^2
         HIP_OK(c, gsx::launch_cg_scalar(0, P0, nv, P1, nsp, st, hist, s));
     }
@@ -1577,12 +1577,12 @@ static gsx_status cg_grid(gsx_matrix *A, int precond, double tau, const double *
             for (; k < kend; ++k) {
                 {
                     KScope ks(c, "spmv_f64");
-                    HIP_OK(c, gsx::launch_sspmv_f64(S.d, S.p, 1, n, f2g, sy, sz, sv, pg, tg, nullptr, P0, st, s));
+                    HIP_OK(c, gsx::launch_sspmv_f64(S.d, S.p, 1, n, f2g, sy, sz, sv, pg, tg, nullptr, P0, st, k, s));
                 }
                 KScope kv(c, "cg_vec");
-                HIP_OK(c, gsx::launch_cg2_update_f64((int)N, xg, rg, pg, tg, zg, jac ? dg : nullptr, tau, P0,
-                                                     nsp, P1, P2, st, k, s));
-                HIP_OK(c, gsx::launch_cg2_direction_f64((int)N, pg, zg, P1, P2, nv, st, k, hist, s));
+                HIP_OK(c, gsx::launch_cg2_update_f64((int)N, xg, rg, pg, tg, zg, jac ? dg : nullptr, tau, P1, P2,
+                                                     st, k, hist, s));
+                HIP_OK(c, gsx::launch_cg2_direction_f64((int)N, pg, zg, st, k, s));
             }
             GSX_TRY(read_state(c, w, h));
             if (h.done) break;
@@ -1610,16 +1610,16 @@ static gsx_status cg_grid(gsx_matrix *A, int precond, double tau, const double *
                                                         p32, t32, nullptr, P0, st, k, s));
                 }
                 KScope kv(c, "cg_vec_f32");
-                HIP_OK(c, gsx::launch_cg2_update_f32((int)N, xc, r32, p32, t32, z32, jac ? d32 : nullptr, tau, P0,
-                                                     nsp, P1, P2, st, k, s));
-                HIP_OK(c, gsx::launch_cg2_direction_f32_mixed((int)N, p32, z32, P1, P2, nv, st, k, s));
+                HIP_OK(c, gsx::launch_cg2_update_f32_mixed((int)N, xc, r32, p32, t32, z32, jac ? d32 : nullptr, tau,
+                                                           P1, P2, st, k, s));
+                HIP_OK(c, gsx::launch_cg2_direction_f32((int)N, p32, z32, st, k, s));
             }
             GSX_TRY(read_state(c, w, h));
             k = h.it;
             if (h.need_update || h.done == 2 || h.err < tol) {
                 KScope kr(c, "reliable_update");
                 HIP_OK(c, gsx::launch_accum_grid(N, xg, xc, s));
-                HIP_OK(c, gsx::launch_sspmv_f64(S.d, S.p, 2, n, f2g, sy, sz, sv, xg, rg, bg, P1, nullptr, s));
+                HIP_OK(c, gsx::launch_sspmv_f64(S.d, S.p, 2, n, f2g, sy, sz, sv, xg, rg, bg, P1, nullptr, 0, s));
                 HIP_OK(c, gsx::launch_cg_reliable(P1, nsp, st, nullptr, s));
                 GSX_TRY(read_state(c, w, h));
                 if (h.done == 1 || h.done == 2) break;

@@ -46,6 +46,43 @@ __device__ inline double sum_partials(const double *__restrict__ p, int np, doub
     return bsum256(a, s);
 }
 
+// True in exactly one block of the launch: the last to arrive at *counter.
+// Every block publishes its partial before arriving, so the last block sees
+// all of them (read through to L2 below); it resets the counter for the
+// next launch.  The sum is in partial order, whichever block is last.
+__device__ inline bool arrive_last(unsigned int *counter)
+{
+    __shared__ int last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned int prev = atomicAdd(counter, 1u);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last) __threadfence();
+    return last;
+}
+
+__device__ inline double sum_partials_l2(const double *p, int np, double *s)
+{
+    double a = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256) a += __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return bsum256(a, s);
+}
+
+// last block of a CG SpMV launch: pAp and alpha = rz_k / pAp
+__device__ inline void finish_alpha(CGState *st, const double *partial, int k, double *s)
+{
+    if (!arrive_last(&st->arrive[0])) return;
+    const double pap = sum_partials_l2(partial, gridDim.x, s);
+    if (threadIdx.x == 0) {
+        st->pap = pap;
+        st->alpha = st->rzs[k & 1] / pap;
+        st->arrive[0] = 0;
+    }
+}
+
 template <typename T>
 __global__ void k_fill(T *__restrict__ a, int64_t n, T v)
 {
@@ -101,7 +138,7 @@ __global__ __launch_bounds__(256) void k_sspmv(int nfree, const int *__restrict_
                                                const T *__restrict__ x, T *__restrict__ y,
                                                const T *__restrict__ b,
                                                double *__restrict__ partial,
-                                               const CGState *__restrict__ st)
+                                               CGState *__restrict__ st, int k)
 {
     __shared__ double s[256];
     if (st && st->done) return;
@@ -147,6 +184,7 @@ __global__ __launch_bounds__(256) void k_sspmv(int nfree, const int *__restrict_
         const double tot = bsum256(contrib, s);
         if (threadIdx.x == 0) partial[blockIdx.x] = tot;
     }
+    if (MODE == 1 && st) finish_alpha(st, partial, k, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -169,7 +207,7 @@ __global__ __launch_bounds__(256) void k_sspmv_sym(int nfree, const int *__restr
                                                    const T *__restrict__ h, const T *__restrict__ x,
                                                    T *__restrict__ y, const T *__restrict__ b,
                                                    double *__restrict__ partial,
-                                                   const CGState *__restrict__ st, int k)
+                                                   CGState *__restrict__ st, int k)
 {
     __shared__ double s[256];
     if (st && (st->done || (st->flag_next && st->flag_next <= k))) return;
@@ -223,6 +261,7 @@ __global__ __launch_bounds__(256) void k_sspmv_sym(int nfree, const int *__restr
         const double tot = bsum256(contrib, s);
         if (threadIdx.x == 0) partial[blockIdx.x] = tot;
     }
+    if (MODE == 1 && st) finish_alpha(st, partial, k, s);
 }
 
 // full FP64 stencil -> FP32 half stencil (slots from the centre on)
@@ -243,24 +282,24 @@ __global__ void k_stencil_half(int nfree, const double *__restrict__ s64, float 
 // grid-layout CG vector kernels (n = grid size; diag = 1 at eliminated
 // functions so z stays 0 there)
 // ---------------------------------------------------------------------------
-template <typename T, typename TX>
+// x += alpha p; r -= alpha t; z = tau r / diag; partials (r, r), (r, z).
+// The last block to finish sums them and takes the iteration's decisions:
+// MIXED == 0: the reference stopping rule (err < tol stops before the
+// direction update); MIXED == 1: the FP32 recurrence of the mixed solver
+// never stops on its own residual but flags a reliable update (and pauses
+// from iteration k + 1 on until it has run).
+template <typename T, typename TX, int MIXED>
 __global__ __launch_bounds__(256) void k_cg2_update(int n, TX *__restrict__ x, T *__restrict__ r,
                                                     const T *__restrict__ p, const T *__restrict__ t,
                                                     T *__restrict__ z, const T *__restrict__ diag,
-                                                    double tau, const double *__restrict__ pap_parts,
-                                                    int npap, double *__restrict__ part_rr,
+                                                    double tau, double *__restrict__ part_rr,
                                                     double *__restrict__ part_rz,
-                                                    CGState *__restrict__ st, int k)
+                                                    CGState *__restrict__ st, int k,
+                                                    double *__restrict__ hist)
 {
     __shared__ double s[256];
     if (st->done || (st->flag_next && st->flag_next <= k)) return;
-    const double pap = sum_partials(pap_parts, npap, s);
-    const double alpha_d = st->rzs[k & 1] / pap;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        st->pap = pap;
-        st->alpha = alpha_d;
-    }
-    const T alpha = (T)alpha_d;
+    const T alpha = (T)st->alpha;
     const T ta = (T)tau;
     double rr = 0.0, rz = 0.0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
@@ -278,46 +317,36 @@ __global__ __launch_bounds__(256) void k_cg2_update(int n, TX *__restrict__ x, T
         part_rr[blockIdx.x] = a;
         part_rz[blockIdx.x] = c;
     }
+    if (!arrive_last(&st->arrive[1])) return;
+    const double rrs = sum_partials_l2(part_rr, gridDim.x, s);
+    const double rzn = sum_partials_l2(part_rz, gridDim.x, s);
+    if (threadIdx.x != 0) return;
+    st->arrive[1] = 0;
+    const double err = sqrt(rrs) / st->bnorm;
+    st->it = k + 1;
+    st->rr = rrs;
+    st->err = err;
+    if (hist) hist[k + 1] = err;
+    if (!MIXED && err < st->tol) {
+        st->done = 1;
+        return;
+    }
+    st->beta = rzn / st->rzs[k & 1];
+    st->rzs[(k + 1) & 1] = rzn;
+    if (MIXED && (rrs < st->rel_since_update * st->rr_ref || err < st->tol)) {
+        st->need_update = 1;
+        st->flag_next = k + 1;
+    }
+    if (k + 1 >= st->maxit) st->done = 2;
 }
 
-// MIXED == 0: reference stopping rule (err < tol stops, before the direction
-// update).  MIXED == 1: FP32 recurrence of the mixed solver: never stops on
-// the recurrence residual, flags a reliable update instead.
-template <typename T, int MIXED>
+// p = z + beta p with the beta of the update launch (skipped once stopped)
+template <typename T>
 __global__ __launch_bounds__(256) void k_cg2_direction(int n, T *__restrict__ p, const T *__restrict__ z,
-                                                       const double *__restrict__ part_rr,
-                                                       const double *__restrict__ part_rz, int np,
-                                                       CGState *__restrict__ st, int k,
-                                                       double *__restrict__ hist)
+                                                       const CGState *__restrict__ st, int k)
 {
-    __shared__ double s[256];
-    // flag_next is only ever set to k + 1 below, so the blocks of this
-    // launch read the same decision whether or not block 0 has written it
     if (st->done || (st->flag_next && st->flag_next <= k)) return;
-    const double rr = sum_partials(part_rr, np, s);
-    const double rzn = sum_partials(part_rz, np, s);
-    const double err = sqrt(rr) / st->bnorm;
-    const bool conv = !MIXED && err < st->tol;
-    const double beta = rzn / st->rzs[k & 1];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        st->it = k + 1;
-        st->rr = rr;
-        st->err = err;
-        if (hist) hist[k + 1] = err;
-        if (conv) {
-            st->done = 1;
-        } else {
-            st->beta = beta;
-            st->rzs[(k + 1) & 1] = rzn;
-            if (MIXED && (rr < st->rel_since_update * st->rr_ref || err < st->tol)) {
-                st->need_update = 1;
-                st->flag_next = k + 1;
-            }
-            if (k + 1 >= st->maxit) st->done = 2;
-        }
-    }
-    if (conv) return;
-    const T tb = (T)beta;
+    const T tb = (T)st->beta;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = z[i] + tb * p[i];
 }
 
@@ -414,10 +443,10 @@ hipError_t launch_gather_f64_from_grid(int nfree, const int *f2g, const double *
 template <typename V, typename T>
 static hipError_t sspmv_dispatch(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz,
                                  const V *sv, const T *x, T *y, const T *b, double *part,
-                                 const CGState *st, hipStream_t s)
+                                 CGState *st, int k, hipStream_t s)
 {
     const dim3 grid(sspmv_blocks(nfree)), blk(256);
-#define GSX_S(D, P, M) if (d == D && p == P && mode == M) { hipLaunchKernelGGL((k_sspmv<D, P, V, T, M>), grid, blk, 0, s, nfree, f2g, sy, sz, sv, x, y, b, part, st); return hipGetLastError(); }
+#define GSX_S(D, P, M) if (d == D && p == P && mode == M) { hipLaunchKernelGGL((k_sspmv<D, P, V, T, M>), grid, blk, 0, s, nfree, f2g, sy, sz, sv, x, y, b, part, st, k); return hipGetLastError(); }
 #define GSX_S3(D, P) GSX_S(D, P, 0) GSX_S(D, P, 1) GSX_S(D, P, 2)
     GSX_S3(2, 1) GSX_S3(2, 2) GSX_S3(2, 3) GSX_S3(2, 4)
     GSX_S3(3, 1) GSX_S3(3, 2) GSX_S3(3, 3) GSX_S3(3, 4)
@@ -428,16 +457,9 @@ static hipError_t sspmv_dispatch(int d, int p, int mode, int nfree, const int *f
 
 hipError_t launch_sspmv_f64(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz,
                             const double *sv, const double *x, double *y, const double *b,
-                            double *part, const CGState *st, hipStream_t s)
+                            double *part, CGState *st, int k, hipStream_t s)
 {
-    return sspmv_dispatch<double, double>(d, p, mode, nfree, f2g, sy, sz, sv, x, y, b, part, st, s);
-}
-
-hipError_t launch_sspmv_f32(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz,
-                            const float *sv, const float *x, float *y, const float *b,
-                            double *part, const CGState *st, hipStream_t s)
-{
-    return sspmv_dispatch<float, float>(d, p, mode, nfree, f2g, sy, sz, sv, x, y, b, part, st, s);
+    return sspmv_dispatch<double, double>(d, p, mode, nfree, f2g, sy, sz, sv, x, y, b, part, st, k, s);
 }
 
 int stencil_half_slots(int d, int p)
@@ -459,7 +481,7 @@ hipError_t launch_stencil_half(int d, int p, int nfree, const double *s64, float
 
 hipError_t launch_sspmv_sym_f32(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz, int fy,
                                 int fz, int64_t hstride, const float *h, const float *x, float *y,
-                                const float *b, double *part, const CGState *st, int k, hipStream_t s)
+                                const float *b, double *part, CGState *st, int k, hipStream_t s)
 {
     const dim3 grid(sspmv_blocks(nfree)), blk(256);
 #define GSX_Y(D, P, M) if (d == D && p == P && mode == M) { hipLaunchKernelGGL((k_sspmv_sym<D, P, float, M>), grid, blk, 0, s, nfree, f2g, sy, sz, fy, fz, hstride, h, x, y, b, part, st, k); return hipGetLastError(); }
@@ -472,38 +494,32 @@ hipError_t launch_sspmv_sym_f32(int d, int p, int mode, int nfree, const int *f2
 }
 
 hipError_t launch_cg2_update_f64(int n, double *x, double *r, const double *p, const double *t,
-                                 double *z, const double *diag, double tau, const double *pap,
-                                 int npap, double *prr, double *prz, CGState *st, int k, hipStream_t s)
+                                 double *z, const double *diag, double tau, double *prr, double *prz,
+                                 CGState *st, int k, double *hist, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cg2_update<double, double>), dim3(vec_blocks(n)), dim3(256), 0, s, n, x, r, p, t, z,
-                       diag, tau, pap, npap, prr, prz, st, k);
+    hipLaunchKernelGGL((k_cg2_update<double, double, 0>), dim3(vec_blocks(n)), dim3(256), 0, s, n, x, r, p, t, z,
+                       diag, tau, prr, prz, st, k, hist);
     return hipGetLastError();
 }
 
-hipError_t launch_cg2_update_f32(int n, float *x, float *r, const float *p, const float *t, float *z,
-                                 const float *diag, double tau, const double *pap, int npap,
-                                 double *prr, double *prz, CGState *st, int k, hipStream_t s)
+hipError_t launch_cg2_update_f32_mixed(int n, float *x, float *r, const float *p, const float *t, float *z,
+                                       const float *diag, double tau, double *prr, double *prz, CGState *st,
+                                       int k, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cg2_update<float, float>), dim3(vec_blocks(n)), dim3(256), 0, s, n, x, r, p, t, z,
-                       diag, tau, pap, npap, prr, prz, st, k);
+    hipLaunchKernelGGL((k_cg2_update<float, float, 1>), dim3(vec_blocks(n)), dim3(256), 0, s, n, x, r, p, t, z,
+                       diag, tau, prr, prz, st, k, nullptr);
     return hipGetLastError();
 }
 
-hipError_t launch_cg2_direction_f64(int n, double *p, const double *z, const double *prr,
-                                    const double *prz, int np, CGState *st, int k, double *hist,
-                                    hipStream_t s)
+hipError_t launch_cg2_direction_f64(int n, double *p, const double *z, const CGState *st, int k, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cg2_direction<double, 0>), dim3(vec_blocks(n)), dim3(256), 0, s, n, p, z, prr, prz,
-                       np, st, k, hist);
+    hipLaunchKernelGGL((k_cg2_direction<double>), dim3(vec_blocks(n)), dim3(256), 0, s, n, p, z, st, k);
     return hipGetLastError();
 }
 
-hipError_t launch_cg2_direction_f32_mixed(int n, float *p, const float *z, const double *prr,
-                                          const double *prz, int np, CGState *st, int k,
-                                          hipStream_t s)
+hipError_t launch_cg2_direction_f32(int n, float *p, const float *z, const CGState *st, int k, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cg2_direction<float, 1>), dim3(vec_blocks(n)), dim3(256), 0, s, n, p, z, prr, prz,
-                       np, st, k, nullptr);
+    hipLaunchKernelGGL((k_cg2_direction<float>), dim3(vec_blocks(n)), dim3(256), 0, s, n, p, z, st, k);
     return hipGetLastError();
 }
 
