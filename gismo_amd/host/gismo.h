@@ -424,11 +424,32 @@ public:
     }
     const std::vector<Interface> &interfaces() const { return m_if; }
     const std::vector<std::pair<int, int>> &boundaries() const { return m_bdr; }
+    /// boundary sides listed under a name (e.g. read from an XML file)
+    const std::map<std::string, std::vector<std::pair<int, int>>> &namedBoundaries() const { return m_named; }
+    /// topology given explicitly (as in an XML MultiPatch); sides neither in
+    /// an interface nor listed are added to the boundary
+    void setTopology(std::vector<Interface> ifs, std::vector<std::pair<int, int>> bdr,
+                     std::map<std::string, std::vector<std::pair<int, int>>> named = {})
+    {
+        m_if = std::move(ifs);
+        m_named = std::move(named);
+        const int d = dim();
+        std::vector<std::vector<int>> used(m_p.size(), std::vector<int>(2 * d + 1, 0));
+        for (auto &f : m_if) used.at(f.p1).at(f.s1) = used.at(f.p2).at(f.s2) = 1;
+        for (auto &b : bdr) used.at(b.first).at(b.second) = 2;
+        m_bdr = std::move(bdr);
+        for (int k = 0; k < (int)m_p.size(); ++k)
+            for (int s = 1; s <= 2 * d; ++s)
+                if (!used[k][s]) m_bdr.push_back({k, s});
+        std::sort(m_bdr.begin(), m_bdr.end());
+        m_bdr.erase(std::unique(m_bdr.begin(), m_bdr.end()), m_bdr.end());
+    }
 
 private:
     std::vector<gsTensorBSplinePatch> m_p;
     std::vector<Interface> m_if;
     std::vector<std::pair<int, int>> m_bdr;
+    std::map<std::string, std::vector<std::pair<int, int>>> m_named;
 };
 
 /// Discretisation bases, one tensor B-spline basis per patch.

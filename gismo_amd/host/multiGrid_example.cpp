@@ -4,6 +4,7 @@
 // drop-in classes of gismo.h.  The domain is built with gsNurbsCreator:
 //   -d 2|3                unit square / unit cube (default 2)
 //   --grid NX NY          NX x NY glued unit squares (2D)
+//   -g FILE               G+Smo XML file holding the (multi)patch geometry
 //   -r N                  refinement steps (default 5)
 //   -p N                  spline degree (default 2)
 //   -l N                  multigrid levels (default 3)
@@ -16,6 +17,7 @@
 //   --mixed               FP32 cycle and CG iterations, FP64 residual updates
 // Prints the iteration count, the residual history and the L2 error.
 #include "gismo.h"
+#include "gsXml.h"
 
 #include <cstdio>
 #include <cstdlib>
@@ -27,7 +29,7 @@ int main(int argc, char *argv[])
 {
     index_t refinements = 5, degree = 2, levels = 3, pre = 1, post = 1, maxIt = 100, dim = 2, gx = 0, gy = 0;
     real_t damping = 0.8, tol = 1e-8;
-    std::string solver = "cg";
+    std::string solver = "cg", geometry;
     bool mixed = false;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
@@ -40,6 +42,7 @@ int main(int argc, char *argv[])
             else if (a == "-p") degree = std::atoi(next());
             else if (a == "-l") levels = std::atoi(next());
             else if (a == "-d") dim = std::atoi(next());
+            else if (a == "-g") geometry = next();
             else if (a == "--grid") { gx = std::atoi(next()); gy = std::atoi(next()); }
             else if (a == "--MG.NumPreSmooth") pre = std::atoi(next());
             else if (a == "--MG.NumPostSmooth") post = std::atoi(next());
@@ -56,7 +59,10 @@ int main(int argc, char *argv[])
     }
     try {
         gsMultiPatch<> mp;
-        if (gx > 0) mp = gsNurbsCreator<>::BSplineSquareGrid(gx, gy);
+        if (!geometry.empty()) {
+            gsFileData<> fd(geometry);
+            fd.getFirst(mp);
+        } else if (gx > 0) mp = gsNurbsCreator<>::BSplineSquareGrid(gx, gy);
         else {
             mp.addPatch(dim == 3 ? gsNurbsCreator<>::BSplineCube() : gsNurbsCreator<>::BSplineSquare());
             mp.computeTopology();

@@ -4,6 +4,7 @@
 // built with gsNurbsCreator instead of read from an XML file:
 //   -d 2|3        unit square / unit cube (default 2)
 //   --grid NX NY  NX x NY unit squares glued at their interfaces (2D)
+//   -f FILE       G+Smo XML file holding the (multi)patch geometry
 //   -e N          degree elevation steps (default 1: degree 2)
 //   -r N          refinement loops (default 5)
 //   --last        solve only on the finest level
@@ -12,6 +13,7 @@
 // Prints one line per level: dofs, CG iterations, relative residual, L2
 // error; then the convergence rates.
 #include "gismo.h"
+#include "gsXml.h"
 
 #include <chrono>
 #include <cstdio>
@@ -24,12 +26,14 @@ int main(int argc, char *argv[])
 {
     index_t numRefine = 5, numElevate = 1, dim = 2, gx = 0, gy = 0;
     bool last = false, full = false, mixed = false;
+    std::string fn;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         if (a == "-r" && i + 1 < argc) numRefine = std::atoi(argv[++i]);
         else if (a == "-e" && i + 1 < argc) numElevate = std::atoi(argv[++i]);
         else if (a == "-d" && i + 1 < argc) dim = std::atoi(argv[++i]);
         else if (a == "--grid" && i + 2 < argc) { gx = std::atoi(argv[++i]); gy = std::atoi(argv[++i]); }
+        else if (a == "-f" && i + 1 < argc) fn = argv[++i];
         else if (a == "--last") last = true;
         else if (a == "--full") full = true;
         else if (a == "--mixed") mixed = true;
@@ -37,7 +41,11 @@ int main(int argc, char *argv[])
     }
     try {
         gsMultiPatch<> mp;
-        if (gx > 0) mp = gsNurbsCreator<>::BSplineSquareGrid(gx, gy);
+        if (!fn.empty()) {
+            gsFileData<> fd(fn);
+            std::printf("Loaded file %s\n", fd.lastPath().c_str());
+            fd.getFirst(mp);
+        } else if (gx > 0) mp = gsNurbsCreator<>::BSplineSquareGrid(gx, gy);
         else {
             mp.addPatch(dim == 3 ? gsNurbsCreator<>::BSplineCube() : gsNurbsCreator<>::BSplineSquare());
             mp.computeTopology();

@@ -1,0 +1,174 @@
+// Host-side checks of gsXml.h (no GPU): the reference's own XML inputs are
+// read, their interface records agree with the topology found by corner
+// matching, the refined Fichera corner has the free-DOF count the
+// reference's gsBasisComponents_test asserts (117, no Dirichlet sides), and
+// matrices, sparse matrices and multipatches survive a write/read round trip.
+// Prints "name value..." lines; exits nonzero on a failed check.
+#include "gsXml.h"
+
+#include <cstdio>
+#include <set>
+#include <tuple>
+
+using namespace gismo;
+
+static int failures = 0;
+#define CHECK(c)                                                    \
+    do {                                                            \
+        if (!(c)) {                                                 \
+            std::printf("FAIL %s:%d %s\n", __FILE__, __LINE__, #c); \
+            ++failures;                                             \
+        }                                                           \
+    } while (0)
+
+typedef std::tuple<int, int, int, int, int, int, int, int> Rec;
+
+// interface as an unordered pair of (patch, side) with the tangential map
+// seen from the lower patch
+static std::set<Rec> canon(const gsMultiPatch<> &mp)
+{
+    std::set<Rec> out;
+    const int d = mp.dim();
+    for (auto f : mp.interfaces()) {
+        if (std::make_pair(f.p1, f.s1) > std::make_pair(f.p2, f.s2)) {
+            // invert the map: tangential direction j of side s2 -> direction of s1
+            const int dir1 = (f.s1 - 1) / 2, dir2 = (f.s2 - 1) / 2;
+            int t1[2], k = 0;
+            for (int j = 0; j < d; ++j)
+                if (j != dir1) t1[k++] = j;
+            int m[2] = {f.m0, f.m1}, fl[2] = {f.f0, f.f1}, nm[2] = {0, 0}, nf[2] = {0, 0};
+            int t2[2];
+            k = 0;
+            for (int j = 0; j < d; ++j)
+                if (j != dir2) t2[k++] = j;
+            for (int a = 0; a < d - 1; ++a)
+                for (int b = 0; b < d - 1; ++b)
+                    if (m[a] == t2[b]) {
+                        nm[b] = t1[a];
+                        nf[b] = fl[a];
+                    }
+            f = {f.p2, f.s2, f.p1, f.s1, nm[0], nf[0], nm[1], nf[1]};
+        }
+        out.insert(Rec(f.p1, f.s1, f.p2, f.s2, f.m0, f.f0, f.m1, f.f1));
+    }
+    return out;
+}
+
+static int64_t free_count(const gsMultiPatch<> &mp, int refine, bool dirichlet)
+{
+    gsMultiBasis<> mb(mp, true);
+    for (int r = 0; r < refine; ++r) mb.uniformRefine();
+    const int np = mp.nPatches(), d = mp.dim();
+    std::vector<int32_t> nbd(3 * np, 1), ds, ifs;
+    int64_t total = 0;
+    for (int k = 0; k < np; ++k) {
+        for (int j = 0; j < d; ++j) nbd[3 * k + j] = mb.knots(k, j).numBasis();
+        total += mb.size(k);
+    }
+    if (dirichlet)
+        for (auto &s : mp.boundaries()) { ds.push_back(s.first); ds.push_back(s.second); }
+    for (auto &f : mp.interfaces()) {
+        const int32_t r[8] = {f.p1, f.s1, f.p2, f.s2, f.m0, f.f0, f.m1, f.f1};
+        ifs.insert(ifs.end(), r, r + 8);
+    }
+    std::vector<int32_t> dofs(total);
+    int64_t nfree = 0, nglued = 0;
+    CHECK(gsx_mapper_build(d, np, nbd.data(), (int32_t)ds.size() / 2, ds.empty() ? nullptr : ds.data(),
+                           (int32_t)ifs.size() / 8, ifs.empty() ? nullptr : ifs.data(), dofs.data(), &nfree,
+                           &nglued) == GSX_OK);
+    return nfree;
+}
+
+int main(int argc, char **argv)
+{
+    const std::string dir = argc > 1 ? argv[1] : "tests/golden/xml";
+    for (const char *name : {"two_squares.xml", "ldomain.xml", "fichera.xml"}) {
+        gsFileData<> fd(dir + "/" + name);
+        gsMultiPatch<> mp;
+        fd.getFirst(mp);
+        gsMultiPatch<> mc = mp;
+        mc.computeTopology();
+        const bool same = canon(mp) == canon(mc) && mp.boundaries() == mc.boundaries();
+        CHECK(same);
+        std::printf("%s patches %d dim %d interfaces %zu boundary %zu topology_match %d\n", name,
+                    (int)mp.nPatches(), mp.dim(), mp.interfaces().size(), mp.boundaries().size(), same ? 1 : 0);
+    }
+    {
+        gsFileData<> fd(dir + "/fichera.xml");
+        gsMultiPatch<> mp;
+        fd.getFirst(mp);
+        const int64_t n = free_count(mp, 1, false);
+        std::printf("fichera_refined_free %lld\n", (long long)n);
+        CHECK(n == 117);
+    }
+    {
+        gsFileData<> fd(dir + "/square.xml");
+        gsMultiPatch<> mp;
+        fd.getFirst(mp);
+        CHECK(mp.nPatches() == 1 && mp.boundaries().size() == 4 && mp.interfaces().empty());
+    }
+    // round trips
+    {
+        gsMatrix<> M(3, 2);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 2; ++j) M(i, j) = 0.1 * (i + 1) + j / 3.0;
+        std::vector<index_t> outer = {0, 2, 3, 5}, inner = {0, 2, 1, 0, 2};
+        std::vector<real_t> val = {4.0, -1.0 / 3.0, 2.5, -1.0 / 3.0, 1e-17};
+        gsSparseMatrix<> S;
+        S.setFromCompressed(3, 3, outer, inner, val);
+        gsMultiPatch<> mp = gsNurbsCreator<>::BSplineSquareGrid(2, 2);
+        gsFileData<> out;
+        out.add(M, 7);
+        out.add(S, 8);
+        out.add(mp);
+        out.save("/tmp/gsxml_roundtrip.xml");
+        gsFileData<> in("/tmp/gsxml_roundtrip.xml");
+        gsMatrix<> M2;
+        gsSparseMatrix<> S2;
+        gsMultiPatch<> mp2;
+        in.getId(7, M2);
+        in.getId(8, S2);
+        in.getFirst(mp2);
+        bool ok = M2.rows() == 3 && M2.cols() == 2;
+        for (int i = 0; i < 3 && ok; ++i)
+            for (int j = 0; j < 2; ++j) ok = ok && M2(i, j) == M(i, j);
+        CHECK(ok);
+        bool sok = S2.nonZeros() == 5;
+        for (int k = 0; k < 5 && sok; ++k)
+            sok = S2.innerIndexPtr()[k] == inner[k] && S2.valuePtr()[k] == val[k];
+        for (int j = 0; j <= 3 && sok; ++j) sok = S2.outerIndexPtr()[j] == outer[j];
+        CHECK(sok);
+        CHECK(canon(mp2) == canon(mp) && mp2.boundaries() == mp.boundaries());
+        bool cok = mp2.nPatches() == 4;
+        for (int k = 0; k < 4 && cok; ++k)
+            for (int i = 0; i < 4; ++i)
+                for (int r = 0; r < 2; ++r) cok = cok && mp2.patch(k).coefs()(i, r) == mp.patch(k).coefs()(i, r);
+        CHECK(cok);
+        std::printf("roundtrip matrix %d sparse %d multipatch %d\n", ok ? 1 : 0, sok ? 1 : 0, cok ? 1 : 0);
+    }
+    // refused inputs
+    {
+        bool threw = false;
+        try {
+            gsFileData<> fd;
+            fd.readString("<xml><Geometry type=\"TensorNurbs2\" id=\"0\"/></xml>");
+            gsTensorBSplinePatch g;
+            fd.getId(0, g);
+        } catch (const std::runtime_error &) {
+            threw = true;
+        }
+        CHECK(threw);
+        threw = false;
+        try {
+            gsFileData<> fd;
+            fd.readString("<xml><Matrix rows=\"2\" cols=\"2\" id=\"0\">1 2 3</Matrix></xml>");
+            gsMatrix<> m;
+            fd.getId(0, m);
+        } catch (const std::runtime_error &) {
+            threw = true;
+        }
+        CHECK(threw);
+    }
+    std::printf("failures %d\n", failures);
+    return failures ? 1 : 0;
+}

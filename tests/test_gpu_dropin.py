@@ -107,3 +107,18 @@ def test_multigrid_example_matches_python_mirror(ctx):
     mg.close()
     A.close()
     gp.close()
+
+
+@pytest.mark.parametrize("fname,args,rate", [("ldomain.xml", ["-e", "1", "-r", "3"], 2.7),
+                                            ("fichera.xml", ["-e", "1", "-r", "2"], 2.5)])
+def test_poisson2_example_reads_reference_xml(fname, args, rate):
+    """The example driver reads the reference's own multipatch XML inputs
+    (L-shaped domain, Fichera corner; u = prod sin(pi x_i) vanishes on their
+    boundaries) and converges at the order p+1 = 3 in L2."""
+    path = os.path.join(ROOT, "tests", "golden", "xml", fname)
+    rows = _levels(_run("poisson2_example", "-f", path, *args))
+    l2 = np.array([r[4] for r in rows])
+    assert all(r[3] < 1e-10 for r in rows)
+    rates = np.log2(l2[:-1] / l2[1:])
+    print(fname, l2, rates, flush=True)
+    assert rates[-1] > rate, rates
