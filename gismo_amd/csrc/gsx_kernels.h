@@ -126,6 +126,11 @@ hipError_t launch_residual_f32(int lpr, int n, const int *rp, const int *ci, con
                                const float *x, const float *b, float *r, hipStream_t s);
 hipError_t launch_conv_f64_f32(int64_t n, const double *a, float *b, hipStream_t s);
 hipError_t launch_conv_f32_f64(int64_t n, const float *a, double *b, hipStream_t s);
+hipError_t launch_smooth_update_f64(int n, double *x, double *d, const double *r, const double *diag, double c1,
+                                    double c2, int first, hipStream_t s);
+hipError_t launch_smooth_update_f32(int n, float *x, float *d, const float *r, const float *diag, float c1, float c2,
+                                    int first, hipStream_t s);
+hipError_t launch_div_sq(int n, double *w, const double *diag, double *part, hipStream_t s);
 
 // structured operator + grid-layout CG (gsx_stencil.hip)
 int sspmv_blocks(int nfree);

@@ -589,4 +589,65 @@ hipError_t launch_conv_f32_f64(int64_t n, const float *a, double *b, hipStream_t
     return hipGetLastError();
 }
 
+
+// ---------------------------------------------------------------------------
+// Polynomial smoothers: d = c1 d + c2 D^-1 r (d = c2 D^-1 r on the first
+// step; D = 1 when diag is null), x += d.  Richardson: c1 = 0, c2 = tau,
+// no diagonal; Chebyshev: the three-term recurrence coefficients.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_smooth_update(int n, T *__restrict__ x, T *__restrict__ d, const T *__restrict__ r,
+                                const T *__restrict__ diag, T c1, T c2, int first)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T ri = diag ? r[i] / diag[i] : r[i];
+    const T di = first ? c2 * ri : c1 * d[i] + c2 * ri;
+    d[i] = di;
+    x[i] += di;
+}
+
+// power iteration step: w /= diag, partial sum of w^2 per block
+__global__ __launch_bounds__(256) void k_div_sq(int n, double *__restrict__ w, const double *__restrict__ diag,
+                                                double *__restrict__ part)
+{
+    __shared__ double s[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double v = 0.0;
+    if (i < n) {
+        const double wi = w[i] / diag[i];
+        w[i] = wi;
+        v = wi * wi;
+    }
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+hipError_t launch_smooth_update_f64(int n, double *x, double *d, const double *r, const double *diag, double c1,
+                                    double c2, int first, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_smooth_update<double>), dim3((n + 255) / 256), dim3(256), 0, s, n, x, d, r, diag, c1, c2,
+                       first);
+    return hipGetLastError();
+}
+
+hipError_t launch_smooth_update_f32(int n, float *x, float *d, const float *r, const float *diag, float c1, float c2,
+                                    int first, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_smooth_update<float>), dim3((n + 255) / 256), dim3(256), 0, s, n, x, d, r, diag, c1, c2,
+                       first);
+    return hipGetLastError();
+}
+
+hipError_t launch_div_sq(int n, double *w, const double *diag, double *part, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_div_sq, dim3((n + 255) / 256), dim3(256), 0, s, n, w, diag, part);
+    return hipGetLastError();
+}
+
 }  // namespace gsx

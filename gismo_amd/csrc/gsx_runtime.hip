@@ -1757,6 +1757,8 @@ struct MGLevel {
     DevPtr prp, pci, pv, pv32, rrp, rci, rv, rv32;
     DevPtr x, b, r, t;                 // FP64
     DevPtr x32, b32, r32, t32;         // FP32
+    DevPtr d, d32;                     // polynomial smoother direction
+    double lmax = 0.0;                 // Chebyshev: bound of lambda(D^-1 A)
 };
 
 struct gsx_mg {
@@ -1766,6 +1768,8 @@ struct gsx_mg {
     double tau = 1.0, cgc = 1.0;
     int precision = GSX_FP64;
     int npad = 0;
+    int smoother = GSX_SMOOTH_JACOBI;
+    double cheb_ratio = 30.0;
     DevPtr Dinv;
     DevPtr zin, zout;                  // finest FP64 staging for the mixed cycle
     ~gsx_mg()
@@ -2133,6 +2137,82 @@ extern "C" gsx_status gsx_mg_level_download(const gsx_mg *mg, int32_t lvl, int32
 
 // One multigrid step at level l on (b, x) with x = 0 on entry (the
 // preconditioner's apply).  FP64 version.
+// Polynomial smoother coefficients of sweep k (Chebyshev recurrence on
+// [lmax / ratio, lmax]; Richardson: constant tau, no diagonal).
+struct SmoothCoef {
+    double c1, c2;
+};
+static SmoothCoef smooth_coef(const gsx_mg *mg, const MGLevel &V, int k, double &rho)
+{
+    if (mg->smoother == GSX_SMOOTH_RICHARDSON) return {0.0, mg->tau};
+    const double l1 = V.lmax, l0 = l1 / mg->cheb_ratio;
+    const double theta = 0.5 * (l1 + l0), delta = 0.5 * (l1 - l0), sigma = theta / delta;
+    if (k == 0) {
+        rho = 1.0 / sigma;
+        return {0.0, 1.0 / theta};
+    }
+    const double rn = 1.0 / (2.0 * sigma - rho);
+    const SmoothCoef c{rn * rho, 2.0 * rn / delta};
+    rho = rn;
+    return c;
+}
+
+// m smoothing sweeps on level V (FP64).  Jacobi ping-pongs x and t (x may
+// come back as t); the polynomial smoothers update x in place.
+static gsx_status smooth64(gsx_mg *mg, MGLevel &V, double *&x, double *&t, const double *b, bool &zero, int m)
+{
+    gsx_ctx *c = mg->ctx;
+    hipStream_t s = c->stream;
+    const int *rp = dp<int>(V.rowptr), *ci = dp<int>(V.col);
+    const double *va = dp<double>(V.val), *dg = dp<double>(V.diag);
+    if (mg->smoother == GSX_SMOOTH_JACOBI) {
+        for (int i = 0; i < m; ++i) {
+            HIP_OK(c, gsx::launch_jacobi_step_f64(V.lpr, V.n, rp, ci, va, dg, b, x, t, mg->tau, zero ? 1 : 0, s));
+            std::swap(x, t);
+            zero = false;
+        }
+        return GSX_OK;
+    }
+    if (m > 0 && zero) HIP_OK(c, gsx::launch_fill_f64(x, V.n, 0.0, s));
+    double *r = dp<double>(V.r), *d = dp<double>(V.d);
+    double rho = 0.0;
+    for (int k = 0; k < m; ++k) {
+        HIP_OK(c, gsx::launch_residual_f64(V.lpr, V.n, rp, ci, va, x, b, r, s));
+        const SmoothCoef cf = smooth_coef(mg, V, k, rho);
+        HIP_OK(c, gsx::launch_smooth_update_f64(V.n, x, d, r, mg->smoother == GSX_SMOOTH_CHEBYSHEV ? dg : nullptr,
+                                                cf.c1, cf.c2, k == 0 ? 1 : 0, s));
+        zero = false;
+    }
+    return GSX_OK;
+}
+
+static gsx_status smooth32(gsx_mg *mg, MGLevel &V, float *&x, float *&t, const float *b, bool &zero, int m)
+{
+    gsx_ctx *c = mg->ctx;
+    hipStream_t s = c->stream;
+    const int *rp = dp<int>(V.rowptr), *ci = dp<int>(V.col);
+    const float *va = dp<float>(V.val32), *dg = dp<float>(V.diag32);
+    if (mg->smoother == GSX_SMOOTH_JACOBI) {
+        for (int i = 0; i < m; ++i) {
+            HIP_OK(c, gsx::launch_jacobi_step_f32(V.lpr, V.n, rp, ci, va, dg, b, x, t, (float)mg->tau, zero ? 1 : 0, s));
+            std::swap(x, t);
+            zero = false;
+        }
+        return GSX_OK;
+    }
+    if (m > 0 && zero) HIP_OK(c, gsx::launch_fill_f32(x, V.n, 0.0f, s));
+    float *r = dp<float>(V.r32), *d = dp<float>(V.d32);
+    double rho = 0.0;
+    for (int k = 0; k < m; ++k) {
+        HIP_OK(c, gsx::launch_residual_f32(V.lpr, V.n, rp, ci, va, x, b, r, s));
+        const SmoothCoef cf = smooth_coef(mg, V, k, rho);
+        HIP_OK(c, gsx::launch_smooth_update_f32(V.n, x, d, r, mg->smoother == GSX_SMOOTH_CHEBYSHEV ? dg : nullptr,
+                                                (float)cf.c1, (float)cf.c2, k == 0 ? 1 : 0, s));
+        zero = false;
+    }
+    return GSX_OK;
+}
+
 static gsx_status mg_cycle64(gsx_mg *mg, int l)
 {
     gsx_ctx *c = mg->ctx;
@@ -2145,15 +2225,11 @@ static gsx_status mg_cycle64(gsx_mg *mg, int l)
         return GSX_OK;
     }
     const int *rp = dp<int>(V.rowptr), *ci = dp<int>(V.col);
-    const double *va = dp<double>(V.val), *dg = dp<double>(V.diag);
+    const double *va = dp<double>(V.val);
     bool zero = true;
     {
         KScope k(c, "mg_smooth");
-        for (int i = 0; i < mg->npre; ++i) {
-            HIP_OK(c, gsx::launch_jacobi_step_f64(V.lpr, V.n, rp, ci, va, dg, b, x, t, mg->tau, zero ? 1 : 0, s));
-            std::swap(x, t);
-            zero = false;
-        }
+        GSX_TRY(smooth64(mg, V, x, t, b, zero, mg->npre));
     }
     MGLevel &C = mg->L[l - 1];
     {
@@ -2172,10 +2248,7 @@ static gsx_status mg_cycle64(gsx_mg *mg, int l)
     }
     {
         KScope k(c, "mg_smooth");
-        for (int i = 0; i < mg->npost; ++i) {
-            HIP_OK(c, gsx::launch_jacobi_step_f64(V.lpr, V.n, rp, ci, va, dg, b, x, t, mg->tau, 0, s));
-            std::swap(x, t);
-        }
+        GSX_TRY(smooth64(mg, V, x, t, b, zero, mg->npost));
     }
     if (x != dp<double>(V.x)) HIP_OK(c, hipMemcpyAsync(dp<double>(V.x), x, sizeof(double) * V.n, hipMemcpyDeviceToDevice, s));
     return GSX_OK;
@@ -2197,15 +2270,11 @@ static gsx_status mg_cycle32(gsx_mg *mg, int l)
         return GSX_OK;
     }
     const int *rp = dp<int>(V.rowptr), *ci = dp<int>(V.col);
-    const float *va = dp<float>(V.val32), *dg = dp<float>(V.diag32);
+    const float *va = dp<float>(V.val32);
     bool zero = true;
     {
         KScope k(c, "mg_smooth");
-        for (int i = 0; i < mg->npre; ++i) {
-            HIP_OK(c, gsx::launch_jacobi_step_f32(V.lpr, V.n, rp, ci, va, dg, b, x, t, (float)mg->tau, zero ? 1 : 0, s));
-            std::swap(x, t);
-            zero = false;
-        }
+        GSX_TRY(smooth32(mg, V, x, t, b, zero, mg->npre));
     }
     MGLevel &C = mg->L[l - 1];
     {
@@ -2223,10 +2292,7 @@ static gsx_status mg_cycle32(gsx_mg *mg, int l)
     }
     {
         KScope k(c, "mg_smooth");
-        for (int i = 0; i < mg->npost; ++i) {
-            HIP_OK(c, gsx::launch_jacobi_step_f32(V.lpr, V.n, rp, ci, va, dg, b, x, t, (float)mg->tau, 0, s));
-            std::swap(x, t);
-        }
+        GSX_TRY(smooth32(mg, V, x, t, b, zero, mg->npost));
     }
     if (x != dp<float>(V.x32)) HIP_OK(c, hipMemcpyAsync(dp<float>(V.x32), x, sizeof(float) * V.n, hipMemcpyDeviceToDevice, s));
     return GSX_OK;
@@ -2248,6 +2314,60 @@ static gsx_status mg_apply_dev(gsx_mg *mg, const double *r, double *z)
         GSX_TRY(mg_cycle64(mg, nl - 1));
         HIP_OK(c, hipMemcpyAsync(z, dp<double>(F.x), sizeof(double) * F.n, hipMemcpyDeviceToDevice, s));
     }
+    return GSX_OK;
+}
+
+extern "C" gsx_status gsx_mg_set_smoother(gsx_mg *mg, int32_t kind, double param)
+{
+    if (!mg || kind < GSX_SMOOTH_JACOBI || kind > GSX_SMOOTH_CHEBYSHEV) return GSX_ERR_ARG;
+    gsx_ctx *c = mg->ctx;
+    hipStream_t s = c->stream;
+    mg->smoother = kind;
+    if (kind == GSX_SMOOTH_JACOBI) return GSX_OK;
+    if (kind == GSX_SMOOTH_CHEBYSHEV) mg->cheb_ratio = param > 1.0 ? param : 30.0;
+    for (size_t l = 1; l < mg->L.size(); ++l) {
+        MGLevel &V = mg->L[l];
+        const int n = V.n;
+        if (!V.d) V.d = dalloc(c, sizeof(double) * n);
+        if (mg->precision == GSX_MIXED && !V.d32) V.d32 = dalloc(c, sizeof(float) * n);
+        if (!V.d || (mg->precision == GSX_MIXED && !V.d32)) return fail(c, GSX_ERR_OOM, "device allocation (smoother)");
+        if (kind != GSX_SMOOTH_CHEBYSHEV) continue;
+        // lambda_max(D^-1 A): 20 power iterations from v0_i = 1 / (1 + i mod 5)
+        std::vector<double> v0(n);
+        double nv = 0.0;
+        for (int i = 0; i < n; ++i) {
+            v0[i] = 1.0 / (1.0 + (double)(i % 5));
+            nv += v0[i] * v0[i];
+        }
+        nv = std::sqrt(nv);
+        for (int i = 0; i < n; ++i) v0[i] /= nv;
+        const int nb = (n + 255) / 256;
+        DevPtr v = dalloc(c, sizeof(double) * n), w = dalloc(c, sizeof(double) * n), part = dalloc(c, sizeof(double) * nb);
+        if (!v || !w || !part) return fail(c, GSX_ERR_OOM, "device allocation (power iteration)");
+        HIP_OK(c, hipMemcpyAsync(v.get(), v0.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+        std::vector<double> ph(nb);
+        double lam = 0.0;
+        for (int it = 0; it < 20; ++it) {
+            HIP_OK(c, gsx::launch_spmv_f64(V.lpr, n, dp<int>(V.rowptr), dp<int>(V.col), dp<double>(V.val), dp<double>(v),
+                                           dp<double>(w), nullptr, nullptr, 0, nullptr, s));
+            HIP_OK(c, gsx::launch_div_sq(n, dp<double>(w), dp<double>(V.diag), dp<double>(part), s));
+            HIP_OK(c, hipMemcpyAsync(ph.data(), part.get(), sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+            HIP_OK(c, hipStreamSynchronize(s));
+            double sq = 0.0;
+            for (int k = 0; k < nb; ++k) sq += ph[k];
+            lam = std::sqrt(sq);
+            HIP_OK(c, gsx::launch_axpby(n, 1.0 / lam, dp<double>(w), 0.0, dp<double>(v), s));
+        }
+        V.lmax = 1.1 * lam;
+    }
+    HIP_OK(c, hipStreamSynchronize(s));
+    return GSX_OK;
+}
+
+extern "C" gsx_status gsx_mg_smoother_bound(const gsx_mg *mg, int32_t lvl, double *lmax)
+{
+    if (!mg || !lmax || lvl < 0 || lvl >= (int)mg->L.size()) return GSX_ERR_ARG;
+    *lmax = mg->L[lvl].lmax;
     return GSX_OK;
 }
 

@@ -111,6 +111,8 @@ SIGNATURES = {
     "gsx_mg_create": [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
                       C.c_int32, _pp],
     "gsx_mg_destroy": [_vp],
+    "gsx_mg_set_smoother": [_vp, C.c_int32, C.c_double],
+    "gsx_mg_smoother_bound": [_vp, C.c_int32, _dp],
     "gsx_mg_apply": [_vp, _vp, _vp],
     "gsx_mg_level_info": [_vp, C.c_int32, _i64p, _i64p, _i64p],
     "gsx_mg_level_download": [_vp, C.c_int32, _i32p, _i32p, _dp, _i32p, _i32p, _dp],
@@ -404,14 +406,23 @@ class Matrix:
 class Multigrid:
     """Geometric multigrid preconditioner (gsMultiGridOp) on the device."""
 
+    SMOOTHERS = {"jacobi": 0, "richardson": 1, "chebyshev": 2}
+
     def __init__(self, problem, matrix, nlevels, npre=1, npost=1, damping=1.0, cgc_damping=1.0,
-                 precision=FP64):
+                 precision=FP64, smoother="jacobi", smoother_param=30.0):
         self.ctx = problem.ctx
         h = C.c_void_p()
         _check(self.ctx.h, lib().gsx_mg_create(problem.h, matrix.h, nlevels, npre, npost, 1, damping,
                                                cgc_damping, precision, C.byref(h)))
         self.h = h
         self.nlevels = nlevels
+        if smoother != "jacobi":
+            _check(self.ctx.h, lib().gsx_mg_set_smoother(h, self.SMOOTHERS[smoother], smoother_param))
+
+    def lmax(self, lvl):
+        v = C.c_double()
+        _check(self.ctx.h, lib().gsx_mg_smoother_bound(self.h, lvl, C.byref(v)))
+        return v.value
 
     def apply(self, r, z):
         _check(self.ctx.h, lib().gsx_mg_apply(self.h, r.ptr, z.ptr))

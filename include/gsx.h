@@ -51,6 +51,12 @@ typedef enum {
     GSX_PREC_MG = 2
 } gsx_precond;
 
+typedef enum {
+    GSX_SMOOTH_JACOBI = 0,      /* damped Jacobi (default)                   */
+    GSX_SMOOTH_RICHARDSON = 1,  /* x += damping (b - A x)                    */
+    GSX_SMOOTH_CHEBYSHEV = 2    /* Jacobi-preconditioned Chebyshev            */
+} gsx_smoother;
+
 typedef struct gsx_ctx gsx_ctx;
 typedef struct gsx_problem gsx_problem;
 typedef struct gsx_matrix gsx_matrix;
@@ -217,6 +223,16 @@ gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nlevels,
                          double damping, double cgc_damping, int32_t precision,
                          gsx_mg **out);
 gsx_status gsx_mg_destroy(gsx_mg *mg);
+/* Smoother of every level but the coarsest (the sweeps npre/npost of
+ * gsx_mg_create).  GSX_SMOOTH_JACOBI: gsJacobiOp::step with the damping of
+ * gsx_mg_create (src/gsSolver/gsSimplePreconditioners.h:178-202);
+ * GSX_SMOOTH_RICHARDSON: gsRichardsonOp::step, x += damping (b - A x)
+ * (same file :33-131); GSX_SMOOTH_CHEBYSHEV (no reference counterpart):
+ * npre/npost-step Chebyshev iteration on D^-1 A over [lmax/param, lmax]
+ * (param <= 1: 30), lmax = 1.1 x a 20-step power-iteration estimate. */
+gsx_status gsx_mg_set_smoother(gsx_mg *mg, int32_t kind, double param);
+/* The Chebyshev bound lmax of level lvl (0 before gsx_mg_set_smoother). */
+gsx_status gsx_mg_smoother_bound(const gsx_mg *mg, int32_t lvl, double *lmax);
 /* z = MG(r): one cycle from a zero start (gsPreconditionerOp::apply,
  * src/gsSolver/gsPreconditioner.h:79-84). */
 gsx_status gsx_mg_apply(gsx_mg *mg, const double *r_dev, double *z_dev);

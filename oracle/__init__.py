@@ -78,6 +78,10 @@ def lib():
                                    C.c_int, C.c_int, C.c_double, C.c_double]
         L.oc_mg_create.restype = C.c_void_p
         L.oc_mg_apply.argtypes = [C.c_void_p, _f64p, _f64p]
+        L.oc_mg_set_smoother.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.oc_mg_set_smoother.restype = None
+        L.oc_mg_lmax.argtypes = [C.c_void_p, C.c_int]
+        L.oc_mg_lmax.restype = C.c_double
         L.oc_mg_matrix.argtypes = [C.c_void_p, C.c_int]
         L.oc_mg_matrix.restype = C.POINTER(OcCsr)
         L.oc_mg_free.argtypes = [C.c_void_p]
@@ -285,7 +289,10 @@ def transfer_multipatch(coarse_patches, dofs_c, nfree_c, dofs_f, nfree_f):
 
 
 class Multigrid:
-    def __init__(self, A_fine, transfers, npre=1, npost=1, damping=1.0, cgc_damping=1.0):
+    SMOOTHERS = {"jacobi": 0, "richardson": 1, "chebyshev": 2}
+
+    def __init__(self, A_fine, transfers, npre=1, npost=1, damping=1.0, cgc_damping=1.0, smoother="jacobi",
+                 smoother_param=30.0):
         self._A = _CsrHolder(A_fine)
         self._P = [_CsrHolder(P) for P in transfers]
         arr = (C.POINTER(OcCsr) * max(1, len(self._P)))()
@@ -295,6 +302,12 @@ class Multigrid:
         self.nlev = len(transfers) + 1
         self.handle = lib().oc_mg_create(C.byref(self._A.s), self.nlev, arr, npre, npost,
                                          damping, cgc_damping)
+        if smoother != "jacobi":
+            lib().oc_mg_set_smoother(self.handle, self.SMOOTHERS[smoother], smoother_param)
+
+    def lmax(self, lvl):
+        """Chebyshev bound of level lvl (1.1 x power-iteration lambda_max(D^-1 A))."""
+        return lib().oc_mg_lmax(self.handle, lvl)
 
     def apply(self, r):
         r = np.ascontiguousarray(r, dtype=np.float64)

@@ -1000,6 +1000,9 @@ oc_csr *oc_transfer_multipatch(int npatch, const oc_patch *coarse,
 struct oc_mg {
     int nlev, npre, npost;
     double tau, cgc;
+    int smoother;       /* 0 Jacobi, 1 Richardson, 2 Chebyshev (Jacobi-preconditioned) */
+    double cheb_ratio;  /* Chebyshev interval [lmax / ratio, lmax] */
+    double *lmax;       /* per level: 1.1 x power-iteration estimate of lambda_max(D^-1 A) */
     oc_csr **A;   /* per level (owned except finest) */
     oc_csr **P;   /* copies of transfers */
     double **dinv;
@@ -1069,6 +1072,8 @@ oc_mg *oc_mg_create(const oc_csr *A_fine, int nlev, const oc_csr *const *P,
     mg->A = (oc_csr **)calloc(nlev, sizeof(oc_csr *));
     mg->P = (oc_csr **)calloc(nlev, sizeof(oc_csr *));
     mg->dinv = (double **)calloc(nlev, sizeof(double *));
+    mg->lmax = (double *)calloc(nlev, sizeof(double));
+    mg->cheb_ratio = 30.0;
     mg->A[nlev - 1] = csr_copy(A_fine);
     for (int l = nlev - 2; l >= 0; --l) {
         mg->P[l] = csr_copy(P[l]);
@@ -1128,6 +1133,85 @@ static void jacobi_step(const oc_mg *mg, int l, const double *b, double *x, doub
     for (int i = 0; i < A->n; ++i) x[i] += mg->tau * (b[i] - tmp[i]) / mg->dinv[l][i];
 }
 
+/* lambda_max(D^-1 A) by power iteration: v0_i = 1 / (1 + i mod 5),
+ * normalised; 20 times w = D^-1 A v, lambda = ||w||, v = w / lambda. */
+static double power_lmax(const oc_csr *A, const double *dg)
+{
+    const int n = A->n;
+    double *v = (double *)malloc(sizeof(double) * n), *w = (double *)malloc(sizeof(double) * n);
+    double nv = 0.0;
+    for (int i = 0; i < n; ++i) { v[i] = 1.0 / (1.0 + (double)(i % 5)); nv += v[i] * v[i]; }
+    nv = sqrt(nv);
+    for (int i = 0; i < n; ++i) v[i] /= nv;
+    double lam = 0.0;
+    for (int it = 0; it < 20; ++it) {
+        oc_csr_spmv(A, v, w);
+        double s = 0.0;
+        for (int i = 0; i < n; ++i) { w[i] /= dg[i]; s += w[i] * w[i]; }
+        lam = sqrt(s);
+        for (int i = 0; i < n; ++i) v[i] = w[i] / lam;
+    }
+    free(v); free(w);
+    return lam;
+}
+
+/* Smoothing sweeps of level l.  Jacobi: x += tau D^-1 (b - A x)
+ * (gsJacobiOp::step, src/gsSolver/gsSimplePreconditioners.h:178-202);
+ * Richardson: x += tau (b - A x) (gsRichardsonOp::step, same file :60-90);
+ * Chebyshev: m-step Jacobi-preconditioned Chebyshev iteration on
+ * [lmax / ratio, lmax] (no reference counterpart):
+ *   theta = (l1 + l0) / 2, delta = (l1 - l0) / 2, sigma = theta / delta,
+ *   rho_1 = 1 / sigma, d_1 = D^-1 r_0 / theta,
+ *   rho_{k+1} = 1 / (2 sigma - rho_k),
+ *   d_{k+1} = rho_{k+1} rho_k d_k + (2 rho_{k+1} / delta) D^-1 r_k,
+ *   x_{k+1} = x_k + d_{k+1}. */
+static void smooth(const oc_mg *mg, int l, const double *b, double *x, int m, double *tmp, double *d)
+{
+    const oc_csr *A = mg->A[l];
+    const int n = A->n;
+    if (mg->smoother == 0) {
+        for (int s = 0; s < m; ++s) jacobi_step(mg, l, b, x, tmp);
+        return;
+    }
+    if (mg->smoother == 1) {
+        for (int s = 0; s < m; ++s) {
+            oc_csr_spmv(A, x, tmp);
+            for (int i = 0; i < n; ++i) x[i] += mg->tau * (b[i] - tmp[i]);
+        }
+        return;
+    }
+    const double l1 = mg->lmax[l], l0 = l1 / mg->cheb_ratio;
+    const double theta = 0.5 * (l1 + l0), delta = 0.5 * (l1 - l0), sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    for (int s = 0; s < m; ++s) {
+        oc_csr_spmv(A, x, tmp);
+        double c1, c2;
+        if (s == 0) { c1 = 0.0; c2 = 1.0 / theta; }
+        else {
+            const double rn = 1.0 / (2.0 * sigma - rho);
+            c1 = rn * rho;
+            c2 = 2.0 * rn / delta;
+            rho = rn;
+        }
+        for (int i = 0; i < n; ++i) {
+            const double ri = (b[i] - tmp[i]) / mg->dinv[l][i];
+            d[i] = (s == 0 ? 0.0 : c1 * d[i]) + c2 * ri;
+            x[i] += d[i];
+        }
+    }
+}
+
+void oc_mg_set_smoother(oc_mg *mg, int kind, double param)
+{
+    mg->smoother = kind;
+    if (kind == 2) {
+        mg->cheb_ratio = param > 1.0 ? param : 30.0;
+        for (int l = 1; l < mg->nlev; ++l) mg->lmax[l] = 1.1 * power_lmax(mg->A[l], mg->dinv[l]);
+    }
+}
+
+double oc_mg_lmax(const oc_mg *mg, int lvl) { return mg->lmax[lvl]; }
+
 static void mg_step(const oc_mg *mg, int l, const double *b, double *x)
 {
     if (l == 0) { coarse_solve(mg, b, x); return; }
@@ -1136,7 +1220,8 @@ static void mg_step(const oc_mg *mg, int l, const double *b, double *x)
     double *res = (double *)malloc(sizeof(double) * nf);
     double *cr = (double *)calloc(nc, sizeof(double));
     double *cc = (double *)calloc(nc, sizeof(double));
-    for (int s = 0; s < mg->npre; ++s) jacobi_step(mg, l, b, x, tmp);
+    double *dd = (double *)calloc(nf, sizeof(double));
+    smooth(mg, l, b, x, mg->npre, tmp, dd);
     oc_csr_spmv(mg->A[l], x, res);
     for (int i = 0; i < nf; ++i) res[i] -= b[i];
     const oc_csr *P = mg->P[l - 1];
@@ -1148,8 +1233,8 @@ static void mg_step(const oc_mg *mg, int l, const double *b, double *x)
         for (int m = P->rowptr[i]; m < P->rowptr[i + 1]; ++m) s += P->val[m] * cc[P->col[m]];
         x[i] -= mg->cgc * s;
     }
-    for (int s = 0; s < mg->npost; ++s) jacobi_step(mg, l, b, x, tmp);
-    free(tmp); free(res); free(cr); free(cc);
+    smooth(mg, l, b, x, mg->npost, tmp, dd);
+    free(tmp); free(res); free(cr); free(cc); free(dd);
 }
 
 /* gsPreconditionerOp::apply: x = 0, one step (src/gsSolver/gsPreconditioner.h:79-84) */
@@ -1170,7 +1255,7 @@ void oc_mg_free(oc_mg *mg)
         if (mg->P[l]) oc_csr_free(mg->P[l]);
         free(mg->dinv[l]);
     }
-    free(mg->A); free(mg->P); free(mg->dinv); free(mg->chol); free(mg);
+    free(mg->A); free(mg->P); free(mg->dinv); free(mg->lmax); free(mg->chol); free(mg);
 }
 
 /* L2 error against u = prod sin(pi x_r): gsExprEvaluator::integral of

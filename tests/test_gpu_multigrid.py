@@ -90,3 +90,48 @@ def test_mg_pcg_matches_oracle(ctx, name):
         xs = x.download()
         assert np.linalg.norm(xs - xo) <= 1e-8 * np.linalg.norm(xo)
         mg.close()
+
+
+@pytest.mark.parametrize("name,smoother", [("grid2x2-p3", "chebyshev"), ("cube-p2", "chebyshev"),
+                                           ("grid3x2-p2", "richardson")])
+def test_polynomial_smoothers_match_oracle(ctx, name, smoother):
+    """Richardson and Chebyshev smoothing: the Chebyshev bounds (power
+    iteration) agree with the oracle's, one cycle agrees with the oracle's,
+    and MG-PCG needs the oracle's iteration count (FP64; FP32 cycle within 3)."""
+    make, ns = CASES[name]
+    levels, maps = _hierarchy(make, ns)
+    Af, bf = O.assemble(levels[-1].patches, maps[-1][0], maps[-1][1], 0)
+    Ps = [O.transfer_multipatch(levels[i].patches, maps[i][0], maps[i][1], maps[i + 1][0], maps[i + 1][1])
+          for i in range(len(levels) - 1)]
+    damping = 0.6 if smoother == "chebyshev" else 0.3 / np.abs(Af.diagonal()).max()
+    omg = O.Multigrid(Af, Ps, npre=2, npost=2, damping=damping, smoother=smoother, smoother_param=20.0)
+    xo, ito, erro = O.cg_solve(Af, bf, tol=1e-10, maxit=300, precond="mg", mg=omg)
+    gp = gsx.Problem(ctx, levels[-1])
+    A, b = gp.assemble()
+    for prec in (gsx.FP64, gsx.MIXED):
+        mg = gsx.Multigrid(gp, A, len(ns), 2, 2, damping, precision=prec, smoother=smoother, smoother_param=20.0)
+        if smoother == "chebyshev":
+            for lvl in range(1, len(ns)):
+                assert mg.lmax(lvl) == pytest.approx(omg.lmax(lvl), rel=1e-12)
+        if prec == gsx.FP64:
+            r = ctx.to_device(bf)
+            z = ctx.array(gp.nfree)
+            mg.apply(r, z)
+            zo = omg.apply(bf)
+            assert np.linalg.norm(z.download() - zo) <= 1e-11 * np.linalg.norm(zo)
+            r.close()
+            z.close()
+        x = ctx.array(gp.nfree).zero()
+        it, rel, ok = gsx.mg_cg_solve(A, mg, b, x, tol=1e-10, maxit=300)
+        print(name, smoother, prec, it, ito, flush=True)
+        assert ok and rel < 1e-10
+        if prec == gsx.FP64:
+            assert abs(it - ito) <= 1
+        else:
+            assert it <= ito + 3
+        assert np.linalg.norm(x.download() - xo) <= 1e-8 * np.linalg.norm(xo)
+        x.close()
+        mg.close()
+    A.close()
+    b.close()
+    gp.close()

@@ -574,3 +574,79 @@ def test_h1_convergence_rate():
         errs.append(O.h1_error(prob.patches, dofs, nfree, x))
     rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
     assert np.all(rates > 1.9) and np.all(rates < 2.3), rates   # order p = 2
+
+
+def _two_grid_setup():
+    coarse = problems.square_grid(2, 2, 2, 2)
+    fine = problems.square_grid(2, 2, 4, 2)
+    dc, nc = O.mapper(coarse.patches, coarse.dir_sides, coarse.ifaces)
+    df, nf = O.mapper(fine.patches, fine.dir_sides, fine.ifaces)
+    Af, bf = O.assemble(fine.patches, df, nf, 1)
+    P = O.transfer_multipatch(coarse.patches, dc, nc, df, nf)
+    return Af, bf, P
+
+
+def _numpy_two_grid(A, b, Pd, smooth):
+    Ac = Pd.T @ A @ Pd
+    x = smooth(np.zeros(len(b)))
+    c = np.linalg.solve(Ac, Pd.T @ (A @ x - b))
+    return smooth(x - Pd @ c)
+
+
+def test_two_grid_richardson_and_chebyshev_match_numpy():
+    """Two-grid cycles with Richardson and Chebyshev smoothing against numpy
+    restatements; the Chebyshev bound against the largest eigenvalue of
+    D^-1 A (power iteration: within 5 %, times the 1.1 safety factor)."""
+    Af, bf, P = _two_grid_setup()
+    A, Pd, D = Af.toarray(), P.toarray(), np.diag(Af.toarray())
+    mg = O.Multigrid(Af, [P], npre=2, npost=2, damping=0.3 / np.abs(D).max(), smoother="richardson")
+    tau = 0.3 / np.abs(D).max()
+
+    def rich(x):
+        for _ in range(2):
+            x = x + tau * (bf - A @ x)
+        return x
+    z = mg.apply(bf)
+    ref = _numpy_two_grid(A, bf, Pd, rich)
+    assert np.linalg.norm(z - ref) <= 1e-11 * np.linalg.norm(ref)
+
+    for m in (1, 3):
+        mg = O.Multigrid(Af, [P], npre=m, npost=m, smoother="chebyshev", smoother_param=20.0)
+        lmax = mg.lmax(1)
+        ev = np.max(np.linalg.eigvals(A / D[:, None]).real)
+        assert 1.1 * ev * 0.95 <= lmax <= 1.1 * ev * 1.0000001, (lmax, 1.1 * ev)
+        l1, l0 = lmax, lmax / 20.0
+        theta, delta = (l1 + l0) / 2, (l1 - l0) / 2
+        sigma = theta / delta
+
+        def cheb(x, m=m):
+            rho = 1.0 / sigma
+            d = (bf - A @ x) / D / theta
+            x = x + d
+            for _ in range(m - 1):
+                rn = 1.0 / (2 * sigma - rho)
+                d = rn * rho * d + 2 * rn / delta * (bf - A @ x) / D
+                x = x + d
+                rho = rn
+            return x
+        z = mg.apply(bf)
+        ref = _numpy_two_grid(A, bf, Pd, cheb)
+        assert np.linalg.norm(z - ref) <= 1e-11 * np.linalg.norm(ref)
+
+
+def test_chebyshev_mg_pcg_beats_jacobi_mg_in_3d():
+    """Cube p=3: Chebyshev(3) smoothing needs fewer MG-PCG iterations than
+    three damped Jacobi sweeps."""
+    levels = [problems.box_patch(3, 3, n) for n in (2, 4, 8)]
+    probs = [problems.Problem([P], problems.all_sides(3), []) for P in levels]
+    maps = [O.mapper(p.patches, p.dir_sides, p.ifaces) for p in probs]
+    Af, bf = O.assemble(probs[-1].patches, maps[-1][0], maps[-1][1], 1)
+    Ps = [O.transfer_multipatch(probs[i].patches, maps[i][0], maps[i][1], maps[i + 1][0], maps[i + 1][1])
+          for i in range(2)]
+    its = {}
+    for name, kw in (("jacobi", dict(damping=0.6)), ("chebyshev", dict(smoother="chebyshev"))):
+        mg = O.Multigrid(Af, Ps, npre=3, npost=3, **kw)
+        x, it, err = O.cg_solve(Af, bf, tol=1e-10, maxit=500, precond="mg", mg=mg)
+        assert err < 1e-10
+        its[name] = it
+    assert its["chebyshev"] < its["jacobi"], its
