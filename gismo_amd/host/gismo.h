@@ -1004,8 +1004,10 @@ std::shared_ptr<gsJacobiOp<T>> makeJacobiOp(const gsSparseMatrix<T> &M, T tau = 
 /// gsMultiGridOp (src/gsMultiGrid/gsMultiGrid.h): built here from the
 /// assembled problem by coarsening its knot vectors (the transfers and
 /// Galerkin products are formed on the device).  Options: NumPreSmooth,
-/// NumPostSmooth, CoarseGridCorrectionDamping, Damping (of the Jacobi
-/// smoother), Precision ("fp64" or "mixed": FP32 cycle).
+/// NumPostSmooth, CoarseGridCorrectionDamping, Damping (of the Jacobi or
+/// Richardson smoother), Smoother ("Jacobi"/"j", "Richardson"/"r",
+/// "Chebyshev"/"c"), ChebyshevRatio, Precision ("fp64" or "mixed": FP32
+/// cycle).
 template <class T = real_t>
 class gsMultiGridOp : public gsPreconditionerOp<T> {
 public:
@@ -1044,6 +1046,14 @@ public:
                                         m_opt.askReal("CoarseGridCorrectionDamping", 1.0), prec, &h),
                           "gsx_mg_create");
             m_mg.reset(h, [](gsx_mg *q) { gsx_mg_destroy(q); });
+            // multiGrid_example's MG.Smoother names (r, j) plus Chebyshev (c)
+            const std::string sm = m_opt.askString("Smoother", "Jacobi");
+            int kind = -1;
+            if (sm == "Jacobi" || sm == "j") kind = GSX_SMOOTH_JACOBI;
+            else if (sm == "Richardson" || sm == "r") kind = GSX_SMOOTH_RICHARDSON;
+            else if (sm == "Chebyshev" || sm == "c") kind = GSX_SMOOTH_CHEBYSHEV;
+            if (kind < 0) throw std::runtime_error("gsMultiGridOp: smoother '" + sm + "' is not supported");
+            detail::check(gsx_mg_set_smoother(h, kind, m_opt.askReal("ChebyshevRatio", 30.0)), "gsx_mg_set_smoother");
         }
         return m_mg.get();
     }

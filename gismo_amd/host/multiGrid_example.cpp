@@ -9,7 +9,8 @@
 //   -p N                  spline degree (default 2)
 //   -l N                  multigrid levels (default 3)
 //   --MG.NumPreSmooth N   (default 1)     --MG.NumPostSmooth N (default 1)
-//   --MG.Damping X        Jacobi damping (default 0.8)
+//   --MG.Damping X        Jacobi / Richardson damping (default 0.8)
+//   -s NAME               MG.Smoother: Jacobi (j), Richardson (r), Chebyshev (c)
 //   -t X                  tolerance (default 1e-8)
 //   --Solver.MaxIterations N (default 100)
 //   -i cg|d               CG preconditioned by multigrid (default) or plain
@@ -29,7 +30,7 @@ int main(int argc, char *argv[])
 {
     index_t refinements = 5, degree = 2, levels = 3, pre = 1, post = 1, maxIt = 100, dim = 2, gx = 0, gy = 0;
     real_t damping = 0.8, tol = 1e-8;
-    std::string solver = "cg", geometry;
+    std::string solver = "cg", geometry, smoother = "Jacobi";
     bool mixed = false;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
@@ -47,6 +48,7 @@ int main(int argc, char *argv[])
             else if (a == "--MG.NumPreSmooth") pre = std::atoi(next());
             else if (a == "--MG.NumPostSmooth") post = std::atoi(next());
             else if (a == "--MG.Damping") damping = std::atof(next());
+            else if (a == "-s" || a == "--MG.Smoother") smoother = next();
             else if (a == "-t") tol = std::atof(next());
             else if (a == "--Solver.MaxIterations") maxIt = std::atoi(next());
             else if (a == "-i") solver = next();
@@ -88,6 +90,7 @@ int main(int argc, char *argv[])
         mgopt.addInt("NumPostSmooth", "", post);
         mgopt.addReal("Damping", "", damping);
         mgopt.addString("Precision", "", mixed ? "mixed" : "fp64");
+        mgopt.addString("Smoother", "", smoother);
         gsMultiGridOp<>::Ptr mg = gsMultiGridOp<>::make(assembler.matrix(), levels, mgopt);
 
         gsOptionList sopt = gsConjugateGradient<>::defaultOptions();
