@@ -212,8 +212,10 @@ __global__ __launch_bounds__(128) void k_elem2d(const PatchDev *__restrict__ pat
         Fbuf[pd.fvec_off + e * NB + t - rg.fshift] = acc;
     }
     if (t == 0) {
-        double m = sDet[0];
-        for (int q = 1; q < NQ; ++q) m = fmin(m, sDet[q]);
+        // oriented determinant: positive at every point of a valid patch of
+        // either orientation (the measure is |det J|)
+        double m = pd.orient * sDet[0];
+        for (int q = 1; q < NQ; ++q) m = fmin(m, pd.orient * sDet[q]);
         detmin[pd.elem_off + e] = m;
     }
 }
@@ -361,8 +363,10 @@ __global__ __launch_bounds__(256) void k_elem3d(const PatchDev *__restrict__ pat
         Fbuf[pd.fvec_off + e * NB + t - rg.fshift] = acc;
     }
     if (t == 0) {
-        double m = sDet[0];
-        for (int q = 1; q < NQ; ++q) m = fmin(m, sDet[q]);
+        // oriented determinant: positive at every point of a valid patch of
+        // either orientation (the measure is |det J|)
+        double m = pd.orient * sDet[0];
+        for (int q = 1; q < NQ; ++q) m = fmin(m, pd.orient * sDet[q]);
         detmin[pd.elem_off + e] = m;
     }
 

@@ -42,6 +42,9 @@ namespace gsx {
 
 namespace {
 
+// see the mixed-precision note above: direction restart threshold
+constexpr double kDirectionReset = 1e-6;
+
 __device__ inline double bsum256(double v, double *s)
 {
     s[threadIdx.x] = v;
@@ -387,6 +390,10 @@ __global__ __launch_bounds__(256) void k_b_rel3(const BChunk *__restrict__ chunk
     const double rr = member_sum(p0, g.c0, g.nc, s);
     const double err = sqrt(rr) / S.bnorm;
     const int decided = err < g.tol ? 1 : (S.stop ? 2 : 0);
+    // S.rr still holds the FP32 recurrence's residual (written by the
+    // direction step; this launch leaves it to R4): a recurrence far below
+    // the true residual leaves no usable direction, which restarts from z
+    const bool reset = S.rr < kDirectionReset * rr;
     double rz = 0.0;
     if (!decided) {
         const float tau = (float)g.tau;
@@ -395,13 +402,13 @@ __global__ __launch_bounds__(256) void k_b_rel3(const BChunk *__restrict__ chunk
             const float z32 = g.d32 ? tau * r32 / g.d32[i] : r32;
             g.r32[i] = r32;
             g.z32[i] = z32;
+            if (reset) g.p32[i] = z32;
             rz += (double)r32 * (double)z32;
         }
     }
     const double a = bsum256(rz, s);
     if (threadIdx.x == 0) p2[blockIdx.x] = a;
     if ((int)blockIdx.x == g.c0 && threadIdx.x == 0) {
-        S.rr = rr;
         S.rr_ref = rr;
         S.err = err;
         if (g.hist) g.hist[S.it_next] = err;
@@ -421,6 +428,7 @@ __global__ __launch_bounds__(256) void k_b_rel4(const BChunk *__restrict__ chunk
     if ((int)blockIdx.x != g.c0) return;
     const double rz = member_sum(p2, g.c0, g.nc, s);
     if (threadIdx.x == 0) {
+        S.rr = S.rr_ref;
         if (S.pending) S.done = S.pending;
         else S.rzs[S.it_next & 1] = rz;
         S.pending = 0;

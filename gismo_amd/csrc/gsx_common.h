@@ -21,7 +21,7 @@ struct PatchDev {
     int nb[3];          // basis functions per direction
     int nel[3];         // elements per direction
     int fkind;          // 1: d*pi^2*prod sin(pi x); 0: felem
-    int pad0;
+    int orient;         // +1 / -1: sign of det J of a valid parametrization
     const double *knots[3];
     const double *ctrl;     // patch control points (nb x d)
     const double *kappa;    // per element or nullptr

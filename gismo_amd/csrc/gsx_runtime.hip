@@ -676,6 +676,24 @@ gsx_status gsx_problem_create(gsx_ctx *c, int32_t npatch, const gsx_patch_desc *
             P->keep.push_back(b);
         }
         pd.fkind = D.fkind;
+        // orientation from the control net at the parametric origin: the
+        // Jacobian there has columns along the net's first edges (open knots)
+        {
+            const double *c0 = D.ctrl_host;
+            const int d = h.d;
+            const int64_t s1 = 1, s2 = h.nb[0], s3 = (int64_t)h.nb[0] * h.nb[1];
+            double e[3][3] = {{0}};
+            for (int r = 0; r < d; ++r) {
+                e[0][r] = c0[s1 * d + r] - c0[r];
+                e[1][r] = c0[s2 * d + r] - c0[r];
+                if (d == 3) e[2][r] = c0[s3 * d + r] - c0[r];
+            }
+            const double det = d == 2 ? e[0][0] * e[1][1] - e[0][1] * e[1][0]
+                                      : e[0][0] * (e[1][1] * e[2][2] - e[1][2] * e[2][1]) -
+                                            e[0][1] * (e[1][0] * e[2][2] - e[1][2] * e[2][0]) +
+                                            e[0][2] * (e[1][0] * e[2][1] - e[1][1] * e[2][0]);
+            pd.orient = det < 0.0 ? -1 : 1;
+        }
         pd.dofs = dp<int>(P->d_dofs) + P->boff[k];
         pd.nelem = h.nelem;
         pd.elem_off = eoff;
@@ -1114,7 +1132,8 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
     P->min_det = mind;
     if (!(mind > 0.0)) {
         char buf[128];
-        std::snprintf(buf, sizeof(buf), "non-positive Jacobian determinant (min det J = %.6e)", mind);
+        std::snprintf(buf, sizeof(buf),
+                      "Jacobian determinant vanishes or changes sign in a patch (min oriented det J = %.6e)", mind);
         return fail(c, GSX_ERR_JACOBIAN, buf);
     }
     A->lpr = gsx::spmv_lpr(A->n ? (double)A->nnz / A->n : 1.0);
@@ -1277,6 +1296,12 @@ static gsx_status cg_workspace(gsx_matrix *A, int maxit, bool mixed, bool grid =
     return GSX_OK;
 }
 
+// Mixed solvers: after a reliable update the FP32 direction is kept (the
+// recurrence continues with the true residual), unless the FP32 residual
+// had fallen below this fraction of the true one: the direction then
+// restarts from the preconditioned residual.
+constexpr double kDirectionReset = 1e-6;
+
 static gsx_status read_state(gsx_ctx *c, const CGWork &w, CGState &h)
 {
     HIP_OK(c, hipMemcpyAsync(&h, w.state.get(), sizeof(CGState), hipMemcpyDeviceToHost, c->stream));
@@ -1438,7 +1463,7 @@ static gsx_status cg_mixed(gsx_matrix *A, int precond, double tau, const double 
                              h.rr < delta * delta * h.rr_ref || h.err < tol;
             if (due) {
                 KScope kr(c, "reliable_update");
-
+                const double rr_rec = h.rr;   // the FP32 recurrence's residual
                 HIP_OK(c, gsx::launch_accum_correction(n, x, xc, s));
                 HIP_OK(c, gsx::launch_spmv_f64(A->lpr, n, rp, ci, va, x, r, b, P1, 2, nullptr, s));
                 HIP_OK(c, gsx::launch_cg_reliable(P1, ns, st, nullptr, s));
@@ -1446,6 +1471,10 @@ static gsx_status cg_mixed(gsx_matrix *A, int precond, double tau, const double 
                 if (h.done == 1 || h.done == 2) break;
                 gsx_status e2 = restart_from_r64();
                 if (e2 != GSX_OK) return e2;
+                // a recurrence that ran far below the true residual (down to
+                // an exact zero on tiny systems) leaves no usable direction
+                if (rr_rec < kDirectionReset * h.rr)
+                    HIP_OK(c, hipMemcpyAsync(p32, jac ? z32 : r32, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
             }
             if (h.it >= maxit) break;
         }
@@ -1618,12 +1647,17 @@ static gsx_status cg_grid(gsx_matrix *A, int precond, double tau, const double *
             k = h.it;
             if (h.need_update || h.done == 2 || h.err < tol) {
                 KScope kr(c, "reliable_update");
+                const double rr_rec = h.rr;   // the FP32 recurrence's residual
                 HIP_OK(c, gsx::launch_accum_grid(N, xg, xc, s));
                 HIP_OK(c, gsx::launch_sspmv_f64(S.d, S.p, 2, n, f2g, sy, sz, sv, xg, rg, bg, P1, nullptr, 0, s));
                 HIP_OK(c, gsx::launch_cg_reliable(P1, nsp, st, nullptr, s));
                 GSX_TRY(read_state(c, w, h));
                 if (h.done == 1 || h.done == 2) break;
                 GSX_TRY(restart(k));
+                // a recurrence that ran far below the true residual (down to
+                // an exact zero on tiny systems) leaves no usable direction
+                if (rr_rec < kDirectionReset * h.rr)
+                    HIP_OK(c, hipMemcpyAsync(p32, z32, sizeof(float) * N, hipMemcpyDeviceToDevice, s));
             }
             if (h.it >= maxit) break;
         }

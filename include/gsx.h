@@ -25,7 +25,7 @@ typedef enum {
     GSX_OK = 0,
     GSX_ERR_ARG = 1,           /* invalid argument / unsupported input       */
     GSX_ERR_OOM = 2,           /* device allocation failed                   */
-    GSX_ERR_JACOBIAN = 3,      /* non-positive Jacobian determinant          */
+    GSX_ERR_JACOBIAN = 3,      /* Jacobian vanishes or changes sign          */
     GSX_ERR_HIP = 4,           /* HIP runtime error                          */
     GSX_ERR_NOT_CONVERGED = 5, /* solver stopped at max iterations           */
     GSX_ERR_NO_DEVICE = 6,     /* no GPU visible                             */
@@ -173,9 +173,12 @@ gsx_status gsx_matrix_download(const gsx_matrix *A, int32_t *rowptr_host,
 /* Device pointers of the CSR arrays (owned by the matrix). */
 gsx_status gsx_matrix_device_ptrs(const gsx_matrix *A, const int32_t **rowptr_dev,
                                   const int32_t **col_dev, const double **val_dev);
-/* Smallest Jacobian determinant met at any Gauss point of the last
- * assembly of prob (a non-positive value makes gsx_assemble fail with
- * GSX_ERR_JACOBIAN). */
+/* Smallest oriented Jacobian determinant (det J times the sign of the
+ * patch's parametrization, taken from its control net at the parametric
+ * origin) met at any Gauss point of the last assembly of prob.  The measure
+ * is |det J| (meas(G), src/gsAssembler/gsExpressions.h:752-781), so patches
+ * of either orientation are valid; a non-positive value (a fold, a
+ * degenerate point) makes gsx_assemble fail with GSX_ERR_JACOBIAN. */
 gsx_status gsx_problem_min_detj(const gsx_problem *prob, double *min_det);
 
 /* ---- operators ----------------------------------------------------------- */

@@ -334,3 +334,44 @@ def test_error_norms_match_oracle(ctx, name):
     for a in (A, b, x, z):
         a.close()
     gp.close()
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_reflected_patch_assembles_like_the_original(ctx, d):
+    """A patch parametrized with the opposite orientation (x -> -x) is valid:
+    the measure is |det J|.  Its matrix equals the original's and the
+    oracle's; a fold inside it is still refused."""
+    prob = _noisy_patch_problem(d, 2, 4, seed=77)
+    ref = gsx.Problem(ctx, prob)
+    A0, b0 = ref.assemble()
+    refl = _noisy_patch_problem(d, 2, 4, seed=77)
+    refl.patches[0].ctrl[:, 0] *= -1.0
+    refl.patches[0].fkind = 0
+    refl.patches[0].felem = np.ones(refl.patches[0].nelem)
+    prob.patches[0].fkind = 0
+    prob.patches[0].felem = np.ones(prob.patches[0].nelem)
+    g0 = gsx.Problem(ctx, prob)
+    g1 = gsx.Problem(ctx, refl)
+    A1, b1 = g0.assemble()
+    A2, b2 = g1.assemble()
+    r1, c1, v1 = A1.download()
+    r2, c2, v2 = A2.download()
+    assert np.array_equal(r1, r2) and np.array_equal(c1, c2)
+    assert np.max(np.abs(v1 - v2)) <= 1e-12 * np.abs(v1).max()
+    assert np.max(np.abs(b1.download() - b2.download())) <= 1e-12 * np.abs(b1.download()).max()
+    assert g1.min_detj() > 0
+    dofs, nfree = O.mapper(refl.patches, refl.dir_sides, refl.ifaces)
+    Ao, bo = O.assemble(refl.patches, dofs, nfree, 0)
+    assert np.max(np.abs(v2 - Ao.data)) <= 1e-12 * np.abs(Ao.data).max()
+    # fold the reflected patch
+    nb = refl.patches[0].nbasis_dir()
+    i = 2 + nb[0] * (2 + (nb[1] * 2 if d == 3 else 0))
+    refl.patches[0].ctrl[i, 0] -= 3.0
+    g2 = gsx.Problem(ctx, refl)
+    with pytest.raises(gsx.GsxError) as e:
+        g2.assemble()
+    assert e.value.status == gsx.GSX_ERR_JACOBIAN
+    for a in (A0, b0, A1, b1, A2, b2):
+        a.close()
+    for g in (ref, g0, g1, g2):
+        g.close()
