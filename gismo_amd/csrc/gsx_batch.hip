@@ -47,14 +47,12 @@ constexpr double kDirectionReset = 1e-6;
 
 __device__ inline double bsum256(double v, double *s)
 {
-    s[threadIdx.x] = v;
-    __syncthreads();
+    // wavefront shuffles, then the 4 wave sums in wave order (fixed order)
 #pragma unroll
-    for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
-        __syncthreads();
-    }
-    const double r = s[0];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const double r = (s[0] + s[1]) + (s[2] + s[3]);
     __syncthreads();
     return r;
 }

@@ -65,9 +65,6 @@ struct CGState {
     // alpha step when an update is due.
     int flag_next;
     int frozen;
-    // grid solver: arrival counters of the SpMV and update launches; the last
-    // block to arrive sums the launch's partials and derives alpha / beta
-    unsigned int arrive[2];
 };
 
 // Element range of one element-kernel launch: elements [e0, e0 + grid) of

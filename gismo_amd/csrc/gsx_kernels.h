@@ -144,27 +144,25 @@ hipError_t launch_scatter_f64_to_grid(int nfree, const int *f2g, const double *s
                                       hipStream_t s);
 hipError_t launch_gather_f64_from_grid(int nfree, const int *f2g, const double *src, double *dst,
                                        hipStream_t s);
-// MODE 1 with st: the last block derives pAp and alpha for iteration k
+// MODE 1 with st: skipped once the solve stopped or paused at iteration k
 hipError_t launch_sspmv_f64(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz,
                             const double *sv, const double *x, double *y, const double *b,
                             double *part, CGState *st, int k, hipStream_t s);
+hipError_t launch_sspmv_f32(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz,
+                            const float *sv, const float *x, float *y, const float *b,
+                            double *part, CGState *st, int k, hipStream_t s);
 hipError_t launch_cg2_update_f64(int n, double *x, double *r, const double *p, const double *t,
-                                 double *z, const double *diag, double tau, double *prr, double *prz,
-                                 CGState *st, int k, double *hist, hipStream_t s);
-hipError_t launch_cg2_update_f32_mixed(int n, float *x, float *r, const float *p, const float *t, float *z,
-                                       const float *diag, double tau, double *prr, double *prz, CGState *st,
-                                       int k, hipStream_t s);
-hipError_t launch_cg2_direction_f64(int n, double *p, const double *z, const CGState *st, int k, hipStream_t s);
-hipError_t launch_cg2_direction_f32(int n, float *p, const float *z, const CGState *st, int k, hipStream_t s);
-// symmetric half stencil (FP32): slots from the centre on, each slot array
-// hstride long with pad zeros before and after its nfree entries; h points
-// at slot 0's first entry.  MODE 1 with st: the last block derives alpha.
-int stencil_half_slots(int d, int p);
-hipError_t launch_stencil_half(int d, int p, int nfree, const double *s64, float *h, int64_t hstride,
-                               hipStream_t s);
-hipError_t launch_sspmv_sym_f32(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz, int fy,
-                                int fz, int64_t hstride, const float *h, const float *x, float *y,
-                                const float *b, double *part, CGState *st, int k, hipStream_t s);
+                                 double *z, const double *diag, double tau, const double *pap,
+                                 int npap, double *prr, double *prz, CGState *st, int k, hipStream_t s);
+hipError_t launch_cg2_update_f32(int n, float *x, float *r, const float *p, const float *t, float *z,
+                                 const float *diag, double tau, const double *pap, int npap,
+                                 double *prr, double *prz, CGState *st, int k, hipStream_t s);
+hipError_t launch_cg2_direction_f64(int n, double *p, const double *z, const double *prr,
+                                    const double *prz, int np, CGState *st, int k, double *hist,
+                                    hipStream_t s);
+hipError_t launch_cg2_direction_f32_mixed(int n, float *p, const float *z, const double *prr,
+                                          const double *prz, int np, CGState *st, int k,
+                                          hipStream_t s);
 hipError_t launch_cg2_precond_f64(int n, const double *r, double *z, const double *diag,
                                   double tau, double *prz, hipStream_t s);
 hipError_t launch_cg2_precond_f32(int n, const float *r, float *z, const float *diag, double tau,

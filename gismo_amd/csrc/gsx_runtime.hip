@@ -869,13 +869,8 @@ struct StencilOp {
     int64_t ngrid = 0;
     int slots = 0;
     std::vector<int> f2g_host;
-    DevPtr f2g, s64;
-    // FP32 symmetric half stencil of the mixed iterations: slots from the
-    // centre on, each hstride = nfree + 2 * hpad long (zero padded)
-    DevPtr h32;
-    bool h32_ready = false;
-    int64_t hstride = 0, hpad = 0;
-    int fy = 1, fz = 1;           // free-box strides (free numbering)
+    DevPtr f2g, s64, s32;
+    bool s32_ready = false;
     // grid-layout workspace
     DevPtr diag64, diag32;
     DevPtr xg, rg, bg, zg, pg, tg;            // FP64
@@ -1050,15 +1045,14 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
         const int W = 2 * S.p + 1;
         S.slots = h.d == 3 ? W * W * W : W * W;
         S.f2g = P->d_f2g;
-        S.fy = P->box_hi[0] - P->box_lo[0] + 1;
-        S.fz = S.fy * (P->box_hi[1] - P->box_lo[1] + 1);
         if (!csr_only) {
-            // FP64 slots from the gather; the FP32 half stencil of mixed
-            // solves is derived from them when a solve first needs it
             S.s64 = dalloc(c, sizeof(double) * (int64_t)S.slots * A->n);
-            if (!S.s64) return fail(c, GSX_ERR_OOM, "device allocation (structured operator)");
+            if (precision == GSX_MIXED) S.s32 = dalloc(c, sizeof(float) * (int64_t)S.slots * A->n);
+            if (!S.s64 || (precision == GSX_MIXED && !S.s32))
+                return fail(c, GSX_ERR_OOM, "device allocation (structured operator)");
         }
         S.s64_ready = !csr_only;
+        S.s32_ready = !csr_only && precision == GSX_MIXED;
         // free rows are lexicographic over the free box: a range of row
         // layers along the slowest direction is a range of rows
         int64_t plane = 1;
@@ -1081,7 +1075,7 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
                                                dp<int>(P->d_rowptr), dp<double>(Kbuf), dp<double>(Fbuf),
                                                dp<double>(A->val), rhs, keep ? dp<unsigned char>(keep) : nullptr,
                                                S.s64 ? dp<double>(S.s64) : nullptr,
-                                               nullptr, s));
+                                               S.s32 ? dp<float>(S.s32) : nullptr, s));
         }
     } else {
         KScope k(c, "gather");
@@ -1511,15 +1505,14 @@ static gsx_status stencil_prepare(gsx_matrix *A, bool mixed)
                                              dp<double>(S.s64), nullptr, s));
         S.s64_ready = true;
     }
-    if (mixed && !S.h32_ready) {
-        const int nh = gsx::stencil_half_slots(S.d, S.p);
-        S.hpad = (int64_t)S.p * (1 + S.fy + (S.d == 3 ? S.fz : 0));
-        S.hstride = n + 2 * S.hpad;
-        S.h32 = dalloc(c, sizeof(float) * nh * S.hstride);
-        if (!S.h32) return fail(c, GSX_ERR_OOM, "device allocation (structured operator, fp32)");
-        HIP_OK(c, gsx::launch_fill_f32(dp<float>(S.h32), nh * S.hstride, 0.0f, s));
-        HIP_OK(c, gsx::launch_stencil_half(S.d, S.p, n, dp<double>(S.s64), dp<float>(S.h32) + S.hpad, S.hstride, s));
-        S.h32_ready = true;
+    if (mixed && !S.s32_ready) {
+        S.s32 = dalloc(c, sizeof(float) * K * n);
+        if (!S.s32) return fail(c, GSX_ERR_OOM, "device allocation (structured operator, fp32)");
+        HIP_OK(c, gsx::launch_fill_f32(dp<float>(S.s32), K * n, 0.0f, s));
+        HIP_OK(c, gsx::launch_csr_to_stencil(S.d, S.p, n, dp<int>(A->rowptr), dp<int>(A->col),
+                                             dp<double>(A->val), dp<int>(S.f2g), S.nx, S.ny, nullptr,
+                                             dp<float>(S.s32), s));
+        S.s32_ready = true;
     }
     if (!S.built) {
         S.diag64 = dalloc(c, sizeof(double) * N);
@@ -1609,9 +1602,9 @@ static gsx_status cg_grid(gsx_matrix *A, int precond, double tau, const double *
                     HIP_OK(c, gsx::launch_sspmv_f64(S.d, S.p, 1, n, f2g, sy, sz, sv, pg, tg, nullptr, P0, st, k, s));
                 }
                 KScope kv(c, "cg_vec");
-                HIP_OK(c, gsx::launch_cg2_update_f64((int)N, xg, rg, pg, tg, zg, jac ? dg : nullptr, tau, P1, P2,
-                                                     st, k, hist, s));
-                HIP_OK(c, gsx::launch_cg2_direction_f64((int)N, pg, zg, st, k, s));
+                HIP_OK(c, gsx::launch_cg2_update_f64((int)N, xg, rg, pg, tg, zg, jac ? dg : nullptr, tau, P0,
+                                                     nsp, P1, P2, st, k, s));
+                HIP_OK(c, gsx::launch_cg2_direction_f64((int)N, pg, zg, P1, P2, nv, st, k, hist, s));
             }
             GSX_TRY(read_state(c, w, h));
             if (h.done) break;
@@ -1620,7 +1613,7 @@ static gsx_status cg_grid(gsx_matrix *A, int precond, double tau, const double *
     } else if (!h.done && mixed) {
         float *r32 = dp<float>(S.r32), *z32 = dp<float>(S.z32), *p32 = dp<float>(S.p32);
         float *t32 = dp<float>(S.t32), *xc = dp<float>(S.xc32);
-        const float *h32 = dp<float>(S.h32) + S.hpad, *d32 = dp<float>(S.diag32);
+        const float *sv32 = dp<float>(S.s32), *d32 = dp<float>(S.diag32);
         auto restart = [&](int slot) -> gsx_status {
             HIP_OK(c, gsx::launch_grid_f64_to_f32(N, rg, r32, s));
             HIP_OK(c, gsx::launch_cg2_precond_f32((int)N, r32, z32, jac ? d32 : nullptr, tau, P0, s));
@@ -1635,13 +1628,12 @@ static gsx_status cg_grid(gsx_matrix *A, int precond, double tau, const double *
             for (int j = 0; j < chunk; ++j, ++k) {
                 {
                     KScope ks(c, "spmv_f32");
-                    HIP_OK(c, gsx::launch_sspmv_sym_f32(S.d, S.p, 1, n, f2g, sy, sz, S.fy, S.fz, S.hstride, h32,
-                                                        p32, t32, nullptr, P0, st, k, s));
+                    HIP_OK(c, gsx::launch_sspmv_f32(S.d, S.p, 1, n, f2g, sy, sz, sv32, p32, t32, nullptr, P0, st, k, s));
                 }
                 KScope kv(c, "cg_vec_f32");
-                HIP_OK(c, gsx::launch_cg2_update_f32_mixed((int)N, xc, r32, p32, t32, z32, jac ? d32 : nullptr, tau,
-                                                           P1, P2, st, k, s));
-                HIP_OK(c, gsx::launch_cg2_direction_f32((int)N, p32, z32, st, k, s));
+                HIP_OK(c, gsx::launch_cg2_update_f32((int)N, xc, r32, p32, t32, z32, jac ? d32 : nullptr, tau, P0,
+                                                     nsp, P1, P2, st, k, s));
+                HIP_OK(c, gsx::launch_cg2_direction_f32_mixed((int)N, p32, z32, P1, P2, nv, st, k, s));
             }
             GSX_TRY(read_state(c, w, h));
             k = h.it;

@@ -25,16 +25,15 @@ namespace gsx {
 
 constexpr int kMaxPartials = 2048;
 
+// block sum of 256 threads: wavefront shuffles, then the 4 wave sums in
+// wave order (fixed order: bitwise reproducible)
 __device__ inline double bsum256(double v, double *s)
 {
-    s[threadIdx.x] = v;
-    __syncthreads();
 #pragma unroll
-    for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
-        __syncthreads();
-    }
-    const double r = s[0];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+    __syncthreads();
+    const double r = (s[0] + s[1]) + (s[2] + s[3]);
     __syncthreads();
     return r;
 }
@@ -44,43 +43,6 @@ __device__ inline double sum_partials(const double *__restrict__ p, int np, doub
     double a = 0.0;
     for (int i = threadIdx.x; i < np; i += 256) a += p[i];
     return bsum256(a, s);
-}
-
-// True in exactly one block of the launch: the last to arrive at *counter.
-// Every block publishes its partial before arriving, so the last block sees
-// all of them (read through to L2 below); it resets the counter for the
-// next launch.  The sum is in partial order, whichever block is last.
-__device__ inline bool arrive_last(unsigned int *counter)
-{
-    __shared__ int last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned int prev = atomicAdd(counter, 1u);
-        last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (last) __threadfence();
-    return last;
-}
-
-__device__ inline double sum_partials_l2(const double *p, int np, double *s)
-{
-    double a = 0.0;
-    for (int i = threadIdx.x; i < np; i += 256) a += __hip_atomic_load(p + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return bsum256(a, s);
-}
-
-// last block of a CG SpMV launch: pAp and alpha = rz_k / pAp
-__device__ inline void finish_alpha(CGState *st, const double *partial, int k, double *s)
-{
-    if (!arrive_last(&st->arrive[0])) return;
-    const double pap = sum_partials_l2(partial, gridDim.x, s);
-    if (threadIdx.x == 0) {
-        st->pap = pap;
-        st->alpha = st->rzs[k & 1] / pap;
-        st->arrive[0] = 0;
-    }
 }
 
 template <typename T>
@@ -140,8 +102,8 @@ __global__ __launch_bounds__(256) void k_sspmv(int nfree, const int *__restrict_
                                                double *__restrict__ partial,
                                                CGState *__restrict__ st, int k)
 {
-    __shared__ double s[256];
-    if (st && st->done) return;
+    __shared__ double s[4];
+    if (st && (st->done || (st->flag_next && st->flag_next <= k))) return;
     constexpr int W = 2 * P + 1;
     double contrib = 0.0;
     for (int r = blockIdx.x * 256 + threadIdx.x; r < nfree; r += gridDim.x * 256) {
@@ -184,98 +146,6 @@ __global__ __launch_bounds__(256) void k_sspmv(int nfree, const int *__restrict_
         const double tot = bsum256(contrib, s);
         if (threadIdx.x == 0) partial[blockIdx.x] = tot;
     }
-    if (MODE == 1 && st) finish_alpha(st, partial, k, s);
-}
-
-// ---------------------------------------------------------------------------
-// Symmetric half-stencil SpMV.  A is symmetric bitwise, so only the centre
-// and the (NS - 1) / 2 slots after it (neighbour offsets lexicographically
-// positive) are stored: h[kk][r] = A(r, r + o_kk) for kk = 0..NH-1.  Row r
-// takes its lower-half coefficients from the rows below it:
-//   A(r, r - o_kk) = A(r - o_kk, r) = h[kk][r - of_kk],
-// where of_kk is the slot offset in the numbering of the free box (the free
-// functions of one patch form a box, numbered lexicographically).  Where the
-// neighbour lies outside the free box that index lands on a padding zero or
-// on another row's coefficient, and the grid value x[g - o_kk] it multiplies
-// is a zero of the grid's halo or eliminated layer.  Each slot array is
-// padded by `pad` zeros on both ends.  Half the coefficient bytes of the
-// full stencil (the mirrored half is re-read from cache).
-// ---------------------------------------------------------------------------
-template <int D, int P, typename T, int MODE>
-__global__ __launch_bounds__(256) void k_sspmv_sym(int nfree, const int *__restrict__ free2grid,
-                                                   int sy, int sz, int fy, int fz, int64_t hstride,
-                                                   const T *__restrict__ h, const T *__restrict__ x,
-                                                   T *__restrict__ y, const T *__restrict__ b,
-                                                   double *__restrict__ partial,
-                                                   CGState *__restrict__ st, int k)
-{
-    __shared__ double s[256];
-    if (st && (st->done || (st->flag_next && st->flag_next <= k))) return;
-    constexpr int W = 2 * P + 1;
-    double contrib = 0.0;
-    for (int r = blockIdx.x * 256 + threadIdx.x; r < nfree; r += gridDim.x * 256) {
-        const int g = free2grid[r];
-        const T *xb = x + g;
-        const T *hb = h + r;
-        T acc = hb[0] * xb[0];
-        // dz = 0 plane: dy = 0 with dx > 0, then dy > 0
-#pragma unroll
-        for (int dx = 1; dx <= P; ++dx) {
-            const int kk = dx;
-            acc += hb[kk * hstride] * xb[dx] + hb[kk * hstride - dx] * xb[-dx];
-        }
-#pragma unroll
-        for (int dy = 1; dy <= P; ++dy) {
-#pragma unroll
-            for (int dx = -P; dx <= P; ++dx) {
-                const int kk = P + (dy - 1) * W + (dx + P) + 1;
-                const int o = dy * sy + dx, of = dy * fy + dx;
-                acc += hb[kk * hstride] * xb[o] + hb[kk * hstride - of] * xb[-o];
-            }
-        }
-        if (D == 3) {
-            constexpr int K0 = P + P * W + 1;   // slots of the dz = 0 half plane
-#pragma unroll 1
-            for (int dz = 1; dz <= P; ++dz) {
-#pragma unroll
-                for (int dy = -P; dy <= P; ++dy) {
-#pragma unroll
-                    for (int dx = -P; dx <= P; ++dx) {
-                        const int kk = K0 + ((dz - 1) * W + (dy + P)) * W + (dx + P);
-                        const int o = dz * sz + dy * sy + dx, of = dz * fz + dy * fy + dx;
-                        acc += hb[kk * hstride] * xb[o] + hb[kk * hstride - of] * xb[-o];
-                    }
-                }
-            }
-        }
-        if (MODE == 2) {
-            const T v = b[g] - acc;
-            y[g] = v;
-            contrib += (double)v * (double)v;
-        } else {
-            y[g] = acc;
-            if (MODE == 1) contrib += (double)x[g] * (double)acc;
-        }
-    }
-    if (MODE != 0) {
-        const double tot = bsum256(contrib, s);
-        if (threadIdx.x == 0) partial[blockIdx.x] = tot;
-    }
-    if (MODE == 1 && st) finish_alpha(st, partial, k, s);
-}
-
-// full FP64 stencil -> FP32 half stencil (slots from the centre on)
-template <int D, int P>
-__global__ void k_stencil_half(int nfree, const double *__restrict__ s64, float *__restrict__ h,
-                               int64_t hstride)
-{
-    constexpr int W = 2 * P + 1;
-    constexpr int NS = (D == 3) ? W * W * W : W * W;
-    constexpr int C = (NS - 1) / 2, NH = NS - C;
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nfree) return;
-#pragma unroll 4
-    for (int kk = 0; kk < NH; ++kk) h[kk * hstride + r] = (float)s64[(int64_t)(C + kk) * nfree + r];
 }
 
 // ---------------------------------------------------------------------------
@@ -283,23 +153,26 @@ __global__ void k_stencil_half(int nfree, const double *__restrict__ s64, float 
 // functions so z stays 0 there)
 // ---------------------------------------------------------------------------
 // x += alpha p; r -= alpha t; z = tau r / diag; partials (r, r), (r, z).
-// The last block to finish sums them and takes the iteration's decisions:
-// MIXED == 0: the reference stopping rule (err < tol stops before the
-// direction update); MIXED == 1: the FP32 recurrence of the mixed solver
-// never stops on its own residual but flags a reliable update (and pauses
-// from iteration k + 1 on until it has run).
-template <typename T, typename TX, int MIXED>
+// Every block derives alpha = rz_k / (p, A p) from the SpMV's partials
+// itself (a bounded count, summed in a fixed order).
+template <typename T, typename TX>
 __global__ __launch_bounds__(256) void k_cg2_update(int n, TX *__restrict__ x, T *__restrict__ r,
                                                     const T *__restrict__ p, const T *__restrict__ t,
                                                     T *__restrict__ z, const T *__restrict__ diag,
-                                                    double tau, double *__restrict__ part_rr,
+                                                    double tau, const double *__restrict__ pap_parts,
+                                                    int npap, double *__restrict__ part_rr,
                                                     double *__restrict__ part_rz,
-                                                    CGState *__restrict__ st, int k,
-                                                    double *__restrict__ hist)
+                                                    CGState *__restrict__ st, int k)
 {
-    __shared__ double s[256];
+    __shared__ double s[4];
     if (st->done || (st->flag_next && st->flag_next <= k)) return;
-    const T alpha = (T)st->alpha;
+    const double pap = sum_partials(pap_parts, npap, s);
+    const double alpha_d = st->rzs[k & 1] / pap;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->pap = pap;
+        st->alpha = alpha_d;
+    }
+    const T alpha = (T)alpha_d;
     const T ta = (T)tau;
     double rr = 0.0, rz = 0.0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
@@ -317,36 +190,47 @@ __global__ __launch_bounds__(256) void k_cg2_update(int n, TX *__restrict__ x, T
         part_rr[blockIdx.x] = a;
         part_rz[blockIdx.x] = c;
     }
-    if (!arrive_last(&st->arrive[1])) return;
-    const double rrs = sum_partials_l2(part_rr, gridDim.x, s);
-    const double rzn = sum_partials_l2(part_rz, gridDim.x, s);
-    if (threadIdx.x != 0) return;
-    st->arrive[1] = 0;
-    const double err = sqrt(rrs) / st->bnorm;
-    st->it = k + 1;
-    st->rr = rrs;
-    st->err = err;
-    if (hist) hist[k + 1] = err;
-    if (!MIXED && err < st->tol) {
-        st->done = 1;
-        return;
-    }
-    st->beta = rzn / st->rzs[k & 1];
-    st->rzs[(k + 1) & 1] = rzn;
-    if (MIXED && (rrs < st->rel_since_update * st->rr_ref || err < st->tol)) {
-        st->need_update = 1;
-        st->flag_next = k + 1;
-    }
-    if (k + 1 >= st->maxit) st->done = 2;
 }
 
-// p = z + beta p with the beta of the update launch (skipped once stopped)
-template <typename T>
+// Stop test and p = z + beta p.  MIXED == 0: reference stopping rule (err <
+// tol stops, before the direction update).  MIXED == 1: FP32 recurrence of
+// the mixed solver: never stops on its own residual, flags a reliable
+// update instead and pauses from iteration k + 1 on until it has run.
+template <typename T, int MIXED>
 __global__ __launch_bounds__(256) void k_cg2_direction(int n, T *__restrict__ p, const T *__restrict__ z,
-                                                       const CGState *__restrict__ st, int k)
+                                                       const double *__restrict__ part_rr,
+                                                       const double *__restrict__ part_rz, int np,
+                                                       CGState *__restrict__ st, int k,
+                                                       double *__restrict__ hist)
 {
+    __shared__ double s[4];
+    // flag_next is only ever set to k + 1 below, so the blocks of this
+    // launch read the same decision whether or not block 0 has written it
     if (st->done || (st->flag_next && st->flag_next <= k)) return;
-    const T tb = (T)st->beta;
+    const double rr = sum_partials(part_rr, np, s);
+    const double rzn = sum_partials(part_rz, np, s);
+    const double err = sqrt(rr) / st->bnorm;
+    const bool conv = !MIXED && err < st->tol;
+    const double beta = rzn / st->rzs[k & 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->it = k + 1;
+        st->rr = rr;
+        st->err = err;
+        if (hist) hist[k + 1] = err;
+        if (conv) {
+            st->done = 1;
+        } else {
+            st->beta = beta;
+            st->rzs[(k + 1) & 1] = rzn;
+            if (MIXED && (rr < st->rel_since_update * st->rr_ref || err < st->tol)) {
+                st->need_update = 1;
+                st->flag_next = k + 1;
+            }
+            if (k + 1 >= st->maxit) st->done = 2;
+        }
+    }
+    if (conv) return;
+    const T tb = (T)beta;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = z[i] + tb * p[i];
 }
 
@@ -356,7 +240,7 @@ __global__ __launch_bounds__(256) void k_cg2_precond(int n, const T *__restrict_
                                                      const T *__restrict__ diag, double tau,
                                                      double *__restrict__ part_rz)
 {
-    __shared__ double s[256];
+    __shared__ double s[4];
     double rz = 0.0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         const T zi = diag ? (T)tau * r[i] / diag[i] : r[i];
@@ -367,9 +251,10 @@ __global__ __launch_bounds__(256) void k_cg2_precond(int n, const T *__restrict_
     if (threadIdx.x == 0) part_rz[blockIdx.x] = c;
 }
 
-__global__ void k_set_rz(const double *__restrict__ parts, int np, CGState *__restrict__ st, int slot)
+__global__ __launch_bounds__(256) void k_set_rz(const double *__restrict__ parts, int np, CGState *__restrict__ st,
+                                                int slot)
 {
-    __shared__ double s[256];
+    __shared__ double s[4];
     const double a = sum_partials(parts, np, s);
     if (threadIdx.x == 0) st->rzs[slot & 1] = a;
 }
@@ -462,64 +347,46 @@ hipError_t launch_sspmv_f64(int d, int p, int mode, int nfree, const int *f2g, i
     return sspmv_dispatch<double, double>(d, p, mode, nfree, f2g, sy, sz, sv, x, y, b, part, st, k, s);
 }
 
-int stencil_half_slots(int d, int p)
+hipError_t launch_sspmv_f32(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz,
+                            const float *sv, const float *x, float *y, const float *b,
+                            double *part, CGState *st, int k, hipStream_t s)
 {
-    const int w = 2 * p + 1, ns = d == 3 ? w * w * w : w * w;
-    return ns - (ns - 1) / 2;
-}
-
-hipError_t launch_stencil_half(int d, int p, int nfree, const double *s64, float *h, int64_t hstride,
-                               hipStream_t s)
-{
-    const dim3 grid((nfree + 255) / 256), blk(256);
-#define GSX_H(D, P) if (d == D && p == P) { hipLaunchKernelGGL((k_stencil_half<D, P>), grid, blk, 0, s, nfree, s64, h, hstride); return hipGetLastError(); }
-    GSX_H(2, 1) GSX_H(2, 2) GSX_H(2, 3) GSX_H(2, 4)
-    GSX_H(3, 1) GSX_H(3, 2) GSX_H(3, 3) GSX_H(3, 4)
-#undef GSX_H
-    return hipErrorInvalidValue;
-}
-
-hipError_t launch_sspmv_sym_f32(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz, int fy,
-                                int fz, int64_t hstride, const float *h, const float *x, float *y,
-                                const float *b, double *part, CGState *st, int k, hipStream_t s)
-{
-    const dim3 grid(sspmv_blocks(nfree)), blk(256);
-#define GSX_Y(D, P, M) if (d == D && p == P && mode == M) { hipLaunchKernelGGL((k_sspmv_sym<D, P, float, M>), grid, blk, 0, s, nfree, f2g, sy, sz, fy, fz, hstride, h, x, y, b, part, st, k); return hipGetLastError(); }
-#define GSX_Y3(D, P) GSX_Y(D, P, 0) GSX_Y(D, P, 1) GSX_Y(D, P, 2)
-    GSX_Y3(2, 1) GSX_Y3(2, 2) GSX_Y3(2, 3) GSX_Y3(2, 4)
-    GSX_Y3(3, 1) GSX_Y3(3, 2) GSX_Y3(3, 3) GSX_Y3(3, 4)
-#undef GSX_Y3
-#undef GSX_Y
-    return hipErrorInvalidValue;
+    return sspmv_dispatch<float, float>(d, p, mode, nfree, f2g, sy, sz, sv, x, y, b, part, st, k, s);
 }
 
 hipError_t launch_cg2_update_f64(int n, double *x, double *r, const double *p, const double *t,
-                                 double *z, const double *diag, double tau, double *prr, double *prz,
-                                 CGState *st, int k, double *hist, hipStream_t s)
+                                 double *z, const double *diag, double tau, const double *pap,
+                                 int npap, double *prr, double *prz, CGState *st, int k, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cg2_update<double, double, 0>), dim3(vec_blocks(n)), dim3(256), 0, s, n, x, r, p, t, z,
-                       diag, tau, prr, prz, st, k, hist);
+    hipLaunchKernelGGL((k_cg2_update<double, double>), dim3(vec_blocks(n)), dim3(256), 0, s, n, x, r, p, t, z,
+                       diag, tau, pap, npap, prr, prz, st, k);
     return hipGetLastError();
 }
 
-hipError_t launch_cg2_update_f32_mixed(int n, float *x, float *r, const float *p, const float *t, float *z,
-                                       const float *diag, double tau, double *prr, double *prz, CGState *st,
-                                       int k, hipStream_t s)
+hipError_t launch_cg2_update_f32(int n, float *x, float *r, const float *p, const float *t, float *z,
+                                 const float *diag, double tau, const double *pap, int npap,
+                                 double *prr, double *prz, CGState *st, int k, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cg2_update<float, float, 1>), dim3(vec_blocks(n)), dim3(256), 0, s, n, x, r, p, t, z,
-                       diag, tau, prr, prz, st, k, nullptr);
+    hipLaunchKernelGGL((k_cg2_update<float, float>), dim3(vec_blocks(n)), dim3(256), 0, s, n, x, r, p, t, z,
+                       diag, tau, pap, npap, prr, prz, st, k);
     return hipGetLastError();
 }
 
-hipError_t launch_cg2_direction_f64(int n, double *p, const double *z, const CGState *st, int k, hipStream_t s)
+hipError_t launch_cg2_direction_f64(int n, double *p, const double *z, const double *prr,
+                                    const double *prz, int np, CGState *st, int k, double *hist,
+                                    hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cg2_direction<double>), dim3(vec_blocks(n)), dim3(256), 0, s, n, p, z, st, k);
+    hipLaunchKernelGGL((k_cg2_direction<double, 0>), dim3(vec_blocks(n)), dim3(256), 0, s, n, p, z, prr, prz,
+                       np, st, k, hist);
     return hipGetLastError();
 }
 
-hipError_t launch_cg2_direction_f32(int n, float *p, const float *z, const CGState *st, int k, hipStream_t s)
+hipError_t launch_cg2_direction_f32_mixed(int n, float *p, const float *z, const double *prr,
+                                          const double *prz, int np, CGState *st, int k,
+                                          hipStream_t s)
 {
-    hipLaunchKernelGGL((k_cg2_direction<float>), dim3(vec_blocks(n)), dim3(256), 0, s, n, p, z, st, k);
+    hipLaunchKernelGGL((k_cg2_direction<float, 1>), dim3(vec_blocks(n)), dim3(256), 0, s, n, p, z, prr, prz,
+                       np, st, k, nullptr);
     return hipGetLastError();
 }
 
