@@ -99,7 +99,11 @@ def main():
     ap.add_argument("--count", type=int, default=512, help="batch: number of problems")
     ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a plain 1 GiB device copy (calibrates the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        return copy_probe()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
               "the path shards by problem/patch across GPUs, which this build does not run.",
@@ -161,9 +165,12 @@ def main():
         xs = x.download()
         err = float(np.linalg.norm(xs - xref) / np.linalg.norm(xref))
         l2 = gp.l2_error(x)
+        # every CG iteration runs one SpMV; the launches a paused mixed
+        # recurrence skips (about 1 us each) are counted in kms but not here
         results[name] = dict(ms=1e3 * float(np.mean(times)), ms_min=1e3 * float(np.min(times)),
                              iters=int(its[-1]), relres=float(rel), rel_err=err, l2=l2,
-                             spmv_launches=nl, spmv_ms=kms, dofs_per_s=ndofs / float(np.mean(times)))
+                             spmv_launches=nl, spmv_working_launches=int(sum(its)), spmv_ms=kms,
+                             dofs_per_s=ndofs / float(np.mean(times)))
 
     # phase split of one mixed step (separate, untimed for `value`)
     ctx.enable_timing(True)
@@ -185,7 +192,7 @@ def main():
     b.close()
     n = gp.nfree
     mixed = results["mixed"]
-    avg_ms = mixed["spmv_ms"] / max(1, mixed["spmv_launches"])
+    avg_ms = mixed["spmv_ms"] / max(1, mixed["spmv_working_launches"])
     slots = (2 * args.p + 1) ** 3
     ngrid = (args.n + 3 * args.p) ** 3   # nb = n + p per direction, padded by p each side
     traffic_alg = stencil_spmv_bytes(n, slots, ngrid, 4, 4)
@@ -215,7 +222,8 @@ def main():
         "phase_ms_one_mixed_step": phases,
         "roofline": {"kernel": "spmv_f32 (structured (2p+1)^3 stencil, FP32 coefficients)", "bound": "hbm",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": pmc_traffic(f"sspmv_f32_cube{args.n}_p{args.p}"),
                      "algorithmic_bytes_per_launch": traffic_alg, "avg_launch_ms": avg_ms},
     }
     if not args.no_cpu:
@@ -223,6 +231,35 @@ def main():
     gp.close()
     ctx.close()
     print(json.dumps(out))
+
+
+def copy_probe():
+    """A plain device copy of known size, three launches: what the HBM
+    counters read for it calibrates their bytes per KiB unit."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+
+
+def pmc_traffic(name):
+    """HBM bytes per launch of a kernel from its committed counter summary
+    (profiles/pmc_<name>.txt, line 'traffic_bytes_per_launch N'), or None."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{name}.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                key, _, val = line.strip().partition(" ")
+                if key == "traffic_bytes_per_launch":
+                    return int(float(val))
+    except OSError:
+        return None
+    return None
 
 
 def _common(args, metric_value, ms, dtype, workload, extra):
@@ -306,8 +343,9 @@ def bench_mg(args):
     gsx.mg_cg_solve(A, mg, b, x, tol=tol, maxit=500)
     ctx.synchronize()
     phases = {}
-    for k in ("pattern_count", "pattern_scan", "pattern_fill", "elem2d", "gather", "mg_setup", "mg_smooth",
-              "mg_transfer", "mg_coarse", "cg_init", "spmv_f64", "cg_vec"):
+    for k in ("pattern_count", "pattern_scan", "pattern_fill", "elem2d", "gather", "mg_setup", "mg_levels",
+              "mg_transfer_build", "mg_coarse_inverse", "mg_smooth", "mg_transfer", "mg_coarse", "cg_init",
+              "spmv_f64", "cg_vec"):
         nl, ms = ctx.kernel_stats(k)
         if nl:
             phases[k] = round(ms, 3)

@@ -34,16 +34,14 @@
 // iteration flag_next on) until the next reliable-update pass, which
 // forms x64 += xc32, r64 = b - A64 x64 in FP64, decides convergence on
 // ||r64|| / ||b||, and restarts the FP32 recurrence from fp32(r64) with the
-// same direction.
+// same direction (unless the recurrence had parted from the true residual:
+// kDirectionReset).
 #include "gsx_common.h"
 #include "gsx_kernels.h"
 
 namespace gsx {
 
 namespace {
-
-// see the mixed-precision note above: direction restart threshold
-constexpr double kDirectionReset = 1e-6;
 
 __device__ inline double bsum256(double v, double *s)
 {
@@ -389,8 +387,9 @@ __global__ __launch_bounds__(256) void k_b_rel3(const BChunk *__restrict__ chunk
     const double err = sqrt(rr) / S.bnorm;
     const int decided = err < g.tol ? 1 : (S.stop ? 2 : 0);
     // S.rr still holds the FP32 recurrence's residual (written by the
-    // direction step; this launch leaves it to R4): a recurrence far below
-    // the true residual leaves no usable direction, which restarts from z
+    // direction step; this launch leaves it to R4): a recurrence that has
+    // parted from the true residual leaves no usable direction, which
+    // restarts from z
     const bool reset = S.rr < kDirectionReset * rr;
     double rz = 0.0;
     if (!decided) {

@@ -39,6 +39,17 @@ struct GaussTab {
     double w[kMaxQ1 + 1][kMaxQ1];
 };
 
+// Mixed-precision CG: a reliable update replaces the FP32 recurrence's
+// residual r_rec by the true FP64 residual r and keeps the search
+// direction, which is consistent as long as r_rec and r agree (they do up
+// to FP32 rounding while the recurrence is above its attainable accuracy).
+// Once the recurrence has run into its floor they part: the fused update
+// r - alpha t leaves a rounding residue (a few percent of r on tiny systems
+// solved in one step) and the kept direction then makes the iteration
+// diverge.  When (r_rec, r_rec) < kDirectionReset (r, r) the direction
+// restarts from the preconditioned residual instead.
+constexpr double kDirectionReset = 0.25;
+
 // Device-resident scalars of one CG solve.
 struct CGState {
     double rz;        // (r, z) of the current iterate

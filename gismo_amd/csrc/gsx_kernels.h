@@ -78,6 +78,16 @@ hipError_t launch_jacobi_apply_f32(int n, const float *r, float *z, const float 
 hipError_t launch_jacobi_step_f64(int lpr, int n, const int *rp, const int *ci, const double *v,
                                   const double *diag, const double *b, const double *xin,
                                   double *xout, double tau, int x_is_zero, hipStream_t s);
+// FP32 values, FP64 vectors (the finest level of a mixed multigrid cycle)
+hipError_t launch_jacobi_step_v32(int lpr, int n, const int *rp, const int *ci, const float *v,
+                                  const double *diag, const double *b, const double *xin,
+                                  double *xout, double tau, int x_is_zero, hipStream_t s);
+hipError_t launch_residual_v32(int lpr, int n, const int *rp, const int *ci, const float *v, const double *x,
+                               const double *b, double *r, hipStream_t s);
+hipError_t launch_restrict_v32(int n, const int *rp, const int *ci, const float *v, const double *x, float *y,
+                               hipStream_t s);
+hipError_t launch_prolong_add_v32(int n, const int *rp, const int *ci, const float *v, const float *x,
+                                  double *y, double alpha, hipStream_t s);
 hipError_t launch_jacobi_step_f32(int lpr, int n, const int *rp, const int *ci, const float *v,
                                   const float *diag, const float *b, const float *xin,
                                   float *xout, float tau, int x_is_zero, hipStream_t s);

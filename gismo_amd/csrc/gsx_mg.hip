@@ -393,15 +393,16 @@ __global__ __launch_bounds__(256) void k_dense_gemv(int n, int npad, const T *__
     if (lane == 0) y[r] = acc;
 }
 
-// y += alpha * (P x) (prolongation), one thread per fine row (rows short)
-template <typename V, typename T>
+// y += alpha * (P x) (prolongation), one thread per fine row (rows short);
+// accumulated in the fine vector's type T
+template <typename V, typename TX, typename T>
 __global__ void k_prolong_add(int n, const int *__restrict__ rp, const int *__restrict__ ci,
-                              const V *__restrict__ v, const T *__restrict__ x, T *__restrict__ y, T alpha)
+                              const V *__restrict__ v, const TX *__restrict__ x, T *__restrict__ y, T alpha)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     T s = 0;
-    for (int k = rp[r]; k < rp[r + 1]; ++k) s += (T)v[k] * x[ci[k]];
+    for (int k = rp[r]; k < rp[r + 1]; ++k) s += (T)v[k] * (T)x[ci[k]];
     y[r] += alpha * s;
 }
 
@@ -422,16 +423,17 @@ __global__ __launch_bounds__(256) void k_residual(int n, const int *__restrict__
     if (row < n && lane == 0) r[row] = b[row] - acc;
 }
 
-// y = R x (restriction), one thread per coarse row
-template <typename V, typename T>
+// y = R x (restriction), one thread per coarse row; accumulated in the fine
+// vector's type T, stored in the coarse vector's type TY
+template <typename V, typename T, typename TY>
 __global__ void k_restrict(int n, const int *__restrict__ rp, const int *__restrict__ ci,
-                           const V *__restrict__ v, const T *__restrict__ x, T *__restrict__ y)
+                           const V *__restrict__ v, const T *__restrict__ x, TY *__restrict__ y)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     T s = 0;
     for (int k = rp[r]; k < rp[r + 1]; ++k) s += (T)v[k] * x[ci[k]];
-    y[r] = s;
+    y[r] = (TY)s;
 }
 
 template <typename S, typename Dd>
@@ -526,28 +528,42 @@ hipError_t launch_dense_gemv_f64(int n, int npad, const double *M, const double 
 hipError_t launch_prolong_add_f64(int n, const int *rp, const int *ci, const double *v, const double *x,
                                   double *y, double alpha, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_prolong_add<double, double>), g1(n), dim3(256), 0, s, n, rp, ci, v, x, y, alpha);
+    hipLaunchKernelGGL((k_prolong_add<double, double, double>), g1(n), dim3(256), 0, s, n, rp, ci, v, x, y, alpha);
     return hipGetLastError();
 }
 
 hipError_t launch_prolong_add_f32(int n, const int *rp, const int *ci, const float *v, const float *x,
                                   float *y, float alpha, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_prolong_add<float, float>), g1(n), dim3(256), 0, s, n, rp, ci, v, x, y, alpha);
+    hipLaunchKernelGGL((k_prolong_add<float, float, float>), g1(n), dim3(256), 0, s, n, rp, ci, v, x, y, alpha);
     return hipGetLastError();
 }
 
 hipError_t launch_restrict_f64(int n, const int *rp, const int *ci, const double *v, const double *x,
                                double *y, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_restrict<double, double>), g1(n), dim3(256), 0, s, n, rp, ci, v, x, y);
+    hipLaunchKernelGGL((k_restrict<double, double, double>), g1(n), dim3(256), 0, s, n, rp, ci, v, x, y);
     return hipGetLastError();
 }
 
 hipError_t launch_restrict_f32(int n, const int *rp, const int *ci, const float *v, const float *x, float *y,
                                hipStream_t s)
 {
-    hipLaunchKernelGGL((k_restrict<float, float>), g1(n), dim3(256), 0, s, n, rp, ci, v, x, y);
+    hipLaunchKernelGGL((k_restrict<float, float, float>), g1(n), dim3(256), 0, s, n, rp, ci, v, x, y);
+    return hipGetLastError();
+}
+
+hipError_t launch_prolong_add_v32(int n, const int *rp, const int *ci, const float *v, const float *x,
+                                  double *y, double alpha, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_prolong_add<float, float, double>), g1(n), dim3(256), 0, s, n, rp, ci, v, x, y, alpha);
+    return hipGetLastError();
+}
+
+hipError_t launch_restrict_v32(int n, const int *rp, const int *ci, const float *v, const double *x, float *y,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL((k_restrict<float, double, float>), g1(n), dim3(256), 0, s, n, rp, ci, v, x, y);
     return hipGetLastError();
 }
 
@@ -569,6 +585,12 @@ hipError_t launch_residual_f64(int lpr, int n, const int *rp, const int *ci, con
                                const double *b, double *r, hipStream_t s)
 {
     return residual_dispatch<double, double>(lpr, n, rp, ci, v, x, b, r, s);
+}
+
+hipError_t launch_residual_v32(int lpr, int n, const int *rp, const int *ci, const float *v, const double *x,
+                               const double *b, double *r, hipStream_t s)
+{
+    return residual_dispatch<float, double>(lpr, n, rp, ci, v, x, b, r, s);
 }
 
 hipError_t launch_residual_f32(int lpr, int n, const int *rp, const int *ci, const float *v, const float *x,

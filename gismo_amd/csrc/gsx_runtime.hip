@@ -1290,12 +1290,6 @@ static gsx_status cg_workspace(gsx_matrix *A, int maxit, bool mixed, bool grid =
     return GSX_OK;
 }
 
-// Mixed solvers: after a reliable update the FP32 direction is kept (the
-// recurrence continues with the true residual), unless the FP32 residual
-// had fallen below this fraction of the true one: the direction then
-// restarts from the preconditioned residual.
-constexpr double kDirectionReset = 1e-6;
-
 static gsx_status read_state(gsx_ctx *c, const CGWork &w, CGState &h)
 {
     HIP_OK(c, hipMemcpyAsync(&h, w.state.get(), sizeof(CGState), hipMemcpyDeviceToHost, c->stream));
@@ -1465,9 +1459,9 @@ static gsx_status cg_mixed(gsx_matrix *A, int precond, double tau, const double 
                 if (h.done == 1 || h.done == 2) break;
                 gsx_status e2 = restart_from_r64();
                 if (e2 != GSX_OK) return e2;
-                // a recurrence that ran far below the true residual (down to
-                // an exact zero on tiny systems) leaves no usable direction
-                if (rr_rec < kDirectionReset * h.rr)
+                // a recurrence that has parted from the true residual (it
+                // ran into its FP32 floor) leaves no usable direction
+                if (rr_rec < gsx::kDirectionReset * h.rr)
                     HIP_OK(c, hipMemcpyAsync(p32, jac ? z32 : r32, sizeof(float) * n, hipMemcpyDeviceToDevice, s));
             }
             if (h.it >= maxit) break;
@@ -1646,9 +1640,9 @@ static gsx_status cg_grid(gsx_matrix *A, int precond, double tau, const double *
                 GSX_TRY(read_state(c, w, h));
                 if (h.done == 1 || h.done == 2) break;
                 GSX_TRY(restart(k));
-                // a recurrence that ran far below the true residual (down to
-                // an exact zero on tiny systems) leaves no usable direction
-                if (rr_rec < kDirectionReset * h.rr)
+                // a recurrence that has parted from the true residual (it
+                // ran into its FP32 floor) leaves no usable direction
+                if (rr_rec < gsx::kDirectionReset * h.rr)
                     HIP_OK(c, hipMemcpyAsync(p32, z32, sizeof(float) * N, hipMemcpyDeviceToDevice, s));
             }
             if (h.it >= maxit) break;
@@ -1934,6 +1928,7 @@ extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nl
     }
     // coarse problems (geometry is not used by the Galerkin levels)
     for (int l = nlevels - 2; l >= 0; --l) {
+        KScope kl(c, "mg_levels");
         std::vector<gsx_patch_desc> desc(npatch);
         std::vector<std::vector<double>> ctrl(npatch);
         for (int k = 0; k < npatch; ++k) {
@@ -1965,6 +1960,7 @@ extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nl
     // transfers and Galerkin products, finest first
     for (int l = nlevels - 2; l >= 0; --l) {
         MGLevel &C = mg->L[l], &F = mg->L[l + 1];
+        KScope kt(c, "mg_transfer_build");
         gsx_problem *fp = (l + 1 == nlevels - 1) ? fine : F.prob;
         gsx_problem *cp = C.prob;
         // 1D transfers per (patch, direction)
@@ -2107,6 +2103,7 @@ extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nl
     }
     // exact coarse solve: dense FP64 inverse of the coarsest operator
     {
+        KScope ki(c, "mg_coarse_inverse");
         MGLevel &C0 = mg->L[0];
         const int n0 = C0.n;
         if ((int64_t)n0 > 32768) return fail(c, GSX_ERR_ARG, "coarsest level too large for the dense inverse");
@@ -2183,17 +2180,27 @@ static SmoothCoef smooth_coef(const gsx_mg *mg, const MGLevel &V, int k, double 
     return c;
 }
 
-// m smoothing sweeps on level V (FP64).  Jacobi ping-pongs x and t (x may
-// come back as t); the polynomial smoothers update x in place.
-static gsx_status smooth64(gsx_mg *mg, MGLevel &V, double *&x, double *&t, const double *b, bool &zero, int m)
+// m smoothing sweeps on level V with FP64 vectors (matrix values FP64, or
+// FP32 when v32: the finest level of a mixed cycle).  Jacobi ping-pongs x
+// and t (x may come back as t); the polynomial smoothers update x in place.
+static gsx_status smooth64(gsx_mg *mg, MGLevel &V, double *&x, double *&t, const double *b, bool &zero, int m,
+                           bool v32 = false)
 {
     gsx_ctx *c = mg->ctx;
     hipStream_t s = c->stream;
     const int *rp = dp<int>(V.rowptr), *ci = dp<int>(V.col);
     const double *va = dp<double>(V.val), *dg = dp<double>(V.diag);
+    const float *va32 = v32 ? dp<float>(V.val32) : nullptr;
+    auto residual = [&](const double *xx, double *rr) {
+        return v32 ? gsx::launch_residual_v32(V.lpr, V.n, rp, ci, va32, xx, b, rr, s)
+                   : gsx::launch_residual_f64(V.lpr, V.n, rp, ci, va, xx, b, rr, s);
+    };
     if (mg->smoother == GSX_SMOOTH_JACOBI) {
         for (int i = 0; i < m; ++i) {
-            HIP_OK(c, gsx::launch_jacobi_step_f64(V.lpr, V.n, rp, ci, va, dg, b, x, t, mg->tau, zero ? 1 : 0, s));
+            if (v32)
+                HIP_OK(c, gsx::launch_jacobi_step_v32(V.lpr, V.n, rp, ci, va32, dg, b, x, t, mg->tau, zero ? 1 : 0, s));
+            else
+                HIP_OK(c, gsx::launch_jacobi_step_f64(V.lpr, V.n, rp, ci, va, dg, b, x, t, mg->tau, zero ? 1 : 0, s));
             std::swap(x, t);
             zero = false;
         }
@@ -2203,7 +2210,7 @@ static gsx_status smooth64(gsx_mg *mg, MGLevel &V, double *&x, double *&t, const
     double *r = dp<double>(V.r), *d = dp<double>(V.d);
     double rho = 0.0;
     for (int k = 0; k < m; ++k) {
-        HIP_OK(c, gsx::launch_residual_f64(V.lpr, V.n, rp, ci, va, x, b, r, s));
+        HIP_OK(c, residual(x, r));
         const SmoothCoef cf = smooth_coef(mg, V, k, rho);
         HIP_OK(c, gsx::launch_smooth_update_f64(V.n, x, d, r, mg->smoother == GSX_SMOOTH_CHEBYSHEV ? dg : nullptr,
                                                 cf.c1, cf.c2, k == 0 ? 1 : 0, s));
@@ -2239,7 +2246,15 @@ static gsx_status smooth32(gsx_mg *mg, MGLevel &V, float *&x, float *&t, const f
     return GSX_OK;
 }
 
-static gsx_status mg_cycle64(gsx_mg *mg, int l)
+static gsx_status mg_cycle32(gsx_mg *mg, int l);
+
+// One V-cycle step at level l with FP64 vectors.  mixed: level l is the
+// finest level of a mixed-precision cycle: its matrix values are read in
+// FP32 and the coarser levels run mg_cycle32.  The fine vectors stay FP64:
+// an FP32 copy of the smooth fine-level iterate carries a rounding error
+// whose residual is of order eps cond(A) ||b||, which costs outer
+// iterations; FP32 matrix values on their own do not.
+static gsx_status mg_cycle64(gsx_mg *mg, int l, bool mixed = false)
 {
     gsx_ctx *c = mg->ctx;
     hipStream_t s = c->stream;
@@ -2255,33 +2270,44 @@ static gsx_status mg_cycle64(gsx_mg *mg, int l)
     bool zero = true;
     {
         KScope k(c, "mg_smooth");
-        GSX_TRY(smooth64(mg, V, x, t, b, zero, mg->npre));
+        GSX_TRY(smooth64(mg, V, x, t, b, zero, mg->npre, mixed));
     }
     MGLevel &C = mg->L[l - 1];
     {
         KScope k(c, "mg_transfer");
         if (zero) HIP_OK(c, hipMemcpyAsync(r, b, sizeof(double) * V.n, hipMemcpyDeviceToDevice, s));
+        else if (mixed) HIP_OK(c, gsx::launch_residual_v32(V.lpr, V.n, rp, ci, dp<float>(V.val32), x, b, r, s));
         else HIP_OK(c, gsx::launch_residual_f64(V.lpr, V.n, rp, ci, va, x, b, r, s));
-        HIP_OK(c, gsx::launch_restrict_f64(C.n, dp<int>(C.rrp), dp<int>(C.rci), dp<double>(C.rv), r, dp<double>(C.b), s));
+        if (mixed)
+            HIP_OK(c, gsx::launch_restrict_v32(C.n, dp<int>(C.rrp), dp<int>(C.rci), dp<float>(C.rv32), r,
+                                               dp<float>(C.b32), s));
+        else
+            HIP_OK(c, gsx::launch_restrict_f64(C.n, dp<int>(C.rrp), dp<int>(C.rci), dp<double>(C.rv), r,
+                                               dp<double>(C.b), s));
     }
     // coarse correction from a zero start (one V-cycle step)
-    GSX_TRY(mg_cycle64(mg, l - 1));
+    if (mixed) GSX_TRY(mg_cycle32(mg, l - 1));
+    else GSX_TRY(mg_cycle64(mg, l - 1));
     {
         KScope k(c, "mg_transfer");
         if (zero) HIP_OK(c, gsx::launch_fill_f64(x, V.n, 0.0, s));
-        HIP_OK(c, gsx::launch_prolong_add_f64(V.n, dp<int>(C.prp),
-                                              dp<int>(C.pci), dp<double>(C.pv), dp<double>(C.x), x, mg->cgc, s));
+        if (mixed)
+            HIP_OK(c, gsx::launch_prolong_add_v32(V.n, dp<int>(C.prp), dp<int>(C.pci), dp<float>(C.pv32),
+                                                  dp<float>(C.x32), x, mg->cgc, s));
+        else
+            HIP_OK(c, gsx::launch_prolong_add_f64(V.n, dp<int>(C.prp), dp<int>(C.pci), dp<double>(C.pv),
+                                                  dp<double>(C.x), x, mg->cgc, s));
     }
     {
         KScope k(c, "mg_smooth");
-        GSX_TRY(smooth64(mg, V, x, t, b, zero, mg->npost));
+        GSX_TRY(smooth64(mg, V, x, t, b, zero, mg->npost, mixed));
     }
     if (x != dp<double>(V.x)) HIP_OK(c, hipMemcpyAsync(dp<double>(V.x), x, sizeof(double) * V.n, hipMemcpyDeviceToDevice, s));
     return GSX_OK;
 }
 
-// FP32 cycle (operators, transfers and vectors in FP32; the coarsest solve
-// in FP64 through conversions).
+// FP32 cycle below the finest level of a mixed cycle (operators, transfers
+// and vectors in FP32; the coarsest solve in FP64 through conversions).
 static gsx_status mg_cycle32(gsx_mg *mg, int l)
 {
     gsx_ctx *c = mg->ctx;
@@ -2331,15 +2357,9 @@ static gsx_status mg_apply_dev(gsx_mg *mg, const double *r, double *z)
     hipStream_t s = c->stream;
     MGLevel &F = mg->L.back();
     const int nl = (int)mg->L.size();
-    if (mg->precision == GSX_MIXED) {
-        HIP_OK(c, gsx::launch_conv_f64_f32(F.n, r, dp<float>(F.b32), s));
-        GSX_TRY(mg_cycle32(mg, nl - 1));
-        HIP_OK(c, gsx::launch_conv_f32_f64(F.n, dp<float>(F.x32), z, s));
-    } else {
-        HIP_OK(c, hipMemcpyAsync(dp<double>(F.b), r, sizeof(double) * F.n, hipMemcpyDeviceToDevice, s));
-        GSX_TRY(mg_cycle64(mg, nl - 1));
-        HIP_OK(c, hipMemcpyAsync(z, dp<double>(F.x), sizeof(double) * F.n, hipMemcpyDeviceToDevice, s));
-    }
+    HIP_OK(c, hipMemcpyAsync(dp<double>(F.b), r, sizeof(double) * F.n, hipMemcpyDeviceToDevice, s));
+    GSX_TRY(mg_cycle64(mg, nl - 1, mg->precision == GSX_MIXED && nl > 1));
+    HIP_OK(c, hipMemcpyAsync(z, dp<double>(F.x), sizeof(double) * F.n, hipMemcpyDeviceToDevice, s));
     return GSX_OK;
 }
 

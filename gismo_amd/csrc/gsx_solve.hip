@@ -518,6 +518,21 @@ hipError_t launch_jacobi_step_f64(int lpr, int n, const int *rp, const int *ci, 
     return hipGetLastError();
 }
 
+hipError_t launch_jacobi_step_v32(int lpr, int n, const int *rp, const int *ci, const float *v,
+                                  const double *diag, const double *b, const double *xin,
+                                  double *xout, double tau, int x_is_zero, hipStream_t s)
+{
+    const dim3 grid(spmv_blocks(n, lpr)), block(256);
+    switch (lpr) {
+    case 64: hipLaunchKernelGGL((k_jacobi_step<64, float, double>), grid, block, 0, s, n, rp, ci, v, diag, b, xin, xout, tau, x_is_zero); break;
+    case 32: hipLaunchKernelGGL((k_jacobi_step<32, float, double>), grid, block, 0, s, n, rp, ci, v, diag, b, xin, xout, tau, x_is_zero); break;
+    case 16: hipLaunchKernelGGL((k_jacobi_step<16, float, double>), grid, block, 0, s, n, rp, ci, v, diag, b, xin, xout, tau, x_is_zero); break;
+    case 8: hipLaunchKernelGGL((k_jacobi_step<8, float, double>), grid, block, 0, s, n, rp, ci, v, diag, b, xin, xout, tau, x_is_zero); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_jacobi_step_f32(int lpr, int n, const int *rp, const int *ci, const float *v,
                                   const float *diag, const float *b, const float *xin,
                                   float *xout, float tau, int x_is_zero, hipStream_t s)

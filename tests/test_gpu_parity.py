@@ -166,7 +166,7 @@ def test_structured_operator_matches_csr(ctx, d, p, n):
     print(f"[case] {prob.name}", flush=True)
     dofs, nfree, Ao, bo = _oracle(prob)
     # relres 1e-12 so that two converged solutions agree to ~cond(A)*1e-12
-    xo, ito, erro = O.cg_solve(Ao, bo, tol=1e-12, maxit=3000)
+    xo, ito, erro, ho = O.cg_solve(Ao, bo, tol=1e-12, maxit=3000, history=True)
     gp = gsx.Problem(ctx, prob)
     A, b = gp.assemble(gsx.PATTERN_EXPR, gsx.MIXED)
     sols = {}
@@ -176,15 +176,22 @@ def test_structured_operator_matches_csr(ctx, d, p, n):
             x = ctx.array(nfree).zero()
             it, rel, ok = A.cg_solve(b, x, tol=1e-12, maxit=3000, precision=prec)
             assert ok and rel < 1e-12, (structured, prec, it, rel)
-            sols[(structured, prec)] = (it, x.download())
-    for key, (it, xs) in sols.items():
+            sols[(structured, prec)] = (it, x.download(), A.history() if prec == gsx.FP64 else None)
+    for key, (it, xs, _) in sols.items():
         assert np.linalg.norm(xs - xo) <= 1e-8 * np.linalg.norm(xo), key
-    # FP64 recurrences: the oracle's iteration count; at relres 1e-12 the
-    # residual history is at rounding level, where the block-wise dot
-    # products may move the last iteration by one (exact equality is
-    # asserted at 1e-10 in test_cg_jacobi_matches_oracle)
-    assert abs(sols[(True, gsx.FP64)][0] - ito) <= 1
-    assert abs(sols[(False, gsx.FP64)][0] - ito) <= 1
+    # FP64 recurrences: the oracle's residual history to rounding level over
+    # the first iterations; further on, the residual norm of an ill-conditioned
+    # CG amplifies the dot products' summation-order rounding (a CPU CG with
+    # pairwise sums departs from the oracle by O(1) at iteration ~100 on the
+    # degree-4 cube), so the stop is checked to lie where the oracle's
+    # residual is within a factor 3 of the tolerance, or within 3% of its
+    # count (exact equality of the count is asserted at 1e-10 on better
+    # conditioned systems in test_cg_jacobi_matches_oracle)
+    first_near = int(np.argmax(ho < 3e-12))
+    for structured in (True, False):
+        it, _, h = sols[(structured, gsx.FP64)]
+        assert np.allclose(h[:10], ho[:10], rtol=1e-9, atol=0), structured
+        assert first_near <= it <= ito + 3 or abs(it - ito) <= 0.03 * ito, (structured, it, first_near, ito)
 
 
 def test_structured_partial_dirichlet(ctx):
@@ -272,22 +279,28 @@ def test_mixed_tiny_patches_converge(ctx, d, p, n):
     """Tiny systems, where the FP32 recurrence reaches rounding level within
     a few iterations: the mixed solvers (structured and CSR, single solve
     and batched) pause their FP32 iterations until the FP64 update and reach
-    the FP64 solution."""
+    the FP64 solution: the true residual b - A x (oracle matrix) is below the
+    tolerance and the error below cond(A) times it."""
     P = problems.box_patch(d, p, n)
     P.fkind = 1
     prob = problems.Problem([P], problems.all_sides(d), [], name=f"tiny-d{d}-p{p}-n{n}")
     dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
     Ao, bo = O.assemble(prob.patches, dofs, nfree, 0)
     xo, _, _ = O.cg_solve(Ao, bo, tol=1e-14, maxit=1000)
+    cond = np.linalg.cond(Ao.toarray())
+
+    def check(xs):
+        assert np.linalg.norm(bo - Ao @ xs) <= 1.01e-10 * np.linalg.norm(bo)
+        assert np.linalg.norm(xs - xo) <= 1.01e-10 * cond * np.linalg.norm(xo)
+
     gp = gsx.Problem(ctx, prob)
     A, b = gp.assemble(gsx.PATTERN_EXPR, gsx.MIXED)
     for structured in (True, False):
         A.set_operator(structured)
         x = ctx.array(gp.nfree).zero()
         it, rel, ok = A.cg_solve(b, x, tol=1e-10, maxit=2000, precision=gsx.MIXED)
-        xs = x.download()
         assert ok and rel < 1e-10 and it < 200, (structured, it, rel)
-        assert np.linalg.norm(xs - xo) <= 1e-8 * np.linalg.norm(xo)
+        check(x.download())
         x.close()
     B = gsx.Batch(ctx, gsx.MIXED, capacity=2)
     x = ctx.array(gp.nfree).zero()
@@ -295,7 +308,7 @@ def test_mixed_tiny_patches_converge(ctx, d, p, n):
     B.run(8)
     st, it, rel = B.result(i)
     assert st == 1 and rel < 1e-10 and it < 200, (it, rel)
-    assert np.linalg.norm(x.download() - xo) <= 1e-8 * np.linalg.norm(xo)
+    check(x.download())
     B.close()
     x.close()
     A.close()
