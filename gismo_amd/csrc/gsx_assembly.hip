@@ -682,7 +682,7 @@ __global__ __launch_bounds__(256) void k_gather(const PatchDev *__restrict__ pat
                         }
                     }
                     __builtin_amdgcn_wave_barrier();
-                    racc += Fbuf[pd.fvec_off + el * NB + lb];
+                    if (Fbuf) racc += Fbuf[pd.fvec_off + el * NB + lb];
                 }
     }
     __builtin_amdgcn_wave_barrier();
@@ -690,7 +690,7 @@ __global__ __launch_bounds__(256) void k_gather(const PatchDev *__restrict__ pat
         val[rb + k] = sAcc[w][k];
         if (keep_flags) keep_flags[rb + k] = sNz[w][k];
     }
-    if (lane == 0) rhs[r] = racc;
+    if (rhs && lane == 0) rhs[r] = racc;
 }
 
 // ---------------------------------------------------------------------------

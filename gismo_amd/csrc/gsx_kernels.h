@@ -118,6 +118,11 @@ hipError_t launch_ap(int d, int n, const int *arp, const int *aci, const double 
 hipError_t launch_rap(int d, int nc, const int *rrp, const int *rci, const double *rv, const int *bcnt,
                       const int *bci, const double *bv, const int *crp, const int *cci, double *cval,
                       int *err, hipStream_t s);
+// element-level Galerkin coarsening (k_coarsen): supported (d, p)
+bool coarsen_supported(int d, int p);
+hipError_t launch_coarsen(int d, int p, const PatchDev *cpatches, const PatchDev *fpatches, const int *gpatch,
+                          const int64_t *gprefix, int ng, int64_t nelem_c, const double *T1, const int64_t *toff,
+                          const double *Kf, double *Kc, hipStream_t s);
 hipError_t launch_dense_inverse(int n, int npad, const int *rp, const int *ci, const double *v,
                                 double *D, double *W, double *Pinv, hipStream_t s);
 hipError_t launch_dense_gemv_f64(int n, int npad, const double *M, const double *x, double *y,

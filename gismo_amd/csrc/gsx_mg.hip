@@ -238,6 +238,112 @@ __global__ void k_rap(int nc, const int *__restrict__ rrp, const int *__restrict
 }
 
 // ---------------------------------------------------------------------------
+// Element-level Galerkin product.  Coarse element E of a dyadically coarsened
+// patch gets K_E = sum over its 2^D child elements e (lexicographic) of
+// T_e^T K_e T_e, where T_e = kron of the 1D blocks T[m][a] = coefficient of
+// coarse function E_k + a in fine function e_k + m (the knot-insertion
+// transfer restricted to the two spans).  Gathered with the coarse DOF map
+// this is P^T A P restricted to the free functions (the product that
+// gsMultiGridOp::init forms with the global transfer,
+// src/gsMultiGrid/gsMultiGrid.hpp:70-94), without forming A P.
+// One block per coarse element; the child matrix is transformed in place in
+// LDS, one line of P+1 entries along one index digit per thread and pass
+// (2 D passes: row digits, then column digits).
+// ---------------------------------------------------------------------------
+template <int D, int P>
+__global__ __launch_bounds__(256) void k_coarsen(const PatchDev *__restrict__ cpatches,
+                                                 const PatchDev *__restrict__ fpatches,
+                                                 const int *__restrict__ gpatch,
+                                                 const int64_t *__restrict__ gprefix, int ng,
+                                                 const double *__restrict__ T1,
+                                                 const int64_t *__restrict__ toff,
+                                                 const double *__restrict__ Kf, double *__restrict__ Kc)
+{
+    constexpr int NB1 = P + 1;
+    constexpr int NB = (D == 3) ? NB1 * NB1 * NB1 : NB1 * NB1;
+    constexpr int N2 = NB * NB;
+    constexpr int PER = (N2 + 255) / 256;
+    constexpr int NLINE = N2 / NB1;
+    __shared__ double sK[N2];
+    __shared__ double sT[3][NB1][NB1];
+    const int tid = threadIdx.x;
+    const int64_t E = blockIdx.x;
+    int lo = 0, hi = ng - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (gprefix[mid] <= E) lo = mid; else hi = mid - 1;
+    }
+    const int pk = gpatch[lo];
+    const PatchDev &cp = cpatches[pk];
+    const PatchDev &fp = fpatches[pk];
+    const int64_t e = E - gprefix[lo];
+    int Ei[3];
+    Ei[0] = (int)(e % cp.nel[0]);
+    Ei[1] = (int)((e / cp.nel[0]) % cp.nel[1]);
+    Ei[2] = (D == 3) ? (int)(e / ((int64_t)cp.nel[0] * cp.nel[1])) : 0;
+    double acc[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) acc[j] = 0.0;
+    for (int child = 0; child < (1 << D); ++child) {
+        int ei[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < D; ++k) ei[k] = 2 * Ei[k] + ((child >> k) & 1);
+        const int64_t fe = ei[0] + (int64_t)fp.nel[0] * (ei[1] + (int64_t)fp.nel[1] * ei[2]);
+        const double *K = Kf + fp.kmat_off + fe * N2;
+        for (int i = tid; i < N2; i += 256) sK[i] = K[i];
+        for (int i = tid; i < D * NB1 * NB1; i += 256) {
+            const int k = i / (NB1 * NB1), q = i % (NB1 * NB1);
+            sT[k][q / NB1][q % NB1] = T1[toff[3 * pk + k] + (int64_t)ei[k] * NB1 * NB1 + q];
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int pass = 0; pass < 2 * D; ++pass) {
+            const int side = pass / D, k = pass % D;
+            int stride = 1;
+            for (int j = 0; j < k; ++j) stride *= NB1;
+            for (int ln = tid; ln < NLINE; ln += 256) {
+                // line ln: the other NB - 1 digits of the transformed index
+                // and the full other index
+                const int other = ln % NB, rest = ln / NB;   // rest in [0, NB / NB1)
+                const int lowd = rest % stride, highd = rest / stride;
+                const int base = lowd + highd * stride * NB1;
+                double v[NB1], w[NB1];
+#pragma unroll
+                for (int m = 0; m < NB1; ++m) {
+                    const int idx = base + m * stride;
+                    v[m] = side == 0 ? sK[idx * NB + other] : sK[other * NB + idx];
+                }
+#pragma unroll
+                for (int a = 0; a < NB1; ++a) {
+                    double sacc = 0.0;
+#pragma unroll
+                    for (int m = 0; m < NB1; ++m) sacc += sT[k][m][a] * v[m];
+                    w[a] = sacc;
+                }
+#pragma unroll
+                for (int a = 0; a < NB1; ++a) {
+                    const int idx = base + a * stride;
+                    if (side == 0) sK[idx * NB + other] = w[a]; else sK[other * NB + idx] = w[a];
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int i = tid + j * 256;
+            if (i < N2) acc[j] += sK[i];
+        }
+        __syncthreads();
+    }
+    double *Ko = Kc + cp.kmat_off + e * N2;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int i = tid + j * 256;
+        if (i < N2) Ko[i] = acc[j];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Dense coarse inverse by blocked Gauss-Jordan (SPD, no pivoting), block 64.
 // ---------------------------------------------------------------------------
 constexpr int kGJ = 64;
@@ -262,33 +368,34 @@ __global__ void k_densify(int n, int npad, const int *__restrict__ rp, const int
     D[i] = x;
 }
 
-// invert the pivot block (kb) in LDS; result to Pinv (64x64, row-major)
+// invert the pivot block (kb) in LDS; result to Pinv (64x64, row-major).
+// In-place Gauss-Jordan: step p maps every entry from the old values (kept
+// in registers between the two barriers), with one reciprocal per step.
 __global__ __launch_bounds__(256) void k_gj_pivot(double *__restrict__ D, int npad, int kb, double *__restrict__ Pinv)
 {
     __shared__ double M[kGJ][kGJ + 1];
+    constexpr int PER = kGJ * kGJ / 256;
     const int t = threadIdx.x;
     for (int i = t; i < kGJ * kGJ; i += 256) M[i / kGJ][i % kGJ] = D[(int64_t)(kb + i / kGJ) * npad + kb + i % kGJ];
     __syncthreads();
     for (int p = 0; p < kGJ; ++p) {
-        const double piv = M[p][p];
-        __syncthreads();
-        // row p scaled; others eliminated (in-place Gauss-Jordan)
-        for (int i = t; i < kGJ * kGJ; i += 256) {
-            const int r = i / kGJ, c = i % kGJ;
-            if (r != p) {
-                const double f = M[r][p] / piv;
-                if (c != p) M[r][c] -= f * M[p][c];
-            }
+        const double inv = 1.0 / M[p][p];
+        double nv[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int i = t + 256 * j, r = i / kGJ, c = i % kGJ;
+            const double a = M[r][c];
+            if (r != p && c != p) nv[j] = a - M[r][p] * inv * M[p][c];
+            else if (r == p && c == p) nv[j] = inv;
+            else if (c == p) nv[j] = -a * inv;
+            else nv[j] = a * inv;
         }
         __syncthreads();
-        for (int i = t; i < kGJ; i += 256) {
-            if (i != p) {
-                M[i][p] = -M[i][p] / piv;
-                M[p][i] = M[p][i] / piv;
-            }
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int i = t + 256 * j;
+            M[i / kGJ][i % kGJ] = nv[j];
         }
-        __syncthreads();
-        if (t == 0) M[p][p] = 1.0 / piv;
         __syncthreads();
     }
     for (int i = t; i < kGJ * kGJ; i += 256) Pinv[i] = M[i / kGJ][i % kGJ];
@@ -502,6 +609,21 @@ hipError_t launch_rap(int d, int nc, const int *rrp, const int *rci, const doubl
     else
         hipLaunchKernelGGL((k_rap<512, 1024>), g1(nc, 64), dim3(64), 0, s, nc, rrp, rci, rv, bcnt, bci, bv, crp, cci, cval, err);
     return hipGetLastError();
+}
+
+bool coarsen_supported(int d, int p) { return (d == 2 && p >= 1 && p <= 4) || (d == 3 && p >= 1 && p <= 3); }
+
+hipError_t launch_coarsen(int d, int p, const PatchDev *cpatches, const PatchDev *fpatches, const int *gpatch,
+                          const int64_t *gprefix, int ng, int64_t nelem_c, const double *T1, const int64_t *toff,
+                          const double *Kf, double *Kc, hipStream_t s)
+{
+    if (nelem_c <= 0) return hipSuccess;
+    const dim3 grid((unsigned)nelem_c), blk(256);
+#define GSX_K(D, P) if (d == D && p == P) { hipLaunchKernelGGL((k_coarsen<D, P>), grid, blk, 0, s, cpatches, fpatches, gpatch, gprefix, ng, T1, toff, Kf, Kc); return hipGetLastError(); }
+    GSX_K(2, 1) GSX_K(2, 2) GSX_K(2, 3) GSX_K(2, 4)
+    GSX_K(3, 1) GSX_K(3, 2) GSX_K(3, 3)
+#undef GSX_K
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_dense_inverse(int n, int npad, const int *rp, const int *ci, const double *v, double *D,

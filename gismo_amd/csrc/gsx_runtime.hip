@@ -888,6 +888,7 @@ struct gsx_matrix {
     StencilOp st;
     bool use_stencil = true;
     bool mixed = false;       // assembled with GSX_MIXED (FP32 operator available)
+    const gsx_problem *src = nullptr;   // problem it was assembled from (values untouched since)
     std::vector<double> hist;
 };
 
@@ -1132,6 +1133,7 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
     }
     A->lpr = gsx::spmv_lpr(A->n ? (double)A->nnz / A->n : 1.0);
     A->mixed = precision == GSX_MIXED;
+    A->src = P;
     if (precision == GSX_MIXED && !A->st.available) {
         A->val32 = dalloc(c, sizeof(float) * std::max<int64_t>(1, A->nnz));
         if (!A->val32) return fail(c, GSX_ERR_OOM, "device allocation (fp32 values)");
@@ -1957,6 +1959,28 @@ extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nl
         st = pattern_build(cp);
         if (st != GSX_OK) return st;
     }
+    // Galerkin levels from element matrices when A is the assembled matrix of
+    // `fine` and the element matrices of two neighbouring levels fit the
+    // element budget: the fine element matrices are recomputed, then each
+    // coarse level's come from its children's (k_coarsen) and are gathered
+    // into the coarse pattern.  Otherwise the global product P^T (A P).
+    bool elem_path = A->src == fine && nlevels > 1;
+    for (const Group &g : fine->groups) elem_path = elem_path && gsx::coarsen_supported(g.d, g.p);
+    if (elem_path) {
+        const int64_t kc = nlevels > 1 ? mg->L[nlevels - 2].prob->kdoubles : 0;
+        elem_path = (fine->kdoubles + fine->fdoubles + kc) * (int64_t)sizeof(double) <= c->elem_budget;
+    }
+    DevPtr Kf;
+    if (elem_path) {
+        KScope ke(c, "mg_elem");
+        Kf = dalloc(c, sizeof(double) * std::max<int64_t>(1, fine->kdoubles));
+        DevPtr Ff = dalloc(c, sizeof(double) * std::max<int64_t>(1, fine->fdoubles));
+        if (!Kf || !Ff) return fail(c, GSX_ERR_OOM, "device allocation (element matrices)");
+        for (const Group &g : fine->groups)
+            HIP_OK(c, gsx::launch_elem(g.d, g.p, dp<PatchDev>(fine->d_patches), dp<int>(g.d_gpatch),
+                                       dp<int64_t>(g.d_gprefix), (int)g.patches.size(), g.nelem, c->gt,
+                                       dp<double>(Kf), dp<double>(Ff), dp<double>(fine->d_detmin), s));
+    }
     // transfers and Galerkin products, finest first
     for (int l = nlevels - 2; l >= 0; --l) {
         MGLevel &C = mg->L[l], &F = mg->L[l + 1];
@@ -1969,6 +1993,10 @@ extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nl
         std::vector<int> trp, tcol;
         std::vector<double> tval;
         std::vector<int> fnb((size_t)npatch * 3, 1), cnb((size_t)npatch * 3, 1), cdoff(npatch, 0);
+        // element path: per (patch, direction, fine span e) the block
+        // T[m][a] = Tm[e + m][e / 2 + a], (p+1)^2 doubles
+        std::vector<double> tblk;
+        std::vector<int64_t> toff((size_t)npatch * 3, 0);
         for (int k = 0; k < npatch; ++k) {
             cdoff[k] = (int)cp->boff[k];
             for (int j = 0; j < 3; ++j) {
@@ -1990,6 +2018,13 @@ extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nl
                 std::vector<double> Tm = transfer_1d(knots[l][3 * k + j], knots[l + 1][3 * k + j], fp->hp[k].p[j], nf, nc);
                 T.nf = nf;
                 T.nc = nc;
+                if (elem_path) {
+                    const int p1 = fp->hp[k].p[j] + 1, nelf = fp->hp[k].nel[j];
+                    toff[3 * k + j] = (int64_t)tblk.size();
+                    for (int e = 0; e < nelf; ++e)
+                        for (int m = 0; m < p1; ++m)
+                            for (int a = 0; a < p1; ++a) tblk.push_back(Tm[(size_t)(e + m) * nc + e / 2 + a]);
+                }
                 int cnt = 0;
                 for (int i = 0; i < nf; ++i) {
                     trp.push_back(cnt);
@@ -2053,22 +2088,43 @@ extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nl
                                         dp<int>(C.rrp), dp<int>(cursor), dp<int>(C.rci), dp<double>(C.rv), 0, s));
         HIP_OK(c, gsx::launch_transpose(2, 0, nullptr, nullptr, nullptr, nullptr, dp<int>(C.rrp), nullptr,
                                         dp<int>(C.rci), dp<double>(C.rv), C.n, s));
-        // Galerkin: B = A P (ELL), A_c = R B into the coarse pattern
-        const int WB = gsx::ap_width(d);
-        DevPtr bcnt = dalloc(c, sizeof(int) * F.n), bci = dalloc(c, sizeof(int) * (size_t)F.n * WB);
-        DevPtr bv = dalloc(c, sizeof(double) * (size_t)F.n * WB);
         C.rowptr = cp->d_rowptr;
         C.col = cp->d_col;
         C.nnz = cp->nnz;
         C.lpr = gsx::spmv_lpr(C.n ? (double)C.nnz / C.n : 1.0);
         C.val = dalloc(c, sizeof(double) * std::max<int64_t>(1, C.nnz));
-        if (!bcnt || !bci || !bv || !C.val) return fail(c, GSX_ERR_OOM, "device allocation (Galerkin product)");
-        HIP_OK(c, gsx::launch_ap(d, F.n, dp<int>(F.rowptr), dp<int>(F.col), dp<double>(F.val), dp<int>(C.prp),
-                                 dp<int>(C.pci), dp<double>(C.pv), dp<int>(bcnt), dp<int>(bci), dp<double>(bv),
-                                 dp<int>(errf), s));
-        HIP_OK(c, gsx::launch_rap(d, C.n, dp<int>(C.rrp), dp<int>(C.rci), dp<double>(C.rv), dp<int>(bcnt),
-                                  dp<int>(bci), dp<double>(bv), dp<int>(C.rowptr), dp<int>(C.col),
-                                  dp<double>(C.val), dp<int>(errf) + 1, s));
+        if (!C.val) return fail(c, GSX_ERR_OOM, "device allocation (Galerkin product)");
+        if (elem_path) {
+            // coarse element matrices from the children's, then the coarse gather
+            KScope kg(c, "mg_galerkin");
+            DevPtr Kc = dalloc(c, sizeof(double) * std::max<int64_t>(1, cp->kdoubles));
+            DevPtr d_tb = dalloc(c, sizeof(double) * std::max<size_t>(1, tblk.size()));
+            DevPtr d_toff = dalloc(c, sizeof(int64_t) * toff.size());
+            if (!Kc || !d_tb || !d_toff) return fail(c, GSX_ERR_OOM, "device allocation (coarse element matrices)");
+            HIP_OK(c, hipMemcpyAsync(d_tb.get(), tblk.data(), sizeof(double) * tblk.size(), hipMemcpyHostToDevice, s));
+            HIP_OK(c, hipMemcpyAsync(d_toff.get(), toff.data(), sizeof(int64_t) * toff.size(), hipMemcpyHostToDevice, s));
+            for (const Group &g : cp->groups)
+                HIP_OK(c, gsx::launch_coarsen(g.d, g.p, dp<PatchDev>(cp->d_patches), dp<PatchDev>(fp->d_patches),
+                                              dp<int>(g.d_gpatch), dp<int64_t>(g.d_gprefix), (int)g.patches.size(),
+                                              g.nelem, dp<double>(d_tb), dp<int64_t>(d_toff), dp<double>(Kf),
+                                              dp<double>(Kc), s));
+            HIP_OK(c, gsx::launch_gather(dp<PatchDev>(cp->d_patches), dp<int>(cp->d_occ_ptr), dp<int>(cp->d_occ_patch),
+                                         dp<int64_t>(cp->d_occ_local), C.n, dp<int>(C.rowptr), dp<int>(C.col),
+                                         dp<double>(Kc), nullptr, dp<double>(C.val), nullptr, nullptr, cp->maxrow, s));
+            Kf = std::move(Kc);
+        } else {
+            // Galerkin: B = A P (ELL), A_c = R B into the coarse pattern
+            const int WB = gsx::ap_width(d);
+            DevPtr bcnt = dalloc(c, sizeof(int) * F.n), bci = dalloc(c, sizeof(int) * (size_t)F.n * WB);
+            DevPtr bv = dalloc(c, sizeof(double) * (size_t)F.n * WB);
+            if (!bcnt || !bci || !bv) return fail(c, GSX_ERR_OOM, "device allocation (Galerkin product)");
+            HIP_OK(c, gsx::launch_ap(d, F.n, dp<int>(F.rowptr), dp<int>(F.col), dp<double>(F.val), dp<int>(C.prp),
+                                     dp<int>(C.pci), dp<double>(C.pv), dp<int>(bcnt), dp<int>(bci), dp<double>(bv),
+                                     dp<int>(errf), s));
+            HIP_OK(c, gsx::launch_rap(d, C.n, dp<int>(C.rrp), dp<int>(C.rci), dp<double>(C.rv), dp<int>(bcnt),
+                                      dp<int>(bci), dp<double>(bv), dp<int>(C.rowptr), dp<int>(C.col),
+                                      dp<double>(C.val), dp<int>(errf) + 1, s));
+        }
         int errh[2] = {0, 0};
         HIP_OK(c, hipMemcpyAsync(errh, errf.get(), sizeof(int) * 2, hipMemcpyDeviceToHost, s));
         HIP_OK(c, hipStreamSynchronize(s));
