@@ -243,10 +243,24 @@ def copy_probe():
     a = torch.ones(n, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
     torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
     for _ in range(3):
         b.copy_(a)
+    e1.record()
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+    copy_ms = e0.elapsed_time(e1) / 3
+    # read-only stream of the same buffer (a reduction)
+    s = a.sum()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(3):
+        s = a.sum()
+    e1.record()
+    torch.cuda.synchronize()
+    read_ms = e0.elapsed_time(e1) / 3
+    print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3,
+                      "copy_GBps": 8 * n / (copy_ms * 1e6), "read_GBps": 4 * n / (read_ms * 1e6)}))
 
 
 def pmc_traffic(name):
@@ -314,7 +328,7 @@ def bench_mg(args):
     results = {}
     for name, prec in (("fp64", gsx.FP64), ("mixed", gsx.MIXED)):
         def step():
-            A, b = gp.assemble(gsx.PATTERN_FULL | gsx.ASM_REBUILD_PATTERN, prec)
+            A, b = gp.assemble(gsx.PATTERN_FULL | gsx.ASM_REBUILD_PATTERN | gsx.ASM_KEEP_ELEMENTS, prec)
             mg = gsx.Multigrid(gp, A, levels, 2, 2, 0.6, precision=prec)
             it, rel, ok = gsx.mg_cg_solve(A, mg, b, x, tol=tol, maxit=500)
             ctx.synchronize()
@@ -340,7 +354,7 @@ def bench_mg(args):
     ctx.timing_filter(None)
     ctx.reset_timing()
     x.zero()
-    A, b = gp.assemble(gsx.PATTERN_FULL | gsx.ASM_REBUILD_PATTERN, gsx.MIXED)
+    A, b = gp.assemble(gsx.PATTERN_FULL | gsx.ASM_REBUILD_PATTERN | gsx.ASM_KEEP_ELEMENTS, gsx.MIXED)
     mg = gsx.Multigrid(gp, A, levels, 2, 2, 0.6, precision=gsx.MIXED)
     gsx.mg_cg_solve(A, mg, b, x, tol=tol, maxit=500)
     ctx.synchronize()

@@ -833,17 +833,17 @@ __global__ __launch_bounds__(256) void k_gather_patch(PatchBox bx, int nfree, in
         if (lane == 0) rhs[r] = racc;
     }
     __syncthreads();
-    // structured operator: slot-major, R consecutive rows per slot
+    // structured operator (tiled slot-major), R consecutive rows per slot
     const int nr = min(R, r1 - rbase);
     if (s64)
         for (int k = threadIdx.x; k < R * NS; k += 256) {
             const int slot = k / R, rr = k % R;
-            if (rr < nr) s64[(int64_t)slot * nfree + rbase + rr] = sAcc[rr][slot];
+            if (rr < nr) s64[stencil_index(slot, rbase + rr, NS)] = sAcc[rr][slot];
         }
     if (s32)
         for (int k = threadIdx.x; k < R * NS; k += 256) {
             const int slot = k / R, rr = k % R;
-            if (rr < nr) s32[(int64_t)slot * nfree + rbase + rr] = (float)sAcc[rr][slot];
+            if (rr < nr) s32[stencil_index(slot, rbase + rr, NS)] = (float)sAcc[rr][slot];
         }
 }
 

@@ -39,6 +39,20 @@ struct GaussTab {
     double w[kMaxQ1 + 1][kMaxQ1];
 };
 
+// Structured operator layout: rows in tiles of kStencilTile; a tile's
+// coefficients are stored slot by slot ([tile][slot][row in tile]), so the
+// block that applies a tile streams one contiguous region.  The last tile is
+// padded to full size (padding rows are never read).
+constexpr int kStencilTile = 256;
+__host__ __device__ inline int64_t stencil_index(int slot, int r, int ns)
+{
+    return ((int64_t)(r / kStencilTile) * ns + slot) * kStencilTile + (r % kStencilTile);
+}
+inline int64_t stencil_size(int n, int ns)
+{
+    return (int64_t)((n + kStencilTile - 1) / kStencilTile) * ns * kStencilTile;
+}
+
 // Mixed-precision CG: a reliable update replaces the FP32 recurrence's
 // residual r_rec by the true FP64 residual r and keeps the search
 // direction, which is consistent as long as r_rec and r agree (they do up

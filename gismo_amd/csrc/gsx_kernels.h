@@ -155,6 +155,9 @@ hipError_t launch_fill_f32(float *a, int64_t n, float v, hipStream_t s);
 hipError_t launch_csr_to_stencil(int d, int p, int nfree, const int *rp, const int *ci,
                                  const double *v, const int *f2g, int nbx, int nby, double *s64,
                                  float *s32, hipStream_t s);
+// centre slot of the structured operator (ns slots) onto the grid
+hipError_t launch_stencil_diag(int nfree, int ns, const int *f2g, const double *s64, double *diag_grid,
+                              hipStream_t s);
 hipError_t launch_scatter_f64_to_grid(int nfree, const int *f2g, const double *src, double *dst,
                                       hipStream_t s);
 hipError_t launch_gather_f64_from_grid(int nfree, const int *f2g, const double *src, double *dst,

@@ -142,22 +142,48 @@ __global__ void k_tr_fill(int n, const int *__restrict__ rp, const int *__restri
     }
 }
 
-__global__ void k_sort_rows(int n, const int *__restrict__ rp, int *__restrict__ ci, double *__restrict__ v)
+// Sort the entries of every row by column (the transpose is filled through
+// atomic cursors, so its in-row order is arbitrary).  One wavefront per row:
+// the row goes to LDS and every entry's position is the count of smaller
+// columns (columns are unique within a row).  Rows longer than the LDS
+// slice are sorted by insertion in place.
+constexpr int kSortCap = 512;
+__global__ __launch_bounds__(256) void k_sort_rows(int n, const int *__restrict__ rp, int *__restrict__ ci,
+                                                   double *__restrict__ v)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int sc[4][kSortCap];
+    __shared__ double sv[4][kSortCap];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int r = blockIdx.x * 4 + w;
     if (r >= n) return;
-    const int a = rp[r], b = rp[r + 1];
-    for (int i = a + 1; i < b; ++i) {
-        const int c = ci[i];
-        const double x = v[i];
-        int j = i - 1;
-        while (j >= a && ci[j] > c) {
-            ci[j + 1] = ci[j];
-            v[j + 1] = v[j];
-            --j;
+    const int a = rp[r], len = rp[r + 1] - a;
+    if (len > kSortCap) {
+        if (lane != 0) return;
+        for (int i = a + 1; i < a + len; ++i) {
+            const int c = ci[i];
+            const double x = v[i];
+            int j = i - 1;
+            while (j >= a && ci[j] > c) {
+                ci[j + 1] = ci[j];
+                v[j + 1] = v[j];
+                --j;
+            }
+            ci[j + 1] = c;
+            v[j + 1] = x;
         }
-        ci[j + 1] = c;
-        v[j + 1] = x;
+        return;
+    }
+    for (int k = lane; k < len; k += kWave) {
+        sc[w][k] = ci[a + k];
+        sv[w][k] = v[a + k];
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (int k = lane; k < len; k += kWave) {
+        const int key = sc[w][k];
+        int rank = 0;
+        for (int j = 0; j < len; ++j) rank += sc[w][j] < key;
+        ci[a + rank] = key;
+        v[a + rank] = sv[w][k];
     }
 }
 
@@ -344,73 +370,86 @@ __global__ __launch_bounds__(256) void k_coarsen(const PatchDev *__restrict__ cp
 }
 
 // ---------------------------------------------------------------------------
-// Dense coarse inverse by blocked Gauss-Jordan (SPD, no pivoting), block 64.
+// Dense coarse inverse by blocked Gauss-Jordan (SPD, no pivoting), block 64,
+// in its symmetric (sweep) form.
 // ---------------------------------------------------------------------------
 constexpr int kGJ = 64;
 
-__global__ void k_densify(int n, int npad, const int *__restrict__ rp, const int *__restrict__ ci,
-                          const double *__restrict__ v, double *__restrict__ D)
+// dense copy of the coarsest operator, padded with the identity: fill, then
+// one thread per row scatters its CSR entries
+__global__ void k_dense_fill(int n, int npad, double *__restrict__ D)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)npad * npad) return;
     const int r = (int)(i / npad), c = (int)(i % npad);
-    double x = 0.0;
-    if (r >= n || c >= n) {
-        x = (r == c) ? 1.0 : 0.0;
-    } else {
-        int lo = rp[r], hi = rp[r + 1] - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (ci[mid] < c) lo = mid + 1; else hi = mid;
-        }
-        if (lo < rp[r + 1] && ci[lo] == c) x = v[lo];
-    }
-    D[i] = x;
+    D[i] = (r == c && r >= n) ? 1.0 : 0.0;
+}
+
+__global__ void k_dense_scatter(int n, int npad, const int *__restrict__ rp, const int *__restrict__ ci,
+                                const double *__restrict__ v, double *__restrict__ D)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    for (int k = rp[r]; k < rp[r + 1]; ++k) D[(int64_t)r * npad + ci[k]] = v[k];
 }
 
 // invert the pivot block (kb) in LDS; result to Pinv (64x64, row-major).
 // In-place Gauss-Jordan: step p maps every entry from the old values (kept
 // in registers between the two barriers), with one reciprocal per step.
-__global__ __launch_bounds__(256) void k_gj_pivot(double *__restrict__ D, int npad, int kb, double *__restrict__ Pinv)
+// The 64 steps are sequential, so the block is wide (1024 threads, 4
+// entries each) to keep each step short.
+constexpr int kPivThreads = 1024;
+__global__ __launch_bounds__(kPivThreads) void k_gj_pivot(double *__restrict__ D, int npad, int kb,
+                                                          double *__restrict__ Pinv)
 {
     __shared__ double M[kGJ][kGJ + 1];
-    constexpr int PER = kGJ * kGJ / 256;
+    constexpr int PER = kGJ * kGJ / kPivThreads;
     const int t = threadIdx.x;
-    for (int i = t; i < kGJ * kGJ; i += 256) M[i / kGJ][i % kGJ] = D[(int64_t)(kb + i / kGJ) * npad + kb + i % kGJ];
+    for (int i = t; i < kGJ * kGJ; i += kPivThreads) M[i / kGJ][i % kGJ] = D[(int64_t)(kb + i / kGJ) * npad + kb + i % kGJ];
     __syncthreads();
     for (int p = 0; p < kGJ; ++p) {
         const double inv = 1.0 / M[p][p];
         double nv[PER];
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
-            const int i = t + 256 * j, r = i / kGJ, c = i % kGJ;
-            const double a = M[r][c];
-            if (r != p && c != p) nv[j] = a - M[r][p] * inv * M[p][c];
-            else if (r == p && c == p) nv[j] = inv;
-            else if (c == p) nv[j] = -a * inv;
-            else nv[j] = a * inv;
+            const int i = t + kPivThreads * j, r = i / kGJ, c = i % kGJ;
+            const double a = M[r][p] * inv, b = M[p][c];
+            const double x = M[r][c];
+            nv[j] = (r != p) ? ((c != p) ? x - a * b : -a) : ((c != p) ? b * inv : inv);
         }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
-            const int i = t + 256 * j;
+            const int i = t + kPivThreads * j;
             M[i / kGJ][i % kGJ] = nv[j];
         }
         __syncthreads();
     }
-    for (int i = t; i < kGJ * kGJ; i += 256) Pinv[i] = M[i / kGJ][i % kGJ];
+    for (int i = t; i < kGJ * kGJ; i += kPivThreads) Pinv[i] = M[i / kGJ][i % kGJ];
 }
 
-// W[r][j] = sum_m Pinv[r][m] D[kb+m][j] (all j); one block per 64 columns
+// Symmetric sweep (Gauss-Jordan in the form that keeps every intermediate
+// matrix symmetric).  Sweeping pivot block k of M:
+//   M'[k][k] = -B^-1,  M'[i][k] = M[i][k] B^-1,  M'[k][j] = B^-1 M[k][j],
+//   M'[i][j] = M[i][j] - M[i][k] B^-1 M[k][j]            (B = M[k][k]);
+// after all blocks M = -A^-1.  Only the tiles I <= J are kept: a tile below
+// the diagonal is read as the transpose of its mirror.
+
+// W[J] = B^-1 M[k][J] for every column tile J != k; one block per tile
 __global__ __launch_bounds__(256) void k_gj_rowpanel(const double *__restrict__ D, int npad, int kb,
                                                      const double *__restrict__ Pinv, double *__restrict__ W)
 {
     __shared__ double sP[kGJ][kGJ + 1];
     __shared__ double sR[kGJ][kGJ + 1];
     const int t = threadIdx.x, j0 = blockIdx.x * kGJ;
+    if (j0 == kb) return;
     for (int i = t; i < kGJ * kGJ; i += 256) {
-        sP[i / kGJ][i % kGJ] = Pinv[i];
-        sR[i / kGJ][i % kGJ] = D[(int64_t)(kb + i / kGJ) * npad + j0 + i % kGJ];
+        const int r = i / kGJ, c = i % kGJ;
+        sP[r][c] = Pinv[i];
+        // M[k][J][r][c]: upper tile D[k][J], or the transpose of D[J][k]
+        // (read along D's rows, transposed into LDS)
+        if (j0 > kb) sR[r][c] = D[(int64_t)(kb + r) * npad + j0 + c];
+        else sR[c][r] = D[(int64_t)(j0 + r) * npad + kb + c];
     }
     __syncthreads();
     for (int i = t; i < kGJ * kGJ; i += 256) {
@@ -421,66 +460,97 @@ __global__ __launch_bounds__(256) void k_gj_rowpanel(const double *__restrict__ 
     }
 }
 
-// D[I][J] -= C[I] W[J] for tiles I, J != kblk (C = column panel of D)
+// D[I][J] -= M[I][k] W[J] for the upper tiles I <= J, I, J != k.  FP64
+// matrix cores: each wavefront forms a 32x32 quadrant of the 64x64 tile as
+// 2x2 blocks of v_mfma_f64_16x16x4f64 over the 64-deep panel.  Operand
+// layout of that instruction: A[i][k] in lane i + 16 k, B[k][j] in lane
+// j + 16 k, D[i][j] in lane j + 16 (i % 4), element i / 4 (the rows of the
+// four result registers interleave).
+typedef double gj_d4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(256) void k_gj_update(double *__restrict__ D, int npad, int kb,
                                                    const double *__restrict__ W)
 {
     const int I = blockIdx.y * kGJ, J = blockIdx.x * kGJ;
-    if (I == kb || J == kb) return;
+    if (I == kb || J == kb || I > J) return;
     __shared__ double sC[kGJ][kGJ + 1];
     __shared__ double sW[kGJ][kGJ + 1];
     const int t = threadIdx.x;
     for (int i = t; i < kGJ * kGJ; i += 256) {
-        sC[i / kGJ][i % kGJ] = D[(int64_t)(I + i / kGJ) * npad + kb + i % kGJ];
-        sW[i / kGJ][i % kGJ] = W[(int64_t)(i / kGJ) * npad + J + i % kGJ];
+        const int r = i / kGJ, c = i % kGJ;
+        // M[I][k]: upper tile D[I][k], or the transpose of D[k][I] (read
+        // along D's rows, transposed into LDS)
+        if (I < kb) sC[r][c] = D[(int64_t)(I + r) * npad + kb + c];
+        else sC[c][r] = D[(int64_t)(kb + r) * npad + I + c];
+        sW[r][c] = W[(int64_t)r * npad + J + c];
     }
     __syncthreads();
-    // each thread 4x4 outputs
-    const int tr = (t / 16) * 4, tc = (t % 16) * 4;
-    double acc[4][4] = {{0}};
-    for (int m = 0; m < kGJ; ++m) {
-        double a[4], b[4];
+    const int w = t / kWave, lane = t % kWave;
+    const int r0 = (w / 2) * 32, c0 = (w % 2) * 32;
+    const int li = lane % 16, lk = lane / 16;
+    gj_d4 acc[2][2];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            a[u] = sC[tr + u][m];
-            b[u] = sW[m][tc + u];
-        }
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[u][v] += a[u] * b[v];
+        for (int v = 0; v < 2; ++v) acc[u][v] = gj_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int ks = 0; ks < kGJ / 4; ++ks) {
+        const int m = 4 * ks + lk;
+        const double a0 = sC[r0 + li][m], a1 = sC[r0 + 16 + li][m];
+        const double b0 = sW[m][c0 + li], b1 = sW[m][c0 + 16 + li];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) D[(int64_t)(I + tr + u) * npad + J + tc + v] -= acc[u][v];
+        for (int v = 0; v < 2; ++v)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = I + r0 + 16 * u + lk + 4 * q, col = J + c0 + 16 * v + li;
+                D[(int64_t)row * npad + col] -= acc[u][v][q];
+            }
 }
 
-// column panel D[I][kb] = -C[I] Pinv, row panel D[kb][J] = W[J], pivot = Pinv
+// panels of the swept block: M'[I][k] = W[I]^T (I < k), M'[k][J] = W[J]
+// (J > k), M'[k][k] = -B^-1
 __global__ __launch_bounds__(256) void k_gj_panels(double *__restrict__ D, int npad, int kb,
                                                    const double *__restrict__ W, const double *__restrict__ Pinv)
 {
     const int I = blockIdx.x * kGJ;
-    __shared__ double sC[kGJ][kGJ + 1];
-    __shared__ double sP[kGJ][kGJ + 1];
     const int t = threadIdx.x;
-    if (I == kb) {
-        for (int i = t; i < kGJ * kGJ; i += 256)
-            D[(int64_t)(kb + i / kGJ) * npad + kb + i % kGJ] = Pinv[i];
+    if (I < kb) {
+        // D[I][k] = W[I]^T through LDS, rows read and written contiguously
+        __shared__ double sT[kGJ][kGJ + 1];
+        for (int i = t; i < kGJ * kGJ; i += 256) sT[i / kGJ][i % kGJ] = W[(int64_t)(i / kGJ) * npad + I + i % kGJ];
+        __syncthreads();
+        for (int i = t; i < kGJ * kGJ; i += 256) {
+            const int r = i / kGJ, c = i % kGJ;
+            D[(int64_t)(I + r) * npad + kb + c] = sT[c][r];
+        }
         return;
     }
     for (int i = t; i < kGJ * kGJ; i += 256) {
-        sC[i / kGJ][i % kGJ] = D[(int64_t)(I + i / kGJ) * npad + kb + i % kGJ];
-        sP[i / kGJ][i % kGJ] = Pinv[i];
+        const int r = i / kGJ, c = i % kGJ;
+        if (I == kb) D[(int64_t)(kb + r) * npad + kb + c] = -Pinv[i];
+        else D[(int64_t)(kb + r) * npad + I + c] = W[(int64_t)r * npad + I + c];
     }
+}
+
+// A^-1 = -M: negate the upper tiles and mirror them into the lower ones
+__global__ __launch_bounds__(256) void k_gj_finish(double *__restrict__ D, int npad)
+{
+    const int I = blockIdx.y * kGJ, J = blockIdx.x * kGJ;
+    if (I > J) return;
+    __shared__ double sT[kGJ][kGJ + 1];
+    const int t = threadIdx.x;
+    for (int i = t; i < kGJ * kGJ; i += 256) sT[i / kGJ][i % kGJ] = -D[(int64_t)(I + i / kGJ) * npad + J + i % kGJ];
     __syncthreads();
     for (int i = t; i < kGJ * kGJ; i += 256) {
         const int r = i / kGJ, c = i % kGJ;
-        double s = 0.0;
-        for (int m = 0; m < kGJ; ++m) s += sC[r][m] * sP[m][c];
-        D[(int64_t)(I + r) * npad + kb + c] = -s;
-        // row panel block (kb rows, columns I..I+63) from W
-        D[(int64_t)(kb + r) * npad + I + c] = W[(int64_t)r * npad + I + c];
+        D[(int64_t)(I + r) * npad + J + c] = sT[r][c];
+        if (I < J) D[(int64_t)(J + r) * npad + I + c] = sT[c][r];
     }
 }
 
@@ -501,16 +571,16 @@ __global__ __launch_bounds__(256) void k_dense_gemv(int n, int npad, const T *__
 }
 
 // y += alpha * (P x) (prolongation), one thread per fine row (rows short);
-// accumulated in the fine vector's type T
+// summed in FP64, stored in the fine vector's type T
 template <typename V, typename TX, typename T>
 __global__ void k_prolong_add(int n, const int *__restrict__ rp, const int *__restrict__ ci,
                               const V *__restrict__ v, const TX *__restrict__ x, T *__restrict__ y, T alpha)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    T s = 0;
-    for (int k = rp[r]; k < rp[r + 1]; ++k) s += (T)v[k] * (T)x[ci[k]];
-    y[r] += alpha * s;
+    double s = 0.0;
+    for (int k = rp[r]; k < rp[r + 1]; ++k) s += (double)v[k] * (double)x[ci[k]];
+    y[r] = (T)((double)y[r] + (double)alpha * s);
 }
 
 // r = b - A x (CSR), one wavefront-group per row
@@ -522,24 +592,24 @@ __global__ __launch_bounds__(256) void k_residual(int n, const int *__restrict__
     constexpr int RPB = 256 / LPR;
     const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
     const int row = blockIdx.x * RPB + g;
-    T acc = 0;
+    double acc = 0.0;   // FP64 sums whatever the storage type (see k_jacobi_step)
     if (row < n)
-        for (int k = rp[row] + lane; k < rp[row + 1]; k += LPR) acc += (T)v[k] * x[ci[k]];
+        for (int k = rp[row] + lane; k < rp[row + 1]; k += LPR) acc += (double)v[k] * (double)x[ci[k]];
 #pragma unroll
     for (int off = LPR / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, LPR);
-    if (row < n && lane == 0) r[row] = b[row] - acc;
+    if (row < n && lane == 0) r[row] = (T)((double)b[row] - acc);
 }
 
-// y = R x (restriction), one thread per coarse row; accumulated in the fine
-// vector's type T, stored in the coarse vector's type TY
+// y = R x (restriction), one thread per coarse row; summed in FP64, stored
+// in the coarse vector's type TY
 template <typename V, typename T, typename TY>
 __global__ void k_restrict(int n, const int *__restrict__ rp, const int *__restrict__ ci,
                            const V *__restrict__ v, const T *__restrict__ x, TY *__restrict__ y)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    T s = 0;
-    for (int k = rp[r]; k < rp[r + 1]; ++k) s += (T)v[k] * x[ci[k]];
+    double s = 0.0;
+    for (int k = rp[r]; k < rp[r + 1]; ++k) s += (double)v[k] * (double)x[ci[k]];
     y[r] = (TY)s;
 }
 
@@ -581,7 +651,7 @@ hipError_t launch_transpose(int phase, int n, const int *rp, const int *ci, cons
     } else if (phase == 1) {
         if (n > 0) hipLaunchKernelGGL(k_tr_fill, g1(n), dim3(256), 0, s, n, rp, ci, v, trp, cursor, tci, tv);
     } else {
-        if (nt > 0) hipLaunchKernelGGL(k_sort_rows, g1(nt, 128), dim3(128), 0, s, nt, trp, tci, tv);
+        if (nt > 0) hipLaunchKernelGGL(k_sort_rows, g1(nt, 4), dim3(256), 0, s, nt, trp, tci, tv);
     }
     return hipGetLastError();
 }
@@ -629,15 +699,17 @@ hipError_t launch_coarsen(int d, int p, const PatchDev *cpatches, const PatchDev
 hipError_t launch_dense_inverse(int n, int npad, const int *rp, const int *ci, const double *v, double *D,
                                 double *W, double *Pinv, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_densify, g1((int64_t)npad * npad), dim3(256), 0, s, n, npad, rp, ci, v, D);
+    hipLaunchKernelGGL(k_dense_fill, g1((int64_t)npad * npad), dim3(256), 0, s, n, npad, D);
+    if (n > 0) hipLaunchKernelGGL(k_dense_scatter, g1(n), dim3(256), 0, s, n, npad, rp, ci, v, D);
     const int nb = npad / kGJ;
     for (int k = 0; k < nb; ++k) {
         const int kb = k * kGJ;
-        hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(256), 0, s, D, npad, kb, Pinv);
+        hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(kPivThreads), 0, s, D, npad, kb, Pinv);
         hipLaunchKernelGGL(k_gj_rowpanel, dim3(nb), dim3(256), 0, s, D, npad, kb, Pinv, W);
         hipLaunchKernelGGL(k_gj_update, dim3(nb, nb), dim3(256), 0, s, D, npad, kb, W);
         hipLaunchKernelGGL(k_gj_panels, dim3(nb), dim3(256), 0, s, D, npad, kb, W, Pinv);
     }
+    hipLaunchKernelGGL(k_gj_finish, dim3(nb, nb), dim3(256), 0, s, D, npad);
     return hipGetLastError();
 }
 

@@ -15,6 +15,7 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <deque>
 #include <vector>
 
 #include "gsx_common.h"
@@ -276,6 +277,10 @@ struct gsx_problem {
     DevPtr d_detmin, d_scan_bsum, d_scan_total;
     int64_t kdoubles = 0, fdoubles = 0;
     double min_det = 0.0;
+    // element matrices of the last assembly (GSX_ASM_KEEP_ELEMENTS), laid
+    // out at kmat_off, and the serial of that assembly
+    DevPtr kept_K;
+    uint64_t asm_serial = 0, kept_serial = 0;
 };
 
 // DOF numbering with the rules of gsDofMapper::finalizeComp
@@ -889,6 +894,7 @@ struct gsx_matrix {
     bool use_stencil = true;
     bool mixed = false;       // assembled with GSX_MIXED (FP32 operator available)
     const gsx_problem *src = nullptr;   // problem it was assembled from (values untouched since)
+    uint64_t src_serial = 0;            // that assembly's serial in the problem
     std::vector<double> hist;
 };
 
@@ -956,15 +962,18 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
 {
     if (!P || !A_out) return GSX_ERR_ARG;
     gsx_ctx *c = P->ctx;
-    if ((mode & ~(GSX_ASM_REBUILD_PATTERN | GSX_ASM_CSR_ONLY)) != GSX_PATTERN_EXPR &&
-        (mode & ~(GSX_ASM_REBUILD_PATTERN | GSX_ASM_CSR_ONLY)) != GSX_PATTERN_FULL)
+    const int32_t flags = GSX_ASM_REBUILD_PATTERN | GSX_ASM_CSR_ONLY | GSX_ASM_KEEP_ELEMENTS;
+    if ((mode & ~flags) != GSX_PATTERN_EXPR && (mode & ~flags) != GSX_PATTERN_FULL)
         return fail(c, GSX_ERR_ARG, "unknown pattern mode");
     if (precision != GSX_FP64 && precision != GSX_MIXED)
         return fail(c, GSX_ERR_ARG, "unknown precision");
     hipStream_t s = c->stream;
     const bool rebuild = (mode & GSX_ASM_REBUILD_PATTERN) != 0;
     const bool csr_only = (mode & GSX_ASM_CSR_ONLY) != 0;
-    mode &= ~(GSX_ASM_REBUILD_PATTERN | GSX_ASM_CSR_ONLY);
+    const bool keep_elements = (mode & GSX_ASM_KEEP_ELEMENTS) != 0;
+    mode &= ~(GSX_ASM_REBUILD_PATTERN | GSX_ASM_CSR_ONLY | GSX_ASM_KEEP_ELEMENTS);
+    P->kept_K.reset();
+    const uint64_t serial = ++P->asm_serial;
     if (!P->have_pattern || rebuild) {
         gsx_status st = pattern_build(P);
         if (st != GSX_OK) return st;
@@ -1047,8 +1056,8 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
         S.slots = h.d == 3 ? W * W * W : W * W;
         S.f2g = P->d_f2g;
         if (!csr_only) {
-            S.s64 = dalloc(c, sizeof(double) * (int64_t)S.slots * A->n);
-            if (precision == GSX_MIXED) S.s32 = dalloc(c, sizeof(float) * (int64_t)S.slots * A->n);
+            S.s64 = dalloc(c, sizeof(double) * gsx::stencil_size(A->n, (int)S.slots));
+            if (precision == GSX_MIXED) S.s32 = dalloc(c, sizeof(float) * gsx::stencil_size(A->n, (int)S.slots));
             if (!S.s64 || (precision == GSX_MIXED && !S.s32))
                 return fail(c, GSX_ERR_OOM, "device allocation (structured operator)");
         }
@@ -1086,7 +1095,12 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
                                      dp<double>(Fbuf), dp<double>(A->val), rhs,
                                      keep ? dp<unsigned char>(keep) : nullptr, P->maxrow, s));
     }
-    // element matrices go back to the cache once the gathers have run
+    // element matrices go back to the cache once the gathers have run, or
+    // stay with the problem when asked for and formed in one pass
+    if (keep_elements && kcap == P->kdoubles) {
+        P->kept_K = std::move(Kbuf);
+        P->kept_serial = serial;
+    }
     Kbuf.reset();
     Fbuf.reset();
     double *dmin_slot = dp<double>(P->d_detmin) + std::max<int64_t>(1, P->nelem);
@@ -1134,6 +1148,7 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
     A->lpr = gsx::spmv_lpr(A->n ? (double)A->nnz / A->n : 1.0);
     A->mixed = precision == GSX_MIXED;
     A->src = P;
+    A->src_serial = serial;
     if (precision == GSX_MIXED && !A->st.available) {
         A->val32 = dalloc(c, sizeof(float) * std::max<int64_t>(1, A->nnz));
         if (!A->val32) return fail(c, GSX_ERR_OOM, "device allocation (fp32 values)");
@@ -1493,18 +1508,20 @@ static gsx_status stencil_prepare(gsx_matrix *A, bool mixed)
     const int64_t K = S.slots, N = S.ngrid;
     KScope k(c, "stencil_build");
     if (!S.s64_ready) {
-        S.s64 = dalloc(c, sizeof(double) * K * n);
+        const int64_t sz = gsx::stencil_size(n, (int)K);
+        S.s64 = dalloc(c, sizeof(double) * sz);
         if (!S.s64) return fail(c, GSX_ERR_OOM, "device allocation (structured operator)");
-        HIP_OK(c, gsx::launch_fill_f64(dp<double>(S.s64), K * n, 0.0, s));
+        HIP_OK(c, gsx::launch_fill_f64(dp<double>(S.s64), sz, 0.0, s));
         HIP_OK(c, gsx::launch_csr_to_stencil(S.d, S.p, n, dp<int>(A->rowptr), dp<int>(A->col),
                                              dp<double>(A->val), dp<int>(S.f2g), S.nx, S.ny,
                                              dp<double>(S.s64), nullptr, s));
         S.s64_ready = true;
     }
     if (mixed && !S.s32_ready) {
-        S.s32 = dalloc(c, sizeof(float) * K * n);
+        const int64_t sz = gsx::stencil_size(n, (int)K);
+        S.s32 = dalloc(c, sizeof(float) * sz);
         if (!S.s32) return fail(c, GSX_ERR_OOM, "device allocation (structured operator, fp32)");
-        HIP_OK(c, gsx::launch_fill_f32(dp<float>(S.s32), K * n, 0.0f, s));
+        HIP_OK(c, gsx::launch_fill_f32(dp<float>(S.s32), sz, 0.0f, s));
         HIP_OK(c, gsx::launch_csr_to_stencil(S.d, S.p, n, dp<int>(A->rowptr), dp<int>(A->col),
                                              dp<double>(A->val), dp<int>(S.f2g), S.nx, S.ny, nullptr,
                                              dp<float>(S.s32), s));
@@ -1522,10 +1539,8 @@ static gsx_status stencil_prepare(gsx_matrix *A, bool mixed)
         if (!S.diag64 || !S.xg || !S.rg || !S.bg || !S.zg || !S.pg || !S.tg || !S.parts)
             return fail(c, GSX_ERR_OOM, "device allocation (grid workspace)");
         // diagonal = centre slot of the structured operator
-        const int64_t centre = (K - 1) / 2;
         HIP_OK(c, gsx::launch_fill_f64(dp<double>(S.diag64), N, 1.0, s));
-        HIP_OK(c, gsx::launch_scatter_f64_to_grid(n, dp<int>(S.f2g), dp<double>(S.s64) + centre * n,
-                                                  dp<double>(S.diag64), s));
+        HIP_OK(c, gsx::launch_stencil_diag(n, (int)K, dp<int>(S.f2g), dp<double>(S.s64), dp<double>(S.diag64), s));
         for (DevPtr *v : {&S.xg, &S.rg, &S.bg, &S.zg, &S.pg, &S.tg})
             HIP_OK(c, gsx::launch_fill_f64(dp<double>(*v), N, 0.0, s));
         S.built = true;
@@ -1811,9 +1826,11 @@ static std::vector<double> transfer_1d(const std::vector<double> &coarse, const 
     nc = (int)coarse.size() - p - 1;
     nf = (int)fine.size() - p - 1;
     std::vector<double> U = coarse;
-    std::vector<double> C((size_t)nc * nc, 0.0), Cn;
-    for (int i = 0; i < nc; ++i) C[(size_t)i * nc + i] = 1.0;
-    int ncur = nc;
+    // rows of the transfer (fine function -> coarse coefficients), one
+    // dense row per function; an insertion replaces the p - 1 rows
+    // k-p+1..k-1 by p new rows and shifts the others
+    std::vector<std::vector<double>> R((size_t)nc, std::vector<double>((size_t)nc, 0.0));
+    for (int i = 0; i < nc; ++i) R[(size_t)i][(size_t)i] = 1.0;
     // knots of `fine` missing from `coarse` (multiset difference, in order)
     std::vector<double> ins;
     {
@@ -1823,25 +1840,27 @@ static std::vector<double> transfer_1d(const std::vector<double> &coarse, const 
             else ins.push_back(fine[i]);
         }
     }
+    std::vector<std::vector<double>> mid;
     for (double u : ins) {
         int k = p;
         while (k + 1 < (int)U.size() && U[k + 1] <= u) ++k;
-        Cn.assign((size_t)(ncur + 1) * nc, 0.0);
-        for (int i = 0; i <= ncur; ++i) {
-            double *dst = &Cn[(size_t)i * nc];
-            if (i <= k - p) {
-                std::memcpy(dst, &C[(size_t)i * nc], sizeof(double) * nc);
-            } else if (i >= k + 1) {
-                std::memcpy(dst, &C[(size_t)(i - 1) * nc], sizeof(double) * nc);
-            } else {
-                const double a = (u - U[i]) / (U[i + p] - U[i]);
-                for (int c = 0; c < nc; ++c) dst[c] = (1.0 - a) * C[(size_t)(i - 1) * nc + c] + a * C[(size_t)i * nc + c];
-            }
+        mid.assign((size_t)p, std::vector<double>((size_t)nc));
+        for (int i = k - p + 1; i <= k; ++i) {
+            const double a = (u - U[i]) / (U[i + p] - U[i]);
+            const std::vector<double> &r0 = R[(size_t)(i - 1)], &r1 = R[(size_t)i];
+            std::vector<double> &dst = mid[(size_t)(i - (k - p + 1))];
+            for (int c = 0; c < nc; ++c) dst[(size_t)c] = (1.0 - a) * r0[(size_t)c] + a * r1[(size_t)c];
         }
-        C.swap(Cn);
-        ++ncur;
+        std::vector<std::vector<double>> Rn;
+        Rn.reserve(R.size() + 1);
+        for (int i = 0; i <= k - p; ++i) Rn.push_back(std::move(R[(size_t)i]));
+        for (auto &m : mid) Rn.push_back(std::move(m));
+        for (size_t i = (size_t)k; i < R.size(); ++i) Rn.push_back(std::move(R[i]));
+        R.swap(Rn);
         U.insert(U.begin() + k + 1, u);
     }
+    std::vector<double> C((size_t)nf * nc);
+    for (int i = 0; i < nf; ++i) std::memcpy(&C[(size_t)i * nc], R[(size_t)i].data(), sizeof(double) * nc);
     return C;
 }
 
@@ -1971,7 +1990,9 @@ extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nl
         elem_path = (fine->kdoubles + fine->fdoubles + kc) * (int64_t)sizeof(double) <= c->elem_budget;
     }
     DevPtr Kf;
-    if (elem_path) {
+    if (elem_path && fine->kept_K && fine->kept_serial == A->src_serial) {
+        Kf = std::move(fine->kept_K);   // this matrix's own element matrices
+    } else if (elem_path) {
         KScope ke(c, "mg_elem");
         Kf = dalloc(c, sizeof(double) * std::max<int64_t>(1, fine->kdoubles));
         DevPtr Ff = dalloc(c, sizeof(double) * std::max<int64_t>(1, fine->fdoubles));
@@ -1981,6 +2002,27 @@ extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nl
                                        dp<int64_t>(g.d_gprefix), (int)g.patches.size(), g.nelem, c->gt,
                                        dp<double>(Kf), dp<double>(Ff), dp<double>(fine->d_detmin), s));
     }
+    // 1D transfers, computed once per distinct (coarse knots, fine knots, p)
+    struct TCache {
+        struct Item { std::vector<double> a, b; int p, nf, nc; std::vector<double> T; };
+        std::deque<Item> items;   // references stay valid as items are added
+        const std::vector<double> &get(const std::vector<double> &a, const std::vector<double> &b, int p, int &nf,
+                                       int &nc)
+        {
+            for (const Item &it : items)
+                if (it.p == p && it.a == a && it.b == b) {
+                    nf = it.nf;
+                    nc = it.nc;
+                    return it.T;
+                }
+            Item it{a, b, p, 0, 0, {}};
+            it.T = transfer_1d(a, b, p, it.nf, it.nc);
+            nf = it.nf;
+            nc = it.nc;
+            items.push_back(std::move(it));
+            return items.back().T;
+        }
+    } tcache;
     // transfers and Galerkin products, finest first
     for (int l = nlevels - 2; l >= 0; --l) {
         MGLevel &C = mg->L[l], &F = mg->L[l + 1];
@@ -2015,7 +2057,8 @@ extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nl
                     continue;
                 }
                 int nf = 0, nc = 0;
-                std::vector<double> Tm = transfer_1d(knots[l][3 * k + j], knots[l + 1][3 * k + j], fp->hp[k].p[j], nf, nc);
+                const std::vector<double> &Tm = tcache.get(knots[l][3 * k + j], knots[l + 1][3 * k + j],
+                                                           fp->hp[k].p[j], nf, nc);
                 T.nf = nf;
                 T.nc = nc;
                 if (elem_path) {

@@ -210,18 +210,21 @@ __global__ __launch_bounds__(256) void k_jacobi_step(int n, const int *__restric
                                                      const T *__restrict__ b, const T *__restrict__ xin,
                                                      T *__restrict__ xout, T tau, int x_is_zero)
 {
+    // products and sums in FP64 whatever the storage type: an FP32 row sum
+    // of a stiffness row cancels to a relative error of eps * cond(A), which
+    // costs a multigrid preconditioner outer iterations
     constexpr int RPB = 256 / LPR;
     const int g = threadIdx.x / LPR, lane = threadIdx.x % LPR;
     const int r = blockIdx.x * RPB + g;
-    T acc = 0;
+    double acc = 0.0;
     if (r < n && !x_is_zero) {
         const int k0 = rowptr[r], k1 = rowptr[r + 1];
-        for (int k = k0 + lane; k < k1; k += LPR) acc += (T)val[k] * xin[col[k]];
+        for (int k = k0 + lane; k < k1; k += LPR) acc += (double)val[k] * (double)xin[col[k]];
     }
-    acc = group_sum<LPR, T>(acc);
+    acc = group_sum<LPR, double>(acc);
     if (r < n && lane == 0) {
-        const T xo = x_is_zero ? (T)0 : xin[r];
-        xout[r] = xo + tau * (b[r] - acc) / diag[r];
+        const double xo = x_is_zero ? 0.0 : (double)xin[r];
+        xout[r] = (T)(xo + (double)tau * ((double)b[r] - acc) / (double)diag[r]);
     }
 }
 

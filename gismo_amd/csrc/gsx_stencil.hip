@@ -60,6 +60,7 @@ __global__ void k_csr_to_stencil(int nfree, const int *__restrict__ rowptr,
                                  double *__restrict__ s64, float *__restrict__ s32)
 {
     constexpr int W = 2 * P + 1;
+    constexpr int NS = (D == 3) ? W * W * W : W * W;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nfree) return;
     const int g = free2grid[r];
@@ -69,9 +70,17 @@ __global__ void k_csr_to_stencil(int nfree, const int *__restrict__ rowptr,
         const int dx = h % nbx - gx, dy = (h / nbx) % nby - gy;
         const int dz = (D == 3) ? h / (nbx * nby) - gz : 0;
         const int slot = (D == 3) ? ((dz + P) * W + (dy + P)) * W + (dx + P) : (dy + P) * W + (dx + P);
-        if (s64) s64[(int64_t)slot * nfree + r] = val[k];
-        if (s32) s32[(int64_t)slot * nfree + r] = (float)val[k];
+        if (s64) s64[stencil_index(slot, r, NS)] = val[k];
+        if (s32) s32[stencil_index(slot, r, NS)] = (float)val[k];
     }
+}
+
+// diagonal (centre slot) of the structured operator onto the grid
+__global__ void k_stencil_diag(int nfree, int ns, int centre, const int *__restrict__ free2grid,
+                               const double *__restrict__ s64, double *__restrict__ dst)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < nfree) dst[free2grid[r]] = s64[stencil_index(centre, r, ns)];
 }
 
 template <typename S, typename Dt>
@@ -90,6 +99,19 @@ __global__ void k_gather_from_grid(int nfree, const int *__restrict__ free2grid,
     if (r < nfree) dst[r] = (double)src[free2grid[r]];
 }
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b to XCD b % 8),
+// each XCD with its own L2.  Logical block of physical block b: XCD x takes
+// the contiguous range [start(x), start(x) + count(x)) of logical blocks, so
+// neighbouring rows, which read the same x planes, share an L2.  A bijection
+// of [0, nb).
+__device__ inline int xcd_block(int b, int nb)
+{
+    const int x = b & 7, j = b >> 3;
+    int start = 0;
+    for (int y = 0; y < x; ++y) start += (nb - y + 7) >> 3;
+    return start + j;
+}
+
 // ---------------------------------------------------------------------------
 // Stencil SpMV.  MODE 0: y = A x; 1: y = A x, partial (x, y); 2: y = b - A x,
 // partial (y, y).  Vectors on the grid; rows = free functions.
@@ -105,11 +127,14 @@ __global__ __launch_bounds__(256) void k_sspmv(int nfree, const int *__restrict_
     __shared__ double s[4];
     if (st && (st->done || (st->flag_next && st->flag_next <= k))) return;
     constexpr int W = 2 * P + 1;
+    constexpr int NS = (D == 3) ? W * W * W : W * W;
+    static_assert(kStencilTile == 256, "one block applies one row tile");
     double contrib = 0.0;
-    for (int r = blockIdx.x * 256 + threadIdx.x; r < nfree; r += gridDim.x * 256) {
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    for (int r = blk * 256 + threadIdx.x; r < nfree; r += gridDim.x * 256) {
         const int g = free2grid[r];
         const T *xb = x + g;
-        const V *vb = sval + r;
+        const V *vb = sval + stencil_index(0, r, NS);
         T acc = 0;
         if (D == 3) {
 #pragma unroll 1
@@ -120,7 +145,7 @@ __global__ __launch_bounds__(256) void k_sspmv(int nfree, const int *__restrict_
                     const int slot0 = ((dz + P) * W + (dy + P)) * W;
 #pragma unroll
                     for (int dx = -P; dx <= P; ++dx)
-                        acc += (T)vb[(int64_t)(slot0 + dx + P) * nfree] * xb[base + dx];
+                        acc += (T)vb[(slot0 + dx + P) * kStencilTile] * xb[base + dx];
                 }
             }
         } else {
@@ -130,7 +155,7 @@ __global__ __launch_bounds__(256) void k_sspmv(int nfree, const int *__restrict_
                 const int slot0 = (dy + P) * W;
 #pragma unroll
                 for (int dx = -P; dx <= P; ++dx)
-                    acc += (T)vb[(int64_t)(slot0 + dx + P) * nfree] * xb[base + dx];
+                    acc += (T)vb[(slot0 + dx + P) * kStencilTile] * xb[base + dx];
             }
         }
         if (MODE == 2) {
@@ -311,6 +336,15 @@ hipError_t launch_csr_to_stencil(int d, int p, int nfree, const int *rp, const i
     GSX_C(3, 1) GSX_C(3, 2) GSX_C(3, 3) GSX_C(3, 4)
 #undef GSX_C
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_stencil_diag(int nfree, int ns, const int *f2g, const double *s64, double *diag_grid,
+                              hipStream_t s)
+{
+    if (nfree <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stencil_diag, dim3((nfree + 255) / 256), dim3(256), 0, s, nfree, ns, (ns - 1) / 2, f2g, s64,
+                       diag_grid);
+    return hipGetLastError();
 }
 
 hipError_t launch_scatter_f64_to_grid(int nfree, const int *f2g, const double *src, double *dst, hipStream_t s)

@@ -32,6 +32,7 @@ PATTERN_EXPR = 0
 PATTERN_FULL = 1
 ASM_REBUILD_PATTERN = 0x100
 ASM_CSR_ONLY = 0x200
+ASM_KEEP_ELEMENTS = 0x400
 PREC_NONE = 0
 PREC_JACOBI = 1
 PREC_MG = 2
@@ -359,6 +360,18 @@ class Matrix:
         n, nnz = C.c_int64(), C.c_int64()
         _check(ctx.h, lib().gsx_matrix_info(h, C.byref(n), C.byref(nnz)))
         self.n, self.nnz = n.value, nnz.value
+
+    @staticmethod
+    def from_csr(ctx, rowptr, col, val, with_fp32=False):
+        """Device matrix from host CSR arrays (sorted columns)."""
+        rp = np.ascontiguousarray(rowptr, np.int32)
+        ci = np.ascontiguousarray(col, np.int32)
+        va = np.ascontiguousarray(val, np.float64)
+        h = C.c_void_p()
+        _check(ctx.h, lib().gsx_matrix_from_csr(ctx.h, len(rp) - 1, rp.ctypes.data_as(_i32p),
+                                                ci.ctypes.data_as(_i32p), va.ctypes.data_as(_dp),
+                                                1 if with_fp32 else 0, C.byref(h)))
+        return Matrix(ctx, h)
 
     def download(self):
         rp = np.zeros(self.n + 1, np.int32)

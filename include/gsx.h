@@ -41,8 +41,13 @@ typedef enum {
     GSX_PATTERN_EXPR = 0,    /* gsExprAssembler: skip exact-zero local entries */
     GSX_PATTERN_FULL = 1,    /* gsPoissonAssembler: every free-free pair       */
     GSX_ASM_REBUILD_PATTERN = 0x100, /* flag: recompute the cached pattern     */
-    GSX_ASM_CSR_ONLY = 0x200         /* flag: no structured operator (CSR
+    GSX_ASM_CSR_ONLY = 0x200,        /* flag: no structured operator (CSR
                                         solves only, e.g. batched members)    */
+    GSX_ASM_KEEP_ELEMENTS = 0x400    /* flag: the problem keeps this assembly's
+                                        element matrices (when they were formed
+                                        in one pass) until its next assembly;
+                                        gsx_mg_create on the returned matrix
+                                        takes them instead of recomputing      */
 } gsx_pattern_mode;
 
 typedef enum {
