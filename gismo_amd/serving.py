@@ -1,19 +1,28 @@
 """Continuous-batching solve server on one GPU.
 
 Requests (a single-patch Poisson/diffusion problem plus 1..8 right-hand
-sides) arrive at any time.  An assembler thread admits them in arrival
-order while their device memory fits the HBM budget, assembles each on its
-own context (its own HIP stream, so assembly overlaps the running solves),
-and hands the resident operator to the solver thread.  The solver thread
-adds one batched-CG member per right-hand side to the running
-`gsx_batch` and advances every member together, a few iterations per
-step; members join and leave between steps.  A request is complete when
-all its right-hand sides have converged and its solutions are back on the
-host; its operator and workspace are then released and its budget freed.
+sides) arrive at any time and are admitted in arrival order while their
+device memory fits the HBM budget.  Two lanes, each with its own context
+(its own HIP stream), run beside the solver thread's batch:
+
+- batch lane: an assembler thread assembles each request and hands the
+  resident operator to the solver thread, which adds one batched-CG member
+  per right-hand side to the running `gsx_batch` and advances every member
+  together, a few iterations per step; members join and leave between
+  steps;
+- multigrid lane: requests of at least `mg_min_dofs` free DOFs whose patch
+  coarsens dyadically to a small coarsest level go to a worker that
+  assembles the operator, builds its multigrid hierarchy once, and solves
+  every right-hand side by MG-preconditioned CG with the hierarchy resident.
+
+A request is complete when all its right-hand sides have converged and its
+solutions are back on the host; its operator, hierarchy and workspace are
+then released and its budget freed.
 
 Every member's iterates are independent of the other members (the batched
-CG reduces per member in a fixed order), so a request's solutions are
-bitwise the same alone and in any mix.
+CG reduces per member in a fixed order; the multigrid lane solves one
+request at a time), so a request's solutions are bitwise the same alone and
+in any mix.
 
 The reference has no serving layer: it assembles and solves one problem at
 a time (src/gsAssembler/gsExprAssembler.h:754-833,
@@ -56,6 +65,7 @@ class Result:
     relres: List[float]
     status: List[int]
     solutions: Optional[List[np.ndarray]] = None
+    lane: str = "batch"           # "batch" (batched Jacobi-CG) or "mg" (multigrid-preconditioned CG)
 
     @property
     def latency(self):
@@ -71,6 +81,37 @@ def estimate_bytes(prob, nrhs, mixed):
     nnz = nfree * (2 * P.p[0] + 1) ** P.d
     per_rhs = nfree * (8 * 6 + (4 * 6 if mixed else 0))
     return int(nnz * (4 + 8 + (4 if mixed else 0)) + nfree * 8 * 3 + nrhs * per_rhs + (64 << 20))
+
+
+def mg_plan(prob, max_coarse=4096):
+    """Levels of a dyadic multigrid hierarchy for a single-patch problem, or
+    0: elements per direction n = m 2^k with the coarsest (m elements per
+    direction) at most max_coarse free DOFs, and element-level Galerkin
+    coarsening available for (d, p) (d = 2: p <= 4; d = 3: p <= 3)."""
+    P = prob.patches[0]
+    d, p = P.d, P.p[0]
+    if len(prob.patches) != 1 or (d == 3 and p > 3) or p > 4:
+        return 0
+    ne = P.nelem_dir()
+    if len(set(ne)) != 1:
+        return 0
+    n = ne[0]
+    levels = 1
+    while n % 2 == 0 and n // 2 >= 2:
+        if max(0, n + p - 2) ** d <= max_coarse:
+            break
+        n //= 2
+        levels += 1
+    if levels < 2 or max(0, n + p - 2) ** d > max_coarse:
+        return 0
+    return levels
+
+
+def mg_element_bytes(prob):
+    """Bytes of the fine element matrices the hierarchy is built from."""
+    P = prob.patches[0]
+    nb = (P.p[0] + 1) ** P.d
+    return P.nelem * nb * nb * 8
 
 
 def rhs_vector(nfree, seed):
@@ -89,20 +130,25 @@ class _Entry:
         self.xs = []
         self.ids = []
         self.bytes = 0
+        self.levels = 0
 
 
 class Server:
     """One GPU, two contexts: one assembles, one solves."""
 
     def __init__(self, precision=gsx.MIXED, budget_bytes=160 << 30, iters_per_step=8, capacity=4096,
-                 keep_solutions=False, device=0):
+                 keep_solutions=False, device=0, mg_min_dofs=20000, elem_budget=16 << 30):
         self.precision = precision
         self.mixed = precision == gsx.MIXED
         self.budget = budget_bytes
         self.iters = iters_per_step
         self.keep = keep_solutions
+        self.mg_min_dofs = mg_min_dofs
+        self.elem_budget = elem_budget
         self.ctx_asm = gsx.Context(device)
         self.ctx = gsx.Context(device)
+        self.ctx_mg = gsx.Context(device)
+        self._mgq = queue.Queue()          # admitted multigrid-lane requests
         self.batch = gsx.Batch(self.ctx, precision, capacity=capacity)
         self.capacity = capacity
         self.t0 = time.perf_counter()
@@ -118,6 +164,8 @@ class Server:
         self._active = {}                  # rid -> entry (solver thread)
         self._asm_thread = threading.Thread(target=self._assembler, daemon=True)
         self._asm_thread.start()
+        self._mg_thread = threading.Thread(target=self._mg_worker, daemon=True)
+        self._mg_thread.start()
 
     def now(self):
         return time.perf_counter() - self.t0
@@ -138,7 +186,10 @@ class Server:
                     pass
                 while pending:
                     req = pending[0]
+                    levels = self._mg_levels(req)
                     need = estimate_bytes(req.problem, len(req.rhs_seeds), self.mixed)
+                    if levels:
+                        need += mg_element_bytes(req.problem) + need // 2
                     with self._lock:
                         if self._used + need > self.budget and self._used > 0:
                             break
@@ -146,6 +197,10 @@ class Server:
                     pending.pop(0)
                     e = _Entry(req, self.now())
                     e.bytes = need
+                    if levels:
+                        e.levels = levels
+                        self._mgq.put(e)
+                        continue
                     e.gp = gsx.Problem(self.ctx_asm, req.problem)
                     e.A, b0 = e.gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_CSR_ONLY, self.precision)
                     e.bs.append(b0)
@@ -154,6 +209,57 @@ class Server:
                     self.ctx_asm.synchronize()
                     e.t_asm = self.now()
                     self._ready.put(e)
+        except Exception as ex:   # reported by the solver thread
+            self._error = ex
+
+    def _mg_levels(self, req):
+        P = req.problem.patches[0]
+        nfree = int(np.prod([nb - 2 for nb in P.nbasis_dir()]))
+        if nfree < self.mg_min_dofs or mg_element_bytes(req.problem) > self.elem_budget:
+            return 0
+        return mg_plan(req.problem)
+
+    # ----- multigrid lane -------------------------------------------------
+    def _mg_worker(self):
+        """Assemble, build the hierarchy once, solve every right-hand side by
+        MG-PCG (V(2,2), damped Jacobi 0.6, exact coarse solve), release."""
+        try:
+            while not self._stop:
+                try:
+                    e = self._mgq.get(timeout=0.002)
+                except queue.Empty:
+                    continue
+                req, ctx = e.req, self.ctx_mg
+                gp = gsx.Problem(ctx, req.problem)
+                A, b0 = gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_KEEP_ELEMENTS, self.precision)
+                mg = gsx.Multigrid(gp, A, e.levels, 2, 2, 0.6, precision=self.precision)
+                ctx.synchronize()
+                t_asm = self.now()
+                its, rels, sts, sols = [], [], [], []
+                for k, seed in enumerate(req.rhs_seeds):
+                    b = b0 if k == 0 else ctx.to_device(rhs_vector(gp.nfree, seed))
+                    x = ctx.array(gp.nfree).zero()
+                    it, rel, ok = gsx.mg_cg_solve(A, mg, b, x, tol=req.tol, maxit=req.maxit)
+                    its.append(it)
+                    rels.append(rel)
+                    sts.append(1 if ok else 2)
+                    sols.append(x.download())
+                    x.close()
+                    if k:
+                        b.close()
+                t_done = self.now()
+                res = Result(rid=req.rid, dofs=gp.nglued, nfree=gp.nfree, nrhs=len(req.rhs_seeds),
+                             arrival=req.arrival, admitted=e.t_admit, assembled=t_asm, done=t_done,
+                             iterations=its, relres=rels, status=sts,
+                             solutions=sols if (self.keep or req.keep) else None, lane="mg")
+                mg.close()
+                A.close()
+                b0.close()
+                gp.close()
+                with self._lock:
+                    self.results.append(res)
+                    self._used -= e.bytes
+                    self._outstanding -= 1
         except Exception as ex:   # reported by the solver thread
             self._error = ex
 
@@ -185,11 +291,11 @@ class Server:
             for i in e.ids:
                 self.batch.remove(i)
                 self._members -= 1
-            self.results.append(Result(
+            result = Result(
                 rid=rid, dofs=e.gp.nglued, nfree=e.gp.nfree, nrhs=len(e.ids), arrival=e.req.arrival,
                 admitted=e.t_admit, assembled=e.t_asm, done=t_done,
                 iterations=[r[1] for r in res], relres=[r[2] for r in res], status=[r[0] for r in res],
-                solutions=sols if (self.keep or e.req.keep) else None))
+                solutions=sols if (self.keep or e.req.keep) else None, lane="batch")
             for x in e.xs:
                 x.close()
             e.A.close()
@@ -198,6 +304,7 @@ class Server:
             e.gp.close()
             del self._active[rid]
             with self._lock:
+                self.results.append(result)
                 self._used -= e.bytes
                 self._outstanding -= 1
         return running
@@ -218,16 +325,29 @@ class Server:
     def close(self):
         self._stop = True
         self._asm_thread.join()
+        self._mg_thread.join()
         self.batch.close()
         self.ctx.close()
         self.ctx_asm.close()
+        self.ctx_mg.close()
 
 
-def make_requests(count, seed=20261202, size_range=(1e3, 2e6), nrhs_range=(1, 8), noise=0.1, sigma=0.5):
+def dyadic_round(n):
+    """n rounded to the nearest m 2^k with 8 <= m < 16 (n itself below 16):
+    at most ~6% away, and such a patch coarsens dyadically k times."""
+    if n < 16:
+        return n
+    k = int(np.floor(np.log2(n))) - 3
+    return int(round(n / 2 ** k)) * 2 ** k
+
+
+def make_requests(count, seed=20261202, size_range=(1e3, 2e6), nrhs_range=(1, 8), noise=0.1, sigma=0.5,
+                  dyadic=True):
     """Requests of the serving workload: size log-uniform in size_range DOFs,
     d uniform in {2, 3}, p uniform in {2, 3, 4}, n = round(size^(1/d)) - p
-    (at least 2), seeded geometry noise, lognormal kappa, random per-element
-    sources, 1..8 right-hand sides."""
+    (at least 2; with dyadic, rounded by dyadic_round so that large requests
+    admit a multigrid hierarchy), seeded geometry noise, lognormal kappa,
+    random per-element sources, 1..8 right-hand sides."""
     out = []
     lo, hi = np.log(size_range[0]), np.log(size_range[1])
     for i in range(count):
@@ -237,6 +357,8 @@ def make_requests(count, seed=20261202, size_range=(1e3, 2e6), nrhs_range=(1, 8)
         d = 2 + int(u[1] * 2)
         p = 2 + int(u[2] * 3)
         n = max(2, int(round(size ** (1.0 / d))) - p)
+        if dyadic:
+            n = dyadic_round(n)
         nrhs = nrhs_range[0] + int(u[3] * (nrhs_range[1] - nrhs_range[0] + 1))
         P = problems.noisy_box_fast(d, p, n, noise, sigma, s)
         prob = problems.Problem([P], problems.all_sides(d), [], name=f"req{i}-d{d}-p{p}-n{n}")
