@@ -632,7 +632,8 @@ __global__ __launch_bounds__(256) void k_gather(const PatchDev *__restrict__ pat
                                                 const double *__restrict__ Kbuf,
                                                 const double *__restrict__ Fbuf,
                                                 double *__restrict__ val, double *__restrict__ rhs,
-                                                unsigned char *__restrict__ keep_flags)
+                                                unsigned char *__restrict__ keep_flags,
+                                                const double *__restrict__ fixed)
 {
     __shared__ int sCol[4][MAXROW];
     __shared__ double sAcc[4][MAXROW];
@@ -666,6 +667,7 @@ __global__ __launch_bounds__(256) void k_gather(const PatchDev *__restrict__ pat
                     const int64_t el = ex + (int64_t)pd.nel[0] * (ey + (int64_t)pd.nel[1] * ez);
                     const int lb = (l[0] - ex) + nb1[0] * ((l[1] - ey) + nb1[1] * (l[2] - ez));
                     const double *Krow = Kbuf + pd.kmat_off + el * NB * NB + (int64_t)lb * NB;
+                    double elim = 0.0;
                     for (int b = lane; b < NB; b += kWave) {
                         const int bx = b % nb1[0], by = (b / nb1[0]) % nb1[1], bz = b / (nb1[0] * nb1[1]);
                         const int64_t m = (ex + bx) + (int64_t)pd.nb[0] * ((ey + by) + (int64_t)pd.nb[1] * (ez + bz));
@@ -679,9 +681,17 @@ __global__ __launch_bounds__(256) void k_gather(const PatchDev *__restrict__ pat
                             const double v = Krow[b];
                             sAcc[w][lo] += v;
                             sNz[w][lo] |= (v != 0.0);
+                        } else if (fixed) {
+                            // eliminated column: K_ij g_j moves to the rhs
+                            elim += Krow[b] * fixed[g - nfree];
                         }
                     }
                     __builtin_amdgcn_wave_barrier();
+                    if (fixed) {
+#pragma unroll
+                        for (int off = 32; off > 0; off >>= 1) elim += __shfl_xor(elim, off, kWave);
+                        racc -= elim;
+                    }
                     if (Fbuf) racc += Fbuf[pd.fvec_off + el * NB + lb];
                 }
     }
@@ -731,7 +741,9 @@ __global__ __launch_bounds__(256) void k_gather_patch(PatchBox bx, int nfree, in
                                                       double *__restrict__ rhs,
                                                       unsigned char *__restrict__ keep,
                                                       double *__restrict__ s64,
-                                                      float *__restrict__ s32)
+                                                      float *__restrict__ s32,
+                                                      const int *__restrict__ dofs,
+                                                      const double *__restrict__ fixed)
 {
     constexpr int W = 2 * P + 1;
     constexpr int NS = GatherCfg<D, P>::NS;
@@ -769,29 +781,46 @@ __global__ __launch_bounds__(256) void k_gather_patch(PatchBox bx, int nfree, in
         // element order
         constexpr int U = (NB > kWave) ? 1 : 8;
         for (int t0 = 0; t0 < ne; t0 += U) {
+            double elimv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) elimv[u] = 0.0;
             for (int b0 = 0; b0 < NB; b0 += kWave) {
                 const int b = b0 + lane;
                 const int bxx = b % NB1, byy = (b / NB1) % NB1, bzz = (D == 3) ? b / (NB1 * NB1) : 0;
-                double v[U];
+                double v[U], el_u[U];
                 int slot[U];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const int t = t0 + u;
                     slot[u] = -1;
                     v[u] = 0.0;
+                    el_u[u] = 0.0;
                     if (t < ne && b < NB) {
                         const int ex = elo[0] + t % ne0, ey = elo[1] + (t / ne0) % ne1;
                         const int ez = elo[2] + t / (ne0 * ne1);
                         const int jx = ex + bxx, jy = ey + byy, jz = ez + bzz;
                         bool fr = jx >= bx.flo[0] && jx <= bx.fhi[0] && jy >= bx.flo[1] && jy <= bx.fhi[1];
                         if (D == 3) fr = fr && jz >= bx.flo[2] && jz <= bx.fhi[2];
+                        const int64_t el = ex + (int64_t)bx.nel[0] * (ey + (int64_t)bx.nel[1] * ez) - elbase;
+                        const int la = (ic[0] - ex) + NB1 * ((ic[1] - ey) + NB1 * (ic[2] - ez));
                         if (fr) {
-                            const int64_t el = ex + (int64_t)bx.nel[0] * (ey + (int64_t)bx.nel[1] * ez) - elbase;
-                            const int la = (ic[0] - ex) + NB1 * ((ic[1] - ey) + NB1 * (ic[2] - ez));
                             v[u] = Kbuf[el * (NB * NB) + (int64_t)la * NB + b];
                             slot[u] = (D == 3) ? ((jz - ic[2] + P) * W + (jy - ic[1] + P)) * W + (jx - ic[0] + P)
                                                : (jy - ic[1] + P) * W + (jx - ic[0] + P);
+                        } else if (fixed) {
+                            // eliminated column: K_ij g_j moves to the rhs
+                            const int64_t m = jx + (int64_t)bx.nb[0] * (jy + (int64_t)bx.nb[1] * jz);
+                            el_u[u] = Kbuf[el * (NB * NB) + (int64_t)la * NB + b] * fixed[dofs[m] - nfree];
                         }
+                    }
+                }
+                if (fixed) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        double e = el_u[u];
+#pragma unroll
+                        for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, kWave);
+                        elimv[u] += e;
                     }
                 }
                 // element order is kept: the slots of one element are
@@ -810,6 +839,7 @@ __global__ __launch_bounds__(256) void k_gather_patch(PatchBox bx, int nfree, in
                 const int ex = elo[0] + t % ne0, ey = elo[1] + (t / ne0) % ne1, ez = elo[2] + t / (ne0 * ne1);
                 const int64_t el = ex + (int64_t)bx.nel[0] * (ey + (int64_t)bx.nel[1] * ez) - elbase;
                 const int la = (ic[0] - ex) + NB1 * ((ic[1] - ey) + NB1 * (ic[2] - ez));
+                if (fixed) racc -= elimv[u];
                 racc += Fbuf[el * NB + la];
             }
         }
@@ -904,7 +934,7 @@ hipError_t launch_gather_patch(int d, int p, const int *flo, const int *fhi, con
                                const int *nel, const int *padded, int nfree, int r0, int r1, int64_t elbase,
                                const int *f2g, const int *rowptr, const double *Kbuf, const double *Fbuf,
                                double *val, double *rhs, unsigned char *keep, double *s64,
-                               float *s32, hipStream_t s)
+                               float *s32, const int *dofs, const double *fixed, hipStream_t s)
 {
     if (r1 <= r0) return hipSuccess;
     PatchBox bx;
@@ -915,7 +945,7 @@ hipError_t launch_gather_patch(int d, int p, const int *flo, const int *fhi, con
         bx.nel[k] = nel[k];
         bx.padded[k] = padded[k];
     }
-#define GSX_G(D, P) if (d == D && p == P) { const dim3 grid((r1 - r0 + GatherCfg<D, P>::R - 1) / GatherCfg<D, P>::R), blk(256); hipLaunchKernelGGL((k_gather_patch<D, P>), grid, blk, 0, s, bx, nfree, r0, r1, elbase, f2g, rowptr, Kbuf, Fbuf, val, rhs, keep, s64, s32); return hipGetLastError(); }
+#define GSX_G(D, P) if (d == D && p == P) { const dim3 grid((r1 - r0 + GatherCfg<D, P>::R - 1) / GatherCfg<D, P>::R), blk(256); hipLaunchKernelGGL((k_gather_patch<D, P>), grid, blk, 0, s, bx, nfree, r0, r1, elbase, f2g, rowptr, Kbuf, Fbuf, val, rhs, keep, s64, s32, dofs, fixed); return hipGetLastError(); }
     GSX_G(2, 1) GSX_G(2, 2) GSX_G(2, 3) GSX_G(2, 4)
     GSX_G(3, 1) GSX_G(3, 2) GSX_G(3, 3) GSX_G(3, 4)
 #undef GSX_G
@@ -987,7 +1017,8 @@ __global__ __launch_bounds__(128) void k_l2err(const PatchDev *__restrict__ patc
                                                const int64_t *__restrict__ gprefix, int ng,
                                                GaussTab gt, const double *__restrict__ sol,
                                                int nfree, double *__restrict__ out,
-                                               double *__restrict__ out_h1)
+                                               double *__restrict__ out_h1,
+                                               const double *__restrict__ fixed)
 {
     constexpr int NB1 = P + 1, NQ1 = P + 1;
     constexpr int NB = (D == 2) ? NB1 * NB1 : NB1 * NB1 * NB1;
@@ -1010,7 +1041,7 @@ __global__ __launch_bounds__(128) void k_l2err(const PatchDev *__restrict__ patc
         const int64_t m = (ei[0] + bx) + (int64_t)pd.nb[0] * ((ei[1] + by) + (int64_t)pd.nb[1] * (ei[2] + bz));
         for (int r = 0; r < D; ++r) sC[b][r] = pd.ctrl[m * D + r];
         const int g = pd.dofs[m];
-        sU[b] = (g < nfree) ? sol[g] : 0.0;
+        sU[b] = (g < nfree) ? sol[g] : (fixed ? fixed[g - nfree] : 0.0);
     }
     __syncthreads();
     double local = 0.0, local_h1 = 0.0;
@@ -1097,6 +1128,200 @@ __global__ __launch_bounds__(128) void k_l2err(const PatchDev *__restrict__ patc
     }
 }
 
+// ---------------------------------------------------------------------------
+// Boundary element terms.  Quadrature of gsGaussRule(basis, 1, 1, dir)
+// (src/gsAssembler/gsGaussRule.hpp:23-60, mapped by gsQuadRule::mapTo,
+// gsQuadRule.h:177-201): p+1 Gauss nodes along each tangential direction,
+// one node of weight 1 on the side across it.  Per element, over the
+// (p+1)^(d-1) functions of the boundary layer (the others vanish on the
+// side):
+//   Dirichlet (L2 projection, gsDirichletValuesByL2Projection,
+//   src/gsAssembler/gsDirichletValues.h:257-435): M_ij = sum_q (w_q m_q N_i)
+//   N_j and F_i = sum_q (w_q m_q N_i) g(x_q), where m_q = |det J| (the map
+//   data carries no side there, so the measure is the volume Jacobian's);
+//   Neumann (gsExprAssembler::assembleBdr with u * g_N.tr() * nv(G),
+//   src/gsAssembler/gsExprAssembler.h:835-895): F_i = sum_q (w_q N_i)
+//   (g_N(x_q) . nv_q), nv the outer normal of gsFunction::computeMap
+//   (src/gsCore/gsFunction.hpp:613-680): component c is alt_c times the
+//   minor of J^T without row dir and column c, alt alternating from
+//   sideOrientation(side) times the sign of det J (first node of the element
+//   with |det J| >= 1e-7).
+// ---------------------------------------------------------------------------
+__device__ inline double bc_value(int kind, const double *val, int d, const double *x)
+{
+    if (kind == 1) {
+        const double pi = 3.14159265358979323846;
+        double v = 1.0;
+        for (int r = 0; r < d; ++r) v *= sin(pi * x[r]);
+        return v;
+    }
+    if (kind == 2) {
+        double v = val[0];
+        for (int r = 0; r < d; ++r) v += val[1 + r] * x[r];
+        return v;
+    }
+    return 0.0;
+}
+
+__device__ inline void bc_flux(int kind, const double *val, int d, const double *x, double *f)
+{
+    const double pi = 3.14159265358979323846;
+    for (int c = 0; c < d; ++c) {
+        if (kind == 1) {
+            double v = pi * cos(pi * x[c]);
+            for (int r = 0; r < d; ++r)
+                if (r != c) v *= sin(pi * x[r]);
+            f[c] = v;
+        } else if (kind == 2) {
+            f[c] = val[1 + c];
+        } else {
+            f[c] = 0.0;
+        }
+    }
+}
+
+template <int D, int P>
+__global__ __launch_bounds__(64) void k_bdry(const PatchDev *__restrict__ patches,
+                                             const BdryItem *__restrict__ items, GaussTab gt,
+                                             double *__restrict__ Mout, double *__restrict__ Fout)
+{
+    constexpr int NB1 = P + 1;
+    constexpr int NB = (D == 3) ? NB1 * NB1 * NB1 : NB1 * NB1;
+    constexpr int NBB = (D == 3) ? NB1 * NB1 : NB1;
+    __shared__ double sV[D][NB1][NB1], sD[D][NB1][NB1], sW[D][NB1];
+    __shared__ double sC[NB][D];
+    __shared__ double sNq[NBB][NBB], sWq[NBB], sGq[NBB], sDet[NBB];
+    __shared__ double sJ[NBB][D][D];
+    const BdryItem it = items[blockIdx.x];
+    const PatchDev &pd = patches[it.patch];
+    const int dir = (it.side - 1) / 2, par = (it.side - 1) % 2;
+    const int t = threadIdx.x;
+    if (t < D * NB1) {
+        const int k = t / NB1, q = t % NB1;
+        const double *U = pd.knots[k];
+        const int span = P + it.ei[k];
+        double u, w;
+        if (k == dir) {
+            u = par ? U[span + 1] : U[span];
+            w = 1.0;
+        } else {
+            const double a = U[span], b = U[span + 1];
+            const double h = (b - a) / 2.0;
+            u = h * (gt.x[NB1][q] + 1.0) + a;
+            w = (0.0 == h ? 0.5 : h) * gt.w[NB1][q];
+        }
+        double N[P + 1], dN[P + 1];
+        deboor_ders<P>(U, span, u, N, dN);
+#pragma unroll
+        for (int i = 0; i <= P; ++i) {
+            sV[k][q][i] = N[i];
+            sD[k][q][i] = dN[i];
+        }
+        sW[k][q] = w;
+    }
+    for (int b = t; b < NB; b += 64) {
+        const int bx = b % NB1, by = (b / NB1) % NB1, bz = (D == 3) ? b / (NB1 * NB1) : 0;
+        const int64_t m = (it.ei[0] + bx) + (int64_t)pd.nb[0] * ((it.ei[1] + by) + (int64_t)pd.nb[1] * (it.ei[2] + bz));
+        for (int r = 0; r < D; ++r) sC[b][r] = pd.ctrl[m * D + r];
+    }
+    __syncthreads();
+    // tangential directions and their node index of boundary point q
+    int tdir[2] = {0, 0}, nt = 0;
+    for (int k = 0; k < D; ++k)
+        if (k != dir) tdir[nt++] = k;
+    double x[3] = {0, 0, 0};
+    if (t < NBB) {
+        int qd[3] = {0, 0, 0};
+        qd[tdir[0]] = t % NB1;
+        if (D == 3) qd[tdir[1]] = t / NB1;
+        double J[3][3] = {{0}};
+        for (int b = 0; b < NB; ++b) {
+            const int bi[3] = {b % NB1, (b / NB1) % NB1, (D == 3) ? b / (NB1 * NB1) : 0};
+            double v = 1.0, g[3] = {1.0, 1.0, 1.0};
+            for (int k = 0; k < D; ++k) {
+                v *= sV[k][qd[k]][bi[k]];
+                for (int c = 0; c < D; ++c) g[c] *= (c == k) ? sD[k][qd[k]][bi[k]] : sV[k][qd[k]][bi[k]];
+            }
+            for (int r = 0; r < D; ++r) {
+                x[r] += sC[b][r] * v;
+                for (int c = 0; c < D; ++c) J[r][c] += sC[b][r] * g[c];
+            }
+        }
+        double det;
+        if (D == 2) det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        else det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) +
+                   J[0][1] * (J[1][2] * J[2][0] - J[1][0] * J[2][2]) +
+                   J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        sDet[t] = det;
+        for (int r = 0; r < D; ++r)
+            for (int c = 0; c < D; ++c) sJ[t][r][c] = J[r][c];
+        double w = sW[tdir[0]][qd[tdir[0]]];
+        if (D == 3) w *= sW[tdir[1]][qd[tdir[1]]];
+        sWq[t] = (it.type == 1) ? w * fabs(det) : w;
+        if (it.type == 1) sGq[t] = bc_value(it.kind, it.value, D, x);
+        // boundary-layer functions: index in dir fixed at 0 (par 0) or P
+        for (int jb = 0; jb < NBB; ++jb) {
+            int bi[3];
+            bi[dir] = par ? P : 0;
+            bi[tdir[0]] = jb % NB1;
+            if (D == 3) bi[tdir[1]] = jb / NB1;
+            double v = 1.0;
+            for (int k = 0; k < D; ++k) v *= sV[k][qd[k]][bi[k]];
+            sNq[t][jb] = v;
+        }
+    }
+    __syncthreads();
+    if (it.type == 2 && t < NBB) {
+        // outer normal (unnormalized): sign of det J from the element's first
+        // node with |det J| >= 1e-7 (else the patch orientation)
+        double dsg = 0.0;
+        for (int q = 0; q < NBB; ++q)
+            if (fabs(sDet[q]) >= 1e-7) {
+                dsg = sDet[q] < 0 ? -1.0 : 1.0;
+                break;
+            }
+        if (dsg == 0.0) dsg = pd.orient < 0 ? -1.0 : 1.0;
+        const int s = it.side;
+        const double so = ((s + (s + 1) / 2) % 2) ? 1.0 : -1.0;
+        double alt = so * dsg, nv[3] = {0, 0, 0};
+        // jacT[a][b] = J[b][a] (row: parametric direction, column: coordinate)
+        for (int c = 0; c < D; ++c) {
+            double mdet;
+            if (D == 2) {
+                const int a = 1 - dir, b = 1 - c;
+                mdet = sJ[t][b][a];
+            } else {
+                int ra[2], cb[2], na = 0, nb = 0;
+                for (int a = 0; a < 3; ++a)
+                    if (a != dir) ra[na++] = a;
+                for (int b = 0; b < 3; ++b)
+                    if (b != c) cb[nb++] = b;
+                mdet = sJ[t][cb[0]][ra[0]] * sJ[t][cb[1]][ra[1]] - sJ[t][cb[1]][ra[0]] * sJ[t][cb[0]][ra[1]];
+            }
+            nv[c] = alt * mdet;
+            alt = -alt;
+        }
+        double f[3];
+        bc_flux(it.kind, it.value, D, x, f);
+        double g = 0.0;
+        for (int c = 0; c < D; ++c) g += f[c] * nv[c];
+        sGq[t] = g;
+    }
+    __syncthreads();
+    if (it.type == 1)
+        for (int idx = t; idx < NBB * NBB; idx += 64) {
+            const int i = idx / NBB, j = idx % NBB;
+            double sacc = 0.0;
+            for (int q = 0; q < NBB; ++q) sacc += (sWq[q] * sNq[q][i]) * sNq[q][j];
+            Mout[(int64_t)blockIdx.x * NBB * NBB + idx] = sacc;
+        }
+    for (int i = t; i < NBB; i += 64) {
+        double sacc = 0.0;
+        for (int q = 0; q < NBB; ++q) sacc += (sWq[q] * sNq[q][i]) * sGq[q];
+        Fout[(int64_t)blockIdx.x * NBB + i] = sacc;
+    }
+}
+
 __global__ void k_sum_serial(const double *__restrict__ v, int64_t n, double *__restrict__ out)
 {
     // deterministic two-level sum: 256 strided partials then a fixed tree
@@ -1148,11 +1373,12 @@ hipError_t launch_elem_range(int d, int p, const PatchDev *patches, const int *g
 
 hipError_t launch_l2err(int d, int p, const PatchDev *patches, const int *gpatch,
                         const int64_t *gprefix, int ng, int64_t nelem, const GaussTab &gt,
-                        const double *sol, int nfree, double *out, double *out_h1, hipStream_t s)
+                        const double *sol, int nfree, double *out, double *out_h1, const double *fixed,
+                        hipStream_t s)
 {
     if (nelem <= 0) return hipSuccess;
     const dim3 grid((unsigned)nelem);
-#define GSX_L(D, P) if (d == D && p == P) { hipLaunchKernelGGL((k_l2err<D, P>), grid, dim3(128), 0, s, patches, gpatch, gprefix, ng, gt, sol, nfree, out, out_h1); return hipGetLastError(); }
+#define GSX_L(D, P) if (d == D && p == P) { hipLaunchKernelGGL((k_l2err<D, P>), grid, dim3(128), 0, s, patches, gpatch, gprefix, ng, gt, sol, nfree, out, out_h1, fixed); return hipGetLastError(); }
     GSX_L(2, 1) GSX_L(2, 2) GSX_L(2, 3) GSX_L(2, 4)
     GSX_L(3, 1) GSX_L(3, 2) GSX_L(3, 3) GSX_L(3, 4)
 #undef GSX_L
@@ -1198,17 +1424,28 @@ int gather_maxrow_bucket(int maxrow)
     return -1;
 }
 
+hipError_t launch_bdry(int d, int p, const PatchDev *patches, const BdryItem *items, int nitems,
+                       const GaussTab &gt, double *M, double *F, hipStream_t s)
+{
+    if (nitems <= 0) return hipSuccess;
+#define GSX_B(D, P) if (d == D && p == P) { hipLaunchKernelGGL((k_bdry<D, P>), dim3(nitems), dim3(64), 0, s, patches, items, gt, M, F); return hipGetLastError(); }
+    GSX_B(2, 1) GSX_B(2, 2) GSX_B(2, 3) GSX_B(2, 4)
+    GSX_B(3, 1) GSX_B(3, 2) GSX_B(3, 3) GSX_B(3, 4)
+#undef GSX_B
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_gather(const PatchDev *patches, const int *occ_ptr, const int *occ_patch,
                          const int64_t *occ_local, int nfree, const int *rowptr, const int *col,
                          const double *Kbuf, const double *Fbuf, double *val, double *rhs,
-                         unsigned char *keep, int maxrow, hipStream_t s)
+                         unsigned char *keep, int maxrow, const double *fixed, hipStream_t s)
 {
     if (nfree <= 0) return hipSuccess;
     const dim3 grid((nfree + 3) / 4), block(256);
     switch (gather_maxrow_bucket(maxrow)) {
-    case 128: hipLaunchKernelGGL(k_gather<128>, grid, block, 0, s, patches, occ_ptr, occ_patch, occ_local, nfree, rowptr, col, Kbuf, Fbuf, val, rhs, keep); break;
-    case 512: hipLaunchKernelGGL(k_gather<512>, grid, block, 0, s, patches, occ_ptr, occ_patch, occ_local, nfree, rowptr, col, Kbuf, Fbuf, val, rhs, keep); break;
-    case 1024: hipLaunchKernelGGL(k_gather<1024>, grid, block, 0, s, patches, occ_ptr, occ_patch, occ_local, nfree, rowptr, col, Kbuf, Fbuf, val, rhs, keep); break;
+    case 128: hipLaunchKernelGGL(k_gather<128>, grid, block, 0, s, patches, occ_ptr, occ_patch, occ_local, nfree, rowptr, col, Kbuf, Fbuf, val, rhs, keep, fixed); break;
+    case 512: hipLaunchKernelGGL(k_gather<512>, grid, block, 0, s, patches, occ_ptr, occ_patch, occ_local, nfree, rowptr, col, Kbuf, Fbuf, val, rhs, keep, fixed); break;
+    case 1024: hipLaunchKernelGGL(k_gather<1024>, grid, block, 0, s, patches, occ_ptr, occ_patch, occ_local, nfree, rowptr, col, Kbuf, Fbuf, val, rhs, keep, fixed); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

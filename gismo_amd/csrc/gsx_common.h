@@ -92,6 +92,15 @@ struct CGState {
     int frozen;
 };
 
+// One boundary element: the element of `patch` adjacent to side `side`
+// (1..2d) at element index ei, the condition's type (1 Dirichlet, 2
+// Neumann), data kind (0 zero, 1 manufactured, 2 affine) and values.
+struct BdryItem {
+    int patch, side, type, kind;
+    int ei[3];
+    double value[4];
+};
+
 // Element range of one element-kernel launch: elements [e0, e0 + grid) of
 // the launch's patch group; element matrices and load vectors are written
 // shifted down by kshift / fshift doubles (a slab of a larger patch).

@@ -15,7 +15,8 @@ hipError_t launch_elem_range(int d, int p, const PatchDev *patches, const int *g
                              double *Kbuf, double *Fbuf, double *detmin, hipStream_t s);
 hipError_t launch_l2err(int d, int p, const PatchDev *patches, const int *gpatch,
                         const int64_t *gprefix, int ng, int64_t nelem, const GaussTab &gt,
-                        const double *sol, int nfree, double *out, double *out_h1, hipStream_t s);
+                        const double *sol, int nfree, double *out, double *out_h1, const double *fixed,
+                        hipStream_t s);
 hipError_t launch_rowcount(const PatchDev *patches, const int *occ_ptr, const int *occ_patch,
                            const int64_t *occ_local, int nfree, int *counts, int *err,
                            hipStream_t s);
@@ -25,10 +26,14 @@ hipError_t launch_rowfill(const PatchDev *patches, const int *occ_ptr, const int
 hipError_t launch_scan(const int *counts, int n, int *rowptr, long long *bsum, long long *total,
                        hipStream_t s);
 int gather_maxrow_bucket(int maxrow);
+// boundary element terms (k_bdry): per item, the boundary-layer mass matrix
+// M ((p+1)^(2(d-1)), Dirichlet items only) and load F ((p+1)^(d-1))
+hipError_t launch_bdry(int d, int p, const PatchDev *patches, const BdryItem *items, int nitems,
+                       const GaussTab &gt, double *M, double *F, hipStream_t s);
 hipError_t launch_gather(const PatchDev *patches, const int *occ_ptr, const int *occ_patch,
                          const int64_t *occ_local, int nfree, const int *rowptr, const int *col,
                          const double *Kbuf, const double *Fbuf, double *val, double *rhs,
-                         unsigned char *keep, int maxrow, hipStream_t s);
+                         unsigned char *keep, int maxrow, const double *fixed, hipStream_t s);
 hipError_t launch_rowfill_patch(int d, int p, const int *flo, const int *fhi, const int *nb,
                                 const int *nel, const int *padded, int nfree, const int *f2g,
                                 const int *rowptr, int *col, hipStream_t s);
@@ -38,7 +43,7 @@ hipError_t launch_gather_patch(int d, int p, const int *flo, const int *fhi, con
                                const int *nel, const int *padded, int nfree, int r0, int r1, int64_t elbase,
                                const int *f2g, const int *rowptr, const double *Kbuf, const double *Fbuf,
                                double *val, double *rhs, unsigned char *keep, double *s64,
-                               float *s32, hipStream_t s);
+                               float *s32, const int *dofs, const double *fixed, hipStream_t s);
 hipError_t launch_keep(const int *rowptr, const int *col, const double *val,
                        const unsigned char *keep, int n, int64_t nnz, int *counts, int *nrowptr,
                        int *ncol, double *nval, long long *bsum, long long *total,

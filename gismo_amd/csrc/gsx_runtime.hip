@@ -277,6 +277,9 @@ struct gsx_problem {
     DevPtr d_detmin, d_scan_bsum, d_scan_total;
     int64_t kdoubles = 0, fdoubles = 0;
     double min_det = 0.0;
+    // boundary data: values of the eliminated DOFs (global index - nfree;
+    // null: homogeneous) and the Neumann load over the free DOFs (null: none)
+    DevPtr d_fixed, d_neu;
     // element matrices of the last assembly (GSX_ASM_KEEP_ELEMENTS), laid
     // out at kmat_off, and the serial of that assembly
     DevPtr kept_K;
@@ -1085,7 +1088,8 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
                                                dp<int>(P->d_rowptr), dp<double>(Kbuf), dp<double>(Fbuf),
                                                dp<double>(A->val), rhs, keep ? dp<unsigned char>(keep) : nullptr,
                                                S.s64 ? dp<double>(S.s64) : nullptr,
-                                               S.s32 ? dp<float>(S.s32) : nullptr, s));
+                                               S.s32 ? dp<float>(S.s32) : nullptr, dp<int>(P->d_dofs),
+                                               P->d_fixed ? dp<double>(P->d_fixed) : nullptr, s));
         }
     } else {
         KScope k(c, "gather");
@@ -1093,7 +1097,8 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
                                      dp<int>(P->d_occ_patch), dp<int64_t>(P->d_occ_local), P->nfree,
                                      dp<int>(P->d_rowptr), dp<int>(P->d_col), dp<double>(Kbuf),
                                      dp<double>(Fbuf), dp<double>(A->val), rhs,
-                                     keep ? dp<unsigned char>(keep) : nullptr, P->maxrow, s));
+                                     keep ? dp<unsigned char>(keep) : nullptr, P->maxrow,
+                                     P->d_fixed ? dp<double>(P->d_fixed) : nullptr, s));
     }
     // element matrices go back to the cache once the gathers have run, or
     // stay with the problem when asked for and formed in one pass
@@ -1752,7 +1757,8 @@ gsx_status gsx_error_norms(gsx_problem *P, const double *sol, double *l2, double
     for (auto &g : P->groups)
         HIP_OK(c, gsx::launch_l2err(g.d, g.p, dp<PatchDev>(P->d_patches), dp<int>(g.d_gpatch),
                                     dp<int64_t>(g.d_gprefix), (int)g.patches.size(), g.nelem, c->gt,
-                                    sol, P->nfree, pl2, h1semi ? ph1 : nullptr, s));
+                                    sol, P->nfree, pl2, h1semi ? ph1 : nullptr,
+                                    P->d_fixed ? dp<double>(P->d_fixed) : nullptr, s));
     HIP_OK(c, gsx::launch_sum(pl2, P->nelem, pl2 + ne, s));
     if (h1semi) HIP_OK(c, gsx::launch_sum(ph1, P->nelem, ph1 + ne, s));
     double tot[2] = {0.0, 0.0};
@@ -2153,7 +2159,8 @@ extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nl
                                               dp<double>(Kc), s));
             HIP_OK(c, gsx::launch_gather(dp<PatchDev>(cp->d_patches), dp<int>(cp->d_occ_ptr), dp<int>(cp->d_occ_patch),
                                          dp<int64_t>(cp->d_occ_local), C.n, dp<int>(C.rowptr), dp<int>(C.col),
-                                         dp<double>(Kc), nullptr, dp<double>(C.val), nullptr, nullptr, cp->maxrow, s));
+                                         dp<double>(Kc), nullptr, dp<double>(C.val), nullptr, nullptr, cp->maxrow,
+                                         nullptr, s));
             Kf = std::move(Kc);
         } else {
             // Galerkin: B = A P (ELL), A_c = R B into the coarse pattern
@@ -2857,5 +2864,242 @@ gsx_status gsx_batch_remove(gsx_batch *B, int32_t id)
     HIP_OK(B->c, hipStreamSynchronize(B->c->stream));
     B->m[id] = BMember();
     B->dirty = true;
+    return GSX_OK;
+}
+
+// ===========================================================================
+// Boundary data: Dirichlet values by L2 projection, Neumann load
+// ===========================================================================
+namespace {
+
+// side elements of (patch, side) in the order of the side's domain iterator
+// (lexicographic over the tangential directions, direction 0 fastest)
+void side_items(const HostPatch &h, int patch, const gsx_bc &b, std::vector<gsx::BdryItem> &out)
+{
+    const int dir = (b.side - 1) / 2, par = (b.side - 1) % 2;
+    int64_t cnt = 1;
+    for (int k = 0; k < h.d; ++k)
+        if (k != dir) cnt *= h.nel[k];
+    for (int64_t t = 0; t < cnt; ++t) {
+        gsx::BdryItem it{};
+        it.patch = patch;
+        it.side = b.side;
+        it.type = b.type;
+        it.kind = b.kind;
+        for (int v = 0; v < 4; ++v) it.value[v] = b.value[v];
+        int64_t r = t;
+        for (int k = 0; k < h.d; ++k) {
+            if (k == dir) {
+                it.ei[k] = par ? h.nel[k] - 1 : 0;
+                continue;
+            }
+            it.ei[k] = (int)(r % h.nel[k]);
+            r /= h.nel[k];
+        }
+        out.push_back(it);
+    }
+}
+
+// patch-local index of boundary-layer function jb of a side item
+int64_t layer_function(const HostPatch &h, const gsx::BdryItem &it, int jb)
+{
+    const int dir = (it.side - 1) / 2, par = (it.side - 1) % 2, p1 = h.p[0] + 1;
+    int bi[3] = {0, 0, 0}, nt = 0;
+    for (int k = 0; k < h.d; ++k) {
+        if (k == dir) {
+            bi[k] = par ? h.p[0] : 0;
+            continue;
+        }
+        bi[k] = nt == 0 ? jb % p1 : jb / p1;
+        ++nt;
+    }
+    int64_t m = 0, str = 1;
+    for (int k = 0; k < h.d; ++k) {
+        m += (it.ei[k] + bi[k]) * str;
+        str *= h.nb[k];
+    }
+    return m;
+}
+
+// boundary element terms of `items` (grouped by the patches' (d, p)) to host
+gsx_status bdry_terms(gsx_problem *P, const std::vector<gsx::BdryItem> &items, std::vector<double> &M,
+                      std::vector<double> &F, int &nbb)
+{
+    gsx_ctx *c = P->ctx;
+    hipStream_t s = c->stream;
+    const int d = P->d, p = P->hp[items[0].patch].p[0];
+    nbb = d == 3 ? (p + 1) * (p + 1) : p + 1;
+    const int n = (int)items.size();
+    DevPtr di = dalloc(c, sizeof(gsx::BdryItem) * n), dM = dalloc(c, sizeof(double) * (size_t)n * nbb * nbb),
+           dF = dalloc(c, sizeof(double) * (size_t)n * nbb);
+    if (!di || !dM || !dF) return fail(c, GSX_ERR_OOM, "device allocation (boundary terms)");
+    HIP_OK(c, hipMemcpyAsync(di.get(), items.data(), sizeof(gsx::BdryItem) * n, hipMemcpyHostToDevice, s));
+    HIP_OK(c, gsx::launch_bdry(d, p, dp<PatchDev>(P->d_patches), dp<gsx::BdryItem>(di), n, c->gt, dp<double>(dM),
+                               dp<double>(dF), s));
+    M.resize((size_t)n * nbb * nbb);
+    F.resize((size_t)n * nbb);
+    HIP_OK(c, hipMemcpyAsync(M.data(), dM.get(), sizeof(double) * M.size(), hipMemcpyDeviceToHost, s));
+    HIP_OK(c, hipMemcpyAsync(F.data(), dF.get(), sizeof(double) * F.size(), hipMemcpyDeviceToHost, s));
+    HIP_OK(c, hipStreamSynchronize(s));
+    return GSX_OK;
+}
+
+}  // namespace
+
+extern "C" gsx_status gsx_problem_set_bc(gsx_problem *P, int32_t nbc, const gsx_bc *bc, int32_t dir_values)
+{
+    if (!P || nbc < 0 || (nbc > 0 && !bc)) return GSX_ERR_ARG;
+    gsx_ctx *c = P->ctx;
+    hipStream_t s = c->stream;
+    if (dir_values != GSX_DIR_HOMOGENEOUS && dir_values != GSX_DIR_L2PROJECTION)
+        return fail(c, GSX_ERR_ARG, "unknown Dirichlet value method");
+    const int d = P->d;
+    std::vector<std::pair<int, int>> dset, iset;
+    for (size_t k = 0; k + 1 < P->dir_sides_h.size(); k += 2) dset.push_back({P->dir_sides_h[k], P->dir_sides_h[k + 1]});
+    for (size_t k = 0; k + 7 < P->ifaces_h.size(); k += 8) {
+        iset.push_back({P->ifaces_h[k], P->ifaces_h[k + 1]});
+        iset.push_back({P->ifaces_h[k + 2], P->ifaces_h[k + 3]});
+    }
+    auto has = [](const std::vector<std::pair<int, int>> &v, int a, int b) {
+        return std::find(v.begin(), v.end(), std::make_pair(a, b)) != v.end();
+    };
+    // Dirichlet data per eliminated side (problem order), Neumann records
+    std::vector<gsx_bc> dir(dset.size()), neu;
+    for (size_t k = 0; k < dset.size(); ++k) {
+        dir[k] = gsx_bc{};
+        dir[k].patch = dset[k].first;
+        dir[k].side = dset[k].second;
+        dir[k].type = GSX_BC_DIRICHLET;
+        dir[k].kind = GSX_BCF_ZERO;
+    }
+    std::vector<std::pair<int, int>> seen;
+    for (int k = 0; k < nbc; ++k) {
+        const gsx_bc &b = bc[k];
+        if (b.patch < 0 || b.patch >= P->npatch || b.side < 1 || b.side > 2 * d ||
+            (b.type != GSX_BC_DIRICHLET && b.type != GSX_BC_NEUMANN) || b.kind < GSX_BCF_ZERO ||
+            b.kind > GSX_BCF_AFFINE)
+            return fail(c, GSX_ERR_ARG, "invalid boundary condition record");
+        if (has(seen, b.patch, b.side)) return fail(c, GSX_ERR_ARG, "two boundary conditions on one side");
+        seen.push_back({b.patch, b.side});
+        if (b.type == GSX_BC_DIRICHLET) {
+            if (!has(dset, b.patch, b.side))
+                return fail(c, GSX_ERR_ARG, "Dirichlet data on a side that the problem does not eliminate");
+            for (size_t j = 0; j < dset.size(); ++j)
+                if (dset[j] == std::make_pair(b.patch, b.side)) dir[j] = b;
+        } else {
+            if (has(dset, b.patch, b.side) || has(iset, b.patch, b.side))
+                return fail(c, GSX_ERR_ARG, "Neumann data on an eliminated or interface side");
+            neu.push_back(b);
+        }
+    }
+    for (const gsx_bc &b : dir)
+        if (P->hp[b.patch].p[0] > gsx::kMaxDeg) return fail(c, GSX_ERR_ARG, "degree not supported");
+    P->d_fixed.reset();
+    P->d_neu.reset();
+    const int64_t nfix = P->nglued - P->nfree;
+    bool inhom = false;
+    for (const gsx_bc &b : dir) inhom = inhom || b.kind != GSX_BCF_ZERO;
+    if (dir_values == GSX_DIR_L2PROJECTION && inhom && nfix > 0) {
+        // boundary mass matrix and load over every Dirichlet side
+        // (gsDirichletValuesByL2Projection, src/gsAssembler/gsDirichletValues.h:257-435)
+        std::vector<std::map<int, double>> rows((size_t)nfix);
+        std::vector<double> rhs((size_t)nfix, 0.0);
+        for (const Group &g : P->groups) {
+            std::vector<gsx::BdryItem> items;
+            for (const gsx_bc &b : dir)
+                if (std::find(g.patches.begin(), g.patches.end(), b.patch) != g.patches.end())
+                    side_items(P->hp[b.patch], b.patch, b, items);
+            if (items.empty()) continue;
+            std::vector<double> M, F;
+            int nbb = 0;
+            GSX_TRY(bdry_terms(P, items, M, F, nbb));
+            for (size_t t = 0; t < items.size(); ++t) {
+                const HostPatch &h = P->hp[items[t].patch];
+                std::vector<int> gl(nbb);
+                for (int j = 0; j < nbb; ++j) {
+                    gl[j] = P->dofs[P->boff[items[t].patch] + layer_function(h, items[t], j)] - P->nfree;
+                    if (gl[j] < 0 || gl[j] >= nfix) return fail(c, GSX_ERR_ARG, "Dirichlet side function not eliminated");
+                }
+                for (int i = 0; i < nbb; ++i) {
+                    for (int j = 0; j < nbb; ++j) rows[gl[i]][gl[j]] += M[(t * nbb + i) * nbb + j];
+                    rhs[gl[i]] += F[t * nbb + i];
+                }
+            }
+        }
+        std::vector<int> rp((size_t)nfix + 1, 0), ci;
+        std::vector<double> va;
+        for (int64_t i = 0; i < nfix; ++i) {
+            for (const auto &kv : rows[i]) {
+                ci.push_back(kv.first);
+                va.push_back(kv.second);
+            }
+            rp[i + 1] = (int)ci.size();
+        }
+        gsx_matrix *Mb = nullptr;
+        GSX_TRY(gsx_matrix_from_csr(c, (int32_t)nfix, rp.data(), ci.data(), va.data(), 0, &Mb));
+        std::unique_ptr<gsx_matrix, gsx_status (*)(gsx_matrix *)> hold(Mb, gsx_matrix_destroy);
+        DevPtr db = dalloc(c, sizeof(double) * nfix), dx = dalloc(c, sizeof(double) * nfix);
+        if (!db || !dx) return fail(c, GSX_ERR_OOM, "device allocation (Dirichlet values)");
+        HIP_OK(c, hipMemcpyAsync(db.get(), rhs.data(), sizeof(double) * nfix, hipMemcpyHostToDevice, s));
+        HIP_OK(c, gsx::launch_fill_f64(dp<double>(dx), nfix, 0.0, s));
+        // gsSparseSolver::CGDiagonal with Eigen's default tolerance (machine
+        // epsilon): Jacobi-CG to rounding level (a boundary mass matrix is
+        // well conditioned: a few dozen iterations)
+        int32_t it = 0;
+        double rel = 0.0;
+        const gsx_status e = gsx_cg_solve(Mb, GSX_PREC_JACOBI, 1.0, nullptr, GSX_FP64, dp<double>(db), dp<double>(dx),
+                                          1e-14, 2000, &it, &rel);
+        if (e != GSX_OK && !(e == GSX_ERR_NOT_CONVERGED && rel < 1e-12)) return e;
+        HIP_OK(c, hipStreamSynchronize(s));
+        P->d_fixed = std::move(dx);
+    }
+    bool any_neu = false;
+    for (const gsx_bc &b : neu) any_neu = any_neu || b.kind != GSX_BCF_ZERO;
+    if (any_neu && P->nfree > 0) {
+        std::vector<double> load((size_t)P->nfree, 0.0);
+        // records in order, side elements in iterator order
+        for (const gsx_bc &b : neu) {
+            std::vector<gsx::BdryItem> items;
+            side_items(P->hp[b.patch], b.patch, b, items);
+            std::vector<double> M, F;
+            int nbb = 0;
+            GSX_TRY(bdry_terms(P, items, M, F, nbb));
+            for (size_t t = 0; t < items.size(); ++t) {
+                const HostPatch &h = P->hp[items[t].patch];
+                for (int j = 0; j < nbb; ++j) {
+                    const int g = P->dofs[P->boff[items[t].patch] + layer_function(h, items[t], j)];
+                    if (g < P->nfree) load[g] += F[t * nbb + j];
+                }
+            }
+        }
+        P->d_neu = dalloc(c, sizeof(double) * P->nfree);
+        if (!P->d_neu) return fail(c, GSX_ERR_OOM, "device allocation (Neumann load)");
+        HIP_OK(c, hipMemcpyAsync(P->d_neu.get(), load.data(), sizeof(double) * P->nfree, hipMemcpyHostToDevice, s));
+        HIP_OK(c, hipStreamSynchronize(s));
+    }
+    return GSX_OK;
+}
+
+extern "C" gsx_status gsx_problem_fixed_values(const gsx_problem *P, double *fixed, int32_t count)
+{
+    if (!P || (count > 0 && !fixed)) return GSX_ERR_ARG;
+    const int64_t nfix = P->nglued - P->nfree;
+    if (count != nfix) return GSX_ERR_ARG;
+    if (!P->d_fixed) {
+        std::fill(fixed, fixed + count, 0.0);
+        return GSX_OK;
+    }
+    gsx_ctx *c = P->ctx;
+    HIP_OK(c, hipMemcpyAsync(fixed, P->d_fixed.get(), sizeof(double) * count, hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(c, hipStreamSynchronize(c->stream));
+    return GSX_OK;
+}
+
+extern "C" gsx_status gsx_assemble_bdr(gsx_problem *P, double *rhs)
+{
+    if (!P || !rhs) return GSX_ERR_ARG;
+    if (!P->d_neu) return GSX_OK;
+    gsx_ctx *c = P->ctx;
+    HIP_OK(c, gsx::launch_axpby(P->nfree, 1.0, dp<double>(P->d_neu), 1.0, rhs, c->stream));
     return GSX_OK;
 }

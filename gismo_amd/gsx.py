@@ -33,6 +33,14 @@ PATTERN_FULL = 1
 ASM_REBUILD_PATTERN = 0x100
 ASM_CSR_ONLY = 0x200
 ASM_KEEP_ELEMENTS = 0x400
+BC_DIRICHLET, BC_NEUMANN = 1, 2
+BCF_ZERO, BCF_MANUFACTURED, BCF_AFFINE = 0, 1, 2
+DIR_HOMOGENEOUS, DIR_L2PROJECTION = 0, 1
+
+
+class GsxBc(C.Structure):
+    _fields_ = [("patch", C.c_int32), ("side", C.c_int32), ("type", C.c_int32), ("kind", C.c_int32),
+                ("value", C.c_double * 4)]
 PREC_NONE = 0
 PREC_JACOBI = 1
 PREC_MG = 2
@@ -108,6 +116,9 @@ SIGNATURES = {
                      _i32p, _dp],
     "gsx_cg_history": [_vp, _dp, C.c_int32, _i32p],
     "gsx_l2_error": [_vp, _vp, _dp],
+    "gsx_problem_set_bc": [_vp, C.c_int32, _vp, C.c_int32],
+    "gsx_problem_fixed_values": [_vp, _dp, C.c_int32],
+    "gsx_assemble_bdr": [_vp, _vp],
     "gsx_error_norms": [_vp, _vp, _dp, _dp],
     "gsx_mg_create": [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
                       C.c_int32, _pp],
@@ -332,6 +343,28 @@ class Problem:
         h = C.c_void_p()
         _check(self.ctx.h, lib().gsx_assemble(self.h, mode, precision, C.byref(h), rhs.ptr))
         return Matrix(self.ctx, h), rhs
+
+    def set_bc(self, bcs, dir_values=1):
+        """Boundary data: bcs = [(patch, side, type, kind, values)] with type
+        BC_DIRICHLET / BC_NEUMANN, kind BCF_ZERO / BCF_MANUFACTURED /
+        BCF_AFFINE (values: c, a_0, a_1, a_2); dir_values DIR_L2PROJECTION
+        (default) or DIR_HOMOGENEOUS."""
+        arr = (GsxBc * max(1, len(bcs)))()
+        for i, (patch, side, typ, kind, vals) in enumerate(bcs):
+            arr[i].patch, arr[i].side, arr[i].type, arr[i].kind = patch, side, typ, kind
+            for k, v in enumerate(list(vals)[:4]):
+                arr[i].value[k] = v
+        _check(self.ctx.h, lib().gsx_problem_set_bc(self.h, len(bcs), C.cast(arr, C.c_void_p), dir_values))
+
+    def fixed_values(self):
+        n = self.nglued - self.nfree
+        out = np.zeros(max(1, n))
+        _check(self.ctx.h, lib().gsx_problem_fixed_values(self.h, out.ctypes.data_as(_dp), n))
+        return out[:n]
+
+    def assemble_bdr(self, rhs):
+        """rhs += Neumann boundary load (gsExprAssembler::assembleBdr)."""
+        _check(self.ctx.h, lib().gsx_assemble_bdr(self.h, rhs.ptr))
 
     def l2_error(self, sol):
         v = C.c_double()

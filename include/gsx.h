@@ -148,6 +148,50 @@ gsx_status gsx_problem_info(const gsx_problem *prob, int64_t *nbasis,
  * patches (gsDofMapper::index, src/gsCore/gsDofMapper.h:325). */
 gsx_status gsx_problem_dofs(const gsx_problem *prob, int32_t *dofs_host);
 
+/* ---- boundary data ------------------------------------------------------ */
+/* Boundary condition of one patch side (gsBoundaryConditions::
+ * addCondition, src/gsPde/gsBoundaryConditions.h).  type: Dirichlet sides
+ * must be among the problem's dir_sides (they fix the DOF map); a Dirichlet
+ * side without a record is homogeneous.  Neumann sides are boundary sides
+ * that are not Dirichlet.  kind and value:
+ *   GSX_BCF_ZERO          g = 0;
+ *   GSX_BCF_MANUFACTURED  Dirichlet g = prod_r sin(pi x_r); Neumann flux
+ *                         grad g (integrand N_i grad g . nv(G));
+ *   GSX_BCF_AFFINE        Dirichlet g = value[0] + sum_r value[1+r] x_r;
+ *                         Neumann flux (value[1], value[2], value[3]).
+ * x are physical coordinates. */
+typedef enum { GSX_BC_DIRICHLET = 1, GSX_BC_NEUMANN = 2 } gsx_bc_type;
+typedef enum { GSX_BCF_ZERO = 0, GSX_BCF_MANUFACTURED = 1, GSX_BCF_AFFINE = 2 } gsx_bc_kind;
+typedef struct {
+    int32_t patch;     /* 0-based */
+    int32_t side;      /* 1..2d: west, east, south, north, front, back */
+    int32_t type;      /* gsx_bc_type */
+    int32_t kind;      /* gsx_bc_kind */
+    double value[4];
+} gsx_bc;
+/* Values of the eliminated Dirichlet DOFs (gsFeSpace::setup(bc,
+ * dir_values, 0), src/gsAssembler/gsDirichletValues.h:27-60):
+ * GSX_DIR_HOMOGENEOUS zero, GSX_DIR_L2PROJECTION the L2 projection of g onto
+ * the Dirichlet sides' functions (gsDirichletValuesByL2Projection,
+ * gsDirichletValues.h:257-435: p+1 Gauss points along the side, weights
+ * w |det J|, solved to rounding level).  Neumann records add the boundary
+ * load of gsExprAssembler::assembleBdr(bc.get("Neumann"), u * g_N.tr() *
+ * nv(G)) (src/gsAssembler/gsExprAssembler.h:835-895) through
+ * gsx_assemble_bdr; every later gsx_assemble moves the fixed columns to the
+ * right-hand side (rhs_i -= K_ij g_j, gsExprAssembler.h:553-630).  Replaces
+ * any earlier boundary data of the problem. */
+typedef enum { GSX_DIR_HOMOGENEOUS = 0, GSX_DIR_L2PROJECTION = 1 } gsx_dir_values;
+gsx_status gsx_problem_set_bc(gsx_problem *prob, int32_t nbc, const gsx_bc *bc,
+                              int32_t dir_values);
+/* Host copy of the fixed (eliminated) DOF values, in the order of their
+ * global index minus nfree (gsFeSpace::fixedPart); count = nglued - nfree. */
+gsx_status gsx_problem_fixed_values(const gsx_problem *prob, double *fixed_host,
+                                    int32_t count);
+/* rhs_dev (nfree, device) += the Neumann boundary load of the problem's
+ * boundary data: gsExprAssembler::assembleBdr(bc.get("Neumann"),
+ * u * g_N.tr() * nv(G)) (src/gsAssembler/gsExprAssembler.h:835-895). */
+gsx_status gsx_assemble_bdr(gsx_problem *prob, double *rhs_dev);
+
 /* ---- assembly ---------------------------------------------------------- */
 /* Replaces gsExprAssembler<T>::assemble(igrad(u,G)*igrad(u,G).tr()*meas(G)
  * [*kappa], u*ff*meas(G)) (src/gsAssembler/gsExprAssembler.h:754-833) and
@@ -295,8 +339,9 @@ gsx_status gsx_batch_history(const gsx_batch *batch, int32_t id, double *hist_ho
 gsx_status gsx_batch_remove(gsx_batch *batch, int32_t id);
 
 /* ---- reporting ---------------------------------------------------------- */
-/* L2 error of the discrete solution (free coefficients, eliminated = 0)
- * against u = prod_r sin(pi x_r) with the assembly quadrature
+/* L2 error of the discrete solution (free coefficients, eliminated ones at
+ * the problem's fixed values) against u = prod_r sin(pi x_r) with the
+ * assembly quadrature
  * (gsExprEvaluator::integral, src/gsAssembler/gsExprEvaluator.h:382-450). */
 gsx_status gsx_l2_error(gsx_problem *prob, const double *sol_dev,
                         double *err);

@@ -85,6 +85,17 @@ def lib():
         L.oc_mg_matrix.argtypes = [C.c_void_p, C.c_int]
         L.oc_mg_matrix.restype = C.POINTER(OcCsr)
         L.oc_mg_free.argtypes = [C.c_void_p]
+        L.oc_assemble_bc.argtypes = [C.c_int, C.POINTER(OcPatch), _i32p, C.c_int, C.c_int, C.c_void_p, _f64p]
+        L.oc_assemble_bc.restype = C.POINTER(OcCsr)
+        L.oc_dirichlet_l2.argtypes = [C.c_int, C.POINTER(OcPatch), _i32p, C.c_int, C.c_int, C.c_int,
+                                      C.c_void_p, _f64p]
+        L.oc_dirichlet_l2.restype = C.c_int
+        L.oc_neumann.argtypes = [C.c_int, C.POINTER(OcPatch), _i32p, C.c_int, C.c_int, C.c_void_p, _f64p]
+        L.oc_neumann.restype = None
+        L.oc_l2_error_bc.argtypes = [C.c_int, C.POINTER(OcPatch), _i32p, C.c_int, _f64p, C.c_void_p]
+        L.oc_l2_error_bc.restype = C.c_double
+        L.oc_h1_error_bc.argtypes = [C.c_int, C.POINTER(OcPatch), _i32p, C.c_int, _f64p, C.c_void_p]
+        L.oc_h1_error_bc.restype = C.c_double
         L.oc_l2_error.argtypes = [C.c_int, C.POINTER(OcPatch), _i32p, C.c_int, _f64p]
         L.oc_l2_error.restype = C.c_double
         L.oc_h1_error.argtypes = [C.c_int, C.POINTER(OcPatch), _i32p, C.c_int, _f64p]
@@ -212,13 +223,53 @@ def _csr_to_scipy(ptr, free=True):
     return M
 
 
-def assemble(patch_list, dofs, nfree, mode=0):
-    """Returns (scipy CSR matrix with sorted indices, rhs)."""
+def assemble(patch_list, dofs, nfree, mode=0, fixed=None):
+    """Returns (scipy CSR matrix with sorted indices, rhs); `fixed` (values
+    of the eliminated DOFs) moves K_ij g_j to the rhs."""
     P = Patches(patch_list)
     rhs = np.zeros(nfree)
     dofs = np.ascontiguousarray(dofs, dtype=np.int32)
-    ptr = lib().oc_assemble(P.n, P.arr, dofs, nfree, mode, rhs)
+    if fixed is None:
+        ptr = lib().oc_assemble(P.n, P.arr, dofs, nfree, mode, rhs)
+    else:
+        fx = np.ascontiguousarray(fixed, np.float64)
+        ptr = lib().oc_assemble_bc(P.n, P.arr, dofs, nfree, mode, fx.ctypes.data_as(C.c_void_p), rhs)
     return _csr_to_scipy(ptr), rhs
+
+
+class OcBc(C.Structure):
+    _fields_ = [("patch", C.c_int), ("side", C.c_int), ("type", C.c_int), ("kind", C.c_int),
+                ("value", C.c_double * 4)]
+
+
+def _bc_array(bcs):
+    arr = (OcBc * max(1, len(bcs)))()
+    for i, (patch, side, typ, kind, vals) in enumerate(bcs):
+        arr[i].patch, arr[i].side, arr[i].type, arr[i].kind = patch, side, typ, kind
+        for k, v in enumerate(list(vals)[:4]):
+            arr[i].value[k] = v
+    return arr
+
+
+def dirichlet_values(patch_list, dofs, nfree, nfix, bcs):
+    """Fixed values (nfix) by the L2 projection over the Dirichlet records
+    bcs = [(patch, side, 1, kind, values), ...] (every Dirichlet side)."""
+    P = Patches(patch_list)
+    out = np.zeros(max(1, nfix))
+    arr = _bc_array(bcs)
+    lib().oc_dirichlet_l2(P.n, P.arr, np.ascontiguousarray(dofs, np.int32), nfree, nfix, len(bcs),
+                          C.cast(arr, C.c_void_p), out)
+    return out[:nfix]
+
+
+def neumann_load(patch_list, dofs, nfree, bcs):
+    """Neumann boundary load (nfree) of the records with type 2."""
+    P = Patches(patch_list)
+    out = np.zeros(max(1, nfree))
+    arr = _bc_array(bcs)
+    lib().oc_neumann(P.n, P.arr, np.ascontiguousarray(dofs, np.int32), nfree, len(bcs),
+                     C.cast(arr, C.c_void_p), out)
+    return out[:nfree]
 
 
 def assemble_rows(patch, dofs, nfree, rows, maxc=1024):
@@ -325,14 +376,23 @@ class Multigrid:
             pass
 
 
-def l2_error(patch_list, dofs, nfree, sol):
+def _fixed_ptr(fixed):
+    if fixed is None:
+        return None, None
+    fx = np.ascontiguousarray(fixed, np.float64)
+    return fx, fx.ctypes.data_as(C.c_void_p)
+
+
+def l2_error(patch_list, dofs, nfree, sol, fixed=None):
     P = Patches(patch_list)
-    return lib().oc_l2_error(P.n, P.arr, np.ascontiguousarray(dofs, np.int32), nfree,
-                             np.ascontiguousarray(sol, np.float64))
+    fx, fp = _fixed_ptr(fixed)
+    return lib().oc_l2_error_bc(P.n, P.arr, np.ascontiguousarray(dofs, np.int32), nfree,
+                                np.ascontiguousarray(sol, np.float64), fp)
 
 
-def h1_error(patch_list, dofs, nfree, sol):
+def h1_error(patch_list, dofs, nfree, sol, fixed=None):
     """H1 seminorm error of the discrete solution against prod sin(pi x_r)."""
     P = Patches(patch_list)
-    return lib().oc_h1_error(P.n, P.arr, np.ascontiguousarray(dofs, np.int32), nfree,
-                             np.ascontiguousarray(sol, np.float64))
+    fx, fp = _fixed_ptr(fixed)
+    return lib().oc_h1_error_bc(P.n, P.arr, np.ascontiguousarray(dofs, np.int32), nfree,
+                                np.ascontiguousarray(sol, np.float64), fp)

@@ -648,6 +648,12 @@ static double source_f(const oc_patch *P, int elem, const double *x)
 oc_csr *oc_assemble(int npatch, const oc_patch *patches, const int *dofs,
                     int nfree, int mode, double *rhs_out)
 {
+    return oc_assemble_bc(npatch, patches, dofs, nfree, mode, NULL, rhs_out);
+}
+
+oc_csr *oc_assemble_bc(int npatch, const oc_patch *patches, const int *dofs,
+                       int nfree, int mode, const double *fixed, double *rhs_out)
+{
     orow *rows = (orow *)calloc(nfree, sizeof(orow));
     for (int i = 0; i < nfree; ++i) rhs_out[i] = 0.0;
     elemdata *D = ed_get();
@@ -704,8 +710,9 @@ oc_csr *oc_assemble(int npatch, const oc_patch *patches, const int *dofs,
                     if (mode == 0 && v == 0.0) continue;
                     const int jj = dofs[off + D->act[j]];
                     if (jj < nfree) orow_add(&rows[ii], jj, v);
-                    /* eliminated columns: homogeneous Dirichlet values, the
-                     * rhs update -= K*0 leaves the rhs unchanged */
+                    /* eliminated column: rhs_i -= K_ij g_j (:613-619);
+                     * homogeneous values (fixed NULL) leave the rhs as is */
+                    else if (fixed && v != 0.0) rhs_out[ii] -= v * fixed[jj - nfree];
                 }
                 rhs_out[ii] += lr[i];
             }
@@ -1270,6 +1277,12 @@ void oc_mg_free(oc_mg *mg)
 double oc_h1_error(int npatch, const oc_patch *patches, const int *dofs,
                    int nfree, const double *sol)
 {
+    return oc_h1_error_bc(npatch, patches, dofs, nfree, sol, NULL);
+}
+
+double oc_h1_error_bc(int npatch, const oc_patch *patches, const int *dofs,
+                      int nfree, const double *sol, const double *fixed)
+{
     const double pi = 3.14159265358979323846;
     elemdata *D = ed_get();
     double total = 0.0;
@@ -1291,8 +1304,9 @@ double oc_h1_error(int npatch, const oc_patch *patches, const int *dofs,
                 double gp[3] = {0, 0, 0};   /* parametric gradient of u_h */
                 for (int b = 0; b < D->nb; ++b) {
                     const int g = dofs[off + D->act[b]];
-                    if (g >= nfree) continue;
-                    for (int a = 0; a < d; ++a) gp[a] += sol[g] * D->dN[(b * d + a) * D->nq + q];
+                    if (g >= nfree && !fixed) continue;
+                    const double cb = g < nfree ? sol[g] : fixed[g - nfree];
+                    for (int a = 0; a < d; ++a) gp[a] += cb * D->dN[(b * d + a) * D->nq + q];
                 }
                 const double *Ji = D->Jinv + q * 9;
                 double diff2 = 0.0;
@@ -1317,6 +1331,12 @@ double oc_h1_error(int npatch, const oc_patch *patches, const int *dofs,
 double oc_l2_error(int npatch, const oc_patch *patches, const int *dofs,
                    int nfree, const double *sol)
 {
+    return oc_l2_error_bc(npatch, patches, dofs, nfree, sol, NULL);
+}
+
+double oc_l2_error_bc(int npatch, const oc_patch *patches, const int *dofs,
+                      int nfree, const double *sol, const double *fixed)
+{
     const double pi = 3.14159265358979323846;
     elemdata *D = ed_get();
     double total = 0.0;
@@ -1338,6 +1358,7 @@ double oc_l2_error(int npatch, const oc_patch *patches, const int *dofs,
                 for (int b = 0; b < D->nb; ++b) {
                     const int g = dofs[off + D->act[b]];
                     if (g < nfree) uh += sol[g] * D->N[b * D->nq + q];
+                    else if (fixed) uh += fixed[g - nfree] * D->N[b * D->nq + q];
                 }
                 double ue = 1.0;
                 for (int r = 0; r < P->d; ++r) ue *= sin(pi * D->x[q * 3 + r]);
@@ -1350,4 +1371,294 @@ double oc_l2_error(int npatch, const oc_patch *patches, const int *dofs,
         off += nbas;
     }
     return sqrt(total);
+}
+
+
+/* ------------------------------------------------------------------------
+ * Boundary data (gsBoundaryConditions with the poisson2_example call
+ * sequence).  Quadrature on one side element: gsGaussRule(basis, 1, 1, dir)
+ * (src/gsAssembler/gsGaussRule.hpp:23-60): p+1 Gauss nodes per tangential
+ * direction, one node on the side in direction dir whose weight 2 is scaled
+ * by 0.5 for the zero-length interval in gsQuadRule::mapTo
+ * (src/gsAssembler/gsQuadRule.h:177-201).  Points lexicographic, direction 0
+ * fastest.  Side elements in the order of the side's domain iterator
+ * (lexicographic over the tangential directions).
+ * ---------------------------------------------------------------------- */
+static double bc_g(const oc_bc *b, int d, const double *x)
+{
+    if (b->kind == 1) {
+        const double pi = 3.14159265358979323846;
+        double v = 1.0;
+        for (int r = 0; r < d; ++r) v *= sin(pi * x[r]);
+        return v;
+    }
+    if (b->kind == 2) {
+        double v = b->value[0];
+        for (int r = 0; r < d; ++r) v += b->value[1 + r] * x[r];
+        return v;
+    }
+    return 0.0;
+}
+
+static void bc_grad(const oc_bc *b, int d, const double *x, double *f)
+{
+    const double pi = 3.14159265358979323846;
+    for (int c = 0; c < d; ++c) {
+        if (b->kind == 1) {
+            double v = pi * cos(pi * x[c]);
+            for (int r = 0; r < d; ++r)
+                if (r != c) v *= sin(pi * x[r]);
+            f[c] = v;
+        } else if (b->kind == 2) {
+            f[c] = b->value[1 + c];
+        } else {
+            f[c] = 0.0;
+        }
+    }
+}
+
+typedef struct {
+    int nb, nq;
+    int act[512];
+    double N[512 * 32];     /* [b][q] */
+    double x[32 * 3], J[32 * 9], det[32], w[32];
+} bdata;
+
+static void bdry_eval(const oc_patch *P, elem1d *E, int dir, int par, const int *ei, bdata *B)
+{
+    const int d = P->d;
+    double v1[3][32][P1MAX], d1[3][32][P1MAX], wq1[3][32];
+    int p1[3] = {1, 1, 1}, nq1[3] = {1, 1, 1};
+    B->nb = 1;
+    B->nq = 1;
+    for (int k = 0; k < d; ++k) {
+        const int p = P->p[k];
+        const double a = E[k].lo[ei[k]], b = E[k].hi[ei[k]];
+        p1[k] = p + 1;
+        if (k == dir) {
+            nq1[k] = 1;
+            double tmp[2 * P1MAX];
+            oc_bspline_ders(P->knots[k], p, E[k].span[ei[k]], par ? b : a, 1, tmp);
+            for (int j = 0; j <= p; ++j) {
+                v1[k][0][j] = tmp[j];
+                d1[k][0][j] = tmp[(p + 1) + j];
+            }
+            wq1[k][0] = 0.5 * 2.0;
+        } else {
+            const int nq = nodes_per_dir(p);
+            const double *gx, *gw;
+            gauss_get(nq, &gx, &gw);
+            const double h = (b - a) / 2.0;
+            nq1[k] = nq;
+            for (int q = 0; q < nq; ++q) {
+                const double u = h * (gx[q] + 1.0) + a;
+                wq1[k][q] = (0.0 == h ? 0.5 : h) * gw[q];
+                double tmp[2 * P1MAX];
+                oc_bspline_ders(P->knots[k], p, E[k].span[ei[k]], u, 1, tmp);
+                for (int j = 0; j <= p; ++j) {
+                    v1[k][q][j] = tmp[j];
+                    d1[k][q][j] = tmp[(p + 1) + j];
+                }
+            }
+        }
+        B->nb *= p1[k];
+        B->nq *= nq1[k];
+    }
+    int nbd[3] = {1, 1, 1};
+    for (int k = 0; k < d; ++k) nbd[k] = oc_patch_nbasis(P, k);
+    for (int bb = 0; bb < B->nb; ++bb) {
+        int r = bb, g = 0, str = 1;
+        for (int k = 0; k < d; ++k) {
+            const int bk = r % p1[k];
+            r /= p1[k];
+            g += (E[k].span[ei[k]] - P->p[k] + bk) * str;
+            str *= nbd[k];
+        }
+        B->act[bb] = g;
+    }
+    for (int q = 0; q < B->nq; ++q) {
+        int qi[3] = {0, 0, 0}, r = q;
+        for (int k = 0; k < d; ++k) { qi[k] = r % nq1[k]; r /= nq1[k]; }
+        double wt = wq1[0][qi[0]];
+        for (int k = 1; k < d; ++k) wt *= wq1[k][qi[k]];
+        B->w[q] = wt;
+        double xx[3] = {0, 0, 0}, J[3][3] = {{0}};
+        for (int bb = 0; bb < B->nb; ++bb) {
+            int bi[3] = {0, 0, 0}, rb = bb;
+            for (int k = 0; k < d; ++k) { bi[k] = rb % p1[k]; rb /= p1[k]; }
+            double v = v1[0][qi[0]][bi[0]];
+            for (int k = 1; k < d; ++k) v *= v1[k][qi[k]][bi[k]];
+            B->N[bb * 32 + q] = v;
+            const double *c = P->ctrl + (size_t)B->act[bb] * d;
+            for (int kk = 0; kk < d; ++kk) {
+                double g = d1[kk][qi[kk]][bi[kk]];
+                for (int i = 0; i < d; ++i)
+                    if (i != kk) g *= v1[i][qi[i]][bi[i]];
+                for (int rr = 0; rr < d; ++rr) J[rr][kk] += c[rr] * g;
+            }
+            for (int rr = 0; rr < d; ++rr) xx[rr] += c[rr] * v;
+        }
+        double det;
+        if (d == 2) det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        else det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) +
+                   J[0][1] * (J[1][2] * J[2][0] - J[1][0] * J[2][2]) +
+                   J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        B->det[q] = det;
+        for (int rr = 0; rr < d; ++rr) {
+            B->x[q * 3 + rr] = xx[rr];
+            for (int a = 0; a < d; ++a) B->J[q * 9 + rr * 3 + a] = J[rr][a];
+        }
+    }
+}
+
+/* visit the elements along `side` of patch P in domain-iterator order */
+typedef void (*side_visit)(const oc_patch *P, elem1d *E, int dir, int par, const int *ei, void *ctx);
+static void for_side_elements(const oc_patch *P, int side, side_visit f, void *ctx)
+{
+    const int d = P->d, dir = (side - 1) / 2, par = (side - 1) % 2;
+    elem1d E[3];
+    int cnt = 1;
+    for (int k = 0; k < d; ++k) {
+        elem1d_build(P, k, &E[k]);
+        if (k != dir) cnt *= E[k].n;
+    }
+    for (int t = 0; t < cnt; ++t) {
+        int ei[3] = {0, 0, 0}, r = t;
+        for (int k = 0; k < d; ++k) {
+            if (k == dir) { ei[k] = par ? E[k].n - 1 : 0; continue; }
+            ei[k] = r % E[k].n;
+            r /= E[k].n;
+        }
+        f(P, E, dir, par, ei, ctx);
+    }
+    for (int k = 0; k < d; ++k) elem1d_free(&E[k]);
+}
+
+typedef struct {
+    const oc_bc *bc;
+    const int *dofs;     /* of this patch (offset applied) */
+    int nfree;
+    orow *rows;          /* boundary-index rows (Dirichlet) */
+    double *rhs;
+} bctx;
+
+static bdata *g_bd = NULL;
+
+/* gsDirichletValuesByL2Projection (src/gsAssembler/gsDirichletValues.h:
+ * 257-435): weight_k = w_k * measure_k with the map data's measure |det J|
+ * (computeMap without a side, src/gsCore/gsFunction.hpp:702-716); entries
+ * (weight N_i) N_j and rhs (weight N_i) g over the eliminated actives. */
+static void dir_visit(const oc_patch *P, elem1d *E, int dir, int par, const int *ei, void *vctx)
+{
+    bctx *C = (bctx *)vctx;
+    bdata *B = g_bd;
+    bdry_eval(P, E, dir, par, ei, B);
+    for (int q = 0; q < B->nq; ++q) {
+        const double wk = B->w[q] * fabs(B->det[q]);
+        const double g = bc_g(C->bc, P->d, B->x + q * 3);
+        for (int i = 0; i < B->nb; ++i) {
+            const int ii = C->dofs[B->act[i]];
+            if (ii < C->nfree) continue;
+            for (int j = 0; j < B->nb; ++j) {
+                const int jj = C->dofs[B->act[j]];
+                if (jj < C->nfree) continue;
+                orow_add(&C->rows[ii - C->nfree], jj - C->nfree, wk * B->N[i * 32 + q] * B->N[j * 32 + q]);
+            }
+            C->rhs[ii - C->nfree] += wk * B->N[i * 32 + q] * g;
+        }
+    }
+}
+
+int oc_dirichlet_l2(int npatch, const oc_patch *patches, const int *dofs, int nfree, int nfix,
+                    int nbc, const oc_bc *bcs, double *fixed_out)
+{
+    if (!g_bd) g_bd = (bdata *)malloc(sizeof(bdata));
+    orow *rows = (orow *)calloc(nfix > 0 ? nfix : 1, sizeof(orow));
+    double *rhs = (double *)calloc(nfix > 0 ? nfix : 1, sizeof(double));
+    int *off = (int *)calloc(npatch + 1, sizeof(int));
+    for (int k = 0; k < npatch; ++k) {
+        int nb = 1;
+        for (int r = 0; r < patches[k].d; ++r) nb *= oc_patch_nbasis(&patches[k], r);
+        off[k + 1] = off[k] + nb;
+    }
+    for (int b = 0; b < nbc; ++b) {
+        if (bcs[b].type != 1) continue;
+        bctx C = {&bcs[b], dofs + off[bcs[b].patch], nfree, rows, rhs};
+        for_side_elements(&patches[bcs[b].patch], bcs[b].side, dir_visit, &C);
+    }
+    oc_csr *M = rows_to_csr(rows, nfix);
+    free(rows);
+    for (int i = 0; i < nfix; ++i) fixed_out[i] = 0.0;
+    double err = 0.0;
+    /* gsSparseSolver::CGDiagonal with Eigen's default tolerance (machine
+     * epsilon): Jacobi-preconditioned CG to rounding level */
+    oc_cg_solve(M, rhs, fixed_out, 1e-15, 2 * nfix + 100, 1, 1.0, NULL, &err, NULL);
+    oc_csr_free(M);
+    free(rhs);
+    free(off);
+    return 0;
+}
+
+/* gsExprAssembler::assembleBdr(bc.get("Neumann"), u * g_N.tr() * nv(G))
+ * (src/gsAssembler/gsExprAssembler.h:835-895): per point N_i (g_N . nv),
+ * nv the outer normal of gsFunction::computeMap (src/gsCore/gsFunction.hpp:
+ * 613-680), quadrature lm = sum_k w_k eval(k), pushed into the free rows. */
+static void neu_visit(const oc_patch *P, elem1d *E, int dir, int par, const int *ei, void *vctx)
+{
+    bctx *C = (bctx *)vctx;
+    bdata *B = g_bd;
+    const int d = P->d;
+    bdry_eval(P, E, dir, par, ei, B);
+    double dsg = 0.0;
+    for (int q = 0; q < B->nq; ++q)
+        if (fabs(B->det[q]) >= 1e-7) { dsg = B->det[q] < 0 ? -1.0 : 1.0; break; }
+    if (dsg == 0.0) dsg = 1.0;
+    const int s = C->bc->side;
+    const double so = ((s + (s + 1) / 2) % 2) ? 1.0 : -1.0;
+    static double lr[512];
+    for (int i = 0; i < B->nb; ++i) lr[i] = 0.0;
+    for (int q = 0; q < B->nq; ++q) {
+        const double *J = B->J + q * 9;   /* J[r][a] = dx_r/dxi_a; jacT[a][r] = J[r][a] */
+        double nv[3] = {0, 0, 0}, alt = so * dsg;
+        for (int c = 0; c < d; ++c) {
+            double m;
+            if (d == 2) {
+                m = J[(1 - c) * 3 + (1 - dir)];
+            } else {
+                int ra[2], cb[2], na = 0, nb = 0;
+                for (int a = 0; a < 3; ++a) if (a != dir) ra[na++] = a;
+                for (int b = 0; b < 3; ++b) if (b != c) cb[nb++] = b;
+                m = J[cb[0] * 3 + ra[0]] * J[cb[1] * 3 + ra[1]] - J[cb[1] * 3 + ra[0]] * J[cb[0] * 3 + ra[1]];
+            }
+            nv[c] = alt * m;
+            alt = -alt;
+        }
+        double f[3];
+        bc_grad(C->bc, d, B->x + q * 3, f);
+        double gn = 0.0;
+        for (int c = 0; c < d; ++c) gn += f[c] * nv[c];
+        for (int i = 0; i < B->nb; ++i) lr[i] += B->w[q] * (B->N[i * 32 + q] * gn);
+    }
+    for (int i = 0; i < B->nb; ++i) {
+        const int ii = C->dofs[B->act[i]];
+        if (ii < C->nfree) C->rhs[ii] += lr[i];
+    }
+}
+
+void oc_neumann(int npatch, const oc_patch *patches, const int *dofs, int nfree, int nbc,
+                const oc_bc *bcs, double *rhs)
+{
+    if (!g_bd) g_bd = (bdata *)malloc(sizeof(bdata));
+    int *off = (int *)calloc(npatch + 1, sizeof(int));
+    for (int k = 0; k < npatch; ++k) {
+        int nb = 1;
+        for (int r = 0; r < patches[k].d; ++r) nb *= oc_patch_nbasis(&patches[k], r);
+        off[k + 1] = off[k] + nb;
+    }
+    for (int b = 0; b < nbc; ++b) {
+        if (bcs[b].type != 2) continue;
+        bctx C = {&bcs[b], dofs + off[bcs[b].patch], nfree, NULL, rhs};
+        for_side_elements(&patches[bcs[b].patch], bcs[b].side, neu_visit, &C);
+    }
+    free(off);
 }

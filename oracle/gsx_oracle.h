@@ -147,6 +147,22 @@ oc_csr *oc_transfer_multipatch(int npatch, const oc_patch *coarse,
                                const int *dofs_c, int nfree_c,
                                const int *dofs_f, int nfree_f);
 
+/* --- boundary data ------------------------------------------------------- */
+/* type 1 Dirichlet, 2 Neumann; kind 0 zero, 1 manufactured (g = prod
+ * sin(pi x_r), Neumann flux grad g), 2 affine (g = value[0] + value[1+r]
+ * x_r, flux (value[1], value[2], value[3])). */
+typedef struct { int patch, side, type, kind; double value[4]; } oc_bc;
+/* Fixed values (nfix = eliminated DOFs) by the L2 projection over the
+ * Dirichlet records (every Dirichlet side, homogeneous ones with kind 0). */
+int  oc_dirichlet_l2(int npatch, const oc_patch *patches, const int *dofs, int nfree, int nfix,
+                     int nbc, const oc_bc *bcs, double *fixed_out);
+/* rhs (nfree) += Neumann boundary load of the Neumann records. */
+void oc_neumann(int npatch, const oc_patch *patches, const int *dofs, int nfree, int nbc,
+                const oc_bc *bcs, double *rhs);
+/* oc_assemble with the fixed values moved to the rhs (fixed NULL: zero). */
+oc_csr *oc_assemble_bc(int npatch, const oc_patch *patches, const int *dofs,
+                       int nfree, int mode, const double *fixed, double *rhs_out);
+
 /* --- reporting ---------------------------------------------------------- */
 /* L2 error of the discrete solution (free coefficients, eliminated = 0)
  * against u = prod sin(pi x_r), with the assembly quadrature. */
@@ -155,6 +171,11 @@ double oc_l2_error(int npatch, const oc_patch *patches, const int *dofs,
 /* H1 seminorm error against the same u (examples/poisson2_example.cpp:173-176). */
 double oc_h1_error(int npatch, const oc_patch *patches, const int *dofs,
                    int nfree, const double *sol);
+/* both with the eliminated coefficients at `fixed` (NULL: zero) */
+double oc_l2_error_bc(int npatch, const oc_patch *patches, const int *dofs,
+                      int nfree, const double *sol, const double *fixed);
+double oc_h1_error_bc(int npatch, const oc_patch *patches, const int *dofs,
+                      int nfree, const double *sol, const double *fixed);
 
 #ifdef __cplusplus
 }
