@@ -2951,7 +2951,8 @@ extern "C" gsx_status gsx_problem_set_bc(gsx_problem *P, int32_t nbc, const gsx_
     if (!P || nbc < 0 || (nbc > 0 && !bc)) return GSX_ERR_ARG;
     gsx_ctx *c = P->ctx;
     hipStream_t s = c->stream;
-    if (dir_values != GSX_DIR_HOMOGENEOUS && dir_values != GSX_DIR_L2PROJECTION)
+    if (dir_values != GSX_DIR_HOMOGENEOUS && dir_values != GSX_DIR_L2PROJECTION &&
+        dir_values != GSX_DIR_L2PROJECTION_INHOM)
         return fail(c, GSX_ERR_ARG, "unknown Dirichlet value method");
     const int d = P->d;
     std::vector<std::pair<int, int>> dset, iset;
@@ -2999,7 +3000,7 @@ extern "C" gsx_status gsx_problem_set_bc(gsx_problem *P, int32_t nbc, const gsx_
     const int64_t nfix = P->nglued - P->nfree;
     bool inhom = false;
     for (const gsx_bc &b : dir) inhom = inhom || b.kind != GSX_BCF_ZERO;
-    if (dir_values == GSX_DIR_L2PROJECTION && inhom && nfix > 0) {
+    if (dir_values != GSX_DIR_HOMOGENEOUS && inhom && nfix > 0) {
         // boundary mass matrix and load over every Dirichlet side
         // (gsDirichletValuesByL2Projection, src/gsAssembler/gsDirichletValues.h:257-435)
         std::vector<std::map<int, double>> rows((size_t)nfix);
@@ -3007,7 +3008,8 @@ extern "C" gsx_status gsx_problem_set_bc(gsx_problem *P, int32_t nbc, const gsx_
         for (const Group &g : P->groups) {
             std::vector<gsx::BdryItem> items;
             for (const gsx_bc &b : dir)
-                if (std::find(g.patches.begin(), g.patches.end(), b.patch) != g.patches.end())
+                if (std::find(g.patches.begin(), g.patches.end(), b.patch) != g.patches.end() &&
+                    !(dir_values == GSX_DIR_L2PROJECTION_INHOM && b.kind == GSX_BCF_ZERO))
                     side_items(P->hp[b.patch], b.patch, b, items);
             if (items.empty()) continue;
             std::vector<double> M, F;
@@ -3028,6 +3030,10 @@ extern "C" gsx_status gsx_problem_set_bc(gsx_problem *P, int32_t nbc, const gsx_
         }
         std::vector<int> rp((size_t)nfix + 1, 0), ci;
         std::vector<double> va;
+        // functions on no projected side (homogeneous sides skipped): an
+        // identity row keeps their value 0
+        for (int64_t i = 0; i < nfix; ++i)
+            if (rows[i].empty()) rows[i][(int)i] = 1.0;
         for (int64_t i = 0; i < nfix; ++i) {
             for (const auto &kv : rows[i]) {
                 ci.push_back(kv.first);

@@ -35,7 +35,7 @@ ASM_CSR_ONLY = 0x200
 ASM_KEEP_ELEMENTS = 0x400
 BC_DIRICHLET, BC_NEUMANN = 1, 2
 BCF_ZERO, BCF_MANUFACTURED, BCF_AFFINE = 0, 1, 2
-DIR_HOMOGENEOUS, DIR_L2PROJECTION = 0, 1
+DIR_HOMOGENEOUS, DIR_L2PROJECTION, DIR_L2PROJECTION_INHOM = 0, 1, 2
 
 
 class GsxBc(C.Structure):
@@ -348,7 +348,9 @@ class Problem:
         """Boundary data: bcs = [(patch, side, type, kind, values)] with type
         BC_DIRICHLET / BC_NEUMANN, kind BCF_ZERO / BCF_MANUFACTURED /
         BCF_AFFINE (values: c, a_0, a_1, a_2); dir_values DIR_L2PROJECTION
-        (default) or DIR_HOMOGENEOUS."""
+        (default: every Dirichlet side projected, gsFeSpace::setup),
+        DIR_L2PROJECTION_INHOM (sides with nonzero data only,
+        gsAssembler::computeDirichletDofs) or DIR_HOMOGENEOUS."""
         arr = (GsxBc * max(1, len(bcs)))()
         for i, (patch, side, typ, kind, vals) in enumerate(bcs):
             arr[i].patch, arr[i].side, arr[i].type, arr[i].kind = patch, side, typ, kind

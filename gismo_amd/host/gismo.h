@@ -38,6 +38,7 @@
 #include <memory>
 #include <sstream>
 #include <stdexcept>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -223,7 +224,9 @@ namespace condition_type {
 enum type { dirichlet = 0, neumann = 1, robin = 2 };
 }
 namespace dirichlet {
-enum strategy { elimination = 11, nitsche = 12, interpolation = 1, l2Projection = 2, homogeneous = 3 };
+// values of gsAssemblerOptions.h:26-45
+enum strategy { elimination = 11, nitsche = 12 };
+enum values { homogeneous = 100, interpolation = 101, l2Projection = 102, user = 103 };
 }
 namespace iFace {
 enum strategy { glue = 1, dg = 2, conforming = 1 };
@@ -558,26 +561,105 @@ template <class T = real_t>
 class gsFunctionExpr {
 public:
     gsFunctionExpr() = default;
-    /// accepts "0" and the manufactured sources "2*pi^2*sin(pi*x)*sin(pi*y)"
-    /// and "3*pi^2*sin(pi*x)*sin(pi*y)*sin(pi*z)" (blanks ignored)
+    /// Scalar functions evaluated on the device: "0"; the manufactured
+    /// solution prod sin(pi x_i) and its source d*pi^2*prod sin(pi x_i); an
+    /// affine function c + a*x + b*y (+ e*z) (blanks ignored)
     gsFunctionExpr(const std::string &expr, short_t dim) : m_dim(dim)
     {
-        std::string e;
-        for (char ch : expr)
-            if (ch != ' ') e += ch;
+        const std::string e = strip(expr);
         if (e == "0") { m_kind = 2; return; }
         const std::string u2 = "sin(pi*x)*sin(pi*y)", u3 = "sin(pi*x)*sin(pi*y)*sin(pi*z)";
         if ((dim == 2 && e == "2*pi^2*" + u2) || (dim == 3 && e == "3*pi^2*" + u3)) { m_kind = 1; return; }
         if ((dim == 2 && e == u2) || (dim == 3 && e == u3)) { m_kind = 3; return; }
-        throw std::runtime_error("gsFunctionExpr: only 0, the manufactured solution prod sin(pi x_i) and its "
-                                 "source d*pi^2*prod sin(pi x_i) are evaluated on the device, got '" + expr + "'");
+        if (affine(e, dim, m_c)) { m_kind = 5; return; }
+        throw std::runtime_error("gsFunctionExpr: only 0, affine functions, the manufactured solution "
+                                 "prod sin(pi x_i) and its source d*pi^2*prod sin(pi x_i) are evaluated on the "
+                                 "device, got '" + expr + "'");
+    }
+    /// Vector functions (Neumann fluxes): the gradient of the manufactured
+    /// solution, or constant components
+    gsFunctionExpr(const std::string &c0, const std::string &c1, short_t dim) : gsFunctionExpr(std::vector<std::string>{c0, c1}, dim) {}
+    gsFunctionExpr(const std::string &c0, const std::string &c1, const std::string &c2, short_t dim)
+        : gsFunctionExpr(std::vector<std::string>{c0, c1, c2}, dim) {}
+    gsFunctionExpr(const std::vector<std::string> &comp, short_t dim) : m_dim(dim), m_target((short_t)comp.size())
+    {
+        if ((int)comp.size() != dim) throw std::runtime_error("gsFunctionExpr: a flux needs one component per direction");
+        static const char *var = "xyz";
+        bool grad = true, constant = true;
+        for (int c = 0; c < dim; ++c) {
+            std::string want = "pi";
+            for (int r = 0; r < dim; ++r) want += std::string("*") + (r == c ? "cos" : "sin") + "(pi*" + var[r] + ")";
+            grad = grad && strip(comp[c]) == want;
+            double v[4] = {0, 0, 0, 0};
+            const bool a = affine(strip(comp[c]), dim, v) && v[1] == 0.0 && v[2] == 0.0 && v[3] == 0.0;
+            constant = constant && a;
+            m_c[1 + c] = v[0];
+        }
+        if (grad) { m_kind = 4; return; }
+        if (constant) { m_kind = 6; return; }
+        throw std::runtime_error("gsFunctionExpr: a flux must be the gradient of prod sin(pi x_i) or constant");
     }
     int kind() const { return m_kind; }
     short_t domainDim() const { return m_dim; }
+    short_t targetDim() const { return m_target; }
+    const double *coefs() const { return m_c; }
 
 private:
-    int m_kind = 2;   // 1 source d pi^2 prod sin, 2 zero, 3 solution prod sin
-    short_t m_dim = 2;
+    static std::string strip(const std::string &e)
+    {
+        std::string o;
+        for (char ch : e)
+            if (ch != ' ') o += ch;
+        return o;
+    }
+    /// c + sum_r a_r x_r written as a sum of terms "n", "n*x", "x", "x*n"
+    static bool affine(const std::string &e, int dim, double *v)
+    {
+        for (int k = 0; k < 4; ++k) v[k] = 0.0;
+        if (e.empty()) return false;
+        size_t i = 0;
+        while (i < e.size()) {
+            double sign = 1.0;
+            if (e[i] == '+' || e[i] == '-') {
+                sign = e[i] == '-' ? -1.0 : 1.0;
+                ++i;
+            }
+            size_t j = i;
+            while (j < e.size() && e[j] != '+' && e[j] != '-') {
+                if ((e[j] == 'e' || e[j] == 'E') && j + 1 < e.size()) ++j;   // exponent sign
+                ++j;
+            }
+            const std::string t = e.substr(i, j - i);
+            if (t.empty()) return false;
+            int var = -1;
+            double num = 1.0;
+            size_t pos = 0;
+            while (pos <= t.size()) {
+                const size_t star = t.find('*', pos);
+                const std::string f = t.substr(pos, star == std::string::npos ? std::string::npos : star - pos);
+                if (f == "x" || f == "y" || f == "z") {
+                    const int r = f[0] - 'x';
+                    if (var >= 0 || r >= dim) return false;
+                    var = r;
+                } else {
+                    char *end = nullptr;
+                    const double x = std::strtod(f.c_str(), &end);
+                    if (f.empty() || end != f.c_str() + f.size()) return false;
+                    num *= x;
+                }
+                if (star == std::string::npos) break;
+                pos = star + 1;
+            }
+            v[var < 0 ? 0 : 1 + var] += sign * num;
+            i = j;
+        }
+        return true;
+    }
+    // 1 source d pi^2 prod sin, 2 zero, 3 solution prod sin, 4 its gradient,
+    // 5 affine c + a.x (m_c = c, a), 6 constant vector (m_c[1..d])
+    int m_kind = 2;
+    short_t m_dim = 2, m_target = 1;
+    double m_c[4] = {0, 0, 0, 0};
 };
 
 /// Piecewise constant per element (patch-wise arrays, lexicographic);
@@ -594,27 +676,61 @@ private:
     std::vector<std::vector<real_t>> m_v;
 };
 
-/// Boundary conditions: homogeneous Dirichlet sides (others natural).
+/// Boundary conditions (gsBoundaryConditions, src/gsPde/gsBoundaryConditions.h):
+/// Dirichlet sides (eliminated; data zero, affine or the manufactured
+/// solution) and Neumann sides (flux: the manufactured solution's gradient
+/// or a constant vector); sides without a condition are natural.
 template <class T = real_t>
 class gsBoundaryConditions {
 public:
+    struct condition {
+        int patch;
+        int side;
+        condition_type::type type;
+        std::shared_ptr<const gsFunctionExpr<T>> f;   // a copy of the data (null: zero)
+    };
+    typedef std::vector<condition> bcRefList;
+
     void addCondition(int patch, boundary::side s, condition_type::type t, const gsFunctionExpr<T> *g = nullptr,
-                      short_t = 0, bool = false, int = -1)
+                      short_t = 0, bool parametric = false, int = -1)
     {
-        if (t != condition_type::dirichlet)
-            throw std::runtime_error("gsBoundaryConditions: only Dirichlet conditions are supported");
-        if (g && g->kind() != 2)
-            throw std::runtime_error("gsBoundaryConditions: only homogeneous Dirichlet values are supported");
-        m_dir.push_back({patch, (int)s});
+        if (t != condition_type::dirichlet && t != condition_type::neumann)
+            throw std::runtime_error("gsBoundaryConditions: Dirichlet and Neumann conditions only");
+        if (parametric) throw std::runtime_error("gsBoundaryConditions: data in physical coordinates only");
+        if (g && t == condition_type::dirichlet && g->kind() != 2 && g->kind() != 3 && g->kind() != 5)
+            throw std::runtime_error("gsBoundaryConditions: Dirichlet data must be 0, affine or prod sin(pi x_i)");
+        if (g && t == condition_type::neumann && g->kind() != 2 && g->kind() != 4 && g->kind() != 6)
+            throw std::runtime_error("gsBoundaryConditions: Neumann data must be a flux (gradient or constant)");
+        m_all.push_back({patch, (int)s, t, g ? std::make_shared<const gsFunctionExpr<T>>(*g) : nullptr});
+        if (t == condition_type::dirichlet) m_dir.push_back({patch, (int)s});
     }
     void addCondition(boundary::side s, condition_type::type t, const gsFunctionExpr<T> *g = nullptr)
     {
         addCondition(0, s, t, g);
     }
+    void setGeoMap(const gsMultiPatch<T> &) {}
     const std::vector<std::pair<int, int>> &dirichletSides() const { return m_dir; }
+    const std::vector<condition> &conditions() const { return m_all; }
+    /// the conditions of one type ("Dirichlet" / "Neumann")
+    bcRefList get(const std::string &label) const
+    {
+        bcRefList out;
+        for (const condition &c : m_all)
+            if ((label == "Dirichlet" && c.type == condition_type::dirichlet) ||
+                (label == "Neumann" && c.type == condition_type::neumann))
+                out.push_back(c);
+        return out;
+    }
+    bool inhomogeneous() const
+    {
+        for (const condition &c : m_all)
+            if (c.f && c.f->kind() != 2) return true;
+        return false;
+    }
 
 private:
     std::vector<std::pair<int, int>> m_dir;
+    std::vector<condition> m_all;
 };
 
 // ---------------------------------------------------------------- device problem
@@ -711,6 +827,27 @@ inline std::shared_ptr<gsx_problem> makeProblem(const gsMultiPatch<real_t> &mp, 
     return std::shared_ptr<gsx_problem>(p, ProblemDeleter());
 }
 
+/// Boundary data of bc on the device problem (gsFeSpace::setup /
+/// gsAssembler::computeDirichletDofs): Dirichlet data by kind, Neumann
+/// fluxes; dir_values one of GSX_DIR_*.
+inline void applyBc(const std::shared_ptr<gsx_problem> &prob, const gsBoundaryConditions<real_t> &bc, int dir_values)
+{
+    std::vector<gsx_bc> recs;
+    for (const auto &c : bc.conditions()) {
+        gsx_bc r{};
+        r.patch = c.patch;
+        r.side = c.side;
+        r.type = c.type == condition_type::dirichlet ? GSX_BC_DIRICHLET : GSX_BC_NEUMANN;
+        const int k = c.f ? c.f->kind() : 2;
+        r.kind = (k == 3 || k == 4) ? GSX_BCF_MANUFACTURED : (k == 5 || k == 6) ? GSX_BCF_AFFINE : GSX_BCF_ZERO;
+        if (c.f)
+            for (int v = 0; v < 4; ++v) r.value[v] = c.f->coefs()[v];
+        recs.push_back(r);
+    }
+    check(gsx_problem_set_bc(prob.get(), (int32_t)recs.size(), recs.empty() ? nullptr : recs.data(), dir_values),
+          "gsx_problem_set_bc");
+}
+
 /// Device assembly -> host gsSparseMatrix (the CSR arrays of a bitwise
 /// symmetric matrix are its CSC arrays) + rhs.
 inline void assembleInto(const std::shared_ptr<gsx_problem> &prob, int mode, int precision,
@@ -754,10 +891,23 @@ struct gsFeSpace {
     // shared by the copies handed out by getSpace, so that setup() on any
     // of them reaches the assembler
     std::shared_ptr<const gsBoundaryConditions<real_t> *> bc = std::make_shared<const gsBoundaryConditions<real_t> *>(nullptr);
+    std::shared_ptr<int> dir_values = std::make_shared<int>(dirichlet::l2Projection);
     /// gsFeSpace::setup (src/gsAssembler/gsExpressions.h:1091-1351):
-    /// interfaces glued (icont = 0), Dirichlet sides eliminated
-    void setup(const gsBoundaryConditions<real_t> &b, int = dirichlet::interpolation, int = 0) const { *bc = &b; }
+    /// interfaces glued (icont = 0), Dirichlet sides eliminated, their values
+    /// by dir_values (gsDirichletValues, src/gsAssembler/gsDirichletValues.h:
+    /// 27-60): homogeneous, or l2Projection (interpolation is not provided)
+    void setup(const gsBoundaryConditions<real_t> &b, int dv, int = 0) const
+    {
+        if (dv != dirichlet::homogeneous && dv != dirichlet::l2Projection)
+            throw std::runtime_error("gsFeSpace::setup: Dirichlet values by l2Projection or homogeneous only");
+        *bc = &b;
+        *dir_values = dv;
+    }
 };
+/// nv(G) and the Neumann form u * g_N.tr() * nv(G) of assembleBdr
+struct nv_expr {};
+struct u_gn { };
+struct neumann_form { };
 /// exact solution u = prod sin(pi x_i) (gsExprEvaluator::getVariable)
 struct exact_var { int kind = 3; };
 /// discrete solution (gsExprAssembler::getSolution): the free coefficients
@@ -809,6 +959,12 @@ inline expr::linear_form operator*(const space &, const expr::kappa_expr &k)
     return l;
 }
 inline expr::linear_form operator*(const expr::linear_form &l, const expr::meas_expr &) { return l; }
+struct bdr_fun_t {
+    bdr_fun_t tr() const { return bdr_fun_t(); }
+};
+inline expr::nv_expr nv(const geometryMap &) { return expr::nv_expr(); }
+inline expr::u_gn operator*(const space &, const bdr_fun_t &) { return expr::u_gn(); }
+inline expr::neumann_form operator*(const expr::u_gn &, const expr::nv_expr &) { return expr::neumann_form(); }
 
 // ---------------------------------------------------------------- assemblers
 /// gsExprAssembler (src/gsAssembler/gsExprAssembler.h:29-833) for the
@@ -843,8 +999,30 @@ public:
         const gsBoundaryConditions<T> *bc = *m_u.bc ? *m_u.bc : m_bc;
         static const gsBoundaryConditions<T> none;
         m_prob = detail::makeProblem(*m_mp, *m_mb, bc ? *bc : none, l.kind, a.kappa, l.felem);
+        if (bc && bc->inhomogeneous())
+            detail::applyBc(m_prob, *bc, *m_u.dir_values == dirichlet::homogeneous ? GSX_DIR_HOMOGENEOUS
+                                                                                   : GSX_DIR_L2PROJECTION);
         const int prec = m_opt.askString("Precision", "fp64") == "mixed" ? GSX_MIXED : GSX_FP64;
         detail::assembleInto(m_prob, GSX_PATTERN_EXPR, prec, m_matrix, m_rhs);
+    }
+    /// g_N of the Neumann conditions in assembleBdr
+    bdr_fun_t getBdrFunction(const geometryMap &) const { return bdr_fun_t(); }
+    /// gsExprAssembler::assembleBdr(bc.get("Neumann"), u * g_N.tr() * nv(G))
+    /// (src/gsAssembler/gsExprAssembler.h:835-895): adds the Neumann load of
+    /// the space's boundary conditions to the right-hand side
+    template <class List>
+    void assembleBdr(const List &neumann, const expr::neumann_form &)
+    {
+        if (!m_prob) throw std::runtime_error("gsExprAssembler::assembleBdr: assemble first");
+        if (neumann.empty()) return;
+        const index_t n = m_rhs.rows();
+        detail::DeviceVec b(sizeof(double) * std::max<index_t>(1, n));
+        std::vector<double> zero((size_t)std::max<index_t>(1, n), 0.0);
+        detail::check(gsx_copy_to_device(detail::ctx(), b.d(), zero.data(), sizeof(double) * n), "upload");
+        detail::check(gsx_assemble_bdr(m_prob.get(), (double *)b.d()), "gsx_assemble_bdr");
+        std::vector<double> load((size_t)std::max<index_t>(1, n));
+        detail::check(gsx_copy_to_host(detail::ctx(), load.data(), b.d(), sizeof(double) * n), "download");
+        for (index_t i = 0; i < n; ++i) m_rhs(i, 0) += load[(size_t)i];
     }
     const gsSparseMatrix<T> &matrix() const { return m_matrix; }
     void matrix_into(gsSparseMatrix<T> &out) const { out = m_matrix; }
@@ -875,7 +1053,7 @@ public:
                        iFace::strategy = iFace::glue)
         : m_mp(&mp), m_mb(&mb), m_bc(&bc), m_f(f)
     {
-        if (ds != dirichlet::elimination && ds != dirichlet::homogeneous)
+        if (ds != dirichlet::elimination)
             throw std::runtime_error("gsPoissonAssembler: only Dirichlet elimination is supported");
     }
     gsOptionList &options() { return m_opt; }
@@ -883,6 +1061,20 @@ public:
     virtual void assemble()
     {
         m_prob = detail::makeProblem(*m_mp, *m_mb, *m_bc, m_f.kind(), nullptr, nullptr);
+        if (m_bc->inhomogeneous()) {
+            // gsAssembler::computeDirichletDofs (src/gsAssembler/gsAssembler.hpp:231-297):
+            // DirichletValues 101 (interpolation, the default) is not provided
+            const int dv = m_opt.askInt("DirichletValues", dirichlet::interpolation);
+            for (const auto &c : m_bc->conditions())
+                if (c.type == condition_type::neumann && c.f && c.f->kind() != 2)
+                    throw std::runtime_error("gsPoissonAssembler: scalar Neumann data are not provided "
+                                             "(gsExprAssembler::assembleBdr takes fluxes)");
+            if (dv != dirichlet::l2Projection && dv != dirichlet::homogeneous)
+                throw std::runtime_error("gsPoissonAssembler: inhomogeneous Dirichlet values need the option "
+                                         "DirichletValues = 102 (l2Projection)");
+            detail::applyBc(m_prob, *m_bc, dv == dirichlet::homogeneous ? GSX_DIR_HOMOGENEOUS
+                                                                        : GSX_DIR_L2PROJECTION_INHOM);
+        }
         const int prec = m_opt.askString("Precision", "fp64") == "mixed" ? GSX_MIXED : GSX_FP64;
         detail::assembleInto(m_prob, GSX_PATTERN_FULL, prec, m_matrix, m_rhs);
     }

@@ -10,10 +10,12 @@
 //   <MultiPatch parDim=.. id=..> with <patches type="id_range|id_index">,
 //     <interfaces> (p1 s1 p2 s2, direction map [d], orientation [d]) and
 //     <boundary name=..> (patch side) lists
-//   <Function type="FunctionExpr" dim=.. id=..>expression</Function>
+//   <Function type="FunctionExpr" dim=.. id=..>expression</Function>, or
+//     vector-valued with <c index=..> components
 //   <OptionList id=..> with <int|real|bool|string label=.. value=..>
-//   <boundaryConditions id=.. multipatch=..> with <bc type="Dirichlet"
-//     name=.. function=..> referring to the MultiPatch's named boundaries
+//   <boundaryConditions id=.. multipatch=..> with <bc type="Dirichlet|
+//     Neumann" name=.. function=..> referring to the MultiPatch's named
+//     boundaries
 //   <Matrix rows cols> (row-major values), <SparseMatrix rows cols>
 //     (row col value per non-zero, column by column)
 // Rational (NURBS) geometries, THB bases and base64 payloads are refused
@@ -577,8 +579,19 @@ private:
     {
         if (n.attr_or("type") != "FunctionExpr")
             throw std::runtime_error("gsFileData: function type '" + n.attr_or("type") + "' not supported");
-        if (!n.children.empty()) throw std::runtime_error("gsFileData: vector-valued FunctionExpr not supported");
-        return gsFunctionExpr<T>(n.text, (short_t)std::atoi(n.attr_or("dim", "2").c_str()));
+        const short_t dim = (short_t)std::atoi(n.attr_or("dim", "2").c_str());
+        if (!n.children.empty()) {
+            // vector-valued: <c index="i"> components (Neumann fluxes)
+            std::vector<std::string> comp;
+            for (auto &c : n.children)
+                if (c.name == "c") {
+                    const size_t i = (size_t)std::atoi(c.attr_or("index", "0").c_str());
+                    if (comp.size() <= i) comp.resize(i + 1);
+                    comp[i] = c.text;
+                }
+            return gsFunctionExpr<T>(comp, dim);
+        }
+        return gsFunctionExpr<T>(n.text, dim);
     }
 
     gsOptionList options(const xml::Node &n) const
@@ -652,16 +665,15 @@ private:
             }
             std::vector<long> v = xml::integers(c.text);
             for (size_t r = 0; r + 1 < v.size(); r += 2) sides.push_back({(int)v[r], (int)v[r + 1]});
-            if (type == "Neumann" || type == "neumann") {
-                // natural: only homogeneous data is consistent with the assembly
-                continue;
-            }
-            if (type != "Dirichlet" && type != "dirichlet")
+            const bool neu = type == "Neumann" || type == "neumann";
+            if (!neu && type != "Dirichlet" && type != "dirichlet")
                 throw std::runtime_error("gsFileData: boundary condition type '" + type + "' not supported");
             const int fi = std::atoi(c.attr_or("function", "0").c_str());
             gsFunctionExpr<T> g("0", mp.dim() ? mp.dim() : 2);
             if (funcs.count(fi)) g = function(*funcs[fi]);
-            for (auto &s : sides) bc.addCondition(s.first, (boundary::side)s.second, condition_type::dirichlet, &g);
+            for (auto &s : sides)
+                bc.addCondition(s.first, (boundary::side)s.second, neu ? condition_type::neumann : condition_type::dirichlet,
+                                &g);
         }
         return bc;
     }

@@ -10,6 +10,14 @@
 //   --last        solve only on the finest level
 //   --full        gsPoissonAssembler (full pattern) instead of gsExprAssembler
 //   --mixed       FP32 iterations with FP64 residual updates
+//   --bvp         the boundary value problem of poisson2d_bvp: the domain
+//                 shifted by 1/4 (u = prod sin(pi x_i) does not vanish on
+//                 it), u on the west/south(/front) sides by L2 projection,
+//                 the flux grad u on the others (assembleBdr); with
+//                 --full u on every side, DirichletValues = l2Projection
+// A file with <boundaryConditions> (the layout of the reference's
+// filedata/pde/poisson2d_bvp.xml: MultiPatch, source id 1, conditions id 2,
+// exact solution id 3) supplies its own data.
 // Prints one line per level: dofs, CG iterations, relative residual, L2
 // error; then the convergence rates.
 #include "gismo.h"
@@ -19,13 +27,14 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 
 using namespace gismo;
 
 int main(int argc, char *argv[])
 {
     index_t numRefine = 5, numElevate = 1, dim = 2, gx = 0, gy = 0;
-    bool last = false, full = false, mixed = false;
+    bool last = false, full = false, mixed = false, bvp = false;
     std::string fn;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
@@ -37,26 +46,50 @@ int main(int argc, char *argv[])
         else if (a == "--last") last = true;
         else if (a == "--full") full = true;
         else if (a == "--mixed") mixed = true;
+        else if (a == "--bvp") bvp = true;
         else { std::fprintf(stderr, "unknown argument %s\n", argv[i]); return 1; }
     }
     try {
         gsMultiPatch<> mp;
+        std::unique_ptr<gsFileData<>> fd;
         if (!fn.empty()) {
-            gsFileData<> fd(fn);
-            std::printf("Loaded file %s\n", fd.lastPath().c_str());
-            fd.getFirst(mp);
-        } else if (gx > 0) mp = gsNurbsCreator<>::BSplineSquareGrid(gx, gy);
+            fd.reset(new gsFileData<>(fn));
+            std::printf("Loaded file %s\n", fd->lastPath().c_str());
+            fd->getFirst(mp);
+        } else if (gx > 0) mp = gsNurbsCreator<>::BSplineSquareGrid(gx, gy, 1, bvp ? 0.25 : 0, bvp ? 0.25 : 0);
         else {
-            mp.addPatch(dim == 3 ? gsNurbsCreator<>::BSplineCube() : gsNurbsCreator<>::BSplineSquare());
+            const real_t o = bvp ? 0.25 : 0;
+            mp.addPatch(dim == 3 ? gsNurbsCreator<>::BSplineCube(1, o, o, o) : gsNurbsCreator<>::BSplineSquare(1, o, o));
             mp.computeTopology();
         }
         const short_t d = mp.dim();
         gsFunctionExpr<> f(d == 3 ? "3*pi^2*sin(pi*x)*sin(pi*y)*sin(pi*z)" : "2*pi^2*sin(pi*x)*sin(pi*y)", d);
         gsFunctionExpr<> ms(d == 3 ? "sin(pi*x)*sin(pi*y)*sin(pi*z)" : "sin(pi*x)*sin(pi*y)", d);
+        gsFunctionExpr<> flux = d == 3 ? gsFunctionExpr<>("pi*cos(pi*x)*sin(pi*y)*sin(pi*z)", "pi*sin(pi*x)*cos(pi*y)*sin(pi*z)",
+                                                          "pi*sin(pi*x)*sin(pi*y)*cos(pi*z)", d)
+                                       : gsFunctionExpr<>("pi*cos(pi*x)*sin(pi*y)", "pi*sin(pi*x)*cos(pi*y)", d);
         gsBoundaryConditions<> bc;
         gsFunctionExpr<> zero("0", d);
+        for (auto &s : mp.boundaries()) {
+            const bool lower = s.second % 2 == 1;   // west, south, front
+            if (!bvp) bc.addCondition(s.first, (boundary::side)s.second, condition_type::dirichlet, &zero);
+            else if (lower) bc.addCondition(s.first, (boundary::side)s.second, condition_type::dirichlet, &ms);
+            else bc.addCondition(s.first, (boundary::side)s.second, condition_type::neumann, &flux);
+        }
+        if (fd && fd->has("boundaryConditions")) {
+            // the layout of filedata/pde/poisson2d_bvp.xml: source id 1,
+            // conditions id 2, exact solution id 3
+            fd->getId(1, f);
+            fd->getId(2, bc);
+            fd->getId(3, ms);
+            std::printf("Boundary conditions from the file\n");
+        }
+        bc.setGeoMap(mp);
+        // gsPoissonAssembler takes scalar Neumann data, which the device does
+        // not evaluate: its --bvp variant fixes u on every side instead
+        gsBoundaryConditions<> bcAll;
         for (auto &s : mp.boundaries())
-            bc.addCondition(s.first, (boundary::side)s.second, condition_type::dirichlet, &zero);
+            bcAll.addCondition(s.first, (boundary::side)s.second, condition_type::dirichlet, bvp ? &ms : &zero);
 
         gsMultiBasis<> dbasis(mp, true);
         dbasis.setDegree(dbasis.maxCwiseDegree() + numElevate);
@@ -91,13 +124,17 @@ int main(int argc, char *argv[])
             const gsMatrix<> *rhs;
             std::unique_ptr<gsPoissonAssembler<>> pa;
             if (full) {
-                pa.reset(new gsPoissonAssembler<>(mp, dbasis, bc, f));
+                pa.reset(new gsPoissonAssembler<>(mp, dbasis, bcAll, f));
                 if (mixed) pa->options().setString("Precision", "mixed");
+                if (bvp) pa->options().setInt("DirichletValues", dirichlet::l2Projection);
                 pa->assemble();
                 K = &pa->matrix();
                 rhs = &pa->rhs();
             } else {
                 A.assemble(igrad(u, G) * igrad(u, G).tr() * meas(G), u * ff * meas(G));
+                // Neumann terms defined on physical space
+                auto g_N = A.getBdrFunction(G);
+                A.assembleBdr(bc.get("Neumann"), u * g_N.tr() * nv(G));
                 K = &A.matrix();
                 rhs = &A.rhs();
             }

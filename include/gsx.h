@@ -180,7 +180,15 @@ typedef struct {
  * gsx_assemble_bdr; every later gsx_assemble moves the fixed columns to the
  * right-hand side (rhs_i -= K_ij g_j, gsExprAssembler.h:553-630).  Replaces
  * any earlier boundary data of the problem. */
-typedef enum { GSX_DIR_HOMOGENEOUS = 0, GSX_DIR_L2PROJECTION = 1 } gsx_dir_values;
+/* GSX_DIR_L2PROJECTION_INHOM: the projection of gsAssembler::
+ * computeDirichletDofsL2Proj (src/gsAssembler/gsAssembler.hpp:392-534, the
+ * gsPoissonAssembler path), which skips the homogeneous Dirichlet sides:
+ * their functions keep 0 unless an inhomogeneous side shares them. */
+typedef enum {
+    GSX_DIR_HOMOGENEOUS = 0,
+    GSX_DIR_L2PROJECTION = 1,
+    GSX_DIR_L2PROJECTION_INHOM = 2
+} gsx_dir_values;
 gsx_status gsx_problem_set_bc(gsx_problem *prob, int32_t nbc, const gsx_bc *bc,
                               int32_t dir_values);
 /* Host copy of the fixed (eliminated) DOF values, in the order of their

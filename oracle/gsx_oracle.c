@@ -842,6 +842,12 @@ int oc_cg_solve(const oc_csr *A, const double *b, double *x, double tol,
     double *t = (double *)malloc(sizeof(double) * n);
     double *dinv = (double *)malloc(sizeof(double) * n);
     for (int i = 0; i < n; ++i) dinv[i] = diag_of(A, i);
+    /* precond 3: Eigen's DiagonalPreconditioner (external/gsEigen/src/
+     * IterativeLinearSolvers/BasicPreconditioners.h:66-77) takes 1 where the
+     * diagonal entry is zero or missing */
+    if (precond == 3)
+        for (int i = 0; i < n; ++i)
+            if (dinv[i] == 0.0) dinv[i] = 1.0;
     int it = 0;
     double err = 0.0;
     const double bnorm = sqrt(dot(b, b, n));
@@ -858,7 +864,7 @@ int oc_cg_solve(const oc_csr *A, const double *b, double *x, double tol,
     if (err < tol) goto done;
 #define APPLY_PREC(in, out)                                                  \
     do {                                                                     \
-        if (precond == 1) for (int i = 0; i < n; ++i) out[i] = tau * in[i] / dinv[i]; \
+        if (precond == 1 || precond == 3) for (int i = 0; i < n; ++i) out[i] = tau * in[i] / dinv[i]; \
         else if (precond == 2) oc_mg_apply(mg, in, out);                     \
         else memcpy(out, in, sizeof(double) * n);                            \
     } while (0)
@@ -1591,8 +1597,11 @@ int oc_dirichlet_l2(int npatch, const oc_patch *patches, const int *dofs, int nf
     for (int i = 0; i < nfix; ++i) fixed_out[i] = 0.0;
     double err = 0.0;
     /* gsSparseSolver::CGDiagonal with Eigen's default tolerance (machine
-     * epsilon): Jacobi-preconditioned CG to rounding level */
-    oc_cg_solve(M, rhs, fixed_out, 1e-15, 2 * nfix + 100, 1, 1.0, NULL, &err, NULL);
+     * epsilon): Jacobi-preconditioned CG to rounding level.  Functions on
+     * no projected side (gsAssembler's variant skips homogeneous sides) have
+     * zero or no rows; Eigen's preconditioner takes 1 there and, from
+     * x = 0 with a zero right-hand side in those rows, they stay 0. */
+    oc_cg_solve(M, rhs, fixed_out, 1e-15, 2 * nfix + 100, 3, 1.0, NULL, &err, NULL);
     oc_csr_free(M);
     free(rhs);
     free(off);

@@ -116,7 +116,8 @@ void oc_csr_spmv(const oc_csr *A, const double *x, double *y);
 /* --- solvers ------------------------------------------------------------ */
 typedef struct oc_mg oc_mg;
 
-/* precond: 0 none, 1 Jacobi(tau), 2 multigrid (mg != NULL).
+/* precond: 0 none, 1 Jacobi(tau), 2 multigrid (mg != NULL), 3 Jacobi with
+ * Eigen's rule for zero diagonal entries (1 there).
  * x: in/out (initial guess).  hist (optional, maxit+1): error history.
  * Returns iterations; *err_out = final ||r||/||b||. */
 int  oc_cg_solve(const oc_csr *A, const double *b, double *x, double tol,

@@ -146,7 +146,55 @@ int main(int argc, char **argv)
         CHECK(cok);
         std::printf("roundtrip matrix %d sparse %d multipatch %d\n", ok ? 1 : 0, sok ? 1 : 0, cok ? 1 : 0);
     }
+    // boundary value problem in the layout of filedata/pde/poisson2d_bvp.xml
+    {
+        gsFileData<> fd(dir + "/square_pair_bvp.xml");
+        gsMultiPatch<> mp;
+        fd.getFirst(mp);
+        gsFunctionExpr<> f, ms;
+        gsBoundaryConditions<> bc;
+        fd.getId(1, f);
+        fd.getId(2, bc);
+        fd.getId(3, ms);
+        CHECK(mp.nPatches() == 2 && mp.interfaces().size() == 1);
+        CHECK(f.kind() == 1 && ms.kind() == 3);
+        int nd = 0, nn = 0;
+        for (const auto &c : bc.conditions()) {
+            if (c.type == condition_type::dirichlet) {
+                ++nd;
+                CHECK(c.f && c.f->kind() == 3);
+                CHECK(c.side == 1 || c.side == 3);
+            } else {
+                ++nn;
+                CHECK(c.f && c.f->kind() == 4 && c.f->targetDim() == 2);
+                CHECK(c.side == 2 || c.side == 4);
+            }
+        }
+        CHECK(bc.inhomogeneous());
+        std::printf("bvp dirichlet %d neumann %d\n", nd, nn);
+    }
     // refused inputs
+    {
+        // a scalar Neumann function (the device takes fluxes)
+        bool threw = false;
+        try {
+            gsFileData<> fd;
+            fd.readString("<xml><MultiPatch parDim=\"2\" id=\"0\"><patches type=\"id_range\">1 1</patches>"
+                          "<boundary name=\"n\">1 2</boundary></MultiPatch>"
+                          "<Geometry type=\"TensorBSpline2\" id=\"1\"><Basis type=\"TensorBSplineBasis2\">"
+                          "<Basis type=\"BSplineBasis\" index=\"0\"><KnotVector degree=\"1\">0 0 1 1</KnotVector></Basis>"
+                          "<Basis type=\"BSplineBasis\" index=\"1\"><KnotVector degree=\"1\">0 0 1 1</KnotVector></Basis>"
+                          "</Basis><coefs geoDim=\"2\">0 0 1 0 0 1 1 1</coefs></Geometry>"
+                          "<boundaryConditions id=\"2\" multipatch=\"0\">"
+                          "<Function type=\"FunctionExpr\" dim=\"2\" index=\"0\">sin(pi*x)*sin(pi*y)</Function>"
+                          "<bc type=\"Neumann\" function=\"0\" name=\"n\"></bc></boundaryConditions></xml>");
+            gsBoundaryConditions<> bc;
+            fd.getId(2, bc);
+        } catch (const std::runtime_error &) {
+            threw = true;
+        }
+        CHECK(threw);
+    }
     {
         bool threw = false;
         try {

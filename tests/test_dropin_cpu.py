@@ -119,3 +119,4 @@ def test_xml_reader_on_reference_inputs():
     assert out.returncode == 0, out.stdout + out.stderr
     assert "fichera_refined_free 117" in out.stdout
     assert out.stdout.count("topology_match 1") == 3
+    assert "bvp dirichlet 3 neumann 3" in out.stdout

@@ -148,3 +148,26 @@ def test_manufactured_mixed_bc_convergence_on_device(ctx):
     rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
     print(errs, rates, flush=True)
     assert rates[-1] > 2.8, rates
+
+
+def test_projection_of_inhomogeneous_sides_only(ctx):
+    """DIR_L2PROJECTION_INHOM (gsAssembler::computeDirichletDofs): only the
+    sides with nonzero data take part in the projection, so the corner
+    functions shared with a homogeneous side keep the projected data of the
+    other side, and functions on homogeneous sides alone stay 0.  Equals the
+    oracle's projection over the inhomogeneous records; differs from the
+    all-sides projection at the shared corners."""
+    P = _curved_patch(2, 3, 6, seed=4)
+    prob = problems.Problem([P], [(0, 1), (0, 2), (0, 3)], [])
+    bcs = [(0, 1, 1, 2, AFFINE), (0, 2, 1, 1, ()), (0, 3, 1, 0, ()), (0, 4, 2, 1, ())]
+    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    nfix = int(max(dofs)) + 1 - nfree
+    want = O.dirichlet_values(prob.patches, dofs, nfree, nfix, [b for b in bcs if b[2] == 1 and b[3] != 0])
+    gp = gsx.Problem(ctx, prob)
+    gp.set_bc(bcs, gsx.DIR_L2PROJECTION_INHOM)
+    got = gp.fixed_values()
+    assert np.abs(got - want).max() <= 1e-12 * np.abs(want).max()
+    gp.set_bc(bcs, gsx.DIR_L2PROJECTION)
+    every = gp.fixed_values()
+    assert np.abs(every - want).max() > 1e-6
+    gp.close()

@@ -137,3 +137,95 @@ def test_multigrid_example_chebyshev_smoother():
         its[sm] = int(s[1])
     print(its, flush=True)
     assert its["c"] < its["j"], its
+
+
+def _bvp_oracle(d, p, n, dsides, nsides, mode):
+    """The --bvp problem of poisson2_example at n elements per direction:
+    u = prod sin(pi x_i) on [0.25, 1.25]^d, projected Dirichlet data on
+    dsides, the flux grad u on nsides."""
+    P = problems.box_patch(d, p, n, [0.25] * d, [1.25] * d)
+    P.fkind = 1
+    prob = problems.Problem([P], [(0, s) for s in dsides], [])
+    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    nfix = int(max(dofs)) + 1 - nfree
+    fixed = O.dirichlet_values(prob.patches, dofs, nfree, nfix, [(0, s, 1, 1, ()) for s in dsides])
+    A, b = O.assemble(prob.patches, dofs, nfree, mode, fixed=fixed)
+    if nsides:
+        b = b + O.neumann_load(prob.patches, dofs, nfree, [(0, s, 2, 1, ()) for s in nsides])
+    xo, ito, _ = O.cg_solve(A, b, tol=1e-10, maxit=10000)
+    return nfree, A.nnz, ito, O.l2_error(prob.patches, dofs, nfree, xo, fixed=fixed)
+
+
+@pytest.mark.parametrize("d,args,rate", [(2, ["-e", "1", "-r", "3"], 2.7),
+                                        (3, ["-d", "3", "-e", "1", "-r", "2"], 2.5)])
+def test_poisson2_example_bvp(d, args, rate):
+    """poisson2_example --bvp: Dirichlet data u on the west/south(/front)
+    sides by L2 projection (gsFeSpace::setup with dirichlet::l2Projection)
+    and the flux on the others through gsExprAssembler::assembleBdr with
+    u * g_N.tr() * nv(G).  Per level the DOFs, pattern, CG iterations and
+    L2 error (fixed values included) agree with the oracle on the same
+    problem; the error falls at order p + 1 = 3."""
+    rows = _levels(_run("poisson2_example", "--bvp", *args))
+    lower = [1, 3, 5][:d]
+    upper = [2, 4, 6][:d]
+    for lvl, (dofs, nnz, its, rel, l2) in enumerate(rows):
+        nfree, onnz, ito, eo = _bvp_oracle(d, 2, 2 ** (lvl + 1), lower, upper, 0)
+        assert (dofs, nnz) == (nfree, onnz)
+        assert abs(its - ito) <= 2, (lvl, its, ito)
+        assert rel < 1e-10
+        assert abs(l2 - eo) <= 1e-4 * eo, (lvl, l2, eo)
+    l2 = np.array([r[4] for r in rows])
+    rates = np.log2(l2[:-1] / l2[1:])
+    print(d, l2, rates, flush=True)
+    assert rates[-1] > rate, rates
+
+
+def test_poisson2_example_bvp_poisson_assembler():
+    """poisson2_example --bvp --full: gsPoissonAssembler with u on every side
+    and the option DirichletValues = 102 (l2Projection), on the full
+    pattern, against the oracle; converges at order 3."""
+    rows = _levels(_run("poisson2_example", "--bvp", "--full", "-e", "1", "-r", "3"))
+    for lvl, (dofs, nnz, its, rel, l2) in enumerate(rows):
+        nfree, onnz, ito, eo = _bvp_oracle(2, 2, 2 ** (lvl + 1), [1, 2, 3, 4], [], 1)
+        assert (dofs, nnz) == (nfree, onnz)
+        assert abs(its - ito) <= 2, (lvl, its, ito)
+        assert abs(l2 - eo) <= 1e-4 * eo, (lvl, l2, eo)
+    l2 = np.array([r[4] for r in rows])
+    rates = np.log2(l2[:-1] / l2[1:])
+    assert rates[-1] > 2.7, rates
+
+
+def test_poisson2_example_bvp_from_xml():
+    """poisson2_example -f square_pair_bvp.xml: source, boundary conditions
+    (Dirichlet u, Neumann flux grad u) and exact solution read from the file
+    in the layout of the reference's poisson2d_bvp.xml; two glued patches.
+    Per level the DOFs, pattern and L2 error agree with the oracle."""
+    path = os.path.join(ROOT, "tests", "golden", "xml", "square_pair_bvp.xml")
+    text = _run("poisson2_example", "-f", path, "-e", "1", "-r", "3")
+    assert "Boundary conditions from the file" in text
+    rows = _levels(text)
+    assert len(rows) == 4
+    dsides = [(0, 1), (0, 3), (1, 3)]
+    nsides = [(0, 4), (1, 4), (1, 2)]
+    for lvl, (dofs, nnz, its, rel, l2) in enumerate(rows):
+        n = 2 ** (lvl + 1)
+        patches = []
+        for i in range(2):
+            P = problems.box_patch(2, 2, n, [0.25 + i, 0.25], [1.25 + i, 1.25])
+            P.fkind = 1
+            patches.append(P)
+        prob = problems.Problem(patches, dsides, [(0, 2, 1, 1, 1, 0, 0, 0)])
+        dmap, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+        nfix = int(max(dmap)) + 1 - nfree
+        fixed = O.dirichlet_values(prob.patches, dmap, nfree, nfix, [(p, s, 1, 1, ()) for p, s in dsides])
+        A, b = O.assemble(prob.patches, dmap, nfree, 0, fixed=fixed)
+        b = b + O.neumann_load(prob.patches, dmap, nfree, [(p, s, 2, 1, ()) for p, s in nsides])
+        xo, ito, _ = O.cg_solve(A, b, tol=1e-10, maxit=10000)
+        eo = O.l2_error(prob.patches, dmap, nfree, xo, fixed=fixed)
+        assert (dofs, nnz) == (nfree, A.nnz)
+        assert abs(its - ito) <= 2, (lvl, its, ito)
+        assert abs(l2 - eo) <= 1e-4 * eo, (lvl, l2, eo)
+    l2 = np.array([r[4] for r in rows])
+    rates = np.log2(l2[:-1] / l2[1:])
+    print(l2, rates, flush=True)
+    assert rates[-1] > 2.7, rates

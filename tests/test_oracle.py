@@ -701,6 +701,38 @@ def test_dirichlet_projection_reproduces_affine_data(d, p):
     assert np.abs(fixed - expect).max() <= 1e-12
 
 
+@pytest.mark.parametrize("d,p,skip", [(2, 3, [3]), (3, 2, [3, 6])])
+def test_projection_skipping_homogeneous_sides(d, p, skip):
+    """gsAssembler::computeDirichletDofsL2Proj (src/gsAssembler/
+    gsAssembler.hpp:393-534) projects the inhomogeneous sides only: with an
+    affine g on those and the homogeneous sides left out, every function
+    with a trace on a projected side (the shared edges and corners
+    included) gets g's coefficient c + a . C_m exactly, and the functions
+    living on the skipped sides alone (zero rows of the projection matrix)
+    stay 0."""
+    sides = list(range(1, 2 * d + 1))
+    prob = _bc_patch(d, p, 4, sides, seed=13)
+    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    nfix = prob.patches[0].nbasis - nfree
+    bcs = [(0, s, 1, 2, AFFINE) for s in sides if s not in skip]
+    fixed = O.dirichlet_values(prob.patches, dofs, nfree, nfix, bcs)
+    _, expect = _affine_at_ctrl(prob, dofs, nfree, 0)
+    P = prob.patches[0]
+    nb = [P.knots[r].size - P.p[r] - 1 for r in range(d)]
+    on_projected = np.zeros(nfix, bool)
+    for m, g in enumerate(dofs):
+        idx = np.unravel_index(m, nb[::-1])[::-1]   # direction 0 fastest
+        for s in sides:
+            if s in skip:
+                continue
+            r, last = (s - 1) // 2, (s - 1) % 2
+            if idx[r] == (nb[r] - 1 if last else 0):
+                on_projected[g - nfree] = True
+    assert on_projected.any() and (~on_projected).any()
+    assert np.abs(fixed[on_projected] - expect[on_projected]).max() <= 1e-12
+    assert np.all(fixed[~on_projected] == 0.0)
+
+
 @pytest.mark.parametrize("d,p,dir_sides", [(2, 2, [1, 3]), (2, 3, [2, 4]), (3, 2, [1, 3, 6])])
 def test_mixed_boundary_patch_test(d, p, dir_sides):
     """Patch test: u = c + a . x solves -div grad u = 0; with u on the
