@@ -776,10 +776,24 @@ __global__ __launch_bounds__(256) void k_gather_patch(PatchBox bx, int nfree, in
         double racc = 0.0;
         const int ne0 = ehi[0] - elo[0] + 1, ne1 = ehi[1] - elo[1] + 1, ne2 = ehi[2] - elo[2] + 1;
         const int ne = ne0 * ne1 * ne2;
+        // element t of the row's lexicographic element range; interior rows
+        // (P+1 elements per direction) decode with constant divisors
+        const bool interior = ne0 == NB1 && ne1 == NB1;
+        auto decode = [&](int t, int &ex, int &ey, int &ez) {
+            if (interior) {
+                ex = elo[0] + t % NB1;
+                ey = elo[1] + (t / NB1) % NB1;
+                ez = elo[2] + t / (NB1 * NB1);
+            } else {
+                ex = elo[0] + t % ne0;
+                ey = elo[1] + (t / ne0) % ne1;
+                ez = elo[2] + t / (ne0 * ne1);
+            }
+        };
         // element rows in flight per lane; with more than one b chunk per
         // element (NB > 64) one element at a time keeps the per-column
         // element order
-        constexpr int U = (NB > kWave) ? 1 : 8;
+        constexpr int U = (NB > kWave) ? 1 : 16;
         for (int t0 = 0; t0 < ne; t0 += U) {
             double elimv[U];
 #pragma unroll
@@ -796,8 +810,8 @@ __global__ __launch_bounds__(256) void k_gather_patch(PatchBox bx, int nfree, in
                     v[u] = 0.0;
                     el_u[u] = 0.0;
                     if (t < ne && b < NB) {
-                        const int ex = elo[0] + t % ne0, ey = elo[1] + (t / ne0) % ne1;
-                        const int ez = elo[2] + t / (ne0 * ne1);
+                        int ex, ey, ez;
+                        decode(t, ex, ey, ez);
                         const int jx = ex + bxx, jy = ey + byy, jz = ez + bzz;
                         bool fr = jx >= bx.flo[0] && jx <= bx.fhi[0] && jy >= bx.flo[1] && jy <= bx.fhi[1];
                         if (D == 3) fr = fr && jz >= bx.flo[2] && jz <= bx.fhi[2];
@@ -836,7 +850,8 @@ __global__ __launch_bounds__(256) void k_gather_patch(PatchBox bx, int nfree, in
             }
             for (int u = 0; u < U && t0 + u < ne; ++u) {
                 const int t = t0 + u;
-                const int ex = elo[0] + t % ne0, ey = elo[1] + (t / ne0) % ne1, ez = elo[2] + t / (ne0 * ne1);
+                int ex, ey, ez;
+                decode(t, ex, ey, ez);
                 const int64_t el = ex + (int64_t)bx.nel[0] * (ey + (int64_t)bx.nel[1] * ez) - elbase;
                 const int la = (ic[0] - ex) + NB1 * ((ic[1] - ey) + NB1 * (ic[2] - ez));
                 if (fixed) racc -= elimv[u];

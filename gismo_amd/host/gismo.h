@@ -918,6 +918,16 @@ struct diff_expr {
     diff_expr sqNorm() const { return *this; }
 };
 struct l2_integrand { diff_expr d; };
+/// igrad(u_ex) - igrad(u_sol, G) of the H1 seminorm error
+struct igrad_exact {};
+struct igrad_sol { const gsMatrix<real_t> *coefs = nullptr; };
+struct grad_diff {
+    const gsMatrix<real_t> *coefs = nullptr;
+    grad_diff sqNorm() const { return *this; }
+};
+struct h1_integrand { grad_diff d; };
+inline grad_diff operator-(const igrad_exact &, const igrad_sol &s) { grad_diff d; d.coefs = s.coefs; return d; }
+inline h1_integrand operator*(const grad_diff &d, const meas_expr &) { h1_integrand i; i.d = d; return i; }
 inline diff_expr operator-(const exact_var &e, const solution_var &s) { diff_expr d; d.kind = e.kind; d.coefs = s.coefs; return d; }
 inline diff_expr operator-(const solution_var &s, const exact_var &e) { return e - s; }
 inline l2_integrand operator*(const diff_expr &d, const meas_expr &) { l2_integrand i; i.d = d; return i; }
@@ -931,6 +941,13 @@ struct igrad_t {
     igrad_t tr() const { igrad_t t; t.e.tr = true; return t; }
 };
 inline igrad_t igrad(const space &, const geometryMap &) { return igrad_t(); }
+inline expr::igrad_exact igrad(const expr::exact_var &) { return expr::igrad_exact(); }
+inline expr::igrad_sol igrad(const expr::solution_var &s, const geometryMap &)
+{
+    expr::igrad_sol g;
+    g.coefs = s.coefs;
+    return g;
+}
 inline expr::meas_expr meas(const geometryMap &) { return expr::meas_expr(); }
 inline expr::gg_expr operator*(const igrad_t &a, const igrad_t &b)
 {
@@ -1340,8 +1357,11 @@ public:
 
 // ---------------------------------------------------------------- evaluation
 /// gsExprEvaluator (src/gsAssembler/gsExprEvaluator.h:38,382-450) for the
-/// L2 error of a discrete solution against u = prod sin(pi x_i):
+/// error of a discrete solution against u = prod sin(pi x_i), the two
+/// integrals of examples/poisson2_example.cpp:173-176:
 ///   math::sqrt(ev.integral((u_ex - u_sol).sqNorm() * meas(G)))
+///   math::sqrt(ev.integral((igrad(u_ex) - igrad(u_sol, G)).sqNorm() * meas(G)))
+/// (fixed Dirichlet values included once boundary data is set)
 template <class T = real_t>
 class gsExprEvaluator {
 public:
@@ -1358,7 +1378,21 @@ public:
         const T err = l2ErrorSinProduct(*e.d.coefs);
         return err * err;
     }
+    T integral(const expr::h1_integrand &e) const
+    {
+        T l2 = 0, semi = 0;
+        errorNorms(*e.d.coefs, l2, semi);
+        return semi * semi;
+    }
     T l2ErrorSinProduct(const gsMatrix<T> &sol) const
+    {
+        T l2 = 0, semi = 0;
+        errorNorms(sol, l2, semi);
+        return l2;
+    }
+
+private:
+    void errorNorms(const gsMatrix<T> &sol, T &l2, T &semi) const
     {
         std::shared_ptr<gsx_problem> prob = m_ea ? m_ea->problem() : m_pa->problem();
         if (!prob) throw std::runtime_error("gsExprEvaluator: nothing assembled yet");
@@ -1367,12 +1401,12 @@ public:
         if (sol.rows() != nfree) throw std::runtime_error("gsExprEvaluator: solution size does not match the space");
         detail::DeviceVec s(sizeof(double) * std::max<int64_t>(1, nfree));
         detail::check(gsx_copy_to_device(detail::ctx(), s.p, sol.data(), sizeof(double) * nfree), "upload");
-        double e = 0;
-        detail::check(gsx_l2_error(prob.get(), s.d(), &e), "gsx_l2_error");
-        return e;
+        double e = 0, h = 0;
+        detail::check(gsx_error_norms(prob.get(), s.d(), &e, &h), "gsx_error_norms");
+        l2 = e;
+        semi = h;
     }
 
-private:
     const gsExprAssembler<T> *m_ea = nullptr;
     const gsPoissonAssembler<T> *m_pa = nullptr;
 };

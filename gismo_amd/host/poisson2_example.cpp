@@ -1,6 +1,6 @@
 // poisson2_example -- the call sequence of G+Smo's examples/poisson2_example.cpp
-// (gsExprAssembler + CG with diagonal preconditioning + L2 error per
-// refinement level) on the drop-in classes of gismo.h.  The domain is
+// (gsExprAssembler + CG with diagonal preconditioning + L2 and H1 errors
+// per refinement level) on the drop-in classes of gismo.h.  The domain is
 // built with gsNurbsCreator instead of read from an XML file:
 //   -d 2|3        unit square / unit cube (default 2)
 //   --grid NX NY  NX x NY unit squares glued at their interfaces (2D)
@@ -19,7 +19,8 @@
 // filedata/pde/poisson2d_bvp.xml: MultiPatch, source id 1, conditions id 2,
 // exact solution id 3) supplies its own data.
 // Prints one line per level: dofs, CG iterations, relative residual, L2
-// error; then the convergence rates.
+// error, H1 error (L2 + H1 seminorm, as the reference example sums them);
+// then the convergence rates.
 #include "gismo.h"
 #include "gsXml.h"
 
@@ -113,7 +114,7 @@ int main(int argc, char *argv[])
         gsMatrix<> solVector;
         solution u_sol = A.getSolution(u, solVector);
 
-        std::vector<double> l2err;
+        std::vector<double> l2err, h1err;
         double ma_time = 0, slv_time = 0;
         for (int r = 0; r <= numRefine; ++r) {
             dbasis.uniformRefine();
@@ -150,16 +151,20 @@ int main(int argc, char *argv[])
             auto t2 = std::chrono::steady_clock::now();
             ma_time += std::chrono::duration<double>(t1 - t0).count();
             slv_time += std::chrono::duration<double>(t2 - t1).count();
-            const double e = full ? gsExprEvaluator<>(*pa).l2ErrorSinProduct(solVector)
-                                  : math::sqrt(ev.integral((u_ex - u_sol).sqNorm() * meas(G)));
+            gsExprEvaluator<> evp = full ? gsExprEvaluator<>(*pa) : ev;
+            const double e = math::sqrt(evp.integral((u_ex - u_sol).sqNorm() * meas(G)));
+            const double h = e + math::sqrt(evp.integral((igrad(u_ex) - igrad(u_sol, G)).sqNorm() * meas(G)));
             l2err.push_back(e);
-            std::printf("level %d dofs %d nnz %d iterations %d relres %.3e L2 %.17g\n", r, (int)K->rows(),
-                        (int)K->nonZeros(), (int)cg.iterations(), cg.error(), e);
+            h1err.push_back(h);
+            std::printf("level %d dofs %d nnz %d iterations %d relres %.3e L2 %.17g H1 %.17g\n", r, (int)K->rows(),
+                        (int)K->nonZeros(), (int)cg.iterations(), cg.error(), e, h);
         }
         std::printf("Assembly: %.4f s  Solving: %.4f s\n", ma_time, slv_time);
         if (l2err.size() > 1) {
             std::printf("EoC (L2):");
             for (size_t i = 1; i < l2err.size(); ++i) std::printf(" %.2f", std::log(l2err[i - 1] / l2err[i]) / std::log(2.0));
+            std::printf("\nEoC (H1):");
+            for (size_t i = 1; i < h1err.size(); ++i) std::printf(" %.2f", std::log(h1err[i - 1] / h1err[i]) / std::log(2.0));
             std::printf("\n");
         }
     } catch (const std::exception &ex) {

@@ -37,6 +37,10 @@ def _levels(text):
     return rows
 
 
+def _h1(text):
+    return np.array([float(m[1]) for m in re.finditer(r"level \d+ .* H1 (\S+)", text)])
+
+
 def test_poisson2_example_square_p2():
     rows = _levels(_run("poisson2_example", "-r", "3", "-e", "1"))
     assert len(rows) == 4
@@ -153,7 +157,8 @@ def _bvp_oracle(d, p, n, dsides, nsides, mode):
     if nsides:
         b = b + O.neumann_load(prob.patches, dofs, nfree, [(0, s, 2, 1, ()) for s in nsides])
     xo, ito, _ = O.cg_solve(A, b, tol=1e-10, maxit=10000)
-    return nfree, A.nnz, ito, O.l2_error(prob.patches, dofs, nfree, xo, fixed=fixed)
+    eo = O.l2_error(prob.patches, dofs, nfree, xo, fixed=fixed)
+    return nfree, A.nnz, ito, eo, eo + O.h1_error(prob.patches, dofs, nfree, xo, fixed=fixed)
 
 
 @pytest.mark.parametrize("d,args,rate", [(2, ["-e", "1", "-r", "3"], 2.7),
@@ -165,19 +170,26 @@ def test_poisson2_example_bvp(d, args, rate):
     u * g_N.tr() * nv(G).  Per level the DOFs, pattern, CG iterations and
     L2 error (fixed values included) agree with the oracle on the same
     problem; the error falls at order p + 1 = 3."""
-    rows = _levels(_run("poisson2_example", "--bvp", *args))
+    text = _run("poisson2_example", "--bvp", *args)
+    rows = _levels(text)
+    h1 = _h1(text)
+    assert len(h1) == len(rows)
     lower = [1, 3, 5][:d]
     upper = [2, 4, 6][:d]
     for lvl, (dofs, nnz, its, rel, l2) in enumerate(rows):
-        nfree, onnz, ito, eo = _bvp_oracle(d, 2, 2 ** (lvl + 1), lower, upper, 0)
+        nfree, onnz, ito, eo, ho = _bvp_oracle(d, 2, 2 ** (lvl + 1), lower, upper, 0)
         assert (dofs, nnz) == (nfree, onnz)
         assert abs(its - ito) <= 2, (lvl, its, ito)
         assert rel < 1e-10
         assert abs(l2 - eo) <= 1e-4 * eo, (lvl, l2, eo)
+        assert abs(h1[lvl] - ho) <= 1e-4 * ho, (lvl, h1[lvl], ho)
     l2 = np.array([r[4] for r in rows])
     rates = np.log2(l2[:-1] / l2[1:])
-    print(d, l2, rates, flush=True)
+    h1rates = np.log2(h1[:-1] / h1[1:])
+    print(d, l2, rates, h1, h1rates, flush=True)
     assert rates[-1] > rate, rates
+    # H1 error falls at order p = 2
+    assert h1rates[-1] > 1.8, h1rates
 
 
 def test_poisson2_example_bvp_poisson_assembler():
@@ -186,7 +198,7 @@ def test_poisson2_example_bvp_poisson_assembler():
     pattern, against the oracle; converges at order 3."""
     rows = _levels(_run("poisson2_example", "--bvp", "--full", "-e", "1", "-r", "3"))
     for lvl, (dofs, nnz, its, rel, l2) in enumerate(rows):
-        nfree, onnz, ito, eo = _bvp_oracle(2, 2, 2 ** (lvl + 1), [1, 2, 3, 4], [], 1)
+        nfree, onnz, ito, eo, _ = _bvp_oracle(2, 2, 2 ** (lvl + 1), [1, 2, 3, 4], [], 1)
         assert (dofs, nnz) == (nfree, onnz)
         assert abs(its - ito) <= 2, (lvl, its, ito)
         assert abs(l2 - eo) <= 1e-4 * eo, (lvl, l2, eo)
