@@ -661,39 +661,71 @@ __global__ __launch_bounds__(256) void k_gather(const PatchDev *__restrict__ pat
             nb1[k] = pd.p[k] + 1;
         }
         const int NB = nb1[0] * nb1[1] * nb1[2];
-        for (int ez = elo[2]; ez <= ehi[2]; ++ez)
-            for (int ey = elo[1]; ey <= ehi[1]; ++ey)
-                for (int ex = elo[0]; ex <= ehi[0]; ++ex) {
+        const int ne0 = ehi[0] - elo[0] + 1, ne1 = ehi[1] - elo[1] + 1;
+        const int ne = ne0 * ne1 * (ehi[2] - elo[2] + 1);
+        // Rows of short elements (NB < 64) are read EPL elements per wave
+        // load (lane = sub-element * NB + b); the sub-elements are then applied
+        // one after the other, so every entry still sums its elements in order.
+        const int EPL = NB >= kWave ? 1 : kWave / NB;
+        // element t's load entry F_e(lb), added to the rhs after its
+        // eliminated columns, element by element
+        auto load = [&](int t) {
+            const int ex = elo[0] + t % ne0, ey = elo[1] + (t / ne0) % ne1, ez = elo[2] + t / (ne0 * ne1);
+            const int64_t el = ex + (int64_t)pd.nel[0] * (ey + (int64_t)pd.nel[1] * ez);
+            const int lb = (l[0] - ex) + nb1[0] * ((l[1] - ey) + nb1[1] * (l[2] - ez));
+            return Fbuf[pd.fvec_off + el * NB + lb];
+        };
+        for (int t0 = 0; t0 < ne; t0 += EPL) {
+            const int nsub = min(EPL, ne - t0);
+            double elim1 = 0.0;   // EPL == 1: eliminated columns over the b chunks
+            for (int b0 = 0; b0 < NB; b0 += kWave) {
+                const int sub = NB >= kWave ? 0 : lane / NB;
+                const int b = NB >= kWave ? b0 + lane : lane % NB;
+                const int t = t0 + sub;
+                int pos = -1;
+                double v = 0.0, el_v = 0.0;
+                if (sub < nsub && b < NB) {
+                    const int ex = elo[0] + t % ne0, ey = elo[1] + (t / ne0) % ne1, ez = elo[2] + t / (ne0 * ne1);
                     const int64_t el = ex + (int64_t)pd.nel[0] * (ey + (int64_t)pd.nel[1] * ez);
                     const int lb = (l[0] - ex) + nb1[0] * ((l[1] - ey) + nb1[1] * (l[2] - ez));
-                    const double *Krow = Kbuf + pd.kmat_off + el * NB * NB + (int64_t)lb * NB;
-                    double elim = 0.0;
-                    for (int b = lane; b < NB; b += kWave) {
-                        const int bx = b % nb1[0], by = (b / nb1[0]) % nb1[1], bz = b / (nb1[0] * nb1[1]);
-                        const int64_t m = (ex + bx) + (int64_t)pd.nb[0] * ((ey + by) + (int64_t)pd.nb[1] * (ez + bz));
-                        const int g = pd.dofs[m];
-                        if (g < nfree) {
-                            int lo = 0, hi = len - 1;
-                            while (lo < hi) {
-                                const int mid = (lo + hi) >> 1;
-                                if (sCol[w][mid] < g) lo = mid + 1; else hi = mid;
-                            }
-                            const double v = Krow[b];
-                            sAcc[w][lo] += v;
-                            sNz[w][lo] |= (v != 0.0);
-                        } else if (fixed) {
-                            // eliminated column: K_ij g_j moves to the rhs
-                            elim += Krow[b] * fixed[g - nfree];
+                    const double kv = Kbuf[pd.kmat_off + el * NB * NB + (int64_t)lb * NB + b];
+                    const int bx = b % nb1[0], by = (b / nb1[0]) % nb1[1], bz = b / (nb1[0] * nb1[1]);
+                    const int64_t m = (ex + bx) + (int64_t)pd.nb[0] * ((ey + by) + (int64_t)pd.nb[1] * (ez + bz));
+                    const int g = pd.dofs[m];
+                    if (g < nfree) {
+                        int lo = 0, hi = len - 1;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if (sCol[w][mid] < g) lo = mid + 1; else hi = mid;
                         }
+                        pos = lo;
+                        v = kv;
+                    } else if (fixed) {
+                        // eliminated column: K_ij g_j moves to the rhs
+                        el_v = kv * fixed[g - nfree];
+                    }
+                }
+                for (int s2 = 0; s2 < nsub; ++s2) {
+                    if (sub == s2 && pos >= 0) {
+                        sAcc[w][pos] += v;
+                        sNz[w][pos] |= (v != 0.0);
                     }
                     __builtin_amdgcn_wave_barrier();
                     if (fixed) {
+                        double e = sub == s2 ? el_v : 0.0;
 #pragma unroll
-                        for (int off = 32; off > 0; off >>= 1) elim += __shfl_xor(elim, off, kWave);
-                        racc -= elim;
+                        for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, kWave);
+                        if (EPL == 1) elim1 += e;
+                        else racc -= e;
                     }
-                    if (Fbuf) racc += Fbuf[pd.fvec_off + el * NB + lb];
+                    if (EPL > 1 && Fbuf) racc += load(t0 + s2);
                 }
+            }
+            if (EPL == 1) {
+                if (fixed) racc -= elim1;
+                if (Fbuf) racc += load(t0);
+            }
+        }
     }
     __builtin_amdgcn_wave_barrier();
     for (int k = lane; k < len; k += kWave) {
