@@ -48,9 +48,16 @@ def cpu_model():
     return platform.processor()
 
 
-def run_step(ctx, gp, gsx, precision, tol, maxit, x, keep=False):
+def run_step(ctx, gp, gsx, precision, tol, maxit, x, keep=False, fd=False):
+    """One step: assemble, then CG to tol with the Jacobi preconditioner, or
+    (fd) with the fast diagonalization built in the step."""
     A, b = gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_REBUILD_PATTERN, precision)
-    it, rel, ok = A.cg_solve(b, x, tol=tol, maxit=maxit, precision=precision)
+    if fd:
+        F = gsx.FastDiag(gp)
+        it, rel, ok = gsx.fd_cg_solve(A, F, b, x, tol=tol, maxit=maxit, precision=precision)
+        F.close()
+    else:
+        it, rel, ok = A.cg_solve(b, x, tol=tol, maxit=maxit, precision=precision)
     ctx.synchronize()
     if keep:
         return A, b, it, rel, ok
@@ -140,10 +147,11 @@ def main():
     xr.close()
 
     results = {}
-    for name, prec in (("fp64", gsx.FP64), ("mixed", gsx.MIXED)):
+    for name, prec, fd in (("fp64", gsx.FP64, False), ("mixed", gsx.MIXED, False),
+                           ("fd_fp64", gsx.FP64, True), ("fd_mixed", gsx.MIXED, True)):
         for _ in range(args.warmup):
             x.zero()
-            run_step(ctx, gp, gsx, prec, args.tol, args.maxit, x)
+            run_step(ctx, gp, gsx, prec, args.tol, args.maxit, x, fd=fd)
         # time the dominant SpMV with HIP events on the context stream
         kname = "spmv_f32" if prec == gsx.MIXED else "spmv_f64"
         ctx.enable_timing(True)
@@ -157,7 +165,7 @@ def main():
             x.zero()
             ctx.synchronize()
             t0 = time.perf_counter()
-            _, _, it, rel, ok = run_step(ctx, gp, gsx, prec, args.tol, args.maxit, x)
+            _, _, it, rel, ok = run_step(ctx, gp, gsx, prec, args.tol, args.maxit, x, fd=fd)
             times.append(time.perf_counter() - t0)
             its.append(it)
         ctx.synchronize()
@@ -180,12 +188,21 @@ def main():
     ctx.reset_timing()
     x.zero()
     run_step(ctx, gp, gsx, gsx.MIXED, args.tol, args.maxit, x)
-    phases = {}
-    for k in ("pattern_count", "pattern_scan", "pattern_fill", "elem3d", "gather", "convert_f32",
-              "stencil_build", "cg_init", "spmv_f32", "cg_vec_f32", "reliable_update"):
-        nl, ms = ctx.kernel_stats(k)
-        if nl:
-            phases[k] = round(ms, 3)
+    keys = ("pattern_count", "pattern_scan", "pattern_fill", "elem3d", "gather", "convert_f32",
+            "stencil_build", "cg_init", "spmv_f32", "cg_vec_f32", "fd_apply_f32", "reliable_update")
+
+    def phase_split():
+        out = {}
+        for k in keys:
+            nl, ms = ctx.kernel_stats(k)
+            if nl:
+                out[k] = round(ms, 3)
+        return out
+    phases = phase_split()
+    ctx.reset_timing()
+    x.zero()
+    run_step(ctx, gp, gsx, gsx.MIXED, args.tol, args.maxit, x, fd=True)
+    phases_fd = phase_split()
     ctx.enable_timing(False)
 
     A, b = gp.assemble(gsx.PATTERN_EXPR, gsx.MIXED)
@@ -220,6 +237,11 @@ def main():
         "fp64": results["fp64"],
         "mixed": mixed,
         "speedup_mixed_vs_fp64": results["fp64"]["ms"] / mixed["ms"],
+        # the same steps with CG preconditioned by the reference's
+        # fastDiagonalizationOp (built in every step) instead of Jacobi
+        "fd_pcg": {"fp64": results["fd_fp64"], "mixed": results["fd_mixed"],
+                   "speedup_vs_jacobi_mixed": mixed["ms"] / results["fd_mixed"]["ms"],
+                   "phase_ms_one_mixed_step": phases_fd},
         "reference_solution": {"relres": rel_ref, "iters": it_ref, "l2_error_vs_exact": l2_ref},
         "phase_ms_one_mixed_step": phases,
         "roofline": {"kernel": "spmv_f32 (structured (2p+1)^3 stencil, FP32 coefficients)", "bound": "hbm",

@@ -186,6 +186,22 @@ hipError_t launch_cg2_direction_f64(int n, double *p, const double *z, const dou
 hipError_t launch_cg2_direction_f32_mixed(int n, float *p, const float *z, const double *prr,
                                           const double *prz, int np, CGState *st, int k,
                                           hipStream_t s);
+hipError_t launch_cg2_dot_f64(int n, const double *a, const double *b, double *part, const CGState *st,
+                              hipStream_t s);
+hipError_t launch_cg2_dot_f32(int n, const float *a, const float *b, double *part, const CGState *st,
+                              hipStream_t s);
+// fast diagonalization: one mode product y_a = sum_i M[i*m + a] x_i along
+// direction dir of the m0 x m1 x m2 box (element (a,b,c) of x at
+// xo + a + b xs1 + c xs2, of y at yo + a + b ys1 + c ys2); with l0 != null
+// the result is divided by alpha + l0[a] + l1[b] (+ l2[c])
+hipError_t launch_fd_mode_f64(int m0, int m1, int m2, int dir, const double *M, const double *x, int64_t xo,
+                              int64_t xs1, int64_t xs2, double *y, int64_t yo, int64_t ys1, int64_t ys2,
+                              const double *l0, const double *l1, const double *l2, double alpha,
+                              const CGState *st, hipStream_t s);
+hipError_t launch_fd_mode_f32(int m0, int m1, int m2, int dir, const float *M, const float *x, int64_t xo,
+                              int64_t xs1, int64_t xs2, float *y, int64_t yo, int64_t ys1, int64_t ys2,
+                              const double *l0, const double *l1, const double *l2, double alpha,
+                              const CGState *st, hipStream_t s);
 hipError_t launch_cg2_precond_f64(int n, const double *r, double *z, const double *diag,
                                   double tau, double *prz, hipStream_t s);
 hipError_t launch_cg2_precond_f32(int n, const float *r, float *z, const float *diag, double tau,

@@ -16,9 +16,11 @@
 #include <numeric>
 #include <string>
 #include <deque>
+#include <type_traits>
 #include <vector>
 
 #include "gsx_common.h"
+#include "gsx_dense.h"
 #include "gsx_kernels.h"
 
 using gsx::CGState;
@@ -1566,11 +1568,81 @@ static gsx_status stencil_prepare(gsx_matrix *A, bool mixed)
     return GSX_OK;
 }
 
+// ===========================================================================
+// Fast diagonalization (gsPatchPreconditionersCreator::fastDiagonalizationOp,
+// src/gsSolver/gsPatchPreconditionersCreator.hpp:258-348) on the free box of
+// a single patch: univariate mass M_k and stiffness beta K_k of each knot
+// vector on the parameter line (Dirichlet ends removed, as
+// eliminateDirichlet1D does), K_k q = lambda M_k q with Q_k^T M_k Q_k = I,
+// and z = (Q_{d-1} x .. x Q_0) diag(1/(alpha + sum lambda)) (..)^T r, which
+// is the exact inverse of alpha M + sum_k M x .. x beta K_k x .. x M.
+// ===========================================================================
+struct gsx_fd {
+    gsx_ctx *ctx = nullptr;
+    int d = 0;
+    int m[3] = {1, 1, 1};
+    int64_t size = 0;
+    double alpha = 0.0;
+    DevPtr Q64[3], QT64[3], Q32[3], QT32[3], lam[3];
+    DevPtr t64[2], t32[2];
+    int grid[3] = {1, 1, 1};
+    int64_t goff = 0, gs1 = 0, gs2 = 0;   // free box on the padded grid
+};
+
+static hipError_t fd_mode(int m0, int m1, int m2, int dir, const double *M, const double *x, int64_t xo,
+                          int64_t xs1, int64_t xs2, double *y, int64_t yo, int64_t ys1, int64_t ys2,
+                          const double *l0, const double *l1, const double *l2, double alpha,
+                          const gsx::CGState *st, hipStream_t s)
+{
+    return gsx::launch_fd_mode_f64(m0, m1, m2, dir, M, x, xo, xs1, xs2, y, yo, ys1, ys2, l0, l1, l2, alpha, st, s);
+}
+
+static hipError_t fd_mode(int m0, int m1, int m2, int dir, const float *M, const float *x, int64_t xo,
+                          int64_t xs1, int64_t xs2, float *y, int64_t yo, int64_t ys1, int64_t ys2,
+                          const double *l0, const double *l1, const double *l2, double alpha,
+                          const gsx::CGState *st, hipStream_t s)
+{
+    return gsx::launch_fd_mode_f32(m0, m1, m2, dir, M, x, xo, xs1, xs2, y, yo, ys1, ys2, l0, l1, l2, alpha, st, s);
+}
+
+// y = FD(x): element (a,b,c) of x at xo + a + b xs1 + c xs2 (likewise y); the
+// 2d mode products alternate between the two compact scratch tensors.
+template <typename T>
+static hipError_t fd_apply(const gsx_fd *F, const T *x, int64_t xo, int64_t xs1, int64_t xs2, T *y, int64_t yo,
+                           int64_t ys1, int64_t ys2, const gsx::CGState *st, hipStream_t s)
+{
+    constexpr bool f64 = std::is_same<T, double>::value;
+    T *buf[2] = {static_cast<T *>(f64 ? F->t64[0].get() : F->t32[0].get()),
+                 static_cast<T *>(f64 ? F->t64[1].get() : F->t32[1].get())};
+    const int D = F->d, m0 = F->m[0], m1 = F->m[1], m2 = F->m[2];
+    const double *l0 = dp<double>(F->lam[0]), *l1 = dp<double>(F->lam[1]);
+    const double *l2 = D == 3 ? dp<double>(F->lam[2]) : nullptr;
+    const T *in = x;
+    int64_t io = xo, is1 = xs1, is2 = xs2;
+    for (int k = 0; k < 2 * D; ++k) {
+        const bool forward = k < D;   // Q^T along 0..D-1, then Q along D-1..0
+        const int dir = forward ? k : 2 * D - 1 - k;
+        const T *M = static_cast<const T *>(forward ? (f64 ? F->Q64[dir].get() : F->Q32[dir].get())
+                                                    : (f64 ? F->QT64[dir].get() : F->QT32[dir].get()));
+        const bool last = k == 2 * D - 1;
+        T *out = last ? y : buf[k & 1];
+        const int64_t oo = last ? yo : 0, os1 = last ? ys1 : m0, os2 = last ? ys2 : (int64_t)m0 * m1;
+        const hipError_t e = fd_mode(m0, m1, m2, dir, M, in, io, is1, is2, out, oo, os1, os2,
+                                     k == D - 1 ? l0 : nullptr, l1, l2, F->alpha, st, s);
+        if (e != hipSuccess) return e;
+        in = out;
+        io = oo;
+        is1 = os1;
+        is2 = os2;
+    }
+    return hipSuccess;
+}
+
 // Grid-layout CG.  mixed == false: FP64 with the reference's recurrences and
 // stopping rule.  mixed == true: FP32 recurrences on the FP32 copy of the
 // operator with FP64 reliable updates (see cg_mixed).
 static gsx_status cg_grid(gsx_matrix *A, int precond, double tau, const double *b, double *x,
-                          double tol, int maxit, bool mixed, int *iters, double *relres)
+                          double tol, int maxit, bool mixed, int *iters, double *relres, const gsx_fd *fd = nullptr)
 {
     gsx_ctx *c = A->ctx;
     hipStream_t s = c->stream;
@@ -1605,8 +1677,20 @@ static gsx_status cg_grid(gsx_matrix *A, int precond, double tau, const double *
     GSX_TRY(read_state(c, w, h));
     if (h.done == 3) HIP_OK(c, gsx::launch_fill_f64(x, n, 0.0, s));
     int k = 0;
+    // the fast diagonalization applies between the update (which then only
+    // moves x and r) and the direction, followed by the (r, z) partials
+    auto fd64 = [&](const gsx::CGState *sp, double *part) -> gsx_status {
+        KScope kf(c, "fd_apply");
+        HIP_OK(c, fd_apply<double>(fd, rg, fd->goff, fd->gs1, fd->gs2, zg, fd->goff, fd->gs1, fd->gs2, sp, s));
+        HIP_OK(c, gsx::launch_cg2_dot_f64((int)N, rg, zg, part, sp, s));
+        return GSX_OK;
+    };
     if (!h.done && !mixed) {
-        HIP_OK(c, gsx::launch_cg2_precond_f64((int)N, rg, zg, jac ? dg : nullptr, tau, P0, s));
+        if (fd) {
+            GSX_TRY(fd64(nullptr, P0));
+        } else {
+            HIP_OK(c, gsx::launch_cg2_precond_f64((int)N, rg, zg, jac ? dg : nullptr, tau, P0, s));
+        }
         HIP_OK(c, gsx::launch_set_rz(P0, nv, st, 0, s));
         HIP_OK(c, hipMemcpyAsync(pg, zg, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
         const int chunk = 16;
@@ -1617,9 +1701,13 @@ static gsx_status cg_grid(gsx_matrix *A, int precond, double tau, const double *
                     KScope ks(c, "spmv_f64");
                     HIP_OK(c, gsx::launch_sspmv_f64(S.d, S.p, 1, n, f2g, sy, sz, sv, pg, tg, nullptr, P0, st, k, s));
                 }
+                {
+                    KScope kv(c, "cg_vec");
+                    HIP_OK(c, gsx::launch_cg2_update_f64((int)N, xg, rg, pg, tg, zg, jac && !fd ? dg : nullptr, tau,
+                                                         P0, nsp, P1, P2, st, k, s));
+                }
+                if (fd) GSX_TRY(fd64(st, P2));
                 KScope kv(c, "cg_vec");
-                HIP_OK(c, gsx::launch_cg2_update_f64((int)N, xg, rg, pg, tg, zg, jac ? dg : nullptr, tau, P0,
-                                                     nsp, P1, P2, st, k, s));
                 HIP_OK(c, gsx::launch_cg2_direction_f64((int)N, pg, zg, P1, P2, nv, st, k, hist, s));
             }
             GSX_TRY(read_state(c, w, h));
@@ -1630,9 +1718,19 @@ static gsx_status cg_grid(gsx_matrix *A, int precond, double tau, const double *
         float *r32 = dp<float>(S.r32), *z32 = dp<float>(S.z32), *p32 = dp<float>(S.p32);
         float *t32 = dp<float>(S.t32), *xc = dp<float>(S.xc32);
         const float *sv32 = dp<float>(S.s32), *d32 = dp<float>(S.diag32);
+        auto fd32 = [&](const gsx::CGState *sp, double *part) -> gsx_status {
+            KScope kf(c, "fd_apply_f32");
+            HIP_OK(c, fd_apply<float>(fd, r32, fd->goff, fd->gs1, fd->gs2, z32, fd->goff, fd->gs1, fd->gs2, sp, s));
+            HIP_OK(c, gsx::launch_cg2_dot_f32((int)N, r32, z32, part, sp, s));
+            return GSX_OK;
+        };
         auto restart = [&](int slot) -> gsx_status {
             HIP_OK(c, gsx::launch_grid_f64_to_f32(N, rg, r32, s));
-            HIP_OK(c, gsx::launch_cg2_precond_f32((int)N, r32, z32, jac ? d32 : nullptr, tau, P0, s));
+            if (fd) {
+                GSX_TRY(fd32(nullptr, P0));
+            } else {
+                HIP_OK(c, gsx::launch_cg2_precond_f32((int)N, r32, z32, jac ? d32 : nullptr, tau, P0, s));
+            }
             HIP_OK(c, gsx::launch_set_rz(P0, nv, st, slot, s));
             return GSX_OK;
         };
@@ -1646,9 +1744,13 @@ static gsx_status cg_grid(gsx_matrix *A, int precond, double tau, const double *
                     KScope ks(c, "spmv_f32");
                     HIP_OK(c, gsx::launch_sspmv_f32(S.d, S.p, 1, n, f2g, sy, sz, sv32, p32, t32, nullptr, P0, st, k, s));
                 }
+                {
+                    KScope kv(c, "cg_vec_f32");
+                    HIP_OK(c, gsx::launch_cg2_update_f32((int)N, xc, r32, p32, t32, z32, jac && !fd ? d32 : nullptr,
+                                                         tau, P0, nsp, P1, P2, st, k, s));
+                }
+                if (fd) GSX_TRY(fd32(st, P2));
                 KScope kv(c, "cg_vec_f32");
-                HIP_OK(c, gsx::launch_cg2_update_f32((int)N, xc, r32, p32, t32, z32, jac ? d32 : nullptr, tau, P0,
-                                                     nsp, P1, P2, st, k, s));
                 HIP_OK(c, gsx::launch_cg2_direction_f32_mixed((int)N, p32, z32, P1, P2, nv, st, k, s));
             }
             GSX_TRY(read_state(c, w, h));
@@ -1731,6 +1833,121 @@ gsx_status gsx_cg_solve(gsx_matrix *A, int32_t precond, double tau, gsx_mg *mg, 
         e = mixed ? cg_mixed(A, precond, tau, b, x, tol, maxit, &it, &rr)
                   : cg_fp64(A, precond, tau, b, x, tol, maxit, &it, &rr);
     }
+    if (iters) *iters = it;
+    if (relres) *relres = rr;
+    return e;
+}
+
+gsx_status gsx_fd_create(gsx_problem *P, double alpha, double beta, gsx_fd **out)
+{
+    if (!P || !out) return GSX_ERR_ARG;
+    gsx_ctx *c = P->ctx;
+    hipStream_t s = c->stream;
+    if (P->npatch != 1 || !P->d_f2g)
+        return fail(c, GSX_ERR_ARG, "fast diagonalization needs a single-patch problem whose free functions form a box");
+    if (!(beta > 0.0) || !(alpha >= 0.0))
+        return fail(c, GSX_ERR_ARG, "fast diagonalization needs beta > 0 and alpha >= 0");
+    const HostPatch &h = P->hp[0];
+    auto F = std::make_unique<gsx_fd>();
+    F->ctx = c;
+    F->d = h.d;
+    F->alpha = alpha;
+    double lmin = alpha;
+    std::vector<std::vector<double>> hold64;   // host sources of the uploads, alive until the sync
+    std::vector<std::vector<float>> hold32;
+    auto up64 = [&](DevPtr &dst, std::vector<double> v) -> bool {
+        dst = dalloc(c, sizeof(double) * std::max<size_t>(1, v.size()));
+        if (!dst) return false;
+        hold64.push_back(std::move(v));
+        return hipMemcpyAsync(dst.get(), hold64.back().data(), sizeof(double) * hold64.back().size(),
+                              hipMemcpyHostToDevice, s) == hipSuccess;
+    };
+    auto up32 = [&](DevPtr &dst, const std::vector<double> &v) -> bool {
+        dst = dalloc(c, sizeof(float) * std::max<size_t>(1, v.size()));
+        if (!dst) return false;
+        hold32.emplace_back(v.begin(), v.end());
+        return hipMemcpyAsync(dst.get(), hold32.back().data(), sizeof(float) * hold32.back().size(),
+                              hipMemcpyHostToDevice, s) == hipSuccess;
+    };
+    for (int k = 0; k < h.d; ++k) {
+        const int p = h.p[k], n = h.nb[k], lo = P->box_lo[k], m = P->box_hi[k] - lo + 1;
+        if (m <= 0) return fail(c, GSX_ERR_ARG, "fast diagonalization: no free functions in a direction");
+        // univariate matrices with the assembly's p+1 Gauss points per span
+        std::vector<double> gx(p + 1), gw(p + 1), K, M;
+        gauss_rule(p + 1, gx.data(), gw.data());
+        gsx::dense::bspline_1d_matrices(P->knots_h[k], p, gx, gw, K, M);
+        std::vector<double> Kf((size_t)m * m), Mf((size_t)m * m), lam, Q;
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j) {
+                Kf[(size_t)i * m + j] = beta * K[(size_t)(lo + i) * n + (lo + j)];
+                Mf[(size_t)i * m + j] = M[(size_t)(lo + i) * n + (lo + j)];
+            }
+        if (!gsx::dense::gen_sym_eig(m, Kf, Mf, lam, Q))
+            return fail(c, GSX_ERR_ARG, "fast diagonalization: the univariate eigenproblem failed");
+        lmin += lam[0];
+        std::vector<double> QT((size_t)m * m);
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j) QT[(size_t)j * m + i] = Q[(size_t)i * m + j];
+        F->m[k] = m;
+        if (!up32(F->Q32[k], Q) || !up32(F->QT32[k], QT) || !up64(F->Q64[k], std::move(Q)) ||
+            !up64(F->QT64[k], std::move(QT)) || !up64(F->lam[k], std::move(lam)))
+            return fail(c, GSX_ERR_OOM, "device allocation (fast diagonalization)");
+    }
+    if (!(lmin > 0.0))
+        return fail(c, GSX_ERR_ARG, "fast diagonalization is singular (no Dirichlet end in any direction and alpha = 0)");
+    F->size = (int64_t)F->m[0] * F->m[1] * F->m[2];
+    if (F->size != P->nfree) return fail(c, GSX_ERR_ARG, "fast diagonalization: free box does not match the problem");
+    for (int j = 0; j < 2; ++j) {
+        F->t64[j] = dalloc(c, sizeof(double) * F->size);
+        F->t32[j] = dalloc(c, sizeof(float) * F->size);
+        if (!F->t64[j] || !F->t32[j]) return fail(c, GSX_ERR_OOM, "device allocation (fast diagonalization)");
+    }
+    const int pg = h.p[0], pz = h.d == 3 ? h.p[0] : 0;
+    for (int k = 0; k < 3; ++k) F->grid[k] = P->grid_dims[k];
+    F->gs1 = P->grid_dims[0];
+    F->gs2 = (int64_t)P->grid_dims[0] * P->grid_dims[1];
+    F->goff = (P->box_lo[0] + pg) + F->gs1 * (P->box_lo[1] + pg) + F->gs2 * (P->box_lo[2] + pz);
+    HIP_OK(c, hipStreamSynchronize(s));
+    *out = F.release();
+    return GSX_OK;
+}
+
+gsx_status gsx_fd_destroy(gsx_fd *F)
+{
+    delete F;
+    return GSX_OK;
+}
+
+gsx_status gsx_fd_apply(gsx_fd *F, const double *r, double *z)
+{
+    if (!F || !r || !z) return GSX_ERR_ARG;
+    gsx_ctx *c = F->ctx;
+    KScope k(c, "fd_apply");
+    HIP_OK(c, fd_apply<double>(F, r, 0, F->m[0], (int64_t)F->m[0] * F->m[1], z, 0, F->m[0],
+                               (int64_t)F->m[0] * F->m[1], nullptr, c->stream));
+    return GSX_OK;
+}
+
+gsx_status gsx_cg_solve_fd(gsx_matrix *A, gsx_fd *F, int32_t precision, const double *b, double *x, double tol,
+                           int32_t maxit, int32_t *iters, double *relres)
+{
+    if (!A || !F || !b || !x || maxit < 0) return GSX_ERR_ARG;
+    gsx_ctx *c = A->ctx;
+    if (F->ctx != c) return fail(c, GSX_ERR_ARG, "matrix and fast diagonalization live on different contexts");
+    if (!A->st.available)
+        return fail(c, GSX_ERR_ARG, "fast diagonalization needs the structured operator of a single-patch matrix");
+    if (A->n != F->size || A->st.nx != F->grid[0] || A->st.ny != F->grid[1] || A->st.nz != F->grid[2])
+        return fail(c, GSX_ERR_ARG, "matrix and fast diagonalization belong to different problems");
+    if (precision == GSX_MIXED && !A->mixed)
+        return fail(c, GSX_ERR_ARG, "mixed precision needs a matrix assembled with GSX_MIXED");
+    const bool mixed = precision == GSX_MIXED;
+    gsx_status e = cg_workspace(A, std::max(1, maxit), false, true);
+    if (e != GSX_OK) return e;
+    e = stencil_prepare(A, mixed);
+    if (e != GSX_OK) return e;
+    int it = 0;
+    double rr = 0.0;
+    e = cg_grid(A, GSX_PREC_NONE, 1.0, b, x, tol, maxit, mixed, &it, &rr, F);
     if (iters) *iters = it;
     if (relres) *relres = rr;
     return e;

@@ -276,6 +276,20 @@ __global__ __launch_bounds__(256) void k_cg2_precond(int n, const T *__restrict_
     if (threadIdx.x == 0) part_rz[blockIdx.x] = c;
 }
 
+// partial (a, b) in the grid solvers' block layout (a preconditioner that
+// is not fused into the update writes z, then this gives the (r, z) partials)
+template <typename T>
+__global__ __launch_bounds__(256) void k_cg2_dot(int n, const T *__restrict__ a, const T *__restrict__ b,
+                                                 double *__restrict__ part, const CGState *__restrict__ st)
+{
+    __shared__ double s[4];
+    if (st && st->done) return;
+    double v = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) v += (double)a[i] * (double)b[i];
+    const double c = bsum256(v, s);
+    if (threadIdx.x == 0) part[blockIdx.x] = c;
+}
+
 __global__ __launch_bounds__(256) void k_set_rz(const double *__restrict__ parts, int np, CGState *__restrict__ st,
                                                 int slot)
 {
@@ -421,6 +435,20 @@ hipError_t launch_cg2_direction_f32_mixed(int n, float *p, const float *z, const
 {
     hipLaunchKernelGGL((k_cg2_direction<float, 1>), dim3(vec_blocks(n)), dim3(256), 0, s, n, p, z, prr, prz,
                        np, st, k, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg2_dot_f64(int n, const double *a, const double *b, double *part, const CGState *st,
+                              hipStream_t s)
+{
+    hipLaunchKernelGGL((k_cg2_dot<double>), dim3(vec_blocks(n)), dim3(256), 0, s, n, a, b, part, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_cg2_dot_f32(int n, const float *a, const float *b, double *part, const CGState *st,
+                              hipStream_t s)
+{
+    hipLaunchKernelGGL((k_cg2_dot<float>), dim3(vec_blocks(n)), dim3(256), 0, s, n, a, b, part, st);
     return hipGetLastError();
 }
 

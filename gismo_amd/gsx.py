@@ -123,6 +123,10 @@ SIGNATURES = {
     "gsx_mg_create": [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
                       C.c_int32, _pp],
     "gsx_mg_destroy": [_vp],
+    "gsx_fd_create": [_vp, C.c_double, C.c_double, _pp],
+    "gsx_fd_destroy": [_vp],
+    "gsx_fd_apply": [_vp, _vp, _vp],
+    "gsx_cg_solve_fd": [_vp, _vp, C.c_int32, _vp, _vp, C.c_double, C.c_int32, _i32p, _dp],
     "gsx_mg_set_smoother": [_vp, C.c_int32, C.c_double],
     "gsx_mg_smoother_bound": [_vp, C.c_int32, _dp],
     "gsx_mg_apply": [_vp, _vp, _vp],
@@ -516,6 +520,36 @@ def mg_cg_solve(matrix, mg, b, x, tol=1e-8, maxit=500):
     rr = C.c_double()
     st = lib().gsx_cg_solve(matrix.h, PREC_MG, 1.0, mg.h, FP64, b.ptr, x.ptr, tol, maxit, C.byref(it),
                             C.byref(rr))
+    _check(matrix.ctx.h, st, (GSX_ERR_NOT_CONVERGED,))
+    return it.value, rr.value, st == GSX_OK
+
+
+class FastDiag:
+    """Fast diagonalization preconditioner of a single-patch problem
+    (gsPatchPreconditionersCreator::fastDiagonalizationOp): the exact inverse
+    of alpha M + beta sum_k M x .. x K_k x .. x M on the parameter domain."""
+
+    def __init__(self, problem, alpha=0.0, beta=1.0):
+        self.ctx = problem.ctx
+        h = C.c_void_p()
+        _check(self.ctx.h, lib().gsx_fd_create(problem.h, alpha, beta, C.byref(h)))
+        self.h = h
+
+    def apply(self, r, z):
+        _check(self.ctx.h, lib().gsx_fd_apply(self.h, r.ptr, z.ptr))
+
+    def close(self):
+        if self.h:
+            lib().gsx_fd_destroy(self.h)
+            self.h = None
+
+
+def fd_cg_solve(matrix, fd, b, x, tol=1e-8, maxit=500, precision=FP64):
+    """CG preconditioned by the fast diagonalization, on the structured
+    operator (FP32 iterations with FP64 reliable updates for MIXED)."""
+    it = C.c_int32()
+    rr = C.c_double()
+    st = lib().gsx_cg_solve_fd(matrix.h, fd.h, precision, b.ptr, x.ptr, tol, maxit, C.byref(it), C.byref(rr))
     _check(matrix.ctx.h, st, (GSX_ERR_NOT_CONVERGED,))
     return it.value, rr.value, st == GSX_OK
 

@@ -63,6 +63,7 @@ typedef enum {
 } gsx_smoother;
 
 typedef struct gsx_ctx gsx_ctx;
+typedef struct gsx_fd gsx_fd;
 typedef struct gsx_problem gsx_problem;
 typedef struct gsx_matrix gsx_matrix;
 typedef struct gsx_mg gsx_mg;
@@ -264,6 +265,27 @@ gsx_status gsx_cg_solve(gsx_matrix *A, int32_t precond, double tau,
                         int32_t *iters, double *relres);
 /* Relative residual history of the last FP64 solve of A (iters+1 values,
  * gsIterativeSolver::solveDetailed, gsIterativeSolver.h:134-165). */
+/* Fast diagonalization preconditioner of a single-patch problem
+ * (gsPatchPreconditionersCreator::fastDiagonalizationOp(basis, bc, opt,
+ * alpha, beta), src/gsSolver/gsPatchPreconditionersCreator.hpp:258-348):
+ * the exact inverse of alpha M + beta sum_k M x .. x K_k x .. x M, with K_k,
+ * M_k the univariate stiffness and mass of knot vector k on the parameter
+ * line (p+1 Gauss points per span, the Dirichlet ends of the problem
+ * removed, gsPatchPreconditionersCreator.hpp:27-126), applied as
+ * (Q x .. x Q) diag(1/(alpha + sum lambda)) (Q x .. x Q)^T with
+ * K_k Q_k = M_k Q_k diag(lambda_k), Q_k^T M_k Q_k = I.  Needs beta > 0 and,
+ * without a Dirichlet end in any direction, alpha > 0 (GSX_ERR_ARG). */
+gsx_status gsx_fd_create(gsx_problem *prob, double alpha, double beta, gsx_fd **out);
+gsx_status gsx_fd_destroy(gsx_fd *fd);
+/* z = FD r over the free DOFs (device vectors, nfree entries; may alias). */
+gsx_status gsx_fd_apply(gsx_fd *fd, const double *r, double *z);
+/* gsConjugateGradient(A, fastDiagonalizationOp(..)).solve(b, x)
+ * (src/gsSolver/gsConjugateGradient.hpp:20-93) on the structured operator of
+ * a matrix assembled from the fast diagonalization's problem; GSX_MIXED runs
+ * FP32 iterations (FP32 operator and factors) with FP64 reliable updates, as
+ * gsx_cg_solve does. */
+gsx_status gsx_cg_solve_fd(gsx_matrix *A, gsx_fd *fd, int32_t precision, const double *b, double *x,
+                           double tol, int32_t maxit, int32_t *iters, double *relres);
 gsx_status gsx_cg_history(gsx_matrix *A, double *hist_host, int32_t cap,
                           int32_t *count);
 

@@ -396,3 +396,127 @@ def h1_error(patch_list, dofs, nfree, sol, fixed=None):
     fx, fp = _fixed_ptr(fixed)
     return lib().oc_h1_error_bc(P.n, P.arr, np.ascontiguousarray(dofs, np.int32), nfree,
                                 np.ascontiguousarray(sol, np.float64), fp)
+
+
+# ---------------------------------------------------------------------------
+# Fast diagonalization (gsPatchPreconditionersCreator, src/gsSolver/
+# gsPatchPreconditionersCreator.hpp) restated with numpy/scipy
+# ---------------------------------------------------------------------------
+
+def tensor_1d(knots, p, lo, hi, beta=1.0):
+    """Univariate stiffness (times beta) and mass of one knot vector on the
+    parameter line, p+1 Gauss points per span (assembleStiffness /
+    assembleMass, gsPatchPreconditionersCreator.hpp:70-96), restricted to
+    the functions lo..hi (eliminateDirichlet1D, :43-66)."""
+    knots = np.ascontiguousarray(knots, dtype=np.float64)
+    n = len(knots) - p - 1
+    gx, gw = gauss_legendre(p + 1)
+    K = np.zeros((n, n))
+    M = np.zeros((n, n))
+    for span in range(p, n):
+        a, b = knots[span], knots[span + 1]
+        if not b > a:
+            continue
+        h = 0.5 * (b - a)
+        for x, w in zip(gx, gw):
+            B = bspline_ders(knots, p, span, a + h * (x + 1.0), 1)
+            sl = slice(span - p, span + 1)
+            M[sl, sl] += h * w * np.outer(B[0], B[0])
+            K[sl, sl] += h * w * np.outer(B[1], B[1])
+    return beta * K[lo:hi + 1, lo:hi + 1], M[lo:hi + 1, lo:hi + 1]
+
+
+def _free_range(patch, k, dir_sides):
+    nb = len(patch.knots[k]) - patch.p[k] - 1
+    lo = 1 if (2 * k + 1) in dir_sides else 0
+    hi = nb - 2 if (2 * k + 2) in dir_sides else nb - 1
+    return lo, hi
+
+
+def kron_sum(patch, dir_sides, alpha=0.0, beta=1.0):
+    """alpha M + beta sum_k M x .. x K_k x .. x M over the free DOFs of a
+    single patch (direction 0 fastest: the last Kronecker factor), the
+    matrix stiffnessMatrix(basis, bc, opt, alpha, beta) assembles
+    (gsPatchPreconditionersCreator.hpp:184-220)."""
+    import scipy.sparse as sp
+    mats = [tensor_1d(patch.knots[k], patch.p[k], *_free_range(patch, k, dir_sides), beta)
+            for k in range(patch.d)]
+    d = patch.d
+
+    def kron_all(pick):
+        out = sp.csr_matrix(np.ones((1, 1)))
+        for k in reversed(range(d)):
+            out = sp.kron(out, sp.csr_matrix(pick(k)), format="csr")
+        return out
+    A = kron_all(lambda k: mats[k][1]) * alpha
+    for j in range(d):
+        A = A + kron_all(lambda k: mats[k][0] if k == j else mats[k][1])
+    return A.tocsr()
+
+
+def fd_operator(patch, dir_sides, alpha=0.0, beta=1.0):
+    """fastDiagonalizationOp (gsPatchPreconditionersCreator.hpp:258-348):
+    per direction K_k q = lambda M_k q with Q^T M Q = I (scipy's eigh), and
+    r -> (Q x .. x Q) diag(1/(alpha + sum lambda)) (Q x .. x Q)^T r over the
+    free DOFs (direction 0 fastest).  Returns the apply function."""
+    import scipy.linalg as sla
+    d = patch.d
+    Qs, lams, ms = [], [], []
+    for k in range(d):
+        lo, hi = _free_range(patch, k, dir_sides)
+        K, M = tensor_1d(patch.knots[k], patch.p[k], lo, hi, beta)
+        lam, Q = sla.eigh(K, M)
+        Qs.append(Q)
+        lams.append(lam)
+        ms.append(hi - lo + 1)
+    shape = tuple(ms[::-1])               # axes: direction d-1 .. 0
+    den = np.full(shape, float(alpha))
+    for k in range(d):
+        ax = d - 1 - k
+        den = den + lams[k].reshape([-1 if a == ax else 1 for a in range(d)])
+
+    def mode(X, M, k):
+        ax = d - 1 - k
+        return np.moveaxis(np.tensordot(M, X, axes=([1], [ax])), 0, ax)
+
+    def apply(r):
+        X = np.asarray(r, dtype=np.float64).reshape(shape)
+        for k in range(d):
+            X = mode(X, Qs[k].T, k)
+        X = X / den
+        for k in range(d):
+            X = mode(X, Qs[k], k)
+        return X.ravel()
+    return apply
+
+
+def pcg(A, b, prec, tol=1e-10, maxit=1000):
+    """gsConjugateGradient with a preconditioner given as a function
+    (src/gsSolver/gsConjugateGradient.hpp:20-93 inside gsIterativeSolver::
+    solve, gsIterativeSolver.h:114-124): the recurrences and stopping rule
+    of oc_cg_solve.  Returns (x, iterations, relative residual)."""
+    x = np.zeros_like(b)
+    r = b.copy()
+    bn = np.linalg.norm(b)
+    if bn == 0.0:
+        return x, 0, 0.0
+    err = np.linalg.norm(r) / bn
+    if err < tol:
+        return x, 0, err
+    p = prec(r)
+    rz = r @ p
+    it = 0
+    while it < maxit:
+        it += 1
+        t = A @ p
+        a = rz / (p @ t)
+        x += a * p
+        r -= a * t
+        err = np.linalg.norm(r) / bn
+        if err < tol:
+            break
+        z = prec(r)
+        rz_new = r @ z
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+    return x, it, err

@@ -136,3 +136,35 @@ def test_mapper_rejects_bad_input():
     prob.ifaces = [(0, 2, 5, 1, 1, 0, 0, 0)]
     with pytest.raises(gsx.GsxError):
         gsx.mapper(prob)
+
+
+def test_dense_eigen_routines():
+    """gismo_amd/csrc/gsx_dense.h on the CPU (g++): symmetric and generalized
+    eigen-decompositions satisfy their identities, and the generalized
+    eigenvalues of the univariate stiffness/mass equal scipy's eigh on the
+    oracle's matrices of the same knot vector."""
+    import shutil
+    import subprocess
+    import tempfile
+    import scipy.linalg as sla
+    import oracle as O
+    cxx = shutil.which("g++")
+    if cxx is None:
+        pytest.skip("g++ not available")
+    with tempfile.TemporaryDirectory() as td:
+        exe = os.path.join(td, "dense_check")
+        subprocess.check_call([cxx, "-O2", "-std=c++17", "-Wall", "-Wextra", "-Werror",
+                               "-I" + os.path.join(ROOT, "gismo_amd", "csrc"),
+                               os.path.join(ROOT, "tests", "cpp", "dense_check.cpp"), "-o", exe])
+        out = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 0, out.stdout + out.stderr
+    for line in out.stdout.splitlines():
+        if not line.startswith("gen_eig"):
+            continue
+        f = line.split()
+        p = int(f[2])
+        lam = np.array([float(v) for v in f[f.index("lambda") + 1:]])
+        knots = O.knots_uniform(9, p, 0.25, 1.75)
+        K, M = O.tensor_1d(knots, p, 1, len(knots) - p - 3)
+        want = sla.eigh(K, M, eigvals_only=True)
+        assert np.abs(lam - want).max() <= 1e-10 * want.max(), p

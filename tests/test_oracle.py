@@ -775,3 +775,48 @@ def test_mixed_boundary_manufactured_convergence():
     rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
     print(errs, rates)
     assert rates[-1] > 2.8, rates
+
+
+# ---------------------------------------------------------------------------
+# fast diagonalization (gsPatchPreconditionersCreator)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("d,p,n", [(2, 2, 6), (3, 3, 4), (3, 2, 5)])
+def test_kron_sum_is_the_unit_box_stiffness(d, p, n):
+    """On the identity map with kappa = 1 the stiffness matrix is the
+    Kronecker sum of the univariate matrices: the oracle's d-dimensional
+    assembly pins the univariate matrices and the Kronecker ordering."""
+    P = problems.box_patch(d, p, n)
+    P.fkind = 1
+    sides = list(range(1, 2 * d + 1))
+    dofs, nfree = O.mapper([P], [(0, s) for s in sides], [])
+    A, _ = O.assemble([P], dofs, nfree, 0)
+    K = O.kron_sum(P, sides)
+    assert K.shape == A.shape
+    assert abs(K - A).max() <= 1e-14 * abs(A).max()
+
+
+@pytest.mark.parametrize("d,p,sides,alpha", [(2, 3, [1, 2, 3, 4], 0.0), (2, 2, [1, 4], 0.3),
+                                             (3, 2, [1, 3, 6], 0.0), (3, 3, [], 1.5)])
+def test_fd_is_the_inverse_of_the_kronecker_sum(d, p, sides, alpha):
+    """fastDiagonalizationOp applies (alpha M + sum_k M x .. x K_k x .. x M)^-1
+    exactly, with Neumann ends kept (and alpha > 0 where no end is fixed)."""
+    import scipy.sparse.linalg as spla
+    P = problems.box_patch(d, p, 5)
+    A = O.kron_sum(P, sides, alpha=alpha, beta=1.0)
+    fd = O.fd_operator(P, sides, alpha=alpha)
+    r = np.random.default_rng(3).standard_normal(A.shape[0])
+    want = spla.spsolve(A.tocsc(), r)
+    assert np.abs(fd(r) - want).max() <= 1e-10 * np.abs(want).max()
+
+
+def test_pcg_restates_the_cg_recurrences():
+    """The callable-preconditioner PCG gives the C oracle's iteration count
+    and solution with the Jacobi preconditioner."""
+    prob = problems.poisson_noisy_cube(n=6, p=2, seed=3)
+    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    A, b = O.assemble(prob.patches, dofs, nfree, 0)
+    x, it, err = O.pcg(A, b, lambda r: r / A.diagonal(), tol=1e-10)
+    xo, ito, erro = O.cg_solve(A, b, tol=1e-10)
+    assert it == ito
+    assert np.abs(x - xo).max() <= 1e-9 * np.abs(xo).max()
