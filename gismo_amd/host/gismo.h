@@ -456,6 +456,26 @@ private:
 };
 
 /// Discretisation bases, one tensor B-spline basis per patch.
+/// gsTensorBSplineBasis (src/gsTensor/gsTensorBSplineBasis.h): the knot
+/// vectors of one patch's tensor basis
+template <class T = real_t>
+class gsTensorBSplineBasis {
+public:
+    explicit gsTensorBSplineBasis(std::vector<gsKnotVector> kv) : m_kv(std::move(kv)) {}
+    short_t dim() const { return (short_t)m_kv.size(); }
+    const gsKnotVector &knots(int dir) const { return m_kv[dir]; }
+    int degree(int dir) const { return m_kv[dir].degree(); }
+    index_t size() const
+    {
+        index_t n = 1;
+        for (auto &k : m_kv) n *= k.numBasis();
+        return n;
+    }
+
+private:
+    std::vector<gsKnotVector> m_kv;
+};
+
 template <class T = real_t>
 class gsMultiBasis {
 public:
@@ -513,6 +533,9 @@ public:
         for (int k = 0; k < nBases(); ++k) n += size(k);
         return n;
     }
+    /// the tensor B-spline basis of one patch (gsMultiBasis::basis / operator[])
+    gsTensorBSplineBasis<T> basis(int patch) const { return gsTensorBSplineBasis<T>(m_kv[patch]); }
+    gsTensorBSplineBasis<T> operator[](int patch) const { return basis(patch); }
 
 private:
     std::vector<std::vector<gsKnotVector>> m_kv;
@@ -1276,6 +1299,102 @@ private:
     mutable std::shared_ptr<gsx_mg> m_mg;
 };
 
+/// The operator of gsPatchPreconditionersCreator::fastDiagonalizationOp
+/// (src/gsSolver/gsPatchPreconditionersCreator.hpp:258-348): the exact
+/// inverse of alpha M + beta sum_k M x .. x K_k x .. x M on the parameter
+/// domain of one patch, built on the device (gsx_fd_create).
+template <class T = real_t>
+class gsFastDiagonalizationOp : public gsPreconditionerOp<T> {
+public:
+    gsFastDiagonalizationOp(const gsTensorBSplineBasis<T> &basis, const gsBoundaryConditions<T> &bc, T alpha, T beta)
+    {
+        const int d = basis.dim();
+        if (d < 2 || d > 3) throw std::runtime_error("fastDiagonalizationOp: 2D and 3D bases only");
+        gsx_patch_desc D;
+        std::memset(&D, 0, sizeof(D));
+        D.d = d;
+        int nb[3] = {1, 1, 1};
+        std::vector<std::vector<double>> gr(d);
+        for (int j = 0; j < d; ++j) {
+            D.p[j] = basis.degree(j);
+            D.nknots[j] = (int32_t)basis.knots(j).size();
+            D.knots_host[j] = basis.knots(j).knots().data();
+            nb[j] = (int)basis.knots(j).numBasis();
+            // Greville abscissae: the identity map (the operator does not
+            // depend on the geometry)
+            const auto &U = basis.knots(j).knots();
+            const int p = D.p[j];
+            for (int i = 0; i < nb[j]; ++i) {
+                double g = 0.0;
+                for (int k = 1; k <= p; ++k) g += U[i + k];
+                gr[j].push_back(p > 0 ? g / p : U[i]);
+            }
+        }
+        const index_t n = (index_t)nb[0] * nb[1] * nb[2];
+        std::vector<double> ctrl((size_t)n * d);
+        for (index_t i = 0; i < n; ++i) {
+            const int c[3] = {(int)(i % nb[0]), (int)((i / nb[0]) % nb[1]), (int)(i / ((index_t)nb[0] * nb[1]))};
+            for (int j = 0; j < d; ++j) ctrl[(size_t)i * d + j] = gr[j][c[j]];
+        }
+        D.ctrl_host = ctrl.data();
+        D.fkind = 1;
+        std::vector<int32_t> ds;
+        for (auto &s : bc.dirichletSides())
+            if (s.first == 0) { ds.push_back(0); ds.push_back(s.second); }
+        gsx_problem *p = nullptr;
+        detail::check(gsx_problem_create(detail::ctx(), 1, &D, (int32_t)ds.size() / 2, ds.empty() ? nullptr : ds.data(),
+                                         0, nullptr, &p),
+                      "gsx_problem_create");
+        m_prob.reset(p, detail::ProblemDeleter());
+        int64_t nbasis = 0, nfree = 0, nelem = 0, nglued = 0;
+        detail::check(gsx_problem_info(p, &nbasis, &nfree, &nelem, &nglued), "gsx_problem_info");
+        m_n = (index_t)nfree;
+        gsx_fd *f = nullptr;
+        detail::check(gsx_fd_create(p, alpha, beta, &f), "gsx_fd_create");
+        m_fd.reset(f, [](gsx_fd *q) { gsx_fd_destroy(q); });
+    }
+    void apply(const gsMatrix<T> &input, gsMatrix<T> &x) const override
+    {
+        detail::DeviceVec a(sizeof(double) * std::max<index_t>(1, m_n)), b(sizeof(double) * std::max<index_t>(1, m_n));
+        detail::check(gsx_copy_to_device(detail::ctx(), a.p, input.data(), sizeof(double) * m_n), "upload");
+        detail::check(gsx_fd_apply(m_fd.get(), a.d(), b.d()), "gsx_fd_apply");
+        x.resize(m_n, 1);
+        detail::check(gsx_copy_to_host(detail::ctx(), x.data(), b.p, sizeof(double) * m_n), "download");
+    }
+    index_t rows() const override { return m_n; }
+    index_t cols() const override { return m_n; }
+    int kind() const override { return GSX_PREC_FD; }
+    gsx_fd *device() const { return m_fd.get(); }
+
+private:
+    index_t m_n = 0;
+    std::shared_ptr<gsx_problem> m_prob;
+    std::shared_ptr<gsx_fd> m_fd;
+};
+
+/// gsPatchPreconditionersCreator (src/gsSolver/gsPatchPreconditionersCreator.h)
+/// subset: fastDiagonalizationOp with DirichletStrategy elimination and
+/// gamma = 0.
+template <class T = real_t>
+struct gsPatchPreconditionersCreator {
+    typedef std::shared_ptr<gsPreconditionerOp<T>> OpUPtr;
+    static gsOptionList defaultOptions()
+    {
+        gsOptionList o;
+        o.setInt("DirichletStrategy", dirichlet::elimination);
+        return o;
+    }
+    static OpUPtr fastDiagonalizationOp(const gsTensorBSplineBasis<T> &basis, const gsBoundaryConditions<T> &bc,
+                                        const gsOptionList &opt = defaultOptions(), T alpha = 0, T beta = 1,
+                                        T gamma = 0)
+    {
+        if (opt.askInt("DirichletStrategy", dirichlet::elimination) != dirichlet::elimination)
+            throw std::runtime_error("fastDiagonalizationOp: Dirichlet elimination only");
+        if (gamma != 0) throw std::runtime_error("fastDiagonalizationOp: gamma != 0 (pure Neumann averaging) is not provided");
+        return std::make_shared<gsFastDiagonalizationOp<T>>(basis, bc, alpha, beta);
+    }
+};
+
 /// gsIterativeSolver / gsConjugateGradient (src/gsSolver/gsIterativeSolver.h:
 /// 20-230, src/gsSolver/gsConjugateGradient.hpp:20-93).  Options:
 /// MaxIterations (1000), Tolerance (1e-10), Precision ("fp64" or "mixed").
@@ -1318,8 +1437,12 @@ protected:
         gsx_mg *mg = nullptr;
         if (kind == GSX_PREC_JACOBI) tau = static_cast<gsJacobiOp<T> *>(m_prec.get())->damping();
         if (kind == GSX_PREC_MG) mg = static_cast<gsMultiGridOp<T> *>(m_prec.get())->device();
-        const gsx_status st = gsx_cg_solve(dev.get(), kind, tau, mg, mixed ? GSX_MIXED : GSX_FP64, b.d(), xd.d(),
-                                           m_tol, m_max, &it, &rel);
+        const gsx_status st =
+            kind == GSX_PREC_FD
+                ? gsx_cg_solve_fd(dev.get(), static_cast<gsFastDiagonalizationOp<T> *>(m_prec.get())->device(),
+                                  mixed ? GSX_MIXED : GSX_FP64, b.d(), xd.d(), m_tol, m_max, &it, &rel)
+                : gsx_cg_solve(dev.get(), kind, tau, mg, mixed ? GSX_MIXED : GSX_FP64, b.d(), xd.d(), m_tol, m_max,
+                               &it, &rel);
         if (st != GSX_OK && st != GSX_ERR_NOT_CONVERGED) detail::check(st, "gsx_cg_solve");
         m_it = it;
         m_err = rel;

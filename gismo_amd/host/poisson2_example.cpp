@@ -10,6 +10,8 @@
 //   --last        solve only on the finest level
 //   --full        gsPoissonAssembler (full pattern) instead of gsExprAssembler
 //   --mixed       FP32 iterations with FP64 residual updates
+//   --fd          CG preconditioned by gsPatchPreconditionersCreator::
+//                 fastDiagonalizationOp (single patch) instead of Jacobi
 //   --bvp         the boundary value problem of poisson2d_bvp: the domain
 //                 shifted by 1/4 (u = prod sin(pi x_i) does not vanish on
 //                 it), u on the west/south(/front) sides by L2 projection,
@@ -35,7 +37,7 @@ using namespace gismo;
 int main(int argc, char *argv[])
 {
     index_t numRefine = 5, numElevate = 1, dim = 2, gx = 0, gy = 0;
-    bool last = false, full = false, mixed = false, bvp = false;
+    bool last = false, full = false, mixed = false, bvp = false, fastdiag = false;
     std::string fn;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
@@ -48,6 +50,7 @@ int main(int argc, char *argv[])
         else if (a == "--full") full = true;
         else if (a == "--mixed") mixed = true;
         else if (a == "--bvp") bvp = true;
+        else if (a == "--fd") fastdiag = true;
         else { std::fprintf(stderr, "unknown argument %s\n", argv[i]); return 1; }
     }
     try {
@@ -140,7 +143,11 @@ int main(int argc, char *argv[])
                 rhs = &A.rhs();
             }
             auto t1 = std::chrono::steady_clock::now();
-            gsConjugateGradient<> cg(*K, makeJacobiOp(*K, 1.0));
+            // preconditioner: diagonal, or (--fd, one patch) the fast
+            // diagonalization of gsPatchPreconditionersCreator
+            gsConjugateGradient<> cg(*K, fastdiag ? gsPatchPreconditionersCreator<>::fastDiagonalizationOp(
+                                                  dbasis.basis(0), full ? bcAll : bc)
+                                            : gsPreconditionerOp<>::Ptr(makeJacobiOp(*K, 1.0)));
             gsOptionList o = gsConjugateGradient<>::defaultOptions();
             o.setReal("Tolerance", 1e-10);
             o.setInt("MaxIterations", 100000);

@@ -53,7 +53,8 @@ typedef enum {
 typedef enum {
     GSX_PREC_NONE = 0,
     GSX_PREC_JACOBI = 1,
-    GSX_PREC_MG = 2
+    GSX_PREC_MG = 2,
+    GSX_PREC_FD = 3      /* fast diagonalization: gsx_cg_solve_fd */
 } gsx_precond;
 
 typedef enum {

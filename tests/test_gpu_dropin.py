@@ -241,3 +241,22 @@ def test_poisson2_example_bvp_from_xml():
     rates = np.log2(l2[:-1] / l2[1:])
     print(l2, rates, flush=True)
     assert rates[-1] > 2.7, rates
+
+
+@pytest.mark.parametrize("args", [["-d", "3", "-e", "2", "-r", "2"], ["--bvp", "-e", "1", "-r", "3"],
+                                  ["--bvp", "--full", "-e", "1", "-r", "2"]])
+def test_poisson2_example_fast_diagonalization(args):
+    """poisson2_example --fd: gsConjugateGradient preconditioned by
+    gsPatchPreconditionersCreator::fastDiagonalizationOp(basis, bc).  The
+    example's domains are affine boxes with unit coefficient, where the
+    stiffness matrix is the Kronecker sum the operator inverts (Neumann
+    ends kept for --bvp): one iteration per level, and the same DOFs and
+    errors as the diagonally preconditioned run."""
+    jac = _levels(_run("poisson2_example", *args))
+    fd = _levels(_run("poisson2_example", "--fd", *args))
+    assert len(fd) == len(jac)
+    for (dj, nj, ij, rj, lj), (df, nf, itf, rf, lf) in zip(jac, fd):
+        assert (dj, nj) == (df, nf)
+        assert itf == 1, (args, itf)
+        assert rf < 1e-10
+        assert abs(lf - lj) <= 1e-6 * lj
