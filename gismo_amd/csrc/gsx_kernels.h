@@ -186,6 +186,13 @@ hipError_t launch_cg2_direction_f64(int n, double *p, const double *z, const dou
 hipError_t launch_cg2_direction_f32_mixed(int n, float *p, const float *z, const double *prr,
                                           const double *prz, int np, CGState *st, int k,
                                           hipStream_t s);
+// Gauss-Seidel: update the cnt rows listed in rows (one level set)
+hipError_t launch_gs_rows_f64(int cnt, const int *rows, const int *rp, const int *ci, const double *va,
+                              const double *b, double *x, hipStream_t s);
+hipError_t launch_gs_rows_v32(int cnt, const int *rows, const int *rp, const int *ci, const float *va,
+                              const double *b, double *x, hipStream_t s);
+hipError_t launch_gs_rows_f32(int cnt, const int *rows, const int *rp, const int *ci, const float *va,
+                              const float *b, float *x, hipStream_t s);
 hipError_t launch_cg2_dot_f64(int n, const double *a, const double *b, double *part, const CGState *st,
                               hipStream_t s);
 hipError_t launch_cg2_dot_f32(int n, const float *a, const float *b, double *part, const CGState *st,

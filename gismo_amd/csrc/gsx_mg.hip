@@ -867,3 +867,62 @@ hipError_t launch_div_sq(int n, double *w, const double *diag, double *part, hip
 }
 
 }  // namespace gsx
+
+namespace gsx {
+
+// ---------------------------------------------------------------------------
+// Gauss-Seidel sweep by level sets (gsGaussSeidelOp, internal::
+// gaussSeidelSweep / reverseGaussSeidelSweep, src/gsSolver/
+// gsSimplePreconditioners.hpp:21-68).  A sweep visits the rows one after
+// the other; rows of one level set have no entry coupling them and every
+// row's earlier neighbours (later ones for the reverse sweep) lie in
+// earlier sets, so updating a set at once reads exactly the values the
+// sequential sweep reads.  One thread per row of the set, the row's sum in
+// storage order.
+// ---------------------------------------------------------------------------
+template <typename V, typename X>
+__global__ __launch_bounds__(256) void k_gs_rows(int cnt, const int *__restrict__ rows, const int *__restrict__ rp,
+                                                 const int *__restrict__ ci, const V *__restrict__ va,
+                                                 const X *__restrict__ b, X *__restrict__ x)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= cnt) return;
+    const int i = rows[k];
+    double diag = 0.0, sum = 0.0;
+    for (int j = rp[i]; j < rp[i + 1]; ++j) {
+        const int c = ci[j];
+        const double a = (double)va[j];
+        sum += a * (double)x[c];
+        if (c == i) diag = a;
+    }
+    x[i] = (X)((double)x[i] + ((double)b[i] - sum) / diag);
+}
+
+template <typename V, typename X>
+static hipError_t launch_gs_t(int cnt, const int *rows, const int *rp, const int *ci, const V *va, const X *b, X *x,
+                              hipStream_t s)
+{
+    if (cnt <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_gs_rows<V, X>), dim3((cnt + 255) / 256), dim3(256), 0, s, cnt, rows, rp, ci, va, b, x);
+    return hipGetLastError();
+}
+
+hipError_t launch_gs_rows_f64(int cnt, const int *rows, const int *rp, const int *ci, const double *va,
+                              const double *b, double *x, hipStream_t s)
+{
+    return launch_gs_t<double, double>(cnt, rows, rp, ci, va, b, x, s);
+}
+
+hipError_t launch_gs_rows_v32(int cnt, const int *rows, const int *rp, const int *ci, const float *va,
+                              const double *b, double *x, hipStream_t s)
+{
+    return launch_gs_t<float, double>(cnt, rows, rp, ci, va, b, x, s);
+}
+
+hipError_t launch_gs_rows_f32(int cnt, const int *rows, const int *rp, const int *ci, const float *va,
+                              const float *b, float *x, hipStream_t s)
+{
+    return launch_gs_t<float, float>(cnt, rows, rp, ci, va, b, x, s);
+}
+
+}  // namespace gsx

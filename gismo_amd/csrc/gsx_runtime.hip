@@ -2019,7 +2019,36 @@ struct MGLevel {
     DevPtr x32, b32, r32, t32;         // FP32
     DevPtr d, d32;                     // polynomial smoother direction
     double lmax = 0.0;                 // Chebyshev: bound of lambda(D^-1 A)
+    // Gauss-Seidel: rows by level set, forward and reverse sweep
+    DevPtr gs_fwd, gs_rev;
+    std::vector<int> gs_fwd_off, gs_rev_off;
 };
+
+// Level sets of one Gauss-Seidel sweep over the CSR pattern: lev(i) = 1 +
+// max lev(j) over the row's earlier columns j < i (forward) or later ones
+// j > i (reverse); rows listed by level, ascending within a level.
+static void gs_schedule(int n, const std::vector<int> &rp, const std::vector<int> &ci, bool reverse,
+                        std::vector<int> &rows, std::vector<int> &off)
+{
+    std::vector<int> lev(n, 0);
+    int maxl = 0;
+    for (int k = 0; k < n; ++k) {
+        const int i = reverse ? n - 1 - k : k;
+        int l = 0;
+        for (int j = rp[i]; j < rp[i + 1]; ++j) {
+            const int c = ci[j];
+            if (reverse ? c > i : c < i) l = std::max(l, lev[c] + 1);
+        }
+        lev[i] = l;
+        maxl = std::max(maxl, l);
+    }
+    off.assign(maxl + 2, 0);
+    for (int i = 0; i < n; ++i) off[lev[i] + 1]++;
+    for (int l = 0; l <= maxl; ++l) off[l + 1] += off[l];
+    rows.assign(n, 0);
+    std::vector<int> at(off.begin(), off.end() - 1);
+    for (int i = 0; i < n; ++i) rows[at[lev[i]]++] = i;
+}
 
 struct gsx_mg {
     gsx_ctx *ctx = nullptr;
@@ -2507,7 +2536,7 @@ static SmoothCoef smooth_coef(const gsx_mg *mg, const MGLevel &V, int k, double 
 // FP32 when v32: the finest level of a mixed cycle).  Jacobi ping-pongs x
 // and t (x may come back as t); the polynomial smoothers update x in place.
 static gsx_status smooth64(gsx_mg *mg, MGLevel &V, double *&x, double *&t, const double *b, bool &zero, int m,
-                           bool v32 = false)
+                           bool v32, bool post)
 {
     gsx_ctx *c = mg->ctx;
     hipStream_t s = c->stream;
@@ -2518,6 +2547,20 @@ static gsx_status smooth64(gsx_mg *mg, MGLevel &V, double *&x, double *&t, const
         return v32 ? gsx::launch_residual_v32(V.lpr, V.n, rp, ci, va32, xx, b, rr, s)
                    : gsx::launch_residual_f64(V.lpr, V.n, rp, ci, va, xx, b, rr, s);
     };
+    if (mg->smoother == GSX_SMOOTH_GAUSS_SEIDEL) {
+        if (m > 0 && zero) HIP_OK(c, gsx::launch_fill_f64(x, V.n, 0.0, s));
+        const int *rows = dp<int>(post ? V.gs_rev : V.gs_fwd);
+        const std::vector<int> &off = post ? V.gs_rev_off : V.gs_fwd_off;
+        for (int k = 0; k < m; ++k)
+            for (size_t l = 0; l + 1 < off.size(); ++l) {
+                if (v32)
+                    HIP_OK(c, gsx::launch_gs_rows_v32(off[l + 1] - off[l], rows + off[l], rp, ci, va32, b, x, s));
+                else
+                    HIP_OK(c, gsx::launch_gs_rows_f64(off[l + 1] - off[l], rows + off[l], rp, ci, va, b, x, s));
+            }
+        if (m > 0) zero = false;
+        return GSX_OK;
+    }
     if (mg->smoother == GSX_SMOOTH_JACOBI) {
         for (int i = 0; i < m; ++i) {
             if (v32)
@@ -2542,12 +2585,23 @@ static gsx_status smooth64(gsx_mg *mg, MGLevel &V, double *&x, double *&t, const
     return GSX_OK;
 }
 
-static gsx_status smooth32(gsx_mg *mg, MGLevel &V, float *&x, float *&t, const float *b, bool &zero, int m)
+static gsx_status smooth32(gsx_mg *mg, MGLevel &V, float *&x, float *&t, const float *b, bool &zero, int m,
+                           bool post)
 {
     gsx_ctx *c = mg->ctx;
     hipStream_t s = c->stream;
     const int *rp = dp<int>(V.rowptr), *ci = dp<int>(V.col);
     const float *va = dp<float>(V.val32), *dg = dp<float>(V.diag32);
+    if (mg->smoother == GSX_SMOOTH_GAUSS_SEIDEL) {
+        if (m > 0 && zero) HIP_OK(c, gsx::launch_fill_f32(x, V.n, 0.0f, s));
+        const int *rows = dp<int>(post ? V.gs_rev : V.gs_fwd);
+        const std::vector<int> &off = post ? V.gs_rev_off : V.gs_fwd_off;
+        for (int k = 0; k < m; ++k)
+            for (size_t l = 0; l + 1 < off.size(); ++l)
+                HIP_OK(c, gsx::launch_gs_rows_f32(off[l + 1] - off[l], rows + off[l], rp, ci, va, b, x, s));
+        if (m > 0) zero = false;
+        return GSX_OK;
+    }
     if (mg->smoother == GSX_SMOOTH_JACOBI) {
         for (int i = 0; i < m; ++i) {
             HIP_OK(c, gsx::launch_jacobi_step_f32(V.lpr, V.n, rp, ci, va, dg, b, x, t, (float)mg->tau, zero ? 1 : 0, s));
@@ -2593,7 +2647,7 @@ static gsx_status mg_cycle64(gsx_mg *mg, int l, bool mixed = false)
     bool zero = true;
     {
         KScope k(c, "mg_smooth");
-        GSX_TRY(smooth64(mg, V, x, t, b, zero, mg->npre, mixed));
+        GSX_TRY(smooth64(mg, V, x, t, b, zero, mg->npre, mixed, false));
     }
     MGLevel &C = mg->L[l - 1];
     {
@@ -2623,7 +2677,7 @@ static gsx_status mg_cycle64(gsx_mg *mg, int l, bool mixed = false)
     }
     {
         KScope k(c, "mg_smooth");
-        GSX_TRY(smooth64(mg, V, x, t, b, zero, mg->npost, mixed));
+        GSX_TRY(smooth64(mg, V, x, t, b, zero, mg->npost, mixed, true));
     }
     if (x != dp<double>(V.x)) HIP_OK(c, hipMemcpyAsync(dp<double>(V.x), x, sizeof(double) * V.n, hipMemcpyDeviceToDevice, s));
     return GSX_OK;
@@ -2649,7 +2703,7 @@ static gsx_status mg_cycle32(gsx_mg *mg, int l)
     bool zero = true;
     {
         KScope k(c, "mg_smooth");
-        GSX_TRY(smooth32(mg, V, x, t, b, zero, mg->npre));
+        GSX_TRY(smooth32(mg, V, x, t, b, zero, mg->npre, false));
     }
     MGLevel &C = mg->L[l - 1];
     {
@@ -2667,7 +2721,7 @@ static gsx_status mg_cycle32(gsx_mg *mg, int l)
     }
     {
         KScope k(c, "mg_smooth");
-        GSX_TRY(smooth32(mg, V, x, t, b, zero, mg->npost));
+        GSX_TRY(smooth32(mg, V, x, t, b, zero, mg->npost, true));
     }
     if (x != dp<float>(V.x32)) HIP_OK(c, hipMemcpyAsync(dp<float>(V.x32), x, sizeof(float) * V.n, hipMemcpyDeviceToDevice, s));
     return GSX_OK;
@@ -2688,11 +2742,31 @@ static gsx_status mg_apply_dev(gsx_mg *mg, const double *r, double *z)
 
 extern "C" gsx_status gsx_mg_set_smoother(gsx_mg *mg, int32_t kind, double param)
 {
-    if (!mg || kind < GSX_SMOOTH_JACOBI || kind > GSX_SMOOTH_CHEBYSHEV) return GSX_ERR_ARG;
+    if (!mg || kind < GSX_SMOOTH_JACOBI || kind > GSX_SMOOTH_GAUSS_SEIDEL) return GSX_ERR_ARG;
     gsx_ctx *c = mg->ctx;
     hipStream_t s = c->stream;
     mg->smoother = kind;
     if (kind == GSX_SMOOTH_JACOBI) return GSX_OK;
+    if (kind == GSX_SMOOTH_GAUSS_SEIDEL) {
+        for (size_t l = 1; l < mg->L.size(); ++l) {
+            MGLevel &V = mg->L[l];
+            if (V.gs_fwd) continue;
+            std::vector<int> rp(V.n + 1), ci((size_t)V.nnz), rows;
+            HIP_OK(c, hipMemcpyAsync(rp.data(), V.rowptr.get(), sizeof(int) * (V.n + 1), hipMemcpyDeviceToHost, s));
+            HIP_OK(c, hipMemcpyAsync(ci.data(), V.col.get(), sizeof(int) * V.nnz, hipMemcpyDeviceToHost, s));
+            HIP_OK(c, hipStreamSynchronize(s));
+            V.gs_fwd = dalloc(c, sizeof(int) * std::max(1, V.n));
+            V.gs_rev = dalloc(c, sizeof(int) * std::max(1, V.n));
+            if (!V.gs_fwd || !V.gs_rev) return fail(c, GSX_ERR_OOM, "device allocation (Gauss-Seidel schedule)");
+            gs_schedule(V.n, rp, ci, false, rows, V.gs_fwd_off);
+            HIP_OK(c, hipMemcpyAsync(V.gs_fwd.get(), rows.data(), sizeof(int) * V.n, hipMemcpyHostToDevice, s));
+            HIP_OK(c, hipStreamSynchronize(s));
+            gs_schedule(V.n, rp, ci, true, rows, V.gs_rev_off);
+            HIP_OK(c, hipMemcpyAsync(V.gs_rev.get(), rows.data(), sizeof(int) * V.n, hipMemcpyHostToDevice, s));
+            HIP_OK(c, hipStreamSynchronize(s));
+        }
+        return GSX_OK;
+    }
     if (kind == GSX_SMOOTH_CHEBYSHEV) mg->cheb_ratio = param > 1.0 ? param : 30.0;
     for (size_t l = 1; l < mg->L.size(); ++l) {
         MGLevel &V = mg->L[l];

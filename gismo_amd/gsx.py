@@ -458,7 +458,7 @@ class Matrix:
 class Multigrid:
     """Geometric multigrid preconditioner (gsMultiGridOp) on the device."""
 
-    SMOOTHERS = {"jacobi": 0, "richardson": 1, "chebyshev": 2}
+    SMOOTHERS = {"jacobi": 0, "richardson": 1, "chebyshev": 2, "gauss_seidel": 3}
 
     def __init__(self, problem, matrix, nlevels, npre=1, npost=1, damping=1.0, cgc_damping=1.0,
                  precision=FP64, smoother="jacobi", smoother_param=30.0):

@@ -1237,7 +1237,7 @@ std::shared_ptr<gsJacobiOp<T>> makeJacobiOp(const gsSparseMatrix<T> &M, T tau = 
 /// assembled problem by coarsening its knot vectors (the transfers and
 /// Galerkin products are formed on the device).  Options: NumPreSmooth,
 /// NumPostSmooth, CoarseGridCorrectionDamping, Damping (of the Jacobi or
-/// Richardson smoother), Smoother ("Jacobi"/"j", "Richardson"/"r",
+/// Richardson smoother), Smoother ("Jacobi"/"j", "GaussSeidel"/"gs", "Richardson"/"r",
 /// "Chebyshev"/"c"), ChebyshevRatio, Precision ("fp64" or "mixed": FP32
 /// cycle).
 template <class T = real_t>
@@ -1278,10 +1278,11 @@ public:
                                         m_opt.askReal("CoarseGridCorrectionDamping", 1.0), prec, &h),
                           "gsx_mg_create");
             m_mg.reset(h, [](gsx_mg *q) { gsx_mg_destroy(q); });
-            // multiGrid_example's MG.Smoother names (r, j) plus Chebyshev (c)
+            // multiGrid_example's MG.Smoother names (r, j, gs) plus Chebyshev (c)
             const std::string sm = m_opt.askString("Smoother", "Jacobi");
             int kind = -1;
             if (sm == "Jacobi" || sm == "j") kind = GSX_SMOOTH_JACOBI;
+            else if (sm == "GaussSeidel" || sm == "gs") kind = GSX_SMOOTH_GAUSS_SEIDEL;
             else if (sm == "Richardson" || sm == "r") kind = GSX_SMOOTH_RICHARDSON;
             else if (sm == "Chebyshev" || sm == "c") kind = GSX_SMOOTH_CHEBYSHEV;
             if (kind < 0) throw std::runtime_error("gsMultiGridOp: smoother '" + sm + "' is not supported");

@@ -10,7 +10,8 @@
 //   -l N                  multigrid levels (default 3)
 //   --MG.NumPreSmooth N   (default 1)     --MG.NumPostSmooth N (default 1)
 //   --MG.Damping X        Jacobi / Richardson damping (default 0.8)
-//   -s NAME               MG.Smoother: Jacobi (j), Richardson (r), Chebyshev (c)
+//   -s NAME               MG.Smoother: GaussSeidel (gs, default), Jacobi (j),
+//                         Richardson (r), Chebyshev (c)
 //   -t X                  tolerance (default 1e-8)
 //   --Solver.MaxIterations N (default 100)
 //   -i cg|d               CG preconditioned by multigrid (default) or plain
@@ -30,7 +31,7 @@ int main(int argc, char *argv[])
 {
     index_t refinements = 5, degree = 2, levels = 3, pre = 1, post = 1, maxIt = 100, dim = 2, gx = 0, gy = 0;
     real_t damping = 0.8, tol = 1e-8;
-    std::string solver = "cg", geometry, smoother = "Jacobi";
+    std::string solver = "cg", geometry, smoother = "GaussSeidel";   // the reference default
     bool mixed = false;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];

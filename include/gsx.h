@@ -60,7 +60,8 @@ typedef enum {
 typedef enum {
     GSX_SMOOTH_JACOBI = 0,      /* damped Jacobi (default)                   */
     GSX_SMOOTH_RICHARDSON = 1,  /* x += damping (b - A x)                    */
-    GSX_SMOOTH_CHEBYSHEV = 2    /* Jacobi-preconditioned Chebyshev            */
+    GSX_SMOOTH_CHEBYSHEV = 2,   /* Jacobi-preconditioned Chebyshev            */
+    GSX_SMOOTH_GAUSS_SEIDEL = 3 /* forward sweeps before, reverse after        */
 } gsx_smoother;
 
 typedef struct gsx_ctx gsx_ctx;
@@ -313,6 +314,14 @@ gsx_status gsx_mg_destroy(gsx_mg *mg);
  * (same file :33-131); GSX_SMOOTH_CHEBYSHEV (no reference counterpart):
  * npre/npost-step Chebyshev iteration on D^-1 A over [lmax/param, lmax]
  * (param <= 1: 30), lmax = 1.1 x a 20-step power-iteration estimate. */
+/* GSX_SMOOTH_GAUSS_SEIDEL: gsGaussSeidelOp (makeGaussSeidelOp, the default
+ * smoother of examples/multiGrid_example.cpp:41) with gsMultiGridOp's
+ * SymmSmooth: forward sweeps (step) before and reverse sweeps (stepT) after
+ * the coarse correction (gsMultiGrid.hpp:166-199,
+ * gsSimplePreconditioners.h:310-329, sweeps gsSimplePreconditioners.hpp:
+ * 21-68), run exactly in the sequential order by level sets of the
+ * triangular dependencies (one launch per set: made for parity, slow on
+ * large levels). */
 gsx_status gsx_mg_set_smoother(gsx_mg *mg, int32_t kind, double param);
 /* The Chebyshev bound lmax of level lvl (0 before gsx_mg_set_smoother). */
 gsx_status gsx_mg_smoother_bound(const gsx_mg *mg, int32_t lvl, double *lmax);

@@ -80,6 +80,8 @@ def lib():
         L.oc_mg_apply.argtypes = [C.c_void_p, _f64p, _f64p]
         L.oc_mg_set_smoother.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.oc_mg_set_smoother.restype = None
+        L.oc_gs_sweep.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int]
+        L.oc_gs_sweep.restype = None
         L.oc_mg_lmax.argtypes = [C.c_void_p, C.c_int]
         L.oc_mg_lmax.restype = C.c_double
         L.oc_mg_matrix.argtypes = [C.c_void_p, C.c_int]
@@ -322,6 +324,15 @@ def cg_solve(A, b, x0=None, tol=1e-10, maxit=1000, precond="jacobi", tau=1.0, mg
     return x, it, err.value
 
 
+def gs_sweep(A, b, x, reverse=False):
+    """One Gauss-Seidel sweep on x (a copy is returned), rows ascending or
+    descending (gsSimplePreconditioners.hpp:21-68)."""
+    H = _CsrHolder(A)
+    x = np.ascontiguousarray(x, dtype=np.float64).copy()
+    lib().oc_gs_sweep(C.byref(H.s), np.ascontiguousarray(b, dtype=np.float64), x, 1 if reverse else 0)
+    return x
+
+
 def transfer_1d(knots, p):
     knots = np.ascontiguousarray(knots, dtype=np.float64)
     nc = len(knots) - p - 1
@@ -340,7 +351,7 @@ def transfer_multipatch(coarse_patches, dofs_c, nfree_c, dofs_f, nfree_f):
 
 
 class Multigrid:
-    SMOOTHERS = {"jacobi": 0, "richardson": 1, "chebyshev": 2}
+    SMOOTHERS = {"jacobi": 0, "richardson": 1, "chebyshev": 2, "gauss_seidel": 3}
 
     def __init__(self, A_fine, transfers, npre=1, npost=1, damping=1.0, cgc_damping=1.0, smoother="jacobi",
                  smoother_param=30.0):

@@ -1178,10 +1178,35 @@ static double power_lmax(const oc_csr *A, const double *dg)
  *   rho_{k+1} = 1 / (2 sigma - rho_k),
  *   d_{k+1} = rho_{k+1} rho_k d_k + (2 rho_{k+1} / delta) D^-1 r_k,
  *   x_{k+1} = x_k + d_{k+1}. */
-static void smooth(const oc_mg *mg, int l, const double *b, double *x, int m, double *tmp, double *d)
+/* One Gauss-Seidel sweep, rows ascending (internal::gaussSeidelSweep) or
+ * descending (internal::reverseGaussSeidelSweep), src/gsSolver/
+ * gsSimplePreconditioners.hpp:21-68: x_i += (f_i - sum_j a_ij x_j) / a_ii
+ * with the row's sum in storage order over the current x. */
+void oc_gs_sweep(const oc_csr *A, const double *b, double *x, int reverse)
+{
+    const int n = A->n;
+    for (int k = 0; k < n; ++k) {
+        const int i = reverse ? n - 1 - k : k;
+        double diag = 0.0, sum = 0.0;
+        for (int j = A->rowptr[i]; j < A->rowptr[i + 1]; ++j) {
+            sum += A->val[j] * x[A->col[j]];
+            if (A->col[j] == i) diag = A->val[j];
+        }
+        x[i] += (b[i] - sum) / diag;
+    }
+}
+
+/* post != 0: post-smoothing, which gsMultiGridOp runs with stepT
+ * (SymmSmooth, gsMultiGrid.hpp:192-199): the reverse sweep for
+ * gsGaussSeidelOp<forward> (gsSimplePreconditioners.h:318-329). */
+static void smooth(const oc_mg *mg, int l, const double *b, double *x, int m, double *tmp, double *d, int post)
 {
     const oc_csr *A = mg->A[l];
     const int n = A->n;
+    if (mg->smoother == 3) {
+        for (int s = 0; s < m; ++s) oc_gs_sweep(A, b, x, post);
+        return;
+    }
     if (mg->smoother == 0) {
         for (int s = 0; s < m; ++s) jacobi_step(mg, l, b, x, tmp);
         return;
@@ -1234,7 +1259,7 @@ static void mg_step(const oc_mg *mg, int l, const double *b, double *x)
     double *cr = (double *)calloc(nc, sizeof(double));
     double *cc = (double *)calloc(nc, sizeof(double));
     double *dd = (double *)calloc(nf, sizeof(double));
-    smooth(mg, l, b, x, mg->npre, tmp, dd);
+    smooth(mg, l, b, x, mg->npre, tmp, dd, 0);
     oc_csr_spmv(mg->A[l], x, res);
     for (int i = 0; i < nf; ++i) res[i] -= b[i];
     const oc_csr *P = mg->P[l - 1];
@@ -1246,7 +1271,7 @@ static void mg_step(const oc_mg *mg, int l, const double *b, double *x)
         for (int m = P->rowptr[i]; m < P->rowptr[i + 1]; ++m) s += P->val[m] * cc[P->col[m]];
         x[i] -= mg->cgc * s;
     }
-    smooth(mg, l, b, x, mg->npost, tmp, dd);
+    smooth(mg, l, b, x, mg->npost, tmp, dd, 1);
     free(tmp); free(res); free(cr); free(cc); free(dd);
 }
 

@@ -135,8 +135,12 @@ oc_mg *oc_mg_create(const oc_csr *A_fine, int nlev, const oc_csr *const *P,
                     int npre, int npost, double damping, double cgc_damping);
 void oc_mg_apply(const oc_mg *mg, const double *r, double *z);
 /* smoother: 0 Jacobi (default), 1 Richardson, 2 Chebyshev with interval
- * [lmax / param, lmax]; lmax per level from 20 power iterations on D^-1 A. */
+ * [lmax / param, lmax]; lmax per level from 20 power iterations on D^-1 A;
+ * 3 Gauss-Seidel (forward sweeps before, reverse sweeps after the coarse
+ * correction). */
 void oc_mg_set_smoother(oc_mg *mg, int kind, double param);
+/* one Gauss-Seidel sweep (reverse != 0: rows descending), x updated in place */
+void oc_gs_sweep(const oc_csr *A, const double *b, double *x, int reverse);
 double oc_mg_lmax(const oc_mg *mg, int lvl);
 const oc_csr *oc_mg_matrix(const oc_mg *mg, int lvl);
 void oc_mg_free(oc_mg *mg);

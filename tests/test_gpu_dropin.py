@@ -88,7 +88,7 @@ def test_poisson2_example_grid_matches_python_mirror(ctx):
 
 
 def test_multigrid_example_matches_python_mirror(ctx):
-    text = _run("multiGrid_example", "--grid", "2", "2", "-r", "3", "-p", "2", "-l", "3",
+    text = _run("multiGrid_example", "--grid", "2", "2", "-r", "3", "-p", "2", "-l", "3", "-s", "j",
                 "--MG.Damping", "0.8", "-t", "1e-8")
     m = re.search(r"dofs (\d+) nnz (\d+)", text)
     s = re.search(r"solver cg iterations (\d+) relres (\S+)", text)
@@ -104,7 +104,7 @@ def test_multigrid_example_matches_python_mirror(ctx):
     assert float(s[2]) < 1e-8
     assert abs(l2 - gp.l2_error(x)) <= 1e-10 * l2
     # the stationary multigrid iteration converges too
-    text = _run("multiGrid_example", "--grid", "2", "2", "-r", "3", "-p", "2", "-l", "3", "-i", "d",
+    text = _run("multiGrid_example", "--grid", "2", "2", "-r", "3", "-p", "2", "-l", "3", "-i", "d", "-s", "j",
                 "--MG.Damping", "0.8", "-t", "1e-8")
     s = re.search(r"solver d iterations (\d+) relres (\S+)", text)
     assert float(s[2]) < 1e-8 and int(s[1]) < 100
@@ -260,3 +260,29 @@ def test_poisson2_example_fast_diagonalization(args):
         assert itf == 1, (args, itf)
         assert rf < 1e-10
         assert abs(lf - lj) <= 1e-6 * lj
+
+
+def test_multigrid_example_default_gauss_seidel(ctx):
+    """multiGrid_example with its default smoother, Gauss-Seidel (the
+    reference example's default): MG-PCG iterations and solution equal the
+    Python mirror's with the same smoother (damped Jacobi printed beside)."""
+    text = _run("multiGrid_example", "--grid", "2", "2", "-r", "3", "-p", "2", "-l", "3", "-t", "1e-8")
+    s = re.search(r"solver cg iterations (\d+) relres (\S+)", text)
+    l2 = float(re.search(r"L2 (\S+)", text)[1])
+    prob = problems.square_grid(2, 2, n=8, p=2)
+    gp = gsx.Problem(ctx, prob)
+    A, b = gp.assemble(gsx.PATTERN_FULL, gsx.FP64)
+    its = {}
+    for sm in ("gauss_seidel", "jacobi"):
+        mg = gsx.Multigrid(gp, A, 3, 1, 1, 0.8, smoother=sm)
+        x = ctx.array(gp.nfree).zero()
+        it, rel, ok = gsx.mg_cg_solve(A, mg, b, x, tol=1e-8, maxit=100)
+        its[sm] = it
+        if sm == "gauss_seidel":
+            assert int(s[1]) == it and ok
+            assert abs(l2 - gp.l2_error(x)) <= 1e-10 * l2
+        mg.close()
+        x.close()
+    print(its, flush=True)
+    A.close()
+    gp.close()

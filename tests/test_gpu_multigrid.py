@@ -149,10 +149,12 @@ def test_mg_pcg_matches_oracle(ctx, name):
 
 
 @pytest.mark.parametrize("name,smoother", [("grid2x2-p3", "chebyshev"), ("cube-p2", "chebyshev"),
-                                           ("grid3x2-p2", "richardson")])
+                                           ("grid3x2-p2", "richardson"), ("grid2x2-p3", "gauss_seidel"),
+                                           ("cube-p2", "gauss_seidel")])
 def test_polynomial_smoothers_match_oracle(ctx, name, smoother):
-    """Richardson and Chebyshev smoothing: the Chebyshev bounds (power
-    iteration) agree with the oracle's, one cycle agrees with the oracle's,
+    """Richardson, Chebyshev and Gauss-Seidel smoothing: the Chebyshev
+    bounds (power iteration) agree with the oracle's, one cycle agrees with
+    the oracle's (Gauss-Seidel by level sets: the sequential sweeps' values),
     and MG-PCG needs the oracle's iteration count (FP64; FP32 cycle within 3)."""
     make, ns = CASES[name]
     levels, maps = _hierarchy(make, ns)

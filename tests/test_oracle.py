@@ -820,3 +820,33 @@ def test_pcg_restates_the_cg_recurrences():
     xo, ito, erro = O.cg_solve(A, b, tol=1e-10)
     assert it == ito
     assert np.abs(x - xo).max() <= 1e-9 * np.abs(xo).max()
+
+
+def test_gauss_seidel_sweeps_and_two_grid_match_closed_forms():
+    """A forward Gauss-Seidel sweep is x <- (D + L)^-1 (b - U x), a reverse
+    sweep x <- (D + U)^-1 (b - L x); the two-grid cycle runs forward sweeps
+    before and reverse sweeps after the coarse correction (gsMultiGridOp's
+    SymmSmooth with stepT)."""
+    Af, bf, P = _two_grid_setup()
+    A, Pd = Af.toarray(), P.toarray()
+    lower, upper = np.tril(A), np.triu(A)
+
+    def fwd(x):
+        return np.linalg.solve(lower, bf - (A - lower) @ x)
+
+    def bwd(x):
+        return np.linalg.solve(upper, bf - (A - upper) @ x)
+    x0 = np.random.default_rng(4).standard_normal(len(bf))
+    assert np.abs(O.gs_sweep(Af, bf, x0) - fwd(x0)).max() <= 1e-12 * np.abs(fwd(x0)).max()
+    assert np.abs(O.gs_sweep(Af, bf, x0, reverse=True) - bwd(x0)).max() <= 1e-12 * np.abs(bwd(x0)).max()
+    Ac = Pd.T @ A @ Pd
+    for m in (1, 2):
+        mg = O.Multigrid(Af, [P], npre=m, npost=m, smoother="gauss_seidel")
+        x = np.zeros(len(bf))
+        for _ in range(m):
+            x = fwd(x)
+        x = x - Pd @ np.linalg.solve(Ac, Pd.T @ (A @ x - bf))
+        for _ in range(m):
+            x = bwd(x)
+        z = mg.apply(bf)
+        assert np.linalg.norm(z - x) <= 1e-11 * np.linalg.norm(x)
