@@ -14,6 +14,9 @@
 //                        Householder tridiagonalisation and the implicit QL
 //                        iteration with Wilkinson-type shifts (Golub & Van
 //                        Loan, Matrix Computations, 4th ed., 8.3), Q = L^-T V
+//   lu_solve, greville,  collocation at the anchors for Dirichlet values by
+//   find_span            interpolation (gsAssembler::
+//                        computeDirichletDofsIntpl)
 #pragma once
 
 #include <algorithm>
@@ -211,6 +214,64 @@ inline bool gen_sym_eig(int n, const std::vector<double> &K, const std::vector<d
             Q[(size_t)i * n + c] = s / L[(size_t)i * n + i];
         }
     return true;
+}
+
+// Solve A X = B in place (A n x n row-major, destroyed; B n x nrhs
+// row-major, overwritten by X) by Gaussian elimination with partial
+// pivoting; false if A is singular.
+inline bool lu_solve(int n, std::vector<double> &A, std::vector<double> &B, int nrhs)
+{
+    for (int k = 0; k < n; ++k) {
+        int piv = k;
+        for (int i = k + 1; i < n; ++i)
+            if (std::fabs(A[(size_t)i * n + k]) > std::fabs(A[(size_t)piv * n + k])) piv = i;
+        if (A[(size_t)piv * n + k] == 0.0) return false;
+        if (piv != k) {
+            for (int j = 0; j < n; ++j) std::swap(A[(size_t)k * n + j], A[(size_t)piv * n + j]);
+            for (int j = 0; j < nrhs; ++j) std::swap(B[(size_t)k * nrhs + j], B[(size_t)piv * nrhs + j]);
+        }
+        for (int i = k + 1; i < n; ++i) {
+            const double f = A[(size_t)i * n + k] / A[(size_t)k * n + k];
+            if (f == 0.0) continue;
+            for (int j = k; j < n; ++j) A[(size_t)i * n + j] -= f * A[(size_t)k * n + j];
+            for (int j = 0; j < nrhs; ++j) B[(size_t)i * nrhs + j] -= f * B[(size_t)k * nrhs + j];
+        }
+    }
+    for (int k = n - 1; k >= 0; --k)
+        for (int j = 0; j < nrhs; ++j) {
+            double v = B[(size_t)k * nrhs + j];
+            for (int i = k + 1; i < n; ++i) v -= A[(size_t)k * n + i] * B[(size_t)i * nrhs + j];
+            B[(size_t)k * nrhs + j] = v / A[(size_t)k * n + k];
+        }
+    return true;
+}
+
+// Greville abscissae (the anchors of gsBSplineBasis) of an open knot vector.
+inline std::vector<double> greville(const std::vector<double> &U, int p)
+{
+    const int n = (int)U.size() - p - 1;
+    std::vector<double> g(n);
+    for (int i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int k = 1; k <= p; ++k) s += U[i + k];
+        g[i] = s / p;
+    }
+    return g;
+}
+
+// Knot span of u in the open knot vector U (the last nonempty span at the
+// right end).
+inline int find_span(const std::vector<double> &U, int p, double u)
+{
+    const int n = (int)U.size() - p - 1;
+    if (u >= U[n]) {
+        int s = n - 1;
+        while (s > p && !(U[s + 1] > U[s])) --s;
+        return s;
+    }
+    int s = p;
+    while (s + 1 < n && U[s + 1] <= u) ++s;
+    return s;
 }
 
 // B-spline values and first derivatives of the p+1 functions active on

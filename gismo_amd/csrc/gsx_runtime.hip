@@ -271,6 +271,7 @@ struct gsx_problem {
     // layers (structured operator of the solvers)
     // host copies of the inputs needed to build coarser levels
     std::vector<std::vector<double>> knots_h;   // npatch * 3 (empty beyond d)
+    std::vector<std::vector<double>> ctrl_h;    // control points per patch (nbasis x d)
     std::vector<int> dir_sides_h, ifaces_h;
     DevPtr d_f2g;
     int grid_dims[3] = {1, 1, 1};
@@ -626,6 +627,9 @@ gsx_status gsx_problem_create(gsx_ctx *c, int32_t npatch, const gsx_patch_desc *
     for (int k = 0; k < npatch; ++k)
         for (int j = 0; j < patches[k].d; ++j)
             P->knots_h[3 * k + j].assign(patches[k].knots_host[j], patches[k].knots_host[j] + patches[k].nknots[j]);
+    P->ctrl_h.resize(npatch);
+    for (int k = 0; k < npatch; ++k)
+        P->ctrl_h[k].assign(patches[k].ctrl_host, patches[k].ctrl_host + P->hp[k].nbasis * patches[k].d);
     if (ndir > 0) P->dir_sides_h.assign(dir_sides, dir_sides + 2 * ndir);
     if (nif > 0) P->ifaces_h.assign(ifaces, ifaces + 8 * nif);
     P->nelem = eoff;
@@ -3237,13 +3241,129 @@ gsx_status bdry_terms(gsx_problem *P, const std::vector<gsx::BdryItem> &items, s
 
 }  // namespace
 
+// Dirichlet values by interpolation (gsAssembler::computeDirichletDofsIntpl,
+// src/gsAssembler/gsAssembler.hpp:314-390): side by side in condition
+// order, g at the grid of the side's anchors (Greville abscissae of the
+// tangential directions, the normal coordinate at the knot vector's end)
+// mapped by the geometry, interpolated by the side's tensor basis; each side
+// writes all of its functions, so a later side overwrites the functions it
+// shares with an earlier one (homogeneous sides write zeros).
+static gsx_status interpolate_sides(gsx_problem *P, const std::vector<gsx_bc> &dir, std::vector<double> &fixed)
+{
+    gsx_ctx *c = P->ctx;
+    const int d = P->d;
+    for (const gsx_bc &b : dir) {
+        const HostPatch &h = P->hp[b.patch];
+        const int nd = (b.side - 1) / 2, par = (b.side - 1) % 2;
+        int od[2] = {0, 0}, no = 0;
+        for (int k = 0; k < d; ++k)
+            if (k != nd) od[no++] = k;
+        const int n0 = h.nb[od[0]], n1 = d == 3 ? h.nb[od[1]] : 1;
+        auto local = [&](int j0, int j1) {
+            int idx[3] = {0, 0, 0};
+            idx[nd] = par ? h.nb[nd] - 1 : 0;
+            idx[od[0]] = j0;
+            if (d == 3) idx[od[1]] = j1;
+            return (int64_t)idx[0] + (int64_t)h.nb[0] * (idx[1] + (int64_t)h.nb[1] * idx[2]);
+        };
+        auto put = [&](int64_t l, double v) -> bool {
+            const int g = P->dofs[P->boff[b.patch] + l] - P->nfree;
+            if (g < 0 || g >= (int)fixed.size()) return false;
+            fixed[g] = v;
+            return true;
+        };
+        if (b.kind == GSX_BCF_ZERO) {
+            for (int j1 = 0; j1 < n1; ++j1)
+                for (int j0 = 0; j0 < n0; ++j0)
+                    if (!put(local(j0, j1), 0.0)) return fail(c, GSX_ERR_ARG, "Dirichlet side function not eliminated");
+            continue;
+        }
+        const std::vector<double> *U[3];
+        for (int k = 0; k < d; ++k) U[k] = &P->knots_h[3 * b.patch + k];
+        std::vector<double> gr[2];
+        gr[0] = gsx::dense::greville(*U[od[0]], h.p[od[0]]);
+        if (d == 3) gr[1] = gsx::dense::greville(*U[od[1]], h.p[od[1]]);
+        const double un = par ? U[nd]->back() : U[nd]->front();
+        // g at the anchor grid: V[j1][j0]
+        std::vector<double> V((size_t)n0 * n1);
+        const std::vector<double> &C = P->ctrl_h[b.patch];
+        std::vector<double> Nv[3];
+        std::vector<int> span(3);
+        for (int j1 = 0; j1 < n1; ++j1)
+            for (int j0 = 0; j0 < n0; ++j0) {
+                double u[3] = {0, 0, 0};
+                u[nd] = un;
+                u[od[0]] = gr[0][j0];
+                if (d == 3) u[od[1]] = gr[1][j1];
+                for (int k = 0; k < d; ++k) {
+                    const int p = h.p[k];
+                    Nv[k].assign(p + 1, 0.0);
+                    std::vector<double> dN(p + 1);
+                    span[k] = gsx::dense::find_span(*U[k], p, u[k]);
+                    gsx::dense::bspline_ders(*U[k], p, span[k], u[k], Nv[k].data(), dN.data());
+                }
+                double x[3] = {0, 0, 0};
+                const int p0 = h.p[0], p1 = h.p[1], p2 = d == 3 ? h.p[2] : 0;
+                for (int c2 = 0; c2 <= p2; ++c2)
+                    for (int c1 = 0; c1 <= p1; ++c1)
+                        for (int c0 = 0; c0 <= p0; ++c0) {
+                            const double w = Nv[0][c0] * Nv[1][c1] * (d == 3 ? Nv[2][c2] : 1.0);
+                            const int64_t m = (span[0] - p0 + c0) +
+                                              (int64_t)h.nb[0] * ((span[1] - p1 + c1) +
+                                                                  (int64_t)h.nb[1] * (d == 3 ? span[2] - p2 + c2 : 0));
+                            for (int r = 0; r < d; ++r) x[r] += w * C[(size_t)m * d + r];
+                        }
+                double g;
+                if (b.kind == GSX_BCF_MANUFACTURED) {
+                    g = 1.0;
+                    for (int r = 0; r < d; ++r) g *= std::sin(M_PI * x[r]);
+                } else {
+                    g = b.value[0];
+                    for (int r = 0; r < d; ++r) g += b.value[1 + r] * x[r];
+                }
+                V[(size_t)j1 * n0 + j0] = g;
+            }
+        // collocation C_t[i][j] = N_j(anchor_i); coefs = C_1^-1 V C_0^-T
+        auto colloc = [&](int t, std::vector<double> &M) {
+            const int k = od[t], p = h.p[k], n = h.nb[k];
+            M.assign((size_t)n * n, 0.0);
+            std::vector<double> N(p + 1), dN(p + 1);
+            for (int i = 0; i < n; ++i) {
+                const int sp = gsx::dense::find_span(*U[k], p, gr[t][i]);
+                gsx::dense::bspline_ders(*U[k], p, sp, gr[t][i], N.data(), dN.data());
+                for (int a = 0; a <= p; ++a) M[(size_t)i * n + sp - p + a] = N[a];
+            }
+        };
+        std::vector<double> M0, W((size_t)n0 * n1);
+        colloc(0, M0);
+        for (int j1 = 0; j1 < n1; ++j1)
+            for (int j0 = 0; j0 < n0; ++j0) W[(size_t)j0 * n1 + j1] = V[(size_t)j1 * n0 + j0];
+        if (!gsx::dense::lu_solve(n0, M0, W, n1)) return fail(c, GSX_ERR_ARG, "singular boundary collocation");
+        // W[j0][j1] = (C_0^-1 V^T)[j0][j1]
+        if (d == 3) {
+            std::vector<double> M1, Z((size_t)n1 * n0);
+            colloc(1, M1);
+            for (int j0 = 0; j0 < n0; ++j0)
+                for (int j1 = 0; j1 < n1; ++j1) Z[(size_t)j1 * n0 + j0] = W[(size_t)j0 * n1 + j1];
+            if (!gsx::dense::lu_solve(n1, M1, Z, n0)) return fail(c, GSX_ERR_ARG, "singular boundary collocation");
+            for (int j1 = 0; j1 < n1; ++j1)
+                for (int j0 = 0; j0 < n0; ++j0) W[(size_t)j0 * n1 + j1] = Z[(size_t)j1 * n0 + j0];
+        }
+        for (int j1 = 0; j1 < n1; ++j1)
+            for (int j0 = 0; j0 < n0; ++j0)
+                if (!put(local(j0, j1), W[(size_t)j0 * n1 + j1]))
+                    return fail(c, GSX_ERR_ARG, "Dirichlet side function not eliminated");
+    }
+    return GSX_OK;
+}
+
 extern "C" gsx_status gsx_problem_set_bc(gsx_problem *P, int32_t nbc, const gsx_bc *bc, int32_t dir_values)
 {
     if (!P || nbc < 0 || (nbc > 0 && !bc)) return GSX_ERR_ARG;
     gsx_ctx *c = P->ctx;
     hipStream_t s = c->stream;
     if (dir_values != GSX_DIR_HOMOGENEOUS && dir_values != GSX_DIR_L2PROJECTION &&
-        dir_values != GSX_DIR_L2PROJECTION_INHOM)
+        dir_values != GSX_DIR_L2PROJECTION_INHOM && dir_values != GSX_DIR_INTERPOLATION)
         return fail(c, GSX_ERR_ARG, "unknown Dirichlet value method");
     const int d = P->d;
     std::vector<std::pair<int, int>> dset, iset;
@@ -3291,7 +3411,14 @@ extern "C" gsx_status gsx_problem_set_bc(gsx_problem *P, int32_t nbc, const gsx_
     const int64_t nfix = P->nglued - P->nfree;
     bool inhom = false;
     for (const gsx_bc &b : dir) inhom = inhom || b.kind != GSX_BCF_ZERO;
-    if (dir_values != GSX_DIR_HOMOGENEOUS && inhom && nfix > 0) {
+    if (dir_values == GSX_DIR_INTERPOLATION && inhom && nfix > 0) {
+        std::vector<double> fixed((size_t)nfix, 0.0);
+        GSX_TRY(interpolate_sides(P, dir, fixed));
+        P->d_fixed = dalloc(c, sizeof(double) * nfix);
+        if (!P->d_fixed) return fail(c, GSX_ERR_OOM, "device allocation (Dirichlet values)");
+        HIP_OK(c, hipMemcpyAsync(P->d_fixed.get(), fixed.data(), sizeof(double) * nfix, hipMemcpyHostToDevice, s));
+        HIP_OK(c, hipStreamSynchronize(s));
+    } else if (dir_values != GSX_DIR_HOMOGENEOUS && inhom && nfix > 0) {
         // boundary mass matrix and load over every Dirichlet side
         // (gsDirichletValuesByL2Projection, src/gsAssembler/gsDirichletValues.h:257-435)
         std::vector<std::map<int, double>> rows((size_t)nfix);

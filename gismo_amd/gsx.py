@@ -35,7 +35,7 @@ ASM_CSR_ONLY = 0x200
 ASM_KEEP_ELEMENTS = 0x400
 BC_DIRICHLET, BC_NEUMANN = 1, 2
 BCF_ZERO, BCF_MANUFACTURED, BCF_AFFINE = 0, 1, 2
-DIR_HOMOGENEOUS, DIR_L2PROJECTION, DIR_L2PROJECTION_INHOM = 0, 1, 2
+DIR_HOMOGENEOUS, DIR_L2PROJECTION, DIR_L2PROJECTION_INHOM, DIR_INTERPOLATION = 0, 1, 2, 3
 
 
 class GsxBc(C.Structure):
@@ -354,7 +354,9 @@ class Problem:
         BCF_AFFINE (values: c, a_0, a_1, a_2); dir_values DIR_L2PROJECTION
         (default: every Dirichlet side projected, gsFeSpace::setup),
         DIR_L2PROJECTION_INHOM (sides with nonzero data only,
-        gsAssembler::computeDirichletDofs) or DIR_HOMOGENEOUS."""
+        gsAssembler::computeDirichletDofs), DIR_INTERPOLATION (interpolation
+        at the side anchors, gsAssembler::computeDirichletDofsIntpl) or
+        DIR_HOMOGENEOUS."""
         arr = (GsxBc * max(1, len(bcs)))()
         for i, (patch, side, typ, kind, vals) in enumerate(bcs):
             arr[i].patch, arr[i].side, arr[i].type, arr[i].kind = patch, side, typ, kind

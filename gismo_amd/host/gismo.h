@@ -918,11 +918,11 @@ struct gsFeSpace {
     /// gsFeSpace::setup (src/gsAssembler/gsExpressions.h:1091-1351):
     /// interfaces glued (icont = 0), Dirichlet sides eliminated, their values
     /// by dir_values (gsDirichletValues, src/gsAssembler/gsDirichletValues.h:
-    /// 27-60): homogeneous, or l2Projection (interpolation is not provided)
+    /// 27-60): homogeneous, interpolation or l2Projection
     void setup(const gsBoundaryConditions<real_t> &b, int dv, int = 0) const
     {
-        if (dv != dirichlet::homogeneous && dv != dirichlet::l2Projection)
-            throw std::runtime_error("gsFeSpace::setup: Dirichlet values by l2Projection or homogeneous only");
+        if (dv != dirichlet::homogeneous && dv != dirichlet::l2Projection && dv != dirichlet::interpolation)
+            throw std::runtime_error("gsFeSpace::setup: Dirichlet values by homogeneous, interpolation or l2Projection");
         *bc = &b;
         *dir_values = dv;
     }
@@ -1039,9 +1039,12 @@ public:
         const gsBoundaryConditions<T> *bc = *m_u.bc ? *m_u.bc : m_bc;
         static const gsBoundaryConditions<T> none;
         m_prob = detail::makeProblem(*m_mp, *m_mb, bc ? *bc : none, l.kind, a.kappa, l.felem);
-        if (bc && bc->inhomogeneous())
-            detail::applyBc(m_prob, *bc, *m_u.dir_values == dirichlet::homogeneous ? GSX_DIR_HOMOGENEOUS
-                                                                                   : GSX_DIR_L2PROJECTION);
+        if (bc && bc->inhomogeneous()) {
+            const int dv = *m_u.dir_values;
+            detail::applyBc(m_prob, *bc, dv == dirichlet::homogeneous     ? GSX_DIR_HOMOGENEOUS
+                                         : dv == dirichlet::interpolation ? GSX_DIR_INTERPOLATION
+                                                                          : GSX_DIR_L2PROJECTION);
+        }
         const int prec = m_opt.askString("Precision", "fp64") == "mixed" ? GSX_MIXED : GSX_FP64;
         detail::assembleInto(m_prob, GSX_PATTERN_EXPR, prec, m_matrix, m_rhs);
     }
@@ -1103,17 +1106,17 @@ public:
         m_prob = detail::makeProblem(*m_mp, *m_mb, *m_bc, m_f.kind(), nullptr, nullptr);
         if (m_bc->inhomogeneous()) {
             // gsAssembler::computeDirichletDofs (src/gsAssembler/gsAssembler.hpp:231-297):
-            // DirichletValues 101 (interpolation, the default) is not provided
+            // DirichletValues 101 (interpolation, the default), 102 or 100
             const int dv = m_opt.askInt("DirichletValues", dirichlet::interpolation);
             for (const auto &c : m_bc->conditions())
                 if (c.type == condition_type::neumann && c.f && c.f->kind() != 2)
                     throw std::runtime_error("gsPoissonAssembler: scalar Neumann data are not provided "
                                              "(gsExprAssembler::assembleBdr takes fluxes)");
-            if (dv != dirichlet::l2Projection && dv != dirichlet::homogeneous)
-                throw std::runtime_error("gsPoissonAssembler: inhomogeneous Dirichlet values need the option "
-                                         "DirichletValues = 102 (l2Projection)");
-            detail::applyBc(m_prob, *m_bc, dv == dirichlet::homogeneous ? GSX_DIR_HOMOGENEOUS
-                                                                        : GSX_DIR_L2PROJECTION_INHOM);
+            if (dv != dirichlet::l2Projection && dv != dirichlet::homogeneous && dv != dirichlet::interpolation)
+                throw std::runtime_error("gsPoissonAssembler: DirichletValues must be 100, 101 or 102");
+            detail::applyBc(m_prob, *m_bc, dv == dirichlet::homogeneous     ? GSX_DIR_HOMOGENEOUS
+                                           : dv == dirichlet::interpolation ? GSX_DIR_INTERPOLATION
+                                                                            : GSX_DIR_L2PROJECTION_INHOM);
         }
         const int prec = m_opt.askString("Precision", "fp64") == "mixed" ? GSX_MIXED : GSX_FP64;
         detail::assembleInto(m_prob, GSX_PATTERN_FULL, prec, m_matrix, m_rhs);

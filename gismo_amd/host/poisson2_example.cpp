@@ -10,13 +10,16 @@
 //   --last        solve only on the finest level
 //   --full        gsPoissonAssembler (full pattern) instead of gsExprAssembler
 //   --mixed       FP32 iterations with FP64 residual updates
+//   --dv N        Dirichlet values: 100 homogeneous, 101 interpolation,
+//                 102 L2 projection (default: gsExprAssembler 102 as in the
+//                 reference example, gsPoissonAssembler its own default 101)
 //   --fd          CG preconditioned by gsPatchPreconditionersCreator::
 //                 fastDiagonalizationOp (single patch) instead of Jacobi
 //   --bvp         the boundary value problem of poisson2d_bvp: the domain
 //                 shifted by 1/4 (u = prod sin(pi x_i) does not vanish on
 //                 it), u on the west/south(/front) sides by L2 projection,
 //                 the flux grad u on the others (assembleBdr); with
-//                 --full u on every side, DirichletValues = l2Projection
+//                 --full u on every side
 // A file with <boundaryConditions> (the layout of the reference's
 // filedata/pde/poisson2d_bvp.xml: MultiPatch, source id 1, conditions id 2,
 // exact solution id 3) supplies its own data.
@@ -38,6 +41,7 @@ int main(int argc, char *argv[])
 {
     index_t numRefine = 5, numElevate = 1, dim = 2, gx = 0, gy = 0;
     bool last = false, full = false, mixed = false, bvp = false, fastdiag = false;
+    int dirValues = -1;   // Dirichlet values method; -1: each assembler's own default
     std::string fn;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
@@ -51,6 +55,7 @@ int main(int argc, char *argv[])
         else if (a == "--mixed") mixed = true;
         else if (a == "--bvp") bvp = true;
         else if (a == "--fd") fastdiag = true;
+        else if (a == "--dv" && i + 1 < argc) dirValues = std::atoi(argv[++i]);
         else { std::fprintf(stderr, "unknown argument %s\n", argv[i]); return 1; }
     }
     try {
@@ -121,7 +126,7 @@ int main(int argc, char *argv[])
         double ma_time = 0, slv_time = 0;
         for (int r = 0; r <= numRefine; ++r) {
             dbasis.uniformRefine();
-            u.setup(bc, dirichlet::l2Projection, 0);
+            u.setup(bc, dirValues < 0 ? (int)dirichlet::l2Projection : dirValues, 0);
             A.initSystem();
             auto t0 = std::chrono::steady_clock::now();
             const gsSparseMatrix<> *K;
@@ -130,7 +135,7 @@ int main(int argc, char *argv[])
             if (full) {
                 pa.reset(new gsPoissonAssembler<>(mp, dbasis, bcAll, f));
                 if (mixed) pa->options().setString("Precision", "mixed");
-                if (bvp) pa->options().setInt("DirichletValues", dirichlet::l2Projection);
+                if (dirValues >= 0) pa->options().setInt("DirichletValues", dirValues);
                 pa->assemble();
                 K = &pa->matrix();
                 rhs = &pa->rhs();

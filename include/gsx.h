@@ -183,6 +183,13 @@ typedef struct {
  * gsx_assemble_bdr; every later gsx_assemble moves the fixed columns to the
  * right-hand side (rhs_i -= K_ij g_j, gsExprAssembler.h:553-630).  Replaces
  * any earlier boundary data of the problem. */
+/* GSX_DIR_INTERPOLATION: gsAssembler::computeDirichletDofsIntpl
+ * (src/gsAssembler/gsAssembler.hpp:314-390, the gsPoissonAssembler default
+ * dirichlet::interpolation): per Dirichlet side in problem order, g at the
+ * side's anchor grid (Greville abscissae, normal coordinate at the knot
+ * vector's end) mapped by the geometry, interpolated by the side's tensor
+ * basis; a later side overwrites the functions it shares with an earlier
+ * one, homogeneous sides write zeros. */
 /* GSX_DIR_L2PROJECTION_INHOM: the projection of gsAssembler::
  * computeDirichletDofsL2Proj (src/gsAssembler/gsAssembler.hpp:392-534, the
  * gsPoissonAssembler path), which skips the homogeneous Dirichlet sides:
@@ -190,7 +197,8 @@ typedef struct {
 typedef enum {
     GSX_DIR_HOMOGENEOUS = 0,
     GSX_DIR_L2PROJECTION = 1,
-    GSX_DIR_L2PROJECTION_INHOM = 2
+    GSX_DIR_L2PROJECTION_INHOM = 2,
+    GSX_DIR_INTERPOLATION = 3
 } gsx_dir_values;
 gsx_status gsx_problem_set_bc(gsx_problem *prob, int32_t nbc, const gsx_bc *bc,
                               int32_t dir_values);

@@ -531,3 +531,91 @@ def pcg(A, b, prec, tol=1e-10, maxit=1000):
         p = z + (rz_new / rz) * p
         rz = rz_new
     return x, it, err
+
+
+# ---------------------------------------------------------------------------
+# Dirichlet values by interpolation (gsAssembler::computeDirichletDofsIntpl)
+# ---------------------------------------------------------------------------
+
+def _greville(knots, p):
+    n = len(knots) - p - 1
+    return np.array([np.sum(knots[i + 1:i + p + 1]) / p for i in range(n)])
+
+
+def geometry_at(patch, u):
+    """x(u) = sum_m C_m N_m(u) of a patch (direction 0 fastest)."""
+    d = patch.d
+    nb = [len(patch.knots[k]) - patch.p[k] - 1 for k in range(d)]
+    spans, vals = [], []
+    for k in range(d):
+        s = find_span(patch.knots[k], patch.p[k], u[k])
+        spans.append(s)
+        vals.append(bspline_ders(patch.knots[k], patch.p[k], s, u[k], 0)[0])
+    x = np.zeros(d)
+    for idx in np.ndindex(*[patch.p[k] + 1 for k in range(d)]):
+        w = np.prod([vals[k][idx[k]] for k in range(d)])
+        m, stride = 0, 1
+        for k in range(d):
+            m += (spans[k] - patch.p[k] + idx[k]) * stride
+            stride *= nb[k]
+        x += w * patch.ctrl[m]
+    return x
+
+
+def dirichlet_interpolation(patch_list, dofs, nfree, nfix, bcs):
+    """Fixed values by interpolation (gsAssembler::computeDirichletDofsIntpl,
+    src/gsAssembler/gsAssembler.hpp:314-390): the Dirichlet records
+    bcs = [(patch, side, 1, kind, values)] in condition order; per side, g at
+    the anchor grid (Greville abscissae of the tangential directions, the
+    normal coordinate at the knot vector's end) mapped by the geometry,
+    interpolated by the side's tensor basis (its collocation matrices); each
+    side writes all of its functions, homogeneous sides zeros."""
+    offs = np.cumsum([0] + [P.nbasis for P in patch_list])
+    out = np.zeros(nfix)
+    for pt, side, _, kind, vals in bcs:
+        P = patch_list[pt]
+        d = P.d
+        nb = [len(P.knots[k]) - P.p[k] - 1 for k in range(d)]
+        nd, par = (side - 1) // 2, (side - 1) % 2
+        od = [k for k in range(d) if k != nd]
+        ns = [nb[k] for k in od]
+
+        def local(js):
+            idx = [0] * 3
+            idx[nd] = nb[nd] - 1 if par else 0
+            for t, k in enumerate(od):
+                idx[k] = js[t]
+            return idx[0] + nb[0] * (idx[1] + (nb[1] if d > 1 else 1) * idx[2])
+        if kind == 0:
+            for js in np.ndindex(*ns[::-1]):
+                out[dofs[offs[pt] + local(js[::-1])] - nfree] = 0.0
+            continue
+        grs = [_greville(P.knots[k], P.p[k]) for k in od]
+        un = P.knots[nd][-1] if par else P.knots[nd][0]
+        G = np.zeros(ns[::-1])   # axes: tangential directions, last one first
+        for js in np.ndindex(*ns[::-1]):
+            jj = js[::-1]
+            u = [0.0] * d
+            u[nd] = un
+            for t, k in enumerate(od):
+                u[k] = grs[t][jj[t]]
+            x = geometry_at(P, u)
+            if kind == 1:
+                g = np.prod(np.sin(np.pi * x))
+            else:
+                g = vals[0] + np.dot(vals[1:1 + d], x)
+            G[js] = g
+        coef = G
+        for t, k in enumerate(od):
+            p = P.p[k]
+            n = nb[k]
+            Cm = np.zeros((n, n))
+            for i, a in enumerate(grs[t]):
+                s = find_span(P.knots[k], p, a)
+                Cm[i, s - p:s + 1] = bspline_ders(P.knots[k], p, s, a, 0)[0]
+            ax = len(od) - 1 - t
+            coef = np.moveaxis(np.linalg.solve(Cm, np.moveaxis(coef, ax, 0).reshape(n, -1)).reshape(
+                np.moveaxis(coef, ax, 0).shape), 0, ax)
+        for js in np.ndindex(*ns[::-1]):
+            out[dofs[offs[pt] + local(js[::-1])] - nfree] = coef[js]
+    return out

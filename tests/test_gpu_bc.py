@@ -171,3 +171,26 @@ def test_projection_of_inhomogeneous_sides_only(ctx):
     every = gp.fixed_values()
     assert np.abs(every - want).max() > 1e-6
     gp.close()
+
+
+
+@pytest.mark.parametrize("name,prob,bcs", CASES, ids=[c[0] for c in CASES])
+def test_dirichlet_interpolation_matches_oracle(ctx, name, prob, bcs):
+    """GSX_DIR_INTERPOLATION (gsAssembler::computeDirichletDofsIntpl): the
+    fixed values equal the oracle's interpolation over the problem's
+    Dirichlet sides in order (homogeneous sides write zeros)."""
+    print(f"[case] interpolation {name}", flush=True)
+    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    nfix = int(max(dofs)) + 1 - nfree
+    dbc = {(b[0], b[1]): b for b in bcs if b[2] == 1}
+    recs = [dbc.get((pt, sd), (pt, sd, 1, 0, ())) for pt, sd in prob.dir_sides]
+    want = O.dirichlet_interpolation(prob.patches, dofs, nfree, nfix, recs)
+    gp = gsx.Problem(ctx, prob)
+    gp.set_bc(bcs, gsx.DIR_INTERPOLATION)
+    got = gp.fixed_values()
+    assert np.abs(got - want).max() <= 1e-12 * max(1.0, np.abs(want).max())
+    A, b = gp.assemble()
+    Ao, bo = O.assemble(prob.patches, dofs, nfree, 0, fixed=want)
+    assert np.abs(b.download() - bo).max() <= 1e-12 * np.abs(bo).max()
+    for o in (A, b, gp):
+        o.close()

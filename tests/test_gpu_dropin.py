@@ -143,7 +143,7 @@ def test_multigrid_example_chebyshev_smoother():
     assert its["c"] < its["j"], its
 
 
-def _bvp_oracle(d, p, n, dsides, nsides, mode):
+def _bvp_oracle(d, p, n, dsides, nsides, mode, interp=False):
     """The --bvp problem of poisson2_example at n elements per direction:
     u = prod sin(pi x_i) on [0.25, 1.25]^d, projected Dirichlet data on
     dsides, the flux grad u on nsides."""
@@ -152,7 +152,9 @@ def _bvp_oracle(d, p, n, dsides, nsides, mode):
     prob = problems.Problem([P], [(0, s) for s in dsides], [])
     dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
     nfix = int(max(dofs)) + 1 - nfree
-    fixed = O.dirichlet_values(prob.patches, dofs, nfree, nfix, [(0, s, 1, 1, ()) for s in dsides])
+    recs = [(0, s, 1, 1, ()) for s in dsides]
+    fixed = (O.dirichlet_interpolation(prob.patches, dofs, nfree, nfix, recs) if interp
+             else O.dirichlet_values(prob.patches, dofs, nfree, nfix, recs))
     A, b = O.assemble(prob.patches, dofs, nfree, mode, fixed=fixed)
     if nsides:
         b = b + O.neumann_load(prob.patches, dofs, nfree, [(0, s, 2, 1, ()) for s in nsides])
@@ -192,13 +194,16 @@ def test_poisson2_example_bvp(d, args, rate):
     assert h1rates[-1] > 1.8, h1rates
 
 
-def test_poisson2_example_bvp_poisson_assembler():
-    """poisson2_example --bvp --full: gsPoissonAssembler with u on every side
-    and the option DirichletValues = 102 (l2Projection), on the full
-    pattern, against the oracle; converges at order 3."""
-    rows = _levels(_run("poisson2_example", "--bvp", "--full", "-e", "1", "-r", "3"))
+@pytest.mark.parametrize("dv", [None, "102"])
+def test_poisson2_example_bvp_poisson_assembler(dv):
+    """poisson2_example --bvp --full: gsPoissonAssembler with u on every side,
+    its default DirichletValues = 101 (interpolation at the side anchors) or
+    102 (L2 projection), on the full pattern, against the oracle; converges
+    at order 3."""
+    extra = [] if dv is None else ["--dv", dv]
+    rows = _levels(_run("poisson2_example", "--bvp", "--full", "-e", "1", "-r", "3", *extra))
     for lvl, (dofs, nnz, its, rel, l2) in enumerate(rows):
-        nfree, onnz, ito, eo, _ = _bvp_oracle(2, 2, 2 ** (lvl + 1), [1, 2, 3, 4], [], 1)
+        nfree, onnz, ito, eo, _ = _bvp_oracle(2, 2, 2 ** (lvl + 1), [1, 2, 3, 4], [], 1, interp=dv is None)
         assert (dofs, nnz) == (nfree, onnz)
         assert abs(its - ito) <= 2, (lvl, its, ito)
         assert abs(l2 - eo) <= 1e-4 * eo, (lvl, l2, eo)

@@ -850,3 +850,43 @@ def test_gauss_seidel_sweeps_and_two_grid_match_closed_forms():
             x = bwd(x)
         z = mg.apply(bf)
         assert np.linalg.norm(z - x) <= 1e-11 * np.linalg.norm(x)
+
+
+@pytest.mark.parametrize("d,p", [(2, 3), (3, 2)])
+def test_dirichlet_interpolation_pins(d, p):
+    """gsAssembler::computeDirichletDofsIntpl: an affine g is reproduced
+    exactly (its trace lies in the side's isoparametric space); the side
+    trace of the interpolant matches a manufactured g at every anchor
+    (scipy's B-splines evaluate the trace); a later side overwrites the
+    functions it shares with an earlier one (a homogeneous side writes
+    zeros)."""
+    from scipy.interpolate import BSpline
+    sides = list(range(1, 2 * d + 1))
+    prob = _bc_patch(d, p, 4, sides, seed=21)
+    P = prob.patches[0]
+    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    nfix = P.nbasis - nfree
+    fx = O.dirichlet_interpolation([P], dofs, nfree, nfix, [(0, s, 1, 2, AFFINE) for s in sides])
+    _, expect = _affine_at_ctrl(prob, dofs, nfree, 0)
+    assert np.abs(fx - expect).max() <= 1e-13
+    # interpolation property on side 1 (west, u_0 = 0) for g = prod sin(pi x)
+    fx = O.dirichlet_interpolation([P], dofs, nfree, nfix, [(0, 1, 1, 1, ())])
+    nb = P.nbasis_dir()
+    coef = np.zeros(nb[1:][::-1])
+    for idx in np.ndindex(*nb[1:][::-1]):
+        m = 0 + nb[0] * (idx[-1] + (nb[1] * idx[0] if d == 3 else 0))
+        coef[idx] = fx[dofs[m] - nfree]
+    grs = [O._greville(P.knots[k], P.p[k]) for k in range(1, d)]
+    for js in np.ndindex(*[len(g) for g in grs]):
+        u = [0.0] + [grs[t][js[t]] for t in range(d - 1)]
+        want = np.prod(np.sin(np.pi * O.geometry_at(P, u)))
+        vals = [np.array([BSpline(P.knots[k], np.eye(nb[k])[j], P.p[k])(u[k]) for j in range(nb[k])])
+                for k in range(1, d)]
+        got = np.einsum('i,i->', vals[0], coef) if d == 2 else np.einsum('j,i,ij->', vals[0], vals[1], coef)
+        assert abs(got - want) <= 1e-12, (js, got, want)
+    # overwrite order at the corner of sides 1 and 3
+    corner = dofs[0] - nfree
+    a = O.dirichlet_interpolation([P], dofs, nfree, nfix, [(0, 1, 1, 2, AFFINE), (0, 3, 1, 0, ())])
+    b = O.dirichlet_interpolation([P], dofs, nfree, nfix, [(0, 3, 1, 0, ()), (0, 1, 1, 2, AFFINE)])
+    assert a[corner] == 0.0
+    assert abs(b[corner] - expect[corner]) <= 1e-13
