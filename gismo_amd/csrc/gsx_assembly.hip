@@ -423,8 +423,9 @@ __global__ __launch_bounds__(256) void k_elem3d(const PatchDev *__restrict__ pat
 #pragma unroll
                         for (int iz = 0; iz < NB1; ++iz) {
                             const double fi = (a == 2) ? sD[2][qz][iz] : sV[2][qz][iz];
+                            // entries iz > jz are the mirror thread's (see the store)
 #pragma unroll
-                            for (int jz = 0; jz < NB1; ++jz) acc[mc][iz][jz] += fi * u[jz];
+                            for (int jz = iz; jz < NB1; ++jz) acc[mc][iz][jz] += fi * u[jz];
                         }
                     }
                 }
@@ -433,6 +434,10 @@ __global__ __launch_bounds__(256) void k_elem3d(const PatchDev *__restrict__ pat
         }
     }
 
+    // Thread (u, v) = ((ix, iy), (jx, jy)) holds the entries (i, j) with
+    // i = (u, iz), j = (v, jz), iz <= jz; the pair with iz > jz is thread
+    // (v, u)'s (jz, iz).  Each unordered pair is written once to both (i, j)
+    // and (j, i) (bitwise symmetric): for iz == jz by the thread with u <= v.
     double *Ke = Kbuf + pd.kmat_off + e * (int64_t)(NB * NB) - rg.kshift;
 #pragma unroll
     for (int mc = 0; mc < MC; ++mc) {
@@ -440,16 +445,16 @@ __global__ __launch_bounds__(256) void k_elem3d(const PatchDev *__restrict__ pat
         if (c < NC) {
             const int ixjx = c / (NB1 * NB1), iyjy = c % (NB1 * NB1);
             const int ix = ixjx / NB1, jx = ixjx % NB1, iy = iyjy / NB1, jy = iyjy % NB1;
+            const bool uv = ix + NB1 * iy <= jx + NB1 * jy;
 #pragma unroll
             for (int iz = 0; iz < NB1; ++iz)
 #pragma unroll
-                for (int jz = 0; jz < NB1; ++jz) {
+                for (int jz = iz; jz < NB1; ++jz) {
+                    if (jz == iz && !uv) continue;
                     const int i = ix + NB1 * (iy + NB1 * iz);
                     const int j = jx + NB1 * (jy + NB1 * jz);
-                    if (i <= j) {
-                        Ke[i * NB + j] = acc[mc][iz][jz];
-                        Ke[j * NB + i] = acc[mc][iz][jz];
-                    }
+                    Ke[i * NB + j] = acc[mc][iz][jz];
+                    Ke[j * NB + i] = acc[mc][iz][jz];
                 }
         }
     }
