@@ -1803,6 +1803,12 @@ gsx_status gsx_cg_solve(gsx_matrix *A, int32_t precond, double tau, gsx_mg *mg, 
 {
     if (!A || !b || !x || maxit < 0) return GSX_ERR_ARG;
     gsx_ctx *c = A->ctx;
+    if (A->n == 0) {   // no free DOFs: nothing to iterate (gsIterativeSolver::solve with b = 0)
+        if (iters) *iters = 0;
+        if (relres) *relres = 0.0;
+        A->hist.assign(1, 0.0);
+        return GSX_OK;
+    }
     if (precond == GSX_PREC_MG || mg) {
         if (!mg) return fail(c, GSX_ERR_ARG, "multigrid preconditioner needs an gsx_mg");
         int it = 0;

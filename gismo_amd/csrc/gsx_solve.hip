@@ -567,6 +567,7 @@ hipError_t launch_diag_f32(int n, const int *rp, const int *ci, const float *v, 
 
 hipError_t launch_f64_to_f32(int64_t n, const double *a, float *b, hipStream_t s)
 {
+    if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL((k_convert<double, float>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a, b);
     return hipGetLastError();
 }
