@@ -303,6 +303,104 @@ inline void bspline_ders(const std::vector<double> &U, int p, int span, double u
     }
 }
 
+// Values and derivatives up to order nder (<= p) of the p+1 functions active
+// on span at u (The NURBS Book A2.3): ders[k][j] = d^k N_{span-p+j} / du^k.
+inline void bspline_ders_all(const std::vector<double> &U, int p, int span, double u, int nder,
+                             std::vector<std::vector<double>> &ders)
+{
+    std::vector<double> left(p + 1), right(p + 1);
+    std::vector<std::vector<double>> ndu(p + 1, std::vector<double>(p + 1, 0.0)), a(2, std::vector<double>(p + 1));
+    ndu[0][0] = 1.0;
+    for (int j = 1; j <= p; ++j) {
+        left[j] = u - U[span + 1 - j];
+        right[j] = U[span + j] - u;
+        double saved = 0.0;
+        for (int r = 0; r < j; ++r) {
+            ndu[j][r] = right[r + 1] + left[j - r];
+            const double tmp = ndu[r][j - 1] / ndu[j][r];
+            ndu[r][j] = saved + right[r + 1] * tmp;
+            saved = left[j - r] * tmp;
+        }
+        ndu[j][j] = saved;
+    }
+    ders.assign(nder + 1, std::vector<double>(p + 1, 0.0));
+    for (int j = 0; j <= p; ++j) ders[0][j] = ndu[j][p];
+    for (int r = 0; r <= p; ++r) {
+        int s1 = 0, s2 = 1;
+        a[0][0] = 1.0;
+        for (int k = 1; k <= nder; ++k) {
+            double d = 0.0;
+            const int rk = r - k, pk = p - k;
+            if (r >= k) {
+                a[s2][0] = a[s1][0] / ndu[pk + 1][rk];
+                d = a[s2][0] * ndu[rk][pk];
+            }
+            const int j1 = rk >= -1 ? 1 : -rk, j2 = (r - 1 <= pk) ? k - 1 : p - r;
+            for (int j = j1; j <= j2; ++j) {
+                a[s2][j] = (a[s1][j] - a[s1][j - 1]) / ndu[pk + 1][rk + j];
+                d += a[s2][j] * ndu[rk + j][pk];
+            }
+            if (r <= pk) {
+                a[s2][k] = -a[s1][k - 1] / ndu[pk + 1][r];
+                d += a[s2][k] * ndu[r][pk];
+            }
+            ders[k][r] = d;
+            std::swap(s1, s2);
+        }
+    }
+    double f = p;
+    for (int k = 1; k <= nder; ++k) {
+        for (int j = 0; j <= p; ++j) ders[k][j] *= f;
+        f *= (p - k);
+    }
+}
+
+// One side's boundary space of the subspace corrected mass smoother
+// (tildeSpaceBasis_oneside, src/gsSolver/gsPatchPreconditionersCreator.hpp:
+// 368-423): the p - b functions at the end u (b = 1 if that end is
+// eliminated) after skipping the one always in the interior space; the
+// matrix D of their odd derivatives of order 1..p-1 at u, scaled by h^i;
+// tilde = an orthonormal basis of the null space of D (the
+// (p+1)/2 - b directions of smallest singular value), compl = the rest.
+// Columns of tilde/compl are vectors over those p - b functions (row-major
+// (p-b) x cols).  The null space is taken from the eigenvectors of D^T D
+// (ascending eigenvalues), which spans the same subspaces as the SVD.
+inline bool tilde_oneside(const std::vector<double> &U, int p, bool left, bool dir, double h,
+                          std::vector<double> &tilde, int &nt, std::vector<double> &compl_, int &nc)
+{
+    const int b = dir ? 1 : 0, m = p - b;
+    tilde.clear();
+    compl_.clear();
+    nt = nc = 0;
+    if (m <= 0) return true;
+    const double u = left ? U.front() : U.back();
+    const int span = find_span(U, p, u);
+    std::vector<std::vector<double>> ders;
+    bspline_ders_all(U, p, span, u, p - 1, ders);
+    std::vector<double> D((size_t)m * m, 0.0);
+    const int offset = left ? b : 1;
+    for (int i = 1; i < p; i += 2)
+        for (int j = 0; j < m; ++j) D[(size_t)(i - b) * m + j] = std::pow(h, i) * ders[i][j + offset];
+    std::vector<double> DtD((size_t)m * m, 0.0), w, V;
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < m; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < m; ++k) s += D[(size_t)k * m + i] * D[(size_t)k * m + j];
+            DtD[(size_t)i * m + j] = s;
+        }
+    if (!sym_eig(m, DtD, w, V)) return false;
+    nt = (p + 1) / 2 - b;
+    if (nt < 0) nt = 0;
+    nc = m - nt;
+    tilde.assign((size_t)m * nt, 0.0);
+    compl_.assign((size_t)m * nc, 0.0);
+    for (int i = 0; i < m; ++i) {
+        for (int j = 0; j < nt; ++j) tilde[(size_t)i * nt + j] = V[(size_t)i * m + j];
+        for (int j = 0; j < nc; ++j) compl_[(size_t)i * nc + j] = V[(size_t)i * m + nt + j];
+    }
+    return true;
+}
+
 // Univariate mass M_ij = int N_i N_j and stiffness K_ij = int N_i' N_j' on
 // the parameter line of the open knot vector U (degree p), with the Gauss
 // rule (gx, gw) on [-1, 1] mapped to every nonempty span; n = |U| - p - 1.

@@ -131,4 +131,65 @@ hipError_t launch_fd_mode_f32(int m0, int m1, int m2, int dir, const float *M, c
     return launch_fd_mode_t<float>(m0, m1, m2, dir, M, x, xo, xs1, xs2, y, yo, ys1, ys2, l0, l1, l2, alpha, st, s);
 }
 
+// Rectangular mode product of the subspace corrected mass smoother: output
+// box o0 x o1 x o2, along direction dir the input has m_in entries and
+// y_a = sum_{i < m_in} M[i*o_dir + a] x_i; optionally divided by
+// alpha + l0[a] + l1[b] + l2[c] (null arrays count 0) and added to y.
+template <typename T>
+__global__ __launch_bounds__(256) void k_tensor_mode(int o0, int o1, int o2, int dir, int m_in,
+                                                     const T *__restrict__ M, const T *__restrict__ x, int64_t xo,
+                                                     int64_t xs1, int64_t xs2, T *__restrict__ y, int64_t yo,
+                                                     int64_t ys1, int64_t ys2, int scale, double alpha,
+                                                     const double *__restrict__ l0, const double *__restrict__ l1,
+                                                     const double *__restrict__ l2, int accumulate,
+                                                     const CGState *__restrict__ st)
+{
+    if (st && st->done) return;
+    const int64_t n = (int64_t)o0 * o1 * o2;
+    const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (o >= n) return;
+    const int a = (int)(o % o0), b = (int)((o / o0) % o1), c = (int)(o / ((int64_t)o0 * o1));
+    const int od = dir == 0 ? o0 : (dir == 1 ? o1 : o2);
+    const int j = dir == 0 ? a : (dir == 1 ? b : c);
+    const int64_t stride = dir == 0 ? 1 : (dir == 1 ? xs1 : xs2);
+    const int64_t base = xo + (dir == 0 ? 0 : a) + (dir == 1 ? 0 : b * xs1) + (dir == 2 ? 0 : c * xs2);
+    double acc = 0.0;
+    for (int i = 0; i < m_in; ++i) acc += (double)M[(int64_t)i * od + j] * (double)x[base + i * stride];
+    if (scale) acc /= alpha + (l0 ? l0[a] : 0.0) + (l1 ? l1[b] : 0.0) + (l2 ? l2[c] : 0.0);
+    const int64_t at = yo + a + b * ys1 + c * ys2;
+    y[at] = accumulate ? (T)((double)y[at] + acc) : (T)acc;
+}
+
+template <typename T>
+hipError_t launch_tensor_mode_t(int o0, int o1, int o2, int dir, int m_in, const T *M, const T *x, int64_t xo,
+                                int64_t xs1, int64_t xs2, T *y, int64_t yo, int64_t ys1, int64_t ys2, int scale,
+                                double alpha, const double *l0, const double *l1, const double *l2, int accumulate,
+                                const CGState *st, hipStream_t s)
+{
+    const int64_t n = (int64_t)o0 * o1 * o2;
+    if (n <= 0) return hipSuccess;
+    if (dir < 0 || dir > 2 || m_in <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_tensor_mode<T>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, o0, o1, o2, dir, m_in,
+                       M, x, xo, xs1, xs2, y, yo, ys1, ys2, scale, alpha, l0, l1, l2, accumulate, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_tensor_mode_f64(int o0, int o1, int o2, int dir, int m_in, const double *M, const double *x,
+                                  int64_t xo, int64_t xs1, int64_t xs2, double *y, int64_t yo, int64_t ys1,
+                                  int64_t ys2, int scale, double alpha, const double *l0, const double *l1,
+                                  const double *l2, int accumulate, const CGState *st, hipStream_t s)
+{
+    return launch_tensor_mode_t<double>(o0, o1, o2, dir, m_in, M, x, xo, xs1, xs2, y, yo, ys1, ys2, scale, alpha, l0,
+                                        l1, l2, accumulate, st, s);
+}
+
+hipError_t launch_tensor_mode_f32(int o0, int o1, int o2, int dir, int m_in, const float *M, const float *x,
+                                  int64_t xo, int64_t xs1, int64_t xs2, float *y, int64_t yo, int64_t ys1,
+                                  int64_t ys2, int scale, double alpha, const double *l0, const double *l1,
+                                  const double *l2, int accumulate, const CGState *st, hipStream_t s)
+{
+    return launch_tensor_mode_t<float>(o0, o1, o2, dir, m_in, M, x, xo, xs1, xs2, y, yo, ys1, ys2, scale, alpha, l0,
+                                       l1, l2, accumulate, st, s);
+}
+
 }  // namespace gsx

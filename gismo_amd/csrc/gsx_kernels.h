@@ -209,6 +209,17 @@ hipError_t launch_fd_mode_f32(int m0, int m1, int m2, int dir, const float *M, c
                               int64_t xs1, int64_t xs2, float *y, int64_t yo, int64_t ys1, int64_t ys2,
                               const double *l0, const double *l1, const double *l2, double alpha,
                               const CGState *st, hipStream_t s);
+// rectangular mode product (output box o0 x o1 x o2, m_in inputs along
+// dir), optional scaling by 1/(alpha + l0[a] + l1[b] + l2[c]) (null: 0)
+// and accumulation into y
+hipError_t launch_tensor_mode_f64(int o0, int o1, int o2, int dir, int m_in, const double *M, const double *x,
+                                  int64_t xo, int64_t xs1, int64_t xs2, double *y, int64_t yo, int64_t ys1,
+                                  int64_t ys2, int scale, double alpha, const double *l0, const double *l1,
+                                  const double *l2, int accumulate, const CGState *st, hipStream_t s);
+hipError_t launch_tensor_mode_f32(int o0, int o1, int o2, int dir, int m_in, const float *M, const float *x,
+                                  int64_t xo, int64_t xs1, int64_t xs2, float *y, int64_t yo, int64_t ys1,
+                                  int64_t ys2, int scale, double alpha, const double *l0, const double *l1,
+                                  const double *l2, int accumulate, const CGState *st, hipStream_t s);
 hipError_t launch_cg2_precond_f64(int n, const double *r, double *z, const double *diag,
                                   double tau, double *prz, hipStream_t s);
 hipError_t launch_cg2_precond_f32(int n, const float *r, float *z, const float *diag, double tau,

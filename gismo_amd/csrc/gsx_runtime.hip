@@ -1583,6 +1583,7 @@ static gsx_status stencil_prepare(gsx_matrix *A, bool mixed)
 // ===========================================================================
 struct gsx_fd {
     gsx_ctx *ctx = nullptr;
+    int kind = 0;                          // 0 fast diagonalization, 1 subspace corrected mass smoother
     int d = 0;
     int m[3] = {1, 1, 1};
     int64_t size = 0;
@@ -1591,6 +1592,12 @@ struct gsx_fd {
     DevPtr t64[2], t32[2];
     int grid[3] = {1, 1, 1};
     int64_t goff = 0, gs1 = 0, gs2 = 0;   // free box on the padded grid
+    // subspace corrected mass smoother, per direction: S = B~ M~^-1 B~^T
+    // (n x n), W = B_compl Q_c (n x cc) and its transpose, lambda_c
+    int ct[3] = {0, 0, 0}, cc[3] = {0, 0, 0};
+    DevPtr S64[3], S32[3], W64[3], W32[3], WT64[3], WT32[3], lc[3];
+    DevPtr xin64;                          // copy of the input of an in-place apply (every type reads it)
+    double beta = 1.0, sigma = 1.0, h = 1.0;
 };
 
 static hipError_t fd_mode(int m0, int m1, int m2, int dir, const double *M, const double *x, int64_t xo,
@@ -1609,12 +1616,104 @@ static hipError_t fd_mode(int m0, int m1, int m2, int dir, const float *M, const
     return gsx::launch_fd_mode_f32(m0, m1, m2, dir, M, x, xo, xs1, xs2, y, yo, ys1, ys2, l0, l1, l2, alpha, st, s);
 }
 
+static hipError_t tensor_mode(int o0, int o1, int o2, int dir, int m_in, const double *M, const double *x,
+                              int64_t xo, int64_t xs1, int64_t xs2, double *y, int64_t yo, int64_t ys1, int64_t ys2,
+                              int scale, double alpha, const double *l0, const double *l1, const double *l2,
+                              int accumulate, const gsx::CGState *st, hipStream_t s)
+{
+    return gsx::launch_tensor_mode_f64(o0, o1, o2, dir, m_in, M, x, xo, xs1, xs2, y, yo, ys1, ys2, scale, alpha, l0,
+                                       l1, l2, accumulate, st, s);
+}
+
+static hipError_t tensor_mode(int o0, int o1, int o2, int dir, int m_in, const float *M, const float *x,
+                              int64_t xo, int64_t xs1, int64_t xs2, float *y, int64_t yo, int64_t ys1, int64_t ys2,
+                              int scale, double alpha, const double *l0, const double *l1, const double *l2,
+                              int accumulate, const gsx::CGState *st, hipStream_t s)
+{
+    return gsx::launch_tensor_mode_f32(o0, o1, o2, dir, m_in, M, x, xo, xs1, xs2, y, yo, ys1, ys2, scale, alpha, l0,
+                                       l1, l2, accumulate, st, s);
+}
+
+// Subspace corrected mass smoother (subspaceCorrectedMassSmootherOp,
+// src/gsSolver/gsPatchPreconditionersCreator.hpp:574-758): the sum over the
+// subspace types J (directions with the boundary complement; the others
+// with the interior space) of T_J C_J^-1 T_J^T.  Per type: for k in J
+// reduce with W_k^T, divide by s + sum_{k in J} lambda_k (s = alpha + beta
+// (d - |J|) / (sigma h^2)), expand with W_k; for k not in J apply S_k (the
+// interior type J = {} divides by s once).  Types missing a factor are
+// skipped; the first type writes y, the others add to it.
+template <typename T>
+static hipError_t scms_apply(const gsx_fd *F, const T *x, int64_t xo, int64_t xs1, int64_t xs2, T *y, int64_t yo,
+                             int64_t ys1, int64_t ys2, const gsx::CGState *st, hipStream_t s)
+{
+    constexpr bool f64 = std::is_same<T, double>::value;
+    T *buf[2] = {static_cast<T *>(f64 ? F->t64[0].get() : F->t32[0].get()),
+                 static_cast<T *>(f64 ? F->t64[1].get() : F->t32[1].get())};
+    const int D = F->d;
+    auto mat = [&](const DevPtr (&a64)[3], const DevPtr (&a32)[3], int k) {
+        return static_cast<const T *>(f64 ? a64[k].get() : a32[k].get());
+    };
+    bool first = true;
+    for (int J = 0; J < (1 << D); ++J) {
+        bool skip = false;
+        int nj = 0;
+        for (int k = 0; k < D; ++k) {
+            const bool inJ = (J >> k) & 1;
+            nj += inJ;
+            if (inJ ? F->cc[k] == 0 : F->ct[k] == 0) skip = true;
+        }
+        if (skip) continue;
+        const double sd = F->alpha + F->beta * (double)(D - nj) / (F->sigma * F->h * F->h);
+        struct Step { int dir; const T *M; int m_in, m_out; bool scale; };
+        Step steps[6];
+        int ns = 0, last_reduce = -1;
+        for (int k = 0; k < D; ++k)
+            if ((J >> k) & 1) {
+                last_reduce = ns;
+                steps[ns++] = {k, mat(F->W64, F->W32, k), F->m[k], F->cc[k], false};
+            }
+        for (int k = 0; k < D; ++k)
+            if ((J >> k) & 1) steps[ns++] = {k, mat(F->WT64, F->WT32, k), F->cc[k], F->m[k], false};
+        for (int k = 0; k < D; ++k)
+            if (!((J >> k) & 1)) steps[ns++] = {k, mat(F->S64, F->S32, k), F->m[k], F->m[k], false};
+        steps[last_reduce >= 0 ? last_reduce : 0].scale = true;
+        const double *lc[3] = {nullptr, nullptr, nullptr};
+        for (int k = 0; k < D; ++k)
+            if ((J >> k) & 1) lc[k] = dp<double>(F->lc[k]);
+        int cur[3] = {F->m[0], F->m[1], D == 3 ? F->m[2] : 1};
+        const T *in = x;
+        int64_t io = xo, is1 = xs1, is2 = xs2;
+        for (int t = 0; t < ns; ++t) {
+            const Step &S = steps[t];
+            int out[3] = {cur[0], cur[1], cur[2]};
+            out[S.dir] = S.m_out;
+            const bool last = t == ns - 1;
+            T *o = last ? y : buf[t & 1];
+            const int64_t oo = last ? yo : 0, os1 = last ? ys1 : out[0], os2 = last ? ys2 : (int64_t)out[0] * out[1];
+            const hipError_t e = tensor_mode(out[0], out[1], out[2], S.dir, S.m_in, S.M, in, io, is1, is2, o, oo, os1,
+                                             os2, S.scale ? 1 : 0, sd, lc[0], lc[1], lc[2], last && !first ? 1 : 0,
+                                             st, s);
+            if (e != hipSuccess) return e;
+            in = o;
+            io = oo;
+            is1 = os1;
+            is2 = os2;
+            cur[0] = out[0];
+            cur[1] = out[1];
+            cur[2] = out[2];
+        }
+        first = false;
+    }
+    return first ? hipErrorInvalidValue : hipSuccess;
+}
+
 // y = FD(x): element (a,b,c) of x at xo + a + b xs1 + c xs2 (likewise y); the
 // 2d mode products alternate between the two compact scratch tensors.
 template <typename T>
 static hipError_t fd_apply(const gsx_fd *F, const T *x, int64_t xo, int64_t xs1, int64_t xs2, T *y, int64_t yo,
                            int64_t ys1, int64_t ys2, const gsx::CGState *st, hipStream_t s)
 {
+    if (F->kind == 1) return scms_apply<T>(F, x, xo, xs1, xs2, y, yo, ys1, ys2, st, s);
     constexpr bool f64 = std::is_same<T, double>::value;
     T *buf[2] = {static_cast<T *>(f64 ? F->t64[0].get() : F->t32[0].get()),
                  static_cast<T *>(f64 ? F->t64[1].get() : F->t32[1].get())};
@@ -1922,6 +2021,168 @@ gsx_status gsx_fd_create(gsx_problem *P, double alpha, double beta, gsx_fd **out
     return GSX_OK;
 }
 
+gsx_status gsx_scms_create(gsx_problem *P, double sigma, double alpha, double beta, gsx_fd **out)
+{
+    if (!P || !out) return GSX_ERR_ARG;
+    gsx_ctx *c = P->ctx;
+    hipStream_t s = c->stream;
+    if (P->npatch != 1 || !P->d_f2g)
+        return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother needs a single-patch problem whose free "
+                                    "functions form a box");
+    if (!(beta != 0.0) || !(sigma > 0.0) || !(alpha >= 0.0))
+        return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother needs beta != 0, sigma > 0, alpha >= 0");
+    const HostPatch &hp = P->hp[0];
+    auto F = std::make_unique<gsx_fd>();
+    F->ctx = c;
+    F->kind = 1;
+    F->d = hp.d;
+    F->alpha = alpha;
+    F->beta = beta;
+    F->sigma = sigma;
+    // h: the smallest knot span of the basis (gsBasis::getMinCellLength)
+    double h = 1e300;
+    for (int k = 0; k < hp.d; ++k) {
+        const std::vector<double> &U = P->knots_h[k];
+        for (size_t i = 0; i + 1 < U.size(); ++i)
+            if (U[i + 1] > U[i]) h = std::min(h, U[i + 1] - U[i]);
+    }
+    F->h = h;
+    std::vector<std::vector<double>> hold64;
+    std::vector<std::vector<float>> hold32;
+    auto up = [&](DevPtr &d64, DevPtr &d32, std::vector<double> v) -> bool {
+        d64 = dalloc(c, sizeof(double) * std::max<size_t>(1, v.size()));
+        d32 = dalloc(c, sizeof(float) * std::max<size_t>(1, v.size()));
+        if (!d64 || !d32) return false;
+        hold32.emplace_back(v.begin(), v.end());
+        hold64.push_back(std::move(v));
+        return hipMemcpyAsync(d64.get(), hold64.back().data(), sizeof(double) * hold64.back().size(),
+                              hipMemcpyHostToDevice, s) == hipSuccess &&
+               hipMemcpyAsync(d32.get(), hold32.back().data(), sizeof(float) * hold32.back().size(),
+                              hipMemcpyHostToDevice, s) == hipSuccess;
+    };
+    namespace dn = gsx::dense;
+    for (int k = 0; k < hp.d; ++k) {
+        const int p = hp.p[k], nb = hp.nb[k], lo = P->box_lo[k], hi = P->box_hi[k], n = hi - lo + 1;
+        const bool dw = lo == 1, de = hi == nb - 2;
+        if (n <= 0) return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: no free functions in a direction");
+        std::vector<double> gx(p + 1), gw(p + 1), K, M;
+        gauss_rule(p + 1, gx.data(), gw.data());
+        dn::bspline_1d_matrices(P->knots_h[k], p, gx, gw, K, M);
+        std::vector<double> Kf((size_t)n * n), Mf((size_t)n * n);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) {
+                Kf[(size_t)i * n + j] = beta * K[(size_t)(lo + i) * nb + (lo + j)];
+                Mf[(size_t)i * n + j] = M[(size_t)(lo + i) * nb + (lo + j)];
+            }
+        // tilde space and complement (tildeSpaceBasis, :416-512); the
+        // derivatives are scaled by powers of the largest knot span
+        double hk = 0.0;
+        for (size_t i = 0; i + 1 < P->knots_h[k].size(); ++i)
+            hk = std::max(hk, P->knots_h[k][i + 1] - P->knots_h[k][i]);
+        std::vector<double> tL, cL, tR, cR;
+        int nL, ncL, nR, ncR;
+        if (!dn::tilde_oneside(P->knots_h[k], p, true, dw, hk, tL, nL, cL, ncL) ||
+            !dn::tilde_oneside(P->knots_h[k], p, false, de, hk, tR, nR, cR, ncR))
+            return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: boundary eigenproblem failed");
+        const int mL = nL + ncL > 0 ? p - (dw ? 1 : 0) : 0, mR = nR + ncR > 0 ? p - (de ? 1 : 0) : 0;
+        const int nI = n - mL - mR;
+        int ct, cc;
+        std::vector<double> Bt, Bl;   // n x ct, n x cc (row-major)
+        if (nI <= 0) {
+            ct = 0;
+            cc = n;
+            Bl.assign((size_t)n * n, 0.0);
+            for (int i = 0; i < n; ++i) Bl[(size_t)i * n + i] = 1.0;
+        } else {
+            ct = nL + nI + nR;
+            cc = ncL + ncR;
+            Bt.assign((size_t)n * ct, 0.0);
+            Bl.assign((size_t)n * std::max(cc, 1), 0.0);
+            for (int i = 0; i < mL; ++i)
+                for (int j = 0; j < nL; ++j) Bt[(size_t)i * ct + j] = tL[(size_t)i * nL + j];
+            for (int j = 0; j < nI; ++j) Bt[(size_t)(mL + j) * ct + nL + j] = 1.0;
+            for (int i = 0; i < mR; ++i)
+                for (int j = 0; j < nR; ++j) Bt[(size_t)(mL + nI + i) * ct + nL + nI + j] = tR[(size_t)i * nR + j];
+            for (int i = 0; i < mL; ++i)
+                for (int j = 0; j < ncL; ++j) Bl[(size_t)i * cc + j] = cL[(size_t)i * ncL + j];
+            for (int i = 0; i < mR; ++i)
+                for (int j = 0; j < ncR; ++j) Bl[(size_t)(mL + nI + i) * cc + ncL + j] = cR[(size_t)i * ncR + j];
+        }
+        F->m[k] = n;
+        F->ct[k] = ct;
+        F->cc[k] = cc;
+        auto gram = [&](const std::vector<double> &B, int cols, const std::vector<double> &A) {
+            std::vector<double> G((size_t)cols * cols, 0.0), AB((size_t)n * cols, 0.0);
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < cols; ++j) {
+                    double v = 0.0;
+                    for (int l = 0; l < n; ++l) v += A[(size_t)i * n + l] * B[(size_t)l * cols + j];
+                    AB[(size_t)i * cols + j] = v;
+                }
+            for (int a = 0; a < cols; ++a)
+                for (int b = 0; b < cols; ++b) {
+                    double v = 0.0;
+                    for (int i = 0; i < n; ++i) v += B[(size_t)i * cols + a] * AB[(size_t)i * cols + b];
+                    G[(size_t)a * cols + b] = v;
+                }
+            return G;
+        };
+        if (ct > 0) {
+            // S = B~ (B~^T M B~)^-1 B~^T
+            std::vector<double> Mt = gram(Bt, ct, Mf), X((size_t)ct * n);
+            for (int a = 0; a < ct; ++a)
+                for (int i = 0; i < n; ++i) X[(size_t)a * n + i] = Bt[(size_t)i * ct + a];
+            if (!dn::lu_solve(ct, Mt, X, n)) return fail(c, GSX_ERR_ARG, "singular interior mass matrix");
+            std::vector<double> S((size_t)n * n, 0.0);
+            for (int i = 0; i < n; ++i)
+                for (int j = 0; j < n; ++j) {
+                    double v = 0.0;
+                    for (int a = 0; a < ct; ++a) v += Bt[(size_t)i * ct + a] * X[(size_t)a * n + j];
+                    S[(size_t)i * n + j] = v;
+                }
+            if (!up(F->S64[k], F->S32[k], std::move(S))) return fail(c, GSX_ERR_OOM, "device allocation (smoother)");
+        }
+        if (cc > 0) {
+            // B_compl = M^-1 B_l2compl; (K_c, M_c) = B_compl^T (K, M) B_compl, K_c Q = M_c Q diag(lambda)
+            std::vector<double> Mcopy = Mf, Bc = Bl;
+            Bc.resize((size_t)n * cc);
+            if (!dn::lu_solve(n, Mcopy, Bc, cc)) return fail(c, GSX_ERR_ARG, "singular mass matrix");
+            std::vector<double> Mc = gram(Bc, cc, Mf), Kc = gram(Bc, cc, Kf), lam, Q;
+            if (!dn::gen_sym_eig(cc, Kc, Mc, lam, Q))
+                return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: boundary eigenproblem failed");
+            std::vector<double> W((size_t)n * cc, 0.0), WT((size_t)cc * n);
+            for (int i = 0; i < n; ++i)
+                for (int a = 0; a < cc; ++a) {
+                    double v = 0.0;
+                    for (int b = 0; b < cc; ++b) v += Bc[(size_t)i * cc + b] * Q[(size_t)b * cc + a];
+                    W[(size_t)i * cc + a] = v;
+                    WT[(size_t)a * n + i] = v;
+                }
+            DevPtr dummy;
+            if (!up(F->W64[k], F->W32[k], std::move(W)) || !up(F->WT64[k], F->WT32[k], std::move(WT)) ||
+                !up(F->lc[k], dummy, std::move(lam)))
+                return fail(c, GSX_ERR_OOM, "device allocation (smoother)");
+        }
+    }
+    F->size = (int64_t)F->m[0] * F->m[1] * F->m[2];
+    if (F->size != P->nfree) return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: free box does not match");
+    for (int j = 0; j < 2; ++j) {
+        F->t64[j] = dalloc(c, sizeof(double) * F->size);
+        F->t32[j] = dalloc(c, sizeof(float) * F->size);
+        if (!F->t64[j] || !F->t32[j]) return fail(c, GSX_ERR_OOM, "device allocation (smoother)");
+    }
+    F->xin64 = dalloc(c, sizeof(double) * F->size);
+    if (!F->xin64) return fail(c, GSX_ERR_OOM, "device allocation (smoother)");
+    const int pg = hp.p[0], pz = hp.d == 3 ? hp.p[0] : 0;
+    for (int k = 0; k < 3; ++k) F->grid[k] = P->grid_dims[k];
+    F->gs1 = P->grid_dims[0];
+    F->gs2 = (int64_t)P->grid_dims[0] * P->grid_dims[1];
+    F->goff = (P->box_lo[0] + pg) + F->gs1 * (P->box_lo[1] + pg) + F->gs2 * (P->box_lo[2] + pz);
+    HIP_OK(c, hipStreamSynchronize(s));
+    *out = F.release();
+    return GSX_OK;
+}
+
 gsx_status gsx_fd_destroy(gsx_fd *F)
 {
     delete F;
@@ -1933,6 +2194,10 @@ gsx_status gsx_fd_apply(gsx_fd *F, const double *r, double *z)
     if (!F || !r || !z) return GSX_ERR_ARG;
     gsx_ctx *c = F->ctx;
     KScope k(c, "fd_apply");
+    if (F->kind == 1 && r == z) {
+        HIP_OK(c, hipMemcpyAsync(F->xin64.get(), r, sizeof(double) * F->size, hipMemcpyDeviceToDevice, c->stream));
+        r = dp<double>(F->xin64);
+    }
     HIP_OK(c, fd_apply<double>(F, r, 0, F->m[0], (int64_t)F->m[0] * F->m[1], z, 0, F->m[0],
                                (int64_t)F->m[0] * F->m[1], nullptr, c->stream));
     return GSX_OK;

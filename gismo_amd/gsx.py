@@ -125,6 +125,7 @@ SIGNATURES = {
     "gsx_mg_destroy": [_vp],
     "gsx_fd_create": [_vp, C.c_double, C.c_double, _pp],
     "gsx_fd_destroy": [_vp],
+    "gsx_scms_create": [_vp, C.c_double, C.c_double, C.c_double, _vp],
     "gsx_fd_apply": [_vp, _vp, _vp],
     "gsx_cg_solve_fd": [_vp, _vp, C.c_int32, _vp, _vp, C.c_double, C.c_int32, _i32p, _dp],
     "gsx_mg_set_smoother": [_vp, C.c_int32, C.c_double],
@@ -544,6 +545,18 @@ class FastDiag:
         if self.h:
             lib().gsx_fd_destroy(self.h)
             self.h = None
+
+
+class SubspaceMassSmoother(FastDiag):
+    """Subspace corrected mass smoother of a single-patch problem
+    (gsPatchPreconditionersCreator::subspaceCorrectedMassSmootherOp(basis,
+    bc, opt, sigma, alpha, beta)): applied and used in CG like FastDiag."""
+
+    def __init__(self, problem, sigma=0.12, alpha=0.0, beta=1.0):
+        self.ctx = problem.ctx
+        h = C.c_void_p()
+        _check(self.ctx.h, lib().gsx_scms_create(problem.h, sigma, alpha, beta, C.byref(h)))
+        self.h = h
 
 
 def fd_cg_solve(matrix, fd, b, x, tol=1e-8, maxit=500, precision=FP64):

@@ -286,6 +286,16 @@ gsx_status gsx_cg_solve(gsx_matrix *A, int32_t precond, double tau,
  * K_k Q_k = M_k Q_k diag(lambda_k), Q_k^T M_k Q_k = I.  Needs beta > 0 and,
  * without a Dirichlet end in any direction, alpha > 0 (GSX_ERR_ARG). */
 gsx_status gsx_fd_create(gsx_problem *prob, double alpha, double beta, gsx_fd **out);
+/* Subspace corrected mass smoother of a single-patch problem
+ * (gsPatchPreconditionersCreator::subspaceCorrectedMassSmootherOp(basis,
+ * bc, opt, sigma, alpha, beta), src/gsSolver/gsPatchPreconditionersCreator.
+ * hpp:574-758, tilde spaces :368-512): per direction the space of splines
+ * with vanishing odd derivatives at the ends and its complement, the sum
+ * over the 2^d subspace combinations of the transfer, the interior mass
+ * inverses scaled by 1/(alpha + beta (d - |J|)/(sigma h^2)) and the
+ * boundary correction.  Returned in the gsx_fd handle: gsx_fd_apply and
+ * gsx_cg_solve_fd use it like the fast diagonalization. */
+gsx_status gsx_scms_create(gsx_problem *prob, double sigma, double alpha, double beta, gsx_fd **out);
 gsx_status gsx_fd_destroy(gsx_fd *fd);
 /* z = FD r over the free DOFs (device vectors, nfree entries; may alias). */
 gsx_status gsx_fd_apply(gsx_fd *fd, const double *r, double *z);

@@ -501,6 +501,106 @@ def fd_operator(patch, dir_sides, alpha=0.0, beta=1.0):
     return apply
 
 
+def tilde_oneside(knots, p, left, dirichlet):
+    """tildeSpaceBasis_oneside (gsPatchPreconditionersCreator.hpp:368-414,
+    odd derivatives): D holds h^i times the i-th derivatives (i = 1, 3, ..
+    < p) at the end of the p - b functions next to it (the function that
+    the Dirichlet condition removes, b = 1, and the (p+1)-th function from
+    the end left out); the tilde basis is the right singular vectors of D
+    for the (p+1)/2 - b smallest singular values (its null space), the
+    complement the rest.  Returns (tilde, compl) as (p-b) x cols arrays."""
+    b = int(bool(dirichlet))
+    m = p - b
+    if m <= 0:
+        return np.zeros((0, 0)), np.zeros((0, 0))
+    knots = np.asarray(knots, dtype=np.float64)
+    h = np.max(np.diff(np.unique(knots)))
+    u = knots[0] if left else knots[-1]
+    ders = bspline_ders(knots, p, find_span(knots, p, u), u, p - 1)
+    D = np.zeros((m, m))
+    off = b if left else 1
+    for i in range(1, p, 2):
+        D[i - b, :] = h ** i * ders[i, off:off + m]
+    V = np.linalg.svd(D)[2].T            # columns by descending singular value
+    nt = (p + 1) // 2 - b
+    return V[:, m - nt:], V[:, :m - nt]
+
+
+def tilde_space(knots, p, dir_west, dir_east):
+    """tildeSpaceBasis (:416-512): B_tilde (n x (n_L + n_I + n_R)) with the
+    left block, the identity on the interior functions and the right block;
+    B_l2compl (n x (n_cL + n_cR)) with the two complement blocks; with no
+    interior functions left, B_tilde is empty and B_l2compl the identity."""
+    nb = len(knots) - p - 1
+    n = nb - int(bool(dir_west)) - int(bool(dir_east))
+    bL, cL = tilde_oneside(knots, p, True, dir_west)
+    bR, cR = tilde_oneside(knots, p, False, dir_east)
+    mL, mR = bL.shape[0], bR.shape[0]
+    nL, nR, ncL, ncR = bL.shape[1], bR.shape[1], cL.shape[1], cR.shape[1]
+    nI = n - nL - nR - ncL - ncR
+    if nI <= 0:
+        return np.zeros((n, 0)), np.eye(n)
+    Bt = np.zeros((n, nL + nI + nR))
+    Bt[:mL, :nL] = bL
+    Bt[mL:mL + nI, nL:nL + nI] = np.eye(nI)
+    Bt[mL + nI:, nL + nI:] = bR
+    Bc = np.zeros((n, ncL + ncR))
+    Bc[:mL, :ncL] = cL
+    Bc[mL + nI:, ncL:] = cR
+    return Bt, Bc
+
+
+def _kron_dirs(mats):
+    """Kronecker product of per-direction matrices, direction 0 fastest."""
+    out = np.ones((1, 1))
+    for A in reversed(mats):
+        out = np.kron(out, A)
+    return out
+
+
+def scms_matrix(patch, dir_sides, sigma=0.12, alpha=0.0, beta=1.0):
+    """subspaceCorrectedMassSmootherOp (gsPatchPreconditionersCreator.hpp:
+    574-758) as a dense matrix over the free DOFs of a single patch: per
+    subspace type J (directions with the complement B_compl = M^-1
+    B_l2compl, the others with B_tilde) the transfer T_J, the correction
+    C_J = s (x)_k N_k + sum_{j in J} (x)_k (K_compl at j, N_k elsewhere)
+    with N_k = B_tilde^T M B_tilde (interior directions) or B_compl^T M
+    B_compl (directions in J) and s = alpha + beta (d - |J|) / (sigma h^2),
+    h the smallest knot span; T_J C_J^-1 T_J^T summed over the types whose
+    transfer has columns.  C_J is built and solved as it stands (no
+    eigen-decomposition)."""
+    d = patch.d
+    h = min(np.min(np.diff(np.unique(np.asarray(patch.knots[k], dtype=np.float64)))) for k in range(d))
+    Bt, Bc, Mt, Mc, Kc = [], [], [], [], []
+    for k in range(d):
+        lo, hi = _free_range(patch, k, dir_sides)
+        nb = len(patch.knots[k]) - patch.p[k] - 1
+        K, M = tensor_1d(patch.knots[k], patch.p[k], lo, hi, beta)
+        bt, bl = tilde_space(patch.knots[k], patch.p[k], lo == 1, hi == nb - 2)
+        bc = np.linalg.solve(M, bl)
+        Bt.append(bt)
+        Bc.append(bc)
+        Mt.append(bt.T @ M @ bt)
+        Mc.append(bc.T @ M @ bc)
+        Kc.append(bc.T @ K @ bc)
+    N = int(np.prod([B.shape[0] for B in Bt]))
+    out = np.zeros((N, N))
+    for J in range(1 << d):
+        inJ = [bool((J >> k) & 1) for k in range(d)]
+        T = [Bc[k] if inJ[k] else Bt[k] for k in range(d)]
+        if any(t.shape[1] == 0 for t in T):
+            continue
+        s = alpha + beta * (d - sum(inJ)) / (sigma * h * h)
+        Nk = [Mc[k] if inJ[k] else Mt[k] for k in range(d)]
+        Cm = s * _kron_dirs(Nk)
+        for j in range(d):
+            if inJ[j]:
+                Cm = Cm + _kron_dirs([Kc[k] if k == j else Nk[k] for k in range(d)])
+        Tf = _kron_dirs(T)
+        out += Tf @ np.linalg.solve(Cm, Tf.T)
+    return out
+
+
 def pcg(A, b, prec, tol=1e-10, maxit=1000):
     """gsConjugateGradient with a preconditioner given as a function
     (src/gsSolver/gsConjugateGradient.hpp:20-93 inside gsIterativeSolver::

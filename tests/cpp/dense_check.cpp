@@ -1,6 +1,7 @@
 // Host checks of gismo_amd/csrc/gsx_dense.h (no GPU): symmetric and
-// generalized eigen-decompositions against their defining identities, and
-// the univariate B-spline mass/stiffness against partition of unity.
+// generalized eigen-decompositions against their defining identities, the
+// univariate B-spline mass/stiffness against partition of unity, and the
+// one-sided tilde spaces (orthonormal; projector printed for comparison).
 // Prints "name value" lines; the generalized eigenvalues of a case are
 // printed for comparison with scipy.  Exits nonzero on a failed check.
 #include "gsx_dense.h"
@@ -130,6 +131,37 @@ int main()
         for (double l : lam) std::printf(" %.17g", l);
         std::printf("\n");
     }
+    // tilde spaces: orthonormal columns; the projector onto the tilde space
+    // is printed for comparison with the oracle's SVD
+    for (int p = 1; p <= 5; ++p)
+        for (int left = 0; left < 2; ++left)
+            for (int dir = 0; dir < 2; ++dir) {
+                std::vector<double> Uk = uniform_knots(p, 9, 0.25, 1.75), T, Cm;
+                int nt = 0, nc = 0;
+                CHECK(tilde_oneside(Uk, p, left == 1, dir == 1, 1.5 / 9, T, nt, Cm, nc));
+                const int m = nt + nc;
+                CHECK(m == std::max(0, p - dir));
+                double eo = 0.0;
+                for (int a = 0; a < m; ++a)
+                    for (int b = 0; b < m; ++b) {
+                        double s = 0.0;
+                        for (int i = 0; i < m; ++i) {
+                            const double va = a < nt ? T[(size_t)i * nt + a] : Cm[(size_t)i * nc + a - nt];
+                            const double vb = b < nt ? T[(size_t)i * nt + b] : Cm[(size_t)i * nc + b - nt];
+                            s += va * vb;
+                        }
+                        eo = std::max(eo, std::fabs(s - (a == b ? 1.0 : 0.0)));
+                    }
+                CHECK(eo < 1e-12);
+                std::printf("tilde p %d left %d dir %d nt %d proj", p, left, dir, nt);
+                for (int i = 0; i < m; ++i)
+                    for (int j = 0; j < m; ++j) {
+                        double s = 0.0;
+                        for (int a = 0; a < nt; ++a) s += T[(size_t)i * nt + a] * T[(size_t)j * nt + a];
+                        std::printf(" %.17g", s);
+                    }
+                std::printf("\n");
+            }
     std::printf("failures %d\n", failures);
     return failures ? 1 : 0;
 }

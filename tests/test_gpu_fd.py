@@ -109,3 +109,122 @@ def test_fd_error_paths(ctx):
     fd = gsx.FastDiag(gp, alpha=1.0)   # with a mass term it is regular
     fd.close()
     gp.close()
+
+
+# ---------------------------------------------------------------------------
+# subspace corrected mass smoother (subspaceCorrectedMassSmootherOp)
+# ---------------------------------------------------------------------------
+
+def _graded_curved(d, p, n, seed, sides):
+    prob = _curved(d, p, n, seed, sides)
+    P = prob.patches[0]
+    rng = np.random.default_rng(seed + 1)
+    knots = []
+    for _ in range(d):
+        w = rng.uniform(1.0, 3.0, n)
+        knots.append(np.concatenate([np.zeros(p + 1), np.cumsum(w)[:-1] / w.sum(), np.ones(p + 1)]))
+    P.knots = knots
+    return prob
+
+
+@pytest.mark.parametrize("d,p,n,sides,sigma,alpha,beta,graded", [
+    (2, 1, 8, [1, 2, 3, 4], 0.12, 0.0, 1.0, False),
+    (2, 2, 9, [1, 4], 0.12, 0.0, 1.0, True),
+    (2, 3, 10, [1, 2, 3, 4], 0.3, 0.5, 2.0, True),
+    (2, 4, 12, [], 0.12, 1.0, 1.0, False),
+    (2, 3, 2, [1, 2, 3, 4], 0.12, 0.0, 1.0, False),   # no interior functions: B_tilde empty
+    (2, 4, 40, [2, 3], 0.12, 0.0, 1.0, True),
+    (3, 2, 5, [1, 2, 3, 4, 5, 6], 0.12, 0.0, 1.0, False),
+    (3, 3, 6, [2, 3, 5], 0.2, 0.0, 1.5, True),
+    (3, 4, 5, [1, 2, 3, 4, 5, 6], 0.12, 0.3, 1.0, False),
+])
+def test_scms_apply_matches_oracle(ctx, d, p, n, sides, sigma, alpha, beta, graded):
+    """The device smoother (per type: reduce and expand with the
+    generalized eigenvectors of the boundary complement, a diagonal
+    scaling, the interior mass inverses) against the oracle's dense
+    restatement that builds and solves each correction as it stands."""
+    print(f"[case] scms apply d={d} p={p} n={n} sides={sides} graded={graded}", flush=True)
+    prob = (_graded_curved if graded else _curved)(d, p, n, 17, sides)
+    gp = gsx.Problem(ctx, prob)
+    sm = gsx.SubspaceMassSmoother(gp, sigma=sigma, alpha=alpha, beta=beta)
+    S = O.scms_matrix(prob.patches[0], sides, sigma=sigma, alpha=alpha, beta=beta)
+    assert S.shape == (gp.nfree, gp.nfree)
+    r = np.random.default_rng(12).standard_normal(gp.nfree)
+    want = S @ r
+    rd = ctx.to_device(r)
+    zd = ctx.array(gp.nfree)
+    sm.apply(rd, zd)
+    got = zd.download()
+    assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
+    sm.apply(rd, rd)   # in place
+    assert np.abs(rd.download() - got).max() <= 1e-14 * np.abs(got).max()
+    for o in (rd, zd, sm, gp):
+        o.close()
+
+
+def test_scms_large_is_symmetric_positive_and_linear(ctx):
+    """At 400^2 free functions (p = 3) the dense oracle is out of reach:
+    (u, S v) = (v, S u), (v, S v) > 0 and S(a u + v) = a S u + S v."""
+    print("[case] scms large 2d p=3 n=400", flush=True)
+    prob = _graded_curved(2, 3, 400, 23, [1, 2, 3, 4])
+    gp = gsx.Problem(ctx, prob)
+    sm = gsx.SubspaceMassSmoother(gp)
+    rng = np.random.default_rng(5)
+    u, v = rng.standard_normal(gp.nfree), rng.standard_normal(gp.nfree)
+    out = []
+    for w in (u, v, 0.75 * u + v):
+        wd, zd = ctx.to_device(w), ctx.array(gp.nfree)
+        sm.apply(wd, zd)
+        out.append(zd.download())
+        wd.close()
+        zd.close()
+    Su, Sv, Sw = out
+    assert abs(u @ Sv - v @ Su) <= 1e-12 * np.linalg.norm(u) * np.linalg.norm(Sv)
+    assert v @ Sv > 0 and u @ Su > 0
+    assert np.abs(Sw - (0.75 * Su + Sv)).max() <= 1e-13 * np.abs(Sw).max()
+    sm.close()
+    gp.close()
+
+
+@pytest.mark.parametrize("d,p,n,sides", [(2, 3, 12, [4]), (3, 2, 6, [1, 2, 3, 4, 5, 6])])
+def test_scms_pcg_matches_oracle(ctx, d, p, n, sides):
+    """CG on the stiffness matrix with the smoother as preconditioner (the
+    reference's preconditioner test uses it so, on a mass-shifted matrix):
+    the oracle's iterations within three and its solution, FP64 and mixed."""
+    print(f"[case] scms pcg d={d} p={p} n={n} sides={sides}", flush=True)
+    prob = _curved(d, p, n, 29, sides)
+    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    Ao, bo = O.assemble(prob.patches, dofs, nfree, 0)
+    S = O.scms_matrix(prob.patches[0], sides)
+    xo, ito, _ = O.pcg(Ao, bo, lambda r: S @ r, tol=1e-10, maxit=1000)
+    gp = gsx.Problem(ctx, prob)
+    A, b = gp.assemble(gsx.PATTERN_EXPR, gsx.MIXED)
+    sm = gsx.SubspaceMassSmoother(gp)
+    x = ctx.array(gp.nfree).zero()
+    it, rel, ok = gsx.fd_cg_solve(A, sm, b, x, tol=1e-10, maxit=1000)
+    assert ok and rel < 1e-10
+    assert abs(it - ito) <= 3, (it, ito)
+    xs = spla.spsolve(Ao.tocsc(), bo)
+    assert np.abs(x.download() - xs).max() <= 1e-8 * np.abs(xs).max()
+    xm = ctx.array(gp.nfree).zero()
+    itm, relm, okm = gsx.fd_cg_solve(A, sm, b, xm, tol=1e-10, maxit=1000, precision=gsx.MIXED)
+    assert okm and relm < 1e-10
+    assert itm <= ito * 3 // 2, (itm, ito)   # FP32 factors: slower once the recurrence nears its floor
+    assert np.abs(xm.download() - xs).max() <= 1e-8 * np.abs(xs).max()
+    for o in (x, xm, sm, A, b, gp):
+        o.close()
+
+
+def test_scms_error_paths(ctx):
+    grid = problems.square_grid(2, 1, n=4, p=2)
+    gp = gsx.Problem(ctx, grid)
+    with pytest.raises(gsx.GsxError):
+        gsx.SubspaceMassSmoother(gp)
+    gp.close()
+    P = problems.box_patch(2, 2, 4)
+    P.fkind = 1
+    gp = gsx.Problem(ctx, problems.Problem([P], problems.all_sides(2), []))
+    for kw in (dict(beta=0.0), dict(sigma=0.0), dict(alpha=-1.0)):
+        with pytest.raises(gsx.GsxError):
+            gsx.SubspaceMassSmoother(gp, **kw)
+    gp.close()

@@ -140,9 +140,10 @@ def test_mapper_rejects_bad_input():
 
 def test_dense_eigen_routines():
     """gismo_amd/csrc/gsx_dense.h on the CPU (g++): symmetric and generalized
-    eigen-decompositions satisfy their identities, and the generalized
+    eigen-decompositions satisfy their identities, the generalized
     eigenvalues of the univariate stiffness/mass equal scipy's eigh on the
-    oracle's matrices of the same knot vector."""
+    oracle's matrices of the same knot vector, and the one-sided tilde
+    spaces equal the oracle's."""
     import shutil
     import subprocess
     import tempfile
@@ -168,3 +169,18 @@ def test_dense_eigen_routines():
         K, M = O.tensor_1d(knots, p, 1, len(knots) - p - 3)
         want = sla.eigh(K, M, eigvals_only=True)
         assert np.abs(lam - want).max() <= 1e-10 * want.max(), p
+    # one-sided tilde spaces: the projector onto the space of vanishing odd
+    # derivatives equals the oracle's (SVD-based) one
+    ntilde = 0
+    for line in out.stdout.splitlines():
+        if not line.startswith("tilde"):
+            continue
+        f = line.split()
+        p, left, dirichlet = int(f[2]), f[4] == "1", f[6] == "1"
+        proj = np.array([float(v) for v in f[f.index("proj") + 1:]])
+        T, _ = O.tilde_oneside(O.knots_uniform(9, p, 0.25, 1.75), p, left, dirichlet)
+        want = (T @ T.T).ravel()
+        assert proj.shape == want.shape, (p, left, dirichlet)
+        assert np.abs(proj - want).max(initial=0.0) <= 1e-10, (p, left, dirichlet)
+        ntilde += 1
+    assert ntilde == 20

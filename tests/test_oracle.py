@@ -810,6 +810,145 @@ def test_fd_is_the_inverse_of_the_kronecker_sum(d, p, sides, alpha):
     assert np.abs(fd(r) - want).max() <= 1e-10 * np.abs(want).max()
 
 
+# ---------------------------------------------------------------------------
+# subspace corrected mass smoother (gsPatchPreconditionersCreator)
+# ---------------------------------------------------------------------------
+
+def _graded_knots(n, p, seed):
+    """Open knot vector on [0, 1] with n spans of random lengths (ratio up
+    to 3) and simple interior knots."""
+    w = np.random.default_rng(seed).uniform(1.0, 3.0, n)
+    inner = np.cumsum(w)[:-1] / w.sum()
+    return np.concatenate([np.zeros(p + 1), inner, np.ones(p + 1)])
+
+
+def _graded_box(d, p, n, seed):
+    P = problems.box_patch(d, p, n)
+    P.knots = [_graded_knots(n, p, seed + k) for k in range(d)]
+    return P
+
+
+@pytest.mark.parametrize("p", [2, 3, 4, 5])
+@pytest.mark.parametrize("dw,de", [(False, False), (True, False), (False, True), (True, True)])
+def test_tilde_space_is_the_space_of_vanishing_odd_derivatives(p, dw, de):
+    """tildeSpaceBasis: every column of B_tilde, as a spline on the free
+    functions, has vanishing derivatives of the odd orders below p at both
+    ends (evaluated with all active functions, so a wrong column offset or
+    a kept Dirichlet function fails); B_tilde and B_l2compl together form
+    an orthonormal-by-blocks basis of R^n, the complement has one column
+    per constraint (p // 2 per end) and is orthogonal to B_tilde."""
+    knots = _graded_knots(9, p, 7 * p + dw + 2 * de)
+    nb = len(knots) - p - 1
+    n = nb - dw - de
+    Bt, Bl = O.tilde_space(knots, p, dw, de)
+    assert Bt.shape == (n, n - 2 * (p // 2)) and Bl.shape == (n, 2 * (p // 2))
+    assert np.abs(Bt.T @ Bl).max() <= 1e-13
+    assert np.linalg.matrix_rank(np.hstack([Bt, Bl])) == n
+    for u in (knots[0], knots[-1]):
+        span = O.find_span(knots, p, u)
+        ders = O.bspline_ders(knots, p, span, u, p - 1)
+        for c in range(Bt.shape[1]):
+            full = np.zeros(nb)
+            full[int(dw):int(dw) + n] = Bt[:, c]
+            act = full[span - p:span + 1]
+            for i in range(1, p, 2):
+                scale = np.abs(ders[i]).max()
+                assert abs(ders[i] @ act) <= 1e-11 * scale, (u, c, i)
+    # the constraints restricted to the complement have full rank
+    rows = []
+    for u in (knots[0], knots[-1]):
+        span = O.find_span(knots, p, u)
+        ders = O.bspline_ders(knots, p, span, u, p - 1)
+        for i in range(1, p, 2):
+            r = np.zeros(nb)
+            r[span - p:span + 1] = ders[i]
+            rows.append(r[int(dw):int(dw) + n])
+    Cmap = np.array(rows).reshape(-1, n)
+    assert np.linalg.matrix_rank(Cmap @ Bl) == Bl.shape[1]
+
+
+@pytest.mark.parametrize("d,sides,sigma,alpha,beta,graded", [(2, [1, 2, 3, 4], 0.12, 0.0, 1.0, False),
+                                                             (2, [2, 3], 0.3, 0.5, 2.0, True),
+                                                             (3, [1, 6], 0.12, 0.0, 1.0, True),
+                                                             (3, [], 0.2, 1.0, 0.5, False)])
+def test_scms_degree_one_is_the_scaled_mass_inverse(d, sides, sigma, alpha, beta, graded):
+    """For p = 1 there are no odd-derivative constraints: the tilde space is
+    everything, only the interior type survives and the smoother is
+    (x)_k M_k^-1 / (alpha + beta d / (sigma h^2)) with h the smallest knot
+    span of the basis."""
+    P = _graded_box(d, 1, 6, 3) if graded else problems.box_patch(d, 1, 6)
+    S = O.scms_matrix(P, sides, sigma=sigma, alpha=alpha, beta=beta)
+    h = min(np.diff(np.unique(P.knots[k])).min() for k in range(d))
+    Minv = np.ones((1, 1))
+    for k in reversed(range(d)):
+        lo, hi = O._free_range(P, k, sides)
+        _, M = O.tensor_1d(P.knots[k], 1, lo, hi)
+        Minv = np.kron(Minv, np.linalg.inv(M))
+    want = Minv / (alpha + beta * d / (sigma * h * h))
+    assert np.abs(S - want).max() <= 1e-12 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("d,p,sides", [(2, 2, [1, 2, 3, 4]), (2, 3, [1, 4]), (2, 4, []), (3, 3, [2, 3, 5]),
+                                       (3, 2, [1, 2, 3, 4, 5, 6])])
+def test_scms_mass_limit_is_the_mass_inverse(d, p, sides):
+    """beta -> 0 leaves sum_J T_J (T_J^T M T_J)^-1 T_J^T / alpha, which is
+    M^-1 / alpha exactly when the subspaces are M-orthogonal and together
+    span the space (B_compl = M^-1 B_l2compl, all 2^d types kept)."""
+    P = _graded_box(d, p, 7 if d == 2 else 5, 11 * p + d)
+    S = O.scms_matrix(P, sides, sigma=0.12, alpha=1.0, beta=1e-13)
+    Minv = np.ones((1, 1))
+    for k in reversed(range(d)):
+        lo, hi = O._free_range(P, k, sides)
+        _, M = O.tensor_1d(P.knots[k], p, lo, hi)
+        Minv = np.kron(Minv, np.linalg.inv(M))
+    assert np.abs(S - Minv).max() <= 1e-8 * np.abs(Minv).max()
+
+
+@pytest.mark.parametrize("d,p,sides,alpha", [(2, 3, [1, 2, 3, 4], 0.0), (2, 2, [3], 0.7), (3, 3, [1, 6], 0.0),
+                                             (3, 4, [], 1.0)])
+def test_scms_is_the_galerkin_subspace_correction(d, p, sides, alpha):
+    """Each correction is the Galerkin restriction of the full tensor
+    operator alpha M + beta sum_{k in J} K_k (x) M_rest + beta (d - |J|) /
+    (sigma h^2) M (stiffness in the boundary directions, scaled mass in the
+    interior ones) to the range of T_J: assembled here from the full
+    Kronecker matrices instead of the univariate blocks, and the result is
+    symmetric positive definite."""
+    sigma, beta = 0.12, 1.3
+    P = _graded_box(d, p, 7 if d == 2 else 5, 5 * p + d)
+    S = O.scms_matrix(P, sides, sigma=sigma, alpha=alpha, beta=beta)
+    h = min(np.diff(np.unique(P.knots[k])).min() for k in range(d))
+    Ks, Ms, Bt, Bc = [], [], [], []
+    for k in range(d):
+        lo, hi = O._free_range(P, k, sides)
+        nb = len(P.knots[k]) - p - 1
+        K, M = O.tensor_1d(P.knots[k], p, lo, hi)
+        bt, bl = O.tilde_space(P.knots[k], p, lo == 1, hi == nb - 2)
+        Ks.append(K)
+        Ms.append(M)
+        Bt.append(bt)
+        Bc.append(np.linalg.solve(M, bl))
+
+    def kr(ms):
+        out = np.ones((1, 1))
+        for m in reversed(ms):
+            out = np.kron(out, m)
+        return out
+    want = np.zeros_like(S)
+    for J in range(1 << d):
+        inJ = [(J >> k) & 1 for k in range(d)]
+        T = kr([Bc[k] if inJ[k] else Bt[k] for k in range(d)])
+        if T.shape[1] == 0:
+            continue
+        A = (alpha + beta * (d - sum(inJ)) / (sigma * h * h)) * kr(Ms)
+        for j in range(d):
+            if inJ[j]:
+                A = A + beta * kr([Ks[k] if k == j else Ms[k] for k in range(d)])
+        want += T @ np.linalg.inv(T.T @ A @ T) @ T.T
+    assert np.abs(S - want).max() <= 1e-10 * np.abs(want).max()
+    assert np.abs(S - S.T).max() <= 1e-12 * np.abs(S).max()
+    assert np.linalg.eigvalsh(0.5 * (S + S.T)).min() > 0
+
+
 def test_pcg_restates_the_cg_recurrences():
     """The callable-preconditioner PCG gives the C oracle's iteration count
     and solution with the Jacobi preconditioner."""
