@@ -2021,28 +2021,48 @@ gsx_status gsx_fd_create(gsx_problem *P, double alpha, double beta, gsx_fd **out
     return GSX_OK;
 }
 
-gsx_status gsx_scms_create(gsx_problem *P, double sigma, double alpha, double beta, gsx_fd **out)
+// Free functions of a single patch as a box: degree, function count, knot
+// vector and free index range per direction.
+struct ScmsBox {
+    int d = 0;
+    int p[3] = {0, 0, 0}, nb[3] = {1, 1, 1}, lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    std::vector<double> knots[3];
+};
+
+static bool scms_box_of(const gsx_problem *P, ScmsBox &B)
 {
-    if (!P || !out) return GSX_ERR_ARG;
-    gsx_ctx *c = P->ctx;
+    if (!P || P->npatch != 1 || !P->d_f2g) return false;
+    const HostPatch &hp = P->hp[0];
+    B.d = hp.d;
+    for (int k = 0; k < hp.d; ++k) {
+        B.p[k] = hp.p[k];
+        B.nb[k] = hp.nb[k];
+        B.lo[k] = P->box_lo[k];
+        B.hi[k] = P->box_hi[k];
+        B.knots[k] = P->knots_h[k];
+    }
+    return true;
+}
+
+// Factors of the subspace corrected mass smoother on the free box (compact
+// vectors, direction 0 fastest); the grid layout fields are left unset.
+static gsx_status scms_build(gsx_ctx *c, const ScmsBox &B, double sigma, double alpha, double beta,
+                             std::unique_ptr<gsx_fd> &out)
+{
     hipStream_t s = c->stream;
-    if (P->npatch != 1 || !P->d_f2g)
-        return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother needs a single-patch problem whose free "
-                                    "functions form a box");
     if (!(beta != 0.0) || !(sigma > 0.0) || !(alpha >= 0.0))
         return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother needs beta != 0, sigma > 0, alpha >= 0");
-    const HostPatch &hp = P->hp[0];
     auto F = std::make_unique<gsx_fd>();
     F->ctx = c;
     F->kind = 1;
-    F->d = hp.d;
+    F->d = B.d;
     F->alpha = alpha;
     F->beta = beta;
     F->sigma = sigma;
     // h: the smallest knot span of the basis (gsBasis::getMinCellLength)
     double h = 1e300;
-    for (int k = 0; k < hp.d; ++k) {
-        const std::vector<double> &U = P->knots_h[k];
+    for (int k = 0; k < B.d; ++k) {
+        const std::vector<double> &U = B.knots[k];
         for (size_t i = 0; i + 1 < U.size(); ++i)
             if (U[i + 1] > U[i]) h = std::min(h, U[i + 1] - U[i]);
     }
@@ -2061,13 +2081,14 @@ gsx_status gsx_scms_create(gsx_problem *P, double sigma, double alpha, double be
                               hipMemcpyHostToDevice, s) == hipSuccess;
     };
     namespace dn = gsx::dense;
-    for (int k = 0; k < hp.d; ++k) {
-        const int p = hp.p[k], nb = hp.nb[k], lo = P->box_lo[k], hi = P->box_hi[k], n = hi - lo + 1;
+    for (int k = 0; k < B.d; ++k) {
+        const std::vector<double> &U = B.knots[k];
+        const int p = B.p[k], nb = B.nb[k], lo = B.lo[k], hi = B.hi[k], n = hi - lo + 1;
         const bool dw = lo == 1, de = hi == nb - 2;
         if (n <= 0) return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: no free functions in a direction");
         std::vector<double> gx(p + 1), gw(p + 1), K, M;
         gauss_rule(p + 1, gx.data(), gw.data());
-        dn::bspline_1d_matrices(P->knots_h[k], p, gx, gw, K, M);
+        dn::bspline_1d_matrices(U, p, gx, gw, K, M);
         std::vector<double> Kf((size_t)n * n), Mf((size_t)n * n);
         for (int i = 0; i < n; ++i)
             for (int j = 0; j < n; ++j) {
@@ -2077,12 +2098,12 @@ gsx_status gsx_scms_create(gsx_problem *P, double sigma, double alpha, double be
         // tilde space and complement (tildeSpaceBasis, :416-512); the
         // derivatives are scaled by powers of the largest knot span
         double hk = 0.0;
-        for (size_t i = 0; i + 1 < P->knots_h[k].size(); ++i)
-            hk = std::max(hk, P->knots_h[k][i + 1] - P->knots_h[k][i]);
+        for (size_t i = 0; i + 1 < U.size(); ++i)
+            hk = std::max(hk, U[i + 1] - U[i]);
         std::vector<double> tL, cL, tR, cR;
         int nL, ncL, nR, ncR;
-        if (!dn::tilde_oneside(P->knots_h[k], p, true, dw, hk, tL, nL, cL, ncL) ||
-            !dn::tilde_oneside(P->knots_h[k], p, false, de, hk, tR, nR, cR, ncR))
+        if (!dn::tilde_oneside(U, p, true, dw, hk, tL, nL, cL, ncL) ||
+            !dn::tilde_oneside(U, p, false, de, hk, tR, nR, cR, ncR))
             return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: boundary eigenproblem failed");
         const int mL = nL + ncL > 0 ? p - (dw ? 1 : 0) : 0, mR = nR + ncR > 0 ? p - (de ? 1 : 0) : 0;
         const int nI = n - mL - mR;
@@ -2165,7 +2186,6 @@ gsx_status gsx_scms_create(gsx_problem *P, double sigma, double alpha, double be
         }
     }
     F->size = (int64_t)F->m[0] * F->m[1] * F->m[2];
-    if (F->size != P->nfree) return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: free box does not match");
     for (int j = 0; j < 2; ++j) {
         F->t64[j] = dalloc(c, sizeof(double) * F->size);
         F->t32[j] = dalloc(c, sizeof(float) * F->size);
@@ -2173,12 +2193,29 @@ gsx_status gsx_scms_create(gsx_problem *P, double sigma, double alpha, double be
     }
     F->xin64 = dalloc(c, sizeof(double) * F->size);
     if (!F->xin64) return fail(c, GSX_ERR_OOM, "device allocation (smoother)");
+    HIP_OK(c, hipStreamSynchronize(s));
+    out = std::move(F);
+    return GSX_OK;
+}
+
+gsx_status gsx_scms_create(gsx_problem *P, double sigma, double alpha, double beta, gsx_fd **out)
+{
+    if (!P || !out) return GSX_ERR_ARG;
+    gsx_ctx *c = P->ctx;
+    ScmsBox B;
+    if (!scms_box_of(P, B))
+        return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother needs a single-patch problem whose free "
+                                    "functions form a box");
+    std::unique_ptr<gsx_fd> F;
+    const gsx_status e = scms_build(c, B, sigma, alpha, beta, F);
+    if (e != GSX_OK) return e;
+    if (F->size != P->nfree) return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: free box does not match");
+    const HostPatch &hp = P->hp[0];
     const int pg = hp.p[0], pz = hp.d == 3 ? hp.p[0] : 0;
     for (int k = 0; k < 3; ++k) F->grid[k] = P->grid_dims[k];
     F->gs1 = P->grid_dims[0];
     F->gs2 = (int64_t)P->grid_dims[0] * P->grid_dims[1];
     F->goff = (P->box_lo[0] + pg) + F->gs1 * (P->box_lo[1] + pg) + F->gs2 * (P->box_lo[2] + pz);
-    HIP_OK(c, hipStreamSynchronize(s));
     *out = F.release();
     return GSX_OK;
 }
@@ -2297,6 +2334,9 @@ struct MGLevel {
     // Gauss-Seidel: rows by level set, forward and reverse sweep
     DevPtr gs_fwd, gs_rev;
     std::vector<int> gs_fwd_off, gs_rev_off;
+    // subspace corrected mass smoother of the level's basis and its damping
+    std::shared_ptr<gsx_fd> scms;
+    double scms_tau = 0.0;
 };
 
 // Level sets of one Gauss-Seidel sweep over the CSR pattern: lev(i) = 1 +
@@ -2336,6 +2376,8 @@ struct gsx_mg {
     double cheb_ratio = 30.0;
     DevPtr Dinv;
     DevPtr zin, zout;                  // finest FP64 staging for the mixed cycle
+    ScmsBox fine_box;                  // free box of a single-patch finest level
+    bool fine_box_ok = false;
     ~gsx_mg()
     {
         for (auto &l : L)
@@ -2445,6 +2487,7 @@ extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nl
     mg->precision = precision;
     const bool mixed = precision == GSX_MIXED;
     mg->L.resize(nlevels);
+    mg->fine_box_ok = scms_box_of(fine, mg->fine_box);
     // finest level = the given matrix
     {
         MGLevel &f = mg->L[nlevels - 1];
@@ -2836,6 +2879,20 @@ static gsx_status smooth64(gsx_mg *mg, MGLevel &V, double *&x, double *&t, const
         if (m > 0) zero = false;
         return GSX_OK;
     }
+    if (mg->smoother == GSX_SMOOTH_SCMS) {
+        // x += tau S (b - A x)
+        if (m > 0 && zero) HIP_OK(c, gsx::launch_fill_f64(x, V.n, 0.0, s));
+        double *r = dp<double>(V.r), *d = dp<double>(V.d);
+        const gsx_fd *F = V.scms.get();
+        for (int k = 0; k < m; ++k) {
+            HIP_OK(c, residual(x, r));
+            HIP_OK(c, fd_apply<double>(F, r, 0, F->m[0], (int64_t)F->m[0] * F->m[1], d, 0, F->m[0],
+                                       (int64_t)F->m[0] * F->m[1], nullptr, s));
+            HIP_OK(c, gsx::launch_smooth_update_f64(V.n, x, d, d, nullptr, 0.0, V.scms_tau, 1, s));
+        }
+        if (m > 0) zero = false;
+        return GSX_OK;
+    }
     if (mg->smoother == GSX_SMOOTH_JACOBI) {
         for (int i = 0; i < m; ++i) {
             if (v32)
@@ -2874,6 +2931,19 @@ static gsx_status smooth32(gsx_mg *mg, MGLevel &V, float *&x, float *&t, const f
         for (int k = 0; k < m; ++k)
             for (size_t l = 0; l + 1 < off.size(); ++l)
                 HIP_OK(c, gsx::launch_gs_rows_f32(off[l + 1] - off[l], rows + off[l], rp, ci, va, b, x, s));
+        if (m > 0) zero = false;
+        return GSX_OK;
+    }
+    if (mg->smoother == GSX_SMOOTH_SCMS) {
+        if (m > 0 && zero) HIP_OK(c, gsx::launch_fill_f32(x, V.n, 0.0f, s));
+        float *r = dp<float>(V.r32), *d = dp<float>(V.d32);
+        const gsx_fd *F = V.scms.get();
+        for (int k = 0; k < m; ++k) {
+            HIP_OK(c, gsx::launch_residual_f32(V.lpr, V.n, rp, ci, va, x, b, r, s));
+            HIP_OK(c, fd_apply<float>(F, r, 0, F->m[0], (int64_t)F->m[0] * F->m[1], d, 0, F->m[0],
+                                      (int64_t)F->m[0] * F->m[1], nullptr, s));
+            HIP_OK(c, gsx::launch_smooth_update_f32(V.n, x, d, d, nullptr, 0.0f, (float)V.scms_tau, 1, s));
+        }
         if (m > 0) zero = false;
         return GSX_OK;
     }
@@ -3017,9 +3087,77 @@ static gsx_status mg_apply_dev(gsx_mg *mg, const double *r, double *z)
 
 extern "C" gsx_status gsx_mg_set_smoother(gsx_mg *mg, int32_t kind, double param)
 {
-    if (!mg || kind < GSX_SMOOTH_JACOBI || kind > GSX_SMOOTH_GAUSS_SEIDEL) return GSX_ERR_ARG;
+    if (!mg || kind < GSX_SMOOTH_JACOBI || kind > GSX_SMOOTH_SCMS) return GSX_ERR_ARG;
     gsx_ctx *c = mg->ctx;
     hipStream_t s = c->stream;
+    if (kind == GSX_SMOOTH_SCMS) {
+        // setupSubspaceCorrectedMassSmoother, examples/multiGrid_example.cpp:
+        // 407-560, for one patch with every side Dirichlet (the interior is
+        // the only component with free functions): per level the smoother
+        // of the level's basis (sigma = MG.Scaling, alpha = 0, beta = 1)
+        // damped by 1 / lambda, lambda from ten power iterations of S A
+        // (gsPreconditionerOp::estimateLargestEigenvalueOfPreconditionedSystem,
+        // src/gsSolver/gsPreconditioner.h:121-132) started from
+        // v0_i = 1 / (1 + i mod 5); once level > nlevels - 5 with more
+        // elements than the degree in every direction, that level's damping
+        // is kept for the finer levels.
+        const double sigma = param > 0.0 ? param : 0.12;
+        const int nl = (int)mg->L.size();
+        double saved = 0.0;
+        for (int l = 1; l < nl; ++l) {
+            MGLevel &V = mg->L[l];
+            ScmsBox B;
+            const bool ok = l == nl - 1 ? (mg->fine_box_ok ? (B = mg->fine_box, true) : false) : scms_box_of(V.prob, B);
+            bool alldir = ok;
+            for (int k = 0; ok && k < B.d; ++k) alldir = alldir && B.lo[k] == 1 && B.hi[k] == B.nb[k] - 2;
+            if (!alldir)
+                return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: single-patch problems with Dirichlet "
+                                            "conditions on every side only");
+            std::unique_ptr<gsx_fd> F;
+            gsx_status e = scms_build(c, B, sigma, 0.0, 1.0, F);
+            if (e != GSX_OK) return e;
+            if (F->size != V.n) return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: level size mismatch");
+            V.scms = std::shared_ptr<gsx_fd>(F.release());
+            const int n = V.n;
+            if (!V.d) V.d = dalloc(c, sizeof(double) * n);
+            if (mg->precision == GSX_MIXED && !V.d32) V.d32 = dalloc(c, sizeof(float) * n);
+            if (!V.d || (mg->precision == GSX_MIXED && !V.d32)) return fail(c, GSX_ERR_OOM, "device allocation (smoother)");
+            if (saved == 0.0) {
+                std::vector<double> xh(n);
+                for (int i = 0; i < n; ++i) xh[i] = 1.0 / (1.0 + (double)(i % 5));
+                DevPtr xv = dalloc(c, sizeof(double) * n), tv = dalloc(c, sizeof(double) * n);
+                if (!xv || !tv) return fail(c, GSX_ERR_OOM, "device allocation (power iteration)");
+                const gsx_fd *Fp = V.scms.get();
+                for (int it = 0; it < 10; ++it) {
+                    double nrm = 0.0;
+                    for (int i = 0; i < n; ++i) nrm += xh[i] * xh[i];
+                    nrm = std::sqrt(nrm);
+                    for (int i = 0; i < n; ++i) xh[i] /= nrm;
+                    HIP_OK(c, hipMemcpyAsync(xv.get(), xh.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+                    HIP_OK(c, gsx::launch_spmv_f64(V.lpr, n, dp<int>(V.rowptr), dp<int>(V.col), dp<double>(V.val),
+                                                   dp<double>(xv), dp<double>(tv), nullptr, nullptr, 0, nullptr, s));
+                    HIP_OK(c, fd_apply<double>(Fp, dp<double>(tv), 0, Fp->m[0], (int64_t)Fp->m[0] * Fp->m[1],
+                                               dp<double>(xv), 0, Fp->m[0], (int64_t)Fp->m[0] * Fp->m[1], nullptr,
+                                               s));
+                    HIP_OK(c, hipMemcpyAsync(xh.data(), xv.get(), sizeof(double) * n, hipMemcpyDeviceToHost, s));
+                    HIP_OK(c, hipStreamSynchronize(s));
+                }
+                double lam = 0.0;
+                for (int i = 0; i < n; ++i) lam += xh[i] * xh[i];
+                lam = std::sqrt(lam);
+                if (!(lam > 0.0)) return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: zero power iterate");
+                V.scms_tau = 1.0 / lam;
+                bool keep = l > nl - 5;
+                for (int k = 0; k < B.d; ++k) keep = keep && (int)B.knots[k].size() - 2 * B.p[k] - 1 > B.p[k];
+                if (keep) saved = V.scms_tau;
+            } else {
+                V.scms_tau = saved;
+            }
+            V.lmax = V.scms_tau;
+        }
+        mg->smoother = kind;
+        return GSX_OK;
+    }
     mg->smoother = kind;
     if (kind == GSX_SMOOTH_JACOBI) return GSX_OK;
     if (kind == GSX_SMOOTH_GAUSS_SEIDEL) {

@@ -461,10 +461,15 @@ class Matrix:
 class Multigrid:
     """Geometric multigrid preconditioner (gsMultiGridOp) on the device."""
 
-    SMOOTHERS = {"jacobi": 0, "richardson": 1, "chebyshev": 2, "gauss_seidel": 3}
+    SMOOTHERS = {"jacobi": 0, "richardson": 1, "chebyshev": 2, "gauss_seidel": 3, "scms": 4}
 
     def __init__(self, problem, matrix, nlevels, npre=1, npost=1, damping=1.0, cgc_damping=1.0,
-                 precision=FP64, smoother="jacobi", smoother_param=30.0):
+                 precision=FP64, smoother="jacobi", smoother_param=None):
+        """smoother_param: the Chebyshev interval ratio (default 30) or the
+        scaling sigma of the subspace corrected mass smoother ("scms",
+        default 0.12)."""
+        if smoother_param is None:
+            smoother_param = 0.12 if smoother == "scms" else 30.0
         self.ctx = problem.ctx
         h = C.c_void_p()
         _check(self.ctx.h, lib().gsx_mg_create(problem.h, matrix.h, nlevels, npre, npost, 1, damping,

@@ -61,7 +61,8 @@ typedef enum {
     GSX_SMOOTH_JACOBI = 0,      /* damped Jacobi (default)                   */
     GSX_SMOOTH_RICHARDSON = 1,  /* x += damping (b - A x)                    */
     GSX_SMOOTH_CHEBYSHEV = 2,   /* Jacobi-preconditioned Chebyshev            */
-    GSX_SMOOTH_GAUSS_SEIDEL = 3 /* forward sweeps before, reverse after        */
+    GSX_SMOOTH_GAUSS_SEIDEL = 3, /* forward sweeps before, reverse after       */
+    GSX_SMOOTH_SCMS = 4          /* subspace corrected mass smoother           */
 } gsx_smoother;
 
 typedef struct gsx_ctx gsx_ctx;
@@ -340,8 +341,19 @@ gsx_status gsx_mg_destroy(gsx_mg *mg);
  * 21-68), run exactly in the sequential order by level sets of the
  * triangular dependencies (one launch per set: made for parity, slow on
  * large levels). */
+/* GSX_SMOOTH_SCMS: the subspace corrected mass smoother as the reference's
+ * multigrid example sets it up (setupSubspaceCorrectedMassSmoother,
+ * examples/multiGrid_example.cpp:407-560) for a single patch with Dirichlet
+ * conditions on every side: x += tau_l S_l (b - A_l x), S_l the smoother of
+ * level l's basis (gsx_scms_create with sigma = param, 0.12 when param <= 0,
+ * alpha = 0, beta = 1), tau_l = 1 / lambda with lambda from ten power
+ * iterations of S_l A_l (src/gsSolver/gsPreconditioner.h:121-132; the
+ * reference starts from a random vector, here v0_i = 1 / (1 + i mod 5));
+ * the first level above nlevels - 5 with more elements than the degree in
+ * every direction keeps its tau for all finer levels. */
 gsx_status gsx_mg_set_smoother(gsx_mg *mg, int32_t kind, double param);
-/* The Chebyshev bound lmax of level lvl (0 before gsx_mg_set_smoother). */
+/* The level parameter of level lvl (0 before gsx_mg_set_smoother): the
+ * Chebyshev bound lmax, or tau_l of GSX_SMOOTH_SCMS. */
 gsx_status gsx_mg_smoother_bound(const gsx_mg *mg, int32_t lvl, double *lmax);
 /* z = MG(r): one cycle from a zero start (gsPreconditionerOp::apply,
  * src/gsSolver/gsPreconditioner.h:79-84). */

@@ -601,6 +601,73 @@ def scms_matrix(patch, dir_sides, sigma=0.12, alpha=0.0, beta=1.0):
     return out
 
 
+def scms_damping(A, S, steps=10):
+    """1 / lambda_max(S A) by estimateLargestEigenvalueOfPreconditionedSystem
+    (src/gsSolver/gsPreconditioner.h:121-132) for the smoother x += S (b -
+    A x): normalise, apply A, one step from zero (x = S (A x)), ten times;
+    lambda = ||x||.  The reference starts from a random vector; here v0_i =
+    1 / (1 + i mod 5)."""
+    n = A.shape[0]
+    x = 1.0 / (1.0 + (np.arange(n) % 5))
+    for _ in range(steps):
+        x = x / np.sqrt(x @ x)
+        x = S @ (A @ x)
+    return 1.0 / np.sqrt(x @ x)
+
+
+def scms_smoother_levels(level_patches, As, sigma=0.12):
+    """setupSubspaceCorrectedMassSmoother (examples/multiGrid_example.cpp:
+    407-560) for one patch with every side Dirichlet, where the interior is
+    the only component with free functions: per level l >= 1 the dense
+    smoother S_l (scms_matrix, alpha = 0, beta = 1) and tau_l = scms_damping;
+    the first level with l > nlev - 5 and more elements than the degree in
+    every direction keeps its tau for the finer levels (:538-553).  Returns
+    [None, (S_1, tau_1), ...]."""
+    nlev = len(As)
+    out = [None]
+    saved = 0.0
+    for l in range(1, nlev):
+        P = level_patches[l]
+        S = scms_matrix(P, list(range(1, 2 * P.d + 1)), sigma=sigma)
+        if saved == 0.0:
+            tau = scms_damping(As[l], S)
+            if l > nlev - 5 and all(len(np.unique(P.knots[k])) - 1 > P.p[k] for k in range(P.d)):
+                saved = tau
+        else:
+            tau = saved
+        out.append((S, tau))
+    return out
+
+
+def mg_vcycle(As, Ps, smoothers, npre=1, npost=1, cgc=1.0):
+    """gsMultiGridOp's V-cycle from a zero start (oc_mg_apply / mg_step of
+    the C oracle: src/gsSolver/gsMultiGrid.hpp:91-199) with smoothers given
+    as functions smoothers[l](b, x, post) updating x in place; As[0] is
+    solved exactly, Ps[l] maps level l to level l + 1.  Returns the apply
+    function."""
+    import scipy.sparse.linalg as spla
+    coarse = spla.factorized(As[0].tocsc())
+
+    def step(l, b, x):
+        if l == 0:
+            x[:] = coarse(b)
+            return
+        for _ in range(npre):
+            smoothers[l](b, x, False)
+        cr = Ps[l - 1].T @ (As[l] @ x - b)
+        cc = np.zeros(As[l - 1].shape[0])
+        step(l - 1, cr, cc)
+        x -= cgc * (Ps[l - 1] @ cc)
+        for _ in range(npost):
+            smoothers[l](b, x, True)
+
+    def apply(r):
+        x = np.zeros(As[-1].shape[0])
+        step(len(As) - 1, np.asarray(r, dtype=np.float64), x)
+        return x
+    return apply
+
+
 def pcg(A, b, prec, tol=1e-10, maxit=1000):
     """gsConjugateGradient with a preconditioner given as a function
     (src/gsSolver/gsConjugateGradient.hpp:20-93 inside gsIterativeSolver::

@@ -193,3 +193,84 @@ def test_polynomial_smoothers_match_oracle(ctx, name, smoother):
     A.close()
     b.close()
     gp.close()
+
+
+SCMS_CASES = {
+    "square-p3": (lambda n: problems.poisson_square(n=n, p=3), (4, 8, 16)),
+    "square-p4": (lambda n: problems.poisson_square(n=n, p=4), (2, 4, 8)),   # level 1: no interior, re-estimated
+    "cube-p2": (lambda n: problems.poisson_noisy_cube(n=n, p=2, seed=4), (2, 4, 8)),
+}
+
+
+@pytest.mark.parametrize("name", list(SCMS_CASES))
+def test_scms_smoother_matches_oracle(ctx, name):
+    """Subspace corrected mass smoothing (the reference multigrid example's
+    scms smoother on one patch with Dirichlet sides): the per-level damping
+    from the power iteration, one cycle, and MG-PCG iterations and solution
+    against the oracle's dense smoother in its numpy V-cycle."""
+    make, ns = SCMS_CASES[name]
+    print(f"[case] scms smoother {name}", flush=True)
+    levels, maps = _hierarchy(make, ns)
+    Af, bf, Ps, omg = _oracle_mg(levels, maps)
+    As = [omg.matrix(lvl) for lvl in range(len(ns))]
+    lv = O.scms_smoother_levels([L.patches[0] for L in levels], As)
+
+    def smoother(lvl):
+        S, tau = lv[lvl]
+
+        def f(b, x, post):
+            x += tau * (S @ (b - As[lvl] @ x))
+        return f
+    cyc = O.mg_vcycle(As, Ps, [None] + [smoother(lvl) for lvl in range(1, len(ns))], npre=1, npost=1)
+    xo, ito, _ = O.pcg(Af, bf, cyc, tol=1e-10, maxit=300)
+    gp = gsx.Problem(ctx, levels[-1])
+    A, b = gp.assemble()
+    for prec in (gsx.FP64, gsx.MIXED):
+        mg = gsx.Multigrid(gp, A, len(ns), 1, 1, 1.0, precision=prec, smoother="scms")
+        for lvl in range(1, len(ns)):
+            assert mg.lmax(lvl) == pytest.approx(lv[lvl][1], rel=1e-10), lvl
+        if prec == gsx.FP64:
+            r = ctx.to_device(bf)
+            z = ctx.array(gp.nfree)
+            mg.apply(r, z)
+            zo = cyc(bf)
+            assert np.linalg.norm(z.download() - zo) <= 1e-11 * np.linalg.norm(zo)
+            r.close()
+            z.close()
+        x = ctx.array(gp.nfree).zero()
+        it, rel, ok = gsx.mg_cg_solve(A, mg, b, x, tol=1e-10, maxit=300)
+        print(name, prec, it, ito, flush=True)
+        assert ok and rel < 1e-10
+        if prec == gsx.FP64:
+            assert abs(it - ito) <= 1
+        else:
+            assert it <= ito + 3
+        assert np.linalg.norm(x.download() - xo) <= 1e-8 * np.linalg.norm(xo)
+        x.close()
+        mg.close()
+    A.close()
+    b.close()
+    gp.close()
+
+
+def test_scms_smoother_error_paths(ctx):
+    """Several patches, or a side without a Dirichlet condition: the
+    interior is not the only component, which the device smoother does not
+    provide."""
+    grid = problems.square_grid(2, 1, n=4, p=2)
+    gp = gsx.Problem(ctx, grid)
+    A, b = gp.assemble()
+    with pytest.raises(gsx.GsxError):
+        gsx.Multigrid(gp, A, 2, smoother="scms")
+    A.close()
+    b.close()
+    gp.close()
+    P = problems.box_patch(2, 2, 8)
+    P.fkind = 1
+    gp = gsx.Problem(ctx, problems.Problem([P], [(0, 1), (0, 2), (0, 3)], []))
+    A, b = gp.assemble()
+    with pytest.raises(gsx.GsxError):
+        gsx.Multigrid(gp, A, 2, smoother="scms")
+    A.close()
+    b.close()
+    gp.close()

@@ -949,6 +949,43 @@ def test_scms_is_the_galerkin_subspace_correction(d, p, sides, alpha):
     assert np.linalg.eigvalsh(0.5 * (S + S.T)).min() > 0
 
 
+def test_mg_vcycle_restates_the_c_oracle_cycle():
+    """The numpy V-cycle with a Richardson smoother equals the C oracle's
+    cycle (gsMultiGridOp with gsRichardsonOp) on a three-level cube."""
+    levels = [problems.poisson_noisy_cube(n=n, p=2, seed=4) for n in (2, 4, 8)]
+    maps = [O.mapper(L.patches, L.dir_sides, L.ifaces) for L in levels]
+    Af, bf = O.assemble(levels[-1].patches, maps[-1][0], maps[-1][1], 0)
+    Ps = [O.transfer_multipatch(levels[i].patches, maps[i][0], maps[i][1], maps[i + 1][0], maps[i + 1][1])
+          for i in range(2)]
+    tau = 0.3 / np.abs(Af.diagonal()).max()
+    omg = O.Multigrid(Af, Ps, npre=2, npost=1, damping=tau, smoother="richardson")
+    As = [omg.matrix(l) for l in range(3)]
+
+    def rich(l):
+        def f(b, x, post):
+            x += tau * (b - As[l] @ x)
+        return f
+    cyc = O.mg_vcycle(As, Ps, [None, rich(1), rich(2)], npre=2, npost=1)
+    z = cyc(bf)
+    zo = omg.apply(bf)
+    assert np.linalg.norm(z - zo) <= 1e-12 * np.linalg.norm(zo)
+
+
+def test_scms_damping_is_the_inverse_spectral_radius():
+    """Ten power iterations of S A reach its largest eigenvalue (S and A
+    symmetric positive definite, so S A has a real spectrum) to within a few
+    percent, from below; the exact value comes from the generalized
+    eigenproblem A v = mu S^-1 v."""
+    import scipy.linalg as sla
+    prob = problems.poisson_square(n=8, p=3)
+    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    A, _ = O.assemble(prob.patches, dofs, nfree, 0)
+    S = O.scms_matrix(prob.patches[0], [1, 2, 3, 4])
+    lam = 1.0 / O.scms_damping(A, S)
+    mu = sla.eigh(A.toarray(), np.linalg.inv(S), eigvals_only=True).max()
+    assert 0.9 * mu <= lam <= mu * (1 + 1e-12)
+
+
 def test_pcg_restates_the_cg_recurrences():
     """The callable-preconditioner PCG gives the C oracle's iteration count
     and solution with the Jacobi preconditioner."""
