@@ -1241,8 +1241,10 @@ std::shared_ptr<gsJacobiOp<T>> makeJacobiOp(const gsSparseMatrix<T> &M, T tau = 
 /// Galerkin products are formed on the device).  Options: NumPreSmooth,
 /// NumPostSmooth, CoarseGridCorrectionDamping, Damping (of the Jacobi or
 /// Richardson smoother), Smoother ("Jacobi"/"j", "GaussSeidel"/"gs", "Richardson"/"r",
-/// "Chebyshev"/"c"), ChebyshevRatio, Precision ("fp64" or "mixed": FP32
-/// cycle).
+/// "Chebyshev"/"c", "SubspaceCorrectedMassSmoother"/"scms": one patch with
+/// Dirichlet sides, damped per level by its power iteration, Damping not
+/// applied), ChebyshevRatio, Scaling (sigma of the subspace corrected mass
+/// smoother, 0.12), Precision ("fp64" or "mixed": FP32 cycle).
 template <class T = real_t>
 class gsMultiGridOp : public gsPreconditionerOp<T> {
 public:
@@ -1281,15 +1283,18 @@ public:
                                         m_opt.askReal("CoarseGridCorrectionDamping", 1.0), prec, &h),
                           "gsx_mg_create");
             m_mg.reset(h, [](gsx_mg *q) { gsx_mg_destroy(q); });
-            // multiGrid_example's MG.Smoother names (r, j, gs) plus Chebyshev (c)
+            // multiGrid_example's MG.Smoother names (r, j, gs, scms) plus Chebyshev (c)
             const std::string sm = m_opt.askString("Smoother", "Jacobi");
             int kind = -1;
             if (sm == "Jacobi" || sm == "j") kind = GSX_SMOOTH_JACOBI;
             else if (sm == "GaussSeidel" || sm == "gs") kind = GSX_SMOOTH_GAUSS_SEIDEL;
             else if (sm == "Richardson" || sm == "r") kind = GSX_SMOOTH_RICHARDSON;
             else if (sm == "Chebyshev" || sm == "c") kind = GSX_SMOOTH_CHEBYSHEV;
+            else if (sm == "SubspaceCorrectedMassSmoother" || sm == "scms") kind = GSX_SMOOTH_SCMS;
             if (kind < 0) throw std::runtime_error("gsMultiGridOp: smoother '" + sm + "' is not supported");
-            detail::check(gsx_mg_set_smoother(h, kind, m_opt.askReal("ChebyshevRatio", 30.0)), "gsx_mg_set_smoother");
+            const double param = kind == GSX_SMOOTH_SCMS ? m_opt.askReal("Scaling", 0.12)
+                                                         : m_opt.askReal("ChebyshevRatio", 30.0);
+            detail::check(gsx_mg_set_smoother(h, kind, param), "gsx_mg_set_smoother");
         }
         return m_mg.get();
     }
@@ -1311,6 +1316,14 @@ template <class T = real_t>
 class gsFastDiagonalizationOp : public gsPreconditionerOp<T> {
 public:
     gsFastDiagonalizationOp(const gsTensorBSplineBasis<T> &basis, const gsBoundaryConditions<T> &bc, T alpha, T beta)
+        : gsFastDiagonalizationOp(basis, bc, alpha, beta, T(0))
+    {
+    }
+
+protected:
+    // sigma > 0: the subspace corrected mass smoother of the same basis
+    gsFastDiagonalizationOp(const gsTensorBSplineBasis<T> &basis, const gsBoundaryConditions<T> &bc, T alpha, T beta,
+                            T sigma)
     {
         const int d = basis.dim();
         if (d < 2 || d > 3) throw std::runtime_error("fastDiagonalizationOp: 2D and 3D bases only");
@@ -1354,9 +1367,14 @@ public:
         detail::check(gsx_problem_info(p, &nbasis, &nfree, &nelem, &nglued), "gsx_problem_info");
         m_n = (index_t)nfree;
         gsx_fd *f = nullptr;
-        detail::check(gsx_fd_create(p, alpha, beta, &f), "gsx_fd_create");
+        if (sigma > 0)
+            detail::check(gsx_scms_create(p, sigma, alpha, beta, &f), "gsx_scms_create");
+        else
+            detail::check(gsx_fd_create(p, alpha, beta, &f), "gsx_fd_create");
         m_fd.reset(f, [](gsx_fd *q) { gsx_fd_destroy(q); });
     }
+
+public:
     void apply(const gsMatrix<T> &input, gsMatrix<T> &x) const override
     {
         detail::DeviceVec a(sizeof(double) * std::max<index_t>(1, m_n)), b(sizeof(double) * std::max<index_t>(1, m_n));
@@ -1376,9 +1394,31 @@ private:
     std::shared_ptr<gsx_fd> m_fd;
 };
 
+/// The operator of gsPatchPreconditionersCreator::subspaceCorrectedMassSmootherOp
+/// (src/gsSolver/gsPatchPreconditionersCreator.hpp:574-758): the sum over the
+/// 2^d combinations of each direction's tilde space (vanishing odd
+/// derivatives at the ends) and its complement, built on the device
+/// (gsx_scms_create); CG uses it like the fast diagonalization.
+template <class T = real_t>
+class gsSubspaceCorrectedMassSmootherOp : public gsFastDiagonalizationOp<T> {
+public:
+    gsSubspaceCorrectedMassSmootherOp(const gsTensorBSplineBasis<T> &basis, const gsBoundaryConditions<T> &bc,
+                                      T sigma, T alpha, T beta)
+        : gsFastDiagonalizationOp<T>(basis, bc, alpha, beta, check_sigma(sigma))
+    {
+    }
+
+private:
+    static T check_sigma(T sigma)
+    {
+        if (!(sigma > 0)) throw std::runtime_error("subspaceCorrectedMassSmootherOp: sigma must be positive");
+        return sigma;
+    }
+};
+
 /// gsPatchPreconditionersCreator (src/gsSolver/gsPatchPreconditionersCreator.h)
-/// subset: fastDiagonalizationOp with DirichletStrategy elimination and
-/// gamma = 0.
+/// subset: fastDiagonalizationOp and subspaceCorrectedMassSmootherOp with
+/// DirichletStrategy elimination (gamma = 0).
 template <class T = real_t>
 struct gsPatchPreconditionersCreator {
     typedef std::shared_ptr<gsPreconditionerOp<T>> OpUPtr;
@@ -1396,6 +1436,17 @@ struct gsPatchPreconditionersCreator {
             throw std::runtime_error("fastDiagonalizationOp: Dirichlet elimination only");
         if (gamma != 0) throw std::runtime_error("fastDiagonalizationOp: gamma != 0 (pure Neumann averaging) is not provided");
         return std::make_shared<gsFastDiagonalizationOp<T>>(basis, bc, alpha, beta);
+    }
+    /// Defaults sigma = 12/100, alpha = 0, beta = 1 (gsPatchPreconditionersCreator.h:125-150).
+    static OpUPtr subspaceCorrectedMassSmootherOp(const gsTensorBSplineBasis<T> &basis,
+                                                  const gsBoundaryConditions<T> &bc,
+                                                  const gsOptionList &opt = defaultOptions(),
+                                                  T sigma = (T)(12) / (T)(100), T alpha = 0, T beta = 1)
+    {
+        if (opt.askInt("DirichletStrategy", dirichlet::elimination) != dirichlet::elimination)
+            throw std::runtime_error("subspaceCorrectedMassSmootherOp: Dirichlet elimination only");
+        if (beta == 0) throw std::runtime_error("subspaceCorrectedMassSmootherOp: beta == 0");
+        return std::make_shared<gsSubspaceCorrectedMassSmootherOp<T>>(basis, bc, sigma, alpha, beta);
     }
 };
 

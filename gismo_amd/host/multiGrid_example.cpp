@@ -11,7 +11,9 @@
 //   --MG.NumPreSmooth N   (default 1)     --MG.NumPostSmooth N (default 1)
 //   --MG.Damping X        Jacobi / Richardson damping (default 0.8)
 //   -s NAME               MG.Smoother: GaussSeidel (gs, default), Jacobi (j),
-//                         Richardson (r), Chebyshev (c)
+//                         Richardson (r), Chebyshev (c),
+//                         SubspaceCorrectedMassSmoother (scms: one patch)
+//   --MG.Scaling X        sigma of the scms smoother (default 0.12)
 //   -t X                  tolerance (default 1e-8)
 //   --Solver.MaxIterations N (default 100)
 //   -i cg|d               CG preconditioned by multigrid (default) or plain
@@ -30,7 +32,7 @@ using namespace gismo;
 int main(int argc, char *argv[])
 {
     index_t refinements = 5, degree = 2, levels = 3, pre = 1, post = 1, maxIt = 100, dim = 2, gx = 0, gy = 0;
-    real_t damping = 0.8, tol = 1e-8;
+    real_t damping = 0.8, tol = 1e-8, scaling = 0.12;
     std::string solver = "cg", geometry, smoother = "GaussSeidel";   // the reference default
     bool mixed = false;
     for (int i = 1; i < argc; ++i) {
@@ -50,6 +52,7 @@ int main(int argc, char *argv[])
             else if (a == "--MG.NumPostSmooth") post = std::atoi(next());
             else if (a == "--MG.Damping") damping = std::atof(next());
             else if (a == "-s" || a == "--MG.Smoother") smoother = next();
+            else if (a == "--MG.Scaling") scaling = std::atof(next());
             else if (a == "-t") tol = std::atof(next());
             else if (a == "--Solver.MaxIterations") maxIt = std::atoi(next());
             else if (a == "-i") solver = next();
@@ -92,6 +95,7 @@ int main(int argc, char *argv[])
         mgopt.addReal("Damping", "", damping);
         mgopt.addString("Precision", "", mixed ? "mixed" : "fp64");
         mgopt.addString("Smoother", "", smoother);
+        mgopt.addReal("Scaling", "", scaling);
         gsMultiGridOp<>::Ptr mg = gsMultiGridOp<>::make(assembler.matrix(), levels, mgopt);
 
         gsOptionList sopt = gsConjugateGradient<>::defaultOptions();

@@ -291,3 +291,26 @@ def test_multigrid_example_default_gauss_seidel(ctx):
     print(its, flush=True)
     A.close()
     gp.close()
+
+
+def test_multigrid_example_scms_smoother(ctx):
+    """multiGrid_example -s scms (the subspace corrected mass smoother,
+    one patch with Dirichlet sides, degree 3): the Python mirror's
+    iteration count, and fewer iterations than damped Jacobi smoothing."""
+    args = ["-d", "2", "-r", "4", "-p", "3", "-l", "3", "-t", "1e-8"]
+    text = _run("multiGrid_example", *args, "-s", "scms")
+    s = re.search(r"solver cg iterations (\d+) relres (\S+)", text)
+    assert float(s[2]) < 1e-8
+    prob = problems.poisson_square(n=16, p=3)
+    gp = gsx.Problem(ctx, prob)
+    A, b = gp.assemble(gsx.PATTERN_FULL, gsx.FP64)
+    mg = gsx.Multigrid(gp, A, 3, 1, 1, smoother="scms")
+    x = ctx.array(gp.nfree).zero()
+    it, rel, ok = gsx.mg_cg_solve(A, mg, b, x, tol=1e-8, maxit=100)
+    assert ok and int(s[1]) == it, (s[1], it)
+    tj = _run("multiGrid_example", *args, "-s", "j")
+    sj = re.search(r"solver cg iterations (\d+) relres (\S+)", tj)
+    print("scms", it, "jacobi", sj[1], flush=True)
+    assert it < int(sj[1])
+    for o in (x, mg, A, b, gp):
+        o.close()
