@@ -68,7 +68,9 @@ def run_step(ctx, gp, gsx, precision, tol, maxit, x, keep=False, fd=False):
 
 def stencil_spmv_bytes(nfree, slots, ngrid, sv, sx):
     """Algorithmic HBM bytes of one structured SpMV: every stored stencil
-    coefficient once, the row -> grid map, x once (grid), y once."""
+    coefficient once (the FP32 operator stores the upper half of the
+    symmetric stencil: slots - slots // 2 per row), the row -> grid map, x
+    once (grid), y once."""
     return nfree * slots * sv + nfree * 4 + ngrid * sx + nfree * sx
 
 
@@ -213,6 +215,7 @@ def main():
     mixed = results["mixed"]
     avg_ms = mixed["spmv_ms"] / max(1, mixed["spmv_working_launches"])
     slots = (2 * args.p + 1) ** 3
+    slots = slots - slots // 2            # FP32 operator: upper half stored
     ngrid = (args.n + 3 * args.p) ** 3   # nb = n + p per direction, padded by p each side
     traffic_alg = stencil_spmv_bytes(n, slots, ngrid, 4, 4)
     achieved = traffic_alg / (avg_ms * 1e-3) / 1e9

@@ -922,11 +922,14 @@ __global__ __launch_bounds__(256) void k_gather_patch(PatchBox bx, int nfree, in
             const int slot = k / R, rr = k % R;
             if (rr < nr) s64[stencil_index(slot, rbase + rr, NS)] = sAcc[rr][slot];
         }
-    if (s32)
-        for (int k = threadIdx.x; k < R * NS; k += 256) {
-            const int slot = k / R, rr = k % R;
-            if (rr < nr) s32[stencil_index(slot, rbase + rr, NS)] = (float)sAcc[rr][slot];
+    if (s32) {
+        // upper half only (stencil_half layout)
+        constexpr int C = NS / 2, NH = stencil_half(NS);
+        for (int k = threadIdx.x; k < R * NH; k += 256) {
+            const int h = k / R, rr = k % R;
+            if (rr < nr) s32[stencil_index(h, rbase + rr, NH)] = (float)sAcc[rr][C + h];
         }
+    }
 }
 
 // Single-patch pattern: row r's columns are the free functions of the

@@ -52,6 +52,11 @@ inline int64_t stencil_size(int n, int ns)
 {
     return (int64_t)((n + kStencilTile - 1) / kStencilTile) * ns * kStencilTile;
 }
+// The FP32 copy of the (bitwise symmetric) structured operator keeps only
+// the slots of offsets lexicographically >= 0: slot s >= ns / 2 is stored as
+// s - ns / 2 in a layout of stencil_half(ns) slots; a lower slot s of row r
+// is slot ns - 1 - s of the row at the opposite offset.
+__host__ __device__ constexpr int stencil_half(int ns) { return ns - ns / 2; }
 
 // Mixed-precision CG: a reliable update replaces the FP32 recurrence's
 // residual r_rec by the true FP64 residual r and keeps the search

@@ -171,8 +171,10 @@ hipError_t launch_gather_f64_from_grid(int nfree, const int *f2g, const double *
 hipError_t launch_sspmv_f64(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz,
                             const double *sv, const double *x, double *y, const double *b,
                             double *part, CGState *st, int k, hipStream_t s);
-hipError_t launch_sspmv_f32(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz,
-                            const float *sv, const float *x, float *y, const float *b,
+// FP32 operator in the half-stored layout (stencil_half); m0 x m1 x m2 = the
+// free box (rows lexicographic), m0 m1 m2 == nfree.
+hipError_t launch_sspmv_f32(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz, int m0, int m1,
+                            int m2, const float *sv, const float *x, float *y, const float *b,
                             double *part, CGState *st, int k, hipStream_t s);
 hipError_t launch_cg2_update_f64(int n, double *x, double *r, const double *p, const double *t,
                                  double *z, const double *diag, double tau, const double *pap,

@@ -880,8 +880,9 @@ struct StencilOp {
     bool s64_ready = false;   // slots filled (by the gather or by conversion)
     int d = 0, p = 0;
     int nx = 1, ny = 1, nz = 1;   // padded grid
+    int m[3] = {1, 1, 1};         // free box (rows lexicographic over it)
     int64_t ngrid = 0;
-    int slots = 0;
+    int slots = 0;                // s64: all slots; s32: stencil_half(slots)
     std::vector<int> f2g_host;
     DevPtr f2g, s64, s32;
     bool s32_ready = false;
@@ -1061,12 +1062,14 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
         S.ny = P->grid_dims[1];
         S.nz = P->grid_dims[2];
         S.ngrid = (int64_t)S.nx * S.ny * S.nz;
+        for (int k = 0; k < h.d; ++k) S.m[k] = P->box_hi[k] - P->box_lo[k] + 1;
         const int W = 2 * S.p + 1;
         S.slots = h.d == 3 ? W * W * W : W * W;
         S.f2g = P->d_f2g;
         if (!csr_only) {
             S.s64 = dalloc(c, sizeof(double) * gsx::stencil_size(A->n, (int)S.slots));
-            if (precision == GSX_MIXED) S.s32 = dalloc(c, sizeof(float) * gsx::stencil_size(A->n, (int)S.slots));
+            if (precision == GSX_MIXED)
+                S.s32 = dalloc(c, sizeof(float) * gsx::stencil_size(A->n, gsx::stencil_half((int)S.slots)));
             if (!S.s64 || (precision == GSX_MIXED && !S.s32))
                 return fail(c, GSX_ERR_OOM, "device allocation (structured operator)");
         }
@@ -1529,7 +1532,7 @@ static gsx_status stencil_prepare(gsx_matrix *A, bool mixed)
         S.s64_ready = true;
     }
     if (mixed && !S.s32_ready) {
-        const int64_t sz = gsx::stencil_size(n, (int)K);
+        const int64_t sz = gsx::stencil_size(n, gsx::stencil_half((int)K));
         S.s32 = dalloc(c, sizeof(float) * sz);
         if (!S.s32) return fail(c, GSX_ERR_OOM, "device allocation (structured operator, fp32)");
         HIP_OK(c, gsx::launch_fill_f32(dp<float>(S.s32), sz, 0.0f, s));
@@ -1845,7 +1848,8 @@ static gsx_status cg_grid(gsx_matrix *A, int precond, double tau, const double *
             for (int j = 0; j < chunk; ++j, ++k) {
                 {
                     KScope ks(c, "spmv_f32");
-                    HIP_OK(c, gsx::launch_sspmv_f32(S.d, S.p, 1, n, f2g, sy, sz, sv32, p32, t32, nullptr, P0, st, k, s));
+                    HIP_OK(c, gsx::launch_sspmv_f32(S.d, S.p, 1, n, f2g, sy, sz, S.m[0], S.m[1], S.m[2], sv32, p32,
+                                                    t32, nullptr, P0, st, k, s));
                 }
                 {
                     KScope kv(c, "cg_vec_f32");

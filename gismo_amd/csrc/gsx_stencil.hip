@@ -71,7 +71,7 @@ __global__ void k_csr_to_stencil(int nfree, const int *__restrict__ rowptr,
         const int dz = (D == 3) ? h / (nbx * nby) - gz : 0;
         const int slot = (D == 3) ? ((dz + P) * W + (dy + P)) * W + (dx + P) : (dy + P) * W + (dx + P);
         if (s64) s64[stencil_index(slot, r, NS)] = val[k];
-        if (s32) s32[stencil_index(slot, r, NS)] = (float)val[k];
+        if (s32 && slot >= NS / 2) s32[stencil_index(slot - NS / 2, r, stencil_half(NS))] = (float)val[k];
     }
 }
 
@@ -156,6 +156,79 @@ __global__ __launch_bounds__(256) void k_sspmv(int nfree, const int *__restrict_
 #pragma unroll
                 for (int dx = -P; dx <= P; ++dx)
                     acc += (T)vb[(slot0 + dx + P) * kStencilTile] * xb[base + dx];
+            }
+        }
+        if (MODE == 2) {
+            const T v = b[g] - acc;
+            y[g] = v;
+            contrib += (double)v * (double)v;
+        } else {
+            y[g] = acc;
+            if (MODE == 1) contrib += (double)x[g] * (double)acc;
+        }
+    }
+    if (MODE != 0) {
+        const double tot = bsum256(contrib, s);
+        if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+    }
+}
+
+// The same SpMV on the half-stored FP32 operator (stencil_half layout): an
+// upper slot is read from the row itself, a lower slot s from slot
+// NS - 1 - s of the neighbour row (free box m0 x m1 x m2, rows
+// lexicographic), which holds the same coefficient since the operator is
+// bitwise symmetric.  Slots are summed in the order of k_sspmv, so y is
+// bitwise the same; neighbours outside the free box contribute 0 (x is 0
+// there).
+template <int D, int P, typename T, int MODE>
+__global__ __launch_bounds__(256) void k_sspmv_half(int nfree, const int *__restrict__ free2grid,
+                                                    int sy, int sz, int m0, int m1, int m2,
+                                                    const float *__restrict__ sval,
+                                                    const T *__restrict__ x, T *__restrict__ y,
+                                                    const T *__restrict__ b,
+                                                    double *__restrict__ partial,
+                                                    CGState *__restrict__ st, int k)
+{
+    __shared__ double s[4];
+    if (st && (st->done || (st->flag_next && st->flag_next <= k))) return;
+    constexpr int W = 2 * P + 1;
+    constexpr int NS = (D == 3) ? W * W * W : W * W;
+    constexpr int C = NS / 2, NH = stencil_half(NS);
+    double contrib = 0.0;
+    const int blk = xcd_block(blockIdx.x, gridDim.x);
+    for (int r = blk * 256 + threadIdx.x; r < nfree; r += gridDim.x * 256) {
+        const int g = free2grid[r];
+        const int ia = r % m0, ib = (r / m0) % m1, ic = (D == 3) ? r / (m0 * m1) : 0;
+        const T *xb = x + g;
+        const float *vb = sval + stencil_index(0, r, NH);
+        T acc = 0;
+        auto coef = [&](int slot, int dx, int dy, int dz) -> float {
+            if (slot >= C) return vb[(slot - C) * kStencilTile];
+            const bool ok = ia + dx >= 0 && ia + dx < m0 && ib + dy >= 0 && ib + dy < m1 &&
+                            (D == 2 || (ic + dz >= 0 && ic + dz < m2));
+            const int rn = r + dx + m0 * (dy + m1 * dz);
+            return ok ? sval[stencil_index(NS - 1 - slot - C, rn, NH)] : 0.0f;
+        };
+        if (D == 3) {
+#pragma unroll 1
+            for (int dz = -P; dz <= P; ++dz) {
+#pragma unroll
+                for (int dy = -P; dy <= P; ++dy) {
+                    const int base = dz * sz + dy * sy;
+                    const int slot0 = ((dz + P) * W + (dy + P)) * W;
+#pragma unroll
+                    for (int dx = -P; dx <= P; ++dx)
+                        acc += (T)coef(slot0 + dx + P, dx, dy, dz) * xb[base + dx];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int dy = -P; dy <= P; ++dy) {
+                const int base = dy * sy;
+                const int slot0 = (dy + P) * W;
+#pragma unroll
+                for (int dx = -P; dx <= P; ++dx)
+                    acc += (T)coef(slot0 + dx + P, dx, dy, 0) * xb[base + dx];
             }
         }
         if (MODE == 2) {
@@ -395,11 +468,19 @@ hipError_t launch_sspmv_f64(int d, int p, int mode, int nfree, const int *f2g, i
     return sspmv_dispatch<double, double>(d, p, mode, nfree, f2g, sy, sz, sv, x, y, b, part, st, k, s);
 }
 
-hipError_t launch_sspmv_f32(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz,
-                            const float *sv, const float *x, float *y, const float *b,
+hipError_t launch_sspmv_f32(int d, int p, int mode, int nfree, const int *f2g, int sy, int sz, int m0, int m1,
+                            int m2, const float *sv, const float *x, float *y, const float *b,
                             double *part, CGState *st, int k, hipStream_t s)
 {
-    return sspmv_dispatch<float, float>(d, p, mode, nfree, f2g, sy, sz, sv, x, y, b, part, st, k, s);
+    if ((int64_t)m0 * m1 * m2 != nfree) return hipErrorInvalidValue;
+    const dim3 grid(sspmv_blocks(nfree)), blk(256);
+#define GSX_S(D, P, M) if (d == D && p == P && mode == M) { hipLaunchKernelGGL((k_sspmv_half<D, P, float, M>), grid, blk, 0, s, nfree, f2g, sy, sz, m0, m1, m2, sv, x, y, b, part, st, k); return hipGetLastError(); }
+#define GSX_S3(D, P) GSX_S(D, P, 0) GSX_S(D, P, 1) GSX_S(D, P, 2)
+    GSX_S3(2, 1) GSX_S3(2, 2) GSX_S3(2, 3) GSX_S3(2, 4)
+    GSX_S3(3, 1) GSX_S3(3, 2) GSX_S3(3, 3) GSX_S3(3, 4)
+#undef GSX_S3
+#undef GSX_S
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_cg2_update_f64(int n, double *x, double *r, const double *p, const double *t,
