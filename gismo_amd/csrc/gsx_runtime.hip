@@ -3111,7 +3111,13 @@ extern "C" gsx_status gsx_mg_set_smoother(gsx_mg *mg, int32_t kind, double param
         for (int l = 1; l < nl; ++l) {
             MGLevel &V = mg->L[l];
             ScmsBox B;
-            const bool ok = l == nl - 1 ? (mg->fine_box_ok ? (B = mg->fine_box, true) : false) : scms_box_of(V.prob, B);
+            bool ok;
+            if (l == nl - 1) {
+                ok = mg->fine_box_ok;
+                if (ok) B = mg->fine_box;
+            } else {
+                ok = scms_box_of(V.prob, B);
+            }
             bool alldir = ok;
             for (int k = 0; ok && k < B.d; ++k) alldir = alldir && B.lo[k] == 1 && B.hi[k] == B.nb[k] - 2;
             if (!alldir)
