@@ -15,6 +15,8 @@
 //                 reference example, gsPoissonAssembler its own default 101)
 //   --fd          CG preconditioned by gsPatchPreconditionersCreator::
 //                 fastDiagonalizationOp (single patch) instead of Jacobi
+//   --scms        CG preconditioned by gsPatchPreconditionersCreator::
+//                 subspaceCorrectedMassSmootherOp (single patch, sigma 0.12)
 //   --bvp         the boundary value problem of poisson2d_bvp: the domain
 //                 shifted by 1/4 (u = prod sin(pi x_i) does not vanish on
 //                 it), u on the west/south(/front) sides by L2 projection,
@@ -40,7 +42,7 @@ using namespace gismo;
 int main(int argc, char *argv[])
 {
     index_t numRefine = 5, numElevate = 1, dim = 2, gx = 0, gy = 0;
-    bool last = false, full = false, mixed = false, bvp = false, fastdiag = false;
+    bool last = false, full = false, mixed = false, bvp = false, fastdiag = false, scms = false;
     int dirValues = -1;   // Dirichlet values method; -1: each assembler's own default
     std::string fn;
     for (int i = 1; i < argc; ++i) {
@@ -55,6 +57,7 @@ int main(int argc, char *argv[])
         else if (a == "--mixed") mixed = true;
         else if (a == "--bvp") bvp = true;
         else if (a == "--fd") fastdiag = true;
+        else if (a == "--scms") scms = true;
         else if (a == "--dv" && i + 1 < argc) dirValues = std::atoi(argv[++i]);
         else { std::fprintf(stderr, "unknown argument %s\n", argv[i]); return 1; }
     }
@@ -148,11 +151,18 @@ int main(int argc, char *argv[])
                 rhs = &A.rhs();
             }
             auto t1 = std::chrono::steady_clock::now();
-            // preconditioner: diagonal, or (--fd, one patch) the fast
-            // diagonalization of gsPatchPreconditionersCreator
-            gsConjugateGradient<> cg(*K, fastdiag ? gsPatchPreconditionersCreator<>::fastDiagonalizationOp(
-                                                  dbasis.basis(0), full ? bcAll : bc)
-                                            : gsPreconditionerOp<>::Ptr(makeJacobiOp(*K, 1.0)));
+            // preconditioner: diagonal, or (--fd / --scms, one patch) the fast
+            // diagonalization or the subspace corrected mass smoother of
+            // gsPatchPreconditionersCreator
+            gsPreconditionerOp<>::Ptr prec;
+            if (fastdiag)
+                prec = gsPatchPreconditionersCreator<>::fastDiagonalizationOp(dbasis.basis(0), full ? bcAll : bc);
+            else if (scms)
+                prec = gsPatchPreconditionersCreator<>::subspaceCorrectedMassSmootherOp(dbasis.basis(0),
+                                                                                       full ? bcAll : bc);
+            else
+                prec = gsPreconditionerOp<>::Ptr(makeJacobiOp(*K, 1.0));
+            gsConjugateGradient<> cg(*K, prec);
             gsOptionList o = gsConjugateGradient<>::defaultOptions();
             o.setReal("Tolerance", 1e-10);
             o.setInt("MaxIterations", 100000);

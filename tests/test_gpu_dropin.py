@@ -314,3 +314,24 @@ def test_multigrid_example_scms_smoother(ctx):
     assert it < int(sj[1])
     for o in (x, mg, A, b, gp):
         o.close()
+
+
+def test_poisson2_example_scms_matches_python_mirror(ctx):
+    """poisson2_example --scms: gsConjugateGradient preconditioned by
+    gsPatchPreconditionersCreator::subspaceCorrectedMassSmootherOp(basis,
+    bc) through the C++ drop-in classes takes the Python mirror's CG
+    iterations on the same square (16 x 16 elements, p = 2)."""
+    rows = _levels(_run("poisson2_example", "--scms", "-r", "3", "-e", "1", "--last"))
+    assert len(rows) == 1
+    dofs, nnz, its, rel, l2 = rows[0]
+    prob = problems.poisson_square(n=16, p=2)
+    gp = gsx.Problem(ctx, prob)
+    A, b = gp.assemble(gsx.PATTERN_EXPR, gsx.FP64)
+    sm = gsx.SubspaceMassSmoother(gp)
+    x = ctx.array(gp.nfree).zero()
+    it, rel_py, ok = gsx.fd_cg_solve(A, sm, b, x, tol=1e-10, maxit=100000)
+    assert ok and (dofs, nnz, its) == (gp.nfree, A.nnz, it), (dofs, its, gp.nfree, it)
+    assert rel < 1e-10
+    assert abs(l2 - gp.l2_error(x)) <= 1e-9 * l2
+    for o in (x, sm, A, b, gp):
+        o.close()
