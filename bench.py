@@ -97,10 +97,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["cube", "mg", "batch", "serve"], default="cube",
+    ap.add_argument("--workload", choices=["cube", "mg", "mg-cube", "batch", "serve"], default="cube",
                     help="cube: single-patch 3D assemble + Jacobi-CG (default); mg: 16-patch 2D "
-                         "assemble + multigrid-preconditioned CG; batch: 512 independent problems; "
+                         "assemble + multigrid-preconditioned CG; mg-cube: the cube's problem with "
+                         "multigrid-preconditioned CG (--smoother); batch: 512 independent problems; "
                          "serve: Poisson-arrival request stream through the continuous-batching server")
+    ap.add_argument("--smoother", choices=["scms", "jacobi", "chebyshev", "gauss_seidel"], default="scms",
+                    help="mg-cube: multigrid smoother (scms: subspace corrected mass smoother)")
     ap.add_argument("--load", type=float, default=0.7, help="serve: arrival rate as a fraction of capacity")
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--p", type=int, default=3)
@@ -122,6 +125,8 @@ def main():
         sys.exit(2)
     if args.workload == "mg":
         return bench_mg(args)
+    if args.workload == "mg-cube":
+        return bench_mg_cube(args)
     if args.workload == "batch":
         return bench_batch(args)
     if args.workload == "serve":
@@ -406,6 +411,79 @@ def bench_mg(args):
     out = _common(args, mixed["dofs_per_s"], mixed["ms"], "f32+f64",
                   f"2D multipatch: 16 degree-{args.p} patches on [0,4]^2, {n}x{n} elements per patch, "
                   f"assemble + MG-preconditioned CG to relres {tol:g}", extra)
+    x.close()
+    gp.close()
+    ctx.close()
+    print(json.dumps(out))
+
+
+def bench_mg_cube(args):
+    """The default workload's cube (n^3 elements, degree p, noisy geometry,
+    lognormal kappa, Dirichlet on every side); one step = assemble +
+    multigrid setup (3 knot-coarsened levels, Galerkin operators, dense
+    coarse inverse, the smoother's own setup) + CG preconditioned by V(1,1)
+    with the chosen smoother to relres tol.  With scms the smoother is the
+    subspace corrected mass smoother of every level, damped by its power
+    iteration (the reference multigrid example's -s scms)."""
+    from gismo_amd import gsx, problems
+    n = 64 if args.n is None else args.n
+    tol = 1e-8 if args.tol is None else args.tol
+    levels = 3
+    damping = 0.6 if args.smoother in ("jacobi", "chebyshev") else 1.0
+    prob = problems.poisson_noisy_cube(n=n, p=args.p)
+    ctx = gsx.Context(0)
+    gp = gsx.Problem(ctx, prob)
+    ndofs = gp.nfree
+    x = ctx.array(gp.nfree)
+    results = {}
+    for name, prec in (("fp64", gsx.FP64), ("mixed", gsx.MIXED)):
+        def step():
+            A, b = gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_KEEP_ELEMENTS, prec)
+            mg = gsx.Multigrid(gp, A, levels, 1, 1, damping, precision=prec, smoother=args.smoother)
+            it, rel, ok = gsx.mg_cg_solve(A, mg, b, x, tol=tol, maxit=500)
+            ctx.synchronize()
+            mg.close(); A.close(); b.close()
+            return it, rel, ok
+        for _ in range(args.warmup):
+            x.zero()
+            step()
+        times = []
+        for _ in range(args.steps):
+            x.zero()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            it, rel, ok = step()
+            times.append(time.perf_counter() - t0)
+        log(f"mg-cube {args.smoother} {name}: {1e3 * np.mean(times):.1f} ms per step, {it} iterations")
+        results[name] = dict(ms=1e3 * float(np.mean(times)), ms_min=1e3 * float(np.min(times)), iters=it,
+                             relres=rel, converged=bool(ok), l2=gp.l2_error(x),
+                             dofs_per_s=ndofs / float(np.mean(times)))
+    ctx.enable_timing(True)
+    ctx.timing_filter(None)
+    ctx.reset_timing()
+    x.zero()
+    A, b = gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_KEEP_ELEMENTS, gsx.MIXED)
+    mg = gsx.Multigrid(gp, A, levels, 1, 1, damping, precision=gsx.MIXED, smoother=args.smoother)
+    gsx.mg_cg_solve(A, mg, b, x, tol=tol, maxit=500)
+    ctx.synchronize()
+    phases = {}
+    for k in ("elem3d", "gather", "mg_setup", "mg_levels", "mg_elem", "mg_transfer_build", "mg_galerkin",
+              "mg_coarse_inverse", "mg_smooth", "mg_transfer", "mg_coarse", "cg_init", "spmv_f64", "cg_vec"):
+        nl, ms = ctx.kernel_stats(k)
+        if nl:
+            phases[k] = round(ms, 3)
+    ctx.enable_timing(False)
+    mg.close(); A.close(); b.close()
+    mixed = results["mixed"]
+    extra = {
+        "config": {"free_dofs": gp.nfree, "levels": levels, "smoother": args.smoother,
+                   "cycle": f"V(1,1), {args.smoother}, exact FP64 coarse solve"},
+        "fp64": results["fp64"], "mixed": mixed,
+        "phase_ms_one_mixed_step": phases,
+    }
+    out = _common(args, mixed["dofs_per_s"], mixed["ms"], "f32+f64",
+                  f"3D Poisson unit cube, p={args.p}, {n}^3 elements, noisy geometry, lognormal kappa; "
+                  f"assemble + MG({args.smoother})-preconditioned CG to relres {tol:g}", extra)
     x.close()
     gp.close()
     ctx.close()
