@@ -15,6 +15,10 @@ measured beside it.  The relative error of each solution against a tight
 FP64 reference solve (relres 1e-13) is reported; the 1e-8 bar of the metric
 is checked.
 
+Other workloads (--workload): mg (16 glued 2D patches, MG-preconditioned
+CG), mg-cube (this cube with MG-preconditioned CG and --smoother), batch
+(512 independent problems), serve (a Poisson-arrival request stream).
+
 Prints ONE JSON line.
 """
 import argparse
