@@ -148,6 +148,26 @@ def lognormal_kappa(nelem, sigma, seed, stream=10):
     return np.exp(sigma * normal01(seed, stream, np.arange(nelem)))
 
 
+def zero_kappa_box(P, lo, hi):
+    """Set kappa = 0 on the elements whose multi-index lies in [lo, hi]
+    (inclusive, per direction): a void in the coefficient.  Their element
+    matrices are exact zeros, so gsExprAssembler's exact-zero skip
+    (src/gsAssembler/gsExprAssembler.h:599) leaves out every entry that only
+    they contribute to."""
+    ne = P.nelem_dir()
+    if P.kappa is None:
+        P.kappa = np.ones(P.nelem)
+    idx = np.arange(P.nelem)
+    inside = np.ones(P.nelem, dtype=bool)
+    r = idx.copy()
+    for k in range(P.d):
+        c = r % ne[k]
+        r //= ne[k]
+        inside &= (c >= lo[k]) & (c <= hi[k])
+    P.kappa = np.where(inside, 0.0, P.kappa)
+    return P
+
+
 def noisy_box_fast(d, p, n, noise, sigma, seed):
     """Degree-p patch on the unit box with n elements per direction, interior
     control points moved by U(+-noise/n), lognormal(0, sigma) kappa and

@@ -66,6 +66,7 @@ class Result:
     status: List[int]
     solutions: Optional[List[np.ndarray]] = None
     lane: str = "batch"           # "batch" (batched Jacobi-CG) or "mg" (multigrid-preconditioned CG)
+    error: Optional[str] = None   # set when the request failed (status FAILED for every rhs)
 
     @property
     def latency(self):
@@ -117,6 +118,12 @@ def mg_element_bytes(prob):
 def rhs_vector(nfree, seed):
     rng = np.random.default_rng(seed)
     return rng.uniform(-1.0, 1.0, nfree)
+
+
+# per right-hand side status of a Result: 1 converged, 2 iteration limit
+# reached (the batched CG's own codes), FAILED: the request raised (for
+# example a folded patch, GSX_ERR_JACOBIAN) and was dropped
+FAILED = 3
 
 
 class _Entry:
@@ -201,16 +208,43 @@ class Server:
                         e.levels = levels
                         self._mgq.put(e)
                         continue
-                    e.gp = gsx.Problem(self.ctx_asm, req.problem)
-                    e.A, b0 = e.gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_CSR_ONLY, self.precision)
-                    e.bs.append(b0)
-                    for s in req.rhs_seeds[1:]:
-                        e.bs.append(self.ctx_asm.to_device(rhs_vector(e.gp.nfree, s)))
-                    self.ctx_asm.synchronize()
+                    try:
+                        e.gp = gsx.Problem(self.ctx_asm, req.problem)
+                        e.A, b0 = e.gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_CSR_ONLY, self.precision)
+                        e.bs.append(b0)
+                        for s in req.rhs_seeds[1:]:
+                            e.bs.append(self.ctx_asm.to_device(rhs_vector(e.gp.nfree, s)))
+                        self.ctx_asm.synchronize()
+                    except gsx.GsxError as ex:
+                        self._fail(e, ex, "batch")
+                        continue
                     e.t_asm = self.now()
                     self._ready.put(e)
-        except Exception as ex:   # reported by the solver thread
+        except Exception as ex:   # not a request's own error: reported by poll()
             self._error = ex
+
+    def _release(self, e, objs):
+        for o in objs:
+            if o is not None:
+                try:
+                    o.close()
+                except Exception:
+                    pass
+
+    def _fail(self, e, ex, lane, objs=()):
+        """A request that raised: free what it holds, return its budget and
+        record a failed result; the server keeps serving the others."""
+        self._release(e, list(objs) + list(e.xs) + list(e.bs) + [e.A, e.gp])
+        e.xs, e.bs, e.A, e.gp = [], [], None, None
+        req = e.req
+        res = Result(rid=req.rid, dofs=0, nfree=0, nrhs=len(req.rhs_seeds), arrival=req.arrival,
+                     admitted=e.t_admit, assembled=self.now(), done=self.now(), iterations=[0] * len(req.rhs_seeds),
+                     relres=[float("nan")] * len(req.rhs_seeds), status=[FAILED] * len(req.rhs_seeds),
+                     lane=lane, error=str(ex))
+        with self._lock:
+            self.results.append(res)
+            self._used -= e.bytes
+            self._outstanding -= 1
 
     def _mg_levels(self, req):
         P = req.problem.patches[0]
@@ -230,23 +264,30 @@ class Server:
                 except queue.Empty:
                     continue
                 req, ctx = e.req, self.ctx_mg
-                gp = gsx.Problem(ctx, req.problem)
-                A, b0 = gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_KEEP_ELEMENTS, self.precision)
-                mg = gsx.Multigrid(gp, A, e.levels, 2, 2, 0.6, precision=self.precision)
-                ctx.synchronize()
-                t_asm = self.now()
-                its, rels, sts, sols = [], [], [], []
-                for k, seed in enumerate(req.rhs_seeds):
-                    b = b0 if k == 0 else ctx.to_device(rhs_vector(gp.nfree, seed))
-                    x = ctx.array(gp.nfree).zero()
-                    it, rel, ok = gsx.mg_cg_solve(A, mg, b, x, tol=req.tol, maxit=req.maxit)
-                    its.append(it)
-                    rels.append(rel)
-                    sts.append(1 if ok else 2)
-                    sols.append(x.download())
-                    x.close()
-                    if k:
-                        b.close()
+                gp = A = b0 = mg = b = x = None
+                try:
+                    gp = gsx.Problem(ctx, req.problem)
+                    A, b0 = gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_KEEP_ELEMENTS, self.precision)
+                    mg = gsx.Multigrid(gp, A, e.levels, 2, 2, 0.6, precision=self.precision)
+                    ctx.synchronize()
+                    t_asm = self.now()
+                    its, rels, sts, sols = [], [], [], []
+                    for k, seed in enumerate(req.rhs_seeds):
+                        b = b0 if k == 0 else ctx.to_device(rhs_vector(gp.nfree, seed))
+                        x = ctx.array(gp.nfree).zero()
+                        it, rel, ok = gsx.mg_cg_solve(A, mg, b, x, tol=req.tol, maxit=req.maxit)
+                        its.append(it)
+                        rels.append(rel)
+                        sts.append(1 if ok else 2)
+                        sols.append(x.download())
+                        x.close()
+                        x = None
+                        if k:
+                            b.close()
+                        b = None
+                except gsx.GsxError as ex:
+                    self._fail(e, ex, "mg", objs=[x, b if b is not b0 else None, mg, b0, A, gp])
+                    continue
                 t_done = self.now()
                 res = Result(rid=req.rid, dofs=gp.nglued, nfree=gp.nfree, nrhs=len(req.rhs_seeds),
                              arrival=req.arrival, admitted=e.t_admit, assembled=t_asm, done=t_done,
@@ -274,11 +315,19 @@ class Server:
                 e = self._ready.get_nowait()
             except queue.Empty:
                 break
-            for b in e.bs:
-                x = self.ctx.array(e.gp.nfree).zero()
-                e.xs.append(x)
-                e.ids.append(self.batch.add(e.A, b, x, tol=e.req.tol, maxit=e.req.maxit))
-                self._members += 1
+            try:
+                for b in e.bs:
+                    x = self.ctx.array(e.gp.nfree).zero()
+                    e.xs.append(x)
+                    e.ids.append(self.batch.add(e.A, b, x, tol=e.req.tol, maxit=e.req.maxit))
+                    self._members += 1
+            except gsx.GsxError as ex:
+                for i in e.ids:
+                    self.batch.remove(i)
+                    self._members -= 1
+                e.ids = []
+                self._fail(e, ex, "batch")
+                continue
             self._active[e.req.rid] = e
         running = self.batch.step(self.iters) if self._active else 0
         for rid in list(self._active):

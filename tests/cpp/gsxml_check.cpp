@@ -1,14 +1,18 @@
 // Host-side checks of gsXml.h (no GPU): the reference's own XML inputs are
 // read, their interface records agree with the topology found by corner
 // matching, the refined Fichera corner has the free-DOF count the
-// reference's gsBasisComponents_test asserts (117, no Dirichlet sides), and
+// reference's gsBasisComponents_test asserts (117, no Dirichlet sides) and
+// its DOF -> topology component golden (printed for the test to compare), and
 // matrices, sparse matrices and multipatches survive a write/read round trip.
 // Prints "name value..." lines; exits nonzero on a failed check.
 #include "gsXml.h"
 
+#include <algorithm>
 #include <cstdio>
+#include <map>
 #include <set>
 #include <tuple>
+#include <vector>
 
 using namespace gismo;
 
@@ -54,7 +58,9 @@ static std::set<Rec> canon(const gsMultiPatch<> &mp)
     return out;
 }
 
-static int64_t free_count(const gsMultiPatch<> &mp, int refine, bool dirichlet)
+// Free-DOF map of the multibasis of mp refined `refine` times (glued
+// interfaces, all sides Dirichlet or none): dofs per patch-local function.
+static int64_t mapper_dofs(const gsMultiPatch<> &mp, int refine, bool dirichlet, std::vector<int32_t> &dofs)
 {
     gsMultiBasis<> mb(mp, true);
     for (int r = 0; r < refine; ++r) mb.uniformRefine();
@@ -71,12 +77,78 @@ static int64_t free_count(const gsMultiPatch<> &mp, int refine, bool dirichlet)
         const int32_t r[8] = {f.p1, f.s1, f.p2, f.s2, f.m0, f.f0, f.m1, f.f1};
         ifs.insert(ifs.end(), r, r + 8);
     }
-    std::vector<int32_t> dofs(total);
+    dofs.assign(total, -1);
     int64_t nfree = 0, nglued = 0;
     CHECK(gsx_mapper_build(d, np, nbd.data(), (int32_t)ds.size() / 2, ds.empty() ? nullptr : ds.data(),
                            (int32_t)ifs.size() / 8, ifs.empty() ? nullptr : ifs.data(), dofs.data(), &nfree,
                            &nglued) == GSX_OK);
     return nfree;
+}
+
+// Topology components of a conforming degree-1 multipatch of boxes in the
+// order of gsBoxTopology::allComponents (src/gsCore/gsBoxTopology.cpp:
+// 294-337): every patch component (index j = sum_d 3^d loc_d with loc 0
+// interior, 1 begin, 2 end; src/gsCore/gsBoundary.h:445-477) is keyed by
+// the sorted canonical ids patch * 2^dim + corner (corner 1-based,
+// src/gsCore/gsBoundary.cpp:80-108) of the corners it contains, where a
+// corner's canonical representative is the coinciding corner of smallest
+// (patch, corner) (getCanonicCorner); components of equal key are one;
+// the list runs from dimension dim down to 0, each in ascending key order.
+// Corners coincide here when their control points do.  Returns the
+// component of every (patch, component index) pair and the counts per
+// dimension.
+static std::vector<std::vector<int>> box_components(const gsMultiPatch<> &mp, std::vector<int> &per_dim)
+{
+    const int np = mp.nPatches(), d = mp.dim(), nc = 1 << d;
+    int ncomp = 1;
+    for (int k = 0; k < d; ++k) ncomp *= 3;
+    auto corner_pt = [&](int k, int c) {
+        std::vector<double> x(d);
+        for (int r = 0; r < d; ++r) x[r] = mp.patch(k).coefs()(c, r);
+        return x;
+    };
+    // canonical (patch, corner) of every patch corner
+    std::vector<std::pair<int, int>> canon_c(np * nc);
+    for (int k = 0; k < np; ++k)
+        for (int c = 0; c < nc; ++c) {
+            std::pair<int, int> best(k, c + 1);
+            const auto x = corner_pt(k, c);
+            for (int k2 = 0; k2 < np; ++k2)
+                for (int c2 = 0; c2 < nc; ++c2)
+                    if (corner_pt(k2, c2) == x && std::make_pair(k2, c2 + 1) < best) best = {k2, c2 + 1};
+            canon_c[k * nc + c] = best;
+        }
+    std::vector<std::map<std::vector<int>, int>> keys(d + 1);
+    std::vector<std::vector<std::vector<int>>> key_of(np, std::vector<std::vector<int>>(ncomp));
+    std::vector<std::vector<int>> dim_of(np, std::vector<int>(ncomp));
+    for (int k = 0; k < np; ++k)
+        for (int j = 0; j < ncomp; ++j) {
+            int cdim = 0;
+            for (int t = 0, jj = j; t < d; ++t, jj /= 3) cdim += (jj % 3 == 0);
+            std::vector<int> key;
+            for (int i = 0; i < (1 << cdim); ++i) {
+                int idx = 0, ii = i, jj = j, cstride = 1;
+                for (int t = 0; t < d; ++t, cstride *= 2) {
+                    if (jj % 3 == 0) { idx += cstride * (ii % 2); ii /= 2; }
+                    else idx += cstride * (jj % 3 - 1);
+                    jj /= 3;
+                }
+                const auto cc = canon_c[k * nc + idx];
+                key.push_back(cc.first * nc + cc.second);
+            }
+            std::sort(key.begin(), key.end());
+            keys[cdim][key] = 0;
+            key_of[k][j] = key;
+            dim_of[k][j] = cdim;
+        }
+    int next = 0;
+    per_dim.assign(d + 1, 0);
+    for (int cd = d; cd >= 0; --cd)
+        for (auto &kv : keys[cd]) { kv.second = next++; ++per_dim[cd]; }
+    std::vector<std::vector<int>> out(np, std::vector<int>(ncomp));
+    for (int k = 0; k < np; ++k)
+        for (int j = 0; j < ncomp; ++j) out[k][j] = keys[dim_of[k][j]][key_of[k][j]];
+    return out;
 }
 
 int main(int argc, char **argv)
@@ -97,9 +169,41 @@ int main(int argc, char **argv)
         gsFileData<> fd(dir + "/fichera.xml");
         gsMultiPatch<> mp;
         fd.getFirst(mp);
-        const int64_t n = free_count(mp, 1, false);
+        std::vector<int32_t> dofs;
+        const int64_t n = mapper_dofs(mp, 1, false, dofs);
         std::printf("fichera_refined_free %lld\n", (long long)n);
         CHECK(n == 117);
+        // component of every free DOF: after one refinement of the degree-1
+        // patches each component holds exactly one function, the one at
+        // local position loc_d = 0 -> begin, 1 -> interior, 2 -> end
+        std::vector<int> per_dim;
+        const auto comp = box_components(mp, per_dim);
+        std::vector<int> of_dof(n, -1);
+        bool consistent = true;
+        std::printf("fichera_local_component");
+        for (int k = 0; k < (int)mp.nPatches(); ++k)
+            for (int l = 0; l < 27; ++l) {
+                int j = 0;
+                for (int t = 0, ll = l, s3 = 1; t < 3; ++t, ll /= 3, s3 *= 3) {
+                    const int pos = ll % 3;
+                    j += s3 * (pos == 1 ? 0 : (pos == 0 ? 1 : 2));
+                }
+                const int g = dofs[27 * k + l], cid = comp[k][j];
+                std::printf(" %d", cid);
+                if (g < 0 || g >= n) { consistent = false; continue; }
+                if (of_dof[g] >= 0 && of_dof[g] != cid) consistent = false;
+                of_dof[g] = cid;
+            }
+        std::printf("\n");
+        CHECK(consistent);
+        std::printf("fichera_components_per_dim");
+        for (int cd = 0; cd <= 3; ++cd) std::printf(" %d", per_dim[cd]);
+        std::printf("\nfichera_component_of_dof");
+        for (int v : of_dof) std::printf(" %d", v);
+        std::printf("\n");
+        for (auto &f : mp.interfaces())
+            std::printf("fichera_iface %d %d %d %d %d %d %d %d\n", f.p1, f.s1, f.p2, f.s2, f.m0, f.f0, f.m1,
+                        f.f1);
     }
     {
         gsFileData<> fd(dir + "/square.xml");

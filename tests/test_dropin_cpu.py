@@ -5,6 +5,7 @@ the geometry mapped into the discretisation basis against the Python
 problem generator (bitwise), and the DOF mapper checked physically (shared
 global index <=> coinciding control points), for glued, flipped, rotated
 and permuted patches in 2D and 3D."""
+import json
 import os
 import shutil
 import subprocess
@@ -12,6 +13,7 @@ import subprocess
 import numpy as np
 import pytest
 
+import oracle as O
 from gismo_amd import gsx, problems
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -100,7 +102,8 @@ def test_xml_reader_on_reference_inputs():
     """gsXml.h reads the reference's own XML inputs (tests/golden/xml, copied
     from its filedata/): their interface records agree with corner-matched
     topology, the refined Fichera corner has the 117 free DOFs asserted by
-    the reference's gsBasisComponents_test, and matrices, sparse matrices and
+    the reference's gsBasisComponents_test and their golden component
+    assignment (host mapper and oracle mapper), and matrices, sparse matrices and
     multipatches survive a write/read round trip."""
     gsx.lib()
     cxx = shutil.which("g++")
@@ -120,3 +123,25 @@ def test_xml_reader_on_reference_inputs():
     assert "fichera_refined_free 117" in out.stdout
     assert out.stdout.count("topology_match 1") == 3
     assert "bvp dirichlet 3 neumann 3" in out.stdout
+    # the reference's golden DOF -> component assignment of the refined
+    # Fichera corner (unittests/gsBasisComponents_test.cpp:41-79)
+    gold = json.load(open(os.path.join(ROOT, "tests", "golden", "fichera_components.json")))
+    lines = {ln.split()[0]: ln.split()[1:] for ln in out.stdout.splitlines() if ln.startswith("fichera_")}
+    assert [int(v) for v in lines["fichera_components_per_dim"]][::-1] == gold["components_per_dim"][::-1]
+    assert [int(v) for v in lines["fichera_component_of_dof"]] == gold["component_of_dof"]
+    # the oracle's mapper on the same interface records gives the same
+    # numbering: each free DOF lands on the golden's component
+    ifaces = [tuple(int(v) for v in ln.split()[1:]) for ln in out.stdout.splitlines()
+              if ln.startswith("fichera_iface")]
+    local = np.array([int(v) for v in lines["fichera_local_component"]]).reshape(7, 27)
+    kn = problems.knots_uniform(2, 1)
+    pats = [problems.Patch(d=3, p=[1, 1, 1], knots=[kn] * 3, ctrl=np.zeros((27, 3))) for _ in range(7)]
+    dofs, nfree = O.mapper(pats, [], ifaces)
+    assert nfree == 117
+    comp = np.full(nfree, -1)
+    for k in range(7):
+        for l in range(27):
+            g = dofs[27 * k + l]
+            assert comp[g] in (-1, local[k, l])
+            comp[g] = local[k, l]
+    assert comp.tolist() == gold["component_of_dof"]

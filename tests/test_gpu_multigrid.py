@@ -38,6 +38,9 @@ CASES = {
     "grid2x2-p3": (lambda n: problems.square_grid(2, 2, n, 3), (2, 4, 8)),
     "grid3x2-p2": (lambda n: problems.square_grid(3, 2, n, 2), (2, 4, 8)),
     "cube-p2": (lambda n: problems.poisson_noisy_cube(n=n, p=2, seed=4), (2, 4, 8)),
+    # the multigrid workload's layout (16 degree-3 patches tiling [0,4]^2)
+    # on three levels: 4, 8, 16 elements per patch and direction
+    "grid4x4-p3": (lambda n: problems.square_grid(4, 4, n, 3), (4, 8, 16)),
 }
 
 
@@ -126,7 +129,7 @@ def test_galerkin_routes_agree(ctx, name):
         o.close()
 
 
-@pytest.mark.parametrize("name", ["grid2x2-p3", "cube-p2"])
+@pytest.mark.parametrize("name", ["grid2x2-p3", "cube-p2", "grid4x4-p3"])
 def test_mg_pcg_matches_oracle(ctx, name):
     make, ns = CASES[name]
     levels, maps = _hierarchy(make, ns)

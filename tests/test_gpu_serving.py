@@ -96,3 +96,41 @@ def test_serving_multigrid_lane():
             err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
             print(P.name, k, mix[r.rid].iterations[k], err, flush=True)
             assert mix[r.rid].status[k] == 1 and err <= 1e-8
+
+
+def _fold(req):
+    """Move one interior control point far across its neighbours: the map
+    folds and assembly raises GSX_ERR_JACOBIAN."""
+    P = req.problem.patches[0]
+    nb = P.nbasis_dir()
+    i = 2 + nb[0] * (2 + (nb[1] * 2 if P.d == 3 else 0))
+    P.ctrl = P.ctrl.copy()
+    P.ctrl[i, 0] += 3.0
+    return req
+
+
+@pytest.mark.parametrize("mg_min_dofs", [20000, 400])
+def test_serving_survives_a_failing_request(mg_min_dofs):
+    """A request that raises (a folded patch) is reported as failed with the
+    library's message and its budget returned; the requests around it, in
+    the batch lane and in the multigrid lane, still complete with the same
+    solutions as without it."""
+    rs = _requests(count=6, seed=11)
+    ref = _serve(rs, mg_min_dofs=mg_min_dofs)
+    bad = [_fold(r) for r in _requests(count=6, seed=11)]
+    mixed = [rs[0], bad[1], rs[2], bad[3], rs[4], rs[5]]
+    srv = serving.Server(precision=gsx.MIXED, iters_per_step=8, mg_min_dofs=mg_min_dofs)
+    for r in mixed:
+        r.arrival = srv.now()
+        srv.submit(r)
+    srv.drain(timeout=120)
+    got = {r.rid: r for r in srv.results}
+    assert srv._used == 0
+    srv.close()
+    assert sorted(got) == list(range(6))
+    for rid in (1, 3):
+        assert got[rid].status == [serving.FAILED] * got[rid].nrhs and "Jacobian" in got[rid].error
+    for rid in (0, 2, 4, 5):
+        assert got[rid].error is None and all(s == 1 for s in got[rid].status)
+        for a, b in zip(got[rid].solutions, ref[rid].solutions):
+            assert np.array_equal(a, b), rid

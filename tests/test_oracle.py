@@ -18,6 +18,7 @@ from scipy.interpolate import BSpline
 
 import oracle as O
 from gismo_amd import problems
+from cases import zero_block_problems
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -330,6 +331,64 @@ def test_expr_pattern_equals_full_pattern_on_configs():
         assert np.array_equal(A0.indptr, A1.indptr) and np.array_equal(A0.indices, A1.indices)
 
 
+def _support_pattern(prob, dofs, nfree, skip_zero_kappa):
+    """Closed-form sparsity pattern: every free pair (i, j) of functions
+    active on one element (open uniform knots: element e of a direction has
+    the functions e..e+p), over all elements, or only over those with a
+    non-zero coefficient.  Independent of the oracle's assembly loop."""
+    rows, cols = [], []
+    off = 0
+    for P in prob.patches:
+        ne, nb = P.nelem_dir(), P.nbasis_dir()
+        for e in range(P.nelem):
+            if skip_zero_kappa and P.kappa is not None and P.kappa[e] == 0.0:
+                continue
+            c = []
+            r = e
+            for k in range(P.d):
+                c.append(r % ne[k])
+                r //= ne[k]
+            loc = np.zeros(1, dtype=np.int64)
+            stride = 1
+            for k in range(P.d):
+                loc = (loc[None, :] + stride * (c[k] + np.arange(P.p[k] + 1))[:, None]).reshape(-1)
+                stride *= nb[k]
+            g = dofs[off + loc]
+            g = g[g < nfree]
+            rows.append(np.repeat(g, g.size))
+            cols.append(np.tile(g, g.size))
+        off += P.nbasis
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    M = sp.csr_matrix((np.ones(r.size), (r, c)), shape=(nfree, nfree))
+    M.sort_indices()
+    return M.indptr, M.indices
+
+
+@pytest.mark.parametrize("prob", zero_block_problems(), ids=lambda p: p.name)
+def test_exact_zero_rule_drops_void_entries(prob):
+    """Mode 0 (gsExprAssembler) keeps exactly the pairs that meet on an
+    element with kappa != 0; mode 1 (gsPoissonAssembler) keeps every pair
+    that meets on any element.  The entries only mode 1 has are exact zeros;
+    the common entries and the load vectors are bitwise equal (skipping a
+    zero addend changes no sum)."""
+    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    A0, b0 = O.assemble(prob.patches, dofs, nfree, 0)
+    A1, b1 = O.assemble(prob.patches, dofs, nfree, 1)
+    ip0, ix0 = _support_pattern(prob, dofs, nfree, True)
+    ip1, ix1 = _support_pattern(prob, dofs, nfree, False)
+    assert np.array_equal(A0.indptr, ip0) and np.array_equal(A0.indices, ix0)
+    assert np.array_equal(A1.indptr, ip1) and np.array_equal(A1.indices, ix1)
+    assert A0.nnz < A1.nnz
+    assert np.array_equal(b0, b1)
+    D1 = A1.toarray()
+    mask = sp.csr_matrix((np.ones(A0.nnz), A0.indices, A0.indptr), shape=A0.shape).toarray() > 0
+    assert np.all(D1[~mask] == 0.0)
+    assert np.array_equal(A0.toarray()[mask], D1[mask])
+    assert np.all(np.diff(A0.indptr) > 0)
+    np.linalg.cholesky(A0.toarray())   # still SPD
+
+
 def test_assemble_rows_matches_full():
     prob = problems.poisson_noisy_cube(n=5, p=3, seed=8)
     dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
@@ -632,6 +691,78 @@ def test_two_grid_richardson_and_chebyshev_match_numpy():
         z = mg.apply(bf)
         ref = _numpy_two_grid(A, bf, Pd, cheb)
         assert np.linalg.norm(z - ref) <= 1e-11 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("m,ratio", [(1, 30.0), (2, 30.0), (3, 30.0), (3, 4.0), (5, 10.0)])
+def test_chebyshev_error_propagator_is_the_chebyshev_polynomial(m, ratio):
+    """The m-step Chebyshev smoother on [l1/ratio, l1] damps the error by
+    p_m(D^-1 A) with p_m(lam) = T_m((theta - lam)/delta) / T_m(theta/delta)
+    (theta, delta: midpoint and half-width of the interval).  Checked through
+    a two-grid cycle whose spectrum is known in closed form: A = D^1/2 C D^1/2
+    with C a symmetric circulant with unit diagonal (D^-1 A is similar to C,
+    eigenvalues 1 + 2 sum_j c_j cos(2 pi j k / n)) and the coarse space
+    D^-1/2 1 (C's eigenvector of k = 0).  Then, in the coordinates
+    D^1/2 x, the cycle's error propagator is p_m(C) (I - q0 q0^T) p_m(C) and
+    the cycle from x = 0 returns z = D^-1/2 (I - E) D^1/2 A^-1 b."""
+    n = 48
+    c = np.zeros(n)
+    c[0], c[1], c[2] = 1.0, -0.31, 0.07
+    c[-1], c[-2] = c[1], c[2]
+    C = np.array([np.roll(c, i) for i in range(n)])
+    k = np.arange(n)
+    lam_closed = 1.0 + 2 * (c[1] * np.cos(2 * np.pi * k / n) + c[2] * np.cos(4 * np.pi * k / n))
+    lam, Q = np.linalg.eigh(C)
+    assert np.allclose(np.sort(lam_closed), lam, atol=1e-14)
+    rng = np.random.default_rng(5)
+    dg = rng.uniform(0.5, 2.0, n)
+    A = np.sqrt(dg)[:, None] * C * np.sqrt(dg)[None, :]
+    P = (1.0 / np.sqrt(dg))[:, None]
+    mg = O.Multigrid(sp.csr_matrix(A), [sp.csr_matrix(P)], npre=m, npost=m, smoother="chebyshev",
+                     smoother_param=ratio)
+    l1 = mg.lmax(1)
+    l0 = l1 / ratio
+    theta, delta = (l1 + l0) / 2, (l1 - l0) / 2
+    Tm = np.polynomial.chebyshev.Chebyshev.basis(m)
+    pm = Tm((theta - lam) / delta) / Tm(theta / delta)
+    pC = (Q * pm) @ Q.T
+    q0 = np.ones(n) / np.sqrt(n)
+    E = pC @ (np.eye(n) - np.outer(q0, q0)) @ pC
+    b = rng.standard_normal(n)
+    xs = np.linalg.solve(A, b)
+    want = (1.0 / np.sqrt(dg)) * ((np.eye(n) - E) @ (np.sqrt(dg) * xs))
+    z = mg.apply(b)
+    assert np.linalg.norm(z - want) <= 1e-12 * np.linalg.norm(want)
+    # a dropped recurrence term (d_k without its rho_k rho_{k-1} d_{k-1}
+    # part) is a different polynomial for m >= 2: the check separates them
+    if m >= 2:
+        assert np.abs(pm).max() < 1.0
+
+
+@pytest.mark.parametrize("smoother,npre,npost", [("jacobi", 2, 2), ("chebyshev", 3, 3),
+                                                  ("gauss_seidel", 1, 1), ("jacobi", 1, 3)])
+def test_vcycle_is_symmetric_and_contracting(smoother, npre, npost):
+    """Three-level V(nu, nu) cycle with a symmetric smoother pair (damped
+    Jacobi, Chebyshev, forward Gauss-Seidel before and reverse after) is a
+    symmetric preconditioner B, and I - BA is a contraction in the energy
+    norm: the eigenvalues of BA lie in (0, 1].  An unbalanced cycle V(1, 3)
+    is not symmetric -- the check can tell."""
+    levels = [problems.square_grid(2, 2, nn, 2) for nn in (1, 2, 4)]
+    maps = [O.mapper(p.patches, p.dir_sides, p.ifaces) for p in levels]
+    Af, _ = O.assemble(levels[-1].patches, maps[-1][0], maps[-1][1], 1)
+    Ps = [O.transfer_multipatch(levels[i].patches, maps[i][0], maps[i][1], maps[i + 1][0], maps[i + 1][1])
+          for i in range(2)]
+    kw = dict(damping=0.6) if smoother == "jacobi" else dict(smoother=smoother)
+    mg = O.Multigrid(Af, Ps, npre=npre, npost=npost, **kw)
+    n = Af.shape[0]
+    B = np.array([mg.apply(np.eye(n)[i]) for i in range(n)]).T
+    asym = np.abs(B - B.T).max() / np.abs(B).max()
+    if npre != npost:
+        assert asym > 1e-6
+        return
+    assert asym <= 1e-12
+    ev = np.linalg.eigvals(B @ Af.toarray())
+    assert np.abs(ev.imag).max() <= 1e-10
+    assert ev.real.min() > 0.0 and ev.real.max() <= 1.0 + 1e-10, (ev.real.min(), ev.real.max())
 
 
 def test_chebyshev_mg_pcg_beats_jacobi_mg_in_3d():
