@@ -123,6 +123,9 @@ hipError_t launch_ap(int d, int n, const int *arp, const int *aci, const double 
 hipError_t launch_rap(int d, int nc, const int *rrp, const int *rci, const double *rv, const int *bcnt,
                       const int *bci, const double *bv, const int *crp, const int *cci, double *cval,
                       int *err, hipStream_t s);
+// pattern of R B (count pass with cnt, fill pass with crp/cci); widths as launch_rap
+hipError_t launch_rap_pattern(int d, int nc, const int *rrp, const int *rci, const int *bcnt, const int *bci,
+                              int *cnt, const int *crp, int *cci, int *err, hipStream_t s);
 // element-level Galerkin coarsening (k_coarsen): supported (d, p)
 bool coarsen_supported(int d, int p);
 hipError_t launch_coarsen(int d, int p, const PatchDev *cpatches, const PatchDev *fpatches, const int *gpatch,

@@ -263,6 +263,48 @@ __global__ void k_rap(int nc, const int *__restrict__ rrp, const int *__restrict
     for (int i = 0; i < len; ++i) cval[a + i] = acc[i];
 }
 
+// Pattern of A_c = R B when the coarse pattern is not known in advance
+// (transfers supplied by the caller): per coarse row the sorted union of
+// the columns of the B rows that R's row references.  cnt != nullptr:
+// write the row lengths; otherwise write the columns at crp[c].
+template <int WB, int MAXC>
+__global__ void k_rap_pattern(int nc, const int *__restrict__ rrp, const int *__restrict__ rci,
+                              const int *__restrict__ bcnt, const int *__restrict__ bci, int *__restrict__ cnt,
+                              const int *__restrict__ crp, int *__restrict__ cci, int *__restrict__ err)
+{
+    const int c1 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c1 >= nc) return;
+    int cols[MAXC];
+    int len = 0;
+    for (int k = rrp[c1]; k < rrp[c1 + 1]; ++k) {
+        const int f1 = rci[k];
+        const int nb = bcnt[f1];
+        for (int m = 0; m < nb; ++m) {
+            const int c2 = bci[(int64_t)f1 * WB + m];
+            int lo = 0, hi = len;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (cols[mid] < c2) lo = mid + 1; else hi = mid;
+            }
+            if (lo < len && cols[lo] == c2) continue;
+            if (len >= MAXC) {
+                err[0] = 4;
+                if (cnt) cnt[c1] = 0;
+                return;
+            }
+            for (int i = len; i > lo; --i) cols[i] = cols[i - 1];
+            cols[lo] = c2;
+            ++len;
+        }
+    }
+    if (cnt) {
+        cnt[c1] = len;
+    } else {
+        const int a = crp[c1];
+        for (int i = 0; i < len; ++i) cci[a + i] = cols[i];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Element-level Galerkin product.  Coarse element E of a dyadically coarsened
 // patch gets K_E = sum over its 2^D child elements e (lexicographic) of
@@ -678,6 +720,19 @@ hipError_t launch_rap(int d, int nc, const int *rrp, const int *rci, const doubl
         hipLaunchKernelGGL((k_rap<128, 128>), g1(nc, 128), dim3(128), 0, s, nc, rrp, rci, rv, bcnt, bci, bv, crp, cci, cval, err);
     else
         hipLaunchKernelGGL((k_rap<512, 1024>), g1(nc, 64), dim3(64), 0, s, nc, rrp, rci, rv, bcnt, bci, bv, crp, cci, cval, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_rap_pattern(int d, int nc, const int *rrp, const int *rci, const int *bcnt, const int *bci,
+                              int *cnt, const int *crp, int *cci, int *err, hipStream_t s)
+{
+    if (nc <= 0) return hipSuccess;
+    if (d == 2)
+        hipLaunchKernelGGL((k_rap_pattern<128, 128>), g1(nc, 128), dim3(128), 0, s, nc, rrp, rci, bcnt, bci, cnt, crp,
+                           cci, err);
+    else
+        hipLaunchKernelGGL((k_rap_pattern<512, 1024>), g1(nc, 64), dim3(64), 0, s, nc, rrp, rci, bcnt, bci, cnt, crp,
+                           cci, err);
     return hipGetLastError();
 }
 

@@ -2341,6 +2341,10 @@ struct MGLevel {
     // subspace corrected mass smoother of the level's basis and its damping
     std::shared_ptr<gsx_fd> scms;
     double scms_tau = 0.0;
+    // this level's smoother (levels >= 1): kind, damping of Jacobi/Richardson,
+    // Chebyshev interval ratio
+    int smoother = GSX_SMOOTH_JACOBI;
+    double tau = 1.0, cheb_ratio = 30.0;
 };
 
 // Level sets of one Gauss-Seidel sweep over the CSR pattern: lev(i) = 1 +
@@ -2376,8 +2380,7 @@ struct gsx_mg {
     double tau = 1.0, cgc = 1.0;
     int precision = GSX_FP64;
     int npad = 0;
-    int smoother = GSX_SMOOTH_JACOBI;
-    double cheb_ratio = 30.0;
+    double scms_saved = 0.0;           // SCMS damping kept for finer levels
     DevPtr Dinv;
     DevPtr zin, zout;                  // finest FP64 staging for the mixed cycle
     ScmsBox fine_box;                  // free box of a single-patch finest level
@@ -2470,6 +2473,54 @@ static gsx_status scan_counts(gsx_ctx *c, const int *counts, int n, DevPtr &rowp
     return GSX_OK;
 }
 
+// Per-level diagonals, FP32 copies and vectors, then the exact coarse solve
+// (dense FP64 inverse of the coarsest operator).
+static gsx_status mg_finish(gsx_mg *mg)
+{
+    gsx_ctx *c = mg->ctx;
+    hipStream_t s = c->stream;
+    // per-level diagonals, FP32 copies, vectors
+    const int nlevels = (int)mg->L.size();
+    const bool mixed = mg->precision == GSX_MIXED;
+    for (int l = 0; l < nlevels; ++l) {
+        mg->L[l].tau = mg->tau;
+        MGLevel &V = mg->L[l];
+        V.diag = dalloc(c, sizeof(double) * std::max(1, V.n));
+        if (!V.diag) return fail(c, GSX_ERR_OOM, "device allocation (diag)");
+        HIP_OK(c, gsx::launch_diag_f64(V.n, dp<int>(V.rowptr), dp<int>(V.col), dp<double>(V.val), dp<double>(V.diag), s));
+        if (mixed) {
+            V.val32 = dalloc(c, sizeof(float) * std::max<int64_t>(1, V.nnz));
+            V.diag32 = dalloc(c, sizeof(float) * std::max(1, V.n));
+            if (!V.val32 || !V.diag32) return fail(c, GSX_ERR_OOM, "device allocation (fp32 level)");
+            HIP_OK(c, gsx::launch_conv_f64_f32(V.nnz, dp<double>(V.val), dp<float>(V.val32), s));
+            HIP_OK(c, gsx::launch_conv_f64_f32(V.n, dp<double>(V.diag), dp<float>(V.diag32), s));
+            if (l < nlevels - 1) {
+                V.pv32 = dalloc(c, sizeof(float) * std::max<int64_t>(1, V.pnnz));
+                V.rv32 = dalloc(c, sizeof(float) * std::max<int64_t>(1, V.pnnz));
+                if (!V.pv32 || !V.rv32) return fail(c, GSX_ERR_OOM, "device allocation (fp32 transfer)");
+                HIP_OK(c, gsx::launch_conv_f64_f32(V.pnnz, dp<double>(V.pv), dp<float>(V.pv32), s));
+                HIP_OK(c, gsx::launch_conv_f64_f32(V.pnnz, dp<double>(V.rv), dp<float>(V.rv32), s));
+            }
+        }
+        GSX_TRY(mg_level_vectors(c, V, mixed));
+    }
+    // exact coarse solve: dense FP64 inverse of the coarsest operator
+    {
+        KScope ki(c, "mg_coarse_inverse");
+        MGLevel &C0 = mg->L[0];
+        const int n0 = C0.n;
+        if ((int64_t)n0 > 32768) return fail(c, GSX_ERR_ARG, "coarsest level too large for the dense inverse");
+        mg->npad = std::max(64, (n0 + 63) / 64 * 64);
+        mg->Dinv = dalloc(c, sizeof(double) * (size_t)mg->npad * mg->npad);
+        DevPtr W = dalloc(c, sizeof(double) * (size_t)64 * mg->npad), Pinv = dalloc(c, sizeof(double) * 64 * 64);
+        if (!mg->Dinv || !W || !Pinv) return fail(c, GSX_ERR_OOM, "device allocation (coarse inverse)");
+        HIP_OK(c, gsx::launch_dense_inverse(n0, mg->npad, dp<int>(C0.rowptr), dp<int>(C0.col), dp<double>(C0.val),
+                                            dp<double>(mg->Dinv), dp<double>(W), dp<double>(Pinv), s));
+        HIP_OK(c, hipStreamSynchronize(s));
+    }
+    return GSX_OK;
+}
+
 extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nlevels, int32_t npre,
                                     int32_t npost, int32_t ncycles, double damping, double cgc_damping,
                                     int32_t precision, gsx_mg **out)
@@ -2489,7 +2540,6 @@ extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nl
     mg->tau = damping;
     mg->cgc = cgc_damping;
     mg->precision = precision;
-    const bool mixed = precision == GSX_MIXED;
     mg->L.resize(nlevels);
     mg->fine_box_ok = scms_box_of(fine, mg->fine_box);
     // finest level = the given matrix
@@ -2753,42 +2803,143 @@ extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nl
             return fail(c, GSX_ERR_ARG, buf);
         }
     }
-    // per-level diagonals, FP32 copies, vectors
-    for (int l = 0; l < nlevels; ++l) {
-        MGLevel &V = mg->L[l];
-        V.diag = dalloc(c, sizeof(double) * std::max(1, V.n));
-        if (!V.diag) return fail(c, GSX_ERR_OOM, "device allocation (diag)");
-        HIP_OK(c, gsx::launch_diag_f64(V.n, dp<int>(V.rowptr), dp<int>(V.col), dp<double>(V.val), dp<double>(V.diag), s));
-        if (mixed) {
-            V.val32 = dalloc(c, sizeof(float) * std::max<int64_t>(1, V.nnz));
-            V.diag32 = dalloc(c, sizeof(float) * std::max(1, V.n));
-            if (!V.val32 || !V.diag32) return fail(c, GSX_ERR_OOM, "device allocation (fp32 level)");
-            HIP_OK(c, gsx::launch_conv_f64_f32(V.nnz, dp<double>(V.val), dp<float>(V.val32), s));
-            HIP_OK(c, gsx::launch_conv_f64_f32(V.n, dp<double>(V.diag), dp<float>(V.diag32), s));
-            if (l < nlevels - 1) {
-                V.pv32 = dalloc(c, sizeof(float) * std::max<int64_t>(1, V.pnnz));
-                V.rv32 = dalloc(c, sizeof(float) * std::max<int64_t>(1, V.pnnz));
-                if (!V.pv32 || !V.rv32) return fail(c, GSX_ERR_OOM, "device allocation (fp32 transfer)");
-                HIP_OK(c, gsx::launch_conv_f64_f32(V.pnnz, dp<double>(V.pv), dp<float>(V.pv32), s));
-                HIP_OK(c, gsx::launch_conv_f64_f32(V.pnnz, dp<double>(V.rv), dp<float>(V.rv32), s));
-            }
-        }
-        GSX_TRY(mg_level_vectors(c, V, mixed));
-    }
-    // exact coarse solve: dense FP64 inverse of the coarsest operator
+    GSX_TRY(mg_finish(mg.get()));
+    *out = mg.release();
+    return GSX_OK;
+}
+
+// gsMultiGridOp::make(fineMatrix, transferMatrices) + init
+// (src/gsMultiGrid/gsMultiGrid.hpp:19-35,70-94): the caller's transfers
+// (transfer l prolongates level l to level l + 1, n_{l+1} x n_l row-major
+// CSR, level nlevels - 1 = A), coarse operators P^T A P formed on the device
+// (B = A P, then the pattern and values of P^T B), restriction P^T.
+extern "C" gsx_status gsx_mg_create_transfers(gsx_matrix *A, int32_t ntransfer, const int32_t *ncoarse,
+                                              const int32_t *const *prow, const int32_t *const *pcol,
+                                              const double *const *pval, int32_t npre, int32_t npost,
+                                              int32_t ncycles, double damping, double cgc_damping,
+                                              int32_t precision, gsx_mg **out)
+{
+    if (!A || !out || ntransfer < 0 || npre < 0 || npost < 0 || ncycles < 1) return GSX_ERR_ARG;
+    if (ntransfer > 0 && (!ncoarse || !prow || !pcol || !pval)) return GSX_ERR_ARG;
+    gsx_ctx *c = A->ctx;
+    if (ncycles != 1) return fail(c, GSX_ERR_ARG, "only V-cycles (NumCycles = 1) are implemented");
+    if (precision != GSX_FP64 && precision != GSX_MIXED) return fail(c, GSX_ERR_ARG, "unknown precision");
+    hipStream_t s = c->stream;
+    KScope ks(c, "mg_setup");
+    const int nlevels = ntransfer + 1;
+    auto mg = std::make_unique<gsx_mg>();
+    mg->ctx = c;
+    mg->npre = npre;
+    mg->npost = npost;
+    mg->ncycles = ncycles;
+    mg->tau = damping;
+    mg->cgc = cgc_damping;
+    mg->precision = precision;
+    mg->L.resize(nlevels);
+    mg->fine_box_ok = A->src && scms_box_of(A->src, mg->fine_box);
     {
-        KScope ki(c, "mg_coarse_inverse");
-        MGLevel &C0 = mg->L[0];
-        const int n0 = C0.n;
-        if ((int64_t)n0 > 32768) return fail(c, GSX_ERR_ARG, "coarsest level too large for the dense inverse");
-        mg->npad = std::max(64, (n0 + 63) / 64 * 64);
-        mg->Dinv = dalloc(c, sizeof(double) * (size_t)mg->npad * mg->npad);
-        DevPtr W = dalloc(c, sizeof(double) * (size_t)64 * mg->npad), Pinv = dalloc(c, sizeof(double) * 64 * 64);
-        if (!mg->Dinv || !W || !Pinv) return fail(c, GSX_ERR_OOM, "device allocation (coarse inverse)");
-        HIP_OK(c, gsx::launch_dense_inverse(n0, mg->npad, dp<int>(C0.rowptr), dp<int>(C0.col), dp<double>(C0.val),
-                                            dp<double>(mg->Dinv), dp<double>(W), dp<double>(Pinv), s));
-        HIP_OK(c, hipStreamSynchronize(s));
+        MGLevel &f = mg->L[nlevels - 1];
+        f.n = A->n;
+        f.nnz = A->nnz;
+        f.lpr = A->lpr;
+        f.rowptr = A->rowptr;
+        f.col = A->col;
+        f.val = A->val;
     }
+    for (int l = nlevels - 2; l >= 0; --l) {
+        MGLevel &C = mg->L[l], &F = mg->L[l + 1];
+        const int nf = F.n, nc = ncoarse[l];
+        const int32_t *rp = prow[l], *ci = pcol[l];
+        if (nc < 0 || !rp || !ci || !pval[l]) return fail(c, GSX_ERR_ARG, "transfer matrix missing");
+        if (rp[0] != 0) return fail(c, GSX_ERR_ARG, "transfer matrix: rowptr[0] != 0");
+        for (int i = 0; i < nf; ++i) {
+            if (rp[i + 1] < rp[i]) return fail(c, GSX_ERR_ARG, "transfer matrix: rowptr not ascending");
+            for (int k = rp[i]; k < rp[i + 1]; ++k)
+                if (ci[k] < 0 || ci[k] >= nc || (k > rp[i] && ci[k] <= ci[k - 1])) {
+                    char buf[160];
+                    std::snprintf(buf, sizeof(buf),
+                                  "transfer matrix %d: row %d has a column out of range or not ascending", l, i);
+                    return fail(c, GSX_ERR_ARG, buf);
+                }
+        }
+        const int64_t pn = rp[nf];
+        C.n = nc;
+        C.nfine = nf;
+        C.pnnz = pn;
+        C.prp = dalloc(c, sizeof(int) * (nf + 1));
+        C.pci = dalloc(c, sizeof(int) * std::max<int64_t>(1, pn));
+        C.pv = dalloc(c, sizeof(double) * std::max<int64_t>(1, pn));
+        if (!C.prp || !C.pci || !C.pv) return fail(c, GSX_ERR_OOM, "device allocation (transfer)");
+        HIP_OK(c, hipMemcpyAsync(C.prp.get(), rp, sizeof(int) * (nf + 1), hipMemcpyHostToDevice, s));
+        HIP_OK(c, hipMemcpyAsync(C.pci.get(), ci, sizeof(int) * pn, hipMemcpyHostToDevice, s));
+        HIP_OK(c, hipMemcpyAsync(C.pv.get(), pval[l], sizeof(double) * pn, hipMemcpyHostToDevice, s));
+        // R = P^T
+        DevPtr tcount = dalloc(c, sizeof(int) * (nc + 1)), cursor = dalloc(c, sizeof(int) * (nc + 1));
+        DevPtr errf = dalloc(c, sizeof(int) * 2);
+        if (!tcount || !cursor || !errf) return fail(c, GSX_ERR_OOM, "device allocation (restriction)");
+        HIP_OK(c, hipMemsetAsync(tcount.get(), 0, sizeof(int) * (nc + 1), s));
+        HIP_OK(c, hipMemsetAsync(cursor.get(), 0, sizeof(int) * (nc + 1), s));
+        HIP_OK(c, hipMemsetAsync(errf.get(), 0, sizeof(int) * 2, s));
+        HIP_OK(c, gsx::launch_transpose(0, nf, dp<int>(C.prp), dp<int>(C.pci), nullptr, dp<int>(tcount), nullptr,
+                                        nullptr, nullptr, nullptr, 0, s));
+        int64_t rn = 0;
+        GSX_TRY(scan_counts(c, dp<int>(tcount), nc, C.rrp, rn));
+        C.rci = dalloc(c, sizeof(int) * std::max<int64_t>(1, rn));
+        C.rv = dalloc(c, sizeof(double) * std::max<int64_t>(1, rn));
+        if (!C.rci || !C.rv) return fail(c, GSX_ERR_OOM, "device allocation (restriction)");
+        HIP_OK(c, gsx::launch_transpose(1, nf, dp<int>(C.prp), dp<int>(C.pci), dp<double>(C.pv), nullptr,
+                                        dp<int>(C.rrp), dp<int>(cursor), dp<int>(C.rci), dp<double>(C.rv), 0, s));
+        HIP_OK(c, gsx::launch_transpose(2, 0, nullptr, nullptr, nullptr, nullptr, dp<int>(C.rrp), nullptr,
+                                        dp<int>(C.rci), dp<double>(C.rv), nc, s));
+        // B = A P with the narrow rows first, the wide ones if those overflow
+        KScope kg(c, "mg_galerkin");
+        int width_class = 2, errh[2] = {0, 0};
+        DevPtr bcnt, bci, bv;
+        for (;;) {
+            const int WB = gsx::ap_width(width_class);
+            bcnt = dalloc(c, sizeof(int) * std::max(1, nf));
+            bci = dalloc(c, sizeof(int) * (size_t)std::max(1, nf) * WB);
+            bv = dalloc(c, sizeof(double) * (size_t)std::max(1, nf) * WB);
+            if (!bcnt || !bci || !bv) return fail(c, GSX_ERR_OOM, "device allocation (Galerkin product)");
+            HIP_OK(c, hipMemsetAsync(errf.get(), 0, sizeof(int) * 2, s));
+            HIP_OK(c, gsx::launch_ap(width_class, nf, dp<int>(F.rowptr), dp<int>(F.col), dp<double>(F.val),
+                                     dp<int>(C.prp), dp<int>(C.pci), dp<double>(C.pv), dp<int>(bcnt), dp<int>(bci),
+                                     dp<double>(bv), dp<int>(errf), s));
+            DevPtr ccount = dalloc(c, sizeof(int) * (nc + 1));
+            if (!ccount) return fail(c, GSX_ERR_OOM, "device allocation (Galerkin pattern)");
+            HIP_OK(c, gsx::launch_rap_pattern(width_class, nc, dp<int>(C.rrp), dp<int>(C.rci), dp<int>(bcnt),
+                                              dp<int>(bci), dp<int>(ccount), nullptr, nullptr, dp<int>(errf), s));
+            HIP_OK(c, hipMemcpyAsync(errh, errf.get(), sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+            HIP_OK(c, hipStreamSynchronize(s));
+            if (errh[0] == 0) {
+                int64_t cn = 0;
+                GSX_TRY(scan_counts(c, dp<int>(ccount), nc, C.rowptr, cn));
+                if (cn > INT32_MAX) return fail(c, GSX_ERR_ARG, "coarse operator too large for int32 indices");
+                C.nnz = cn;
+                C.col = dalloc(c, sizeof(int) * std::max<int64_t>(1, cn));
+                C.val = dalloc(c, sizeof(double) * std::max<int64_t>(1, cn));
+                if (!C.col || !C.val) return fail(c, GSX_ERR_OOM, "device allocation (coarse operator)");
+                HIP_OK(c, gsx::launch_rap_pattern(width_class, nc, dp<int>(C.rrp), dp<int>(C.rci), dp<int>(bcnt),
+                                                  dp<int>(bci), nullptr, dp<int>(C.rowptr), dp<int>(C.col),
+                                                  dp<int>(errf), s));
+                HIP_OK(c, gsx::launch_rap(width_class, nc, dp<int>(C.rrp), dp<int>(C.rci), dp<double>(C.rv),
+                                          dp<int>(bcnt), dp<int>(bci), dp<double>(bv), dp<int>(C.rowptr),
+                                          dp<int>(C.col), dp<double>(C.val), dp<int>(errf) + 1, s));
+                HIP_OK(c, hipMemcpyAsync(errh, errf.get(), sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+                HIP_OK(c, hipStreamSynchronize(s));
+                if (errh[0] == 0 && errh[1] == 0) break;
+            }
+            if (width_class == 3) {
+                char buf[160];
+                std::snprintf(buf, sizeof(buf), "multigrid level %d: Galerkin product rows too wide (codes %d, %d)", l,
+                              errh[0], errh[1]);
+                return fail(c, GSX_ERR_ARG, buf);
+            }
+            width_class = 3;
+        }
+        C.lpr = gsx::spmv_lpr(C.n ? (double)C.nnz / C.n : 1.0);
+    }
+    GSX_TRY(mg_finish(mg.get()));
     *out = mg.release();
     return GSX_OK;
 }
@@ -2841,8 +2992,9 @@ struct SmoothCoef {
 };
 static SmoothCoef smooth_coef(const gsx_mg *mg, const MGLevel &V, int k, double &rho)
 {
-    if (mg->smoother == GSX_SMOOTH_RICHARDSON) return {0.0, mg->tau};
-    const double l1 = V.lmax, l0 = l1 / mg->cheb_ratio;
+    (void)mg;
+    if (V.smoother == GSX_SMOOTH_RICHARDSON) return {0.0, V.tau};
+    const double l1 = V.lmax, l0 = l1 / V.cheb_ratio;
     const double theta = 0.5 * (l1 + l0), delta = 0.5 * (l1 - l0), sigma = theta / delta;
     if (k == 0) {
         rho = 1.0 / sigma;
@@ -2869,7 +3021,7 @@ static gsx_status smooth64(gsx_mg *mg, MGLevel &V, double *&x, double *&t, const
         return v32 ? gsx::launch_residual_v32(V.lpr, V.n, rp, ci, va32, xx, b, rr, s)
                    : gsx::launch_residual_f64(V.lpr, V.n, rp, ci, va, xx, b, rr, s);
     };
-    if (mg->smoother == GSX_SMOOTH_GAUSS_SEIDEL) {
+    if (V.smoother == GSX_SMOOTH_GAUSS_SEIDEL) {
         if (m > 0 && zero) HIP_OK(c, gsx::launch_fill_f64(x, V.n, 0.0, s));
         const int *rows = dp<int>(post ? V.gs_rev : V.gs_fwd);
         const std::vector<int> &off = post ? V.gs_rev_off : V.gs_fwd_off;
@@ -2883,7 +3035,7 @@ static gsx_status smooth64(gsx_mg *mg, MGLevel &V, double *&x, double *&t, const
         if (m > 0) zero = false;
         return GSX_OK;
     }
-    if (mg->smoother == GSX_SMOOTH_SCMS) {
+    if (V.smoother == GSX_SMOOTH_SCMS) {
         // x += tau S (b - A x)
         if (m > 0 && zero) HIP_OK(c, gsx::launch_fill_f64(x, V.n, 0.0, s));
         double *r = dp<double>(V.r), *d = dp<double>(V.d);
@@ -2897,12 +3049,12 @@ static gsx_status smooth64(gsx_mg *mg, MGLevel &V, double *&x, double *&t, const
         if (m > 0) zero = false;
         return GSX_OK;
     }
-    if (mg->smoother == GSX_SMOOTH_JACOBI) {
+    if (V.smoother == GSX_SMOOTH_JACOBI) {
         for (int i = 0; i < m; ++i) {
             if (v32)
-                HIP_OK(c, gsx::launch_jacobi_step_v32(V.lpr, V.n, rp, ci, va32, dg, b, x, t, mg->tau, zero ? 1 : 0, s));
+                HIP_OK(c, gsx::launch_jacobi_step_v32(V.lpr, V.n, rp, ci, va32, dg, b, x, t, V.tau, zero ? 1 : 0, s));
             else
-                HIP_OK(c, gsx::launch_jacobi_step_f64(V.lpr, V.n, rp, ci, va, dg, b, x, t, mg->tau, zero ? 1 : 0, s));
+                HIP_OK(c, gsx::launch_jacobi_step_f64(V.lpr, V.n, rp, ci, va, dg, b, x, t, V.tau, zero ? 1 : 0, s));
             std::swap(x, t);
             zero = false;
         }
@@ -2914,7 +3066,7 @@ static gsx_status smooth64(gsx_mg *mg, MGLevel &V, double *&x, double *&t, const
     for (int k = 0; k < m; ++k) {
         HIP_OK(c, residual(x, r));
         const SmoothCoef cf = smooth_coef(mg, V, k, rho);
-        HIP_OK(c, gsx::launch_smooth_update_f64(V.n, x, d, r, mg->smoother == GSX_SMOOTH_CHEBYSHEV ? dg : nullptr,
+        HIP_OK(c, gsx::launch_smooth_update_f64(V.n, x, d, r, V.smoother == GSX_SMOOTH_CHEBYSHEV ? dg : nullptr,
                                                 cf.c1, cf.c2, k == 0 ? 1 : 0, s));
         zero = false;
     }
@@ -2928,7 +3080,7 @@ static gsx_status smooth32(gsx_mg *mg, MGLevel &V, float *&x, float *&t, const f
     hipStream_t s = c->stream;
     const int *rp = dp<int>(V.rowptr), *ci = dp<int>(V.col);
     const float *va = dp<float>(V.val32), *dg = dp<float>(V.diag32);
-    if (mg->smoother == GSX_SMOOTH_GAUSS_SEIDEL) {
+    if (V.smoother == GSX_SMOOTH_GAUSS_SEIDEL) {
         if (m > 0 && zero) HIP_OK(c, gsx::launch_fill_f32(x, V.n, 0.0f, s));
         const int *rows = dp<int>(post ? V.gs_rev : V.gs_fwd);
         const std::vector<int> &off = post ? V.gs_rev_off : V.gs_fwd_off;
@@ -2938,7 +3090,7 @@ static gsx_status smooth32(gsx_mg *mg, MGLevel &V, float *&x, float *&t, const f
         if (m > 0) zero = false;
         return GSX_OK;
     }
-    if (mg->smoother == GSX_SMOOTH_SCMS) {
+    if (V.smoother == GSX_SMOOTH_SCMS) {
         if (m > 0 && zero) HIP_OK(c, gsx::launch_fill_f32(x, V.n, 0.0f, s));
         float *r = dp<float>(V.r32), *d = dp<float>(V.d32);
         const gsx_fd *F = V.scms.get();
@@ -2951,9 +3103,9 @@ static gsx_status smooth32(gsx_mg *mg, MGLevel &V, float *&x, float *&t, const f
         if (m > 0) zero = false;
         return GSX_OK;
     }
-    if (mg->smoother == GSX_SMOOTH_JACOBI) {
+    if (V.smoother == GSX_SMOOTH_JACOBI) {
         for (int i = 0; i < m; ++i) {
-            HIP_OK(c, gsx::launch_jacobi_step_f32(V.lpr, V.n, rp, ci, va, dg, b, x, t, (float)mg->tau, zero ? 1 : 0, s));
+            HIP_OK(c, gsx::launch_jacobi_step_f32(V.lpr, V.n, rp, ci, va, dg, b, x, t, (float)V.tau, zero ? 1 : 0, s));
             std::swap(x, t);
             zero = false;
         }
@@ -2965,7 +3117,7 @@ static gsx_status smooth32(gsx_mg *mg, MGLevel &V, float *&x, float *&t, const f
     for (int k = 0; k < m; ++k) {
         HIP_OK(c, gsx::launch_residual_f32(V.lpr, V.n, rp, ci, va, x, b, r, s));
         const SmoothCoef cf = smooth_coef(mg, V, k, rho);
-        HIP_OK(c, gsx::launch_smooth_update_f32(V.n, x, d, r, mg->smoother == GSX_SMOOTH_CHEBYSHEV ? dg : nullptr,
+        HIP_OK(c, gsx::launch_smooth_update_f32(V.n, x, d, r, V.smoother == GSX_SMOOTH_CHEBYSHEV ? dg : nullptr,
                                                 (float)cf.c1, (float)cf.c2, k == 0 ? 1 : 0, s));
         zero = false;
     }
@@ -3089,145 +3241,166 @@ static gsx_status mg_apply_dev(gsx_mg *mg, const double *r, double *z)
     return GSX_OK;
 }
 
-extern "C" gsx_status gsx_mg_set_smoother(gsx_mg *mg, int32_t kind, double param)
+// Set-up of level l's smoother (1 <= l < nlevels).  GSX_SMOOTH_SCMS: the
+// subspace corrected mass smoother of setupSubspaceCorrectedMassSmoother
+// (examples/multiGrid_example.cpp:407-560) for one patch with every side
+// Dirichlet (the interior is the only component with free functions): the
+// smoother of the level's basis (sigma = MG.Scaling, alpha = 0, beta = 1)
+// damped by 1 / lambda, lambda from ten power iterations of S A
+// (gsPreconditionerOp::estimateLargestEigenvalueOfPreconditionedSystem,
+// src/gsSolver/gsPreconditioner.h:121-132) started from v0_i = 1 / (1 + i
+// mod 5); once a level above nlevels - 5 has more elements than the degree
+// in every direction, its damping is kept (mg->scms_saved) for the finer
+// levels, as the example's patchLocalDampingParameters are.
+static gsx_status mg_setup_level_smoother(gsx_mg *mg, int l, int kind, double param, double damping)
 {
-    if (!mg || kind < GSX_SMOOTH_JACOBI || kind > GSX_SMOOTH_SCMS) return GSX_ERR_ARG;
     gsx_ctx *c = mg->ctx;
     hipStream_t s = c->stream;
+    const int nl = (int)mg->L.size();
+    MGLevel &V = mg->L[l];
+    const int n = V.n;
     if (kind == GSX_SMOOTH_SCMS) {
-        // setupSubspaceCorrectedMassSmoother, examples/multiGrid_example.cpp:
-        // 407-560, for one patch with every side Dirichlet (the interior is
-        // the only component with free functions): per level the smoother
-        // of the level's basis (sigma = MG.Scaling, alpha = 0, beta = 1)
-        // damped by 1 / lambda, lambda from ten power iterations of S A
-        // (gsPreconditionerOp::estimateLargestEigenvalueOfPreconditionedSystem,
-        // src/gsSolver/gsPreconditioner.h:121-132) started from
-        // v0_i = 1 / (1 + i mod 5); once level > nlevels - 5 with more
-        // elements than the degree in every direction, that level's damping
-        // is kept for the finer levels.
         const double sigma = param > 0.0 ? param : 0.12;
-        const int nl = (int)mg->L.size();
-        double saved = 0.0;
-        for (int l = 1; l < nl; ++l) {
-            MGLevel &V = mg->L[l];
-            ScmsBox B;
-            bool ok;
-            if (l == nl - 1) {
-                ok = mg->fine_box_ok;
-                if (ok) B = mg->fine_box;
-            } else {
-                ok = scms_box_of(V.prob, B);
-            }
-            bool alldir = ok;
-            for (int k = 0; ok && k < B.d; ++k) alldir = alldir && B.lo[k] == 1 && B.hi[k] == B.nb[k] - 2;
-            if (!alldir)
-                return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: single-patch problems with Dirichlet "
-                                            "conditions on every side only");
-            std::unique_ptr<gsx_fd> F;
-            gsx_status e = scms_build(c, B, sigma, 0.0, 1.0, F);
-            if (e != GSX_OK) return e;
-            if (F->size != V.n) return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: level size mismatch");
-            V.scms = std::shared_ptr<gsx_fd>(F.release());
-            const int n = V.n;
-            if (!V.d) V.d = dalloc(c, sizeof(double) * n);
-            if (mg->precision == GSX_MIXED && !V.d32) V.d32 = dalloc(c, sizeof(float) * n);
-            if (!V.d || (mg->precision == GSX_MIXED && !V.d32)) return fail(c, GSX_ERR_OOM, "device allocation (smoother)");
-            if (saved == 0.0) {
-                std::vector<double> xh(n);
-                for (int i = 0; i < n; ++i) xh[i] = 1.0 / (1.0 + (double)(i % 5));
-                DevPtr xv = dalloc(c, sizeof(double) * n), tv = dalloc(c, sizeof(double) * n);
-                if (!xv || !tv) return fail(c, GSX_ERR_OOM, "device allocation (power iteration)");
-                const gsx_fd *Fp = V.scms.get();
-                for (int it = 0; it < 10; ++it) {
-                    double nrm = 0.0;
-                    for (int i = 0; i < n; ++i) nrm += xh[i] * xh[i];
-                    nrm = std::sqrt(nrm);
-                    for (int i = 0; i < n; ++i) xh[i] /= nrm;
-                    HIP_OK(c, hipMemcpyAsync(xv.get(), xh.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
-                    HIP_OK(c, gsx::launch_spmv_f64(V.lpr, n, dp<int>(V.rowptr), dp<int>(V.col), dp<double>(V.val),
-                                                   dp<double>(xv), dp<double>(tv), nullptr, nullptr, 0, nullptr, s));
-                    HIP_OK(c, fd_apply<double>(Fp, dp<double>(tv), 0, Fp->m[0], (int64_t)Fp->m[0] * Fp->m[1],
-                                               dp<double>(xv), 0, Fp->m[0], (int64_t)Fp->m[0] * Fp->m[1], nullptr,
-                                               s));
-                    HIP_OK(c, hipMemcpyAsync(xh.data(), xv.get(), sizeof(double) * n, hipMemcpyDeviceToHost, s));
-                    HIP_OK(c, hipStreamSynchronize(s));
-                }
-                double lam = 0.0;
-                for (int i = 0; i < n; ++i) lam += xh[i] * xh[i];
-                lam = std::sqrt(lam);
-                if (!(lam > 0.0)) return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: zero power iterate");
-                V.scms_tau = 1.0 / lam;
-                bool keep = l > nl - 5;
-                for (int k = 0; k < B.d; ++k) keep = keep && (int)B.knots[k].size() - 2 * B.p[k] - 1 > B.p[k];
-                if (keep) saved = V.scms_tau;
-            } else {
-                V.scms_tau = saved;
-            }
-            V.lmax = V.scms_tau;
+        ScmsBox B;
+        bool ok;
+        if (l == nl - 1) {
+            ok = mg->fine_box_ok;
+            if (ok) B = mg->fine_box;
+        } else {
+            ok = scms_box_of(V.prob, B);
         }
-        mg->smoother = kind;
-        return GSX_OK;
-    }
-    mg->smoother = kind;
-    if (kind == GSX_SMOOTH_JACOBI) return GSX_OK;
-    if (kind == GSX_SMOOTH_GAUSS_SEIDEL) {
-        for (size_t l = 1; l < mg->L.size(); ++l) {
-            MGLevel &V = mg->L[l];
-            if (V.gs_fwd) continue;
-            std::vector<int> rp(V.n + 1), ci((size_t)V.nnz), rows;
-            HIP_OK(c, hipMemcpyAsync(rp.data(), V.rowptr.get(), sizeof(int) * (V.n + 1), hipMemcpyDeviceToHost, s));
-            HIP_OK(c, hipMemcpyAsync(ci.data(), V.col.get(), sizeof(int) * V.nnz, hipMemcpyDeviceToHost, s));
-            HIP_OK(c, hipStreamSynchronize(s));
-            V.gs_fwd = dalloc(c, sizeof(int) * std::max(1, V.n));
-            V.gs_rev = dalloc(c, sizeof(int) * std::max(1, V.n));
-            if (!V.gs_fwd || !V.gs_rev) return fail(c, GSX_ERR_OOM, "device allocation (Gauss-Seidel schedule)");
-            gs_schedule(V.n, rp, ci, false, rows, V.gs_fwd_off);
-            HIP_OK(c, hipMemcpyAsync(V.gs_fwd.get(), rows.data(), sizeof(int) * V.n, hipMemcpyHostToDevice, s));
-            HIP_OK(c, hipStreamSynchronize(s));
-            gs_schedule(V.n, rp, ci, true, rows, V.gs_rev_off);
-            HIP_OK(c, hipMemcpyAsync(V.gs_rev.get(), rows.data(), sizeof(int) * V.n, hipMemcpyHostToDevice, s));
-            HIP_OK(c, hipStreamSynchronize(s));
-        }
-        return GSX_OK;
-    }
-    if (kind == GSX_SMOOTH_CHEBYSHEV) mg->cheb_ratio = param > 1.0 ? param : 30.0;
-    for (size_t l = 1; l < mg->L.size(); ++l) {
-        MGLevel &V = mg->L[l];
-        const int n = V.n;
+        bool alldir = ok;
+        for (int k = 0; ok && k < B.d; ++k) alldir = alldir && B.lo[k] == 1 && B.hi[k] == B.nb[k] - 2;
+        if (!alldir)
+            return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: single-patch problems with Dirichlet "
+                                        "conditions on every side only, on a hierarchy built by gsx_mg_create");
+        std::unique_ptr<gsx_fd> F;
+        gsx_status e = scms_build(c, B, sigma, 0.0, 1.0, F);
+        if (e != GSX_OK) return e;
+        if (F->size != V.n) return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: level size mismatch");
+        V.scms = std::shared_ptr<gsx_fd>(F.release());
         if (!V.d) V.d = dalloc(c, sizeof(double) * n);
         if (mg->precision == GSX_MIXED && !V.d32) V.d32 = dalloc(c, sizeof(float) * n);
         if (!V.d || (mg->precision == GSX_MIXED && !V.d32)) return fail(c, GSX_ERR_OOM, "device allocation (smoother)");
-        if (kind != GSX_SMOOTH_CHEBYSHEV) continue;
-        // lambda_max(D^-1 A): 20 power iterations from v0_i = 1 / (1 + i mod 5)
-        std::vector<double> v0(n);
-        double nv = 0.0;
-        for (int i = 0; i < n; ++i) {
-            v0[i] = 1.0 / (1.0 + (double)(i % 5));
-            nv += v0[i] * v0[i];
+        if (mg->scms_saved == 0.0) {
+            std::vector<double> xh(n);
+            for (int i = 0; i < n; ++i) xh[i] = 1.0 / (1.0 + (double)(i % 5));
+            DevPtr xv = dalloc(c, sizeof(double) * n), tv = dalloc(c, sizeof(double) * n);
+            if (!xv || !tv) return fail(c, GSX_ERR_OOM, "device allocation (power iteration)");
+            const gsx_fd *Fp = V.scms.get();
+            for (int it = 0; it < 10; ++it) {
+                double nrm = 0.0;
+                for (int i = 0; i < n; ++i) nrm += xh[i] * xh[i];
+                nrm = std::sqrt(nrm);
+                for (int i = 0; i < n; ++i) xh[i] /= nrm;
+                HIP_OK(c, hipMemcpyAsync(xv.get(), xh.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+                HIP_OK(c, gsx::launch_spmv_f64(V.lpr, n, dp<int>(V.rowptr), dp<int>(V.col), dp<double>(V.val),
+                                               dp<double>(xv), dp<double>(tv), nullptr, nullptr, 0, nullptr, s));
+                HIP_OK(c, fd_apply<double>(Fp, dp<double>(tv), 0, Fp->m[0], (int64_t)Fp->m[0] * Fp->m[1],
+                                           dp<double>(xv), 0, Fp->m[0], (int64_t)Fp->m[0] * Fp->m[1], nullptr, s));
+                HIP_OK(c, hipMemcpyAsync(xh.data(), xv.get(), sizeof(double) * n, hipMemcpyDeviceToHost, s));
+                HIP_OK(c, hipStreamSynchronize(s));
+            }
+            double lam = 0.0;
+            for (int i = 0; i < n; ++i) lam += xh[i] * xh[i];
+            lam = std::sqrt(lam);
+            if (!(lam > 0.0)) return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: zero power iterate");
+            V.scms_tau = 1.0 / lam;
+            // elements per direction: the distinct non-empty knot spans
+            // (gsTensorBSplineBasis::component(j).numElements())
+            bool keep = l > nl - 5;
+            for (int k = 0; k < B.d; ++k) {
+                int nel = 0;
+                for (size_t q = 0; q + 1 < B.knots[k].size(); ++q) nel += B.knots[k][q + 1] > B.knots[k][q];
+                keep = keep && nel > B.p[k];
+            }
+            if (keep) mg->scms_saved = V.scms_tau;
+        } else {
+            V.scms_tau = mg->scms_saved;
         }
-        nv = std::sqrt(nv);
-        for (int i = 0; i < n; ++i) v0[i] /= nv;
-        const int nb = (n + 255) / 256;
-        DevPtr v = dalloc(c, sizeof(double) * n), w = dalloc(c, sizeof(double) * n), part = dalloc(c, sizeof(double) * nb);
-        if (!v || !w || !part) return fail(c, GSX_ERR_OOM, "device allocation (power iteration)");
-        HIP_OK(c, hipMemcpyAsync(v.get(), v0.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
-        std::vector<double> ph(nb);
-        double lam = 0.0;
-        for (int it = 0; it < 20; ++it) {
-            HIP_OK(c, gsx::launch_spmv_f64(V.lpr, n, dp<int>(V.rowptr), dp<int>(V.col), dp<double>(V.val), dp<double>(v),
-                                           dp<double>(w), nullptr, nullptr, 0, nullptr, s));
-            HIP_OK(c, gsx::launch_div_sq(n, dp<double>(w), dp<double>(V.diag), dp<double>(part), s));
-            HIP_OK(c, hipMemcpyAsync(ph.data(), part.get(), sizeof(double) * nb, hipMemcpyDeviceToHost, s));
-            HIP_OK(c, hipStreamSynchronize(s));
-            double sq = 0.0;
-            for (int k = 0; k < nb; ++k) sq += ph[k];
-            lam = std::sqrt(sq);
-            HIP_OK(c, gsx::launch_axpby(n, 1.0 / lam, dp<double>(w), 0.0, dp<double>(v), s));
-        }
-        V.lmax = 1.1 * lam;
+        V.lmax = V.scms_tau;
+        V.smoother = kind;
+        return GSX_OK;
     }
+    V.smoother = kind;
+    V.tau = damping;
+    if (kind == GSX_SMOOTH_JACOBI) return GSX_OK;
+    if (kind == GSX_SMOOTH_GAUSS_SEIDEL) {
+        if (V.gs_fwd) return GSX_OK;
+        std::vector<int> rp(V.n + 1), ci((size_t)V.nnz), rows;
+        HIP_OK(c, hipMemcpyAsync(rp.data(), V.rowptr.get(), sizeof(int) * (V.n + 1), hipMemcpyDeviceToHost, s));
+        HIP_OK(c, hipMemcpyAsync(ci.data(), V.col.get(), sizeof(int) * V.nnz, hipMemcpyDeviceToHost, s));
+        HIP_OK(c, hipStreamSynchronize(s));
+        V.gs_fwd = dalloc(c, sizeof(int) * std::max(1, V.n));
+        V.gs_rev = dalloc(c, sizeof(int) * std::max(1, V.n));
+        if (!V.gs_fwd || !V.gs_rev) return fail(c, GSX_ERR_OOM, "device allocation (Gauss-Seidel schedule)");
+        gs_schedule(V.n, rp, ci, false, rows, V.gs_fwd_off);
+        HIP_OK(c, hipMemcpyAsync(V.gs_fwd.get(), rows.data(), sizeof(int) * V.n, hipMemcpyHostToDevice, s));
+        HIP_OK(c, hipStreamSynchronize(s));
+        gs_schedule(V.n, rp, ci, true, rows, V.gs_rev_off);
+        HIP_OK(c, hipMemcpyAsync(V.gs_rev.get(), rows.data(), sizeof(int) * V.n, hipMemcpyHostToDevice, s));
+        HIP_OK(c, hipStreamSynchronize(s));
+        return GSX_OK;
+    }
+    if (kind == GSX_SMOOTH_CHEBYSHEV) V.cheb_ratio = param > 1.0 ? param : 30.0;
+    if (!V.d) V.d = dalloc(c, sizeof(double) * n);
+    if (mg->precision == GSX_MIXED && !V.d32) V.d32 = dalloc(c, sizeof(float) * n);
+    if (!V.d || (mg->precision == GSX_MIXED && !V.d32)) return fail(c, GSX_ERR_OOM, "device allocation (smoother)");
+    if (kind != GSX_SMOOTH_CHEBYSHEV) return GSX_OK;
+    // lambda_max(D^-1 A): 20 power iterations from v0_i = 1 / (1 + i mod 5)
+    std::vector<double> v0(n);
+    double nv = 0.0;
+    for (int i = 0; i < n; ++i) {
+        v0[i] = 1.0 / (1.0 + (double)(i % 5));
+        nv += v0[i] * v0[i];
+    }
+    nv = std::sqrt(nv);
+    for (int i = 0; i < n; ++i) v0[i] /= nv;
+    const int nb = (n + 255) / 256;
+    DevPtr v = dalloc(c, sizeof(double) * n), w = dalloc(c, sizeof(double) * n), part = dalloc(c, sizeof(double) * nb);
+    if (!v || !w || !part) return fail(c, GSX_ERR_OOM, "device allocation (power iteration)");
+    HIP_OK(c, hipMemcpyAsync(v.get(), v0.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    std::vector<double> ph(nb);
+    double lam = 0.0;
+    for (int it = 0; it < 20; ++it) {
+        HIP_OK(c, gsx::launch_spmv_f64(V.lpr, n, dp<int>(V.rowptr), dp<int>(V.col), dp<double>(V.val), dp<double>(v),
+                                       dp<double>(w), nullptr, nullptr, 0, nullptr, s));
+        HIP_OK(c, gsx::launch_div_sq(n, dp<double>(w), dp<double>(V.diag), dp<double>(part), s));
+        HIP_OK(c, hipMemcpyAsync(ph.data(), part.get(), sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+        HIP_OK(c, hipStreamSynchronize(s));
+        double sq = 0.0;
+        for (int k = 0; k < nb; ++k) sq += ph[k];
+        lam = std::sqrt(sq);
+        HIP_OK(c, gsx::launch_axpby(n, 1.0 / lam, dp<double>(w), 0.0, dp<double>(v), s));
+    }
+    V.lmax = 1.1 * lam;
     HIP_OK(c, hipStreamSynchronize(s));
     return GSX_OK;
+}
+
+extern "C" gsx_status gsx_mg_set_smoother(gsx_mg *mg, int32_t kind, double param)
+{
+    if (!mg || kind < GSX_SMOOTH_JACOBI || kind > GSX_SMOOTH_SCMS) return GSX_ERR_ARG;
+    mg->scms_saved = 0.0;
+    for (int l = 1; l < (int)mg->L.size(); ++l) GSX_TRY(mg_setup_level_smoother(mg, l, kind, param, mg->tau));
+    return GSX_OK;
+}
+
+extern "C" gsx_status gsx_mg_set_cycle(gsx_mg *mg, int32_t npre, int32_t npost, double cgc_damping)
+{
+    if (!mg || npre < 0 || npost < 0) return GSX_ERR_ARG;
+    mg->npre = npre;
+    mg->npost = npost;
+    mg->cgc = cgc_damping;
+    return GSX_OK;
+}
+
+extern "C" gsx_status gsx_mg_set_level_smoother(gsx_mg *mg, int32_t lvl, int32_t kind, double damping, double param)
+{
+    if (!mg || kind < GSX_SMOOTH_JACOBI || kind > GSX_SMOOTH_SCMS) return GSX_ERR_ARG;
+    if (lvl < 1 || lvl >= (int)mg->L.size()) return fail(mg->ctx, GSX_ERR_ARG, "smoother level out of range");
+    if (lvl == 1) mg->scms_saved = 0.0;
+    return mg_setup_level_smoother(mg, lvl, kind, param, damping);
 }
 
 extern "C" gsx_status gsx_mg_smoother_bound(const gsx_mg *mg, int32_t lvl, double *lmax)

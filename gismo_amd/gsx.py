@@ -123,6 +123,10 @@ SIGNATURES = {
     "gsx_mg_create": [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
                       C.c_int32, _pp],
     "gsx_mg_destroy": [_vp],
+    "gsx_mg_create_transfers": [_vp, C.c_int32, _i32p, C.POINTER(_i32p), C.POINTER(_i32p), C.POINTER(_dp),
+                                C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32, _pp],
+    "gsx_mg_set_level_smoother": [_vp, C.c_int32, C.c_int32, C.c_double, C.c_double],
+    "gsx_mg_set_cycle": [_vp, C.c_int32, C.c_int32, C.c_double],
     "gsx_fd_create": [_vp, C.c_double, C.c_double, _pp],
     "gsx_fd_destroy": [_vp],
     "gsx_scms_create": [_vp, C.c_double, C.c_double, C.c_double, _vp],
@@ -478,6 +482,45 @@ class Multigrid:
         self.nlevels = nlevels
         if smoother != "jacobi":
             _check(self.ctx.h, lib().gsx_mg_set_smoother(h, self.SMOOTHERS[smoother], smoother_param))
+
+    @classmethod
+    def from_transfers(cls, ctx, matrix, transfers, npre=1, npost=1, damping=1.0, cgc_damping=1.0,
+                       precision=FP64):
+        """gsMultiGridOp::make(fineMatrix, transferMatrices): transfers[l]
+        (scipy CSR, n_{l+1} x n_l) prolongates level l to level l + 1, the
+        last one to the matrix's level."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        nt = len(transfers)
+        keep = []
+        rps = (_i32p * max(1, nt))()
+        cis = (_i32p * max(1, nt))()
+        vas = (_dp * max(1, nt))()
+        nc = np.zeros(max(1, nt), np.int32)
+        for l, P in enumerate(transfers):
+            P = P.tocsr()
+            P.sort_indices()
+            rp = np.ascontiguousarray(P.indptr, np.int32)
+            ci = np.ascontiguousarray(P.indices, np.int32)
+            va = np.ascontiguousarray(P.data, np.float64)
+            keep += [rp, ci, va]
+            rps[l] = rp.ctypes.data_as(_i32p)
+            cis[l] = ci.ctypes.data_as(_i32p)
+            vas[l] = va.ctypes.data_as(_dp)
+            nc[l] = P.shape[1]
+        h = C.c_void_p()
+        _check(ctx.h, lib().gsx_mg_create_transfers(matrix.h, nt, nc.ctypes.data_as(_i32p), rps, cis, vas, npre,
+                                                    npost, 1, damping, cgc_damping, precision, C.byref(h)))
+        self.h = h
+        self.nlevels = nt + 1
+        return self
+
+    def set_level_smoother(self, lvl, smoother, damping=1.0, param=0.0):
+        """gsMultiGridOp::setSmoother(lvl, op): one level's smoother."""
+        _check(self.ctx.h, lib().gsx_mg_set_level_smoother(self.h, lvl, self.SMOOTHERS[smoother], damping, param))
+
+    def set_cycle(self, npre, npost, cgc_damping=1.0):
+        _check(self.ctx.h, lib().gsx_mg_set_cycle(self.h, npre, npost, cgc_damping))
 
     def lmax(self, lvl):
         v = C.c_double()

@@ -34,6 +34,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <iostream>
 #include <map>
 #include <memory>
 #include <sstream>
@@ -47,6 +48,9 @@ namespace gismo {
 typedef double real_t;
 typedef int index_t;
 typedef short short_t;
+
+/// the reference's info stream (src/gsCore/gsDebug.h)
+inline std::ostream &gsInfo = std::cout;
 
 // ---------------------------------------------------------------- errors
 namespace detail {
@@ -123,6 +127,12 @@ public:
         return std::sqrt(s);
     }
     static gsMatrix Zero(index_t r, index_t c) { return gsMatrix(r, c); }
+    /// uniform values in [-1, 1] from std::rand (Eigen's setRandom)
+    void setRandom(index_t r, index_t c)
+    {
+        resize(r, c);
+        for (T &v : m_data) v = T(2) * (T)std::rand() / (T)RAND_MAX - T(1);
+    }
 
 private:
     index_t m_rows = 0, m_cols = 0;
@@ -133,17 +143,26 @@ template <class T = real_t>
 using gsVector = gsMatrix<T>;
 
 // ---------------------------------------------------------------- gsSparseMatrix
-/// Column-major compressed sparse matrix with int indices (Eigen's
-/// compressed storage accessors).  Matrices produced by the assemblers are
-/// symmetric bitwise and keep a handle to their device copy, which the
-/// solvers use without a host round trip.
-template <class T = real_t>
+/// Storage orders of gsSparseMatrix (Eigen's ColMajor / RowMajor flags).
+enum { ColMajor = 0, RowMajor = 1 };
+
+/// Compressed sparse matrix with int indices (Eigen's compressed storage
+/// accessors), column-major by default, row-major with RowMajor (the
+/// multigrid transfers, gsSparseMatrix<T, RowMajor>).  Matrices produced by
+/// the assemblers are symmetric bitwise and keep a handle to their device
+/// copy, which the solvers use without a host round trip.
+template <class T = real_t, int _Options = ColMajor>
 class gsSparseMatrix {
 public:
+    struct Triplet {
+        index_t row, col;
+        T value;
+    };
     gsSparseMatrix() = default;
-    gsSparseMatrix(index_t r, index_t c) : m_rows(r), m_cols(c), m_outer(c + 1, 0) {}
+    gsSparseMatrix(index_t r, index_t c) : m_rows(r), m_cols(c), m_outer((_Options == RowMajor ? r : c) + 1, 0) {}
     index_t rows() const { return m_rows; }
     index_t cols() const { return m_cols; }
+    index_t outerSize() const { return _Options == RowMajor ? m_rows : m_cols; }
     index_t nonZeros() const { return (index_t)m_val.size(); }
     bool isCompressed() const { return true; }
     const index_t *outerIndexPtr() const { return m_outer.data(); }
@@ -151,12 +170,13 @@ public:
     const T *valuePtr() const { return m_val.data(); }
     T coeff(index_t i, index_t j) const
     {
-        auto b = m_inner.begin() + m_outer[j], e = m_inner.begin() + m_outer[j + 1];
-        auto it = std::lower_bound(b, e, i);
-        return (it != e && *it == i) ? m_val[it - m_inner.begin()] : T(0);
+        const index_t o = _Options == RowMajor ? i : j, in = _Options == RowMajor ? j : i;
+        auto b = m_inner.begin() + m_outer[o], e = m_inner.begin() + m_outer[o + 1];
+        auto it = std::lower_bound(b, e, in);
+        return (it != e && *it == in) ? m_val[it - m_inner.begin()] : T(0);
     }
     T at(index_t i, index_t j) const { return coeff(i, j); }
-    /// Fill from compressed columns (takes the arrays).
+    /// Fill from compressed outer vectors (takes the arrays).
     void setFromCompressed(index_t r, index_t c, std::vector<index_t> outer, std::vector<index_t> inner,
                            std::vector<T> val)
     {
@@ -167,13 +187,77 @@ public:
         m_val = std::move(val);
         m_dev.reset();
     }
+    /// setFromTriplets with take_first (the first entry of a duplicated
+    /// position is kept, src/gsCore/gsMultiBasis.hpp:175-176), or summing
+    /// duplicates when takeFirst is false (Eigen's default).
+    void setFromTriplets(index_t r, index_t c, const std::vector<Triplet> &t, bool takeFirst)
+    {
+        m_rows = r;
+        m_cols = c;
+        const index_t no = outerSize();
+        std::vector<std::vector<std::pair<index_t, T>>> v((size_t)no);
+        for (const Triplet &e : t) {
+            const index_t o = _Options == RowMajor ? e.row : e.col, in = _Options == RowMajor ? e.col : e.row;
+            auto &vec = v[(size_t)o];
+            auto it = std::find_if(vec.begin(), vec.end(), [&](const std::pair<index_t, T> &q) { return q.first == in; });
+            if (it == vec.end()) vec.push_back({in, e.value});
+            else if (!takeFirst) it->second += e.value;
+        }
+        m_outer.assign(no + 1, 0);
+        m_inner.clear();
+        m_val.clear();
+        for (index_t o = 0; o < no; ++o) {
+            auto &vec = v[(size_t)o];
+            std::sort(vec.begin(), vec.end(),
+                      [](const std::pair<index_t, T> &x, const std::pair<index_t, T> &y) { return x.first < y.first; });
+            for (auto &q : vec) {
+                m_inner.push_back(q.first);
+                m_val.push_back(q.second);
+            }
+            m_outer[o + 1] = (index_t)m_inner.size();
+        }
+        m_dev.reset();
+    }
     /// y = A x on the host
     gsMatrix<T> operator*(const gsMatrix<T> &x) const
     {
         gsMatrix<T> y(m_rows, 1);
-        for (index_t j = 0; j < m_cols; ++j)
-            for (index_t k = m_outer[j]; k < m_outer[j + 1]; ++k) y(m_inner[k]) += m_val[k] * x(j);
+        for (index_t o = 0; o < outerSize(); ++o)
+            for (index_t k = m_outer[o]; k < m_outer[o + 1]; ++k) {
+                if (_Options == RowMajor) y(o) += m_val[k] * x(m_inner[k]);
+                else y(m_inner[k]) += m_val[k] * x(o);
+            }
         return y;
+    }
+    /// the transpose: the same arrays read in the other storage order
+    gsSparseMatrix<T, 1 - _Options> transpose() const
+    {
+        gsSparseMatrix<T, 1 - _Options> t;
+        t.setFromCompressed(m_cols, m_rows, m_outer, m_inner, m_val);
+        return t;
+    }
+    /// row-major CSR arrays of this matrix (for a ColMajor matrix the
+    /// compressed columns are transposed)
+    void toCsr(std::vector<int32_t> &rp, std::vector<int32_t> &ci, std::vector<double> &v) const
+    {
+        if (_Options == RowMajor) {
+            rp.assign(m_outer.begin(), m_outer.end());
+            ci.assign(m_inner.begin(), m_inner.end());
+            v.assign(m_val.begin(), m_val.end());
+            return;
+        }
+        rp.assign(m_rows + 1, 0);
+        ci.resize(m_inner.size());
+        v.resize(m_val.size());
+        for (size_t k = 0; k < m_inner.size(); ++k) rp[m_inner[k] + 1]++;
+        for (index_t i = 0; i < m_rows; ++i) rp[i + 1] += rp[i];
+        std::vector<int32_t> pos(rp.begin(), rp.end() - 1);
+        for (index_t j = 0; j < m_cols; ++j)
+            for (index_t k = m_outer[j]; k < m_outer[j + 1]; ++k) {
+                const int r = m_inner[k];
+                ci[pos[r]] = j;
+                v[pos[r]++] = m_val[k];
+            }
     }
     // device twin (set by the assemblers)
     std::shared_ptr<gsx_matrix> devicePtr() const { return m_dev; }
@@ -189,6 +273,10 @@ private:
     std::shared_ptr<gsx_matrix> m_dev;
     std::shared_ptr<gsx_problem> m_prob;
 };
+
+/// give(x): move semantics of the reference's give() (src/gsCore/gsMemory.h)
+template <class X>
+X give(X &x) { return std::move(x); }
 
 // ---------------------------------------------------------------- options
 /// Typed key/value options (gsOptionList, src/gsIO/gsOptionList.h:32).
@@ -487,6 +575,28 @@ public:
             for (int j = 0; j < mp.patch(k).domainDim(); ++j) kv.push_back(mp.patch(k).knots(j));
             m_kv.push_back(kv);
         }
+        m_if = mp.interfaces();   // the box topology the reference's gsMultiBasis keeps
+    }
+    const std::vector<typename gsMultiPatch<T>::Interface> &interfaces() const { return m_if; }
+    /// remove every other interior knot of every direction (gsKnotVector::
+    /// coarsen(1, 1), src/gsNurbs/gsKnotVector.hpp:951-985: the distinct
+    /// interior knots number 1, 3, 5, ... are removed)
+    void uniformCoarsen(int numKnots = 1)
+    {
+        if (numKnots != 1) throw std::runtime_error("gsMultiBasis::uniformCoarsen: one knot per span only");
+        for (auto &p : m_kv)
+            for (auto &k : p) {
+                const std::vector<real_t> &U = k.knots();
+                std::vector<real_t> uniq;
+                for (real_t u : U)
+                    if (uniq.empty() || u != uniq.back()) uniq.push_back(u);
+                std::vector<real_t> removed;
+                for (size_t i = 1; i + 1 < uniq.size(); i += 2) removed.push_back(uniq[i]);
+                std::vector<real_t> out;
+                for (real_t u : U)
+                    if (!std::binary_search(removed.begin(), removed.end(), u)) out.push_back(u);
+                k = gsKnotVector(out, k.degree());
+            }
     }
     index_t nBases() const { return (index_t)m_kv.size(); }
     void degreeElevate(int r = 1)
@@ -539,6 +649,7 @@ public:
 
 private:
     std::vector<std::vector<gsKnotVector>> m_kv;
+    std::vector<typename gsMultiPatch<T>::Interface> m_if;
 };
 
 /// gsNurbsCreator subset (src/gsNurbs/gsNurbsCreator.hpp:655-745): degree-1
@@ -565,13 +676,14 @@ struct gsNurbsCreator {
         mp.computeTopology();
         return mp;
     }
+    /// cube of side r centred at (x, y, z) (src/gsNurbs/gsNurbsCreator.hpp:731-745)
     static std::unique_ptr<gsTensorBSplinePatch> BSplineCube(real_t r = 1, real_t x = 0, real_t y = 0, real_t z = 0)
     {
         gsMatrix<real_t> C(8, 3);
         for (int i = 0; i < 8; ++i) {
-            C(i, 0) = r * (i & 1) + x;
-            C(i, 1) = r * ((i >> 1) & 1) + y;
-            C(i, 2) = r * ((i >> 2) & 1) + z;
+            C(i, 0) = r * ((real_t)(i & 1) - 0.5) + x;
+            C(i, 1) = r * ((real_t)((i >> 1) & 1) - 0.5) + y;
+            C(i, 2) = r * ((real_t)((i >> 2) & 1) - 0.5) + z;
         }
         return std::unique_ptr<gsTensorBSplinePatch>(new gsTensorBSplinePatch(
             {gsKnotVector(0, 1, 0, 2), gsKnotVector(0, 1, 0, 2), gsKnotVector(0, 1, 0, 2)}, C));
@@ -1094,7 +1206,7 @@ public:
     gsPoissonAssembler(const gsMultiPatch<T> &mp, const gsMultiBasis<T> &mb, const gsBoundaryConditions<T> &bc,
                        const gsFunctionExpr<T> &f, dirichlet::strategy ds = dirichlet::elimination,
                        iFace::strategy = iFace::glue)
-        : m_mp(&mp), m_mb(&mb), m_bc(&bc), m_f(f)
+        : m_mp(&mp), m_mb(mb), m_bc(&bc), m_f(f)
     {
         if (ds != dirichlet::elimination)
             throw std::runtime_error("gsPoissonAssembler: only Dirichlet elimination is supported");
@@ -1103,7 +1215,7 @@ public:
     virtual ~gsPoissonAssembler() = default;
     virtual void assemble()
     {
-        m_prob = detail::makeProblem(*m_mp, *m_mb, *m_bc, m_f.kind(), nullptr, nullptr);
+        m_prob = detail::makeProblem(*m_mp, m_mb, *m_bc, m_f.kind(), nullptr, nullptr);
         if (m_bc->inhomogeneous()) {
             // gsAssembler::computeDirichletDofs (src/gsAssembler/gsAssembler.hpp:231-297):
             // DirichletValues 101 (interpolation, the default), 102 or 100
@@ -1125,16 +1237,258 @@ public:
     const gsMatrix<T> &rhs() const { return m_rhs; }
     index_t numDofs() const { return m_matrix.rows(); }
     std::shared_ptr<gsx_problem> problem() const { return m_prob; }
+    const gsMultiPatch<T> &patches() const { return *m_mp; }
+    /// gsAssembler::constructSolution (src/gsAssembler/gsAssembler.hpp:
+    /// 537-581): one scalar patch per patch over the discretisation basis;
+    /// the coefficient of a free function is its entry of solVector, of an
+    /// eliminated one its fixed (Dirichlet) value.
+    void constructSolution(const gsMatrix<T> &solVector, gsMultiPatch<T> &result) const
+    {
+        if (!m_prob) throw std::runtime_error("gsPoissonAssembler::constructSolution: nothing assembled yet");
+        int64_t nbasis = 0, nfree = 0, nelem = 0, nglued = 0;
+        detail::check(gsx_problem_info(m_prob.get(), &nbasis, &nfree, &nelem, &nglued), "gsx_problem_info");
+        if (solVector.rows() != nfree)
+            throw std::runtime_error("gsPoissonAssembler::constructSolution: solution size does not match the space");
+        std::vector<int32_t> dofs((size_t)nbasis);
+        detail::check(gsx_problem_dofs(m_prob.get(), dofs.data()), "gsx_problem_dofs");
+        std::vector<double> fixed((size_t)std::max<int64_t>(1, nglued - nfree), 0.0);
+        if (nglued > nfree)
+            detail::check(gsx_problem_fixed_values(m_prob.get(), fixed.data(), (int32_t)(nglued - nfree)),
+                          "gsx_problem_fixed_values");
+        result = gsMultiPatch<T>();
+        int64_t off = 0;
+        for (index_t k = 0; k < m_mb.nBases(); ++k) {
+            const index_t nb = m_mb.size(k);
+            std::vector<gsKnotVector> kv;
+            for (int j = 0; j < m_mb.basis(k).dim(); ++j) kv.push_back(m_mb.knots(k, j));
+            gsMatrix<real_t> C(nb, 1);
+            for (index_t i = 0; i < nb; ++i) {
+                const int32_t g = dofs[(size_t)(off + i)];
+                C(i, 0) = g < nfree ? solVector(g) : fixed[(size_t)(g - nfree)];
+            }
+            result.addPatch(gsTensorBSplinePatch(kv, C));
+            off += nb;
+        }
+    }
 
 private:
     const gsMultiPatch<T> *m_mp;
-    const gsMultiBasis<T> *m_mb;
+    gsMultiBasis<T> m_mb;   // a copy, as gsAssembler keeps its bases
     const gsBoundaryConditions<T> *m_bc;
     gsFunctionExpr<T> m_f;
     gsOptionList m_opt;
     std::shared_ptr<gsx_problem> m_prob;
     gsSparseMatrix<T> m_matrix;
     gsMatrix<T> m_rhs;
+};
+
+// ---------------------------------------------------------------- grid hierarchy
+namespace detail {
+
+/// Rows of the 1D knot-insertion transfer (coarse -> fine coefficients,
+/// nf x nc): each knot of `fine` missing from `coarse` is inserted once,
+/// new coefficients Q_i = a_i P_i + (1 - a_i) P_{i-1}, a_i = (u - U_i) /
+/// (U_{i+p} - U_i) for the p functions touching the span (Boehm's
+/// insertion, the transfer of gsBSplineBasis::refine_withTransfer).
+inline std::vector<std::vector<double>> insertionRows(const std::vector<real_t> &coarse,
+                                                      const std::vector<real_t> &fine, int p)
+{
+    std::vector<real_t> U = coarse, ins;
+    size_t a = 0;
+    for (real_t u : fine) {
+        if (a < coarse.size() && coarse[a] == u) ++a;
+        else ins.push_back(u);
+    }
+    if (a != coarse.size()) throw std::runtime_error("gsGridHierarchy: the coarse knots are not nested in the fine ones");
+    const int nc = (int)coarse.size() - p - 1;
+    std::vector<std::vector<double>> R((size_t)nc, std::vector<double>((size_t)nc, 0.0));
+    for (int i = 0; i < nc; ++i) R[(size_t)i][(size_t)i] = 1.0;
+    for (real_t u : ins) {
+        int k = p;
+        while (k + 1 < (int)U.size() - p - 1 && U[(size_t)k + 1] <= u) ++k;
+        std::vector<std::vector<double>> out;
+        out.reserve(R.size() + 1);
+        for (int i = 0; i <= k - p; ++i) out.push_back(R[(size_t)i]);
+        for (int i = k - p + 1; i <= k; ++i) {
+            const double al = (u - U[(size_t)i]) / (U[(size_t)i + p] - U[(size_t)i]);
+            std::vector<double> row((size_t)nc);
+            for (int c = 0; c < nc; ++c) row[(size_t)c] = al * R[(size_t)i][(size_t)c] + (1.0 - al) * R[(size_t)i - 1][(size_t)c];
+            out.push_back(std::move(row));
+        }
+        for (size_t i = (size_t)k; i < R.size(); ++i) out.push_back(R[i]);
+        R.swap(out);
+        U.insert(U.begin() + k + 1, u);
+    }
+    return R;
+}
+
+/// The DOF map of a multibasis (gsMultiBasis::getMapper with Dirichlet
+/// elimination and glued interfaces, src/gsCore/gsMultiBasis.hpp:373-424).
+template <class T>
+inline std::vector<int32_t> multiBasisDofs(const gsMultiBasis<T> &mb, const gsBoundaryConditions<T> &bc, int64_t &nfree)
+{
+    const int np = mb.nBases();
+    const int d = np ? (int)mb.basis(0).dim() : 0;
+    std::vector<int32_t> nb((size_t)np * 3, 1), ds, ifs;
+    int64_t tot = 0;
+    for (int k = 0; k < np; ++k) {
+        for (int j = 0; j < d; ++j) nb[(size_t)3 * k + j] = mb.knots(k, j).numBasis();
+        tot += mb.size(k);
+    }
+    for (auto &sd : bc.dirichletSides()) { ds.push_back(sd.first); ds.push_back(sd.second); }
+    for (auto &f : mb.interfaces()) {
+        const int32_t r[8] = {f.p1, f.s1, f.p2, f.s2, f.m0, f.f0, f.m1, f.f1};
+        ifs.insert(ifs.end(), r, r + 8);
+    }
+    std::vector<int32_t> dofs((size_t)tot);
+    int64_t nglued = 0;
+    check(gsx_mapper_build(d, np, nb.data(), (int32_t)ds.size() / 2, ds.empty() ? nullptr : ds.data(),
+                           (int32_t)ifs.size() / 8, ifs.empty() ? nullptr : ifs.data(), dofs.data(), &nfree, &nglued),
+          "gsx_mapper_build");
+    return dofs;
+}
+
+/// The transfer between two nested multibases restricted to free DOFs
+/// (gsMultiBasis::combineTransferMatrices, src/gsCore/gsMultiBasis.hpp:
+/// 137-178): per patch the tensor product of the 1D insertion transfers,
+/// rows in local order, entries whose fine and coarse DOFs are both free,
+/// the first of duplicated positions kept (take_first).
+template <class T>
+inline gsSparseMatrix<T, RowMajor> combinedTransfer(const gsMultiBasis<T> &coarse, const gsMultiBasis<T> &fine,
+                                                    const gsBoundaryConditions<T> &bc)
+{
+    int64_t nfc = 0, nff = 0;
+    const std::vector<int32_t> dc = multiBasisDofs(coarse, bc, nfc), df = multiBasisDofs(fine, bc, nff);
+    std::vector<typename gsSparseMatrix<T, RowMajor>::Triplet> trip;
+    int64_t offc = 0, offf = 0;
+    for (int k = 0; k < fine.nBases(); ++k) {
+        const int d = fine.basis(k).dim();
+        std::vector<std::vector<std::vector<double>>> T1(3);
+        int nf[3] = {1, 1, 1}, nc[3] = {1, 1, 1};
+        for (int j = 0; j < 3; ++j) {
+            if (j < d) {
+                T1[j] = insertionRows(coarse.knots(k, j).knots(), fine.knots(k, j).knots(), fine.knots(k, j).degree());
+                nf[j] = (int)T1[j].size();
+                nc[j] = coarse.knots(k, j).numBasis();
+            } else {
+                T1[j] = {{1.0}};
+            }
+        }
+        for (int iz = 0; iz < nf[2]; ++iz)
+            for (int iy = 0; iy < nf[1]; ++iy)
+                for (int ix = 0; ix < nf[0]; ++ix) {
+                    const int64_t fi = ix + (int64_t)nf[0] * (iy + (int64_t)nf[1] * iz);
+                    const int fdof = df[(size_t)(offf + fi)];
+                    if (fdof >= nff) continue;
+                    for (int cz = 0; cz < nc[2]; ++cz) {
+                        const double vz = T1[2][(size_t)iz][(size_t)cz];
+                        if (vz == 0.0) continue;
+                        for (int cy = 0; cy < nc[1]; ++cy) {
+                            const double vy = T1[1][(size_t)iy][(size_t)cy];
+                            if (vy == 0.0) continue;
+                            for (int cx = 0; cx < nc[0]; ++cx) {
+                                const double vx = T1[0][(size_t)ix][(size_t)cx];
+                                if (vx == 0.0) continue;
+                                const int64_t ci = cx + (int64_t)nc[0] * (cy + (int64_t)nc[1] * cz);
+                                const int cdof = dc[(size_t)(offc + ci)];
+                                if (cdof >= nfc) continue;
+                                trip.push_back({fdof, cdof, (T)(vx * vy * vz)});
+                            }
+                        }
+                    }
+                }
+        offc += coarse.size(k);
+        offf += fine.size(k);
+    }
+    gsSparseMatrix<T, RowMajor> P;
+    P.setFromTriplets((index_t)nff, (index_t)nfc, trip, true);
+    return P;
+}
+
+}  // namespace detail
+
+/// gsGridHierarchy (src/gsMultiGrid/gsGridHierarchy.h:35-200,
+/// gsGridHierarchy.hpp:27-101): the multibases of a multigrid hierarchy,
+/// coarsest first, and the transfers between neighbouring levels (restricted
+/// to free DOFs; Dirichlet elimination and glued interfaces).
+template <class T = real_t>
+class gsGridHierarchy {
+public:
+    static gsGridHierarchy buildByCoarsening(gsMultiBasis<T> mBasis, const gsBoundaryConditions<T> &bc,
+                                             const gsOptionList &options)
+    {
+        return buildByCoarsening(std::move(mBasis), bc, options, options.askInt("Levels", 3),
+                                 options.askInt("DegreesOfFreedom", 0), 0);
+    }
+    static gsGridHierarchy buildByCoarsening(gsMultiBasis<T> mBasis, const gsBoundaryConditions<T> &bc,
+                                             const gsOptionList &options, index_t levels, index_t degreesOfFreedom = 0,
+                                             index_t = 0)
+    {
+        checkStrategies(options);
+        gsGridHierarchy r;
+        r.m_mBases.push_back(std::move(mBasis));
+        index_t lastSize = r.m_mBases[0].totalSize();
+        for (index_t i = 0; i < levels - 1 && lastSize > degreesOfFreedom; ++i) {
+            gsMultiBasis<T> coarse = r.m_mBases[(size_t)i];
+            coarse.uniformCoarsen(1);
+            const index_t newSize = coarse.totalSize();
+            if (lastSize <= newSize && degreesOfFreedom > 0) break;
+            lastSize = newSize;
+            r.m_transfers.push_back(detail::combinedTransfer(coarse, r.m_mBases[(size_t)i], bc));
+            r.m_mBases.push_back(std::move(coarse));
+        }
+        std::reverse(r.m_mBases.begin(), r.m_mBases.end());
+        std::reverse(r.m_transfers.begin(), r.m_transfers.end());
+        return r;
+    }
+    static gsGridHierarchy buildByRefinement(gsMultiBasis<T> mBasis, const gsBoundaryConditions<T> &bc,
+                                             const gsOptionList &options)
+    {
+        return buildByRefinement(std::move(mBasis), bc, options, options.askInt("Levels", 3),
+                                 options.askInt("NumberOfKnotsToBeInserted", 1),
+                                 options.askInt("MultiplicityOfKnotsToBeInserted", 1), 0);
+    }
+    static gsGridHierarchy buildByRefinement(gsMultiBasis<T> mBasis, const gsBoundaryConditions<T> &bc,
+                                             const gsOptionList &options, index_t levels,
+                                             index_t numberOfKnotsToBeInserted = 1,
+                                             index_t multiplicityOfKnotsToBeInserted = 1, index_t = 0)
+    {
+        checkStrategies(options);
+        if (multiplicityOfKnotsToBeInserted != 1)
+            throw std::runtime_error("gsGridHierarchy::buildByRefinement: knot multiplicity 1 only");
+        gsGridHierarchy r;
+        r.m_mBases.push_back(std::move(mBasis));
+        for (index_t i = 1; i < levels; ++i) {
+            gsMultiBasis<T> fine = r.m_mBases.back();
+            fine.uniformRefine((int)numberOfKnotsToBeInserted);
+            r.m_transfers.push_back(detail::combinedTransfer(r.m_mBases.back(), fine, bc));
+            r.m_mBases.push_back(std::move(fine));
+        }
+        return r;
+    }
+    gsGridHierarchy &moveMultiBasesTo(std::vector<gsMultiBasis<T>> &out)
+    {
+        out = std::move(m_mBases);
+        return *this;
+    }
+    gsGridHierarchy &moveTransferMatricesTo(std::vector<gsSparseMatrix<T, RowMajor>> &out)
+    {
+        out = std::move(m_transfers);
+        return *this;
+    }
+    const std::vector<gsMultiBasis<T>> &getMultiBases() const { return m_mBases; }
+    const std::vector<gsSparseMatrix<T, RowMajor>> &getTransferMatrices() const { return m_transfers; }
+
+private:
+    static void checkStrategies(const gsOptionList &o)
+    {
+        if (o.askInt("DirichletStrategy", dirichlet::elimination) != dirichlet::elimination)
+            throw std::runtime_error("gsGridHierarchy: only Dirichlet elimination is supported");
+        if (o.askInt("InterfaceStrategy", iFace::glue) != iFace::glue)
+            throw std::runtime_error("gsGridHierarchy: only glued (conforming) interfaces are supported");
+    }
+    std::vector<gsMultiBasis<T>> m_mBases;
+    std::vector<gsSparseMatrix<T, RowMajor>> m_transfers;
 };
 
 // ---------------------------------------------------------------- operators / solvers
@@ -1147,26 +1501,18 @@ public:
     virtual void apply(const gsMatrix<T> &input, gsMatrix<T> &x) const = 0;
     virtual index_t rows() const = 0;
     virtual index_t cols() const = 0;
+    virtual void setOptions(const gsOptionList &) {}
 };
 
 namespace detail {
 inline std::shared_ptr<gsx_matrix> deviceMatrix(const gsSparseMatrix<real_t> &M, bool fp32)
 {
     if (M.devicePtr()) return M.devicePtr();
-    // column-major arrays -> row-major arrays
     const index_t n = M.rows();
     if (M.cols() != n) throw std::runtime_error("gismo (libgsx): square matrices only");
-    std::vector<int32_t> rp(n + 1, 0), ci(M.nonZeros());
-    std::vector<double> v(M.nonZeros());
-    for (index_t k = 0; k < M.nonZeros(); ++k) rp[M.innerIndexPtr()[k] + 1]++;
-    for (index_t i = 0; i < n; ++i) rp[i + 1] += rp[i];
-    std::vector<int32_t> pos(rp.begin(), rp.end() - 1);
-    for (index_t j = 0; j < n; ++j)
-        for (index_t k = M.outerIndexPtr()[j]; k < M.outerIndexPtr()[j + 1]; ++k) {
-            const int r = M.innerIndexPtr()[k];
-            ci[pos[r]] = j;
-            v[pos[r]++] = M.valuePtr()[k];
-        }
+    std::vector<int32_t> rp, ci;
+    std::vector<double> v;
+    M.toCsr(rp, ci, v);
     gsx_matrix *A = nullptr;
     check(gsx_matrix_from_csr(ctx(), n, rp.data(), ci.data(), v.data(), fp32 ? 1 : 0, &A), "gsx_matrix_from_csr");
     return std::shared_ptr<gsx_matrix>(A, MatrixDeleter());
@@ -1206,6 +1552,14 @@ class gsPreconditionerOp : public gsLinearOperator<T> {
 public:
     typedef std::shared_ptr<gsPreconditionerOp> Ptr;
     virtual int kind() const = 0;
+    /// as a multigrid smoother: GSX_SMOOTH_* and its damping (-1: not a smoother)
+    virtual int smootherKind() const { return -1; }
+    virtual T smootherDamping() const { return T(1); }
+    /// sweeps per step (gsPreconditionerOp::setNumOfSweeps); one is provided
+    void setNumOfSweeps(index_t n)
+    {
+        if (n != 1) throw std::runtime_error("gsPreconditionerOp: one sweep per step is provided");
+    }
 };
 
 /// gsJacobiOp (src/gsSolver/gsSimplePreconditioners.h:146-244)
@@ -1221,9 +1575,11 @@ public:
     index_t rows() const override { return m_M->rows(); }
     index_t cols() const override { return m_M->cols(); }
     int kind() const override { return GSX_PREC_JACOBI; }
+    int smootherKind() const override { return GSX_SMOOTH_JACOBI; }
+    T smootherDamping() const override { return m_tau; }
     void setDamping(T tau) { m_tau = tau; }
     T damping() const { return m_tau; }
-    void setOptions(const gsOptionList &o) { m_tau = o.askReal("Damping", m_tau); }
+    void setOptions(const gsOptionList &o) override { m_tau = o.askReal("Damping", m_tau); }
 
 private:
     const gsSparseMatrix<T> *m_M;
@@ -1236,19 +1592,170 @@ std::shared_ptr<gsJacobiOp<T>> makeJacobiOp(const gsSparseMatrix<T> &M, T tau = 
     return std::make_shared<gsJacobiOp<T>>(M, tau);
 }
 
-/// gsMultiGridOp (src/gsMultiGrid/gsMultiGrid.h): built here from the
-/// assembled problem by coarsening its knot vectors (the transfers and
-/// Galerkin products are formed on the device).  Options: NumPreSmooth,
-/// NumPostSmooth, CoarseGridCorrectionDamping, Damping (of the Jacobi or
-/// Richardson smoother), Smoother ("Jacobi"/"j", "GaussSeidel"/"gs", "Richardson"/"r",
-/// "Chebyshev"/"c", "SubspaceCorrectedMassSmoother"/"scms": one patch with
-/// Dirichlet sides, damped per level by its power iteration, Damping not
-/// applied), ChebyshevRatio, Scaling (sigma of the subspace corrected mass
-/// smoother, 0.12), Precision ("fp64" or "mixed": FP32 cycle).
+/// gsRichardsonOp (src/gsSolver/gsSimplePreconditioners.h:33-131):
+/// x = tau r; as a smoother x += tau (b - A x).
+template <class T = real_t>
+class gsRichardsonOp : public gsPreconditionerOp<T> {
+public:
+    explicit gsRichardsonOp(const gsSparseMatrix<T> &M, T tau = 1) : m_M(&M), m_tau(tau) {}
+    void apply(const gsMatrix<T> &input, gsMatrix<T> &x) const override
+    {
+        x.resize(input.rows(), 1);
+        for (index_t i = 0; i < input.rows(); ++i) x(i) = m_tau * input(i);
+    }
+    index_t rows() const override { return m_M->rows(); }
+    index_t cols() const override { return m_M->cols(); }
+    int kind() const override { return GSX_PREC_NONE; }
+    int smootherKind() const override { return GSX_SMOOTH_RICHARDSON; }
+    T smootherDamping() const override { return m_tau; }
+    void setOptions(const gsOptionList &o) override { m_tau = o.askReal("Damping", m_tau); }
+
+private:
+    const gsSparseMatrix<T> *m_M;
+    T m_tau;
+};
+
+template <class T>
+std::shared_ptr<gsRichardsonOp<T>> makeRichardsonOp(const gsSparseMatrix<T> &M, T tau = 1)
+{
+    return std::make_shared<gsRichardsonOp<T>>(M, tau);
+}
+
+/// gsGaussSeidelOp (src/gsSolver/gsSimplePreconditioners.h:278-395): a
+/// multigrid smoother here (forward sweeps before, reverse sweeps after the
+/// coarse correction); apply is one forward sweep from zero on the host.
+template <class T = real_t>
+class gsGaussSeidelOp : public gsPreconditionerOp<T> {
+public:
+    explicit gsGaussSeidelOp(const gsSparseMatrix<T> &M) : m_M(&M) {}
+    void apply(const gsMatrix<T> &input, gsMatrix<T> &x) const override
+    {
+        // column-major storage: solve (D + L) x = r column by column
+        const index_t n = m_M->rows();
+        x.resize(n, 1);
+        for (index_t i = 0; i < n; ++i) x(i) = input(i);
+        for (index_t j = 0; j < n; ++j) {
+            x(j) /= m_M->coeff(j, j);
+            for (index_t k = m_M->outerIndexPtr()[j]; k < m_M->outerIndexPtr()[j + 1]; ++k) {
+                const index_t r = m_M->innerIndexPtr()[k];
+                if (r > j) x(r) -= m_M->valuePtr()[k] * x(j);
+            }
+        }
+    }
+    index_t rows() const override { return m_M->rows(); }
+    index_t cols() const override { return m_M->cols(); }
+    int kind() const override { return GSX_PREC_NONE; }
+    int smootherKind() const override { return GSX_SMOOTH_GAUSS_SEIDEL; }
+
+private:
+    const gsSparseMatrix<T> *m_M;
+};
+
+template <class T>
+std::shared_ptr<gsGaussSeidelOp<T>> makeGaussSeidelOp(const gsSparseMatrix<T> &M)
+{
+    return std::make_shared<gsGaussSeidelOp<T>>(M);
+}
+
+/// Exact solvers of gsMatrixOp.h (makeSparseCholeskySolver /
+/// makeSparseLUSolver, src/gsSolver/gsMatrixOp.h:306-370): apply solves with
+/// a dense FP64 factorisation on the host.  As the coarse solver of a
+/// gsMultiGridOp they select its exact coarsest-level solve, which runs on
+/// the device (the dense FP64 inverse of the coarsest operator).
+template <class T = real_t>
+class gsExactSolverOp : public gsLinearOperator<T> {
+public:
+    gsExactSolverOp(const gsSparseMatrix<T> &M, bool cholesky) : m_n(M.rows()), m_chol(cholesky)
+    {
+        const index_t n = m_n;
+        m_F.assign((size_t)n * n, 0.0);
+        for (index_t j = 0; j < M.cols(); ++j)
+            for (index_t k = M.outerIndexPtr()[j]; k < M.outerIndexPtr()[j + 1]; ++k)
+                m_F[(size_t)M.innerIndexPtr()[k] * n + j] = M.valuePtr()[k];
+        m_piv.resize((size_t)n);
+        for (index_t c = 0; c < n; ++c) {
+            index_t pr = c;
+            if (!m_chol)
+                for (index_t r = c + 1; r < n; ++r)
+                    if (std::fabs(m_F[(size_t)r * n + c]) > std::fabs(m_F[(size_t)pr * n + c])) pr = r;
+            if (m_F[(size_t)pr * n + c] == 0.0 || (m_chol && m_F[(size_t)c * n + c] <= 0.0))
+                throw std::runtime_error(m_chol ? "makeSparseCholeskySolver: matrix not positive definite"
+                                                : "makeSparseLUSolver: singular matrix");
+            m_piv[(size_t)c] = pr;
+            if (pr != c)
+                for (index_t q = 0; q < n; ++q) std::swap(m_F[(size_t)c * n + q], m_F[(size_t)pr * n + q]);
+            for (index_t r = c + 1; r < n; ++r) {
+                const double l = m_F[(size_t)r * n + c] / m_F[(size_t)c * n + c];
+                m_F[(size_t)r * n + c] = l;
+                if (l != 0.0)
+                    for (index_t q = c + 1; q < n; ++q) m_F[(size_t)r * n + q] -= l * m_F[(size_t)c * n + q];
+            }
+        }
+    }
+    void apply(const gsMatrix<T> &input, gsMatrix<T> &x) const override
+    {
+        const index_t n = m_n;
+        x.resize(n, 1);
+        for (index_t i = 0; i < n; ++i) x(i) = input(i);
+        for (index_t c = 0; c < n; ++c)
+            if (m_piv[(size_t)c] != c) std::swap(x(c), x(m_piv[(size_t)c]));
+        for (index_t r = 0; r < n; ++r)
+            for (index_t c = 0; c < r; ++c) x(r) -= m_F[(size_t)r * n + c] * x(c);
+        for (index_t r = n - 1; r >= 0; --r) {
+            for (index_t c = r + 1; c < n; ++c) x(r) -= m_F[(size_t)r * n + c] * x(c);
+            x(r) /= m_F[(size_t)r * n + r];
+        }
+    }
+    index_t rows() const override { return m_n; }
+    index_t cols() const override { return m_n; }
+
+private:
+    index_t m_n;
+    bool m_chol;
+    std::vector<double> m_F;
+    std::vector<index_t> m_piv;
+};
+
+template <class T>
+typename gsLinearOperator<T>::Ptr makeSparseCholeskySolver(const gsSparseMatrix<T> &M)
+{
+    return std::make_shared<gsExactSolverOp<T>>(M, true);
+}
+template <class T>
+typename gsLinearOperator<T>::Ptr makeSparseLUSolver(const gsSparseMatrix<T> &M)
+{
+    return std::make_shared<gsExactSolverOp<T>>(M, false);
+}
+
+/// gsMultiGridOp (src/gsMultiGrid/gsMultiGrid.h:57-395) on the device.
+/// Two ways to build it:
+///  - make(fineMatrix, transferMatrices[, coarseSolver]), the reference's
+///    (gsMultiGrid.h:136-141): the caller's transfers (gsGridHierarchy) are
+///    uploaded and the Galerkin products P^T A P formed on the device;
+///  - make(fineMatrix, numLevels[, opt]): the hierarchy from coarsening the
+///    knot vectors of the problem the matrix was assembled on (transfers
+///    and Galerkin products built on the device, from the kept element
+///    matrices when they fit).
+/// Per level, setSmoother(i, makeJacobiOp / makeRichardsonOp /
+/// makeGaussSeidelOp (mg->matrix(i))) selects that level's smoother;
+/// without it the Smoother option applies to every level.  Options
+/// (setOptions): NumPreSmooth, NumPostSmooth, CoarseGridCorrectionDamping,
+/// Damping (default Jacobi/Richardson damping), Smoother ("Jacobi"/"j",
+/// "GaussSeidel"/"gs", "Richardson"/"r", "Chebyshev"/"c",
+/// "SubspaceCorrectedMassSmoother"/"scms": one patch with Dirichlet sides,
+/// damped per level by its power iteration), ChebyshevRatio, Scaling (sigma
+/// of the scms smoother, 0.12), Precision ("fp64" or "mixed": FP32 cycle).
+/// The coarsest level is always solved exactly (dense FP64 inverse on the
+/// device); setCoarseSolver accepts makeSparseCholeskySolver /
+/// makeSparseLUSolver.
 template <class T = real_t>
 class gsMultiGridOp : public gsPreconditionerOp<T> {
 public:
     typedef std::shared_ptr<gsMultiGridOp> Ptr;
+    typedef std::shared_ptr<gsMultiGridOp> uPtr;
+    typedef typename gsLinearOperator<T>::Ptr OpPtr;
+    typedef typename gsPreconditionerOp<T>::Ptr PrecondPtr;
+
     static Ptr make(const gsSparseMatrix<T> &fine, index_t numLevels, const gsOptionList &opt = gsOptionList())
     {
         Ptr p(new gsMultiGridOp());
@@ -1259,9 +1766,88 @@ public:
             throw std::runtime_error("gsMultiGridOp: the matrix must come from gsExprAssembler/gsPoissonAssembler");
         p->m_levels = numLevels;
         p->m_opt = opt;
+        p->m_smoothers.assign((size_t)numLevels, nullptr);
         return p;
     }
-    void setOptions(const gsOptionList &o) { m_opt = o; m_mg.reset(); }
+    static Ptr make(const gsSparseMatrix<T> &fine, std::vector<gsSparseMatrix<T, RowMajor>> transferMatrices,
+                    OpPtr coarseSolver = OpPtr())
+    {
+        Ptr p(new gsMultiGridOp());
+        p->m_n = fine.rows();
+        p->m_A = detail::deviceMatrix(fine, false);
+        p->m_transfers = std::move(transferMatrices);
+        p->m_byTransfers = true;
+        p->m_levels = (index_t)p->m_transfers.size() + 1;
+        for (size_t l = 0; l < p->m_transfers.size(); ++l) {
+            const index_t nf = l + 1 < p->m_transfers.size() ? p->m_transfers[l + 1].cols() : p->m_n;
+            if (p->m_transfers[l].rows() != nf)
+                throw std::runtime_error("gsMultiGridOp: transfer matrix sizes do not fit the fine matrix");
+        }
+        p->m_smoothers.assign((size_t)p->m_levels, nullptr);
+        if (coarseSolver) p->setCoarseSolver(coarseSolver);
+        return p;
+    }
+    void setOptions(const gsOptionList &o) override
+    {
+        const bool precChanged = o.askString("Precision", "fp64") != m_opt.askString("Precision", "fp64");
+        m_opt = o;
+        if (!m_mg) return;
+        if (precChanged || !m_byTransfers) {
+            m_mg.reset();   // rebuilt with the new options on the next use
+            return;
+        }
+        detail::check(gsx_mg_set_cycle(m_mg.get(), m_opt.askInt("NumPreSmooth", 1), m_opt.askInt("NumPostSmooth", 1),
+                                       m_opt.askReal("CoarseGridCorrectionDamping", 1.0)),
+                      "gsx_mg_set_cycle");
+        applySmoothers(m_mg.get());
+    }
+    index_t numLevels() const { return m_levels; }
+    /// the operator of level lvl (0 = coarsest), a host copy of the device's
+    const gsSparseMatrix<T> &matrix(index_t lvl) const
+    {
+        if (lvl < 0 || lvl >= m_levels) throw std::runtime_error("gsMultiGridOp::matrix: level out of range");
+        if (m_mats.empty()) m_mats.resize((size_t)m_levels);
+        std::shared_ptr<gsSparseMatrix<T>> &M = m_mats[(size_t)lvl];
+        if (!M) {
+            gsx_mg *h = device();
+            int64_t n = 0, nnz = 0, pn = 0;
+            detail::check(gsx_mg_level_info(h, (int32_t)lvl, &n, &nnz, &pn), "gsx_mg_level_info");
+            std::vector<int32_t> rp((size_t)n + 1), ci((size_t)nnz);
+            std::vector<double> v((size_t)nnz);
+            detail::check(gsx_mg_level_download(h, (int32_t)lvl, rp.data(), ci.data(), v.data(), nullptr, nullptr,
+                                                nullptr),
+                          "gsx_mg_level_download");
+            gsSparseMatrix<T, RowMajor> R;
+            R.setFromCompressed((index_t)n, (index_t)n, std::vector<index_t>(rp.begin(), rp.end()),
+                                std::vector<index_t>(ci.begin(), ci.end()), std::move(v));
+            // row-major -> column-major storage
+            std::vector<typename gsSparseMatrix<T>::Triplet> t;
+            t.reserve((size_t)nnz);
+            for (index_t i = 0; i < (index_t)n; ++i)
+                for (index_t k = R.outerIndexPtr()[i]; k < R.outerIndexPtr()[i + 1]; ++k)
+                    t.push_back({i, R.innerIndexPtr()[k], R.valuePtr()[k]});
+            M = std::make_shared<gsSparseMatrix<T>>();
+            M->setFromTriplets((index_t)n, (index_t)n, t, false);
+        }
+        return *M;
+    }
+    OpPtr underlyingOp(index_t lvl) const { return makeMatrixOp(matrix(lvl)); }
+    void setSmoother(index_t lvl, const PrecondPtr &sm)
+    {
+        if (lvl < 1 || lvl >= m_levels) throw std::runtime_error("gsMultiGridOp::setSmoother: level out of range");
+        if (!sm || sm->smootherKind() < 0)
+            throw std::runtime_error("gsMultiGridOp::setSmoother: Jacobi, Richardson or Gauss-Seidel smoothers only");
+        m_smoothers[(size_t)lvl] = sm;
+        if (m_mg) applySmoothers(m_mg.get());
+    }
+    PrecondPtr smoother(index_t lvl) const { return m_smoothers.at((size_t)lvl); }
+    void setCoarseSolver(const OpPtr &op)
+    {
+        if (!std::dynamic_pointer_cast<gsExactSolverOp<T>>(op))
+            throw std::runtime_error("gsMultiGridOp::setCoarseSolver: exact solvers (makeSparseCholeskySolver, "
+                                     "makeSparseLUSolver) only");
+        m_coarse = op;
+    }
     void apply(const gsMatrix<T> &input, gsMatrix<T> &x) const override
     {
         detail::DeviceVec a(sizeof(double) * m_n), b(sizeof(double) * m_n);
@@ -1278,10 +1864,30 @@ public:
         if (!m_mg) {
             gsx_mg *h = nullptr;
             const int prec = m_opt.askString("Precision", "fp64") == "mixed" ? GSX_MIXED : GSX_FP64;
-            detail::check(gsx_mg_create(m_prob.get(), m_A.get(), m_levels, m_opt.askInt("NumPreSmooth", 1),
-                                        m_opt.askInt("NumPostSmooth", 1), 1, m_opt.askReal("Damping", 1.0),
-                                        m_opt.askReal("CoarseGridCorrectionDamping", 1.0), prec, &h),
-                          "gsx_mg_create");
+            const int npre = m_opt.askInt("NumPreSmooth", 1), npost = m_opt.askInt("NumPostSmooth", 1);
+            const double tau = m_opt.askReal("Damping", 1.0), cgc = m_opt.askReal("CoarseGridCorrectionDamping", 1.0);
+            if (m_byTransfers) {
+                const int nt = (int)m_transfers.size();
+                std::vector<std::vector<int32_t>> rp(nt), ci(nt);
+                std::vector<std::vector<double>> v(nt);
+                std::vector<const int32_t *> prp(nt), pci(nt);
+                std::vector<const double *> pv(nt);
+                std::vector<int32_t> nc(nt);
+                for (int l = 0; l < nt; ++l) {
+                    m_transfers[(size_t)l].toCsr(rp[l], ci[l], v[l]);
+                    prp[l] = rp[l].data();
+                    pci[l] = ci[l].data();
+                    pv[l] = v[l].data();
+                    nc[l] = m_transfers[(size_t)l].cols();
+                }
+                std::shared_ptr<gsx_matrix> A = m_A;
+                detail::check(gsx_mg_create_transfers(A.get(), nt, nc.data(), prp.data(), pci.data(), pv.data(), npre,
+                                                      npost, 1, tau, cgc, prec, &h),
+                              "gsx_mg_create_transfers");
+            } else {
+                detail::check(gsx_mg_create(m_prob.get(), m_A.get(), m_levels, npre, npost, 1, tau, cgc, prec, &h),
+                              "gsx_mg_create");
+            }
             m_mg.reset(h, [](gsx_mg *q) { gsx_mg_destroy(q); });
             // multiGrid_example's MG.Smoother names (r, j, gs, scms) plus Chebyshev (c)
             const std::string sm = m_opt.askString("Smoother", "Jacobi");
@@ -1292,19 +1898,38 @@ public:
             else if (sm == "Chebyshev" || sm == "c") kind = GSX_SMOOTH_CHEBYSHEV;
             else if (sm == "SubspaceCorrectedMassSmoother" || sm == "scms") kind = GSX_SMOOTH_SCMS;
             if (kind < 0) throw std::runtime_error("gsMultiGridOp: smoother '" + sm + "' is not supported");
-            const double param = kind == GSX_SMOOTH_SCMS ? m_opt.askReal("Scaling", 0.12)
-                                                         : m_opt.askReal("ChebyshevRatio", 30.0);
-            detail::check(gsx_mg_set_smoother(h, kind, param), "gsx_mg_set_smoother");
+            bool perLevel = false;
+            for (auto &q : m_smoothers) perLevel = perLevel || q;
+            if (!perLevel) {
+                const double param = kind == GSX_SMOOTH_SCMS ? m_opt.askReal("Scaling", 0.12)
+                                                             : m_opt.askReal("ChebyshevRatio", 30.0);
+                detail::check(gsx_mg_set_smoother(h, kind, param), "gsx_mg_set_smoother");
+            }
+            applySmoothers(h);
         }
         return m_mg.get();
     }
 
 private:
+    void applySmoothers(gsx_mg *h) const
+    {
+        for (index_t l = 1; l < m_levels; ++l) {
+            const PrecondPtr &q = m_smoothers[(size_t)l];
+            if (!q) continue;
+            detail::check(gsx_mg_set_level_smoother(h, (int32_t)l, q->smootherKind(), (double)q->smootherDamping(), 0.0),
+                          "gsx_mg_set_level_smoother");
+        }
+    }
     index_t m_n = 0;
     index_t m_levels = 1;
+    bool m_byTransfers = false;
     gsOptionList m_opt;
     std::shared_ptr<gsx_matrix> m_A;
     std::shared_ptr<gsx_problem> m_prob;
+    std::vector<gsSparseMatrix<T, RowMajor>> m_transfers;
+    std::vector<PrecondPtr> m_smoothers;
+    OpPtr m_coarse;
+    mutable std::vector<std::shared_ptr<gsSparseMatrix<T>>> m_mats;
     mutable std::shared_ptr<gsx_mg> m_mg;
 };
 

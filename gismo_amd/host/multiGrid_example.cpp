@@ -70,7 +70,7 @@ int main(int argc, char *argv[])
             fd.getFirst(mp);
         } else if (gx > 0) mp = gsNurbsCreator<>::BSplineSquareGrid(gx, gy);
         else {
-            mp.addPatch(dim == 3 ? gsNurbsCreator<>::BSplineCube() : gsNurbsCreator<>::BSplineSquare());
+            mp.addPatch(dim == 3 ? gsNurbsCreator<>::BSplineCube(1, 0.5, 0.5, 0.5) : gsNurbsCreator<>::BSplineSquare());
             mp.computeTopology();
         }
         const short_t d = mp.dim();
@@ -90,13 +90,50 @@ int main(int argc, char *argv[])
         std::printf("dofs %d nnz %d\n", (int)assembler.numDofs(), (int)assembler.matrix().nonZeros());
 
         gsOptionList mgopt;
+        mgopt.addInt("Levels", "", levels);
         mgopt.addInt("NumPreSmooth", "", pre);
         mgopt.addInt("NumPostSmooth", "", post);
         mgopt.addReal("Damping", "", damping);
+        mgopt.addInt("DirichletStrategy", "", dirichlet::elimination);
+        mgopt.addInt("InterfaceStrategy", "", iFace::conforming);
         mgopt.addString("Precision", "", mixed ? "mixed" : "fp64");
         mgopt.addString("Smoother", "", smoother);
         mgopt.addReal("Scaling", "", scaling);
-        gsMultiGridOp<>::Ptr mg = gsMultiGridOp<>::make(assembler.matrix(), levels, mgopt);
+
+        // examples/multiGrid_example.cpp:260-327: hierarchy by coarsening,
+        // gsMultiGridOp::make(matrix, transfers), Cholesky coarse solver,
+        // one smoother per level.  The subspace corrected mass smoother and
+        // Chebyshev come from the hierarchy the device builds from the
+        // problem (make(matrix, levels, options)).
+        gsMultiGridOp<>::Ptr mg;
+        const bool perLevel = smoother == "Jacobi" || smoother == "j" || smoother == "Richardson" ||
+                              smoother == "r" || smoother == "GaussSeidel" || smoother == "gs";
+        if (perLevel) {
+            std::vector<gsSparseMatrix<real_t, RowMajor>> transferMatrices;
+            std::vector<gsMultiBasis<real_t>> multiBases;
+            gsGridHierarchy<>::buildByCoarsening(give(mb), bc, mgopt)
+                .moveMultiBasesTo(multiBases)
+                .moveTransferMatricesTo(transferMatrices);
+            mg = gsMultiGridOp<>::make(assembler.matrix(), transferMatrices);
+            mg->setOptions(mgopt);
+            mg->setCoarseSolver(makeSparseCholeskySolver(mg->matrix(0)));
+            for (index_t i = 1; i < mg->numLevels(); ++i) {
+                gsPreconditionerOp<>::Ptr smootherOp;
+                if (smoother == "Richardson" || smoother == "r")
+                    smootherOp = makeRichardsonOp(mg->matrix(i));
+                else if (smoother == "Jacobi" || smoother == "j")
+                    smootherOp = makeJacobiOp(mg->matrix(i));
+                else
+                    smootherOp = makeGaussSeidelOp(mg->matrix(i));
+                smootherOp->setOptions(mgopt);
+                mg->setSmoother(i, smootherOp);
+            }
+            std::printf("hierarchy levels %d transfers", (int)mg->numLevels());
+            for (auto &t : transferMatrices) std::printf(" %dx%d", (int)t.rows(), (int)t.cols());
+            std::printf("\n");
+        } else {
+            mg = gsMultiGridOp<>::make(assembler.matrix(), levels, mgopt);
+        }
 
         gsOptionList sopt = gsConjugateGradient<>::defaultOptions();
         sopt.setReal("Tolerance", tol);

@@ -71,7 +71,7 @@ int main(int argc, char *argv[])
         } else if (gx > 0) mp = gsNurbsCreator<>::BSplineSquareGrid(gx, gy, 1, bvp ? 0.25 : 0, bvp ? 0.25 : 0);
         else {
             const real_t o = bvp ? 0.25 : 0;
-            mp.addPatch(dim == 3 ? gsNurbsCreator<>::BSplineCube(1, o, o, o) : gsNurbsCreator<>::BSplineSquare(1, o, o));
+            mp.addPatch(dim == 3 ? gsNurbsCreator<>::BSplineCube(1, o + 0.5, o + 0.5, o + 0.5) : gsNurbsCreator<>::BSplineSquare(1, o, o));
             mp.computeTopology();
         }
         const short_t d = mp.dim();

@@ -325,6 +325,21 @@ gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nlevels,
                          int32_t npre, int32_t npost, int32_t ncycles,
                          double damping, double cgc_damping, int32_t precision,
                          gsx_mg **out);
+/* gsMultiGridOp::make(fineMatrix, transferMatrices) and its init
+ * (src/gsMultiGrid/gsMultiGrid.h:136-141, gsMultiGrid.hpp:19-35,70-94):
+ * the hierarchy given by the caller's transfers.  Transfer l (0 <= l <
+ * ntransfer) prolongates level l to level l + 1: a row-major CSR matrix
+ * (host arrays prow[l] of n_{l+1} + 1 entries, pcol[l]/pval[l], columns
+ * ascending within a row) of n_{l+1} rows and ncoarse[l] columns, where
+ * level ntransfer is A.  The coarse operators P^T A P and the restrictions
+ * P^T are formed on the device; smoothing and the exact coarse solve are as
+ * in gsx_mg_create.  The subspace corrected mass smoother needs the
+ * hierarchy of gsx_mg_create. */
+gsx_status gsx_mg_create_transfers(gsx_matrix *A, int32_t ntransfer, const int32_t *ncoarse,
+                                   const int32_t *const *prow, const int32_t *const *pcol,
+                                   const double *const *pval, int32_t npre, int32_t npost,
+                                   int32_t ncycles, double damping, double cgc_damping,
+                                   int32_t precision, gsx_mg **out);
 gsx_status gsx_mg_destroy(gsx_mg *mg);
 /* Smoother of every level but the coarsest (the sweeps npre/npost of
  * gsx_mg_create).  GSX_SMOOTH_JACOBI: gsJacobiOp::step with the damping of
@@ -352,6 +367,17 @@ gsx_status gsx_mg_destroy(gsx_mg *mg);
  * the first level above nlevels - 5 with more elements than the degree in
  * every direction keeps its tau for all finer levels. */
 gsx_status gsx_mg_set_smoother(gsx_mg *mg, int32_t kind, double param);
+/* gsMultiGridOp::setSmoother(lvl, op) (src/gsMultiGrid/gsMultiGrid.h:295):
+ * the smoother of level lvl alone (1 <= lvl < nlevels), kind and param as in
+ * gsx_mg_set_smoother, damping = the Jacobi/Richardson damping of that
+ * level's operator (gsJacobiOp/gsRichardsonOp option Damping).  Levels are
+ * set up in ascending order as the reference's example does; a scms level
+ * keeps the damping of the first qualifying coarser level. */
+gsx_status gsx_mg_set_level_smoother(gsx_mg *mg, int32_t lvl, int32_t kind, double damping, double param);
+/* gsMultiGridOp::setOptions (src/gsMultiGrid/gsMultiGrid.hpp:300-330):
+ * NumPreSmooth, NumPostSmooth and CoarseGridCorrectionDamping of an
+ * existing hierarchy. */
+gsx_status gsx_mg_set_cycle(gsx_mg *mg, int32_t npre, int32_t npost, double cgc_damping);
 /* The level parameter of level lvl (0 before gsx_mg_set_smoother): the
  * Chebyshev bound lmax, or tau_l of GSX_SMOOTH_SCMS. */
 gsx_status gsx_mg_smoother_bound(const gsx_mg *mg, int32_t lvl, double *lmax);

@@ -149,7 +149,7 @@ int main()
     // 3D: a cube and a neighbour with permuted, flipped directions
     {
         gsMultiPatch<> mp;
-        mp.addPatch(gsNurbsCreator<>::BSplineCube());
+        mp.addPatch(gsNurbsCreator<>::BSplineCube(1, 0.5, 0.5, 0.5));   // [0,1]^3
         gsMatrix<> C(8, 3);
         for (int c = 0; c < 8; ++c) {
             const int b0 = c & 1, b1 = (c >> 1) & 1, b2 = (c >> 2) & 1;
@@ -170,11 +170,51 @@ int main()
     {
         gsMultiPatch<> mp;
         for (int i = 0; i < 2; ++i)
-            for (int j = 0; j < 2; ++j) mp.addPatch(gsNurbsCreator<>::BSplineCube(1, i, j, 0));
+            for (int j = 0; j < 2; ++j) mp.addPatch(gsNurbsCreator<>::BSplineCube(1, i + 0.5, j + 0.5, 0.5));
         mp.computeTopology();
         print_topology("cubes2x2", mp);
         CHECK(mp.interfaces().size() == 4);
         check_mapper("cubes2x2", mp, 2, 2);
+    }
+
+    // multigrid hierarchy by coarsening (gsGridHierarchy::buildByCoarsening):
+    // 2 x 2 grid of degree-2 squares with 8 x 8 elements each, three levels;
+    // one cube of degree 3 with 4^3 elements, two levels.  Each transfer
+    // prints as rows cols | rowptr | col | val.
+    {
+        struct Case { const char *name; int grid, dim, p, r, levels; };
+        const Case cases[] = {{"hier2d", 2, 2, 2, 3, 3}, {"hier3d", 0, 3, 3, 2, 2}};
+        for (const Case &cs : cases) {
+            gsMultiPatch<> mp;
+            if (cs.grid) mp = gsNurbsCreator<>::BSplineSquareGrid(cs.grid, cs.grid);
+            else {
+                mp.addPatch(gsNurbsCreator<>::BSplineCube(1, 0.5, 0.5, 0.5));
+                mp.computeTopology();
+            }
+            gsBoundaryConditions<> bc;
+            for (auto &sd : mp.boundaries()) bc.addCondition(sd.first, (boundary::side)sd.second, condition_type::dirichlet);
+            gsMultiBasis<> mb(mp, true);
+            mb.setDegree(cs.p);
+            for (int r = 0; r < cs.r; ++r) mb.uniformRefine();
+            gsOptionList opt;
+            opt.addInt("Levels", "", cs.levels);
+            std::vector<gsSparseMatrix<real_t, RowMajor>> T;
+            std::vector<gsMultiBasis<real_t>> bases;
+            gsGridHierarchy<>::buildByCoarsening(give(mb), bc, opt).moveMultiBasesTo(bases).moveTransferMatricesTo(T);
+            CHECK((int)T.size() == cs.levels - 1 && (int)bases.size() == cs.levels);
+            CHECK(mb.nBases() == 0);   // moved into the hierarchy
+            for (size_t l = 0; l < T.size(); ++l) {
+                std::printf("%s_T%d %d %d |", cs.name, (int)l, (int)T[l].rows(), (int)T[l].cols());
+                for (index_t i = 0; i <= T[l].rows(); ++i) std::printf(" %d", T[l].outerIndexPtr()[i]);
+                std::printf(" |");
+                for (index_t k = 0; k < T[l].nonZeros(); ++k) std::printf(" %d", T[l].innerIndexPtr()[k]);
+                std::printf(" |");
+                for (index_t k = 0; k < T[l].nonZeros(); ++k) std::printf(" %.17g", T[l].valuePtr()[k]);
+                std::printf("\n");
+            }
+            for (size_t l = 0; l < bases.size(); ++l) print_knots((std::string(cs.name) + "_kv" + std::to_string(l)).c_str(),
+                                                                   bases[l].knots(0, 0));
+        }
     }
 
     // source strings

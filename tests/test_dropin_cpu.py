@@ -145,3 +145,35 @@ def test_xml_reader_on_reference_inputs():
             assert comp[g] in (-1, local[k, l])
             comp[g] = local[k, l]
     assert comp.tolist() == gold["component_of_dof"]
+
+
+def _transfer(tokens):
+    parts = " ".join(tokens).split("|")
+    rows, cols = (int(t) for t in parts[0].split())
+    rp = np.array([int(t) for t in parts[1].split()])
+    ci = np.array([int(t) for t in parts[2].split()], dtype=np.int64)
+    va = np.array([float(t) for t in parts[3].split()])
+    import scipy.sparse as sp
+    return sp.csr_matrix((va, ci, rp), shape=(rows, cols))
+
+
+@pytest.mark.parametrize("name,grid,p,nfine,levels", [("hier2d", 2, 2, 8, 3), ("hier3d", 0, 3, 4, 2)])
+def test_grid_hierarchy_transfers_match_oracle(host_check, name, grid, p, nfine, levels):
+    """gsGridHierarchy::buildByCoarsening's transfers (the C++ host:
+    coarsened knots, 1D knot insertion, tensor product, free DOFs, take_first
+    at interfaces) are bitwise the oracle's combined transfers
+    (oc_transfer_multipatch) between the same levels, coarsest first."""
+    for lvl in range(levels - 1):
+        nc = nfine >> (levels - 1 - lvl)
+        mk = (lambda n: problems.square_grid(grid, grid, n, p)) if grid else \
+             (lambda n: problems.Problem([problems.box_patch(3, p, n)], problems.all_sides(3), []))
+        coarse, fine = mk(nc), mk(2 * nc)
+        dc, ncf = O.mapper(coarse.patches, coarse.dir_sides, coarse.ifaces)
+        df, nff = O.mapper(fine.patches, fine.dir_sides, fine.ifaces)
+        Po = O.transfer_multipatch(coarse.patches, dc, ncf, df, nff)
+        P = _transfer(host_check[f"{name}_T{lvl}"])
+        assert P.shape == Po.shape
+        assert np.array_equal(P.indptr, Po.indptr) and np.array_equal(P.indices, Po.indices)
+        assert np.array_equal(P.data, Po.data)
+        deg, k = _knots(host_check[f"{name}_kv{lvl}"])
+        assert deg == p and np.array_equal(k, problems.knots_uniform(nc, p))

@@ -103,6 +103,21 @@ def test_multigrid_example_matches_python_mirror(ctx):
     assert int(s[1]) == it and ok
     assert float(s[2]) < 1e-8
     assert abs(l2 - gp.l2_error(x)) <= 1e-10 * l2
+    # the reference's sequence ran: gsGridHierarchy::buildByCoarsening, then
+    # gsMultiGridOp::make(matrix, transfers) with a Cholesky coarse solver
+    # and one Jacobi smoother per level; same iterations as the oracle's
+    # multigrid on its own transfers
+    h = re.search(r"hierarchy levels (\d+) transfers (.*)", text)
+    assert h and int(h[1]) == 3
+    levels = [problems.square_grid(2, 2, n, 2) for n in (2, 4, 8)]
+    maps = [O.mapper(q.patches, q.dir_sides, q.ifaces) for q in levels]
+    Ps = [O.transfer_multipatch(levels[i].patches, maps[i][0], maps[i][1], maps[i + 1][0], maps[i + 1][1])
+          for i in range(2)]
+    assert h[2].split() == [f"{P.shape[0]}x{P.shape[1]}" for P in Ps]
+    Af, bf = O.assemble(levels[-1].patches, maps[-1][0], maps[-1][1], 1)
+    omg = O.Multigrid(Af, Ps, npre=1, npost=1, damping=0.8)
+    _, ito, _ = O.cg_solve(Af, bf, tol=1e-8, maxit=100, precond="mg", mg=omg)
+    assert int(s[1]) == ito
     # the stationary multigrid iteration converges too
     text = _run("multiGrid_example", "--grid", "2", "2", "-r", "3", "-p", "2", "-l", "3", "-i", "d", "-s", "j",
                 "--MG.Damping", "0.8", "-t", "1e-8")

@@ -277,3 +277,81 @@ def test_scms_smoother_error_paths(ctx):
     A.close()
     b.close()
     gp.close()
+
+
+@pytest.mark.parametrize("name", ["grid2x2-p3", "cube-p2", "grid4x4-p3"])
+def test_multigrid_from_caller_transfers(ctx, name):
+    """gsMultiGridOp::make(fineMatrix, transferMatrices) (the reference's
+    construction, gsMultiGrid.hpp:19-35,70-94): the oracle's transfers
+    uploaded, P^T A P formed on the device.  The level operators equal the
+    oracle's Galerkin products to 1e-12 (row-scaled), one V(2,2) cycle its
+    cycle, and the MG-PCG iteration count its count."""
+    make, ns = CASES[name]
+    print(f"[case] {name}", flush=True)
+    levels, maps = _hierarchy(make, ns)
+    Af, bf, Ps, omg = _oracle_mg(levels, maps)
+    gp = gsx.Problem(ctx, levels[-1])
+    A, b = gp.assemble(gsx.PATTERN_EXPR, gsx.FP64)
+    mg = gsx.Multigrid.from_transfers(ctx, A, Ps, 2, 2, 0.6)
+    for lvl in range(len(ns)):
+        Ag, Pg = mg.level(lvl)
+        assert _rowscaled(Ag, omg.matrix(lvl)) <= 1e-12, lvl
+        if Pg is not None:
+            Po = Ps[lvl].tocsr()
+            Po.sort_indices()
+            assert np.array_equal(Pg.indptr, Po.indptr) and np.array_equal(Pg.indices, Po.indices)
+            assert np.array_equal(Pg.data, Po.data)
+    rng = np.random.default_rng(3)
+    r = rng.standard_normal(gp.nfree)
+    rd, zd = ctx.to_device(r), ctx.array(gp.nfree)
+    mg.apply(rd, zd)
+    zo = omg.apply(r)
+    assert np.linalg.norm(zd.download() - zo) <= 1e-10 * np.linalg.norm(zo)
+    x = ctx.array(gp.nfree).zero()
+    it, rel, ok = gsx.mg_cg_solve(A, mg, b, x, tol=1e-8, maxit=200)
+    xo, ito, erro = O.cg_solve(Af, bf, tol=1e-8, maxit=200, precond="mg", mg=omg)
+    assert ok and it == ito, (it, ito)
+    assert np.linalg.norm(x.download() - xo) <= 1e-8 * np.linalg.norm(xo)
+    mg.close()
+    A.close()
+    gp.close()
+
+
+def test_per_level_smoothers(ctx):
+    """gsMultiGridOp::setSmoother(lvl, op): setting every level to the same
+    smoother one level at a time gives bitwise the cycle of
+    gsx_mg_set_smoother; levels may mix Gauss-Seidel, Richardson and damped
+    Jacobi smoothing, and the mixed hierarchy still preconditions CG."""
+    make, ns = CASES["grid2x2-p3"]
+    levels, maps = _hierarchy(make, ns)
+    Af, bf, Ps, omg = _oracle_mg(levels, maps)
+    gp = gsx.Problem(ctx, levels[-1])
+    A, b = gp.assemble(gsx.PATTERN_EXPR, gsx.FP64)
+    rng = np.random.default_rng(5)
+    rd = ctx.to_device(rng.standard_normal(gp.nfree))
+    outs = []
+    for per_level in (False, True):
+        mg = gsx.Multigrid.from_transfers(ctx, A, Ps, 1, 1, 0.7)
+        if per_level:
+            for lvl in range(1, len(ns)):
+                mg.set_level_smoother(lvl, "gauss_seidel")
+        else:
+            lib_ok = gsx.lib().gsx_mg_set_smoother(mg.h, gsx.Multigrid.SMOOTHERS["gauss_seidel"], 0.0)
+            assert lib_ok == 0
+        zd = ctx.array(gp.nfree)
+        mg.apply(rd, zd)
+        outs.append(zd.download())
+        mg.close()
+    assert np.array_equal(outs[0], outs[1])
+    # mixed: coarse-most smoothed level Richardson, the finest damped Jacobi
+    mg = gsx.Multigrid.from_transfers(ctx, A, Ps, 2, 2, 0.6)
+    mg.set_level_smoother(1, "richardson", damping=0.05)
+    mg.set_level_smoother(2, "jacobi", damping=0.6)
+    x = ctx.array(gp.nfree).zero()
+    it, rel, ok = gsx.mg_cg_solve(A, mg, b, x, tol=1e-8, maxit=300)
+    xo, ito, erro = O.cg_solve(Af, bf, tol=1e-8, maxit=3000, precond="jacobi")
+    assert ok and it < ito, (it, ito)
+    assert np.linalg.norm(x.download() - xo) <= 1e-6 * np.linalg.norm(xo)
+    mg.close()
+    A.close()
+    gp.close()
