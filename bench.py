@@ -78,22 +78,39 @@ def stencil_spmv_bytes(nfree, slots, ngrid, sv, sx):
     return nfree * slots * sv + nfree * 4 + ngrid * sx + nfree * sx
 
 
+def cpu_threads():
+    """Host threads this process may use: the CPU affinity, capped by
+    OMP_NUM_THREADS when set (the GPU box gives one GPU a share of 16)."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        n = min(n, int(os.environ.get("OMP_NUM_THREADS", n)))
+    except ValueError:
+        pass
+    return max(1, n)
+
+
 def cpu_baseline(n_sample, tol):
-    """Oracle (single-threaded FP64 restatement) on the same generator at
-    n_sample^3 elements: assemble + Jacobi-CG to tol."""
+    """The oracle built for speed (libgsx_cpuref.so: -O3, AVX2/FMA, OpenMP)
+    on the same generator at n_sample^3 elements: the reference-style
+    threaded assembly (element stride per thread, insertion under a lock)
+    and Jacobi-CG to tol, on all host threads and on one."""
     import oracle as O
     from gismo_amd import problems
     prob = problems.poisson_noisy_cube(n=n_sample, p=3, seed=20261202)
-    t0 = time.perf_counter()
-    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
-    A, b = O.assemble(prob.patches, dofs, nfree, 0)
-    x, it, err = O.cg_solve(A, b, tol=tol, maxit=5000)
-    dt = time.perf_counter() - t0
     nb = prob.patches[0].nbasis
-    return {"value": nb / dt, "unit": "DOFs/s", "cores": 1, "kind": "port",
-            "sample": f"oracle assemble+Jacobi-CG({tol:g}) of the same generator at {n_sample}^3 "
-                      f"elements, p=3 ({nb} DOFs, {it} CG iterations, {dt:.1f} s), "
-                      f"1 thread of {cpu_model()}"}
+    nt = cpu_threads()
+    runs = {}
+    for threads in (nt, 1):
+        ta, ts, it, nfree = O.baseline_assemble_solve(prob.patches, prob.dir_sides, prob.ifaces, tol, threads)
+        runs[threads] = (ta, ts, it)
+    ta, ts, it = runs[nt]
+    t1a, t1s, _ = runs[1]
+    return {"value": nb / (ta + ts), "unit": "DOFs/s", "cores": nt, "kind": "port",
+            "one_thread_value": nb / (t1a + t1s),
+            "sample": f"oracle (gcc -O3 -march=x86-64-v3 -fopenmp) assemble on {nt} threads + Jacobi-CG({tol:g}) "
+                      f"of the same generator at {n_sample}^3 elements, p=3 ({nb} DOFs, {it} CG iterations): "
+                      f"assembly {ta:.2f} s + solve {ts:.2f} s on {nt} threads, {t1a:.2f} s + {t1s:.2f} s on 1 "
+                      f"thread, {cpu_model()}"}
 
 
 def main():
@@ -137,7 +154,7 @@ def main():
         return bench_serve(args)
     args.n = 64 if args.n is None else args.n
     args.tol = 1e-8 if args.tol is None else args.tol
-    args.cpu_sample = 26 if args.cpu_sample is None else args.cpu_sample
+    args.cpu_sample = 24 if args.cpu_sample is None else args.cpu_sample
 
     from gismo_amd import gsx, problems
 
@@ -199,7 +216,8 @@ def main():
     ctx.reset_timing()
     x.zero()
     run_step(ctx, gp, gsx, gsx.MIXED, args.tol, args.maxit, x)
-    keys = ("pattern_count", "pattern_scan", "pattern_fill", "elem3d", "gather", "convert_f32",
+    keys = ("pattern_count", "pattern_scan", "pattern_fill", "sf_assemble", "sf_finish", "elem3d", "gather",
+            "convert_f32",
             "stencil_build", "cg_init", "spmv_f32", "cg_vec_f32", "fd_apply_f32", "reliable_update")
 
     def phase_split():

@@ -5,6 +5,26 @@
 
 namespace gsx {
 
+// sum-factorised single-patch 3D assembly (gsx_sfasm.hip): the patch's
+// basis sizes, free box [flo, fhi] (m = its extent), Gauss points per
+// direction g, offsets of the 1D tables, tiles of 2 x 2 rows in (y, z)
+struct SfBox {
+    int nb[3], nel[3], flo[3], fhi[3], m[3], g[3];
+    int64_t toff[3];
+    int nty, ntz;
+};
+bool sf_supported(int d, int p);
+int64_t sf_table_doubles(int p, const int *nel);
+// tables, element load vectors + det check, stencil rows (FP64, all slots of
+// free rows whose column is in the patch)
+hipError_t launch_sf_assemble(int p, const PatchDev *patches, const GaussTab &gt, double *tab, const SfBox &bx,
+                              double *Fbuf, double *detmin, double *s64, int64_t nelem, hipStream_t s);
+// CSR values, FP32 half stencil (may be null), rhs; zeroes the slots of
+// columns outside the free box; err |= 1 on a CSR row length mismatch
+hipError_t launch_sf_finish(int p, const SfBox &bx, int nfree, const int *rowptr, double *s64, float *s32,
+                            double *val, double *rhs, const double *Fbuf, const int *dofs, const double *fixed,
+                            int *err, hipStream_t s);
+
 // assembly
 hipError_t launch_elem(int d, int p, const PatchDev *patches, const int *gpatch,
                        const int64_t *gprefix, int ng, int64_t nelem, const GaussTab &gt,

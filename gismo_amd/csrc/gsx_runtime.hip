@@ -287,6 +287,7 @@ struct gsx_problem {
     // out at kmat_off, and the serial of that assembly
     DevPtr kept_K;
     uint64_t asm_serial = 0, kept_serial = 0;
+    bool zero_kappa = false;        // some element has kappa exactly 0
 };
 
 // DOF numbering with the rules of gsDofMapper::finalizeComp
@@ -674,6 +675,7 @@ gsx_status gsx_problem_create(gsx_ctx *c, int32_t npatch, const gsx_patch_desc *
         pd.kappa = nullptr;
         pd.felem = nullptr;
         if (D.kappa_host) {
+            for (int64_t q = 0; q < h.nelem; ++q) P->zero_kappa = P->zero_kappa || D.kappa_host[q] == 0.0;
             DevPtr b = dalloc(c, sizeof(double) * h.nelem);
             if (!b) return fail(c, GSX_ERR_OOM, "device allocation (kappa)");
             HIP_OK(c, hipMemcpyAsync(b.get(), D.kappa_host, sizeof(double) * h.nelem,
@@ -972,7 +974,8 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
 {
     if (!P || !A_out) return GSX_ERR_ARG;
     gsx_ctx *c = P->ctx;
-    const int32_t flags = GSX_ASM_REBUILD_PATTERN | GSX_ASM_CSR_ONLY | GSX_ASM_KEEP_ELEMENTS;
+    const int32_t flags =
+        GSX_ASM_REBUILD_PATTERN | GSX_ASM_CSR_ONLY | GSX_ASM_KEEP_ELEMENTS | GSX_ASM_ELEMENTWISE | GSX_ASM_ROWWISE;
     if ((mode & ~flags) != GSX_PATTERN_EXPR && (mode & ~flags) != GSX_PATTERN_FULL)
         return fail(c, GSX_ERR_ARG, "unknown pattern mode");
     if (precision != GSX_FP64 && precision != GSX_MIXED)
@@ -981,7 +984,9 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
     const bool rebuild = (mode & GSX_ASM_REBUILD_PATTERN) != 0;
     const bool csr_only = (mode & GSX_ASM_CSR_ONLY) != 0;
     const bool keep_elements = (mode & GSX_ASM_KEEP_ELEMENTS) != 0;
-    mode &= ~(GSX_ASM_REBUILD_PATTERN | GSX_ASM_CSR_ONLY | GSX_ASM_KEEP_ELEMENTS);
+    const bool elementwise = (mode & GSX_ASM_ELEMENTWISE) != 0 || (mode & GSX_ASM_ROWWISE) == 0;
+    mode &= ~(GSX_ASM_REBUILD_PATTERN | GSX_ASM_CSR_ONLY | GSX_ASM_KEEP_ELEMENTS | GSX_ASM_ELEMENTWISE |
+              GSX_ASM_ROWWISE);
     P->kept_K.reset();
     const uint64_t serial = ++P->asm_serial;
     if (!P->have_pattern || rebuild) {
@@ -995,6 +1000,10 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
         P->d_detmin = dalloc(c, sizeof(double) * (std::max<int64_t>(1, P->nelem) + 260));
         if (!P->d_detmin) return fail(c, GSX_ERR_OOM, "device allocation (Jacobian check)");
     }
+    // single 3D patch: sum-factorised rows, no element matrices
+    const bool use_sf = single && !elementwise && !keep_elements && !P->zero_kappa && P->hp[0].d == 3 &&
+                        P->hp[0].p[0] == P->hp[0].p[1] && P->hp[0].p[0] == P->hp[0].p[2] &&
+                        gsx::sf_supported(3, P->hp[0].p[0]);
     // Element matrices live only during the assembly, in a buffer of at most
     // the context's budget: a single patch is assembled in slabs of element
     // layers along its slowest direction (each slab recomputes the p element
@@ -1007,7 +1016,9 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
     for (int k = 0; k < sd; ++k) layer_elems *= h0.nel[k];
     const int nb0 = h0.d == 3 ? (h0.p[0] + 1) * (h0.p[0] + 1) * (h0.p[0] + 1) : (h0.p[0] + 1) * (h0.p[0] + 1);
     int slab_rows = 0;   // row layers per slab (single patch)
-    if (single) {
+    if (use_sf) {
+        slab_rows = 0;
+    } else if (single) {
         const int64_t layer_bytes = layer_elems * ((int64_t)nb0 * nb0 + nb0) * (int64_t)sizeof(double);
         const int64_t fit = c->elem_budget / std::max<int64_t>(1, layer_bytes);
         const int free_layers = P->box_hi[sd] - P->box_lo[sd] + 1;
@@ -1018,10 +1029,12 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
         return fail(c, GSX_ERR_BUDGET, "element matrices of a multipatch assembly exceed the element budget "
                                        "(gsx_ctx_set_elem_budget)");
     }
-    const int64_t kcap = single ? std::min<int64_t>(h0.nel[sd], slab_rows + h0.p[0]) * layer_elems * nb0 * nb0
-                                : P->kdoubles;
-    const int64_t fcap = single ? std::min<int64_t>(h0.nel[sd], slab_rows + h0.p[0]) * layer_elems * nb0
-                                : P->fdoubles;
+    const int64_t kcap = use_sf ? 0
+                         : single ? std::min<int64_t>(h0.nel[sd], slab_rows + h0.p[0]) * layer_elems * nb0 * nb0
+                                  : P->kdoubles;
+    const int64_t fcap = use_sf ? P->nelem * nb0
+                         : single ? std::min<int64_t>(h0.nel[sd], slab_rows + h0.p[0]) * layer_elems * nb0
+                                  : P->fdoubles;
     DevPtr Kbuf = dalloc(c, sizeof(double) * std::max<int64_t>(1, kcap));
     DevPtr Fbuf = dalloc(c, sizeof(double) * std::max<int64_t>(1, fcap));
     if (!Kbuf || !Fbuf) return fail(c, GSX_ERR_OOM, "device allocation (element matrices)");
@@ -1048,8 +1061,8 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
         rhs = dp<double>(rhs_tmp);
     }
     DevPtr keep;
-    if (mode == GSX_PATTERN_EXPR) keep = dalloc(c, std::max<int64_t>(1, P->nnz));
-    if (!A->val || !rhs || (mode == GSX_PATTERN_EXPR && !keep))
+    if (mode == GSX_PATTERN_EXPR && !use_sf) keep = dalloc(c, std::max<int64_t>(1, P->nnz));
+    if (!A->val || !rhs || (mode == GSX_PATTERN_EXPR && !use_sf && !keep))
         return fail(c, GSX_ERR_OOM, "device allocation (values)");
     if (single) {
         // structured operator filled by the same gather (unless CSR only)
@@ -1075,12 +1088,55 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
         }
         S.s64_ready = !csr_only;
         S.s32_ready = !csr_only && precision == GSX_MIXED;
+        if (use_sf) {
+            gsx::SfBox sb;
+            const int p = h.p[0];
+            for (int k = 0; k < 3; ++k) {
+                sb.nb[k] = h.nb[k];
+                sb.nel[k] = h.nel[k];
+                sb.flo[k] = P->box_lo[k];
+                sb.fhi[k] = P->box_hi[k];
+                sb.m[k] = P->box_hi[k] - P->box_lo[k] + 1;
+                sb.g[k] = h.nel[k] * (p + 1);
+            }
+            sb.toff[0] = 0;
+            sb.toff[1] = (int64_t)sb.g[0] * (2 * (p + 1) + 1);
+            sb.toff[2] = sb.toff[1] + (int64_t)sb.g[1] * (2 * (p + 1) + 1);
+            sb.nty = (h.nb[1] + 1) / 2;
+            sb.ntz = (h.nb[2] + 1) / 2;
+            DevPtr tab = dalloc(c, sizeof(double) * gsx::sf_table_doubles(p, h.nel));
+            DevPtr s64tmp;
+            double *s64 = dp<double>(S.s64);
+            if (!s64) {
+                s64tmp = dalloc(c, sizeof(double) * gsx::stencil_size(A->n, (int)S.slots));
+                s64 = dp<double>(s64tmp);
+            }
+            DevPtr errf = dalloc(c, sizeof(int));
+            if (!tab || !s64 || !errf) return fail(c, GSX_ERR_OOM, "device allocation (row assembly)");
+            HIP_OK(c, hipMemsetAsync(errf.get(), 0, sizeof(int), s));
+            {
+                KScope k(c, "sf_assemble");
+                HIP_OK(c, gsx::launch_sf_assemble(p, dp<PatchDev>(P->d_patches), c->gt, dp<double>(tab), sb,
+                                                  dp<double>(Fbuf), dp<double>(P->d_detmin), s64, P->nelem, s));
+            }
+            {
+                KScope k(c, "sf_finish");
+                HIP_OK(c, gsx::launch_sf_finish(p, sb, P->nfree, dp<int>(P->d_rowptr), s64,
+                                                S.s32 ? dp<float>(S.s32) : nullptr, dp<double>(A->val), rhs,
+                                                dp<double>(Fbuf), dp<int>(P->d_dofs),
+                                                P->d_fixed ? dp<double>(P->d_fixed) : nullptr, dp<int>(errf), s));
+            }
+            int errh = 0;
+            HIP_OK(c, hipMemcpyAsync(&errh, errf.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+            HIP_OK(c, hipStreamSynchronize(s));
+            if (errh) return fail(c, GSX_ERR_ARG, "row assembly: CSR row length does not match the stencil window");
+        }
         // free rows are lexicographic over the free box: a range of row
         // layers along the slowest direction is a range of rows
         int64_t plane = 1;
         for (int k = 0; k < sd; ++k) plane *= P->box_hi[k] - P->box_lo[k] + 1;
         const Group &g = P->groups[0];
-        for (int L0 = P->box_lo[sd]; L0 <= P->box_hi[sd]; L0 += slab_rows) {
+        for (int L0 = P->box_lo[sd]; !use_sf && L0 <= P->box_hi[sd]; L0 += slab_rows) {
             const int L1 = std::min(L0 + slab_rows, P->box_hi[sd] + 1);
             const int64_t E0 = std::max(0, L0 - h.p[0]), E1 = std::min(h.nel[sd], L1);
             {
@@ -1121,7 +1177,7 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
     HIP_OK(c, gsx::launch_min(dp<double>(P->d_detmin), P->nelem, dmin_slot, s));
     double mind = 0.0;
     HIP_OK(c, hipMemcpyAsync(&mind, dmin_slot, sizeof(double), hipMemcpyDeviceToHost, s));
-    if (mode == GSX_PATTERN_EXPR) {
+    if (mode == GSX_PATTERN_EXPR && keep) {
         DevPtr dropped = dalloc(c, sizeof(unsigned long long));
         if (!dropped) return fail(c, GSX_ERR_OOM, "device allocation");
         HIP_OK(c, hipMemsetAsync(dropped.get(), 0, sizeof(unsigned long long), s));
@@ -1328,20 +1384,31 @@ static gsx_status read_state(gsx_ctx *c, const CGWork &w, CGState &h)
     return GSX_OK;
 }
 
+// Device buffers of the FP64 CSR CG: four vectors, the diagonal, partial
+// sums (4 x nparts), the state and the residual history (maxit + 1).
+struct CgBufs {
+    double *r, *z, *p, *t;
+    const double *diag;
+    double *parts;
+    int nparts;
+    CGState *state;
+    double *hist;
+};
+
 // FP64 preconditioned CG with the reference's step order and stopping rule.
-static gsx_status cg_fp64(gsx_matrix *A, int precond, double tau, const double *b, double *x,
-                          double tol, int maxit, int *iters, double *relres)
+// keep_hist: copy the residual history into A->hist.
+static gsx_status cg_fp64_run(gsx_matrix *A, const CgBufs &B, int precond, double tau, const double *b, double *x,
+                              double tol, int maxit, int *iters, double *relres, bool keep_hist)
 {
     gsx_ctx *c = A->ctx;
     hipStream_t s = c->stream;
-    CGWork &w = A->ws;
     const int n = A->n;
-    const int np = w.nparts;
+    const int np = B.nparts;
     // partial counts: SpMV kernels write ns per-block partials, vector kernels nv
     const int ns = gsx::spmv_blocks(n, A->lpr), nv = (n + 255) / 256;
-    double *P0 = dp<double>(w.parts), *P1 = P0 + np, *P2 = P1 + np;
-    CGState *st = dp<CGState>(w.state);
-    double *hist = dp<double>(w.hist);
+    double *P0 = B.parts, *P1 = P0 + np, *P2 = P1 + np;
+    CGState *st = B.state;
+    double *hist = B.hist;
     CGState h{};
     h.tol = tol;
     h.tau = tau;
@@ -1349,8 +1416,13 @@ static gsx_status cg_fp64(gsx_matrix *A, int precond, double tau, const double *
     HIP_OK(c, hipMemcpyAsync(st, &h, sizeof(CGState), hipMemcpyHostToDevice, s));
     const int *rp = dp<int>(A->rowptr), *ci = dp<int>(A->col);
     const double *va = dp<double>(A->val);
-    double *r = dp<double>(w.r), *z = dp<double>(w.z), *p = dp<double>(w.p), *t = dp<double>(w.t);
-    const double *diag = dp<double>(A->diag);
+    double *r = B.r, *z = B.z, *p = B.p, *t = B.t;
+    const double *diag = B.diag;
+    auto state = [&](CGState &out) -> gsx_status {
+        HIP_OK(c, hipMemcpyAsync(&out, st, sizeof(CGState), hipMemcpyDeviceToHost, s));
+        HIP_OK(c, hipStreamSynchronize(s));
+        return GSX_OK;
+    };
     {
         KScope k(c, "cg_init");
         // r = b - A x ; ||r||^2 ; ||b||^2
@@ -1358,10 +1430,8 @@ static gsx_status cg_fp64(gsx_matrix *A, int precond, double tau, const double *
         HIP_OK(c, gsx::launch_dot_f64(n, b, b, P0, nullptr, s));
         HIP_OK(c, gsx::launch_cg_scalar(0, P0, nv, P1, ns, st, hist, s));
     }
-    GSX_TRY(read_state(c, w, h));
-    if (h.done == 3) {
-        HIP_OK(c, hipMemsetAsync(x, 0, sizeof(double) * n, s));
-    }
+    GSX_TRY(state(h));
+    if (h.done == 3) HIP_OK(c, gsx::launch_fill_f64(x, n, 0.0, s));
     if (!h.done) {
         if (precond == GSX_PREC_JACOBI) {
             HIP_OK(c, gsx::launch_jacobi_apply_f64(n, r, p, diag, tau, P0, st, s));
@@ -1391,18 +1461,29 @@ static gsx_status cg_fp64(gsx_matrix *A, int precond, double tau, const double *
                 }
             }
             done_it = k_end;
-            GSX_TRY(read_state(c, w, h));
+            GSX_TRY(state(h));
             if (h.done) break;
         }
     }
     *iters = h.it;
     *relres = h.err;
-    A->hist.resize(h.it + 1);
-    HIP_OK(c, hipMemcpyAsync(A->hist.data(), hist, sizeof(double) * (h.it + 1), hipMemcpyDeviceToHost, s));
+    if (keep_hist) {
+        A->hist.resize(h.it + 1);
+        HIP_OK(c, hipMemcpyAsync(A->hist.data(), hist, sizeof(double) * (h.it + 1), hipMemcpyDeviceToHost, s));
+    }
     HIP_OK(c, hipStreamSynchronize(s));
     if (h.done == 1 || h.done == 3) return GSX_OK;
     if (h.err < tol) return GSX_OK;
     return fail(c, GSX_ERR_NOT_CONVERGED, "CG reached the maximum number of iterations");
+}
+
+static gsx_status cg_fp64(gsx_matrix *A, int precond, double tau, const double *b, double *x,
+                          double tol, int maxit, int *iters, double *relres)
+{
+    CGWork &w = A->ws;
+    const CgBufs B{dp<double>(w.r), dp<double>(w.z), dp<double>(w.p), dp<double>(w.t), dp<double>(A->diag),
+                   dp<double>(w.parts), w.nparts, dp<CGState>(w.state), dp<double>(w.hist)};
+    return cg_fp64_run(A, B, precond, tau, b, x, tol, maxit, iters, relres, true);
 }
 
 // Mixed precision: CG iterations on the FP32 copy of A with FP64 reliable

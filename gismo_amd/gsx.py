@@ -33,6 +33,8 @@ PATTERN_FULL = 1
 ASM_REBUILD_PATTERN = 0x100
 ASM_CSR_ONLY = 0x200
 ASM_KEEP_ELEMENTS = 0x400
+ASM_ELEMENTWISE = 0x800
+ASM_ROWWISE = 0x1000
 BC_DIRICHLET, BC_NEUMANN = 1, 2
 BCF_ZERO, BCF_MANUFACTURED, BCF_AFFINE = 0, 1, 2
 DIR_HOMOGENEOUS, DIR_L2PROJECTION, DIR_L2PROJECTION_INHOM, DIR_INTERPOLATION = 0, 1, 2, 3

@@ -43,11 +43,17 @@ typedef enum {
     GSX_ASM_REBUILD_PATTERN = 0x100, /* flag: recompute the cached pattern     */
     GSX_ASM_CSR_ONLY = 0x200,        /* flag: no structured operator (CSR
                                         solves only, e.g. batched members)    */
-    GSX_ASM_KEEP_ELEMENTS = 0x400    /* flag: the problem keeps this assembly's
+    GSX_ASM_KEEP_ELEMENTS = 0x400,   /* flag: the problem keeps this assembly's
                                         element matrices (when they were formed
                                         in one pass) until its next assembly;
                                         gsx_mg_create on the returned matrix
                                         takes them instead of recomputing      */
+    GSX_ASM_ELEMENTWISE = 0x800,     /* flag: element matrices + row gather
+                                        (summed in element order) also where
+                                        the sum-factorised row assembly of a
+                                        single 3D patch would run              */
+    GSX_ASM_ROWWISE = 0x1000         /* flag: the sum-factorised row assembly
+                                        where it applies (single 3D patch)     */
 } gsx_pattern_mode;
 
 typedef enum {
@@ -221,7 +227,15 @@ gsx_status gsx_assemble_bdr(gsx_problem *prob, double *rhs_dev);
  * bitwise, so its CSR arrays are also its CSC arrays (gsSparseMatrix is
  * column-major, src/gsCore/gsForwardDeclarations.h:279-280).
  * precision GSX_MIXED additionally keeps an FP32 copy of the values for the
- * inner iterations.  rhs_dev: nfree doubles (device) or NULL. */
+ * inner iterations.  rhs_dev: nfree doubles (device) or NULL.
+ * With GSX_ASM_ROWWISE a single 3D patch of equal degree 1..4 in every
+ * direction whose kappa has no exact zero is assembled row by row by sum
+ * factorisation, without element matrices (the same sums over the same
+ * Gauss points, grouped by point columns instead of elements: values agree
+ * with the element path to rounding).  There no entry is dropped by the
+ * exact-zero rule: an entry with only exact-zero element contributions
+ * needs kappa = 0 on every element they share.  Other problems,
+ * GSX_ASM_KEEP_ELEMENTS and GSX_ASM_ELEMENTWISE take the element path. */
 gsx_status gsx_assemble(gsx_problem *prob, int32_t pattern_mode,
                         int32_t precision, gsx_matrix **A_out,
                         double *rhs_dev);
@@ -309,7 +323,6 @@ gsx_status gsx_cg_solve_fd(gsx_matrix *A, gsx_fd *fd, int32_t precision, const d
                            double tol, int32_t maxit, int32_t *iters, double *relres);
 gsx_status gsx_cg_history(gsx_matrix *A, double *hist_host, int32_t cap,
                           int32_t *count);
-
 /* ---- geometric multigrid -------------------------------------------------- */
 /* gsMultiGridOp::make + init (src/gsMultiGrid/gsMultiGrid.hpp:19-94) on a
  * hierarchy of nlevels obtained by coarsening every patch's knot vectors

@@ -8,12 +8,15 @@ cpu_baseline leg of bench.py may import this module; the product path
 import ctypes as C
 import os
 import subprocess
+import time
 
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libgsx_oracle.so")
+_CPUREF_PATH = os.path.join(_HERE, "libgsx_cpuref.so")
 _lib = None
+_cpuref = None
 
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -184,6 +187,51 @@ def element_geometry(patch, elem):
     wq = np.zeros(nq)
     lib().oc_element_geometry(P.arr, elem, xq, jac, meas, wq)
     return xq.reshape(nq, d), jac.reshape(nq, d, d), meas, wq
+
+
+def cpuref_lib():
+    """The oracle built for timing (-O3, AVX2/FMA, OpenMP): libgsx_cpuref.so."""
+    global _cpuref
+    if _cpuref is None:
+        if not os.path.exists(_CPUREF_PATH):
+            build()
+        L = C.CDLL(_CPUREF_PATH)
+        L.oc_mapper_build.argtypes = [C.c_int, C.POINTER(OcPatch), C.c_int, _i32p, C.c_int, _i32p, _i32p]
+        L.oc_mapper_build.restype = C.c_int
+        L.oc_assemble_par.argtypes = [C.c_int, C.POINTER(OcPatch), _i32p, C.c_int, C.c_int, _f64p, C.c_int]
+        L.oc_assemble_par.restype = C.POINTER(OcCsr)
+        L.oc_cg_solve.argtypes = [C.POINTER(OcCsr), _f64p, _f64p, C.c_double, C.c_int, C.c_int,
+                                  C.c_double, C.c_void_p, C.POINTER(C.c_double), C.c_void_p]
+        L.oc_cg_solve.restype = C.c_int
+        L.oc_csr_free.argtypes = [C.POINTER(OcCsr)]
+        _cpuref = L
+    return _cpuref
+
+
+def baseline_assemble_solve(patch_list, dir_sides, ifaces, tol, nthreads, maxit=5000):
+    """CPU baseline run (timing only): DOF map, the reference-style threaded
+    assembly on nthreads threads and Jacobi-CG to tol (serial, as the
+    reference's column-major sparse product is), all from libgsx_cpuref.so.
+    Returns (assembly seconds, solve seconds, CG iterations, free DOFs)."""
+    L = cpuref_lib()
+    P = Patches(patch_list)
+    total = sum(int(np.prod([len(q.knots[k]) - q.p[k] - 1 for k in range(q.d)])) for q in patch_list)
+    ds = np.ascontiguousarray(np.asarray(dir_sides, dtype=np.int32).reshape(-1))
+    ifs = np.ascontiguousarray(np.asarray(ifaces, dtype=np.int32).reshape(-1))
+    ds = ds if ds.size else np.zeros(2, np.int32)
+    ifs = ifs if ifs.size else np.zeros(8, np.int32)
+    dofs = np.zeros(total, np.int32)
+    t0 = time.perf_counter()
+    nfree = L.oc_mapper_build(P.n, P.arr, len(dir_sides), ds, len(ifaces), ifs, dofs)
+    rhs = np.zeros(nfree)
+    ptr = L.oc_assemble_par(P.n, P.arr, dofs, nfree, 0, rhs, nthreads)
+    t1 = time.perf_counter()
+    x = np.zeros(nfree)
+    err = C.c_double(0)
+    it = L.oc_cg_solve(ptr, rhs, x, tol, maxit, 1, 1.0, None, C.byref(err), None)
+    t2 = time.perf_counter()
+    L.oc_csr_free(ptr)
+    return t1 - t0, t2 - t1, it, nfree
 
 
 def mapper(patch_list, dir_sides, ifaces):

@@ -9,6 +9,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define P1MAX (OC_MAXP + 1)
 
@@ -651,6 +654,64 @@ oc_csr *oc_assemble(int npatch, const oc_patch *patches, const int *dofs,
     return oc_assemble_bc(npatch, patches, dofs, nfree, mode, NULL, rhs_out);
 }
 
+/* Local matrix lm (nb x nb) and load lr of element e: per point eval =
+ * (G G^T)*meas [*kappa], quadrature lm = w_0 eval(0) + sum_k w_k eval(k)
+ * (gsExprAssembler.h:497-505).  G is scratch of nb x 3. */
+static void elem_local(const oc_patch *P, elem1d *E, int e, elemdata *D, double *lm, double *lr, double *G)
+{
+    const int d = P->d;
+    elem_eval(P, E, e, D);
+    const int nb = D->nb, nq = D->nq;
+    const double kap = P->kappa ? P->kappa[e] : 1.0;
+    for (int q = 0; q < nq; ++q) {
+        /* G = grad(u) * jacInv(G): G[b][c] = sum_a dN_a(b) Jinv[a][c] */
+        const double *Ji = D->Jinv + q * 9;
+        for (int b = 0; b < nb; ++b)
+            for (int c = 0; c < d; ++c) {
+                double s = 0.0;
+                for (int a = 0; a < d; ++a) s += D->dN[(b * d + a) * nq + q] * Ji[a * 3 + c];
+                G[b * 3 + c] = s;
+            }
+        const double wq = D->w[q], ms = D->meas[q];
+        for (int i = 0; i < nb; ++i)
+            for (int j = 0; j < nb; ++j) {
+                double s = 0.0;
+                for (int c = 0; c < d; ++c) s += G[i * 3 + c] * G[j * 3 + c];
+                double ev = s * ms;
+                if (P->kappa) ev *= kap;
+                if (q == 0) lm[i * nb + j] = wq * ev;
+                else lm[i * nb + j] += wq * ev;
+            }
+        const double fx = source_f(P, e, D->x + q * 3);
+        for (int i = 0; i < nb; ++i) {
+            const double ev = (D->N[i * nq + q] * fx) * ms;
+            if (q == 0) lr[i] = wq * ev;
+            else lr[i] += wq * ev;
+        }
+    }
+}
+
+/* push (gsExprAssembler.h:553-630) of one element's lm/lr */
+static void elem_push(const elemdata *D, const int *dofs, int off, int nfree, int mode, const double *fixed,
+                      const double *lm, const double *lr, orow *rows, double *rhs_out)
+{
+    const int nb = D->nb;
+    for (int i = 0; i < nb; ++i) {
+        const int ii = dofs[off + D->act[i]];
+        if (ii >= nfree) continue;
+        for (int j = 0; j < nb; ++j) {
+            const double v = lm[i * nb + j];
+            if (mode == 0 && v == 0.0) continue;
+            const int jj = dofs[off + D->act[j]];
+            if (jj < nfree) orow_add(&rows[ii], jj, v);
+            /* eliminated column: rhs_i -= K_ij g_j (:613-619);
+             * homogeneous values (fixed NULL) leave the rhs as is */
+            else if (fixed && v != 0.0) rhs_out[ii] -= v * fixed[jj - nfree];
+        }
+        rhs_out[ii] += lr[i];
+    }
+}
+
 oc_csr *oc_assemble_bc(int npatch, const oc_patch *patches, const int *dofs,
                        int nfree, int mode, const double *fixed, double *rhs_out)
 {
@@ -672,52 +733,63 @@ oc_csr *oc_assemble_bc(int npatch, const oc_patch *patches, const int *dofs,
             nbas *= oc_patch_nbasis(P, k);
         }
         for (int e = 0; e < nel; ++e) {
-            elem_eval(P, E, e, D);
-            const int nb = D->nb, nq = D->nq;
-            const double kap = P->kappa ? P->kappa[e] : 1.0;
-            for (int q = 0; q < nq; ++q) {
-                /* G = grad(u) * jacInv(G): G[b][c] = sum_a dN_a(b) Jinv[a][c] */
-                const double *Ji = D->Jinv + q * 9;
-                for (int b = 0; b < nb; ++b)
-                    for (int c = 0; c < d; ++c) {
-                        double s = 0.0;
-                        for (int a = 0; a < d; ++a) s += D->dN[(b * d + a) * nq + q] * Ji[a * 3 + c];
-                        G[b * 3 + c] = s;
-                    }
-                const double wq = D->w[q], ms = D->meas[q];
-                for (int i = 0; i < nb; ++i)
-                    for (int j = 0; j < nb; ++j) {
-                        double s = 0.0;
-                        for (int c = 0; c < d; ++c) s += G[i * 3 + c] * G[j * 3 + c];
-                        double ev = s * ms;
-                        if (P->kappa) ev *= kap;
-                        if (q == 0) lm[i * nb + j] = wq * ev;
-                        else lm[i * nb + j] += wq * ev;
-                    }
-                const double fx = source_f(P, e, D->x + q * 3);
-                for (int i = 0; i < nb; ++i) {
-                    const double ev = (D->N[i * nq + q] * fx) * ms;
-                    if (q == 0) lr[i] = wq * ev;
-                    else lr[i] += wq * ev;
-                }
-            }
-            /* push */
-            for (int i = 0; i < nb; ++i) {
-                const int ii = dofs[off + D->act[i]];
-                if (ii >= nfree) continue;
-                for (int j = 0; j < nb; ++j) {
-                    const double v = lm[i * nb + j];
-                    if (mode == 0 && v == 0.0) continue;
-                    const int jj = dofs[off + D->act[j]];
-                    if (jj < nfree) orow_add(&rows[ii], jj, v);
-                    /* eliminated column: rhs_i -= K_ij g_j (:613-619);
-                     * homogeneous values (fixed NULL) leave the rhs as is */
-                    else if (fixed && v != 0.0) rhs_out[ii] -= v * fixed[jj - nfree];
-                }
-                rhs_out[ii] += lr[i];
-            }
+            elem_local(P, E, e, D, lm, lr, G);
+            elem_push(D, dofs, off, nfree, mode, fixed, lm, lr, rows, rhs_out);
         }
         for (int k = 0; k < d; ++k) elem1d_free(&E[k]);
+        off += nbas;
+    }
+    return rows_to_csr(rows, nfree);
+}
+
+/* The reference's threaded assembly (gsExprAssembler.h:761-796): a team of
+ * nthreads threads, thread t takes elements t, t + nt, ... of each patch
+ * with its own evaluation buffers, and the push runs under one lock (omp
+ * critical, :607).  Used only as the CPU baseline's timing; the order in
+ * which elements reach the matrix depends on the schedule, so values agree
+ * with oc_assemble to rounding.  Without OpenMP it runs on one thread. */
+oc_csr *oc_assemble_par(int npatch, const oc_patch *patches, const int *dofs, int nfree, int mode,
+                        double *rhs_out, int nthreads)
+{
+    orow *rows = (orow *)calloc(nfree, sizeof(orow));
+    for (int i = 0; i < nfree; ++i) rhs_out[i] = 0.0;
+    for (int n = 1; n < 32; ++n) {   /* Gauss tables before the team starts */
+        const double *x, *w;
+        gauss_get(n, &x, &w);
+    }
+    int off = 0;
+    for (int pi = 0; pi < npatch; ++pi) {
+        const oc_patch *P = &patches[pi];
+        const int d = P->d;
+        int nel = 1, nbas = 1;
+        for (int k = 0; k < d; ++k) {
+            nel *= oc_patch_nelem(P, k);
+            nbas *= oc_patch_nbasis(P, k);
+        }
+#pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
+        {
+            int tid = 0, nt = 1;
+#ifdef _OPENMP
+            tid = omp_get_thread_num();
+            nt = omp_get_num_threads();
+#endif
+            elemdata *D = (elemdata *)malloc(sizeof(elemdata));
+            double *lm = (double *)malloc(sizeof(double) * 512 * 512);
+            double *lr = (double *)malloc(sizeof(double) * 512);
+            double *G = (double *)malloc(sizeof(double) * 512 * 3);
+            elem1d E[3];
+            for (int k = 0; k < d; ++k) elem1d_build(P, k, &E[k]);
+            for (int e = tid; e < nel; e += nt) {
+                elem_local(P, E, e, D, lm, lr, G);
+#pragma omp critical
+                elem_push(D, dofs, off, nfree, mode, NULL, lm, lr, rows, rhs_out);
+            }
+            for (int k = 0; k < d; ++k) elem1d_free(&E[k]);
+            free(D);
+            free(lm);
+            free(lr);
+            free(G);
+        }
         off += nbas;
     }
     return rows_to_csr(rows, nfree);

@@ -167,6 +167,12 @@ void oc_neumann(int npatch, const oc_patch *patches, const int *dofs, int nfree,
 /* oc_assemble with the fixed values moved to the rhs (fixed NULL: zero). */
 oc_csr *oc_assemble_bc(int npatch, const oc_patch *patches, const int *dofs,
                        int nfree, int mode, const double *fixed, double *rhs_out);
+/* The same assembly on a team of nthreads OpenMP threads, element-strided
+ * with the push under a lock, as the reference threads it
+ * (gsExprAssembler.h:761-796).  CPU baseline timing only; fixed values are
+ * not applied.  Built with OpenMP only in libgsx_cpuref.so. */
+oc_csr *oc_assemble_par(int npatch, const oc_patch *patches, const int *dofs, int nfree, int mode,
+                        double *rhs_out, int nthreads);
 
 /* --- reporting ---------------------------------------------------------- */
 /* L2 error of the discrete solution (free coefficients, eliminated = 0)

@@ -222,21 +222,24 @@ def _noisy_patch_problem(d, p, n, seed):
 
 @pytest.mark.parametrize("d,p,n", [(2, 3, 24), (3, 2, 10), (3, 4, 6)])
 def test_slab_assembly_is_identical(ctx, d, p, n):
-    """A single patch assembled in slabs of element layers (element budget
-    smaller than the patch) gives bitwise the same matrix, load vector and
-    structured operator as in one pass."""
+    """A single patch assembled on the element path in slabs of element
+    layers (element budget smaller than the patch) gives bitwise the same
+    matrix, load vector and structured operator as in one pass.  (3D patches
+    take the sum-factorised row assembly by default, which forms no element
+    matrices; GSX_ASM_ELEMENTWISE selects the element path.)"""
     prob = _noisy_patch_problem(d, p, n, seed=40 + d + p)
     gp = gsx.Problem(ctx, prob)
-    A1, b1 = gp.assemble(gsx.PATTERN_EXPR, gsx.MIXED)
+    ew = gsx.PATTERN_EXPR | gsx.ASM_ELEMENTWISE
+    A1, b1 = gp.assemble(ew, gsx.MIXED)
     nel_layer = n ** (d - 1)
     nb = (p + 1) ** d
     layer_bytes = nel_layer * (nb * nb + nb) * 8
     try:
         ctx.set_elem_budget(layer_bytes * (p + 2))   # slabs of 2 row layers
-        A2, b2 = gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_REBUILD_PATTERN, gsx.MIXED)
+        A2, b2 = gp.assemble(ew | gsx.ASM_REBUILD_PATTERN, gsx.MIXED)
         with pytest.raises(gsx.GsxError):
             ctx.set_elem_budget(layer_bytes * p)      # not even one row layer fits
-            gp.assemble(gsx.PATTERN_EXPR, gsx.FP64)
+            gp.assemble(ew, gsx.FP64)
     finally:
         ctx.set_elem_budget(16 << 30)
     r1, c1, v1 = A1.download()

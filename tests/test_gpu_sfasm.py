@@ -1,0 +1,135 @@
+"""Sum-factorised row assembly of single 3D patches (GSX_ASM_ROWWISE)
+against the element path (element matrices + row gather in element order,
+GSX_ASM_ELEMENTWISE) and the oracle: patterns bit-exact, values and loads
+within rounding of the element path and within 1e-12 of the oracle, the
+FP64 stencil bitwise symmetric, inhomogeneous Dirichlet columns moved to the
+rhs, and the structured operators (FP64 and the half-stored FP32 one) giving
+the CSR operator's CG iterations."""
+import numpy as np
+import pytest
+
+import oracle as O
+from gismo_amd import gsx, problems
+
+pytestmark = pytest.mark.gpu
+
+ENTRY_TOL = 1e-12    # against the oracle (row-scaled)
+PATH_TOL = 1e-13     # against the element path (row-scaled)
+
+
+def _row_scaled_err(rp, va, ref):
+    err = 0.0
+    for i in range(len(rp) - 1):
+        a, b = rp[i], rp[i + 1]
+        if b > a:
+            s = np.abs(ref[a:b]).max()
+            err = max(err, np.abs(va[a:b] - ref[a:b]).max() / s)
+    return err
+
+
+CASES = [(1, 5, True), (2, 4, True), (3, 5, True), (4, 3, True), (3, 7, False), (2, 1, True)]
+
+
+@pytest.mark.parametrize("p,n,kappa", CASES)
+def test_rows_match_element_path_and_oracle(ctx, p, n, kappa):
+    prob = problems.poisson_noisy_cube(n=n, p=p, seed=31 + p, with_kappa=kappa)
+    print(f"[case] sf p={p} n={n} kappa={kappa}", flush=True)
+    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    Ao, bo = O.assemble(prob.patches, dofs, nfree, 0)
+    gp = gsx.Problem(ctx, prob)
+    A1, b1 = gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_ROWWISE, gsx.MIXED)
+    A2, b2 = gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_ELEMENTWISE, gsx.MIXED)
+    rp1, ci1, va1 = A1.download()
+    rp2, ci2, va2 = A2.download()
+    assert np.array_equal(rp1, Ao.indptr) and np.array_equal(ci1, Ao.indices)
+    assert np.array_equal(rp1, rp2) and np.array_equal(ci1, ci2)
+    assert _row_scaled_err(rp1, va1, va2) <= PATH_TOL
+    assert _row_scaled_err(rp1, va1, Ao.data) <= ENTRY_TOL
+    g1, g2 = b1.download()[:nfree], b2.download()[:nfree]
+    assert np.abs(g1 - g2).max() <= PATH_TOL * np.abs(g2).max()
+    assert np.abs(g1 - bo).max() <= ENTRY_TOL * np.abs(bo).max()
+    import scipy.sparse as sp
+    M = sp.csr_matrix((va1, ci1, rp1), shape=(nfree, nfree))
+    assert (M != M.T).nnz == 0
+    assert abs(gp.min_detj()) > 0
+    for a in (A1, A2, b1, b2):
+        a.close()
+    gp.close()
+
+
+@pytest.mark.parametrize("p", [2, 3])
+def test_structured_operators_and_cg(ctx, p):
+    """FP64 CG on the row-assembled structured operator takes the CSR
+    operator's iterations and solution; mixed CG on the half-stored FP32
+    stencil takes the FP32 CSR operator's iterations (the FP32 operator is
+    pinned: a wrong mirror slot would change the iterates)."""
+    prob = problems.poisson_noisy_cube(n=8, p=p, seed=5)
+    gp = gsx.Problem(ctx, prob)
+    A, b = gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_ROWWISE, gsx.MIXED)
+    res = {}
+    for structured in (True, False):
+        A.set_operator(structured)
+        for prec in (gsx.FP64, gsx.MIXED):
+            x = ctx.array(gp.nfree).zero()
+            it, rel, ok = A.cg_solve(b, x, tol=1e-10, maxit=2000, precision=prec)
+            assert ok, (structured, prec, rel)
+            res[(structured, prec)] = (it, x.download())
+            x.close()
+    for prec in (gsx.FP64, gsx.MIXED):
+        (i1, x1), (i2, x2) = res[(True, prec)], res[(False, prec)]
+        assert i1 == i2, (prec, i1, i2)
+        assert np.linalg.norm(x1 - x2) <= 1e-9 * np.linalg.norm(x2)
+    A.close()
+    b.close()
+    gp.close()
+
+
+def test_inhomogeneous_dirichlet_columns(ctx):
+    """Affine Dirichlet data on all faces of a curved cube (projected): the
+    values of the eliminated functions move to the rhs (rhs_i -= K_ij g_j)
+    as on the element path and in the oracle, and the solution is the
+    oracle's.  (In 3D the Gauss rule does not integrate the curved map's
+    cofactors exactly, so the affine function itself is not reproduced.)"""
+    import scipy.sparse.linalg as spla
+    prob = problems.poisson_noisy_cube(n=6, p=3, seed=9, with_kappa=False)
+    P = prob.patches[0]
+    P.fkind = 0
+    P.felem = np.zeros(P.nelem)
+    aff = (0.5, 1.0, -2.0, 0.25)
+    bcs = [(0, s, gsx.BC_DIRICHLET, gsx.BCF_AFFINE, aff) for s in range(1, 7)]
+    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    fixed = O.dirichlet_values(prob.patches, dofs, nfree, P.nbasis - nfree, bcs)
+    Ao, bo = O.assemble(prob.patches, dofs, nfree, 0, fixed=fixed)
+    xo = spla.spsolve(Ao.tocsc(), bo)
+    gp = gsx.Problem(ctx, prob)
+    gp.set_bc(bcs, gsx.DIR_L2PROJECTION)
+    A1, b1 = gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_ROWWISE, gsx.FP64)
+    A2, b2 = gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_ELEMENTWISE, gsx.FP64)
+    g1, g2 = b1.download()[:nfree], b2.download()[:nfree]
+    assert np.abs(g1 - g2).max() <= PATH_TOL * np.abs(g2).max()
+    assert np.abs(g1 - bo).max() <= ENTRY_TOL * np.abs(bo).max()
+    x = ctx.array(gp.nfree).zero()
+    it, rel, ok = A1.cg_solve(b1, x, tol=1e-13, maxit=3000)
+    assert ok
+    assert np.abs(x.download() - xo).max() <= 1e-9 * np.abs(xo).max()
+    for a in (A1, A2, b1, b2, x):
+        a.close()
+    gp.close()
+
+
+def test_other_problems_keep_the_element_path(ctx):
+    """A kappa with exact zeros keeps the element path (the exact-zero rule
+    drops the entries only voids contribute to): pattern and values are the
+    oracle's."""
+    prob = problems.poisson_noisy_cube(n=4, p=2, seed=3)
+    problems.zero_kappa_box(prob.patches[0], (0, 0, 0), (2, 2, 2))
+    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    Ao, bo = O.assemble(prob.patches, dofs, nfree, 0)
+    gp = gsx.Problem(ctx, prob)
+    A, b = gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_ROWWISE, gsx.FP64)
+    rp, ci, va = A.download()
+    assert np.array_equal(rp, Ao.indptr) and np.array_equal(ci, Ao.indices)
+    assert _row_scaled_err(rp, va, Ao.data) <= ENTRY_TOL
+    A.close()
+    b.close()
+    gp.close()
