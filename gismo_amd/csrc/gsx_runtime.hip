@@ -94,6 +94,7 @@ struct gsx_ctx {
     std::shared_ptr<Counter> mem = std::make_shared<Counter>();
     GaussTab gt;
     int64_t elem_budget = int64_t(16) << 30;   // bytes of element matrices per assembly pass
+    void *scratch = nullptr;   // this device's address of the allocation-free workspace
 };
 
 namespace {
@@ -452,6 +453,33 @@ gsx_status gsx_mapper_build(int32_t d, int32_t npatch, const int32_t *nbasis_dir
     return GSX_OK;
 }
 
+// Workspace of gsx_cg_solve_noalloc: a fixed device array, one copy per GPU,
+// alive until the process ends.  Host code uses only the address that
+// hipGetSymbolAddress returns for the current device (kept per device in
+// g_scratch_addr and in every context of that device), never the symbol.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static void *g_scratch_addr[64] = {};
+
+// device address of `scratch` on the context's device (looked up once per
+// device, with that device current)
+static gsx_status scratch_setup(gsx_ctx *c)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (c->device < 0 || c->device >= 64) return fail(c, GSX_ERR_ARG, "device index out of range");
+    if (!g_scratch_addr[c->device]) {
+        void *ptr = nullptr;
+        HIP_OK(c, hipSetDevice(c->device));
+        const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(c, GSX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_addr[c->device] = ptr;
+    }
+    c->scratch = g_scratch_addr[c->device];
+    return GSX_OK;
+}
+
 gsx_status gsx_ctx_create(int32_t device, gsx_ctx **out)
 {
     if (!out) return GSX_ERR_ARG;
@@ -469,6 +497,11 @@ gsx_status gsx_ctx_create(int32_t device, gsx_ctx **out)
     }
     std::memset(&c->gt, 0, sizeof(c->gt));
     for (int q = 1; q <= gsx::kMaxQ1; ++q) gauss_rule(q, c->gt.x[q], c->gt.w[q]);
+    if (scratch_setup(c) != GSX_OK) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return GSX_ERR_HIP;
+    }
     *out = c;
     return GSX_OK;
 }
@@ -2027,6 +2060,60 @@ gsx_status gsx_cg_solve(gsx_matrix *A, int32_t precond, double tau, gsx_mg *mg, 
         e = mixed ? cg_mixed(A, precond, tau, b, x, tol, maxit, &it, &rr)
                   : cg_fp64(A, precond, tau, b, x, tol, maxit, &it, &rr);
     }
+    if (iters) *iters = it;
+    if (relres) *relres = rr;
+    return e;
+}
+
+// gsx_cg_solve's FP64 CSR CG (GSX_PREC_NONE or GSX_PREC_JACOBI) with its
+// workspace -- the four vectors, the diagonal, the partial sums, the state
+// and the residual history -- in `scratch` instead of device allocations.
+// Layout (256-byte aligned pieces): r, z, p, t, diag (n doubles each),
+// partials (4 x nparts), history (maxit + 1), state.
+gsx_status gsx_cg_solve_noalloc(gsx_matrix *A, int32_t precond, double tau, const double *b, double *x,
+                                double tol, int32_t maxit, int32_t *iters, double *relres)
+{
+    if (!A || !b || !x || maxit < 0) return GSX_ERR_ARG;
+    gsx_ctx *c = A->ctx;
+    if (precond != GSX_PREC_NONE && precond != GSX_PREC_JACOBI)
+        return fail(c, GSX_ERR_ARG, "allocation-free CG: GSX_PREC_NONE or GSX_PREC_JACOBI only");
+    if (!c->scratch) return fail(c, GSX_ERR_HIP, "allocation-free CG: workspace address not set up");
+    const int n = A->n;
+    if (n == 0) {
+        if (iters) *iters = 0;
+        if (relres) *relres = 0.0;
+        return GSX_OK;
+    }
+    const int nparts = std::max(gsx::spmv_blocks(n, A->lpr), (n + 255) / 256) + 1;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~size_t(255);
+        return o;
+    };
+    const size_t vec = sizeof(double) * (size_t)n;
+    const size_t o_r = take(vec), o_z = take(vec), o_p = take(vec), o_t = take(vec), o_d = take(vec);
+    const size_t o_parts = take(sizeof(double) * 4 * (size_t)nparts);
+    const size_t o_hist = take(sizeof(double) * ((size_t)maxit + 1));
+    const size_t o_state = take(sizeof(CGState));
+    if (off > kScratchBytes) {
+        char buf[200];
+        std::snprintf(buf, sizeof(buf),
+                      "allocation-free CG: %d rows with maxit %d need %zu bytes, more than the %zu-byte workspace",
+                      n, maxit, off, kScratchBytes);
+        return fail(c, GSX_ERR_ARG, buf);
+    }
+    unsigned char *base = static_cast<unsigned char *>(c->scratch);
+    double *diag = reinterpret_cast<double *>(base + o_d);
+    if (precond == GSX_PREC_JACOBI)
+        HIP_OK(c, gsx::launch_diag_f64(n, dp<int>(A->rowptr), dp<int>(A->col), dp<double>(A->val), diag, c->stream));
+    const CgBufs B{reinterpret_cast<double *>(base + o_r), reinterpret_cast<double *>(base + o_z),
+                   reinterpret_cast<double *>(base + o_p), reinterpret_cast<double *>(base + o_t), diag,
+                   reinterpret_cast<double *>(base + o_parts), nparts, reinterpret_cast<CGState *>(base + o_state),
+                   reinterpret_cast<double *>(base + o_hist)};
+    int it = 0;
+    double rr = 0.0;
+    const gsx_status e = cg_fp64_run(A, B, precond, tau, b, x, tol, maxit, &it, &rr, false);
     if (iters) *iters = it;
     if (relres) *relres = rr;
     return e;

@@ -117,6 +117,7 @@ SIGNATURES = {
     "gsx_cg_solve": [_vp, C.c_int32, C.c_double, _vp, C.c_int32, _vp, _vp, C.c_double, C.c_int32,
                      _i32p, _dp],
     "gsx_cg_history": [_vp, _dp, C.c_int32, _i32p],
+    "gsx_cg_solve_noalloc": [_vp, C.c_int32, C.c_double, _vp, _vp, C.c_double, C.c_int32, _i32p, _dp],
     "gsx_l2_error": [_vp, _vp, _dp],
     "gsx_problem_set_bc": [_vp, C.c_int32, _vp, C.c_int32],
     "gsx_problem_fixed_values": [_vp, _dp, C.c_int32],

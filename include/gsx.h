@@ -323,6 +323,21 @@ gsx_status gsx_cg_solve_fd(gsx_matrix *A, gsx_fd *fd, int32_t precision, const d
                            double tol, int32_t maxit, int32_t *iters, double *relres);
 gsx_status gsx_cg_history(gsx_matrix *A, double *hist_host, int32_t cap,
                           int32_t *count);
+/* Allocation-free CG: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): gsx_cg_solve's FP64 CSR
+ * CG (precond GSX_PREC_NONE or GSX_PREC_JACOBI, damping tau) with its whole
+ * workspace in a fixed 64 MiB device array per GPU (held until the process
+ * ends): 5 n + 4 nparts + maxit + 1 doubles plus the solver state, nparts
+ * about n / 256, each piece rounded up to 256 bytes -- about 1.6 M rows at
+ * maxit 1000.  Larger requests return GSX_ERR_ARG.  No hipMalloc, hipFree
+ * or allocating library call is made; the caller passes b and x (device,
+ * n doubles).  The residual history is not kept (gsx_cg_history does not
+ * see it).  Every call on one GPU uses the same array: calls that overlap on
+ * one GPU (several host threads or streams at once) race. */
+gsx_status gsx_cg_solve_noalloc(gsx_matrix *A, int32_t precond, double tau, const double *b, double *x,
+                                double tol, int32_t maxit, int32_t *iters, double *relres);
+
 /* ---- geometric multigrid -------------------------------------------------- */
 /* gsMultiGridOp::make + init (src/gsMultiGrid/gsMultiGrid.hpp:19-94) on a
  * hierarchy of nlevels obtained by coarsening every patch's knot vectors
