@@ -134,6 +134,8 @@ def main():
                     help="batch: number of problems (512); serve: number of timed requests (48)")
     ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="default workload only: skip the multigrid and batch legs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -145,11 +147,13 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "mg":
-        return bench_mg(args)
+        print(json.dumps(bench_mg(args)))
+        return None
     if args.workload == "mg-cube":
         return bench_mg_cube(args)
     if args.workload == "batch":
-        return bench_batch(args)
+        print(json.dumps(bench_batch(args)))
+        return None
     if args.workload == "serve":
         return bench_serve(args)
     args.n = 64 if args.n is None else args.n
@@ -284,6 +288,20 @@ def main():
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample, args.tol)
     gp.close()
     ctx.close()
+    if not args.no_legs:
+        # the multigrid and batch workloads at their own sizes, timed the same
+        # way (their figures beside the headline, not in `value`)
+        import copy
+        legs = {}
+        for key, fn in (("mg", bench_mg), ("batch", bench_batch)):
+            a = copy.copy(args)
+            a.n, a.tol, a.count, a.cpu_sample, a.no_cpu = None, None, None, None, True
+            a.steps = max(1, min(args.steps, 2)) if key == "batch" else args.steps
+            r = fn(a)
+            legs[key] = {k: r[k] for k in ("metric", "value", "unit", "ms_per_step", "dtype", "config", "fp64",
+                                           "mixed", "speedup_mixed_vs_fp64", "roofline") if k in r}
+            log(f"leg {key}: {r['value']:.4g} DOFs/s, {r['ms_per_step']:.1f} ms per step")
+        out["legs"] = legs
     print(json.dumps(out))
 
 
@@ -389,6 +407,10 @@ def bench_mg(args):
         for _ in range(args.warmup):
             x.zero()
             step()
+        # the outer CG's fine-level SpMV timed with HIP events on the stream
+        ctx.enable_timing(True)
+        ctx.timing_filter("spmv_f64")
+        ctx.reset_timing()
         times = []
         for _ in range(args.steps):
             x.zero()
@@ -396,11 +418,14 @@ def bench_mg(args):
             t0 = time.perf_counter()
             it, rel = step()
             times.append(time.perf_counter() - t0)
+        nl, kms = ctx.kernel_stats("spmv_f64")
+        ctx.enable_timing(False)
         log(f"mg {name}: {1e3 * np.mean(times):.1f} ms per step, {it} iterations")
         xs = x.download()
         results[name] = dict(ms=1e3 * float(np.mean(times)), ms_min=1e3 * float(np.min(times)), iters=it,
                              relres=rel, rel_err=float(np.linalg.norm(xs - xref) / np.linalg.norm(xref)),
-                             l2=gp.l2_error(x), dofs_per_s=ndofs / float(np.mean(times)))
+                             l2=gp.l2_error(x), dofs_per_s=ndofs / float(np.mean(times)),
+                             fine_spmv_launches=nl, fine_spmv_ms=kms)
     # phase split of one mixed step
     ctx.enable_timing(True)
     ctx.timing_filter(None)
@@ -420,7 +445,14 @@ def bench_mg(args):
     ctx.enable_timing(False)
     mg.close(); A.close(); b.close()
     mixed = results["mixed"]
+    nf = gp.nfree
+    spmv_bytes = nnz * (8 + 4) + (nf + 1) * 4 + 2 * nf * 8
+    avg_ms = mixed["fine_spmv_ms"] / max(1, mixed["fine_spmv_launches"])
+    achieved = spmv_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     extra = {
+        "roofline": {"kernel": "fine-level CSR SpMV of the outer CG (FP64)", "bound": "hbm",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "algorithmic_bytes_per_launch": spmv_bytes, "avg_launch_ms": avg_ms},
         "config": {"dofs": ndofs, "free_dofs": gp.nfree, "nnz": nnz, "levels": levels,
                    "cycle": "V(2,2), damped Jacobi 0.6, exact FP64 coarse solve"},
         "fp64": results["fp64"], "mixed": mixed,
@@ -436,7 +468,7 @@ def bench_mg(args):
     x.close()
     gp.close()
     ctx.close()
-    print(json.dumps(out))
+    return out
 
 
 def bench_mg_cube(args):
@@ -578,6 +610,7 @@ def bench_batch(args):
             assert off == gp.nfree
         B.run(8)
         res = [B.result(i) for i in range(len(probs))]
+        launches[0] = B.graph_launches()
         ctx.synchronize()
         B.close()
         nnz = sum(A.nnz for A, _ in mats)
@@ -585,6 +618,7 @@ def bench_batch(args):
             A.close(); b.close()
         return res, nnz
 
+    launches = [0]
     xs = [ctx.array(gp.nfree) for _, gp in gps]
     for x in xs:
         x.zero()
@@ -618,6 +652,7 @@ def bench_batch(args):
                 off += nf
         its = [r[1] for r in res]
         results[name] = dict(ms=1e3 * float(np.mean(times)), ms_min=1e3 * float(np.min(times)),
+                             graph_launches_per_step=launches[0],
                              max_rel_err=max(errs), converged=sum(r[0] == 1 for r in res),
                              iters_max=max(its), iters_mean=float(np.mean(its)),
                              dofs_per_s=ndofs / float(np.mean(times)))
@@ -637,7 +672,7 @@ def bench_batch(args):
     for _, gp in gps:
         gp.close()
     ctx.close()
-    print(json.dumps(out))
+    return out
 
 
 def cpu_baseline_batch(probs, tol, budget_s=15.0):

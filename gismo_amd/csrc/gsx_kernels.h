@@ -14,11 +14,16 @@ struct SfBox {
     int nty, ntz;
 };
 bool sf_supported(int d, int p);
+bool sf_mfma(int p);   // degrees with the matrix-core variant (the default row assembly)
 int64_t sf_table_doubles(int p, const int *nel);
-// tables, element load vectors + det check, stencil rows (FP64, all slots of
-// free rows whose column is in the patch)
-hipError_t launch_sf_assemble(int p, const PatchDev *patches, const GaussTab &gt, double *tab, const SfBox &bx,
-                              double *Fbuf, double *detmin, double *s64, int64_t nelem, hipStream_t s);
+// doubles of the per-Gauss-point D (6 per point) of the matrix-core variant
+// (degrees 2, 3), 0 for the others
+int64_t sf_dq_doubles(int p, const int *nel);
+// tables, element load vectors + det check (+ D), stencil rows (FP64, all
+// slots of free rows whose column is in the patch); the tiling (nty, ntz)
+// is chosen here
+hipError_t launch_sf_assemble(int p, const PatchDev *patches, const GaussTab &gt, double *tab, SfBox bx,
+                              double *Fbuf, double *detmin, double *Dq, double *s64, int64_t nelem, hipStream_t s);
 // CSR values, FP32 half stencil (may be null), rhs; zeroes the slots of
 // columns outside the free box; err |= 1 on a CSR row length mismatch
 hipError_t launch_sf_finish(int p, const SfBox &bx, int nfree, const int *rowptr, double *s64, float *s32,

@@ -1017,7 +1017,8 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
     const bool rebuild = (mode & GSX_ASM_REBUILD_PATTERN) != 0;
     const bool csr_only = (mode & GSX_ASM_CSR_ONLY) != 0;
     const bool keep_elements = (mode & GSX_ASM_KEEP_ELEMENTS) != 0;
-    const bool elementwise = (mode & GSX_ASM_ELEMENTWISE) != 0 || (mode & GSX_ASM_ROWWISE) == 0;
+    const bool rowwise_forced = (mode & GSX_ASM_ROWWISE) != 0;
+    bool elementwise = (mode & GSX_ASM_ELEMENTWISE) != 0;
     mode &= ~(GSX_ASM_REBUILD_PATTERN | GSX_ASM_CSR_ONLY | GSX_ASM_KEEP_ELEMENTS | GSX_ASM_ELEMENTWISE |
               GSX_ASM_ROWWISE);
     P->kept_K.reset();
@@ -1034,6 +1035,8 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
         if (!P->d_detmin) return fail(c, GSX_ERR_OOM, "device allocation (Jacobian check)");
     }
     // single 3D patch: sum-factorised rows, no element matrices
+    // (by default where its matrix-core variant runs: degrees 2 and 3)
+    if (!rowwise_forced && !gsx::sf_mfma(P->hp[0].p[0])) elementwise = true;
     const bool use_sf = single && !elementwise && !keep_elements && !P->zero_kappa && P->hp[0].d == 3 &&
                         P->hp[0].p[0] == P->hp[0].p[1] && P->hp[0].p[0] == P->hp[0].p[2] &&
                         gsx::sf_supported(3, P->hp[0].p[0]);
@@ -1138,6 +1141,7 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
             sb.nty = (h.nb[1] + 1) / 2;
             sb.ntz = (h.nb[2] + 1) / 2;
             DevPtr tab = dalloc(c, sizeof(double) * gsx::sf_table_doubles(p, h.nel));
+            DevPtr dq = dalloc(c, sizeof(double) * std::max<int64_t>(1, gsx::sf_dq_doubles(p, h.nel)));
             DevPtr s64tmp;
             double *s64 = dp<double>(S.s64);
             if (!s64) {
@@ -1145,12 +1149,13 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
                 s64 = dp<double>(s64tmp);
             }
             DevPtr errf = dalloc(c, sizeof(int));
-            if (!tab || !s64 || !errf) return fail(c, GSX_ERR_OOM, "device allocation (row assembly)");
+            if (!tab || !dq || !s64 || !errf) return fail(c, GSX_ERR_OOM, "device allocation (row assembly)");
             HIP_OK(c, hipMemsetAsync(errf.get(), 0, sizeof(int), s));
             {
                 KScope k(c, "sf_assemble");
                 HIP_OK(c, gsx::launch_sf_assemble(p, dp<PatchDev>(P->d_patches), c->gt, dp<double>(tab), sb,
-                                                  dp<double>(Fbuf), dp<double>(P->d_detmin), s64, P->nelem, s));
+                                                  dp<double>(Fbuf), dp<double>(P->d_detmin), dp<double>(dq), s64,
+                                                  P->nelem, s));
             }
             {
                 KScope k(c, "sf_finish");
@@ -3681,6 +3686,22 @@ struct gsx_batch {
     bool dirty = true;
     std::vector<gsx::BChunk> chunks_h;
     std::vector<gsx::BState> state_h;
+    // one step (init, iters iterations, reliable update) captured as a HIP
+    // graph for the current member layout; replayed while it holds
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_iters = -1;
+    int64_t graph_launches = 0;
+    void drop_graph()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+        graph_iters = -1;
+    }
+    ~gsx_batch() { drop_graph(); }
 };
 
 gsx_status gsx_batch_create(gsx_ctx *c, int32_t precision, int32_t capacity, gsx_batch **out)
@@ -3848,25 +3869,47 @@ gsx_status gsx_batch_step(gsx_batch *B, int32_t iters, int32_t *running)
     if (!B || iters < 0) return GSX_ERR_ARG;
     gsx_ctx *c = B->c;
     hipStream_t s = c->stream;
-    if (B->dirty) GSX_TRY(batch_rebuild(B));
+    if (B->dirty) {
+        B->drop_graph();
+        GSX_TRY(batch_rebuild(B));
+    }
     const int nc = B->nchunks;
     if (nc > 0) {
         const gsx::BChunk *ch = dp<gsx::BChunk>(B->d_chunks);
         const gsx::BSeg *sg = dp<gsx::BSeg>(B->d_segs);
         gsx::BState *st = dp<gsx::BState>(B->d_state);
         double *P0 = dp<double>(B->d_parts), *P1 = P0 + B->chunk_cap, *P2 = P1 + B->chunk_cap;
-        {
-            KScope k(c, "batch_init");
-            HIP_OK(c, gsx::launch_batch_init(nc, ch, sg, st, P0, P1, P2, B->mixed, s));
-        }
-        {
-            KScope k(c, B->mixed ? "batch_iter_f32" : "batch_iter_f64");
-            for (int i = 0; i < iters; ++i)
-                HIP_OK(c, gsx::launch_batch_iteration(nc, ch, sg, st, P0, P1, P2, B->mixed, 1e-4, s));
-        }
-        if (B->mixed) {
-            KScope k(c, "batch_reliable");
-            HIP_OK(c, gsx::launch_batch_reliable(nc, ch, sg, st, P0, P2, s));
+        auto enqueue = [&]() -> hipError_t {
+            hipError_t e = gsx::launch_batch_init(nc, ch, sg, st, P0, P1, P2, B->mixed, s);
+            for (int i = 0; e == hipSuccess && i < iters; ++i)
+                e = gsx::launch_batch_iteration(nc, ch, sg, st, P0, P1, P2, B->mixed, 1e-4, s);
+            if (e == hipSuccess && B->mixed) e = gsx::launch_batch_reliable(nc, ch, sg, st, P0, P2, s);
+            return e;
+        };
+        if (B->use_graph && !c->timing) {
+            // the step's launches depend only on the member layout and iters:
+            // capture them once, replay every step until the layout changes
+            if (!B->gexec || B->graph_iters != iters) {
+                B->drop_graph();
+                HIP_OK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                const hipError_t e = enqueue();
+                hipGraph_t g = nullptr;
+                const hipError_t e2 = hipStreamEndCapture(s, &g);
+                if (e != hipSuccess || e2 != hipSuccess) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return fail(c, GSX_ERR_HIP, std::string("batch graph capture: ") +
+                                                    hipGetErrorString(e != hipSuccess ? e : e2));
+                }
+                B->graph = g;
+                HIP_OK(c, hipGraphInstantiate(&B->gexec, B->graph, nullptr, nullptr, 0));
+                B->graph_iters = iters;
+            }
+            KScope k(c, "batch_graph");
+            HIP_OK(c, hipGraphLaunch(B->gexec, s));
+            ++B->graph_launches;
+        } else {
+            KScope k(c, B->mixed ? "batch_step_f32" : "batch_step_f64");
+            HIP_OK(c, enqueue());
         }
         HIP_OK(c, hipMemcpyAsync(B->state_h.data(), st, sizeof(gsx::BState) * B->cap, hipMemcpyDeviceToHost, s));
         HIP_OK(c, hipStreamSynchronize(s));
@@ -3884,6 +3927,21 @@ gsx_status gsx_batch_step(gsx_batch *B, int32_t iters, int32_t *running)
         }
     }
     if (running) *running = run;
+    return GSX_OK;
+}
+
+gsx_status gsx_batch_set_graph(gsx_batch *B, int32_t enable)
+{
+    if (!B) return GSX_ERR_ARG;
+    B->use_graph = enable != 0;
+    if (!B->use_graph) B->drop_graph();
+    return GSX_OK;
+}
+
+gsx_status gsx_batch_graph_launches(const gsx_batch *B, int64_t *count)
+{
+    if (!B || !count) return GSX_ERR_ARG;
+    *count = B->graph_launches;
     return GSX_OK;
 }
 

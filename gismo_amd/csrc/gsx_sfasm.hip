@@ -367,12 +367,286 @@ __global__ __launch_bounds__(256) void k_sfasm3d(const PatchDev *__restrict__ pa
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same row assembly with the three contractions on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64; operand layout: A[i][k] in lane i + 16 k, B[k][j]
+// in lane j + 16 k, result [i][j] in lane j + 16 (i % 4), element i / 4).
+// D comes precomputed per Gauss point (k_load3d).  A block of 8 waves owns a
+// 4 x 4 tile of basis rows in (y, z); per Gauss point column gx:
+//   z:  T1_arr[gy][c]   = sum_gz D_s(arr)[gz][gy] Wz_f(arr)[gz][c]        (c = (izl, oz >= 0))
+//   y:  T3_cls[e][c]    = sum_(a,b) in cls sum_gy Wy_f(a,b)[e][gy] T1_arr(a,b)[gy][c]   (e = (iyl, oy))
+//   x:  acc[(ox, r)][(e, c)] += sum_cls X_cls[(ox, r)] T3_cls[e][c]
+// with Wz, Wy the products of the 1D basis values/derivatives of row and
+// column (x-independent, built once per block) and X_cls the same products
+// along x for the current point.  The x result rows are i = 4 r + g with
+// ox = 4 t + g - p: element r of a lane's result register is the ring
+// position (row ix = e + r), so moving to the next element shifts the four
+// elements of each register -- no data crosses lanes.
+// ---------------------------------------------------------------------------
+typedef double sf_d4 __attribute__((ext_vector_type(4)));
+
+template <int P>
+struct SfmCfg {
+    static constexpr int NB1 = P + 1, W = 2 * P + 1, NS = W * W * W;
+    static constexpr int TY = 4, TZ = 4;
+    static constexpr int NGY = (TY + P) * NB1, NGZ = (TZ + P) * NB1;
+    static constexpr int KZ = (NGZ + 3) / 4 * 4;       // z contraction depth (gz)
+    static constexpr int MY = (NGY + 15) / 16 * 16;    // z stage rows (gy)
+    static constexpr int KY = (NGY + 3) / 4 * 4;       // y contraction depth (gy)
+    static constexpr int NE = TY * W;                  // y stage rows (iyl, oy)
+    static constexpr int ME = (NE + 15) / 16 * 16;
+    static constexpr int NC = TZ * NB1;                // columns (izl, oz >= 0)
+    static constexpr int MTF = (W + 3) / 4;            // x stage row tiles (ox)
+    static constexpr int NTF = NE;                     // x stage column tiles (one per e)
+    static constexpr int NWAVE = 8;
+    static constexpr int FPW = (MTF * NTF + NWAVE - 1) / NWAVE;
+    static_assert(NC <= 16 && P <= 3, "one column tile, ring of at most four rows");
+};
+
+template <int P>
+__global__ __launch_bounds__(512) void k_sfasm3d_mfma(const double *__restrict__ tab,
+                                                      const double *__restrict__ Dq, SfBox bx,
+                                                      double *__restrict__ s64)
+{
+    using C = SfmCfg<P>;
+    constexpr int NB1 = C::NB1, W = C::W, NS = C::NS, TY = C::TY, TZ = C::TZ;
+    constexpr int KZ = C::KZ, MY = C::MY, KY = C::KY, NE = C::NE, ME = C::ME, NC = C::NC;
+    constexpr int MTF = C::MTF, NTF = C::NTF, FPW = C::FPW;
+    __shared__ double sWz[4][KZ][16];
+    __shared__ double sWy[4][ME][KY];
+    __shared__ double sD[6][KZ][MY];
+    __shared__ double sT1[8][KY][16];
+    __shared__ double sT3[5][ME][16];
+    __shared__ double sVx[NB1][NB1], sDx[NB1][NB1];
+
+    const int t = threadIdx.x, w = t / kWave, l = t % kWave;
+    const int ty = blockIdx.x % bx.nty, tz = blockIdx.x / bx.nty;
+    const int Y0 = ty * TY, Z0 = tz * TZ;
+    const int nbx = bx.nb[0], nby = bx.nb[1], nbz = bx.nb[2];
+    const int nelx = bx.nel[0], nely = bx.nel[1], nelz = bx.nel[2];
+    const int ey_lo = max(0, Y0 - P), ey_hi = min(nely - 1, Y0 + TY - 1);
+    const int ez_lo = max(0, Z0 - P), ez_hi = min(nelz - 1, Z0 + TZ - 1);
+    const int ngy = (ey_hi - ey_lo + 1) * NB1, ngz = (ez_hi - ez_lo + 1) * NB1;
+    const double *tx = tab + bx.toff[0], *tyb = tab + bx.toff[1], *tzb = tab + bx.toff[2];
+    const int Gx = bx.g[0], Gy = bx.g[1], Gz = bx.g[2];
+
+    // Wz[f][gz][c]: Bz^fa(iz, gz) Bz^fb(iz + oz, gz), c = izl (p+1) + oz
+    for (int k = t; k < 4 * KZ * 16; k += 512) {
+        const int c = k % 16, gz = (k / 16) % KZ, f = k / (16 * KZ);
+        double v = 0.0;
+        if (c < NC && gz < ngz) {
+            const int izl = c / NB1, oz = c % NB1;
+            const int iz = Z0 + izl, jz = iz + oz, ez = ez_lo + gz / NB1;
+            const int m1 = iz - ez, m2 = jz - ez;
+            if (iz < nbz && jz < nbz && m1 >= 0 && m1 <= P && m2 >= 0 && m2 <= P) {
+                const int64_t g = (int64_t)ez_lo * NB1 + gz;
+                const double a = (f & 2) ? tzb[(int64_t)Gz * NB1 + g * NB1 + m1] : tzb[g * NB1 + m1];
+                const double b = (f & 1) ? tzb[(int64_t)Gz * NB1 + g * NB1 + m2] : tzb[g * NB1 + m2];
+                v = a * b;
+            }
+        }
+        sWz[f][gz][c] = v;
+    }
+    // Wy[f][e][gy]: By^fa(iy, gy) By^fb(iy + oy, gy), e = iyl (2p+1) + oy + p
+    for (int k = t; k < 4 * ME * KY; k += 512) {
+        const int gy = k % KY, e = (k / KY) % ME, f = k / (KY * ME);
+        double v = 0.0;
+        if (e < NE && gy < ngy) {
+            const int iyl = e / W, oy = e % W - P;
+            const int iy = Y0 + iyl, jy = iy + oy, ey = ey_lo + gy / NB1;
+            const int m1 = iy - ey, m2 = jy - ey;
+            if (iy < nby && jy >= 0 && jy < nby && m1 >= 0 && m1 <= P && m2 >= 0 && m2 <= P) {
+                const int64_t g = (int64_t)ey_lo * NB1 + gy;
+                const double a = (f & 2) ? tyb[(int64_t)Gy * NB1 + g * NB1 + m1] : tyb[g * NB1 + m1];
+                const double b = (f & 1) ? tyb[(int64_t)Gy * NB1 + g * NB1 + m2] : tyb[g * NB1 + m2];
+                v = a * b;
+            }
+        }
+        sWy[f][e][gy] = v;
+    }
+
+    sf_d4 acc[FPW];
+#pragma unroll
+    for (int u = 0; u < FPW; ++u) acc[u] = sf_d4{0.0, 0.0, 0.0, 0.0};
+    const int li = l % 16, lk = l / 16;
+
+    // D of point column gc for the tile's (gz, gy) window, zero-padded:
+    // each thread holds its DPT entries in registers one column ahead
+    constexpr int DN = 6 * KZ * MY, DPT = (DN + 511) / 512;
+    double dpre[DPT];
+    auto d_load = [&](int64_t gc) {
+#pragma unroll
+        for (int u = 0; u < DPT; ++u) {
+            const int k = t + 512 * u;
+            double v = 0.0;
+            if (k < DN && gc < (int64_t)Gx) {
+                const int gy = k % MY, gz = (k / MY) % KZ, sc = k / (MY * KZ);
+                if (gy < ngy && gz < ngz) v = Dq[((gc * 6 + sc) * Gz + ez_lo * NB1 + gz) * Gy + ey_lo * NB1 + gy];
+            }
+            dpre[u] = v;
+        }
+    };
+    auto d_store = [&]() {
+#pragma unroll
+        for (int u = 0; u < DPT; ++u) {
+            const int k = t + 512 * u;
+            if (k < DN) (&sD[0][0][0])[k] = dpre[u];
+        }
+    };
+    auto x_tables = [&](int ex) {
+        if (t < NB1 * NB1) {
+            const int q = t / NB1, m = t % NB1;
+            const int64_t g = (int64_t)ex * NB1 + q;
+            sVx[q][m] = tx[g * NB1 + m];
+            sDx[q][m] = tx[(int64_t)Gx * NB1 + g * NB1 + m];
+        }
+    };
+    d_load(0);
+    d_store();
+    d_load(1);
+    x_tables(0);
+    __syncthreads();
+
+    // y-stage work units: (class, row tile, pair subset); class 3 in two halves
+    // (planes 3 and 4 of sT3, summed when the x stage reads them)
+    constexpr int NMT = ME / 16, NUNIT = 5 * NMT;
+    for (int ex = 0; ex < nelx; ++ex) {
+        for (int qx = 0; qx < NB1; ++qx) {
+            const int64_t gc = (int64_t)ex * NB1 + qx;
+            // z: arrays (a,b) = 00, 01 (= 10), 11 [z flags 00]; 20, 21 [10]; 02, 12 [01]; 22 [11];
+            // the wave's row tiles interleaved (independent accumulators)
+            {
+                constexpr int NT_D = MY / 16;
+                const int arr = w;
+                const int sidx = arr == 0 ? 0 : arr == 1 ? 1 : arr == 2 ? 3 : (arr == 3 || arr == 5) ? 2
+                                 : (arr == 4 || arr == 6) ? 4 : 5;
+                const int f = ((arr == 3 || arr == 4 || arr == 7) ? 2 : 0) + ((arr == 5 || arr == 6 || arr == 7) ? 1 : 0);
+                sf_d4 a4[NT_D];
+#pragma unroll
+                for (int mt = 0; mt < NT_D; ++mt) a4[mt] = sf_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int ks = 0; ks < KZ / 4; ++ks) {
+                    const double bv = sWz[f][4 * ks + lk][li];
+#pragma unroll
+                    for (int mt = 0; mt < NT_D; ++mt) {
+                        const double av = sD[sidx][4 * ks + lk][16 * mt + li];
+                        a4[mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, a4[mt], 0, 0, 0);
+                    }
+                }
+#pragma unroll
+                for (int mt = 0; mt < NT_D; ++mt)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int gy = 16 * mt + 4 * q + lk;
+                        if (gy < KY) sT1[arr][gy][li] = a4[mt][q];
+                    }
+            }
+            __syncthreads();
+            // sD is free: the next column's D goes in, the one after into registers
+            d_store();
+            d_load(gc + 2);
+            if (qx == 0) x_tables(ex);   // the previous element's x stage is done
+            // y: x-flag classes 0: (0,0); 1: (0,1),(0,2); 2: (1,0),(2,0); 3: (1,1),(1,2) | (2,1),(2,2)
+            for (int unit = w; unit < NUNIT; unit += 8) {
+                const int plane = (unit < 4 * NMT) ? 1 + unit / NMT : 0;   // units 0..: planes 1,2,3,4 then 0
+                const int mt = unit % NMT;
+                const int npair = plane == 0 ? 1 : 2;
+                sf_d4 a4 = sf_d4{0.0, 0.0, 0.0, 0.0}, b4 = sf_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int pp = 0; pp < 2; ++pp) {
+                    if (pp >= npair) break;
+                    int a, b;
+                    if (plane == 0) { a = 0; b = 0; }
+                    else if (plane == 1) { a = 0; b = 1 + pp; }
+                    else if (plane == 2) { a = 1 + pp; b = 0; }
+                    else if (plane == 3) { a = 1; b = 1 + pp; }
+                    else { a = 2; b = 1 + pp; }
+                    const int arr = (a == 0 && b == 0) ? 0 : (a + b == 1) ? 1 : (a == 1 && b == 1) ? 2
+                                    : (a == 2 && b == 0) ? 3 : (a == 2 && b == 1) ? 4 : (a == 0 && b == 2) ? 5
+                                    : (a == 1 && b == 2) ? 6 : 7;
+                    const int f = (a == 1 ? 2 : 0) + (b == 1 ? 1 : 0);
+#pragma unroll
+                    for (int ks = 0; ks < KY / 4; ++ks) {
+                        const double av = sWy[f][16 * mt + li][4 * ks + lk];
+                        const double bv = sT1[arr][4 * ks + lk][li];
+                        if (ks & 1) b4 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, b4, 0, 0, 0);
+                        else a4 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, a4, 0, 0, 0);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 4; ++q) sT3[plane][16 * mt + 4 * q + lk][li] = a4[q] + b4[q];
+            }
+            __syncthreads();
+            // x: rows i = 4 r + g (ox = 4 t + g - p), depth = the four classes
+            double xa[MTF];
+#pragma unroll
+            for (int tt = 0; tt < MTF; ++tt) {
+                const int r = li / 4, g = li % 4, ox = 4 * tt + g - P, r2 = r + ox, cls = lk;
+                double v = 0.0;
+                if (r <= P && ox <= P && r2 >= 0 && r2 <= P) {
+                    const bool fa = cls == 0 || cls == 1, fb = cls == 0 || cls == 2;
+                    v = (fa ? sDx[qx][r] : sVx[qx][r]) * (fb ? sDx[qx][r2] : sVx[qx][r2]);
+                }
+                xa[tt] = v;
+            }
+#pragma unroll
+            for (int u = 0; u < FPW; ++u) {
+                const int fi = w + 8 * u;
+                if (fi < MTF * NTF) {
+                    const int tt = fi / NTF, n = fi % NTF;
+                    const double bv = lk == 3 ? sT3[3][n][li] + sT3[4][n][li] : sT3[lk][n][li];
+                    acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[tt], bv, acc[u], 0, 0, 0);
+                }
+            }
+        }
+        // rows complete after this element: ring position 0 (all at the last element)
+        const int nrow = ex == nelx - 1 ? NB1 : 1;
+#pragma unroll
+        for (int u = 0; u < FPW; ++u) {
+            const int fi = w + 8 * u;
+            if (fi < MTF * NTF) {
+                const int tt = fi / NTF, n = fi % NTF;
+                const int ox = 4 * tt + lk - P;
+                const int iyl = n / W, oy = n % W - P, izl = li / NB1, oz = li % NB1;
+                const bool upper = oz > 0 || oy > 0 || (oy == 0 && ox >= 0);
+                if (li < NC && ox <= P && upper) {
+                    const int iy = Y0 + iyl, iz = Z0 + izl;
+                    for (int r = 0; r < nrow; ++r) {
+                        const int ix = ex + r;
+                        if (ix >= nbx || iy >= nby || iz >= nbz) continue;
+                        const double v = acc[u][r];
+                        const int slot = ((oz + P) * W + (oy + P)) * W + (ox + P);
+                        const bool fi_free = ix >= bx.flo[0] && ix <= bx.fhi[0] && iy >= bx.flo[1] &&
+                                             iy <= bx.fhi[1] && iz >= bx.flo[2] && iz <= bx.fhi[2];
+                        if (fi_free) {
+                            const int ri = (ix - bx.flo[0]) + bx.m[0] * ((iy - bx.flo[1]) + bx.m[1] * (iz - bx.flo[2]));
+                            s64[stencil_index(slot, ri, NS)] = v;
+                        }
+                        if (slot != NS / 2) {
+                            const int jx = ix + ox, jy = iy + oy, jz = iz + oz;
+                            const bool fj = jx >= bx.flo[0] && jx <= bx.fhi[0] && jy >= bx.flo[1] &&
+                                            jy <= bx.fhi[1] && jz >= bx.flo[2] && jz <= bx.fhi[2];
+                            if (fj) {
+                                const int rj = (jx - bx.flo[0]) +
+                                               bx.m[0] * ((jy - bx.flo[1]) + bx.m[1] * (jz - bx.flo[2]));
+                                s64[stencil_index(NS - 1 - slot, rj, NS)] = v;
+                            }
+                        }
+                    }
+                }
+                acc[u] = sf_d4{acc[u][1], acc[u][2], acc[u][3], 0.0};
+            }
+        }
+    }
+}
+
 // Element load vectors and the oriented minimum of det J (the load part of
 // k_elem3d): the geometry by sum factorisation over the element's control
 // points, f at the mapped points, F_e(i) = sum_q w_q ((N_i f) |det J|).
 template <int P>
 __global__ __launch_bounds__(128) void k_load3d(const PatchDev *__restrict__ patches, GaussTab gt,
-                                                double *__restrict__ Fbuf, double *__restrict__ detmin)
+                                                double *__restrict__ Fbuf, double *__restrict__ detmin,
+                                                double *__restrict__ Dq, int Gy, int Gz)
 {
     constexpr int NB1 = P + 1, NB = NB1 * NB1 * NB1, NQ = NB;
     __shared__ double sV[3][NB1][NB1], sD[3][NB1][NB1], sW[3][NB1];
@@ -439,6 +713,32 @@ __global__ __launch_bounds__(128) void k_load3d(const PatchDev *__restrict__ pat
         sMeas[q] = fabs(det);
         sDet[q] = det;
         sF[q] = source_value(pd.fkind, 3, x, pd.felem, e);
+        if (Dq) {
+            // D_ab = w |det J| kappa (J^-1 J^-T)_ab at the Gauss point, stored
+            // as [gx][ab][gz][gy] (gx = e_x (p+1) + q_x, likewise y, z)
+            double inv[3][3];
+            inv[0][0] = c00 / det;
+            inv[1][0] = c01 / det;
+            inv[2][0] = c02 / det;
+            inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+            inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+            inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+            inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+            inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+            inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+            double cq = sWq[q] * fabs(det);
+            if (pd.kappa) cq *= pd.kappa[e];
+            const int64_t gx = (int64_t)ei[0] * NB1 + qx, gy = (int64_t)ei[1] * NB1 + qy, gz = (int64_t)ei[2] * NB1 + qz;
+            int sidx = 0;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = a; b < 3; ++b) {
+                    const double m = inv[a][0] * inv[b][0] + inv[a][1] * inv[b][1] + inv[a][2] * inv[b][2];
+                    Dq[((gx * 6 + sidx) * Gz + gz) * Gy + gy] = cq * m;
+                    ++sidx;
+                }
+        }
     }
     __syncthreads();
     for (int i = t; i < NB; i += 128) {
@@ -524,17 +824,34 @@ int64_t sf_table_doubles(int p, const int *nel)
     return n;
 }
 
-hipError_t launch_sf_assemble(int p, const PatchDev *patches, const GaussTab &gt, double *tab, const SfBox &bx,
-                              double *Fbuf, double *detmin, double *s64, int64_t nelem, hipStream_t s)
+bool sf_mfma(int p) { return p == 2 || p == 3; }
+
+int64_t sf_dq_doubles(int p, const int *nel)
+{
+    if (!sf_mfma(p)) return 0;
+    return (int64_t)nel[0] * (p + 1) * 6 * ((int64_t)nel[1] * (p + 1)) * ((int64_t)nel[2] * (p + 1));
+}
+
+hipError_t launch_sf_assemble(int p, const PatchDev *patches, const GaussTab &gt, double *tab, SfBox bx,
+                              double *Fbuf, double *detmin, double *Dq, double *s64, int64_t nelem, hipStream_t s)
 {
     const int G = bx.g[0] + bx.g[1] + bx.g[2];
-    const int nblk = bx.nty * bx.ntz;
 #define GSX_SF(PP)                                                                                              \
     if (p == PP) {                                                                                              \
         hipLaunchKernelGGL((k_sf_tables<PP>), dim3((G + 255) / 256), dim3(256), 0, s, patches, gt, tab, bx.g[0], \
                            bx.g[1], bx.g[2]);                                                                   \
-        hipLaunchKernelGGL((k_load3d<PP>), dim3((unsigned)nelem), dim3(128), 0, s, patches, gt, Fbuf, detmin);  \
-        hipLaunchKernelGGL((k_sfasm3d<PP>), dim3(nblk), dim3(256), 0, s, patches, tab, bx, s64);                \
+        hipLaunchKernelGGL((k_load3d<PP>), dim3((unsigned)nelem), dim3(128), 0, s, patches, gt, Fbuf, detmin,   \
+                           sf_mfma(PP) ? Dq : nullptr, bx.g[1], bx.g[2]);                                       \
+        if (sf_mfma(PP)) {                                                                                      \
+            bx.nty = (bx.nb[1] + SfmCfg<(PP < 4 ? PP : 3)>::TY - 1) / SfmCfg<(PP < 4 ? PP : 3)>::TY;            \
+            bx.ntz = (bx.nb[2] + SfmCfg<(PP < 4 ? PP : 3)>::TZ - 1) / SfmCfg<(PP < 4 ? PP : 3)>::TZ;            \
+            hipLaunchKernelGGL((k_sfasm3d_mfma<(PP < 4 ? PP : 3)>), dim3(bx.nty * bx.ntz), dim3(512), 0, s, tab, \
+                               Dq, bx, s64);                                                                    \
+        } else {                                                                                                \
+            bx.nty = (bx.nb[1] + 1) / 2;                                                                        \
+            bx.ntz = (bx.nb[2] + 1) / 2;                                                                        \
+            hipLaunchKernelGGL((k_sfasm3d<PP>), dim3(bx.nty * bx.ntz), dim3(256), 0, s, patches, tab, bx, s64); \
+        }                                                                                                       \
         return hipGetLastError();                                                                               \
     }
     GSX_SF(1) GSX_SF(2) GSX_SF(3) GSX_SF(4)

@@ -145,6 +145,8 @@ SIGNATURES = {
     "gsx_batch_add": [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_double, _vp, _vp, C.c_double,
                       C.c_int32, _i32p],
     "gsx_batch_step": [_vp, C.c_int32, _i32p],
+    "gsx_batch_set_graph": [_vp, C.c_int32],
+    "gsx_batch_graph_launches": [_vp, _i64p],
     "gsx_batch_result": [_vp, C.c_int32, _i32p, _i32p, _dp],
     "gsx_batch_history": [_vp, C.c_int32, _dp, C.c_int32, _i32p],
     "gsx_batch_remove": [_vp, C.c_int32],
@@ -662,6 +664,15 @@ class Batch:
         out = np.zeros(n.value)
         _check(self.ctx.h, lib().gsx_batch_history(self.h, i, out.ctypes.data_as(_dp), n.value, C.byref(n)))
         return out
+
+    def set_graph(self, enable):
+        """Capture each step's launches as a HIP graph (default) or not."""
+        _check(self.ctx.h, lib().gsx_batch_set_graph(self.h, 1 if enable else 0))
+
+    def graph_launches(self):
+        n = C.c_int64()
+        _check(self.ctx.h, lib().gsx_batch_graph_launches(self.h, C.byref(n)))
+        return n.value
 
     def remove(self, i):
         _check(self.ctx.h, lib().gsx_batch_remove(self.h, i))

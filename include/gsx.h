@@ -53,7 +53,8 @@ typedef enum {
                                         the sum-factorised row assembly of a
                                         single 3D patch would run              */
     GSX_ASM_ROWWISE = 0x1000         /* flag: the sum-factorised row assembly
-                                        where it applies (single 3D patch)     */
+                                        also for degrees 1 and 4 (single 3D
+                                        patch; degrees 2, 3 take it anyway)    */
 } gsx_pattern_mode;
 
 typedef enum {
@@ -228,14 +229,15 @@ gsx_status gsx_assemble_bdr(gsx_problem *prob, double *rhs_dev);
  * column-major, src/gsCore/gsForwardDeclarations.h:279-280).
  * precision GSX_MIXED additionally keeps an FP32 copy of the values for the
  * inner iterations.  rhs_dev: nfree doubles (device) or NULL.
- * With GSX_ASM_ROWWISE a single 3D patch of equal degree 1..4 in every
- * direction whose kappa has no exact zero is assembled row by row by sum
- * factorisation, without element matrices (the same sums over the same
- * Gauss points, grouped by point columns instead of elements: values agree
- * with the element path to rounding).  There no entry is dropped by the
- * exact-zero rule: an entry with only exact-zero element contributions
- * needs kappa = 0 on every element they share.  Other problems,
- * GSX_ASM_KEEP_ELEMENTS and GSX_ASM_ELEMENTWISE take the element path. */
+ * A single 3D patch of equal degree 2 or 3 in every direction (1..4 with
+ * GSX_ASM_ROWWISE) whose kappa has no exact zero is assembled row by row by
+ * sum factorisation on the FP64 matrix cores, without element matrices (the
+ * same sums over the same Gauss points, grouped by point columns instead of
+ * elements: values agree with the element path to rounding).  There no
+ * entry is dropped by the exact-zero rule: an entry with only exact-zero
+ * element contributions needs kappa = 0 on every element they share.  Other
+ * problems, GSX_ASM_KEEP_ELEMENTS and GSX_ASM_ELEMENTWISE take the element
+ * path. */
 gsx_status gsx_assemble(gsx_problem *prob, int32_t pattern_mode,
                         int32_t precision, gsx_matrix **A_out,
                         double *rhs_dev);
@@ -451,8 +453,15 @@ gsx_status gsx_batch_add(gsx_batch *batch, gsx_matrix *A, int32_t row0,
                          int32_t maxit, int32_t *id);
 /* Initialise newly added members, run iters CG iterations of every running
  * member (members that finish stop on their own), then the reliable update
- * (mixed).  running: members still iterating afterwards. */
+ * (mixed).  running: members still iterating afterwards.  The step's
+ * launches are captured once as a HIP graph for the current set of members
+ * and replayed by every following step (one graph launch per step) until a
+ * member is added, finishes or is removed; with timing enabled on the
+ * context (gsx_ctx_enable_timing) the launches go out one by one instead. */
 gsx_status gsx_batch_step(gsx_batch *batch, int32_t iters, int32_t *running);
+/* Graph capture of the steps on (default) or off; graph launches so far. */
+gsx_status gsx_batch_set_graph(gsx_batch *batch, int32_t enable);
+gsx_status gsx_batch_graph_launches(const gsx_batch *batch, int64_t *count);
 /* status: 0 running, 1 converged, 2 iteration limit, 3 zero right-hand side. */
 gsx_status gsx_batch_result(const gsx_batch *batch, int32_t id, int32_t *status,
                             int32_t *iters, double *relres);

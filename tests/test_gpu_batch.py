@@ -188,3 +188,32 @@ def test_batch_zero_rhs_and_iteration_limit(ctx):
     xz.close()
     for m in members:
         m.close()
+
+
+@pytest.mark.parametrize("precision", [gsx.FP64, gsx.MIXED])
+def test_batch_graph_replays_equal_eager_launches(ctx, precision):
+    """Each batch step is one HIP graph launch, replayed until the member set
+    changes (a member finishing re-captures): the iterates of every replay
+    equal those of launching the same kernels one by one, bit for bit, for
+    FP64 and mixed members; the state is read back between replays."""
+    probs = _small_batch(count=10, seed=21)
+    members = [_Member(ctx, p, precision) for p in probs]
+    outs = []
+    for graph in (True, False):
+        B = gsx.Batch(ctx, precision, capacity=len(members))
+        B.set_graph(graph)
+        ids = []
+        for m in members:
+            m.x.zero()
+            ids.append(B.add(m.A, m.b, m.x, tol=1e-10, maxit=5000))
+        B.run(6)
+        launches = B.graph_launches()
+        assert (launches > 0) if graph else (launches == 0)
+        outs.append(([B.result(i) for i in ids], [m.x.download() for m in members]))
+        B.close()
+    (r1, x1), (r2, x2) = outs
+    assert r1 == r2
+    for a, b in zip(x1, x2):
+        assert np.array_equal(a, b)
+    for m in members:
+        m.close()

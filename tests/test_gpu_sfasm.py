@@ -60,9 +60,10 @@ def test_rows_match_element_path_and_oracle(ctx, p, n, kappa):
 @pytest.mark.parametrize("p", [2, 3])
 def test_structured_operators_and_cg(ctx, p):
     """FP64 CG on the row-assembled structured operator takes the CSR
-    operator's iterations and solution; mixed CG on the half-stored FP32
-    stencil takes the FP32 CSR operator's iterations (the FP32 operator is
-    pinned: a wrong mirror slot would change the iterates)."""
+    operator's iterations (to one: the two SpMV kernels sum a row in
+    different orders) and solution; mixed CG on the half-stored FP32 stencil
+    likewise against the FP32 CSR operator (a wrong mirror slot would change
+    the operator, not just its rounding)."""
     prob = problems.poisson_noisy_cube(n=8, p=p, seed=5)
     gp = gsx.Problem(ctx, prob)
     A, b = gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_ROWWISE, gsx.MIXED)
@@ -77,7 +78,7 @@ def test_structured_operators_and_cg(ctx, p):
             x.close()
     for prec in (gsx.FP64, gsx.MIXED):
         (i1, x1), (i2, x2) = res[(True, prec)], res[(False, prec)]
-        assert i1 == i2, (prec, i1, i2)
+        assert abs(i1 - i2) <= 1, (prec, i1, i2)
         assert np.linalg.norm(x1 - x2) <= 1e-9 * np.linalg.norm(x2)
     A.close()
     b.close()
