@@ -643,6 +643,7 @@ __global__ __launch_bounds__(512) void k_sfasm3d_mfma(const double *__restrict__
 // Element load vectors and the oriented minimum of det J (the load part of
 // k_elem3d): the geometry by sum factorisation over the element's control
 // points, f at the mapped points, F_e(i) = sum_q w_q ((N_i f) |det J|).
+// One block per element; F is stored [i][e].
 template <int P>
 __global__ __launch_bounds__(128) void k_load3d(const PatchDev *__restrict__ patches, GaussTab gt,
                                                 double *__restrict__ Fbuf, double *__restrict__ detmin,
@@ -749,7 +750,7 @@ __global__ __launch_bounds__(128) void k_load3d(const PatchDev *__restrict__ pat
             const double v = sV[0][qx][bx] * sV[1][qy][by] * sV[2][qz][bz];
             a += sWq[q] * ((v * sF[q]) * sMeas[q]);
         }
-        Fbuf[e * NB + i] = a;
+        Fbuf[(int64_t)i * gridDim.x + e] = a;   // [local function][element]: coalesced in the finish
     }
     if (t == 0) {
         double m = pd.orient * sDet[0];
@@ -758,61 +759,119 @@ __global__ __launch_bounds__(128) void k_load3d(const PatchDev *__restrict__ pat
     }
 }
 
+template <int P> struct SfFinish { static constexpr int R = P >= 4 ? 8 : 16; };
+
 // Free rows from the stencil: CSR values (free columns in slot order =
 // ascending columns), the FP32 upper half, the rhs (element load vectors
 // summed in element order, minus K_ij g_j over the eliminated columns), and
 // the slots of columns outside the free box set to 0 in the FP64 stencil.
+// A block takes R consecutive rows (within one stencil tile): their slots are
+// read slot by slot (R consecutive rows each) into LDS, and the CSR values of
+// the R rows, one contiguous range, are written entry by entry from there.
 template <int P>
 __global__ __launch_bounds__(256) void k_sf_finish3d(SfBox bx, int nfree, const int *__restrict__ rowptr,
                                                      double *__restrict__ s64, float *__restrict__ s32,
                                                      double *__restrict__ val, double *__restrict__ rhs,
-                                                     const double *__restrict__ Fbuf,
+                                                     const double *__restrict__ Fbuf, int64_t nelem,
                                                      const int *__restrict__ dofs,
                                                      const double *__restrict__ fixed, int *__restrict__ err)
 {
-    constexpr int NB1 = P + 1, W = 2 * P + 1, NS = W * W * W, NB = NB1 * NB1 * NB1;
+    constexpr int NB1 = P + 1, W = 2 * P + 1, NS = W * W * W;
     constexpr int C = NS / 2, NH = stencil_half(NS);
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nfree) return;
-    const int ix = bx.flo[0] + r % bx.m[0], iy = bx.flo[1] + (r / bx.m[0]) % bx.m[1],
-              iz = bx.flo[2] + r / (bx.m[0] * bx.m[1]);
-    const int base = rowptr[r], len = rowptr[r + 1] - base;
-    int pos = 0;
-    double corr = 0.0;
-    for (int s = 0; s < NS; ++s) {
-        const int ox = s % W - P, oy = (s / W) % W - P, oz = s / (W * W) - P;
-        const int jx = ix + ox, jy = iy + oy, jz = iz + oz;
-        const bool inb = jx >= 0 && jx < bx.nb[0] && jy >= 0 && jy < bx.nb[1] && jz >= 0 && jz < bx.nb[2];
-        const bool fr = jx >= bx.flo[0] && jx <= bx.fhi[0] && jy >= bx.flo[1] && jy <= bx.fhi[1] &&
-                        jz >= bx.flo[2] && jz <= bx.fhi[2];
-        const int64_t si = stencil_index(s, r, NS);
-        if (fr) {
-            const double v = s64[si];
-            if (pos < len) val[base + pos] = v;
-            ++pos;
-            if (s32 && s >= C) s32[stencil_index(s - C, r, NH)] = (float)v;
-        } else {
-            if (inb && fixed) {
-                const int64_t m = jx + (int64_t)bx.nb[0] * (jy + (int64_t)bx.nb[1] * jz);
-                corr += s64[si] * fixed[dofs[m] - nfree];
-            }
-            s64[si] = 0.0;
-            if (s32 && s >= C) s32[stencil_index(s - C, r, NH)] = 0.0f;
+    constexpr int R = SfFinish<P>::R, LD = R + 1;   // LD: odd stride in doubles against bank conflicts
+    static_assert(kStencilTile % R == 0, "a block's rows lie in one tile");
+    __shared__ double L[NS * LD];
+    __shared__ int sBase[R + 1], sLo[R][3], sN[R][3];
+    const int r0 = blockIdx.x * R, t = threadIdx.x;
+    const int nr = min(R, nfree - r0);
+    if (t <= nr) sBase[t] = rowptr[r0 + t];
+    if (t < nr) {
+        const int r = r0 + t;
+        const int i3[3] = {bx.flo[0] + r % bx.m[0], bx.flo[1] + (r / bx.m[0]) % bx.m[1],
+                           bx.flo[2] + r / (bx.m[0] * bx.m[1])};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int lo = max(-P, bx.flo[k] - i3[k]), hi = min(P, bx.fhi[k] - i3[k]);
+            sLo[t][k] = lo;
+            sN[t][k] = hi - lo + 1;
         }
     }
-    if (pos != len) atomicOr(err, 1);
-    double racc = 0.0;
-    const int ex0 = max(0, ix - P), ex1 = min(bx.nel[0] - 1, ix);
-    const int ey0 = max(0, iy - P), ey1 = min(bx.nel[1] - 1, iy);
-    const int ez0 = max(0, iz - P), ez1 = min(bx.nel[2] - 1, iz);
-    for (int ez = ez0; ez <= ez1; ++ez)
-        for (int ey = ey0; ey <= ey1; ++ey)
-            for (int ex = ex0; ex <= ex1; ++ex) {
-                const int64_t e = ex + (int64_t)bx.nel[0] * (ey + (int64_t)bx.nel[1] * ez);
-                const int la = (ix - ex) + NB1 * ((iy - ey) + NB1 * (iz - ez));
-                racc += Fbuf[e * NB + la];
+    for (int q = t; q < NS * R; q += 256) {
+        const int s = q / R, rr = q % R;
+        if (rr < nr) L[s * LD + rr] = s64[stencil_index(s, r0 + rr, NS)];
+    }
+    __syncthreads();
+    if (t < nr && sN[t][0] * sN[t][1] * sN[t][2] != sBase[t + 1] - sBase[t]) atomicOr(err, 1);
+    // CSR values: entry k of the block's range -> (row, position) -> slot
+    const int b0 = sBase[0], tot = sBase[nr] - b0;
+    int rr = 0;
+    for (int k = t; k < tot; k += 256) {
+        while (rr + 1 < nr && sBase[rr + 1] - b0 <= k) ++rr;
+        const int pos = k - (sBase[rr] - b0);
+        const int nx = sN[rr][0], ny = sN[rr][1];
+        if (pos >= nx * ny * sN[rr][2]) continue;   // row length mismatch, flagged above
+        const int ox = sLo[rr][0] + pos % nx, oy = sLo[rr][1] + (pos / nx) % ny, oz = sLo[rr][2] + pos / (nx * ny);
+        val[b0 + k] = L[(((oz + P) * W + (oy + P)) * W + (ox + P)) * LD + rr];
+    }
+    // FP32 upper half; FP64 slots of columns outside the free box to 0
+    for (int q = t; q < NS * R; q += 256) {
+        const int s = q / R, rw = q % R;
+        if (rw >= nr) continue;
+        const int ox = s % W - P - sLo[rw][0], oy = (s / W) % W - P - sLo[rw][1], oz = s / (W * W) - P - sLo[rw][2];
+        const bool fr = ox >= 0 && ox < sN[rw][0] && oy >= 0 && oy < sN[rw][1] && oz >= 0 && oz < sN[rw][2];
+        if (!fr) s64[stencil_index(s, r0 + rw, NS)] = 0.0;
+        if (s32 && s >= C) s32[stencil_index(s - C, r0 + rw, NH)] = fr ? (float)L[s * LD + rw] : 0.0f;
+    }
+    // rhs: thread (row, c, b) sums the loads of the elements (ex, iy - P + b,
+    // iz - P + c) touching the row over ex; the row then sums its (c, b)
+    // partials in order.  Likewise the eliminated columns' K_ij g_j, by slot
+    // classes.
+    constexpr int NYZ = NB1 * NB1;
+    static_assert(R * NYZ <= 256, "one thread per (row, c, b)");
+    __shared__ double sF[R][NYZ], sC[R][NYZ];
+    if (t < R * NYZ) {
+        const int rw = t % R, j = t / R, c = j / NB1, b = j % NB1;   // consecutive lanes: consecutive rows
+        double f = 0.0, g = 0.0;
+        if (rw < nr) {
+            const int r = r0 + rw;
+            const int ix = bx.flo[0] + r % bx.m[0], iy = bx.flo[1] + (r / bx.m[0]) % bx.m[1],
+                      iz = bx.flo[2] + r / (bx.m[0] * bx.m[1]);
+            const int ey = iy - P + b, ez = iz - P + c;
+            if (ey >= 0 && ey < bx.nel[1] && ez >= 0 && ez < bx.nel[2]) {
+                const int64_t eyz = (int64_t)bx.nel[0] * (ey + (int64_t)bx.nel[1] * ez);
+                const int lyz = NB1 * ((P - b) + NB1 * (P - c));
+#pragma unroll
+                for (int a = 0; a <= P; ++a) {
+                    const int ex = ix - P + a;
+                    if (ex >= 0 && ex < bx.nel[0]) f += Fbuf[(int64_t)(lyz + P - a) * nelem + eyz + ex];
+                }
             }
-    rhs[r] = racc - corr;
+            if (fixed) {
+                for (int s = j; s < NS; s += NYZ) {
+                    const int ox = s % W - P, oy = (s / W) % W - P, oz = s / (W * W) - P;
+                    const int jx = ix + ox, jy = iy + oy, jz = iz + oz;
+                    const bool inb = jx >= 0 && jx < bx.nb[0] && jy >= 0 && jy < bx.nb[1] && jz >= 0 && jz < bx.nb[2];
+                    const bool fr = jx >= bx.flo[0] && jx <= bx.fhi[0] && jy >= bx.flo[1] && jy <= bx.fhi[1] &&
+                                    jz >= bx.flo[2] && jz <= bx.fhi[2];
+                    if (inb && !fr) {
+                        const int64_t m = jx + (int64_t)bx.nb[0] * (jy + (int64_t)bx.nb[1] * jz);
+                        g += L[s * LD + rw] * fixed[dofs[m] - nfree];
+                    }
+                }
+            }
+        }
+        sF[rw][j] = f;
+        sC[rw][j] = g;
+    }
+    __syncthreads();
+    if (t < nr) {
+        double racc = 0.0, corr = 0.0;
+        for (int j = 0; j < NYZ; ++j) {
+            racc += sF[t][j];
+            corr += sC[t][j];
+        }
+        rhs[r0 + t] = racc - corr;
+    }
 }
 
 bool sf_supported(int d, int p) { return d == 3 && p >= 1 && p <= 4; }
@@ -864,11 +923,12 @@ hipError_t launch_sf_finish(int p, const SfBox &bx, int nfree, const int *rowptr
                             int *err, hipStream_t s)
 {
     if (nfree <= 0) return hipSuccess;
-    const dim3 grid((nfree + 255) / 256), blk(256);
+    const int64_t nelem = (int64_t)bx.nel[0] * bx.nel[1] * bx.nel[2];
 #define GSX_SFF(PP)                                                                                             \
     if (p == PP) {                                                                                              \
+        const dim3 grid((nfree + SfFinish<PP>::R - 1) / SfFinish<PP>::R), blk(256);                             \
         hipLaunchKernelGGL((k_sf_finish3d<PP>), grid, blk, 0, s, bx, nfree, rowptr, s64, s32, val, rhs, Fbuf,   \
-                           dofs, fixed, err);                                                                   \
+                           nelem, dofs, fixed, err);                                                            \
         return hipGetLastError();                                                                               \
     }
     GSX_SFF(1) GSX_SFF(2) GSX_SFF(3) GSX_SFF(4)

@@ -174,12 +174,19 @@ __global__ __launch_bounds__(256) void k_sspmv(int nfree, const int *__restrict_
 }
 
 // The same SpMV on the half-stored FP32 operator (stencil_half layout): an
-// upper slot is read from the row itself, a lower slot s from slot
-// NS - 1 - s of the neighbour row (free box m0 x m1 x m2, rows
-// lexicographic), which holds the same coefficient since the operator is
-// bitwise symmetric.  Slots are summed in the order of k_sspmv, so y is
-// bitwise the same; neighbours outside the free box contribute 0 (x is 0
-// there).
+// upper slot is read from the row itself, a lower slot from the neighbour row
+// at that offset (free box m0 x m1 x m2, rows lexicographic), whose mirrored
+// upper slot holds the same coefficient since the operator is bitwise
+// symmetric.  Neighbours outside the free box contribute 0 (x is 0 there).
+//
+// Order: each upper x-line of the stencil, (dz, dy) > 0 lexicographically
+// (the centre line from dx = 0), is taken together with its mirror line
+// (-dz, -dy).  Both read the same slot plane of the operator, the mirror one
+// at the rows shifted by the offset, so a block reads the plane at the same
+// point of its loop as the blocks that own those rows (up to 3 z-planes back,
+// on the same XCD through xcd_block) and the second read hits the L2.  With
+// the lexicographic order the z-lower mirrors were read at the start of the
+// loop and their owners read them at its end: each was fetched from HBM twice.
 template <int D, int P, typename T, int MODE>
 __global__ __launch_bounds__(256) void k_sspmv_half(int nfree, const int *__restrict__ free2grid,
                                                     int sy, int sz, int m0, int m1, int m2,
@@ -194,6 +201,9 @@ __global__ __launch_bounds__(256) void k_sspmv_half(int nfree, const int *__rest
     constexpr int W = 2 * P + 1;
     constexpr int NS = (D == 3) ? W * W * W : W * W;
     constexpr int C = NS / 2, NH = stencil_half(NS);
+    // full upper x-lines after the centre one: dz = 0, dy = 1..P, then (3D)
+    // dz = 1..P, dy = -P..P
+    constexpr int NL = (D == 3) ? P + P * W : P;
     double contrib = 0.0;
     const int blk = xcd_block(blockIdx.x, gridDim.x);
     for (int r = blk * 256 + threadIdx.x; r < nfree; r += gridDim.x * 256) {
@@ -202,33 +212,26 @@ __global__ __launch_bounds__(256) void k_sspmv_half(int nfree, const int *__rest
         const T *xb = x + g;
         const float *vb = sval + stencil_index(0, r, NH);
         T acc = 0;
-        auto coef = [&](int slot, int dx, int dy, int dz) -> float {
-            if (slot >= C) return vb[(slot - C) * kStencilTile];
-            const bool ok = ia + dx >= 0 && ia + dx < m0 && ib + dy >= 0 && ib + dy < m1 &&
-                            (D == 2 || (ic + dz >= 0 && ic + dz < m2));
-            const int rn = r + dx + m0 * (dy + m1 * dz);
-            return ok ? sval[stencil_index(NS - 1 - slot - C, rn, NH)] : 0.0f;
-        };
-        if (D == 3) {
+        // centre line: own slots dx = 0..P (half slots 0..P), mirrors dx = -P..-1
+#pragma unroll
+        for (int dx = 0; dx <= P; ++dx) acc += (T)vb[dx * kStencilTile] * xb[dx];
+#pragma unroll
+        for (int dx = -P; dx < 0; ++dx)
+            acc += (T)(ia + dx >= 0 ? sval[stencil_index(-dx, r + dx, NH)] : 0.0f) * xb[dx];
 #pragma unroll 1
-            for (int dz = -P; dz <= P; ++dz) {
+        for (int u = 0; u < NL; ++u) {
+            const int dz = (D == 2 || u < P) ? 0 : 1 + (u - P) / W;
+            const int dy = (D == 2 || u < P) ? u + 1 : (u - P) % W - P;
+            const int h0 = (D == 3 ? ((dz + P) * W + (dy + P)) * W : (dy + P) * W) - C;   // (dz, dy, -P)
+            const int base = dz * sz + dy * sy;
 #pragma unroll
-                for (int dy = -P; dy <= P; ++dy) {
-                    const int base = dz * sz + dy * sy;
-                    const int slot0 = ((dz + P) * W + (dy + P)) * W;
+            for (int dx = -P; dx <= P; ++dx) acc += (T)vb[(h0 + dx + P) * kStencilTile] * xb[base + dx];
+            const bool okyz = ib - dy >= 0 && ib - dy < m1 && ic - dz >= 0;
+            const int rm = r - m0 * (dy + m1 * dz);
 #pragma unroll
-                    for (int dx = -P; dx <= P; ++dx)
-                        acc += (T)coef(slot0 + dx + P, dx, dy, dz) * xb[base + dx];
-                }
-            }
-        } else {
-#pragma unroll
-            for (int dy = -P; dy <= P; ++dy) {
-                const int base = dy * sy;
-                const int slot0 = (dy + P) * W;
-#pragma unroll
-                for (int dx = -P; dx <= P; ++dx)
-                    acc += (T)coef(slot0 + dx + P, dx, dy, 0) * xb[base + dx];
+            for (int dx = -P; dx <= P; ++dx) {
+                const bool ok = okyz && ia + dx >= 0 && ia + dx < m0;
+                acc += (T)(ok ? sval[stencil_index(h0 - dx + P, rm + dx, NH)] : 0.0f) * xb[dx - base];
             }
         }
         if (MODE == 2) {
