@@ -124,6 +124,135 @@ __device__ inline void chunk_residual_any(const BSeg &g, int lo, int hi, double 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Structured members: one thread per row, a chunk is one stencil tile of the
+// member (256 rows from row0).  Vectors are indexed by row; the neighbour at
+// offset (dx, dy, dz) of member row l is row l + dx + m0 (dy + m1 dz) when it
+// lies in the box.  Slots whose neighbour is outside the box hold 0, and the
+// vector is then read at the row itself (finite), so the product is 0.
+// ---------------------------------------------------------------------------
+template <int D, int P>
+struct BoxGeo {
+    static constexpr int W = 2 * P + 1;
+    static constexpr int NS = (D == 3) ? W * W * W : W * W;
+    static constexpr int C = NS / 2, NH = stencil_half(NS);
+};
+
+// FP64, all slots in lexicographic (= column) order: y = A x or y = b - A x
+template <int D, int P, bool RESID>
+__device__ inline void box_apply64(const BSeg &g, int lo, int hi, const double *__restrict__ x,
+                                   double *__restrict__ y, double &dot)
+{
+    using G = BoxGeo<D, P>;
+    const int m0 = g.bm[0], m1 = g.bm[1], m2 = g.bm[2];
+    const int r = lo + threadIdx.x;
+    if (r >= hi) return;
+    const int l = r - g.row0;
+    const int ia = l % m0, ib = (l / m0) % m1, ic = (D == 3) ? l / (m0 * m1) : 0;
+    const double *vb = g.s64 + stencil_index(0, l, G::NS);
+    double acc = 0.0;
+#pragma unroll 1
+    for (int dz = (D == 3 ? -P : 0); dz <= (D == 3 ? P : 0); ++dz) {
+#pragma unroll
+        for (int dy = -P; dy <= P; ++dy) {
+            const bool okyz = ib + dy >= 0 && ib + dy < m1 && ic + dz >= 0 && ic + dz < m2;
+            const int rb = r + m0 * (dy + m1 * dz);
+            const int slot0 = ((D == 3 ? (dz + P) * G::W : 0) + (dy + P)) * G::W;
+#pragma unroll
+            for (int dx = -P; dx <= P; ++dx) {
+                const bool ok = okyz && ia + dx >= 0 && ia + dx < m0;
+                acc += vb[(slot0 + dx + P) * kStencilTile] * x[ok ? rb + dx : r];
+            }
+        }
+    }
+    if (RESID) {
+        const double v = g.b[r] - acc;
+        y[r] = v;
+        dot += v * v;
+    } else {
+        y[r] = acc;
+        dot += x[r] * acc;
+    }
+}
+
+// FP32 half stencil: each upper x-line with its mirror line (the order of the
+// single-patch kernel k_sspmv_half)
+template <int D, int P>
+__device__ inline void box_apply32(const BSeg &g, int lo, int hi, const float *__restrict__ x,
+                                   float *__restrict__ y, double &dot)
+{
+    using G = BoxGeo<D, P>;
+    constexpr int W = G::W, C = G::C, NH = G::NH;
+    constexpr int NL = (D == 3) ? P + P * W : P;
+    const int m0 = g.bm[0], m1 = g.bm[1], m2 = g.bm[2];
+    const int r = lo + threadIdx.x;
+    if (r >= hi) return;
+    const int l = r - g.row0;
+    const int ia = l % m0, ib = (l / m0) % m1, ic = (D == 3) ? l / (m0 * m1) : 0;
+    const float *sv = g.s32;
+    const float *vb = sv + stencil_index(0, l, NH);
+    float acc = 0.0f;
+#pragma unroll
+    for (int dx = 0; dx <= P; ++dx) acc += vb[dx * kStencilTile] * x[ia + dx < m0 ? r + dx : r];
+#pragma unroll
+    for (int dx = -P; dx < 0; ++dx)
+        acc += (ia + dx >= 0 ? sv[stencil_index(-dx, l + dx, NH)] * x[r + dx] : 0.0f);
+#pragma unroll 1
+    for (int u = 0; u < NL; ++u) {
+        const int dz = (D == 2 || u < P) ? 0 : 1 + (u - P) / W;
+        const int dy = (D == 2 || u < P) ? u + 1 : (u - P) % W - P;
+        const int h0 = (D == 3 ? ((dz + P) * W + (dy + P)) * W : (dy + P) * W) - C;   // (dz, dy, -P)
+        const int off = m0 * (dy + m1 * dz);
+        const bool upyz = ib + dy >= 0 && ib + dy < m1 && ic + dz < m2;
+#pragma unroll
+        for (int dx = -P; dx <= P; ++dx) {
+            const bool ok = upyz && ia + dx >= 0 && ia + dx < m0;
+            acc += vb[(h0 + dx + P) * kStencilTile] * x[ok ? r + off + dx : r];
+        }
+        const bool okyz = ib - dy >= 0 && ib - dy < m1 && ic - dz >= 0;
+#pragma unroll
+        for (int dx = -P; dx <= P; ++dx) {
+            const bool ok = okyz && ia + dx >= 0 && ia + dx < m0;
+            acc += (ok ? sv[stencil_index(h0 - dx + P, l - off + dx, NH)] * x[r - off + dx] : 0.0f);
+        }
+    }
+    y[r] = acc;
+    dot += (double)x[r] * (double)acc;
+}
+
+#define GSX_BOX_CASES(MACRO) \
+    MACRO(2, 1) MACRO(2, 2) MACRO(2, 3) MACRO(2, 4) MACRO(3, 1) MACRO(3, 2) MACRO(3, 3) MACRO(3, 4)
+
+template <bool RESID>
+__device__ inline void box_apply64_any(const BSeg &g, int lo, int hi, const double *x, double *y, double &dot)
+{
+    switch (g.bd * 8 + g.bp) {
+#define GSX_C(D, P) \
+    case D * 8 + P: box_apply64<D, P, RESID>(g, lo, hi, x, y, dot); break;
+        GSX_BOX_CASES(GSX_C)
+#undef GSX_C
+    default: break;
+    }
+}
+
+__device__ inline void box_apply32_any(const BSeg &g, int lo, int hi, const float *x, float *y, double &dot)
+{
+    switch (g.bd * 8 + g.bp) {
+#define GSX_C(D, P) \
+    case D * 8 + P: box_apply32<D, P>(g, lo, hi, x, y, dot); break;
+        GSX_BOX_CASES(GSX_C)
+#undef GSX_C
+    default: break;
+    }
+}
+
+// r = b - A x (FP64) of a chunk of either kind
+__device__ inline void chunk_residual_member(const BSeg &g, int lo, int hi, double &rr)
+{
+    if (g.bd) box_apply64_any<true>(g, lo, hi, g.x, g.r, rr);
+    else chunk_residual_any(g, lo, hi, rr);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -141,7 +270,7 @@ __global__ __launch_bounds__(256) void k_b_init1(const BChunk *__restrict__ chun
     if (st[ch.seg].phase != 0) return;
     const BSeg g = segs[ch.seg];
     double rr = 0.0, bb = 0.0;
-    chunk_residual_any(g, ch.lo, ch.hi, rr);
+    chunk_residual_member(g, ch.lo, ch.hi, rr);
     for (int i = ch.lo + threadIdx.x; i < ch.hi; i += 256) bb += g.b[i] * g.b[i];
     const double a = bsum256(rr, s), c = bsum256(bb, s);
     if (threadIdx.x == 0) {
@@ -242,8 +371,13 @@ __global__ __launch_bounds__(256) void k_b_spmv(const BChunk *__restrict__ chunk
     if ((int)blockIdx.x == g.c0 && threadIdx.x == 0) S.it = S.it_next;
     if (S.flag_next && S.flag_next <= S.it_next) return;
     double dot = 0.0;
-    if constexpr (sizeof(T) == 4) chunk_spmv_any<float, float>(g, ch.lo, ch.hi, g.v32, g.p32, g.t32, dot);
-    else chunk_spmv_any<double, double>(g, ch.lo, ch.hi, g.v64, g.p, g.t, dot);
+    if constexpr (sizeof(T) == 4) {
+        if (g.bd) box_apply32_any(g, ch.lo, ch.hi, g.p32, g.t32, dot);
+        else chunk_spmv_any<float, float>(g, ch.lo, ch.hi, g.v32, g.p32, g.t32, dot);
+    } else {
+        if (g.bd) box_apply64_any<false>(g, ch.lo, ch.hi, g.p, g.t, dot);
+        else chunk_spmv_any<double, double>(g, ch.lo, ch.hi, g.v64, g.p, g.t, dot);
+    }
     const double a = bsum256(dot, s);
     if (threadIdx.x == 0) p0[blockIdx.x] = a;
 }
@@ -369,7 +503,7 @@ __global__ __launch_bounds__(256) void k_b_rel2(const BChunk *__restrict__ chunk
     if (S.phase != 1 || S.done || !S.need_update) return;
     const BSeg g = segs[ch.seg];
     double rr = 0.0;
-    chunk_residual_any(g, ch.lo, ch.hi, rr);
+    chunk_residual_member(g, ch.lo, ch.hi, rr);
     const double a = bsum256(rr, s);
     if (threadIdx.x == 0) p0[blockIdx.x] = a;
 }
@@ -446,6 +580,39 @@ __global__ __launch_bounds__(256) void k_b_check(int row0, int nrows, const int 
     if (bad) flag[0] = 1;
 }
 
+// Structured operator of a box member from its CSR rows: every entry of
+// member row l must couple to a row of the box within the stencil window
+// (else flag[0] = 1); the FP64 stencil takes all slots, the FP32 one the
+// upper half.  Both arrays are zero beforehand (slots without an entry).
+template <int D, int P>
+__global__ __launch_bounds__(256) void k_b_box_build(int row0, int n, int m0, int m1, const int *__restrict__ rp,
+                                                     const int *__restrict__ ci, const double *__restrict__ val,
+                                                     double *__restrict__ s64, float *__restrict__ s32,
+                                                     int *__restrict__ flag)
+{
+    using G = BoxGeo<D, P>;
+    const int l = blockIdx.x * 256 + threadIdx.x;
+    if (l >= n) return;
+    const int ia = l % m0, ib = (l / m0) % m1, ic = l / (m0 * m1);
+    bool bad = false;
+    for (int k = rp[row0 + l]; k < rp[row0 + l + 1]; ++k) {
+        const int j = ci[k] - row0;
+        if (j < 0 || j >= n) {
+            bad = true;
+            continue;
+        }
+        const int dx = j % m0 - ia, dy = (j / m0) % m1 - ib, dz = j / (m0 * m1) - ic;
+        if (dx < -P || dx > P || dy < -P || dy > P || dz < (D == 3 ? -P : 0) || dz > (D == 3 ? P : 0)) {
+            bad = true;
+            continue;
+        }
+        const int slot = ((D == 3 ? (dz + P) * G::W : 0) + (dy + P)) * G::W + (dx + P);
+        s64[stencil_index(slot, l, G::NS)] = val[k];
+        if (s32 && slot >= G::C) s32[stencil_index(slot - G::C, l, G::NH)] = (float)val[k];
+    }
+    if (bad) flag[0] = 1;
+}
+
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
@@ -455,6 +622,26 @@ hipError_t launch_batch_check(int row0, int nrows, const int *rp, const int *ci,
 {
     hipLaunchKernelGGL(k_b_check, dim3((nrows + 3) / 4), dim3(256), 0, s, row0, nrows, rp, ci, flag);
     return hipGetLastError();
+}
+
+int64_t batch_box_slots(int d, int p, bool half)
+{
+    const int w = 2 * p + 1, ns = d == 3 ? w * w * w : w * w;
+    return half ? stencil_half(ns) : ns;
+}
+
+hipError_t launch_batch_box_build(int d, int p, int row0, int n, int m0, int m1, const int *rp, const int *ci,
+                                  const double *val, double *s64, float *s32, int *flag, hipStream_t s)
+{
+    const dim3 grid((n + 255) / 256), blk(256);
+#define GSX_C(D, P)                                                                                             \
+    if (d == D && p == P) {                                                                                     \
+        hipLaunchKernelGGL((k_b_box_build<D, P>), grid, blk, 0, s, row0, n, m0, m1, rp, ci, val, s64, s32, flag); \
+        return hipGetLastError();                                                                               \
+    }
+    GSX_BOX_CASES(GSX_C)
+#undef GSX_C
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_batch_init(int nchunks, const BChunk *chunks, const BSeg *segs, BState *st, double *p0,

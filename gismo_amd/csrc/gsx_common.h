@@ -134,6 +134,15 @@ struct BSeg {
     int maxit;
     int lpr;             // lanes per row of the SpMV
     int c0, nc;          // first chunk and chunk count
+    // Structured member (bd > 0): the rows are the free functions of one
+    // tensor-product patch, a bm[0] x bm[1] x bm[2] box in lexicographic
+    // order starting at row0, coupled through the (2 bp + 1)^bd stencil.
+    // Coefficients in stencil layout over the member's rows: s64 every
+    // slot, s32 the upper half (zero where the neighbour is outside the box).
+    int bd, bp, row0;
+    int bm[3];
+    const double *s64;
+    const float *s32;
 };
 
 struct BChunk {

@@ -3664,6 +3664,7 @@ struct BMember {
     int row0 = 0, nrows = 0;
     int maxit = 0;
     DevPtr r, z, p, t, r32, z32, p32, t32, xc32, d32, hist;
+    DevPtr s64, s32;          // structured operator of a box member
     gsx::BSeg seg{};
     gsx::BState st{};
 };
@@ -3732,12 +3733,20 @@ gsx_status gsx_batch_destroy(gsx_batch *B)
     return GSX_OK;
 }
 
-gsx_status gsx_batch_add(gsx_batch *B, gsx_matrix *A, int32_t row0, int32_t nrows, int32_t precond, double tau,
-                         const double *b, double *x, double tol, int32_t maxit, int32_t *id)
+static gsx_status batch_add(gsx_batch *B, gsx_matrix *A, int32_t row0, int32_t nrows, const int32_t *box,
+                            int32_t precond, double tau, const double *b, double *x, double tol, int32_t maxit,
+                            int32_t *id)
 {
     if (!B || !A || !b || !x || !id || maxit < 1 || nrows <= 0 || row0 < 0 || (int64_t)row0 + nrows > A->n)
         return GSX_ERR_ARG;
     gsx_ctx *c = B->c;
+    if (box) {
+        const int d = box[0], p = box[1];
+        const int64_t cells = (int64_t)box[2] * box[3] * (d == 3 ? box[4] : 1);
+        if ((d != 2 && d != 3) || p < 1 || p > 4 || box[2] < 1 || box[3] < 1 || (d == 3 && box[4] < 1) ||
+            (d == 2 && box[4] != 1) || cells != nrows)
+            return fail(c, GSX_ERR_ARG, "box member: d in {2, 3}, p in 1..4, m0 m1 m2 = nrows (m2 = 1 in 2D)");
+    }
     if (A->ctx->device != c->device) return fail(c, GSX_ERR_ARG, "matrix lives on another device");
     if (precond != GSX_PREC_NONE && precond != GSX_PREC_JACOBI)
         return fail(c, GSX_ERR_ARG, "batched CG supports no preconditioner or Jacobi");
@@ -3821,6 +3830,31 @@ gsx_status gsx_batch_add(gsx_batch *B, gsx_matrix *A, int32_t row0, int32_t nrow
     g.tol = tol;
     g.maxit = maxit;
     g.lpr = gsx::spmv_lpr((double)(rp2[1] - rp2[0]) / nrows);
+    g.row0 = row0;
+    if (box) {
+        // the member's structured operator, built from its CSR rows (which
+        // must lie in the box's stencil window)
+        const int d = box[0], p = box[1];
+        const int64_t n64 = gsx::stencil_size(nrows, (int)gsx::batch_box_slots(d, p, false));
+        const int64_t n32 = gsx::stencil_size(nrows, (int)gsx::batch_box_slots(d, p, true));
+        M.s64 = dalloc(c, sizeof(double) * n64);
+        if (B->mixed) M.s32 = dalloc(c, sizeof(float) * n32);
+        if (!M.s64 || (B->mixed && !M.s32)) return fail(c, GSX_ERR_OOM, "device allocation (box member operator)");
+        HIP_OK(c, gsx::launch_fill_f64(dp<double>(M.s64), n64, 0.0, s));
+        if (B->mixed) HIP_OK(c, gsx::launch_fill_f32(dp<float>(M.s32), n32, 0.0f, s));
+        HIP_OK(c, hipMemsetAsync(B->d_bad.get(), 0, sizeof(int), s));
+        HIP_OK(c, gsx::launch_batch_box_build(d, p, row0, nrows, box[2], box[3], dp<int>(A->rowptr), dp<int>(A->col),
+                                              dp<double>(A->val), dp<double>(M.s64),
+                                              B->mixed ? dp<float>(M.s32) : nullptr, dp<int>(B->d_bad), s));
+        HIP_OK(c, hipMemcpyAsync(&bad, B->d_bad.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_OK(c, hipStreamSynchronize(s));
+        if (bad) return fail(c, GSX_ERR_ARG, "box member: an entry lies outside the box's stencil window");
+        g.bd = d;
+        g.bp = p;
+        for (int k = 0; k < 3; ++k) g.bm[k] = box[2 + k];
+        g.s64 = dp<double>(M.s64);
+        g.s32 = dp<float>(M.s32);
+    }
     std::memset(&M.st, 0, sizeof(M.st));
     // fresh state: phase 0 (initialised by the next step)
     HIP_OK(c, hipMemcpyAsync(dp<gsx::BState>(B->d_state) + slot, &M.st, sizeof(gsx::BState),
@@ -3832,6 +3866,20 @@ gsx_status gsx_batch_add(gsx_batch *B, gsx_matrix *A, int32_t row0, int32_t nrow
     return GSX_OK;
 }
 
+gsx_status gsx_batch_add(gsx_batch *B, gsx_matrix *A, int32_t row0, int32_t nrows, int32_t precond, double tau,
+                         const double *b, double *x, double tol, int32_t maxit, int32_t *id)
+{
+    return batch_add(B, A, row0, nrows, nullptr, precond, tau, b, x, tol, maxit, id);
+}
+
+gsx_status gsx_batch_add_box(gsx_batch *B, gsx_matrix *A, int32_t row0, int32_t nrows, int32_t d, int32_t p,
+                             int32_t m0, int32_t m1, int32_t m2, int32_t precond, double tau, const double *b,
+                             double *x, double tol, int32_t maxit, int32_t *id)
+{
+    const int32_t box[5] = {d, p, m0, m1, m2};
+    return batch_add(B, A, row0, nrows, box, precond, tau, b, x, tol, maxit, id);
+}
+
 static gsx_status batch_rebuild(gsx_batch *B)
 {
     gsx_ctx *c = B->c;
@@ -3840,7 +3888,8 @@ static gsx_status batch_rebuild(gsx_batch *B)
     for (int k = 0; k < B->cap; ++k) {
         BMember &M = B->m[k];
         if (!M.used || M.status) continue;
-        const int rows = gsx::batch_chunk_rows(M.seg.lpr);
+        // box members: one stencil tile of the member per chunk
+        const int rows = M.seg.bd ? gsx::kStencilTile : gsx::batch_chunk_rows(M.seg.lpr);
         M.seg.c0 = (int)B->chunks_h.size();
         for (int lo = M.row0; lo < M.row0 + M.nrows; lo += rows)
             B->chunks_h.push_back({k, lo, std::min(lo + rows, M.row0 + M.nrows)});

@@ -144,6 +144,8 @@ SIGNATURES = {
     "gsx_batch_destroy": [_vp],
     "gsx_batch_add": [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_double, _vp, _vp, C.c_double,
                       C.c_int32, _i32p],
+    "gsx_batch_add_box": [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                          C.c_int32, C.c_double, _vp, _vp, C.c_double, C.c_int32, C.POINTER(C.c_int32)],
     "gsx_batch_step": [_vp, C.c_int32, _i32p],
     "gsx_batch_set_graph": [_vp, C.c_int32],
     "gsx_batch_graph_launches": [_vp, _i64p],
@@ -633,11 +635,20 @@ class Batch:
         _check(ctx.h, lib().gsx_batch_create(ctx.h, precision, capacity, C.byref(h)))
         self.h = h
 
-    def add(self, matrix, b, x, row0=0, nrows=None, tol=1e-10, maxit=1000, precond=PREC_JACOBI, tau=1.0):
+    def add(self, matrix, b, x, row0=0, nrows=None, tol=1e-10, maxit=1000, precond=PREC_JACOBI, tau=1.0,
+            box=None):
+        """box = (d, p, m0, m1, m2): the rows are the free functions of one
+        tensor-product patch (gsx_batch_add_box): the member iterates on a
+        structured copy of its block."""
         nrows = matrix.n - row0 if nrows is None else nrows
         i = C.c_int32()
-        _check(self.ctx.h, lib().gsx_batch_add(self.h, matrix.h, row0, nrows, precond, tau, b.ptr, x.ptr, tol,
-                                               maxit, C.byref(i)))
+        if box is None:
+            _check(self.ctx.h, lib().gsx_batch_add(self.h, matrix.h, row0, nrows, precond, tau, b.ptr, x.ptr, tol,
+                                                   maxit, C.byref(i)))
+        else:
+            d, p, m0, m1, m2 = box
+            _check(self.ctx.h, lib().gsx_batch_add_box(self.h, matrix.h, row0, nrows, d, p, m0, m1, m2, precond,
+                                                       tau, b.ptr, x.ptr, tol, maxit, C.byref(i)))
         return i.value
 
     def step(self, iters=8):

@@ -451,6 +451,19 @@ gsx_status gsx_batch_add(gsx_batch *batch, gsx_matrix *A, int32_t row0,
                          int32_t nrows, int32_t precond, double tau,
                          const double *b_dev, double *x_dev, double tol,
                          int32_t maxit, int32_t *id);
+/* The same for a member whose rows are the free functions of one
+ * tensor-product patch of degree p (the d-dimensional box m0 x m1 x m2 of
+ * free functions in lexicographic order, m2 = 1 in 2D), e.g. a single patch
+ * with Dirichlet sides or one of several disjoint patches: its CSR rows must
+ * couple only within the box's (2p+1)^d stencil window (else GSX_ERR_ARG).
+ * The member then iterates on a structured copy of its block (all stencil
+ * slots in FP64, the upper half in FP32 for GSX_MIXED; no column indices),
+ * built here from A's values. */
+gsx_status gsx_batch_add_box(gsx_batch *batch, gsx_matrix *A, int32_t row0,
+                             int32_t nrows, int32_t d, int32_t p, int32_t m0,
+                             int32_t m1, int32_t m2, int32_t precond, double tau,
+                             const double *b_dev, double *x_dev, double tol,
+                             int32_t maxit, int32_t *id);
 /* Initialise newly added members, run iters CG iterations of every running
  * member (members that finish stop on their own), then the reliable update
  * (mixed).  running: members still iterating afterwards.  The step's

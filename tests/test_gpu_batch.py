@@ -217,3 +217,97 @@ def test_batch_graph_replays_equal_eager_launches(ctx, precision):
         assert np.array_equal(a, b)
     for m in members:
         m.close()
+
+
+def _box(prob):
+    P = prob.patches[0]
+    m = [nb - 2 for nb in P.nbasis_dir()]
+    return (P.d, P.p[0], m[0], m[1], m[2] if P.d == 3 else 1)
+
+
+@pytest.mark.parametrize("precision", [gsx.FP64, gsx.MIXED])
+def test_box_members_reach_the_oracle_solution(ctx, precision):
+    """Members added with their patch box (gsx_batch_add_box) iterate on a
+    structured copy of their block: the oracle's solution and (FP64) its
+    iteration count, as the CSR members; a box member's iterates are bitwise
+    the same alone and in a mix with CSR and box members."""
+    probs = _small_batch(count=12, seed=5)
+    members = [_Member(ctx, p, precision) for p in probs]
+    tol = 1e-10 if precision == gsx.FP64 else 1e-12
+    B = gsx.Batch(ctx, precision, capacity=2 * len(members))
+    ids = [B.add(m.A, m.b, m.x.zero(), tol=tol, maxit=5000, box=_box(m.prob)) for m in members]
+    B.run(8)
+    res = [B.result(i) for i in ids]
+    xs = [m.x.download() for m in members]
+    B.close()
+    for prob, (st, it, rr), x in zip(probs, res, xs):
+        _, _, xo, ito = _oracle(prob, tol)
+        print(prob.name, _box(prob), it, ito, rr, flush=True)
+        assert st == 1 and rr < tol
+        if precision == gsx.FP64:
+            assert abs(it - ito) <= max(1, 0.02 * ito), (prob.name, it, ito)
+        assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+    # alone vs. in a mix of box and CSR members: bitwise
+    for k in (0, 5):
+        m = members[k]
+        B = gsx.Batch(ctx, precision, capacity=4)
+        B.add(m.A, m.b, m.x.zero(), tol=tol, maxit=5000, box=_box(m.prob))
+        B.run(8)
+        alone = m.x.download()
+        B.close()
+        assert np.array_equal(alone, xs[k])
+        B = gsx.Batch(ctx, precision, capacity=4)
+        other = members[k + 1]
+        B.add(other.A, other.b, other.x.zero(), tol=tol, maxit=5000)
+        B.add(m.A, m.b, m.x.zero(), tol=tol, maxit=5000, box=_box(m.prob))
+        B.add(members[k + 2].A, members[k + 2].b, members[k + 2].x.zero(), tol=tol, maxit=5000,
+              box=_box(members[k + 2].prob))
+        B.run(8)
+        assert np.array_equal(m.x.download(), xs[k])
+        B.close()
+    for m in members:
+        m.close()
+
+
+def test_box_members_of_a_disjoint_patch_matrix_and_wrong_boxes(ctx):
+    """Box members as blocks of one matrix assembled from disjoint patches
+    give the separately assembled members' solutions; a box that does not
+    match the block's couplings is refused."""
+    probs = [p for p in _small_batch(count=12, seed=21) if p.d == 3][:4]
+    joint = problems.Problem([p.patches[0] for p in probs],
+                             [(k, s) for k in range(len(probs)) for s in range(1, 7)], [], name="joint3")
+    gj = gsx.Problem(ctx, joint)
+    Aj, bj = gj.assemble(gsx.PATTERN_EXPR, gsx.MIXED)
+    xj = ctx.array(gj.nfree).zero()
+    B = gsx.Batch(ctx, gsx.MIXED, capacity=len(probs))
+    off, offs = 0, []
+    for p in probs:
+        bx = _box(p)
+        n = bx[2] * bx[3] * bx[4]
+        B.add(Aj, bj.view(off, n), xj.view(off, n), row0=off, nrows=n, tol=1e-12, maxit=5000, box=bx)
+        offs.append(off)
+        off += n
+    assert off == gj.nfree
+    B.run(8)
+    xall = xj.download()
+    for p, o in zip(probs, offs):
+        _, _, xo, _ = _oracle(p, 1e-14)
+        xb = xall[o:o + len(xo)]
+        assert np.linalg.norm(xb - xo) <= 1e-8 * np.linalg.norm(xo)
+    B.close()
+    B = gsx.Batch(ctx, gsx.MIXED, capacity=4)
+    d, p, m0, m1, m2 = _box(probs[0])
+    n = m0 * m1 * m2
+    with pytest.raises(gsx.GsxError):   # the box spans two patches' rows
+        B.add(Aj, bj.view(0, n), xj.view(0, n), row0=offs[1] - 1, nrows=n, box=(d, p, m0, m1, m2))
+    if m0 != m1:
+        with pytest.raises(gsx.GsxError):   # transposed box: couplings outside the window
+            B.add(Aj, bj.view(0, n), xj.view(0, n), row0=0, nrows=n, box=(d, p, m1, m0, m2))
+    with pytest.raises(gsx.GsxError):   # degree too small for the couplings
+        B.add(Aj, bj.view(0, n), xj.view(0, n), row0=0, nrows=n, box=(d, p - 1, m0, m1, m2))
+    with pytest.raises(gsx.GsxError):   # cells != rows
+        B.add(Aj, bj.view(0, n), xj.view(0, n), row0=0, nrows=n - 1, box=(d, p, m0, m1, m2))
+    B.close()
+    for a in (Aj, bj, xj):
+        a.close()
+    gj.close()
