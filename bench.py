@@ -134,6 +134,8 @@ def main():
                     help="batch: number of problems (512); serve: number of timed requests (48)")
     ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-grouped", action="store_true",
+                    help="batch: assemble disjoint patches in a few multipatch groups (CSR blocks)")
     ap.add_argument("--batch-csr", action="store_true",
                     help="batch: members iterate on their CSR blocks instead of structured copies")
     ap.add_argument("--no-legs", action="store_true",
@@ -570,8 +572,10 @@ def bench_batch(args):
     """Independent single-patch problems (half 2D with 8..96, half 3D with
     8..32 elements per direction, degree 2..4, +-0.1h noise, lognormal
     kappa, random per-element sources).  One step = assembly of every
-    problem (disjoint patches grouped into a few multipatch assemblies up to
-    an element-matrix budget) + one batched Jacobi-CG over all of them."""
+    problem (each as its own patch; with --batch-grouped, disjoint patches
+    grouped into a few multipatch assemblies up to an element-matrix budget)
+    + one batched Jacobi-CG over all of them, members on structured operators
+    with the tile shapes of the autotuner's table."""
     from gismo_amd import gsx, problems
     # relres 1e-11 keeps every member's error vs the tight solve under 1e-8
     # (at 1e-10 the worst-conditioned 3D p=4 members reach ~1.1e-8)
@@ -579,24 +583,33 @@ def bench_batch(args):
     args.count = 512 if args.count is None else args.count
     probs = problems.batch_problems(count=args.count, seed=7)
     ctx = gsx.Context(0)
-    budget = 12e9   # bytes of element matrices per assembly (the context allows 16 GiB)
-    groups, cur, cur_bytes = [], [], 0.0
-    for d in (2, 3):
-        for pr in [q for q in probs if q.d == d]:
-            P = pr.patches[0]
-            bytes_ = P.nelem * (P.p[0] + 1) ** (2 * d) * 8.0
-            if cur and (cur_bytes + bytes_ > budget or cur[0].d != d):
-                groups.append(cur)
-                cur, cur_bytes = [], 0.0
-            cur.append(pr)
-            cur_bytes += bytes_
-    if cur:
-        groups.append(cur)
-    gps = []
-    for g in groups:
-        joint = problems.Problem([q.patches[0] for q in g],
-                                 [(k, s) for k in range(len(g)) for s in range(1, 2 * g[0].d + 1)], [])
-        gps.append((g, gsx.Problem(ctx, joint)))
+    from gismo_amd import autotune
+    table = autotune.Table.load(ctx.arch())   # tile shapes per member class (defaults if not tuned)
+    if args.batch_grouped:
+        # disjoint patches grouped into multipatch assemblies (element path,
+        # CSR), members built from the blocks
+        budget = 12e9   # bytes of element matrices per assembly (the context allows 16 GiB)
+        groups, cur, cur_bytes = [], [], 0.0
+        for d in (2, 3):
+            for pr in [q for q in probs if q.d == d]:
+                P = pr.patches[0]
+                bytes_ = P.nelem * (P.p[0] + 1) ** (2 * d) * 8.0
+                if cur and (cur_bytes + bytes_ > budget or cur[0].d != d):
+                    groups.append(cur)
+                    cur, cur_bytes = [], 0.0
+                cur.append(pr)
+                cur_bytes += bytes_
+        if cur:
+            groups.append(cur)
+        gps = []
+        for g in groups:
+            joint = problems.Problem([q.patches[0] for q in g],
+                                     [(k, s) for k in range(len(g)) for s in range(1, 2 * g[0].d + 1)], [])
+            gps.append((g, gsx.Problem(ctx, joint)))
+    else:
+        # every problem assembled as its own single patch: the assembly fills
+        # its structured operator, on which the box member iterates
+        gps = [([q], gsx.Problem(ctx, q)) for q in probs]
     ndofs = sum(gp.nglued for _, gp in gps)
     nfree = sum(gp.nfree for _, gp in gps)
 
@@ -611,9 +624,14 @@ def bench_batch(args):
                 P = q.patches[0]
                 m = [nb - 2 for nb in P.nbasis_dir()] + [1]
                 nf = int(np.prod(m))
-                box = None if args.batch_csr else (P.d, P.p[0], m[0], m[1], m[2])
+                kind, lanes, half = table.lookup(prec == gsx.MIXED, P.d, P.p[0], nf)
+                if args.batch_csr or kind == "csr":
+                    box, tile = None, (lanes if kind == "csr" else None)
+                    tile = None if tile is None else (tile, 1)
+                else:
+                    box, tile = (P.d, P.p[0], m[0], m[1], m[2]), (lanes, half)
                 B.add(A, b.view(off, nf), xg.view(off, nf), row0=off, nrows=nf, tol=tol_, maxit=args.maxit,
-                      box=box)
+                      box=box, tile=tile)
                 off += nf
             assert off == gp.nfree
         B.run(8)
@@ -664,9 +682,30 @@ def bench_batch(args):
                              max_rel_err=max(errs), converged=sum(r[0] == 1 for r in res),
                              iters_max=max(its), iters_mean=float(np.mean(its)),
                              dofs_per_s=ndofs / float(np.mean(times)))
+    # phase split of one mixed step (launches one by one with events: no graph)
+    for x in xs:
+        x.zero()
+    ctx.enable_timing(True)
+    ctx.timing_filter(None)
+    ctx.reset_timing()
+    t0 = time.perf_counter()
+    run(gsx.MIXED, tol, xs)
+    ctx.synchronize()
+    phases = {"wall_ms": round(1e3 * (time.perf_counter() - t0), 1)}
+    for k in ("pattern_count", "pattern_scan", "pattern_fill", "elem2d", "elem3d", "gather", "sf_assemble",
+              "sf_finish", "stencil_build", "convert_f32", "batch_box_build", "batch_step_f32"):
+        nl, ms = ctx.kernel_stats(k)
+        if nl:
+            phases[k] = round(ms, 3)
+    ctx.enable_timing(False)
     mixed = results["mixed"]
+    mixed["phase_ms_one_step"] = phases
     extra = {"config": {"problems": len(probs), "dofs": ndofs, "free_dofs": nfree, "nnz": nnz,
-                        "assemblies": len(gps)},
+                        "assemblies": len(gps),
+                        "assembly": "multipatch groups" if args.batch_grouped else "one single-patch assembly per problem",
+                        "tile_table": (f"gismo_amd/tuning/batch_tiles_{table.arch.split(':')[0]}.json"
+                                       if table.entries else "defaults (not tuned)"),
+                        "members": "CSR blocks" if args.batch_csr else "structured (box) members"},
              "fp64": results["fp64"], "mixed": mixed,
              "speedup_mixed_vs_fp64": results["fp64"]["ms"] / mixed["ms"],
              "reference": {"tol": 1e-12, "converged": sum(r[0] == 1 for r in ref_res)}}

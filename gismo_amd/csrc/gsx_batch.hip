@@ -9,8 +9,11 @@
 //
 // Layout.  A member ("segment") is one CG solve: rows [row0, row0 + nrows)
 // of a CSR matrix and its own vectors (base pointers indexed by row and by
-// column).  Its rows are cut into chunks of (256 / lpr) * 64 rows, one
-// thread block per chunk, and chunks never straddle two members.  Every
+// column).  Its rows are cut into chunks of (256 / lpr) * 64 rows (CSR) or
+// 256 / tpr rows (box members), one thread block per chunk, and chunks
+// never straddle two members.  Blocks map to chunks through xcd_block, so
+// each XCD takes a contiguous range of chunks and a member's neighbouring
+// rows share an L2.  Every
 // reduction is per chunk (block partial) and then per member (the member's
 // chunk partials summed in chunk order by every block that needs the
 // scalar), so a member's arithmetic does not depend on which other members
@@ -138,67 +141,74 @@ struct BoxGeo {
     static constexpr int C = NS / 2, NH = stencil_half(NS);
 };
 
-// FP64, all slots in lexicographic (= column) order: y = A x or y = b - A x
-template <int D, int P, bool RESID>
-__device__ inline void box_apply64(const BSeg &g, int lo, int hi, const double *__restrict__ x,
-                                   double *__restrict__ y, double &dot)
+// Tile shape (chosen per member): tpr threads per row (1, 2 or 4; a chunk
+// holds 256 / tpr rows), each summing every tpr-th stencil line (x-line of
+// 2p + 1 slots) of the row; the partial sums are added by butterfly
+// shuffles within the row's group, so the result depends on tpr but not on
+// the other members.
+
+// sum over the row's tpr lanes (groups are aligned: xor stays inside)
+__device__ inline double row_sum(double v, int tpr)
 {
-    using G = BoxGeo<D, P>;
-    const int m0 = g.bm[0], m1 = g.bm[1], m2 = g.bm[2];
-    const int r = lo + threadIdx.x;
-    if (r >= hi) return;
-    const int l = r - g.row0;
-    const int ia = l % m0, ib = (l / m0) % m1, ic = (D == 3) ? l / (m0 * m1) : 0;
-    const double *vb = g.s64 + stencil_index(0, l, G::NS);
-    double acc = 0.0;
-#pragma unroll 1
-    for (int dz = (D == 3 ? -P : 0); dz <= (D == 3 ? P : 0); ++dz) {
-#pragma unroll
-        for (int dy = -P; dy <= P; ++dy) {
-            const bool okyz = ib + dy >= 0 && ib + dy < m1 && ic + dz >= 0 && ic + dz < m2;
-            const int rb = r + m0 * (dy + m1 * dz);
-            const int slot0 = ((D == 3 ? (dz + P) * G::W : 0) + (dy + P)) * G::W;
-#pragma unroll
-            for (int dx = -P; dx <= P; ++dx) {
-                const bool ok = okyz && ia + dx >= 0 && ia + dx < m0;
-                acc += vb[(slot0 + dx + P) * kStencilTile] * x[ok ? rb + dx : r];
-            }
-        }
-    }
-    if (RESID) {
-        const double v = g.b[r] - acc;
-        y[r] = v;
-        dot += v * v;
-    } else {
-        y[r] = acc;
-        dot += x[r] * acc;
-    }
+    for (int off = tpr >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ inline float row_sum(float v, int tpr)
+{
+    for (int off = tpr >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
 }
 
-// FP32 half stencil: each upper x-line with its mirror line (the order of the
-// single-patch kernel k_sspmv_half)
+// All slots of the row (s is the FP64 stencil or the full FP32 one), line by
+// line in lexicographic (= column) order.
+template <int D, int P, typename T>
+__device__ inline T box_full_rows(const BSeg &g, const T *__restrict__ sv, int r, int sub, int tpr,
+                                  const T *__restrict__ x)
+{
+    using G = BoxGeo<D, P>;
+    constexpr int NLINES = (D == 3) ? G::W * G::W : G::W;
+    const int m0 = g.bm[0], m1 = g.bm[1], m2 = g.bm[2];
+    const int l = r - g.row0;
+    const int ia = l % m0, ib = (l / m0) % m1, ic = (D == 3) ? l / (m0 * m1) : 0;
+    const T *vb = sv + stencil_index(0, l, G::NS);
+    T acc = 0;
+#pragma unroll 1
+    for (int li = sub; li < NLINES; li += tpr) {
+        const int dz = (D == 3) ? li / G::W - P : 0, dy = li % G::W - P;
+        const bool okyz = ib + dy >= 0 && ib + dy < m1 && ic + dz >= 0 && ic + dz < m2;
+        const int rb = r + m0 * (dy + m1 * dz);
+#pragma unroll
+        for (int dx = -P; dx <= P; ++dx) {
+            const bool ok = okyz && ia + dx >= 0 && ia + dx < m0;
+            acc += vb[(li * G::W + dx + P) * kStencilTile] * x[ok ? rb + dx : r];
+        }
+    }
+    return acc;
+}
+
+// FP32 upper half: the centre line (lane 0 of the row) and each upper line
+// with its mirror line (the order of the single-patch kernel k_sspmv_half)
 template <int D, int P>
-__device__ inline void box_apply32(const BSeg &g, int lo, int hi, const float *__restrict__ x,
-                                   float *__restrict__ y, double &dot)
+__device__ inline float box_half_rows(const BSeg &g, int r, int sub, int tpr, const float *__restrict__ x)
 {
     using G = BoxGeo<D, P>;
     constexpr int W = G::W, C = G::C, NH = G::NH;
     constexpr int NL = (D == 3) ? P + P * W : P;
     const int m0 = g.bm[0], m1 = g.bm[1], m2 = g.bm[2];
-    const int r = lo + threadIdx.x;
-    if (r >= hi) return;
     const int l = r - g.row0;
     const int ia = l % m0, ib = (l / m0) % m1, ic = (D == 3) ? l / (m0 * m1) : 0;
     const float *sv = g.s32;
     const float *vb = sv + stencil_index(0, l, NH);
     float acc = 0.0f;
+    if (sub == 0) {
 #pragma unroll
-    for (int dx = 0; dx <= P; ++dx) acc += vb[dx * kStencilTile] * x[ia + dx < m0 ? r + dx : r];
+        for (int dx = 0; dx <= P; ++dx) acc += vb[dx * kStencilTile] * x[ia + dx < m0 ? r + dx : r];
 #pragma unroll
-    for (int dx = -P; dx < 0; ++dx)
-        acc += (ia + dx >= 0 ? sv[stencil_index(-dx, l + dx, NH)] * x[r + dx] : 0.0f);
+        for (int dx = -P; dx < 0; ++dx)
+            acc += (ia + dx >= 0 ? sv[stencil_index(-dx, l + dx, NH)] * x[r + dx] : 0.0f);
+    }
 #pragma unroll 1
-    for (int u = 0; u < NL; ++u) {
+    for (int u = sub; u < NL; u += tpr) {
         const int dz = (D == 2 || u < P) ? 0 : 1 + (u - P) / W;
         const int dy = (D == 2 || u < P) ? u + 1 : (u - P) % W - P;
         const int h0 = (D == 3 ? ((dz + P) * W + (dy + P)) * W : (dy + P) * W) - C;   // (dz, dy, -P)
@@ -216,6 +226,39 @@ __device__ inline void box_apply32(const BSeg &g, int lo, int hi, const float *_
             acc += (ok ? sv[stencil_index(h0 - dx + P, l - off + dx, NH)] * x[r - off + dx] : 0.0f);
         }
     }
+    return acc;
+}
+
+// FP64: y = A x, partial (x, y); or (RESID) y = b - A x, partial (y, y)
+template <int D, int P, bool RESID>
+__device__ inline void box_apply64(const BSeg &g, int lo, int hi, const double *__restrict__ x,
+                                   double *__restrict__ y, double &dot)
+{
+    const int tpr = g.tpr, r = lo + (int)threadIdx.x / tpr, sub = (int)threadIdx.x % tpr;
+    if (r >= hi) return;   // the row's whole group leaves together
+    const double acc = row_sum(box_full_rows<D, P, double>(g, g.s64, r, sub, tpr, x), tpr);
+    if (sub != 0) return;
+    if (RESID) {
+        const double v = g.b[r] - acc;
+        y[r] = v;
+        dot += v * v;
+    } else {
+        y[r] = acc;
+        dot += x[r] * acc;
+    }
+}
+
+// FP32 on the half (default) or full stencil copy
+template <int D, int P>
+__device__ inline void box_apply32(const BSeg &g, int lo, int hi, const float *__restrict__ x,
+                                   float *__restrict__ y, double &dot)
+{
+    const int tpr = g.tpr, r = lo + (int)threadIdx.x / tpr, sub = (int)threadIdx.x % tpr;
+    if (r >= hi) return;
+    const float part = g.half ? box_half_rows<D, P>(g, r, sub, tpr, x)
+                              : box_full_rows<D, P, float>(g, g.s32, r, sub, tpr, x);
+    const float acc = row_sum(part, tpr);
+    if (sub != 0) return;
     y[r] = acc;
     dot += (double)x[r] * (double)acc;
 }
@@ -266,7 +309,8 @@ __global__ __launch_bounds__(256) void k_b_init1(const BChunk *__restrict__ chun
                                                  double *__restrict__ p0, double *__restrict__ p1)
 {
     __shared__ double s[256];
-    const BChunk ch = chunks[blockIdx.x];
+    const int cb = xcd_block(blockIdx.x, gridDim.x);   // logical chunk
+    const BChunk ch = chunks[cb];
     if (st[ch.seg].phase != 0) return;
     const BSeg g = segs[ch.seg];
     double rr = 0.0, bb = 0.0;
@@ -274,8 +318,8 @@ __global__ __launch_bounds__(256) void k_b_init1(const BChunk *__restrict__ chun
     for (int i = ch.lo + threadIdx.x; i < ch.hi; i += 256) bb += g.b[i] * g.b[i];
     const double a = bsum256(rr, s), c = bsum256(bb, s);
     if (threadIdx.x == 0) {
-        p0[blockIdx.x] = a;
-        p1[blockIdx.x] = c;
+        p0[cb] = a;
+        p1[cb] = c;
     }
 }
 
@@ -285,7 +329,8 @@ __global__ __launch_bounds__(256) void k_b_init2(const BChunk *__restrict__ chun
                                                  double *__restrict__ p2, int mixed)
 {
     __shared__ double s[256];
-    const BChunk ch = chunks[blockIdx.x];
+    const int cb = xcd_block(blockIdx.x, gridDim.x);   // logical chunk
+    const BChunk ch = chunks[cb];
     BState &S = st[ch.seg];
     if (S.phase != 0) return;
     const BSeg g = segs[ch.seg];
@@ -319,8 +364,8 @@ __global__ __launch_bounds__(256) void k_b_init2(const BChunk *__restrict__ chun
         }
     }
     const double a = bsum256(rz, s);
-    if (threadIdx.x == 0) p2[blockIdx.x] = a;
-    if (blockIdx.x == (unsigned)g.c0 && threadIdx.x == 0) {
+    if (threadIdx.x == 0) p2[cb] = a;
+    if (cb == g.c0 && threadIdx.x == 0) {
         S.bnorm = bnorm;
         S.rr = rr;
         S.rr_ref = rr;
@@ -341,11 +386,12 @@ __global__ __launch_bounds__(256) void k_b_init3(const BChunk *__restrict__ chun
                                                  const double *__restrict__ p2)
 {
     __shared__ double s[256];
-    const BChunk ch = chunks[blockIdx.x];
+    const int cb = xcd_block(blockIdx.x, gridDim.x);   // logical chunk
+    const BChunk ch = chunks[cb];
     BState &S = st[ch.seg];
     if (S.phase != 0) return;
     const BSeg g = segs[ch.seg];
-    if ((int)blockIdx.x != g.c0) return;
+    if (cb != g.c0) return;
     const double rz = member_sum(p2, g.c0, g.nc, s);
     if (threadIdx.x == 0) {
         S.rzs[0] = rz;
@@ -362,13 +408,14 @@ __global__ __launch_bounds__(256) void k_b_spmv(const BChunk *__restrict__ chunk
                                                 double *__restrict__ p0)
 {
     __shared__ double s[256];
-    const BChunk ch = chunks[blockIdx.x];
+    const int cb = xcd_block(blockIdx.x, gridDim.x);   // logical chunk
+    const BChunk ch = chunks[cb];
     BState &S = st[ch.seg];
     if (S.phase != 1 || S.done || S.stop) return;
     const BSeg g = segs[ch.seg];
     // the iteration number advances even for a paused member, so that the
     // update and direction launches of this iteration see it paused too
-    if ((int)blockIdx.x == g.c0 && threadIdx.x == 0) S.it = S.it_next;
+    if (cb == g.c0 && threadIdx.x == 0) S.it = S.it_next;
     if (S.flag_next && S.flag_next <= S.it_next) return;
     double dot = 0.0;
     if constexpr (sizeof(T) == 4) {
@@ -379,7 +426,7 @@ __global__ __launch_bounds__(256) void k_b_spmv(const BChunk *__restrict__ chunk
         else chunk_spmv_any<double, double>(g, ch.lo, ch.hi, g.v64, g.p, g.t, dot);
     }
     const double a = bsum256(dot, s);
-    if (threadIdx.x == 0) p0[blockIdx.x] = a;
+    if (threadIdx.x == 0) p0[cb] = a;
 }
 
 template <typename T>
@@ -389,7 +436,8 @@ __global__ __launch_bounds__(256) void k_b_update(const BChunk *__restrict__ chu
                                                   double *__restrict__ p2)
 {
     __shared__ double s[256];
-    const BChunk ch = chunks[blockIdx.x];
+    const int cb = xcd_block(blockIdx.x, gridDim.x);   // logical chunk
+    const BChunk ch = chunks[cb];
     const BState &S = st[ch.seg];
     if (S.phase != 1 || S.done || S.stop || (S.flag_next && S.flag_next <= S.it)) return;
     const BSeg g = segs[ch.seg];
@@ -420,8 +468,8 @@ __global__ __launch_bounds__(256) void k_b_update(const BChunk *__restrict__ chu
     }
     const double a = bsum256(rr, s), c = bsum256(rz, s);
     if (threadIdx.x == 0) {
-        p1[blockIdx.x] = a;
-        p2[blockIdx.x] = c;
+        p1[cb] = a;
+        p2[cb] = c;
     }
 }
 
@@ -432,7 +480,8 @@ __global__ __launch_bounds__(256) void k_b_direction(const BChunk *__restrict__ 
                                                      const double *__restrict__ p2, double delta2)
 {
     __shared__ double s[256];
-    const BChunk ch = chunks[blockIdx.x];
+    const int cb = xcd_block(blockIdx.x, gridDim.x);   // logical chunk
+    const BChunk ch = chunks[cb];
     BState &S = st[ch.seg];
     // flag_next is only set to it + 1 below: every block of this launch
     // takes the same decision whether or not the first block has written it
@@ -453,7 +502,7 @@ __global__ __launch_bounds__(256) void k_b_direction(const BChunk *__restrict__ 
             for (int i = ch.lo + threadIdx.x; i < ch.hi; i += 256) g.p[i] = g.z[i] + beta * g.p[i];
         }
     }
-    if ((int)blockIdx.x == g.c0 && threadIdx.x == 0) {
+    if (cb == g.c0 && threadIdx.x == 0) {
         S.it_next = k + 1;
         S.rr = rr;
         S.err = err;
@@ -483,7 +532,8 @@ __global__ __launch_bounds__(256) void k_b_direction(const BChunk *__restrict__ 
 __global__ __launch_bounds__(256) void k_b_rel1(const BChunk *__restrict__ chunks,
                                                 const BSeg *__restrict__ segs, BState *__restrict__ st)
 {
-    const BChunk ch = chunks[blockIdx.x];
+    const int cb = xcd_block(blockIdx.x, gridDim.x);   // logical chunk
+    const BChunk ch = chunks[cb];
     const BState &S = st[ch.seg];
     if (S.phase != 1 || S.done || !S.need_update) return;
     const BSeg g = segs[ch.seg];
@@ -498,14 +548,15 @@ __global__ __launch_bounds__(256) void k_b_rel2(const BChunk *__restrict__ chunk
                                                 double *__restrict__ p0)
 {
     __shared__ double s[256];
-    const BChunk ch = chunks[blockIdx.x];
+    const int cb = xcd_block(blockIdx.x, gridDim.x);   // logical chunk
+    const BChunk ch = chunks[cb];
     const BState &S = st[ch.seg];
     if (S.phase != 1 || S.done || !S.need_update) return;
     const BSeg g = segs[ch.seg];
     double rr = 0.0;
     chunk_residual_member(g, ch.lo, ch.hi, rr);
     const double a = bsum256(rr, s);
-    if (threadIdx.x == 0) p0[blockIdx.x] = a;
+    if (threadIdx.x == 0) p0[cb] = a;
 }
 
 __global__ __launch_bounds__(256) void k_b_rel3(const BChunk *__restrict__ chunks,
@@ -513,7 +564,8 @@ __global__ __launch_bounds__(256) void k_b_rel3(const BChunk *__restrict__ chunk
                                                 const double *__restrict__ p0, double *__restrict__ p2)
 {
     __shared__ double s[256];
-    const BChunk ch = chunks[blockIdx.x];
+    const int cb = xcd_block(blockIdx.x, gridDim.x);   // logical chunk
+    const BChunk ch = chunks[cb];
     BState &S = st[ch.seg];
     if (S.phase != 1 || S.done || !S.need_update) return;
     const BSeg g = segs[ch.seg];
@@ -538,8 +590,8 @@ __global__ __launch_bounds__(256) void k_b_rel3(const BChunk *__restrict__ chunk
         }
     }
     const double a = bsum256(rz, s);
-    if (threadIdx.x == 0) p2[blockIdx.x] = a;
-    if ((int)blockIdx.x == g.c0 && threadIdx.x == 0) {
+    if (threadIdx.x == 0) p2[cb] = a;
+    if (cb == g.c0 && threadIdx.x == 0) {
         S.rr_ref = rr;
         S.err = err;
         if (g.hist) g.hist[S.it_next] = err;
@@ -552,11 +604,12 @@ __global__ __launch_bounds__(256) void k_b_rel4(const BChunk *__restrict__ chunk
                                                 const double *__restrict__ p2)
 {
     __shared__ double s[256];
-    const BChunk ch = chunks[blockIdx.x];
+    const int cb = xcd_block(blockIdx.x, gridDim.x);   // logical chunk
+    const BChunk ch = chunks[cb];
     BState &S = st[ch.seg];
     if (S.phase != 1 || S.done || !S.need_update) return;
     const BSeg g = segs[ch.seg];
-    if ((int)blockIdx.x != g.c0) return;
+    if (cb != g.c0) return;
     const double rz = member_sum(p2, g.c0, g.nc, s);
     if (threadIdx.x == 0) {
         S.rr = S.rr_ref;
@@ -585,7 +638,7 @@ __global__ __launch_bounds__(256) void k_b_check(int row0, int nrows, const int 
 // (else flag[0] = 1); the FP64 stencil takes all slots, the FP32 one the
 // upper half.  Both arrays are zero beforehand (slots without an entry).
 template <int D, int P>
-__global__ __launch_bounds__(256) void k_b_box_build(int row0, int n, int m0, int m1, const int *__restrict__ rp,
+__global__ __launch_bounds__(256) void k_b_box_build(int row0, int n, int m0, int m1, int half, const int *__restrict__ rp,
                                                      const int *__restrict__ ci, const double *__restrict__ val,
                                                      double *__restrict__ s64, float *__restrict__ s32,
                                                      int *__restrict__ flag)
@@ -608,9 +661,22 @@ __global__ __launch_bounds__(256) void k_b_box_build(int row0, int n, int m0, in
         }
         const int slot = ((D == 3 ? (dz + P) * G::W : 0) + (dy + P)) * G::W + (dx + P);
         s64[stencil_index(slot, l, G::NS)] = val[k];
-        if (s32 && slot >= G::C) s32[stencil_index(slot - G::C, l, G::NH)] = (float)val[k];
+        if (s32 && half && slot >= G::C) s32[stencil_index(slot - G::C, l, G::NH)] = (float)val[k];
+        if (s32 && !half) s32[stencil_index(slot, l, G::NS)] = (float)val[k];
     }
     if (bad) flag[0] = 1;
+}
+
+// FP32 copy of a box member's FP64 stencil: the upper half (half = 1) or
+// all ns slots, same tile layout
+__global__ __launch_bounds__(256) void k_b_s32_from_s64(int64_t n32, int ns, int ns32, int c,
+                                                        const double *__restrict__ s64, float *__restrict__ s32)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n32) return;
+    const int64_t tile = i / ((int64_t)ns32 * kStencilTile);
+    const int slot = (int)((i / kStencilTile) % ns32), row = (int)(i % kStencilTile);
+    s32[i] = (float)s64[(tile * ns + slot + c) * kStencilTile + row];
 }
 
 // ---------------------------------------------------------------------------
@@ -630,18 +696,29 @@ int64_t batch_box_slots(int d, int p, bool half)
     return half ? stencil_half(ns) : ns;
 }
 
-hipError_t launch_batch_box_build(int d, int p, int row0, int n, int m0, int m1, const int *rp, const int *ci,
-                                  const double *val, double *s64, float *s32, int *flag, hipStream_t s)
+hipError_t launch_batch_box_build(int d, int p, int row0, int n, int m0, int m1, bool half, const int *rp,
+                                  const int *ci, const double *val, double *s64, float *s32, int *flag,
+                                  hipStream_t s)
 {
     const dim3 grid((n + 255) / 256), blk(256);
 #define GSX_C(D, P)                                                                                             \
     if (d == D && p == P) {                                                                                     \
-        hipLaunchKernelGGL((k_b_box_build<D, P>), grid, blk, 0, s, row0, n, m0, m1, rp, ci, val, s64, s32, flag); \
+        hipLaunchKernelGGL((k_b_box_build<D, P>), grid, blk, 0, s, row0, n, m0, m1, half ? 1 : 0, rp, ci, val,  \
+                           s64, s32, flag);                                                                     \
         return hipGetLastError();                                                                               \
     }
     GSX_BOX_CASES(GSX_C)
 #undef GSX_C
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_batch_box_s32(int d, int p, int n, bool half, const double *s64, float *s32, hipStream_t s)
+{
+    const int ns = (int)batch_box_slots(d, p, false), ns32 = half ? (int)batch_box_slots(d, p, true) : ns;
+    const int64_t n32 = stencil_size(n, ns32);
+    hipLaunchKernelGGL(k_b_s32_from_s64, dim3((unsigned)((n32 + 255) / 256)), dim3(256), 0, s, n32, ns, ns32,
+                       half ? ns / 2 : 0, s64, s32);
+    return hipGetLastError();
 }
 
 hipError_t launch_batch_init(int nchunks, const BChunk *chunks, const BSeg *segs, BState *st, double *p0,

@@ -115,6 +115,19 @@ struct ElemRange {
     int64_t fshift;
 };
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b to XCD b % 8),
+// each XCD with its own L2.  Logical block of physical block b: XCD x takes
+// the contiguous range [start(x), start(x) + count(x)) of logical blocks, so
+// neighbouring rows, which read the same x planes, share an L2.  A bijection
+// of [0, nb).
+__device__ inline int xcd_block(int b, int nb)
+{
+    const int x = b & 7, j = b >> 3;
+    int start = 0;
+    for (int y = 0; y < x; ++y) start += (nb - y + 7) >> 3;
+    return start + j;
+}
+
 // One member of a batched CG solve: rows [row0, row0 + nrows) of a CSR
 // matrix whose columns stay inside that range.  Vector pointers are bases
 // indexed by row/column number (the member's own arrays shifted by -row0).
@@ -141,6 +154,8 @@ struct BSeg {
     // slot, s32 the upper half (zero where the neighbour is outside the box).
     int bd, bp, row0;
     int bm[3];
+    int tpr;             // threads per row (1, 2, 4); chunks of 256 / tpr rows
+    int half;            // FP32 operator: 1 upper half of the stencil, 0 all slots
     const double *s64;
     const float *s32;
 };

@@ -262,8 +262,10 @@ hipError_t launch_accum_grid(int64_t n, double *x, float *xc, hipStream_t s);
 int batch_chunk_rows(int lpr);
 hipError_t launch_batch_check(int row0, int nrows, const int *rp, const int *ci, int *flag, hipStream_t s);
 int64_t batch_box_slots(int d, int p, bool half);
-hipError_t launch_batch_box_build(int d, int p, int row0, int n, int m0, int m1, const int *rp, const int *ci,
-                                  const double *val, double *s64, float *s32, int *flag, hipStream_t s);
+hipError_t launch_batch_box_build(int d, int p, int row0, int n, int m0, int m1, bool half, const int *rp,
+                                  const int *ci, const double *val, double *s64, float *s32, int *flag,
+                                  hipStream_t s);
+hipError_t launch_batch_box_s32(int d, int p, int n, bool half, const double *s64, float *s32, hipStream_t s);
 hipError_t launch_batch_init(int nchunks, const BChunk *chunks, const BSeg *segs, BState *st, double *p0,
                              double *p1, double *p2, int mixed, hipStream_t s);
 hipError_t launch_batch_iteration(int nchunks, const BChunk *chunks, const BSeg *segs, BState *st, double *p0,

@@ -642,17 +642,23 @@ __global__ __launch_bounds__(256) void k_residual(int n, const int *__restrict__
     if (row < n && lane == 0) r[row] = (T)((double)b[row] - acc);
 }
 
-// y = R x (restriction), one thread per coarse row; summed in FP64, stored
-// in the coarse vector's type TY
+// y = R x (restriction), kRestrictLanes lanes per coarse row (consecutive
+// lanes read consecutive entries of the row); lane partials summed in FP64
+// by butterfly shuffles (a fixed order), stored in the coarse vector's type TY
+constexpr int kRestrictLanes = 8;
 template <typename V, typename T, typename TY>
-__global__ void k_restrict(int n, const int *__restrict__ rp, const int *__restrict__ ci,
-                           const V *__restrict__ v, const T *__restrict__ x, TY *__restrict__ y)
+__global__ __launch_bounds__(256) void k_restrict(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                                                  const V *__restrict__ v, const T *__restrict__ x,
+                                                  TY *__restrict__ y)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
+    constexpr int L = kRestrictLanes;
+    const int r = (int)((blockIdx.x * (int64_t)blockDim.x + threadIdx.x) / L), lane = threadIdx.x % L;
+    if (r >= n) return;   // the row's whole group leaves together
     double s = 0.0;
-    for (int k = rp[r]; k < rp[r + 1]; ++k) s += (double)v[k] * (double)x[ci[k]];
-    y[r] = (TY)s;
+    for (int k = rp[r] + lane; k < rp[r + 1]; k += L) s += (double)v[k] * (double)x[ci[k]];
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) y[r] = (TY)s;
 }
 
 template <typename S, typename Dd>
@@ -791,14 +797,14 @@ hipError_t launch_prolong_add_f32(int n, const int *rp, const int *ci, const flo
 hipError_t launch_restrict_f64(int n, const int *rp, const int *ci, const double *v, const double *x,
                                double *y, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_restrict<double, double, double>), g1(n), dim3(256), 0, s, n, rp, ci, v, x, y);
+    hipLaunchKernelGGL((k_restrict<double, double, double>), g1((int64_t)n * kRestrictLanes), dim3(256), 0, s, n, rp, ci, v, x, y);
     return hipGetLastError();
 }
 
 hipError_t launch_restrict_f32(int n, const int *rp, const int *ci, const float *v, const float *x, float *y,
                                hipStream_t s)
 {
-    hipLaunchKernelGGL((k_restrict<float, float, float>), g1(n), dim3(256), 0, s, n, rp, ci, v, x, y);
+    hipLaunchKernelGGL((k_restrict<float, float, float>), g1((int64_t)n * kRestrictLanes), dim3(256), 0, s, n, rp, ci, v, x, y);
     return hipGetLastError();
 }
 
@@ -812,7 +818,7 @@ hipError_t launch_prolong_add_v32(int n, const int *rp, const int *ci, const flo
 hipError_t launch_restrict_v32(int n, const int *rp, const int *ci, const float *v, const double *x, float *y,
                                hipStream_t s)
 {
-    hipLaunchKernelGGL((k_restrict<float, double, float>), g1(n), dim3(256), 0, s, n, rp, ci, v, x, y);
+    hipLaunchKernelGGL((k_restrict<float, double, float>), g1((int64_t)n * kRestrictLanes), dim3(256), 0, s, n, rp, ci, v, x, y);
     return hipGetLastError();
 }
 

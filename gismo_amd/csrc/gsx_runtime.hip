@@ -513,6 +513,15 @@ gsx_status gsx_ctx_set_elem_budget(gsx_ctx *c, int64_t bytes)
     return GSX_OK;
 }
 
+gsx_status gsx_ctx_arch(const gsx_ctx *c, char *buf, int32_t cap)
+{
+    if (!c || !buf || cap < 1) return GSX_ERR_ARG;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) return GSX_ERR_HIP;
+    std::snprintf(buf, (size_t)cap, "%s", prop.gcnArchName);
+    return GSX_OK;
+}
+
 gsx_status gsx_ctx_destroy(gsx_ctx *c)
 {
     if (!c) return GSX_ERR_ARG;
@@ -3741,6 +3750,8 @@ static gsx_status batch_add(gsx_batch *B, gsx_matrix *A, int32_t row0, int32_t n
         return GSX_ERR_ARG;
     gsx_ctx *c = B->c;
     if (box) {
+        if ((box[5] != 1 && box[5] != 2 && box[5] != 4) || (box[6] != 0 && box[6] != 1))
+            return fail(c, GSX_ERR_ARG, "box member: lanes 1, 2 or 4 and half 0 or 1");
         const int d = box[0], p = box[1];
         const int64_t cells = (int64_t)box[2] * box[3] * (d == 3 ? box[4] : 1);
         if ((d != 2 && d != 3) || p < 1 || p > 4 || box[2] < 1 || box[3] < 1 || (d == 3 && box[4] < 1) ||
@@ -3759,16 +3770,25 @@ static gsx_status batch_add(gsx_batch *B, gsx_matrix *A, int32_t row0, int32_t n
         }
     if (slot < 0) return fail(c, GSX_ERR_ARG, "batch is full");
     hipStream_t s = c->stream;
-    // the member's rows must only couple to its own rows
+    // A box member that is the whole of a single-patch matrix whose
+    // structured operator the assembly has filled (the same box) iterates on
+    // that operator itself: no copy, no checks (the rows are the patch's)
+    const StencilOp &S = A->st;
+    const bool own = box && row0 == 0 && nrows == A->n && S.available && S.s64_ready && S.d == box[0] &&
+                     S.p == box[1] && S.m[0] == box[2] && S.m[1] == box[3] && (S.d == 2 || S.m[2] == box[4]) &&
+                     (!B->mixed || S.s32_ready);
     int rp2[2] = {0, 0};
-    HIP_OK(c, hipMemcpyAsync(&rp2[0], dp<int>(A->rowptr) + row0, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_OK(c, hipMemcpyAsync(&rp2[1], dp<int>(A->rowptr) + row0 + nrows, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_OK(c, hipMemsetAsync(B->d_bad.get(), 0, sizeof(int), s));
-    HIP_OK(c, gsx::launch_batch_check(row0, nrows, dp<int>(A->rowptr), dp<int>(A->col), dp<int>(B->d_bad), s));
     int bad = 0;
-    HIP_OK(c, hipMemcpyAsync(&bad, B->d_bad.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-    HIP_OK(c, hipStreamSynchronize(s));
-    if (bad) return fail(c, GSX_ERR_ARG, "member rows couple to rows outside [row0, row0 + nrows)");
+    if (!own) {
+        // the member's rows must only couple to its own rows
+        HIP_OK(c, hipMemcpyAsync(&rp2[0], dp<int>(A->rowptr) + row0, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_OK(c, hipMemcpyAsync(&rp2[1], dp<int>(A->rowptr) + row0 + nrows, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_OK(c, hipMemsetAsync(B->d_bad.get(), 0, sizeof(int), s));
+        HIP_OK(c, gsx::launch_batch_check(row0, nrows, dp<int>(A->rowptr), dp<int>(A->col), dp<int>(B->d_bad), s));
+        HIP_OK(c, hipMemcpyAsync(&bad, B->d_bad.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_OK(c, hipStreamSynchronize(s));
+        if (bad) return fail(c, GSX_ERR_ARG, "member rows couple to rows outside [row0, row0 + nrows)");
+    }
     if (!A->diag && precond == GSX_PREC_JACOBI) {
         A->diag = dalloc(c, sizeof(double) * A->n);
         if (!A->diag) return fail(c, GSX_ERR_OOM, "device allocation (diag)");
@@ -3829,28 +3849,50 @@ static gsx_status batch_add(gsx_batch *B, gsx_matrix *A, int32_t row0, int32_t n
     g.tau = tau;
     g.tol = tol;
     g.maxit = maxit;
-    g.lpr = gsx::spmv_lpr((double)(rp2[1] - rp2[0]) / nrows);
+    g.lpr = own ? 8 : gsx::spmv_lpr((double)(rp2[1] - rp2[0]) / nrows);
     g.row0 = row0;
-    if (box) {
+    if (own) {
+        const int d = box[0], p = box[1];
+        g.s64 = dp<double>(S.s64);
+        g.s32 = B->mixed ? dp<float>(S.s32) : nullptr;
+        if (B->mixed && box[6] == 0) {
+            // the full FP32 form asked for: a copy from the FP64 slots
+            const int64_t ns = gsx::batch_box_slots(d, p, false);
+            M.s32 = dalloc(c, sizeof(float) * gsx::stencil_size(nrows, (int)ns));
+            if (!M.s32) return fail(c, GSX_ERR_OOM, "device allocation (box member operator)");
+            HIP_OK(c, gsx::launch_batch_box_s32(d, p, nrows, false, dp<double>(S.s64),
+                                                dp<float>(M.s32), s));
+            g.s32 = dp<float>(M.s32);
+        }
+        g.bd = d;
+        g.bp = p;
+        g.tpr = box[5];
+        g.half = box[6];
+        for (int k = 0; k < 3; ++k) g.bm[k] = box[2 + k];
+    } else if (box) {
         // the member's structured operator, built from its CSR rows (which
         // must lie in the box's stencil window)
         const int d = box[0], p = box[1];
+        const bool half = box[6] != 0;
         const int64_t n64 = gsx::stencil_size(nrows, (int)gsx::batch_box_slots(d, p, false));
-        const int64_t n32 = gsx::stencil_size(nrows, (int)gsx::batch_box_slots(d, p, true));
+        const int64_t n32 = gsx::stencil_size(nrows, (int)gsx::batch_box_slots(d, p, half));
         M.s64 = dalloc(c, sizeof(double) * n64);
         if (B->mixed) M.s32 = dalloc(c, sizeof(float) * n32);
         if (!M.s64 || (B->mixed && !M.s32)) return fail(c, GSX_ERR_OOM, "device allocation (box member operator)");
+        KScope ks(c, "batch_box_build");
         HIP_OK(c, gsx::launch_fill_f64(dp<double>(M.s64), n64, 0.0, s));
         if (B->mixed) HIP_OK(c, gsx::launch_fill_f32(dp<float>(M.s32), n32, 0.0f, s));
         HIP_OK(c, hipMemsetAsync(B->d_bad.get(), 0, sizeof(int), s));
-        HIP_OK(c, gsx::launch_batch_box_build(d, p, row0, nrows, box[2], box[3], dp<int>(A->rowptr), dp<int>(A->col),
-                                              dp<double>(A->val), dp<double>(M.s64),
+        HIP_OK(c, gsx::launch_batch_box_build(d, p, row0, nrows, box[2], box[3], half, dp<int>(A->rowptr),
+                                              dp<int>(A->col), dp<double>(A->val), dp<double>(M.s64),
                                               B->mixed ? dp<float>(M.s32) : nullptr, dp<int>(B->d_bad), s));
         HIP_OK(c, hipMemcpyAsync(&bad, B->d_bad.get(), sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_OK(c, hipStreamSynchronize(s));
         if (bad) return fail(c, GSX_ERR_ARG, "box member: an entry lies outside the box's stencil window");
         g.bd = d;
         g.bp = p;
+        g.tpr = box[5];
+        g.half = half ? 1 : 0;
         for (int k = 0; k < 3; ++k) g.bm[k] = box[2 + k];
         g.s64 = dp<double>(M.s64);
         g.s32 = dp<float>(M.s32);
@@ -3873,11 +3915,44 @@ gsx_status gsx_batch_add(gsx_batch *B, gsx_matrix *A, int32_t row0, int32_t nrow
 }
 
 gsx_status gsx_batch_add_box(gsx_batch *B, gsx_matrix *A, int32_t row0, int32_t nrows, int32_t d, int32_t p,
-                             int32_t m0, int32_t m1, int32_t m2, int32_t precond, double tau, const double *b,
-                             double *x, double tol, int32_t maxit, int32_t *id)
+                             int32_t m0, int32_t m1, int32_t m2, int32_t lanes, int32_t half, int32_t precond,
+                             double tau, const double *b, double *x, double tol, int32_t maxit, int32_t *id)
 {
-    const int32_t box[5] = {d, p, m0, m1, m2};
+    const int32_t box[7] = {d, p, m0, m1, m2, lanes, half};
     return batch_add(B, A, row0, nrows, box, precond, tau, b, x, tol, maxit, id);
+}
+
+gsx_status gsx_batch_set_tile(gsx_batch *B, int32_t id, int32_t lanes, int32_t half)
+{
+    if (!B || id < 0 || id >= B->cap || !B->m[id].used) return GSX_ERR_ARG;
+    gsx_ctx *c = B->c;
+    BMember &M = B->m[id];
+    gsx::BSeg &g = M.seg;
+    if (M.st.phase != 0) return fail(c, GSX_ERR_ARG, "tile shape: set it before the member's first step");
+    if (!g.bd) {
+        if (lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
+            return fail(c, GSX_ERR_ARG, "tile shape: CSR members take 8, 16, 32 or 64 lanes per row");
+        g.lpr = lanes;
+    } else {
+        if (lanes != 1 && lanes != 2 && lanes != 4)
+            return fail(c, GSX_ERR_ARG, "tile shape: box members take 1, 2 or 4 threads per row");
+        if (half != 0 && half != 1) return GSX_ERR_ARG;
+        if (B->mixed && half != g.half) {
+            // the FP32 copy in the other form, from the FP64 stencil
+            HIP_OK(c, hipStreamSynchronize(c->stream));   // kernels may still read the old copy
+            const int64_t ns = gsx::batch_box_slots(g.bd, g.bp, half != 0);
+            DevPtr s32 = dalloc(c, sizeof(float) * gsx::stencil_size(M.nrows, (int)ns));
+            if (!s32) return fail(c, GSX_ERR_OOM, "device allocation (box member operator)");
+            HIP_OK(c, gsx::launch_batch_box_s32(g.bd, g.bp, M.nrows, half != 0, dp<double>(M.s64), dp<float>(s32),
+                                                c->stream));
+            M.s32 = std::move(s32);
+            g.s32 = dp<float>(M.s32);
+        }
+        g.tpr = lanes;
+        g.half = half;
+    }
+    B->dirty = true;
+    return GSX_OK;
 }
 
 static gsx_status batch_rebuild(gsx_batch *B)
@@ -3889,7 +3964,7 @@ static gsx_status batch_rebuild(gsx_batch *B)
         BMember &M = B->m[k];
         if (!M.used || M.status) continue;
         // box members: one stencil tile of the member per chunk
-        const int rows = M.seg.bd ? gsx::kStencilTile : gsx::batch_chunk_rows(M.seg.lpr);
+        const int rows = M.seg.bd ? gsx::kStencilTile / M.seg.tpr : gsx::batch_chunk_rows(M.seg.lpr);
         M.seg.c0 = (int)B->chunks_h.size();
         for (int lo = M.row0; lo < M.row0 + M.nrows; lo += rows)
             B->chunks_h.push_back({k, lo, std::min(lo + rows, M.row0 + M.nrows)});

@@ -99,19 +99,6 @@ __global__ void k_gather_from_grid(int nfree, const int *__restrict__ free2grid,
     if (r < nfree) dst[r] = (double)src[free2grid[r]];
 }
 
-// Workgroups are dispatched to the 8 XCDs round-robin (block b to XCD b % 8),
-// each XCD with its own L2.  Logical block of physical block b: XCD x takes
-// the contiguous range [start(x), start(x) + count(x)) of logical blocks, so
-// neighbouring rows, which read the same x planes, share an L2.  A bijection
-// of [0, nb).
-__device__ inline int xcd_block(int b, int nb)
-{
-    const int x = b & 7, j = b >> 3;
-    int start = 0;
-    for (int y = 0; y < x; ++y) start += (nb - y + 7) >> 3;
-    return start + j;
-}
-
 // ---------------------------------------------------------------------------
 // Stencil SpMV.  MODE 0: y = A x; 1: y = A x, partial (x, y); 2: y = b - A x,
 // partial (y, y).  Vectors on the grid; rows = free functions.

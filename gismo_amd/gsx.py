@@ -145,7 +145,10 @@ SIGNATURES = {
     "gsx_batch_add": [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_double, _vp, _vp, C.c_double,
                       C.c_int32, _i32p],
     "gsx_batch_add_box": [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
-                          C.c_int32, C.c_double, _vp, _vp, C.c_double, C.c_int32, C.POINTER(C.c_int32)],
+                          C.c_int32, C.c_int32, C.c_int32, C.c_double, _vp, _vp, C.c_double, C.c_int32,
+                          C.POINTER(C.c_int32)],
+    "gsx_batch_set_tile": [_vp, C.c_int32, C.c_int32, C.c_int32],
+    "gsx_ctx_arch": [_vp, C.c_char_p, C.c_int32],
     "gsx_batch_step": [_vp, C.c_int32, _i32p],
     "gsx_batch_set_graph": [_vp, C.c_int32],
     "gsx_batch_graph_launches": [_vp, _i64p],
@@ -262,6 +265,12 @@ class Context:
     def to_device(self, a, dtype=np.float64):
         a = np.asarray(a, dtype=dtype)
         return DeviceArray(self, a.size, dtype).upload(a)
+
+    def arch(self):
+        """The device's architecture name (gfx950 on MI355X)."""
+        buf = C.create_string_buffer(128)
+        _check(self.h, lib().gsx_ctx_arch(self.h, buf, 128))
+        return buf.value.decode()
 
     def synchronize(self):
         _check(self.h, lib().gsx_ctx_synchronize(self.h))
@@ -636,20 +645,27 @@ class Batch:
         self.h = h
 
     def add(self, matrix, b, x, row0=0, nrows=None, tol=1e-10, maxit=1000, precond=PREC_JACOBI, tau=1.0,
-            box=None):
+            box=None, tile=None):
         """box = (d, p, m0, m1, m2): the rows are the free functions of one
         tensor-product patch (gsx_batch_add_box): the member iterates on a
-        structured copy of its block."""
+        structured copy of its block.  tile = (lanes, half): the SpMV tile
+        shape (gsx_batch_set_tile)."""
         nrows = matrix.n - row0 if nrows is None else nrows
         i = C.c_int32()
         if box is None:
             _check(self.ctx.h, lib().gsx_batch_add(self.h, matrix.h, row0, nrows, precond, tau, b.ptr, x.ptr, tol,
                                                    maxit, C.byref(i)))
+            if tile is not None:
+                self.set_tile(i.value, *tile)
         else:
             d, p, m0, m1, m2 = box
-            _check(self.ctx.h, lib().gsx_batch_add_box(self.h, matrix.h, row0, nrows, d, p, m0, m1, m2, precond,
-                                                       tau, b.ptr, x.ptr, tol, maxit, C.byref(i)))
+            lanes, half = (1, 1) if tile is None else tile
+            _check(self.ctx.h, lib().gsx_batch_add_box(self.h, matrix.h, row0, nrows, d, p, m0, m1, m2, lanes,
+                                                       half, precond, tau, b.ptr, x.ptr, tol, maxit, C.byref(i)))
         return i.value
+
+    def set_tile(self, i, lanes, half=1):
+        _check(self.ctx.h, lib().gsx_batch_set_tile(self.h, i, lanes, half))
 
     def step(self, iters=8):
         run = C.c_int32()

@@ -102,6 +102,9 @@ gsx_status gsx_ctx_create(int32_t device, gsx_ctx **out);
 gsx_status gsx_ctx_destroy(gsx_ctx *ctx);
 const char *gsx_last_error(const gsx_ctx *ctx);
 gsx_status gsx_ctx_synchronize(gsx_ctx *ctx);
+/* The device's architecture name (e.g. "gfx950:sramecc+:xnack-"), NUL-
+ * terminated in buf (cap bytes); keys the autotuner's tables. */
+gsx_status gsx_ctx_arch(const gsx_ctx *ctx, char *buf, int32_t cap);
 /* Kernel timing: when enabled, every launch is bracketed by HIP events on
  * the context's stream; gsx_ctx_kernel_stats returns, for kernel name
  * `name`, the number of launches and the summed milliseconds since the last
@@ -457,13 +460,25 @@ gsx_status gsx_batch_add(gsx_batch *batch, gsx_matrix *A, int32_t row0,
  * with Dirichlet sides or one of several disjoint patches: its CSR rows must
  * couple only within the box's (2p+1)^d stencil window (else GSX_ERR_ARG).
  * The member then iterates on a structured copy of its block (all stencil
- * slots in FP64, the upper half in FP32 for GSX_MIXED; no column indices),
- * built here from A's values. */
+ * slots in FP64; for GSX_MIXED the upper half (half = 1) or all slots
+ * (half = 0) in FP32; no column indices), built here from A's values.
+ * lanes: threads per row of its SpMV (1, 2 or 4), as gsx_batch_set_tile. */
 gsx_status gsx_batch_add_box(gsx_batch *batch, gsx_matrix *A, int32_t row0,
                              int32_t nrows, int32_t d, int32_t p, int32_t m0,
-                             int32_t m1, int32_t m2, int32_t precond, double tau,
-                             const double *b_dev, double *x_dev, double tol,
-                             int32_t maxit, int32_t *id);
+                             int32_t m1, int32_t m2, int32_t lanes, int32_t half,
+                             int32_t precond, double tau, const double *b_dev,
+                             double *x_dev, double tol, int32_t maxit,
+                             int32_t *id);
+/* Tile shape of a member's SpMV, set before its first step.  CSR members:
+ * lanes per row 8, 16, 32 or 64 (default from the mean row length).  Box
+ * members: 1, 2 or 4 threads per row (default 1; a chunk holds 256 / lanes
+ * rows and each thread sums every lanes-th stencil line), and for GSX_MIXED
+ * the FP32 operator as the upper half of the stencil (half = 1, default) or
+ * all slots (half = 0).  The choice changes the summation order, not the
+ * operator: a member's iterates depend on its own shape only.  The project's
+ * autotuner (gismo_amd/autotune.py) times the shapes per (d, p, rows) class
+ * on the device and keeps the table in gismo_amd/tuning/. */
+gsx_status gsx_batch_set_tile(gsx_batch *batch, int32_t id, int32_t lanes, int32_t half);
 /* Initialise newly added members, run iters CG iterations of every running
  * member (members that finish stop on their own), then the reliable update
  * (mixed).  running: members still iterating afterwards.  The step's

@@ -311,3 +311,108 @@ def test_box_members_of_a_disjoint_patch_matrix_and_wrong_boxes(ctx):
     for a in (Aj, bj, xj):
         a.close()
     gj.close()
+
+
+@pytest.mark.parametrize("precision", [gsx.FP64, gsx.MIXED])
+def test_tile_shapes_reach_the_oracle_solution(ctx, precision):
+    """Every SpMV tile shape of box members (1, 2, 4 threads per row; half
+    or full FP32 stencil) and of CSR members (8..64 lanes per row) reaches
+    the oracle's solution; a member with a non-default shape is bitwise the
+    same alone and in a mix; bad shapes and late changes are refused."""
+    probs = _small_batch(count=6, seed=13)
+    members = [_Member(ctx, p, precision) for p in probs]
+    tol = 1e-10 if precision == gsx.FP64 else 1e-12
+    shapes = [((1, 1), None), ((2, 1), None), ((4, 1), None), (None, 16), (None, 64)]
+    if precision == gsx.MIXED:
+        shapes += [((1, 0), None), ((4, 0), None)]
+    oracle = [_oracle(p, 1e-14)[2] for p in probs]
+    sols = {}
+    for box_tile, lanes in shapes:
+        B = gsx.Batch(ctx, precision, capacity=len(members))
+        ids = []
+        for m in members:
+            if box_tile is not None:
+                ids.append(B.add(m.A, m.b, m.x.zero(), tol=tol, maxit=5000, box=_box(m.prob), tile=box_tile))
+            else:
+                ids.append(B.add(m.A, m.b, m.x.zero(), tol=tol, maxit=5000, tile=(lanes, 1)))
+        B.run(8)
+        for m, i, xo in zip(members, ids, oracle):
+            st, it, rr = B.result(i)
+            assert st == 1, (box_tile, lanes, m.prob.name)
+            x = m.x.download()
+            assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo), (box_tile, lanes, m.prob.name)
+        sols[(box_tile, lanes)] = [m.x.download() for m in members]
+        B.close()
+    shape = (4, 0) if precision == gsx.MIXED else (4, 1)
+    m = members[2]
+    B = gsx.Batch(ctx, precision, capacity=4)
+    B.add(members[0].A, members[0].b, members[0].x.zero(), tol=tol, maxit=5000, box=_box(members[0].prob))
+    B.add(m.A, m.b, m.x.zero(), tol=tol, maxit=5000, box=_box(m.prob), tile=shape)
+    B.add(members[3].A, members[3].b, members[3].x.zero(), tol=tol, maxit=5000, tile=(32, 1))
+    B.run(8)
+    assert np.array_equal(m.x.download(), sols[(shape, None)][2])
+    with pytest.raises(gsx.GsxError):
+        B.set_tile(1, 4, 1)   # after the first step
+    B.close()
+    B = gsx.Batch(ctx, precision, capacity=4)
+    i = B.add(m.A, m.b, m.x.zero(), tol=tol, maxit=5000, box=_box(m.prob))
+    j = B.add(m.A, m.b, m.x.zero(), tol=tol, maxit=5000)
+    for bad in ((i, 3, 1), (i, 8, 1), (j, 4, 1), (i, 2, 2)):
+        with pytest.raises(gsx.GsxError):
+            B.set_tile(*bad)
+    B.close()
+    for mm in members:
+        mm.close()
+
+
+def test_autotuner_runs_and_picks_the_fastest(ctx):
+    """A reduced tuning run (2D, p = 2, every size bucket) times every
+    candidate shape and picks the fastest per class."""
+    from gismo_amd import autotune
+    classes = autotune.tune(ctx, iters=4, reps=1, degrees=(2,), dims=(2,), log=lambda s: print(s, flush=True))
+    assert set(classes) == {f"{pr}-d2-p2-{b}" for pr in ("mixed", "fp64") for b in ("s", "m", "l")}
+    for key, e in classes.items():
+        times = e["us_per_iteration"]
+        assert len(times) == len(autotune.candidates(key.startswith("mixed")))
+        assert times[autotune._name(e["best"])] == min(times.values())
+
+
+@pytest.mark.parametrize("precision", [gsx.FP64, gsx.MIXED])
+def test_box_member_on_its_own_patch_operator_equals_block_copy(ctx, precision):
+    """A box member that is a whole single-patch matrix iterates on the
+    operator its assembly filled (no copy); the same 2D patches as blocks of
+    a disjoint-patch matrix get copies built from the CSR values.  Both
+    assemblies sum the same element matrices in the same order, so the
+    operators, and the members' iterates, are bitwise equal."""
+    probs = [p for p in _small_batch(count=12, seed=21) if p.d == 2][:4]
+    tol = 1e-10 if precision == gsx.FP64 else 1e-12
+    members = [_Member(ctx, p, precision) for p in probs]
+    B = gsx.Batch(ctx, precision, capacity=len(members))
+    for m in members:
+        B.add(m.A, m.b, m.x.zero(), tol=tol, maxit=5000, box=_box(m.prob), tile=(1, 1))
+    B.run(8)
+    own = [m.x.download() for m in members]
+    B.close()
+    joint = problems.Problem([p.patches[0] for p in probs],
+                             [(k, s) for k in range(len(probs)) for s in range(1, 5)], [], name="joint2")
+    gj = gsx.Problem(ctx, joint)
+    Aj, bj = gj.assemble(gsx.PATTERN_EXPR, precision)
+    xj = ctx.array(gj.nfree).zero()
+    B = gsx.Batch(ctx, precision, capacity=len(probs))
+    off, offs = 0, []
+    for m in members:
+        n = m.gp.nfree
+        B.add(Aj, bj.view(off, n), xj.view(off, n), row0=off, nrows=n, tol=tol, maxit=5000, box=_box(m.prob),
+              tile=(1, 1))
+        offs.append(off)
+        off += n
+    B.run(8)
+    xall = xj.download()
+    for m, o, x in zip(members, offs, own):
+        assert np.array_equal(xall[o:o + m.gp.nfree], x), m.prob.name
+    B.close()
+    for a in (Aj, bj, xj):
+        a.close()
+    gj.close()
+    for m in members:
+        m.close()
