@@ -7,9 +7,10 @@ device memory fits the HBM budget.  Two lanes, each with its own context
 
 - batch lane: an assembler thread assembles each request and hands the
   resident operator to the solver thread, which adds one batched-CG member
-  per right-hand side to the running `gsx_batch` and advances every member
-  together, a few iterations per step; members join and leave between
-  steps;
+  per right-hand side to the running `gsx_batch` (box members on the
+  request's structured operator, with the autotuner's tile shape) and
+  advances every member together, a few iterations per step; members join
+  and leave between steps;
 - multigrid lane: requests of at least `mg_min_dofs` free DOFs whose patch
   coarsens dyadically to a small coarsest level go to a worker that
   assembles the operator, builds its multigrid hierarchy once, and solves
@@ -75,13 +76,25 @@ class Result:
 
 def estimate_bytes(prob, nrhs, mixed):
     """Device bytes a request keeps resident: CSR pattern + values (+ FP32
-    copy), the load vector, and per right-hand side the solution, the rhs
-    and the CG workspace."""
+    copy), the structured operator (FP64 slots, FP32 upper half), the load
+    vector, and per right-hand side the solution, the rhs and the CG
+    workspace."""
     P = prob.patches[0]
     nfree = int(np.prod([nb - 2 for nb in P.nbasis_dir()]))
-    nnz = nfree * (2 * P.p[0] + 1) ** P.d
+    slots = (2 * P.p[0] + 1) ** P.d
+    nnz = nfree * slots
+    tiles = (nfree + 255) // 256 * 256
+    stencil = tiles * slots * 8 + (tiles * (slots - slots // 2) * 4 if mixed else 0)
     per_rhs = nfree * (8 * 6 + (4 * 6 if mixed else 0))
-    return int(nnz * (4 + 8 + (4 if mixed else 0)) + nfree * 8 * 3 + nrhs * per_rhs + (64 << 20))
+    return int(nnz * (4 + 8 + (4 if mixed else 0)) + stencil + nfree * 8 * 3 + nrhs * per_rhs + (64 << 20))
+
+
+def box_of(prob):
+    """(d, p, m0, m1, m2): the free functions of a single patch with
+    Dirichlet sides (the box of a box member)."""
+    P = prob.patches[0]
+    m = [nb - 2 for nb in P.nbasis_dir()] + [1]
+    return (P.d, P.p[0], m[0], m[1], m[2])
 
 
 def mg_plan(prob, max_coarse=4096):
@@ -157,6 +170,8 @@ class Server:
         self.ctx_mg = gsx.Context(device)
         self._mgq = queue.Queue()          # admitted multigrid-lane requests
         self.batch = gsx.Batch(self.ctx, precision, capacity=capacity)
+        from . import autotune
+        self.tiles = autotune.Table.load(self.ctx.arch())   # SpMV tile shape per member class
         self.capacity = capacity
         self.t0 = time.perf_counter()
         self._incoming = queue.Queue()     # submitted requests
@@ -210,7 +225,9 @@ class Server:
                         continue
                     try:
                         e.gp = gsx.Problem(self.ctx_asm, req.problem)
-                        e.A, b0 = e.gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_CSR_ONLY, self.precision)
+                        # the assembly fills the structured operator the
+                        # request's batch members iterate on
+                        e.A, b0 = e.gp.assemble(gsx.PATTERN_EXPR, self.precision)
                         e.bs.append(b0)
                         for s in req.rhs_seeds[1:]:
                             e.bs.append(self.ctx_asm.to_device(rhs_vector(e.gp.nfree, s)))
@@ -316,10 +333,16 @@ class Server:
             except queue.Empty:
                 break
             try:
+                box = box_of(e.req.problem)
+                kind, lanes, half = self.tiles.lookup(self.mixed, box[0], box[1], e.gp.nfree)
                 for b in e.bs:
                     x = self.ctx.array(e.gp.nfree).zero()
                     e.xs.append(x)
-                    e.ids.append(self.batch.add(e.A, b, x, tol=e.req.tol, maxit=e.req.maxit))
+                    if kind == "box":
+                        e.ids.append(self.batch.add(e.A, b, x, tol=e.req.tol, maxit=e.req.maxit, box=box,
+                                                    tile=(lanes, half)))
+                    else:
+                        e.ids.append(self.batch.add(e.A, b, x, tol=e.req.tol, maxit=e.req.maxit, tile=(lanes, 1)))
                     self._members += 1
             except gsx.GsxError as ex:
                 for i in e.ids:
