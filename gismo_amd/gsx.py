@@ -72,12 +72,36 @@ class PatchDesc(C.Structure):
     ]
 
 
+
+
 _lib = None
 _vp = C.c_void_p
 _pp = C.POINTER(C.c_void_p)
 _i64p = C.POINTER(C.c_int64)
 _i32p = C.POINTER(C.c_int32)
 _dp = C.POINTER(C.c_double)
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("device", C.c_int32), ("precision", C.c_int32), ("budget_bytes", C.c_int64),
+                ("iters_per_step", C.c_int32), ("capacity", C.c_int32), ("mg_min_free", C.c_int64),
+                ("mg_max_coarse", C.c_int64), ("elem_budget", C.c_int64)]
+
+
+class ServerRequest(C.Structure):
+    _fields_ = [("patch", C.POINTER(PatchDesc)), ("nrhs", C.c_int32), ("rhs_host", C.POINTER(_dp)),
+                ("x_host", C.POINTER(_dp)), ("tol", C.c_double), ("maxit", C.c_int32),
+                ("iters_host", _i32p), ("relres_host", _dp), ("status_host", _i32p)]
+
+
+class ServerResult(C.Structure):
+    _fields_ = [("rid", C.c_int64), ("status", C.c_int32), ("lane", C.c_int32), ("nrhs", C.c_int32),
+                ("iters_max", C.c_int32), ("nbasis", C.c_int64), ("nfree", C.c_int64), ("relres_max", C.c_double),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_assembled", C.c_double),
+                ("t_done", C.c_double), ("error", C.c_char * 160)]
+
+
+SERVER_FAILED = 3
 
 # (name, argtypes) of every exported function; restype is c_int (gsx_status)
 # unless listed in _RESTYPES.
@@ -149,6 +173,14 @@ SIGNATURES = {
                           C.POINTER(C.c_int32)],
     "gsx_batch_set_tile": [_vp, C.c_int32, C.c_int32, C.c_int32],
     "gsx_ctx_arch": [_vp, C.c_char_p, C.c_int32],
+    "gsx_server_create": [C.POINTER(ServerOpts), _pp],
+    "gsx_server_set_tile": [_vp, C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_int32, C.c_int32],
+    "gsx_server_submit": [_vp, C.POINTER(ServerRequest), _i64p],
+    "gsx_server_poll": [_vp, C.c_int32, C.POINTER(ServerResult), _i32p],
+    "gsx_server_status": [_vp, _i64p, _i64p, C.POINTER(C.c_double)],
+    "gsx_server_drain": [_vp, C.c_double],
+    "gsx_server_destroy": [_vp],
+    "gsx_server_error": [_vp],
     "gsx_batch_step": [_vp, C.c_int32, _i32p],
     "gsx_batch_set_graph": [_vp, C.c_int32],
     "gsx_batch_graph_launches": [_vp, _i64p],
@@ -156,7 +188,7 @@ SIGNATURES = {
     "gsx_batch_history": [_vp, C.c_int32, _dp, C.c_int32, _i32p],
     "gsx_batch_remove": [_vp, C.c_int32],
 }
-_RESTYPES = {"gsx_last_error": C.c_char_p}
+_RESTYPES = {"gsx_last_error": C.c_char_p, "gsx_server_error": C.c_char_p}
 
 
 def lib():
@@ -306,6 +338,33 @@ class Context:
             self.h = None
 
 
+def fill_patch_desc(o, P, keep):
+    """Fill the PatchDesc o from a problems.Patch; the host arrays it points
+    to are appended to keep (they must outlive the call that reads o)."""
+    o.d = P.d
+    for k in range(3):
+        if k < P.d:
+            kn = np.ascontiguousarray(P.knots[k], dtype=np.float64)
+            keep.append(kn)
+            o.p[k] = int(P.p[k])
+            o.nknots[k] = kn.size
+            o.knots_host[k] = kn.ctypes.data_as(_dp)
+    ctrl = np.ascontiguousarray(P.ctrl, dtype=np.float64)
+    keep.append(ctrl)
+    o.ctrl_host = ctrl.ctypes.data_as(_dp)
+    for name, attr in (("kappa_host", "kappa"), ("felem_host", "felem")):
+        a = getattr(P, attr, None)
+        if a is None:
+            setattr(o, name, _dp())
+        else:
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            keep.append(a)
+            setattr(o, name, a.ctypes.data_as(_dp))
+    o.fkind = int(P.fkind)
+
+
+
+
 class Problem:
     """Patches + DOF mapper on the device (gsMultiPatch, gsMultiBasis,
     gsFeSpace::setup with glued interfaces and eliminated Dirichlet sides)."""
@@ -318,27 +377,7 @@ class Problem:
         arr = (PatchDesc * n)()
         self._keep = []
         for i, P in enumerate(patches):
-            o = arr[i]
-            o.d = P.d
-            for k in range(3):
-                if k < P.d:
-                    kn = np.ascontiguousarray(P.knots[k], dtype=np.float64)
-                    self._keep.append(kn)
-                    o.p[k] = int(P.p[k])
-                    o.nknots[k] = kn.size
-                    o.knots_host[k] = kn.ctypes.data_as(_dp)
-            ctrl = np.ascontiguousarray(P.ctrl, dtype=np.float64)
-            self._keep.append(ctrl)
-            o.ctrl_host = ctrl.ctypes.data_as(_dp)
-            for name, attr in (("kappa_host", "kappa"), ("felem_host", "felem")):
-                a = getattr(P, attr, None)
-                if a is None:
-                    setattr(o, name, _dp())
-                else:
-                    a = np.ascontiguousarray(a, dtype=np.float64)
-                    self._keep.append(a)
-                    setattr(o, name, a.ctypes.data_as(_dp))
-            o.fkind = int(P.fkind)
+            fill_patch_desc(arr[i], P, self._keep)
         ds = np.ascontiguousarray(np.asarray(problem.dir_sides, dtype=np.int32).reshape(-1))
         ifs = np.ascontiguousarray(np.asarray(problem.ifaces, dtype=np.int32).reshape(-1))
         self._keep += [ds, ifs]

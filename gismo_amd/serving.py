@@ -1,28 +1,26 @@
-"""Continuous-batching solve server on one GPU.
+"""Continuous-batching solve server on one GPU: the Python face of the
+native server (gsx_server_* in the C ABI, gismo_amd/csrc/gsx_server.hip).
 
 Requests (a single-patch Poisson/diffusion problem plus 1..8 right-hand
 sides) arrive at any time and are admitted in arrival order while their
-device memory fits the HBM budget.  Two lanes, each with its own context
-(its own HIP stream), run beside the solver thread's batch:
+estimated device memory fits the HBM budget.  The native server runs three
+threads, each with its own context (its own HIP stream):
 
-- batch lane: an assembler thread assembles each request and hands the
-  resident operator to the solver thread, which adds one batched-CG member
-  per right-hand side to the running `gsx_batch` (box members on the
-  request's structured operator, with the autotuner's tile shape) and
-  advances every member together, a few iterations per step; members join
-  and leave between steps;
+- assembler: admits requests and assembles each one (its structured operator
+  included) with its right-hand sides uploaded;
+- solver (batch lane): one box member per right-hand side on the request's
+  own operator, with the autotuner's tile shape, in one running gsx_batch
+  whose steps advance every member together; members join and leave
+  between steps;
 - multigrid lane: requests of at least `mg_min_dofs` free DOFs whose patch
-  coarsens dyadically to a small coarsest level go to a worker that
-  assembles the operator, builds its multigrid hierarchy once, and solves
-  every right-hand side by MG-preconditioned CG with the hierarchy resident.
+  coarsens dyadically to a small coarsest level: the operator is assembled,
+  its hierarchy built once, and every right-hand side solved by
+  MG-preconditioned CG with the hierarchy resident.
 
-A request is complete when all its right-hand sides have converged and its
-solutions are back on the host; its operator, hierarchy and workspace are
-then released and its budget freed.
-
-Every member's iterates are independent of the other members (the batched
-CG reduces per member in a fixed order; the multigrid lane solves one
-request at a time), so a request's solutions are bitwise the same alone and
+A request is complete when all its right-hand sides have finished and its
+solutions are in host memory; its operator, hierarchy and workspace are then
+released and its budget freed.  Every member's iterates are independent of
+the other members, so a request's solutions are bitwise the same alone and
 in any mix.
 
 The reference has no serving layer: it assembles and solves one problem at
@@ -30,8 +28,7 @@ a time (src/gsAssembler/gsExprAssembler.h:754-833,
 src/gsSolver/gsConjugateGradient.hpp:20-93); each member runs that
 algorithm.
 """
-import queue
-import threading
+import ctypes as C
 import time
 from dataclasses import dataclass
 from typing import List, Optional
@@ -74,60 +71,6 @@ class Result:
         return self.done - self.arrival
 
 
-def estimate_bytes(prob, nrhs, mixed):
-    """Device bytes a request keeps resident: CSR pattern + values (+ FP32
-    copy), the structured operator (FP64 slots, FP32 upper half), the load
-    vector, and per right-hand side the solution, the rhs and the CG
-    workspace."""
-    P = prob.patches[0]
-    nfree = int(np.prod([nb - 2 for nb in P.nbasis_dir()]))
-    slots = (2 * P.p[0] + 1) ** P.d
-    nnz = nfree * slots
-    tiles = (nfree + 255) // 256 * 256
-    stencil = tiles * slots * 8 + (tiles * (slots - slots // 2) * 4 if mixed else 0)
-    per_rhs = nfree * (8 * 6 + (4 * 6 if mixed else 0))
-    return int(nnz * (4 + 8 + (4 if mixed else 0)) + stencil + nfree * 8 * 3 + nrhs * per_rhs + (64 << 20))
-
-
-def box_of(prob):
-    """(d, p, m0, m1, m2): the free functions of a single patch with
-    Dirichlet sides (the box of a box member)."""
-    P = prob.patches[0]
-    m = [nb - 2 for nb in P.nbasis_dir()] + [1]
-    return (P.d, P.p[0], m[0], m[1], m[2])
-
-
-def mg_plan(prob, max_coarse=4096):
-    """Levels of a dyadic multigrid hierarchy for a single-patch problem, or
-    0: elements per direction n = m 2^k with the coarsest (m elements per
-    direction) at most max_coarse free DOFs, and element-level Galerkin
-    coarsening available for (d, p) (d = 2: p <= 4; d = 3: p <= 3)."""
-    P = prob.patches[0]
-    d, p = P.d, P.p[0]
-    if len(prob.patches) != 1 or (d == 3 and p > 3) or p > 4:
-        return 0
-    ne = P.nelem_dir()
-    if len(set(ne)) != 1:
-        return 0
-    n = ne[0]
-    levels = 1
-    while n % 2 == 0 and n // 2 >= 2:
-        if max(0, n + p - 2) ** d <= max_coarse:
-            break
-        n //= 2
-        levels += 1
-    if levels < 2 or max(0, n + p - 2) ** d > max_coarse:
-        return 0
-    return levels
-
-
-def mg_element_bytes(prob):
-    """Bytes of the fine element matrices the hierarchy is built from."""
-    P = prob.patches[0]
-    nb = (P.p[0] + 1) ** P.d
-    return P.nelem * nb * nb * 8
-
-
 def rhs_vector(nfree, seed):
     rng = np.random.default_rng(seed)
     return rng.uniform(-1.0, 1.0, nfree)
@@ -139,269 +82,128 @@ def rhs_vector(nfree, seed):
 FAILED = 3
 
 
-class _Entry:
-    def __init__(self, req, t_admit):
-        self.req = req
-        self.t_admit = t_admit
-        self.t_asm = None
-        self.gp = None
-        self.A = None
-        self.bs = []
-        self.xs = []
-        self.ids = []
-        self.bytes = 0
-        self.levels = 0
-
-
 class Server:
-    """One GPU, two contexts: one assembles, one solves."""
+    """The native continuous-batching server (gsx_server_create ...)."""
 
     def __init__(self, precision=gsx.MIXED, budget_bytes=160 << 30, iters_per_step=8, capacity=4096,
-                 keep_solutions=False, device=0, mg_min_dofs=20000, elem_budget=16 << 30):
+                 keep_solutions=False, device=0, mg_min_dofs=20000, elem_budget=16 << 30, mg_max_coarse=4096):
         self.precision = precision
-        self.mixed = precision == gsx.MIXED
-        self.budget = budget_bytes
-        self.iters = iters_per_step
         self.keep = keep_solutions
-        self.mg_min_dofs = mg_min_dofs
-        self.elem_budget = elem_budget
-        self.ctx_asm = gsx.Context(device)
-        self.ctx = gsx.Context(device)
-        self.ctx_mg = gsx.Context(device)
-        self._mgq = queue.Queue()          # admitted multigrid-lane requests
-        self.batch = gsx.Batch(self.ctx, precision, capacity=capacity)
+        o = gsx.ServerOpts(device=device, precision=precision, budget_bytes=int(budget_bytes),
+                           iters_per_step=iters_per_step, capacity=capacity, mg_min_free=int(mg_min_dofs),
+                           mg_max_coarse=int(mg_max_coarse), elem_budget=int(elem_budget))
+        h = C.c_void_p()
+        _check_server(None, gsx.lib().gsx_server_create(C.byref(o), C.byref(h)))
+        self.h = h
+        # tile shapes from the autotuner's table (box members' classes)
         from . import autotune
-        self.tiles = autotune.Table.load(self.ctx.arch())   # SpMV tile shape per member class
-        self.capacity = capacity
-        self.t0 = time.perf_counter()
-        self._incoming = queue.Queue()     # submitted requests
-        self._ready = queue.Queue()        # assembled entries
-        self._lock = threading.Lock()
-        self._used = 0                     # admitted bytes
-        self._members = 0                  # batch slots in use (solver thread)
-        self._outstanding = 0              # submitted and not yet finished
-        self._stop = False
-        self._error = None
+        ctx = gsx.Context(device)
+        table = autotune.Table.load(ctx.arch())
+        ctx.close()
+        pname = "mixed" if precision == gsx.MIXED else "fp64"
+        bounds, lo = {}, 0
+        for name, lim in autotune.BUCKETS:
+            bounds[name] = (lo, lim if lim is not None else 1 << 62)
+            lo = lim if lim is not None else lo
+        for key, e in table.entries.items():
+            pr, d, p, bname = key.split("-")
+            b = e["best"]
+            if pr != pname or b["kind"] != "box":
+                continue
+            lo_, hi_ = bounds[bname]
+            gsx.lib().gsx_server_set_tile(self.h, int(d[1:]), int(p[1:]), lo_, hi_, b["lanes"], b["half"])
         self.results = []
-        self._active = {}                  # rid -> entry (solver thread)
-        self._asm_thread = threading.Thread(target=self._assembler, daemon=True)
-        self._asm_thread.start()
-        self._mg_thread = threading.Thread(target=self._mg_worker, daemon=True)
-        self._mg_thread.start()
+        self._inflight = {}     # native rid -> (request, host buffers)
 
     def now(self):
-        return time.perf_counter() - self.t0
+        clk = C.c_double()
+        _check_server(self, gsx.lib().gsx_server_status(self.h, None, None, C.byref(clk)))
+        return clk.value
 
     def submit(self, req):
-        with self._lock:
-            self._outstanding += 1
-        self._incoming.put(req)
-
-    # ----- assembler thread --------------------------------------------
-    def _assembler(self):
-        pending = []
-        try:
-            while not self._stop:
-                try:
-                    pending.append(self._incoming.get(timeout=0.002))
-                except queue.Empty:
-                    pass
-                while pending:
-                    req = pending[0]
-                    levels = self._mg_levels(req)
-                    need = estimate_bytes(req.problem, len(req.rhs_seeds), self.mixed)
-                    if levels:
-                        need += mg_element_bytes(req.problem) + need // 2
-                    with self._lock:
-                        if self._used + need > self.budget and self._used > 0:
-                            break
-                        self._used += need
-                    pending.pop(0)
-                    e = _Entry(req, self.now())
-                    e.bytes = need
-                    if levels:
-                        e.levels = levels
-                        self._mgq.put(e)
-                        continue
-                    try:
-                        e.gp = gsx.Problem(self.ctx_asm, req.problem)
-                        # the assembly fills the structured operator the
-                        # request's batch members iterate on
-                        e.A, b0 = e.gp.assemble(gsx.PATTERN_EXPR, self.precision)
-                        e.bs.append(b0)
-                        for s in req.rhs_seeds[1:]:
-                            e.bs.append(self.ctx_asm.to_device(rhs_vector(e.gp.nfree, s)))
-                        self.ctx_asm.synchronize()
-                    except gsx.GsxError as ex:
-                        self._fail(e, ex, "batch")
-                        continue
-                    e.t_asm = self.now()
-                    self._ready.put(e)
-        except Exception as ex:   # not a request's own error: reported by poll()
-            self._error = ex
-
-    def _release(self, e, objs):
-        for o in objs:
-            if o is not None:
-                try:
-                    o.close()
-                except Exception:
-                    pass
-
-    def _fail(self, e, ex, lane, objs=()):
-        """A request that raised: free what it holds, return its budget and
-        record a failed result; the server keeps serving the others."""
-        self._release(e, list(objs) + list(e.xs) + list(e.bs) + [e.A, e.gp])
-        e.xs, e.bs, e.A, e.gp = [], [], None, None
-        req = e.req
-        res = Result(rid=req.rid, dofs=0, nfree=0, nrhs=len(req.rhs_seeds), arrival=req.arrival,
-                     admitted=e.t_admit, assembled=self.now(), done=self.now(), iterations=[0] * len(req.rhs_seeds),
-                     relres=[float("nan")] * len(req.rhs_seeds), status=[FAILED] * len(req.rhs_seeds),
-                     lane=lane, error=str(ex))
-        with self._lock:
-            self.results.append(res)
-            self._used -= e.bytes
-            self._outstanding -= 1
-
-    def _mg_levels(self, req):
+        """Queue a request; its arrival is req.arrival (server clock)."""
         P = req.problem.patches[0]
+        if len(req.problem.patches) != 1:
+            raise ValueError("the server takes single-patch requests")
+        keep = []
+        desc = gsx.PatchDesc()
+        gsx.fill_patch_desc(desc, P, keep)
         nfree = int(np.prod([nb - 2 for nb in P.nbasis_dir()]))
-        if nfree < self.mg_min_dofs or mg_element_bytes(req.problem) > self.elem_budget:
-            return 0
-        return mg_plan(req.problem)
+        nrhs = len(req.rhs_seeds)
+        rhs = [None] + [np.ascontiguousarray(rhs_vector(nfree, s)) for s in req.rhs_seeds[1:]]
+        xs = [np.empty(nfree) for _ in range(nrhs)]
+        its = np.zeros(nrhs, dtype=np.int32)
+        rels = np.zeros(nrhs)
+        sts = np.zeros(nrhs, dtype=np.int32)
+        rhs_p = (gsx._dp * nrhs)(*[gsx._dp() if a is None else a.ctypes.data_as(gsx._dp) for a in rhs])
+        x_p = (gsx._dp * nrhs)(*[a.ctypes.data_as(gsx._dp) for a in xs])
+        r = gsx.ServerRequest(patch=C.pointer(desc), nrhs=nrhs, rhs_host=rhs_p, x_host=x_p, tol=req.tol,
+                              maxit=req.maxit, iters_host=its.ctypes.data_as(gsx._i32p),
+                              relres_host=rels.ctypes.data_as(gsx._dp), status_host=sts.ctypes.data_as(gsx._i32p))
+        rid = C.c_int64()
+        _check_server(self, gsx.lib().gsx_server_submit(self.h, C.byref(r), C.byref(rid)))
+        self._inflight[rid.value] = (req, xs, its, rels, sts, keep, rhs, rhs_p, x_p)
 
-    # ----- multigrid lane -------------------------------------------------
-    def _mg_worker(self):
-        """Assemble, build the hierarchy once, solve every right-hand side by
-        MG-PCG (V(2,2), damped Jacobi 0.6, exact coarse solve), release."""
-        try:
-            while not self._stop:
-                try:
-                    e = self._mgq.get(timeout=0.002)
-                except queue.Empty:
-                    continue
-                req, ctx = e.req, self.ctx_mg
-                gp = A = b0 = mg = b = x = None
-                try:
-                    gp = gsx.Problem(ctx, req.problem)
-                    A, b0 = gp.assemble(gsx.PATTERN_EXPR | gsx.ASM_KEEP_ELEMENTS, self.precision)
-                    mg = gsx.Multigrid(gp, A, e.levels, 2, 2, 0.6, precision=self.precision)
-                    ctx.synchronize()
-                    t_asm = self.now()
-                    its, rels, sts, sols = [], [], [], []
-                    for k, seed in enumerate(req.rhs_seeds):
-                        b = b0 if k == 0 else ctx.to_device(rhs_vector(gp.nfree, seed))
-                        x = ctx.array(gp.nfree).zero()
-                        it, rel, ok = gsx.mg_cg_solve(A, mg, b, x, tol=req.tol, maxit=req.maxit)
-                        its.append(it)
-                        rels.append(rel)
-                        sts.append(1 if ok else 2)
-                        sols.append(x.download())
-                        x.close()
-                        x = None
-                        if k:
-                            b.close()
-                        b = None
-                except gsx.GsxError as ex:
-                    self._fail(e, ex, "mg", objs=[x, b if b is not b0 else None, mg, b0, A, gp])
-                    continue
-                t_done = self.now()
-                res = Result(rid=req.rid, dofs=gp.nglued, nfree=gp.nfree, nrhs=len(req.rhs_seeds),
-                             arrival=req.arrival, admitted=e.t_admit, assembled=t_asm, done=t_done,
-                             iterations=its, relres=rels, status=sts,
-                             solutions=sols if (self.keep or req.keep) else None, lane="mg")
-                mg.close()
-                A.close()
-                b0.close()
-                gp.close()
-                with self._lock:
-                    self.results.append(res)
-                    self._used -= e.bytes
-                    self._outstanding -= 1
-        except Exception as ex:   # reported by the solver thread
-            self._error = ex
-
-    # ----- solver thread (the caller) -----------------------------------
     def poll(self):
-        """Admit assembled requests into the batch, run one step, collect the
-        requests that finished.  Returns the number of running members."""
-        if self._error is not None:
-            raise self._error
-        while self._members < self.capacity:
-            try:
-                e = self._ready.get_nowait()
-            except queue.Empty:
+        """Collect the requests that finished; returns the number still
+        outstanding."""
+        buf = (gsx.ServerResult * 64)()
+        n = C.c_int32()
+        while True:
+            _check_server(self, gsx.lib().gsx_server_poll(self.h, 64, buf, C.byref(n)))
+            for k in range(n.value):
+                r = buf[k]
+                req, xs, its, rels, sts, *_ = self._inflight.pop(r.rid)
+                failed = r.status == gsx.SERVER_FAILED
+                self.results.append(Result(
+                    rid=req.rid, dofs=int(r.nbasis), nfree=int(r.nfree), nrhs=int(r.nrhs), arrival=req.arrival,
+                    admitted=r.t_admit, assembled=r.t_assembled if r.t_assembled > 0 else r.t_done,
+                    done=r.t_done, iterations=[int(v) for v in its], relres=[float(v) for v in rels],
+                    status=[FAILED] * int(r.nrhs) if failed else [int(v) for v in sts],
+                    solutions=None if failed or not (self.keep or req.keep) else xs,
+                    lane="mg" if r.lane == 1 else "batch", error=r.error.decode() if failed else None))
+            if n.value < 64:
                 break
-            try:
-                box = box_of(e.req.problem)
-                kind, lanes, half = self.tiles.lookup(self.mixed, box[0], box[1], e.gp.nfree)
-                for b in e.bs:
-                    x = self.ctx.array(e.gp.nfree).zero()
-                    e.xs.append(x)
-                    if kind == "box":
-                        e.ids.append(self.batch.add(e.A, b, x, tol=e.req.tol, maxit=e.req.maxit, box=box,
-                                                    tile=(lanes, half)))
-                    else:
-                        e.ids.append(self.batch.add(e.A, b, x, tol=e.req.tol, maxit=e.req.maxit, tile=(lanes, 1)))
-                    self._members += 1
-            except gsx.GsxError as ex:
-                for i in e.ids:
-                    self.batch.remove(i)
-                    self._members -= 1
-                e.ids = []
-                self._fail(e, ex, "batch")
-                continue
-            self._active[e.req.rid] = e
-        running = self.batch.step(self.iters) if self._active else 0
-        for rid in list(self._active):
-            e = self._active[rid]
-            res = [self.batch.result(i) for i in e.ids]
-            if any(r[0] == 0 for r in res):
-                continue
-            sols = [x.download() for x in e.xs]   # results go back to the host
-            t_done = self.now()
-            for i in e.ids:
-                self.batch.remove(i)
-                self._members -= 1
-            result = Result(
-                rid=rid, dofs=e.gp.nglued, nfree=e.gp.nfree, nrhs=len(e.ids), arrival=e.req.arrival,
-                admitted=e.t_admit, assembled=e.t_asm, done=t_done,
-                iterations=[r[1] for r in res], relres=[r[2] for r in res], status=[r[0] for r in res],
-                solutions=sols if (self.keep or e.req.keep) else None, lane="batch")
-            for x in e.xs:
-                x.close()
-            e.A.close()
-            for b in e.bs:
-                b.close()
-            e.gp.close()
-            del self._active[rid]
-            with self._lock:
-                self.results.append(result)
-                self._used -= e.bytes
-                self._outstanding -= 1
-        return running
+        out = C.c_int64()
+        _check_server(self, gsx.lib().gsx_server_status(self.h, C.byref(out), None, None))
+        return out.value
 
     def idle(self):
-        """No request submitted and unfinished (queued, being assembled,
-        waiting for a batch slot or solving)."""
-        with self._lock:
-            return self._outstanding == 0
+        """No request submitted and unfinished."""
+        return self.poll() == 0
+
+    @property
+    def _used(self):
+        used = C.c_int64()
+        _check_server(self, gsx.lib().gsx_server_status(self.h, None, C.byref(used), None))
+        return used.value
 
     def drain(self, timeout=600.0):
         t_end = time.perf_counter() + timeout
-        while not self.idle():
-            self.poll()
+        while self.poll() > 0:
             if time.perf_counter() > t_end:
                 raise RuntimeError("server did not drain")
+            gsx.lib().gsx_server_drain(self.h, 0.05)
+        self.poll()
 
     def close(self):
-        self._stop = True
-        self._asm_thread.join()
-        self._mg_thread.join()
-        self.batch.close()
-        self.ctx.close()
-        self.ctx_asm.close()
-        self.ctx_mg.close()
+        if self.h:
+            gsx.lib().gsx_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _check_server(srv, st):
+    if st != 0:
+        msg = ""
+        if srv is not None and srv.h:
+            msg = (gsx.lib().gsx_server_error(srv.h) or b"").decode()
+        raise gsx.GsxError(st, msg)
 
 
 def dyadic_round(n):

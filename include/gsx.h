@@ -499,6 +499,83 @@ gsx_status gsx_batch_history(const gsx_batch *batch, int32_t id, double *hist_ho
 /* Release a member's slot and workspace (its x_dev keeps the solution). */
 gsx_status gsx_batch_remove(gsx_batch *batch, int32_t id);
 
+/* ---- serving ---------------------------------------------------------- */
+/* Continuous-batching server of single-patch requests on one GPU (native
+ * threads; no reference counterpart: the reference assembles and solves one
+ * problem at a time, src/gsAssembler/gsExprAssembler.h:754-833,
+ * src/gsSolver/gsConjugateGradient.hpp:20-93).  An assembler thread admits
+ * requests in arrival order while their estimated device bytes fit
+ * budget_bytes and assembles them; a solver thread runs every right-hand
+ * side as a box member of one gsx_batch (Jacobi-CG, members join and leave
+ * between steps of iters_per_step iterations); requests with at least
+ * mg_min_free free DOFs whose elements coarsen dyadically to at most
+ * mg_max_coarse coarse DOFs go to a multigrid thread (V(2,2), damped Jacobi
+ * 0.6, hierarchy built once per request).  Three contexts on `device`.  A
+ * request's solutions are bitwise the same alone and in any mix. */
+typedef struct gsx_server gsx_server;
+typedef struct {
+    int32_t device;
+    int32_t precision;        /* GSX_FP64 or GSX_MIXED                          */
+    int64_t budget_bytes;     /* estimated device bytes of admitted requests     */
+    int32_t iters_per_step;   /* batched CG iterations per step                  */
+    int32_t capacity;         /* batch members (right-hand sides) at one time    */
+    int64_t mg_min_free;      /* multigrid lane from this many free DOFs; 0: off */
+    int64_t mg_max_coarse;    /* coarsest-level free DOFs at most (0: 4096)      */
+    int64_t elem_budget;      /* multigrid lane element-matrix bytes (0: 16 GiB) */
+} gsx_server_opts;
+/* One patch with homogeneous Dirichlet conditions on every side; nrhs
+ * right-hand sides of nfree values (rhs_host NULL or rhs_host[0] NULL: the
+ * assembled load vector for the first; the others must be given), copied at
+ * submission; x_host (or an entry) NULL: the solution is not returned,
+ * otherwise x_host[k] receives nfree values and must stay valid until the
+ * request's result has been polled. */
+typedef struct {
+    const gsx_patch_desc *patch;
+    int32_t nrhs;
+    const double *const *rhs_host;
+    double *const *x_host;
+    double tol;
+    int32_t maxit;
+    /* optional (NULL), nrhs entries each, written when the request finishes:
+     * iterations, relative residual and status (1, 2) per right-hand side */
+    int32_t *iters_host;
+    double *relres_host;
+    int32_t *status_host;
+} gsx_server_request;
+enum { GSX_SERVER_FAILED = 3 };
+typedef struct {
+    int64_t rid;
+    int32_t status;           /* 1 converged, 2 a right-hand side reached maxit,
+                                 GSX_SERVER_FAILED (error says why)              */
+    int32_t lane;             /* 0 batched CG, 1 multigrid                        */
+    int32_t nrhs;
+    int32_t iters_max;
+    int64_t nbasis, nfree;
+    double relres_max;
+    double t_submit, t_admit, t_assembled, t_done;   /* server clock, seconds    */
+    char error[160];
+} gsx_server_result;
+gsx_status gsx_server_create(const gsx_server_opts *opts, gsx_server **out);
+/* Tile shape (gsx_batch_set_tile) of members of degree p in d dimensions
+ * with rows in [rows_lo, rows_hi); first matching rule wins; default 1, 1. */
+gsx_status gsx_server_set_tile(gsx_server *server, int32_t d, int32_t p,
+                               int64_t rows_lo, int64_t rows_hi, int32_t lanes,
+                               int32_t half);
+gsx_status gsx_server_submit(gsx_server *server, const gsx_server_request *req,
+                             int64_t *rid);
+/* Up to max finished requests since the last poll (n of them in out). */
+gsx_status gsx_server_poll(gsx_server *server, int32_t max, gsx_server_result *out,
+                           int32_t *n);
+gsx_status gsx_server_status(gsx_server *server, int64_t *outstanding,
+                             int64_t *used_bytes, double *clock);
+/* Message of a failure of the server itself (GSX_ERR_HIP from poll/status). */
+const char *gsx_server_error(const gsx_server *server);
+/* Wait until every submitted request has finished (GSX_ERR_NOT_CONVERGED on
+ * timeout). */
+gsx_status gsx_server_drain(gsx_server *server, double timeout_s);
+/* Stops the threads; requests not finished are dropped and released. */
+gsx_status gsx_server_destroy(gsx_server *server);
+
 /* ---- reporting ---------------------------------------------------------- */
 /* L2 error of the discrete solution (free coefficients, eliminated ones at
  * the problem's fixed values) against u = prod_r sin(pi x_r) with the
