@@ -134,6 +134,12 @@ def main():
                     help="batch: number of problems (512); serve: number of timed requests (48)")
     ap.add_argument("--cpu-sample", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-mg-min", type=int, default=20000,
+                    help="serve: multigrid lane from this many free DOFs (0: no multigrid lane)")
+    ap.add_argument("--serve-warm", type=int, default=None, help="serve: warm-up requests (count // 3)")
+    ap.add_argument("--serve-group", type=int, default=1,
+                    help="serve: right-hand sides of a request sharing SpMV launches (gsx_batch_set_group)")
+    ap.add_argument("--serve-mixed-only", action="store_true", help="serve: skip the FP64 run")
     ap.add_argument("--batch-grouped", action="store_true",
                     help="batch: assemble disjoint patches in a few multipatch groups (CSR blocks)")
     ap.add_argument("--batch-csr", action="store_true",
@@ -754,7 +760,7 @@ def bench_serve(args):
     from gismo_amd import gsx, serving
     tol = 1e-10 if args.tol is None else args.tol
     args.count = 48 if args.count is None else args.count
-    nwarm = max(16, args.count // 3)
+    nwarm = max(16, args.count // 3) if args.serve_warm is None else args.serve_warm
     reqs = serving.make_requests(nwarm + args.count, seed=20261202)
     for r in reqs:
         r.tol = tol
@@ -766,7 +772,8 @@ def bench_serve(args):
         r.keep = True
 
     def run(prec, rate, rs):
-        srv = serving.Server(precision=prec, iters_per_step=8)
+        srv = serving.Server(precision=prec, iters_per_step=8, mg_min_dofs=args.serve_mg_min,
+                             group_max=args.serve_group)
         t0 = srv.now()
         if rate is None:
             for r in rs:
@@ -795,7 +802,8 @@ def bench_serve(args):
         return res
 
     out_runs = {}
-    for name, prec in (("mixed", gsx.MIXED), ("fp64", gsx.FP64)):
+    precs = (("mixed", gsx.MIXED),) if args.serve_mixed_only else (("mixed", gsx.MIXED), ("fp64", gsx.FP64))
+    for name, prec in precs:
         res = run(prec, None, warm)
         span = max(r.done for r in res) - min(r.arrival for r in res)
         work = sum(r.dofs * r.nrhs for r in res)
@@ -807,7 +815,14 @@ def bench_serve(args):
         span = max(r.done for r in res) - min(r.arrival for r in res)
         work = sum(r.dofs * r.nrhs for r in res)
         ok = all(all(s == 1 for s in r.status) for r in res)
-        out_runs[name] = dict(requests=len(res), rate_rps=rate, capacity_rps=cap_rps,
+        lanes = {}
+        for r in res:
+            ln = lanes.setdefault(r.lane, dict(requests=0, dof_solves=0, service_s=0.0, max_latency_ms=0.0))
+            ln["requests"] += 1
+            ln["dof_solves"] += r.dofs * r.nrhs
+            ln["service_s"] = round(ln["service_s"] + (r.done - r.admitted), 3)
+            ln["max_latency_ms"] = round(max(ln["max_latency_ms"], 1e3 * r.latency), 1)
+        out_runs[name] = dict(requests=len(res), rate_rps=rate, capacity_rps=cap_rps, lanes=lanes,
                               p50_ms=1e3 * float(np.percentile(lat, 50)),
                               p99_ms=1e3 * float(np.percentile(lat, 99)),
                               mean_ms=1e3 * float(lat.mean()), dofs_per_s=work / span,
@@ -839,8 +854,9 @@ def bench_serve(args):
     mixed = out_runs["mixed"]
     extra = {"config": {"requests": args.count, "warmup_requests": nwarm, "load": args.load,
                         "sizes": "log-uniform 1e3..2e6 DOFs, d in {2,3}, p in {2,3,4}, 1..8 rhs",
-                        "tol": tol, "iterations_per_step": 8},
-             "mixed": mixed, "fp64": out_runs["fp64"],
+                        "tol": tol, "iterations_per_step": 8, "mg_lane_min_free_dofs": args.serve_mg_min,
+                        "spmv_group_max": args.serve_group},
+             "mixed": mixed, "fp64": out_runs.get("fp64"),
              "p50_ms": mixed["p50_ms"], "p99_ms": mixed["p99_ms"],
              "sampled_max_rel_err_vs_fp64": max(errs) if errs else None,
              "sampled_solutions": len(errs)}

@@ -159,11 +159,13 @@ __device__ inline float row_sum(float v, int tpr)
     return v;
 }
 
-// All slots of the row (s is the FP64 stencil or the full FP32 one), line by
-// line in lexicographic (= column) order.
-template <int D, int P, typename T>
-__device__ inline T box_full_rows(const BSeg &g, const T *__restrict__ sv, int r, int sub, int tpr,
-                                  const T *__restrict__ x)
+// All slots of the row (sv: the FP64 stencil or the full FP32 one), line by
+// line in lexicographic (= column) order, applied to nv vectors at once
+// (acc[j] for xs[j]): each coefficient is read once for all of them, and
+// each vector's sum runs in the same order whatever nv is.
+template <int D, int P, typename T, int KMAX>
+__device__ inline void box_full_rows(const BSeg &g, const T *__restrict__ sv, int r, int sub, int tpr,
+                                     const T *const *xs, int nv, T *acc)
 {
     using G = BoxGeo<D, P>;
     constexpr int NLINES = (D == 3) ? G::W * G::W : G::W;
@@ -171,7 +173,6 @@ __device__ inline T box_full_rows(const BSeg &g, const T *__restrict__ sv, int r
     const int l = r - g.row0;
     const int ia = l % m0, ib = (l / m0) % m1, ic = (D == 3) ? l / (m0 * m1) : 0;
     const T *vb = sv + stencil_index(0, l, G::NS);
-    T acc = 0;
 #pragma unroll 1
     for (int li = sub; li < NLINES; li += tpr) {
         const int dz = (D == 3) ? li / G::W - P : 0, dy = li % G::W - P;
@@ -180,16 +181,21 @@ __device__ inline T box_full_rows(const BSeg &g, const T *__restrict__ sv, int r
 #pragma unroll
         for (int dx = -P; dx <= P; ++dx) {
             const bool ok = okyz && ia + dx >= 0 && ia + dx < m0;
-            acc += vb[(li * G::W + dx + P) * kStencilTile] * x[ok ? rb + dx : r];
+            const T c = vb[(li * G::W + dx + P) * kStencilTile];
+            const int col = ok ? rb + dx : r;
+#pragma unroll
+            for (int j = 0; j < KMAX; ++j)
+                if (j < nv) acc[j] += c * xs[j][col];
         }
     }
-    return acc;
 }
 
 // FP32 upper half: the centre line (lane 0 of the row) and each upper line
-// with its mirror line (the order of the single-patch kernel k_sspmv_half)
-template <int D, int P>
-__device__ inline float box_half_rows(const BSeg &g, int r, int sub, int tpr, const float *__restrict__ x)
+// with its mirror line (the order of the single-patch kernel k_sspmv_half),
+// applied to nv vectors at once
+template <int D, int P, int KMAX>
+__device__ inline void box_half_rows(const BSeg &g, int r, int sub, int tpr, const float *const *xs, int nv,
+                                     float *acc)
 {
     using G = BoxGeo<D, P>;
     constexpr int W = G::W, C = G::C, NH = G::NH;
@@ -199,13 +205,17 @@ __device__ inline float box_half_rows(const BSeg &g, int r, int sub, int tpr, co
     const int ia = l % m0, ib = (l / m0) % m1, ic = (D == 3) ? l / (m0 * m1) : 0;
     const float *sv = g.s32;
     const float *vb = sv + stencil_index(0, l, NH);
-    float acc = 0.0f;
+    auto add = [&](float c, int col) {
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j)
+            if (j < nv) acc[j] += c * xs[j][col];
+    };
     if (sub == 0) {
 #pragma unroll
-        for (int dx = 0; dx <= P; ++dx) acc += vb[dx * kStencilTile] * x[ia + dx < m0 ? r + dx : r];
+        for (int dx = 0; dx <= P; ++dx) add(vb[dx * kStencilTile], ia + dx < m0 ? r + dx : r);
 #pragma unroll
         for (int dx = -P; dx < 0; ++dx)
-            acc += (ia + dx >= 0 ? sv[stencil_index(-dx, l + dx, NH)] * x[r + dx] : 0.0f);
+            if (ia + dx >= 0) add(sv[stencil_index(-dx, l + dx, NH)], r + dx);
     }
 #pragma unroll 1
     for (int u = sub; u < NL; u += tpr) {
@@ -217,72 +227,80 @@ __device__ inline float box_half_rows(const BSeg &g, int r, int sub, int tpr, co
 #pragma unroll
         for (int dx = -P; dx <= P; ++dx) {
             const bool ok = upyz && ia + dx >= 0 && ia + dx < m0;
-            acc += vb[(h0 + dx + P) * kStencilTile] * x[ok ? r + off + dx : r];
+            add(vb[(h0 + dx + P) * kStencilTile], ok ? r + off + dx : r);
         }
         const bool okyz = ib - dy >= 0 && ib - dy < m1 && ic - dz >= 0;
 #pragma unroll
         for (int dx = -P; dx <= P; ++dx) {
             const bool ok = okyz && ia + dx >= 0 && ia + dx < m0;
-            acc += (ok ? sv[stencil_index(h0 - dx + P, l - off + dx, NH)] * x[r - off + dx] : 0.0f);
+            if (ok) add(sv[stencil_index(h0 - dx + P, l - off + dx, NH)], r - off + dx);
         }
     }
-    return acc;
 }
 
-// FP64: y = A x, partial (x, y); or (RESID) y = b - A x, partial (y, y)
-template <int D, int P, bool RESID>
-__device__ inline void box_apply64(const BSeg &g, int lo, int hi, const double *__restrict__ x,
-                                   double *__restrict__ y, double &dot)
+// FP64 residual r = b - A x of one box member, partial (r, r)
+template <int D, int P>
+__device__ inline void box_resid64(const BSeg &g, int lo, int hi, double &dot)
 {
     const int tpr = g.tpr, r = lo + (int)threadIdx.x / tpr, sub = (int)threadIdx.x % tpr;
     if (r >= hi) return;   // the row's whole group leaves together
-    const double acc = row_sum(box_full_rows<D, P, double>(g, g.s64, r, sub, tpr, x), tpr);
+    const double *xs[1] = {g.x};
+    double acc[1] = {0.0};
+    box_full_rows<D, P, double, 1>(g, g.s64, r, sub, tpr, xs, 1, acc);
+    const double a = row_sum(acc[0], tpr);
     if (sub != 0) return;
-    if (RESID) {
-        const double v = g.b[r] - acc;
-        y[r] = v;
-        dot += v * v;
-    } else {
-        y[r] = acc;
-        dot += x[r] * acc;
-    }
+    const double v = g.b[r] - a;
+    g.r[r] = v;
+    dot += v * v;
 }
 
-// FP32 on the half (default) or full stencil copy
-template <int D, int P>
-__device__ inline void box_apply32(const BSeg &g, int lo, int hi, const float *__restrict__ x,
-                                   float *__restrict__ y, double &dot)
+// y_j = A x_j for the nv members of a group (FP64: all slots; FP32: the half
+// or full copy), partials (x_j, y_j)
+template <int D, int P, typename T, int KMAX>
+__device__ inline void box_apply_group(const BSeg &g, int lo, int hi, const T *const *xs, T *const *ys, int nv,
+                                       double *dot)
 {
     const int tpr = g.tpr, r = lo + (int)threadIdx.x / tpr, sub = (int)threadIdx.x % tpr;
     if (r >= hi) return;
-    const float part = g.half ? box_half_rows<D, P>(g, r, sub, tpr, x)
-                              : box_full_rows<D, P, float>(g, g.s32, r, sub, tpr, x);
-    const float acc = row_sum(part, tpr);
+    T acc[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) acc[j] = 0;
+    if constexpr (sizeof(T) == 8) box_full_rows<D, P, double, KMAX>(g, g.s64, r, sub, tpr, xs, nv, acc);
+    else if (g.half) box_half_rows<D, P, KMAX>(g, r, sub, tpr, xs, nv, acc);
+    else box_full_rows<D, P, float, KMAX>(g, g.s32, r, sub, tpr, xs, nv, acc);
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j)
+        if (j < nv) acc[j] = row_sum(acc[j], tpr);
     if (sub != 0) return;
-    y[r] = acc;
-    dot += (double)x[r] * (double)acc;
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j)
+        if (j < nv) {
+            ys[j][r] = acc[j];
+            dot[j] += (double)xs[j][r] * (double)acc[j];
+        }
 }
 
 #define GSX_BOX_CASES(MACRO) \
     MACRO(2, 1) MACRO(2, 2) MACRO(2, 3) MACRO(2, 4) MACRO(3, 1) MACRO(3, 2) MACRO(3, 3) MACRO(3, 4)
 
-template <bool RESID>
-__device__ inline void box_apply64_any(const BSeg &g, int lo, int hi, const double *x, double *y, double &dot)
+__device__ inline void box_resid64_any(const BSeg &g, int lo, int hi, double &dot)
 {
     switch (g.bd * 8 + g.bp) {
 #define GSX_C(D, P) \
-    case D * 8 + P: box_apply64<D, P, RESID>(g, lo, hi, x, y, dot); break;
+    case D * 8 + P: box_resid64<D, P>(g, lo, hi, dot); break;
         GSX_BOX_CASES(GSX_C)
 #undef GSX_C
     default: break;
     }
 }
 
-__device__ inline void box_apply32_any(const BSeg &g, int lo, int hi, const float *x, float *y, double &dot)
+template <typename T, int KMAX>
+__device__ inline void box_apply_group_any(const BSeg &g, int lo, int hi, const T *const *xs, T *const *ys,
+                                           int nv, double *dot)
 {
     switch (g.bd * 8 + g.bp) {
 #define GSX_C(D, P) \
-    case D * 8 + P: box_apply32<D, P>(g, lo, hi, x, y, dot); break;
+    case D * 8 + P: box_apply_group<D, P, T, KMAX>(g, lo, hi, xs, ys, nv, dot); break;
         GSX_BOX_CASES(GSX_C)
 #undef GSX_C
     default: break;
@@ -292,7 +310,7 @@ __device__ inline void box_apply32_any(const BSeg &g, int lo, int hi, const floa
 // r = b - A x (FP64) of a chunk of either kind
 __device__ inline void chunk_residual_member(const BSeg &g, int lo, int hi, double &rr)
 {
-    if (g.bd) box_apply64_any<true>(g, lo, hi, g.x, g.r, rr);
+    if (g.bd) box_resid64_any(g, lo, hi, rr);
     else chunk_residual_any(g, lo, hi, rr);
 }
 
@@ -402,31 +420,63 @@ __global__ __launch_bounds__(256) void k_b_init3(const BChunk *__restrict__ chun
 // ---------------------------------------------------------------------------
 // One CG iteration (three launches).
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void k_b_spmv(const BChunk *__restrict__ chunks,
+// KMAX = 1: groups of one member (the SpMV of most batches); KMAX =
+// kGroupMax: groups of several members, which hold KMAX accumulators per
+// row (a separate launch, so that the single-member launch keeps its
+// register budget and occupancy).
+template <typename T, int KMAX>
+__global__ __launch_bounds__(256) void k_b_spmv(const BSChunk *__restrict__ schunks,
+                                                const BGroup *__restrict__ groups,
                                                 const BSeg *__restrict__ segs, BState *__restrict__ st,
                                                 double *__restrict__ p0)
 {
     __shared__ double s[256];
-    const int cb = xcd_block(blockIdx.x, gridDim.x);   // logical chunk
-    const BChunk ch = chunks[cb];
-    BState &S = st[ch.seg];
-    if (S.phase != 1 || S.done || S.stop) return;
-    const BSeg g = segs[ch.seg];
-    // the iteration number advances even for a paused member, so that the
-    // update and direction launches of this iteration see it paused too
-    if (cb == g.c0 && threadIdx.x == 0) S.it = S.it_next;
-    if (S.flag_next && S.flag_next <= S.it_next) return;
-    double dot = 0.0;
-    if constexpr (sizeof(T) == 4) {
-        if (g.bd) box_apply32_any(g, ch.lo, ch.hi, g.p32, g.t32, dot);
-        else chunk_spmv_any<float, float>(g, ch.lo, ch.hi, g.v32, g.p32, g.t32, dot);
-    } else {
-        if (g.bd) box_apply64_any<false>(g, ch.lo, ch.hi, g.p, g.t, dot);
-        else chunk_spmv_any<double, double>(g, ch.lo, ch.hi, g.v64, g.p, g.t, dot);
+    const int cb = xcd_block(blockIdx.x, gridDim.x);   // logical group chunk
+    const BSChunk ch = schunks[cb];
+    const BGroup G = groups[ch.grp];
+    // the group's members running this iteration (the same set in every
+    // thread: the state fields read here are not written by this launch)
+    int act[KMAX];
+    int na = 0;
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) {
+        if (j >= G.k) break;
+        BState &S = st[G.seg[j]];
+        if (S.phase != 1 || S.done || S.stop) continue;
+        // the iteration number advances even for a paused member, so that
+        // the update and direction launches of this iteration see it paused
+        if (ch.q == 0 && threadIdx.x == 0) S.it = S.it_next;
+        if (S.flag_next && S.flag_next <= S.it_next) continue;
+        act[na++] = j;
     }
-    const double a = bsum256(dot, s);
-    if (threadIdx.x == 0) p0[cb] = a;
+    if (na == 0) return;
+    const BSeg g = segs[G.seg[act[0]]];
+    double dot[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; ++j) dot[j] = 0.0;
+    if (!g.bd) {   // a CSR member (groups of one)
+        if constexpr (sizeof(T) == 4) chunk_spmv_any<float, float>(g, ch.lo, ch.hi, g.v32, g.p32, g.t32, dot[0]);
+        else chunk_spmv_any<double, double>(g, ch.lo, ch.hi, g.v64, g.p, g.t, dot[0]);
+    } else {
+        const T *xs[KMAX];
+        T *ys[KMAX];
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j) {
+            const int sid = G.seg[act[j < na ? j : 0]];
+            if constexpr (sizeof(T) == 4) {
+                xs[j] = segs[sid].p32;
+                ys[j] = segs[sid].t32;
+            } else {
+                xs[j] = segs[sid].p;
+                ys[j] = segs[sid].t;
+            }
+        }
+        box_apply_group_any<T, KMAX>(g, ch.lo, ch.hi, xs, ys, na, dot);
+    }
+    for (int j = 0; j < na; ++j) {
+        const double a = bsum256(dot[j], s);
+        if (threadIdx.x == 0) p0[segs[G.seg[act[j]]].c0 + ch.q] = a;
+    }
 }
 
 template <typename T>
@@ -731,17 +781,27 @@ hipError_t launch_batch_init(int nchunks, const BChunk *chunks, const BSeg *segs
     return hipGetLastError();
 }
 
-hipError_t launch_batch_iteration(int nchunks, const BChunk *chunks, const BSeg *segs, BState *st, double *p0,
-                                  double *p1, double *p2, int mixed, double delta2, hipStream_t s)
+hipError_t launch_batch_iteration(int nchunks, const BChunk *chunks, int nsingle, int nmulti,
+                                  const BSChunk *schunks, const BGroup *groups, const BSeg *segs, BState *st,
+                                  double *p0, double *p1, double *p2, int mixed, double delta2, hipStream_t s)
 {
     if (nchunks <= 0) return hipSuccess;
+    // schunks: the single-member groups' chunks, then the others'
     if (mixed) {
-        hipLaunchKernelGGL(k_b_spmv<float>, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p0);
+        if (nsingle)
+            hipLaunchKernelGGL((k_b_spmv<float, 1>), dim3(nsingle), dim3(256), 0, s, schunks, groups, segs, st, p0);
+        if (nmulti)
+            hipLaunchKernelGGL((k_b_spmv<float, kGroupMax>), dim3(nmulti), dim3(256), 0, s, schunks + nsingle,
+                               groups, segs, st, p0);
         hipLaunchKernelGGL(k_b_update<float>, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p0, p1, p2);
         hipLaunchKernelGGL(k_b_direction<float>, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p1, p2,
                            delta2);
     } else {
-        hipLaunchKernelGGL(k_b_spmv<double>, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p0);
+        if (nsingle)
+            hipLaunchKernelGGL((k_b_spmv<double, 1>), dim3(nsingle), dim3(256), 0, s, schunks, groups, segs, st, p0);
+        if (nmulti)
+            hipLaunchKernelGGL((k_b_spmv<double, kGroupMax>), dim3(nmulti), dim3(256), 0, s, schunks + nsingle,
+                               groups, segs, st, p0);
         hipLaunchKernelGGL(k_b_update<double>, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p0, p1, p2);
         hipLaunchKernelGGL(k_b_direction<double>, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p1, p2,
                            delta2);

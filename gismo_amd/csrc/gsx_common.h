@@ -160,6 +160,21 @@ struct BSeg {
     const float *s32;
 };
 
+// Members that iterate on one structured operator with one tile shape (the
+// right-hand sides of one problem) share their SpMV launches: a group chunk
+// covers rows [lo, hi) of every member of the group, which is chunk q of
+// each of them.  Other members form groups of one.
+constexpr int kGroupMax = 8;
+struct BGroup {
+    int k;
+    int seg[kGroupMax];
+};
+struct BSChunk {
+    int grp;
+    int lo, hi;
+    int q;
+};
+
 struct BChunk {
     int seg;
     int lo, hi;          // rows [lo, hi)

@@ -268,8 +268,9 @@ hipError_t launch_batch_box_build(int d, int p, int row0, int n, int m0, int m1,
 hipError_t launch_batch_box_s32(int d, int p, int n, bool half, const double *s64, float *s32, hipStream_t s);
 hipError_t launch_batch_init(int nchunks, const BChunk *chunks, const BSeg *segs, BState *st, double *p0,
                              double *p1, double *p2, int mixed, hipStream_t s);
-hipError_t launch_batch_iteration(int nchunks, const BChunk *chunks, const BSeg *segs, BState *st, double *p0,
-                                  double *p1, double *p2, int mixed, double delta2, hipStream_t s);
+hipError_t launch_batch_iteration(int nchunks, const BChunk *chunks, int nsingle, int nmulti,
+                                  const BSChunk *schunks, const BGroup *groups, const BSeg *segs, BState *st,
+                                  double *p0, double *p1, double *p2, int mixed, double delta2, hipStream_t s);
 hipError_t launch_batch_reliable(int nchunks, const BChunk *chunks, const BSeg *segs, BState *st, double *p0,
                                  double *p2, hipStream_t s);
 

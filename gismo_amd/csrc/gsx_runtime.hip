@@ -930,6 +930,7 @@ struct StencilOp {
     std::vector<int> f2g_host;
     DevPtr f2g, s64, s32;
     bool s32_ready = false;
+    DevPtr s32full;                   // all slots in FP32 (batch members that ask for it)
     // grid-layout workspace
     DevPtr diag64, diag32;
     DevPtr xg, rg, bg, zg, pg, tg;            // FP64
@@ -3691,10 +3692,13 @@ struct gsx_batch {
     bool mixed = false;
     int cap = 0;
     std::vector<BMember> m;
-    DevPtr d_segs, d_state, d_chunks, d_parts, d_bad;
-    int chunk_cap = 0, nchunks = 0;
+    DevPtr d_segs, d_state, d_chunks, d_parts, d_bad, d_groups, d_schunks;
+    int chunk_cap = 0, nchunks = 0, nschunks = 0, nsingle = 0;
+    int group_max = 1;                   // members per SpMV group (1: no groups)
     bool dirty = true;
     std::vector<gsx::BChunk> chunks_h;
+    std::vector<gsx::BGroup> groups_h;   // members sharing an operator and tile shape
+    std::vector<gsx::BSChunk> schunks_h; // the SpMV launch's chunks (one per group chunk)
     std::vector<gsx::BState> state_h;
     // one step (init, iters iterations, reliable update) captured as a HIP
     // graph for the current member layout; replayed while it holds
@@ -3724,9 +3728,10 @@ gsx_status gsx_batch_create(gsx_ctx *c, int32_t precision, int32_t capacity, gsx
     B->m.resize(capacity);
     B->state_h.resize(capacity);
     B->d_segs = dalloc(c, sizeof(gsx::BSeg) * capacity);
+    B->d_groups = dalloc(c, sizeof(gsx::BGroup) * capacity);
     B->d_state = dalloc(c, sizeof(gsx::BState) * capacity);
     B->d_bad = dalloc(c, sizeof(int));
-    if (!B->d_segs || !B->d_state || !B->d_bad) {
+    if (!B->d_segs || !B->d_state || !B->d_bad || !B->d_groups) {
         delete B;
         return fail(c, GSX_ERR_OOM, "device allocation (batch)");
     }
@@ -3856,13 +3861,16 @@ static gsx_status batch_add(gsx_batch *B, gsx_matrix *A, int32_t row0, int32_t n
         g.s64 = dp<double>(S.s64);
         g.s32 = B->mixed ? dp<float>(S.s32) : nullptr;
         if (B->mixed && box[6] == 0) {
-            // the full FP32 form asked for: a copy from the FP64 slots
-            const int64_t ns = gsx::batch_box_slots(d, p, false);
-            M.s32 = dalloc(c, sizeof(float) * gsx::stencil_size(nrows, (int)ns));
-            if (!M.s32) return fail(c, GSX_ERR_OOM, "device allocation (box member operator)");
-            HIP_OK(c, gsx::launch_batch_box_s32(d, p, nrows, false, dp<double>(S.s64),
-                                                dp<float>(M.s32), s));
-            g.s32 = dp<float>(M.s32);
+            // the full FP32 form asked for: one copy from the FP64 slots,
+            // kept with the matrix for every member on it
+            StencilOp &So = A->st;
+            if (!So.s32full) {
+                const int64_t ns = gsx::batch_box_slots(d, p, false);
+                So.s32full = dalloc(c, sizeof(float) * gsx::stencil_size(nrows, (int)ns));
+                if (!So.s32full) return fail(c, GSX_ERR_OOM, "device allocation (box member operator)");
+                HIP_OK(c, gsx::launch_batch_box_s32(d, p, nrows, false, dp<double>(S.s64), dp<float>(So.s32full), s));
+            }
+            g.s32 = dp<float>(So.s32full);
         }
         g.bd = d;
         g.bp = p;
@@ -3972,13 +3980,62 @@ static gsx_status batch_rebuild(gsx_batch *B)
         HIP_OK(c, hipMemcpyAsync(dp<gsx::BSeg>(B->d_segs) + k, &M.seg, sizeof(gsx::BSeg), hipMemcpyHostToDevice, s));
     }
     B->nchunks = (int)B->chunks_h.size();
+    // SpMV groups: box members on one operator with one tile shape (up to
+    // kGroupMax), every other member alone
+    B->groups_h.clear();
+    B->schunks_h.clear();
+    B->nsingle = 0;
+    std::vector<int> gfirst;   // first member of each group
+    for (int k = 0; k < B->cap; ++k) {
+        const BMember &M = B->m[k];
+        if (!M.used || M.status) continue;
+        const gsx::BSeg &g = M.seg;
+        int gid = -1;
+        if (g.bd)
+            for (size_t q = 0; q < B->groups_h.size(); ++q) {
+                const gsx::BSeg &h = B->m[gfirst[q]].seg;
+                if (B->groups_h[q].k < B->group_max && h.bd && h.s64 == g.s64 && h.s32 == g.s32 &&
+                    h.tpr == g.tpr && h.half == g.half && h.row0 == g.row0 && h.nc == g.nc) {
+                    gid = (int)q;
+                    break;
+                }
+            }
+        if (gid < 0) {
+            gid = (int)B->groups_h.size();
+            B->groups_h.push_back(gsx::BGroup{});
+            gfirst.push_back(k);
+        }
+        gsx::BGroup &G = B->groups_h[gid];
+        G.seg[G.k++] = k;
+    }
+    // chunks of the single-member groups first, then those of the others
+    // (two launches: the single-member kernel keeps its register budget)
+    for (int pass = 0; pass < 2; ++pass)
+        for (size_t q = 0; q < B->groups_h.size(); ++q) {
+            if ((B->groups_h[q].k > 1) != (pass == 1)) continue;
+            const gsx::BSeg &h = B->m[gfirst[q]].seg;
+            for (int i = 0; i < h.nc; ++i) {
+                const gsx::BChunk &ch = B->chunks_h[h.c0 + i];
+                B->schunks_h.push_back({(int)q, ch.lo, ch.hi, i});
+            }
+            if (pass == 0) B->nsingle = (int)B->schunks_h.size();
+        }
+    B->nschunks = (int)B->schunks_h.size();
     if (B->nchunks > B->chunk_cap) {
         HIP_OK(c, hipStreamSynchronize(s));
         B->chunk_cap = std::max(B->nchunks, 2 * B->chunk_cap);
         B->d_chunks = dalloc(c, sizeof(gsx::BChunk) * B->chunk_cap);
+        B->d_schunks = dalloc(c, sizeof(gsx::BSChunk) * B->chunk_cap);
         B->d_parts = dalloc(c, sizeof(double) * 3 * B->chunk_cap);
-        if (!B->d_chunks || !B->d_parts) return fail(c, GSX_ERR_OOM, "device allocation (batch chunks)");
+        if (!B->d_chunks || !B->d_schunks || !B->d_parts)
+            return fail(c, GSX_ERR_OOM, "device allocation (batch chunks)");
     }
+    if (!B->groups_h.empty())
+        HIP_OK(c, hipMemcpyAsync(B->d_groups.get(), B->groups_h.data(), sizeof(gsx::BGroup) * B->groups_h.size(),
+                                 hipMemcpyHostToDevice, s));
+    if (B->nschunks)
+        HIP_OK(c, hipMemcpyAsync(B->d_schunks.get(), B->schunks_h.data(), sizeof(gsx::BSChunk) * B->nschunks,
+                                 hipMemcpyHostToDevice, s));
     if (B->nchunks)
         HIP_OK(c, hipMemcpyAsync(B->d_chunks.get(), B->chunks_h.data(), sizeof(gsx::BChunk) * B->nchunks,
                                  hipMemcpyHostToDevice, s));
@@ -4006,7 +4063,9 @@ gsx_status gsx_batch_step(gsx_batch *B, int32_t iters, int32_t *running)
         auto enqueue = [&]() -> hipError_t {
             hipError_t e = gsx::launch_batch_init(nc, ch, sg, st, P0, P1, P2, B->mixed, s);
             for (int i = 0; e == hipSuccess && i < iters; ++i)
-                e = gsx::launch_batch_iteration(nc, ch, sg, st, P0, P1, P2, B->mixed, 1e-4, s);
+                e = gsx::launch_batch_iteration(nc, ch, B->nsingle, B->nschunks - B->nsingle,
+                                                dp<gsx::BSChunk>(B->d_schunks), dp<gsx::BGroup>(B->d_groups), sg,
+                                                st, P0, P1, P2, B->mixed, 1e-4, s);
             if (e == hipSuccess && B->mixed) e = gsx::launch_batch_reliable(nc, ch, sg, st, P0, P2, s);
             return e;
         };
@@ -4051,6 +4110,14 @@ gsx_status gsx_batch_step(gsx_batch *B, int32_t iters, int32_t *running)
         }
     }
     if (running) *running = run;
+    return GSX_OK;
+}
+
+gsx_status gsx_batch_set_group(gsx_batch *B, int32_t group_max)
+{
+    if (!B || group_max < 1 || group_max > gsx::kGroupMax) return GSX_ERR_ARG;
+    B->group_max = group_max;
+    B->dirty = true;
     return GSX_OK;
 }
 

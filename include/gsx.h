@@ -487,6 +487,14 @@ gsx_status gsx_batch_set_tile(gsx_batch *batch, int32_t id, int32_t lanes, int32
  * member is added, finishes or is removed; with timing enabled on the
  * context (gsx_ctx_enable_timing) the launches go out one by one instead. */
 gsx_status gsx_batch_step(gsx_batch *batch, int32_t iters, int32_t *running);
+/* Members on one structured operator with one tile shape (box members on
+ * the same single-patch matrix: several right-hand sides) share their SpMV
+ * launch in groups of up to group_max (1..8; default 1, no groups): each
+ * coefficient is read once for the group, each member's x separately.  A
+ * member's iterates do not depend on its group.  Groups pay while the
+ * group's x vectors around the rows in flight fit the L2 (small and medium
+ * operators); on large ones separate launches are faster. */
+gsx_status gsx_batch_set_group(gsx_batch *batch, int32_t group_max);
 /* Graph capture of the steps on (default) or off; graph launches so far. */
 gsx_status gsx_batch_set_graph(gsx_batch *batch, int32_t enable);
 gsx_status gsx_batch_graph_launches(const gsx_batch *batch, int64_t *count);
@@ -522,6 +530,7 @@ typedef struct {
     int64_t mg_min_free;      /* multigrid lane from this many free DOFs; 0: off */
     int64_t mg_max_coarse;    /* coarsest-level free DOFs at most (0: 4096)      */
     int64_t elem_budget;      /* multigrid lane element-matrix bytes (0: 16 GiB) */
+    int32_t group_max;        /* gsx_batch_set_group of the batch lane (0: 1)    */
 } gsx_server_opts;
 /* One patch with homogeneous Dirichlet conditions on every side; nrhs
  * right-hand sides of nfree values (rhs_host NULL or rhs_host[0] NULL: the

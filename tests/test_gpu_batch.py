@@ -416,3 +416,48 @@ def test_box_member_on_its_own_patch_operator_equals_block_copy(ctx, precision):
     gj.close()
     for m in members:
         m.close()
+
+
+@pytest.mark.parametrize("precision,tile", [(gsx.FP64, (1, 1)), (gsx.MIXED, (1, 1)), (gsx.MIXED, (2, 0))])
+def test_members_sharing_an_operator_share_the_spmv(ctx, precision, tile):
+    """Several right-hand sides on one box operator form one SpMV group (each
+    coefficient read once for all of them); every member's iterates are
+    bitwise those of the same member solved alone (a group of one), also
+    when the group shrinks as members finish, and reach the oracle's
+    solution."""
+    prob = [p for p in _small_batch(count=12, seed=5) if p.d == 3][1]
+    m = _Member(ctx, prob, precision)
+    n = m.gp.nfree
+    rng = np.random.default_rng(4)
+    bs = [m.b] + [ctx.to_device(rng.uniform(-1, 1, n)) for _ in range(4)]
+    tol = 1e-10 if precision == gsx.FP64 else 1e-12
+    maxits = [5000, 5000, 40, 5000, 5000]   # one member stops early: the group shrinks
+    alone = []
+    for b, mi in zip(bs, maxits):
+        x = ctx.array(n).zero()
+        B = gsx.Batch(ctx, precision, capacity=2)
+        B.add(m.A, b, x, tol=tol, maxit=mi, box=_box(prob), tile=tile)
+        B.run(8)
+        alone.append((x.download(), B.result(0)))
+        B.close()
+        x.close()
+    xs = [ctx.array(n).zero() for _ in bs]
+    B = gsx.Batch(ctx, precision, capacity=8)
+    B.set_group(4)   # groups of 4 and 1
+    ids = [B.add(m.A, b, x, tol=tol, maxit=mi, box=_box(prob), tile=tile) for b, x, mi in zip(bs, xs, maxits)]
+    B.run(8)
+    A_o = None
+    for k, (i, x) in enumerate(zip(ids, xs)):
+        got = x.download()
+        assert np.array_equal(got, alone[k][0]), k
+        assert B.result(i) == alone[k][1], k
+    B.close()
+    dofs, nfree = O.mapper(prob.patches, prob.dir_sides, prob.ifaces)
+    A_o, _ = O.assemble(prob.patches, dofs, nfree, 0)
+    import scipy.sparse.linalg as spla
+    for k in (1, 3):
+        xo = spla.spsolve(A_o.tocsc(), bs[k].download())
+        assert np.linalg.norm(alone[k][0] - xo) <= 1e-8 * np.linalg.norm(xo)
+    for a in bs[1:] + xs:
+        a.close()
+    m.close()

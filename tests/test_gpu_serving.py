@@ -21,8 +21,9 @@ def _requests(count=8, seed=99):
     return rs
 
 
-def _serve(rs, budget=160 << 30, precision=gsx.MIXED, stagger=0.0, mg_min_dofs=20000):
-    srv = serving.Server(precision=precision, budget_bytes=budget, iters_per_step=8, mg_min_dofs=mg_min_dofs)
+def _serve(rs, budget=160 << 30, precision=gsx.MIXED, stagger=0.0, mg_min_dofs=20000, group_max=1):
+    srv = serving.Server(precision=precision, budget_bytes=budget, iters_per_step=8, mg_min_dofs=mg_min_dofs,
+                         group_max=group_max)
     for r in rs:
         r.arrival = srv.now()
         srv.submit(r)
@@ -134,3 +135,17 @@ def test_serving_survives_a_failing_request(mg_min_dofs):
         assert got[rid].error is None and all(s == 1 for s in got[rid].status)
         for a, b in zip(got[rid].solutions, ref[rid].solutions):
             assert np.array_equal(a, b), rid
+
+
+def test_serving_shared_spmv_groups_give_the_same_solutions():
+    """With the right-hand sides of a request sharing SpMV launches
+    (group_max 4) every solution is bitwise the one served without groups."""
+    rs = _requests(count=6, seed=5)
+    for r in rs:
+        r.rhs_seeds = (r.rhs_seeds + [r.rhs_seeds[0] * 7 + k for k in range(1, 4)])[:4]
+    plain = _serve(rs)
+    grouped = _serve(rs, group_max=4)
+    for r in rs:
+        assert len(grouped[r.rid].solutions) == 4
+        for a, b in zip(grouped[r.rid].solutions, plain[r.rid].solutions):
+            assert np.array_equal(a, b), r.rid
