@@ -137,6 +137,7 @@ def main():
     ap.add_argument("--serve-mg-min", type=int, default=20000,
                     help="serve: multigrid lane from this many free DOFs (0: no multigrid lane)")
     ap.add_argument("--serve-warm", type=int, default=None, help="serve: warm-up requests (count // 3)")
+    ap.add_argument("--serve-mg-workers", type=int, default=1, help="serve: multigrid-lane threads")
     ap.add_argument("--serve-group", type=int, default=1,
                     help="serve: right-hand sides of a request sharing SpMV launches (gsx_batch_set_group)")
     ap.add_argument("--serve-mixed-only", action="store_true", help="serve: skip the FP64 run")
@@ -773,7 +774,7 @@ def bench_serve(args):
 
     def run(prec, rate, rs):
         srv = serving.Server(precision=prec, iters_per_step=8, mg_min_dofs=args.serve_mg_min,
-                             group_max=args.serve_group)
+                             group_max=args.serve_group, mg_workers=args.serve_mg_workers)
         t0 = srv.now()
         if rate is None:
             for r in rs:
@@ -855,7 +856,8 @@ def bench_serve(args):
     extra = {"config": {"requests": args.count, "warmup_requests": nwarm, "load": args.load,
                         "sizes": "log-uniform 1e3..2e6 DOFs, d in {2,3}, p in {2,3,4}, 1..8 rhs",
                         "tol": tol, "iterations_per_step": 8, "mg_lane_min_free_dofs": args.serve_mg_min,
-                        "spmv_group_max": args.serve_group},
+                        "spmv_group_max": args.serve_group,
+                        "mg_workers": args.serve_mg_workers},
              "mixed": mixed, "fp64": out_runs.get("fp64"),
              "p50_ms": mixed["p50_ms"], "p99_ms": mixed["p99_ms"],
              "sampled_max_rel_err_vs_fp64": max(errs) if errs else None,

@@ -15,9 +15,10 @@
 //     gsx_server_set_tile), steps the batch, and completes a request when all
 //     its members have finished: solutions to the caller's host buffers,
 //     operator and workspace released, budget returned;
-//   multigrid thread (its own context): assembles with the element matrices
-//     kept, builds the hierarchy once and solves every right-hand side by
-//     MG-preconditioned CG with it resident.
+//   multigrid threads (mg_workers of them, each with its own context): each
+//     takes a request, assembles it with the element matrices kept, builds
+//     the hierarchy once and solves every right-hand side by MG-preconditioned
+//     CG with it resident.
 //
 // A member's iterates depend only on its own rows and tile shape, so a
 // request's solutions are bitwise the same alone and in any mix.
@@ -80,7 +81,8 @@ std::string status_text(gsx_ctx *c, gsx_status st, const char *what)
 
 struct gsx_server {
     gsx_server_opts o{};
-    gsx_ctx *ca = nullptr, *cb = nullptr, *cm = nullptr;
+    gsx_ctx *ca = nullptr, *cb = nullptr;
+    std::vector<gsx_ctx *> cms;      // one context per multigrid worker
     gsx_batch *batch = nullptr;
     std::vector<TileRule> tiles;
     std::chrono::steady_clock::time_point t0;
@@ -93,7 +95,8 @@ struct gsx_server {
     int members = 0;                 // solver thread only
     bool stop = false;
     std::string fatal;
-    std::thread tasm, tsol, tmg;
+    std::thread tasm, tsol;
+    std::vector<std::thread> tmg;
 
     double now() const
     {
@@ -375,7 +378,7 @@ struct gsx_server {
         }
     }
 
-    void mg_worker()
+    void mg_worker(gsx_ctx *cm)
     {
         while (true) {
             Req *q = nullptr;
@@ -436,20 +439,26 @@ gsx_status gsx_server_create(const gsx_server_opts *opts, gsx_server **out)
     if (s->o.mg_max_coarse <= 0) s->o.mg_max_coarse = 4096;
     gsx_status st = gsx_ctx_create(opts->device, &s->ca);
     if (st == GSX_OK) st = gsx_ctx_create(opts->device, &s->cb);
-    if (st == GSX_OK) st = gsx_ctx_create(opts->device, &s->cm);
+    const int nmg = s->o.mg_min_free > 0 ? std::max(1, s->o.mg_workers) : 0;
+    for (int k = 0; k < nmg && st == GSX_OK; ++k) {
+        gsx_ctx *c = nullptr;
+        st = gsx_ctx_create(opts->device, &c);
+        if (c) s->cms.push_back(c);
+    }
     if (st == GSX_OK) st = gsx_batch_create(s->cb, opts->precision, opts->capacity, &s->batch);
     if (st == GSX_OK) st = gsx_batch_set_group(s->batch, opts->group_max > 0 ? opts->group_max : 1);
     if (st != GSX_OK) {
         if (s->batch) gsx_batch_destroy(s->batch);
-        for (gsx_ctx *c : {s->ca, s->cb, s->cm})
+        for (gsx_ctx *c : {s->ca, s->cb})
             if (c) gsx_ctx_destroy(c);
+        for (gsx_ctx *c : s->cms) gsx_ctx_destroy(c);
         delete s;
         return st;
     }
     s->t0 = std::chrono::steady_clock::now();
     s->tasm = std::thread([s] { s->assembler(); });
     s->tsol = std::thread([s] { s->solver(); });
-    s->tmg = std::thread([s] { s->mg_worker(); });
+    for (gsx_ctx *c : s->cms) s->tmg.emplace_back([s, c] { s->mg_worker(c); });
     *out = s;
     return GSX_OK;
 }
@@ -585,7 +594,7 @@ gsx_status gsx_server_destroy(gsx_server *s)
     s->cv.notify_all();
     s->tasm.join();
     s->tsol.join();
-    s->tmg.join();
+    for (std::thread &th : s->tmg) th.join();
     // requests still held: free their device objects
     for (Req *q : s->active) {
         s->release(q);
@@ -599,7 +608,7 @@ gsx_status gsx_server_destroy(gsx_server *s)
     gsx_batch_destroy(s->batch);
     gsx_ctx_destroy(s->ca);
     gsx_ctx_destroy(s->cb);
-    gsx_ctx_destroy(s->cm);
+    for (gsx_ctx *c : s->cms) gsx_ctx_destroy(c);
     delete s;
     return GSX_OK;
 }

@@ -517,9 +517,10 @@ gsx_status gsx_batch_remove(gsx_batch *batch, int32_t id);
  * side as a box member of one gsx_batch (Jacobi-CG, members join and leave
  * between steps of iters_per_step iterations); requests with at least
  * mg_min_free free DOFs whose elements coarsen dyadically to at most
- * mg_max_coarse coarse DOFs go to a multigrid thread (V(2,2), damped Jacobi
- * 0.6, hierarchy built once per request).  Three contexts on `device`.  A
- * request's solutions are bitwise the same alone and in any mix. */
+ * mg_max_coarse coarse DOFs go to the multigrid threads (V(2,2), damped
+ * Jacobi 0.6, hierarchy built once per request; mg_workers requests at a
+ * time).  One context per thread on `device`.  A request's solutions are
+ * bitwise the same alone and in any mix. */
 typedef struct gsx_server gsx_server;
 typedef struct {
     int32_t device;
@@ -531,6 +532,7 @@ typedef struct {
     int64_t mg_max_coarse;    /* coarsest-level free DOFs at most (0: 4096)      */
     int64_t elem_budget;      /* multigrid lane element-matrix bytes (0: 16 GiB) */
     int32_t group_max;        /* gsx_batch_set_group of the batch lane (0: 1)    */
+    int32_t mg_workers;       /* multigrid-lane threads, one context each (0: 1) */
 } gsx_server_opts;
 /* One patch with homogeneous Dirichlet conditions on every side; nrhs
  * right-hand sides of nfree values (rhs_host NULL or rhs_host[0] NULL: the
