@@ -54,8 +54,8 @@ __device__ int transfer_row(int f, const int *__restrict__ occ_ptr, const int *_
     }
     const int cnx = coarse_nb[3 * k], cny = coarse_nb[3 * k + 1];
     int cnt = 0;
-    int cc[kMaxPRow];
-    double cv[kMaxPRow];
+    // entries go straight to the row's own output slots (sorted there by
+    // insertion); no per-thread arrays, so the kernels need no scratch
     for (int iz = b0[2]; iz < e0[2]; ++iz)
         for (int iy = b0[1]; iy < e0[1]; ++iy)
             for (int ix = b0[0]; ix < e0[0]; ++ix) {
@@ -74,22 +74,18 @@ __device__ int transfer_row(int f, const int *__restrict__ occ_ptr, const int *_
                 const int g = cdofs[cdofs_off[k] + cx + cnx * (cy + cny * cz)];
                 if (g >= nfree_c) continue;
                 if (cnt >= kMaxPRow) return -1;
-                // insertion by column
-                int pos = cnt;
-                while (pos > 0 && cc[pos - 1] > g) {
-                    cc[pos] = cc[pos - 1];
-                    cv[pos] = cv[pos - 1];
-                    --pos;
+                if (WRITE) {   // insertion by column
+                    int pos = cnt;
+                    while (pos > 0 && ocol[pos - 1] > g) {
+                        ocol[pos] = ocol[pos - 1];
+                        oval[pos] = oval[pos - 1];
+                        --pos;
+                    }
+                    ocol[pos] = g;
+                    oval[pos] = v;
                 }
-                cc[pos] = g;
-                cv[pos] = v;
                 ++cnt;
             }
-    if (WRITE)
-        for (int i = 0; i < cnt; ++i) {
-            ocol[i] = cc[i];
-            oval[i] = cv[i];
-        }
     return cnt;
 }
 

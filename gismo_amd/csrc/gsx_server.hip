@@ -35,6 +35,8 @@
 #include <thread>
 #include <vector>
 
+#include <hip/hip_runtime.h>
+
 #include "gsx.h"
 
 namespace {
@@ -251,8 +253,25 @@ struct gsx_server {
         return st;
     }
 
+    // device memory still free after admitting `need` bytes stays above the
+    // reserve (kernels with private arrays need the runtime to allocate
+    // scratch at dispatch: a device without free memory aborts the queue);
+    // the contexts' cached blocks are released first when that helps
+    bool device_room(int64_t need)
+    {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return true;
+        if ((int64_t)fr - need >= o.reserve_bytes) return true;
+        gsx_ctx_release_cache(ca);
+        gsx_ctx_release_cache(cb);
+        for (gsx_ctx *c : cms) gsx_ctx_release_cache(c);
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return true;
+        return (int64_t)fr - need >= o.reserve_bytes;
+    }
+
     void assembler()
     {
+        (void)hipSetDevice(o.device);
         while (true) {
             Req *q = nullptr;
             {
@@ -263,6 +282,17 @@ struct gsx_server {
                     const Req *h = incoming.front();
                     return used == 0 || used + h->bytes <= o.budget_bytes;
                 });
+                if (stop) return;
+                const int64_t need = incoming.front()->bytes;
+                if (used > 0) {
+                    lk.unlock();
+                    const bool room = device_room(need);
+                    lk.lock();
+                    if (!room) {   // wait for a request to finish, then look again
+                        cv.wait_for(lk, std::chrono::milliseconds(20));
+                        continue;
+                    }
+                }
                 if (stop) return;
                 q = incoming.front();
                 incoming.pop_front();
@@ -437,6 +467,7 @@ gsx_status gsx_server_create(const gsx_server_opts *opts, gsx_server **out)
     s->o = *opts;
     if (s->o.elem_budget <= 0) s->o.elem_budget = (int64_t)16 << 30;
     if (s->o.mg_max_coarse <= 0) s->o.mg_max_coarse = 4096;
+    if (s->o.reserve_bytes <= 0) s->o.reserve_bytes = (int64_t)16 << 30;
     gsx_status st = gsx_ctx_create(opts->device, &s->ca);
     if (st == GSX_OK) st = gsx_ctx_create(opts->device, &s->cb);
     const int nmg = s->o.mg_min_free > 0 ? std::max(1, s->o.mg_workers) : 0;

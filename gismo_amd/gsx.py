@@ -86,7 +86,7 @@ class ServerOpts(C.Structure):
     _fields_ = [("device", C.c_int32), ("precision", C.c_int32), ("budget_bytes", C.c_int64),
                 ("iters_per_step", C.c_int32), ("capacity", C.c_int32), ("mg_min_free", C.c_int64),
                 ("mg_max_coarse", C.c_int64), ("elem_budget", C.c_int64), ("group_max", C.c_int32),
-                ("mg_workers", C.c_int32)]
+                ("mg_workers", C.c_int32), ("reserve_bytes", C.c_int64)]
 
 
 class ServerRequest(C.Structure):

@@ -533,6 +533,8 @@ typedef struct {
     int64_t elem_budget;      /* multigrid lane element-matrix bytes (0: 16 GiB) */
     int32_t group_max;        /* gsx_batch_set_group of the batch lane (0: 1)    */
     int32_t mg_workers;       /* multigrid-lane threads, one context each (0: 1) */
+    int64_t reserve_bytes;    /* device memory kept free beside the admitted
+                                 requests (runtime scratch; 0: 16 GiB)           */
 } gsx_server_opts;
 /* One patch with homogeneous Dirichlet conditions on every side; nrhs
  * right-hand sides of nfree values (rhs_host NULL or rhs_host[0] NULL: the
