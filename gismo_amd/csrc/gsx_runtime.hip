@@ -1045,7 +1045,7 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
         if (!P->d_detmin) return fail(c, GSX_ERR_OOM, "device allocation (Jacobian check)");
     }
     // single 3D patch: sum-factorised rows, no element matrices
-    // (by default where its matrix-core variant runs: degrees 2 and 3)
+    // (by default where its matrix-core variant runs: degrees 2 to 4)
     if (!rowwise_forced && !gsx::sf_mfma(P->hp[0].p[0])) elementwise = true;
     const bool use_sf = single && !elementwise && !keep_elements && !P->zero_kappa && P->hp[0].d == 3 &&
                         P->hp[0].p[0] == P->hp[0].p[1] && P->hp[0].p[0] == P->hp[0].p[2] &&

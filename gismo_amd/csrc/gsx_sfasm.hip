@@ -388,7 +388,9 @@ typedef double sf_d4 __attribute__((ext_vector_type(4)));
 template <int P>
 struct SfmCfg {
     static constexpr int NB1 = P + 1, W = 2 * P + 1, NS = W * W * W;
-    static constexpr int TY = 4, TZ = 4;
+    // row tile in (y, z): 4 x 4 up to degree 3; 2 x 3 at degree 4 (the z
+    // stage's 16 columns hold TZ (p + 1) <= 16, and the LDS arrays fit)
+    static constexpr int TY = P == 4 ? 2 : 4, TZ = P == 4 ? 3 : 4;
     static constexpr int NGY = (TY + P) * NB1, NGZ = (TZ + P) * NB1;
     static constexpr int KZ = (NGZ + 3) / 4 * 4;       // z contraction depth (gz)
     static constexpr int MY = (NGY + 15) / 16 * 16;    // z stage rows (gy)
@@ -400,7 +402,7 @@ struct SfmCfg {
     static constexpr int NTF = NE;                     // x stage column tiles (one per e)
     static constexpr int NWAVE = 8;
     static constexpr int FPW = (MTF * NTF + NWAVE - 1) / NWAVE;
-    static_assert(NC <= 16 && P <= 3, "one column tile, ring of at most four rows");
+    static_assert(NC <= 16 && P <= 4, "one column tile; the ring holds at most five rows");
 };
 
 template <int P>
@@ -465,15 +467,34 @@ __global__ __launch_bounds__(512) void k_sfasm3d_mfma(const double *__restrict__
         sWy[f][e][gy] = v;
     }
 
-    sf_d4 acc[FPW];
+    // ring positions 0..3 in the four elements of acc; at degree 4 the fifth
+    // (row ex + 4, first touched by element ex) accumulates in element 0 of
+    // acc4 and moves into element 3 of acc when the ring shifts
+    // (only ox <= 0, the x tiles 0 and 1, reach position 4: fi = w + 8 u < 2 NTF)
+    constexpr int NA4 = P == 4 ? (2 * NTF + 7) / 8 : 1;
+    sf_d4 acc[FPW], acc4[NA4];
 #pragma unroll
     for (int u = 0; u < FPW; ++u) acc[u] = sf_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int u = 0; u < NA4; ++u) acc4[u] = sf_d4{0.0, 0.0, 0.0, 0.0};
     const int li = l % 16, lk = l / 16;
 
     // D of point column gc for the tile's (gz, gy) window, zero-padded:
     // each thread holds its DPT entries in registers one column ahead
-    constexpr int DN = 6 * KZ * MY, DPT = (DN + 511) / 512;
+    // (degree 4 keeps no registers to spare: D goes straight to LDS there)
+    constexpr bool PREFETCH = P < 4;
+    constexpr int DN = 6 * KZ * MY, DPT = PREFETCH ? (DN + 511) / 512 : 1;
     double dpre[DPT];
+    auto d_direct = [&](int64_t gc) {
+        for (int k = t; k < DN; k += 512) {
+            double v = 0.0;
+            if (gc < (int64_t)Gx) {
+                const int gy = k % MY, gz = (k / MY) % KZ, sc = k / (MY * KZ);
+                if (gy < ngy && gz < ngz) v = Dq[((gc * 6 + sc) * Gz + ez_lo * NB1 + gz) * Gy + ey_lo * NB1 + gy];
+            }
+            (&sD[0][0][0])[k] = v;
+        }
+    };
     auto d_load = [&](int64_t gc) {
 #pragma unroll
         for (int u = 0; u < DPT; ++u) {
@@ -501,9 +522,13 @@ __global__ __launch_bounds__(512) void k_sfasm3d_mfma(const double *__restrict__
             sDx[q][m] = tx[(int64_t)Gx * NB1 + g * NB1 + m];
         }
     };
-    d_load(0);
-    d_store();
-    d_load(1);
+    if constexpr (PREFETCH) {
+        d_load(0);
+        d_store();
+        d_load(1);
+    } else {
+        d_direct(0);
+    }
     x_tables(0);
     __syncthreads();
 
@@ -543,8 +568,12 @@ __global__ __launch_bounds__(512) void k_sfasm3d_mfma(const double *__restrict__
             }
             __syncthreads();
             // sD is free: the next column's D goes in, the one after into registers
-            d_store();
-            d_load(gc + 2);
+            if constexpr (PREFETCH) {
+                d_store();
+                d_load(gc + 2);
+            } else {
+                d_direct(gc + 1);
+            }
             if (qx == 0) x_tables(ex);   // the previous element's x stage is done
             // y: x-flag classes 0: (0,0); 1: (0,1),(0,2); 2: (1,0),(2,0); 3: (1,1),(1,2) | (2,1),(2,2)
             for (int unit = w; unit < NUNIT; unit += 8) {
@@ -578,16 +607,21 @@ __global__ __launch_bounds__(512) void k_sfasm3d_mfma(const double *__restrict__
             }
             __syncthreads();
             // x: rows i = 4 r + g (ox = 4 t + g - p), depth = the four classes
-            double xa[MTF];
+            double xa[MTF], xa4[MTF];
 #pragma unroll
             for (int tt = 0; tt < MTF; ++tt) {
                 const int r = li / 4, g = li % 4, ox = 4 * tt + g - P, r2 = r + ox, cls = lk;
+                const bool fa = cls == 0 || cls == 1, fb = cls == 0 || cls == 2;
                 double v = 0.0;
-                if (r <= P && ox <= P && r2 >= 0 && r2 <= P) {
-                    const bool fa = cls == 0 || cls == 1, fb = cls == 0 || cls == 2;
+                if (r <= P && ox <= P && r2 >= 0 && r2 <= P)
                     v = (fa ? sDx[qx][r] : sVx[qx][r]) * (fb ? sDx[qx][r2] : sVx[qx][r2]);
-                }
                 xa[tt] = v;
+                // degree 4: rows g of the second operand hold ring position 4
+                const int r4 = P, s2 = r4 + ox;
+                double v4 = 0.0;
+                if (P == 4 && li < 4 && ox <= P && s2 >= 0 && s2 <= P)
+                    v4 = (fa ? sDx[qx][r4] : sVx[qx][r4]) * (fb ? sDx[qx][s2] : sVx[qx][s2]);
+                xa4[tt] = v4;
             }
 #pragma unroll
             for (int u = 0; u < FPW; ++u) {
@@ -596,6 +630,10 @@ __global__ __launch_bounds__(512) void k_sfasm3d_mfma(const double *__restrict__
                     const int tt = fi / NTF, n = fi % NTF;
                     const double bv = lk == 3 ? sT3[3][n][li] + sT3[4][n][li] : sT3[lk][n][li];
                     acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[tt], bv, acc[u], 0, 0, 0);
+                    if constexpr (P == 4)
+                        if (u < NA4 && tt < 2)
+                            acc4[u < NA4 ? u : 0] =
+                                __builtin_amdgcn_mfma_f64_16x16x4f64(xa4[tt], bv, acc4[u < NA4 ? u : 0], 0, 0, 0);
                 }
             }
         }
@@ -614,7 +652,9 @@ __global__ __launch_bounds__(512) void k_sfasm3d_mfma(const double *__restrict__
                     for (int r = 0; r < nrow; ++r) {
                         const int ix = ex + r;
                         if (ix >= nbx || iy >= nby || iz >= nbz) continue;
-                        const double v = acc[u][r];
+                        double v;
+                        if constexpr (P == 4) v = r < 4 ? acc[u][r] : (u < NA4 && tt < 2 ? acc4[u < NA4 ? u : 0][0] : 0.0);
+                        else v = acc[u][r];
                         const int slot = ((oz + P) * W + (oy + P)) * W + (ox + P);
                         const bool fi_free = ix >= bx.flo[0] && ix <= bx.fhi[0] && iy >= bx.flo[1] &&
                                              iy <= bx.fhi[1] && iz >= bx.flo[2] && iz <= bx.fhi[2];
@@ -634,7 +674,13 @@ __global__ __launch_bounds__(512) void k_sfasm3d_mfma(const double *__restrict__
                         }
                     }
                 }
-                acc[u] = sf_d4{acc[u][1], acc[u][2], acc[u][3], 0.0};
+                if constexpr (P == 4) {
+                    const bool has4 = u < NA4 && tt < 2;
+                    acc[u] = sf_d4{acc[u][1], acc[u][2], acc[u][3], has4 ? acc4[u < NA4 ? u : 0][0] : 0.0};
+                    if (has4) acc4[u < NA4 ? u : 0] = sf_d4{0.0, 0.0, 0.0, 0.0};
+                } else {
+                    acc[u] = sf_d4{acc[u][1], acc[u][2], acc[u][3], 0.0};
+                }
             }
         }
     }
@@ -883,7 +929,7 @@ int64_t sf_table_doubles(int p, const int *nel)
     return n;
 }
 
-bool sf_mfma(int p) { return p == 2 || p == 3; }
+bool sf_mfma(int p) { return p >= 2 && p <= 4; }
 
 int64_t sf_dq_doubles(int p, const int *nel)
 {
@@ -902,9 +948,9 @@ hipError_t launch_sf_assemble(int p, const PatchDev *patches, const GaussTab &gt
         hipLaunchKernelGGL((k_load3d<PP>), dim3((unsigned)nelem), dim3(128), 0, s, patches, gt, Fbuf, detmin,   \
                            sf_mfma(PP) ? Dq : nullptr, bx.g[1], bx.g[2]);                                       \
         if (sf_mfma(PP)) {                                                                                      \
-            bx.nty = (bx.nb[1] + SfmCfg<(PP < 4 ? PP : 3)>::TY - 1) / SfmCfg<(PP < 4 ? PP : 3)>::TY;            \
-            bx.ntz = (bx.nb[2] + SfmCfg<(PP < 4 ? PP : 3)>::TZ - 1) / SfmCfg<(PP < 4 ? PP : 3)>::TZ;            \
-            hipLaunchKernelGGL((k_sfasm3d_mfma<(PP < 4 ? PP : 3)>), dim3(bx.nty * bx.ntz), dim3(512), 0, s, tab, \
+            bx.nty = (bx.nb[1] + SfmCfg<(PP > 1 ? PP : 2)>::TY - 1) / SfmCfg<(PP > 1 ? PP : 2)>::TY;            \
+            bx.ntz = (bx.nb[2] + SfmCfg<(PP > 1 ? PP : 2)>::TZ - 1) / SfmCfg<(PP > 1 ? PP : 2)>::TZ;            \
+            hipLaunchKernelGGL((k_sfasm3d_mfma<(PP > 1 ? PP : 2)>), dim3(bx.nty * bx.ntz), dim3(512), 0, s, tab, \
                                Dq, bx, s64);                                                                    \
         } else {                                                                                                \
             bx.nty = (bx.nb[1] + 1) / 2;                                                                        \

@@ -232,8 +232,8 @@ gsx_status gsx_assemble_bdr(gsx_problem *prob, double *rhs_dev);
  * column-major, src/gsCore/gsForwardDeclarations.h:279-280).
  * precision GSX_MIXED additionally keeps an FP32 copy of the values for the
  * inner iterations.  rhs_dev: nfree doubles (device) or NULL.
- * A single 3D patch of equal degree 2 or 3 in every direction (1..4 with
- * GSX_ASM_ROWWISE) whose kappa has no exact zero is assembled row by row by
+ * A single 3D patch of equal degree 2, 3 or 4 in every direction (also
+ * degree 1 with GSX_ASM_ROWWISE) whose kappa has no exact zero is assembled row by row by
  * sum factorisation on the FP64 matrix cores, without element matrices (the
  * same sums over the same Gauss points, grouped by point columns instead of
  * elements: values agree with the element path to rounding).  There no
