@@ -274,4 +274,29 @@ hipError_t launch_batch_iteration(int nchunks, const BChunk *chunks, int nsingle
 hipError_t launch_batch_reliable(int nchunks, const BChunk *chunks, const BSeg *segs, BState *st, double *p0,
                                  double *p2, hipStream_t s);
 
+// assembly with the geometry of another problem (gsx_pmg.hip): elements
+// [e0, e1) of patch ip of the discretisation patches ups, geometry patch ip
+// of gps; K (may be null) at kmat_off + e NB^2 - kshift, f (may be null) at
+// fvec_off + e NB - fshift with f = 1 when unit_f (the lumped mass)
+hipError_t launch_geo_elem(int d, const PatchDev *ups, const PatchDev *gps, int ip, int64_t e0, int64_t e1,
+                           const GaussTab &gt, double *Kbuf, double *Fbuf, double *detmin, int64_t kshift,
+                           int64_t fshift, int unit_f, hipStream_t s);
+// mixed mass element matrices (U elements x U functions x V functions) of patch ip at moff
+hipError_t launch_geo_mixed(int d, const PatchDev *ups, const PatchDev *vps, const PatchDev *gps, int ip,
+                            int64_t nelem, const GaussTab &gt, double *Mbuf, int64_t moff, hipStream_t s);
+// mixed mass pattern (rows: U's free DOFs, columns: V's free DOFs): phase 0
+// counts candidates (counts), 1 writes them at cptr, sorts and dedups
+// (counts = kept), 2 compacts into (rowptr, col)
+hipError_t launch_mix_pattern(int phase, int d, const PatchDev *ups, const PatchDev *vps, const int *occ_ptr,
+                              const int *occ_patch, const int64_t *occ_local, int nrows, int nfree_v,
+                              const int *cptr, int *cand, int *counts, const int *rowptr, int *col, hipStream_t s);
+hipError_t launch_mix_gather(int d, const PatchDev *ups, const PatchDev *vps, const int *occ_ptr, const int *occ_patch,
+                             const int64_t *occ_local, int nrows, int nfree_v, const int *rowptr, const int *col,
+                             const double *Mbuf, const int64_t *moff, double *val, hipStream_t s);
+// per free DOF: sum of the element vector entries of its functions
+hipError_t launch_vec_gather(int d, const PatchDev *ups, const int *occ_ptr, const int *occ_patch,
+                             const int64_t *occ_local, int nrows, const double *Fbuf, double *out, hipStream_t s);
+// val(row r) *= 1 / L(r); err[0] = 1 on L(r) == 0
+hipError_t launch_scale_rows(int nrows, const int *rowptr, const double *L, double *val, int *err, hipStream_t s);
+
 }  // namespace gsx

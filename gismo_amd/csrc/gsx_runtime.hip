@@ -289,6 +289,9 @@ struct gsx_problem {
     DevPtr kept_K;
     uint64_t asm_serial = 0, kept_serial = 0;
     bool zero_kappa = false;        // some element has kappa exactly 0
+    // geometry, kappa and source of another problem (a level of a
+    // p-multigrid hierarchy): this problem's patches give only the basis
+    gsx_problem *geo = nullptr;
 };
 
 // DOF numbering with the rules of gsDofMapper::finalizeComp
@@ -866,6 +869,46 @@ no_structure:
     return GSX_OK;
 }
 
+// A problem discretised with the bases in `patches` (degrees and knot
+// vectors; control points, kappa and sources are not read) over the geometry
+// map, kappa and source of `geo` (gsCDRAssembler(mp, bases[i], ...) of
+// examples/pMultiGrid_example.cpp:445-460, where the level bases differ from
+// the geometry's).  Each patch's parameter box must be the geometry patch's.
+gsx_status gsx_problem_create_geo(gsx_problem *geo, int32_t npatch, const gsx_patch_desc *patches, int32_t ndir,
+                                  const int32_t *dir_sides, int32_t nif, const int32_t *ifaces, gsx_problem **out)
+{
+    if (!geo || !patches || !out) return GSX_ERR_ARG;
+    gsx_ctx *c = geo->ctx;
+    if (geo->geo) return fail(c, GSX_ERR_ARG, "the geometry problem must carry its own geometry");
+    if (npatch != geo->npatch) return fail(c, GSX_ERR_ARG, "patch count differs from the geometry's");
+    std::vector<gsx_patch_desc> desc(patches, patches + npatch);
+    std::vector<std::vector<double>> zeros((size_t)npatch);
+    for (int k = 0; k < npatch; ++k) {
+        gsx_patch_desc &D = desc[(size_t)k];
+        if (D.d != geo->d) return fail(c, GSX_ERR_ARG, "dimension differs from the geometry's");
+        int64_t nbas = 1;
+        for (int j = 0; j < D.d; ++j) {
+            const std::vector<double> &G = geo->knots_h[(size_t)3 * k + j];
+            const int nk = D.nknots[j];
+            if (!D.knots_host[j] || nk < 2) return fail(c, GSX_ERR_ARG, "knot vector too short");
+            if (D.knots_host[j][0] != G.front() || D.knots_host[j][nk - 1] != G.back())
+                return fail(c, GSX_ERR_ARG, "parameter box differs from the geometry patch's");
+            nbas *= nk - D.p[j] - 1;
+        }
+        zeros[(size_t)k].assign((size_t)std::max<int64_t>(1, nbas) * D.d, 0.0);
+        D.ctrl_host = zeros[(size_t)k].data();
+        D.kappa_host = nullptr;
+        D.felem_host = nullptr;
+        D.fkind = 1;
+    }
+    gsx_problem *P = nullptr;
+    GSX_TRY(gsx_problem_create(c, npatch, desc.data(), ndir, dir_sides, nif, ifaces, &P));
+    P->geo = geo;
+    P->zero_kappa = geo->zero_kappa;
+    *out = P;
+    return GSX_OK;
+}
+
 gsx_status gsx_problem_destroy(gsx_problem *P)
 {
     if (!P) return GSX_ERR_ARG;
@@ -1047,6 +1090,13 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
     // single 3D patch: sum-factorised rows, no element matrices
     // (by default where its matrix-core variant runs: degrees 2 to 4)
     if (!rowwise_forced && !gsx::sf_mfma(P->hp[0].p[0])) elementwise = true;
+    if (P->geo) {
+        // geometry in another problem's basis: element matrices by the
+        // two-basis kernel (the row assembly evaluates the map in the
+        // discretisation basis)
+        elementwise = true;
+        if (P->geo->npatch != P->npatch) return fail(c, GSX_ERR_ARG, "geometry problem changed");
+    }
     const bool use_sf = single && !elementwise && !keep_elements && !P->zero_kappa && P->hp[0].d == 3 &&
                         P->hp[0].p[0] == P->hp[0].p[1] && P->hp[0].p[0] == P->hp[0].p[2] &&
                         gsx::sf_supported(3, P->hp[0].p[0]);
@@ -1084,7 +1134,13 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
     DevPtr Kbuf = dalloc(c, sizeof(double) * std::max<int64_t>(1, kcap));
     DevPtr Fbuf = dalloc(c, sizeof(double) * std::max<int64_t>(1, fcap));
     if (!Kbuf || !Fbuf) return fail(c, GSX_ERR_OOM, "device allocation (element matrices)");
-    if (!single) {
+    if (!single && P->geo) {
+        KScope k(c, "geo_elem");
+        for (int ip = 0; ip < P->npatch; ++ip)
+            HIP_OK(c, gsx::launch_geo_elem(P->d, dp<PatchDev>(P->d_patches), dp<PatchDev>(P->geo->d_patches), ip, 0,
+                                           P->hp[ip].nelem, c->gt, dp<double>(Kbuf), dp<double>(Fbuf),
+                                           dp<double>(P->d_detmin), 0, 0, 0, s));
+    } else if (!single) {
         for (auto &g : P->groups) {
             KScope k(c, g.d == 3 ? "elem3d" : "elem2d");
             HIP_OK(c, gsx::launch_elem(g.d, g.p, dp<PatchDev>(P->d_patches), dp<int>(g.d_gpatch),
@@ -1187,7 +1243,15 @@ gsx_status gsx_assemble(gsx_problem *P, int32_t mode, int32_t precision, gsx_mat
         for (int L0 = P->box_lo[sd]; !use_sf && L0 <= P->box_hi[sd]; L0 += slab_rows) {
             const int L1 = std::min(L0 + slab_rows, P->box_hi[sd] + 1);
             const int64_t E0 = std::max(0, L0 - h.p[0]), E1 = std::min(h.nel[sd], L1);
-            {
+            if (P->geo) {
+                KScope k(c, "geo_elem");
+                // element matrices of the slab from offset 0 (the kernel
+                // adds the patch offsets, as the range element kernels do)
+                HIP_OK(c, gsx::launch_geo_elem(h.d, dp<PatchDev>(P->d_patches), dp<PatchDev>(P->geo->d_patches), 0,
+                                               E0 * layer_elems, E1 * layer_elems, c->gt, dp<double>(Kbuf),
+                                               dp<double>(Fbuf), dp<double>(P->d_detmin),
+                                               E0 * layer_elems * nb0 * nb0, E0 * layer_elems * nb0, 0, s));
+            } else {
                 KScope k(c, h.d == 3 ? "elem3d" : "elem2d");
                 HIP_OK(c, gsx::launch_elem_range(h.d, h.p[0], dp<PatchDev>(P->d_patches), dp<int>(g.d_gpatch),
                                                  dp<int64_t>(g.d_gprefix), 1, E0 * layer_elems,
@@ -2465,6 +2529,7 @@ gsx_status gsx_error_norms(gsx_problem *P, const double *sol, double *l2, double
 {
     if (!P || !sol || (!l2 && !h1semi)) return GSX_ERR_ARG;
     gsx_ctx *c = P->ctx;
+    if (P->geo) return fail(c, GSX_ERR_ARG, "error norms need the problem's own geometry");
     hipStream_t s = c->stream;
     const int64_t ne = std::max<int64_t>(1, P->nelem);
     DevPtr per = dalloc(c, sizeof(double) * 2 * (ne + 1));
@@ -2704,6 +2769,171 @@ static gsx_status mg_finish(gsx_mg *mg)
     return GSX_OK;
 }
 
+// 1D knot-insertion transfers, computed once per distinct (coarse knots,
+// fine knots, p)
+struct TransferCache {
+    struct Item { std::vector<double> a, b; int p, nf, nc; std::vector<double> T; };
+    std::deque<Item> items;   // references stay valid as items are added
+    const std::vector<double> &get(const std::vector<double> &a, const std::vector<double> &b, int p, int &nf,
+                                   int &nc)
+    {
+        for (const Item &it : items)
+            if (it.p == p && it.a == a && it.b == b) {
+                nf = it.nf;
+                nc = it.nc;
+                return it.T;
+            }
+        Item it{a, b, p, 0, 0, {}};
+        it.T = transfer_1d(a, b, p, it.nf, it.nc);
+        nf = it.nf;
+        nc = it.nc;
+        items.push_back(std::move(it));
+        return items.back().T;
+    }
+};
+
+// P: the knot-insertion transfer from the basis of cp to that of fp (same
+// degrees, cp's knots a subset of fp's) restricted to free DOFs, rows = fp's
+// nf free DOFs (gsMultiBasis::combineTransferMatrices, src/gsCore/
+// gsMultiBasis.hpp:137-178), into C.prp/pci/pv.  With tblk, also the per
+// (patch, direction, fine element) 1D blocks of the element Galerkin path
+// at toff.
+static gsx_status mg_knot_transfer(gsx_ctx *c, gsx_problem *fp, gsx_problem *cp, int nf, MGLevel &C,
+                                   TransferCache &tc, std::vector<double> *tblk, std::vector<int64_t> *toff)
+{
+    hipStream_t s = c->stream;
+    const int npatch = fp->npatch, d = fp->d;
+    // 1D transfers per (patch, direction)
+    struct T1Dh { int rp_off, ent_off, nf, nc; };
+    std::vector<T1Dh> t1d((size_t)npatch * 3, T1Dh{0, 0, 1, 1});
+    std::vector<int> trp, tcol;
+    std::vector<double> tval;
+    std::vector<int> fnb((size_t)npatch * 3, 1), cnb((size_t)npatch * 3, 1), cdoff(npatch, 0);
+    // element path: per (patch, direction, fine span e) the block
+    // T[m][a] = Tm[e + m][e / 2 + a], (p+1)^2 doubles
+    if (toff) toff->assign((size_t)npatch * 3, 0);
+    for (int k = 0; k < npatch; ++k) {
+        cdoff[k] = (int)cp->boff[k];
+        for (int j = 0; j < 3; ++j) {
+            fnb[3 * k + j] = fp->hp[k].nb[j];
+            cnb[3 * k + j] = cp->hp[k].nb[j];
+        }
+        for (int j = 0; j < 3; ++j) {
+            T1Dh &T = t1d[3 * k + j];
+            T.rp_off = (int)trp.size();
+            T.ent_off = (int)tcol.size();
+            if (j >= d) {
+                trp.push_back(0);
+                trp.push_back(1);
+                tcol.push_back(0);
+                tval.push_back(1.0);
+                continue;
+            }
+            int nf1 = 0, nc1 = 0;   // 1D fine and coarse function counts
+            const std::vector<double> &Tm = tc.get(cp->knots_h[(size_t)3 * k + j],
+                                                   fp->knots_h[(size_t)3 * k + j], fp->hp[k].p[j], nf1, nc1);
+            T.nf = nf1;
+            T.nc = nc1;
+            if (tblk) {
+                const int p1 = fp->hp[k].p[j] + 1, nelf = fp->hp[k].nel[j];
+                (*toff)[(size_t)3 * k + j] = (int64_t)tblk->size();
+                for (int e = 0; e < nelf; ++e)
+                    for (int m = 0; m < p1; ++m)
+                        for (int a = 0; a < p1; ++a) tblk->push_back(Tm[(size_t)(e + m) * nc1 + e / 2 + a]);
+            }
+            int cnt = 0;
+            for (int i = 0; i < nf1; ++i) {
+                trp.push_back(cnt);
+                for (int cc = 0; cc < nc1; ++cc)
+                    if (Tm[(size_t)i * nc1 + cc] != 0.0) {
+                        tcol.push_back(cc);
+                        tval.push_back(Tm[(size_t)i * nc1 + cc]);
+                        ++cnt;
+                    }
+            }
+            trp.push_back(cnt);
+        }
+    }
+    // rp entries are relative to ent_off
+    DevPtr d_t1d = dalloc(c, sizeof(T1Dh) * t1d.size());
+    DevPtr d_trp = dalloc(c, sizeof(int) * trp.size()), d_tcol = dalloc(c, sizeof(int) * tcol.size());
+    DevPtr d_tval = dalloc(c, sizeof(double) * tval.size());
+    DevPtr d_fnb = dalloc(c, sizeof(int) * fnb.size()), d_cnb = dalloc(c, sizeof(int) * cnb.size());
+    DevPtr d_cdoff = dalloc(c, sizeof(int) * npatch);
+    DevPtr counts = dalloc(c, sizeof(int) * (nf + 1)), errf = dalloc(c, sizeof(int) * 2);
+    if (!d_t1d || !d_trp || !d_tcol || !d_tval || !d_fnb || !d_cnb || !d_cdoff || !counts || !errf)
+        return fail(c, GSX_ERR_OOM, "device allocation (transfer)");
+    HIP_OK(c, hipMemcpyAsync(d_t1d.get(), t1d.data(), sizeof(T1Dh) * t1d.size(), hipMemcpyHostToDevice, s));
+    HIP_OK(c, hipMemcpyAsync(d_trp.get(), trp.data(), sizeof(int) * trp.size(), hipMemcpyHostToDevice, s));
+    HIP_OK(c, hipMemcpyAsync(d_tcol.get(), tcol.data(), sizeof(int) * tcol.size(), hipMemcpyHostToDevice, s));
+    HIP_OK(c, hipMemcpyAsync(d_tval.get(), tval.data(), sizeof(double) * tval.size(), hipMemcpyHostToDevice, s));
+    HIP_OK(c, hipMemcpyAsync(d_fnb.get(), fnb.data(), sizeof(int) * fnb.size(), hipMemcpyHostToDevice, s));
+    HIP_OK(c, hipMemcpyAsync(d_cnb.get(), cnb.data(), sizeof(int) * cnb.size(), hipMemcpyHostToDevice, s));
+    HIP_OK(c, hipMemcpyAsync(d_cdoff.get(), cdoff.data(), sizeof(int) * npatch, hipMemcpyHostToDevice, s));
+    HIP_OK(c, hipMemsetAsync(errf.get(), 0, sizeof(int) * 2, s));
+    HIP_OK(c, gsx::launch_transfer(0, nf, dp<int>(fp->d_occ_ptr), dp<int>(fp->d_occ_patch),
+                                   dp<int64_t>(fp->d_occ_local), dp<int>(d_fnb), d_t1d.get(), dp<int>(d_trp),
+                                   dp<int>(d_tcol), dp<double>(d_tval), dp<int>(d_cdoff), dp<int>(cp->d_dofs),
+                                   dp<int>(d_cnb), d, C.n, dp<int>(counts), dp<int>(errf), nullptr, nullptr,
+                                   nullptr, s));
+    int64_t pn = 0;
+    GSX_TRY(scan_counts(c, dp<int>(counts), nf, C.prp, pn));
+    C.nfine = nf;
+    C.pnnz = pn;
+    C.pci = dalloc(c, sizeof(int) * std::max<int64_t>(1, pn));
+    C.pv = dalloc(c, sizeof(double) * std::max<int64_t>(1, pn));
+    if (!C.pci || !C.pv) return fail(c, GSX_ERR_OOM, "device allocation (transfer)");
+    HIP_OK(c, gsx::launch_transfer(1, nf, dp<int>(fp->d_occ_ptr), dp<int>(fp->d_occ_patch),
+                                   dp<int64_t>(fp->d_occ_local), dp<int>(d_fnb), d_t1d.get(), dp<int>(d_trp),
+                                   dp<int>(d_tcol), dp<double>(d_tval), dp<int>(d_cdoff), dp<int>(cp->d_dofs),
+                                   dp<int>(d_cnb), d, C.n, nullptr, nullptr, dp<int>(C.prp), dp<int>(C.pci),
+                                   dp<double>(C.pv), s));
+    int errh[2] = {0, 0};
+    HIP_OK(c, hipMemcpyAsync(errh, errf.get(), sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    HIP_OK(c, hipStreamSynchronize(s));
+    if (errh[0] || errh[1]) return fail(c, GSX_ERR_ARG, "multigrid transfer: row buffer overflow");
+    return GSX_OK;
+}
+
+// R = P^T (rows sorted by column) into C.rrp/rci/rv; C.n coarse rows, nf fine.
+static gsx_status mg_transpose_transfer(gsx_ctx *c, MGLevel &C, int nf)
+{
+    hipStream_t s = c->stream;
+    DevPtr tcount = dalloc(c, sizeof(int) * (C.n + 1)), cursor = dalloc(c, sizeof(int) * (C.n + 1));
+    if (!tcount || !cursor) return fail(c, GSX_ERR_OOM, "device allocation (restriction)");
+    HIP_OK(c, hipMemsetAsync(tcount.get(), 0, sizeof(int) * (C.n + 1), s));
+    HIP_OK(c, hipMemsetAsync(cursor.get(), 0, sizeof(int) * (C.n + 1), s));
+    HIP_OK(c, gsx::launch_transpose(0, nf, dp<int>(C.prp), dp<int>(C.pci), nullptr, dp<int>(tcount), nullptr,
+                                    nullptr, nullptr, nullptr, 0, s));
+    int64_t rn = 0;
+    GSX_TRY(scan_counts(c, dp<int>(tcount), C.n, C.rrp, rn));
+    C.rci = dalloc(c, sizeof(int) * std::max<int64_t>(1, rn));
+    C.rv = dalloc(c, sizeof(double) * std::max<int64_t>(1, rn));
+    if (!C.rci || !C.rv) return fail(c, GSX_ERR_OOM, "device allocation (restriction)");
+    HIP_OK(c, gsx::launch_transpose(1, nf, dp<int>(C.prp), dp<int>(C.pci), dp<double>(C.pv), nullptr,
+                                    dp<int>(C.rrp), dp<int>(cursor), dp<int>(C.rci), dp<double>(C.rv), 0, s));
+    HIP_OK(c, gsx::launch_transpose(2, 0, nullptr, nullptr, nullptr, nullptr, dp<int>(C.rrp), nullptr,
+                                    dp<int>(C.rci), dp<double>(C.rv), C.n, s));
+    return GSX_OK;
+}
+
+// Galerkin operator A_c = R (A_f P) (src/gsMultiGrid/gsMultiGrid.hpp:70-104)
+// into the coarse pattern C.rowptr/col, values C.val: B = A_f P row by row
+// in ELL form, then R B; overflow codes in err[0] (B) and err[1] (pattern).
+static gsx_status mg_galerkin_sparse(gsx_ctx *c, MGLevel &C, const MGLevel &F, int d, int *err)
+{
+    hipStream_t s = c->stream;
+    const int WB = gsx::ap_width(d);
+    DevPtr bcnt = dalloc(c, sizeof(int) * std::max(1, F.n)), bci = dalloc(c, sizeof(int) * (size_t)std::max(1, F.n) * WB);
+    DevPtr bv = dalloc(c, sizeof(double) * (size_t)std::max(1, F.n) * WB);
+    if (!bcnt || !bci || !bv) return fail(c, GSX_ERR_OOM, "device allocation (Galerkin product)");
+    HIP_OK(c, gsx::launch_ap(d, F.n, dp<int>(F.rowptr), dp<int>(F.col), dp<double>(F.val), dp<int>(C.prp),
+                             dp<int>(C.pci), dp<double>(C.pv), dp<int>(bcnt), dp<int>(bci), dp<double>(bv), err, s));
+    HIP_OK(c, gsx::launch_rap(d, C.n, dp<int>(C.rrp), dp<int>(C.rci), dp<double>(C.rv), dp<int>(bcnt), dp<int>(bci),
+                              dp<double>(bv), dp<int>(C.rowptr), dp<int>(C.col), dp<double>(C.val), err + 1, s));
+    return GSX_OK;
+}
+
 extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nlevels, int32_t npre,
                                     int32_t npost, int32_t ncycles, double damping, double cgc_damping,
                                     int32_t precision, gsx_mg **out)
@@ -2809,135 +3039,20 @@ extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nl
                                        dp<int64_t>(g.d_gprefix), (int)g.patches.size(), g.nelem, c->gt,
                                        dp<double>(Kf), dp<double>(Ff), dp<double>(fine->d_detmin), s));
     }
-    // 1D transfers, computed once per distinct (coarse knots, fine knots, p)
-    struct TCache {
-        struct Item { std::vector<double> a, b; int p, nf, nc; std::vector<double> T; };
-        std::deque<Item> items;   // references stay valid as items are added
-        const std::vector<double> &get(const std::vector<double> &a, const std::vector<double> &b, int p, int &nf,
-                                       int &nc)
-        {
-            for (const Item &it : items)
-                if (it.p == p && it.a == a && it.b == b) {
-                    nf = it.nf;
-                    nc = it.nc;
-                    return it.T;
-                }
-            Item it{a, b, p, 0, 0, {}};
-            it.T = transfer_1d(a, b, p, it.nf, it.nc);
-            nf = it.nf;
-            nc = it.nc;
-            items.push_back(std::move(it));
-            return items.back().T;
-        }
-    } tcache;
+    TransferCache tcache;
     // transfers and Galerkin products, finest first
     for (int l = nlevels - 2; l >= 0; --l) {
         MGLevel &C = mg->L[l], &F = mg->L[l + 1];
         KScope kt(c, "mg_transfer_build");
         gsx_problem *fp = (l + 1 == nlevels - 1) ? fine : F.prob;
         gsx_problem *cp = C.prob;
-        // 1D transfers per (patch, direction)
-        struct T1Dh { int rp_off, ent_off, nf, nc; };
-        std::vector<T1Dh> t1d((size_t)npatch * 3, T1Dh{0, 0, 1, 1});
-        std::vector<int> trp, tcol;
-        std::vector<double> tval;
-        std::vector<int> fnb((size_t)npatch * 3, 1), cnb((size_t)npatch * 3, 1), cdoff(npatch, 0);
-        // element path: per (patch, direction, fine span e) the block
-        // T[m][a] = Tm[e + m][e / 2 + a], (p+1)^2 doubles
         std::vector<double> tblk;
-        std::vector<int64_t> toff((size_t)npatch * 3, 0);
-        for (int k = 0; k < npatch; ++k) {
-            cdoff[k] = (int)cp->boff[k];
-            for (int j = 0; j < 3; ++j) {
-                fnb[3 * k + j] = fp->hp[k].nb[j];
-                cnb[3 * k + j] = cp->hp[k].nb[j];
-            }
-            for (int j = 0; j < 3; ++j) {
-                T1Dh &T = t1d[3 * k + j];
-                T.rp_off = (int)trp.size();
-                T.ent_off = (int)tcol.size();
-                if (j >= d) {
-                    trp.push_back(0);
-                    trp.push_back(1);
-                    tcol.push_back(0);
-                    tval.push_back(1.0);
-                    continue;
-                }
-                int nf = 0, nc = 0;
-                const std::vector<double> &Tm = tcache.get(knots[l][3 * k + j], knots[l + 1][3 * k + j],
-                                                           fp->hp[k].p[j], nf, nc);
-                T.nf = nf;
-                T.nc = nc;
-                if (elem_path) {
-                    const int p1 = fp->hp[k].p[j] + 1, nelf = fp->hp[k].nel[j];
-                    toff[3 * k + j] = (int64_t)tblk.size();
-                    for (int e = 0; e < nelf; ++e)
-                        for (int m = 0; m < p1; ++m)
-                            for (int a = 0; a < p1; ++a) tblk.push_back(Tm[(size_t)(e + m) * nc + e / 2 + a]);
-                }
-                int cnt = 0;
-                for (int i = 0; i < nf; ++i) {
-                    trp.push_back(cnt);
-                    for (int cc = 0; cc < nc; ++cc)
-                        if (Tm[(size_t)i * nc + cc] != 0.0) {
-                            tcol.push_back(cc);
-                            tval.push_back(Tm[(size_t)i * nc + cc]);
-                            ++cnt;
-                        }
-                }
-                trp.push_back(cnt);
-            }
-        }
-        // rp entries are relative to ent_off
-        DevPtr d_t1d = dalloc(c, sizeof(T1Dh) * t1d.size());
-        DevPtr d_trp = dalloc(c, sizeof(int) * trp.size()), d_tcol = dalloc(c, sizeof(int) * tcol.size());
-        DevPtr d_tval = dalloc(c, sizeof(double) * tval.size());
-        DevPtr d_fnb = dalloc(c, sizeof(int) * fnb.size()), d_cnb = dalloc(c, sizeof(int) * cnb.size());
-        DevPtr d_cdoff = dalloc(c, sizeof(int) * npatch);
-        DevPtr counts = dalloc(c, sizeof(int) * (F.n + 1)), errf = dalloc(c, sizeof(int) * 2);
-        if (!d_t1d || !d_trp || !d_tcol || !d_tval || !d_fnb || !d_cnb || !d_cdoff || !counts || !errf)
-            return fail(c, GSX_ERR_OOM, "device allocation (transfer)");
-        HIP_OK(c, hipMemcpyAsync(d_t1d.get(), t1d.data(), sizeof(T1Dh) * t1d.size(), hipMemcpyHostToDevice, s));
-        HIP_OK(c, hipMemcpyAsync(d_trp.get(), trp.data(), sizeof(int) * trp.size(), hipMemcpyHostToDevice, s));
-        HIP_OK(c, hipMemcpyAsync(d_tcol.get(), tcol.data(), sizeof(int) * tcol.size(), hipMemcpyHostToDevice, s));
-        HIP_OK(c, hipMemcpyAsync(d_tval.get(), tval.data(), sizeof(double) * tval.size(), hipMemcpyHostToDevice, s));
-        HIP_OK(c, hipMemcpyAsync(d_fnb.get(), fnb.data(), sizeof(int) * fnb.size(), hipMemcpyHostToDevice, s));
-        HIP_OK(c, hipMemcpyAsync(d_cnb.get(), cnb.data(), sizeof(int) * cnb.size(), hipMemcpyHostToDevice, s));
-        HIP_OK(c, hipMemcpyAsync(d_cdoff.get(), cdoff.data(), sizeof(int) * npatch, hipMemcpyHostToDevice, s));
+        std::vector<int64_t> toff;
+        GSX_TRY(mg_knot_transfer(c, fp, cp, F.n, C, tcache, elem_path ? &tblk : nullptr, &toff));
+        GSX_TRY(mg_transpose_transfer(c, C, F.n));
+        DevPtr errf = dalloc(c, sizeof(int) * 2);
+        if (!errf) return fail(c, GSX_ERR_OOM, "device allocation (transfer)");
         HIP_OK(c, hipMemsetAsync(errf.get(), 0, sizeof(int) * 2, s));
-        HIP_OK(c, gsx::launch_transfer(0, F.n, dp<int>(fp->d_occ_ptr), dp<int>(fp->d_occ_patch),
-                                       dp<int64_t>(fp->d_occ_local), dp<int>(d_fnb), d_t1d.get(), dp<int>(d_trp),
-                                       dp<int>(d_tcol), dp<double>(d_tval), dp<int>(d_cdoff), dp<int>(cp->d_dofs),
-                                       dp<int>(d_cnb), d, C.n, dp<int>(counts), dp<int>(errf), nullptr, nullptr,
-                                       nullptr, s));
-        int64_t pn = 0;
-        GSX_TRY(scan_counts(c, dp<int>(counts), F.n, C.prp, pn));
-        C.nfine = F.n;
-        C.pnnz = pn;
-        C.pci = dalloc(c, sizeof(int) * std::max<int64_t>(1, pn));
-        C.pv = dalloc(c, sizeof(double) * std::max<int64_t>(1, pn));
-        if (!C.pci || !C.pv) return fail(c, GSX_ERR_OOM, "device allocation (transfer)");
-        HIP_OK(c, gsx::launch_transfer(1, F.n, dp<int>(fp->d_occ_ptr), dp<int>(fp->d_occ_patch),
-                                       dp<int64_t>(fp->d_occ_local), dp<int>(d_fnb), d_t1d.get(), dp<int>(d_trp),
-                                       dp<int>(d_tcol), dp<double>(d_tval), dp<int>(d_cdoff), dp<int>(cp->d_dofs),
-                                       dp<int>(d_cnb), d, C.n, nullptr, nullptr, dp<int>(C.prp), dp<int>(C.pci),
-                                       dp<double>(C.pv), s));
-        // R = P^T
-        DevPtr tcount = dalloc(c, sizeof(int) * (C.n + 1)), cursor = dalloc(c, sizeof(int) * (C.n + 1));
-        if (!tcount || !cursor) return fail(c, GSX_ERR_OOM, "device allocation (restriction)");
-        HIP_OK(c, hipMemsetAsync(tcount.get(), 0, sizeof(int) * (C.n + 1), s));
-        HIP_OK(c, hipMemsetAsync(cursor.get(), 0, sizeof(int) * (C.n + 1), s));
-        HIP_OK(c, gsx::launch_transpose(0, F.n, dp<int>(C.prp), dp<int>(C.pci), nullptr, dp<int>(tcount), nullptr,
-                                        nullptr, nullptr, nullptr, 0, s));
-        int64_t rn = 0;
-        GSX_TRY(scan_counts(c, dp<int>(tcount), C.n, C.rrp, rn));
-        C.rci = dalloc(c, sizeof(int) * std::max<int64_t>(1, rn));
-        C.rv = dalloc(c, sizeof(double) * std::max<int64_t>(1, rn));
-        if (!C.rci || !C.rv) return fail(c, GSX_ERR_OOM, "device allocation (restriction)");
-        HIP_OK(c, gsx::launch_transpose(1, F.n, dp<int>(C.prp), dp<int>(C.pci), dp<double>(C.pv), nullptr,
-                                        dp<int>(C.rrp), dp<int>(cursor), dp<int>(C.rci), dp<double>(C.rv), 0, s));
-        HIP_OK(c, gsx::launch_transpose(2, 0, nullptr, nullptr, nullptr, nullptr, dp<int>(C.rrp), nullptr,
-                                        dp<int>(C.rci), dp<double>(C.rv), C.n, s));
         C.rowptr = cp->d_rowptr;
         C.col = cp->d_col;
         C.nnz = cp->nnz;
@@ -2964,17 +3079,7 @@ extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nl
                                          nullptr, s));
             Kf = std::move(Kc);
         } else {
-            // Galerkin: B = A P (ELL), A_c = R B into the coarse pattern
-            const int WB = gsx::ap_width(d);
-            DevPtr bcnt = dalloc(c, sizeof(int) * F.n), bci = dalloc(c, sizeof(int) * (size_t)F.n * WB);
-            DevPtr bv = dalloc(c, sizeof(double) * (size_t)F.n * WB);
-            if (!bcnt || !bci || !bv) return fail(c, GSX_ERR_OOM, "device allocation (Galerkin product)");
-            HIP_OK(c, gsx::launch_ap(d, F.n, dp<int>(F.rowptr), dp<int>(F.col), dp<double>(F.val), dp<int>(C.prp),
-                                     dp<int>(C.pci), dp<double>(C.pv), dp<int>(bcnt), dp<int>(bci), dp<double>(bv),
-                                     dp<int>(errf), s));
-            HIP_OK(c, gsx::launch_rap(d, C.n, dp<int>(C.rrp), dp<int>(C.rci), dp<double>(C.rv), dp<int>(bcnt),
-                                      dp<int>(bci), dp<double>(bv), dp<int>(C.rowptr), dp<int>(C.col),
-                                      dp<double>(C.val), dp<int>(errf) + 1, s));
+            GSX_TRY(mg_galerkin_sparse(c, C, F, d, dp<int>(errf)));
         }
         int errh[2] = {0, 0};
         HIP_OK(c, hipMemcpyAsync(errh, errf.get(), sizeof(int) * 2, hipMemcpyDeviceToHost, s));
@@ -3127,6 +3232,271 @@ extern "C" gsx_status gsx_mg_create_transfers(gsx_matrix *A, int32_t ntransfer, 
     return GSX_OK;
 }
 
+// ---- p-multigrid (examples/pMultiGrid_example.cpp:296-560) ----------------
+
+// Lumped mass L(i) = int phi_i dx over the free DOFs of U, with U's own
+// quadrature and the geometry of G (assembleLumpedMass,
+// examples/pMultiGrid_example.cpp:803-826).
+static gsx_status pmg_lumped(gsx_ctx *c, gsx_problem *U, gsx_problem *G, DevPtr &L)
+{
+    hipStream_t s = c->stream;
+    DevPtr Fb = dalloc(c, sizeof(double) * std::max<int64_t>(1, U->fdoubles));
+    L = dalloc(c, sizeof(double) * std::max(1, U->nfree));
+    if (!Fb || !L) return fail(c, GSX_ERR_OOM, "device allocation (lumped mass)");
+    for (int ip = 0; ip < U->npatch; ++ip)
+        HIP_OK(c, gsx::launch_geo_elem(U->d, dp<PatchDev>(U->d_patches), dp<PatchDev>(G->d_patches), ip, 0,
+                                       U->hp[ip].nelem, c->gt, nullptr, dp<double>(Fb), nullptr, 0, 0, 1, s));
+    HIP_OK(c, gsx::launch_vec_gather(U->d, dp<PatchDev>(U->d_patches), dp<int>(U->d_occ_ptr), dp<int>(U->d_occ_patch),
+                                     dp<int64_t>(U->d_occ_local), U->nfree, dp<double>(Fb), dp<double>(L), s));
+    return GSX_OK;
+}
+
+// Mixed mass M(i, j) = int phi_i psi_j dx, rows the free DOFs of U (the finer
+// level), columns those of V, integrated on U's elements with U's quadrature
+// over the geometry of G (assembleMixedMass, pMultiGrid_example.cpp:855-880,
+// whose matrix().transpose() is this U x V matrix), into C.prp/pci/pv.
+static gsx_status pmg_mixed(gsx_ctx *c, gsx_problem *U, gsx_problem *V, gsx_problem *G, MGLevel &C)
+{
+    hipStream_t s = c->stream;
+    const int d = U->d, nrows = U->nfree;
+    std::vector<int64_t> moff((size_t)U->npatch);
+    int64_t total = 0;
+    for (int k = 0; k < U->npatch; ++k) {
+        const HostPatch &hu = U->hp[(size_t)k], &hv = V->hp[(size_t)k];
+        int64_t nbu = 1, nbv = 1;
+        for (int j = 0; j < d; ++j) {
+            nbu *= hu.p[j] + 1;
+            nbv *= hv.p[j] + 1;
+            // V's breakpoints must be breakpoints of U
+            const std::vector<double> &ku = U->knots_h[(size_t)3 * k + j], &kv = V->knots_h[(size_t)3 * k + j];
+            for (double v : kv)
+                if (!std::binary_search(ku.begin(), ku.end(), v))
+                    return fail(c, GSX_ERR_ARG, "p-multigrid: a coarser level's mesh must be nested in the finer one's");
+        }
+        moff[(size_t)k] = total;
+        total += hu.nelem * nbu * nbv;
+    }
+    DevPtr Mb = dalloc(c, sizeof(double) * std::max<int64_t>(1, total));
+    DevPtr d_moff = dalloc(c, sizeof(int64_t) * moff.size());
+    if (!Mb || !d_moff) return fail(c, GSX_ERR_OOM, "device allocation (mixed mass)");
+    HIP_OK(c, hipMemcpyAsync(d_moff.get(), moff.data(), sizeof(int64_t) * moff.size(), hipMemcpyHostToDevice, s));
+    {
+        KScope k(c, "pmg_mixed_elem");
+        for (int ip = 0; ip < U->npatch; ++ip)
+            HIP_OK(c, gsx::launch_geo_mixed(d, dp<PatchDev>(U->d_patches), dp<PatchDev>(V->d_patches),
+                                            dp<PatchDev>(G->d_patches), ip, U->hp[(size_t)ip].nelem, c->gt,
+                                            dp<double>(Mb), moff[(size_t)ip], s));
+    }
+    // pattern: candidates per row, sorted and made unique, then compacted
+    DevPtr counts = dalloc(c, sizeof(int) * (nrows + 1));
+    if (!counts) return fail(c, GSX_ERR_OOM, "device allocation (mixed mass pattern)");
+    const PatchDev *ups = dp<PatchDev>(U->d_patches), *vps = dp<PatchDev>(V->d_patches);
+    HIP_OK(c, gsx::launch_mix_pattern(0, d, ups, vps, dp<int>(U->d_occ_ptr), dp<int>(U->d_occ_patch),
+                                      dp<int64_t>(U->d_occ_local), nrows, V->nfree, nullptr, nullptr, dp<int>(counts),
+                                      nullptr, nullptr, s));
+    DevPtr cptr;
+    int64_t ncand = 0;
+    GSX_TRY(scan_counts(c, dp<int>(counts), nrows, cptr, ncand));
+    if (ncand > INT32_MAX) return fail(c, GSX_ERR_ARG, "mixed mass too large for int32 indices");
+    DevPtr cand = dalloc(c, sizeof(int) * std::max<int64_t>(1, ncand));
+    if (!cand) return fail(c, GSX_ERR_OOM, "device allocation (mixed mass pattern)");
+    HIP_OK(c, gsx::launch_mix_pattern(1, d, ups, vps, dp<int>(U->d_occ_ptr), dp<int>(U->d_occ_patch),
+                                      dp<int64_t>(U->d_occ_local), nrows, V->nfree, dp<int>(cptr), dp<int>(cand),
+                                      dp<int>(counts), nullptr, nullptr, s));
+    int64_t nnz = 0;
+    GSX_TRY(scan_counts(c, dp<int>(counts), nrows, C.prp, nnz));
+    C.pci = dalloc(c, sizeof(int) * std::max<int64_t>(1, nnz));
+    C.pv = dalloc(c, sizeof(double) * std::max<int64_t>(1, nnz));
+    if (!C.pci || !C.pv) return fail(c, GSX_ERR_OOM, "device allocation (mixed mass)");
+    HIP_OK(c, gsx::launch_mix_pattern(2, d, ups, vps, nullptr, nullptr, nullptr, nrows, V->nfree, dp<int>(cptr),
+                                      dp<int>(cand), nullptr, dp<int>(C.prp), dp<int>(C.pci), s));
+    {
+        KScope k(c, "pmg_mixed_gather");
+        HIP_OK(c, gsx::launch_mix_gather(d, ups, vps, dp<int>(U->d_occ_ptr), dp<int>(U->d_occ_patch),
+                                         dp<int64_t>(U->d_occ_local), nrows, V->nfree, dp<int>(C.prp),
+                                         dp<int>(C.pci), dp<double>(Mb), dp<int64_t>(d_moff), dp<double>(C.pv), s));
+    }
+    C.nfine = nrows;
+    C.pnnz = nnz;
+    return GSX_OK;
+}
+
+// Open knot vector of degree q over the breakpoints of U (simple interior
+// knots: the maximal smoothness of gsKnotVector::degreeIncrease/
+// degreeReduce on such a vector).
+static std::vector<double> pmg_knots(const std::vector<double> &U, int q)
+{
+    std::vector<double> bp;
+    for (double u : U)
+        if (bp.empty() || u != bp.back()) bp.push_back(u);
+    std::vector<double> K((size_t)q, bp.front());
+    K.insert(K.end(), bp.begin(), bp.end());
+    K.insert(K.end(), (size_t)q, bp.back());
+    return K;
+}
+
+extern "C" gsx_status gsx_pmg_create(gsx_problem *fine, gsx_matrix *A, int32_t nlevels, const char *coarsening,
+                                     int32_t projection, int32_t coarse_operator, int32_t npre, int32_t npost,
+                                     double damping, double cgc_damping, int32_t precision, gsx_mg **out)
+{
+    if (!fine || !A || !out || !coarsening || nlevels < 1 || npre < 0 || npost < 0) return GSX_ERR_ARG;
+    gsx_ctx *c = fine->ctx;
+    if (fine->geo) return fail(c, GSX_ERR_ARG, "the finest problem must carry its own geometry");
+    if (A->n != fine->nfree) return fail(c, GSX_ERR_ARG, "matrix does not belong to the problem");
+    if (precision != GSX_FP64 && precision != GSX_MIXED) return fail(c, GSX_ERR_ARG, "unknown precision");
+    if (projection != 1 && projection != 2) return fail(c, GSX_ERR_ARG, "projection must be 1 or 2");
+    if (coarse_operator != 1 && coarse_operator != 2) return fail(c, GSX_ERR_ARG, "coarse operator must be 1 or 2");
+    if ((int)std::strlen(coarsening) != nlevels - 1)
+        return fail(c, GSX_ERR_ARG, "coarsening needs one of h, p, z per level transition (nlevels - 1)");
+    int npz = 0;
+    for (int i = 0; i < nlevels - 1; ++i) {
+        const char t = coarsening[i];
+        if (t != 'h' && t != 'p' && t != 'z') return fail(c, GSX_ERR_ARG, "coarsening: unknown type (h, p or z)");
+        npz += t != 'h';
+    }
+    if (projection == 1 && npz > 1)
+        return fail(c, GSX_ERR_ARG, "projection 1 allows only one p or z transition");
+    hipStream_t s = c->stream;
+    KScope ks(c, "pmg_setup");
+    const int npatch = fine->npatch, d = fine->d;
+    // level bases, finest first: knots per (patch, direction) and degree per patch
+    std::vector<std::vector<std::vector<double>>> knots((size_t)nlevels);
+    std::vector<std::vector<int>> deg((size_t)nlevels);
+    knots[(size_t)nlevels - 1] = fine->knots_h;
+    for (int k = 0; k < npatch; ++k) deg[(size_t)nlevels - 1].push_back(fine->hp[(size_t)k].p[0]);
+    for (int l = nlevels - 2; l >= 0; --l) {
+        const char t = coarsening[l];
+        knots[(size_t)l].resize((size_t)npatch * 3);
+        deg[(size_t)l].resize((size_t)npatch);
+        for (int k = 0; k < npatch; ++k) {
+            const int pf = deg[(size_t)l + 1][(size_t)k];
+            int pc = pf;
+            if (t != 'h') {
+                pc = projection == 1 ? 1 : pf - 1;
+                if (pc < 1 || pc >= pf) return fail(c, GSX_ERR_ARG, "p-multigrid: degree cannot be reduced further");
+            }
+            deg[(size_t)l][(size_t)k] = pc;
+            for (int j = 0; j < d; ++j) {
+                std::vector<double> U = knots[(size_t)l + 1][(size_t)3 * k + j];
+                if (t != 'p') {
+                    const int nel = (int)U.size() - 2 * pf - 1;
+                    if (nel % 2 != 0 || nel < 2)
+                        return fail(c, GSX_ERR_ARG,
+                                    "multigrid needs an even number of elements per direction on every level");
+                    std::vector<double> V(U.begin(), U.begin() + pf + 1);
+                    for (int e = 2; e < nel; e += 2) V.push_back(U[(size_t)(pf + e)]);
+                    V.insert(V.end(), U.end() - pf - 1, U.end());
+                    U.swap(V);
+                }
+                knots[(size_t)l][(size_t)3 * k + j] = pmg_knots(U, pc);
+            }
+        }
+    }
+    auto mg = std::make_unique<gsx_mg>();
+    mg->ctx = c;
+    mg->npre = npre;
+    mg->npost = npost;
+    mg->ncycles = 1;
+    mg->tau = damping;
+    mg->cgc = cgc_damping;
+    mg->precision = precision;
+    mg->L.resize((size_t)nlevels);
+    mg->fine_box_ok = scms_box_of(fine, mg->fine_box);
+    {
+        MGLevel &f = mg->L[(size_t)nlevels - 1];
+        f.n = A->n;
+        f.nnz = A->nnz;
+        f.lpr = A->lpr;
+        f.rowptr = A->rowptr;
+        f.col = A->col;
+        f.val = A->val;
+    }
+    // level problems over the fine geometry
+    for (int l = nlevels - 2; l >= 0; --l) {
+        std::vector<gsx_patch_desc> desc((size_t)npatch);
+        for (int k = 0; k < npatch; ++k) {
+            gsx_patch_desc &D = desc[(size_t)k];
+            std::memset(&D, 0, sizeof(D));
+            D.d = d;
+            for (int j = 0; j < d; ++j) {
+                D.p[j] = deg[(size_t)l][(size_t)k];
+                D.nknots[j] = (int)knots[(size_t)l][(size_t)3 * k + j].size();
+                D.knots_host[j] = knots[(size_t)l][(size_t)3 * k + j].data();
+            }
+            D.fkind = 1;
+        }
+        gsx_problem *cp = nullptr;
+        GSX_TRY(gsx_problem_create_geo(fine, npatch, desc.data(), (int)fine->dir_sides_h.size() / 2,
+                                       fine->dir_sides_h.empty() ? nullptr : fine->dir_sides_h.data(),
+                                       (int)fine->ifaces_h.size() / 8,
+                                       fine->ifaces_h.empty() ? nullptr : fine->ifaces_h.data(), &cp));
+        mg->L[(size_t)l].prob = cp;
+        mg->L[(size_t)l].n = cp->nfree;
+    }
+    // operators and transfers, finest first
+    TransferCache tcache;
+    for (int l = nlevels - 2; l >= 0; --l) {
+        MGLevel &C = mg->L[(size_t)l], &F = mg->L[(size_t)l + 1];
+        gsx_problem *fp = (l + 1 == nlevels - 1) ? fine : F.prob;
+        gsx_problem *cp = C.prob;
+        const char t = coarsening[l];
+        const bool galerkin = t == 'h' && coarse_operator == 2;
+        if (galerkin) {
+            if (!cp->have_pattern) GSX_TRY(pattern_build(cp));
+            C.rowptr = cp->d_rowptr;
+            C.col = cp->d_col;
+            C.nnz = cp->nnz;
+            C.val = dalloc(c, sizeof(double) * std::max<int64_t>(1, C.nnz));
+            if (!C.val) return fail(c, GSX_ERR_OOM, "device allocation (Galerkin product)");
+        } else {
+            // rediscretisation on the level's basis over the fine geometry
+            gsx_matrix *Al = nullptr;
+            GSX_TRY(gsx_assemble(cp, GSX_PATTERN_FULL | GSX_ASM_CSR_ONLY, GSX_FP64, &Al, nullptr));
+            C.rowptr = Al->rowptr;
+            C.col = Al->col;
+            C.val = Al->val;
+            C.nnz = Al->nnz;
+            gsx_matrix_destroy(Al);
+        }
+        C.lpr = gsx::spmv_lpr(C.n ? (double)C.nnz / C.n : 1.0);
+        KScope kt(c, "pmg_transfer_build");
+        if (t == 'h') {
+            // canonical embedding and its transpose
+            GSX_TRY(mg_knot_transfer(c, fp, cp, F.n, C, tcache, nullptr, nullptr));
+            GSX_TRY(mg_transpose_transfer(c, C, F.n));
+        } else {
+            // L2 projections with lumped mass: P = L_f^-1 M, R = L_c^-1 M^T
+            DevPtr Lf, Lc;
+            GSX_TRY(pmg_mixed(c, fp, cp, fine, C));
+            GSX_TRY(mg_transpose_transfer(c, C, F.n));
+            GSX_TRY(pmg_lumped(c, fp, fine, Lf));
+            GSX_TRY(pmg_lumped(c, cp, fine, Lc));
+            DevPtr errf = dalloc(c, sizeof(int));
+            if (!errf) return fail(c, GSX_ERR_OOM, "device allocation");
+            HIP_OK(c, hipMemsetAsync(errf.get(), 0, sizeof(int), s));
+            HIP_OK(c, gsx::launch_scale_rows(F.n, dp<int>(C.prp), dp<double>(Lf), dp<double>(C.pv), dp<int>(errf), s));
+            HIP_OK(c, gsx::launch_scale_rows(C.n, dp<int>(C.rrp), dp<double>(Lc), dp<double>(C.rv), dp<int>(errf), s));
+            int errh = 0;
+            HIP_OK(c, hipMemcpyAsync(&errh, errf.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+            HIP_OK(c, hipStreamSynchronize(s));
+            if (errh) return fail(c, GSX_ERR_ARG, "p-multigrid: a lumped mass entry is zero");
+        }
+        if (galerkin) {
+            DevPtr errf = dalloc(c, sizeof(int) * 2);
+            if (!errf) return fail(c, GSX_ERR_OOM, "device allocation");
+            HIP_OK(c, hipMemsetAsync(errf.get(), 0, sizeof(int) * 2, s));
+            GSX_TRY(mg_galerkin_sparse(c, C, F, d, dp<int>(errf)));
+            int errh[2] = {0, 0};
+            HIP_OK(c, hipMemcpyAsync(errh, errf.get(), sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+            HIP_OK(c, hipStreamSynchronize(s));
+            if (errh[0] || errh[1]) return fail(c, GSX_ERR_ARG, "p-multigrid: Galerkin product buffer overflow");
+        }
+    }
+    GSX_TRY(mg_finish(mg.get()));
+    *out = mg.release();
+    return GSX_OK;
+}
+
 extern "C" gsx_status gsx_mg_destroy(gsx_mg *mg)
 {
     if (!mg) return GSX_ERR_ARG;
@@ -3162,6 +3532,19 @@ extern "C" gsx_status gsx_mg_level_download(const gsx_mg *mg, int32_t lvl, int32
         if (pci) HIP_OK(c, hipMemcpyAsync(pci, l.pci.get(), sizeof(int) * l.pnnz, hipMemcpyDeviceToHost, s));
         if (pv) HIP_OK(c, hipMemcpyAsync(pv, l.pv.get(), sizeof(double) * l.pnnz, hipMemcpyDeviceToHost, s));
     }
+    HIP_OK(c, hipStreamSynchronize(s));
+    return GSX_OK;
+}
+
+extern "C" gsx_status gsx_mg_restriction_download(const gsx_mg *mg, int32_t lvl, int32_t *rp, int32_t *ci, double *v)
+{
+    if (!mg || lvl < 0 || lvl >= (int)mg->L.size() - 1) return GSX_ERR_ARG;
+    gsx_ctx *c = mg->ctx;
+    hipStream_t s = c->stream;
+    const MGLevel &l = mg->L[lvl];
+    if (rp) HIP_OK(c, hipMemcpyAsync(rp, l.rrp.get(), sizeof(int) * (l.n + 1), hipMemcpyDeviceToHost, s));
+    if (ci) HIP_OK(c, hipMemcpyAsync(ci, l.rci.get(), sizeof(int) * l.pnnz, hipMemcpyDeviceToHost, s));
+    if (v) HIP_OK(c, hipMemcpyAsync(v, l.rv.get(), sizeof(double) * l.pnnz, hipMemcpyDeviceToHost, s));
     HIP_OK(c, hipStreamSynchronize(s));
     return GSX_OK;
 }
@@ -4370,6 +4753,7 @@ extern "C" gsx_status gsx_problem_set_bc(gsx_problem *P, int32_t nbc, const gsx_
 {
     if (!P || nbc < 0 || (nbc > 0 && !bc)) return GSX_ERR_ARG;
     gsx_ctx *c = P->ctx;
+    if (P->geo) return fail(c, GSX_ERR_ARG, "boundary data need the problem's own geometry");
     hipStream_t s = c->stream;
     if (dir_values != GSX_DIR_HOMOGENEOUS && dir_values != GSX_DIR_L2PROJECTION &&
         dir_values != GSX_DIR_L2PROJECTION_INHOM && dir_values != GSX_DIR_INTERPOLATION)

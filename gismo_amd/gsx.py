@@ -126,6 +126,7 @@ SIGNATURES = {
     "gsx_ctx_set_elem_budget": [_vp, C.c_int64],
     "gsx_problem_create": [_vp, C.c_int32, C.POINTER(PatchDesc), C.c_int32, _i32p, C.c_int32,
                            _i32p, _pp],
+    "gsx_problem_create_geo": [_vp, C.c_int32, C.POINTER(PatchDesc), C.c_int32, _i32p, C.c_int32, _i32p, _pp],
     "gsx_problem_destroy": [_vp],
     "gsx_problem_info": [_vp, _i64p, _i64p, _i64p, _i64p],
     "gsx_problem_dofs": [_vp, _i32p],
@@ -165,6 +166,9 @@ SIGNATURES = {
     "gsx_mg_apply": [_vp, _vp, _vp],
     "gsx_mg_level_info": [_vp, C.c_int32, _i64p, _i64p, _i64p],
     "gsx_mg_level_download": [_vp, C.c_int32, _i32p, _i32p, _dp, _i32p, _i32p, _dp],
+    "gsx_pmg_create": [_vp, _vp, C.c_int32, C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                       C.c_double, C.c_int32, _pp],
+    "gsx_mg_restriction_download": [_vp, C.c_int32, _i32p, _i32p, _dp],
     "gsx_batch_create": [_vp, C.c_int32, C.c_int32, _pp],
     "gsx_batch_destroy": [_vp],
     "gsx_batch_add": [_vp, _vp, C.c_int32, C.c_int32, C.c_int32, C.c_double, _vp, _vp, C.c_double,
@@ -392,6 +396,43 @@ class Problem:
         _check(ctx.h, lib().gsx_problem_info(h, C.byref(nb), C.byref(nf), C.byref(ne), C.byref(ng)))
         self.nbasis, self.nfree, self.nelem, self.nglued = nb.value, nf.value, ne.value, ng.value
 
+    @classmethod
+    def with_geometry(cls, geo, bases, dir_sides=None, ifaces=None):
+        """gsx_problem_create_geo: the bases (objects with d, p, knots, one per
+        patch of geo) over the geometry, kappa and source of the Problem geo;
+        Dirichlet sides and interfaces default to geo's."""
+        self = cls.__new__(cls)
+        self.ctx = geo.ctx
+        self.src = None
+        self.geo = geo
+        n = len(bases)
+        arr = (PatchDesc * n)()
+        self._keep = []
+        for i, B in enumerate(bases):
+            o = arr[i]
+            o.d = B.d
+            for k in range(B.d):
+                kn = np.ascontiguousarray(B.knots[k], dtype=np.float64)
+                self._keep.append(kn)
+                o.p[k] = int(B.p[k])
+                o.nknots[k] = kn.size
+                o.knots_host[k] = kn.ctypes.data_as(_dp)
+            o.fkind = 1
+        src = geo.src
+        ds = np.ascontiguousarray(np.asarray(src.dir_sides if dir_sides is None else dir_sides,
+                                             dtype=np.int32).reshape(-1))
+        ifs = np.ascontiguousarray(np.asarray(src.ifaces if ifaces is None else ifaces, dtype=np.int32).reshape(-1))
+        self._keep += [ds, ifs]
+        h = C.c_void_p()
+        _check(self.ctx.h, lib().gsx_problem_create_geo(
+            geo.h, n, arr, ds.size // 2, ds.ctypes.data_as(_i32p) if ds.size else None, ifs.size // 8,
+            ifs.ctypes.data_as(_i32p) if ifs.size else None, C.byref(h)))
+        self.h = h
+        nb, nf, ne, ng = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        _check(self.ctx.h, lib().gsx_problem_info(h, C.byref(nb), C.byref(nf), C.byref(ne), C.byref(ng)))
+        self.nbasis, self.nfree, self.nelem, self.nglued = nb.value, nf.value, ne.value, ng.value
+        return self
+
     def dofs(self):
         out = np.zeros(self.nbasis, np.int32)
         _check(self.ctx.h, lib().gsx_problem_dofs(self.h, out.ctypes.data_as(_i32p)))
@@ -571,6 +612,46 @@ class Multigrid:
         self.h = h
         self.nlevels = nt + 1
         return self
+
+    @classmethod
+    def pmg(cls, problem, matrix, nlevels, coarsening=None, projection=2, coarse_operator=2, npre=1, npost=1,
+            damping=1.0, cgc_damping=1.0, precision=FP64):
+        """p-multigrid (gsx_pmg_create) with the transitions of
+        examples/pMultiGrid_example.cpp: coarsening[l] in 'h', 'p', 'z' from
+        level l to l + 1; the default is the example's (projection 2:
+        'h' * (nlevels - p) + 'z' * (p - 1), or 'z' * (nlevels - 1) when
+        nlevels <= p; projection 1: 'h' * (nlevels - 2) + 'z')."""
+        if coarsening is None:
+            p = int(problem.src.patches[0].p[0])
+            if projection == 1:
+                coarsening = "h" * (nlevels - 2) + "z"
+            elif nlevels > p:
+                coarsening = "h" * (nlevels - p) + "z" * (p - 1)
+            else:
+                coarsening = "z" * (nlevels - 1)
+        self = cls.__new__(cls)
+        self.ctx = problem.ctx
+        h = C.c_void_p()
+        _check(self.ctx.h, lib().gsx_pmg_create(problem.h, matrix.h, nlevels, coarsening.encode(), projection,
+                                                coarse_operator, npre, npost, damping, cgc_damping, precision,
+                                                C.byref(h)))
+        self.h = h
+        self.nlevels = nlevels
+        self.coarsening = coarsening
+        return self
+
+    def restriction(self, lvl):
+        """The restriction from level lvl + 1 to level lvl (scipy CSR)."""
+        import scipy.sparse as sp
+        n, pnnz, nf = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(self.ctx.h, lib().gsx_mg_level_info(self.h, lvl, C.byref(n), None, C.byref(pnnz)))
+        _check(self.ctx.h, lib().gsx_mg_level_info(self.h, lvl + 1, C.byref(nf), None, None))
+        rp = np.zeros(n.value + 1, np.int32)
+        ci = np.zeros(max(1, pnnz.value), np.int32)
+        va = np.zeros(max(1, pnnz.value))
+        _check(self.ctx.h, lib().gsx_mg_restriction_download(self.h, lvl, rp.ctypes.data_as(_i32p),
+                                                             ci.ctypes.data_as(_i32p), va.ctypes.data_as(_dp)))
+        return sp.csr_matrix((va[:pnnz.value], ci[:pnnz.value], rp), shape=(n.value, nf.value))
 
     def set_level_smoother(self, lvl, smoother, damping=1.0, param=0.0):
         """gsMultiGridOp::setSmoother(lvl, op): one level's smoother."""

@@ -154,6 +154,19 @@ gsx_status gsx_problem_create(gsx_ctx *ctx, int32_t npatch,
                               const int32_t *dir_sides_host, int32_t nif,
                               const int32_t *ifaces_host, gsx_problem **out);
 gsx_status gsx_problem_destroy(gsx_problem *prob);
+/* A problem discretised with the bases of `patches` (degree and knot
+ * vectors; control points, kappa and sources are not read) over the
+ * geometry map, kappa and source of `geo`: gsCDRAssembler(mp, bases[i], ...)
+ * of examples/pMultiGrid_example.cpp:445-460, where a level's basis differs
+ * from the geometry's in degree or mesh.  Each patch covers the parameter box
+ * of geo's patch; the quadrature follows the discretisation basis (q + 1
+ * Gauss points per direction on its elements) and the geometry is evaluated
+ * in its own basis at every point.  gsx_assemble takes the element path;
+ * geo must outlive the problem.  Boundary data and error norms need the
+ * problem's own geometry (GSX_ERR_ARG). */
+gsx_status gsx_problem_create_geo(gsx_problem *geo, int32_t npatch, const gsx_patch_desc *patches,
+                                  int32_t ndir, const int32_t *dir_sides_host, int32_t nif,
+                                  const int32_t *ifaces_host, gsx_problem **out);
 /* nbasis: all basis functions (summed over patches, before gluing);
  * nfree: free DOFs; nelem: elements; nglued: glued space dimension. */
 gsx_status gsx_problem_info(const gsx_problem *prob, int64_t *nbasis,
@@ -373,6 +386,32 @@ gsx_status gsx_mg_create_transfers(gsx_matrix *A, int32_t ntransfer, const int32
                                    const double *const *pval, int32_t npre, int32_t npost,
                                    int32_t ncycles, double damping, double cgc_damping,
                                    int32_t precision, gsx_mg **out);
+/* p-multigrid as examples/pMultiGrid_example.cpp:296-560 sets it up, on the
+ * fine problem and its assembled matrix A (nlevels levels, level
+ * nlevels - 1 = A).  coarsening[l] (l = 0 .. nlevels - 2) is the transition
+ * from level l to level l + 1 (--Coarsening): 'h' a coarser mesh (every other
+ * knot, same degree), 'p' a lower degree (same mesh), 'z' both.  projection
+ * (--Projection): 1 lowers the degree straight to 1 (one p or z transition),
+ * 2 by one.  Level bases keep simple interior knots (the smoothness of
+ * gsMultiBasis::degreeIncrease over them).  Operators: p and z levels are
+ * assembled on their own basis over the fine geometry (gsx_problem_create_geo);
+ * h levels too when coarse_operator = 1, or P^T A P when coarse_operator = 2
+ * (--CoarseOperator).  Transfers (--Lumping 1): p and z, L2 projections
+ * with lumped mass, P = L_f^-1 M_fc and R = L_c^-1 M_fc^T with
+ * M_fc(i, j) = int phi_i psi_j on the finer basis' elements and
+ * L(i) = int phi_i (assembleMixedMass/assembleLumpedMass, :803-880);
+ * h, the canonical embedding and its transpose.  The coarsest level is
+ * solved exactly (dense FP64 inverse); V-cycles with the smoothers of
+ * gsx_mg_set_smoother / gsx_mg_set_level_smoother (damped Jacobi by
+ * default).  fine must outlive the call. */
+gsx_status gsx_pmg_create(gsx_problem *fine, gsx_matrix *A, int32_t nlevels, const char *coarsening,
+                          int32_t projection, int32_t coarse_operator, int32_t npre, int32_t npost,
+                          double damping, double cgc_damping, int32_t precision, gsx_mg **out);
+/* Host copy of the restriction of level lvl (0 <= lvl < nlevels - 1): CSR
+ * with one row per free DOF of level lvl, columns of level lvl + 1 (P^T for
+ * h transitions, L_c^-1 M_fc^T for p and z); any pointer may be NULL. */
+gsx_status gsx_mg_restriction_download(const gsx_mg *mg, int32_t lvl, int32_t *rowptr_host,
+                                       int32_t *col_host, double *val_host);
 gsx_status gsx_mg_destroy(gsx_mg *mg);
 /* Smoother of every level but the coarsest (the sweeps npre/npost of
  * gsx_mg_create).  GSX_SMOOTH_JACOBI: gsJacobiOp::step with the damping of
