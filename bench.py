@@ -125,6 +125,9 @@ def main():
                          "serve: Poisson-arrival request stream through the continuous-batching server")
     ap.add_argument("--smoother", choices=["scms", "jacobi", "chebyshev", "gauss_seidel"], default="scms",
                     help="mg-cube: multigrid smoother (scms: subspace corrected mass smoother)")
+    ap.add_argument("--mg-kind", choices=["h", "p"], default="h",
+                    help="mg: h-multigrid (knot coarsening) or p-multigrid (the coarsening of "
+                         "examples/pMultiGrid_example.cpp: z transitions down to degree 1, then h)")
     ap.add_argument("--load", type=float, default=0.7, help="serve: arrival rate as a fraction of capacity")
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--p", type=int, default=3)
@@ -383,20 +386,29 @@ def _common(args, metric_value, ms, dtype, workload, extra):
 def bench_mg(args):
     """16 degree-3 patches tiling [0,4]^2, n elements per patch direction,
     C0-glued; one step = assemble + multigrid setup (knot-coarsened
-    hierarchy, Galerkin levels, dense coarse inverse) + CG preconditioned by
-    V(2,2) with damped Jacobi (omega 0.6) to relres tol."""
+    hierarchy, Galerkin levels, dense coarse inverse; with --mg-kind p the
+    p-multigrid hierarchy: levels of lower degree assembled over the fine
+    geometry, lumped-mass transfers) + CG preconditioned by V(2,2) with
+    damped Jacobi (omega 0.6) to relres tol."""
     from gismo_amd import gsx, problems
     n = 256 if args.n is None else args.n
     tol = 1e-8 if args.tol is None else args.tol
     levels = int(round(np.log2(n))) - 3   # coarsest 16 elements per patch direction
     prob = problems.square_grid(4, 4, n=n, p=args.p)
+    pmg = args.mg_kind == "p"
+    coarsening = ("h" * (levels - args.p) + "z" * (args.p - 1)) if levels > args.p else "z" * (levels - 1)
+
+    def make_mg(A, prec=gsx.FP64):
+        if pmg:
+            return gsx.Multigrid.pmg(gp, A, levels, coarsening, npre=2, npost=2, damping=0.6, precision=prec)
+        return gsx.Multigrid(gp, A, levels, 2, 2, 0.6, precision=prec)
     ctx = gsx.Context(0)
     gp = gsx.Problem(ctx, prob)
     ndofs = gp.nglued
 
     # tight FP64 reference (outside the timing)
     A, b = gp.assemble(gsx.PATTERN_FULL, gsx.FP64)
-    mg = gsx.Multigrid(gp, A, levels, 2, 2, 0.6)
+    mg = make_mg(A)
     xr = ctx.array(gp.nfree).zero()
     it_ref, rel_ref, _ = gsx.mg_cg_solve(A, mg, b, xr, tol=1e-13, maxit=500)
     xref = xr.download()
@@ -410,7 +422,7 @@ def bench_mg(args):
     for name, prec in (("fp64", gsx.FP64), ("mixed", gsx.MIXED)):
         def step():
             A, b = gp.assemble(gsx.PATTERN_FULL | gsx.ASM_REBUILD_PATTERN | gsx.ASM_KEEP_ELEMENTS, prec)
-            mg = gsx.Multigrid(gp, A, levels, 2, 2, 0.6, precision=prec)
+            mg = make_mg(A, prec)
             it, rel, ok = gsx.mg_cg_solve(A, mg, b, x, tol=tol, maxit=500)
             ctx.synchronize()
             mg.close(); A.close(); b.close()
@@ -443,13 +455,14 @@ def bench_mg(args):
     ctx.reset_timing()
     x.zero()
     A, b = gp.assemble(gsx.PATTERN_FULL | gsx.ASM_REBUILD_PATTERN | gsx.ASM_KEEP_ELEMENTS, gsx.MIXED)
-    mg = gsx.Multigrid(gp, A, levels, 2, 2, 0.6, precision=gsx.MIXED)
+    mg = make_mg(A, gsx.MIXED)
     gsx.mg_cg_solve(A, mg, b, x, tol=tol, maxit=500)
     ctx.synchronize()
     phases = {}
     for k in ("pattern_count", "pattern_scan", "pattern_fill", "elem2d", "gather", "mg_setup", "mg_levels",
               "mg_elem", "mg_transfer_build", "mg_galerkin", "mg_coarse_inverse", "mg_smooth", "mg_transfer",
-              "mg_coarse", "cg_init", "spmv_f64", "cg_vec"):
+              "mg_coarse", "cg_init", "spmv_f64", "cg_vec", "pmg_setup", "geo_elem", "pmg_mixed_elem",
+              "pmg_mixed_gather", "pmg_transfer_build"):
         nl, ms = ctx.kernel_stats(k)
         if nl:
             phases[k] = round(ms, 3)
@@ -465,6 +478,7 @@ def bench_mg(args):
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "algorithmic_bytes_per_launch": spmv_bytes, "avg_launch_ms": avg_ms},
         "config": {"dofs": ndofs, "free_dofs": gp.nfree, "nnz": nnz, "levels": levels,
+                   "hierarchy": f"p-multigrid, coarsening {coarsening}" if pmg else "h-multigrid",
                    "cycle": "V(2,2), damped Jacobi 0.6, exact FP64 coarse solve"},
         "fp64": results["fp64"], "mixed": mixed,
         "speedup_mixed_vs_fp64": results["fp64"]["ms"] / mixed["ms"],
@@ -475,7 +489,7 @@ def bench_mg(args):
         extra["cpu_baseline"] = cpu_baseline_mg(16 if args.cpu_sample is None else args.cpu_sample, args.p)
     out = _common(args, mixed["dofs_per_s"], mixed["ms"], "f32+f64",
                   f"2D multipatch: 16 degree-{args.p} patches on [0,4]^2, {n}x{n} elements per patch, "
-                  f"assemble + MG-preconditioned CG to relres {tol:g}", extra)
+                  f"assemble + {'p-' if pmg else ''}MG-preconditioned CG to relres {tol:g}", extra)
     x.close()
     gp.close()
     ctx.close()
