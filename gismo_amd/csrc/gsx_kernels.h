@@ -296,6 +296,9 @@ hipError_t launch_mix_gather(int d, const PatchDev *ups, const PatchDev *vps, co
 // per free DOF: sum of the element vector entries of its functions
 hipError_t launch_vec_gather(int d, const PatchDev *ups, const int *occ_ptr, const int *occ_patch,
                              const int64_t *occ_local, int nrows, const double *Fbuf, double *out, hipStream_t s);
+// out[i] = in[idx[i]]; x[idx[i]] += scale z[i] (distinct indices)
+hipError_t launch_gather_idx(int n, const int *idx, const double *in, double *out, hipStream_t s);
+hipError_t launch_scatter_add_idx(int n, const int *idx, const double *z, double scale, double *x, hipStream_t s);
 // val(row r) *= 1 / L(r); err[0] = 1 on L(r) == 0
 hipError_t launch_scale_rows(int nrows, const int *rowptr, const double *L, double *val, int *err, hipStream_t s);
 

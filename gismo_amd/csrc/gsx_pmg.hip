@@ -513,9 +513,38 @@ __global__ void k_scale_rows(int nrows, const int *__restrict__ rowptr, const do
     for (int k = rowptr[r]; k < rowptr[r + 1]; ++k) val[k] = inv * val[k];
 }
 
+__global__ void k_gather_idx(int n, const int *__restrict__ idx, const double *__restrict__ in,
+                             double *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[idx[i]];
+}
+
+// x[idx[i]] += scale * z[i]; the indices of one call are distinct
+__global__ void k_scatter_add_idx(int n, const int *__restrict__ idx, const double *__restrict__ z, double scale,
+                                  double *__restrict__ x)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[idx[i]] += scale * z[i];
+}
+
 inline dim3 grid1(int64_t n, int b) { return dim3((unsigned)((n + b - 1) / b)); }
 
 }  // namespace
+
+hipError_t launch_gather_idx(int n, const int *idx, const double *in, double *out, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_idx, grid1(n, 256), dim3(256), 0, s, n, idx, in, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_add_idx(int n, const int *idx, const double *z, double scale, double *x, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_add_idx, grid1(n, 256), dim3(256), 0, s, n, idx, z, scale, x);
+    return hipGetLastError();
+}
 
 hipError_t launch_geo_elem(int d, const PatchDev *ups, const PatchDev *gps, int ip, int64_t e0, int64_t e1,
                            const GaussTab &gt, double *Kbuf, double *Fbuf, double *detmin, int64_t kshift,

@@ -2569,6 +2569,45 @@ gsx_status gsx_l2_error(gsx_problem *P, const double *sol, double *err)
 // 137-178); coarse operators are Galerkin products P^T A P computed on the
 // device; the coarsest level is solved exactly with a dense FP64 inverse.
 // ===========================================================================
+// Host copy of what the component decomposition of the subspace corrected
+// mass smoother needs from a problem: patch sizes, knots and the DOF map.
+struct PatchLayout {
+    int d = 0, npatch = 0, nfree = 0;
+    std::vector<HostPatch> hp;
+    std::vector<std::vector<double>> knots;   // npatch * 3
+    std::vector<int> dofs;
+    std::vector<int64_t> boff;
+};
+
+static PatchLayout layout_of(const gsx_problem *P)
+{
+    PatchLayout L;
+    L.d = P->d;
+    L.npatch = P->npatch;
+    L.nfree = P->nfree;
+    L.hp = P->hp;
+    L.knots = P->knots_h;
+    L.dofs = P->dofs;
+    L.boff = P->boff;
+    return L;
+}
+
+// One component of the subspace corrected mass smoother: its free DOFs
+// (global indices, component order) and local operator, the smoother of a
+// box damped by tau, or the exact inverse of A restricted to the component.
+struct ScmsComp {
+    int n = 0, dim = 0;
+    DevPtr idx;
+    std::shared_ptr<gsx_fd> F;
+    double tau = 1.0;
+    DevPtr Ainv;   // n x n (direct components)
+};
+struct ScmsSplit {
+    std::vector<ScmsComp> comps;
+    int nmax = 0;
+    DevPtr rc, zc;
+};
+
 struct MGLevel {
     gsx_problem *prob = nullptr;       // coarse levels own their problem
     int n = 0;
@@ -2588,6 +2627,8 @@ struct MGLevel {
     std::vector<int> gs_fwd_off, gs_rev_off;
     // subspace corrected mass smoother of the level's basis and its damping
     std::shared_ptr<gsx_fd> scms;
+    // several components (patches, interfaces, vertices): section per component
+    std::shared_ptr<ScmsSplit> split;
     double scms_tau = 0.0;
     // this level's smoother (levels >= 1): kind, damping of Jacobi/Richardson,
     // Chebyshev interval ratio
@@ -2629,6 +2670,9 @@ struct gsx_mg {
     int precision = GSX_FP64;
     int npad = 0;
     double scms_saved = 0.0;           // SCMS damping kept for finer levels
+    std::vector<double> scms_saved_comp;   // the same per component (several components)
+    PatchLayout fine_layout;           // finest level's patches and DOF map
+    bool fine_layout_ok = false;
     DevPtr Dinv;
     DevPtr zin, zout;                  // finest FP64 staging for the mixed cycle
     ScmsBox fine_box;                  // free box of a single-patch finest level
@@ -2955,6 +2999,8 @@ extern "C" gsx_status gsx_mg_create(gsx_problem *fine, gsx_matrix *A, int32_t nl
     mg->precision = precision;
     mg->L.resize(nlevels);
     mg->fine_box_ok = scms_box_of(fine, mg->fine_box);
+    mg->fine_layout = layout_of(fine);
+    mg->fine_layout_ok = true;
     // finest level = the given matrix
     {
         MGLevel &f = mg->L[nlevels - 1];
@@ -3402,6 +3448,8 @@ extern "C" gsx_status gsx_pmg_create(gsx_problem *fine, gsx_matrix *A, int32_t n
     mg->precision = precision;
     mg->L.resize((size_t)nlevels);
     mg->fine_box_ok = scms_box_of(fine, mg->fine_box);
+    mg->fine_layout = layout_of(fine);
+    mg->fine_layout_ok = true;
     {
         MGLevel &f = mg->L[(size_t)nlevels - 1];
         f.n = A->n;
@@ -3598,6 +3646,30 @@ static gsx_status smooth64(gsx_mg *mg, MGLevel &V, double *&x, double *&t, const
                 else
                     HIP_OK(c, gsx::launch_gs_rows_f64(off[l + 1] - off[l], rows + off[l], rp, ci, va, b, x, s));
             }
+        if (m > 0) zero = false;
+        return GSX_OK;
+    }
+    if (V.smoother == GSX_SMOOTH_SCMS && V.split) {
+        // x += sum_c T_c op_c T_c^T (b - A x): the components partition the
+        // DOFs, so their updates do not overlap
+        if (m > 0 && zero) HIP_OK(c, gsx::launch_fill_f64(x, V.n, 0.0, s));
+        double *r = dp<double>(V.r);
+        ScmsSplit &S = *V.split;
+        double *rc = dp<double>(S.rc), *zc = dp<double>(S.zc);
+        for (int k = 0; k < m; ++k) {
+            HIP_OK(c, residual(x, r));
+            for (const ScmsComp &q : S.comps) {
+                HIP_OK(c, gsx::launch_gather_idx(q.n, dp<int>(q.idx), r, rc, s));
+                if (q.F) {
+                    const gsx_fd *F = q.F.get();
+                    HIP_OK(c, fd_apply<double>(F, rc, 0, F->m[0], (int64_t)F->m[0] * F->m[1], zc, 0, F->m[0],
+                                               (int64_t)F->m[0] * F->m[1], nullptr, s));
+                } else {
+                    HIP_OK(c, gsx::launch_dense_gemv_f64(q.n, q.n, dp<double>(q.Ainv), rc, zc, s));
+                }
+                HIP_OK(c, gsx::launch_scatter_add_idx(q.n, dp<int>(q.idx), zc, q.tau, x, s));
+            }
+        }
         if (m > 0) zero = false;
         return GSX_OK;
     }
@@ -3807,6 +3879,236 @@ static gsx_status mg_apply_dev(gsx_mg *mg, const double *r, double *z)
     return GSX_OK;
 }
 
+// Components of setupSubspaceCorrectedMassSmoother
+// (examples/multiGrid_example.cpp:407-567) with conforming interfaces:
+// gsBoxTopology::allComponents(true) (src/gsCore/gsBoxTopology.cpp:294-340)
+// splits every patch into its 3^d sub-boxes (interior, sides, edges,
+// corners); sub-boxes of glued patches with the same corners are one
+// component, and all corners are one.  A component's DOFs are the free
+// functions of its sub-boxes whose index is interior in the sub-box's open
+// directions (componentBasis_withIndices with no_lower,
+// src/gsCore/gsMultiBasis.hpp:262-328), the first patch's in its
+// lexicographic order, then the others' not yet listed.  Patch interiors get
+// the smoother of the patch basis (sigma, alpha = 0, beta = 1), faces of a
+// 3D patch that of the face basis with alpha = p/h, beta = h/p (h the
+// patch's smallest knot span, p its degree); all others the exact solve of
+// A restricted to the component (makeSparseCholeskySolver).
+struct CompDesc {
+    int dim = 0, patch = 0;
+    std::vector<int> idx;
+    std::vector<char> seen;
+    bool box = false;
+    ScmsBox B;
+    int boxsize = 0;
+    double alpha = 0.0, beta = 1.0;
+};
+
+static std::vector<CompDesc> scms_components(const PatchLayout &L)
+{
+    const int d = L.d;
+    int ncode = 1;
+    for (int j = 0; j < d; ++j) ncode *= 3;
+    std::map<std::pair<int, std::vector<int>>, size_t> where;
+    std::vector<CompDesc> comps;
+    for (int k = 0; k < L.npatch; ++k) {
+        const HostPatch &h = L.hp[(size_t)k];
+        for (int code = 0; code < ncode; ++code) {
+            int loc[3] = {1, 1, 1}, dim = 0;   // per direction 0: first index, 1: interior, 2: last
+            for (int j = 0, cc = code; j < d; ++j, cc /= 3) {
+                loc[j] = cc % 3;
+                dim += loc[j] == 1;
+            }
+            // corner functions in the sub-box's closure identify it across patches
+            std::vector<int> key;
+            for (int cn = 0; cn < (1 << d); ++cn) {
+                bool in = true;
+                int64_t m = 0, stride = 1;
+                for (int j = 0; j < d; ++j) {
+                    const int last = (cn >> j) & 1;
+                    if ((loc[j] == 0 && last) || (loc[j] == 2 && !last)) in = false;
+                    m += (last ? h.nb[j] - 1 : 0) * stride;
+                    stride *= h.nb[j];
+                }
+                if (in) key.push_back(L.dofs[(size_t)(L.boff[(size_t)k] + m)]);
+            }
+            std::sort(key.begin(), key.end());
+            if (dim == d) key.insert(key.begin(), -1 - k);   // one interior per patch
+            if (dim == 0) key.clear();                       // all corners are one component
+            auto ins = where.emplace(std::make_pair(dim, key), comps.size());
+            if (ins.second) {
+                CompDesc cd;
+                cd.dim = dim;
+                cd.patch = k;
+                cd.seen.assign((size_t)L.nfree, 0);
+                cd.box = dim == d || (dim == d - 1 && dim >= 2);
+                if (cd.box) {
+                    double hmin = 1e300;
+                    int pmax = 0;
+                    for (int j = 0; j < d; ++j) {
+                        const std::vector<double> &U = L.knots[(size_t)3 * k + j];
+                        for (size_t i = 0; i + 1 < U.size(); ++i)
+                            if (U[i + 1] > U[i]) hmin = std::min(hmin, U[i + 1] - U[i]);
+                        pmax = std::max(pmax, h.p[j]);
+                    }
+                    cd.B.d = dim;
+                    cd.boxsize = 1;
+                    for (int j = 0, o = 0; j < d; ++j) {
+                        if (loc[j] != 1) continue;
+                        cd.B.p[o] = h.p[j];
+                        cd.B.nb[o] = h.nb[j];
+                        cd.B.lo[o] = 1;
+                        cd.B.hi[o] = h.nb[j] - 2;
+                        cd.B.knots[o] = L.knots[(size_t)3 * k + j];
+                        cd.boxsize *= std::max(0, h.nb[j] - 2);
+                        ++o;
+                    }
+                    if (dim < d) {
+                        cd.alpha = pmax / hmin;
+                        cd.beta = hmin / pmax;
+                    }
+                }
+                comps.push_back(std::move(cd));
+            }
+            CompDesc &cd = comps[ins.first->second];
+            int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+            for (int j = 0; j < d; ++j) {
+                lo[j] = loc[j] == 0 ? 0 : loc[j] == 2 ? h.nb[j] - 1 : 1;
+                hi[j] = loc[j] == 0 ? 0 : loc[j] == 2 ? h.nb[j] - 1 : h.nb[j] - 2;
+            }
+            for (int z = lo[2]; z <= hi[2]; ++z)
+                for (int y = lo[1]; y <= hi[1]; ++y)
+                    for (int x = lo[0]; x <= hi[0]; ++x) {
+                        const int64_t m = x + (int64_t)h.nb[0] * (y + (int64_t)h.nb[1] * z);
+                        const int g = L.dofs[(size_t)(L.boff[(size_t)k] + m)];
+                        if (g < L.nfree && !cd.seen[(size_t)g]) {
+                            cd.seen[(size_t)g] = 1;
+                            cd.idx.push_back(g);
+                        }
+                    }
+        }
+    }
+    // highest dimension first, as allComponents lists them
+    std::stable_sort(comps.begin(), comps.end(), [](const CompDesc &a, const CompDesc &b) { return a.dim > b.dim; });
+    for (CompDesc &cd : comps) std::vector<char>().swap(cd.seen);
+    return comps;
+}
+
+// The components' operators on level l: boxes get their smoother damped by
+// 1 / lambda, lambda from ten power iterations of S_c A_cc from
+// v0_i = 1 / (1 + i mod 5) (estimateLargestEigenvalueOfPreconditionedSystem,
+// src/gsSolver/gsPreconditioner.h:121-132), kept per component for the finer
+// levels under the example's rule (level above nlevels - 5, more elements
+// than the degree in every direction of the component's basis); the others
+// the dense inverse of A_cc.
+static gsx_status scms_split_build(gsx_mg *mg, int l, const PatchLayout &L, double sigma)
+{
+    gsx_ctx *c = mg->ctx;
+    hipStream_t s = c->stream;
+    const int nl = (int)mg->L.size();
+    MGLevel &V = mg->L[l];
+    if (L.nfree != V.n) return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: level size mismatch");
+    std::vector<CompDesc> comps = scms_components(L);
+    if (mg->scms_saved_comp.size() != comps.size()) mg->scms_saved_comp.assign(comps.size(), 0.0);
+    std::vector<int> rp((size_t)V.n + 1), ci((size_t)V.nnz);
+    std::vector<double> va((size_t)V.nnz);
+    HIP_OK(c, hipMemcpyAsync(rp.data(), V.rowptr.get(), sizeof(int) * (V.n + 1), hipMemcpyDeviceToHost, s));
+    HIP_OK(c, hipMemcpyAsync(ci.data(), V.col.get(), sizeof(int) * V.nnz, hipMemcpyDeviceToHost, s));
+    HIP_OK(c, hipMemcpyAsync(va.data(), V.val.get(), sizeof(double) * V.nnz, hipMemcpyDeviceToHost, s));
+    HIP_OK(c, hipStreamSynchronize(s));
+    auto S = std::make_shared<ScmsSplit>();
+    std::vector<int> pos((size_t)V.n, -1);
+    int covered = 0;
+    for (size_t i = 0; i < comps.size(); ++i) {
+        const CompDesc &cd = comps[i];
+        const int n = (int)cd.idx.size();
+        if (n == 0) continue;
+        covered += n;
+        ScmsComp q;
+        q.n = n;
+        q.dim = cd.dim;
+        q.idx = dalloc(c, sizeof(int) * n);
+        if (!q.idx) return fail(c, GSX_ERR_OOM, "device allocation (smoother components)");
+        HIP_OK(c, hipMemcpyAsync(q.idx.get(), cd.idx.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+        if (cd.box) {
+            if (n != cd.boxsize)
+                return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: a patch or face component is not a "
+                                            "box of free functions (conforming interfaces only)");
+            std::unique_ptr<gsx_fd> F;
+            GSX_TRY(scms_build(c, cd.B, sigma, cd.alpha, cd.beta, F));
+            if (F->size != n) return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: component size mismatch");
+            q.F = std::shared_ptr<gsx_fd>(F.release());
+            double &saved = mg->scms_saved_comp[i];
+            if (saved == 0.0) {
+                std::vector<double> xh((size_t)n);
+                for (int j = 0; j < n; ++j) xh[(size_t)j] = 1.0 / (1.0 + (double)(j % 5));
+                DevPtr cv = dalloc(c, sizeof(double) * n), cw = dalloc(c, sizeof(double) * n);
+                DevPtr gv = dalloc(c, sizeof(double) * V.n), gt = dalloc(c, sizeof(double) * V.n);
+                if (!cv || !cw || !gv || !gt) return fail(c, GSX_ERR_OOM, "device allocation (power iteration)");
+                const gsx_fd *Fp = q.F.get();
+                for (int it = 0; it < 10; ++it) {
+                    double nrm = 0.0;
+                    for (double v : xh) nrm += v * v;
+                    nrm = std::sqrt(nrm);
+                    for (double &v : xh) v /= nrm;
+                    HIP_OK(c, hipMemcpyAsync(cv.get(), xh.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+                    HIP_OK(c, gsx::launch_fill_f64(dp<double>(gv), V.n, 0.0, s));
+                    HIP_OK(c, gsx::launch_scatter_add_idx(n, dp<int>(q.idx), dp<double>(cv), 1.0, dp<double>(gv), s));
+                    HIP_OK(c, gsx::launch_spmv_f64(V.lpr, V.n, dp<int>(V.rowptr), dp<int>(V.col), dp<double>(V.val),
+                                                   dp<double>(gv), dp<double>(gt), nullptr, nullptr, 0, nullptr, s));
+                    HIP_OK(c, gsx::launch_gather_idx(n, dp<int>(q.idx), dp<double>(gt), dp<double>(cw), s));
+                    HIP_OK(c, fd_apply<double>(Fp, dp<double>(cw), 0, Fp->m[0], (int64_t)Fp->m[0] * Fp->m[1],
+                                               dp<double>(cv), 0, Fp->m[0], (int64_t)Fp->m[0] * Fp->m[1], nullptr, s));
+                    HIP_OK(c, hipMemcpyAsync(xh.data(), cv.get(), sizeof(double) * n, hipMemcpyDeviceToHost, s));
+                    HIP_OK(c, hipStreamSynchronize(s));
+                }
+                double lam = 0.0;
+                for (double v : xh) lam += v * v;
+                lam = std::sqrt(lam);
+                if (!(lam > 0.0)) return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: zero power iterate");
+                q.tau = 1.0 / lam;
+                bool keep = l > nl - 5;
+                for (int j = 0; j < cd.B.d; ++j) {
+                    int nel = 0;
+                    for (size_t t = 0; t + 1 < cd.B.knots[j].size(); ++t) nel += cd.B.knots[j][t + 1] > cd.B.knots[j][t];
+                    keep = keep && nel > cd.B.p[j];
+                }
+                if (keep) saved = q.tau;
+            } else {
+                q.tau = saved;
+            }
+        } else {
+            if (n > 2048) return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: a lower-dimensional "
+                                                      "component has more than 2048 DOFs (dense solve)");
+            for (int j = 0; j < n; ++j) pos[(size_t)cd.idx[(size_t)j]] = j;
+            std::vector<double> Acc((size_t)n * n, 0.0), X((size_t)n * n, 0.0);
+            for (int a = 0; a < n; ++a) {
+                const int r = cd.idx[(size_t)a];
+                for (int t = rp[(size_t)r]; t < rp[(size_t)r + 1]; ++t) {
+                    const int b = pos[(size_t)ci[(size_t)t]];
+                    if (b >= 0) Acc[(size_t)a * n + b] = va[(size_t)t];
+                }
+                X[(size_t)a * n + a] = 1.0;
+            }
+            for (int j = 0; j < n; ++j) pos[(size_t)cd.idx[(size_t)j]] = -1;
+            if (!gsx::dense::lu_solve(n, Acc, X, n))
+                return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: singular component matrix");
+            q.Ainv = dalloc(c, sizeof(double) * (size_t)n * n);
+            if (!q.Ainv) return fail(c, GSX_ERR_OOM, "device allocation (component inverse)");
+            HIP_OK(c, hipMemcpyAsync(q.Ainv.get(), X.data(), sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice, s));
+        }
+        S->nmax = std::max(S->nmax, n);
+        S->comps.push_back(std::move(q));
+    }
+    if (covered != V.n) return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: components do not cover the DOFs");
+    S->rc = dalloc(c, sizeof(double) * std::max(1, S->nmax));
+    S->zc = dalloc(c, sizeof(double) * std::max(1, S->nmax));
+    if (!S->rc || !S->zc) return fail(c, GSX_ERR_OOM, "device allocation (smoother)");
+    HIP_OK(c, hipStreamSynchronize(s));
+    V.split = S;
+    V.scms.reset();
+    return GSX_OK;
+}
+
 // Set-up of level l's smoother (1 <= l < nlevels).  GSX_SMOOTH_SCMS: the
 // subspace corrected mass smoother of setupSubspaceCorrectedMassSmoother
 // (examples/multiGrid_example.cpp:407-560) for one patch with every side
@@ -3837,9 +4139,20 @@ static gsx_status mg_setup_level_smoother(gsx_mg *mg, int l, int kind, double pa
         }
         bool alldir = ok;
         for (int k = 0; ok && k < B.d; ++k) alldir = alldir && B.lo[k] == 1 && B.hi[k] == B.nb[k] - 2;
-        if (!alldir)
-            return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: single-patch problems with Dirichlet "
-                                        "conditions on every side only, on a hierarchy built by gsx_mg_create");
+        if (!alldir) {
+            // several components: patches, interfaces, sides without a
+            // Dirichlet condition, edges, corners
+            if (mg->precision == GSX_MIXED)
+                return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother with several components: FP64 "
+                                            "cycles only");
+            if (l == nl - 1 ? !mg->fine_layout_ok : !V.prob)
+                return fail(c, GSX_ERR_ARG, "subspace corrected mass smoother: needs a hierarchy built from a "
+                                            "problem (gsx_mg_create or gsx_pmg_create)");
+            GSX_TRY(scms_split_build(mg, l, l == nl - 1 ? mg->fine_layout : layout_of(V.prob), sigma));
+            V.lmax = 0.0;
+            V.smoother = kind;
+            return GSX_OK;
+        }
         std::unique_ptr<gsx_fd> F;
         gsx_status e = scms_build(c, B, sigma, 0.0, 1.0, F);
         if (e != GSX_OK) return e;

@@ -247,6 +247,19 @@ def square_grid(nx=4, ny=4, n=16, p=3, r=1.0):
     return Problem(patches, dir_sides, ifaces, name=f"grid{nx}x{ny}-p{p}-n{n}")
 
 
+def two_cubes(n=4, p=2):
+    """[0,2] x [0,1]^2 as two glued unit cubes, n elements per direction
+    each (patch 0's east side on patch 1's west side, tangential directions
+    1 and 2 unchanged), homogeneous Dirichlet on the outer boundary,
+    f = 3 pi^2 sin sin sin."""
+    a = box_patch(3, p, n)
+    b = box_patch(3, p, n, lo=[1.0, 0.0, 0.0], hi=[2.0, 1.0, 1.0])
+    for P in (a, b):
+        P.fkind = 1
+    dir_sides = [(0, s) for s in (1, 3, 4, 5, 6)] + [(1, s) for s in (2, 3, 4, 5, 6)]
+    return Problem([a, b], dir_sides, [(0, 2, 1, 1, 1, 0, 2, 0)], name=f"two-cubes-p{p}-n{n}")
+
+
 def batch_problems(count=512, seed=7, n2d=(8, 96), n3d=(8, 32), degrees=(2, 3, 4),
                    noise=0.1, sigma=0.5):
     """Independent single-patch problems: the first half 2D, the rest 3D,

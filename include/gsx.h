@@ -430,14 +430,24 @@ gsx_status gsx_mg_destroy(gsx_mg *mg);
  * large levels). */
 /* GSX_SMOOTH_SCMS: the subspace corrected mass smoother as the reference's
  * multigrid example sets it up (setupSubspaceCorrectedMassSmoother,
- * examples/multiGrid_example.cpp:407-560) for a single patch with Dirichlet
- * conditions on every side: x += tau_l S_l (b - A_l x), S_l the smoother of
- * level l's basis (gsx_scms_create with sigma = param, 0.12 when param <= 0,
- * alpha = 0, beta = 1), tau_l = 1 / lambda with lambda from ten power
- * iterations of S_l A_l (src/gsSolver/gsPreconditioner.h:121-132; the
- * reference starts from a random vector, here v0_i = 1 / (1 + i mod 5));
- * the first level above nlevels - 5 with more elements than the degree in
- * every direction keeps its tau for all finer levels. */
+ * examples/multiGrid_example.cpp:407-567, conforming interfaces).  For a
+ * single patch with Dirichlet conditions on every side:
+ * x += tau_l S_l (b - A_l x), S_l the smoother of level l's basis
+ * (gsx_scms_create with sigma = param, 0.12 when param <= 0, alpha = 0,
+ * beta = 1), tau_l = 1 / lambda with lambda from ten power iterations of
+ * S_l A_l (src/gsSolver/gsPreconditioner.h:121-132; the reference starts
+ * from a random vector, here v0_i = 1 / (1 + i mod 5)); the first level
+ * above nlevels - 5 with more elements than the degree in every direction
+ * keeps its tau for all finer levels.  Otherwise (several patches, or sides
+ * without a Dirichlet condition) the free DOFs split into the components of
+ * gsBoxTopology::allComponents(true): patch interiors, sides (glued sides
+ * are one), edges of 3D patches, and all corners together;
+ * x += sum_c T_c op_c T_c^T (b - A_l x) with op_c the smoother of the patch
+ * interior or 3D face basis (faces: alpha = p/h, beta = h/p) damped per
+ * component as above, and the exact inverse of A_l restricted to the other
+ * components (at most 2048 DOFs each).  Several components need FP64
+ * cycles and a hierarchy built from a problem (gsx_mg_create,
+ * gsx_pmg_create). */
 gsx_status gsx_mg_set_smoother(gsx_mg *mg, int32_t kind, double param);
 /* gsMultiGridOp::setSmoother(lvl, op) (src/gsMultiGrid/gsMultiGrid.h:295):
  * the smoother of level lvl alone (1 <= lvl < nlevels), kind and param as in

@@ -687,6 +687,110 @@ def scms_smoother_levels(level_patches, As, sigma=0.12):
     return out
 
 
+def scms_components(patch_list, dofs, nfree, combine_corners=True):
+    """The components of setupSubspaceCorrectedMassSmoother
+    (examples/multiGrid_example.cpp:407-567) with conforming interfaces: the
+    3^d sub-boxes of every patch (gsBoxTopology::allComponents,
+    src/gsCore/gsBoxTopology.cpp:294-340), those of glued patches with the
+    same corner functions being one component, all corners one when
+    combine_corners; a component's DOFs are its free functions interior to
+    the sub-box in its open directions (componentBasis_withIndices with
+    no_lower, src/gsCore/gsMultiBasis.hpp:262-328), first patch first.
+    Returns [(dim, idx, box)] ordered by dimension (highest first), box =
+    (patch, open directions) for patch interiors and the faces of 3D
+    patches (the components with a smoother), else None; empty components
+    are kept (idx of length 0)."""
+    import itertools
+    groups, order, offs, off = {}, [], [], 0
+    for P in patch_list:
+        offs.append(off)
+        off += int(np.prod([len(P.knots[k]) - P.p[k] - 1 for k in range(P.d)]))
+    for k, P in enumerate(patch_list):
+        d = P.d
+        nb = [len(P.knots[j]) - P.p[j] - 1 for j in range(d)]
+
+        def flat(ii):
+            m, stride = 0, 1
+            for j in range(d):
+                m += ii[j] * stride
+                stride *= nb[j]
+            return offs[k] + m
+        for loc in itertools.product((0, 1, 2), repeat=d):      # first index, interior, last index
+            dim = sum(v == 1 for v in loc)
+            corners = frozenset(int(dofs[flat([nb[j] - 1 if cn[j] else 0 for j in range(d)])])
+                                for cn in itertools.product((0, 1), repeat=d)
+                                if not any((loc[j] == 0 and cn[j]) or (loc[j] == 2 and not cn[j]) for j in range(d)))
+            if dim == d:
+                key = (dim, ("patch", k))
+            elif dim == 0 and combine_corners:
+                key = (0, "corners")
+            else:
+                key = (dim, corners)
+            if key not in groups:
+                dirs = [j for j in range(d) if loc[j] == 1]
+                box = (k, dirs) if (dim == d or (dim == d - 1 and dim >= 2)) else None
+                groups[key] = dict(dim=dim, idx=[], seen=set(), box=box)
+                order.append(key)
+            g = groups[key]
+            ranges = [range(0, 1) if loc[j] == 0 else range(nb[j] - 1, nb[j]) if loc[j] == 2 else
+                      range(1, nb[j] - 1) for j in range(d)]
+            for t in itertools.product(*reversed(ranges)):
+                gd = int(dofs[flat(t[::-1])])
+                if gd < nfree and gd not in g["seen"]:
+                    g["seen"].add(gd)
+                    g["idx"].append(gd)
+    comps = sorted((groups[key] for key in order), key=lambda g: -g["dim"])
+    return [(g["dim"], np.array(g["idx"], dtype=np.int64), g["box"]) for g in comps]
+
+
+def scms_split_levels(level_problems, As, sigma=0.12):
+    """setupSubspaceCorrectedMassSmoother on every level l >= 1 of a
+    hierarchy (level_problems[l]: the problem of level l, As[l] its
+    operator) as dense smoothers S_l = sum_c T_c op_c T_c^T: for patch
+    interiors and 3D faces op_c = tau_c scms_matrix of the component's
+    basis (Dirichlet on all its sides; faces with alpha = p/h, beta = h/p, h
+    the patch's smallest knot span, p its degree) with tau_c = scms_damping
+    of A_cc, kept per component under the example's rule (:536-554); for
+    the others op_c = A_cc^-1.  Returns [None, S_1, ...]."""
+    import types
+    nlev = len(As)
+    out = [None]
+    saved = None
+    for l in range(1, nlev):
+        prob = level_problems[l]
+        dofs, nfree = mapper(prob.patches, prob.dir_sides, prob.ifaces)
+        comps = scms_components(prob.patches, dofs, nfree)
+        if saved is None:
+            saved = [0.0] * len(comps)
+        A = As[l].toarray() if hasattr(As[l], "toarray") else np.asarray(As[l])
+        S = np.zeros((nfree, nfree))
+        for i, (dim, idx, box) in enumerate(comps):
+            if len(idx) == 0:
+                continue
+            Acc = A[np.ix_(idx, idx)]
+            if box is None:
+                S[np.ix_(idx, idx)] += np.linalg.inv(Acc)
+                continue
+            k, dirs = box
+            P = prob.patches[k]
+            B = types.SimpleNamespace(d=len(dirs), p=[P.p[j] for j in dirs], knots=[P.knots[j] for j in dirs])
+            if dim == P.d:
+                Sc = scms_matrix(B, list(range(1, 2 * B.d + 1)), sigma=sigma)
+            else:
+                h = min(np.min(np.diff(np.unique(np.asarray(P.knots[j], dtype=np.float64)))) for j in range(P.d))
+                pm = max(P.p[:P.d])
+                Sc = scms_matrix(B, list(range(1, 2 * B.d + 1)), sigma=sigma, alpha=pm / h, beta=h / pm)
+            if saved[i] == 0.0:
+                tau = scms_damping(Acc, Sc)
+                if l > nlev - 5 and all(len(np.unique(B.knots[j])) - 1 > B.p[j] for j in range(B.d)):
+                    saved[i] = tau
+            else:
+                tau = saved[i]
+            S[np.ix_(idx, idx)] += tau * Sc
+        out.append(S)
+    return out
+
+
 def mg_vcycle(As, Ps, smoothers, npre=1, npost=1, cgc=1.0):
     """gsMultiGridOp's V-cycle from a zero start (oc_mg_apply / mg_step of
     the C oracle: src/gsSolver/gsMultiGrid.hpp:91-199) with smoothers given

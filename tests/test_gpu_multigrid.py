@@ -257,26 +257,69 @@ def test_scms_smoother_matches_oracle(ctx, name):
 
 
 def test_scms_smoother_error_paths(ctx):
-    """Several patches, or a side without a Dirichlet condition: the
-    interior is not the only component, which the device smoother does not
-    provide."""
+    """Several components (patches, interfaces, sides without a Dirichlet
+    condition) run FP64 cycles only, and need a hierarchy built from a
+    problem (not from the caller's transfers)."""
     grid = problems.square_grid(2, 1, n=4, p=2)
     gp = gsx.Problem(ctx, grid)
-    A, b = gp.assemble()
+    A, b = gp.assemble(gsx.PATTERN_EXPR, gsx.MIXED)
     with pytest.raises(gsx.GsxError):
-        gsx.Multigrid(gp, A, 2, smoother="scms")
+        gsx.Multigrid(gp, A, 2, precision=gsx.MIXED, smoother="scms")
+    levels, maps = _hierarchy(lambda n: problems.square_grid(2, 1, n, 2), (2, 4))
+    _, _, Ps, _ = _oracle_mg(levels, maps)
+    mg = gsx.Multigrid.from_transfers(ctx, A, Ps)
+    with pytest.raises(gsx.GsxError):
+        mg.set_level_smoother(1, "scms")
+    mg.close()
     A.close()
     b.close()
     gp.close()
-    P = problems.box_patch(2, 2, 8)
-    P.fkind = 1
-    gp = gsx.Problem(ctx, problems.Problem([P], [(0, 1), (0, 2), (0, 3)], []))
+
+
+SPLIT_CASES = {
+    "grid2x2-p3": (lambda n: problems.square_grid(2, 2, n, 3), (2, 4, 8)),
+    "square-neumann-p3": (lambda n: problems.Problem([problems.poisson_square(n=n, p=3).patches[0]],
+                                                     [(0, 1), (0, 2), (0, 3)], []), (4, 8, 16)),
+    "two-cubes-p2": (lambda n: problems.two_cubes(n, 2), (2, 4)),
+}
+
+
+@pytest.mark.parametrize("name", list(SPLIT_CASES))
+def test_multipatch_scms_smoother_matches_oracle(ctx, name):
+    """The subspace corrected mass smoother with several components (patch
+    interiors and 3D faces with their box smoothers and power-iteration
+    dampings, lower-dimensional components solved exactly): one cycle and
+    MG-PCG iterations and solution against the oracle's dense split
+    smoother in its numpy V-cycle."""
+    make, ns = SPLIT_CASES[name]
+    print(f"[case] split scms {name}", flush=True)
+    levels, maps = _hierarchy(make, ns)
+    Af, bf, Ps, omg = _oracle_mg(levels, maps)
+    As = [omg.matrix(lvl) for lvl in range(len(ns))]
+    split = O.scms_split_levels(levels, As)
+
+    def smoother(lvl):
+        def f(b, x, post):
+            x += split[lvl] @ (b - As[lvl] @ x)
+        return f
+    cyc = O.mg_vcycle(As, Ps, [None] + [smoother(lvl) for lvl in range(1, len(ns))], npre=1, npost=1)
+    xo, ito, _ = O.pcg(Af, bf, cyc, tol=1e-10, maxit=300)
+    gp = gsx.Problem(ctx, levels[-1])
     A, b = gp.assemble()
-    with pytest.raises(gsx.GsxError):
-        gsx.Multigrid(gp, A, 2, smoother="scms")
-    A.close()
-    b.close()
-    gp.close()
+    mg = gsx.Multigrid(gp, A, len(ns), 1, 1, 1.0, smoother="scms")
+    r = ctx.to_device(bf)
+    z = ctx.array(gp.nfree)
+    mg.apply(r, z)
+    zo = cyc(bf)
+    print("  cycle", np.linalg.norm(z.download() - zo) / np.linalg.norm(zo), flush=True)
+    assert np.linalg.norm(z.download() - zo) <= 1e-11 * np.linalg.norm(zo)
+    x = ctx.array(gp.nfree).zero()
+    it, rel, ok = gsx.mg_cg_solve(A, mg, b, x, tol=1e-10, maxit=300)
+    print(f"  {it} iterations, oracle {ito}", flush=True)
+    assert ok and abs(it - ito) <= 1
+    assert np.linalg.norm(x.download() - xo) <= 1e-8 * np.linalg.norm(xo)
+    for o in (r, z, x, mg, A, b, gp):
+        o.close()
 
 
 @pytest.mark.parametrize("name", ["grid2x2-p3", "cube-p2", "grid4x4-p3"])
