@@ -1742,8 +1742,10 @@ typename gsLinearOperator<T>::Ptr makeSparseLUSolver(const gsSparseMatrix<T> &M)
 /// (setOptions): NumPreSmooth, NumPostSmooth, CoarseGridCorrectionDamping,
 /// Damping (default Jacobi/Richardson damping), Smoother ("Jacobi"/"j",
 /// "GaussSeidel"/"gs", "Richardson"/"r", "Chebyshev"/"c",
-/// "SubspaceCorrectedMassSmoother"/"scms": one patch with Dirichlet sides,
-/// damped per level by its power iteration), ChebyshevRatio, Scaling (sigma
+/// "SubspaceCorrectedMassSmoother"/"scms": the example's components --
+/// patch interiors and 3D faces with box smoothers damped by their power
+/// iteration, sides, edges and corners solved exactly; several components
+/// need "fp64"), ChebyshevRatio, Scaling (sigma
 /// of the scms smoother, 0.12), Precision ("fp64" or "mixed": FP32 cycle).
 /// The coarsest level is always solved exactly (dense FP64 inverse on the
 /// device); setCoarseSolver accepts makeSparseCholeskySolver /

@@ -12,7 +12,7 @@
 //   --MG.Damping X        Jacobi / Richardson damping (default 0.8)
 //   -s NAME               MG.Smoother: GaussSeidel (gs, default), Jacobi (j),
 //                         Richardson (r), Chebyshev (c),
-//                         SubspaceCorrectedMassSmoother (scms: one patch)
+//                         SubspaceCorrectedMassSmoother (scms)
 //   --MG.Scaling X        sigma of the scms smoother (default 0.12)
 //   -t X                  tolerance (default 1e-8)
 //   --Solver.MaxIterations N (default 100)
