@@ -156,8 +156,13 @@ bool coarsen_supported(int d, int p);
 hipError_t launch_coarsen(int d, int p, const PatchDev *cpatches, const PatchDev *fpatches, const int *gpatch,
                           const int64_t *gprefix, int ng, int64_t nelem_c, const double *T1, const int64_t *toff,
                           const double *Kf, double *Kc, hipStream_t s);
+// dense inverse of an SPD CSR matrix (n rows) into D (npad x npad, padded
+// with the identity) by symmetric Gauss-Jordan sweeps; npad from
+// dense_inverse_npad(n), work of dense_inverse_work_doubles(npad) doubles
+int dense_inverse_npad(int n);
+int64_t dense_inverse_work_doubles(int npad);
 hipError_t launch_dense_inverse(int n, int npad, const int *rp, const int *ci, const double *v,
-                                double *D, double *W, double *Pinv, hipStream_t s);
+                                double *D, double *work, hipStream_t s);
 hipError_t launch_dense_gemv_f64(int n, int npad, const double *M, const double *x, double *y,
                                  hipStream_t s);
 hipError_t launch_prolong_add_f64(int n, const int *rp, const int *ci, const double *v,

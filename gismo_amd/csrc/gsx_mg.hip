@@ -592,6 +592,129 @@ __global__ __launch_bounds__(256) void k_gj_finish(double *__restrict__ D, int n
     }
 }
 
+// Two-level sweep for large coarse matrices: outer pivot blocks of 256.
+// Sweeping the 256-block K at once equals its four 64-sweeps one after the
+// other, but touches each tile of D once instead of four times: B^-1 by the
+// 64-sweep on a 256 x 256 copy, W = B^-1 M[K][.] (row panel), the rank-256
+// update M[I][J] -= M[I][K] W[J], then the swept panels.
+constexpr int kGJB = 256;
+
+// the block K of D as a full symmetric 256 x 256 matrix (a tile below the
+// diagonal is the transpose of its kept mirror)
+__global__ void k_gjb_take_block(const double *__restrict__ D, int npad, int kb, double *__restrict__ B)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kGJB * kGJB) return;
+    const int r = i / kGJB, c = i % kGJB;
+    B[i] = (r / kGJ <= c / kGJ) ? D[(int64_t)(kb + r) * npad + kb + c] : D[(int64_t)(kb + c) * npad + kb + r];
+}
+
+// MODE 0 (row panel): W[64 ri + .][J + .] = sum_c Binv[64 ri + .][64 c + .] M[kb + 64 c + .][J + .]
+//   (grid: column tiles x 4 row tiles ri; tiles J inside the block skipped)
+// MODE 1 (update): D[I][J] -= sum_c M[I][kb + 64 c + .] W[64 c + .][J]
+//   (grid: column tiles x row tiles, upper tiles outside the block)
+// Each 64 x 64 output tile accumulates the four 64-deep chunks on the FP64
+// matrix cores (the operand layout of k_gj_update) and is written once.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_gjb_gemm(double *__restrict__ D, int npad, int kb,
+                                                  const double *__restrict__ Binv, double *__restrict__ W)
+{
+    const int J = blockIdx.x * kGJ, I = blockIdx.y * kGJ;
+    const bool inJ = J >= kb && J < kb + kGJB;
+    if (MODE == 0) {
+        if (inJ) return;
+    } else {
+        const bool inI = I >= kb && I < kb + kGJB;
+        if (inI || inJ || I > J) return;
+    }
+    __shared__ double sA[kGJ][kGJ + 1];
+    __shared__ double sB[kGJ][kGJ + 1];
+    const int t = threadIdx.x;
+    const int w = t / kWave, lane = t % kWave;
+    const int r0 = (w / 2) * 32, c0 = (w % 2) * 32;
+    const int li = lane % 16, lk = lane / 16;
+    gj_d4 acc[2][2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) acc[u][v] = gj_d4{0.0, 0.0, 0.0, 0.0};
+    for (int kc = 0; kc < kGJB / kGJ; ++kc) {
+        const int kcol = kb + kc * kGJ;
+        for (int i = t; i < kGJ * kGJ; i += 256) {
+            const int r = i / kGJ, c = i % kGJ;
+            if (MODE == 0) {
+                sA[r][c] = Binv[(int64_t)(I + r) * kGJB + kc * kGJ + c];
+                // M[kcol + r][J + c]: the upper tile D[kcol][J], or the
+                // transpose of D[J][kcol]
+                if (J > kb) sB[r][c] = D[(int64_t)(kcol + r) * npad + J + c];
+                else sB[c][r] = D[(int64_t)(J + r) * npad + kcol + c];
+            } else {
+                // M[I + r][kcol + c]: the upper tile D[I][kcol], or the
+                // transpose of D[kcol][I]
+                if (I < kb) sA[r][c] = D[(int64_t)(I + r) * npad + kcol + c];
+                else sA[c][r] = D[(int64_t)(kcol + r) * npad + I + c];
+                sB[r][c] = W[(int64_t)(kc * kGJ + r) * npad + J + c];
+            }
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int ks = 0; ks < kGJ / 4; ++ks) {
+            const int m = 4 * ks + lk;
+            const double a0 = sA[r0 + li][m], a1 = sA[r0 + 16 + li][m];
+            const double b0 = sB[m][c0 + li], b1 = sB[m][c0 + 16 + li];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int row = r0 + 16 * u + lk + 4 * q, col = J + c0 + 16 * v + li;
+                if (MODE == 0) W[(int64_t)(I + row) * npad + col] = acc[u][v][q];
+                else D[(int64_t)(I + row) * npad + col] -= acc[u][v][q];
+            }
+}
+
+// swept panels of block K: M'[I][K] = W[I]^T (I before K), M'[K][J] = W[J]
+// (J after K), M'[K][K] = -B^-1 (its upper tiles); grid: column tiles x 4
+// row tiles of the block
+__global__ __launch_bounds__(256) void k_gjb_panels(double *__restrict__ D, int npad, int kb,
+                                                    const double *__restrict__ W, const double *__restrict__ Binv)
+{
+    const int I = blockIdx.x * kGJ, ri = blockIdx.y, rb = kb + ri * kGJ;
+    const int t = threadIdx.x;
+    if (I < kb) {
+        __shared__ double sT[kGJ][kGJ + 1];
+        for (int i = t; i < kGJ * kGJ; i += 256)
+            sT[i / kGJ][i % kGJ] = W[(int64_t)(ri * kGJ + i / kGJ) * npad + I + i % kGJ];
+        __syncthreads();
+        for (int i = t; i < kGJ * kGJ; i += 256) {
+            const int r = i / kGJ, c = i % kGJ;
+            D[(int64_t)(I + r) * npad + rb + c] = sT[c][r];
+        }
+        return;
+    }
+    if (I < kb + kGJB) {
+        const int ci = (I - kb) / kGJ;
+        if (ri > ci) return;
+        for (int i = t; i < kGJ * kGJ; i += 256) {
+            const int r = i / kGJ, c = i % kGJ;
+            D[(int64_t)(rb + r) * npad + I + c] = -Binv[(int64_t)(ri * kGJ + r) * kGJB + ci * kGJ + c];
+        }
+        return;
+    }
+    for (int i = t; i < kGJ * kGJ; i += 256) {
+        const int r = i / kGJ, c = i % kGJ;
+        D[(int64_t)(rb + r) * npad + I + c] = W[(int64_t)(ri * kGJ + r) * npad + I + c];
+    }
+}
+
 // y = M x, dense n x n (row stride npad), one wavefront per row
 template <typename T>
 __global__ __launch_bounds__(256) void k_dense_gemv(int n, int npad, const T *__restrict__ M,
@@ -753,11 +876,10 @@ hipError_t launch_coarsen(int d, int p, const PatchDev *cpatches, const PatchDev
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_dense_inverse(int n, int npad, const int *rp, const int *ci, const double *v, double *D,
-                                double *W, double *Pinv, hipStream_t s)
+// the 64-block symmetric sweep of the npad x npad matrix D (M = -A^-1 on
+// the upper tiles)
+static void gj_sweep64(double *D, int npad, double *W, double *Pinv, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_dense_fill, g1((int64_t)npad * npad), dim3(256), 0, s, n, npad, D);
-    if (n > 0) hipLaunchKernelGGL(k_dense_scatter, g1(n), dim3(256), 0, s, n, npad, rp, ci, v, D);
     const int nb = npad / kGJ;
     for (int k = 0; k < nb; ++k) {
         const int kb = k * kGJ;
@@ -765,6 +887,41 @@ hipError_t launch_dense_inverse(int n, int npad, const int *rp, const int *ci, c
         hipLaunchKernelGGL(k_gj_rowpanel, dim3(nb), dim3(256), 0, s, D, npad, kb, Pinv, W);
         hipLaunchKernelGGL(k_gj_update, dim3(nb, nb), dim3(256), 0, s, D, npad, kb, W);
         hipLaunchKernelGGL(k_gj_panels, dim3(nb), dim3(256), 0, s, D, npad, kb, W, Pinv);
+    }
+}
+
+int dense_inverse_npad(int n)
+{
+    if (n <= kGJB) return n <= kGJ ? kGJ : (n + kGJ - 1) / kGJ * kGJ;
+    return (n + kGJB - 1) / kGJB * kGJB;
+}
+
+int64_t dense_inverse_work_doubles(int npad)
+{
+    if (npad <= kGJB) return (int64_t)kGJ * npad + kGJ * kGJ;
+    return (int64_t)kGJB * npad + kGJB * kGJB + kGJ * kGJB + kGJ * kGJ;
+}
+
+hipError_t launch_dense_inverse(int n, int npad, const int *rp, const int *ci, const double *v, double *D,
+                                double *work, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dense_fill, g1((int64_t)npad * npad), dim3(256), 0, s, n, npad, D);
+    if (n > 0) hipLaunchKernelGGL(k_dense_scatter, g1(n), dim3(256), 0, s, n, npad, rp, ci, v, D);
+    const int nb = npad / kGJ;
+    if (npad <= kGJB) {
+        gj_sweep64(D, npad, work, work + (int64_t)kGJ * npad, s);
+    } else {
+        double *W = work, *Bb = W + (int64_t)kGJB * npad, *Wi = Bb + kGJB * kGJB, *Pi = Wi + kGJ * kGJB;
+        const int nbb = kGJB / kGJ;
+        for (int kb = 0; kb < npad; kb += kGJB) {
+            // B^-1 of the pivot block on its own copy
+            hipLaunchKernelGGL(k_gjb_take_block, g1(kGJB * kGJB), dim3(256), 0, s, D, npad, kb, Bb);
+            gj_sweep64(Bb, kGJB, Wi, Pi, s);
+            hipLaunchKernelGGL(k_gj_finish, dim3(nbb, nbb), dim3(256), 0, s, Bb, kGJB);
+            hipLaunchKernelGGL(k_gjb_gemm<0>, dim3(nb, nbb), dim3(256), 0, s, D, npad, kb, Bb, W);
+            hipLaunchKernelGGL(k_gjb_gemm<1>, dim3(nb, nb), dim3(256), 0, s, D, npad, kb, Bb, W);
+            hipLaunchKernelGGL(k_gjb_panels, dim3(nb, nbb), dim3(256), 0, s, D, npad, kb, W, Bb);
+        }
     }
     hipLaunchKernelGGL(k_gj_finish, dim3(nb, nb), dim3(256), 0, s, D, npad);
     return hipGetLastError();

@@ -2802,12 +2802,12 @@ static gsx_status mg_finish(gsx_mg *mg)
         MGLevel &C0 = mg->L[0];
         const int n0 = C0.n;
         if ((int64_t)n0 > 32768) return fail(c, GSX_ERR_ARG, "coarsest level too large for the dense inverse");
-        mg->npad = std::max(64, (n0 + 63) / 64 * 64);
+        mg->npad = gsx::dense_inverse_npad(n0);
         mg->Dinv = dalloc(c, sizeof(double) * (size_t)mg->npad * mg->npad);
-        DevPtr W = dalloc(c, sizeof(double) * (size_t)64 * mg->npad), Pinv = dalloc(c, sizeof(double) * 64 * 64);
-        if (!mg->Dinv || !W || !Pinv) return fail(c, GSX_ERR_OOM, "device allocation (coarse inverse)");
+        DevPtr W = dalloc(c, sizeof(double) * (size_t)gsx::dense_inverse_work_doubles(mg->npad));
+        if (!mg->Dinv || !W) return fail(c, GSX_ERR_OOM, "device allocation (coarse inverse)");
         HIP_OK(c, gsx::launch_dense_inverse(n0, mg->npad, dp<int>(C0.rowptr), dp<int>(C0.col), dp<double>(C0.val),
-                                            dp<double>(mg->Dinv), dp<double>(W), dp<double>(Pinv), s));
+                                            dp<double>(mg->Dinv), dp<double>(W), s));
         HIP_OK(c, hipStreamSynchronize(s));
     }
     return GSX_OK;
