@@ -295,9 +295,11 @@ hipError_t launch_geo_mixed(int d, const PatchDev *ups, const PatchDev *vps, con
 hipError_t launch_mix_pattern(int phase, int d, const PatchDev *ups, const PatchDev *vps, const int *occ_ptr,
                               const int *occ_patch, const int64_t *occ_local, int nrows, int nfree_v,
                               const int *cptr, int *cand, int *counts, const int *rowptr, int *col, hipStream_t s);
+// values of the mixed mass: one wavefront per row when every row has at
+// most 1024 entries (maxrow), else one thread per row (the same sums)
 hipError_t launch_mix_gather(int d, const PatchDev *ups, const PatchDev *vps, const int *occ_ptr, const int *occ_patch,
                              const int64_t *occ_local, int nrows, int nfree_v, const int *rowptr, const int *col,
-                             const double *Mbuf, const int64_t *moff, double *val, hipStream_t s);
+                             const double *Mbuf, const int64_t *moff, double *val, int maxrow, hipStream_t s);
 // per free DOF: sum of the element vector entries of its functions
 hipError_t launch_vec_gather(int d, const PatchDev *ups, const int *occ_ptr, const int *occ_patch,
                              const int64_t *occ_local, int nrows, const double *Fbuf, double *out, hipStream_t s);

@@ -141,9 +141,11 @@ __global__ void k_tr_fill(int n, const int *__restrict__ rp, const int *__restri
 // Sort the entries of every row by column (the transpose is filled through
 // atomic cursors, so its in-row order is arbitrary).  One wavefront per row:
 // the row goes to LDS and every entry's position is the count of smaller
-// columns (columns are unique within a row).  Rows longer than the LDS
-// slice are sorted by insertion in place.
+// columns (columns are unique within a row).  Rows of kSortCap to kSortLong
+// entries (restrictions of p-multigrid) are ranked the same way by a whole
+// block (k_sort_rows_long); longer ones are sorted by insertion in place.
 constexpr int kSortCap = 512;
+constexpr int kSortLong = 4096;
 __global__ __launch_bounds__(256) void k_sort_rows(int n, const int *__restrict__ rp, int *__restrict__ ci,
                                                    double *__restrict__ v)
 {
@@ -154,7 +156,7 @@ __global__ __launch_bounds__(256) void k_sort_rows(int n, const int *__restrict_
     if (r >= n) return;
     const int a = rp[r], len = rp[r + 1] - a;
     if (len > kSortCap) {
-        if (lane != 0) return;
+        if (len <= kSortLong || lane != 0) return;
         for (int i = a + 1; i < a + len; ++i) {
             const int c = ci[i];
             const double x = v[i];
@@ -180,6 +182,29 @@ __global__ __launch_bounds__(256) void k_sort_rows(int n, const int *__restrict_
         for (int j = 0; j < len; ++j) rank += sc[w][j] < key;
         ci[a + rank] = key;
         v[a + rank] = sv[w][k];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sort_rows_long(int n, const int *__restrict__ rp, int *__restrict__ ci,
+                                                        double *__restrict__ v)
+{
+    __shared__ int sc[kSortLong];
+    __shared__ double sv[kSortLong];
+    const int r = blockIdx.x;
+    if (r >= n) return;
+    const int a = rp[r], len = rp[r + 1] - a;
+    if (len <= kSortCap || len > kSortLong) return;
+    for (int k = threadIdx.x; k < len; k += blockDim.x) {
+        sc[k] = ci[a + k];
+        sv[k] = v[a + k];
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < len; k += blockDim.x) {
+        const int c = sc[k];
+        int pos = 0;
+        for (int m = 0; m < len; ++m) pos += sc[m] < c;
+        ci[a + pos] = c;
+        v[a + pos] = sv[k];
     }
 }
 
@@ -818,7 +843,10 @@ hipError_t launch_transpose(int phase, int n, const int *rp, const int *ci, cons
     } else if (phase == 1) {
         if (n > 0) hipLaunchKernelGGL(k_tr_fill, g1(n), dim3(256), 0, s, n, rp, ci, v, trp, cursor, tci, tv);
     } else {
-        if (nt > 0) hipLaunchKernelGGL(k_sort_rows, g1(nt, 4), dim3(256), 0, s, nt, trp, tci, tv);
+        if (nt > 0) {
+            hipLaunchKernelGGL(k_sort_rows, g1(nt, 4), dim3(256), 0, s, nt, trp, tci, tv);
+            hipLaunchKernelGGL(k_sort_rows_long, dim3((unsigned)nt), dim3(256), 0, s, nt, trp, tci, tv);
+        }
     }
     return hipGetLastError();
 }

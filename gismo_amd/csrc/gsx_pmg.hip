@@ -465,6 +465,67 @@ __global__ void k_mix_gather(const PatchDev *__restrict__ ups, const PatchDev *_
     }
 }
 
+// The same with one wavefront per row (4 rows per block): the row's columns
+// and sums in LDS, the lanes taking the coarse functions of each element
+// (distinct columns), the elements one after the other; each entry sums its
+// elements in the order of k_mix_gather, so the values are the same bits.
+constexpr int kMixRowMax = 1024;
+template <int D>
+__global__ __launch_bounds__(256) void k_mix_gather_w(const PatchDev *__restrict__ ups, const PatchDev *__restrict__ vps,
+                                                      const int *__restrict__ occ_ptr, const int *__restrict__ occ_patch,
+                                                      const int64_t *__restrict__ occ_local, int nrows, int nfree_v,
+                                                      const int *__restrict__ rowptr, const int *__restrict__ col,
+                                                      const double *__restrict__ Mbuf, const int64_t *__restrict__ moff,
+                                                      double *__restrict__ val)
+{
+    __shared__ int sCol[4][kMixRowMax];
+    __shared__ double sAcc[4][kMixRowMax];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int r = blockIdx.x * 4 + w;
+    if (r >= nrows) return;
+    const int rb = rowptr[r], len = rowptr[r + 1] - rb;
+    for (int k = lane; k < len; k += kWave) {
+        sCol[w][k] = col[rb + k];
+        sAcc[w][k] = 0.0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (int o = occ_ptr[r]; o < occ_ptr[r + 1]; ++o) {
+        const int ip = occ_patch[o];
+        const PatchDev &U = ups[ip];
+        const PatchDev &V = vps[ip];
+        int l[3], elo[3], ehi[3], vlo[3], vhi[3];
+        coords_of(U, occ_local[o], l);
+        mix_ranges<D>(U, V, l, elo, ehi, vlo, vhi);
+        const int u0 = U.p[0] + 1, u1 = U.p[1] + 1, u2 = D == 3 ? U.p[2] + 1 : 1;
+        const int w0 = V.p[0] + 1, w1 = V.p[1] + 1, w2 = D == 3 ? V.p[2] + 1 : 1;
+        const int NBU = u0 * u1 * u2, NBV = w0 * w1 * w2;
+        for (int ez = elo[2]; ez <= ehi[2]; ++ez)
+            for (int ey = elo[1]; ey <= ehi[1]; ++ey)
+                for (int ex = elo[0]; ex <= ehi[0]; ++ex) {
+                    const int64_t el = ex + (int64_t)U.nel[0] * (ey + (int64_t)U.nel[1] * ez);
+                    const int a = (l[0] - ex) + u0 * ((l[1] - ey) + u1 * (l[2] - ez));
+                    const int f0 = v_elem_of(U, V, 0, ex), f1 = v_elem_of(U, V, 1, ey);
+                    const int f2 = D == 3 ? v_elem_of(U, V, 2, ez) : 0;
+                    const double *Me = Mbuf + moff[ip] + el * (int64_t)NBU * NBV + (int64_t)a * NBV;
+                    for (int b = lane; b < NBV; b += kWave) {
+                        const int bx = b % w0, by = (b / w0) % w1, bz = b / (w0 * w1);
+                        const int64_t m = (f0 + bx) + (int64_t)V.nb[0] * ((f1 + by) + (int64_t)V.nb[1] * (f2 + bz));
+                        const int g = V.dofs[m];
+                        if (g >= nfree_v) continue;
+                        int lo = 0, hi = len - 1;
+                        while (lo < hi) {
+                            const int mid = (lo + hi) >> 1;
+                            if (sCol[w][mid] < g) lo = mid + 1; else hi = mid;
+                        }
+                        sAcc[w][lo] += Me[b];
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                }
+    }
+    __builtin_amdgcn_wave_barrier();
+    for (int k = lane; k < len; k += kWave) val[rb + k] = sAcc[w][k];
+}
+
 // Row r of an element vector field: sum over the row's patches and elements
 // in element order (the load part of the assembly gather, without
 // eliminated columns).
@@ -602,9 +663,19 @@ hipError_t launch_mix_pattern(int phase, int d, const PatchDev *ups, const Patch
 
 hipError_t launch_mix_gather(int d, const PatchDev *ups, const PatchDev *vps, const int *occ_ptr, const int *occ_patch,
                              const int64_t *occ_local, int nrows, int nfree_v, const int *rowptr, const int *col,
-                             const double *Mbuf, const int64_t *moff, double *val, hipStream_t s)
+                             const double *Mbuf, const int64_t *moff, double *val, int maxrow, hipStream_t s)
 {
     if (nrows <= 0) return hipSuccess;
+    if (maxrow <= kMixRowMax) {
+        const dim3 g = grid1(nrows, 4), b(256);
+        if (d == 2)
+            hipLaunchKernelGGL(k_mix_gather_w<2>, g, b, 0, s, ups, vps, occ_ptr, occ_patch, occ_local, nrows, nfree_v,
+                               rowptr, col, Mbuf, moff, val);
+        else
+            hipLaunchKernelGGL(k_mix_gather_w<3>, g, b, 0, s, ups, vps, occ_ptr, occ_patch, occ_local, nrows, nfree_v,
+                               rowptr, col, Mbuf, moff, val);
+        return hipGetLastError();
+    }
     const dim3 g = grid1(nrows, 128), b(128);
     if (d == 2)
         hipLaunchKernelGGL(k_mix_gather<2>, g, b, 0, s, ups, vps, occ_ptr, occ_patch, occ_local, nrows, nfree_v,

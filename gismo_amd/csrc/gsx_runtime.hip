@@ -3357,10 +3357,23 @@ static gsx_status pmg_mixed(gsx_ctx *c, gsx_problem *U, gsx_problem *V, gsx_prob
     HIP_OK(c, gsx::launch_mix_pattern(2, d, ups, vps, nullptr, nullptr, nullptr, nrows, V->nfree, dp<int>(cptr),
                                       dp<int>(cand), nullptr, dp<int>(C.prp), dp<int>(C.pci), s));
     {
+        // one wavefront per row pays where an element has at least 27 coarse
+        // functions (the lanes take them); it holds rows of up to 1024
+        int nbv = 1;
+        for (int j = 0; j < d; ++j) nbv *= V->hp[0].p[j] + 1;
+        int maxrow = INT32_MAX;
+        if (nbv >= 27) {
+            std::vector<int> rp((size_t)nrows + 1);
+            HIP_OK(c, hipMemcpyAsync(rp.data(), C.prp.get(), sizeof(int) * (nrows + 1), hipMemcpyDeviceToHost, s));
+            HIP_OK(c, hipStreamSynchronize(s));
+            maxrow = 0;
+            for (int i = 0; i < nrows; ++i) maxrow = std::max(maxrow, rp[(size_t)i + 1] - rp[(size_t)i]);
+        }
         KScope k(c, "pmg_mixed_gather");
         HIP_OK(c, gsx::launch_mix_gather(d, ups, vps, dp<int>(U->d_occ_ptr), dp<int>(U->d_occ_patch),
                                          dp<int64_t>(U->d_occ_local), nrows, V->nfree, dp<int>(C.prp),
-                                         dp<int>(C.pci), dp<double>(Mb), dp<int64_t>(d_moff), dp<double>(C.pv), s));
+                                         dp<int>(C.pci), dp<double>(Mb), dp<int64_t>(d_moff), dp<double>(C.pv),
+                                         maxrow, s));
     }
     C.nfine = nrows;
     C.pnnz = nnz;
