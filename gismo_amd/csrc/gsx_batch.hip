@@ -159,13 +159,56 @@ __device__ inline float row_sum(float v, int tpr)
     return v;
 }
 
+// The x values of a group's nv vectors at a column: separate arrays (one
+// member, or the generic path), or the group's interleaved copy (kGroupMax
+// values per row: one coefficient's x values come in two 16-byte loads in
+// FP32 instead of nv scattered ones).  Both add c * x to acc[j] in the same
+// order, so a member's sums are the same bits either way.
+template <typename T>
+struct XSep {
+    const T *const *xs;
+    template <int KMAX>
+    __device__ inline void fma(T c, int col, T *acc, int nv) const
+    {
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j)
+            if (j < nv) acc[j] += c * xs[j][col];
+    }
+    __device__ inline T at(int j, int r) const { return xs[j][r]; }
+};
+template <typename T>
+struct XIl {
+    const T *X;
+    int row0;
+    template <int KMAX>
+    __device__ inline void fma(T c, int col, T *acc, int nv) const
+    {
+        static_assert(KMAX == kGroupMax, "interleaved rows hold kGroupMax values");
+        const T *q = X + (int64_t)(col - row0) * kGroupMax;
+        T v[kGroupMax];
+        if constexpr (sizeof(T) == 4) {
+            typedef float f4 __attribute__((ext_vector_type(4)));
+            const f4 a = *reinterpret_cast<const f4 *>(q), b = *reinterpret_cast<const f4 *>(q + 4);
+            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+            v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < kGroupMax; ++j) v[j] = q[j];
+        }
+        (void)nv;   // slots >= nv hold stale values; their sums are not read
+#pragma unroll
+        for (int j = 0; j < KMAX; ++j) acc[j] += c * v[j];
+    }
+    __device__ inline T at(int j, int r) const { return X[(int64_t)(r - row0) * kGroupMax + j]; }
+};
+
 // All slots of the row (sv: the FP64 stencil or the full FP32 one), line by
 // line in lexicographic (= column) order, applied to nv vectors at once
 // (acc[j] for xs[j]): each coefficient is read once for all of them, and
 // each vector's sum runs in the same order whatever nv is.
-template <int D, int P, typename T, int KMAX>
+template <int D, int P, typename T, int KMAX, typename XA>
 __device__ inline void box_full_rows(const BSeg &g, const T *__restrict__ sv, int r, int sub, int tpr,
-                                     const T *const *xs, int nv, T *acc)
+                                     const XA &xa, int nv, T *acc)
 {
     using G = BoxGeo<D, P>;
     constexpr int NLINES = (D == 3) ? G::W * G::W : G::W;
@@ -183,9 +226,7 @@ __device__ inline void box_full_rows(const BSeg &g, const T *__restrict__ sv, in
             const bool ok = okyz && ia + dx >= 0 && ia + dx < m0;
             const T c = vb[(li * G::W + dx + P) * kStencilTile];
             const int col = ok ? rb + dx : r;
-#pragma unroll
-            for (int j = 0; j < KMAX; ++j)
-                if (j < nv) acc[j] += c * xs[j][col];
+            xa.template fma<KMAX>(c, col, acc, nv);
         }
     }
 }
@@ -193,9 +234,8 @@ __device__ inline void box_full_rows(const BSeg &g, const T *__restrict__ sv, in
 // FP32 upper half: the centre line (lane 0 of the row) and each upper line
 // with its mirror line (the order of the single-patch kernel k_sspmv_half),
 // applied to nv vectors at once
-template <int D, int P, int KMAX>
-__device__ inline void box_half_rows(const BSeg &g, int r, int sub, int tpr, const float *const *xs, int nv,
-                                     float *acc)
+template <int D, int P, int KMAX, typename XA>
+__device__ inline void box_half_rows(const BSeg &g, int r, int sub, int tpr, const XA &xa, int nv, float *acc)
 {
     using G = BoxGeo<D, P>;
     constexpr int W = G::W, C = G::C, NH = G::NH;
@@ -205,11 +245,7 @@ __device__ inline void box_half_rows(const BSeg &g, int r, int sub, int tpr, con
     const int ia = l % m0, ib = (l / m0) % m1, ic = (D == 3) ? l / (m0 * m1) : 0;
     const float *sv = g.s32;
     const float *vb = sv + stencil_index(0, l, NH);
-    auto add = [&](float c, int col) {
-#pragma unroll
-        for (int j = 0; j < KMAX; ++j)
-            if (j < nv) acc[j] += c * xs[j][col];
-    };
+    auto add = [&](float c, int col) { xa.template fma<KMAX>(c, col, acc, nv); };
     if (sub == 0) {
 #pragma unroll
         for (int dx = 0; dx <= P; ++dx) add(vb[dx * kStencilTile], ia + dx < m0 ? r + dx : r);
@@ -246,7 +282,7 @@ __device__ inline void box_resid64(const BSeg &g, int lo, int hi, double &dot)
     if (r >= hi) return;   // the row's whole group leaves together
     const double *xs[1] = {g.x};
     double acc[1] = {0.0};
-    box_full_rows<D, P, double, 1>(g, g.s64, r, sub, tpr, xs, 1, acc);
+    box_full_rows<D, P, double, 1>(g, g.s64, r, sub, tpr, XSep<double>{xs}, 1, acc);
     const double a = row_sum(acc[0], tpr);
     if (sub != 0) return;
     const double v = g.b[r] - a;
@@ -256,8 +292,8 @@ __device__ inline void box_resid64(const BSeg &g, int lo, int hi, double &dot)
 
 // y_j = A x_j for the nv members of a group (FP64: all slots; FP32: the half
 // or full copy), partials (x_j, y_j)
-template <int D, int P, typename T, int KMAX>
-__device__ inline void box_apply_group(const BSeg &g, int lo, int hi, const T *const *xs, T *const *ys, int nv,
+template <int D, int P, typename T, int KMAX, typename XA>
+__device__ inline void box_apply_group(const BSeg &g, int lo, int hi, const XA &xa, T *const *ys, int nv,
                                        double *dot)
 {
     const int tpr = g.tpr, r = lo + (int)threadIdx.x / tpr, sub = (int)threadIdx.x % tpr;
@@ -265,9 +301,9 @@ __device__ inline void box_apply_group(const BSeg &g, int lo, int hi, const T *c
     T acc[KMAX];
 #pragma unroll
     for (int j = 0; j < KMAX; ++j) acc[j] = 0;
-    if constexpr (sizeof(T) == 8) box_full_rows<D, P, double, KMAX>(g, g.s64, r, sub, tpr, xs, nv, acc);
-    else if (g.half) box_half_rows<D, P, KMAX>(g, r, sub, tpr, xs, nv, acc);
-    else box_full_rows<D, P, float, KMAX>(g, g.s32, r, sub, tpr, xs, nv, acc);
+    if constexpr (sizeof(T) == 8) box_full_rows<D, P, double, KMAX>(g, g.s64, r, sub, tpr, xa, nv, acc);
+    else if (g.half) box_half_rows<D, P, KMAX>(g, r, sub, tpr, xa, nv, acc);
+    else box_full_rows<D, P, float, KMAX>(g, g.s32, r, sub, tpr, xa, nv, acc);
 #pragma unroll
     for (int j = 0; j < KMAX; ++j)
         if (j < nv) acc[j] = row_sum(acc[j], tpr);
@@ -276,7 +312,7 @@ __device__ inline void box_apply_group(const BSeg &g, int lo, int hi, const T *c
     for (int j = 0; j < KMAX; ++j)
         if (j < nv) {
             ys[j][r] = acc[j];
-            dot[j] += (double)xs[j][r] * (double)acc[j];
+            dot[j] += (double)xa.at(j, r) * (double)acc[j];
         }
 }
 
@@ -294,13 +330,13 @@ __device__ inline void box_resid64_any(const BSeg &g, int lo, int hi, double &do
     }
 }
 
-template <typename T, int KMAX>
-__device__ inline void box_apply_group_any(const BSeg &g, int lo, int hi, const T *const *xs, T *const *ys,
+template <typename T, int KMAX, typename XA>
+__device__ inline void box_apply_group_any(const BSeg &g, int lo, int hi, const XA &xa, T *const *ys,
                                            int nv, double *dot)
 {
     switch (g.bd * 8 + g.bp) {
 #define GSX_C(D, P) \
-    case D * 8 + P: box_apply_group<D, P, T, KMAX>(g, lo, hi, xs, ys, nv, dot); break;
+    case D * 8 + P: box_apply_group<D, P, T, KMAX>(g, lo, hi, xa, ys, nv, dot); break;
         GSX_BOX_CASES(GSX_C)
 #undef GSX_C
     default: break;
@@ -424,6 +460,36 @@ __global__ __launch_bounds__(256) void k_b_init3(const BChunk *__restrict__ chun
 // kGroupMax: groups of several members, which hold KMAX accumulators per
 // row (a separate launch, so that the single-member launch keeps its
 // register budget and occupancy).
+// Interleave the SpMV inputs of each group's active members (the same set
+// k_b_spmv forms: the state it reads is not written between the two
+// launches): X[(r - xrow0) kGroupMax + j] = p_j[r] over the chunk's rows.
+template <typename T>
+__global__ __launch_bounds__(256) void k_b_interleave(const BSChunk *__restrict__ schunks,
+                                                      const BGroup *__restrict__ groups,
+                                                      const BSeg *__restrict__ segs, const BState *__restrict__ st)
+{
+    const BSChunk ch = schunks[blockIdx.x];
+    const BGroup G = groups[ch.grp];
+    if (!G.xi) return;
+    const T *src[kGroupMax];
+    int na = 0;
+    for (int j = 0; j < G.k; ++j) {
+        const BState &S = st[G.seg[j]];
+        if (S.phase != 1 || S.done || S.stop) continue;
+        if (S.flag_next && S.flag_next <= S.it_next) continue;
+        const BSeg &m = segs[G.seg[j]];
+        if constexpr (sizeof(T) == 4) src[na++] = m.p32;
+        else src[na++] = m.p;
+    }
+    if (na == 0) return;
+    T *X = static_cast<T *>(G.xi);
+    const int n = (ch.hi - ch.lo) * kGroupMax;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const int r = ch.lo + t / kGroupMax, j = t % kGroupMax;
+        if (j < na) X[(int64_t)(r - G.xrow0) * kGroupMax + j] = src[j][r];
+    }
+}
+
 template <typename T, int KMAX>
 __global__ __launch_bounds__(256) void k_b_spmv(const BSChunk *__restrict__ schunks,
                                                 const BGroup *__restrict__ groups,
@@ -471,7 +537,17 @@ __global__ __launch_bounds__(256) void k_b_spmv(const BSChunk *__restrict__ schu
                 ys[j] = segs[sid].t;
             }
         }
-        box_apply_group_any<T, KMAX>(g, ch.lo, ch.hi, xs, ys, na, dot);
+        if constexpr (KMAX > 1) {
+            // the inputs of the active members, interleaved by k_b_interleave
+            if (G.xi) {
+                box_apply_group_any<T, KMAX>(g, ch.lo, ch.hi, XIl<T>{static_cast<const T *>(G.xi), G.xrow0}, ys, na,
+                                             dot);
+            } else {
+                box_apply_group_any<T, KMAX>(g, ch.lo, ch.hi, XSep<T>{xs}, ys, na, dot);
+            }
+        } else {
+            box_apply_group_any<T, KMAX>(g, ch.lo, ch.hi, XSep<T>{xs}, ys, na, dot);
+        }
     }
     for (int j = 0; j < na; ++j) {
         const double a = bsum256(dot[j], s);
@@ -790,18 +866,24 @@ hipError_t launch_batch_iteration(int nchunks, const BChunk *chunks, int nsingle
     if (mixed) {
         if (nsingle)
             hipLaunchKernelGGL((k_b_spmv<float, 1>), dim3(nsingle), dim3(256), 0, s, schunks, groups, segs, st, p0);
-        if (nmulti)
+        if (nmulti) {
+            hipLaunchKernelGGL(k_b_interleave<float>, dim3(nmulti), dim3(256), 0, s, schunks + nsingle, groups, segs,
+                               st);
             hipLaunchKernelGGL((k_b_spmv<float, kGroupMax>), dim3(nmulti), dim3(256), 0, s, schunks + nsingle,
                                groups, segs, st, p0);
+        }
         hipLaunchKernelGGL(k_b_update<float>, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p0, p1, p2);
         hipLaunchKernelGGL(k_b_direction<float>, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p1, p2,
                            delta2);
     } else {
         if (nsingle)
             hipLaunchKernelGGL((k_b_spmv<double, 1>), dim3(nsingle), dim3(256), 0, s, schunks, groups, segs, st, p0);
-        if (nmulti)
+        if (nmulti) {
+            hipLaunchKernelGGL(k_b_interleave<double>, dim3(nmulti), dim3(256), 0, s, schunks + nsingle, groups, segs,
+                               st);
             hipLaunchKernelGGL((k_b_spmv<double, kGroupMax>), dim3(nmulti), dim3(256), 0, s, schunks + nsingle,
                                groups, segs, st, p0);
+        }
         hipLaunchKernelGGL(k_b_update<double>, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p0, p1, p2);
         hipLaunchKernelGGL(k_b_direction<double>, dim3(nchunks), dim3(256), 0, s, chunks, segs, st, p1, p2,
                            delta2);

@@ -168,6 +168,10 @@ constexpr int kGroupMax = 8;
 struct BGroup {
     int k;
     int seg[kGroupMax];
+    // groups of several box members: their SpMV inputs interleaved,
+    // kGroupMax values per row (row - xrow0), refreshed before each SpMV
+    void *xi;
+    int xrow0;
 };
 struct BSChunk {
     int grp;

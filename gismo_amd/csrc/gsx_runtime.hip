@@ -4404,9 +4404,11 @@ struct gsx_batch {
     DevPtr d_segs, d_state, d_chunks, d_parts, d_bad, d_groups, d_schunks;
     int chunk_cap = 0, nchunks = 0, nschunks = 0, nsingle = 0;
     int group_max = 1;                   // members per SpMV group (1: no groups)
+    int group_rows = 131072;             // rows from which a member joins a group
     bool dirty = true;
     std::vector<gsx::BChunk> chunks_h;
     std::vector<gsx::BGroup> groups_h;   // members sharing an operator and tile shape
+    std::vector<DevPtr> group_bufs;       // their interleaved SpMV inputs
     std::vector<gsx::BSChunk> schunks_h; // the SpMV launch's chunks (one per group chunk)
     std::vector<gsx::BState> state_h;
     // one step (init, iters iterations, reliable update) captured as a HIP
@@ -4700,7 +4702,7 @@ static gsx_status batch_rebuild(gsx_batch *B)
         if (!M.used || M.status) continue;
         const gsx::BSeg &g = M.seg;
         int gid = -1;
-        if (g.bd)
+        if (g.bd && M.nrows >= B->group_rows)
             for (size_t q = 0; q < B->groups_h.size(); ++q) {
                 const gsx::BSeg &h = B->m[gfirst[q]].seg;
                 if (B->groups_h[q].k < B->group_max && h.bd && h.s64 == g.s64 && h.s32 == g.s32 &&
@@ -4716,6 +4718,21 @@ static gsx_status batch_rebuild(gsx_batch *B)
         }
         gsx::BGroup &G = B->groups_h[gid];
         G.seg[G.k++] = k;
+    }
+    // interleaved SpMV inputs of the groups of several members (kGroupMax
+    // values per row of their box)
+    B->group_bufs.clear();
+    for (size_t q = 0; q < B->groups_h.size(); ++q) {
+        gsx::BGroup &G = B->groups_h[q];
+        G.xi = nullptr;
+        G.xrow0 = 0;
+        if (G.k < 2) continue;
+        const BMember &M = B->m[gfirst[q]];
+        DevPtr buf = dalloc(c, (B->mixed ? sizeof(float) : sizeof(double)) * (size_t)M.nrows * gsx::kGroupMax);
+        if (!buf) return fail(c, GSX_ERR_OOM, "device allocation (group inputs)");
+        G.xi = buf.get();
+        G.xrow0 = M.row0;
+        B->group_bufs.push_back(std::move(buf));
     }
     // chunks of the single-member groups first, then those of the others
     // (two launches: the single-member kernel keeps its register budget)
@@ -4826,6 +4843,14 @@ gsx_status gsx_batch_set_group(gsx_batch *B, int32_t group_max)
 {
     if (!B || group_max < 1 || group_max > gsx::kGroupMax) return GSX_ERR_ARG;
     B->group_max = group_max;
+    B->dirty = true;
+    return GSX_OK;
+}
+
+gsx_status gsx_batch_set_group_rows(gsx_batch *B, int64_t rows)
+{
+    if (!B || rows < 1) return GSX_ERR_ARG;
+    B->group_rows = (int)std::min<int64_t>(rows, INT32_MAX);
     B->dirty = true;
     return GSX_OK;
 }

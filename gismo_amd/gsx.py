@@ -86,7 +86,7 @@ class ServerOpts(C.Structure):
     _fields_ = [("device", C.c_int32), ("precision", C.c_int32), ("budget_bytes", C.c_int64),
                 ("iters_per_step", C.c_int32), ("capacity", C.c_int32), ("mg_min_free", C.c_int64),
                 ("mg_max_coarse", C.c_int64), ("elem_budget", C.c_int64), ("group_max", C.c_int32),
-                ("mg_workers", C.c_int32), ("reserve_bytes", C.c_int64)]
+                ("mg_workers", C.c_int32), ("reserve_bytes", C.c_int64), ("group_rows", C.c_int64)]
 
 
 class ServerRequest(C.Structure):
@@ -178,6 +178,7 @@ SIGNATURES = {
                           C.POINTER(C.c_int32)],
     "gsx_batch_set_tile": [_vp, C.c_int32, C.c_int32, C.c_int32],
     "gsx_batch_set_group": [_vp, C.c_int32],
+    "gsx_batch_set_group_rows": [_vp, C.c_int64],
     "gsx_ctx_arch": [_vp, C.c_char_p, C.c_int32],
     "gsx_server_create": [C.POINTER(ServerOpts), _pp],
     "gsx_server_set_tile": [_vp, C.c_int32, C.c_int32, C.c_int64, C.c_int64, C.c_int32, C.c_int32],
@@ -814,10 +815,13 @@ class Batch:
         _check(self.ctx.h, lib().gsx_batch_history(self.h, i, out.ctypes.data_as(_dp), n.value, C.byref(n)))
         return out
 
-    def set_group(self, group_max):
+    def set_group(self, group_max, rows=None):
         """Members on one operator share SpMV launches in groups of up to
-        group_max (gsx_batch_set_group)."""
+        group_max (gsx_batch_set_group); members of at least rows rows
+        (gsx_batch_set_group_rows; None: the default 131072)."""
         _check(self.ctx.h, lib().gsx_batch_set_group(self.h, group_max))
+        if rows is not None:
+            _check(self.ctx.h, lib().gsx_batch_set_group_rows(self.h, int(rows)))
 
     def set_graph(self, enable):
         """Capture each step's launches as a HIP graph (default) or not."""

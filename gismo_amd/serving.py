@@ -87,13 +87,13 @@ class Server:
 
     def __init__(self, precision=gsx.MIXED, budget_bytes=160 << 30, iters_per_step=8, capacity=4096,
                  keep_solutions=False, device=0, mg_min_dofs=20000, elem_budget=16 << 30, mg_max_coarse=4096,
-                 group_max=1, mg_workers=1):
+                 group_max=1, mg_workers=1, group_rows=0):
         self.precision = precision
         self.keep = keep_solutions
         o = gsx.ServerOpts(device=device, precision=precision, budget_bytes=int(budget_bytes),
                            iters_per_step=iters_per_step, capacity=capacity, mg_min_free=int(mg_min_dofs),
                            mg_max_coarse=int(mg_max_coarse), elem_budget=int(elem_budget), group_max=group_max,
-                           mg_workers=mg_workers)
+                           mg_workers=mg_workers, group_rows=int(group_rows))
         h = C.c_void_p()
         _check_server(None, gsx.lib().gsx_server_create(C.byref(o), C.byref(h)))
         self.h = h

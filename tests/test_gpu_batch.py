@@ -443,7 +443,7 @@ def test_members_sharing_an_operator_share_the_spmv(ctx, precision, tile):
         x.close()
     xs = [ctx.array(n).zero() for _ in bs]
     B = gsx.Batch(ctx, precision, capacity=8)
-    B.set_group(4)   # groups of 4 and 1
+    B.set_group(4, rows=1)   # groups of 4 and 1 (small operators group too)
     ids = [B.add(m.A, b, x, tol=tol, maxit=mi, box=_box(prob), tile=tile) for b, x, mi in zip(bs, xs, maxits)]
     B.run(8)
     A_o = None

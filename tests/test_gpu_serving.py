@@ -22,8 +22,9 @@ def _requests(count=8, seed=99):
 
 
 def _serve(rs, budget=160 << 30, precision=gsx.MIXED, stagger=0.0, mg_min_dofs=20000, group_max=1):
+    # group_rows 1: the small test operators group too
     srv = serving.Server(precision=precision, budget_bytes=budget, iters_per_step=8, mg_min_dofs=mg_min_dofs,
-                         group_max=group_max)
+                         group_max=group_max, group_rows=1)
     for r in rs:
         r.arrival = srv.now()
         srv.submit(r)
