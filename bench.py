@@ -141,7 +141,7 @@ def main():
                     help="serve: multigrid lane from this many free DOFs (0: no multigrid lane)")
     ap.add_argument("--serve-warm", type=int, default=None, help="serve: warm-up requests (count // 3)")
     ap.add_argument("--serve-mg-workers", type=int, default=1, help="serve: multigrid-lane threads")
-    ap.add_argument("--serve-group", type=int, default=1,
+    ap.add_argument("--serve-group", type=int, default=8,
                     help="serve: right-hand sides of a request sharing SpMV launches (gsx_batch_set_group)")
     ap.add_argument("--serve-mixed-only", action="store_true", help="serve: skip the FP64 run")
     ap.add_argument("--batch-grouped", action="store_true",

@@ -477,7 +477,7 @@ gsx_status gsx_server_create(const gsx_server_opts *opts, gsx_server **out)
         if (c) s->cms.push_back(c);
     }
     if (st == GSX_OK) st = gsx_batch_create(s->cb, opts->precision, opts->capacity, &s->batch);
-    if (st == GSX_OK) st = gsx_batch_set_group(s->batch, opts->group_max > 0 ? opts->group_max : 1);
+    if (st == GSX_OK) st = gsx_batch_set_group(s->batch, opts->group_max > 0 ? opts->group_max : 8);
     if (st == GSX_OK && opts->group_rows > 0) st = gsx_batch_set_group_rows(s->batch, opts->group_rows);
     if (st != GSX_OK) {
         if (s->batch) gsx_batch_destroy(s->batch);

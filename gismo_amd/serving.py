@@ -87,7 +87,7 @@ class Server:
 
     def __init__(self, precision=gsx.MIXED, budget_bytes=160 << 30, iters_per_step=8, capacity=4096,
                  keep_solutions=False, device=0, mg_min_dofs=20000, elem_budget=16 << 30, mg_max_coarse=4096,
-                 group_max=1, mg_workers=1, group_rows=0):
+                 group_max=8, mg_workers=1, group_rows=0):
         self.precision = precision
         self.keep = keep_solutions
         o = gsx.ServerOpts(device=device, precision=precision, budget_bytes=int(budget_bytes),

@@ -584,7 +584,7 @@ typedef struct {
     int64_t mg_min_free;      /* multigrid lane from this many free DOFs; 0: off */
     int64_t mg_max_coarse;    /* coarsest-level free DOFs at most (0: 4096)      */
     int64_t elem_budget;      /* multigrid lane element-matrix bytes (0: 16 GiB) */
-    int32_t group_max;        /* gsx_batch_set_group of the batch lane (0: 1)    */
+    int32_t group_max;        /* gsx_batch_set_group of the batch lane (0: 8)    */
     int32_t mg_workers;       /* multigrid-lane threads, one context each (0: 1) */
     int64_t reserve_bytes;    /* device memory kept free beside the admitted
                                  requests (runtime scratch; 0: 16 GiB)           */
