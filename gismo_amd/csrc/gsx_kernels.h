@@ -156,6 +156,14 @@ bool coarsen_supported(int d, int p);
 hipError_t launch_coarsen(int d, int p, const PatchDev *cpatches, const PatchDev *fpatches, const int *gpatch,
                           const int64_t *gprefix, int ng, int64_t nelem_c, const double *T1, const int64_t *toff,
                           const double *Kf, double *Kc, hipStream_t s);
+// banded Cholesky of an SPD CSR matrix of n rows, rows ordered by pos (row
+// i of A is row pos[i] of the factor), half-bandwidth b in that order: Lb of
+// n (b + 1) doubles; err[0] = 1 on a pivot <= 0.  Solve x = A^-1 rhs with a
+// work vector of n doubles.
+hipError_t launch_band_factor(int n, int b, const int *rp, const int *ci, const double *v, const int *pos,
+                              double *Lb, int *err, hipStream_t s);
+hipError_t launch_band_solve(int n, int b, const int *pos, const double *Lb, const double *rhs, double *x,
+                             double *work, hipStream_t s);
 // dense inverse of an SPD CSR matrix (n rows) into D (npad x npad, padded
 // with the identity) by symmetric Gauss-Jordan sweeps; npad from
 // dense_inverse_npad(n), work of dense_inverse_work_doubles(npad) doubles

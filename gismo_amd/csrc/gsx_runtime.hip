@@ -94,6 +94,7 @@ struct gsx_ctx {
     std::shared_ptr<Counter> mem = std::make_shared<Counter>();
     GaussTab gt;
     int64_t elem_budget = int64_t(16) << 30;   // bytes of element matrices per assembly pass
+    int64_t coarse_dense_max = 32768;          // largest coarsest level solved by its dense inverse
     void *scratch = nullptr;   // this device's address of the allocation-free workspace
 };
 
@@ -513,6 +514,13 @@ gsx_status gsx_ctx_set_elem_budget(gsx_ctx *c, int64_t bytes)
 {
     if (!c || bytes <= 0) return GSX_ERR_ARG;
     c->elem_budget = bytes;
+    return GSX_OK;
+}
+
+gsx_status gsx_ctx_set_coarse_dense_max(gsx_ctx *c, int64_t rows)
+{
+    if (!c || rows < 1 || rows > 32768) return GSX_ERR_ARG;
+    c->coarse_dense_max = rows;
     return GSX_OK;
 }
 
@@ -2669,6 +2677,10 @@ struct gsx_mg {
     double tau = 1.0, cgc = 1.0;
     int precision = GSX_FP64;
     int npad = 0;
+    // exact coarse solve by a banded Cholesky factor (band_b > 0) in reverse
+    // Cuthill-McKee order, else by the dense inverse Dinv
+    int band_b = 0;
+    DevPtr band_pos, band_L, band_y;
     double scms_saved = 0.0;           // SCMS damping kept for finer levels
     std::vector<double> scms_saved_comp;   // the same per component (several components)
     PatchLayout fine_layout;           // finest level's patches and DOF map
@@ -2765,8 +2777,70 @@ static gsx_status scan_counts(gsx_ctx *c, const int *counts, int n, DevPtr &rowp
     return GSX_OK;
 }
 
+// Reverse Cuthill-McKee order of a symmetric pattern (George and Liu,
+// "Computer Solution of Large Sparse Positive Definite Systems", 1981):
+// per connected part a breadth-first search from a node of least degree,
+// restarted once from the last node that search reaches (a far node), with
+// neighbours by ascending degree; the whole order reversed; ties go to the
+// lower index.  pos[i] = position of row i; returns the half-bandwidth in
+// that order.
+static int rcm_order(int n, const std::vector<int> &rp, const std::vector<int> &ci, std::vector<int> &pos)
+{
+    std::vector<int> deg((size_t)n), order, level((size_t)n, -1);
+    for (int i = 0; i < n; ++i) deg[(size_t)i] = rp[(size_t)i + 1] - rp[(size_t)i];
+    std::vector<char> seen((size_t)n, 0);
+    order.reserve((size_t)n);
+    std::vector<int> nb;
+    auto bfs = [&](int s, std::vector<int> &out) {
+        // breadth-first over the unseen nodes from s; out in visiting order
+        out.clear();
+        std::vector<int> mark;
+        out.push_back(s);
+        level[(size_t)s] = 0;
+        mark.push_back(s);
+        for (size_t h = 0; h < out.size(); ++h) {
+            const int v = out[h];
+            nb.clear();
+            for (int k = rp[(size_t)v]; k < rp[(size_t)v + 1]; ++k) {
+                const int u = ci[(size_t)k];
+                if (u != v && !seen[(size_t)u] && level[(size_t)u] < 0) {
+                    level[(size_t)u] = level[(size_t)v] + 1;
+                    mark.push_back(u);
+                    nb.push_back(u);
+                }
+            }
+            std::sort(nb.begin(), nb.end(), [&](int a, int b) {
+                return deg[(size_t)a] != deg[(size_t)b] ? deg[(size_t)a] < deg[(size_t)b] : a < b;
+            });
+            out.insert(out.end(), nb.begin(), nb.end());
+        }
+        for (int m : mark) level[(size_t)m] = -1;
+    };
+    std::vector<int> part;
+    for (;;) {
+        int s = -1;
+        for (int i = 0; i < n; ++i)
+            if (!seen[(size_t)i] && (s < 0 || deg[(size_t)i] < deg[(size_t)s])) s = i;
+        if (s < 0) break;
+        bfs(s, part);
+        bfs(part.back(), part);
+        for (int v : part) {
+            seen[(size_t)v] = 1;
+            order.push_back(v);
+        }
+    }
+    std::reverse(order.begin(), order.end());
+    pos.assign((size_t)n, 0);
+    for (int k = 0; k < n; ++k) pos[(size_t)order[(size_t)k]] = k;
+    int b = 0;
+    for (int i = 0; i < n; ++i)
+        for (int k = rp[(size_t)i]; k < rp[(size_t)i + 1]; ++k) b = std::max(b, std::abs(pos[(size_t)i] - pos[(size_t)ci[(size_t)k]]));
+    return b;
+}
+
 // Per-level diagonals, FP32 copies and vectors, then the exact coarse solve
-// (dense FP64 inverse of the coarsest operator).
+// (the dense inverse of the coarsest operator or, above the context's
+// coarse_dense_max rows, its banded FP64 Cholesky factor).
 static gsx_status mg_finish(gsx_mg *mg)
 {
     gsx_ctx *c = mg->ctx;
@@ -2796,12 +2870,41 @@ static gsx_status mg_finish(gsx_mg *mg)
         }
         GSX_TRY(mg_level_vectors(c, V, mixed));
     }
-    // exact coarse solve: dense FP64 inverse of the coarsest operator
+    // exact coarse solve: the dense inverse up to coarse_dense_max rows (one
+    // GEMV per solve); above, a banded FP64 Cholesky factor in reverse
+    // Cuthill-McKee order (its two triangular solves run panel after panel,
+    // slower than the GEMV where both fit)
     {
         KScope ki(c, "mg_coarse_inverse");
         MGLevel &C0 = mg->L[0];
         const int n0 = C0.n;
-        if ((int64_t)n0 > 32768) return fail(c, GSX_ERR_ARG, "coarsest level too large for the dense inverse");
+        if ((int64_t)n0 > c->coarse_dense_max) {
+            std::vector<int> rp((size_t)n0 + 1), ci((size_t)C0.nnz), pos;
+            HIP_OK(c, hipMemcpyAsync(rp.data(), C0.rowptr.get(), sizeof(int) * (n0 + 1), hipMemcpyDeviceToHost, s));
+            HIP_OK(c, hipMemcpyAsync(ci.data(), C0.col.get(), sizeof(int) * C0.nnz, hipMemcpyDeviceToHost, s));
+            HIP_OK(c, hipStreamSynchronize(s));
+            const int b = rcm_order(n0, rp, ci, pos);
+            if (b <= 4096) {
+                mg->band_b = b;
+                mg->band_pos = dalloc(c, sizeof(int) * n0);
+                mg->band_L = dalloc(c, sizeof(double) * (size_t)n0 * (b + 1));
+                mg->band_y = dalloc(c, sizeof(double) * n0);
+                DevPtr errf = dalloc(c, sizeof(int));
+                if (!mg->band_pos || !mg->band_L || !mg->band_y || !errf)
+                    return fail(c, GSX_ERR_OOM, "device allocation (coarse factor)");
+                HIP_OK(c, hipMemcpyAsync(mg->band_pos.get(), pos.data(), sizeof(int) * n0, hipMemcpyHostToDevice, s));
+                HIP_OK(c, hipMemsetAsync(errf.get(), 0, sizeof(int), s));
+                HIP_OK(c, gsx::launch_band_factor(n0, b, dp<int>(C0.rowptr), dp<int>(C0.col), dp<double>(C0.val),
+                                                  dp<int>(mg->band_pos), dp<double>(mg->band_L), dp<int>(errf), s));
+                int errh = 0;
+                HIP_OK(c, hipMemcpyAsync(&errh, errf.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+                HIP_OK(c, hipStreamSynchronize(s));
+                if (errh) return fail(c, GSX_ERR_ARG, "coarsest operator is not positive definite (Cholesky pivot)");
+                return GSX_OK;
+            }
+        }
+        if ((int64_t)n0 > 32768)
+            return fail(c, GSX_ERR_ARG, "coarsest level too large for the dense inverse and its band too wide");
         mg->npad = gsx::dense_inverse_npad(n0);
         mg->Dinv = dalloc(c, sizeof(double) * (size_t)mg->npad * mg->npad);
         DevPtr W = dalloc(c, sizeof(double) * (size_t)gsx::dense_inverse_work_doubles(mg->npad));
@@ -3791,7 +3894,11 @@ static gsx_status mg_cycle64(gsx_mg *mg, int l, bool mixed = false)
     double *x = dp<double>(V.x), *t = dp<double>(V.t), *b = dp<double>(V.b), *r = dp<double>(V.r);
     if (l == 0) {
         KScope k(c, "mg_coarse");
-        HIP_OK(c, gsx::launch_dense_gemv_f64(V.n, mg->npad, dp<double>(mg->Dinv), b, x, s));
+        if (mg->band_b > 0)
+            HIP_OK(c, gsx::launch_band_solve(V.n, mg->band_b, dp<int>(mg->band_pos), dp<double>(mg->band_L), b, x,
+                                             dp<double>(mg->band_y), s));
+        else
+            HIP_OK(c, gsx::launch_dense_gemv_f64(V.n, mg->npad, dp<double>(mg->Dinv), b, x, s));
         return GSX_OK;
     }
     const int *rp = dp<int>(V.rowptr), *ci = dp<int>(V.col);
@@ -3846,7 +3953,12 @@ static gsx_status mg_cycle32(gsx_mg *mg, int l)
     if (l == 0) {
         KScope k(c, "mg_coarse");
         HIP_OK(c, gsx::launch_conv_f32_f64(V.n, b, dp<double>(V.b), s));
-        HIP_OK(c, gsx::launch_dense_gemv_f64(V.n, mg->npad, dp<double>(mg->Dinv), dp<double>(V.b), dp<double>(V.x), s));
+        if (mg->band_b > 0)
+            HIP_OK(c, gsx::launch_band_solve(V.n, mg->band_b, dp<int>(mg->band_pos), dp<double>(mg->band_L),
+                                             dp<double>(V.b), dp<double>(V.x), dp<double>(mg->band_y), s));
+        else
+            HIP_OK(c, gsx::launch_dense_gemv_f64(V.n, mg->npad, dp<double>(mg->Dinv), dp<double>(V.b),
+                                                 dp<double>(V.x), s));
         HIP_OK(c, gsx::launch_conv_f64_f32(V.n, dp<double>(V.x), x, s));
         return GSX_OK;
     }

@@ -124,6 +124,7 @@ SIGNATURES = {
     "gsx_copy_to_host": [_vp, _vp, _vp, C.c_int64],
     "gsx_ctx_memory": [_vp, _i64p],
     "gsx_ctx_set_elem_budget": [_vp, C.c_int64],
+    "gsx_ctx_set_coarse_dense_max": [_vp, C.c_int64],
     "gsx_problem_create": [_vp, C.c_int32, C.POINTER(PatchDesc), C.c_int32, _i32p, C.c_int32,
                            _i32p, _pp],
     "gsx_problem_create_geo": [_vp, C.c_int32, C.POINTER(PatchDesc), C.c_int32, _i32p, C.c_int32, _i32p, _pp],
@@ -333,6 +334,11 @@ class Context:
         """Bytes of element matrices per assembly pass (single patches larger
         than this are assembled in slabs)."""
         _check(self.h, lib().gsx_ctx_set_elem_budget(self.h, int(nbytes)))
+
+    def set_coarse_dense_max(self, rows):
+        """Largest coarsest multigrid level solved through its dense inverse
+        (default 32768 rows); larger ones use a banded Cholesky factor."""
+        _check(self.h, lib().gsx_ctx_set_coarse_dense_max(self.h, int(rows)))
 
     def memory(self):
         b = C.c_int64()

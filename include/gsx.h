@@ -128,6 +128,14 @@ gsx_status gsx_ctx_memory(gsx_ctx *ctx, int64_t *bytes_in_use);
  * does not fit fails with GSX_ERR_BUDGET.  No reference counterpart (the
  * reference assembles element by element on the host). */
 gsx_status gsx_ctx_set_elem_budget(gsx_ctx *ctx, int64_t bytes);
+/* Largest coarsest multigrid level (rows, 1..32768, default 32768) solved
+ * exactly through its dense FP64 inverse, one GEMV per coarse solve.  A
+ * larger coarsest level is factored as a banded FP64 Cholesky in reverse
+ * Cuthill-McKee order (half-bandwidth at most 4096); both are exact solves,
+ * like the reference's sparse Cholesky of its coarsest level
+ * (src/gsSolver/gsMatrixOp.h:357-370, makeSparseCholeskySolver).  Read when
+ * a multigrid is created. */
+gsx_status gsx_ctx_set_coarse_dense_max(gsx_ctx *ctx, int64_t rows);
 
 /* ---- host-only building blocks (no device needed) ---------------------- */
 /* Gauss-Legendre rule with n nodes on [-1,1], ascending: the values of
