@@ -663,25 +663,48 @@ __global__ __launch_bounds__(256) void k_gjb_gemm(double *__restrict__ D, int np
     for (int u = 0; u < 2; ++u)
 #pragma unroll
         for (int v = 0; v < 2; ++v) acc[u][v] = gj_d4{0.0, 0.0, 0.0, 0.0};
-    for (int kc = 0; kc < kGJB / kGJ; ++kc) {
+    // the next 64-deep chunk is fetched into registers while the current
+    // one is multiplied from LDS (16 + 16 values per thread)
+    constexpr int NQ = kGJ * kGJ / 256;
+    double ra[NQ], rb[NQ];
+    auto fetch = [&](int kc) {
         const int kcol = kb + kc * kGJ;
-        for (int i = t; i < kGJ * kGJ; i += 256) {
-            const int r = i / kGJ, c = i % kGJ;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int i = t + 256 * q, r = i / kGJ, c = i % kGJ;
             if (MODE == 0) {
-                sA[r][c] = Binv[(int64_t)(I + r) * kGJB + kc * kGJ + c];
-                // M[kcol + r][J + c]: the upper tile D[kcol][J], or the
-                // transpose of D[J][kcol]
-                if (J > kb) sB[r][c] = D[(int64_t)(kcol + r) * npad + J + c];
-                else sB[c][r] = D[(int64_t)(J + r) * npad + kcol + c];
+                ra[q] = Binv[(int64_t)(I + r) * kGJB + kc * kGJ + c];
+                // M[kcol + r][J + c]: the upper tile D[kcol][J], or (stored
+                // transposed below) D[J][kcol]
+                rb[q] = J > kb ? D[(int64_t)(kcol + r) * npad + J + c] : D[(int64_t)(J + r) * npad + kcol + c];
             } else {
-                // M[I + r][kcol + c]: the upper tile D[I][kcol], or the
-                // transpose of D[kcol][I]
-                if (I < kb) sA[r][c] = D[(int64_t)(I + r) * npad + kcol + c];
-                else sA[c][r] = D[(int64_t)(kcol + r) * npad + I + c];
-                sB[r][c] = W[(int64_t)(kc * kGJ + r) * npad + J + c];
+                // M[I + r][kcol + c]: the upper tile D[I][kcol], or (stored
+                // transposed below) D[kcol][I]
+                ra[q] = I < kb ? D[(int64_t)(I + r) * npad + kcol + c] : D[(int64_t)(kcol + r) * npad + I + c];
+                rb[q] = W[(int64_t)(kc * kGJ + r) * npad + J + c];
             }
         }
+    };
+    auto stash = [&]() {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int i = t + 256 * q, r = i / kGJ, c = i % kGJ;
+            if (MODE == 0) {
+                sA[r][c] = ra[q];
+                if (J > kb) sB[r][c] = rb[q];
+                else sB[c][r] = rb[q];
+            } else {
+                if (I < kb) sA[r][c] = ra[q];
+                else sA[c][r] = ra[q];
+                sB[r][c] = rb[q];
+            }
+        }
+    };
+    fetch(0);
+    for (int kc = 0; kc < kGJB / kGJ; ++kc) {
+        stash();
         __syncthreads();
+        if (kc + 1 < kGJB / kGJ) fetch(kc + 1);
 #pragma unroll 4
         for (int ks = 0; ks < kGJ / 4; ++ks) {
             const int m = 4 * ks + lk;
