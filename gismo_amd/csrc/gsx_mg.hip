@@ -457,10 +457,14 @@ __global__ void k_dense_scatter(int n, int npad, const int *__restrict__ rp, con
 }
 
 // invert the pivot block (kb) in LDS; result to Pinv (64x64, row-major).
-// In-place Gauss-Jordan: step p maps every entry from the old values (kept
-// in registers between the two barriers), with one reciprocal per step.
-// The 64 steps are sequential, so the block is wide (1024 threads, 4
-// entries each) to keep each step short.
+// In-place Gauss-Jordan over pairs of pivots: step p sweeps rows/columns
+// K = {p, p + 1} at once with the 2 x 2 inverse Q of M[K][K] (positive
+// determinant: the block is SPD),
+//   M'[K][K] = Q,  M'[K][c] = Q M[K][c],  M'[r][K] = -M[r][K] Q,
+//   M'[r][c] = M[r][c] - M[r][K] Q M[K][c],
+// which equals the two single sweeps in exact arithmetic and halves the
+// sequential steps.  Every entry is mapped from the old values (kept in
+// registers between the two barriers); 1024 threads, 4 entries each.
 constexpr int kPivThreads = 1024;
 __global__ __launch_bounds__(kPivThreads) void k_gj_pivot(double *__restrict__ D, int npad, int kb,
                                                           double *__restrict__ Pinv)
@@ -470,15 +474,26 @@ __global__ __launch_bounds__(kPivThreads) void k_gj_pivot(double *__restrict__ D
     const int t = threadIdx.x;
     for (int i = t; i < kGJ * kGJ; i += kPivThreads) M[i / kGJ][i % kGJ] = D[(int64_t)(kb + i / kGJ) * npad + kb + i % kGJ];
     __syncthreads();
-    for (int p = 0; p < kGJ; ++p) {
-        const double inv = 1.0 / M[p][p];
+    for (int p = 0; p < kGJ; p += 2) {
+        const double a = M[p][p], b = M[p][p + 1], c = M[p + 1][p], d = M[p + 1][p + 1];
+        const double id = 1.0 / (a * d - b * c);
+        const double q00 = d * id, q01 = -b * id, q10 = -c * id, q11 = a * id;
         double nv[PER];
 #pragma unroll
         for (int j = 0; j < PER; ++j) {
-            const int i = t + kPivThreads * j, r = i / kGJ, c = i % kGJ;
-            const double a = M[r][p] * inv, b = M[p][c];
-            const double x = M[r][c];
-            nv[j] = (r != p) ? ((c != p) ? x - a * b : -a) : ((c != p) ? b * inv : inv);
+            const int i = t + kPivThreads * j, r = i / kGJ, col = i % kGJ;
+            const bool rin = (r >> 1) == (p >> 1), cin = (col >> 1) == (p >> 1);
+            const double m0 = M[p][col], m1 = M[p + 1][col];
+            double x;
+            if (rin) {
+                const double e0 = (r == p) ? q00 : q10, e1 = (r == p) ? q01 : q11;
+                x = cin ? ((col == p) ? e0 : e1) : e0 * m0 + e1 * m1;
+            } else {
+                const double u0 = M[r][p] * q00 + M[r][p + 1] * q10;
+                const double u1 = M[r][p] * q01 + M[r][p + 1] * q11;
+                x = cin ? -((col == p) ? u0 : u1) : M[r][col] - (u0 * m0 + u1 * m1);
+            }
+            nv[j] = x;
         }
         __syncthreads();
 #pragma unroll
